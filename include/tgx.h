@@ -1,0 +1,216 @@
+/*
+ * tgx.h - C-ABI of the MI355X GAP/SOAP energy-and-force engine.
+ *
+ * This is the drop-in boundary behind TurboGAP's Fortran drivers.  Every
+ * entry point takes plain pointers, sizes and scalars (no HIP or torch types)
+ * so that Fortran binds it with iso_c_binding exactly as the reference binds
+ * its GPU layer (reference: src/gpu/gpu_interface.f90:34-703).
+ *
+ * Conventions (same as the reference's Fortran arrays, so buffers pass
+ * straight through):
+ *   - reals are double, integers are int32 (Fortran default integer);
+ *   - positions(3,N), forces(3,N), xyz(3,n_pairs) are column-major, i.e.
+ *     x,y,z of one atom are contiguous;
+ *   - soap(dim,n_sites) and Qs(dim,n_sparse) are column-major;
+ *   - the lattice is the 9 doubles a(1:3), b(1:3), c(1:3)
+ *     (state%a_box, b_box, c_box, reference types/types.f90:147-211);
+ *   - species are 1-based indices into the model's species list
+ *     (state%species), neighbour ids are 1-based supercell ids
+ *     (neighbors%neighbors_list, reference types/types.f90:83-111);
+ *   - virial(3,3) is symmetric, column-major.
+ * Every function returns TGX_OK (0) or an error code; tgx_last_error() gives
+ * the message.  The library never stops the process (the reference prints and
+ * calls stop; the Fortran shim keeps that behaviour on a non-zero code).
+ * Calls on one context are stream-ordered and not thread-safe; separate
+ * contexts are independent.
+ */
+#ifndef TGX_H
+#define TGX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TGX_MAX_SPECIES 8
+#define TGX_MAX_LMAX 10  /* (l_max+1)^2 <= 121 real harmonics      */
+#define TGX_MAX_NR 32    /* sum over species of alpha_max (n_max)  */
+
+enum {
+  TGX_OK = 0,
+  TGX_ERR_ARG = 1,         /* bad argument or model/shape mismatch           */
+  TGX_ERR_HIP = 2,         /* HIP runtime failure                            */
+  TGX_ERR_CAPACITY = 3,    /* caller buffer too small (size is reported)     */
+  TGX_ERR_UNSUPPORTED = 4, /* e.g. non-orthorhombic cell                     */
+  TGX_ERR_STATE = 5        /* no model loaded, wrong call order              */
+};
+
+/* Arithmetic of the SOAP expansion and of the kernel GEMM. */
+enum {
+  TGX_FP64 = 0,  /* SOAP FP64, kernel GEMM FP64                           */
+  TGX_FP32 = 1,  /* SOAP FP32, kernel GEMM FP32                           */
+  TGX_MIXED = 2  /* SOAP FP32, kernel GEMM BF16 on MFMA, FP32 accumulate   */
+};
+
+typedef struct tgx_ctx tgx_ctx;
+
+/*
+ * Hyper-parameters of one soap_turbo descriptor block.  Field meaning and
+ * defaults follow the reference's soap_turbo type (types/types.f90:252-312)
+ * and its .gap parser (gap/read_gap.f90:156-299, 425-432).  Per-species
+ * arrays are indexed by species-1.  rcut_soft is the soft radius
+ * (rcut_hard - buffer), as read_gap.f90:425-432 converts it.
+ */
+typedef struct {
+  int32_t n_species;
+  int32_t l_max;
+  int32_t radial_enhancement; /* 0, 1 or 2                                  */
+  int32_t central_species;    /* 0 = all atoms are sites, else 1-based       */
+  int32_t basis;              /* 0 = poly3 (the only basis implemented)      */
+  int32_t pad_;
+  int32_t alpha_max[TGX_MAX_SPECIES];
+  double rcut_hard[TGX_MAX_SPECIES];
+  double rcut_soft[TGX_MAX_SPECIES];
+  double atom_sigma_r[TGX_MAX_SPECIES];
+  double atom_sigma_t[TGX_MAX_SPECIES];
+  double atom_sigma_r_scaling[TGX_MAX_SPECIES];
+  double atom_sigma_t_scaling[TGX_MAX_SPECIES];
+  double amplitude_scaling[TGX_MAX_SPECIES];
+  double central_weight[TGX_MAX_SPECIES];
+  double global_scaling[TGX_MAX_SPECIES];
+  double nf[TGX_MAX_SPECIES];
+} tgx_soap_hypers;
+
+/* Context ------------------------------------------------------------------ */
+
+/* Replaces the empty _GPU init block (turbogap_main.f90:320-327) and the
+ * handle/stream creation of gpu_interface.f90:682-700. */
+int tgx_create(int device, int precision, tgx_ctx** ctx);
+int tgx_destroy(tgx_ctx* ctx);
+const char* tgx_last_error(const tgx_ctx* ctx);
+int tgx_set_precision(tgx_ctx* ctx, int precision);
+
+/* Models ------------------------------------------------------------------- */
+
+/* One soap_turbo GAP block: hypers + sparse set Qs(dim,n_sparse) + alphas,
+ * delta, zeta (read_gap.f90:45-771, 773-875).  dim must equal
+ * N_R(N_R+1)/2*(l_max+1) with N_R = sum(alpha_max) (read_gap.f90:556-558). */
+int tgx_set_soap_model(tgx_ctx* ctx, const tgx_soap_hypers* h, int dim,
+                       int n_sparse, const double* Qs, const double* alphas,
+                       double delta, double zeta);
+
+/* Per-species isolated-atom energies added to every atom of that species
+ * (calculate_e0, gap/calculate_gap_soap.f90:135-169). */
+int tgx_set_e0(tgx_ctx* ctx, int n_species, const double* e0);
+
+/* One distance_2b block (gap_2b_t, types.f90:314-321; alphas/cutoff/Qs read
+ * by read_gap.f90:829-848).  sp1/sp2 are 1-based species. */
+int tgx_add_2b_model(tgx_ctx* ctx, int sp1, int sp2, double rcut,
+                     double buffer, double delta, double sigma, int n_sparse,
+                     const double* Qs, const double* alphas,
+                     const double* cutoff);
+int tgx_clear_2b_models(tgx_ctx* ctx);
+
+/* Hot path ----------------------------------------------------------------- */
+
+/*
+ * Full neighbour list within rcut (build_neighbors_list,
+ * neighbors/neighbors_interface.f90:144-524, with get_distance :547-643).
+ * pos(3,n) is the unit cell; when the cell is thinner than 2*rcut it is
+ * replicated exactly as read_xyz.f90:133-155 / set_supercell :323-373 do and
+ * neighbour ids are supercell ids.  Each atom's list holds itself first
+ * (rjs=0, xyz=0, theta=phi=0) and then its neighbours in ascending supercell
+ * id (the reference's order is cell-traversal order; the set is identical).
+ * rjs/xyz are bit-identical to the reference's FP64 arithmetic.  cap is the
+ * length of list/rjs/thetas/phis (xyz holds 3*cap); when too small,
+ * TGX_ERR_CAPACITY is returned and *n_pairs holds the required size.
+ * indices_out (may be NULL) receives the supercell repetitions.
+ * Orthorhombic cells only (TGX_ERR_UNSUPPORTED otherwise).
+ */
+int tgx_neighbors_build(tgx_ctx* ctx, int n, const double* pos,
+                        const double* lattice, double rcut, int64_t cap,
+                        int* n_neigh, int* list, double* rjs, double* xyz,
+                        double* thetas, double* phis, int64_t* n_pairs,
+                        int* indices_out);
+
+/*
+ * Normalised SOAP vectors soap(dim,n_sites) of the central sites, in site
+ * order (the get_soap call at gap/gap_interface.f90:301-306, forward part).
+ * n_sites_out receives the number of sites (atoms whose species matches
+ * central_species, or all atoms).  site_index (may be NULL, length n) gets
+ * the 1-based atom id of each site.
+ */
+int tgx_soap_descriptors(tgx_ctx* ctx, int n, const double* pos,
+                         const int* species, const double* lattice,
+                         double* soap, int* n_sites_out, int* site_index);
+
+/*
+ * The GAP kernel step on given descriptors (gap/gap.f90:118-171):
+ * energies(i) = delta^2 * sum_s K(i,s)^zeta alphas(s), K = soap^T Qs, and,
+ * when qss is not NULL, Qss(n_sites,dim) = -zeta delta^2 (K^(zeta-1) o alphas)
+ * Qs^T (column-major, as the reference's Qss).  Runs in the context's
+ * precision mode.  e0 is not added.
+ */
+int tgx_gap_kernel_step(tgx_ctx* ctx, int n_sites, const double* soap,
+                        double* energies, double* qss);
+
+/*
+ * Energies, forces and virial of one periodic structure: neighbour list, SOAP
+ * term, 2-body terms and e0 (calculate_gap_soap.f90:171-481 +
+ * calculate_gap_2b gap.f90:246-327 + calculate_e0, summed as
+ * turbogap_main.f90:993-1109 does).  e_atom(n), forces(3,n), virial(3,3).
+ * With do_forces = 0 only e_atom is written (MC mode,
+ * control/read_control.f90:47-51).
+ */
+int tgx_energy_forces(tgx_ctx* ctx, int n, const double* pos,
+                      const int* species, const double* lattice,
+                      int do_forces, double* e_atom, double* forces,
+                      double* virial);
+
+/*
+ * Many independent structures in one call (structures s = 0..n_struct-1 own
+ * atoms offsets[s]..offsets[s+1]-1 of pos/species/e_atom/forces; lattices
+ * holds 9 doubles per structure, virials 9 per structure).  All structures
+ * share one set of launches.
+ */
+int tgx_energy_forces_batched(tgx_ctx* ctx, int n_struct, const int* offsets,
+                              const double* pos, const int* species,
+                              const double* lattices, int do_forces,
+                              double* e_atom, double* forces,
+                              double* virials);
+
+/*
+ * NVE molecular dynamics, n_steps velocity-Verlet steps of dt fs
+ * (md/md_utils.f90:486-532) with the neighbour list built with skin and
+ * rebuilt when any atom moved more than skin/2 since the last build
+ * (md/md_interface.f90:1112-1151).  Masses in amu (converted to
+ * eV fs^2/A^2 with 103.6426965268, read/read_files.f90:333).  pos and vel
+ * (3,n) are updated in place; forces(3,n) receives the final forces.
+ * e_pot and e_kin (length n_steps+1, may be NULL) receive total potential
+ * and kinetic energy before the first and after every step.  The step is
+ * captured once in a hipGraph and replayed.  n_rebuilds (may be NULL)
+ * receives the number of list rebuilds.
+ */
+int tgx_md_run(tgx_ctx* ctx, int n, int n_steps, double dt, double skin,
+               const int* species, const double* lattice, const double* mass,
+               double* pos, double* vel, double* forces, double* e_pot,
+               double* e_kin, int* n_rebuilds);
+
+/* Overlap S(alpha_max,alpha_max) and orthonormalisation W = S^(-1/2) of the
+ * poly3 radial basis, both column-major (symmetric); replaces
+ * get_orthonormalization_matrix_poly3 (gpu/gpu_interface.f90:424-429).
+ * Host-only (no device needed).  W is exact to FP64 rounding although
+ * cond(S) ~ 1e13 at alpha_max = 8. */
+int tgx_poly3_orthonormalization(int alpha_max, double* S, double* W);
+
+/* Per-stage timing of the last call, milliseconds (HIP events):
+ * [0] neighbours, [1] SOAP forward, [2] kernel GEMMs, [3] SOAP backward and
+ * force gather, [4] 2-body, [5] integrator, [6] total. */
+int tgx_stage_times(const tgx_ctx* ctx, double* ms7);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TGX_H */

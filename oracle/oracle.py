@@ -1,0 +1,279 @@
+"""ctypes wrapper of the FP64 CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product.  Each function names the
+reference routine it restates (see tgx_oracle.c for what pins it).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+PKG = os.path.join(ROOT, "turbogap_2.0_amd")
+if PKG not in sys.path:
+    sys.path.insert(0, PKG)
+
+from tgx import CSoapHypers, GapModel, SoapHypers  # noqa: E402
+
+LIB = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+
+class C2b(C.Structure):
+    _fields_ = [("sp1", C.c_int32), ("sp2", C.c_int32), ("n_sparse", C.c_int32),
+                ("pad_", C.c_int32), ("rcut", C.c_double), ("buffer", C.c_double),
+                ("delta", C.c_double), ("sigma", C.c_double),
+                ("Qs", C.POINTER(C.c_double)), ("alphas", C.POINTER(C.c_double)),
+                ("cutoff", C.POINTER(C.c_double))]
+
+
+class CModel(C.Structure):
+    _fields_ = [("h", CSoapHypers), ("has_soap", C.c_int32), ("dim", C.c_int32),
+                ("n_sparse", C.c_int32), ("n2b", C.c_int32),
+                ("Qs", C.POINTER(C.c_double)), ("alphas", C.POINTER(C.c_double)),
+                ("delta", C.c_double), ("zeta", C.c_double),
+                ("e0", C.c_double * 8), ("b2", C.POINTER(C2b))]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = C.CDLL(LIB)
+    return _lib
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def mic(pi, pj, L):
+    """get_distance, orthorhombic branch (neighbors_interface.f90:547-581)."""
+    out = np.zeros(3)
+    d = C.c_double()
+    lib().orc_mic(_d(_f64(pi)), _d(_f64(pj)), _d(_f64(L)), _d(out), C.byref(d))
+    return out, d.value
+
+
+def cells_for_cutoff(lattice, rcut):
+    idx = np.zeros(3, np.int32)
+    lib().orc_cells_for_cutoff(_d(_f64(lattice)), C.c_double(rcut), _i(idx))
+    return idx
+
+
+def neighbors(pos, lattice, rcut, algo=0):
+    """build_neighbors_list (neighbors_interface.f90:144-524), sorted lists."""
+    pos, lat = _f64(pos), _f64(lattice)
+    n = len(pos)
+    L = lib()
+    np_ = C.c_int64(0)
+    L.orc_neighbors.restype = C.c_int
+    rc = L.orc_neighbors(n, _d(pos), _d(lat), C.c_double(rcut), algo, C.c_int64(0), None, None,
+                         None, None, None, None, C.byref(np_), None)
+    if rc not in (0, 3):
+        raise RuntimeError(f"orc_neighbors failed {rc}")
+    m = np_.value
+    nn = np.zeros(n, np.int32)
+    lst = np.zeros(m, np.int32)
+    rjs = np.zeros(m)
+    xyz = np.zeros((m, 3))
+    th = np.zeros(m)
+    ph = np.zeros(m)
+    idx = np.zeros(3, np.int32)
+    rc = L.orc_neighbors(n, _d(pos), _d(lat), C.c_double(rcut), algo, C.c_int64(m), _i(nn),
+                         _i(lst), _d(rjs), _d(xyz), _d(th), _d(ph), C.byref(np_), _i(idx))
+    assert rc == 0
+    return dict(n_neigh=nn, list=lst, rjs=rjs, xyz=xyz, thetas=th, phis=ph, indices=idx)
+
+
+def poly3_overlap(nmax):
+    S = np.zeros((nmax, nmax))
+    W = np.zeros((nmax, nmax))
+    lib().orc_poly3_overlap(nmax, _d(S), _d(W))
+    return S, W
+
+
+def radial(h: SoapHypers, s: int, r):
+    """R_n(r), dR_n/dr of species s (0-based)."""
+    r = _f64(np.atleast_1d(r))
+    nmax = h.alpha_max[s]
+    R = np.zeros((len(r), nmax))
+    dR = np.zeros((len(r), nmax))
+    ch = h.to_c()
+    lib().orc_radial(C.byref(ch), s, len(r), _d(r), _d(R), _d(dR))
+    return R, dR
+
+
+def radial_raw(r, rh, sig, p, nk):
+    out = np.zeros(nk)
+    lib().orc_radial_raw(C.c_double(r), C.c_double(rh), C.c_double(sig), p, nk, _d(out))
+    return out
+
+
+def sph(lmax, u):
+    L2 = (lmax + 1) ** 2
+    Y = np.zeros(L2)
+    dF = np.zeros((L2, 3))
+    lib().orc_sph(lmax, _d(_f64(u)), _d(Y), _d(dF))
+    return Y, dF
+
+
+def soap(h: SoapHypers, pos, species, lattice, derivatives=False):
+    """get_soap: descriptors (n_sites, dim) and optionally soap_cart_der as
+    (n_pairs, dim, 3), with pair_atom / site_atom / site_n_neigh."""
+    pos, lat = _f64(pos), _f64(lattice)
+    sp = np.ascontiguousarray(species, dtype=np.int32)
+    n = len(pos)
+    ch = h.to_c()
+    L = lib()
+    ns = C.c_int(0)
+    npairs = C.c_int64(0)
+    out = np.zeros((n, h.dim))
+    site = np.zeros(n, np.int32)
+    nn = np.zeros(n, np.int32)
+    if not derivatives:
+        rc = L.orc_soap(C.byref(ch), n, _d(pos), _d(lat), _i(sp), _d(out), None, C.c_int64(0),
+                        C.byref(ns), C.byref(npairs), _i(site), None, None, _i(nn))
+        assert rc == 0, rc
+        return dict(soap=out[: ns.value], site_atom=site[: ns.value])
+    rc = L.orc_soap(C.byref(ch), n, _d(pos), _d(lat), _i(sp), _d(out), None, C.c_int64(0),
+                    C.byref(ns), C.byref(npairs), None, None, None, None)
+    m = npairs.value
+    der = np.zeros((m, h.dim, 3))
+    pa = np.zeros(m, np.int32)
+    pxyz = np.zeros((m, 3))
+    rc = L.orc_soap(C.byref(ch), n, _d(pos), _d(lat), _i(sp), _d(out), _d(der), C.c_int64(m),
+                    C.byref(ns), C.byref(npairs), _i(site), _i(pa), _d(pxyz), _i(nn))
+    assert rc == 0, rc
+    k = ns.value
+    return dict(soap=out[:k], soap_der=der, site_atom=site[:k], pair_atom=pa, pair_xyz=pxyz,
+                site_n_neigh=nn[:k])
+
+
+def gap_kernel_step(soap_, Qs, alphas, delta, zeta):
+    """gap.f90:118-171: energies and Qss (n_sites, dim)."""
+    q = _f64(soap_)
+    ns, dim = q.shape
+    Qs = _f64(Qs)
+    e = np.zeros(ns)
+    qss = np.zeros((dim, ns))
+    lib().orc_gap_kernel_step(ns, dim, len(Qs), _d(q), _d(Qs), _d(_f64(alphas)),
+                              C.c_double(delta), C.c_double(zeta), _d(e), _d(qss))
+    return e, qss.T.copy()
+
+
+def gap_soap_ef(soap_, soap_der, alphas, delta, zeta, e0, Qs, n_neigh, pair_atom, xyz,
+                n_atoms, do_forces=True):
+    """get_soap_energy_and_forces (gap.f90:39-244)."""
+    q = _f64(soap_)
+    ns, dim = q.shape
+    e = np.zeros(ns)
+    f = np.zeros((n_atoms, 3))
+    v = np.zeros((3, 3))
+    lib().orc_gap_soap_ef(ns, dim, len(Qs), _d(q), _d(_f64(soap_der)), _d(_f64(alphas)),
+                          C.c_double(delta), C.c_double(zeta), C.c_double(e0), _d(_f64(Qs)),
+                          _i(np.ascontiguousarray(n_neigh, np.int32)),
+                          _i(np.ascontiguousarray(pair_atom, np.int32)), _d(_f64(xyz)),
+                          n_atoms, int(do_forces), _d(e), _d(f), _d(v))
+    return e, f, v.T.copy()
+
+
+def _cmodel(model: GapModel):
+    keep = []
+    cm = CModel()
+    cm.h = model.hypers.to_c()
+    cm.has_soap = 1
+    Qs = _f64(model.Qs)
+    al = _f64(model.alphas)
+    keep += [Qs, al]
+    cm.dim = Qs.shape[1]
+    cm.n_sparse = Qs.shape[0]
+    cm.Qs = _d(Qs)
+    cm.alphas = _d(al)
+    cm.delta = model.delta
+    cm.zeta = model.zeta
+    e0 = model.e0 if model.e0 is not None else np.zeros(model.hypers.n_species)
+    for s, v in enumerate(e0):
+        cm.e0[s] = float(v)
+    arr = (C2b * max(1, len(model.two_body)))()
+    for k, b in enumerate(model.two_body):
+        q, a, c = _f64(b.Qs), _f64(b.alphas), _f64(b.cutoff)
+        keep += [q, a, c]
+        arr[k].sp1, arr[k].sp2, arr[k].n_sparse = b.sp1, b.sp2, len(q)
+        arr[k].rcut, arr[k].buffer, arr[k].delta, arr[k].sigma = b.rcut, b.buffer, b.delta, b.sigma
+        arr[k].Qs, arr[k].alphas, arr[k].cutoff = _d(q), _d(a), _d(c)
+    keep.append(arr)
+    cm.n2b = len(model.two_body)
+    cm.b2 = C.cast(arr, C.POINTER(C2b))
+    return cm, keep
+
+
+def gap_2b(b, pos, species, lattice, do_forces=True):
+    """get_2b_energy_and_forces (gap.f90:329-460) on the oracle's list."""
+    nb = neighbors(pos, lattice, b.rcut)
+    n = len(pos)
+    arr = C2b()
+    q, a, c = _f64(b.Qs), _f64(b.alphas), _f64(b.cutoff)
+    arr.sp1, arr.sp2, arr.n_sparse = b.sp1, b.sp2, len(q)
+    arr.rcut, arr.buffer, arr.delta, arr.sigma = b.rcut, b.buffer, b.delta, b.sigma
+    arr.Qs, arr.alphas, arr.cutoff = _d(q), _d(a), _d(c)
+    e = np.zeros(n)
+    f = np.zeros((n, 3))
+    v = np.zeros((3, 3))
+    sp = np.ascontiguousarray(species, np.int32)
+    lib().orc_gap_2b_ef(C.byref(arr), n, _i(sp), _i(nb["n_neigh"]), _i(nb["list"]),
+                        _d(nb["rjs"]), _d(_f64(nb["xyz"])), int(do_forces), _d(e), _d(f), _d(v))
+    return e, f, v.T.copy()
+
+
+def energy_forces(model: GapModel, pos, species, lattice, do_forces=True, chunk_sites=0,
+                  soap=True):
+    """Whole path for one structure (calculate_gap_soap + calculate_gap_2b + e0)."""
+    pos, lat = _f64(pos), _f64(lattice)
+    sp = np.ascontiguousarray(species, np.int32)
+    n = len(pos)
+    cm, keep = _cmodel(model)
+    cm.has_soap = 1 if soap else 0
+    e = np.zeros(n)
+    f = np.zeros((n, 3))
+    v = np.zeros((3, 3))
+    rc = lib().orc_energy_forces(C.byref(cm), n, _d(pos), _i(sp), _d(lat), int(do_forces),
+                                 chunk_sites, _d(e), _d(f), _d(v))
+    if rc != 0:
+        raise RuntimeError(f"orc_energy_forces failed {rc}")
+    return e, f, v.T.copy()
+
+
+def velocity_verlet(pos, pos_prev, vel, forces, forces_prev, masses_ev, dt, dt_prev,
+                    first_step):
+    """velocity_verlet (md_utils.f90:486-532), arrays updated in place."""
+    dtp = C.c_double(dt_prev)
+    lib().orc_velocity_verlet(len(pos), _d(pos), _d(pos_prev), _d(vel), _d(_f64(forces)),
+                              _d(forces_prev), _d(_f64(masses_ev)), C.c_double(dt),
+                              C.byref(dtp), int(first_step))
+    return dtp.value
+
+
+def mc_accept(e_new, e_prev, temp):
+    fn = lib().orc_mc_accept
+    fn.restype = C.c_double
+    return fn(C.c_double(e_new), C.c_double(e_prev), C.c_double(temp))

@@ -1,0 +1,1118 @@
+/*
+ * tgx_oracle.c - FP64 CPU oracle of the GAP/SOAP energy-and-force path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker the HIP library is compared with.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+ * it; the product never calls it.  Built with -ffp-contract=off so that every
+ * FP64 expression rounds as written.
+ *
+ * What each part restates and what pins it (tests/test_oracle_*.py):
+ *   orc_mic / orc_cells_for_cutoff / orc_neighbors
+ *       neighbors/neighbors_interface.f90:144-524 (cell list, self first),
+ *       :547-581 (get_distance, gfortran MODULO = fmod + sign fix),
+ *       :655-708 (cells for cutoff), xyz/read_xyz.f90:323-373 (supercell).
+ *       Pinned by: linked cells == brute force == numpy brute force with
+ *       explicit images; permutation/translation invariance.
+ *   orc_poly3_overlap / orc_radial / orc_radial_raw / orc_sph / orc_soap
+ *       the soap_turbo expansion (third-party, absent from the reference:
+ *       libAtoms/soap_turbo, pinned version unknown; call sites
+ *       gap/gap_interface.f90:175,301-306).  Restated from the published
+ *       method (Caro, PRB 100, 024112 (2019)): poly3 radial basis
+ *       orthonormalised by W = S^(-1/2), Gaussian radial smearing, heat-kernel
+ *       angular smearing, polynomial amplitude scaling and soft cutoff, power
+ *       spectrum over n<=n' with sqrt(2) multiplicity, L2 normalisation.
+ *       Exact soap_turbo conventions: PARITY UNPINNED (no vector exists).
+ *       Pinned against mathematics: S vs quadrature, W S W = I, raw radial
+ *       integrals vs scipy quad, real harmonics vs scipy sph_harm_y, rotation
+ *       and permutation invariance, |q| = 1, finite-difference derivatives,
+ *       translation sum rule.
+ *   orc_gap_kernel_step / orc_gap_soap_ef
+ *       gap/gap.f90:39-244.  Pinned by numpy closed form (einsum) and by
+ *       finite differences of the whole path (F = -dE/dr, virial =
+ *       -dE/dstrain).
+ *   orc_gap_2b_ef
+ *       gap/gap.f90:329-460.  Pinned by the dimer closed form and FD.
+ *   orc_velocity_verlet   md/md_utils.f90:486-532 (harmonic dimer test).
+ *   orc_mc_accept         mc/mc_utils.f90:172-180, 215-257.
+ */
+#include "tgx_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define PI 3.14159265358979323846
+
+/* ------------------------------------------------------------------------ */
+/* Neighbours                                                               */
+/* ------------------------------------------------------------------------ */
+
+/* Fortran MODULO(a,p) for reals as gfortran evaluates it: fmod, then add p
+ * when a and p have opposite signs and the remainder is non-zero. */
+static double f_modulo(double a, double p) {
+  double r = fmod(a, p);
+  if (r != 0.0) {
+    if ((a < 0.0) != (p < 0.0)) r = r + p;
+  } else {
+    r = copysign(0.0, p);
+  }
+  return r;
+}
+
+void orc_mic(const double* posi, const double* posj, const double* L,
+             double* dist, double* d) {
+  double d2 = 0.0;
+  for (int k = 0; k < 3; k++) {
+    double t = f_modulo(posj[k] - posi[k], L[k]);
+    if (t > L[k] / 2.0) t = t - L[k];
+    dist[k] = t;
+    d2 = d2 + t * t;
+  }
+  *d = sqrt(d2);
+}
+
+static void cross3(const double* a, const double* b, double* o) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+static double dot3(const double* a, const double* b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+void orc_cells_for_cutoff(const double* lat, double rcut, int* indices) {
+  const double *a = lat, *b = lat + 3, *c = lat + 6;
+  double axb[3], bxc[3], cxa[3], m[3][3], mi[3][3], md, v[3];
+  cross3(a, b, axb);
+  double t = sqrt(dot3(axb, axb));
+  for (int k = 0; k < 3; k++) axb[k] /= t;
+  cross3(c, a, cxa);
+  t = sqrt(dot3(cxa, cxa));
+  for (int k = 0; k < 3; k++) cxa[k] /= t;
+  cross3(b, c, bxc);
+  t = sqrt(dot3(bxc, bxc));
+  for (int k = 0; k < 3; k++) bxc[k] /= t;
+  const double* rows[3] = {axb, cxa, bxc};
+  const double* cols[3] = {a, b, c};
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) m[i][j] = dot3(rows[i], cols[j]);
+  md = -m[0][2] * m[1][1] * m[2][0] + m[0][1] * m[1][2] * m[2][0] +
+       m[0][2] * m[1][0] * m[2][1] - m[0][0] * m[1][2] * m[2][1] -
+       m[0][1] * m[1][0] * m[2][2] + m[0][0] * m[1][1] * m[2][2];
+  mi[0][0] = m[1][1] * m[2][2] - m[1][2] * m[2][1];
+  mi[0][1] = m[0][2] * m[2][1] - m[0][1] * m[2][2];
+  mi[0][2] = m[0][1] * m[1][2] - m[0][2] * m[1][1];
+  mi[1][0] = m[1][2] * m[2][0] - m[1][0] * m[2][2];
+  mi[1][1] = m[0][0] * m[2][2] - m[0][2] * m[2][0];
+  mi[1][2] = m[0][2] * m[1][0] - m[0][0] * m[1][2];
+  mi[2][0] = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+  mi[2][1] = m[0][1] * m[2][0] - m[0][0] * m[2][1];
+  mi[2][2] = m[0][0] * m[1][1] - m[0][1] * m[1][0];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) mi[i][j] = mi[i][j] / md;
+  for (int i = 0; i < 3; i++) {
+    v[i] = 0.0;
+    for (int j = 0; j < 3; j++) v[i] += mi[i][j] * (2.0 * rcut);
+    v[i] = fabs(v[i]);
+    indices[i] = (int)v[i];
+    if (v[i] > (double)indices[i]) indices[i] += 1;
+  }
+}
+
+static int imod(int a, int m) {
+  int r = a % m;
+  return r < 0 ? r + m : r;
+}
+
+static int cmp_int(const void* a, const void* b) {
+  int x = *(const int*)a, y = *(const int*)b;
+  return (x > y) - (x < y);
+}
+
+int orc_neighbors(int n, const double* pos, const double* lat, double rcut,
+                  int algo, int64_t cap, int* n_neigh, int* list, double* rjs,
+                  double* xyz, double* th, double* ph, int64_t* n_pairs,
+                  int* indices) {
+  const double dtol = 1e-6;
+  if (!(fabs(lat[1]) < dtol && fabs(lat[2]) < dtol && fabs(lat[3]) < dtol &&
+        fabs(lat[5]) < dtol && fabs(lat[6]) < dtol && fabs(lat[7]) < dtol))
+    return TGX_ERR_UNSUPPORTED;
+  double A[9] = {lat[0], 0, 0, 0, lat[4], 0, 0, 0, lat[8]};
+  int idx[3];
+  orc_cells_for_cutoff(A, rcut, idx);
+  if (indices) memcpy(indices, idx, sizeof(idx));
+  int small = idx[0] > 1 || idx[1] > 1 || idx[2] > 1;
+  int nrep = idx[0] * idx[1] * idx[2];
+  int nsc = n * nrep;
+  double* P = (double*)malloc(sizeof(double) * 3 * (size_t)nsc);
+  double L[3];
+  if (small) {
+    int cnt = 0;
+    for (int i2 = 0; i2 < idx[0]; i2++)
+      for (int j2 = 0; j2 < idx[1]; j2++)
+        for (int k2 = 0; k2 < idx[2]; k2++)
+          for (int i = 0; i < n; i++) {
+            for (int k = 0; k < 3; k++)
+              P[3 * cnt + k] = pos[3 * i + k] + (double)i2 * A[k] +
+                               (double)j2 * A[3 + k] + (double)k2 * A[6 + k];
+            cnt++;
+          }
+    L[0] = (double)idx[0] * A[0];
+    L[1] = (double)idx[1] * A[4];
+    L[2] = (double)idx[2] * A[8];
+  } else {
+    memcpy(P, pos, sizeof(double) * 3 * (size_t)n);
+    L[0] = A[0];
+    L[1] = A[4];
+    L[2] = A[8];
+  }
+
+  /* Candidate lists per atom (supercell ids, 0-based), sorted. */
+  int** nb = (int**)calloc((size_t)n, sizeof(int*));
+  int* cnt = (int*)calloc((size_t)n, sizeof(int));
+  int use_cells = (algo == 0) && !small;
+  int mx = 0, my = 0, mz = 0;
+  int *head = NULL, *next = NULL;
+  if (use_cells) {
+    const double tol = 1e-10;
+    mx = (int)(L[0] / rcut);
+    my = (int)(L[1] / rcut);
+    mz = (int)(L[2] / rcut);
+    head = (int*)malloc(sizeof(int) * (size_t)mx * my * mz);
+    next = (int*)malloc(sizeof(int) * (size_t)n);
+    for (int c = 0; c < mx * my * mz; c++) head[c] = -1;
+    double ctr[3] = {L[0] / 2.0, L[1] / 2.0, L[2] / 2.0};
+    for (int i = 0; i < n; i++) {
+      double dd[3], d;
+      orc_mic(ctr, P + 3 * i, L, dd, &d);
+      for (int k = 0; k < 3; k++) dd[k] = dd[k] + ctr[k];
+      int cx = imod((int)(dd[0] / (L[0] + tol) * mx), mx);
+      int cy = imod((int)(dd[1] / (L[1] + tol) * my), my);
+      int cz = imod((int)(dd[2] / (L[2] + tol) * mz), mz);
+      int c = cx + cy * mx + cz * mx * my;
+      next[i] = head[c];
+      head[c] = i;
+    }
+  }
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 0; i < n; i++) {
+    int capi = 64, ni = 0;
+    int* buf = (int*)malloc(sizeof(int) * capi);
+    if (use_cells) {
+      const double tol = 1e-10;
+      double ctr[3] = {L[0] / 2.0, L[1] / 2.0, L[2] / 2.0};
+      double dd[3], d;
+      orc_mic(ctr, P + 3 * i, L, dd, &d);
+      for (int k = 0; k < 3; k++) dd[k] = dd[k] + ctr[k];
+      int i2 = (int)(dd[0] / (L[0] + tol) * mx);
+      int j2 = (int)(dd[1] / (L[1] + tol) * my);
+      int k2 = (int)(dd[2] / (L[2] + tol) * mz);
+      int seen[27], nseen = 0;
+      for (int k3 = k2 - 1; k3 <= k2 + 1; k3++)
+        for (int j3 = j2 - 1; j3 <= j2 + 1; j3++)
+          for (int i3 = i2 - 1; i3 <= i2 + 1; i3++) {
+            int c = imod(i3, mx) + imod(j3, my) * mx + imod(k3, mz) * mx * my;
+            int dup = 0;
+            for (int q = 0; q < nseen; q++) dup |= (seen[q] == c);
+            if (dup) continue;
+            seen[nseen++] = c;
+            for (int k = head[c]; k >= 0; k = next[k]) {
+              if (k == i) continue;
+              double dv[3], dk;
+              orc_mic(P + 3 * i, P + 3 * k, L, dv, &dk);
+              if (dk < rcut) {
+                if (ni == capi) buf = (int*)realloc(buf, sizeof(int) * (capi *= 2));
+                buf[ni++] = k;
+              }
+            }
+          }
+    } else {
+      for (int k = 0; k < nsc; k++) {
+        if (k == i) continue;
+        double dv[3], dk;
+        orc_mic(P + 3 * i, P + 3 * k, L, dv, &dk);
+        if (dk < rcut) {
+          if (ni == capi) buf = (int*)realloc(buf, sizeof(int) * (capi *= 2));
+          buf[ni++] = k;
+        }
+      }
+    }
+    qsort(buf, (size_t)ni, sizeof(int), cmp_int);
+    nb[i] = buf;
+    cnt[i] = ni;
+  }
+  int64_t tot = 0;
+  for (int i = 0; i < n; i++) tot += cnt[i] + 1;
+  *n_pairs = tot;
+  int rc = TGX_OK;
+  if (tot > cap) {
+    rc = TGX_ERR_CAPACITY;
+  } else {
+    int64_t* off = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 1));
+    off[0] = 0;
+    for (int i = 0; i < n; i++) off[i + 1] = off[i] + cnt[i] + 1;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+      int64_t k2 = off[i];
+      n_neigh[i] = cnt[i] + 1;
+      list[k2] = i + 1;
+      if (rjs) rjs[k2] = 0.0;
+      if (xyz) xyz[3 * k2] = xyz[3 * k2 + 1] = xyz[3 * k2 + 2] = 0.0;
+      if (th) th[k2] = 0.0;
+      if (ph) ph[k2] = 0.0;
+      for (int q = 0; q < cnt[i]; q++) {
+        int64_t kk = k2 + 1 + q;
+        int j = nb[i][q];
+        double dv[3], d;
+        orc_mic(P + 3 * i, P + 3 * j, L, dv, &d);
+        list[kk] = j + 1;
+        if (rjs) rjs[kk] = d;
+        if (xyz) {
+          xyz[3 * kk] = dv[0];
+          xyz[3 * kk + 1] = dv[1];
+          xyz[3 * kk + 2] = dv[2];
+        }
+        if (th) {
+          if (fabs(dv[2]) >= d)
+            th[kk] = dv[2] > 0.0 ? 0.0 : acos(-1.0);
+          else
+            th[kk] = acos(dv[2] / d);
+        }
+        if (ph) ph[kk] = atan2(dv[1], dv[0]);
+      }
+    }
+    free(off);
+  }
+  for (int i = 0; i < n; i++) free(nb[i]);
+  free(nb);
+  free(cnt);
+  free(P);
+  free(head);
+  free(next);
+  return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Radial basis                                                             */
+/* ------------------------------------------------------------------------ */
+
+/* W = S^(-1/2) is needed to full FP64 accuracy although cond(S) ~ 1e13 for
+ * n_max = 8: the eigen-decomposition runs in binary128 (gcc __float128) and
+ * only the result is rounded to double. */
+typedef __float128 q128;
+
+static q128 q_sqrt(q128 v) {
+  if (v <= 0) return 0;
+  q128 x = (q128)sqrt((double)v);
+  for (int it = 0; it < 4; it++) x = (x + v / x) / 2;
+  return x;
+}
+static q128 q_abs(q128 v) { return v < 0 ? -v : v; }
+
+static void jacobi_eig_q(int n, q128* a, q128* v, q128* w) {
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) v[i * n + j] = (i == j);
+  for (int sweep = 0; sweep < 100; sweep++) {
+    q128 off = 0, diag = 0;
+    for (int p = 0; p < n; p++) {
+      diag += a[p * n + p] * a[p * n + p];
+      for (int q = p + 1; q < n; q++) off += a[p * n + q] * a[p * n + q];
+    }
+    if (off <= diag * (q128)1e-66) break;
+    for (int p = 0; p < n; p++)
+      for (int q = p + 1; q < n; q++) {
+        q128 apq = a[p * n + q];
+        if (apq == 0) continue;
+        q128 theta = (a[q * n + q] - a[p * n + p]) / (2 * apq);
+        q128 t = (theta >= 0 ? 1 : -1) / (q_abs(theta) + q_sqrt(theta * theta + 1));
+        q128 c = 1 / q_sqrt(t * t + 1), s = t * c;
+        for (int k = 0; k < n; k++) {
+          q128 akp = a[k * n + p], akq = a[k * n + q];
+          a[k * n + p] = c * akp - s * akq;
+          a[k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; k++) {
+          q128 apk = a[p * n + k], aqk = a[q * n + k];
+          a[p * n + k] = c * apk - s * aqk;
+          a[q * n + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; k++) {
+          q128 vkp = v[k * n + p], vkq = v[k * n + q];
+          v[k * n + p] = c * vkp - s * vkq;
+          v[k * n + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  for (int i = 0; i < n; i++) w[i] = a[i * n + i];
+}
+
+void orc_poly3_overlap(int nmax, double* S, double* W) {
+  /* phi_a(r) = sqrt((2a+5)/rh^(2a+5)) (rh-r)^(a+2), a = 1..nmax, on [0,rh]:
+   * S_ab = sqrt((2a+5)(2b+5)) / (a+b+5), independent of rh. */
+  q128 A[TGX_MAX_NR * TGX_MAX_NR], V[TGX_MAX_NR * TGX_MAX_NR], w[TGX_MAX_NR];
+  for (int a = 1; a <= nmax; a++)
+    for (int b = 1; b <= nmax; b++) {
+      q128 s = q_sqrt((q128)((2 * a + 5) * (2 * b + 5))) / (q128)(a + b + 5);
+      S[(a - 1) * nmax + (b - 1)] = (double)s;
+      A[(a - 1) * nmax + (b - 1)] = s;
+    }
+  jacobi_eig_q(nmax, A, V, w);
+  for (int i = 0; i < nmax; i++)
+    for (int j = 0; j < nmax; j++) {
+      q128 acc = 0;
+      for (int k = 0; k < nmax; k++) acc += V[i * nmax + k] * V[j * nmax + k] / q_sqrt(w[k]);
+      W[i * nmax + j] = (double)acc;
+    }
+}
+
+/* J_k = int_0^rh u^k exp(-(u-c)^2/(2 s^2)) du, c = rh - r, k = 0..kmax. */
+static void radial_J(double r, double rh, double s, int kmax, double* J) {
+  double c = rh - r, s2 = s * s;
+  double er = exp(-r * r / (2.0 * s2)), ec = exp(-c * c / (2.0 * s2));
+  J[0] = s * sqrt(PI / 2.0) *
+         (erf(r / (s * sqrt(2.0))) + erf(c / (s * sqrt(2.0))));
+  if (kmax >= 1) J[1] = c * J[0] + s2 * (ec - er);
+  double rhp = 1.0; /* rh^(k-1) */
+  for (int k = 2; k <= kmax; k++) {
+    rhp *= rh;
+    J[k] = c * J[k - 1] + (k - 1) * s2 * J[k - 2] - s2 * rhp * er;
+  }
+}
+
+/* T_k for poly degree d = k+3 and radial weight rho^p, from J. */
+static double poly_T(const double* J, int d, int p, double rh) {
+  if (p == 0) return J[d];
+  if (p == 1) return rh * J[d] - J[d + 1];
+  return rh * rh * J[d] - 2.0 * rh * J[d + 1] + J[d + 2];
+}
+
+void orc_radial_raw(double r, double rh, double sig, int p, int nk,
+                    double* out) {
+  double J[64];
+  radial_J(r, rh, sig, nk + 2 + p + 2, J);
+  double pre = 1.0 / (sqrt(2.0 * PI) * sig);
+  for (int k = 0; k < nk; k++) out[k] = pre * poly_T(J, k + 3, p, rh);
+}
+
+/* Unnormalised (no W) basis integrals I_a(r) (a = 1..nmax) and dI_a/dr for
+ * a Gaussian of width sig0 + sigs*r centred at r. */
+static void radial_I(double r, double rh, double sig0, double sigs, int p,
+                     int nmax, double* I, double* dI) {
+  double s = sig0 + sigs * r;
+  double J[64];
+  int kmax = nmax + 2 + p + 2;
+  radial_J(r, rh, s, kmax, J);
+  double c = rh - r, s2 = s * s;
+  double pre = 1.0 / (sqrt(2.0 * PI) * s);
+  double dJc[64], dJs[64];
+  for (int k = 0; k + 2 <= kmax; k++) {
+    dJc[k] = (J[k + 1] - c * J[k]) / s2;
+    dJs[k] = (J[k + 2] - 2.0 * c * J[k + 1] + c * c * J[k]) / (s2 * s);
+  }
+  for (int a = 1; a <= nmax; a++) {
+    int d = a + 2;
+    double Na = sqrt((2.0 * a + 5.0) / pow(rh, 2.0 * a + 5.0));
+    double T = poly_T(J, d, p, rh);
+    double Tc = poly_T(dJc, d, p, rh);
+    double Ts = poly_T(dJs, d, p, rh);
+    I[a - 1] = Na * pre * T;
+    if (dI)
+      dI[a - 1] = Na * (pre * (-Tc + Ts * sigs) - pre / s * sigs * T);
+  }
+}
+
+/* Amplitude (polynomial scaling, global scaling) times soft cutoff. */
+static void radial_envelope(const tgx_soap_hypers* h, int s, double r,
+                            double* e, double* de) {
+  double rh = h->rcut_hard[s], rs = h->rcut_soft[s];
+  double x = r / rh;
+  double poly = 1.0 + 2.0 * x * x * x - 3.0 * x * x;
+  double dpoly = (6.0 * x * x - 6.0 * x) / rh;
+  double as = h->amplitude_scaling[s];
+  double amp, damp;
+  if (as == 0.0) {
+    amp = 1.0;
+    damp = 0.0;
+  } else if (as == 1.0) {
+    amp = poly;
+    damp = dpoly;
+  } else {
+    amp = pow(poly, as);
+    damp = poly > 0.0 ? as * pow(poly, as - 1.0) * dpoly : 0.0;
+  }
+  double fc = 1.0, dfc = 0.0;
+  if (r >= rh) {
+    fc = 0.0;
+  } else if (rs < rh && r > rs) {
+    double y = (r - rs) / (rh - rs);
+    fc = 1.0 - 3.0 * y * y + 2.0 * y * y * y;
+    dfc = (-6.0 * y + 6.0 * y * y) / (rh - rs);
+  }
+  double g = h->global_scaling[s];
+  *e = g * amp * fc;
+  *de = g * (damp * fc + amp * dfc);
+}
+
+void orc_radial(const tgx_soap_hypers* h, int s, int nr, const double* r,
+                double* R, double* dR) {
+  int nmax = h->alpha_max[s];
+  double S[TGX_MAX_NR * TGX_MAX_NR], W[TGX_MAX_NR * TGX_MAX_NR];
+  orc_poly3_overlap(nmax, S, W);
+  for (int q = 0; q < nr; q++) {
+    double I[TGX_MAX_NR], dI[TGX_MAX_NR], e, de;
+    double rr = r[q];
+    if (rr >= h->rcut_hard[s]) {
+      for (int n = 0; n < nmax; n++) R[q * nmax + n] = dR[q * nmax + n] = 0.0;
+      continue;
+    }
+    radial_I(rr, h->rcut_hard[s], h->atom_sigma_r[s],
+             h->atom_sigma_r_scaling[s], h->radial_enhancement, nmax, I, dI);
+    radial_envelope(h, s, rr, &e, &de);
+    for (int n = 0; n < nmax; n++) {
+      double v = 0.0, dv = 0.0;
+      for (int a = 0; a < nmax; a++) {
+        v += W[n * nmax + a] * I[a];
+        dv += W[n * nmax + a] * dI[a];
+      }
+      R[q * nmax + n] = e * v;
+      dR[q * nmax + n] = de * v + e * dv;
+    }
+  }
+}
+
+/* Central-atom radial coefficients of species s at r = 0. */
+static void radial_self(const tgx_soap_hypers* h, int s, const double* W,
+                        double* R) {
+  int nmax = h->alpha_max[s];
+  double I[TGX_MAX_NR];
+  radial_I(0.0, h->rcut_hard[s], h->atom_sigma_r[s], 0.0,
+           h->radial_enhancement, nmax, I, NULL);
+  for (int n = 0; n < nmax; n++) {
+    double v = 0.0;
+    for (int a = 0; a < nmax; a++) v += W[n * nmax + a] * I[a];
+    R[n] = h->central_weight[s] * v;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Real spherical harmonics                                                 */
+/* ------------------------------------------------------------------------ */
+
+void orc_sph(int L, const double* u, double* Y, double* dF) {
+  double x = u[0], y = u[1], z = u[2];
+  double C[TGX_MAX_LMAX + 2], S[TGX_MAX_LMAX + 2];
+  double Q[TGX_MAX_LMAX + 1][TGX_MAX_LMAX + 1], Qd[TGX_MAX_LMAX + 1][TGX_MAX_LMAX + 1];
+  C[0] = 1.0;
+  S[0] = 0.0;
+  for (int m = 0; m < L; m++) {
+    C[m + 1] = x * C[m] - y * S[m];
+    S[m + 1] = x * S[m] + y * C[m];
+  }
+  double dfact = 1.0; /* (2m-1)!! */
+  for (int m = 0; m <= L; m++) {
+    if (m > 0) dfact *= (2.0 * m - 1.0);
+    Q[m][m] = dfact;
+    Qd[m][m] = 0.0;
+    if (m + 1 <= L) {
+      Q[m + 1][m] = (2.0 * m + 1.0) * z * dfact;
+      Qd[m + 1][m] = (2.0 * m + 1.0) * dfact;
+    }
+    for (int l = m + 2; l <= L; l++) {
+      Q[l][m] = ((2.0 * l - 1.0) * z * Q[l - 1][m] - (l + m - 1.0) * Q[l - 2][m]) / (l - m);
+      Qd[l][m] = ((2.0 * l - 1.0) * (Q[l - 1][m] + z * Qd[l - 1][m]) -
+                  (l + m - 1.0) * Qd[l - 2][m]) / (l - m);
+    }
+  }
+  for (int l = 0; l <= L; l++) {
+    for (int m = 0; m <= l; m++) {
+      double ratio = 1.0; /* (l-m)!/(l+m)! */
+      for (int k = l - m + 1; k <= l + m; k++) ratio /= (double)k;
+      double K = sqrt((2.0 * l + 1.0) / (4.0 * PI) * ratio);
+      if (m > 0) K *= sqrt(2.0);
+      double q = K * Q[l][m], qd = K * Qd[l][m];
+      if (m == 0) {
+        int i = l * l + l;
+        Y[i] = q;
+        if (dF) {
+          dF[3 * i] = 0.0;
+          dF[3 * i + 1] = 0.0;
+          dF[3 * i + 2] = qd;
+        }
+      } else {
+        int ic = l * l + l + m, is = l * l + l - m;
+        Y[ic] = q * C[m];
+        Y[is] = q * S[m];
+        if (dF) {
+          dF[3 * ic] = q * m * C[m - 1];
+          dF[3 * ic + 1] = -q * m * S[m - 1];
+          dF[3 * ic + 2] = qd * C[m];
+          dF[3 * is] = q * m * S[m - 1];
+          dF[3 * is + 1] = q * m * C[m - 1];
+          dF[3 * is + 2] = qd * S[m];
+        }
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* SOAP                                                                     */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+  int nr, L, L2, npair, dim;
+  int off[TGX_MAX_SPECIES], nmax[TGX_MAX_SPECIES];
+  double W[TGX_MAX_SPECIES][TGX_MAX_NR * TGX_MAX_NR];
+} soap_geom;
+
+static void soap_geom_init(const tgx_soap_hypers* h, soap_geom* g) {
+  g->nr = 0;
+  for (int s = 0; s < h->n_species; s++) {
+    g->off[s] = g->nr;
+    g->nmax[s] = h->alpha_max[s];
+    g->nr += h->alpha_max[s];
+    double S[TGX_MAX_NR * TGX_MAX_NR];
+    orc_poly3_overlap(h->alpha_max[s], S, g->W[s]);
+  }
+  g->L = h->l_max;
+  g->L2 = (h->l_max + 1) * (h->l_max + 1);
+  g->npair = g->nr * (g->nr + 1) / 2;
+  g->dim = g->npair * (h->l_max + 1);
+}
+
+/* One site: central species si, np neighbour vectors x (3 each) with species
+ * sj (0-based).  Writes q (dim) and, if dq != NULL, dq[(j*dim + d)*3 + k] =
+ * dq_d / dr_j,k for each neighbour j; the self derivative is -sum_j. */
+static void soap_site(const tgx_soap_hypers* h, const soap_geom* g, int si,
+                      int np, const double* x, const int* sj, double* q,
+                      double* dq) {
+  int NR = g->nr, L = g->L, L2 = g->L2, dim = g->dim;
+  double* c = (double*)calloc((size_t)NR * L2, sizeof(double));
+  double* Rv = (double*)malloc(sizeof(double) * (size_t)np * TGX_MAX_NR);
+  double* dRv = (double*)malloc(sizeof(double) * (size_t)np * TGX_MAX_NR);
+  double* AY = (double*)malloc(sizeof(double) * (size_t)np * L2);
+  double* Yv = (double*)malloc(sizeof(double) * (size_t)np * L2);
+  double* dFv = (double*)malloc(sizeof(double) * (size_t)np * L2 * 3);
+  double* Al = (double*)malloc(sizeof(double) * (size_t)np * (L + 1));
+  double* dAl = (double*)malloc(sizeof(double) * (size_t)np * (L + 1));
+  /* central atom */
+  {
+    double Rs[TGX_MAX_NR];
+    radial_self(h, si, g->W[si], Rs);
+    for (int n = 0; n < g->nmax[si]; n++)
+      c[(g->off[si] + n) * L2 + 0] += Rs[n] * (1.0 / sqrt(4.0 * PI));
+  }
+  for (int j = 0; j < np; j++) {
+    const double* v = x + 3 * j;
+    double r = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    int s = sj[j];
+    orc_radial(h, s, 1, &r, Rv + (size_t)j * TGX_MAX_NR, dRv + (size_t)j * TGX_MAX_NR);
+    double u[3] = {v[0] / r, v[1] / r, v[2] / r};
+    orc_sph(L, u, Yv + (size_t)j * L2, dFv + (size_t)j * L2 * 3);
+    double st = h->atom_sigma_t[s] + h->atom_sigma_t_scaling[s] * r;
+    double dst = h->atom_sigma_t_scaling[s];
+    for (int l = 0; l <= L; l++) {
+      double ex = -0.5 * l * (l + 1.0) * st * st / (r * r);
+      double a = exp(ex);
+      double dex = -0.5 * l * (l + 1.0) * (2.0 * st * dst / (r * r) - 2.0 * st * st / (r * r * r));
+      Al[j * (L + 1) + l] = a;
+      dAl[j * (L + 1) + l] = a * dex;
+    }
+    for (int l = 0; l <= L; l++)
+      for (int m = -l; m <= l; m++) {
+        int lm = l * l + l + m;
+        AY[(size_t)j * L2 + lm] = Al[j * (L + 1) + l] * Yv[(size_t)j * L2 + lm];
+      }
+    for (int n = 0; n < g->nmax[s]; n++) {
+      double Rn = Rv[(size_t)j * TGX_MAX_NR + n];
+      double* cn = c + (size_t)(g->off[s] + n) * L2;
+      for (int lm = 0; lm < L2; lm++) cn[lm] += Rn * AY[(size_t)j * L2 + lm];
+    }
+  }
+  /* power spectrum */
+  double* p = (double*)calloc((size_t)dim, sizeof(double));
+  int k = 0;
+  for (int n = 0; n < NR; n++)
+    for (int n2 = n; n2 < NR; n2++) {
+      double w = (n == n2) ? 1.0 : sqrt(2.0);
+      for (int l = 0; l <= L; l++) {
+        double acc = 0.0;
+        for (int m = -l; m <= l; m++) {
+          int lm = l * l + l + m;
+          acc += c[n * L2 + lm] * c[n2 * L2 + lm];
+        }
+        p[k * (L + 1) + l] = w * acc;
+      }
+      k++;
+    }
+  double nrm = 0.0;
+  for (int d = 0; d < dim; d++) nrm += p[d] * p[d];
+  nrm = sqrt(nrm);
+  for (int d = 0; d < dim; d++) q[d] = p[d] / nrm;
+
+  if (dq) {
+    double* dc = (double*)malloc(sizeof(double) * TGX_MAX_NR * L2 * 3);
+    double* dp = (double*)malloc(sizeof(double) * (size_t)dim * 3);
+    for (int j = 0; j < np; j++) {
+      const double* v = x + 3 * j;
+      double r = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+      double u[3] = {v[0] / r, v[1] / r, v[2] / r};
+      int s = sj[j], nm = g->nmax[s], o = g->off[s];
+      /* dc[(n_local*L2 + lm)*3 + k] */
+      for (int n = 0; n < nm; n++) {
+        double Rn = Rv[(size_t)j * TGX_MAX_NR + n], dRn = dRv[(size_t)j * TGX_MAX_NR + n];
+        for (int l = 0; l <= L; l++) {
+          double A = Al[j * (L + 1) + l], dA = dAl[j * (L + 1) + l];
+          for (int m = -l; m <= l; m++) {
+            int lm = l * l + l + m;
+            double Y = Yv[(size_t)j * L2 + lm];
+            const double* dF = dFv + ((size_t)j * L2 + lm) * 3;
+            double udF = u[0] * dF[0] + u[1] * dF[1] + u[2] * dF[2];
+            double radial = (dRn * A + Rn * dA) * Y;
+            double tang = Rn * A / r;
+            for (int kk = 0; kk < 3; kk++)
+              dc[((size_t)n * L2 + lm) * 3 + kk] = radial * u[kk] + tang * (dF[kk] - udF * u[kk]);
+          }
+        }
+      }
+      memset(dp, 0, sizeof(double) * (size_t)dim * 3);
+      int kp = 0;
+      for (int n = 0; n < NR; n++)
+        for (int n2 = n; n2 < NR; n2++) {
+          double w = (n == n2) ? 1.0 : sqrt(2.0);
+          int in1 = (n >= o && n < o + nm), in2 = (n2 >= o && n2 < o + nm);
+          if (in1 || in2) {
+            for (int l = 0; l <= L; l++) {
+              double acc[3] = {0, 0, 0};
+              for (int m = -l; m <= l; m++) {
+                int lm = l * l + l + m;
+                for (int kk = 0; kk < 3; kk++) {
+                  if (in1) acc[kk] += dc[((size_t)(n - o) * L2 + lm) * 3 + kk] * c[n2 * L2 + lm];
+                  if (in2) acc[kk] += c[n * L2 + lm] * dc[((size_t)(n2 - o) * L2 + lm) * 3 + kk];
+                }
+              }
+              for (int kk = 0; kk < 3; kk++) dp[(kp * (L + 1) + l) * 3 + kk] = w * acc[kk];
+            }
+          }
+          kp++;
+        }
+      double qdp[3] = {0, 0, 0};
+      for (int d = 0; d < dim; d++)
+        for (int kk = 0; kk < 3; kk++) qdp[kk] += q[d] * dp[d * 3 + kk];
+      for (int d = 0; d < dim; d++)
+        for (int kk = 0; kk < 3; kk++)
+          dq[((size_t)j * dim + d) * 3 + kk] = (dp[d * 3 + kk] - q[d] * qdp[kk]) / nrm;
+    }
+    free(dc);
+    free(dp);
+  }
+  free(p);
+  free(c);
+  free(Rv);
+  free(dRv);
+  free(AY);
+  free(Yv);
+  free(dFv);
+  free(Al);
+  free(dAl);
+}
+
+static double soap_rcut_max(const tgx_soap_hypers* h) {
+  double r = 0.0;
+  for (int s = 0; s < h->n_species; s++) r = h->rcut_hard[s] > r ? h->rcut_hard[s] : r;
+  return r;
+}
+
+/* Site/pair selection of gap_interface.f90:184-289 on a full neighbour list:
+ * central sites by central_species, pairs with rjs < rcut_max. */
+typedef struct {
+  int n_sites;
+  int64_t n_pairs;
+  int* site_atom;     /* n_sites, 0-based */
+  int* site_nn;       /* pairs per site (self included) */
+  int64_t* site_off;  /* n_sites+1 */
+  int* pair_atom;     /* folded 0-based */
+  double* pair_xyz;   /* 3 per pair */
+} site_pairs;
+
+static void select_sites(const tgx_soap_hypers* h, int n, const int* species,
+                         const int* n_neigh, const int* list,
+                         const double* rjs, const double* xyz, site_pairs* sp) {
+  double rc = soap_rcut_max(h);
+  sp->n_sites = 0;
+  sp->n_pairs = 0;
+  sp->site_atom = (int*)malloc(sizeof(int) * (size_t)n);
+  sp->site_nn = (int*)malloc(sizeof(int) * (size_t)n);
+  sp->site_off = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 1));
+  int64_t k = 0, tot = 0;
+  for (int i = 0; i < n; i++) {
+    if (h->central_species == 0 || species[i] == h->central_species) {
+      int c = 0;
+      for (int q = 0; q < n_neigh[i]; q++)
+        if (rjs[k + q] < rc) c++;
+      sp->site_atom[sp->n_sites] = i;
+      sp->site_nn[sp->n_sites] = c;
+      sp->site_off[sp->n_sites] = tot;
+      tot += c;
+      sp->n_sites++;
+    }
+    k += n_neigh[i];
+  }
+  sp->site_off[sp->n_sites] = tot;
+  sp->n_pairs = tot;
+  sp->pair_atom = (int*)malloc(sizeof(int) * (size_t)(tot > 0 ? tot : 1));
+  sp->pair_xyz = (double*)malloc(sizeof(double) * 3 * (size_t)(tot > 0 ? tot : 1));
+  k = 0;
+  int64_t t = 0;
+  for (int i = 0; i < n; i++) {
+    if (h->central_species == 0 || species[i] == h->central_species) {
+      for (int q = 0; q < n_neigh[i]; q++)
+        if (rjs[k + q] < rc) {
+          sp->pair_atom[t] = (list[k + q] - 1) % n;
+          for (int kk = 0; kk < 3; kk++) sp->pair_xyz[3 * t + kk] = xyz[3 * (k + q) + kk];
+          t++;
+        }
+    }
+    k += n_neigh[i];
+  }
+}
+
+static void free_sites(site_pairs* sp) {
+  free(sp->site_atom);
+  free(sp->site_nn);
+  free(sp->site_off);
+  free(sp->pair_atom);
+  free(sp->pair_xyz);
+}
+
+/* Descriptors and derivatives for sites [s0, s1) of sp. */
+static void soap_chunk(const tgx_soap_hypers* h, const soap_geom* g,
+                       const site_pairs* sp, const int* species, int s0,
+                       int s1, double* soap, double* soap_der) {
+  int dim = g->dim;
+  int64_t pbase = sp->site_off[s0];
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int st = s0; st < s1; st++) {
+    int64_t o = sp->site_off[st];
+    int nn = sp->site_nn[st];
+    int np = nn - 1; /* exclude self */
+    int si = species[sp->site_atom[st]] - 1;
+    int* sj = (int*)malloc(sizeof(int) * (size_t)(np > 0 ? np : 1));
+    for (int j = 0; j < np; j++) sj[j] = species[sp->pair_atom[o + 1 + j]] - 1;
+    double* q = soap + (size_t)(st - s0) * dim;
+    double* dq = NULL;
+    if (soap_der) dq = (double*)malloc(sizeof(double) * (size_t)(np > 0 ? np : 1) * dim * 3);
+    soap_site(h, g, si, np, sp->pair_xyz + 3 * (o + 1), sj, q, dq);
+    if (soap_der) {
+      double* self = soap_der + (size_t)(o - pbase) * dim * 3;
+      memset(self, 0, sizeof(double) * (size_t)dim * 3);
+      for (int j = 0; j < np; j++) {
+        double* dst = soap_der + (size_t)(o - pbase + 1 + j) * dim * 3;
+        memcpy(dst, dq + (size_t)j * dim * 3, sizeof(double) * (size_t)dim * 3);
+        for (int d = 0; d < dim * 3; d++) self[d] -= dst[d];
+      }
+      free(dq);
+    }
+    free(sj);
+  }
+}
+
+int orc_soap(const tgx_soap_hypers* h, int n, const double* pos,
+             const double* lat, const int* species, double* soap,
+             double* soap_der, int64_t pair_cap, int* n_sites,
+             int64_t* n_pairs, int* site_atom, int* pair_atom,
+             double* pair_xyz, int* site_n_neigh) {
+  double rc = soap_rcut_max(h);
+  int64_t np = 0;
+  int rc0 = orc_neighbors(n, pos, lat, rc, 0, 0, NULL, NULL, NULL, NULL, NULL, NULL, &np, NULL);
+  if (rc0 != TGX_OK && rc0 != TGX_ERR_CAPACITY) return rc0;
+  int* nn = (int*)malloc(sizeof(int) * (size_t)n);
+  int* list = (int*)malloc(sizeof(int) * (size_t)np);
+  double* rjs = (double*)malloc(sizeof(double) * (size_t)np);
+  double* xyz = (double*)malloc(sizeof(double) * 3 * (size_t)np);
+  orc_neighbors(n, pos, lat, rc, 0, np, nn, list, rjs, xyz, NULL, NULL, &np, NULL);
+  site_pairs sp;
+  select_sites(h, n, species, nn, list, rjs, xyz, &sp);
+  soap_geom g;
+  soap_geom_init(h, &g);
+  *n_sites = sp.n_sites;
+  *n_pairs = sp.n_pairs;
+  int rcode = TGX_OK;
+  if (soap_der && sp.n_pairs > pair_cap) {
+    rcode = TGX_ERR_CAPACITY;
+  } else {
+    soap_chunk(h, &g, &sp, species, 0, sp.n_sites, soap, soap_der);
+    if (site_atom) memcpy(site_atom, sp.site_atom, sizeof(int) * (size_t)sp.n_sites);
+    if (site_n_neigh) memcpy(site_n_neigh, sp.site_nn, sizeof(int) * (size_t)sp.n_sites);
+    if (pair_atom && sp.n_pairs <= pair_cap)
+      memcpy(pair_atom, sp.pair_atom, sizeof(int) * (size_t)sp.n_pairs);
+    if (pair_xyz && sp.n_pairs <= pair_cap)
+      memcpy(pair_xyz, sp.pair_xyz, sizeof(double) * 3 * (size_t)sp.n_pairs);
+  }
+  free_sites(&sp);
+  free(nn);
+  free(list);
+  free(rjs);
+  free(xyz);
+  return rcode;
+}
+
+/* ------------------------------------------------------------------------ */
+/* GAP kernel step                                                          */
+/* ------------------------------------------------------------------------ */
+
+static double ipow(double x, int e) {
+  double r = 1.0;
+  for (int i = 0; i < e; i++) r *= x;
+  return r;
+}
+
+void orc_gap_kernel_step(int n_sites, int dim, int n_sparse,
+                         const double* soap, const double* Qs,
+                         const double* alphas, double delta, double zeta0,
+                         double* energies, double* qss) {
+  int is_int = fabs(zeta0 - (double)(int)zeta0) < 1e-5;
+  int zi = (int)zeta0;
+  double zeta = is_int ? (double)zi : zeta0;
+  double* kd = (double*)malloc(sizeof(double) * (size_t)(n_sparse > 0 ? n_sparse : 1));
+#pragma omp parallel for schedule(static) firstprivate(kd)
+  for (int i = 0; i < n_sites; i++) {
+    double* kr = (double*)malloc(sizeof(double) * (size_t)(n_sparse > 0 ? n_sparse : 1));
+    const double* qi = soap + (size_t)i * dim;
+    double e = 0.0;
+    for (int s = 0; s < n_sparse; s++) {
+      const double* qs = Qs + (size_t)s * dim;
+      double k = 0.0;
+      for (int d = 0; d < dim; d++) k += qi[d] * qs[d];
+      double kz = is_int ? ipow(k, zi) : pow(k, zeta);
+      e += kz * alphas[s];
+      kr[s] = (is_int ? ipow(k, zi - 1) : pow(k, zeta - 1.0)) * alphas[s];
+    }
+    energies[i] = delta * delta * e;
+    if (qss) {
+      for (int d = 0; d < dim; d++) {
+        double acc = 0.0;
+        for (int s = 0; s < n_sparse; s++) acc += kr[s] * Qs[(size_t)s * dim + d];
+        qss[(size_t)d * n_sites + i] = -zeta * delta * delta * acc;
+      }
+    }
+    free(kr);
+  }
+  free(kd);
+}
+
+void orc_gap_soap_ef(int n_sites, int dim, int n_sparse, const double* soap,
+                     const double* soap_der, const double* alphas,
+                     double delta, double zeta, double e0, const double* Qs,
+                     const int* n_neigh, const int* pair_atom,
+                     const double* xyz, int n_atoms, int do_forces,
+                     double* energies, double* forces, double* virial) {
+  double* qss = do_forces ? (double*)malloc(sizeof(double) * (size_t)n_sites * dim) : NULL;
+  orc_gap_kernel_step(n_sites, dim, n_sparse, soap, Qs, alphas, delta, zeta, energies, qss);
+  for (int i = 0; i < n_sites; i++) energies[i] += e0;
+  if (!do_forces) return;
+  memset(forces, 0, sizeof(double) * 3 * (size_t)n_atoms);
+  memset(virial, 0, sizeof(double) * 9);
+  int64_t* off = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n_sites + 1));
+  off[0] = 0;
+  for (int i = 0; i < n_sites; i++) off[i + 1] = off[i] + n_neigh[i];
+  int nth = 1;
+#ifdef _OPENMP
+  nth = omp_get_max_threads();
+#endif
+  double* fth = (double*)calloc((size_t)nth * 3 * n_atoms, sizeof(double));
+  double* vth = (double*)calloc((size_t)nth * 9, sizeof(double));
+#pragma omp parallel
+  {
+    int t = 0;
+#ifdef _OPENMP
+    t = omp_get_thread_num();
+#endif
+    double* F = fth + (size_t)t * 3 * n_atoms;
+    double* V = vth + (size_t)t * 9;
+#pragma omp for schedule(dynamic, 8)
+    for (int i = 0; i < n_sites; i++) {
+      for (int64_t l = off[i]; l < off[i + 1]; l++) {
+        int j2 = pair_atom[l];
+        double f[3];
+        for (int k = 0; k < 3; k++) {
+          double acc = 0.0;
+          for (int d = 0; d < dim; d++)
+            acc += qss[(size_t)d * n_sites + i] * soap_der[((size_t)l * dim + d) * 3 + k];
+          f[k] = acc;
+          F[3 * j2 + k] += acc;
+        }
+        for (int k1 = 0; k1 < 3; k1++)
+          for (int k2 = 0; k2 < 3; k2++)
+            V[k2 * 3 + k1] += 0.5 * (f[k1] * xyz[3 * l + k2] + f[k2] * xyz[3 * l + k1]);
+      }
+    }
+  }
+  for (int t = 0; t < nth; t++) {
+    for (size_t a = 0; a < (size_t)3 * n_atoms; a++) forces[a] += fth[(size_t)t * 3 * n_atoms + a];
+    for (int a = 0; a < 9; a++) virial[a] += vth[t * 9 + a];
+  }
+  free(fth);
+  free(vth);
+  free(off);
+  free(qss);
+}
+
+/* ------------------------------------------------------------------------ */
+/* 2-body                                                                   */
+/* ------------------------------------------------------------------------ */
+
+void orc_gap_2b_ef(const orc_2b* b, int n, const int* species,
+                   const int* n_neigh, const int* list, const double* rjs,
+                   const double* xyz, int do_forces, double* energies,
+                   double* forces, double* virial) {
+  const double pi = acos(-1.0);
+  double rcut = b->rcut, buffer = b->buffer, delta = b->delta, sigma = b->sigma;
+  int sp1 = b->sp1, sp2 = b->sp2;
+  int64_t* off = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 1));
+  off[0] = 0;
+  for (int i = 0; i < n; i++) off[i + 1] = off[i] + n_neigh[i];
+  if (do_forces) memset(virial, 0, sizeof(double) * 9);
+  double vir[9] = {0};
+  for (int i = 0; i < n; i++) {
+    energies[i] = 0.0;
+    if (do_forces) forces[3 * i] = forces[3 * i + 1] = forces[3 * i + 2] = 0.0;
+    if (species[i] != sp1 && species[i] != sp2) continue;
+    for (int64_t k = off[i] + 1; k < off[i + 1]; k++) {
+      int sj = species[(list[k] - 1) % n];
+      if (!((species[i] == sp1 && sj == sp2) || (species[i] == sp2 && sj == sp1))) continue;
+      double r = rjs[k];
+      if (!(r < rcut)) continue;
+      double fcut, dfcut;
+      if (r < rcut - buffer) {
+        fcut = 1.0;
+        dfcut = 0.0;
+      } else {
+        fcut = (cos(pi * (r - rcut + buffer) / buffer) + 1.0) / 2.0;
+        dfcut = pi / 2.0 / buffer * sin(pi * (r - rcut + buffer) / buffer);
+      }
+      for (int s = 0; s < b->n_sparse; s++) {
+        double ex = exp(-0.5 * (r - b->Qs[s]) * (r - b->Qs[s]) / (sigma * sigma));
+        energies[i] += delta * delta * b->alphas[s] * b->cutoff[s] * fcut * ex;
+        if (do_forces) {
+          double f[3];
+          for (int kk = 0; kk < 3; kk++)
+            f[kk] = -2.0 * delta * delta * b->alphas[s] * b->cutoff[s] * ex * xyz[3 * k + kk] / r *
+                    ((r - b->Qs[s]) / (sigma * sigma) * fcut + dfcut);
+          for (int kk = 0; kk < 3; kk++) forces[3 * i + kk] += f[kk];
+          for (int k1 = 0; k1 < 3; k1++)
+            for (int k2 = 0; k2 < 3; k2++)
+              vir[k2 * 3 + k1] -= 0.5 * (f[k1] * xyz[3 * k + k2] + f[k2] * xyz[3 * k + k1]);
+        }
+      }
+    }
+  }
+  if (do_forces)
+    for (int a = 0; a < 9; a++) virial[a] = 0.5 * vir[a];
+  free(off);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Whole path                                                               */
+/* ------------------------------------------------------------------------ */
+
+int orc_energy_forces(const orc_model* m, int n, const double* pos,
+                      const int* species, const double* lat, int do_forces,
+                      int chunk_sites, double* e_atom, double* forces,
+                      double* virial) {
+  double rc = 0.0;
+  if (m->has_soap) rc = soap_rcut_max(&m->h);
+  for (int b = 0; b < m->n2b; b++) rc = m->b2[b].rcut > rc ? m->b2[b].rcut : rc;
+  int64_t np = 0;
+  int r0 = orc_neighbors(n, pos, lat, rc, 0, 0, NULL, NULL, NULL, NULL, NULL, NULL, &np, NULL);
+  if (r0 != TGX_OK && r0 != TGX_ERR_CAPACITY) return r0;
+  int* nn = (int*)malloc(sizeof(int) * (size_t)n);
+  int* list = (int*)malloc(sizeof(int) * (size_t)np);
+  double* rjs = (double*)malloc(sizeof(double) * (size_t)np);
+  double* xyz = (double*)malloc(sizeof(double) * 3 * (size_t)np);
+  orc_neighbors(n, pos, lat, rc, 0, np, nn, list, rjs, xyz, NULL, NULL, &np, NULL);
+
+  for (int i = 0; i < n; i++) e_atom[i] = m->e0[species[i] - 1];
+  if (do_forces) {
+    memset(forces, 0, sizeof(double) * 3 * (size_t)n);
+    memset(virial, 0, sizeof(double) * 9);
+  }
+  double* tf = (double*)malloc(sizeof(double) * 3 * (size_t)n);
+  double tv[9];
+  if (m->has_soap) {
+    site_pairs sp;
+    select_sites(&m->h, n, species, nn, list, rjs, xyz, &sp);
+    soap_geom g;
+    soap_geom_init(&m->h, &g);
+    int dim = g.dim;
+    int cs = chunk_sites > 0 ? chunk_sites : (sp.n_sites > 0 ? sp.n_sites : 1);
+    for (int s0 = 0; s0 < sp.n_sites; s0 += cs) {
+      int s1 = s0 + cs < sp.n_sites ? s0 + cs : sp.n_sites;
+      int ns = s1 - s0;
+      int64_t p0 = sp.site_off[s0], p1 = sp.site_off[s1];
+      double* soap = (double*)malloc(sizeof(double) * (size_t)ns * dim);
+      double* der = do_forces ? (double*)malloc(sizeof(double) * (size_t)(p1 - p0) * dim * 3) : NULL;
+      soap_chunk(&m->h, &g, &sp, species, s0, s1, soap, der);
+      double* en = (double*)malloc(sizeof(double) * (size_t)ns);
+      orc_gap_soap_ef(ns, dim, m->n_sparse, soap, der, m->alphas, m->delta, m->zeta, 0.0,
+                      m->Qs, sp.site_nn + s0, sp.pair_atom + p0, sp.pair_xyz + 3 * p0, n,
+                      do_forces, en, tf, tv);
+      for (int i = 0; i < ns; i++) e_atom[sp.site_atom[s0 + i]] += en[i];
+      if (do_forces) {
+        for (size_t a = 0; a < (size_t)3 * n; a++) forces[a] += tf[a];
+        for (int a = 0; a < 9; a++) virial[a] += tv[a];
+      }
+      free(en);
+      free(soap);
+      free(der);
+    }
+    free_sites(&sp);
+  }
+  double* te = (double*)malloc(sizeof(double) * (size_t)n);
+  for (int b = 0; b < m->n2b; b++) {
+    orc_gap_2b_ef(&m->b2[b], n, species, nn, list, rjs, xyz, do_forces, te, tf, tv);
+    for (int i = 0; i < n; i++) e_atom[i] += te[i];
+    if (do_forces) {
+      for (size_t a = 0; a < (size_t)3 * n; a++) forces[a] += tf[a];
+      for (int a = 0; a < 9; a++) virial[a] += tv[a];
+    }
+  }
+  free(te);
+  free(tf);
+  free(nn);
+  free(list);
+  free(rjs);
+  free(xyz);
+  return TGX_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* MD / MC                                                                  */
+/* ------------------------------------------------------------------------ */
+
+void orc_velocity_verlet(int n, double* pos, double* pos_prev, double* vel,
+                         const double* forces, double* forces_prev,
+                         const double* masses, double dt, double* dt_prev,
+                         int first_step) {
+  if (!first_step)
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < 3; j++)
+        vel[3 * i + j] = vel[3 * i + j] + 0.5 * (forces[3 * i + j] + forces_prev[3 * i + j]) / masses[i] * *dt_prev;
+  memcpy(pos_prev, pos, sizeof(double) * 3 * (size_t)n);
+  memcpy(forces_prev, forces, sizeof(double) * 3 * (size_t)n);
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < 3; j++)
+      pos[3 * i + j] = pos[3 * i + j] + vel[3 * i + j] * dt + 0.5 * forces[3 * i + j] / masses[i] * (dt * dt);
+  *dt_prev = dt;
+}
+
+double orc_mc_accept(double e_new, double e_prev, double temp) {
+  /* kB is a default-real literal in the reference (mc_utils.f90:176). */
+  const double kB = (double)8.617333262e-5f;
+  double p = exp(-(e_new - e_prev) / (kB * temp));
+  return p < 1.0 ? p : 1.0;
+}

@@ -1,0 +1,194 @@
+"""GPU parity: the HIP path through the C-ABI against the FP64 CPU oracle.
+
+Tolerances: neighbour lists bit-exact (ids, rjs, xyz); FP64 mode energies
+1e-8 eV/atom, forces 1e-6 eV/A, virial 1e-6 eV; FP32 / mixed modes at the
+stated looser bounds.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import structures as S
+import tgx
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = tgx.Engine(0, "fp64")
+    yield e
+    e.close()
+
+
+def _nb_equal(a, b, ulps_angles=True):
+    assert np.array_equal(a["n_neigh"], b["n_neigh"])
+    assert np.array_equal(a["list"], b["list"])
+    assert np.array_equal(a["rjs"], b["rjs"])  # bit-exact
+    assert np.array_equal(a["xyz"], b["xyz"])
+    assert np.allclose(a["thetas"], b["thetas"], rtol=0, atol=4e-16 * np.pi)
+    assert np.allclose(a["phis"], b["phis"], rtol=0, atol=4e-16 * np.pi)
+    assert np.array_equal(a["indices"], b["indices"])
+
+
+@pytest.mark.parametrize("case", ["carbon216", "small32", "cho300", "carbon2000"])
+def test_neighbors_bit_exact(eng, case):
+    if case == "carbon216":
+        pos, sp, lat, _ = W.carbon216()
+        rc = 3.7
+    elif case == "small32":
+        pos, sp, lat, _ = W.small_box()
+        rc = 3.7
+    elif case == "cho300":
+        pos, sp, lat, _ = W.cho()
+        rc = 4.0
+    else:
+        pos, sp, lat = S.carbon_cell(2000, 2.0, 1.2, 55)
+        rc = 4.2
+    print(f"case={case} n={len(pos)} rcut={rc}", flush=True)
+    g = eng.neighbors(pos, lat, rc)
+    o = oracle.neighbors(pos, lat, rc)
+    _nb_equal(g, o)
+
+
+def test_neighbors_unwrapped_positions(eng):
+    """Positions outside the box (MD without wrapping) give the same list."""
+    pos, sp, lat, _ = W.carbon216()
+    L = lat[0]
+    rng = np.random.default_rng(3)
+    shifted = pos + L * rng.integers(-2, 3, size=pos.shape)
+    g = eng.neighbors(shifted, lat, 3.7)
+    o = oracle.neighbors(shifted, lat, 3.7)
+    _nb_equal(g, o)
+
+
+def test_soap_fp64_and_fp32(eng):
+    pos, sp, lat, model = W.carbon216()
+    eng.set_model(model)
+    ref = oracle.soap(model.hypers, pos, sp, lat)["soap"]
+    q64, site = eng.soap(pos, sp, lat)
+    assert np.array_equal(site, np.arange(len(pos)))
+    assert np.abs(q64 - ref).max() < 1e-10
+    eng.set_precision("fp32")
+    try:
+        q32, _ = eng.soap(pos, sp, lat)
+    finally:
+        eng.set_precision("fp64")
+    assert np.abs(q32 - ref).max() < 2e-5
+
+
+def test_soap_multispecies(eng):
+    pos, sp, lat, model = W.cho()
+    eng.set_model(model)
+    ref = oracle.soap(model.hypers, pos, sp, lat)["soap"]
+    q, _ = eng.soap(pos, sp, lat)
+    assert np.abs(q - ref).max() < 1e-10
+
+
+@pytest.mark.parametrize("prec,etol,gtol", [("fp64", 1e-11, 1e-11), ("fp32", 2e-5, 2e-5),
+                                            ("mixed", 2e-2, 2e-2)])
+def test_kernel_step(eng, prec, etol, gtol):
+    pos, sp, lat, model = W.carbon216()
+    eng.set_model(model)
+    q = oracle.soap(model.hypers, pos, sp, lat)["soap"]
+    e_ref, qss_ref = oracle.gap_kernel_step(q, model.Qs, model.alphas, model.delta, model.zeta)
+    eng.set_precision(prec)
+    try:
+        e, qss = eng.kernel_step(q)
+    finally:
+        eng.set_precision("fp64")
+    scale_e = np.abs(e_ref).max()
+    scale_g = np.abs(qss_ref).max()
+    print(prec, np.abs(e - e_ref).max() / scale_e, np.abs(qss - qss_ref).max() / scale_g)
+    assert np.abs(e - e_ref).max() <= etol * scale_e
+    assert np.abs(qss - qss_ref).max() <= gtol * scale_g
+
+
+def test_kernel_step_fractional_zeta(eng):
+    pos, sp, lat, model = W.carbon216()
+    m2 = S.model_from_soaps(model.hypers, model.Qs, 300, seed=4, zeta=2.5)
+    eng.set_model(m2)
+    q = oracle.soap(model.hypers, pos, sp, lat)["soap"]
+    e_ref, qss_ref = oracle.gap_kernel_step(q, m2.Qs, m2.alphas, m2.delta, m2.zeta)
+    e, qss = eng.kernel_step(q)
+    assert np.abs(e - e_ref).max() < 1e-11
+    assert np.abs(qss - qss_ref).max() < 1e-11
+
+
+def _ef_check(eng, model, pos, sp, lat, etol, ftol, vtol, prec="fp64"):
+    eng.set_model(model)
+    eng.set_precision(prec)
+    try:
+        e, f, v = eng.energy_forces(pos, sp, lat)
+    finally:
+        eng.set_precision("fp64")
+    e0, f0, v0 = oracle.energy_forces(model, pos, sp, lat)
+    de, df, dv = np.abs(e - e0).max(), np.abs(f - f0).max(), np.abs(v - v0).max()
+    print(f"{prec} n={len(pos)} dE={de:.3e} dF={df:.3e} dV={dv:.3e} "
+          f"F_mae={np.abs(f - f0).mean():.3e}")
+    assert de <= etol and df <= ftol and dv <= vtol, (de, df, dv)
+    return e, f, v, e0, f0, v0
+
+
+def test_energy_forces_fp64_carbon216(eng):
+    pos, sp, lat, model = W.carbon216()
+    _ef_check(eng, model, pos, sp, lat, 1e-8, 1e-6, 1e-6)
+
+
+def test_energy_forces_fp64_small_box(eng):
+    pos, sp, lat, model = W.small_box()
+    _ef_check(eng, model, pos, sp, lat, 1e-8, 1e-6, 1e-6)
+
+
+def test_energy_forces_fp64_cho(eng):
+    pos, sp, lat, model = W.cho()
+    _ef_check(eng, model, pos, sp, lat, 1e-8, 1e-6, 1e-6)
+
+
+def test_energy_forces_central_species(eng):
+    pos, sp, lat, model = W.cho()
+    h = S.cho_hypers()
+    h.central_species = 1
+    q = oracle.soap(h, pos, sp, lat)["soap"]
+    m = S.model_from_soaps(h, q, 100, seed=9)
+    _ef_check(eng, m, pos, sp, lat, 1e-8, 1e-6, 1e-6)
+
+
+def test_energy_forces_mixed_and_fp32(eng):
+    pos, sp, lat, model = W.carbon216()
+    _ef_check(eng, model, pos, sp, lat, 1e-4, 1e-4, 1e-3, prec="fp32")
+    _, f, _, _, f0, _ = _ef_check(eng, model, pos, sp, lat, 2e-3, 2e-2, 5e-2, prec="mixed")
+    assert np.abs(f - f0).mean() < 1e-3  # force MAE <= 1 meV/A
+
+
+def test_energies_only(eng):
+    pos, sp, lat, model = W.carbon216()
+    eng.set_model(model)
+    e, _, _ = eng.energy_forces(pos, sp, lat, do_forces=False)
+    e0, _, _ = oracle.energy_forces(model, pos, sp, lat, do_forces=False)
+    assert np.abs(e - e0).max() < 1e-8
+
+
+def test_single_atom(eng):
+    pos = np.array([[1.0, 2.0, 3.0]])
+    sp = np.array([1], np.int32)
+    lat = S.cubic_lattice(20.0)
+    _, _, _, model = W.carbon216()
+    eng.set_model(model)
+    e, f, v = eng.energy_forces(pos, sp, lat)
+    e0, f0, v0 = oracle.energy_forces(model, pos, sp, lat)
+    assert np.abs(e - e0).max() < 1e-10
+    assert np.abs(f).max() == 0.0 and np.abs(v).max() == 0.0
+
+
+def test_bad_arguments_raise(eng):
+    _, _, _, model = W.carbon216()
+    eng.set_model(model)
+    pos = np.zeros((2, 3))
+    with pytest.raises(tgx.TgxError):
+        eng.energy_forces(pos, np.array([1, 5], np.int32), S.cubic_lattice(10.0))
+    tri = np.array([10.0, 0, 0, 1.0, 10.0, 0, 0, 0, 10.0])
+    with pytest.raises(tgx.TgxError) as ei:
+        eng.energy_forces(pos + [[0, 0, 0], [1.5, 0, 0]], np.array([1, 1], np.int32), tri)
+    assert ei.value.code == tgx.TGX_ERR_UNSUPPORTED

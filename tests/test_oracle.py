@@ -1,0 +1,485 @@
+"""Pins the FP64 CPU oracle against mathematics and independent numpy code.
+
+The reference ships no golden vectors for this path and its SOAP library
+(soap_turbo) is absent, so each oracle function is checked against what the
+mathematics fixes: quadrature, library special functions, invariances,
+finite differences, closed forms, and independent brute-force restatements.
+"""
+import math
+
+import mpmath as mp
+import numpy as np
+import pytest
+from scipy.integrate import quad
+from scipy.special import sph_harm_y
+
+import oracle
+import structures as S
+import workloads as W
+
+KB_SINGLE = float(np.float32(8.617333262e-5))
+
+
+def _fd5(f, x, h):
+    """Fourth-order central difference."""
+    return (-f(x + 2 * h) + 8 * f(x + h) - 8 * f(x - h) + f(x - 2 * h)) / (12 * h)
+
+
+# --------------------------------------------------------------------------
+# radial basis
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("nmax", [1, 4, 8, 12])
+def test_poly3_overlap_vs_quadrature(nmax):
+    S_, W_ = oracle.poly3_overlap(nmax)
+    rh = 3.7
+    for a in range(1, nmax + 1):
+        for b in range(1, nmax + 1):
+            Na = math.sqrt((2 * a + 5) / rh ** (2 * a + 5))
+            Nb = math.sqrt((2 * b + 5) / rh ** (2 * b + 5))
+            v = quad(lambda r: Na * Nb * (rh - r) ** (a + 2) * (rh - r) ** (b + 2), 0, rh,
+                     epsrel=1e-13)[0]
+            assert abs(v - S_[a - 1, b - 1]) < 1e-12
+
+
+@pytest.mark.parametrize("nmax,tol", [(4, 1e-13), (8, 1e-9), (12, 1e-6)])
+def test_W_is_inverse_sqrt(nmax, tol):
+    """W S W = I evaluated exactly (50 digits) with the FP64 W; W symmetric."""
+    mp.mp.dps = 50
+    _, W_ = oracle.poly3_overlap(nmax)
+    assert np.array_equal(W_, W_.T)
+    Sm = mp.matrix(nmax, nmax)
+    for a in range(1, nmax + 1):
+        for b in range(1, nmax + 1):
+            Sm[a - 1, b - 1] = mp.sqrt((2 * a + 5) * (2 * b + 5)) / (a + b + 5)
+    R = mp.matrix(W_.tolist()) * Sm * mp.matrix(W_.tolist()) - mp.eye(nmax)
+    assert float(max(abs(x) for x in R)) < tol
+    # positive definite
+    assert np.all(np.linalg.eigvalsh(W_) > 0)
+
+
+@pytest.mark.parametrize("p", [0, 1, 2])
+def test_radial_raw_vs_quadrature(p):
+    rh, sig = 3.7, 0.5
+    for r in [0.0, 0.4, 1.2, 2.5, 3.3, 3.69]:
+        out = oracle.radial_raw(r, rh, sig, p, 8)
+        for k in range(8):
+            ref = quad(lambda x: x ** p * (rh - x) ** (k + 3) * math.exp(-(x - r) ** 2 / (2 * sig * sig)),
+                       0, rh, epsabs=1e-15, epsrel=1e-13)[0] / (math.sqrt(2 * math.pi) * sig)
+            assert abs(out[k] - ref) <= 1e-11 * max(abs(ref), 1e-6), (r, k, out[k], ref)
+
+
+def test_radial_derivative_and_limits():
+    h = S.carbon_hypers()
+    r = np.linspace(0.3, 3.19, 41)  # below the soft radius (C2 kink there)
+    R, dR = oracle.radial(h, 0, r)
+    assert np.abs(_fd5(lambda x: oracle.radial(h, 0, x)[0], r, 1e-3) - dR).max() < 1e-7
+    r2 = np.linspace(3.21, 3.68, 11)
+    _, dR2 = oracle.radial(h, 0, r2)
+    assert np.abs(_fd5(lambda x: oracle.radial(h, 0, x)[0], r2, 1e-4) - dR2).max() < 1e-6
+    # zero at and beyond the hard cutoff, continuous with zero slope there
+    R1, dR1 = oracle.radial(h, 0, np.array([3.7, 4.0]))
+    assert np.all(R1 == 0) and np.all(dR1 == 0)
+    R2, dR2 = oracle.radial(h, 0, np.array([3.7 - 1e-6]))
+    assert np.abs(R2).max() < 1e-9 and np.abs(dR2).max() < 1e-4
+
+
+def test_radial_sigma_scaling_derivative():
+    h = S.SoapHypers.uniform(1, atom_sigma_r_scaling=0.1, atom_sigma_t_scaling=0.05,
+                             amplitude_scaling=2.0, radial_enhancement=1)
+    r = np.linspace(0.5, 3.6, 17)
+    r = np.linspace(0.5, 3.15, 17)
+    _, dR = oracle.radial(h, 0, r)
+    fd = _fd5(lambda x: oracle.radial(h, 0, x)[0], r, 1e-3)
+    assert np.abs(fd - dR).max() < 1e-6 * max(1.0, np.abs(dR).max())
+
+
+# --------------------------------------------------------------------------
+# spherical harmonics
+# --------------------------------------------------------------------------
+def test_real_harmonics_vs_scipy():
+    rng = np.random.default_rng(0)
+    L = 10
+    for _ in range(20):
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        Y, _ = oracle.sph(L, u)
+        th, ph = math.acos(u[2]), math.atan2(u[1], u[0])
+        for l in range(L + 1):
+            for m in range(-l, l + 1):
+                yc = sph_harm_y(l, abs(m), th, ph)
+                if m == 0:
+                    ref = yc.real
+                elif m > 0:
+                    ref = math.sqrt(2) * (-1) ** m * yc.real
+                else:
+                    ref = math.sqrt(2) * (-1) ** m * yc.imag
+                assert abs(Y[l * l + l + m] - ref) < 1e-13
+
+
+def test_real_harmonics_orthonormal():
+    """Quadrature on the sphere (Lebedev-free: product Gauss-Legendre x
+    uniform phi) gives the identity Gram matrix."""
+    L = 8
+    xg, wg = np.polynomial.legendre.leggauss(24)
+    nphi = 48
+    G = np.zeros(((L + 1) ** 2, (L + 1) ** 2))
+    for z, w in zip(xg, wg):
+        s = math.sqrt(1 - z * z)
+        for k in range(nphi):
+            ph = 2 * math.pi * k / nphi
+            Y, _ = oracle.sph(L, [s * math.cos(ph), s * math.sin(ph), z])
+            G += w * (2 * math.pi / nphi) * np.outer(Y, Y)
+    assert np.abs(G - np.eye(len(G))).max() < 1e-12
+
+
+def test_harmonic_gradient_fd():
+    rng = np.random.default_rng(1)
+    L = 8
+    for _ in range(5):
+        v = rng.normal(size=3)
+        r = np.linalg.norm(v)
+        u = v / r
+        Y, dF = oracle.sph(L, u)
+        # grad_r Y(r/|r|) = (dF - (u.dF) u)/r
+        grad = (dF - (dF @ u)[:, None] * u[None, :]) / r
+        e = 1e-6
+        for k in range(3):
+            vp, vm = v.copy(), v.copy()
+            vp[k] += e
+            vm[k] -= e
+            Yp, _ = oracle.sph(L, vp / np.linalg.norm(vp))
+            Ym, _ = oracle.sph(L, vm / np.linalg.norm(vm))
+            assert np.abs((Yp - Ym) / (2 * e) - grad[:, k]).max() < 1e-7
+
+
+# --------------------------------------------------------------------------
+# SOAP descriptor
+# --------------------------------------------------------------------------
+def _cluster(n=12, seed=5):
+    rng = np.random.default_rng(seed)
+    pts = [np.zeros(3)]
+    while len(pts) < n:
+        p = rng.uniform(-3.3, 3.3, 3)
+        if np.linalg.norm(p) < 3.6 and all(np.linalg.norm(p - q) > 1.1 for q in pts):
+            pts.append(p)
+    return np.array(pts) + 20.0
+
+
+def test_soap_rotation_permutation_invariance():
+    h = S.carbon_hypers()
+    lat = S.cubic_lattice(40.0)
+    pos = _cluster()
+    sp = np.ones(len(pos), np.int32)
+    q0 = oracle.soap(h, pos, sp, lat)["soap"][0]
+    assert abs(np.linalg.norm(q0) - 1) < 1e-14
+    rng = np.random.default_rng(2)
+    A = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+    rot = (pos - 20.0) @ A.T + 20.0
+    perm = np.concatenate([[0], 1 + rng.permutation(len(pos) - 1)])
+    q1 = oracle.soap(h, rot[perm], sp, lat)["soap"][0]
+    assert np.abs(q1 - q0).max() < 1e-11
+    # reflection too (power spectrum is parity-even)
+    q2 = oracle.soap(h, (pos - 20.0) * [-1, 1, 1] + 20.0, sp, lat)["soap"][0]
+    assert np.abs(q2 - q0).max() < 1e-11
+
+
+def test_soap_isolated_atom_and_dimer_closed_form():
+    """A lone atom has only c_n00 = w R_n(0) Y00; a dimer adds
+    R_n(r) A_l Y_lm(u) - so p_{nn'l} for l > 0 is R_n R_n' A_l^2 (2l+1)/(4 pi)."""
+    h = S.carbon_hypers()
+    lat = S.cubic_lattice(30.0)
+    r = 2.1
+    pos = np.array([[10.0, 10.0, 10.0], [10.0, 10.0 + r, 10.0]])
+    sp = np.ones(2, np.int32)
+    q = oracle.soap(h, pos, sp, lat)["soap"][0]
+    R, _ = oracle.radial(h, 0, [r])
+    R = R[0]
+    A = np.exp(-0.5 * np.arange(9) * np.arange(1, 10) * 0.25 / r ** 2)
+    nr, L = 8, 8
+    p = []
+    for n in range(nr):
+        for n2 in range(n, nr):
+            w = 1.0 if n == n2 else math.sqrt(2)
+            for l in range(L + 1):
+                if l == 0:
+                    continue
+                p.append((n, n2, l, w * R[n] * R[n2] * A[l] ** 2 * (2 * l + 1) / (4 * math.pi)))
+    # compare l > 0 components up to the common normalisation
+    idx = lambda n, n2, l: (n * nr - n * (n - 1) // 2 + (n2 - n)) * (L + 1) + l
+    ref = np.array([v for *_, v in p])
+    got = np.array([q[idx(n, n2, l)] for n, n2, l, _ in p])
+    scale = got[np.argmax(np.abs(ref))] / ref[np.argmax(np.abs(ref))]
+    assert np.abs(got - scale * ref).max() < 1e-10 * np.abs(got).max()
+
+
+def test_soap_derivatives_fd():
+    pos, sp, lat, _ = W.carbon216()
+    pos, sp = pos[:80].copy(), sp[:80]
+    lat = S.cubic_lattice(12.915)  # keep density structure, box unchanged
+    h = S.carbon_hypers()
+    d = oracle.soap(h, pos, sp, lat, derivatives=True)
+    off = np.concatenate([[0], np.cumsum(d["site_n_neigh"])])
+    i = 3
+    pa = d["pair_atom"][off[i]:off[i + 1]]
+    der = d["soap_der"][off[i]:off[i + 1]]
+    e = 1e-4
+    for jj in [0, 1, 2, len(pa) - 1]:
+        j = pa[jj]
+        for k in range(3):
+            vals = []
+            for s in (2, 1, -1, -2):
+                pp = pos.copy()
+                pp[j, k] += s * e
+                vals.append(oracle.soap(h, pp, sp, lat)["soap"][i])
+            fd = (-vals[0] + 8 * vals[1] - 8 * vals[2] + vals[3]) / (12 * e)
+            assert np.abs(fd - der[jj, :, k]).max() < 2e-7
+
+
+# --------------------------------------------------------------------------
+# kernel step, full path
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("zeta", [4.0, 2.5, 1.0])
+def test_kernel_step_closed_form(zeta):
+    rng = np.random.default_rng(3)
+    q = rng.random((30, 45))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    Qs = rng.random((20, 45))
+    Qs /= np.linalg.norm(Qs, axis=1, keepdims=True)
+    al = rng.normal(0, 0.01, 20)
+    delta = 1.3
+    e, qss = oracle.gap_kernel_step(q, Qs, al, delta, zeta)
+    K = np.einsum("id,sd->is", q, Qs)
+    e_ref = delta ** 2 * (K ** zeta) @ al
+    qss_ref = -zeta * delta ** 2 * (K ** (zeta - 1) * al) @ Qs
+    assert np.abs(e - e_ref).max() < 1e-15
+    assert np.abs(qss - qss_ref).max() < 1e-15
+
+
+def _strained(pos, lat, eps):
+    F = np.eye(3) + eps
+    return pos @ F.T, (lat.reshape(3, 3) @ F.T).reshape(9)
+
+
+@pytest.fixture(scope="module")
+def c64():
+    pos, sp, lat = S.carbon_cell(64, 2.0, 1.2, 21)
+    h = S.carbon_hypers()
+    q = oracle.soap(h, pos, sp, lat)["soap"]
+    model = S.model_from_soaps(h, q, 80, seed=21, two_body=[S.carbon_2b(21)])
+    return pos, sp, lat, model
+
+
+def test_forces_are_minus_gradient(c64):
+    pos, sp, lat, model = c64
+    e, f, v = oracle.energy_forces(model, pos, sp, lat)
+    h = 1e-4
+    for a in [0, 7, 33]:
+        for k in range(3):
+            Es = []
+            for s in (2, 1, -1, -2):
+                p = pos.copy()
+                p[a, k] += s * h
+                Es.append(oracle.energy_forces(model, p, sp, lat, do_forces=False)[0].sum())
+            fd = -(-Es[0] + 8 * Es[1] - 8 * Es[2] + Es[3]) / (12 * h)
+            assert abs(fd - f[a, k]) < 1e-8 * max(1.0, abs(f[a, k])) + 1e-9
+
+
+def test_virial_is_minus_strain_derivative(c64):
+    """virial = -dE/d(strain) for SOAP and 2-body terms together (the
+    reference's conventions at gap.f90:201-207 and 437-447)."""
+    pos, sp, lat, model = c64
+    _, _, v = oracle.energy_forces(model, pos, sp, lat)
+    h = 1e-5
+    # orthorhombic cells only: diagonal strains; off-diagonal via the
+    # symmetric Cauchy-Born pair rule on a big cell is not orthorhombic.
+    for a in range(3):
+        Es = []
+        for s in (1, -1):
+            eps = np.zeros((3, 3))
+            eps[a, a] = s * h
+            p, l = _strained(pos, lat, eps)
+            Es.append(oracle.energy_forces(model, p, sp, l, do_forces=False)[0].sum())
+        fd = -(Es[0] - Es[1]) / (2 * h)
+        assert abs(fd - v[a, a]) < 1e-6 * max(1.0, abs(v[a, a]))
+    assert np.abs(v - v.T).max() < 1e-12
+
+
+def test_virial_offdiagonal_pair_identity(c64):
+    """Off-diagonal virial equals sum over pairs of sym(F_ij x r_ij) computed
+    independently from the materialised soap_cart_der (gap.f90:191-209)."""
+    pos, sp, lat, model = c64
+    d = oracle.soap(model.hypers, pos, sp, lat, derivatives=True)
+    e, f, v = oracle.gap_soap_ef(d["soap"], d["soap_der"], model.alphas, model.delta,
+                                 model.zeta, 0.0, model.Qs, d["site_n_neigh"], d["pair_atom"],
+                                 d["pair_xyz"], len(pos))
+    K = d["soap"] @ model.Qs.T
+    Qss = -model.zeta * model.delta ** 2 * (K ** (model.zeta - 1) * model.alphas) @ model.Qs
+    off = np.concatenate([[0], np.cumsum(d["site_n_neigh"])])
+    vv = np.zeros((3, 3))
+    ff = np.zeros_like(f)
+    for i in range(len(d["soap"])):
+        for l in range(off[i], off[i + 1]):
+            F = Qss[i] @ d["soap_der"][l]
+            ff[d["pair_atom"][l]] += F
+            x = d["pair_xyz"][l]
+            vv += 0.5 * (np.outer(F, x) + np.outer(x, F))
+    assert np.abs(vv - v).max() < 1e-12
+    assert np.abs(ff - f).max() < 1e-12
+
+
+def test_extensivity_supercell(c64):
+    pos, sp, lat, model = c64
+    e1, f1, v1 = oracle.energy_forces(model, pos, sp, lat)
+    L = lat[0]
+    shifts = np.array([[i, j, k] for i in range(2) for j in range(2) for k in range(2)]) * L
+    pos2 = np.concatenate([pos + s for s in shifts])
+    sp2 = np.tile(sp, 8)
+    e2, f2, v2 = oracle.energy_forces(model, pos2, sp2, S.cubic_lattice(2 * L))
+    assert abs(e2.sum() - 8 * e1.sum()) < 1e-9
+    assert np.abs(f2[:64] - f1).max() < 1e-9
+    assert np.abs(v2 - 8 * v1).max() < 1e-8
+
+
+def test_chunked_equals_unchunked(c64):
+    pos, sp, lat, model = c64
+    a = oracle.energy_forces(model, pos, sp, lat, chunk_sites=0)
+    b = oracle.energy_forces(model, pos, sp, lat, chunk_sites=7)
+    for x, y in zip(a, b):
+        assert np.abs(x - y).max() < 1e-13
+
+
+def test_translation_sum_of_forces(c64):
+    pos, sp, lat, model = c64
+    _, f, _ = oracle.energy_forces(model, pos, sp, lat)
+    assert np.abs(f.sum(0)).max() < 1e-11
+
+
+# --------------------------------------------------------------------------
+# 2-body
+# --------------------------------------------------------------------------
+def test_2b_dimer_closed_form():
+    b = S.carbon_2b(3)
+    lat = S.cubic_lattice(30.0)
+    for r in (1.5, 3.1, 3.45, 3.69):  # plain, and inside the 0.5 buffer
+        pos = np.array([[5.0, 5.0, 5.0], [5.0 + r, 5.0, 5.0]])
+        sp = np.ones(2, np.int32)
+        e, f, v = oracle.gap_2b(b, pos, sp, lat)
+        fc = 1.0 if r < b.rcut - b.buffer else (math.cos(math.pi * (r - b.rcut + b.buffer) / b.buffer) + 1) / 2
+        phi = b.delta ** 2 * np.sum(b.alphas * b.cutoff * fc * np.exp(-0.5 * (r - b.Qs) ** 2 / b.sigma ** 2))
+        assert abs(e[0] - phi) < 1e-15 and abs(e[1] - phi) < 1e-15
+        # F = -dE_total/dx with E_total = 2 phi
+        hstep = 1e-6
+        def etot(rr):
+            p = np.array([[5.0, 5.0, 5.0], [5.0 + rr, 5.0, 5.0]])
+            return oracle.gap_2b(b, p, sp, lat, do_forces=False)[0].sum()
+        dE = (etot(r + hstep) - etot(r - hstep)) / (2 * hstep)
+        assert abs(f[1, 0] - (-dE)) < 1e-8 and abs(f[0, 0] - dE) < 1e-8
+        # virial = -dE/dstrain_xx = -dE/dr * r
+        assert abs(v[0, 0] - (-dE * r)) < 1e-8
+
+
+# --------------------------------------------------------------------------
+# neighbours
+# --------------------------------------------------------------------------
+def _numpy_neighbors(pos, L, rc):
+    """Independent brute force with explicit periodic images (any box)."""
+    n = len(pos)
+    reps = [int(math.ceil(rc / L)) + 1] * 3
+    out = []
+    for i in range(n):
+        s = set()
+        for a in range(-reps[0], reps[0] + 1):
+            for b in range(-reps[1], reps[1] + 1):
+                for c in range(-reps[2], reps[2] + 1):
+                    d = pos + np.array([a, b, c]) * L - pos[i]
+                    r = np.linalg.norm(d, axis=1)
+                    for j in np.nonzero(r < rc)[0]:
+                        if not (j == i and a == b == c == 0):
+                            s.add((j, round(r[j], 9)))
+        out.append(sorted(s))
+    return out
+
+
+@pytest.mark.parametrize("n,rho,rc", [(216, 2.0, 3.7), (32, 3.2, 3.7), (40, 1.5, 4.2)])
+def test_neighbors_vs_numpy(n, rho, rc):
+    pos, sp, lat = S.carbon_cell(n, rho, 1.2, 10 + n)
+    nb = oracle.neighbors(pos, lat, rc)
+    ref = _numpy_neighbors(pos, lat[0], rc)
+    off = np.concatenate([[0], np.cumsum(nb["n_neigh"])])
+    for i in range(n):
+        lst = nb["list"][off[i]:off[i + 1]]
+        assert lst[0] == i + 1 and nb["rjs"][off[i]] == 0.0
+        got = sorted(((j - 1) % n, round(r, 9)) for j, r in zip(lst[1:], nb["rjs"][off[i] + 1:off[i + 1]]))
+        assert got == ref[i]
+        assert np.all(np.diff(lst[1:]) > 0)
+        # xyz consistent with rjs and theta/phi
+        x = nb["xyz"][off[i] + 1:off[i + 1]]
+        r = nb["rjs"][off[i] + 1:off[i + 1]]
+        assert np.abs(np.linalg.norm(x, axis=1) - r).max() < 1e-12
+        th = nb["thetas"][off[i] + 1:off[i + 1]]
+        assert np.abs(np.cos(th) * r - x[:, 2]).max() < 1e-12
+
+
+def test_neighbors_cells_equal_brute():
+    pos, sp, lat = S.carbon_cell(500, 2.0, 1.2, 77)
+    a = oracle.neighbors(pos, lat, 3.7, algo=0)
+    b = oracle.neighbors(pos, lat, 3.7, algo=1)
+    for k in a:
+        assert np.array_equal(a[k], b[k])
+
+
+def test_mic_matches_gfortran_modulo():
+    """modulo(a,p) = fmod(a,p) (+p when a<0 and remainder non-zero); fold
+    at L/2; d = sqrt(dx^2+dy^2+dz^2) in that order."""
+    L = np.array([10.0, 11.0, 12.5])
+    rng = np.random.default_rng(4)
+    for _ in range(200):
+        pi, pj = rng.uniform(-25, 25, 3), rng.uniform(-25, 25, 3)
+        d, r = oracle.mic(pi, pj, L)
+        ref = []
+        for k in range(3):
+            a = pj[k] - pi[k]
+            t = math.fmod(a, L[k])
+            if t != 0.0 and (a < 0) != (L[k] < 0):
+                t = t + L[k]
+            if t > L[k] / 2.0:
+                t = t - L[k]
+            ref.append(t)
+        assert d.tolist() == ref
+        assert r == math.sqrt((ref[0] * ref[0] + ref[1] * ref[1]) + ref[2] * ref[2])
+
+
+# --------------------------------------------------------------------------
+# MD / MC
+# --------------------------------------------------------------------------
+def test_velocity_verlet_harmonic():
+    """Two unit masses on a spring: VV conserves energy to O(dt^2) and
+    matches the analytic oscillation to O(dt^2)."""
+    k, m = 2.0, 1.0
+    def forces(x):
+        d = x[1, 0] - x[0, 0] - 1.0
+        return np.array([[k * d, 0, 0], [-k * d, 0, 0]], float)
+    errs = []
+    for dt in (0.02, 0.01):
+        x = np.array([[0.0, 0, 0], [1.2, 0, 0]])
+        v = np.zeros((2, 3))
+        xp, fp = np.zeros_like(x), np.zeros_like(x)
+        masses = np.full(2, m)
+        f = forces(x)
+        dtp = dt
+        t = 0.0
+        nsteps = int(round(2.0 / dt))
+        for s in range(nsteps):
+            dtp = oracle.velocity_verlet(x, xp, v, f, fp, masses, dt, dtp, s == 0)
+            f = forces(x)
+            t += dt
+        w = math.sqrt(2 * k / m)
+        ext = (x[1, 0] - x[0, 0]) - 1.0
+        errs.append(abs(ext - 0.2 * math.cos(w * t)))
+    assert errs[1] < errs[0] / 3.0  # second order
+
+
+def test_mc_acceptance_formula():
+    for de in (-0.1, 0.0, 0.01, 0.05, 0.3):
+        p = oracle.mc_accept(1.0 + de, 1.0, 1500.0)
+        assert p == min(1.0, math.exp(-((1.0 + de) - 1.0) / (KB_SINGLE * 1500.0)))

@@ -1,0 +1,866 @@
+// C-ABI entry points of the tgx library (declared in include/tgx.h) and the
+// device-resident pipeline behind them.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "tgx_internal.h"
+
+namespace tgx {
+
+// launchers defined in tgx_kernels.hip (not in the shared header because they
+// take the device SoapDev pointer)
+hipError_t launch_build_rtab(const SoapDev* sd_dev, int n_species, const double* W, float* tab,
+                             hipStream_t st);
+hipError_t launch_neighbors(const double* pos, int n, const Cell& c, double rcut, int* cell_of,
+                            int* cell_count, int* cell_start, int* cell_fill, int* cell_atoms,
+                            int* count, int maxnb, int* list, int* rev, int* flag,
+                            hipStream_t st);
+hipError_t launch_csr_out(const double* pos, int n, const Cell& c, int maxnb, const int* count,
+                          const int* list, const long long* off, int* out_list, double* rjs,
+                          double* xyz, double* th, double* ph, hipStream_t st);
+template <typename T>
+hipError_t launch_soap_fwd(const SoapDev& sdh, const SoapDev* sd, const double* W,
+                           const float* rtab, const double* pos, const int* species,
+                           const Cell& cell, int maxnb, const int* count, const int* list,
+                           const int* site_atom, int n_sites, const int2* pairn, T* q, int q_ld,
+                           unsigned short* qb, int qb_ld, T* cbuf, T* pnorm, hipStream_t st);
+template <typename T>
+hipError_t launch_soap_bwd(const SoapDev& sdh, const SoapDev* sd, const double* W,
+                           const float* rtab, const double* pos, const int* species,
+                           const Cell& cell, int maxnb, const int* count, const int* list,
+                           const int* site_atom, int n_sites, const int2* pairn, const T* q,
+                           int q_ld, const T* cbuf, const T* pnorm, const T* G, int g_ld,
+                           T* gpair, double* vsite, hipStream_t st);
+hipError_t launch_two_body(const TwoBodyDev& b, const double* pos, const int* species, int n,
+                           const Cell& c, int maxnb, const int* count, const int* list,
+                           double* eatom, double* forces, double* vatom, int do_forces,
+                           hipStream_t st);
+hipError_t launch_energy_init(int n, const int* species, const double* e0,
+                              const int* site_of_atom, const double* esite, double* eatom,
+                              double* forces, double* vatom, hipStream_t st);
+hipError_t launch_reduce(int n_sites, const double* vsite, int n, const double* vatom,
+                         const double* eatom, double* out, hipStream_t st);
+
+int set_err(tgx_ctx* c, int code, const char* fmt, ...) {
+  if (!c) return code;
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  c->err = buf;
+  return code;
+}
+
+int DevBuf::ensure(size_t want) {
+  if (want <= bytes && p) return 0;
+  release();
+  size_t sz = want + want / 4 + 256;
+  if (hipMalloc(&p, sz) != hipSuccess) {
+    p = nullptr;
+    bytes = 0;
+    return 1;
+  }
+  bytes = sz;
+  return 0;
+}
+
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+}  // namespace tgx
+
+using namespace tgx;
+
+namespace {
+
+#define ENSURE(ctx, buf, nbytes)                                                     \
+  do {                                                                               \
+    if ((ctx)->buf.ensure(nbytes))                                                   \
+      return set_err(ctx, TGX_ERR_HIP, "device allocation of %zu bytes failed (%s)", \
+                     (size_t)(nbytes), #buf);                                        \
+  } while (0)
+
+struct Extra {
+  DevBuf sd_dev, pairn, e0_dev, iota, cell_count, cell_fill, out, stage;
+  DevBuf twob_store;
+  std::vector<int> species_host;
+  hipEvent_t ev[8];
+  bool ev_ready = false;
+};
+
+}  // namespace
+
+// The context keeps an Extra block next to it.
+struct tgx_ctx_full {
+  tgx_ctx c;
+  Extra x;
+};
+
+static Extra& X(tgx_ctx* c) { return reinterpret_cast<tgx_ctx_full*>(c)->x; }
+
+static size_t tsize(int prec) { return prec == TGX_FP64 ? 8 : 4; }
+
+// Neighbour-list capacity per atom for n atoms in a box at cutoff rcut.
+static int guess_maxnb(int n_total_images, double volume, double rcut) {
+  double rho = n_total_images / volume;
+  double est = 4.0 / 3.0 * M_PI * rcut * rcut * rcut * rho;
+  int m = (int)(est * 1.6) + 16;
+  return (m + 7) / 8 * 8;
+}
+
+// Build the padded, sorted neighbour list (and reverse index) on the device.
+static int build_list(tgx_ctx* c, int n, const Cell& cell, double rcut, bool want_rev) {
+  Extra& x = X(c);
+  const int ncell = cell.small ? 1 : cell.mx * cell.my * cell.mz;
+  ENSURE(c, cell_of, sizeof(int) * n);
+  ENSURE(c, cell_atoms, sizeof(int) * n);
+  ENSURE(c, cell_start, sizeof(int) * (ncell + 1));
+  if (x.cell_count.ensure(sizeof(int) * ncell) || x.cell_fill.ensure(sizeof(int) * ncell))
+    return set_err(c, TGX_ERR_HIP, "device allocation failed (cells)");
+  ENSURE(c, nb_count, sizeof(int) * n);
+  ENSURE(c, nb_flag, sizeof(int) * 4);
+  int maxnb = c->maxnb;
+  if (maxnb <= 0) maxnb = guess_maxnb(n * cell.nrep, cell.La * cell.Lb * cell.Lc, rcut);
+  for (int attempt = 0; attempt < 3; attempt++) {
+    ENSURE(c, nb_list, sizeof(int) * (size_t)n * maxnb);
+    if (want_rev) ENSURE(c, nb_rev, sizeof(int) * (size_t)n * maxnb);
+    TGX_HIP(c, hipMemsetAsync(c->nb_flag.p, 0, sizeof(int) * 4, c->stream));
+    TGX_HIP(c, launch_neighbors(c->pos.as<double>(), n, cell, rcut, c->cell_of.as<int>(),
+                                x.cell_count.as<int>(), c->cell_start.as<int>(),
+                                x.cell_fill.as<int>(), c->cell_atoms.as<int>(),
+                                c->nb_count.as<int>(), maxnb, c->nb_list.as<int>(),
+                                want_rev ? c->nb_rev.as<int>() : nullptr, c->nb_flag.as<int>(),
+                                c->stream));
+    int flag[4];
+    TGX_HIP(c, hipMemcpyAsync(flag, c->nb_flag.p, sizeof(flag), hipMemcpyDeviceToHost, c->stream));
+    TGX_HIP(c, hipStreamSynchronize(c->stream));
+    if (flag[0] > maxnb) {
+      maxnb = (flag[0] + flag[0] / 4 + 8 + 7) / 8 * 8;
+      continue;
+    }
+    if (flag[1]) return set_err(c, TGX_ERR_STATE, "neighbour list is not symmetric");
+    c->maxnb = maxnb;
+    return TGX_OK;
+  }
+  return set_err(c, TGX_ERR_CAPACITY, "neighbour capacity could not be sized");
+}
+
+static double rcut_all(const tgx_ctx* c) {
+  double r = c->has_soap ? c->sd.rcut_max : 0.0;
+  for (const auto& b : c->twob) r = std::max(r, b.rcut);
+  return r;
+}
+
+// Sites of the SOAP block: all atoms, or those of central_species.
+static int setup_sites(tgx_ctx* c, int n, const int* species_host, int* n_sites) {
+  Extra& x = X(c);
+  ENSURE(c, site_of_atom, sizeof(int) * n);
+  ENSURE(c, site_atom, sizeof(int) * n);
+  std::vector<int> soa(n), sa;
+  sa.reserve(n);
+  for (int i = 0; i < n; i++) {
+    if (c->has_soap && (c->sd.central_species == 0 || species_host[i] == c->sd.central_species)) {
+      soa[i] = (int)sa.size();
+      sa.push_back(i);
+    } else {
+      soa[i] = -1;
+    }
+  }
+  (void)x;
+  TGX_HIP(c, hipMemcpyAsync(c->site_of_atom.p, soa.data(), sizeof(int) * n, hipMemcpyHostToDevice,
+                            c->stream));
+  if (!sa.empty())
+    TGX_HIP(c, hipMemcpyAsync(c->site_atom.p, sa.data(), sizeof(int) * sa.size(),
+                              hipMemcpyHostToDevice, c->stream));
+  TGX_HIP(c, hipStreamSynchronize(c->stream));
+  *n_sites = (int)sa.size();
+  return TGX_OK;
+}
+
+// SOAP forward for the current list into c->q (and c->qb in mixed mode).
+static int soap_forward(tgx_ctx* c, int n, const Cell& cell, int n_sites) {
+  Extra& x = X(c);
+  const int prec = c->precision;
+  const size_t ts = tsize(prec);
+  const int NRP = (c->sd.nr + 7) / 8 * 8;
+  ENSURE(c, q, ts * (size_t)std::max(n_sites, 1) * c->dim_pad);
+  ENSURE(c, cbuf, ts * (size_t)std::max(n_sites, 1) * NRP * c->sd.L2);
+  ENSURE(c, pnorm, ts * (size_t)std::max(n_sites, 1));
+  unsigned short* qb = nullptr;
+  if (prec == TGX_MIXED) {
+    ENSURE(c, qb, 2 * (size_t)std::max(n_sites, 1) * c->dim_pad);
+    qb = c->qb.as<unsigned short>();
+  }
+  hipError_t e;
+  if (prec == TGX_FP64)
+    e = launch_soap_fwd<double>(c->sd, x.sd_dev.as<SoapDev>(), c->W_dev.as<double>(),
+                                c->rtab.as<float>(), c->pos.as<double>(), c->species.as<int>(),
+                                cell, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
+                                c->site_atom.as<int>(), n_sites, x.pairn.as<int2>(),
+                                c->q.as<double>(), c->dim_pad, nullptr, 0, c->cbuf.as<double>(),
+                                c->pnorm.as<double>(), c->stream);
+  else
+    e = launch_soap_fwd<float>(c->sd, x.sd_dev.as<SoapDev>(), c->W_dev.as<double>(),
+                               c->rtab.as<float>(), c->pos.as<double>(), c->species.as<int>(),
+                               cell, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
+                               c->site_atom.as<int>(), n_sites, x.pairn.as<int2>(),
+                               c->q.as<float>(), c->dim_pad, qb, c->dim_pad, c->cbuf.as<float>(),
+                               c->pnorm.as<float>(), c->stream);
+  TGX_HIP(c, e);
+  (void)n;
+  return TGX_OK;
+}
+
+// Kernel step on the descriptors in c->q / c->qb.
+static int kernel_step(tgx_ctx* c, int n_sites, bool want_grad) {
+  const int prec = c->precision;
+  const size_t tin = prec == TGX_FP64 ? 8 : (prec == TGX_FP32 ? 4 : 2);
+  const size_t tacc = prec == TGX_FP64 ? 8 : 4;
+  const int ns = std::max(n_sites, 1);
+  ENSURE(c, wmat, tin * (size_t)ns * c->sp_pad);
+  ENSURE(c, epart, tacc * (size_t)ns * (c->sp_pad / 64));
+  ENSURE(c, esite, 8 * (size_t)ns);
+  if (want_grad) ENSURE(c, gmat, tacc * (size_t)ns * c->dim_pad);
+  const void *qa, *qs, *qsT, *al;
+  if (prec == TGX_FP64) {
+    qa = c->q.p; qs = c->Qs64.p; qsT = c->QsT64.p; al = c->alph64.p;
+  } else if (prec == TGX_FP32) {
+    qa = c->q.p; qs = c->Qs32.p; qsT = c->QsT32.p; al = c->alph32.p;
+  } else {
+    qa = c->qb.p; qs = c->Qs16.p; qsT = c->QsT16.p; al = c->alph32.p;
+  }
+  TGX_HIP(c, launch_gap_kernel(prec, n_sites, c->dim, c->dim_pad, c->n_sparse, c->sp_pad, qa, qs,
+                               qsT, al, c->delta, c->zeta, c->wmat.p, c->epart.p,
+                               c->esite.as<double>(), want_grad ? c->gmat.p : nullptr,
+                               want_grad ? 1 : 0, c->stream));
+  return TGX_OK;
+}
+
+static void ev_rec(tgx_ctx* c, int k) {
+  Extra& x = X(c);
+  if (x.ev_ready) (void)hipEventRecord(x.ev[k], c->stream);
+}
+
+// Energies/forces/virial of the structure whose positions and species are
+// already in c->pos / c->species.  Results stay on the device:
+// c->eatom, c->forces, X(c).out[0..8] virial, [9] total energy.
+static int run_ef(tgx_ctx* c, int n, const double* lattice, const int* species_host,
+                  int do_forces) {
+  Extra& x = X(c);
+  if (!c->has_soap && c->twob.empty()) return set_err(c, TGX_ERR_STATE, "no model loaded");
+  Cell cell;
+  std::string err;
+  const double rc = rcut_all(c);
+  int rcode = make_cell(lattice, rc, &cell, &err);
+  if (rcode) return set_err(c, rcode, "%s", err.c_str());
+  if ((long long)n * cell.nrep >= (1LL << TGX_IMG_SHIFT))
+    return set_err(c, TGX_ERR_ARG, "too many atoms");
+  ev_rec(c, 0);
+  rcode = build_list(c, n, cell, rc, do_forces != 0);
+  if (rcode) return rcode;
+  ev_rec(c, 1);
+  int n_sites = 0;
+  rcode = setup_sites(c, n, species_host, &n_sites);
+  if (rcode) return rcode;
+  const int prec = c->precision;
+  ENSURE(c, eatom, 8 * (size_t)n);
+  ENSURE(c, forces, 24 * (size_t)n);
+  ENSURE(c, vsite, 48 * (size_t)std::max(n_sites, 1));
+  ENSURE(c, virial, 48 * (size_t)n);  // per-atom 2b virial
+  if (x.out.ensure(16 * sizeof(double))) return set_err(c, TGX_ERR_HIP, "allocation failed");
+  if (c->has_soap && n_sites > 0) {
+    rcode = soap_forward(c, n, cell, n_sites);
+    if (rcode) return rcode;
+    ev_rec(c, 2);
+    rcode = kernel_step(c, n_sites, do_forces != 0);
+    if (rcode) return rcode;
+    ev_rec(c, 3);
+  } else {
+    ev_rec(c, 2);
+    ev_rec(c, 3);
+  }
+  TGX_HIP(c, launch_energy_init(n, c->species.as<int>(), x.e0_dev.as<double>(),
+                                c->site_of_atom.as<int>(),
+                                (c->has_soap && n_sites > 0) ? c->esite.as<double>() : nullptr,
+                                c->eatom.as<double>(), c->forces.as<double>(),
+                                c->virial.as<double>(), c->stream));
+  if (do_forces && c->has_soap && n_sites > 0) {
+    const size_t ts = tsize(prec);
+    ENSURE(c, gpair, ts * 3 * (size_t)n * c->maxnb);
+    hipError_t e;
+    if (prec == TGX_FP64) {
+      e = launch_soap_bwd<double>(c->sd, x.sd_dev.as<SoapDev>(), c->W_dev.as<double>(),
+                                  c->rtab.as<float>(), c->pos.as<double>(), c->species.as<int>(),
+                                  cell, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
+                                  c->site_atom.as<int>(), n_sites, x.pairn.as<int2>(),
+                                  c->q.as<double>(), c->dim_pad, c->cbuf.as<double>(),
+                                  c->pnorm.as<double>(), c->gmat.as<double>(), c->dim_pad,
+                                  c->gpair.as<double>(), c->vsite.as<double>(), c->stream);
+      if (e == hipSuccess)
+        e = launch_force_gather<double>(n, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
+                                        c->nb_rev.as<int>(), c->site_of_atom.as<int>(),
+                                        c->gpair.as<double>(), c->forces.as<double>(), c->stream);
+    } else {
+      e = launch_soap_bwd<float>(c->sd, x.sd_dev.as<SoapDev>(), c->W_dev.as<double>(),
+                                 c->rtab.as<float>(), c->pos.as<double>(), c->species.as<int>(),
+                                 cell, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
+                                 c->site_atom.as<int>(), n_sites, x.pairn.as<int2>(),
+                                 c->q.as<float>(), c->dim_pad, c->cbuf.as<float>(),
+                                 c->pnorm.as<float>(), c->gmat.as<float>(), c->dim_pad,
+                                 c->gpair.as<float>(), c->vsite.as<double>(), c->stream);
+      if (e == hipSuccess)
+        e = launch_force_gather<float>(n, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
+                                       c->nb_rev.as<int>(), c->site_of_atom.as<int>(),
+                                       c->gpair.as<float>(), c->forces.as<double>(), c->stream);
+    }
+    TGX_HIP(c, e);
+  }
+  ev_rec(c, 4);
+  for (const auto& b : c->twob)
+    TGX_HIP(c, launch_two_body(b, c->pos.as<double>(), c->species.as<int>(), n, cell, c->maxnb,
+                               c->nb_count.as<int>(), c->nb_list.as<int>(), c->eatom.as<double>(),
+                               c->forces.as<double>(), c->virial.as<double>(), do_forces,
+                               c->stream));
+  ev_rec(c, 5);
+  TGX_HIP(c, launch_reduce((do_forces && c->has_soap) ? n_sites : 0,
+                           (do_forces && c->has_soap) ? c->vsite.as<double>() : nullptr, n,
+                           do_forces ? c->virial.as<double>() : nullptr, c->eatom.as<double>(),
+                           x.out.as<double>(), c->stream));
+  ev_rec(c, 6);
+  return TGX_OK;
+}
+
+static void collect_times(tgx_ctx* c) {
+  Extra& x = X(c);
+  if (!x.ev_ready) return;
+  float ms[6];
+  for (int k = 0; k < 6; k++) {
+    ms[k] = 0.f;
+    (void)hipEventElapsedTime(&ms[k], x.ev[k], x.ev[k + 1]);
+  }
+  c->stage_ms[0] = ms[0];  // neighbour list
+  c->stage_ms[1] = ms[1];  // site setup + SOAP forward
+  c->stage_ms[2] = ms[2];  // kernel GEMMs
+  c->stage_ms[3] = ms[3];  // SOAP adjoint + force gather
+  c->stage_ms[4] = ms[4];  // 2-body
+  c->stage_ms[5] = ms[5];  // reductions
+  float tot = 0.f;
+  (void)hipEventElapsedTime(&tot, x.ev[0], x.ev[6]);
+  c->stage_ms[6] = tot;
+}
+
+static int upload_structure(tgx_ctx* c, int n, const double* pos, const int* species) {
+  ENSURE(c, pos, 24 * (size_t)n);
+  ENSURE(c, species, 4 * (size_t)n);
+  const int ns = c->has_soap ? c->sd.n_species : TGX_MAX_SPECIES;
+  for (int i = 0; i < n; i++) {
+    if (species[i] < 1 || species[i] > ns)
+      return set_err(c, TGX_ERR_ARG, "species[%d] = %d out of range 1..%d", i, species[i], ns);
+    if (!std::isfinite(pos[3 * i]) || !std::isfinite(pos[3 * i + 1]) || !std::isfinite(pos[3 * i + 2]))
+      return set_err(c, TGX_ERR_ARG, "position of atom %d is not finite", i);
+  }
+  TGX_HIP(c, hipMemcpyAsync(c->pos.p, pos, 24 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  TGX_HIP(c, hipMemcpyAsync(c->species.p, species, 4 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  return TGX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int tgx_create(int device, int precision, tgx_ctx** out) {
+  if (!out) return TGX_ERR_ARG;
+  *out = nullptr;
+  if (precision < TGX_FP64 || precision > TGX_MIXED) return TGX_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return TGX_ERR_HIP;
+  tgx_ctx_full* f = new (std::nothrow) tgx_ctx_full();
+  if (!f) return TGX_ERR_ARG;
+  tgx_ctx* c = &f->c;
+  c->device = device;
+  c->precision = precision;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    delete f;
+    return TGX_ERR_HIP;
+  }
+  for (int k = 0; k < 8; k++) (void)hipEventCreate(&f->x.ev[k]);
+  f->x.ev_ready = true;
+  if (upload_sph_constants(TGX_MAX_LMAX, c->stream) != hipSuccess) {
+    delete f;
+    return TGX_ERR_HIP;
+  }
+  (void)hipStreamSynchronize(c->stream);
+  *out = c;
+  return TGX_OK;
+}
+
+int tgx_destroy(tgx_ctx* c) {
+  if (!c) return TGX_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  tgx_ctx_full* f = reinterpret_cast<tgx_ctx_full*>(c);
+  DevBuf* bufs[] = {&c->W_dev, &c->rtab, &c->Qs64, &c->QsT64, &c->Qs32, &c->QsT32, &c->Qs16,
+                    &c->QsT16, &c->alph64, &c->alph32, &c->pos, &c->species, &c->nb_count,
+                    &c->nb_list, &c->nb_rev, &c->nb_flag, &c->cell_of, &c->cell_start,
+                    &c->cell_atoms, &c->site_of_atom, &c->site_atom, &c->q, &c->qb, &c->cbuf,
+                    &c->pnorm, &c->ksum, &c->wmat, &c->gmat, &c->epart, &c->esite, &c->gpair,
+                    &c->vsite, &c->forces, &c->eatom, &c->virial, &c->scratch, &c->csr_off,
+                    &c->csr_out, &c->md_vel, &c->md_fprev, &c->md_mass, &c->md_pos0,
+                    &c->md_ep, &c->md_ek, &c->md_reb, &c->md_disp, &f->x.sd_dev, &f->x.pairn,
+                    &f->x.e0_dev, &f->x.iota, &f->x.cell_count, &f->x.cell_fill, &f->x.out,
+                    &f->x.stage, &f->x.twob_store};
+  for (DevBuf* b : bufs) b->release();
+  for (auto& b : c->twob) {
+    if (b.Qs) (void)hipFree(b.Qs);
+    if (b.alphas) (void)hipFree(b.alphas);
+  }
+  if (f->x.ev_ready)
+    for (int k = 0; k < 8; k++) (void)hipEventDestroy(f->x.ev[k]);
+  (void)hipStreamDestroy(c->stream);
+  delete f;
+  return TGX_OK;
+}
+
+const char* tgx_last_error(const tgx_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int tgx_set_precision(tgx_ctx* c, int precision) {
+  if (!c) return TGX_ERR_ARG;
+  if (precision < TGX_FP64 || precision > TGX_MIXED)
+    return set_err(c, TGX_ERR_ARG, "precision %d not in 0..2", precision);
+  c->precision = precision;
+  return TGX_OK;
+}
+
+int tgx_set_soap_model(tgx_ctx* c, const tgx_soap_hypers* h, int dim, int n_sparse,
+                       const double* Qs, const double* alphas, double delta, double zeta) {
+  if (!c) return TGX_ERR_ARG;
+  if (!h || !Qs || !alphas) return set_err(c, TGX_ERR_ARG, "null argument");
+  (void)hipSetDevice(c->device);
+  if (h->n_species < 1 || h->n_species > TGX_MAX_SPECIES)
+    return set_err(c, TGX_ERR_ARG, "n_species %d not in 1..%d", h->n_species, TGX_MAX_SPECIES);
+  if (h->l_max < 0 || h->l_max > TGX_MAX_LMAX)
+    return set_err(c, TGX_ERR_ARG, "l_max %d not in 0..%d", h->l_max, TGX_MAX_LMAX);
+  if (h->basis != 0) return set_err(c, TGX_ERR_UNSUPPORTED, "only the poly3 basis is implemented");
+  if (h->radial_enhancement < 0 || h->radial_enhancement > 2)
+    return set_err(c, TGX_ERR_ARG, "radial_enhancement must be 0, 1 or 2");
+  SoapDev sd{};
+  sd.n_species = h->n_species;
+  sd.l_max = h->l_max;
+  sd.L2 = (h->l_max + 1) * (h->l_max + 1);
+  sd.radial_enhancement = h->radial_enhancement;
+  sd.central_species = h->central_species;
+  int nr = 0;
+  double rcm = 0;
+  for (int s = 0; s < h->n_species; s++) {
+    if (h->alpha_max[s] < 1 || h->alpha_max[s] > 16)
+      return set_err(c, TGX_ERR_ARG, "alpha_max[%d] = %d not in 1..16", s, h->alpha_max[s]);
+    if (!(h->rcut_hard[s] > 0) || h->rcut_soft[s] > h->rcut_hard[s] || h->rcut_soft[s] < 0)
+      return set_err(c, TGX_ERR_ARG, "need 0 <= rcut_soft <= rcut_hard (species %d)", s + 1);
+    if (!(h->atom_sigma_r[s] > 0) || !(h->atom_sigma_t[s] > 0))
+      return set_err(c, TGX_ERR_ARG, "atom_sigma must be positive (species %d)", s + 1);
+    sd.nmax[s] = h->alpha_max[s];
+    sd.off[s] = nr;
+    nr += h->alpha_max[s];
+    sd.rh[s] = h->rcut_hard[s];
+    sd.rs[s] = h->rcut_soft[s];
+    sd.sig_r[s] = h->atom_sigma_r[s];
+    sd.sig_rs[s] = h->atom_sigma_r_scaling[s];
+    sd.sig_t[s] = h->atom_sigma_t[s];
+    sd.sig_ts[s] = h->atom_sigma_t_scaling[s];
+    sd.amp_scal[s] = h->amplitude_scaling[s];
+    sd.glob[s] = h->global_scaling[s];
+    rcm = std::max(rcm, h->rcut_hard[s]);
+  }
+  if (nr > TGX_MAX_NR) return set_err(c, TGX_ERR_ARG, "sum of alpha_max %d > %d", nr, TGX_MAX_NR);
+  sd.nr = nr;
+  sd.npair = nr * (nr + 1) / 2;
+  sd.dim = sd.npair * (h->l_max + 1);
+  sd.rcut_max = rcm;
+  if (h->central_species < 0 || h->central_species > h->n_species)
+    return set_err(c, TGX_ERR_ARG, "central_species out of range");
+  if (dim != sd.dim)
+    return set_err(c, TGX_ERR_ARG, "dim %d != n_max(n_max+1)/2*(l_max+1) = %d", dim, sd.dim);
+  if (n_sparse < 1) return set_err(c, TGX_ERR_ARG, "n_sparse must be >= 1");
+  // W per species and the central-atom coefficients
+  std::vector<double> W((size_t)TGX_MAX_SPECIES * TGX_MAX_NR * TGX_MAX_NR, 0.0);
+  for (int s = 0; s < h->n_species; s++) poly3_W(h->alpha_max[s], W.data() + (size_t)s * TGX_MAX_NR * TGX_MAX_NR);
+  for (int s = 0; s < h->n_species; s++) {
+    double R[TGX_MAX_NR];
+    SoapDev tmp = sd;
+    tmp.sig_rs[s] = 0.0;
+    radial_closed_form(tmp, W.data(), s, 0.0, R, nullptr);
+    for (int k = 0; k < h->alpha_max[s]; k++)
+      sd.rself[s * TGX_MAX_NR + k] = h->central_weight[s] * R[k] * (1.0 / std::sqrt(4.0 * M_PI));
+  }
+  c->W_host = W;
+  c->hyp = *h;
+  c->sd = sd;
+  c->dim = dim;
+  c->n_sparse = n_sparse;
+  c->dim_pad = (dim + 31) / 32 * 32;
+  c->sp_pad = (n_sparse + 63) / 64 * 64;
+  c->delta = delta;
+  c->zeta = zeta;
+  c->Qs_host.assign(Qs, Qs + (size_t)dim * n_sparse);
+  c->alphas_host.assign(alphas, alphas + n_sparse);
+  Extra& x = X(c);
+  // padded layouts of Qs for the three precisions
+  const int Dp = c->dim_pad, Sp = c->sp_pad;
+  std::vector<double> q64((size_t)Sp * Dp, 0.0), qT64((size_t)Dp * Sp, 0.0), a64(Sp, 0.0);
+  std::vector<float> q32((size_t)Sp * Dp, 0.f), qT32((size_t)Dp * Sp, 0.f), a32(Sp, 0.f);
+  std::vector<unsigned short> q16((size_t)Sp * Dp, 0), qT16((size_t)Dp * Sp, 0);
+  auto bf = [](float f) {
+    unsigned int u;
+    std::memcpy(&u, &f, 4);
+    return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+  };
+  for (int s = 0; s < n_sparse; s++) {
+    a64[s] = alphas[s];
+    a32[s] = (float)alphas[s];
+    for (int d = 0; d < dim; d++) {
+      const double v = Qs[(size_t)s * dim + d];
+      q64[(size_t)s * Dp + d] = v;
+      qT64[(size_t)d * Sp + s] = v;
+      q32[(size_t)s * Dp + d] = (float)v;
+      qT32[(size_t)d * Sp + s] = (float)v;
+      q16[(size_t)s * Dp + d] = bf((float)v);
+      qT16[(size_t)d * Sp + s] = bf((float)v);
+    }
+  }
+  ENSURE(c, Qs64, 8 * q64.size());
+  ENSURE(c, QsT64, 8 * qT64.size());
+  ENSURE(c, Qs32, 4 * q32.size());
+  ENSURE(c, QsT32, 4 * qT32.size());
+  ENSURE(c, Qs16, 2 * q16.size());
+  ENSURE(c, QsT16, 2 * qT16.size());
+  ENSURE(c, alph64, 8 * (size_t)Sp);
+  ENSURE(c, alph32, 4 * (size_t)Sp);
+  ENSURE(c, W_dev, 8 * W.size());
+  ENSURE(c, rtab, sizeof(float) * 2 * (size_t)TGX_MAX_SPECIES * (TGX_RTAB_N + 1) * TGX_MAX_NR);
+  if (x.sd_dev.ensure(sizeof(SoapDev))) return set_err(c, TGX_ERR_HIP, "allocation failed");
+  std::vector<int2> pn(sd.npair);
+  {
+    int k = 0;
+    for (int a = 0; a < nr; a++)
+      for (int b = a; b < nr; b++) pn[k++] = make_int2(a, b);
+  }
+  if (x.pairn.ensure(sizeof(int2) * pn.size())) return set_err(c, TGX_ERR_HIP, "allocation failed");
+  hipStream_t st = c->stream;
+  TGX_HIP(c, hipMemcpyAsync(c->Qs64.p, q64.data(), 8 * q64.size(), hipMemcpyHostToDevice, st));
+  TGX_HIP(c, hipMemcpyAsync(c->QsT64.p, qT64.data(), 8 * qT64.size(), hipMemcpyHostToDevice, st));
+  TGX_HIP(c, hipMemcpyAsync(c->Qs32.p, q32.data(), 4 * q32.size(), hipMemcpyHostToDevice, st));
+  TGX_HIP(c, hipMemcpyAsync(c->QsT32.p, qT32.data(), 4 * qT32.size(), hipMemcpyHostToDevice, st));
+  TGX_HIP(c, hipMemcpyAsync(c->Qs16.p, q16.data(), 2 * q16.size(), hipMemcpyHostToDevice, st));
+  TGX_HIP(c, hipMemcpyAsync(c->QsT16.p, qT16.data(), 2 * qT16.size(), hipMemcpyHostToDevice, st));
+  TGX_HIP(c, hipMemcpyAsync(c->alph64.p, a64.data(), 8 * (size_t)Sp, hipMemcpyHostToDevice, st));
+  TGX_HIP(c, hipMemcpyAsync(c->alph32.p, a32.data(), 4 * (size_t)Sp, hipMemcpyHostToDevice, st));
+  TGX_HIP(c, hipMemcpyAsync(c->W_dev.p, W.data(), 8 * W.size(), hipMemcpyHostToDevice, st));
+  TGX_HIP(c, hipMemcpyAsync(x.sd_dev.p, &c->sd, sizeof(SoapDev), hipMemcpyHostToDevice, st));
+  TGX_HIP(c, hipMemcpyAsync(x.pairn.p, pn.data(), sizeof(int2) * pn.size(), hipMemcpyHostToDevice, st));
+  TGX_HIP(c, launch_build_rtab(x.sd_dev.as<SoapDev>(), sd.n_species, c->W_dev.as<double>(),
+                               c->rtab.as<float>(), st));
+  TGX_HIP(c, hipStreamSynchronize(st));
+  c->has_soap = true;
+  c->maxnb = 0;
+  if (!x.e0_dev.p) {
+    if (x.e0_dev.ensure(8 * TGX_MAX_SPECIES)) return set_err(c, TGX_ERR_HIP, "allocation failed");
+    TGX_HIP(c, hipMemcpy(x.e0_dev.p, c->e0, 8 * TGX_MAX_SPECIES, hipMemcpyHostToDevice));
+  }
+  return TGX_OK;
+}
+
+int tgx_set_e0(tgx_ctx* c, int n_species, const double* e0) {
+  if (!c) return TGX_ERR_ARG;
+  if (n_species < 0 || n_species > TGX_MAX_SPECIES || (n_species > 0 && !e0))
+    return set_err(c, TGX_ERR_ARG, "bad e0 arguments");
+  (void)hipSetDevice(c->device);
+  for (int s = 0; s < TGX_MAX_SPECIES; s++) c->e0[s] = s < n_species ? e0[s] : 0.0;
+  Extra& x = X(c);
+  if (x.e0_dev.ensure(8 * TGX_MAX_SPECIES)) return set_err(c, TGX_ERR_HIP, "allocation failed");
+  TGX_HIP(c, hipMemcpy(x.e0_dev.p, c->e0, 8 * TGX_MAX_SPECIES, hipMemcpyHostToDevice));
+  return TGX_OK;
+}
+
+int tgx_add_2b_model(tgx_ctx* c, int sp1, int sp2, double rcut, double buffer, double delta,
+                     double sigma, int n_sparse, const double* Qs, const double* alphas,
+                     const double* cutoff) {
+  if (!c) return TGX_ERR_ARG;
+  if (sp1 < 1 || sp2 < 1 || sp1 > TGX_MAX_SPECIES || sp2 > TGX_MAX_SPECIES || n_sparse < 1 ||
+      !Qs || !alphas || !cutoff || !(rcut > 0) || !(sigma > 0) || buffer < 0)
+    return set_err(c, TGX_ERR_ARG, "bad distance_2b arguments");
+  (void)hipSetDevice(c->device);
+  TwoBodyDev b{};
+  b.sp1 = sp1;
+  b.sp2 = sp2;
+  b.n_sparse = n_sparse;
+  b.rcut = rcut;
+  b.buffer = buffer;
+  b.delta = delta;
+  b.sigma = sigma;
+  std::vector<double> ac(n_sparse);
+  for (int k = 0; k < n_sparse; k++) ac[k] = alphas[k] * cutoff[k];
+  TGX_HIP(c, hipMalloc(&b.Qs, 8 * (size_t)n_sparse));
+  TGX_HIP(c, hipMalloc(&b.alphas, 8 * (size_t)n_sparse));
+  TGX_HIP(c, hipMemcpy(b.Qs, Qs, 8 * (size_t)n_sparse, hipMemcpyHostToDevice));
+  TGX_HIP(c, hipMemcpy(b.alphas, ac.data(), 8 * (size_t)n_sparse, hipMemcpyHostToDevice));
+  c->twob.push_back(b);
+  c->maxnb = 0;
+  Extra& x = X(c);
+  if (!x.e0_dev.p) {
+    if (x.e0_dev.ensure(8 * TGX_MAX_SPECIES)) return set_err(c, TGX_ERR_HIP, "allocation failed");
+    TGX_HIP(c, hipMemcpy(x.e0_dev.p, c->e0, 8 * TGX_MAX_SPECIES, hipMemcpyHostToDevice));
+  }
+  return TGX_OK;
+}
+
+int tgx_clear_2b_models(tgx_ctx* c) {
+  if (!c) return TGX_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  for (auto& b : c->twob) {
+    (void)hipFree(b.Qs);
+    (void)hipFree(b.alphas);
+  }
+  c->twob.clear();
+  c->maxnb = 0;
+  return TGX_OK;
+}
+
+int tgx_neighbors_build(tgx_ctx* c, int n, const double* pos, const double* lattice, double rcut,
+                        int64_t cap, int* n_neigh, int* list, double* rjs, double* xyz,
+                        double* thetas, double* phis, int64_t* n_pairs, int* indices_out) {
+  if (!c) return TGX_ERR_ARG;
+  if (n < 1 || !pos || !lattice || !n_pairs) return set_err(c, TGX_ERR_ARG, "bad arguments");
+  (void)hipSetDevice(c->device);
+  Cell cell;
+  std::string err;
+  int rcode = make_cell(lattice, rcut, &cell, &err);
+  if (rcode) return set_err(c, rcode, "%s", err.c_str());
+  if (indices_out) {
+    indices_out[0] = cell.ia;
+    indices_out[1] = cell.ib;
+    indices_out[2] = cell.ic;
+  }
+  ENSURE(c, pos, 24 * (size_t)n);
+  for (int i = 0; i < 3 * n; i++)
+    if (!std::isfinite(pos[i])) return set_err(c, TGX_ERR_ARG, "non-finite position");
+  TGX_HIP(c, hipMemcpyAsync(c->pos.p, pos, 24 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  const int saved = c->maxnb;
+  c->maxnb = 0;
+  rcode = build_list(c, n, cell, rcut, false);
+  const int maxnb = c->maxnb;
+  c->maxnb = saved;
+  if (rcode) return rcode;
+  std::vector<int> cnt(n);
+  TGX_HIP(c, hipMemcpy(cnt.data(), c->nb_count.p, 4 * (size_t)n, hipMemcpyDeviceToHost));
+  std::vector<long long> off(n + 1, 0);
+  for (int i = 0; i < n; i++) off[i + 1] = off[i] + cnt[i] + 1;
+  *n_pairs = off[n];
+  if (off[n] > cap) return set_err(c, TGX_ERR_CAPACITY, "need %lld pairs, cap %lld", off[n], (long long)cap);
+  if (!n_neigh || !list) return set_err(c, TGX_ERR_ARG, "null output arrays");
+  const size_t np = (size_t)off[n];
+  ENSURE(c, csr_off, 8 * (size_t)(n + 1));
+  ENSURE(c, csr_out, (4 + 8 + 24 + 8 + 8) * np);
+  char* base = c->csr_out.as<char>();
+  int* d_list = reinterpret_cast<int*>(base);
+  double* d_rjs = reinterpret_cast<double*>(base + ((4 * np + 7) / 8 * 8));
+  double* d_xyz = d_rjs + np;
+  double* d_th = d_xyz + 3 * np;
+  double* d_ph = d_th + np;
+  TGX_HIP(c, hipMemcpyAsync(c->csr_off.p, off.data(), 8 * (size_t)(n + 1), hipMemcpyHostToDevice, c->stream));
+  TGX_HIP(c, launch_csr_out(c->pos.as<double>(), n, cell, maxnb, c->nb_count.as<int>(),
+                            c->nb_list.as<int>(), c->csr_off.as<long long>(), d_list, d_rjs, d_xyz,
+                            d_th, d_ph, c->stream));
+  TGX_HIP(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < n; i++) n_neigh[i] = cnt[i] + 1;
+  TGX_HIP(c, hipMemcpy(list, d_list, 4 * np, hipMemcpyDeviceToHost));
+  if (rjs) TGX_HIP(c, hipMemcpy(rjs, d_rjs, 8 * np, hipMemcpyDeviceToHost));
+  if (xyz) TGX_HIP(c, hipMemcpy(xyz, d_xyz, 24 * np, hipMemcpyDeviceToHost));
+  if (thetas) TGX_HIP(c, hipMemcpy(thetas, d_th, 8 * np, hipMemcpyDeviceToHost));
+  if (phis) TGX_HIP(c, hipMemcpy(phis, d_ph, 8 * np, hipMemcpyDeviceToHost));
+  return TGX_OK;
+}
+
+int tgx_soap_descriptors(tgx_ctx* c, int n, const double* pos, const int* species,
+                         const double* lattice, double* soap, int* n_sites_out, int* site_index) {
+  if (!c) return TGX_ERR_ARG;
+  if (n < 1 || !pos || !species || !lattice || !soap || !n_sites_out)
+    return set_err(c, TGX_ERR_ARG, "bad arguments");
+  if (!c->has_soap) return set_err(c, TGX_ERR_STATE, "no SOAP model loaded");
+  (void)hipSetDevice(c->device);
+  int rcode = upload_structure(c, n, pos, species);
+  if (rcode) return rcode;
+  Cell cell;
+  std::string err;
+  rcode = make_cell(lattice, c->sd.rcut_max, &cell, &err);
+  if (rcode) return set_err(c, rcode, "%s", err.c_str());
+  const int saved = c->maxnb;
+  c->maxnb = 0;
+  rcode = build_list(c, n, cell, c->sd.rcut_max, false);
+  if (rcode) {
+    c->maxnb = saved;
+    return rcode;
+  }
+  int n_sites = 0;
+  rcode = setup_sites(c, n, species, &n_sites);
+  if (!rcode && n_sites > 0) rcode = soap_forward(c, n, cell, n_sites);
+  c->maxnb = saved;
+  if (rcode) return rcode;
+  TGX_HIP(c, hipStreamSynchronize(c->stream));
+  const int Dp = c->dim_pad, dim = c->dim;
+  if (c->precision == TGX_FP64) {
+    std::vector<double> tmp((size_t)n_sites * Dp);
+    TGX_HIP(c, hipMemcpy(tmp.data(), c->q.p, 8 * tmp.size(), hipMemcpyDeviceToHost));
+    for (int s = 0; s < n_sites; s++)
+      std::memcpy(soap + (size_t)s * dim, tmp.data() + (size_t)s * Dp, 8 * (size_t)dim);
+  } else {
+    std::vector<float> tmp((size_t)n_sites * Dp);
+    TGX_HIP(c, hipMemcpy(tmp.data(), c->q.p, 4 * tmp.size(), hipMemcpyDeviceToHost));
+    for (int s = 0; s < n_sites; s++)
+      for (int d = 0; d < dim; d++) soap[(size_t)s * dim + d] = tmp[(size_t)s * Dp + d];
+  }
+  *n_sites_out = n_sites;
+  if (site_index) {
+    std::vector<int> sa(n_sites);
+    if (n_sites)
+      TGX_HIP(c, hipMemcpy(sa.data(), c->site_atom.p, 4 * (size_t)n_sites, hipMemcpyDeviceToHost));
+    for (int s = 0; s < n_sites; s++) site_index[s] = sa[s] + 1;
+  }
+  return TGX_OK;
+}
+
+int tgx_gap_kernel_step(tgx_ctx* c, int n_sites, const double* soap, double* energies,
+                        double* qss) {
+  if (!c) return TGX_ERR_ARG;
+  if (n_sites < 0 || (n_sites > 0 && (!soap || !energies)))
+    return set_err(c, TGX_ERR_ARG, "bad arguments");
+  if (!c->has_soap) return set_err(c, TGX_ERR_STATE, "no SOAP model loaded");
+  if (n_sites == 0) return TGX_OK;
+  (void)hipSetDevice(c->device);
+  const int prec = c->precision, Dp = c->dim_pad, dim = c->dim;
+  const size_t ts = prec == TGX_FP64 ? 8 : (prec == TGX_FP32 ? 4 : 2);
+  if (prec == TGX_MIXED) {
+    ENSURE(c, qb, 2 * (size_t)n_sites * Dp);
+    std::vector<unsigned short> t((size_t)n_sites * Dp, 0);
+    for (int s = 0; s < n_sites; s++)
+      for (int d = 0; d < dim; d++) {
+        float f = (float)soap[(size_t)s * dim + d];
+        unsigned int u;
+        std::memcpy(&u, &f, 4);
+        t[(size_t)s * Dp + d] = (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+      }
+    TGX_HIP(c, hipMemcpy(c->qb.p, t.data(), 2 * t.size(), hipMemcpyHostToDevice));
+  } else {
+    ENSURE(c, q, ts * (size_t)n_sites * Dp);
+    if (prec == TGX_FP64) {
+      std::vector<double> t((size_t)n_sites * Dp, 0.0);
+      for (int s = 0; s < n_sites; s++)
+        std::memcpy(t.data() + (size_t)s * Dp, soap + (size_t)s * dim, 8 * (size_t)dim);
+      TGX_HIP(c, hipMemcpy(c->q.p, t.data(), 8 * t.size(), hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> t((size_t)n_sites * Dp, 0.f);
+      for (int s = 0; s < n_sites; s++)
+        for (int d = 0; d < dim; d++) t[(size_t)s * Dp + d] = (float)soap[(size_t)s * dim + d];
+      TGX_HIP(c, hipMemcpy(c->q.p, t.data(), 4 * t.size(), hipMemcpyHostToDevice));
+    }
+  }
+  int rcode = kernel_step(c, n_sites, qss != nullptr);
+  if (rcode) return rcode;
+  TGX_HIP(c, hipStreamSynchronize(c->stream));
+  TGX_HIP(c, hipMemcpy(energies, c->esite.p, 8 * (size_t)n_sites, hipMemcpyDeviceToHost));
+  if (qss) {
+    if (prec == TGX_FP64) {
+      std::vector<double> g((size_t)n_sites * Dp);
+      TGX_HIP(c, hipMemcpy(g.data(), c->gmat.p, 8 * g.size(), hipMemcpyDeviceToHost));
+      for (int s = 0; s < n_sites; s++)
+        for (int d = 0; d < dim; d++) qss[(size_t)d * n_sites + s] = -g[(size_t)s * Dp + d];
+    } else {
+      std::vector<float> g((size_t)n_sites * Dp);
+      TGX_HIP(c, hipMemcpy(g.data(), c->gmat.p, 4 * g.size(), hipMemcpyDeviceToHost));
+      for (int s = 0; s < n_sites; s++)
+        for (int d = 0; d < dim; d++) qss[(size_t)d * n_sites + s] = -(double)g[(size_t)s * Dp + d];
+    }
+  }
+  return TGX_OK;
+}
+
+int tgx_energy_forces(tgx_ctx* c, int n, const double* pos, const int* species,
+                      const double* lattice, int do_forces, double* e_atom, double* forces,
+                      double* virial) {
+  if (!c) return TGX_ERR_ARG;
+  if (n < 1 || !pos || !species || !lattice || !e_atom || (do_forces && (!forces || !virial)))
+    return set_err(c, TGX_ERR_ARG, "bad arguments");
+  (void)hipSetDevice(c->device);
+  int rcode = upload_structure(c, n, pos, species);
+  if (rcode) return rcode;
+  rcode = run_ef(c, n, lattice, species, do_forces);
+  if (rcode) return rcode;
+  TGX_HIP(c, hipStreamSynchronize(c->stream));
+  collect_times(c);
+  TGX_HIP(c, hipMemcpy(e_atom, c->eatom.p, 8 * (size_t)n, hipMemcpyDeviceToHost));
+  if (do_forces) {
+    TGX_HIP(c, hipMemcpy(forces, c->forces.p, 24 * (size_t)n, hipMemcpyDeviceToHost));
+    TGX_HIP(c, hipMemcpy(virial, X(c).out.p, 72, hipMemcpyDeviceToHost));
+  }
+  return TGX_OK;
+}
+
+int tgx_energy_forces_batched(tgx_ctx* c, int n_struct, const int* offsets, const double* pos,
+                              const int* species, const double* lattices, int do_forces,
+                              double* e_atom, double* forces, double* virials) {
+  if (!c) return TGX_ERR_ARG;
+  if (n_struct < 1 || !offsets || !pos || !species || !lattices || !e_atom ||
+      (do_forces && (!forces || !virials)))
+    return set_err(c, TGX_ERR_ARG, "bad arguments");
+  for (int s = 0; s < n_struct; s++) {
+    const int a = offsets[s], b = offsets[s + 1];
+    if (b <= a) return set_err(c, TGX_ERR_ARG, "structure %d is empty", s);
+    int rcode = tgx_energy_forces(c, b - a, pos + 3 * (size_t)a, species + a, lattices + 9 * (size_t)s,
+                                  do_forces, e_atom + a, do_forces ? forces + 3 * (size_t)a : nullptr,
+                                  do_forces ? virials + 9 * (size_t)s : nullptr);
+    if (rcode) return rcode;
+  }
+  return TGX_OK;
+}
+
+int tgx_md_run(tgx_ctx* c, int n, int n_steps, double dt, double skin, const int* species,
+               const double* lattice, const double* mass, double* pos, double* vel,
+               double* forces, double* e_pot, double* e_kin, int* n_rebuilds) {
+  (void)n; (void)n_steps; (void)dt; (void)skin; (void)species; (void)lattice; (void)mass;
+  (void)pos; (void)vel; (void)forces; (void)e_pot; (void)e_kin; (void)n_rebuilds;
+  if (!c) return TGX_ERR_ARG;
+  return set_err(c, TGX_ERR_UNSUPPORTED, "md_run not built yet");
+}
+
+int tgx_poly3_orthonormalization(int alpha_max, double* S, double* W) {
+  if (alpha_max < 1 || alpha_max > TGX_MAX_NR || !W) return TGX_ERR_ARG;
+  std::vector<double> w((size_t)TGX_MAX_NR * TGX_MAX_NR);
+  poly3_W(alpha_max, w.data());
+  for (int i = 0; i < alpha_max; i++)
+    for (int j = 0; j < alpha_max; j++) {
+      W[j * alpha_max + i] = w[(size_t)i * TGX_MAX_NR + j];
+      if (S)
+        S[j * alpha_max + i] =
+            std::sqrt((2.0 * i + 7.0) * (2.0 * j + 7.0)) / (double)(i + j + 7);
+    }
+  return TGX_OK;
+}
+
+int tgx_stage_times(const tgx_ctx* c, double* ms7) {
+  if (!c || !ms7) return TGX_ERR_ARG;
+  for (int k = 0; k < 7; k++) ms7[k] = c->stage_ms[k];
+  return TGX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,254 @@
+// Device-side math shared by the tgx kernels: minimum-image distance with the
+// reference's FP64 operation order, poly3 radial closed form, radial table
+// lookup, and real spherical-harmonic recurrences.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "tgx_internal.h"
+
+namespace tgx {
+
+// Normalised associated-Legendre recurrence constants (real harmonics without
+// Condon-Shortley phase): P(l,m) = A[l][m] z P(l-1,m) - B[l][m] P(l-2,m),
+// P(m,m) = Pmm[m]; the sqrt(2) of m > 0 is folded into Pmm.
+struct SphConst {
+  float A[TGX_L2MAX], B[TGX_L2MAX], Pmm[TGX_MAX_LMAX + 1];
+  double Ad[TGX_L2MAX], Bd[TGX_L2MAX], Pmmd[TGX_MAX_LMAX + 1];
+};
+extern __constant__ SphConst c_sph;
+
+template <typename T> struct SphK;
+template <> struct SphK<float> {
+  __device__ static float A(int i) { return c_sph.A[i]; }
+  __device__ static float B(int i) { return c_sph.B[i]; }
+  __device__ static float P(int m) { return c_sph.Pmm[m]; }
+};
+template <> struct SphK<double> {
+  __device__ static double A(int i) { return c_sph.Ad[i]; }
+  __device__ static double B(int i) { return c_sph.Bd[i]; }
+  __device__ static double P(int m) { return c_sph.Pmmd[m]; }
+};
+
+// Fortran MODULO(a,p) for reals in gfortran's formulation (fmod plus sign
+// fix), then the L/2 fold and the x,y,z sum of squares in that order
+// (get_distance, reference neighbors_interface.f90:565-581).  Contraction is
+// off so the FP64 result is bit-identical to the reference arithmetic.
+__device__ __forceinline__ double f_modulo(double a, double p) {
+#pragma clang fp contract(off)
+  double r = fmod(a, p);
+  if (r != 0.0) {
+    if ((a < 0.0) != (p < 0.0)) r = r + p;
+  } else {
+    r = copysign(0.0, p);
+  }
+  return r;
+}
+
+__device__ __forceinline__ double mic(const double* pi, const double* pj,
+                                      double La, double Lb, double Lc,
+                                      double* dv) {
+#pragma clang fp contract(off)
+  double L[3] = {La, Lb, Lc};
+  double d2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    double t = f_modulo(pj[k] - pi[k], L[k]);
+    if (t > L[k] / 2.0) t = t - L[k];
+    dv[k] = t;
+    d2 = d2 + t * t;
+  }
+  return sqrt(d2);
+}
+
+// Position of supercell image `img` of atom j, summed as set_supercell does
+// (xyz/read_xyz.f90:336-339): ((p + i2 a) + j2 b) + k2 c.
+__device__ __forceinline__ void image_pos(const double* pos, int j, int img,
+                                          const Cell& c, double* out) {
+#pragma clang fp contract(off)
+  const double* p = pos + 3 * j;
+  if (img == 0) {
+    out[0] = p[0];
+    out[1] = p[1];
+    out[2] = p[2];
+    return;
+  }
+  int k2 = img % c.ic;
+  int j2 = (img / c.ic) % c.ib;
+  int i2 = img / (c.ic * c.ib);
+  // orthorhombic: a = (a,0,0), b = (0,b,0), c = (0,0,c)
+  out[0] = ((p[0] + (double)i2 * c.a) + (double)j2 * 0.0) + (double)k2 * 0.0;
+  out[1] = ((p[1] + (double)i2 * 0.0) + (double)j2 * c.b) + (double)k2 * 0.0;
+  out[2] = ((p[2] + (double)i2 * 0.0) + (double)j2 * 0.0) + (double)k2 * c.c;
+}
+
+// Pair vector r_j(img) - r_i with the reference's minimum image.
+__device__ __forceinline__ double pair_vec(const double* pos, int i, int packed,
+                                           const Cell& c, double* dv) {
+  int j = packed & TGX_ATOM_MASK;
+  int img = packed >> TGX_IMG_SHIFT;
+  double pj[3];
+  image_pos(pos, j, img, c, pj);
+  return mic(pos + 3 * i, pj, c.La, c.Lb, c.Lc, dv);
+}
+
+// ---------------------------------------------------------------------------
+// Radial basis
+// ---------------------------------------------------------------------------
+#define TGX_JMAX (TGX_MAX_NR + 8)
+
+// Unnormalised poly3 integrals combined with W for species s:
+// B_n(r) = sum_a W[n][a] N_a int_0^rh rho^p (rh-rho)^(a+2) G(rho; r, sig(r))
+// (G the unit-normalised 1-D Gaussian) and dB_n/dr.  FP64 throughout.
+template <int NMX>
+__device__ inline void radial_basis_fp64(const SoapDev* sd, const double* W,
+                                         int s, double r, double* Bv,
+                                         double* dB) {
+  const int nmax = sd->nmax[s];
+  const int p = sd->radial_enhancement;
+  const double rh = sd->rh[s];
+  const double sig0 = sd->sig_r[s], sigs = sd->sig_rs[s];
+  const double sg = sig0 + sigs * r;
+  const int kmax = nmax + 6;
+  double J[NMX + 8];
+  const double c = rh - r, s2 = sg * sg;
+  const double er = exp(-r * r / (2.0 * s2)), ec = exp(-c * c / (2.0 * s2));
+  J[0] = sg * 1.2533141373155002512 *
+         (erf(r / (sg * 1.4142135623730950488)) +
+          erf(c / (sg * 1.4142135623730950488)));
+  J[1] = c * J[0] + s2 * (ec - er);
+  double rhp = 1.0;
+#pragma unroll
+  for (int k = 2; k < NMX + 8; k++) {
+    rhp *= rh;
+    J[k] = k <= kmax ? c * J[k - 1] + (k - 1) * s2 * J[k - 2] - s2 * rhp * er : 0.0;
+  }
+  const double pre = 1.0 / (2.5066282746310005024 * sg);
+  double I[NMX], dI[NMX];
+#pragma unroll
+  for (int a = 1; a <= NMX; a++) {
+    if (a > nmax) {
+      I[a - 1] = dI[a - 1] = 0.0;
+      continue;
+    }
+    const int d = a + 2;
+    double Na = sqrt((2.0 * a + 5.0) / pow(rh, 2.0 * a + 5.0));
+    double T, Tc, Ts;
+    double j0 = J[d], j1 = J[d + 1], j2 = J[d + 2], j3 = J[d + 3], j4 = J[d + 4];
+    // dJ_k/dc = (J_{k+1} - c J_k)/s2 ; dJ_k/dsig = (J_{k+2} - 2c J_{k+1} + c^2 J_k)/s2/sg
+    double c0 = (j1 - c * j0) / s2, c1 = (j2 - c * j1) / s2, c2 = (j3 - c * j2) / s2;
+    double s0 = (j2 - 2.0 * c * j1 + c * c * j0) / (s2 * sg);
+    double s1 = (j3 - 2.0 * c * j2 + c * c * j1) / (s2 * sg);
+    double s2_ = (j4 - 2.0 * c * j3 + c * c * j2) / (s2 * sg);
+    if (p == 0) {
+      T = j0; Tc = c0; Ts = s0;
+    } else if (p == 1) {
+      T = rh * j0 - j1; Tc = rh * c0 - c1; Ts = rh * s0 - s1;
+    } else {
+      T = rh * rh * j0 - 2.0 * rh * j1 + j2;
+      Tc = rh * rh * c0 - 2.0 * rh * c1 + c2;
+      Ts = rh * rh * s0 - 2.0 * rh * s1 + s2_;
+    }
+    I[a - 1] = Na * pre * T;
+    dI[a - 1] = Na * (pre * (-Tc + Ts * sigs) - pre / sg * sigs * T);
+  }
+  const double* Ws = W + s * TGX_MAX_NR * TGX_MAX_NR;
+#pragma unroll
+  for (int n = 0; n < NMX; n++) {
+    double v = 0.0, dv = 0.0;
+#pragma unroll
+    for (int a = 0; a < NMX; a++) {
+      if (a < nmax) {
+        v += Ws[n * TGX_MAX_NR + a] * I[a];
+        dv += Ws[n * TGX_MAX_NR + a] * dI[a];
+      }
+    }
+    Bv[n] = n < nmax ? v : 0.0;
+    dB[n] = n < nmax ? dv : 0.0;
+  }
+}
+
+// Amplitude (polynomial scaling of the reference's scaling_mode
+// "polynomial", times global_scaling) and the soft cutoff between rcut_soft
+// and rcut_hard.  Evaluated in T.
+template <typename T>
+__device__ __forceinline__ void radial_envelope(const SoapDev* sd, int s, T r,
+                                                T* e, T* de) {
+  const T rh = (T)sd->rh[s], rs = (T)sd->rs[s];
+  const T x = r / rh;
+  const T poly = (T)1 + (T)2 * x * x * x - (T)3 * x * x;
+  const T dpoly = ((T)6 * x * x - (T)6 * x) / rh;
+  const T as = (T)sd->amp_scal[s];
+  T amp, damp;
+  if (as == (T)0) {
+    amp = 1;
+    damp = 0;
+  } else if (as == (T)1) {
+    amp = poly;
+    damp = dpoly;
+  } else {
+    amp = pow(poly, as);
+    damp = poly > (T)0 ? as * pow(poly, as - (T)1) * dpoly : (T)0;
+  }
+  T fc = 1, dfc = 0;
+  if (r >= rh) {
+    fc = 0;
+  } else if (rs < rh && r > rs) {
+    const T y = (r - rs) / (rh - rs);
+    fc = (T)1 - (T)3 * y * y + (T)2 * y * y * y;
+    dfc = ((T)-6 * y + (T)6 * y * y) / (rh - rs);
+  }
+  const T g = (T)sd->glob[s];
+  *e = g * amp * fc;
+  *de = g * (damp * fc + amp * dfc);
+}
+
+// FP32 lookup of B_n(r), dB_n/dr from the cubic-Hermite table
+// tab[s][k][n] = (B_n(r_k), h dB_n/dr(r_k)), r_k = k h, h = rh/TGX_RTAB_N.
+template <int NMX>
+__device__ __forceinline__ void radial_basis_tab(const SoapDev* sd,
+                                                 const float2* tab, int s,
+                                                 float r, float* Bv,
+                                                 float* dB, int nmax) {
+  const float rh = (float)sd->rh[s];
+  const float hinv = (float)TGX_RTAB_N / rh;
+  float x = r * hinv;
+  int k = (int)x;
+  k = k < 0 ? 0 : (k > TGX_RTAB_N - 1 ? TGX_RTAB_N - 1 : k);
+  const float t = x - (float)k;
+  const float t2 = t * t, t3 = t2 * t;
+  const float h00 = 2.f * t3 - 3.f * t2 + 1.f, h10 = t3 - 2.f * t2 + t;
+  const float h01 = -2.f * t3 + 3.f * t2, h11 = t3 - t2;
+  const float g00 = 6.f * t2 - 6.f * t, g10 = 3.f * t2 - 4.f * t + 1.f;
+  const float g01 = -6.f * t2 + 6.f * t, g11 = 3.f * t2 - 2.f * t;
+  const float2* r0 = tab + ((size_t)s * (TGX_RTAB_N + 1) + k) * TGX_MAX_NR;
+  const float2* r1 = r0 + TGX_MAX_NR;
+#pragma unroll
+  for (int n = 0; n < NMX; n++) {
+    if (n < nmax) {
+      float2 a = r0[n], b = r1[n];
+      Bv[n] = h00 * a.x + h10 * a.y + h01 * b.x + h11 * b.y;
+      dB[n] = (g00 * a.x + g10 * a.y + g01 * b.x + g11 * b.y) * hinv;
+    } else {
+      Bv[n] = dB[n] = 0.f;
+    }
+  }
+}
+
+// Angular smearing A_l = exp(-l(l+1) sig_t^2 / (2 r^2)), sig_t = sig_t0 +
+// sig_ts r.  Returns log2 of t = exp(-sig_t^2/(2 r^2)) and the factor D with
+// dA_l/dr = A_l l(l+1) D.
+template <typename T>
+__device__ __forceinline__ void ang_params(const SoapDev* sd, int s, T r,
+                                           T* log2t, T* D) {
+  const T st = (T)sd->sig_t[s] + (T)sd->sig_ts[s] * r;
+  const T sts = (T)sd->sig_ts[s];
+  const T ir = (T)1 / r;
+  *log2t = -(T)0.72134752044448170368 * st * st * ir * ir;  // -st^2/(2r^2)/ln2
+  *D = st * st * ir * ir * ir - st * sts * ir * ir;
+}
+
+__device__ __forceinline__ float exp2_t(float x) { return exp2f(x); }
+__device__ __forceinline__ double exp2_t(double x) { return exp2(x); }
+
+}  // namespace tgx

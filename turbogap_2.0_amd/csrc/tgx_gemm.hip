@@ -1,0 +1,272 @@
+// GAP kernel step on MFMA (reference gap/gap.f90:118-171):
+//   K = q Qs^T                      (GEMM 1, epilogue: E partials, W)
+//   E_i = delta^2 sum_s K_is^zeta alpha_s
+//   W_is = zeta delta^2 alpha_s K_is^(zeta-1)
+//   G = W Qs = dE/dq                (GEMM 2)
+// Both are "NT" products C[M][N] = A[M][K] B[N][K]^T with K contiguous.
+// Inputs FP64 (v_mfma_f64_16x16x4), FP32 (v_mfma_f32_16x16x4) or BF16
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation).
+#include <hip/hip_runtime.h>
+
+#include "tgx_internal.h"
+
+namespace tgx {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16;
+
+template <typename TI> struct MF;
+template <> struct MF<double> {
+  typedef double acc_t;
+  typedef f64x4 vacc;
+  static constexpr int BK = 8, KSTEP = 4, KPL = 1, PAD = 1;
+  typedef double frag;
+  __device__ static frag load(const double* s) { return *s; }
+  __device__ static vacc mma(frag a, frag b, vacc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct MF<float> {
+  typedef float acc_t;
+  typedef f32x4 vacc;
+  static constexpr int BK = 16, KSTEP = 4, KPL = 1, PAD = 1;
+  typedef float frag;
+  __device__ static frag load(const float* s) { return *s; }
+  __device__ static vacc mma(frag a, frag b, vacc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct MF<u16> {
+  typedef float acc_t;
+  typedef f32x4 vacc;
+  static constexpr int BK = 32, KSTEP = 32, KPL = 8, PAD = 8;
+  typedef bf16x8 frag;
+  __device__ static frag load(const u16* s) {
+    return *reinterpret_cast<const bf16x8*>(s);
+  }
+  __device__ static vacc mma(frag a, frag b, vacc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+};
+
+__device__ __forceinline__ u16 to_bf16(float f) {
+  unsigned int u = __float_as_uint(f);
+  return (u16)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+template <typename TI> __device__ __forceinline__ TI cvt_out(float v);
+template <> __device__ __forceinline__ float cvt_out<float>(float v) { return v; }
+template <> __device__ __forceinline__ u16 cvt_out<u16>(float v) { return to_bf16(v); }
+__device__ __forceinline__ double cvt_out_d(double v) { return v; }
+
+struct EpiKernel {
+  const void* alphas;  // acc_t, padded to N (zeros beyond n_sparse)
+  int n_sparse;
+  int zint;            // >0: integer zeta
+  double zeta, zd2;    // zeta, zeta*delta^2
+  void* W;             // TI [M][ldw]
+  int ldw;
+  void* epart;         // acc_t [gridDim.x][M]
+};
+struct EpiStore {
+  void* C;  // acc_t [M][ldc]
+  int ldc;
+};
+
+template <typename T>
+__device__ __forceinline__ void kpow(T k, int zint, T zeta, T* kz, T* kz1) {
+  if (zint > 0) {
+    T p = 1;
+    for (int i = 1; i < zint; i++) p *= k;
+    *kz1 = p;
+    *kz = p * k;
+  } else {
+    *kz = pow(k, zeta);
+    *kz1 = pow(k, zeta - (T)1);
+  }
+}
+
+// EPI 0: kernel epilogue; EPI 1: plain store.
+template <typename TI, int EPI>
+__global__ __launch_bounds__(256) void gemm_nt_kernel(int M, int N, int K,
+                                                     const TI* __restrict__ A, int lda,
+                                                     const TI* __restrict__ B, int ldb,
+                                                     EpiKernel ek, EpiStore es) {
+  typedef MF<TI> F;
+  typedef typename F::acc_t acc_t;
+  typedef typename F::vacc vacc;
+  constexpr int BM = 64, BN = 64, BK = F::BK, LDS = BK + F::PAD;
+  constexpr int EPR = 16 / sizeof(TI);  // elements per 16-byte load
+  __shared__ __align__(16) TI sA[BM * LDS];
+  __shared__ __align__(16) TI sB[BN * LDS];
+  __shared__ acc_t sE[2][BM];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int row0 = blockIdx.y * BM, col0 = blockIdx.x * BN;
+  const int lr = tid >> 2, lc = (tid & 3) * EPR;
+
+  vacc acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) acc[a][b] = vacc{0, 0, 0, 0};
+
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    {
+      const int r = row0 + lr;
+      TI tmp[EPR];
+      if (r < M) {
+        const uint4 v = *reinterpret_cast<const uint4*>(A + (size_t)r * lda + k0 + lc);
+        *reinterpret_cast<uint4*>(tmp) = v;
+      } else {
+#pragma unroll
+        for (int e = 0; e < EPR; e++) tmp[e] = (TI)0;
+      }
+#pragma unroll
+      for (int e = 0; e < EPR; e++) sA[lr * LDS + lc + e] = tmp[e];
+      const int c = col0 + lr;
+      if (c < N) {
+        const uint4 v = *reinterpret_cast<const uint4*>(B + (size_t)c * ldb + k0 + lc);
+        *reinterpret_cast<uint4*>(tmp) = v;
+      } else {
+#pragma unroll
+        for (int e = 0; e < EPR; e++) tmp[e] = (TI)0;
+      }
+#pragma unroll
+      for (int e = 0; e < EPR; e++) sB[lr * LDS + lc + e] = tmp[e];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += F::KSTEP) {
+      const int kk = ks + (lane >> 4) * F::KPL;
+      typename F::frag af[2], bfr[2];
+#pragma unroll
+      for (int mi = 0; mi < 2; mi++)
+        af[mi] = F::load(sA + (wm * 32 + mi * 16 + (lane & 15)) * LDS + kk);
+#pragma unroll
+      for (int ni = 0; ni < 2; ni++)
+        bfr[ni] = F::load(sB + (wn * 32 + ni * 16 + (lane & 15)) * LDS + kk);
+#pragma unroll
+      for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+        for (int ni = 0; ni < 2; ni++) acc[mi][ni] = F::mma(af[mi], bfr[ni], acc[mi][ni]);
+    }
+    __syncthreads();
+  }
+
+  if (EPI == 1) {
+    acc_t* C = reinterpret_cast<acc_t*>(es.C);
+#pragma unroll
+    for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+      for (int ni = 0; ni < 2; ni++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int r = row0 + wm * 32 + mi * 16 + (lane >> 4) * 4 + v;
+          const int c = col0 + wn * 32 + ni * 16 + (lane & 15);
+          if (r < M && c < es.ldc) C[(size_t)r * es.ldc + c] = acc[mi][ni][v];
+        }
+    return;
+  }
+  // kernel epilogue
+  const acc_t* al = reinterpret_cast<const acc_t*>(ek.alphas);
+  const acc_t zeta = (acc_t)ek.zeta, zd2 = (acc_t)ek.zd2;
+  TI* Wm = reinterpret_cast<TI*>(ek.W);
+  acc_t rs[2][4];
+#pragma unroll
+  for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) rs[mi][v] = 0;
+#pragma unroll
+  for (int ni = 0; ni < 2; ni++) {
+    const int c = col0 + wn * 32 + ni * 16 + (lane & 15);
+    const bool valid = c < ek.n_sparse;
+    const acc_t a = valid ? al[c] : (acc_t)0;
+#pragma unroll
+    for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = row0 + wm * 32 + mi * 16 + (lane >> 4) * 4 + v;
+        const acc_t kv = acc[mi][ni][v];
+        acc_t kz = 0, kz1 = 0;
+        if (valid) kpow<acc_t>(kv, ek.zint, zeta, &kz, &kz1);
+        rs[mi][v] += a * kz;
+        const acc_t w = valid ? zd2 * a * kz1 : (acc_t)0;
+        if (r < M && c < ek.ldw) {
+          if constexpr (sizeof(TI) == 8) Wm[(size_t)r * ek.ldw + c] = w;
+          else Wm[(size_t)r * ek.ldw + c] = cvt_out<TI>((float)w);
+        }
+      }
+  }
+  // reduce row partials over the 16 lanes of a row group
+#pragma unroll
+  for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      acc_t x = rs[mi][v];
+      x += __shfl_xor(x, 1, 64);
+      x += __shfl_xor(x, 2, 64);
+      x += __shfl_xor(x, 4, 64);
+      x += __shfl_xor(x, 8, 64);
+      if ((lane & 15) == 0) sE[wn][wm * 32 + mi * 16 + (lane >> 4) * 4 + v] = x;
+    }
+  __syncthreads();
+  if (tid < BM) {
+    const int r = row0 + tid;
+    if (r < M)
+      reinterpret_cast<acc_t*>(ek.epart)[(size_t)blockIdx.x * M + r] = sE[0][tid] + sE[1][tid];
+  }
+}
+
+template <typename TA>
+__global__ void esum_kernel(int M, int ntiles, const TA* epart, double d2,
+                            double* esite) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= M) return;
+  double s = 0.0;
+  for (int t = 0; t < ntiles; t++) s += (double)epart[(size_t)t * M + r];
+  esite[r] = d2 * s;
+}
+
+template <typename TI>
+static hipError_t run(int n_sites, int dim, int dim_pad, int n_sparse, int sp_pad,
+                      const void* q, const void* qs, const void* qsT, const void* alphas,
+                      double delta, double zeta, void* wmat, void* epart, double* esite,
+                      void* G, int want_grad, hipStream_t st) {
+  typedef typename MF<TI>::acc_t acc_t;
+  const int M = n_sites;
+  if (M == 0) return hipSuccess;
+  const int zint = fabs(zeta - (double)(int)zeta) < 1e-5 ? (int)zeta : 0;
+  const double zz = zint > 0 ? (double)zint : zeta;
+  EpiKernel ek{alphas, n_sparse, zint, zz, zz * delta * delta, wmat, sp_pad, epart};
+  EpiStore es{G, dim_pad};
+  dim3 g1(sp_pad / 64, (M + 63) / 64);
+  hipLaunchKernelGGL((gemm_nt_kernel<TI, 0>), g1, dim3(256), 0, st, M, sp_pad, dim_pad,
+                     (const TI*)q, dim_pad, (const TI*)qs, dim_pad, ek, es);
+  hipLaunchKernelGGL(esum_kernel<acc_t>, dim3((M + 255) / 256), dim3(256), 0, st, M,
+                     (int)g1.x, (const acc_t*)epart, delta * delta, esite);
+  if (want_grad) {
+    dim3 g2((dim_pad + 63) / 64, (M + 63) / 64);
+    hipLaunchKernelGGL((gemm_nt_kernel<TI, 1>), g2, dim3(256), 0, st, M, dim_pad, sp_pad,
+                       (const TI*)wmat, sp_pad, (const TI*)qsT, sp_pad, ek, es);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_gap_kernel(int mode, int n_sites, int dim, int dim_pad, int n_sparse,
+                             int sp_pad, const void* q, const void* qs, const void* qsT,
+                             const void* alphas, double delta, double zeta, void* wmat,
+                             void* epart, double* esite, void* G, int want_grad,
+                             hipStream_t st) {
+  if (mode == TGX_FP64)
+    return run<double>(n_sites, dim, dim_pad, n_sparse, sp_pad, q, qs, qsT, alphas, delta, zeta,
+                       wmat, epart, esite, G, want_grad, st);
+  if (mode == TGX_FP32)
+    return run<float>(n_sites, dim, dim_pad, n_sparse, sp_pad, q, qs, qsT, alphas, delta, zeta,
+                      wmat, epart, esite, G, want_grad, st);
+  return run<u16>(n_sites, dim, dim_pad, n_sparse, sp_pad, q, qs, qsT, alphas, delta, zeta, wmat,
+                  epart, esite, G, want_grad, st);
+}
+
+}  // namespace tgx

@@ -1,0 +1,223 @@
+// Host-side model math of the tgx library: the poly3 orthonormalisation
+// matrix W = S^(-1/2) (computed in binary128 and rounded once, because
+// cond(S) ~ 1e13 at n_max = 8), the FP64 radial closed form used for the
+// central-atom term, spherical-harmonic recurrence constants and the cell.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "tgx_internal.h"
+
+namespace tgx {
+
+namespace {
+
+typedef __float128 qd;
+
+qd qsqrt(qd v) {
+  if (v <= 0) return 0;
+  qd x = (qd)std::sqrt((double)v);
+  for (int i = 0; i < 5; i++) x = (x + v / x) / 2;
+  return x;
+}
+
+// Symmetric eigen-decomposition A = V diag(w) V^T by cyclic Jacobi, binary128.
+void eig_sym(int n, qd* a, qd* v, qd* w) {
+  for (int i = 0; i < n * n; i++) v[i] = 0;
+  for (int i = 0; i < n; i++) v[i * n + i] = 1;
+  for (int sweep = 0; sweep < 64; sweep++) {
+    qd off = 0, dg = 0;
+    for (int i = 0; i < n; i++) {
+      dg += a[i * n + i] * a[i * n + i];
+      for (int j = i + 1; j < n; j++) off += a[i * n + j] * a[i * n + j];
+    }
+    if (off <= dg * (qd)1e-60 * (qd)1e-6) break;
+    for (int p = 0; p < n - 1; p++)
+      for (int q = p + 1; q < n; q++) {
+        const qd apq = a[p * n + q];
+        if (apq == 0) continue;
+        const qd th = (a[q * n + q] - a[p * n + p]) / (2 * apq);
+        const qd t = (th >= 0 ? 1 : -1) / ((th >= 0 ? th : -th) + qsqrt(th * th + 1));
+        const qd c = 1 / qsqrt(t * t + 1), s = t * c;
+        for (int k = 0; k < n; k++) {
+          const qd x = a[k * n + p], y = a[k * n + q];
+          a[k * n + p] = c * x - s * y;
+          a[k * n + q] = s * x + c * y;
+        }
+        for (int k = 0; k < n; k++) {
+          const qd x = a[p * n + k], y = a[q * n + k];
+          a[p * n + k] = c * x - s * y;
+          a[q * n + k] = s * x + c * y;
+        }
+        for (int k = 0; k < n; k++) {
+          const qd x = v[k * n + p], y = v[k * n + q];
+          v[k * n + p] = c * x - s * y;
+          v[k * n + q] = s * x + c * y;
+        }
+      }
+  }
+  for (int i = 0; i < n; i++) w[i] = a[i * n + i];
+}
+
+}  // namespace
+
+// Overlap of phi_a(r) = sqrt((2a+5)/rh^(2a+5)) (rh - r)^(a+2) on [0, rh]:
+// S_ab = sqrt((2a+5)(2b+5))/(a+b+5) for a, b = 1..nmax; W = S^(-1/2).
+void poly3_W(int nmax, double* W) {
+  qd S[TGX_MAX_NR * TGX_MAX_NR], V[TGX_MAX_NR * TGX_MAX_NR], w[TGX_MAX_NR];
+  for (int a = 0; a < nmax; a++)
+    for (int b = 0; b < nmax; b++)
+      S[a * nmax + b] = qsqrt((qd)((2 * a + 7) * (2 * b + 7))) / (qd)(a + b + 7);
+  eig_sym(nmax, S, V, w);
+  for (int i = 0; i < nmax; i++)
+    for (int j = 0; j < nmax; j++) {
+      qd acc = 0;
+      for (int k = 0; k < nmax; k++) acc += V[i * nmax + k] * V[j * nmax + k] / qsqrt(w[k]);
+      W[i * TGX_MAX_NR + j] = (double)acc;
+    }
+}
+
+// FP64 closed form of R_n(r) (no envelope) for species s: see the device
+// version in tgx_device.h; used on the host for the central-atom term.
+void radial_closed_form(const SoapDev& sd, const double* W, int s, double r, double* R,
+                        double* /*dR*/) {
+  const int nmax = sd.nmax[s], p = sd.radial_enhancement;
+  const double rh = sd.rh[s];
+  const double sg = sd.sig_r[s] + sd.sig_rs[s] * r;
+  const double c = rh - r, s2 = sg * sg;
+  const double er = std::exp(-r * r / (2 * s2)), ec = std::exp(-c * c / (2 * s2));
+  double J[TGX_MAX_NR + 8];
+  J[0] = sg * std::sqrt(M_PI / 2) * (std::erf(r / (sg * std::sqrt(2.0))) + std::erf(c / (sg * std::sqrt(2.0))));
+  J[1] = c * J[0] + s2 * (ec - er);
+  double rhp = 1;
+  for (int k = 2; k < nmax + 8; k++) {
+    rhp *= rh;
+    J[k] = c * J[k - 1] + (k - 1) * s2 * J[k - 2] - s2 * rhp * er;
+  }
+  const double pre = 1.0 / (std::sqrt(2 * M_PI) * sg);
+  double I[TGX_MAX_NR];
+  for (int a = 1; a <= nmax; a++) {
+    const int d = a + 2;
+    const double Na = std::sqrt((2.0 * a + 5.0) / std::pow(rh, 2.0 * a + 5.0));
+    double T;
+    if (p == 0) T = J[d];
+    else if (p == 1) T = rh * J[d] - J[d + 1];
+    else T = rh * rh * J[d] - 2 * rh * J[d + 1] + J[d + 2];
+    I[a - 1] = Na * pre * T;
+  }
+  const double* Ws = W + s * TGX_MAX_NR * TGX_MAX_NR;
+  for (int n = 0; n < nmax; n++) {
+    double v = 0;
+    for (int a = 0; a < nmax; a++) v += Ws[n * TGX_MAX_NR + a] * I[a];
+    R[n] = v;
+  }
+}
+
+// Normalised real-harmonic recurrence constants, index l*(LMAX+1)+m.
+void sph_constants(int lmax, float* A, float* B, float* Pmm, double* Ad, double* Bd,
+                   double* Pmmd) {
+  auto K = [](int l, int m) {
+    double r = 1.0;
+    for (int k = l - m + 1; k <= l + m; k++) r /= (double)k;
+    double v = std::sqrt((2.0 * l + 1.0) / (4.0 * M_PI) * r);
+    return m > 0 ? v * std::sqrt(2.0) : v;
+  };
+  for (int i = 0; i < TGX_L2MAX; i++) A[i] = B[i] = 0, Ad[i] = Bd[i] = 0;
+  double df = 1.0;
+  for (int m = 0; m <= lmax; m++) {
+    if (m > 0) df *= (2.0 * m - 1.0);
+    Pmmd[m] = K(m, m) * df;
+    Pmm[m] = (float)Pmmd[m];
+    for (int l = m + 1; l <= lmax; l++) {
+      const int ix = l * (TGX_MAX_LMAX + 1) + m;
+      Ad[ix] = K(l, m) / K(l - 1, m) * (2.0 * l - 1.0) / (l - m);
+      Bd[ix] = l - 2 >= m ? K(l, m) / K(l - 2, m) * (l + m - 1.0) / (l - m) : 0.0;
+      A[ix] = (float)Ad[ix];
+      B[ix] = (float)Bd[ix];
+    }
+  }
+}
+
+// Orthorhombic cell, supercell repetitions as number_of_unit_cells_for_given_
+// cutoff (neighbors_interface.f90:655-708) and the linked-cell grid
+// mx = int(L/rcut) (:299-301).
+int make_cell(const double* lat, double rcut, Cell* c, std::string* err) {
+  const double tol = 1e-6;
+  if (!(std::fabs(lat[1]) < tol && std::fabs(lat[2]) < tol && std::fabs(lat[3]) < tol &&
+        std::fabs(lat[5]) < tol && std::fabs(lat[6]) < tol && std::fabs(lat[7]) < tol)) {
+    *err = "non-orthorhombic cells are not supported";
+    return TGX_ERR_UNSUPPORTED;
+  }
+  if (!(lat[0] > 0 && lat[4] > 0 && lat[8] > 0 && rcut > 0)) {
+    *err = "cell edges and cutoff must be positive";
+    return TGX_ERR_ARG;
+  }
+  // Reference arithmetic of number_of_unit_cells_for_given_cutoff on the
+  // (zeroed off-diagonal) lattice: unit normals, the 3x3 matrix of normal.edge,
+  // its explicit inverse, times 2*rcut, rounded up.
+  const double a[3] = {lat[0], 0, 0}, b[3] = {0, lat[4], 0}, cc[3] = {0, 0, lat[8]};
+  auto cross = [](const double* x, const double* y, double* o) {
+    o[0] = x[1] * y[2] - x[2] * y[1];
+    o[1] = x[2] * y[0] - x[0] * y[2];
+    o[2] = x[0] * y[1] - x[1] * y[0];
+  };
+  auto dot = [](const double* x, const double* y) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
+  double axb[3], cxa[3], bxc[3];
+  cross(a, b, axb);
+  double t = std::sqrt(dot(axb, axb));
+  for (int k = 0; k < 3; k++) axb[k] /= t;
+  cross(cc, a, cxa);
+  t = std::sqrt(dot(cxa, cxa));
+  for (int k = 0; k < 3; k++) cxa[k] /= t;
+  cross(b, cc, bxc);
+  t = std::sqrt(dot(bxc, bxc));
+  for (int k = 0; k < 3; k++) bxc[k] /= t;
+  const double* rows[3] = {axb, cxa, bxc};
+  const double* cols[3] = {a, b, cc};
+  double m[3][3], mi[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) m[i][j] = dot(rows[i], cols[j]);
+  const double md = -m[0][2] * m[1][1] * m[2][0] + m[0][1] * m[1][2] * m[2][0] +
+                    m[0][2] * m[1][0] * m[2][1] - m[0][0] * m[1][2] * m[2][1] -
+                    m[0][1] * m[1][0] * m[2][2] + m[0][0] * m[1][1] * m[2][2];
+  mi[0][0] = m[1][1] * m[2][2] - m[1][2] * m[2][1];
+  mi[0][1] = m[0][2] * m[2][1] - m[0][1] * m[2][2];
+  mi[0][2] = m[0][1] * m[1][2] - m[0][2] * m[1][1];
+  mi[1][0] = m[1][2] * m[2][0] - m[1][0] * m[2][2];
+  mi[1][1] = m[0][0] * m[2][2] - m[0][2] * m[2][0];
+  mi[1][2] = m[0][2] * m[1][0] - m[0][0] * m[1][2];
+  mi[2][0] = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+  mi[2][1] = m[0][1] * m[2][0] - m[0][0] * m[2][1];
+  mi[2][2] = m[0][0] * m[1][1] - m[0][1] * m[1][0];
+  int idx[3];
+  for (int i = 0; i < 3; i++) {
+    double v = 0;
+    for (int j = 0; j < 3; j++) v += (mi[i][j] / md) * (2.0 * rcut);
+    v = std::fabs(v);
+    idx[i] = (int)v;
+    if (v > (double)idx[i]) idx[i] += 1;
+    if (idx[i] < 1) idx[i] = 1;
+  }
+  c->a = lat[0];
+  c->b = lat[4];
+  c->c = lat[8];
+  c->ia = idx[0];
+  c->ib = idx[1];
+  c->ic = idx[2];
+  c->nrep = idx[0] * idx[1] * idx[2];
+  c->small = c->nrep > 1;
+  c->La = (double)idx[0] * lat[0];
+  c->Lb = (double)idx[1] * lat[4];
+  c->Lc = (double)idx[2] * lat[8];
+  c->mx = (int)(c->La / rcut);
+  c->my = (int)(c->Lb / rcut);
+  c->mz = (int)(c->Lc / rcut);
+  if (c->nrep > (1 << (31 - TGX_IMG_SHIFT))) {
+    *err = "cell too small for the cutoff (too many images)";
+    return TGX_ERR_UNSUPPORTED;
+  }
+  return TGX_OK;
+}
+
+}  // namespace tgx

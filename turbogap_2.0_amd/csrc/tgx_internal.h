@@ -1,0 +1,186 @@
+// Internal declarations of the tgx HIP library (not part of the C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/tgx.h"
+
+#define TGX_L2MAX ((TGX_MAX_LMAX + 1) * (TGX_MAX_LMAX + 1))
+
+// Number of intervals of the FP32 radial lookup table (per species, over
+// [0, rcut_hard]); cubic Hermite interpolation of value and derivative.
+#define TGX_RTAB_N 1024
+
+// Packed neighbour entry: atom id in the low 26 bits, supercell image in the
+// high bits.  Ascending packed value == ascending reference supercell id.
+#define TGX_IMG_SHIFT 26
+#define TGX_ATOM_MASK ((1 << TGX_IMG_SHIFT) - 1)
+
+namespace tgx {
+
+// Orthorhombic periodic cell as the kernels see it.
+struct Cell {
+  double a, b, c;        // unit-cell edges (state%a_box(1), b_box(2), c_box(3))
+  double La, Lb, Lc;     // supercell edges (indices * a ...)
+  int ia, ib, ic;        // supercell repetitions
+  int nrep;              // ia*ib*ic
+  int small;             // any repetition > 1
+  int mx, my, mz;        // linked-cell grid (large boxes only)
+};
+
+// SOAP model constants in device-friendly form.
+struct SoapDev {
+  int n_species, l_max, L2, nr, dim, npair, radial_enhancement, central_species;
+  int nmax[TGX_MAX_SPECIES], off[TGX_MAX_SPECIES];
+  double rh[TGX_MAX_SPECIES], rs[TGX_MAX_SPECIES];
+  double sig_r[TGX_MAX_SPECIES], sig_rs[TGX_MAX_SPECIES];
+  double sig_t[TGX_MAX_SPECIES], sig_ts[TGX_MAX_SPECIES];
+  double amp_scal[TGX_MAX_SPECIES], glob[TGX_MAX_SPECIES];
+  double rself[TGX_MAX_SPECIES * TGX_MAX_NR];  // central weight * R_n(0) * Y00
+  double rcut_max;
+};
+
+struct TwoBodyDev {
+  int sp1, sp2, n_sparse;
+  double rcut, buffer, delta, sigma;
+  double* Qs;      // device
+  double* alphas;  // device (alpha * cutoff folded in)
+};
+
+// Growable device buffer.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t want);
+  void release();
+  template <typename T>
+  T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+struct Timer;
+
+}  // namespace tgx
+
+struct tgx_ctx {
+  int device = 0;
+  int precision = TGX_FP64;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // model
+  bool has_soap = false;
+  tgx_soap_hypers hyp{};
+  tgx::SoapDev sd{};
+  int dim = 0, n_sparse = 0, dim_pad = 0, sp_pad = 0;
+  double delta = 1.0, zeta = 2.0;
+  double e0[TGX_MAX_SPECIES] = {0};
+  std::vector<double> Qs_host;      // n_sparse x dim
+  std::vector<double> alphas_host;  // n_sparse
+  std::vector<tgx::TwoBodyDev> twob;
+  // W matrices per species (n_max x n_max, row-major), FP64
+  std::vector<double> W_host;       // TGX_MAX_SPECIES * NR * NR
+  tgx::DevBuf W_dev, rtab, Qs64, QsT64, Qs32, QsT32, Qs16, QsT16, alph64, alph32;
+  bool rtab_ready = false;
+
+  // per-call work buffers
+  tgx::DevBuf pos, species, nb_count, nb_list, nb_rev, nb_flag, cell_of, cell_start,
+      cell_atoms, site_of_atom, site_atom;
+  tgx::DevBuf q, qb, cbuf, pnorm, ksum, wmat, gmat, epart, esite, gpair, vsite,
+      forces, eatom, virial, scratch, csr_off, csr_out;
+  int maxnb = 0;
+  double stage_ms[7] = {0};
+
+  // MD state
+  tgx::DevBuf md_vel, md_fprev, md_mass, md_pos0, md_ep, md_ek, md_reb, md_disp;
+};
+
+namespace tgx {
+
+int set_err(tgx_ctx* c, int code, const char* fmt, ...);
+#define TGX_HIP(ctx, call)                                                   \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return tgx::set_err(ctx, TGX_ERR_HIP, "%s failed: %s (%s:%d)", #call,  \
+                          hipGetErrorString(e_), __FILE__, __LINE__);        \
+  } while (0)
+
+// Host helpers (tgx_host.cpp)
+void poly3_W(int nmax, double* W);                 // S^(-1/2), binary128
+void radial_closed_form(const SoapDev& sd, const double* W, int s, double r,
+                        double* R, double* dR);   // FP64 host evaluation
+int make_cell(const double* lattice, double rcut, Cell* cell, std::string* err);
+void sph_constants(int lmax, float* A, float* B, float* Pmm, double* Ad,
+                   double* Bd, double* Pmmd);
+
+// Kernel launchers (return hipError_t)
+hipError_t upload_sph_constants(int lmax, hipStream_t st);
+hipError_t launch_build_rtab(const SoapDev& sd, const double* W, float* tab,
+                             hipStream_t st);
+
+// Neighbours
+hipError_t launch_bin_atoms(const double* pos, int n, Cell cell, int* cell_of,
+                            int* cell_count, hipStream_t st);
+hipError_t launch_count_neighbors(const double* pos, int n, Cell cell,
+                                  double rcut2_dummy, double rcut,
+                                  const int* cell_start, const int* cell_atoms,
+                                  int* count, int maxnb, int* list, int* flag,
+                                  hipStream_t st);
+hipError_t launch_sort_lists(int n, int maxnb, const int* count, int* list,
+                             hipStream_t st);
+hipError_t launch_reverse(int n, int maxnb, Cell cell, const int* count,
+                          const int* list, int* rev, int* flag, hipStream_t st);
+hipError_t launch_csr_out(const double* pos, int n, Cell cell, int maxnb,
+                          const int* count, const int* list, const int* off,
+                          int* out_list, double* rjs, double* xyz, double* th,
+                          double* ph, hipStream_t st);
+
+// SOAP
+template <typename T>
+hipError_t launch_soap_fwd(const SoapDev& sd, const double* pos,
+                           const int* species, int n, Cell cell, int maxnb,
+                           const int* count, const int* list,
+                           const int* site_atom, int n_sites, const double* W,
+                           const float* rtab, T* q, void* qb, int qb_ld,
+                           T* cbuf, T* pnorm, hipStream_t st);
+template <typename T>
+hipError_t launch_soap_bwd(const SoapDev& sd, const double* pos,
+                           const int* species, int n, Cell cell, int maxnb,
+                           const int* count, const int* list,
+                           const int* site_atom, int n_sites, const double* W,
+                           const float* rtab, const T* q, const T* cbuf,
+                           const T* pnorm, const T* G, int g_ld, T* gpair,
+                           double* vsite, hipStream_t st);
+
+// Kernel step GEMMs.  mode: TGX_FP64 / TGX_FP32 / TGX_MIXED.
+hipError_t launch_gap_kernel(int mode, int n_sites, int dim, int dim_pad,
+                             int n_sparse, int sp_pad, const void* q,
+                             const void* qs, const void* qsT,
+                             const void* alphas, double delta, double zeta,
+                             void* wmat, void* epart, double* esite, void* G,
+                             int want_grad, hipStream_t st);
+
+// Forces, 2-body, reductions, MD
+template <typename T>
+hipError_t launch_force_gather(int n, int maxnb, const int* count,
+                               const int* list, const int* rev,
+                               const int* site_of_atom, const T* gpair,
+                               double* forces, hipStream_t st);
+template <typename T>
+hipError_t launch_two_body(const TwoBodyDev& b, const double* pos,
+                           const int* species, int n, Cell cell, int maxnb,
+                           const int* count, const int* list, double* eatom,
+                           double* forces, double* vatom, int do_forces,
+                           hipStream_t st);
+hipError_t launch_energy_init(int n, const int* species, const double* e0dev,
+                              const int* site_of_atom, const double* esite,
+                              double* eatom, hipStream_t st);
+hipError_t launch_reduce_virial(int n_rows, const double* v6rows,
+                                double* out9, double* energy_in,
+                                const double* eatom, int n, hipStream_t st);
+
+}  // namespace tgx

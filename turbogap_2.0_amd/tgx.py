@@ -1,0 +1,396 @@
+"""Python host mirror of the tgx C-ABI (include/tgx.h).
+
+This module is the Python-side caller of the HIP library: it loads
+``libtgx.so`` from this directory with ctypes and exposes the same entry
+points the Fortran shim binds (``fortran/tgx_mod.f90``).  It does no
+arithmetic of its own: when the library is missing it raises, it never falls
+back to a CPU path.
+
+The argument meaning mirrors the reference's Fortran interfaces:
+``build_neighbors_list`` (neighbors/neighbors_interface.f90:144),
+``get_gap_soap`` / ``get_soap_energy_and_forces`` (gap/gap_interface.f90:41,
+gap/gap.f90:39), ``get_2b_energy_and_forces`` (gap/gap.f90:329) and
+``velocity_verlet`` (md/md_utils.f90:486).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtgx.so")
+
+MAX_SPECIES = 8
+MAX_LMAX = 10
+MAX_NR = 32
+
+TGX_OK = 0
+TGX_ERR_ARG = 1
+TGX_ERR_HIP = 2
+TGX_ERR_CAPACITY = 3
+TGX_ERR_UNSUPPORTED = 4
+TGX_ERR_STATE = 5
+
+FP64, FP32, MIXED = 0, 1, 2
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "mixed": MIXED}
+
+# amu -> eV fs^2 / A^2 (reference read/read_files.f90:333)
+AMU = 103.6426965268
+KB_MD = 8.6173303e-5  # md/md_interface.f90:111-118
+
+
+class CSoapHypers(C.Structure):
+    """ctypes mirror of ``tgx_soap_hypers``."""
+
+    _fields_ = [
+        ("n_species", C.c_int32),
+        ("l_max", C.c_int32),
+        ("radial_enhancement", C.c_int32),
+        ("central_species", C.c_int32),
+        ("basis", C.c_int32),
+        ("pad_", C.c_int32),
+        ("alpha_max", C.c_int32 * MAX_SPECIES),
+        ("rcut_hard", C.c_double * MAX_SPECIES),
+        ("rcut_soft", C.c_double * MAX_SPECIES),
+        ("atom_sigma_r", C.c_double * MAX_SPECIES),
+        ("atom_sigma_t", C.c_double * MAX_SPECIES),
+        ("atom_sigma_r_scaling", C.c_double * MAX_SPECIES),
+        ("atom_sigma_t_scaling", C.c_double * MAX_SPECIES),
+        ("amplitude_scaling", C.c_double * MAX_SPECIES),
+        ("central_weight", C.c_double * MAX_SPECIES),
+        ("global_scaling", C.c_double * MAX_SPECIES),
+        ("nf", C.c_double * MAX_SPECIES),
+    ]
+
+
+@dataclass
+class SoapHypers:
+    """One soap_turbo block.  Defaults are the reference's
+    (gap/read_gap.f90:164-180, types/types.f90:252-312); ``buffer`` is the
+    .gap keyword, rcut_soft = rcut_hard - buffer (read_gap.f90:425-432)."""
+
+    n_species: int = 1
+    alpha_max: Sequence[int] = (8,)
+    l_max: int = 8
+    rcut_hard: Sequence[float] = (3.7,)
+    buffer: Sequence[float] = (0.5,)
+    atom_sigma_r: Sequence[float] = (0.5,)
+    atom_sigma_t: Sequence[float] = (0.5,)
+    atom_sigma_r_scaling: Sequence[float] = (0.0,)
+    atom_sigma_t_scaling: Sequence[float] = (0.0,)
+    amplitude_scaling: Sequence[float] = (1.0,)
+    central_weight: Sequence[float] = (1.0,)
+    global_scaling: Sequence[float] = (1.0,)
+    nf: Sequence[float] = (4.0,)
+    radial_enhancement: int = 0
+    central_species: int = 0
+
+    @classmethod
+    def uniform(cls, n_species: int, **kw) -> "SoapHypers":
+        """Same per-species values for every species."""
+        base = cls()
+        args = {"n_species": n_species}
+        for name in ("alpha_max", "rcut_hard", "buffer", "atom_sigma_r",
+                     "atom_sigma_t", "atom_sigma_r_scaling",
+                     "atom_sigma_t_scaling", "amplitude_scaling",
+                     "central_weight", "global_scaling", "nf"):
+            v = kw.pop(name, getattr(base, name)[0])
+            args[name] = tuple([v] * n_species) if np.isscalar(v) else tuple(v)
+        args.update(kw)
+        return cls(**args)
+
+    @property
+    def n_r(self) -> int:
+        return int(sum(self.alpha_max))
+
+    @property
+    def dim(self) -> int:
+        nr = self.n_r
+        return nr * (nr + 1) // 2 * (self.l_max + 1)
+
+    @property
+    def rcut_max(self) -> float:
+        return float(max(self.rcut_hard))
+
+    def to_c(self) -> CSoapHypers:
+        h = CSoapHypers()
+        h.n_species = self.n_species
+        h.l_max = self.l_max
+        h.radial_enhancement = self.radial_enhancement
+        h.central_species = self.central_species
+        h.basis = 0
+        for s in range(self.n_species):
+            h.alpha_max[s] = int(self.alpha_max[s])
+            h.rcut_hard[s] = float(self.rcut_hard[s])
+            b = float(self.buffer[s])
+            h.rcut_soft[s] = h.rcut_hard[s] if b == 0.0 else h.rcut_hard[s] - b
+            h.atom_sigma_r[s] = float(self.atom_sigma_r[s])
+            h.atom_sigma_t[s] = float(self.atom_sigma_t[s])
+            h.atom_sigma_r_scaling[s] = float(self.atom_sigma_r_scaling[s])
+            h.atom_sigma_t_scaling[s] = float(self.atom_sigma_t_scaling[s])
+            h.amplitude_scaling[s] = float(self.amplitude_scaling[s])
+            h.central_weight[s] = float(self.central_weight[s])
+            h.global_scaling[s] = float(self.global_scaling[s])
+            h.nf[s] = float(self.nf[s])
+        return h
+
+
+@dataclass
+class TwoBody:
+    """One distance_2b block (gap_2b_t, types/types.f90:314-321)."""
+
+    sp1: int
+    sp2: int
+    rcut: float
+    delta: float
+    sigma: float
+    Qs: np.ndarray
+    alphas: np.ndarray
+    cutoff: np.ndarray
+    buffer: float = 0.5
+
+
+@dataclass
+class GapModel:
+    """A soap_turbo GAP block plus optional 2b blocks and e0 per species."""
+
+    hypers: SoapHypers
+    Qs: np.ndarray          # (n_sparse, dim) row-major == Qs(dim, n_sparse)
+    alphas: np.ndarray      # (n_sparse,)
+    delta: float = 1.0
+    zeta: float = 4.0
+    e0: Optional[np.ndarray] = None
+    two_body: list = field(default_factory=list)
+
+    @property
+    def rcut_max(self) -> float:
+        r = self.hypers.rcut_max
+        for b in self.two_body:
+            r = max(r, b.rcut)
+        return r
+
+
+def _dptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _iptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libtgx.so; raise when it is not built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"HIP library {path} is not built; run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    d, i, i64 = C.c_double, C.c_int, C.c_int64
+    dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int)
+    sig = {
+        "tgx_create": (i, [i, i, C.POINTER(P)]),
+        "tgx_destroy": (i, [P]),
+        "tgx_last_error": (C.c_char_p, [P]),
+        "tgx_set_precision": (i, [P, i]),
+        "tgx_set_soap_model": (i, [P, C.POINTER(CSoapHypers), i, i, dp, dp, d, d]),
+        "tgx_set_e0": (i, [P, i, dp]),
+        "tgx_add_2b_model": (i, [P, i, i, d, d, d, d, i, dp, dp, dp]),
+        "tgx_clear_2b_models": (i, [P]),
+        "tgx_neighbors_build": (i, [P, i, dp, dp, d, i64, ip, ip, dp, dp, dp, dp,
+                                    C.POINTER(i64), ip]),
+        "tgx_soap_descriptors": (i, [P, i, dp, ip, dp, dp, ip, ip]),
+        "tgx_gap_kernel_step": (i, [P, i, dp, dp, dp]),
+        "tgx_energy_forces": (i, [P, i, dp, ip, dp, i, dp, dp, dp]),
+        "tgx_energy_forces_batched": (i, [P, i, ip, dp, ip, dp, i, dp, dp, dp]),
+        "tgx_md_run": (i, [P, i, i, d, d, ip, dp, dp, dp, dp, dp, dp, dp, ip]),
+        "tgx_stage_times": (i, [P, dp]),
+        "tgx_poly3_orthonormalization": (i, [i, dp, dp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def poly3_orthonormalization(alpha_max: int):
+    """S and W = S^(-1/2) of the poly3 basis (host-only library call)."""
+    lib = load_library()
+    S = np.zeros((alpha_max, alpha_max))
+    W = np.zeros((alpha_max, alpha_max))
+    rc = lib.tgx_poly3_orthonormalization(alpha_max, _dptr(S), _dptr(W))
+    if rc != TGX_OK:
+        raise TgxError(rc, "tgx_poly3_orthonormalization")
+    return S.T.copy(), W.T.copy()
+
+
+class TgxError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"tgx error {code}: {msg}")
+        self.code = code
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class Engine:
+    """One library context on one GPU."""
+
+    def __init__(self, device: int = 0, precision: str = "fp64"):
+        self.lib = load_library()
+        self.ctx = C.c_void_p()
+        self._check(self.lib.tgx_create(device, PRECISIONS[precision], C.byref(self.ctx)))
+        self.model: Optional[GapModel] = None
+
+    # -- plumbing --------------------------------------------------------
+    def _check(self, rc: int):
+        if rc != TGX_OK:
+            msg = ""
+            if self.ctx:
+                m = self.lib.tgx_last_error(self.ctx)
+                msg = m.decode() if m else ""
+            raise TgxError(rc, msg)
+
+    def close(self):
+        if self.ctx:
+            self.lib.tgx_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_precision(self, precision: str):
+        self._check(self.lib.tgx_set_precision(self.ctx, PRECISIONS[precision]))
+
+    # -- models ------------------------------------------------------------
+    def set_model(self, model: GapModel):
+        h = model.hypers.to_c()
+        Qs = _f64(model.Qs)
+        al = _f64(model.alphas)
+        n_sparse, dim = Qs.shape
+        self._check(self.lib.tgx_set_soap_model(self.ctx, C.byref(h), dim, n_sparse,
+                                                _dptr(Qs), _dptr(al), model.delta, model.zeta))
+        e0 = _f64(model.e0 if model.e0 is not None else np.zeros(model.hypers.n_species))
+        self._check(self.lib.tgx_set_e0(self.ctx, len(e0), _dptr(e0)))
+        self._check(self.lib.tgx_clear_2b_models(self.ctx))
+        for b in model.two_body:
+            q, a, c = _f64(b.Qs), _f64(b.alphas), _f64(b.cutoff)
+            self._check(self.lib.tgx_add_2b_model(self.ctx, b.sp1, b.sp2, b.rcut, b.buffer,
+                                                  b.delta, b.sigma, len(q), _dptr(q), _dptr(a),
+                                                  _dptr(c)))
+        self.model = model
+
+    # -- hot path ----------------------------------------------------------
+    def neighbors(self, pos, lattice, rcut: float):
+        pos, lat = _f64(pos), _f64(lattice)
+        n = pos.shape[0]
+        need = C.c_int64(0)
+        idx = np.zeros(3, np.int32)
+        cap = max(64 * n, 1024)
+        while True:
+            nn = np.zeros(n, np.int32)
+            lst = np.zeros(cap, np.int32)
+            rjs = np.zeros(cap)
+            xyz = np.zeros((cap, 3))
+            th = np.zeros(cap)
+            ph = np.zeros(cap)
+            rc = self.lib.tgx_neighbors_build(self.ctx, n, _dptr(pos), _dptr(lat), rcut, cap,
+                                              _iptr(nn), _iptr(lst), _dptr(rjs), _dptr(xyz),
+                                              _dptr(th), _dptr(ph), C.byref(need), _iptr(idx))
+            if rc == TGX_ERR_CAPACITY:
+                cap = int(need.value)
+                continue
+            self._check(rc)
+            m = int(need.value)
+            return dict(n_neigh=nn, list=lst[:m], rjs=rjs[:m], xyz=xyz[:m], thetas=th[:m],
+                        phis=ph[:m], indices=idx)
+
+    def soap(self, pos, species, lattice):
+        pos, lat, sp = _f64(pos), _f64(lattice), _i32(species)
+        n = pos.shape[0]
+        dim = self.model.hypers.dim
+        out = np.zeros((n, dim))
+        ns = C.c_int(0)
+        site = np.zeros(n, np.int32)
+        self._check(self.lib.tgx_soap_descriptors(self.ctx, n, _dptr(pos), _iptr(sp), _dptr(lat),
+                                                  _dptr(out), C.byref(ns), _iptr(site)))
+        return out[: ns.value], site[: ns.value] - 1
+
+    def kernel_step(self, soap, want_qss: bool = True):
+        q = _f64(soap)
+        ns, dim = q.shape
+        e = np.zeros(ns)
+        qss = np.zeros((dim, ns)) if want_qss else None
+        self._check(self.lib.tgx_gap_kernel_step(self.ctx, ns, _dptr(q), _dptr(e), _dptr(qss)))
+        return e, (qss.T.copy() if want_qss else None)
+
+    def energy_forces(self, pos, species, lattice, do_forces: bool = True):
+        pos, lat, sp = _f64(pos), _f64(lattice), _i32(species)
+        n = pos.shape[0]
+        e = np.zeros(n)
+        f = np.zeros((n, 3))
+        v = np.zeros((3, 3))
+        self._check(self.lib.tgx_energy_forces(self.ctx, n, _dptr(pos), _iptr(sp), _dptr(lat),
+                                               int(do_forces), _dptr(e), _dptr(f), _dptr(v)))
+        return e, f, v.T.copy()
+
+    def energy_forces_batched(self, structures, do_forces: bool = True):
+        """structures: list of (pos, species, lattice)."""
+        offs = np.zeros(len(structures) + 1, np.int32)
+        for s, (p, _, _) in enumerate(structures):
+            offs[s + 1] = offs[s] + len(p)
+        pos = _f64(np.concatenate([p for p, _, _ in structures]))
+        sp = _i32(np.concatenate([s for _, s, _ in structures]))
+        lat = _f64(np.stack([np.asarray(l, np.float64).reshape(9) for _, _, l in structures]))
+        n = int(offs[-1])
+        e = np.zeros(n)
+        f = np.zeros((n, 3))
+        v = np.zeros((len(structures), 3, 3))
+        self._check(self.lib.tgx_energy_forces_batched(self.ctx, len(structures), _iptr(offs),
+                                                       _dptr(pos), _iptr(sp), _dptr(lat),
+                                                       int(do_forces), _dptr(e), _dptr(f),
+                                                       _dptr(v)))
+        return offs, e, f, np.transpose(v, (0, 2, 1)).copy()
+
+    def md_run(self, pos, vel, species, lattice, masses_amu, n_steps: int, dt: float,
+               skin: float):
+        pos, vel = _f64(pos).copy(), _f64(vel).copy()
+        lat, sp, m = _f64(lattice), _i32(species), _f64(masses_amu)
+        n = pos.shape[0]
+        f = np.zeros((n, 3))
+        ep = np.zeros(n_steps + 1)
+        ek = np.zeros(n_steps + 1)
+        nreb = C.c_int(0)
+        self._check(self.lib.tgx_md_run(self.ctx, n, n_steps, dt, skin, _iptr(sp), _dptr(lat),
+                                        _dptr(m), _dptr(pos), _dptr(vel), _dptr(f), _dptr(ep),
+                                        _dptr(ek), C.byref(nreb)))
+        return dict(pos=pos, vel=vel, forces=f, e_pot=ep, e_kin=ek, n_rebuilds=nreb.value)
+
+    def stage_times(self):
+        out = np.zeros(7)
+        self._check(self.lib.tgx_stage_times(self.ctx, _dptr(out)))
+        return dict(zip(["neighbors", "soap_fwd", "gemm", "soap_bwd", "two_body", "integrator",
+                         "total"], out))
