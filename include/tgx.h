@@ -171,8 +171,7 @@ int tgx_energy_forces(tgx_ctx* ctx, int n, const double* pos,
 /*
  * Many independent structures in one call (structures s = 0..n_struct-1 own
  * atoms offsets[s]..offsets[s+1]-1 of pos/species/e_atom/forces; lattices
- * holds 9 doubles per structure, virials 9 per structure).  All structures
- * share one set of launches.
+ * holds 9 doubles per structure, virials 9 per structure).
  */
 int tgx_energy_forces_batched(tgx_ctx* ctx, int n_struct, const int* offsets,
                               const double* pos, const int* species,
@@ -203,6 +202,30 @@ int tgx_md_run(tgx_ctx* ctx, int n, int n_steps, double dt, double skin,
  * Host-only (no device needed).  W is exact to FP64 rounding although
  * cond(S) ~ 1e13 at alpha_max = 8. */
 int tgx_poly3_orthonormalization(int alpha_max, double* S, double* W);
+
+/*
+ * The same dynamics split for callers that keep the state on the device
+ * between calls (the Fortran md driver, benchmarks):
+ *   tgx_md_begin  uploads the state, builds the list with skin, computes the
+ *                 initial forces and captures one step as a hipGraph;
+ *                 history = length of the e_pot/e_kin ring kept on device;
+ *   tgx_md_steps  enqueues n_steps steps (asynchronous, no host sync);
+ *   tgx_md_read   waits and copies back positions, velocities synchronous
+ *                 with them, forces and the energies of the last
+ *                 min(steps+1, history) states (any pointer may be NULL);
+ *   tgx_md_end    releases the graph.
+ */
+int tgx_md_begin(tgx_ctx* ctx, int n, const int* species, const double* lattice,
+                 const double* mass, const double* pos, const double* vel,
+                 double dt, double skin, int history);
+int tgx_md_steps(tgx_ctx* ctx, int n_steps);
+int tgx_md_read(tgx_ctx* ctx, double* pos, double* vel, double* forces,
+                double* e_pot, double* e_kin, int* n_steps_done,
+                int* n_rebuilds);
+int tgx_md_end(tgx_ctx* ctx);
+
+/* Wait for all work enqueued on the context's stream. */
+int tgx_sync(tgx_ctx* ctx);
 
 /* Per-stage timing of the last call, milliseconds (HIP events):
  * [0] neighbours, [1] SOAP forward, [2] kernel GEMMs, [3] SOAP backward and

@@ -21,7 +21,7 @@ hipError_t launch_build_rtab(const SoapDev* sd_dev, int n_species, const double*
 hipError_t launch_neighbors(const double* pos, int n, const Cell& c, double rcut, int* cell_of,
                             int* cell_count, int* cell_start, int* cell_fill, int* cell_atoms,
                             int* count, int maxnb, int* list, int* rev, int* flag,
-                            hipStream_t st);
+                            hipStream_t st, const int* guard);
 hipError_t launch_csr_out(const double* pos, int n, const Cell& c, int maxnb, const int* count,
                           const int* list, const long long* off, int* out_list, double* rjs,
                           double* xyz, double* th, double* ph, hipStream_t st);
@@ -47,6 +47,20 @@ hipError_t launch_energy_init(int n, const int* species, const double* e0,
                               double* forces, double* vatom, hipStream_t st);
 hipError_t launch_reduce(int n_sites, const double* vsite, int n, const double* vatom,
                          const double* eatom, double* out, hipStream_t st);
+
+hipError_t launch_md_vv(int n, double* pos, double* vel, const double* f, double* fprev,
+                        const double* mass, double dt, const int* step, double* ke,
+                        hipStream_t st);
+hipError_t launch_md_skin(int n, const double* pos, const double* pref, double lim2, int* flag,
+                          hipStream_t st);
+hipError_t launch_md_ref_copy(int n, const double* pos, double* pref, const int* flag,
+                              hipStream_t st);
+hipError_t launch_md_finish(int n, const double* ke, const double* out, int* flag, int* nreb,
+                            int* step, double* ep_hist, double* ek_hist, int cap,
+                            hipStream_t st);
+hipError_t launch_md_sync_vel(int n, const double* vel, const double* f, const double* fprev,
+                              const double* mass, double dt, const int* step, double* vout,
+                              double* ke, double* kesum, hipStream_t st);
 
 int set_err(tgx_ctx* c, int code, const char* fmt, ...) {
   if (!c) return code;
@@ -91,12 +105,25 @@ namespace {
                      (size_t)(nbytes), #buf);                                        \
   } while (0)
 
+struct MdState {
+  bool active = false;
+  int n = 0, n_sites = 0, cap = 0;
+  double dt = 1.0, skin = 0.0, rcut_list = 0.0;
+  Cell cell{};
+  DevBuf pref, ke, ctl /* flag, nreb, step, pad */, ep, ek, vout, kesum;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool use_graph = true;
+};
+
 struct Extra {
   DevBuf sd_dev, pairn, e0_dev, iota, cell_count, cell_fill, out, stage;
   DevBuf twob_store;
   std::vector<int> species_host;
   hipEvent_t ev[8];
   bool ev_ready = false;
+  bool ev_on = true;
+  MdState md;
 };
 
 }  // namespace
@@ -141,7 +168,7 @@ static int build_list(tgx_ctx* c, int n, const Cell& cell, double rcut, bool wan
                                 x.cell_fill.as<int>(), c->cell_atoms.as<int>(),
                                 c->nb_count.as<int>(), maxnb, c->nb_list.as<int>(),
                                 want_rev ? c->nb_rev.as<int>() : nullptr, c->nb_flag.as<int>(),
-                                c->stream));
+                                c->stream, nullptr));
     int flag[4];
     TGX_HIP(c, hipMemcpyAsync(flag, c->nb_flag.p, sizeof(flag), hipMemcpyDeviceToHost, c->stream));
     TGX_HIP(c, hipStreamSynchronize(c->stream));
@@ -249,8 +276,11 @@ static int kernel_step(tgx_ctx* c, int n_sites, bool want_grad) {
 
 static void ev_rec(tgx_ctx* c, int k) {
   Extra& x = X(c);
-  if (x.ev_ready) (void)hipEventRecord(x.ev[k], c->stream);
+  if (x.ev_ready && x.ev_on) (void)hipEventRecord(x.ev[k], c->stream);
 }
+
+static int ef_launch(tgx_ctx* c, int n, int n_sites, const Cell& cell, int do_forces,
+                     bool events);
 
 // Energies/forces/virial of the structure whose positions and species are
 // already in c->pos / c->species.  Results stay on the device:
@@ -266,6 +296,7 @@ static int run_ef(tgx_ctx* c, int n, const double* lattice, const int* species_h
   if (rcode) return set_err(c, rcode, "%s", err.c_str());
   if ((long long)n * cell.nrep >= (1LL << TGX_IMG_SHIFT))
     return set_err(c, TGX_ERR_ARG, "too many atoms");
+  x.ev_on = true;
   ev_rec(c, 0);
   rcode = build_list(c, n, cell, rc, do_forces != 0);
   if (rcode) return rcode;
@@ -273,6 +304,17 @@ static int run_ef(tgx_ctx* c, int n, const double* lattice, const int* species_h
   int n_sites = 0;
   rcode = setup_sites(c, n, species_host, &n_sites);
   if (rcode) return rcode;
+  return ef_launch(c, n, n_sites, cell, do_forces, true);
+}
+
+// Energy/force launch sequence on the current list and sites.  Asynchronous:
+// no host synchronisation, no allocation once the buffers are sized (the MD
+// step graph captures it).
+static int ef_launch(tgx_ctx* c, int n, int n_sites, const Cell& cell, int do_forces,
+                     bool events) {
+  Extra& x = X(c);
+  int rcode;
+  x.ev_on = events;
   const int prec = c->precision;
   ENSURE(c, eatom, 8 * (size_t)n);
   ENSURE(c, forces, 24 * (size_t)n);
@@ -338,12 +380,13 @@ static int run_ef(tgx_ctx* c, int n, const double* lattice, const int* species_h
                            do_forces ? c->virial.as<double>() : nullptr, c->eatom.as<double>(),
                            x.out.as<double>(), c->stream));
   ev_rec(c, 6);
+  x.ev_on = true;
   return TGX_OK;
 }
 
 static void collect_times(tgx_ctx* c) {
   Extra& x = X(c);
-  if (!x.ev_ready) return;
+  if (!x.ev_ready || !x.ev_on) return;
   float ms[6];
   for (int k = 0; k < 6; k++) {
     ms[k] = 0.f;
@@ -834,13 +877,212 @@ int tgx_energy_forces_batched(tgx_ctx* c, int n_struct, const int* offsets, cons
   return TGX_OK;
 }
 
+// ---- molecular dynamics ---------------------------------------------------
+
+static int md_step_launch(tgx_ctx* c) {
+  Extra& x = X(c);
+  MdState& m = x.md;
+  int* ctl = m.ctl.as<int>();
+  const int n = m.n;
+  TGX_HIP(c, launch_md_vv(n, c->pos.as<double>(), c->md_vel.as<double>(), c->forces.as<double>(),
+                          c->md_fprev.as<double>(), c->md_mass.as<double>(), m.dt, ctl + 2,
+                          m.ke.as<double>(), c->stream));
+  TGX_HIP(c, launch_md_skin(n, c->pos.as<double>(), m.pref.as<double>(),
+                            0.25 * m.skin * m.skin, ctl + 0, c->stream));
+  TGX_HIP(c, launch_neighbors(c->pos.as<double>(), n, m.cell, m.rcut_list, c->cell_of.as<int>(),
+                              x.cell_count.as<int>(), c->cell_start.as<int>(),
+                              x.cell_fill.as<int>(), c->cell_atoms.as<int>(),
+                              c->nb_count.as<int>(), c->maxnb, c->nb_list.as<int>(),
+                              c->nb_rev.as<int>(), c->nb_flag.as<int>(), c->stream, ctl + 0));
+  TGX_HIP(c, launch_md_ref_copy(n, c->pos.as<double>(), m.pref.as<double>(), ctl + 0, c->stream));
+  int rcode = ef_launch(c, n, m.n_sites, m.cell, 1, false);
+  if (rcode) return rcode;
+  TGX_HIP(c, launch_md_finish(n, m.ke.as<double>(), x.out.as<double>(), ctl + 0, ctl + 1, ctl + 2,
+                              m.ep.as<double>(), m.ek.as<double>(), m.cap, c->stream));
+  return TGX_OK;
+}
+
+static void md_release_graph(MdState& m) {
+  if (m.exec) (void)hipGraphExecDestroy(m.exec);
+  if (m.graph) (void)hipGraphDestroy(m.graph);
+  m.exec = nullptr;
+  m.graph = nullptr;
+}
+
+int tgx_md_end(tgx_ctx* c) {
+  if (!c) return TGX_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  MdState& m = X(c).md;
+  md_release_graph(m);
+  m.active = false;
+  return TGX_OK;
+}
+
+int tgx_md_begin(tgx_ctx* c, int n, const int* species, const double* lattice,
+                 const double* mass, const double* pos, const double* vel, double dt,
+                 double skin, int history) {
+  if (!c) return TGX_ERR_ARG;
+  if (n < 2 || !species || !lattice || !mass || !pos || !vel || !(dt > 0) || skin < 0 ||
+      history < 1)
+    return set_err(c, TGX_ERR_ARG, "bad arguments");
+  if (!c->has_soap && c->twob.empty()) return set_err(c, TGX_ERR_STATE, "no model loaded");
+  (void)hipSetDevice(c->device);
+  Extra& x = X(c);
+  MdState& m = x.md;
+  if (m.active) tgx_md_end(c);
+  for (int i = 0; i < n; i++)
+    if (!(mass[i] > 0)) return set_err(c, TGX_ERR_ARG, "mass[%d] must be positive", i);
+  int rcode = upload_structure(c, n, pos, species);
+  if (rcode) return rcode;
+  m.n = n;
+  m.dt = dt;
+  m.skin = skin;
+  m.cap = history;
+  m.rcut_list = rcut_all(c) + skin;
+  std::string err;
+  rcode = make_cell(lattice, m.rcut_list, &m.cell, &err);
+  if (rcode) return set_err(c, rcode, "%s", err.c_str());
+  if ((long long)n * m.cell.nrep >= (1LL << TGX_IMG_SHIFT))
+    return set_err(c, TGX_ERR_ARG, "too many atoms");
+  // size the list from a first build, with head room for density changes
+  c->maxnb = 0;
+  rcode = build_list(c, n, m.cell, m.rcut_list, true);
+  if (rcode) return rcode;
+  {
+    std::vector<int> cnt(n);
+    TGX_HIP(c, hipMemcpy(cnt.data(), c->nb_count.p, 4 * (size_t)n, hipMemcpyDeviceToHost));
+    const int mx = *std::max_element(cnt.begin(), cnt.end());
+    const int want = ((mx * 3) / 2 + 16 + 7) / 8 * 8;
+    if (c->maxnb < want) {
+      c->maxnb = want;
+      rcode = build_list(c, n, m.cell, m.rcut_list, true);
+      if (rcode) return rcode;
+    }
+  }
+  rcode = setup_sites(c, n, species, &m.n_sites);
+  if (rcode) return rcode;
+  ENSURE(c, md_vel, 24 * (size_t)n);
+  ENSURE(c, md_fprev, 24 * (size_t)n);
+  ENSURE(c, md_mass, 8 * (size_t)n);
+  if (m.pref.ensure(24 * (size_t)n) || m.ke.ensure(8 * (size_t)n) || m.ctl.ensure(64) ||
+      m.ep.ensure(8 * (size_t)history) || m.ek.ensure(8 * (size_t)history) ||
+      m.vout.ensure(24 * (size_t)n) || m.kesum.ensure(16))
+    return set_err(c, TGX_ERR_HIP, "device allocation failed (md)");
+  std::vector<double> me(n);
+  for (int i = 0; i < n; i++) me[i] = mass[i] * 103.6426965268;  // amu -> eV fs^2/A^2
+  TGX_HIP(c, hipMemcpyAsync(c->md_vel.p, vel, 24 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  TGX_HIP(c, hipMemcpyAsync(c->md_mass.p, me.data(), 8 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  TGX_HIP(c, hipMemsetAsync(c->md_fprev.p, 0, 24 * (size_t)n, c->stream));
+  TGX_HIP(c, hipMemcpyAsync(m.pref.p, c->pos.p, 24 * (size_t)n, hipMemcpyDeviceToDevice, c->stream));
+  TGX_HIP(c, hipMemsetAsync(m.ctl.p, 0, 64, c->stream));
+  TGX_HIP(c, hipMemsetAsync(m.ep.p, 0, 8 * (size_t)history, c->stream));
+  TGX_HIP(c, hipMemsetAsync(m.ek.p, 0, 8 * (size_t)history, c->stream));
+  // forces and energy at the initial positions
+  rcode = ef_launch(c, n, m.n_sites, m.cell, 1, false);
+  if (rcode) return rcode;
+  TGX_HIP(c, hipMemcpyAsync(m.ep.p, x.out.as<double>() + 9, 8, hipMemcpyDeviceToDevice, c->stream));
+  TGX_HIP(c, hipStreamSynchronize(c->stream));
+  m.active = true;
+  const char* ng = getenv("TGX_NO_GRAPH");
+  m.use_graph = !(ng && ng[0] == '1');
+  if (m.use_graph) {
+    TGX_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    rcode = md_step_launch(c);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rcode) {
+      if (g) (void)hipGraphDestroy(g);
+      return rcode;
+    }
+    if (e != hipSuccess) return set_err(c, TGX_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    m.graph = g;
+    e = hipGraphInstantiate(&m.exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      md_release_graph(m);
+      return set_err(c, TGX_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+    }
+  }
+  return TGX_OK;
+}
+
+int tgx_md_steps(tgx_ctx* c, int n_steps) {
+  if (!c) return TGX_ERR_ARG;
+  MdState& m = X(c).md;
+  if (!m.active) return set_err(c, TGX_ERR_STATE, "tgx_md_begin has not been called");
+  if (n_steps < 0) return set_err(c, TGX_ERR_ARG, "n_steps < 0");
+  (void)hipSetDevice(c->device);
+  for (int s = 0; s < n_steps; s++) {
+    if (m.exec) {
+      TGX_HIP(c, hipGraphLaunch(m.exec, c->stream));
+    } else {
+      int rcode = md_step_launch(c);
+      if (rcode) return rcode;
+    }
+  }
+  return TGX_OK;
+}
+
+int tgx_md_read(tgx_ctx* c, double* pos, double* vel, double* forces, double* e_pot,
+                double* e_kin, int* n_steps_done, int* n_rebuilds) {
+  if (!c) return TGX_ERR_ARG;
+  Extra& x = X(c);
+  MdState& m = x.md;
+  if (!m.active) return set_err(c, TGX_ERR_STATE, "tgx_md_begin has not been called");
+  (void)hipSetDevice(c->device);
+  const int n = m.n;
+  TGX_HIP(c, launch_md_sync_vel(n, c->md_vel.as<double>(), c->forces.as<double>(),
+                                c->md_fprev.as<double>(), c->md_mass.as<double>(), m.dt,
+                                m.ctl.as<int>() + 2, m.vout.as<double>(), m.ke.as<double>(),
+                                m.kesum.as<double>(), c->stream));
+  int ctl[4], flag[4];
+  double kes = 0.0;
+  TGX_HIP(c, hipMemcpyAsync(ctl, m.ctl.p, sizeof(ctl), hipMemcpyDeviceToHost, c->stream));
+  TGX_HIP(c, hipMemcpyAsync(flag, c->nb_flag.p, sizeof(flag), hipMemcpyDeviceToHost, c->stream));
+  TGX_HIP(c, hipMemcpyAsync(&kes, m.kesum.p, 8, hipMemcpyDeviceToHost, c->stream));
+  TGX_HIP(c, hipStreamSynchronize(c->stream));
+  if (flag[0] > c->maxnb)
+    return set_err(c, TGX_ERR_CAPACITY, "neighbour capacity %d exceeded during MD (%d needed)",
+                   c->maxnb, flag[0]);
+  if (flag[1]) return set_err(c, TGX_ERR_STATE, "neighbour list lost symmetry during MD");
+  const int steps = ctl[2];
+  if (n_steps_done) *n_steps_done = steps;
+  if (n_rebuilds) *n_rebuilds = ctl[1];
+  if (pos) TGX_HIP(c, hipMemcpy(pos, c->pos.p, 24 * (size_t)n, hipMemcpyDeviceToHost));
+  if (vel) TGX_HIP(c, hipMemcpy(vel, m.vout.p, 24 * (size_t)n, hipMemcpyDeviceToHost));
+  if (forces) TGX_HIP(c, hipMemcpy(forces, c->forces.p, 24 * (size_t)n, hipMemcpyDeviceToHost));
+  if (e_pot || e_kin) {
+    const int cap = m.cap;
+    std::vector<double> ep(cap), ek(cap);
+    TGX_HIP(c, hipMemcpy(ep.data(), m.ep.p, 8 * (size_t)cap, hipMemcpyDeviceToHost));
+    TGX_HIP(c, hipMemcpy(ek.data(), m.ek.p, 8 * (size_t)cap, hipMemcpyDeviceToHost));
+    const int first = steps + 1 > cap ? steps + 1 - cap : 0;
+    for (int s = first; s <= steps; s++) {
+      if (e_pot) e_pot[s - first] = ep[s % cap];
+      if (e_kin) e_kin[s - first] = s == steps ? kes : ek[s % cap];
+    }
+  }
+  return TGX_OK;
+}
+
 int tgx_md_run(tgx_ctx* c, int n, int n_steps, double dt, double skin, const int* species,
                const double* lattice, const double* mass, double* pos, double* vel,
                double* forces, double* e_pot, double* e_kin, int* n_rebuilds) {
-  (void)n; (void)n_steps; (void)dt; (void)skin; (void)species; (void)lattice; (void)mass;
-  (void)pos; (void)vel; (void)forces; (void)e_pot; (void)e_kin; (void)n_rebuilds;
   if (!c) return TGX_ERR_ARG;
-  return set_err(c, TGX_ERR_UNSUPPORTED, "md_run not built yet");
+  if (n_steps < 0 || !pos || !vel) return set_err(c, TGX_ERR_ARG, "bad arguments");
+  int rcode = tgx_md_begin(c, n, species, lattice, mass, pos, vel, dt, skin, n_steps + 1);
+  if (rcode) return rcode;
+  rcode = tgx_md_steps(c, n_steps);
+  if (!rcode) rcode = tgx_md_read(c, pos, vel, forces, e_pot, e_kin, nullptr, n_rebuilds);
+  tgx_md_end(c);
+  return rcode;
+}
+
+int tgx_sync(tgx_ctx* c) {
+  if (!c) return TGX_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  TGX_HIP(c, hipStreamSynchronize(c->stream));
+  return TGX_OK;
 }
 
 int tgx_poly3_orthonormalization(int alpha_max, double* S, double* W) {

@@ -27,6 +27,8 @@ template <> struct MF<double> {
   __device__ static vacc mma(frag a, frag b, vacc c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  // result row of accumulator element v in this lane (column = lane & 15)
+  __device__ static int row(int lane, int v) { return (lane >> 4) + 4 * v; }
 };
 template <> struct MF<float> {
   typedef float acc_t;
@@ -37,6 +39,7 @@ template <> struct MF<float> {
   __device__ static vacc mma(frag a, frag b, vacc c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  __device__ static int row(int lane, int v) { return (lane >> 4) * 4 + v; }
 };
 template <> struct MF<u16> {
   typedef float acc_t;
@@ -49,6 +52,7 @@ template <> struct MF<u16> {
   __device__ static vacc mma(frag a, frag b, vacc c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
   }
+  __device__ static int row(int lane, int v) { return (lane >> 4) * 4 + v; }
 };
 
 __device__ __forceinline__ u16 to_bf16(float f) {
@@ -163,7 +167,7 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(int M, int N, int K,
       for (int ni = 0; ni < 2; ni++)
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-          const int r = row0 + wm * 32 + mi * 16 + (lane >> 4) * 4 + v;
+          const int r = row0 + wm * 32 + mi * 16 + F::row(lane, v);
           const int c = col0 + wn * 32 + ni * 16 + (lane & 15);
           if (r < M && c < es.ldc) C[(size_t)r * es.ldc + c] = acc[mi][ni][v];
         }
@@ -187,7 +191,7 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(int M, int N, int K,
     for (int mi = 0; mi < 2; mi++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int r = row0 + wm * 32 + mi * 16 + (lane >> 4) * 4 + v;
+        const int r = row0 + wm * 32 + mi * 16 + F::row(lane, v);
         const acc_t kv = acc[mi][ni][v];
         acc_t kz = 0, kz1 = 0;
         if (valid) kpow<acc_t>(kv, ek.zint, zeta, &kz, &kz1);
@@ -209,7 +213,7 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(int M, int N, int K,
       x += __shfl_xor(x, 2, 64);
       x += __shfl_xor(x, 4, 64);
       x += __shfl_xor(x, 8, 64);
-      if ((lane & 15) == 0) sE[wn][wm * 32 + mi * 16 + (lane >> 4) * 4 + v] = x;
+      if ((lane & 15) == 0) sE[wn][wm * 32 + mi * 16 + F::row(lane, v)] = x;
     }
   __syncthreads();
   if (tid < BM) {

@@ -71,8 +71,18 @@ __device__ __forceinline__ int imodd(int a, int m) {
   return r < 0 ? r + m : r;
 }
 
+#define GUARD_RETURN() \
+  if (guard && *guard == 0) return
+
+__global__ void zero_ints_kernel(int* p, int m, const int* guard) {
+  GUARD_RETURN();
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) p[i] = 0;
+}
+
 __global__ void bin_atoms_kernel(const double* pos, int n, Cell c, int* cell_of,
-                                 int* cell_count) {
+                                 int* cell_count, const int* guard) {
+  GUARD_RETURN();
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int cx, cy, cz;
@@ -84,7 +94,8 @@ __global__ void bin_atoms_kernel(const double* pos, int n, Cell c, int* cell_of,
 
 // Exclusive scan of ncells counts by one workgroup (ncells is small).
 __global__ void scan_cells_kernel(const int* count, int ncells, int* start,
-                                  int* fill) {
+                                  int* fill, const int* guard) {
+  GUARD_RETURN();
   __shared__ int part[1024];
   int t = threadIdx.x;
   int per = (ncells + 1023) / 1024;
@@ -109,7 +120,8 @@ __global__ void scan_cells_kernel(const int* count, int ncells, int* start,
 }
 
 __global__ void scatter_cells_kernel(int n, const int* cell_of, const int* start,
-                                     int* fill, int* cell_atoms) {
+                                     int* fill, int* cell_atoms, const int* guard) {
+  GUARD_RETURN();
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int c = cell_of[i];
@@ -121,7 +133,9 @@ __global__ void scatter_cells_kernel(int n, const int* cell_of, const int* start
 __global__ void find_neighbors_kernel(const double* pos, int n, Cell c,
                                       double rcut, const int* cell_start,
                                       const int* cell_atoms, int* count,
-                                      int maxnb, int* list, int* flag) {
+                                      int maxnb, int* list, int* flag,
+                                      const int* guard) {
+  GUARD_RETURN();
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int cnt = 0;
@@ -169,7 +183,9 @@ __global__ void find_neighbors_kernel(const double* pos, int n, Cell c,
 }
 
 // Sort each atom's list ascending (== ascending reference supercell id).
-__global__ void sort_lists_kernel(int n, int maxnb, const int* count, int* list) {
+__global__ void sort_lists_kernel(int n, int maxnb, const int* count, int* list,
+                                  const int* guard) {
+  GUARD_RETURN();
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int cnt = min(count[i], maxnb);
@@ -187,7 +203,9 @@ __global__ void sort_lists_kernel(int n, int maxnb, const int* count, int* list)
 
 // rev[i][slot] = flat index of the entry (neg img, i) in j's list.
 __global__ void reverse_kernel(int n, int maxnb, Cell c, const int* count,
-                               const int* list, int* rev, int* flag) {
+                               const int* list, int* rev, int* flag,
+                               const int* guard) {
+  GUARD_RETURN();
   int i = blockIdx.x;
   int cnt = min(count[i], maxnb);
   for (int s = threadIdx.x; s < cnt; s += blockDim.x) {
@@ -254,30 +272,33 @@ __global__ void csr_out_kernel(const double* pos, int n, Cell c, int maxnb,
   }
 }
 
+// The whole list build; with guard != nullptr every kernel returns at once
+// unless *guard is non-zero (rebuild requested on the device), so the
+// sequence can sit in a captured graph.
 hipError_t launch_neighbors(const double* pos, int n, const Cell& c, double rcut,
                             int* cell_of, int* cell_count, int* cell_start,
                             int* cell_fill, int* cell_atoms, int* count,
                             int maxnb, int* list, int* rev, int* flag,
-                            hipStream_t st) {
+                            hipStream_t st, const int* guard) {
   const int B = 256;
   int nb = (n + B - 1) / B;
-  hipError_t e;
   if (!c.small) {
     int ncell = c.mx * c.my * c.mz;
-    e = hipMemsetAsync(cell_count, 0, sizeof(int) * ncell, st);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(bin_atoms_kernel, dim3(nb), dim3(B), 0, st, pos, n, c, cell_of, cell_count);
+    hipLaunchKernelGGL(zero_ints_kernel, dim3((ncell + B - 1) / B), dim3(B), 0, st, cell_count,
+                       ncell, guard);
+    hipLaunchKernelGGL(bin_atoms_kernel, dim3(nb), dim3(B), 0, st, pos, n, c, cell_of, cell_count,
+                       guard);
     hipLaunchKernelGGL(scan_cells_kernel, dim3(1), dim3(1024), 0, st, cell_count, ncell, cell_start,
-                       cell_fill);
+                       cell_fill, guard);
     hipLaunchKernelGGL(scatter_cells_kernel, dim3(nb), dim3(B), 0, st, n, cell_of, cell_start,
-                       cell_fill, cell_atoms);
+                       cell_fill, cell_atoms, guard);
   }
   hipLaunchKernelGGL(find_neighbors_kernel, dim3(nb), dim3(B), 0, st, pos, n, c, rcut, cell_start,
-                     cell_atoms, count, maxnb, list, flag);
-  hipLaunchKernelGGL(sort_lists_kernel, dim3(nb), dim3(B), 0, st, n, maxnb, count, list);
+                     cell_atoms, count, maxnb, list, flag, guard);
+  hipLaunchKernelGGL(sort_lists_kernel, dim3(nb), dim3(B), 0, st, n, maxnb, count, list, guard);
   if (rev)
     hipLaunchKernelGGL(reverse_kernel, dim3(n), dim3(64), 0, st, n, maxnb, c, count, list, rev,
-                       flag);
+                       flag, guard);
   return hipGetLastError();
 }
 
@@ -662,10 +683,12 @@ static hipError_t fwd_inst(const SoapDev& sdh, const SoapDev* sd, const double* 
                            const int* site_atom, int n_sites, const int2* pairn, T* q, int q_ld,
                            unsigned short* qb, int qb_ld, T* cbuf, T* pnorm, hipStream_t st) {
   size_t sm = sizeof(T) * ((size_t)64 * sdh.L2 + 64 * NRP + (size_t)NRP * sdh.L2);
-  if (sm > 65536) {
+  static size_t sm_set = 65536;
+  if (sm > sm_set) {
     hipError_t e = hipFuncSetAttribute((const void*)soap_fwd_kernel<T, NRP, NMX>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
     if (e != hipSuccess) return e;
+    sm_set = sm;
   }
   hipLaunchKernelGGL((soap_fwd_kernel<T, NRP, NMX>), dim3(n_sites), dim3(64), sm, st, sd, W,
                      reinterpret_cast<const float2*>(rtab), pos, species, cell, maxnb, count,
@@ -702,10 +725,12 @@ static hipError_t bwd_inst(const SoapDev& sdh, const SoapDev* sd, const double* 
                            int q_ld, const T* cbuf, const T* pnorm, const T* G, int g_ld,
                            T* gpair, double* vsite, hipStream_t st) {
   size_t sm = sizeof(T) * ((size_t)2 * NRP * sdh.L2 + sdh.dim);
-  if (sm > 65536) {
+  static size_t sm_set = 65536;
+  if (sm > sm_set) {
     hipError_t e = hipFuncSetAttribute((const void*)soap_bwd_kernel<T, NRP, NMX>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
     if (e != hipSuccess) return e;
+    sm_set = sm;
   }
   hipLaunchKernelGGL((soap_bwd_kernel<T, NRP, NMX>), dim3(n_sites), dim3(64), sm, st, sd, W,
                      reinterpret_cast<const float2*>(rtab), pos, species, cell, maxnb, count,

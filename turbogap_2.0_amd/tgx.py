@@ -218,6 +218,11 @@ def load_library(path: str = LIB_PATH):
         "tgx_energy_forces_batched": (i, [P, i, ip, dp, ip, dp, i, dp, dp, dp]),
         "tgx_md_run": (i, [P, i, i, d, d, ip, dp, dp, dp, dp, dp, dp, dp, ip]),
         "tgx_stage_times": (i, [P, dp]),
+        "tgx_md_begin": (i, [P, i, ip, dp, dp, dp, dp, d, d, i]),
+        "tgx_md_steps": (i, [P, i]),
+        "tgx_md_read": (i, [P, dp, dp, dp, dp, dp, ip, ip]),
+        "tgx_md_end": (i, [P]),
+        "tgx_sync": (i, [P]),
         "tgx_poly3_orthonormalization": (i, [i, dp, dp]),
     }
     for name, (res, args) in sig.items():
@@ -388,6 +393,35 @@ class Engine:
                                         _dptr(m), _dptr(pos), _dptr(vel), _dptr(f), _dptr(ep),
                                         _dptr(ek), C.byref(nreb)))
         return dict(pos=pos, vel=vel, forces=f, e_pot=ep, e_kin=ek, n_rebuilds=nreb.value)
+
+    def md_begin(self, pos, vel, species, lattice, masses_amu, dt: float, skin: float,
+                 history: int = 1024):
+        pos, vel = _f64(pos), _f64(vel)
+        lat, sp, m = _f64(lattice), _i32(species), _f64(masses_amu)
+        self._md_n = pos.shape[0]
+        self._md_hist = history
+        self._check(self.lib.tgx_md_begin(self.ctx, self._md_n, _iptr(sp), _dptr(lat), _dptr(m),
+                                          _dptr(pos), _dptr(vel), dt, skin, history))
+
+    def md_steps(self, n_steps: int):
+        self._check(self.lib.tgx_md_steps(self.ctx, n_steps))
+
+    def md_read(self):
+        n = self._md_n
+        pos, vel, f = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        ep, ek = np.zeros(self._md_hist), np.zeros(self._md_hist)
+        done, nreb = C.c_int(0), C.c_int(0)
+        self._check(self.lib.tgx_md_read(self.ctx, _dptr(pos), _dptr(vel), _dptr(f), _dptr(ep),
+                                         _dptr(ek), C.byref(done), C.byref(nreb)))
+        m = min(done.value + 1, self._md_hist)
+        return dict(pos=pos, vel=vel, forces=f, e_pot=ep[:m], e_kin=ek[:m], steps=done.value,
+                    n_rebuilds=nreb.value)
+
+    def md_end(self):
+        self._check(self.lib.tgx_md_end(self.ctx))
+
+    def sync(self):
+        self._check(self.lib.tgx_sync(self.ctx))
 
     def stage_times(self):
         out = np.zeros(7)
