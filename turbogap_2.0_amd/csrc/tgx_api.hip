@@ -41,7 +41,7 @@ hipError_t launch_soap_bwd(const SoapDev& sdh, const SoapDev* sd, const double* 
 hipError_t launch_two_body(const TwoBodyDev& b, const double* pos, const int* species, int n,
                            const Cell& c, int maxnb, const int* count, const int* list,
                            double* eatom, double* forces, double* vatom, int do_forces,
-                           hipStream_t st);
+                           int exact, hipStream_t st);
 hipError_t launch_energy_init(int n, const int* species, const double* e0,
                               const int* site_of_atom, const double* esite, double* eatom,
                               double* forces, double* vatom, hipStream_t st);
@@ -320,7 +320,8 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Cell& cell, int do_fo
   ENSURE(c, forces, 24 * (size_t)n);
   ENSURE(c, vsite, 48 * (size_t)std::max(n_sites, 1));
   ENSURE(c, virial, 48 * (size_t)n);  // per-atom 2b virial
-  if (x.out.ensure(16 * sizeof(double))) return set_err(c, TGX_ERR_HIP, "allocation failed");
+  if (x.out.ensure((16 + 8 * 128) * sizeof(double)))
+    return set_err(c, TGX_ERR_HIP, "allocation failed");
   if (c->has_soap && n_sites > 0) {
     rcode = soap_forward(c, n, cell, n_sites);
     if (rcode) return rcode;
@@ -373,7 +374,7 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Cell& cell, int do_fo
     TGX_HIP(c, launch_two_body(b, c->pos.as<double>(), c->species.as<int>(), n, cell, c->maxnb,
                                c->nb_count.as<int>(), c->nb_list.as<int>(), c->eatom.as<double>(),
                                c->forces.as<double>(), c->virial.as<double>(), do_forces,
-                               c->stream));
+                               prec == TGX_FP64, c->stream));
   ev_rec(c, 5);
   TGX_HIP(c, launch_reduce((do_forces && c->has_soap) ? n_sites : 0,
                            (do_forces && c->has_soap) ? c->vsite.as<double>() : nullptr, n,
@@ -468,6 +469,7 @@ int tgx_destroy(tgx_ctx* c) {
   for (auto& b : c->twob) {
     if (b.Qs) (void)hipFree(b.Qs);
     if (b.alphas) (void)hipFree(b.alphas);
+    if (b.tab) (void)hipFree(b.tab);
   }
   if (f->x.ev_ready)
     for (int k = 0; k < 8; k++) (void)hipEventDestroy(f->x.ev[k]);
@@ -612,6 +614,7 @@ int tgx_set_soap_model(tgx_ctx* c, const tgx_soap_hypers* h, int dim, int n_spar
   TGX_HIP(c, hipMemcpyAsync(c->W_dev.p, W.data(), 8 * W.size(), hipMemcpyHostToDevice, st));
   TGX_HIP(c, hipMemcpyAsync(x.sd_dev.p, &c->sd, sizeof(SoapDev), hipMemcpyHostToDevice, st));
   TGX_HIP(c, hipMemcpyAsync(x.pairn.p, pn.data(), sizeof(int2) * pn.size(), hipMemcpyHostToDevice, st));
+  TGX_HIP(c, hipMemsetAsync(c->rtab.p, 0, c->rtab.bytes, st));
   TGX_HIP(c, launch_build_rtab(x.sd_dev.as<SoapDev>(), sd.n_species, c->W_dev.as<double>(),
                                c->rtab.as<float>(), st));
   TGX_HIP(c, hipStreamSynchronize(st));
@@ -654,10 +657,30 @@ int tgx_add_2b_model(tgx_ctx* c, int sp1, int sp2, double rcut, double buffer, d
   b.sigma = sigma;
   std::vector<double> ac(n_sparse);
   for (int k = 0; k < n_sparse; k++) ac[k] = alphas[k] * cutoff[k];
+  // FP32 table of g(r) = delta^2 sum_s alpha_s c_s exp(-(r-Q_s)^2/(2 sigma^2))
+  // and h g'(r) on [0, rcut], evaluated in FP64 (the cutoff is applied
+  // analytically by the kernel).
+  b.nt = 4096;
+  std::vector<float> tab(2 * (size_t)(b.nt + 1));
+  const double hh = rcut / b.nt, is2 = 1.0 / (sigma * sigma);
+  for (int k = 0; k <= b.nt; k++) {
+    const double r = k * hh;
+    double g = 0.0, dg = 0.0;
+    for (int s = 0; s < n_sparse; s++) {
+      const double dr = r - Qs[s];
+      const double ex = std::exp(-0.5 * dr * dr * is2);
+      g += ac[s] * ex;
+      dg -= ac[s] * ex * dr * is2;
+    }
+    tab[2 * k] = (float)(delta * delta * g);
+    tab[2 * k + 1] = (float)(hh * delta * delta * dg);
+  }
   TGX_HIP(c, hipMalloc(&b.Qs, 8 * (size_t)n_sparse));
   TGX_HIP(c, hipMalloc(&b.alphas, 8 * (size_t)n_sparse));
+  TGX_HIP(c, hipMalloc(&b.tab, sizeof(float) * tab.size()));
   TGX_HIP(c, hipMemcpy(b.Qs, Qs, 8 * (size_t)n_sparse, hipMemcpyHostToDevice));
   TGX_HIP(c, hipMemcpy(b.alphas, ac.data(), 8 * (size_t)n_sparse, hipMemcpyHostToDevice));
+  TGX_HIP(c, hipMemcpy(b.tab, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice));
   c->twob.push_back(b);
   c->maxnb = 0;
   Extra& x = X(c);
@@ -674,6 +697,7 @@ int tgx_clear_2b_models(tgx_ctx* c) {
   for (auto& b : c->twob) {
     (void)hipFree(b.Qs);
     (void)hipFree(b.alphas);
+    (void)hipFree(b.tab);
   }
   c->twob.clear();
   c->maxnb = 0;

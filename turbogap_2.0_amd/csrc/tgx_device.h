@@ -204,7 +204,8 @@ __device__ __forceinline__ void radial_envelope(const SoapDev* sd, int s, T r,
 }
 
 // FP32 lookup of B_n(r), dB_n/dr from the cubic-Hermite table
-// tab[s][k][n] = (B_n(r_k), h dB_n/dr(r_k)), r_k = k h, h = rh/TGX_RTAB_N.
+// tab[s][k][n] = (B_n(r_k), h dB_n/dr(r_k)), r_k = k h, h = rh/TGX_RTAB_N;
+// entries with n >= n_max(s) are zero.
 template <int NMX>
 __device__ __forceinline__ void radial_basis_tab(const SoapDev* sd,
                                                  const float2* tab, int s,
@@ -225,13 +226,10 @@ __device__ __forceinline__ void radial_basis_tab(const SoapDev* sd,
   const float2* r1 = r0 + TGX_MAX_NR;
 #pragma unroll
   for (int n = 0; n < NMX; n++) {
-    if (n < nmax) {
-      float2 a = r0[n], b = r1[n];
-      Bv[n] = h00 * a.x + h10 * a.y + h01 * b.x + h11 * b.y;
-      dB[n] = (g00 * a.x + g10 * a.y + g01 * b.x + g11 * b.y) * hinv;
-    } else {
-      Bv[n] = dB[n] = 0.f;
-    }
+    // rows are zero beyond the species' n_max, so no guard is needed
+    const float2 a = r0[n], b = r1[n];
+    Bv[n] = h00 * a.x + h10 * a.y + h01 * b.x + h11 * b.y;
+    dB[n] = (g00 * a.x + g10 * a.y + g01 * b.x + g11 * b.y) * hinv;
   }
 }
 

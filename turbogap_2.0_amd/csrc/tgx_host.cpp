@@ -210,6 +210,9 @@ int make_cell(const double* lat, double rcut, Cell* c, std::string* err) {
   c->La = (double)idx[0] * lat[0];
   c->Lb = (double)idx[1] * lat[4];
   c->Lc = (double)idx[2] * lat[8];
+  c->iLa = 1.0 / c->La;
+  c->iLb = 1.0 / c->Lb;
+  c->iLc = 1.0 / c->Lc;
   c->mx = (int)(c->La / rcut);
   c->my = (int)(c->Lb / rcut);
   c->mz = (int)(c->Lc / rcut);

@@ -30,6 +30,7 @@ struct Cell {
   int nrep;              // ia*ib*ic
   int small;             // any repetition > 1
   int mx, my, mz;        // linked-cell grid (large boxes only)
+  double iLa, iLb, iLc;  // 1 / supercell edges
 };
 
 // SOAP model constants in device-friendly form.
@@ -49,6 +50,8 @@ struct TwoBodyDev {
   double rcut, buffer, delta, sigma;
   double* Qs;      // device
   double* alphas;  // device (alpha * cutoff folded in)
+  float2* tab;     // device: (g(r_k), h g'(r_k)), r_k = k rcut/nt, k = 0..nt
+  int nt;
 };
 
 // Growable device buffer.
@@ -60,8 +63,6 @@ struct DevBuf {
   template <typename T>
   T* as() const { return reinterpret_cast<T*>(p); }
 };
-
-struct Timer;
 
 }  // namespace tgx
 
@@ -117,70 +118,18 @@ int make_cell(const double* lattice, double rcut, Cell* cell, std::string* err);
 void sph_constants(int lmax, float* A, float* B, float* Pmm, double* Ad,
                    double* Bd, double* Pmmd);
 
-// Kernel launchers (return hipError_t)
+// Kernel launchers are declared next to their callers in tgx_api.hip.
 hipError_t upload_sph_constants(int lmax, hipStream_t st);
-hipError_t launch_build_rtab(const SoapDev& sd, const double* W, float* tab,
-                             hipStream_t st);
-
-// Neighbours
-hipError_t launch_bin_atoms(const double* pos, int n, Cell cell, int* cell_of,
-                            int* cell_count, hipStream_t st);
-hipError_t launch_count_neighbors(const double* pos, int n, Cell cell,
-                                  double rcut2_dummy, double rcut,
-                                  const int* cell_start, const int* cell_atoms,
-                                  int* count, int maxnb, int* list, int* flag,
-                                  hipStream_t st);
-hipError_t launch_sort_lists(int n, int maxnb, const int* count, int* list,
-                             hipStream_t st);
-hipError_t launch_reverse(int n, int maxnb, Cell cell, const int* count,
-                          const int* list, int* rev, int* flag, hipStream_t st);
-hipError_t launch_csr_out(const double* pos, int n, Cell cell, int maxnb,
-                          const int* count, const int* list, const int* off,
-                          int* out_list, double* rjs, double* xyz, double* th,
-                          double* ph, hipStream_t st);
-
-// SOAP
-template <typename T>
-hipError_t launch_soap_fwd(const SoapDev& sd, const double* pos,
-                           const int* species, int n, Cell cell, int maxnb,
-                           const int* count, const int* list,
-                           const int* site_atom, int n_sites, const double* W,
-                           const float* rtab, T* q, void* qb, int qb_ld,
-                           T* cbuf, T* pnorm, hipStream_t st);
-template <typename T>
-hipError_t launch_soap_bwd(const SoapDev& sd, const double* pos,
-                           const int* species, int n, Cell cell, int maxnb,
-                           const int* count, const int* list,
-                           const int* site_atom, int n_sites, const double* W,
-                           const float* rtab, const T* q, const T* cbuf,
-                           const T* pnorm, const T* G, int g_ld, T* gpair,
-                           double* vsite, hipStream_t st);
-
-// Kernel step GEMMs.  mode: TGX_FP64 / TGX_FP32 / TGX_MIXED.
 hipError_t launch_gap_kernel(int mode, int n_sites, int dim, int dim_pad,
                              int n_sparse, int sp_pad, const void* q,
                              const void* qs, const void* qsT,
                              const void* alphas, double delta, double zeta,
                              void* wmat, void* epart, double* esite, void* G,
                              int want_grad, hipStream_t st);
-
-// Forces, 2-body, reductions, MD
 template <typename T>
 hipError_t launch_force_gather(int n, int maxnb, const int* count,
                                const int* list, const int* rev,
                                const int* site_of_atom, const T* gpair,
                                double* forces, hipStream_t st);
-template <typename T>
-hipError_t launch_two_body(const TwoBodyDev& b, const double* pos,
-                           const int* species, int n, Cell cell, int maxnb,
-                           const int* count, const int* list, double* eatom,
-                           double* forces, double* vatom, int do_forces,
-                           hipStream_t st);
-hipError_t launch_energy_init(int n, const int* species, const double* e0dev,
-                              const int* site_of_atom, const double* esite,
-                              double* eatom, hipStream_t st);
-hipError_t launch_reduce_virial(int n_rows, const double* v6rows,
-                                double* out9, double* energy_in,
-                                const double* eatom, int n, hipStream_t st);
 
 }  // namespace tgx

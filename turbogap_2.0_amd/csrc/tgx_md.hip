@@ -11,25 +11,37 @@ namespace tgx {
 
 // v(t) = v(t-dt) + 1/2 (F(t) + F(t-dt))/m dt  (skipped on the first step),
 // x(t+dt) = x(t) + v dt + 1/2 F/m dt^2, with the reference's operation order.
-__global__ void md_vv_kernel(int n, double* pos, double* vel, const double* f,
-                             double* fprev, const double* mass, double dt,
-                             const int* step, double* ke) {
+// ke_part[block] = kinetic energy of the block's atoms (fixed tree).
+__global__ __launch_bounds__(256) void md_vv_kernel(int n, double* pos, double* vel,
+                                                    const double* f, double* fprev,
+                                                    const double* mass, double dt,
+                                                    const int* step, double* ke_part) {
 #pragma clang fp contract(off)
+  __shared__ double sh[256];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const bool first = (*step == 0);
-  const double m = mass[i];
-  double v2 = 0.0;
-  for (int k = 0; k < 3; k++) {
-    const int a = 3 * i + k;
-    double v = vel[a];
-    if (!first) v = v + 0.5 * (f[a] + fprev[a]) / m * dt;
-    vel[a] = v;
-    v2 = v2 + v * v;
-    fprev[a] = f[a];
-    pos[a] = pos[a] + v * dt + 0.5 * f[a] / m * (dt * dt);
+  double kei = 0.0;
+  if (i < n) {
+    const bool first = (*step == 0);
+    const double m = mass[i];
+    double v2 = 0.0;
+    for (int k = 0; k < 3; k++) {
+      const int a = 3 * i + k;
+      double v = vel[a];
+      if (!first) v = v + 0.5 * (f[a] + fprev[a]) / m * dt;
+      vel[a] = v;
+      v2 = v2 + v * v;
+      fprev[a] = f[a];
+      pos[a] = pos[a] + v * dt + 0.5 * f[a] / m * (dt * dt);
+    }
+    kei = 0.5 * m * v2;
   }
-  ke[i] = 0.5 * m * v2;
+  sh[threadIdx.x] = kei;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) ke_part[blockIdx.x] = sh[0];
 }
 
 // Rebuild when any atom moved more than skin/2 since the last build.
@@ -57,7 +69,8 @@ __global__ void md_finish_kernel(int n, const double* ke, const double* out,
   __shared__ double sh[256];
   const int t = threadIdx.x;
   double a = 0.0;
-  for (int i = t; i < n; i += 256) a += ke[i];
+  const int nb = (n + 255) / 256;  // per-block partials written by md_vv_kernel
+  for (int i = t; i < nb; i += 256) a += ke[i];
   sh[t] = a;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
