@@ -29,13 +29,13 @@ template <typename T>
 hipError_t launch_soap_fwd(const SoapDev& sdh, const SoapDev* sd, const double* W,
                            const float* rtab, const double* pos, const int* species,
                            const Cell& cell, int maxnb, const int* count, const int* list,
-                           const int* site_atom, int n_sites, const int2* pairn, T* q, int q_ld,
+                           const int* site_atom, int n_sites, const int* dmap, T* q, int q_ld,
                            unsigned short* qb, int qb_ld, T* cbuf, T* pnorm, hipStream_t st);
 template <typename T>
 hipError_t launch_soap_bwd(const SoapDev& sdh, const SoapDev* sd, const double* W,
                            const float* rtab, const double* pos, const int* species,
                            const Cell& cell, int maxnb, const int* count, const int* list,
-                           const int* site_atom, int n_sites, const int2* pairn, const T* q,
+                           const int* site_atom, int n_sites, const int* dmap, const T* q,
                            int q_ld, const T* cbuf, const T* pnorm, const T* G, int g_ld,
                            T* gpair, double* vsite, hipStream_t st);
 hipError_t launch_two_body(const TwoBodyDev& b, const double* pos, const int* species, int n,
@@ -220,7 +220,7 @@ static int soap_forward(tgx_ctx* c, int n, const Cell& cell, int n_sites) {
   Extra& x = X(c);
   const int prec = c->precision;
   const size_t ts = tsize(prec);
-  const int NRP = (c->sd.nr + 7) / 8 * 8;
+  const int NRP = c->sd.nrp;
   ENSURE(c, q, ts * (size_t)std::max(n_sites, 1) * c->dim_pad);
   ENSURE(c, cbuf, ts * (size_t)std::max(n_sites, 1) * NRP * c->sd.L2);
   ENSURE(c, pnorm, ts * (size_t)std::max(n_sites, 1));
@@ -234,14 +234,14 @@ static int soap_forward(tgx_ctx* c, int n, const Cell& cell, int n_sites) {
     e = launch_soap_fwd<double>(c->sd, x.sd_dev.as<SoapDev>(), c->W_dev.as<double>(),
                                 c->rtab.as<float>(), c->pos.as<double>(), c->species.as<int>(),
                                 cell, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
-                                c->site_atom.as<int>(), n_sites, x.pairn.as<int2>(),
+                                c->site_atom.as<int>(), n_sites, x.pairn.as<int>(),
                                 c->q.as<double>(), c->dim_pad, nullptr, 0, c->cbuf.as<double>(),
                                 c->pnorm.as<double>(), c->stream);
   else
     e = launch_soap_fwd<float>(c->sd, x.sd_dev.as<SoapDev>(), c->W_dev.as<double>(),
                                c->rtab.as<float>(), c->pos.as<double>(), c->species.as<int>(),
                                cell, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
-                               c->site_atom.as<int>(), n_sites, x.pairn.as<int2>(),
+                               c->site_atom.as<int>(), n_sites, x.pairn.as<int>(),
                                c->q.as<float>(), c->dim_pad, qb, c->dim_pad, c->cbuf.as<float>(),
                                c->pnorm.as<float>(), c->stream);
   TGX_HIP(c, e);
@@ -346,7 +346,7 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Cell& cell, int do_fo
       e = launch_soap_bwd<double>(c->sd, x.sd_dev.as<SoapDev>(), c->W_dev.as<double>(),
                                   c->rtab.as<float>(), c->pos.as<double>(), c->species.as<int>(),
                                   cell, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
-                                  c->site_atom.as<int>(), n_sites, x.pairn.as<int2>(),
+                                  c->site_atom.as<int>(), n_sites, x.pairn.as<int>(),
                                   c->q.as<double>(), c->dim_pad, c->cbuf.as<double>(),
                                   c->pnorm.as<double>(), c->gmat.as<double>(), c->dim_pad,
                                   c->gpair.as<double>(), c->vsite.as<double>(), c->stream);
@@ -358,7 +358,7 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Cell& cell, int do_fo
       e = launch_soap_bwd<float>(c->sd, x.sd_dev.as<SoapDev>(), c->W_dev.as<double>(),
                                  c->rtab.as<float>(), c->pos.as<double>(), c->species.as<int>(),
                                  cell, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
-                                 c->site_atom.as<int>(), n_sites, x.pairn.as<int2>(),
+                                 c->site_atom.as<int>(), n_sites, x.pairn.as<int>(),
                                  c->q.as<float>(), c->dim_pad, c->cbuf.as<float>(),
                                  c->pnorm.as<float>(), c->gmat.as<float>(), c->dim_pad,
                                  c->gpair.as<float>(), c->vsite.as<double>(), c->stream);
@@ -531,6 +531,16 @@ int tgx_set_soap_model(tgx_ctx* c, const tgx_soap_hypers* h, int dim, int n_spar
   if (nr > TGX_MAX_NR) return set_err(c, TGX_ERR_ARG, "sum of alpha_max %d > %d", nr, TGX_MAX_NR);
   sd.nr = nr;
   sd.npair = nr * (nr + 1) / 2;
+  {
+    int nmx = 0, need = 0;
+    for (int s = 0; s < h->n_species; s++) nmx = std::max(nmx, h->alpha_max[s]);
+    sd.nmx = nmx <= 8 ? 8 : 16;
+    for (int s = 0; s < h->n_species; s++) need = std::max(need, sd.off[s] + sd.nmx);
+    sd.nrp = (std::max(need, nr) + 7) / 8 * 8;
+    if (sd.nrp > TGX_MAX_NR)
+      return set_err(c, TGX_ERR_UNSUPPORTED, "radial channels (padded %d) exceed %d", sd.nrp,
+                     TGX_MAX_NR);
+  }
   sd.dim = sd.npair * (h->l_max + 1);
   sd.rcut_max = rcm;
   if (h->central_species < 0 || h->central_species > h->n_species)
@@ -595,13 +605,15 @@ int tgx_set_soap_model(tgx_ctx* c, const tgx_soap_hypers* h, int dim, int n_spar
   ENSURE(c, W_dev, 8 * W.size());
   ENSURE(c, rtab, sizeof(float) * 2 * (size_t)TGX_MAX_SPECIES * (TGX_RTAB_N + 1) * TGX_MAX_NR);
   if (x.sd_dev.ensure(sizeof(SoapDev))) return set_err(c, TGX_ERR_HIP, "allocation failed");
-  std::vector<int2> pn(sd.npair);
+  // component d of the power spectrum -> (n, n', l), packed n | n' << 8 | l << 16
+  std::vector<int> pn(sd.dim);
   {
     int k = 0;
     for (int a = 0; a < nr; a++)
-      for (int b = a; b < nr; b++) pn[k++] = make_int2(a, b);
+      for (int b = a; b < nr; b++)
+        for (int l = 0; l <= sd.l_max; l++) pn[k++] = a | (b << 8) | (l << 16);
   }
-  if (x.pairn.ensure(sizeof(int2) * pn.size())) return set_err(c, TGX_ERR_HIP, "allocation failed");
+  if (x.pairn.ensure(sizeof(int) * pn.size())) return set_err(c, TGX_ERR_HIP, "allocation failed");
   hipStream_t st = c->stream;
   TGX_HIP(c, hipMemcpyAsync(c->Qs64.p, q64.data(), 8 * q64.size(), hipMemcpyHostToDevice, st));
   TGX_HIP(c, hipMemcpyAsync(c->QsT64.p, qT64.data(), 8 * qT64.size(), hipMemcpyHostToDevice, st));
@@ -613,7 +625,7 @@ int tgx_set_soap_model(tgx_ctx* c, const tgx_soap_hypers* h, int dim, int n_spar
   TGX_HIP(c, hipMemcpyAsync(c->alph32.p, a32.data(), 4 * (size_t)Sp, hipMemcpyHostToDevice, st));
   TGX_HIP(c, hipMemcpyAsync(c->W_dev.p, W.data(), 8 * W.size(), hipMemcpyHostToDevice, st));
   TGX_HIP(c, hipMemcpyAsync(x.sd_dev.p, &c->sd, sizeof(SoapDev), hipMemcpyHostToDevice, st));
-  TGX_HIP(c, hipMemcpyAsync(x.pairn.p, pn.data(), sizeof(int2) * pn.size(), hipMemcpyHostToDevice, st));
+  TGX_HIP(c, hipMemcpyAsync(x.pairn.p, pn.data(), sizeof(int) * pn.size(), hipMemcpyHostToDevice, st));
   TGX_HIP(c, hipMemsetAsync(c->rtab.p, 0, c->rtab.bytes, st));
   TGX_HIP(c, launch_build_rtab(x.sd_dev.as<SoapDev>(), sd.n_species, c->W_dev.as<double>(),
                                c->rtab.as<float>(), st));

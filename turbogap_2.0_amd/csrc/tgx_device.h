@@ -15,6 +15,7 @@ namespace tgx {
 struct SphConst {
   float A[TGX_L2MAX], B[TGX_L2MAX], Pmm[TGX_MAX_LMAX + 1];
   double Ad[TGX_L2MAX], Bd[TGX_L2MAX], Pmmd[TGX_MAX_LMAX + 1];
+  int lm2l[TGX_L2MAX];  // l of the combined index lm = l*l + l + m
 };
 extern __constant__ SphConst c_sph;
 

@@ -43,6 +43,10 @@ struct SoapDev {
   double amp_scal[TGX_MAX_SPECIES], glob[TGX_MAX_SPECIES];
   double rself[TGX_MAX_SPECIES * TGX_MAX_NR];  // central weight * R_n(0) * Y00
   double rcut_max;
+  // Kernel shape: nmx = per-species n_max bound (8 or 16); nrp = radial
+  // channels padded so that off[s] + nmx <= nrp for every species (rows of
+  // width nrp can then be read nmx wide from any species offset).
+  int nmx, nrp;
 };
 
 struct TwoBodyDev {

@@ -117,75 +117,105 @@ __global__ void scatter_cells_kernel(int n, const int* cell_of, const int* start
   cell_atoms[start[c] + slot] = i;
 }
 
-// Candidates of atom i within rcut, unsorted, packed (img<<26 | j).
+// Candidates of atom i within rcut, unsorted, packed (img<<26 | j).  Four
+// lanes per atom split the 27 neighbouring cells; a rounding minimum image
+// screens candidates and only those within rcut (plus a margin far above
+// rounding) are decided with the reference's arithmetic (mic), so the pair
+// set is exactly the reference's.  count[] must be zero on entry.
+#define TGX_NB_LANES 4
 __global__ void find_neighbors_kernel(const double* pos, int n, Cell c,
                                       double rcut, const int* cell_start,
                                       const int* cell_atoms, int* count,
                                       int maxnb, int* list, int* flag,
                                       const int* guard) {
   GUARD_RETURN();
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = t / TGX_NB_LANES, part = t % TGX_NB_LANES;
   if (i >= n) return;
-  int cnt = 0;
   int* out = list + (size_t)i * maxnb;
   const double* pi = pos + 3 * i;
+  const double screen2 = (rcut + 1e-6) * (rcut + 1e-6);
   if (!c.small) {
     int cx, cy, cz;
     cell_coords(pi, c, &cx, &cy, &cz);
-    int seen[27];
-    int ns = 0;
-    for (int k3 = cz - 1; k3 <= cz + 1; k3++)
-      for (int j3 = cy - 1; j3 <= cy + 1; j3++)
-        for (int i3 = cx - 1; i3 <= cx + 1; i3++) {
-          int cell = imodd(i3, c.mx) + imodd(j3, c.my) * c.mx + imodd(k3, c.mz) * c.mx * c.my;
-          bool dup = false;
-          for (int q = 0; q < ns; q++) dup |= (seen[q] == cell);
-          if (dup) continue;
-          seen[ns++] = cell;
-          for (int a = cell_start[cell]; a < cell_start[cell + 1]; a++) {
-            int k = cell_atoms[a];
-            if (k == i) continue;
-            double dv[3];
-            double d = mic(pi, pos + 3 * k, c.La, c.Lb, c.Lc, dv);
-            if (d < rcut) {
-              if (cnt < maxnb) out[cnt] = k;
-              cnt++;
-            }
-          }
+    for (int q = part; q < 27; q += TGX_NB_LANES) {
+      const int i3 = cx - 1 + q % 3, j3 = cy - 1 + (q / 3) % 3, k3 = cz - 1 + q / 9;
+      const int cell = imodd(i3, c.mx) + imodd(j3, c.my) * c.mx + imodd(k3, c.mz) * c.mx * c.my;
+      // skip a cell already visited by a smaller q (grids of 1 or 2 cells)
+      bool dup = false;
+      for (int q2 = 0; q2 < q; q2++) {
+        const int a3 = cx - 1 + q2 % 3, b3 = cy - 1 + (q2 / 3) % 3, c3 = cz - 1 + q2 / 9;
+        dup |= (imodd(a3, c.mx) + imodd(b3, c.my) * c.mx + imodd(c3, c.mz) * c.mx * c.my) == cell;
+      }
+      if (dup) continue;
+      for (int a = cell_start[cell]; a < cell_start[cell + 1]; a++) {
+        const int k = cell_atoms[a];
+        if (k == i) continue;
+        const double* pk = pos + 3 * k;
+        double dx = pk[0] - pi[0], dy = pk[1] - pi[1], dz = pk[2] - pi[2];
+        dx -= c.La * rint(dx * c.iLa);
+        dy -= c.Lb * rint(dy * c.iLb);
+        dz -= c.Lc * rint(dz * c.iLc);
+        if (dx * dx + dy * dy + dz * dz > screen2) continue;
+        double dv[3];
+        const double d = mic(pi, pk, c.La, c.Lb, c.Lc, dv);
+        if (d < rcut) {
+          const int slot = atomicAdd(count + i, 1);
+          if (slot < maxnb) out[slot] = k;
+          else atomicMax(flag, slot + 1);
         }
+      }
+    }
   } else {
     for (int img = 0; img < c.nrep; img++)
-      for (int k = 0; k < n; k++) {
+      for (int k = part; k < n; k += TGX_NB_LANES) {
         if (img == 0 && k == i) continue;
         double pk[3], dv[3];
         image_pos(pos, k, img, c, pk);
-        double d = mic(pi, pk, c.La, c.Lb, c.Lc, dv);
+        const double d = mic(pi, pk, c.La, c.Lb, c.Lc, dv);
         if (d < rcut) {
-          if (cnt < maxnb) out[cnt] = (img << TGX_IMG_SHIFT) | k;
-          cnt++;
+          const int slot = atomicAdd(count + i, 1);
+          if (slot < maxnb) out[slot] = (img << TGX_IMG_SHIFT) | k;
+          else atomicMax(flag, slot + 1);
         }
       }
   }
-  count[i] = cnt;
-  if (cnt > maxnb) atomicMax(flag, cnt);
 }
 
-// Sort each atom's list ascending (== ascending reference supercell id).
-__global__ void sort_lists_kernel(int n, int maxnb, const int* count, int* list,
-                                  const int* guard) {
+// Sort each atom's list ascending (== ascending reference supercell id):
+// one wave per atom, bitonic network over 64 lanes; lists longer than 64
+// entries fall back to an insertion sort by lane 0.
+__global__ __launch_bounds__(256) void sort_lists_kernel(int n, int maxnb, const int* count,
+                                                         int* list, const int* guard) {
   GUARD_RETURN();
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (i >= n) return;
-  int cnt = min(count[i], maxnb);
+  const int cnt = min(count[i], maxnb);
   int* a = list + (size_t)i * maxnb;
-  for (int p = 1; p < cnt; p++) {
-    int v = a[p];
-    int q = p - 1;
-    while (q >= 0 && a[q] > v) {
-      a[q + 1] = a[q];
-      q--;
+  if (cnt <= 64) {
+    int v = lane < cnt ? a[lane] : 0x7fffffff;
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        const int o = __shfl_xor(v, j, 64);
+        const bool up = (lane & k) == 0;
+        const bool lower = (lane & j) == 0;
+        v = (lower == up) ? min(v, o) : max(v, o);
+      }
     }
-    a[q + 1] = v;
+    if (lane < cnt) a[lane] = v;
+  } else if (lane == 0) {
+    for (int p = 1; p < cnt; p++) {
+      const int v = a[p];
+      int q = p - 1;
+      while (q >= 0 && a[q] > v) {
+        a[q + 1] = a[q];
+        q--;
+      }
+      a[q + 1] = v;
+    }
   }
 }
 
@@ -281,9 +311,11 @@ hipError_t launch_neighbors(const double* pos, int n, const Cell& c, double rcut
     hipLaunchKernelGGL(scatter_cells_kernel, dim3(nb), dim3(B), 0, st, n, cell_of, cell_start,
                        cell_fill, cell_atoms, guard);
   }
-  hipLaunchKernelGGL(find_neighbors_kernel, dim3(nb), dim3(B), 0, st, pos, n, c, rcut, cell_start,
-                     cell_atoms, count, maxnb, list, flag, guard);
-  hipLaunchKernelGGL(sort_lists_kernel, dim3(nb), dim3(B), 0, st, n, maxnb, count, list, guard);
+  hipLaunchKernelGGL(zero_ints_kernel, dim3(nb), dim3(B), 0, st, count, n, guard);
+  hipLaunchKernelGGL(find_neighbors_kernel, dim3((n * TGX_NB_LANES + B - 1) / B), dim3(B), 0, st,
+                     pos, n, c, rcut, cell_start, cell_atoms, count, maxnb, list, flag, guard);
+  hipLaunchKernelGGL(sort_lists_kernel, dim3((n + 3) / 4), dim3(256), 0, st, n, maxnb, count,
+                     list, guard);
   if (rev)
     hipLaunchKernelGGL(reverse_kernel, dim3(n), dim3(64), 0, st, n, maxnb, c, count, list, rev,
                        flag, guard);
@@ -307,16 +339,17 @@ hipError_t launch_csr_out(const double* pos, int n, const Cell& c, int maxnb,
 // Forces: F_k = sum_slots (g[k][slot] - g[rev[k][slot]]), fixed order, FP64.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void force_gather_kernel(int n, int maxnb, const int* count,
-                                    const int* list, const int* rev,
-                                    const int* site_of_atom, const T* gpair,
-                                    double* forces) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void force_gather_kernel(int n, int maxnb, const int* count,
+                                                           const int* list, const int* rev,
+                                                           const int* site_of_atom,
+                                                           const T* gpair, double* forces) {
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (k >= n) return;
   const int cnt = min(count[k], maxnb);
   const bool ksite = site_of_atom[k] >= 0;
   double f0 = 0, f1 = 0, f2 = 0;
-  for (int s = 0; s < cnt; s++) {
+  for (int s = lane; s < cnt; s += 64) {
     const size_t p = (size_t)k * maxnb + s;
     if (ksite) {
       f0 += (double)gpair[p * 3];
@@ -331,16 +364,24 @@ __global__ void force_gather_kernel(int n, int maxnb, const int* count,
       f2 -= (double)gpair[rp * 3 + 2];
     }
   }
-  forces[3 * k] += f0;
-  forces[3 * k + 1] += f1;
-  forces[3 * k + 2] += f2;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    f0 += __shfl_xor(f0, o, 64);
+    f1 += __shfl_xor(f1, o, 64);
+    f2 += __shfl_xor(f2, o, 64);
+  }
+  if (lane == 0) {
+    forces[3 * k] += f0;
+    forces[3 * k + 1] += f1;
+    forces[3 * k + 2] += f2;
+  }
 }
 
 template <typename T>
 hipError_t launch_force_gather(int n, int maxnb, const int* count, const int* list,
                                const int* rev, const int* site_of_atom, const T* gpair,
                                double* forces, hipStream_t st) {
-  hipLaunchKernelGGL(force_gather_kernel<T>, dim3((n + 255) / 256), dim3(256), 0, st, n, maxnb,
+  hipLaunchKernelGGL(force_gather_kernel<T>, dim3((n + 3) / 4), dim3(256), 0, st, n, maxnb,
                      count, list, rev, site_of_atom, gpair, forces);
   return hipGetLastError();
 }

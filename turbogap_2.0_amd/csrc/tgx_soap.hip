@@ -1,8 +1,10 @@
 // SOAP expansion (forward) and its adjoint (forces, virial) on the device.
 //
 // Layout: a workgroup runs up to four independent 64-lane waves (fewer when
-// a large model needs more LDS per site), one site (central atom) per wave,
-// each with its own LDS region, so waves never wait for each other.  Forward, per site:
+// a large model needs more LDS per site); a wave serves two sites with 32
+// lanes each (one site with 64 lanes for models with more than 16 radial
+// channels), each site with its own LDS region, so nothing waits across
+// sites.  Forward, per site:
 //   A  lanes = neighbour slots: minimum-image vector, radial values R_n(r)
 //      and A_l(r) Y_lm(u) for the valid pairs, compacted into LDS in chunks;
 //   B  lanes = lm: c[n][lm] += R_n(j) A_l Y_lm(j) accumulated in registers;
@@ -28,6 +30,8 @@ hipError_t upload_sph_constants(int /*lmax*/, hipStream_t st) {
   static bool ready = false;
   if (!ready) {
     sph_constants(TGX_MAX_LMAX, h.A, h.B, h.Pmm, h.Ad, h.Bd, h.Pmmd);
+    for (int l = 0; l <= TGX_MAX_LMAX; l++)
+      for (int k = l * l; k < (l + 1) * (l + 1); k++) h.lm2l[k] = l;
     ready = true;
   }
   return hipMemcpyToSymbolAsync(HIP_SYMBOL(c_sph), &h, sizeof(h), 0, hipMemcpyHostToDevice, st);
@@ -108,17 +112,29 @@ __device__ __forceinline__ double pair_vec_fast(const double* __restrict__ pos, 
   return sqrt(dx * dx + dy * dy + dz * dz);
 }
 
-// Pairs per LDS chunk and per-wave LDS footprint (elements of T).
-template <typename T> struct Chunk { static constexpr int CH = 32; };
-template <> struct Chunk<double> { static constexpr int CH = 16; };
+// Sites per 64-lane wave: two (32 lanes each) when the model has at most 16
+// radial channels, else one.  Pairs per LDS chunk and the per-site LDS
+// footprint (elements of T).
+template <int NRP> struct Spw { static constexpr int v = NRP <= 16 ? 2 : 1; };
+template <typename T, int SPW> struct Chunk { static constexpr int CH = SPW == 2 ? 24 : 32; };
+template <int SPW> struct Chunk<double, SPW> { static constexpr int CH = 16; };
 
 template <typename T, int NRP>
-__host__ __device__ constexpr size_t fwd_wave_elems(int L2) {
-  return (size_t)(Chunk<T>::CH > NRP ? Chunk<T>::CH : NRP) * L2 + (size_t)Chunk<T>::CH * NRP;
+__host__ __device__ constexpr size_t fwd_site_elems(int L2) {
+  constexpr int CH = Chunk<T, Spw<NRP>::v>::CH;
+  return (size_t)(CH > NRP ? CH : NRP) * L2 + (size_t)CH * NRP;
 }
 template <int NRP>
-__host__ __device__ constexpr size_t bwd_wave_elems(int L2, int dim) {
-  return (size_t)2 * NRP * L2 + dim;
+__host__ __device__ constexpr size_t bwd_site_elems(int L2, int Lp) {
+  return (size_t)2 * NRP * L2 + (size_t)Lp * NRP * NRP;
+}
+
+// Sum over the LW lanes of a sub-wave.
+template <int LW, typename T>
+__device__ __forceinline__ T sub_sum(T v) {
+#pragma unroll
+  for (int o = LW / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
 }
 
 // A_l = t^(l(l+1)), t = exp(-sig_t^2/(2 r^2)), for l = 0..LMX.
@@ -140,27 +156,32 @@ __global__ __launch_bounds__(256) void soap_fwd_kernel(
     const float2* __restrict__ rtab, const double* __restrict__ pos,
     const int* __restrict__ species, Cell cell, int maxnb, const int* __restrict__ count,
     const int* __restrict__ list, const int* __restrict__ site_atom, int n_sites,
-    const int2* __restrict__ pairn, T* __restrict__ q, int q_ld, u16* __restrict__ qb,
+    const int* __restrict__ dmap, T* __restrict__ q, int q_ld, u16* __restrict__ qb,
     int qb_ld, T* __restrict__ cbuf, T* __restrict__ pnorm, int wpb) {
-  constexpr int CH = Chunk<T>::CH;
+  constexpr int SPW = Spw<NRP>::v, LW = 64 / SPW;
+  constexpr int CH = Chunk<T, SPW>::CH;
+  constexpr int NS = ((LMX + 1) * (LMX + 1) + LW - 1) / LW;  // lm slots per lane
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int L = sd->l_max, L2 = sd->L2;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  T* sAY = reinterpret_cast<T*>(smem_raw) + (size_t)wave * fwd_wave_elems<T, NRP>(L2);
+  const int wave = threadIdx.x >> 6, lane64 = threadIdx.x & 63;
+  const int sub = lane64 / LW, lane = lane64 % LW;
+  T* sAY = reinterpret_cast<T*>(smem_raw) +
+           (size_t)(wave * SPW + sub) * fwd_site_elems<T, NRP>(L2);
   T* sR = sAY + (size_t)(CH > NRP ? CH : NRP) * L2;
   T* sC = sAY;  // reused once the accumulation is done
-  const int site = blockIdx.x * wpb + wave;
+  const int site = (blockIdx.x * wpb + wave) * SPW + sub;
   if (site >= n_sites) return;
   const int i = site_atom[site];
   const int si = species[i] - 1;
   const int cnt = min(count[i], maxnb);
-  const int lm0 = lane, lm1 = lane + 64;
 
-  T acc0[NRP], acc1[NRP];
+  T acc[NS][NRP];
 #pragma unroll
-  for (int n = 0; n < NRP; n++) acc0[n] = acc1[n] = (T)0;
+  for (int k = 0; k < NS; k++)
+#pragma unroll
+    for (int n = 0; n < NRP; n++) acc[k][n] = (T)0;
 
-  for (int base = 0; base < cnt; base += 64) {
+  for (int base = 0; base < cnt; base += LW) {
     const int slot = base + lane;
     double dv[3] = {0, 0, 0};
     double rd = 1e30;
@@ -171,7 +192,8 @@ __global__ __launch_bounds__(256) void soap_fwd_kernel(
       s = species[e & TGX_ATOM_MASK] - 1;
     }
     const bool valid = slot < cnt && rd < sd->rh[s];
-    const unsigned long long mask = __ballot(valid);
+    unsigned long long mask = __ballot(valid);
+    if (SPW == 2) mask = (mask >> (sub * 32)) & 0xffffffffull;
     const int jc = __popcll(mask & ((1ull << lane) - 1ull));
     const int npv = __popcll(mask);
     for (int c0 = 0; c0 < npv; c0 += CH) {
@@ -182,12 +204,12 @@ __global__ __launch_bounds__(256) void soap_fwd_kernel(
         const T ux = (T)dv[0] * ir, uy = (T)dv[1] * ir, uz = (T)dv[2] * ir;
         T R[NMX], dR[NMX];
         Radial<T>::template eval<NMX>(sd, W, rtab, s, r, R, dR);
-        const int o = sd->off[s], nm = sd->nmax[s];
+        const int o = sd->off[s];
+        // R[n] = 0 for n >= n_max(s) and o + NMX <= NRP: no guard needed
 #pragma unroll
         for (int n = 0; n < NRP; n++) sR[jj * NRP + n] = (T)0;
 #pragma unroll
-        for (int n = 0; n < NMX; n++)
-          if (n < nm) sR[jj * NRP + o + n] = R[n];
+        for (int n = 0; n < NMX; n++) sR[jj * NRP + o + n] = R[n];
         T log2t, D;
         ang_params<T>(sd, s, r, &log2t, &D);
         T Al[LMX + 1];
@@ -222,13 +244,17 @@ __global__ __launch_bounds__(256) void soap_fwd_kernel(
       wave_sync();
       const int nj = min(CH, npv - c0);
       for (int j = 0; j < nj; j++) {
-        const T y0 = sAY[j * L2 + lm0];
-        const T y1 = lm1 < L2 ? sAY[j * L2 + lm1] : (T)0;
+        T y[NS];
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+          const int lm = lane + k * LW;
+          y[k] = lm < L2 ? sAY[j * L2 + lm] : (T)0;
+        }
 #pragma unroll
         for (int n = 0; n < NRP; n++) {
           const T rv = sR[j * NRP + n];
-          acc0[n] += rv * y0;
-          acc1[n] += rv * y1;
+#pragma unroll
+          for (int k = 0; k < NS; k++) acc[k][n] += rv * y[k];
         }
       }
       wave_sync();
@@ -240,45 +266,45 @@ __global__ __launch_bounds__(256) void soap_fwd_kernel(
     for (int n = 0; n < sd->nmax[si]; n++) {
 #pragma unroll
       for (int k = 0; k < NRP; k++)
-        if (k == o + n) acc0[k] += (T)sd->rself[si * TGX_MAX_NR + n];
+        if (k == o + n) acc[0][k] += (T)sd->rself[si * TGX_MAX_NR + n];
     }
   }
   T* cs = cbuf + (size_t)site * NRP * L2;
 #pragma unroll
-  for (int n = 0; n < NRP; n++) {
-    if (lm0 < L2) {
-      sC[n * L2 + lm0] = acc0[n];
-      cs[n * L2 + lm0] = acc0[n];
-    }
-    if (lm1 < L2) {
-      sC[n * L2 + lm1] = acc1[n];
-      cs[n * L2 + lm1] = acc1[n];
+  for (int k = 0; k < NS; k++) {
+    const int lm = lane + k * LW;
+    if (lm < L2) {
+#pragma unroll
+      for (int n = 0; n < NRP; n++) {
+        sC[n * L2 + lm] = acc[k][n];
+        cs[n * L2 + lm] = acc[k][n];
+      }
     }
   }
   wave_sync();
-  const int dim = sd->dim, Lp = L + 1;
+  const int dim = sd->dim;
   T* qs = q + (size_t)site * q_ld;
   T ss = 0;
-  for (int d = lane; d < dim; d += 64) {
-    const int kp = d / Lp, l = d - kp * Lp;
-    const int2 nn = pairn[kp];
-    const T* ca = sC + nn.x * L2 + l * l;
-    const T* cb = sC + nn.y * L2 + l * l;
+  for (int d = lane; d < dim; d += LW) {
+    const int dm = dmap[d];
+    const int n1 = dm & 255, n2 = (dm >> 8) & 255, l = dm >> 16;
+    const T* ca = sC + n1 * L2 + l * l;
+    const T* cb = sC + n2 * L2 + l * l;
     T a = 0;
     for (int m = 0; m <= 2 * l; m++) a += ca[m] * cb[m];
-    const T p = (nn.x == nn.y ? (T)1 : (T)1.41421356237309504880) * a;
+    const T p = (n1 == n2 ? (T)1 : (T)1.41421356237309504880) * a;
     qs[d] = p;
     ss += p * p;
   }
-  ss = wave_sum(ss);
+  ss = sub_sum<LW>(ss);
   const T inv = (T)1 / sqrt(ss);
-  for (int d = lane; d < q_ld; d += 64) {
+  for (int d = lane; d < q_ld; d += LW) {
     const T v = d < dim ? qs[d] * inv : (T)0;
     qs[d] = v;
     if (qb) qb[(size_t)site * qb_ld + d] = f2bf((float)v);
   }
   if (qb)
-    for (int d = q_ld + lane; d < qb_ld; d += 64) qb[(size_t)site * qb_ld + d] = 0;
+    for (int d = q_ld + lane; d < qb_ld; d += LW) qb[(size_t)site * qb_ld + d] = 0;
   if (lane == 0) pnorm[site] = sqrt(ss);
 }
 
@@ -288,51 +314,65 @@ __global__ __launch_bounds__(256) void soap_bwd_kernel(
     const float2* __restrict__ rtab, const double* __restrict__ pos,
     const int* __restrict__ species, Cell cell, int maxnb, const int* __restrict__ count,
     const int* __restrict__ list, const int* __restrict__ site_atom, int n_sites,
-    const int2* __restrict__ pairn, const T* __restrict__ q, int q_ld,
+    const int* __restrict__ dmap, const T* __restrict__ q, int q_ld,
     const T* __restrict__ cbuf, const T* __restrict__ pnorm, const T* __restrict__ G, int g_ld,
     T* __restrict__ gpair, double* __restrict__ vsite, int wpb) {
+  constexpr int SPW = Spw<NRP>::v, LW = 64 / SPW;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int L = sd->l_max, L2 = sd->L2, NR = sd->nr, dim = sd->dim, Lp = L + 1;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  T* sC = reinterpret_cast<T*>(smem_raw) + (size_t)wave * bwd_wave_elems<NRP>(L2, dim);
+  const int L = sd->l_max, L2 = sd->L2, dim = sd->dim, Lp = L + 1;
+  const int wave = threadIdx.x >> 6, lane64 = threadIdx.x & 63;
+  const int sub = lane64 / LW, lane = lane64 % LW;
+  T* sC = reinterpret_cast<T*>(smem_raw) + (size_t)(wave * SPW + sub) * bwd_site_elems<NRP>(L2, Lp);
   T* sGC = sC + NRP * L2;  // [L2][NRP]
-  T* sGP = sGC + NRP * L2; // [dim]
-  const int site = blockIdx.x * wpb + wave;
+  T* sM = sGC + NRP * L2;  // [Lp][NRP][NRP]: dE/dc = M_l c
+  const int site = (blockIdx.x * wpb + wave) * SPW + sub;
   if (site >= n_sites) return;
   const int i = site_atom[site];
   const int cnt = min(count[i], maxnb);
   const T* cs = cbuf + (size_t)site * NRP * L2;
-  for (int k = lane; k < NRP * L2; k += 64) sC[k] = cs[k];
+  for (int k = lane; k < NRP * L2; k += LW) sC[k] = cs[k];
+  for (int k = lane; k < Lp * NRP * NRP; k += LW) sM[k] = (T)0;
   const T* qs = q + (size_t)site * q_ld;
   const T* gs = G + (size_t)site * g_ld;
   T gq = 0;
-  for (int d = lane; d < dim; d += 64) gq += gs[d] * qs[d];
-  gq = wave_sum(gq);
+  for (int d = lane; d < dim; d += LW) gq += gs[d] * qs[d];
+  gq = sub_sum<LW>(gq);
   const T inv = (T)1 / pnorm[site];
-  for (int d = lane; d < dim; d += 64) sGP[d] = (gs[d] - gq * qs[d]) * inv;
   wave_sync();
-  // dE/dc[n][lm] = sum_n2 M_l[n][n2] c[n2][lm], M_l from dE/dp
-  for (int lm = lane; lm < L2; lm += 64) {
-    int l = 0;
-    while ((l + 1) * (l + 1) <= lm) l++;
+  // dE/dp -> M_l[n][n2] = w dE/dp[(n,n2),l] (symmetric; w = 2 on the diagonal,
+  // sqrt(2) off it, the derivative of the multiplicity-weighted product)
+  for (int d = lane; d < dim; d += LW) {
+    const T gp = (gs[d] - gq * qs[d]) * inv;
+    const int dm = dmap[d];
+    const int n1 = dm & 255, n2 = (dm >> 8) & 255, l = dm >> 16;
+    T* Ml = sM + l * NRP * NRP;
+    if (n1 == n2) {
+      Ml[n1 * NRP + n1] = (T)2 * gp;
+    } else {
+      const T w = (T)1.41421356237309504880 * gp;
+      Ml[n1 * NRP + n2] = w;
+      Ml[n2 * NRP + n1] = w;
+    }
+  }
+  wave_sync();
+  for (int lm = lane; lm < L2; lm += LW) {
+    const int l = c_sph.lm2l[lm];
+    const T* Ml = sM + l * NRP * NRP;
+    T cl[NRP];
+#pragma unroll
+    for (int n2 = 0; n2 < NRP; n2++) cl[n2] = sC[n2 * L2 + lm];
 #pragma unroll
     for (int n = 0; n < NRP; n++) {
       T a = 0;
-      if (n < NR) {
-        for (int n2 = 0; n2 < NR; n2++) {
-          const int lo = n < n2 ? n : n2, hi = n < n2 ? n2 : n;
-          const int kp = lo * NR - lo * (lo - 1) / 2 + (hi - lo);
-          const T w = (n == n2) ? (T)2 : (T)1.41421356237309504880;
-          a += w * sGP[kp * Lp + l] * sC[n2 * L2 + lm];
-        }
-      }
+#pragma unroll
+      for (int n2 = 0; n2 < NRP; n2++) a += Ml[n * NRP + n2] * cl[n2];
       sGC[lm * NRP + n] = a;
     }
   }
   wave_sync();
   double v[6] = {0, 0, 0, 0, 0, 0};
   T* gp_out = gpair + (size_t)i * maxnb * 3;
-  for (int base = 0; base < cnt; base += 64) {
+  for (int base = 0; base < cnt; base += LW) {
     const int slot = base + lane;
     if (slot >= cnt) continue;
     const int e = list[(size_t)i * maxnb + slot];
@@ -346,7 +386,7 @@ __global__ __launch_bounds__(256) void soap_bwd_kernel(
       const T ux = (T)dv[0] * ir, uy = (T)dv[1] * ir, uz = (T)dv[2] * ir;
       T R[NMX], dR[NMX];
       Radial<T>::template eval<NMX>(sd, W, rtab, s, r, R, dR);
-      const int o = sd->off[s], nm = sd->nmax[s];
+      const int o = sd->off[s];
       T log2t, D;
       ang_params<T>(sd, s, r, &log2t, &D);
       T Al[LMX + 1];
@@ -378,10 +418,8 @@ __global__ __launch_bounds__(256) void soap_bwd_kernel(
                 T h = 0, hd = 0;
 #pragma unroll
                 for (int n = 0; n < NMX; n++) {
-                  if (n < nm) {
-                    h += gc[n] * R[n];
-                    hd += gc[n] * dR[n];
-                  }
+                  h += gc[n] * R[n];
+                  hd += gc[n] * dR[n];
                 }
                 s1 += (hd * A + h * dA) * (P * Cm);
                 const T hA = h * A;
@@ -396,10 +434,8 @@ __global__ __launch_bounds__(256) void soap_bwd_kernel(
                 T h = 0, hd = 0;
 #pragma unroll
                 for (int n = 0; n < NMX; n++) {
-                  if (n < nm) {
-                    h += gc[n] * R[n];
-                    hd += gc[n] * dR[n];
-                  }
+                  h += gc[n] * R[n];
+                  hd += gc[n] * dR[n];
                 }
                 s1 += (hd * A + h * dA) * (P * Sm);
                 const T hA = h * A;
@@ -436,7 +472,7 @@ __global__ __launch_bounds__(256) void soap_bwd_kernel(
   }
 #pragma unroll
   for (int k = 0; k < 6; k++) {
-    const double t = wave_sum(v[k]);
+    const double t = sub_sum<LW>(v[k]);
     if (lane == 0) vsite[(size_t)site * 6 + k] = t;
   }
 }
@@ -466,17 +502,19 @@ template <typename T, int NRP, int NMX, int LMX>
 static hipError_t fwd_inst(const SoapDev& sdh, const SoapDev* sd, const double* W,
                            const float* rtab, const double* pos, const int* species,
                            const Cell& cell, int maxnb, const int* count, const int* list,
-                           const int* site_atom, int n_sites, const int2* pairn, T* q, int q_ld,
+                           const int* site_atom, int n_sites, const int* dmap, T* q, int q_ld,
                            u16* qb, int qb_ld, T* cbuf, T* pnorm, hipStream_t st) {
-  const size_t per_wave = sizeof(T) * fwd_wave_elems<T, NRP>(sdh.L2);
+  constexpr int SPW = Spw<NRP>::v;
+  const size_t per_wave = SPW * sizeof(T) * fwd_site_elems<T, NRP>(sdh.L2);
   const int wpb = waves_per_block(per_wave);
   const size_t sm = wpb * per_wave;
   static size_t sm_set = 65536;
   hipError_t e = ensure_lds(soap_fwd_kernel<T, NRP, NMX, LMX>, sm, &sm_set);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((soap_fwd_kernel<T, NRP, NMX, LMX>), dim3((n_sites + wpb - 1) / wpb),
+  const int per_block = wpb * SPW;
+  hipLaunchKernelGGL((soap_fwd_kernel<T, NRP, NMX, LMX>), dim3((n_sites + per_block - 1) / per_block),
                      dim3(64 * wpb), sm, st, sd, W, reinterpret_cast<const float2*>(rtab), pos,
-                     species, cell, maxnb, count, list, site_atom, n_sites, pairn, q, q_ld, qb,
+                     species, cell, maxnb, count, list, site_atom, n_sites, dmap, q, q_ld, qb,
                      qb_ld, cbuf, pnorm, wpb);
   return hipGetLastError();
 }
@@ -485,30 +523,29 @@ template <typename T, int NRP, int NMX, int LMX>
 static hipError_t bwd_inst(const SoapDev& sdh, const SoapDev* sd, const double* W,
                            const float* rtab, const double* pos, const int* species,
                            const Cell& cell, int maxnb, const int* count, const int* list,
-                           const int* site_atom, int n_sites, const int2* pairn, const T* q,
+                           const int* site_atom, int n_sites, const int* dmap, const T* q,
                            int q_ld, const T* cbuf, const T* pnorm, const T* G, int g_ld,
                            T* gpair, double* vsite, hipStream_t st) {
-  const size_t per_wave = sizeof(T) * bwd_wave_elems<NRP>(sdh.L2, sdh.dim);
+  constexpr int SPW = Spw<NRP>::v;
+  const size_t per_wave = SPW * sizeof(T) * bwd_site_elems<NRP>(sdh.L2, sdh.l_max + 1);
   const int wpb = waves_per_block(per_wave);
   const size_t sm = wpb * per_wave;
   static size_t sm_set = 65536;
   hipError_t e = ensure_lds(soap_bwd_kernel<T, NRP, NMX, LMX>, sm, &sm_set);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((soap_bwd_kernel<T, NRP, NMX, LMX>), dim3((n_sites + wpb - 1) / wpb),
+  const int per_block = wpb * SPW;
+  hipLaunchKernelGGL((soap_bwd_kernel<T, NRP, NMX, LMX>), dim3((n_sites + per_block - 1) / per_block),
                      dim3(64 * wpb), sm, st, sd, W, reinterpret_cast<const float2*>(rtab), pos,
-                     species, cell, maxnb, count, list, site_atom, n_sites, pairn, q, q_ld, cbuf,
+                     species, cell, maxnb, count, list, site_atom, n_sites, dmap, q, q_ld, cbuf,
                      pnorm, G, g_ld, gpair, vsite, wpb);
   return hipGetLastError();
 }
 
-// Compile-time shape of a model: NRP = sum n_max rounded up to 8, NMX =
-// largest per-species n_max (8 or 16), LMX = l_max bound (8 or 10).
+// Compile-time shape of a model: NRP = padded radial channels (SoapDev::nrp),
+// NMX = per-species n_max bound (8 or 16), LMX = l_max bound (8 or 10).
 #define TGX_SOAP_DISPATCH(FN, ...)                                                      \
   do {                                                                                 \
-    int nrp = (sdh.nr + 7) / 8 * 8;                                                    \
-    int nmx = 0;                                                                       \
-    for (int s_ = 0; s_ < sdh.n_species; s_++) nmx = sdh.nmax[s_] > nmx ? sdh.nmax[s_] : nmx; \
-    nmx = nmx <= 8 ? 8 : 16;                                                           \
+    const int nrp = sdh.nrp, nmx = sdh.nmx;                                            \
     const bool l8 = sdh.l_max <= 8;                                                    \
     if (nrp == 8) return l8 ? FN<T, 8, 8, 8>(__VA_ARGS__) : FN<T, 8, 8, 10>(__VA_ARGS__); \
     if (nrp == 16 && nmx == 8) return l8 ? FN<T, 16, 8, 8>(__VA_ARGS__) : FN<T, 16, 8, 10>(__VA_ARGS__); \
@@ -524,28 +561,28 @@ template <typename T>
 hipError_t launch_soap_fwd(const SoapDev& sdh, const SoapDev* sd, const double* W,
                            const float* rtab, const double* pos, const int* species,
                            const Cell& cell, int maxnb, const int* count, const int* list,
-                           const int* site_atom, int n_sites, const int2* pairn, T* q, int q_ld,
+                           const int* site_atom, int n_sites, const int* dmap, T* q, int q_ld,
                            u16* qb, int qb_ld, T* cbuf, T* pnorm, hipStream_t st) {
   if (n_sites == 0) return hipSuccess;
   TGX_SOAP_DISPATCH(fwd_inst, sdh, sd, W, rtab, pos, species, cell, maxnb, count, list,
-                    site_atom, n_sites, pairn, q, q_ld, qb, qb_ld, cbuf, pnorm, st);
+                    site_atom, n_sites, dmap, q, q_ld, qb, qb_ld, cbuf, pnorm, st);
 }
 
 template <typename T>
 hipError_t launch_soap_bwd(const SoapDev& sdh, const SoapDev* sd, const double* W,
                            const float* rtab, const double* pos, const int* species,
                            const Cell& cell, int maxnb, const int* count, const int* list,
-                           const int* site_atom, int n_sites, const int2* pairn, const T* q,
+                           const int* site_atom, int n_sites, const int* dmap, const T* q,
                            int q_ld, const T* cbuf, const T* pnorm, const T* G, int g_ld,
                            T* gpair, double* vsite, hipStream_t st) {
   if (n_sites == 0) return hipSuccess;
   TGX_SOAP_DISPATCH(bwd_inst, sdh, sd, W, rtab, pos, species, cell, maxnb, count, list,
-                    site_atom, n_sites, pairn, q, q_ld, cbuf, pnorm, G, g_ld, gpair, vsite, st);
+                    site_atom, n_sites, dmap, q, q_ld, cbuf, pnorm, G, g_ld, gpair, vsite, st);
 }
 
-template hipError_t launch_soap_fwd<float>(const SoapDev&, const SoapDev*, const double*, const float*, const double*, const int*, const Cell&, int, const int*, const int*, const int*, int, const int2*, float*, int, u16*, int, float*, float*, hipStream_t);
-template hipError_t launch_soap_fwd<double>(const SoapDev&, const SoapDev*, const double*, const float*, const double*, const int*, const Cell&, int, const int*, const int*, const int*, int, const int2*, double*, int, u16*, int, double*, double*, hipStream_t);
-template hipError_t launch_soap_bwd<float>(const SoapDev&, const SoapDev*, const double*, const float*, const double*, const int*, const Cell&, int, const int*, const int*, const int*, int, const int2*, const float*, int, const float*, const float*, const float*, int, float*, double*, hipStream_t);
-template hipError_t launch_soap_bwd<double>(const SoapDev&, const SoapDev*, const double*, const float*, const double*, const int*, const Cell&, int, const int*, const int*, const int*, int, const int2*, const double*, int, const double*, const double*, const double*, int, double*, double*, hipStream_t);
+template hipError_t launch_soap_fwd<float>(const SoapDev&, const SoapDev*, const double*, const float*, const double*, const int*, const Cell&, int, const int*, const int*, const int*, int, const int*, float*, int, u16*, int, float*, float*, hipStream_t);
+template hipError_t launch_soap_fwd<double>(const SoapDev&, const SoapDev*, const double*, const float*, const double*, const int*, const Cell&, int, const int*, const int*, const int*, int, const int*, double*, int, u16*, int, double*, double*, hipStream_t);
+template hipError_t launch_soap_bwd<float>(const SoapDev&, const SoapDev*, const double*, const float*, const double*, const int*, const Cell&, int, const int*, const int*, const int*, int, const int*, const float*, int, const float*, const float*, const float*, int, float*, double*, hipStream_t);
+template hipError_t launch_soap_bwd<double>(const SoapDev&, const SoapDev*, const double*, const float*, const double*, const int*, const Cell&, int, const int*, const int*, const int*, int, const int*, const double*, int, const double*, const double*, const double*, int, double*, double*, hipStream_t);
 
 }  // namespace tgx
