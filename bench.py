@@ -30,7 +30,40 @@ import tgx  # noqa: E402
 
 METRIC = "atom-steps/s (E+F) at 32k C atoms, mixed vs FP64; force MAE vs FP64 oracle"
 PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def recorded_traffic(kernel, precision, atoms):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    measurement (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel
+    calibration), or None when no measurement matches this run."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            rec = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    for r in rec.get("kernels", []):
+        if r.get("kernel") == kernel and r.get("precision") == precision \
+                and r.get("atoms") == atoms:
+            return r.get("bytes_per_launch")
+    return None
+
+
+def copy_probe(mbytes):
+    """A plain device copy of known size, for calibrating the PMC byte
+    counters (FETCH_SIZE/WRITE_SIZE) against algorithmic bytes."""
+    import torch
+
+    n = mbytes * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n,
+                      "launches": 3}))
 
 
 def build_model(eng):
@@ -44,18 +77,21 @@ def build_model(eng):
 
 
 def soap_costs(h, n_sites, n_pairs):
-    """Algorithmic FP32 operations of the SOAP expansion and its adjoint
-    (multiply-add = 2).  Per pair within the cutoff: radial lookup
+    """Algorithmic floating-point operations of the SOAP expansion and its
+    adjoint (multiply-add = 2), L2 = (l_max+1)^2 real harmonics, n_max of
+    the neighbour's species.  Per pair within the cutoff: radial lookup
     (n_max x 16), harmonics and angular factors (L2 x 6), c accumulation
     (n_max x L2 x 2); adjoint: h and h' (2 x n_max x L2 x 2), harmonic
     gradients and contraction (L2 x 16).  Per site: power spectrum
-    (dim x (2l+1) x 2 summed over l), adjoint G_c (N_R^2 x L2 x 2)."""
+    (sum over l of N_R(N_R+1)/2 x (2l+1) x 2) plus normalisation (3 x dim);
+    adjoint: dE/dc = M_l c (N_R^2 x L2 x 2) plus projection (4 x dim)."""
     L2 = (h.l_max + 1) ** 2
     nr = h.n_r
+    nm = max(h.alpha_max)
     npair_nn = nr * (nr + 1) // 2
     ps = npair_nn * sum(2 * (2 * l + 1) for l in range(h.l_max + 1))
-    fwd = n_pairs * (8 * 16 + L2 * 6 + 8 * L2 * 2) + n_sites * (ps + 3 * h.dim)
-    bwd = n_pairs * (8 * 16 + 2 * 8 * L2 * 2 + L2 * 16) + n_sites * (nr * nr * L2 * 2 + 4 * h.dim)
+    fwd = n_pairs * (nm * 16 + L2 * 6 + nm * L2 * 2) + n_sites * (ps + 3 * h.dim)
+    bwd = n_pairs * (nm * 16 + 2 * nm * L2 * 2 + L2 * 16) + n_sites * (nr * nr * L2 * 2 + 4 * h.dim)
     return fwd, bwd
 
 
@@ -68,7 +104,12 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp32", "fp64"])
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
+                    help="only run a plain device copy of MB MiB (PMC calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
@@ -112,22 +153,37 @@ def main():
         print(f"[bench] fp64: {1e3 * t64 / k64:.3f} ms/step, {fp64:.4g} atom-steps/s",
               file=sys.stderr, flush=True)
 
-    # per-stage timing of one instrumented E+F (HIP events on the library stream)
+    # Dominant kernel: single-kernel device times (HIP events on the library
+    # stream, bracketing only that kernel) averaged over instrumented E+F calls.
     eng.set_precision(args.precision)
-    e_m, f_m, v_m = eng.energy_forces(pos, sp, lat)
+    kt = {"soap_fwd": 0.0, "gemm": 0.0, "soap_bwd": 0.0, "force_gather": 0.0}
+    n_inst = 5
+    t_host = 0.0
+    for _ in range(n_inst):
+        t0 = time.perf_counter()
+        e_m, f_m, v_m = eng.energy_forces(pos, sp, lat)
+        t_host += time.perf_counter() - t0
+        for k, v in eng.kernel_times().items():
+            kt[k] += v / n_inst
+    # host buffers in and out (PCIe, list rebuilt from scratch every call)
+    host_rate = n * n_inst / t_host
     st = eng.stage_times()
     nb = eng.neighbors(pos, lat, model.hypers.rcut_max)
     n_pairs = int(np.sum(nb["rjs"] > 0))
     fwd_flop, bwd_flop = soap_costs(model.hypers, n, n_pairs)
-    stage_ms = {"soap_fwd": st["soap_fwd"], "gemm": st["gemm"], "soap_bwd": st["soap_bwd"]}
-    dom = max(stage_ms, key=stage_ms.get)
+    dom = max(("soap_fwd", "gemm", "soap_bwd"), key=kt.get)
     D, Sn = model.hypers.dim, len(model.alphas)
     gemm_flop = 4.0 * n * Sn * D
     flops = {"soap_fwd": fwd_flop, "gemm": gemm_flop, "soap_bwd": bwd_flop}[dom]
-    ach = flops / (stage_ms[dom] * 1e-3) / 1e12
+    ach = flops / (kt[dom] * 1e-3) / 1e12
     peak = PEAK_FP32_TFLOPS if dom != "gemm" or args.precision != "mixed" else 2500.0
+    if args.precision == "fp64":
+        peak = PEAK_FP64_TFLOPS
     roofline = {"bound": "mfma", "kernel": dom, "achieved": round(ach, 3), "peak": peak,
-                "unit": "TFLOP/s", "frac": round(ach / peak, 5), "traffic": None,
+                "unit": "TFLOP/s", "frac": round(ach / peak, 5),
+                "traffic": recorded_traffic(dom, args.precision, n),
+                "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
+                "flop_per_launch": flops,
                 "stage_ms": {k: round(v, 4) for k, v in st.items()}}
 
     cpu = None
@@ -170,6 +226,7 @@ def main():
         "mixed_over_fp64": None if fp64 is None else round(value / fp64, 3),
         "accuracy_vs_fp64_oracle": mae,
         "energy_drift_eV_per_atom": drift,
+        "host_buffer_atom_evals_per_s": round(host_rate, 1),
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -232,6 +232,14 @@ int tgx_sync(tgx_ctx* ctx);
  * force gather, [4] 2-body, [5] integrator, [6] total. */
 int tgx_stage_times(const tgx_ctx* ctx, double* ms7);
 
+/* Device time of single kernels in the last tgx_energy_forces call,
+ * milliseconds, each bracketed by HIP events on the context's stream:
+ * [0] SOAP forward, [1] kernel GEMMs (both), [2] SOAP adjoint, [3] force
+ * gather.  0 for a kernel that did not run.  (Instrumentation; the
+ * reference's equivalent is the do_timing wall-clock printout,
+ * gap/gap.f90:230-242.) */
+int tgx_kernel_times(const tgx_ctx* ctx, double* ms4);
+
 #ifdef __cplusplus
 }
 #endif

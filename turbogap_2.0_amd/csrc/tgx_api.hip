@@ -121,6 +121,7 @@ struct Extra {
   DevBuf twob_store;
   std::vector<int> species_host;
   hipEvent_t ev[8];
+  hipEvent_t kev[8];  // brackets of single kernels: fwd, GEMMs, adjoint, gather
   bool ev_ready = false;
   bool ev_on = true;
   MdState md;
@@ -215,6 +216,8 @@ static int setup_sites(tgx_ctx* c, int n, const int* species_host, int* n_sites)
   return TGX_OK;
 }
 
+static void kev_rec(tgx_ctx* c, int k);
+
 // SOAP forward for the current list into c->q (and c->qb in mixed mode).
 static int soap_forward(tgx_ctx* c, int n, const Cell& cell, int n_sites) {
   Extra& x = X(c);
@@ -230,6 +233,7 @@ static int soap_forward(tgx_ctx* c, int n, const Cell& cell, int n_sites) {
     qb = c->qb.as<unsigned short>();
   }
   hipError_t e;
+  kev_rec(c, 0);
   if (prec == TGX_FP64)
     e = launch_soap_fwd<double>(c->sd, x.sd_dev.as<SoapDev>(), c->W_dev.as<double>(),
                                 c->rtab.as<float>(), c->pos.as<double>(), c->species.as<int>(),
@@ -245,6 +249,7 @@ static int soap_forward(tgx_ctx* c, int n, const Cell& cell, int n_sites) {
                                c->q.as<float>(), c->dim_pad, qb, c->dim_pad, c->cbuf.as<float>(),
                                c->pnorm.as<float>(), c->stream);
   TGX_HIP(c, e);
+  kev_rec(c, 1);
   (void)n;
   return TGX_OK;
 }
@@ -267,16 +272,22 @@ static int kernel_step(tgx_ctx* c, int n_sites, bool want_grad) {
   } else {
     qa = c->qb.p; qs = c->Qs16.p; qsT = c->QsT16.p; al = c->alph32.p;
   }
+  kev_rec(c, 2);
   TGX_HIP(c, launch_gap_kernel(prec, n_sites, c->dim, c->dim_pad, c->n_sparse, c->sp_pad, qa, qs,
                                qsT, al, c->delta, c->zeta, c->wmat.p, c->epart.p,
                                c->esite.as<double>(), want_grad ? c->gmat.p : nullptr,
                                want_grad ? 1 : 0, c->stream));
+  kev_rec(c, 3);
   return TGX_OK;
 }
 
 static void ev_rec(tgx_ctx* c, int k) {
   Extra& x = X(c);
   if (x.ev_ready && x.ev_on) (void)hipEventRecord(x.ev[k], c->stream);
+}
+static void kev_rec(tgx_ctx* c, int k) {
+  Extra& x = X(c);
+  if (x.ev_ready && x.ev_on) (void)hipEventRecord(x.kev[k], c->stream);
 }
 
 static int ef_launch(tgx_ctx* c, int n, int n_sites, const Cell& cell, int do_forces,
@@ -342,6 +353,7 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Cell& cell, int do_fo
     const size_t ts = tsize(prec);
     ENSURE(c, gpair, ts * 3 * (size_t)n * c->maxnb);
     hipError_t e;
+    kev_rec(c, 4);
     if (prec == TGX_FP64) {
       e = launch_soap_bwd<double>(c->sd, x.sd_dev.as<SoapDev>(), c->W_dev.as<double>(),
                                   c->rtab.as<float>(), c->pos.as<double>(), c->species.as<int>(),
@@ -350,6 +362,8 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Cell& cell, int do_fo
                                   c->q.as<double>(), c->dim_pad, c->cbuf.as<double>(),
                                   c->pnorm.as<double>(), c->gmat.as<double>(), c->dim_pad,
                                   c->gpair.as<double>(), c->vsite.as<double>(), c->stream);
+      kev_rec(c, 5);
+      kev_rec(c, 6);
       if (e == hipSuccess)
         e = launch_force_gather<double>(n, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
                                         c->nb_rev.as<int>(), c->site_of_atom.as<int>(),
@@ -362,12 +376,15 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Cell& cell, int do_fo
                                  c->q.as<float>(), c->dim_pad, c->cbuf.as<float>(),
                                  c->pnorm.as<float>(), c->gmat.as<float>(), c->dim_pad,
                                  c->gpair.as<float>(), c->vsite.as<double>(), c->stream);
+      kev_rec(c, 5);
+      kev_rec(c, 6);
       if (e == hipSuccess)
         e = launch_force_gather<float>(n, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
                                        c->nb_rev.as<int>(), c->site_of_atom.as<int>(),
                                        c->gpair.as<float>(), c->forces.as<double>(), c->stream);
     }
     TGX_HIP(c, e);
+    kev_rec(c, 7);
   }
   ev_rec(c, 4);
   for (const auto& b : c->twob)
@@ -402,6 +419,11 @@ static void collect_times(tgx_ctx* c) {
   float tot = 0.f;
   (void)hipEventElapsedTime(&tot, x.ev[0], x.ev[6]);
   c->stage_ms[6] = tot;
+  for (int k = 0; k < 4; k++) {
+    float t = 0.f;
+    c->kernel_ms[k] = hipEventElapsedTime(&t, x.kev[2 * k], x.kev[2 * k + 1]) == hipSuccess ? t : 0.0;
+  }
+  (void)hipGetLastError();
 }
 
 static int upload_structure(tgx_ctx* c, int n, const double* pos, const int* species) {
@@ -440,11 +462,8 @@ int tgx_create(int device, int precision, tgx_ctx** out) {
     return TGX_ERR_HIP;
   }
   for (int k = 0; k < 8; k++) (void)hipEventCreate(&f->x.ev[k]);
+  for (int k = 0; k < 8; k++) (void)hipEventCreate(&f->x.kev[k]);
   f->x.ev_ready = true;
-  if (upload_sph_constants(TGX_MAX_LMAX, c->stream) != hipSuccess) {
-    delete f;
-    return TGX_ERR_HIP;
-  }
   (void)hipStreamSynchronize(c->stream);
   *out = c;
   return TGX_OK;
@@ -473,6 +492,7 @@ int tgx_destroy(tgx_ctx* c) {
   }
   if (f->x.ev_ready)
     for (int k = 0; k < 8; k++) (void)hipEventDestroy(f->x.ev[k]);
+    for (int k = 0; k < 8; k++) (void)hipEventDestroy(f->x.kev[k]);
   (void)hipStreamDestroy(c->stream);
   delete f;
   return TGX_OK;
@@ -1138,6 +1158,12 @@ int tgx_poly3_orthonormalization(int alpha_max, double* S, double* W) {
 int tgx_stage_times(const tgx_ctx* c, double* ms7) {
   if (!c || !ms7) return TGX_ERR_ARG;
   for (int k = 0; k < 7; k++) ms7[k] = c->stage_ms[k];
+  return TGX_OK;
+}
+
+int tgx_kernel_times(const tgx_ctx* c, double* ms4) {
+  if (!c || !ms4) return TGX_ERR_ARG;
+  for (int k = 0; k < 4; k++) ms4[k] = c->kernel_ms[k];
   return TGX_OK;
 }
 

@@ -11,24 +11,67 @@ namespace tgx {
 
 // Normalised associated-Legendre recurrence constants (real harmonics without
 // Condon-Shortley phase): P(l,m) = A[l][m] z P(l-1,m) - B[l][m] P(l-2,m),
-// P(m,m) = Pmm[m]; the sqrt(2) of m > 0 is folded into Pmm.
-struct SphConst {
+// P(m,m) = Pmm[m]; the sqrt(2) of m > 0 is folded into Pmm.  With
+// K(l,m) = sqrt((2l+1)/(4 pi) (l-m)!/(l+m)!):
+//   A = sqrt((2l+1)(2l-1)/((l+m)(l-m))),
+//   B = sqrt((2l+1)(l+m-1)(l-m-1)/((2l-3)(l+m)(l-m))),
+//   Pmm = K(m,m) (2m-1)!! (times sqrt 2 for m > 0).
+// The table is a compile-time constant: in the fully unrolled recurrences the
+// entries fold into instruction literals instead of occupying scalar registers.
+constexpr double cx_sqrt(double a) {
+  if (a <= 0.0) return 0.0;
+  double x = a > 1.0 ? a : 1.0, prev = 0.0;
+  for (int it = 0; it < 200 && x != prev; it++) {
+    prev = x;
+    x = 0.5 * (x + a / x);
+  }
+  return x;
+}
+struct SphTab {
   float A[TGX_L2MAX], B[TGX_L2MAX], Pmm[TGX_MAX_LMAX + 1];
   double Ad[TGX_L2MAX], Bd[TGX_L2MAX], Pmmd[TGX_MAX_LMAX + 1];
   int lm2l[TGX_L2MAX];  // l of the combined index lm = l*l + l + m
 };
-extern __constant__ SphConst c_sph;
+constexpr SphTab make_sph_tab() {
+  SphTab t{};
+  const double pi = 3.14159265358979323846;
+  double df = 1.0;  // (2m-1)!!
+  for (int m = 0; m <= TGX_MAX_LMAX; m++) {
+    if (m > 0) df *= (2.0 * m - 1.0);
+    double f = 1.0;  // (l-m)!/(l+m)! at l = m
+    for (int k = 1; k <= 2 * m; k++) f /= (double)k;
+    double p = cx_sqrt((2.0 * m + 1.0) / (4.0 * pi) * f) * df;
+    if (m > 0) p *= cx_sqrt(2.0);
+    t.Pmmd[m] = p;
+    t.Pmm[m] = (float)p;
+    for (int l = m + 1; l <= TGX_MAX_LMAX; l++) {
+      const int ix = l * (TGX_MAX_LMAX + 1) + m;
+      const double a = cx_sqrt((2.0 * l + 1.0) * (2.0 * l - 1.0) / ((double)(l + m) * (l - m)));
+      const double b = l - 2 >= m ? cx_sqrt((2.0 * l + 1.0) * (l + m - 1.0) * (l - m - 1.0) /
+                                            ((2.0 * l - 3.0) * (l + m) * (l - m)))
+                                  : 0.0;
+      t.Ad[ix] = a;
+      t.Bd[ix] = b;
+      t.A[ix] = (float)a;
+      t.B[ix] = (float)b;
+    }
+  }
+  for (int l = 0; l <= TGX_MAX_LMAX; l++)
+    for (int k = l * l; k < (l + 1) * (l + 1); k++) t.lm2l[k] = l;
+  return t;
+}
+constexpr SphTab kSph = make_sph_tab();
 
 template <typename T> struct SphK;
 template <> struct SphK<float> {
-  __device__ static float A(int i) { return c_sph.A[i]; }
-  __device__ static float B(int i) { return c_sph.B[i]; }
-  __device__ static float P(int m) { return c_sph.Pmm[m]; }
+  __device__ static constexpr float A(int i) { return kSph.A[i]; }
+  __device__ static constexpr float B(int i) { return kSph.B[i]; }
+  __device__ static constexpr float P(int m) { return kSph.Pmm[m]; }
 };
 template <> struct SphK<double> {
-  __device__ static double A(int i) { return c_sph.Ad[i]; }
-  __device__ static double B(int i) { return c_sph.Bd[i]; }
-  __device__ static double P(int m) { return c_sph.Pmmd[m]; }
+  __device__ static constexpr double A(int i) { return kSph.Ad[i]; }
+  __device__ static constexpr double B(int i) { return kSph.Bd[i]; }
+  __device__ static constexpr double P(int m) { return kSph.Pmmd[m]; }
 };
 
 // Fortran MODULO(a,p) for reals in gfortran's formulation (fmod plus sign

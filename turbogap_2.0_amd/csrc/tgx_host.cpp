@@ -114,31 +114,6 @@ void radial_closed_form(const SoapDev& sd, const double* W, int s, double r, dou
   }
 }
 
-// Normalised real-harmonic recurrence constants, index l*(LMAX+1)+m.
-void sph_constants(int lmax, float* A, float* B, float* Pmm, double* Ad, double* Bd,
-                   double* Pmmd) {
-  auto K = [](int l, int m) {
-    double r = 1.0;
-    for (int k = l - m + 1; k <= l + m; k++) r /= (double)k;
-    double v = std::sqrt((2.0 * l + 1.0) / (4.0 * M_PI) * r);
-    return m > 0 ? v * std::sqrt(2.0) : v;
-  };
-  for (int i = 0; i < TGX_L2MAX; i++) A[i] = B[i] = 0, Ad[i] = Bd[i] = 0;
-  double df = 1.0;
-  for (int m = 0; m <= lmax; m++) {
-    if (m > 0) df *= (2.0 * m - 1.0);
-    Pmmd[m] = K(m, m) * df;
-    Pmm[m] = (float)Pmmd[m];
-    for (int l = m + 1; l <= lmax; l++) {
-      const int ix = l * (TGX_MAX_LMAX + 1) + m;
-      Ad[ix] = K(l, m) / K(l - 1, m) * (2.0 * l - 1.0) / (l - m);
-      Bd[ix] = l - 2 >= m ? K(l, m) / K(l - 2, m) * (l + m - 1.0) / (l - m) : 0.0;
-      A[ix] = (float)Ad[ix];
-      B[ix] = (float)Bd[ix];
-    }
-  }
-}
-
 // Orthorhombic cell, supercell repetitions as number_of_unit_cells_for_given_
 // cutoff (neighbors_interface.f90:655-708) and the linked-cell grid
 // mx = int(L/rcut) (:299-301).

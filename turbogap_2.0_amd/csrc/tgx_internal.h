@@ -98,6 +98,7 @@ struct tgx_ctx {
       forces, eatom, virial, scratch, csr_off, csr_out;
   int maxnb = 0;
   double stage_ms[7] = {0};
+  double kernel_ms[4] = {0};  // SOAP forward, kernel GEMMs, SOAP adjoint, force gather
 
   // MD state
   tgx::DevBuf md_vel, md_fprev, md_mass, md_pos0, md_ep, md_ek, md_reb, md_disp;
@@ -119,11 +120,8 @@ void poly3_W(int nmax, double* W);                 // S^(-1/2), binary128
 void radial_closed_form(const SoapDev& sd, const double* W, int s, double r,
                         double* R, double* dR);   // FP64 host evaluation
 int make_cell(const double* lattice, double rcut, Cell* cell, std::string* err);
-void sph_constants(int lmax, float* A, float* B, float* Pmm, double* Ad,
-                   double* Bd, double* Pmmd);
 
 // Kernel launchers are declared next to their callers in tgx_api.hip.
-hipError_t upload_sph_constants(int lmax, hipStream_t st);
 hipError_t launch_gap_kernel(int mode, int n_sites, int dim, int dim_pad,
                              int n_sparse, int sp_pad, const void* q,
                              const void* qs, const void* qsT,

@@ -23,20 +23,6 @@ namespace tgx {
 
 typedef unsigned short u16;
 
-__constant__ SphConst c_sph;
-
-hipError_t upload_sph_constants(int /*lmax*/, hipStream_t st) {
-  static SphConst h;
-  static bool ready = false;
-  if (!ready) {
-    sph_constants(TGX_MAX_LMAX, h.A, h.B, h.Pmm, h.Ad, h.Bd, h.Pmmd);
-    for (int l = 0; l <= TGX_MAX_LMAX; l++)
-      for (int k = l * l; k < (l + 1) * (l + 1); k++) h.lm2l[k] = l;
-    ready = true;
-  }
-  return hipMemcpyToSymbolAsync(HIP_SYMBOL(c_sph), &h, sizeof(h), 0, hipMemcpyHostToDevice, st);
-}
-
 template <typename T> struct Radial;
 template <> struct Radial<double> {
   // FP64: closed form with the FP64 W for every pair.
@@ -356,7 +342,7 @@ __global__ __launch_bounds__(256) void soap_bwd_kernel(
   }
   wave_sync();
   for (int lm = lane; lm < L2; lm += LW) {
-    const int l = c_sph.lm2l[lm];
+    const int l = kSph.lm2l[lm];
     const T* Ml = sM + l * NRP * NRP;
     T cl[NRP];
 #pragma unroll

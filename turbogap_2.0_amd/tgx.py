@@ -218,6 +218,7 @@ def load_library(path: str = LIB_PATH):
         "tgx_energy_forces_batched": (i, [P, i, ip, dp, ip, dp, i, dp, dp, dp]),
         "tgx_md_run": (i, [P, i, i, d, d, ip, dp, dp, dp, dp, dp, dp, dp, ip]),
         "tgx_stage_times": (i, [P, dp]),
+        "tgx_kernel_times": (i, [P, dp]),
         "tgx_md_begin": (i, [P, i, ip, dp, dp, dp, dp, d, d, i]),
         "tgx_md_steps": (i, [P, i]),
         "tgx_md_read": (i, [P, dp, dp, dp, dp, dp, ip, ip]),
@@ -428,3 +429,10 @@ class Engine:
         self._check(self.lib.tgx_stage_times(self.ctx, _dptr(out)))
         return dict(zip(["neighbors", "soap_fwd", "gemm", "soap_bwd", "two_body", "integrator",
                          "total"], out))
+
+    def kernel_times(self):
+        """Event-bracketed device time (ms) of single kernels in the last
+        energy_forces call."""
+        out = np.zeros(4)
+        self._check(self.lib.tgx_kernel_times(self.ctx, _dptr(out)))
+        return dict(zip(["soap_fwd", "gemm", "soap_bwd", "force_gather"], out))
