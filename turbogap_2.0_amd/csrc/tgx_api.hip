@@ -28,20 +28,28 @@ hipError_t launch_csr_out(const double* pos, int n, const Cell& c, int maxnb, co
 template <typename T>
 hipError_t launch_soap_fwd(const SoapDev& sdh, const SoapDev* sd, const double* W,
                            const float* rtab, const double* pos, const int* species,
-                           const Cell& cell, int maxnb, const int* count, const int* list,
+                           const Geom& g, int maxnb, const int* count, const int* list,
                            const int* site_atom, int n_sites, const int* dmap, T* q, int q_ld,
                            unsigned short* qb, int qb_ld, T* cbuf, T* pnorm, hipStream_t st);
 template <typename T>
 hipError_t launch_soap_bwd(const SoapDev& sdh, const SoapDev* sd, const double* W,
                            const float* rtab, const double* pos, const int* species,
-                           const Cell& cell, int maxnb, const int* count, const int* list,
+                           const Geom& g, int maxnb, const int* count, const int* list,
                            const int* site_atom, int n_sites, const int* dmap, const T* q,
                            int q_ld, const T* cbuf, const T* pnorm, const T* G, int g_ld,
                            T* gpair, double* vsite, hipStream_t st);
 hipError_t launch_two_body(const TwoBodyDev& b, const double* pos, const int* species, int n,
-                           const Cell& c, int maxnb, const int* count, const int* list,
+                           const Geom& g, int maxnb, const int* count, const int* list,
                            double* eatom, double* forces, double* vatom, int do_forces,
                            int exact, hipStream_t st);
+hipError_t launch_neighbors_many(const double* pos, int n, const Cell* cells, const int* sid,
+                                 const int* boff, const int* aoff, int n_bins, double rcut,
+                                 int* cell_of, int* cell_count, int* cell_start, int* cell_fill,
+                                 int* cell_atoms, int* count, int maxnb, int* list, int* rev,
+                                 int* flag, hipStream_t st);
+hipError_t launch_reduce_many(int n_struct, const int* aoff, const int* site_of_atom,
+                              const double* vsite, const double* vatom, const double* eatom,
+                              double* out, hipStream_t st);
 hipError_t launch_energy_init(int n, const int* species, const double* e0,
                               const int* site_of_atom, const double* esite, double* eatom,
                               double* forces, double* vatom, hipStream_t st);
@@ -118,6 +126,7 @@ struct MdState {
 
 struct Extra {
   DevBuf sd_dev, pairn, e0_dev, iota, cell_count, cell_fill, out, stage;
+  DevBuf b_cells, b_sid, b_boff, b_aoff, b_out;  // batched structures
   DevBuf twob_store;
   std::vector<int> species_host;
   hipEvent_t ev[8];
@@ -219,7 +228,7 @@ static int setup_sites(tgx_ctx* c, int n, const int* species_host, int* n_sites)
 static void kev_rec(tgx_ctx* c, int k);
 
 // SOAP forward for the current list into c->q (and c->qb in mixed mode).
-static int soap_forward(tgx_ctx* c, int n, const Cell& cell, int n_sites) {
+static int soap_forward(tgx_ctx* c, int n, const Geom& cell, int n_sites) {
   Extra& x = X(c);
   const int prec = c->precision;
   const size_t ts = tsize(prec);
@@ -290,8 +299,8 @@ static void kev_rec(tgx_ctx* c, int k) {
   if (x.ev_ready && x.ev_on) (void)hipEventRecord(x.kev[k], c->stream);
 }
 
-static int ef_launch(tgx_ctx* c, int n, int n_sites, const Cell& cell, int do_forces,
-                     bool events);
+static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_forces,
+                     bool events, int n_struct = 0);
 
 // Energies/forces/virial of the structure whose positions and species are
 // already in c->pos / c->species.  Results stay on the device:
@@ -307,6 +316,8 @@ static int run_ef(tgx_ctx* c, int n, const double* lattice, const int* species_h
   if (rcode) return set_err(c, rcode, "%s", err.c_str());
   if ((long long)n * cell.nrep >= (1LL << TGX_IMG_SHIFT))
     return set_err(c, TGX_ERR_ARG, "too many atoms");
+  if (cell.nrep >= (1 << (31 - TGX_IMG_SHIFT)))
+    return set_err(c, TGX_ERR_UNSUPPORTED, "box too small for the cutoff (%d images)", cell.nrep);
   x.ev_on = true;
   ev_rec(c, 0);
   rcode = build_list(c, n, cell, rc, do_forces != 0);
@@ -315,14 +326,14 @@ static int run_ef(tgx_ctx* c, int n, const double* lattice, const int* species_h
   int n_sites = 0;
   rcode = setup_sites(c, n, species_host, &n_sites);
   if (rcode) return rcode;
-  return ef_launch(c, n, n_sites, cell, do_forces, true);
+  return ef_launch(c, n, n_sites, Geom{cell}, do_forces, true);
 }
 
 // Energy/force launch sequence on the current list and sites.  Asynchronous:
 // no host synchronisation, no allocation once the buffers are sized (the MD
 // step graph captures it).
-static int ef_launch(tgx_ctx* c, int n, int n_sites, const Cell& cell, int do_forces,
-                     bool events) {
+static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_forces,
+                     bool events, int n_struct) {
   Extra& x = X(c);
   int rcode;
   x.ev_on = events;
@@ -393,10 +404,20 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Cell& cell, int do_fo
                                c->forces.as<double>(), c->virial.as<double>(), do_forces,
                                prec == TGX_FP64, c->stream));
   ev_rec(c, 5);
-  TGX_HIP(c, launch_reduce((do_forces && c->has_soap) ? n_sites : 0,
-                           (do_forces && c->has_soap) ? c->vsite.as<double>() : nullptr, n,
-                           do_forces ? c->virial.as<double>() : nullptr, c->eatom.as<double>(),
-                           x.out.as<double>(), c->stream));
+  if (cell.many) {
+    if (x.b_out.ensure(sizeof(double) * 10 * (size_t)n_struct))
+      return set_err(c, TGX_ERR_HIP, "allocation failed");
+    TGX_HIP(c, launch_reduce_many(n_struct, x.b_aoff.as<int>(), c->site_of_atom.as<int>(),
+                                  (do_forces && c->has_soap && n_sites > 0) ? c->vsite.as<double>()
+                                                                            : nullptr,
+                                  do_forces ? c->virial.as<double>() : nullptr,
+                                  c->eatom.as<double>(), x.b_out.as<double>(), c->stream));
+  } else {
+    TGX_HIP(c, launch_reduce((do_forces && c->has_soap) ? n_sites : 0,
+                             (do_forces && c->has_soap) ? c->vsite.as<double>() : nullptr, n,
+                             do_forces ? c->virial.as<double>() : nullptr, c->eatom.as<double>(),
+                             x.out.as<double>(), c->stream));
+  }
   ev_rec(c, 6);
   x.ev_on = true;
   return TGX_OK;
@@ -813,7 +834,7 @@ int tgx_soap_descriptors(tgx_ctx* c, int n, const double* pos, const int* specie
   }
   int n_sites = 0;
   rcode = setup_sites(c, n, species, &n_sites);
-  if (!rcode && n_sites > 0) rcode = soap_forward(c, n, cell, n_sites);
+  if (!rcode && n_sites > 0) rcode = soap_forward(c, n, Geom{cell}, n_sites);
   c->maxnb = saved;
   if (rcode) return rcode;
   TGX_HIP(c, hipStreamSynchronize(c->stream));
@@ -922,13 +943,96 @@ int tgx_energy_forces_batched(tgx_ctx* c, int n_struct, const int* offsets, cons
   if (n_struct < 1 || !offsets || !pos || !species || !lattices || !e_atom ||
       (do_forces && (!forces || !virials)))
     return set_err(c, TGX_ERR_ARG, "bad arguments");
+  if (offsets[0] != 0) return set_err(c, TGX_ERR_ARG, "offsets[0] must be 0");
+  for (int s = 0; s < n_struct; s++)
+    if (offsets[s + 1] <= offsets[s]) return set_err(c, TGX_ERR_ARG, "structure %d is empty", s);
+  const int n = offsets[n_struct];
+  if (n >= (1 << TGX_IMG_SHIFT)) return set_err(c, TGX_ERR_ARG, "too many atoms");
+  if (!c->has_soap && c->twob.empty()) return set_err(c, TGX_ERR_STATE, "no model loaded");
+  (void)hipSetDevice(c->device);
+  Extra& x = X(c);
+  int rcode = upload_structure(c, n, pos, species);
+  if (rcode) return rcode;
+  // per-structure cells, bin offsets and the structure of every atom
+  const double rc = rcut_all(c);
+  std::vector<Cell> cells(n_struct);
+  std::vector<int> boff(n_struct + 1), sid(n);
+  int maxnb = c->maxnb;
+  int guess = 0;
   for (int s = 0; s < n_struct; s++) {
-    const int a = offsets[s], b = offsets[s + 1];
-    if (b <= a) return set_err(c, TGX_ERR_ARG, "structure %d is empty", s);
-    int rcode = tgx_energy_forces(c, b - a, pos + 3 * (size_t)a, species + a, lattices + 9 * (size_t)s,
-                                  do_forces, e_atom + a, do_forces ? forces + 3 * (size_t)a : nullptr,
-                                  do_forces ? virials + 9 * (size_t)s : nullptr);
-    if (rcode) return rcode;
+    std::string err;
+    rcode = make_cell(lattices + 9 * (size_t)s, rc, &cells[s], &err);
+    if (rcode) return set_err(c, rcode, "structure %d: %s", s, err.c_str());
+    if (cells[s].nrep >= (1 << (31 - TGX_IMG_SHIFT)))
+      return set_err(c, TGX_ERR_UNSUPPORTED, "structure %d: box too small for the cutoff", s);
+    const int ns = offsets[s + 1] - offsets[s];
+    boff[s + 1] = boff[s] + (cells[s].small ? 1 : cells[s].mx * cells[s].my * cells[s].mz);
+    for (int i = offsets[s]; i < offsets[s + 1]; i++) sid[i] = s;
+    guess = std::max(guess, guess_maxnb(ns * cells[s].nrep,
+                                        cells[s].La * cells[s].Lb * cells[s].Lc, rc));
+  }
+  if (maxnb <= 0 || maxnb < guess) maxnb = guess;
+  const int n_bins = boff[n_struct];
+  if (x.b_cells.ensure(sizeof(Cell) * n_struct) || x.b_sid.ensure(sizeof(int) * n) ||
+      x.b_boff.ensure(sizeof(int) * (n_struct + 1)) ||
+      x.b_aoff.ensure(sizeof(int) * (n_struct + 1)) || x.cell_count.ensure(sizeof(int) * n_bins) ||
+      x.cell_fill.ensure(sizeof(int) * n_bins))
+    return set_err(c, TGX_ERR_HIP, "device allocation failed");
+  ENSURE(c, cell_of, sizeof(int) * n);
+  ENSURE(c, cell_atoms, sizeof(int) * n);
+  ENSURE(c, cell_start, sizeof(int) * (n_bins + 1));
+  ENSURE(c, nb_count, sizeof(int) * n);
+  ENSURE(c, nb_flag, sizeof(int) * 4);
+  TGX_HIP(c, hipMemcpyAsync(x.b_cells.p, cells.data(), sizeof(Cell) * n_struct,
+                            hipMemcpyHostToDevice, c->stream));
+  TGX_HIP(c, hipMemcpyAsync(x.b_sid.p, sid.data(), sizeof(int) * n, hipMemcpyHostToDevice,
+                            c->stream));
+  TGX_HIP(c, hipMemcpyAsync(x.b_boff.p, boff.data(), sizeof(int) * (n_struct + 1),
+                            hipMemcpyHostToDevice, c->stream));
+  TGX_HIP(c, hipMemcpyAsync(x.b_aoff.p, offsets, sizeof(int) * (n_struct + 1),
+                            hipMemcpyHostToDevice, c->stream));
+  x.ev_on = true;
+  ev_rec(c, 0);
+  bool sized = false;
+  for (int attempt = 0; attempt < 3 && !sized; attempt++) {
+    ENSURE(c, nb_list, sizeof(int) * (size_t)n * maxnb);
+    if (do_forces) ENSURE(c, nb_rev, sizeof(int) * (size_t)n * maxnb);
+    TGX_HIP(c, hipMemsetAsync(c->nb_flag.p, 0, sizeof(int) * 4, c->stream));
+    TGX_HIP(c, launch_neighbors_many(c->pos.as<double>(), n, x.b_cells.as<Cell>(),
+                                     x.b_sid.as<int>(), x.b_boff.as<int>(), x.b_aoff.as<int>(),
+                                     n_bins, rc, c->cell_of.as<int>(), x.cell_count.as<int>(),
+                                     c->cell_start.as<int>(), x.cell_fill.as<int>(),
+                                     c->cell_atoms.as<int>(), c->nb_count.as<int>(), maxnb,
+                                     c->nb_list.as<int>(), do_forces ? c->nb_rev.as<int>() : nullptr,
+                                     c->nb_flag.as<int>(), c->stream));
+    int flag[4];
+    TGX_HIP(c, hipMemcpyAsync(flag, c->nb_flag.p, sizeof(flag), hipMemcpyDeviceToHost, c->stream));
+    TGX_HIP(c, hipStreamSynchronize(c->stream));
+    if (flag[0] > maxnb) {
+      maxnb = (flag[0] + flag[0] / 4 + 8 + 7) / 8 * 8;
+      continue;
+    }
+    if (flag[1]) return set_err(c, TGX_ERR_STATE, "neighbour list is not symmetric");
+    sized = true;
+  }
+  if (!sized) return set_err(c, TGX_ERR_CAPACITY, "neighbour capacity could not be sized");
+  c->maxnb = maxnb;
+  ev_rec(c, 1);
+  int n_sites = 0;
+  rcode = setup_sites(c, n, species, &n_sites);
+  if (rcode) return rcode;
+  rcode = ef_launch(c, n, n_sites, Geom{cells[0], x.b_cells.as<Cell>(), x.b_sid.as<int>()},
+                    do_forces, true, n_struct);
+  if (rcode) return rcode;
+  TGX_HIP(c, hipStreamSynchronize(c->stream));
+  collect_times(c);
+  TGX_HIP(c, hipMemcpy(e_atom, c->eatom.p, 8 * (size_t)n, hipMemcpyDeviceToHost));
+  if (do_forces) {
+    TGX_HIP(c, hipMemcpy(forces, c->forces.p, 24 * (size_t)n, hipMemcpyDeviceToHost));
+    std::vector<double> o(10 * (size_t)n_struct);
+    TGX_HIP(c, hipMemcpy(o.data(), x.b_out.p, sizeof(double) * o.size(), hipMemcpyDeviceToHost));
+    for (int s = 0; s < n_struct; s++)
+      for (int k = 0; k < 9; k++) virials[9 * (size_t)s + k] = o[10 * (size_t)s + k];
   }
   return TGX_OK;
 }
@@ -951,7 +1055,7 @@ static int md_step_launch(tgx_ctx* c) {
                               c->nb_count.as<int>(), c->maxnb, c->nb_list.as<int>(),
                               c->nb_rev.as<int>(), c->nb_flag.as<int>(), c->stream, ctl + 0));
   TGX_HIP(c, launch_md_ref_copy(n, c->pos.as<double>(), m.pref.as<double>(), ctl + 0, c->stream));
-  int rcode = ef_launch(c, n, m.n_sites, m.cell, 1, false);
+  int rcode = ef_launch(c, n, m.n_sites, Geom{m.cell}, 1, false);
   if (rcode) return rcode;
   TGX_HIP(c, launch_md_finish(n, m.ke.as<double>(), x.out.as<double>(), ctl + 0, ctl + 1, ctl + 2,
                               m.ep.as<double>(), m.ek.as<double>(), m.cap, c->stream));
@@ -1035,7 +1139,7 @@ int tgx_md_begin(tgx_ctx* c, int n, const int* species, const double* lattice,
   TGX_HIP(c, hipMemsetAsync(m.ep.p, 0, 8 * (size_t)history, c->stream));
   TGX_HIP(c, hipMemsetAsync(m.ek.p, 0, 8 * (size_t)history, c->stream));
   // forces and energy at the initial positions
-  rcode = ef_launch(c, n, m.n_sites, m.cell, 1, false);
+  rcode = ef_launch(c, n, m.n_sites, Geom{m.cell}, 1, false);
   if (rcode) return rcode;
   TGX_HIP(c, hipMemcpyAsync(m.ep.p, x.out.as<double>() + 9, 8, hipMemcpyDeviceToDevice, c->stream));
   TGX_HIP(c, hipStreamSynchronize(c->stream));

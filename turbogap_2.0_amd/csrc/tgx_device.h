@@ -74,6 +74,17 @@ template <> struct SphK<double> {
   __device__ static constexpr double P(int m) { return kSph.Pmmd[m]; }
 };
 
+// Pair geometry as seen by a kernel: of(i) is the periodic cell of atom i.
+struct CellOne {
+  Cell c;
+  __device__ __forceinline__ const Cell& of(int) const { return c; }
+};
+struct CellMany {
+  const Cell* cells;
+  const int* sid;
+  __device__ __forceinline__ Cell of(int i) const { return cells[sid[i]]; }
+};
+
 // Fortran MODULO(a,p) for reals in gfortran's formulation (fmod plus sign
 // fix), then the L/2 fold and the x,y,z sum of squares in that order
 // (get_distance, reference neighbors_interface.f90:565-581).  Contraction is

@@ -96,7 +96,8 @@ template <typename TI, int EPI>
 __global__ __launch_bounds__(256) void gemm_nt_kernel(int M, int N, int K,
                                                      const TI* __restrict__ A, int lda,
                                                      const TI* __restrict__ B, int ldb,
-                                                     EpiKernel ek, EpiStore es) {
+                                                     EpiKernel ek, EpiStore es, int ntx,
+                                                     int ntiles) {
   typedef MF<TI> F;
   typedef typename F::acc_t acc_t;
   typedef typename F::vacc vacc;
@@ -107,7 +108,15 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(int M, int N, int K,
   __shared__ acc_t sE[2][BM];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wm = wave >> 1, wn = wave & 1;
-  const int row0 = blockIdx.y * BM, col0 = blockIdx.x * BN;
+  // Workgroups are dealt round-robin to the 8 XCDs (id % 8).  Renumber so
+  // that a contiguous run of tile ids, i.e. the column tiles of one row
+  // strip, lands on one XCD and the strip of A is fetched into that XCD's L2
+  // once instead of once per column tile.  gridDim.x is a multiple of 8.
+  const int w = blockIdx.x;
+  const int t = (w & 7) * (gridDim.x >> 3) + (w >> 3);
+  if (t >= ntiles) return;
+  const int tx = t % ntx, ty = t / ntx;
+  const int row0 = ty * BM, col0 = tx * BN;
   const int lr = tid >> 2, lc = (tid & 3) * EPR;
 
   vacc acc[2][2];
@@ -219,7 +228,7 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(int M, int N, int K,
   if (tid < BM) {
     const int r = row0 + tid;
     if (r < M)
-      reinterpret_cast<acc_t*>(ek.epart)[(size_t)blockIdx.x * M + r] = sE[0][tid] + sE[1][tid];
+      reinterpret_cast<acc_t*>(ek.epart)[(size_t)tx * M + r] = sE[0][tid] + sE[1][tid];
   }
 }
 
@@ -245,15 +254,18 @@ static hipError_t run(int n_sites, int dim, int dim_pad, int n_sparse, int sp_pa
   const double zz = zint > 0 ? (double)zint : zeta;
   EpiKernel ek{alphas, n_sparse, zint, zz, zz * delta * delta, wmat, sp_pad, epart};
   EpiStore es{G, dim_pad};
-  dim3 g1(sp_pad / 64, (M + 63) / 64);
-  hipLaunchKernelGGL((gemm_nt_kernel<TI, 0>), g1, dim3(256), 0, st, M, sp_pad, dim_pad,
-                     (const TI*)q, dim_pad, (const TI*)qs, dim_pad, ek, es);
-  hipLaunchKernelGGL(esum_kernel<acc_t>, dim3((M + 255) / 256), dim3(256), 0, st, M,
-                     (int)g1.x, (const acc_t*)epart, delta * delta, esite);
+  const int mt = (M + 63) / 64;
+  const int nt1 = sp_pad / 64;
+  hipLaunchKernelGGL((gemm_nt_kernel<TI, 0>), dim3((nt1 * mt + 7) / 8 * 8), dim3(256), 0, st, M,
+                     sp_pad, dim_pad, (const TI*)q, dim_pad, (const TI*)qs, dim_pad, ek, es, nt1,
+                     nt1 * mt);
+  hipLaunchKernelGGL(esum_kernel<acc_t>, dim3((M + 255) / 256), dim3(256), 0, st, M, nt1,
+                     (const acc_t*)epart, delta * delta, esite);
   if (want_grad) {
-    dim3 g2((dim_pad + 63) / 64, (M + 63) / 64);
-    hipLaunchKernelGGL((gemm_nt_kernel<TI, 1>), g2, dim3(256), 0, st, M, dim_pad, sp_pad,
-                       (const TI*)wmat, sp_pad, (const TI*)qsT, sp_pad, ek, es);
+    const int nt2 = (dim_pad + 63) / 64;
+    hipLaunchKernelGGL((gemm_nt_kernel<TI, 1>), dim3((nt2 * mt + 7) / 8 * 8), dim3(256), 0, st,
+                       M, dim_pad, sp_pad, (const TI*)wmat, sp_pad, (const TI*)qsT, sp_pad, ek,
+                       es, nt2, nt2 * mt);
   }
   return hipGetLastError();
 }

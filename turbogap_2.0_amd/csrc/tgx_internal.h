@@ -33,6 +33,15 @@ struct Cell {
   double iLa, iLb, iLc;  // 1 / supercell edges
 };
 
+// Pair geometry handed to the launchers: one periodic cell for every atom,
+// or (batched structures) a device array of cells with sid[atom] = the
+// atom's structure.
+struct Geom {
+  Cell one;
+  const Cell* many = nullptr;
+  const int* sid = nullptr;
+};
+
 // SOAP model constants in device-friendly form.
 struct SoapDev {
   int n_species, l_max, L2, nr, dim, npair, radial_enhancement, central_species;

@@ -220,11 +220,13 @@ __global__ __launch_bounds__(256) void sort_lists_kernel(int n, int maxnb, const
 }
 
 // rev[i][slot] = flat index of the entry (neg img, i) in j's list.
-__global__ void reverse_kernel(int n, int maxnb, Cell c, const int* count,
+template <typename G>
+__global__ void reverse_kernel(int n, int maxnb, G geo, const int* count,
                                const int* list, int* rev, int* flag,
                                const int* guard) {
   GUARD_RETURN();
   int i = blockIdx.x;
+  const Cell c = geo.of(i);
   int cnt = min(count[i], maxnb);
   for (int s = threadIdx.x; s < cnt; s += blockDim.x) {
     int e = list[(size_t)i * maxnb + s];
@@ -317,8 +319,117 @@ hipError_t launch_neighbors(const double* pos, int n, const Cell& c, double rcut
   hipLaunchKernelGGL(sort_lists_kernel, dim3((n + 3) / 4), dim3(256), 0, st, n, maxnb, count,
                      list, guard);
   if (rev)
-    hipLaunchKernelGGL(reverse_kernel, dim3(n), dim3(64), 0, st, n, maxnb, c, count, list, rev,
-                       flag, guard);
+    hipLaunchKernelGGL(reverse_kernel<CellOne>, dim3(n), dim3(64), 0, st, n, maxnb, CellOne{c},
+                       count, list, rev, flag, guard);
+  return hipGetLastError();
+}
+
+// Batched structures.  Every structure s has its own cell cells[s], atoms
+// aoff[s]..aoff[s+1]-1 and, for a box of at least 2 rcut, its own linked-cell
+// grid at bins boff[s]..; a smaller box gets one bin and brute force over
+// its atoms and supercell images, as for a single structure.
+__global__ void bin_atoms_many_kernel(const double* pos, int n, const Cell* cells,
+                                      const int* sid, const int* boff, int* cell_of,
+                                      int* cell_count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s = sid[i];
+  const Cell c = cells[s];
+  int cell = boff[s];
+  if (!c.small) {
+    int cx, cy, cz;
+    cell_coords(pos + 3 * i, c, &cx, &cy, &cz);
+    cell += imodd(cx, c.mx) + imodd(cy, c.my) * c.mx + imodd(cz, c.mz) * c.mx * c.my;
+  }
+  cell_of[i] = cell;
+  atomicAdd(cell_count + cell, 1);
+}
+
+__global__ void find_neighbors_many_kernel(const double* pos, int n, const Cell* cells,
+                                           const int* sid, const int* boff, const int* aoff,
+                                           double rcut, const int* cell_start,
+                                           const int* cell_atoms, int* count, int maxnb,
+                                           int* list, int* flag) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = t / TGX_NB_LANES, part = t % TGX_NB_LANES;
+  if (i >= n) return;
+  const int s = sid[i];
+  const Cell c = cells[s];
+  int* out = list + (size_t)i * maxnb;
+  const double* pi = pos + 3 * i;
+  const double screen2 = (rcut + 1e-6) * (rcut + 1e-6);
+  if (!c.small) {
+    int cx, cy, cz;
+    cell_coords(pi, c, &cx, &cy, &cz);
+    for (int q = part; q < 27; q += TGX_NB_LANES) {
+      const int i3 = cx - 1 + q % 3, j3 = cy - 1 + (q / 3) % 3, k3 = cz - 1 + q / 9;
+      const int cell = imodd(i3, c.mx) + imodd(j3, c.my) * c.mx + imodd(k3, c.mz) * c.mx * c.my;
+      bool dup = false;
+      for (int q2 = 0; q2 < q; q2++) {
+        const int a3 = cx - 1 + q2 % 3, b3 = cy - 1 + (q2 / 3) % 3, c3 = cz - 1 + q2 / 9;
+        dup |= (imodd(a3, c.mx) + imodd(b3, c.my) * c.mx + imodd(c3, c.mz) * c.mx * c.my) == cell;
+      }
+      if (dup) continue;
+      const int gc = boff[s] + cell;
+      for (int a = cell_start[gc]; a < cell_start[gc + 1]; a++) {
+        const int k = cell_atoms[a];
+        if (k == i) continue;
+        const double* pk = pos + 3 * k;
+        double dx = pk[0] - pi[0], dy = pk[1] - pi[1], dz = pk[2] - pi[2];
+        dx -= c.La * rint(dx * c.iLa);
+        dy -= c.Lb * rint(dy * c.iLb);
+        dz -= c.Lc * rint(dz * c.iLc);
+        if (dx * dx + dy * dy + dz * dz > screen2) continue;
+        double dv[3];
+        const double d = mic(pi, pk, c.La, c.Lb, c.Lc, dv);
+        if (d < rcut) {
+          const int slot = atomicAdd(count + i, 1);
+          if (slot < maxnb) out[slot] = k;
+          else atomicMax(flag, slot + 1);
+        }
+      }
+    }
+  } else {
+    const int k0 = aoff[s], k1 = aoff[s + 1];
+    for (int img = 0; img < c.nrep; img++)
+      for (int k = k0 + part; k < k1; k += TGX_NB_LANES) {
+        if (img == 0 && k == i) continue;
+        double pk[3], dv[3];
+        image_pos(pos, k, img, c, pk);
+        const double d = mic(pi, pk, c.La, c.Lb, c.Lc, dv);
+        if (d < rcut) {
+          const int slot = atomicAdd(count + i, 1);
+          if (slot < maxnb) out[slot] = (img << TGX_IMG_SHIFT) | k;
+          else atomicMax(flag, slot + 1);
+        }
+      }
+  }
+}
+
+hipError_t launch_neighbors_many(const double* pos, int n, const Cell* cells, const int* sid,
+                                 const int* boff, const int* aoff, int n_bins, double rcut,
+                                 int* cell_of, int* cell_count, int* cell_start, int* cell_fill,
+                                 int* cell_atoms, int* count, int maxnb, int* list, int* rev,
+                                 int* flag, hipStream_t st) {
+  const int B = 256;
+  const int nb = (n + B - 1) / B;
+  hipLaunchKernelGGL(zero_ints_kernel, dim3((n_bins + B - 1) / B), dim3(B), 0, st, cell_count,
+                     n_bins, nullptr);
+  hipLaunchKernelGGL(bin_atoms_many_kernel, dim3(nb), dim3(B), 0, st, pos, n, cells, sid, boff,
+                     cell_of, cell_count);
+  hipLaunchKernelGGL(scan_cells_kernel, dim3(1), dim3(1024), 0, st, cell_count, n_bins,
+                     cell_start, cell_fill, nullptr);
+  hipLaunchKernelGGL(scatter_cells_kernel, dim3(nb), dim3(B), 0, st, n, cell_of, cell_start,
+                     cell_fill, cell_atoms, nullptr);
+  hipLaunchKernelGGL(zero_ints_kernel, dim3(nb), dim3(B), 0, st, count, n, nullptr);
+  hipLaunchKernelGGL(find_neighbors_many_kernel, dim3((n * TGX_NB_LANES + B - 1) / B), dim3(B),
+                     0, st, pos, n, cells, sid, boff, aoff, rcut, cell_start, cell_atoms, count,
+                     maxnb, list, flag);
+  hipLaunchKernelGGL(sort_lists_kernel, dim3((n + 3) / 4), dim3(256), 0, st, n, maxnb, count,
+                     list, nullptr);
+  if (rev)
+    hipLaunchKernelGGL(reverse_kernel<CellMany>, dim3(n), dim3(64), 0, st, n, maxnb,
+                       CellMany{cells, sid}, count, list, rev, flag, nullptr);
   return hipGetLastError();
 }
 
@@ -404,9 +515,9 @@ __device__ __forceinline__ T wsum(T v) {
   return v;
 }
 
-template <int EXACT>
+template <int EXACT, typename G>
 __global__ __launch_bounds__(256) void two_body_kernel(TwoBodyDev b, const double* pos,
-                                                       const int* species, int n, Cell c,
+                                                       const int* species, int n, G geo,
                                                        int maxnb, const int* count,
                                                        const int* list, double* eatom,
                                                        double* forces, double* vatom,
@@ -417,6 +528,7 @@ __global__ __launch_bounds__(256) void two_body_kernel(TwoBodyDev b, const doubl
   const double pi = 3.14159265358979323846;
   const int si = species[i];
   if (si != b.sp1 && si != b.sp2) return;
+  const Cell c = geo.of(i);
   const int cnt = min(count[i], maxnb);
   double e = 0, f[3] = {0, 0, 0}, v[6] = {0, 0, 0, 0, 0, 0};
   for (int s = lane; s < cnt; s += 64) {
@@ -501,17 +613,30 @@ __global__ __launch_bounds__(256) void two_body_kernel(TwoBodyDev b, const doubl
   }
 }
 
-hipError_t launch_two_body(const TwoBodyDev& b, const double* pos, const int* species, int n,
-                           const Cell& c, int maxnb, const int* count, const int* list,
-                           double* eatom, double* forces, double* vatom, int do_forces,
-                           int exact, hipStream_t st) {
+template <typename G>
+static void two_body_go(const TwoBodyDev& b, const double* pos, const int* species, int n,
+                        const G& g, int maxnb, const int* count, const int* list, double* eatom,
+                        double* forces, double* vatom, int do_forces, int exact,
+                        hipStream_t st) {
   const dim3 grid((n + 3) / 4), block(256);
   if (exact || !b.tab)
-    hipLaunchKernelGGL(two_body_kernel<1>, grid, block, 0, st, b, pos, species, n, c, maxnb,
+    hipLaunchKernelGGL((two_body_kernel<1, G>), grid, block, 0, st, b, pos, species, n, g, maxnb,
                        count, list, eatom, forces, vatom, do_forces);
   else
-    hipLaunchKernelGGL(two_body_kernel<0>, grid, block, 0, st, b, pos, species, n, c, maxnb,
+    hipLaunchKernelGGL((two_body_kernel<0, G>), grid, block, 0, st, b, pos, species, n, g, maxnb,
                        count, list, eatom, forces, vatom, do_forces);
+}
+
+hipError_t launch_two_body(const TwoBodyDev& b, const double* pos, const int* species, int n,
+                           const Geom& g, int maxnb, const int* count, const int* list,
+                           double* eatom, double* forces, double* vatom, int do_forces,
+                           int exact, hipStream_t st) {
+  if (g.many)
+    two_body_go(b, pos, species, n, CellMany{g.many, g.sid}, maxnb, count, list, eatom, forces,
+                vatom, do_forces, exact, st);
+  else
+    two_body_go(b, pos, species, n, CellOne{g.one}, maxnb, count, list, eatom, forces, vatom,
+                do_forces, exact, st);
   return hipGetLastError();
 }
 
@@ -595,6 +720,52 @@ hipError_t launch_reduce(int n_sites, const double* vsite, int n, const double* 
   hipLaunchKernelGGL(reduce_part_kernel, dim3(TGX_RB), dim3(256), 0, st, n_sites, vsite, n, vatom,
                      eatom, out + 16);
   hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(TGX_RB), 0, st, out + 16, out);
+  return hipGetLastError();
+}
+
+// Batched structures: per-structure energy and virial, one workgroup per
+// structure summing its atoms in a fixed order (out[10*s + 0..8] virial,
+// out[10*s + 9] energy).
+__global__ void reduce_many_kernel(const int* aoff, const int* site_of_atom,
+                                   const double* vsite, const double* vatom,
+                                   const double* eatom, double* out) {
+  __shared__ double sh[256][7];
+  const int t = threadIdx.x, s = blockIdx.x;
+  double a[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int i = aoff[s] + t; i < aoff[s + 1]; i += 256) {
+    if (vsite) {
+      const int st = site_of_atom[i];
+      if (st >= 0)
+        for (int k = 0; k < 6; k++) a[k] += vsite[(size_t)st * 6 + k];
+    }
+    if (vatom)
+      for (int k = 0; k < 6; k++) a[k] += vatom[(size_t)i * 6 + k];
+    a[6] += eatom[i];
+  }
+  for (int k = 0; k < 7; k++) sh[t][k] = a[k];
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (t < off)
+      for (int k = 0; k < 7; k++) sh[t][k] += sh[t + off][k];
+    __syncthreads();
+  }
+  if (t == 0) {
+    double* o = out + 10 * (size_t)s;
+    o[0] = sh[0][0];
+    o[4] = sh[0][1];
+    o[8] = sh[0][2];
+    o[1] = o[3] = sh[0][3];
+    o[2] = o[6] = sh[0][4];
+    o[5] = o[7] = sh[0][5];
+    o[9] = sh[0][6];
+  }
+}
+
+hipError_t launch_reduce_many(int n_struct, const int* aoff, const int* site_of_atom,
+                              const double* vsite, const double* vatom, const double* eatom,
+                              double* out, hipStream_t st) {
+  hipLaunchKernelGGL(reduce_many_kernel, dim3(n_struct), dim3(256), 0, st, aoff, site_of_atom,
+                     vsite, vatom, eatom, out);
   return hipGetLastError();
 }
 

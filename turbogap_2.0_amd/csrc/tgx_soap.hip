@@ -1,574 +1,58 @@
-// SOAP expansion (forward) and its adjoint (forces, virial) on the device.
-//
-// Layout: a workgroup runs up to four independent 64-lane waves (fewer when
-// a large model needs more LDS per site); a wave serves two sites with 32
-// lanes each (one site with 64 lanes for models with more than 16 radial
-// channels), each site with its own LDS region, so nothing waits across
-// sites.  Forward, per site:
-//   A  lanes = neighbour slots: minimum-image vector, radial values R_n(r)
-//      and A_l(r) Y_lm(u) for the valid pairs, compacted into LDS in chunks;
-//   B  lanes = lm: c[n][lm] += R_n(j) A_l Y_lm(j) accumulated in registers;
-//   C  power spectrum p[(n<=n'),l], its norm and q = p/|p|.
-// Adjoint, per site: dE/dp from G = dE/dq, dE/dc (lanes = lm), then lanes =
-// pairs recompute the basis with derivatives and contract it with dE/dc into
-// g_ij = dE_i/dr_ij; the site virial is reduced over the wave.
-// l_max and n_max are compile-time bounds (loops fully unrolled, guarded by
-// the runtime values) so that the recurrences live in registers.
-#include <hip/hip_runtime.h>
-
-#include "tgx_device.h"
-#include "tgx_internal.h"
+// SOAP launchers for one periodic cell, and the entry that picks the pair
+// geometry (one cell, or a cell per structure for batched structures).
+#include "tgx_soap_impl.h"
 
 namespace tgx {
 
-typedef unsigned short u16;
-
-template <typename T> struct Radial;
-template <> struct Radial<double> {
-  // FP64: closed form with the FP64 W for every pair.
-  template <int NMX>
-  __device__ __forceinline__ static void eval(const SoapDev* sd, const double* W, const float2*, int s,
-                              double r, double* R, double* dR) {
-    double Bv[NMX], dB[NMX], e, de;
-    radial_basis_fp64<NMX>(sd, W, s, r, Bv, dB);
-    radial_envelope<double>(sd, s, r, &e, &de);
-#pragma unroll
-    for (int n = 0; n < NMX; n++) {
-      R[n] = e * Bv[n];
-      dR[n] = de * Bv[n] + e * dB[n];
-    }
-  }
-};
-template <> struct Radial<float> {
-  // FP32: table of the smooth part (built in FP64), analytic envelope.
-  template <int NMX>
-  __device__ __forceinline__ static void eval(const SoapDev* sd, const double*, const float2* tab, int s,
-                              float r, float* R, float* dR) {
-    float Bv[NMX], dB[NMX], e, de;
-    radial_basis_tab<NMX>(sd, tab, s, r, Bv, dB, sd->nmax[s]);
-    radial_envelope<float>(sd, s, r, &e, &de);
-#pragma unroll
-    for (int n = 0; n < NMX; n++) {
-      R[n] = e * Bv[n];
-      dR[n] = de * Bv[n] + e * dB[n];
-    }
-  }
-};
-
+// Defined in tgx_soap_batch.hip.
 template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-// LDS written by other lanes of this wave is visible after this point.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ __forceinline__ u16 f2bf(float f) {
-  unsigned int u = __float_as_uint(f);
-  return (u16)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
-}
-
-// Minimum-image pair vector by rounding (the neighbour list already fixed the
-// pair set with the reference's arithmetic; here only the vector is needed).
-__device__ __forceinline__ double pair_vec_fast(const double* __restrict__ pos, int i,
-                                                int packed, const Cell& c, double* dv) {
-  const int j = packed & TGX_ATOM_MASK, img = packed >> TGX_IMG_SHIFT;
-  double pj[3];
-  if (img == 0) {
-    pj[0] = pos[3 * j];
-    pj[1] = pos[3 * j + 1];
-    pj[2] = pos[3 * j + 2];
-  } else {
-    image_pos(pos, j, img, c, pj);
-  }
-  double dx = pj[0] - pos[3 * i], dy = pj[1] - pos[3 * i + 1], dz = pj[2] - pos[3 * i + 2];
-  dx -= c.La * rint(dx * c.iLa);
-  dy -= c.Lb * rint(dy * c.iLb);
-  dz -= c.Lc * rint(dz * c.iLc);
-  dv[0] = dx;
-  dv[1] = dy;
-  dv[2] = dz;
-  return sqrt(dx * dx + dy * dy + dz * dz);
-}
-
-// Sites per 64-lane wave: two (32 lanes each) when the model has at most 16
-// radial channels, else one.  Pairs per LDS chunk and the per-site LDS
-// footprint (elements of T).
-template <int NRP> struct Spw { static constexpr int v = NRP <= 16 ? 2 : 1; };
-template <typename T, int SPW> struct Chunk { static constexpr int CH = SPW == 2 ? 24 : 32; };
-template <int SPW> struct Chunk<double, SPW> { static constexpr int CH = 16; };
-
-template <typename T, int NRP>
-__host__ __device__ constexpr size_t fwd_site_elems(int L2) {
-  constexpr int CH = Chunk<T, Spw<NRP>::v>::CH;
-  return (size_t)(CH > NRP ? CH : NRP) * L2 + (size_t)CH * NRP;
-}
-template <int NRP>
-__host__ __device__ constexpr size_t bwd_site_elems(int L2, int Lp) {
-  return (size_t)2 * NRP * L2 + (size_t)Lp * NRP * NRP;
-}
-
-// Sum over the LW lanes of a sub-wave.
-template <int LW, typename T>
-__device__ __forceinline__ T sub_sum(T v) {
-#pragma unroll
-  for (int o = LW / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-// A_l = t^(l(l+1)), t = exp(-sig_t^2/(2 r^2)), for l = 0..LMX.
-template <typename T, int LMX>
-__device__ __forceinline__ void ang_factors(T log2t, T* Al) {
-  const T t2 = exp2_t((T)2 * log2t);
-  T Tl = 1;
-  Al[0] = 1;
-#pragma unroll
-  for (int l = 1; l <= LMX; l++) {
-    Tl *= t2;
-    Al[l] = Al[l - 1] * Tl;
-  }
-}
-
-template <typename T, int NRP, int NMX, int LMX>
-__global__ __launch_bounds__(256) void soap_fwd_kernel(
-    const SoapDev* __restrict__ sd, const double* __restrict__ W,
-    const float2* __restrict__ rtab, const double* __restrict__ pos,
-    const int* __restrict__ species, Cell cell, int maxnb, const int* __restrict__ count,
-    const int* __restrict__ list, const int* __restrict__ site_atom, int n_sites,
-    const int* __restrict__ dmap, T* __restrict__ q, int q_ld, u16* __restrict__ qb,
-    int qb_ld, T* __restrict__ cbuf, T* __restrict__ pnorm, int wpb) {
-  constexpr int SPW = Spw<NRP>::v, LW = 64 / SPW;
-  constexpr int CH = Chunk<T, SPW>::CH;
-  constexpr int NS = ((LMX + 1) * (LMX + 1) + LW - 1) / LW;  // lm slots per lane
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int L = sd->l_max, L2 = sd->L2;
-  const int wave = threadIdx.x >> 6, lane64 = threadIdx.x & 63;
-  const int sub = lane64 / LW, lane = lane64 % LW;
-  T* sAY = reinterpret_cast<T*>(smem_raw) +
-           (size_t)(wave * SPW + sub) * fwd_site_elems<T, NRP>(L2);
-  T* sR = sAY + (size_t)(CH > NRP ? CH : NRP) * L2;
-  T* sC = sAY;  // reused once the accumulation is done
-  const int site = (blockIdx.x * wpb + wave) * SPW + sub;
-  if (site >= n_sites) return;
-  const int i = site_atom[site];
-  const int si = species[i] - 1;
-  const int cnt = min(count[i], maxnb);
-
-  T acc[NS][NRP];
-#pragma unroll
-  for (int k = 0; k < NS; k++)
-#pragma unroll
-    for (int n = 0; n < NRP; n++) acc[k][n] = (T)0;
-
-  for (int base = 0; base < cnt; base += LW) {
-    const int slot = base + lane;
-    double dv[3] = {0, 0, 0};
-    double rd = 1e30;
-    int s = 0;
-    if (slot < cnt) {
-      const int e = list[(size_t)i * maxnb + slot];
-      rd = pair_vec_fast(pos, i, e, cell, dv);
-      s = species[e & TGX_ATOM_MASK] - 1;
-    }
-    const bool valid = slot < cnt && rd < sd->rh[s];
-    unsigned long long mask = __ballot(valid);
-    if (SPW == 2) mask = (mask >> (sub * 32)) & 0xffffffffull;
-    const int jc = __popcll(mask & ((1ull << lane) - 1ull));
-    const int npv = __popcll(mask);
-    for (int c0 = 0; c0 < npv; c0 += CH) {
-      if (valid && jc >= c0 && jc < c0 + CH) {
-        const int jj = jc - c0;
-        const T r = (T)rd;
-        const T ir = (T)1 / r;
-        const T ux = (T)dv[0] * ir, uy = (T)dv[1] * ir, uz = (T)dv[2] * ir;
-        T R[NMX], dR[NMX];
-        Radial<T>::template eval<NMX>(sd, W, rtab, s, r, R, dR);
-        const int o = sd->off[s];
-        // R[n] = 0 for n >= n_max(s) and o + NMX <= NRP: no guard needed
-#pragma unroll
-        for (int n = 0; n < NRP; n++) sR[jj * NRP + n] = (T)0;
-#pragma unroll
-        for (int n = 0; n < NMX; n++) sR[jj * NRP + o + n] = R[n];
-        T log2t, D;
-        ang_params<T>(sd, s, r, &log2t, &D);
-        T Al[LMX + 1];
-        ang_factors<T, LMX>(log2t, Al);
-        T* ay = sAY + jj * L2;
-        T Cm = 1, Sm = 0;
-#pragma unroll
-        for (int m = 0; m <= LMX; m++) {
-          if (m <= L) {
-            T pm2 = 0, pm1 = SphK<T>::P(m);
-#pragma unroll
-            for (int l = m; l <= LMX; l++) {
-              if (l <= L) {
-                T P = pm1;
-                if (l > m) {
-                  const int ix = l * (TGX_MAX_LMAX + 1) + m;
-                  P = SphK<T>::A(ix) * uz * pm1 - SphK<T>::B(ix) * pm2;
-                  pm2 = pm1;
-                  pm1 = P;
-                }
-                const T AP = Al[l] * P;
-                ay[l * l + l + m] = AP * Cm;
-                if (m > 0) ay[l * l + l - m] = AP * Sm;
-              }
-            }
-            const T Cn = ux * Cm - uy * Sm;
-            Sm = ux * Sm + uy * Cm;
-            Cm = Cn;
-          }
-        }
-      }
-      wave_sync();
-      const int nj = min(CH, npv - c0);
-      for (int j = 0; j < nj; j++) {
-        T y[NS];
-#pragma unroll
-        for (int k = 0; k < NS; k++) {
-          const int lm = lane + k * LW;
-          y[k] = lm < L2 ? sAY[j * L2 + lm] : (T)0;
-        }
-#pragma unroll
-        for (int n = 0; n < NRP; n++) {
-          const T rv = sR[j * NRP + n];
-#pragma unroll
-          for (int k = 0; k < NS; k++) acc[k][n] += rv * y[k];
-        }
-      }
-      wave_sync();
-    }
-  }
-  // central atom: l = 0 only
-  if (lane == 0) {
-    const int o = sd->off[si];
-    for (int n = 0; n < sd->nmax[si]; n++) {
-#pragma unroll
-      for (int k = 0; k < NRP; k++)
-        if (k == o + n) acc[0][k] += (T)sd->rself[si * TGX_MAX_NR + n];
-    }
-  }
-  T* cs = cbuf + (size_t)site * NRP * L2;
-#pragma unroll
-  for (int k = 0; k < NS; k++) {
-    const int lm = lane + k * LW;
-    if (lm < L2) {
-#pragma unroll
-      for (int n = 0; n < NRP; n++) {
-        sC[n * L2 + lm] = acc[k][n];
-        cs[n * L2 + lm] = acc[k][n];
-      }
-    }
-  }
-  wave_sync();
-  const int dim = sd->dim;
-  T* qs = q + (size_t)site * q_ld;
-  T ss = 0;
-  for (int d = lane; d < dim; d += LW) {
-    const int dm = dmap[d];
-    const int n1 = dm & 255, n2 = (dm >> 8) & 255, l = dm >> 16;
-    const T* ca = sC + n1 * L2 + l * l;
-    const T* cb = sC + n2 * L2 + l * l;
-    T a = 0;
-    for (int m = 0; m <= 2 * l; m++) a += ca[m] * cb[m];
-    const T p = (n1 == n2 ? (T)1 : (T)1.41421356237309504880) * a;
-    qs[d] = p;
-    ss += p * p;
-  }
-  ss = sub_sum<LW>(ss);
-  const T inv = (T)1 / sqrt(ss);
-  for (int d = lane; d < q_ld; d += LW) {
-    const T v = d < dim ? qs[d] * inv : (T)0;
-    qs[d] = v;
-    if (qb) qb[(size_t)site * qb_ld + d] = f2bf((float)v);
-  }
-  if (qb)
-    for (int d = q_ld + lane; d < qb_ld; d += LW) qb[(size_t)site * qb_ld + d] = 0;
-  if (lane == 0) pnorm[site] = sqrt(ss);
-}
-
-template <typename T, int NRP, int NMX, int LMX>
-__global__ __launch_bounds__(256) void soap_bwd_kernel(
-    const SoapDev* __restrict__ sd, const double* __restrict__ W,
-    const float2* __restrict__ rtab, const double* __restrict__ pos,
-    const int* __restrict__ species, Cell cell, int maxnb, const int* __restrict__ count,
-    const int* __restrict__ list, const int* __restrict__ site_atom, int n_sites,
-    const int* __restrict__ dmap, const T* __restrict__ q, int q_ld,
-    const T* __restrict__ cbuf, const T* __restrict__ pnorm, const T* __restrict__ G, int g_ld,
-    T* __restrict__ gpair, double* __restrict__ vsite, int wpb) {
-  constexpr int SPW = Spw<NRP>::v, LW = 64 / SPW;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int L = sd->l_max, L2 = sd->L2, dim = sd->dim, Lp = L + 1;
-  const int wave = threadIdx.x >> 6, lane64 = threadIdx.x & 63;
-  const int sub = lane64 / LW, lane = lane64 % LW;
-  T* sC = reinterpret_cast<T*>(smem_raw) + (size_t)(wave * SPW + sub) * bwd_site_elems<NRP>(L2, Lp);
-  T* sGC = sC + NRP * L2;  // [L2][NRP]
-  T* sM = sGC + NRP * L2;  // [Lp][NRP][NRP]: dE/dc = M_l c
-  const int site = (blockIdx.x * wpb + wave) * SPW + sub;
-  if (site >= n_sites) return;
-  const int i = site_atom[site];
-  const int cnt = min(count[i], maxnb);
-  const T* cs = cbuf + (size_t)site * NRP * L2;
-  for (int k = lane; k < NRP * L2; k += LW) sC[k] = cs[k];
-  for (int k = lane; k < Lp * NRP * NRP; k += LW) sM[k] = (T)0;
-  const T* qs = q + (size_t)site * q_ld;
-  const T* gs = G + (size_t)site * g_ld;
-  T gq = 0;
-  for (int d = lane; d < dim; d += LW) gq += gs[d] * qs[d];
-  gq = sub_sum<LW>(gq);
-  const T inv = (T)1 / pnorm[site];
-  wave_sync();
-  // dE/dp -> M_l[n][n2] = w dE/dp[(n,n2),l] (symmetric; w = 2 on the diagonal,
-  // sqrt(2) off it, the derivative of the multiplicity-weighted product)
-  for (int d = lane; d < dim; d += LW) {
-    const T gp = (gs[d] - gq * qs[d]) * inv;
-    const int dm = dmap[d];
-    const int n1 = dm & 255, n2 = (dm >> 8) & 255, l = dm >> 16;
-    T* Ml = sM + l * NRP * NRP;
-    if (n1 == n2) {
-      Ml[n1 * NRP + n1] = (T)2 * gp;
-    } else {
-      const T w = (T)1.41421356237309504880 * gp;
-      Ml[n1 * NRP + n2] = w;
-      Ml[n2 * NRP + n1] = w;
-    }
-  }
-  wave_sync();
-  for (int lm = lane; lm < L2; lm += LW) {
-    const int l = kSph.lm2l[lm];
-    const T* Ml = sM + l * NRP * NRP;
-    T cl[NRP];
-#pragma unroll
-    for (int n2 = 0; n2 < NRP; n2++) cl[n2] = sC[n2 * L2 + lm];
-#pragma unroll
-    for (int n = 0; n < NRP; n++) {
-      T a = 0;
-#pragma unroll
-      for (int n2 = 0; n2 < NRP; n2++) a += Ml[n * NRP + n2] * cl[n2];
-      sGC[lm * NRP + n] = a;
-    }
-  }
-  wave_sync();
-  double v[6] = {0, 0, 0, 0, 0, 0};
-  T* gp_out = gpair + (size_t)i * maxnb * 3;
-  for (int base = 0; base < cnt; base += LW) {
-    const int slot = base + lane;
-    if (slot >= cnt) continue;
-    const int e = list[(size_t)i * maxnb + slot];
-    double dv[3];
-    const double rd = pair_vec_fast(pos, i, e, cell, dv);
-    const int s = species[e & TGX_ATOM_MASK] - 1;
-    T g0 = 0, g1 = 0, g2 = 0;
-    if (rd < sd->rh[s]) {
-      const T r = (T)rd;
-      const T ir = (T)1 / r;
-      const T ux = (T)dv[0] * ir, uy = (T)dv[1] * ir, uz = (T)dv[2] * ir;
-      T R[NMX], dR[NMX];
-      Radial<T>::template eval<NMX>(sd, W, rtab, s, r, R, dR);
-      const int o = sd->off[s];
-      T log2t, D;
-      ang_params<T>(sd, s, r, &log2t, &D);
-      T Al[LMX + 1];
-      ang_factors<T, LMX>(log2t, Al);
-      T s1 = 0, Vx = 0, Vy = 0, Vz = 0;
-      T Cm = 1, Sm = 0, Cp = 0, Sp = 0;  // C_m, S_m and C_{m-1}, S_{m-1}
-#pragma unroll
-      for (int m = 0; m <= LMX; m++) {
-        if (m <= L) {
-          T pm2 = 0, pm1 = SphK<T>::P(m), dm2 = 0, dm1 = 0;
-#pragma unroll
-          for (int l = m; l <= LMX; l++) {
-            if (l <= L) {
-              T P = pm1, dP = dm1;
-              if (l > m) {
-                const int ix = l * (TGX_MAX_LMAX + 1) + m;
-                const T a = SphK<T>::A(ix), b = SphK<T>::B(ix);
-                P = a * uz * pm1 - b * pm2;
-                dP = a * (pm1 + uz * dm1) - b * dm2;
-                pm2 = pm1;
-                pm1 = P;
-                dm2 = dm1;
-                dm1 = dP;
-              }
-              const T A = Al[l];
-              const T dA = A * (T)(l * (l + 1)) * D;
-              {
-                const T* gc = sGC + (l * l + l + m) * NRP + o;
-                T h = 0, hd = 0;
-#pragma unroll
-                for (int n = 0; n < NMX; n++) {
-                  h += gc[n] * R[n];
-                  hd += gc[n] * dR[n];
-                }
-                s1 += (hd * A + h * dA) * (P * Cm);
-                const T hA = h * A;
-                if (m > 0) {
-                  Vx += hA * P * (T)m * Cp;
-                  Vy -= hA * P * (T)m * Sp;
-                }
-                Vz += hA * dP * Cm;
-              }
-              if (m > 0) {
-                const T* gc = sGC + (l * l + l - m) * NRP + o;
-                T h = 0, hd = 0;
-#pragma unroll
-                for (int n = 0; n < NMX; n++) {
-                  h += gc[n] * R[n];
-                  hd += gc[n] * dR[n];
-                }
-                s1 += (hd * A + h * dA) * (P * Sm);
-                const T hA = h * A;
-                Vx += hA * P * (T)m * Sp;
-                Vy += hA * P * (T)m * Cp;
-                Vz += hA * dP * Sm;
-              }
-            }
-          }
-          const T Cn = ux * Cm - uy * Sm;
-          const T Sn = ux * Sm + uy * Cm;
-          Cp = Cm;
-          Sp = Sm;
-          Cm = Cn;
-          Sm = Sn;
-        }
-      }
-      const T uV = ux * Vx + uy * Vy + uz * Vz;
-      g0 = s1 * ux + (Vx - uV * ux) * ir;
-      g1 = s1 * uy + (Vy - uV * uy) * ir;
-      g2 = s1 * uz + (Vz - uV * uz) * ir;
-      const double gd[3] = {(double)g0, (double)g1, (double)g2};
-      // virial += 1/2 (F x + x F), F = -g the force on the neighbour
-      v[0] -= gd[0] * dv[0];
-      v[1] -= gd[1] * dv[1];
-      v[2] -= gd[2] * dv[2];
-      v[3] -= 0.5 * (gd[0] * dv[1] + gd[1] * dv[0]);
-      v[4] -= 0.5 * (gd[0] * dv[2] + gd[2] * dv[0]);
-      v[5] -= 0.5 * (gd[1] * dv[2] + gd[2] * dv[1]);
-    }
-    gp_out[slot * 3 + 0] = g0;
-    gp_out[slot * 3 + 1] = g1;
-    gp_out[slot * 3 + 2] = g2;
-  }
-#pragma unroll
-  for (int k = 0; k < 6; k++) {
-    const double t = sub_sum<LW>(v[k]);
-    if (lane == 0) vsite[(size_t)site * 6 + k] = t;
-  }
-}
-
-// Sites per workgroup: as many waves as fit in 64 KB of LDS (at least one).
-static int waves_per_block(size_t per_wave_bytes) {
-  int w = 4;
-  while (w > 1 && (size_t)w * per_wave_bytes > 65536) w >>= 1;
-  return w;
-}
-
-template <typename K>
-static hipError_t ensure_lds(K kernel, size_t sm, size_t* set) {
-  if (sm <= *set) return hipSuccess;
-  if (sm > 160 * 1024) return hipErrorInvalidValue;
-  hipError_t e = hipFuncSetAttribute((const void*)kernel,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-  if (e != hipSuccess) {
-    (void)hipGetLastError();  // do not leave the failure as the sticky error
-    return e;
-  }
-  *set = sm;
-  return hipSuccess;
-}
-
-template <typename T, int NRP, int NMX, int LMX>
-static hipError_t fwd_inst(const SoapDev& sdh, const SoapDev* sd, const double* W,
-                           const float* rtab, const double* pos, const int* species,
-                           const Cell& cell, int maxnb, const int* count, const int* list,
-                           const int* site_atom, int n_sites, const int* dmap, T* q, int q_ld,
-                           u16* qb, int qb_ld, T* cbuf, T* pnorm, hipStream_t st) {
-  constexpr int SPW = Spw<NRP>::v;
-  const size_t per_wave = SPW * sizeof(T) * fwd_site_elems<T, NRP>(sdh.L2);
-  const int wpb = waves_per_block(per_wave);
-  const size_t sm = wpb * per_wave;
-  static size_t sm_set = 65536;
-  hipError_t e = ensure_lds(soap_fwd_kernel<T, NRP, NMX, LMX>, sm, &sm_set);
-  if (e != hipSuccess) return e;
-  const int per_block = wpb * SPW;
-  hipLaunchKernelGGL((soap_fwd_kernel<T, NRP, NMX, LMX>), dim3((n_sites + per_block - 1) / per_block),
-                     dim3(64 * wpb), sm, st, sd, W, reinterpret_cast<const float2*>(rtab), pos,
-                     species, cell, maxnb, count, list, site_atom, n_sites, dmap, q, q_ld, qb,
-                     qb_ld, cbuf, pnorm, wpb);
-  return hipGetLastError();
-}
-
-template <typename T, int NRP, int NMX, int LMX>
-static hipError_t bwd_inst(const SoapDev& sdh, const SoapDev* sd, const double* W,
-                           const float* rtab, const double* pos, const int* species,
-                           const Cell& cell, int maxnb, const int* count, const int* list,
-                           const int* site_atom, int n_sites, const int* dmap, const T* q,
-                           int q_ld, const T* cbuf, const T* pnorm, const T* G, int g_ld,
-                           T* gpair, double* vsite, hipStream_t st) {
-  constexpr int SPW = Spw<NRP>::v;
-  const size_t per_wave = SPW * sizeof(T) * bwd_site_elems<NRP>(sdh.L2, sdh.l_max + 1);
-  const int wpb = waves_per_block(per_wave);
-  const size_t sm = wpb * per_wave;
-  static size_t sm_set = 65536;
-  hipError_t e = ensure_lds(soap_bwd_kernel<T, NRP, NMX, LMX>, sm, &sm_set);
-  if (e != hipSuccess) return e;
-  const int per_block = wpb * SPW;
-  hipLaunchKernelGGL((soap_bwd_kernel<T, NRP, NMX, LMX>), dim3((n_sites + per_block - 1) / per_block),
-                     dim3(64 * wpb), sm, st, sd, W, reinterpret_cast<const float2*>(rtab), pos,
-                     species, cell, maxnb, count, list, site_atom, n_sites, dmap, q, q_ld, cbuf,
-                     pnorm, G, g_ld, gpair, vsite, wpb);
-  return hipGetLastError();
-}
-
-// Compile-time shape of a model: NRP = padded radial channels (SoapDev::nrp),
-// NMX = per-species n_max bound (8 or 16), LMX = l_max bound (8 or 10).
-#define TGX_SOAP_DISPATCH(FN, ...)                                                      \
-  do {                                                                                 \
-    const int nrp = sdh.nrp, nmx = sdh.nmx;                                            \
-    const bool l8 = sdh.l_max <= 8;                                                    \
-    if (nrp == 8) return l8 ? FN<T, 8, 8, 8>(__VA_ARGS__) : FN<T, 8, 8, 10>(__VA_ARGS__); \
-    if (nrp == 16 && nmx == 8) return l8 ? FN<T, 16, 8, 8>(__VA_ARGS__) : FN<T, 16, 8, 10>(__VA_ARGS__); \
-    if (nrp == 16) return l8 ? FN<T, 16, 16, 8>(__VA_ARGS__) : FN<T, 16, 16, 10>(__VA_ARGS__); \
-    if (nrp == 24 && nmx == 8) return l8 ? FN<T, 24, 8, 8>(__VA_ARGS__) : FN<T, 24, 8, 10>(__VA_ARGS__); \
-    if (nrp == 32 && nmx == 8) return l8 ? FN<T, 32, 8, 8>(__VA_ARGS__) : FN<T, 32, 8, 10>(__VA_ARGS__); \
-    if (nrp == 24) return l8 ? FN<T, 24, 16, 8>(__VA_ARGS__) : FN<T, 24, 16, 10>(__VA_ARGS__); \
-    if (nrp == 32) return l8 ? FN<T, 32, 16, 8>(__VA_ARGS__) : FN<T, 32, 16, 10>(__VA_ARGS__); \
-    return hipErrorInvalidValue;                                                       \
-  } while (0)
+hipError_t launch_soap_fwd_many(const SoapDev& sdh, const SoapDev* sd, const double* W,
+                                const float* rtab, const double* pos, const int* species,
+                                const CellMany& g, int maxnb, const int* count, const int* list,
+                                const int* site_atom, int n_sites, const int* dmap, T* q,
+                                int q_ld, u16* qb, int qb_ld, T* cbuf, T* pnorm, hipStream_t st);
+template <typename T>
+hipError_t launch_soap_bwd_many(const SoapDev& sdh, const SoapDev* sd, const double* W,
+                                const float* rtab, const double* pos, const int* species,
+                                const CellMany& g, int maxnb, const int* count, const int* list,
+                                const int* site_atom, int n_sites, const int* dmap, const T* q,
+                                int q_ld, const T* cbuf, const T* pnorm, const T* G, int g_ld,
+                                T* gpair, double* vsite, hipStream_t st);
 
 template <typename T>
 hipError_t launch_soap_fwd(const SoapDev& sdh, const SoapDev* sd, const double* W,
                            const float* rtab, const double* pos, const int* species,
-                           const Cell& cell, int maxnb, const int* count, const int* list,
+                           const Geom& g, int maxnb, const int* count, const int* list,
                            const int* site_atom, int n_sites, const int* dmap, T* q, int q_ld,
                            u16* qb, int qb_ld, T* cbuf, T* pnorm, hipStream_t st) {
-  if (n_sites == 0) return hipSuccess;
-  TGX_SOAP_DISPATCH(fwd_inst, sdh, sd, W, rtab, pos, species, cell, maxnb, count, list,
-                    site_atom, n_sites, dmap, q, q_ld, qb, qb_ld, cbuf, pnorm, st);
+  if (g.many)
+    return launch_soap_fwd_many<T>(sdh, sd, W, rtab, pos, species, CellMany{g.many, g.sid},
+                                   maxnb, count, list, site_atom, n_sites, dmap, q, q_ld, qb,
+                                   qb_ld, cbuf, pnorm, st);
+  return launch_soap_fwd_g<T, CellOne>(sdh, sd, W, rtab, pos, species, CellOne{g.one}, maxnb,
+                                       count, list, site_atom, n_sites, dmap, q, q_ld, qb, qb_ld,
+                                       cbuf, pnorm, st);
 }
 
 template <typename T>
 hipError_t launch_soap_bwd(const SoapDev& sdh, const SoapDev* sd, const double* W,
                            const float* rtab, const double* pos, const int* species,
-                           const Cell& cell, int maxnb, const int* count, const int* list,
+                           const Geom& g, int maxnb, const int* count, const int* list,
                            const int* site_atom, int n_sites, const int* dmap, const T* q,
                            int q_ld, const T* cbuf, const T* pnorm, const T* G, int g_ld,
                            T* gpair, double* vsite, hipStream_t st) {
-  if (n_sites == 0) return hipSuccess;
-  TGX_SOAP_DISPATCH(bwd_inst, sdh, sd, W, rtab, pos, species, cell, maxnb, count, list,
-                    site_atom, n_sites, dmap, q, q_ld, cbuf, pnorm, G, g_ld, gpair, vsite, st);
+  if (g.many)
+    return launch_soap_bwd_many<T>(sdh, sd, W, rtab, pos, species, CellMany{g.many, g.sid},
+                                   maxnb, count, list, site_atom, n_sites, dmap, q, q_ld, cbuf,
+                                   pnorm, G, g_ld, gpair, vsite, st);
+  return launch_soap_bwd_g<T, CellOne>(sdh, sd, W, rtab, pos, species, CellOne{g.one}, maxnb,
+                                       count, list, site_atom, n_sites, dmap, q, q_ld, cbuf,
+                                       pnorm, G, g_ld, gpair, vsite, st);
 }
 
-template hipError_t launch_soap_fwd<float>(const SoapDev&, const SoapDev*, const double*, const float*, const double*, const int*, const Cell&, int, const int*, const int*, const int*, int, const int*, float*, int, u16*, int, float*, float*, hipStream_t);
-template hipError_t launch_soap_fwd<double>(const SoapDev&, const SoapDev*, const double*, const float*, const double*, const int*, const Cell&, int, const int*, const int*, const int*, int, const int*, double*, int, u16*, int, double*, double*, hipStream_t);
-template hipError_t launch_soap_bwd<float>(const SoapDev&, const SoapDev*, const double*, const float*, const double*, const int*, const Cell&, int, const int*, const int*, const int*, int, const int*, const float*, int, const float*, const float*, const float*, int, float*, double*, hipStream_t);
-template hipError_t launch_soap_bwd<double>(const SoapDev&, const SoapDev*, const double*, const float*, const double*, const int*, const Cell&, int, const int*, const int*, const int*, int, const int*, const double*, int, const double*, const double*, const double*, int, double*, double*, hipStream_t);
+template hipError_t launch_soap_fwd<float>(const SoapDev&, const SoapDev*, const double*, const float*, const double*, const int*, const Geom&, int, const int*, const int*, const int*, int, const int*, float*, int, u16*, int, float*, float*, hipStream_t);
+template hipError_t launch_soap_fwd<double>(const SoapDev&, const SoapDev*, const double*, const float*, const double*, const int*, const Geom&, int, const int*, const int*, const int*, int, const int*, double*, int, u16*, int, double*, double*, hipStream_t);
+template hipError_t launch_soap_bwd<float>(const SoapDev&, const SoapDev*, const double*, const float*, const double*, const int*, const Geom&, int, const int*, const int*, const int*, int, const int*, const float*, int, const float*, const float*, const float*, int, float*, double*, hipStream_t);
+template hipError_t launch_soap_bwd<double>(const SoapDev&, const SoapDev*, const double*, const float*, const double*, const int*, const Geom&, int, const int*, const int*, const int*, int, const int*, const double*, int, const double*, const double*, const double*, int, double*, double*, hipStream_t);
 
 }  // namespace tgx
