@@ -156,6 +156,9 @@ def main():
     # Dominant kernel: single-kernel device times (HIP events on the library
     # stream, bracketing only that kernel) averaged over instrumented E+F calls.
     eng.set_precision(args.precision)
+    # lists with the MD skin, so that each kernel sees the MD step's pair
+    # slots and the profile's per-kernel averages cover one workload
+    eng.set_list_buffer(0.5)
     kt = {"soap_fwd": 0.0, "gemm": 0.0, "soap_bwd": 0.0, "force_gather": 0.0}
     n_inst = 5
     t_host = 0.0
@@ -168,6 +171,7 @@ def main():
     # host buffers in and out (PCIe, list rebuilt from scratch every call)
     host_rate = n * n_inst / t_host
     st = eng.stage_times()
+    eng.set_list_buffer(0.0)
     nb = eng.neighbors(pos, lat, model.hypers.rcut_max)
     n_pairs = int(np.sum(nb["rjs"] > 0))
     fwd_flop, bwd_flop = soap_costs(model.hypers, n, n_pairs)
@@ -192,14 +196,18 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
 
-        t0 = time.perf_counter()
-        e_o, f_o, v_o = oracle.energy_forces(model, pos, sp, lat, chunk_sites=2048)
-        t_cpu = time.perf_counter() - t0
+        # repeat full FP64 E+F evaluations of the whole cell for >= 10 s of CPU work
+        reps, t_cpu = 0, 0.0
+        while reps == 0 or (t_cpu < 10.0 and reps < 20):
+            t0 = time.perf_counter()
+            e_o, f_o, v_o = oracle.energy_forces(model, pos, sp, lat, chunk_sites=2048)
+            t_cpu += time.perf_counter() - t0
+            reps += 1
         cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        cpu = {"value": round(n / t_cpu, 2), "unit": "atom-steps/s", "cores": cores,
+        cpu = {"value": round(n * reps / t_cpu, 2), "unit": "atom-steps/s", "cores": cores,
                "kind": "port",
-               "sample": f"one FP64 E+F of the full {n}-atom cell by the oracle "
-                         f"(linked cells, SOAP with materialised dq/dr, kernel GEMMs, "
+               "sample": f"{reps} FP64 E+F evaluations of the full {n}-atom cell by the "
+                         f"oracle (linked cells, SOAP with materialised dq/dr, kernel GEMMs, "
                          f"force loop), {t_cpu:.1f} s"}
         mae = {"force_mae_eV_A": float(np.abs(f_m - f_o).mean()),
                "force_max_eV_A": float(np.abs(f_m - f_o).max()),

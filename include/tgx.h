@@ -196,6 +196,29 @@ int tgx_md_run(tgx_ctx* ctx, int n, int n_steps, double dt, double skin,
                double* pos, double* vel, double* forces, double* e_pot,
                double* e_kin, int* n_rebuilds);
 
+/*
+ * Metropolis Monte Carlo with single-atom moves, energies only (replaces the
+ * "move" branch of calculate_mc_step, mc/mc_interface.f90:183-447, with
+ * mc_get_atom_disp mc/mc_utils.f90:259-277 and the acceptance of
+ * get_mc_acceptance / monte_carlo_move :172-180, 215-257).  Trial t
+ * (0-based) draws u_k = u(seed, t, k), k = 0..4, where u is the top 53 bits
+ * of the splitmix64 finaliser of seed + 0x9E3779B97F4A7C15 * (5t + k + 1)
+ * divided by 2^53; it moves atom floor(u_0 n) by move_max (u_1..u_3 - 1/2)
+ * (positions + disp, mc_utils.f90:696-698) and accepts when
+ * min(1, exp(-(E_trial - E)/(kB T))) > u_4, kB = 8.617333262e-5 rounded to
+ * single precision as the reference's literal is.  Each trial sees the state
+ * left by the previous one; the energy change is computed from the sites
+ * whose environment changed (the reference recomputes the whole system).
+ * The neighbour list carries a skin (> sqrt(3) move_max) and is rebuilt when
+ * an atom leaves it; the box must be at least 2 (rcut + skin).  pos (3,n) is
+ * updated in place.  e_trace (n_trials + 1, may be NULL): energy before the
+ * first trial and after every trial.  n_accepted / n_rebuilds may be NULL.
+ */
+int tgx_mc_run(tgx_ctx* ctx, int n, const int* species, const double* lattice,
+               double* pos, int n_trials, double temp, double move_max,
+               double skin, uint64_t seed, double* e_trace, int* n_accepted,
+               int* n_rebuilds);
+
 /* Overlap S(alpha_max,alpha_max) and orthonormalisation W = S^(-1/2) of the
  * poly3 radial basis, both column-major (symmetric); replaces
  * get_orthonormalization_matrix_poly3 (gpu/gpu_interface.f90:424-429).
@@ -226,6 +249,12 @@ int tgx_md_end(tgx_ctx* ctx);
 
 /* Wait for all work enqueued on the context's stream. */
 int tgx_sync(tgx_ctx* ctx);
+
+/* Neighbour-list buffer of tgx_energy_forces and tgx_energy_forces_batched:
+ * lists are built to the largest model cutoff + buffer (neighbors%buffer,
+ * utils/misc.f90:79-81; default 0).  Results do not depend on it: pairs
+ * beyond a term's cutoff contribute nothing. */
+int tgx_set_list_buffer(tgx_ctx* ctx, double buffer);
 
 /* Per-stage timing of the last call, milliseconds (HIP events):
  * [0] neighbours, [1] SOAP forward, [2] kernel GEMMs, [3] SOAP backward and

@@ -277,3 +277,30 @@ def mc_accept(e_new, e_prev, temp):
     fn = lib().orc_mc_accept
     fn.restype = C.c_double
     return fn(C.c_double(e_new), C.c_double(e_prev), C.c_double(temp))
+
+
+def mc_uniform(seed, t, k):
+    fn = lib().orc_mc_uniform
+    fn.restype = C.c_double
+    fn.argtypes = [C.c_uint64, C.c_int64, C.c_int]
+    return fn(seed, t, k)
+
+
+def mc_run(model: GapModel, pos, species, lattice, n_trials, temp, move_max, seed):
+    """Sequential single-atom-move Metropolis MC with a full-system energy per
+    trial (mc_interface.f90:183-447).  Returns final positions, the energy
+    trace (n_trials + 1) and the accepted count."""
+    pos = _f64(pos).copy()
+    lat = _f64(lattice)
+    sp = np.ascontiguousarray(species, np.int32)
+    cm, keep = _cmodel(model)
+    tr = np.zeros(n_trials + 1)
+    acc = C.c_int(0)
+    fn = lib().orc_mc_run
+    fn.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
+                   C.c_double, C.c_uint64, C.c_void_p, C.c_void_p]
+    rc = fn(C.addressof(cm), len(pos), pos.ctypes.data, sp.ctypes.data, lat.ctypes.data,
+            n_trials, temp, move_max, seed, tr.ctypes.data, C.addressof(acc))
+    if rc != 0:
+        raise RuntimeError(f"orc_mc_run failed {rc}")
+    return pos, tr, acc.value

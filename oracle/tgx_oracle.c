@@ -34,6 +34,14 @@
  *       gap/gap.f90:329-460.  Pinned by the dimer closed form and FD.
  *   orc_velocity_verlet   md/md_utils.f90:486-532 (harmonic dimer test).
  *   orc_mc_accept         mc/mc_utils.f90:172-180, 215-257.
+ *   orc_mc_uniform        the trial stream of tgx_mc_run (splitmix64 of
+ *       seed + golden * (5t + k + 1)); pinned by splitmix64's published
+ *       outputs for state 0.
+ *   orc_mc_run            mc/mc_interface.f90:183-447 with single-atom
+ *       moves (mc_utils.f90:259-277, 696-698): one full-system energy per
+ *       trial, accept when min(1, p) > u.  Pinned by its limits (T -> inf
+ *       accepts every move, T -> 0 only downhill ones) and by recomputing
+ *       the final energy.
  */
 #include "tgx_oracle.h"
 
@@ -457,11 +465,10 @@ static void radial_envelope(const tgx_soap_hypers* h, int s, double r,
   *de = g * (damp * fc + amp * dfc);
 }
 
-void orc_radial(const tgx_soap_hypers* h, int s, int nr, const double* r,
-                double* R, double* dR) {
+/* R_n, dR_n/dr with a given W (alpha_max x alpha_max, row n). */
+static void radial_W(const tgx_soap_hypers* h, int s, const double* W, int nr,
+                     const double* r, double* R, double* dR) {
   int nmax = h->alpha_max[s];
-  double S[TGX_MAX_NR * TGX_MAX_NR], W[TGX_MAX_NR * TGX_MAX_NR];
-  orc_poly3_overlap(nmax, S, W);
   for (int q = 0; q < nr; q++) {
     double I[TGX_MAX_NR], dI[TGX_MAX_NR], e, de;
     double rr = r[q];
@@ -482,6 +489,13 @@ void orc_radial(const tgx_soap_hypers* h, int s, int nr, const double* r,
       dR[q * nmax + n] = de * v + e * dv;
     }
   }
+}
+
+void orc_radial(const tgx_soap_hypers* h, int s, int nr, const double* r,
+                double* R, double* dR) {
+  double S[TGX_MAX_NR * TGX_MAX_NR], W[TGX_MAX_NR * TGX_MAX_NR];
+  orc_poly3_overlap(h->alpha_max[s], S, W);
+  radial_W(h, s, W, nr, r, R, dR);
 }
 
 /* Central-atom radial coefficients of species s at r = 0. */
@@ -610,7 +624,7 @@ static void soap_site(const tgx_soap_hypers* h, const soap_geom* g, int si,
     const double* v = x + 3 * j;
     double r = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
     int s = sj[j];
-    orc_radial(h, s, 1, &r, Rv + (size_t)j * TGX_MAX_NR, dRv + (size_t)j * TGX_MAX_NR);
+    radial_W(h, s, g->W[s], 1, &r, Rv + (size_t)j * TGX_MAX_NR, dRv + (size_t)j * TGX_MAX_NR);
     double u[3] = {v[0] / r, v[1] / r, v[2] / r};
     orc_sph(L, u, Yv + (size_t)j * L2, dFv + (size_t)j * L2 * 3);
     double st = h->atom_sigma_t[s] + h->atom_sigma_t_scaling[s] * r;
@@ -1115,4 +1129,59 @@ double orc_mc_accept(double e_new, double e_prev, double temp) {
   const double kB = (double)8.617333262e-5f;
   double p = exp(-(e_new - e_prev) / (kB * temp));
   return p < 1.0 ? p : 1.0;
+}
+
+/* u(seed, t, k) in [0, 1): the top 53 bits of the splitmix64 finaliser
+ * applied to seed + 0x9E3779B97F4A7C15 * (5t + k + 1). */
+double orc_mc_uniform(uint64_t seed, int64_t t, int k) {
+  uint64_t z = seed + 0x9E3779B97F4A7C15ull * (uint64_t)(5 * t + k + 1);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+static double orc_total_energy(const orc_model* m, int n, const double* pos,
+                               const int* species, const double* lat,
+                               double* e_atom, int* rc) {
+  double e = 0.0;
+  *rc = orc_energy_forces(m, n, pos, species, lat, 0, 0, e_atom, NULL, NULL);
+  for (int i = 0; i < n; i++) e += e_atom[i];
+  return e;
+}
+
+/* Trial t: atom floor(u0 n), disp = move_max (u1..u3 - 1/2), positions +
+ * disp (mc_utils.f90:267-273, 696-698); E of the whole trial structure;
+ * p = min(1, exp(-(E_trial - E_current)/(kB T))), accepted when p > u4
+ * (mc_interface.f90:245-299).  e_trace[0] = initial energy, e_trace[t+1]
+ * = current energy after trial t. */
+int orc_mc_run(const orc_model* m, int n, double* pos, const int* species,
+               const double* lat, int n_trials, double temp, double move_max,
+               uint64_t seed, double* e_trace, int* n_accepted) {
+  double* ea = (double*)malloc(sizeof(double) * (size_t)n);
+  int rc = 0, acc = 0;
+  double e_cur = orc_total_energy(m, n, pos, species, lat, ea, &rc);
+  if (e_trace) e_trace[0] = e_cur;
+  for (int t = 0; t < n_trials && rc == 0; t++) {
+    int idx = (int)floor(orc_mc_uniform(seed, t, 0) * n);
+    if (idx > n - 1) idx = n - 1;
+    double old[3], disp[3];
+    for (int k = 0; k < 3; k++) {
+      disp[k] = move_max * (orc_mc_uniform(seed, t, 1 + k) - 0.5);
+      old[k] = pos[3 * idx + k];
+      pos[3 * idx + k] = old[k] + disp[k];
+    }
+    const double e_new = orc_total_energy(m, n, pos, species, lat, ea, &rc);
+    const double p = orc_mc_accept(e_new, e_cur, temp);
+    if (p > orc_mc_uniform(seed, t, 4)) {
+      e_cur = e_new;
+      acc++;
+    } else {
+      for (int k = 0; k < 3; k++) pos[3 * idx + k] = old[k];
+    }
+    if (e_trace) e_trace[t + 1] = e_cur;
+  }
+  free(ea);
+  if (n_accepted) *n_accepted = acc;
+  return rc;
 }

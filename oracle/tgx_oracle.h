@@ -125,6 +125,15 @@ void orc_velocity_verlet(int n, double* pos, double* pos_prev, double* vel,
 /* Metropolis acceptance of mc_utils.f90:172-180,215-257 ("move"). */
 double orc_mc_accept(double e_new, double e_prev, double temp);
 
+/* Trial random stream of single-atom-move MC: u(seed, t, k) in [0, 1). */
+double orc_mc_uniform(uint64_t seed, int64_t t, int k);
+
+/* Sequential Metropolis MC with single-atom moves and one full-system energy
+ * per trial (mc/mc_interface.f90:183-447); pos updated in place. */
+int orc_mc_run(const orc_model* m, int n, double* pos, const int* species,
+               const double* lat, int n_trials, double temp, double move_max,
+               uint64_t seed, double* e_trace, int* n_accepted);
+
 #ifdef __cplusplus
 }
 #endif

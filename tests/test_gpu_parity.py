@@ -196,3 +196,20 @@ def test_bad_arguments_raise(eng):
     with pytest.raises(tgx.TgxError) as ei:
         eng.energy_forces(pos + [[0, 0, 0], [1.5, 0, 0]], np.array([1, 1], np.int32), tri)
     assert ei.value.code == tgx.TGX_ERR_UNSUPPORTED
+
+
+def test_list_buffer_does_not_change_results(eng):
+    """Lists built with a buffer hold extra pairs beyond the cutoff; they add
+    exact zeros, but shift which lane sums which pair, so results agree to
+    rounding of the lane reductions."""
+    pos, sp, lat, model = W.carbon216()
+    eng.set_model(model)
+    e0, f0, v0 = eng.energy_forces(pos, sp, lat)
+    eng.set_list_buffer(0.7)
+    try:
+        e1, f1, v1 = eng.energy_forces(pos, sp, lat)
+    finally:
+        eng.set_list_buffer(0.0)
+    assert np.abs(e0 - e1).max() <= 1e-13
+    assert np.abs(f0 - f1).max() <= 1e-12
+    assert np.abs(v0 - v1).max() <= 1e-12 * np.abs(v0).max()

@@ -483,3 +483,49 @@ def test_mc_acceptance_formula():
     for de in (-0.1, 0.0, 0.01, 0.05, 0.3):
         p = oracle.mc_accept(1.0 + de, 1.0, 1500.0)
         assert p == min(1.0, math.exp(-((1.0 + de) - 1.0) / (KB_SINGLE * 1500.0)))
+
+
+# ---------------------------------------------------------------------------
+# Monte Carlo trial stream and the sequential Metropolis restatement
+# ---------------------------------------------------------------------------
+
+def test_mc_uniform_is_splitmix64():
+    """u(0, 0, k) are the first outputs of splitmix64 from state 0 (published
+    values 0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F), top
+    53 bits over 2^53."""
+    for k, v in enumerate([0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]):
+        assert oracle.mc_uniform(0, 0, k) == (v >> 11) / 2.0 ** 53
+    # trial t continues the same counter: u(0, t, k) = output 5t + k + 1
+    assert oracle.mc_uniform(0, 1, 0) == oracle.mc_uniform(0, 0, 5)
+    u = np.array([oracle.mc_uniform(7, t, k) for t in range(2000) for k in range(5)])
+    assert 0.0 <= u.min() and u.max() < 1.0 and abs(u.mean() - 0.5) < 0.01
+
+
+def _mc_system(n=24, seed=11):
+    pos, sp, lat = S.carbon_cell(n, 2.0, 1.2, seed)
+    _, _, _, model = W.carbon216()
+    return pos, sp, lat, model
+
+
+def test_mc_hot_limit_accepts_everything():
+    pos, sp, lat, model = _mc_system()
+    p, tr, acc = oracle.mc_run(model, pos, sp, lat, 12, 1e12, 0.6, 5)
+    assert acc == 12
+    # every move applied: atom floor(u0 n) moved by move_max (u - 1/2)
+    q = pos.copy()
+    for t in range(12):
+        m = int(np.floor(oracle.mc_uniform(5, t, 0) * len(q)))
+        q[m] += [0.6 * (oracle.mc_uniform(5, t, 1 + k) - 0.5) for k in range(3)]
+    assert np.array_equal(p, q)
+    e, _, _ = oracle.energy_forces(model, p, sp, lat, do_forces=False)
+    assert abs(e.sum() - tr[-1]) < 1e-12
+
+
+def test_mc_cold_limit_only_downhill():
+    pos, sp, lat, model = _mc_system()
+    p, tr, acc = oracle.mc_run(model, pos, sp, lat, 20, 1e-9, 0.6, 9)
+    d = np.diff(tr)
+    assert np.all(d <= 0.0)
+    assert acc == int(np.sum(d < 0))
+    e, _, _ = oracle.energy_forces(model, p, sp, lat, do_forces=False)
+    assert abs(e.sum() - tr[-1]) < 1e-12

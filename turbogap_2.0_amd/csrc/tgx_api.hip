@@ -50,6 +50,19 @@ hipError_t launch_neighbors_many(const double* pos, int n, const Cell* cells, co
 hipError_t launch_reduce_many(int n_struct, const int* aoff, const int* site_of_atom,
                               const double* vsite, const double* vatom, const double* eatom,
                               double* out, hipStream_t st);
+hipError_t launch_mc_propose(int n, double* pos, const double* pref, unsigned long long seed,
+                             double move_max, double lim2, int* ctl, double* dctl,
+                             hipStream_t st);
+hipError_t launch_mc_affected(const int* ctl, const int* count, const int* list, int maxnb,
+                              const int* species, int central, int* aff, int cap, int* n_aff,
+                              hipStream_t st);
+hipError_t launch_mc_decide(int n, double* pos, const int* species, const Cell& cell, int maxnb,
+                            const int* count, const int* list, const int* aff, const int* n_aff,
+                            const double* esite, double* e_cur, const TwoBodyDev* twob, int n2b,
+                            unsigned long long seed, double kT, int* ctl, double* dctl,
+                            double* trace, hipStream_t st);
+hipError_t launch_mc_init(int n, const int* site_of_atom, const double* esite, double* e_cur,
+                          hipStream_t st);
 hipError_t launch_energy_init(int n, const int* species, const double* e0,
                               const int* site_of_atom, const double* esite, double* eatom,
                               double* forces, double* vatom, hipStream_t st);
@@ -127,6 +140,7 @@ struct MdState {
 struct Extra {
   DevBuf sd_dev, pairn, e0_dev, iota, cell_count, cell_fill, out, stage;
   DevBuf b_cells, b_sid, b_boff, b_aoff, b_out;  // batched structures
+  DevBuf mc_pref, mc_ctl, mc_dctl, mc_ecur, mc_naff, mc_trace, mc_twob;  // Monte Carlo
   DevBuf twob_store;
   std::vector<int> species_host;
   hipEvent_t ev[8];
@@ -311,7 +325,7 @@ static int run_ef(tgx_ctx* c, int n, const double* lattice, const int* species_h
   if (!c->has_soap && c->twob.empty()) return set_err(c, TGX_ERR_STATE, "no model loaded");
   Cell cell;
   std::string err;
-  const double rc = rcut_all(c);
+  const double rc = rcut_all(c) + c->list_buffer;
   int rcode = make_cell(lattice, rc, &cell, &err);
   if (rcode) return set_err(c, rcode, "%s", err.c_str());
   if ((long long)n * cell.nrep >= (1LL << TGX_IMG_SHIFT))
@@ -954,7 +968,7 @@ int tgx_energy_forces_batched(tgx_ctx* c, int n_struct, const int* offsets, cons
   int rcode = upload_structure(c, n, pos, species);
   if (rcode) return rcode;
   // per-structure cells, bin offsets and the structure of every atom
-  const double rc = rcut_all(c);
+  const double rc = rcut_all(c) + c->list_buffer;
   std::vector<Cell> cells(n_struct);
   std::vector<int> boff(n_struct + 1), sid(n);
   int maxnb = c->maxnb;
@@ -1034,6 +1048,195 @@ int tgx_energy_forces_batched(tgx_ctx* c, int n_struct, const int* offsets, cons
     for (int s = 0; s < n_struct; s++)
       for (int k = 0; k < 9; k++) virials[9 * (size_t)s + k] = o[10 * (size_t)s + k];
   }
+  return TGX_OK;
+}
+
+// ---- Monte Carlo -----------------------------------------------------------
+
+namespace {
+struct McRun {
+  int n, cap, central, n2b;
+  Cell cell;
+  double rcut_list, move_max, lim2, kT;
+  unsigned long long seed;
+};
+}  // namespace
+
+// One trial: propose, guarded list rebuild, affected sites, SOAP forward and
+// energy GEMM on them, decision.  Fixed launch shapes (capturable).
+static int mc_trial_launch(tgx_ctx* c, const McRun& r) {
+  Extra& x = X(c);
+  int* ctl = x.mc_ctl.as<int>();
+  double* dctl = x.mc_dctl.as<double>();
+  TGX_HIP(c, launch_mc_propose(r.n, c->pos.as<double>(), x.mc_pref.as<double>(), r.seed,
+                               r.move_max, r.lim2, ctl, dctl, c->stream));
+  TGX_HIP(c, launch_neighbors(c->pos.as<double>(), r.n, r.cell, r.rcut_list,
+                              c->cell_of.as<int>(), x.cell_count.as<int>(),
+                              c->cell_start.as<int>(), x.cell_fill.as<int>(),
+                              c->cell_atoms.as<int>(), c->nb_count.as<int>(), c->maxnb,
+                              c->nb_list.as<int>(), nullptr, c->nb_flag.as<int>(), c->stream,
+                              ctl + MC_REBUILD));
+  TGX_HIP(c, launch_md_ref_copy(r.n, c->pos.as<double>(), x.mc_pref.as<double>(),
+                                ctl + MC_REBUILD, c->stream));
+  if (c->has_soap) {
+    TGX_HIP(c, launch_mc_affected(ctl, c->nb_count.as<int>(), c->nb_list.as<int>(), c->maxnb,
+                                  c->species.as<int>(), r.central, c->site_atom.as<int>(), r.cap,
+                                  x.mc_naff.as<int>(), c->stream));
+    int rcode = soap_forward(c, r.n, Geom{r.cell}, r.cap);
+    if (rcode) return rcode;
+    rcode = kernel_step(c, r.cap, false);
+    if (rcode) return rcode;
+  }
+  TGX_HIP(c, launch_mc_decide(r.n, c->pos.as<double>(), c->species.as<int>(), r.cell, c->maxnb,
+                              c->nb_count.as<int>(), c->nb_list.as<int>(),
+                              c->site_atom.as<int>(), x.mc_naff.as<int>(),
+                              c->has_soap ? c->esite.as<double>() : nullptr,
+                              x.mc_ecur.as<double>(), x.mc_twob.as<TwoBodyDev>(), r.n2b, r.seed,
+                              r.kT, ctl, dctl, x.mc_trace.as<double>(), c->stream));
+  return TGX_OK;
+}
+
+int tgx_mc_run(tgx_ctx* c, int n, const int* species, const double* lattice, double* pos,
+               int n_trials, double temp, double move_max, double skin, uint64_t seed,
+               double* e_trace, int* n_accepted, int* n_rebuilds) {
+  if (!c) return TGX_ERR_ARG;
+  if (n < 2 || !species || !lattice || !pos || n_trials < 0 || !(temp > 0) ||
+      !(move_max > 0))
+    return set_err(c, TGX_ERR_ARG, "bad arguments");
+  if (!(skin > std::sqrt(3.0) * move_max))
+    return set_err(c, TGX_ERR_ARG, "skin must exceed sqrt(3) * move_max (%g)",
+                   std::sqrt(3.0) * move_max);
+  if (!c->has_soap && c->twob.empty()) return set_err(c, TGX_ERR_STATE, "no model loaded");
+  if (X(c).md.active) return set_err(c, TGX_ERR_STATE, "end the MD run first (tgx_md_end)");
+  (void)hipSetDevice(c->device);
+  Extra& x = X(c);
+  int rcode = upload_structure(c, n, pos, species);
+  if (rcode) return rcode;
+  McRun r{};
+  r.n = n;
+  r.move_max = move_max;
+  r.lim2 = 0.25 * skin * skin;
+  r.kT = (double)8.617333262e-5f * temp;  // the reference's default-real kB
+  r.seed = seed;
+  r.n2b = (int)c->twob.size();
+  r.central = c->has_soap ? c->sd.central_species : 0;
+  r.rcut_list = rcut_all(c) + skin;
+  std::string err;
+  rcode = make_cell(lattice, r.rcut_list, &r.cell, &err);
+  if (rcode) return set_err(c, rcode, "%s", err.c_str());
+  if (r.cell.small)
+    return set_err(c, TGX_ERR_UNSUPPORTED,
+                   "Monte Carlo needs a box of at least 2 (rcut + skin) = %g A",
+                   2 * r.rcut_list);
+  c->maxnb = 0;
+  rcode = build_list(c, n, r.cell, r.rcut_list, false);
+  if (rcode) return rcode;
+  {
+    std::vector<int> cnt(n);
+    TGX_HIP(c, hipMemcpy(cnt.data(), c->nb_count.p, 4 * (size_t)n, hipMemcpyDeviceToHost));
+    const int mx = *std::max_element(cnt.begin(), cnt.end());
+    const int want = ((mx * 3) / 2 + 16 + 7) / 8 * 8;
+    if (c->maxnb < want) {
+      c->maxnb = want;
+      rcode = build_list(c, n, r.cell, r.rcut_list, false);
+      if (rcode) return rcode;
+    }
+  }
+  r.cap = c->maxnb + 1;
+  if (r.cap > 1024) return set_err(c, TGX_ERR_UNSUPPORTED, "neighbour lists too long for MC");
+  int n_sites = 0;
+  rcode = setup_sites(c, n, species, &n_sites);
+  if (rcode) return rcode;
+  // energy of the initial state; per-atom SOAP site energies
+  x.ev_on = false;
+  rcode = ef_launch(c, n, n_sites, Geom{r.cell}, 0, false);
+  if (rcode) return rcode;
+  if (x.mc_pref.ensure(24 * (size_t)n) || x.mc_ctl.ensure(sizeof(int) * 16) ||
+      x.mc_dctl.ensure(sizeof(double) * MC_NDCTL) || x.mc_ecur.ensure(8 * (size_t)n) ||
+      x.mc_naff.ensure(sizeof(int) * 4) || x.mc_trace.ensure(8 * ((size_t)n_trials + 1)) ||
+      x.mc_twob.ensure(sizeof(TwoBodyDev) * std::max(1, r.n2b)))
+    return set_err(c, TGX_ERR_HIP, "device allocation failed (mc)");
+  ENSURE(c, site_atom, sizeof(int) * (size_t)std::max(n, r.cap));
+  TGX_HIP(c, launch_mc_init(n, c->site_of_atom.as<int>(),
+                            (c->has_soap && n_sites > 0) ? c->esite.as<double>() : nullptr,
+                            x.mc_ecur.as<double>(), c->stream));
+  if (r.n2b)
+    TGX_HIP(c, hipMemcpyAsync(x.mc_twob.p, c->twob.data(), sizeof(TwoBodyDev) * r.n2b,
+                              hipMemcpyHostToDevice, c->stream));
+  TGX_HIP(c, hipMemcpyAsync(x.mc_pref.p, c->pos.p, 24 * (size_t)n, hipMemcpyDeviceToDevice,
+                            c->stream));
+  double e0;
+  TGX_HIP(c, hipMemcpyAsync(&e0, x.out.as<double>() + 9, 8, hipMemcpyDeviceToHost, c->stream));
+  TGX_HIP(c, hipMemsetAsync(x.mc_ctl.p, 0, sizeof(int) * 16, c->stream));
+  TGX_HIP(c, hipMemsetAsync(c->nb_flag.p, 0, sizeof(int) * 4, c->stream));
+  TGX_HIP(c, hipMemsetAsync(x.mc_naff.p, 0, sizeof(int) * 4, c->stream));
+  TGX_HIP(c, hipMemsetAsync(c->site_atom.p, 0, sizeof(int) * (size_t)r.cap, c->stream));
+  TGX_HIP(c, hipStreamSynchronize(c->stream));
+  // a dry trial (MC_END = 0: every control kernel returns at once) sizes the
+  // SOAP and GEMM buffers before anything is captured; no timing events
+  x.ev_on = false;
+  rcode = mc_trial_launch(c, r);
+  if (rcode) return rcode;
+  std::vector<double> d(MC_NDCTL, 0.0);
+  d[MC_ETOT] = e0;
+  int ctl_h[16] = {0};
+  ctl_h[MC_END] = n_trials;
+  TGX_HIP(c, hipMemcpyAsync(x.mc_dctl.p, d.data(), sizeof(double) * MC_NDCTL,
+                            hipMemcpyHostToDevice, c->stream));
+  TGX_HIP(c, hipMemcpyAsync(x.mc_ctl.p, ctl_h, sizeof(ctl_h), hipMemcpyHostToDevice, c->stream));
+  TGX_HIP(c, hipMemcpyAsync(x.mc_trace.p, &e0, 8, hipMemcpyHostToDevice, c->stream));
+  TGX_HIP(c, hipStreamSynchronize(c->stream));
+  const int TB = std::max(1, std::min(64, n_trials));
+  const char* ng = getenv("TGX_NO_GRAPH");
+  const bool use_graph = !(ng && ng[0] == '1') && n_trials > 0;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  if (use_graph) {
+    TGX_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int b = 0; b < TB && !rcode; b++) rcode = mc_trial_launch(c, r);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rcode || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      x.ev_on = true;
+      return rcode ? rcode
+                   : set_err(c, TGX_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      x.ev_on = true;
+      return set_err(c, TGX_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+    }
+    for (int t = 0; t < n_trials; t += TB) {
+      hipError_t le = hipGraphLaunch(ge, c->stream);
+      if (le != hipSuccess) {
+        rcode = set_err(c, TGX_ERR_HIP, "graph launch failed: %s", hipGetErrorString(le));
+        break;
+      }
+    }
+  } else {
+    for (int t = 0; t < n_trials && !rcode; t++) rcode = mc_trial_launch(c, r);
+  }
+  hipError_t se = hipStreamSynchronize(c->stream);
+  if (ge) (void)hipGraphExecDestroy(ge);
+  if (g) (void)hipGraphDestroy(g);
+  x.ev_on = true;
+  if (rcode) return rcode;
+  if (se != hipSuccess) return set_err(c, TGX_ERR_HIP, "%s", hipGetErrorString(se));
+  int flag[4];
+  TGX_HIP(c, hipMemcpy(flag, c->nb_flag.p, sizeof(flag), hipMemcpyDeviceToHost));
+  if (flag[0] > c->maxnb)
+    return set_err(c, TGX_ERR_CAPACITY, "neighbour capacity exceeded during the run (%d > %d)",
+                   flag[0], c->maxnb);
+  TGX_HIP(c, hipMemcpy(ctl_h, x.mc_ctl.p, sizeof(ctl_h), hipMemcpyDeviceToHost));
+  TGX_HIP(c, hipMemcpy(pos, c->pos.p, 24 * (size_t)n, hipMemcpyDeviceToHost));
+  if (e_trace)
+    TGX_HIP(c, hipMemcpy(e_trace, x.mc_trace.p, 8 * ((size_t)n_trials + 1),
+                         hipMemcpyDeviceToHost));
+  if (n_accepted) *n_accepted = ctl_h[MC_NACC];
+  if (n_rebuilds) *n_rebuilds = ctl_h[MC_NREB];
+  if (ctl_h[MC_STEP] != n_trials)
+    return set_err(c, TGX_ERR_STATE, "ran %d of %d trials", ctl_h[MC_STEP], n_trials);
   return TGX_OK;
 }
 
@@ -1262,6 +1465,14 @@ int tgx_poly3_orthonormalization(int alpha_max, double* S, double* W) {
 int tgx_stage_times(const tgx_ctx* c, double* ms7) {
   if (!c || !ms7) return TGX_ERR_ARG;
   for (int k = 0; k < 7; k++) ms7[k] = c->stage_ms[k];
+  return TGX_OK;
+}
+
+int tgx_set_list_buffer(tgx_ctx* c, double buffer) {
+  if (!c) return TGX_ERR_ARG;
+  if (!(buffer >= 0.0) || buffer > 10.0) return set_err(c, TGX_ERR_ARG, "buffer out of range");
+  c->list_buffer = buffer;
+  c->maxnb = 0;  // re-sized at the next build
   return TGX_OK;
 }
 

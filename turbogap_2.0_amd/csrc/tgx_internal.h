@@ -67,6 +67,10 @@ struct TwoBodyDev {
   int nt;
 };
 
+// Monte Carlo control words on the device (int and double arrays).
+enum { MC_REBUILD = 0, MC_STEP, MC_END, MC_LIVE, MC_ATOM, MC_NREB, MC_NACC, MC_NCTL };
+enum { MC_OLD = 0, MC_ETOT = 3, MC_NDCTL = 8 };
+
 // Growable device buffer.
 struct DevBuf {
   void* p = nullptr;
@@ -92,6 +96,7 @@ struct tgx_ctx {
   int dim = 0, n_sparse = 0, dim_pad = 0, sp_pad = 0;
   double delta = 1.0, zeta = 2.0;
   double e0[TGX_MAX_SPECIES] = {0};
+  double list_buffer = 0.0;  // list cutoff = largest model cutoff + list_buffer
   std::vector<double> Qs_host;      // n_sparse x dim
   std::vector<double> alphas_host;  // n_sparse
   std::vector<tgx::TwoBodyDev> twob;

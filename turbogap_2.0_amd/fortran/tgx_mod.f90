@@ -44,7 +44,7 @@ module tgx_mod
        tgx_soap_descriptors, tgx_gap_kernel_step, tgx_energy_forces, &
        tgx_energy_forces_batched, tgx_md_run, tgx_md_begin, tgx_md_steps, tgx_md_read, &
        tgx_md_end, tgx_sync, tgx_poly3_orthonormalization, tgx_stage_times, tgx_check, &
-       tgx_error_string
+       tgx_error_string, tgx_kernel_times, tgx_mc_run, tgx_set_list_buffer
 
   interface
      integer(c_int) function tgx_create(device, precision, ctx) bind(C, name="tgx_create")
@@ -211,6 +211,34 @@ module tgx_mod
        type(c_ptr), value :: ctx
        real(c_double) :: ms7(7)
      end function tgx_stage_times
+
+     integer(c_int) function tgx_set_list_buffer(ctx, buffer) bind(C, name="tgx_set_list_buffer")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: buffer
+     end function tgx_set_list_buffer
+
+     integer(c_int) function tgx_kernel_times(ctx, ms4) bind(C, name="tgx_kernel_times")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double) :: ms4(4)
+     end function tgx_kernel_times
+
+     ! Single-atom-move Metropolis MC (calculate_mc_step "move",
+     ! mc/mc_interface.f90:183-447); pos(3,n) updated in place.
+     integer(c_int) function tgx_mc_run(ctx, n, species, lattice, pos, n_trials, temp, &
+          move_max, skin, seed, e_trace, n_accepted, n_rebuilds) bind(C, name="tgx_mc_run")
+       import :: c_int, c_ptr, c_double, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: n, n_trials
+       integer(c_int), intent(in) :: species(*)
+       real(c_double), intent(in) :: lattice(9)
+       real(c_double), intent(inout) :: pos(*)
+       real(c_double), value :: temp, move_max, skin
+       integer(c_int64_t), value :: seed
+       real(c_double) :: e_trace(*)
+       integer(c_int) :: n_accepted, n_rebuilds
+     end function tgx_mc_run
 
      integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
        import :: c_size_t, c_ptr

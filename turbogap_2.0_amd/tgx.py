@@ -219,6 +219,8 @@ def load_library(path: str = LIB_PATH):
         "tgx_md_run": (i, [P, i, i, d, d, ip, dp, dp, dp, dp, dp, dp, dp, ip]),
         "tgx_stage_times": (i, [P, dp]),
         "tgx_kernel_times": (i, [P, dp]),
+        "tgx_set_list_buffer": (i, [P, d]),
+        "tgx_mc_run": (i, [P, i, ip, dp, dp, i, d, d, d, C.c_uint64, dp, ip, ip]),
         "tgx_md_begin": (i, [P, i, ip, dp, dp, dp, dp, d, d, i]),
         "tgx_md_steps": (i, [P, i]),
         "tgx_md_read": (i, [P, dp, dp, dp, dp, dp, ip, ip]),
@@ -381,6 +383,20 @@ class Engine:
                                                        _dptr(v)))
         return offs, e, f, np.transpose(v, (0, 2, 1)).copy()
 
+    def mc_run(self, pos, species, lattice, n_trials: int, temp: float, move_max: float,
+               skin: float, seed: int):
+        """Single-atom-move Metropolis MC (energies only).  Returns dict with
+        the final positions, the energy trace (n_trials + 1), accepted and
+        list-rebuild counts."""
+        pos = _f64(pos).copy()
+        sp, lat = _i32(species), _f64(lattice)
+        tr = np.zeros(n_trials + 1)
+        acc, nreb = C.c_int(0), C.c_int(0)
+        self._check(self.lib.tgx_mc_run(self.ctx, len(pos), _iptr(sp), _dptr(lat), _dptr(pos),
+                                        n_trials, temp, move_max, skin, seed, _dptr(tr),
+                                        C.byref(acc), C.byref(nreb)))
+        return dict(pos=pos, e_trace=tr, n_accepted=acc.value, n_rebuilds=nreb.value)
+
     def md_run(self, pos, vel, species, lattice, masses_amu, n_steps: int, dt: float,
                skin: float):
         pos, vel = _f64(pos).copy(), _f64(vel).copy()
@@ -429,6 +445,9 @@ class Engine:
         self._check(self.lib.tgx_stage_times(self.ctx, _dptr(out)))
         return dict(zip(["neighbors", "soap_fwd", "gemm", "soap_bwd", "two_body", "integrator",
                          "total"], out))
+
+    def set_list_buffer(self, buffer: float):
+        self._check(self.lib.tgx_set_list_buffer(self.ctx, float(buffer)))
 
     def kernel_times(self):
         """Event-bracketed device time (ms) of single kernels in the last
