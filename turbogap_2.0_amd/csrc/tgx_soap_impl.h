@@ -108,14 +108,54 @@ template <int NRP> struct Spw { static constexpr int v = NRP <= 16 ? 2 : 1; };
 template <typename T, int SPW> struct Chunk { static constexpr int CH = SPW == 2 ? 24 : 32; };
 template <int SPW> struct Chunk<double, SPW> { static constexpr int CH = 16; };
 
+// Compacted valid-pair slots per site (two scan rounds of the site's lanes).
+template <int NRP> struct Capv { static constexpr int v = 2 * (64 / Spw<NRP>::v); };
+
+// Per-site LDS footprint in bytes (16-byte multiple): forward = harmonics
+// and radial values of a chunk of pairs (the first part reused for c) plus
+// the compacted slot list; adjoint = dE/dc [L2][NRP], the coupling matrices
+// M_l [Lp][NRP][NRP] and the slot list.
 template <typename T, int NRP>
-__host__ __device__ constexpr size_t fwd_site_elems(int L2) {
+__host__ __device__ constexpr size_t fwd_site_bytes(int L2) {
   constexpr int CH = Chunk<T, Spw<NRP>::v>::CH;
-  return (size_t)(CH > NRP ? CH : NRP) * L2 + (size_t)CH * NRP;
+  return ((sizeof(T) * ((size_t)(CH > NRP ? CH : NRP) * L2 + (size_t)CH * NRP) +
+           4 * Capv<NRP>::v) + 15) / 16 * 16;
 }
-template <int NRP>
-__host__ __device__ constexpr size_t bwd_site_elems(int L2, int Lp) {
-  return (size_t)2 * NRP * L2 + (size_t)Lp * NRP * NRP;
+template <typename T, int NRP>
+__host__ __device__ constexpr size_t bwd_site_bytes(int L2, int Lp) {
+  return ((sizeof(T) * ((size_t)NRP * L2 + (size_t)Lp * NRP * NRP) + 4 * Capv<NRP>::v) + 15) /
+         16 * 16;
+}
+
+// Appends, in slot order, the slots whose pair lies within the neighbour
+// species' cutoff to idx[], scanning the list from *next in steps of LW
+// slots while at least LW entries are free.  Returns the count appended;
+// the same on every lane of the site.  on_skip(slot) runs for slots that
+// are scanned but not appended.
+template <int LW, int SPW, typename GEO, typename F>
+__device__ __forceinline__ int scan_pairs(const SoapDev* sd, const double* pos,
+                                          const int* species, const Cell& cell, const int* row,
+                                          int i, int cnt, int* next, int* idx, int capv,
+                                          int sub, int lane, F on_skip) {
+  int nv = 0;
+  while (*next < cnt && nv <= capv - LW) {
+    const int slot = *next + lane;
+    bool valid = false;
+    if (slot < cnt) {
+      const int e = row[slot];
+      double dv[3];
+      const double rd = pair_vec_fast(pos, i, e, cell, dv);
+      valid = rd < sd->rh[species[e & TGX_ATOM_MASK] - 1];
+      if (!valid) on_skip(slot);
+    }
+    unsigned long long mask = __ballot(valid);
+    if (SPW == 2) mask = (mask >> (sub * 32)) & 0xffffffffull;
+    const int jc = __popcll(mask & ((1ull << lane) - 1ull));
+    if (valid) idx[nv + jc] = slot;
+    nv += __popcll(mask);
+    *next += LW;
+  }
+  return nv;
 }
 
 // Sum over the LW lanes of a sub-wave.
@@ -154,9 +194,10 @@ __global__ __launch_bounds__(256) void soap_fwd_kernel(
   const int L = sd->l_max, L2 = sd->L2;
   const int wave = threadIdx.x >> 6, lane64 = threadIdx.x & 63;
   const int sub = lane64 / LW, lane = lane64 % LW;
-  T* sAY = reinterpret_cast<T*>(smem_raw) +
-           (size_t)(wave * SPW + sub) * fwd_site_elems<T, NRP>(L2);
+  constexpr int CAPV = Capv<NRP>::v;
+  T* sAY = reinterpret_cast<T*>(smem_raw + (size_t)(wave * SPW + sub) * fwd_site_bytes<T, NRP>(L2));
   T* sR = sAY + (size_t)(CH > NRP ? CH : NRP) * L2;
+  int* sIdx = reinterpret_cast<int*>(sR + CH * NRP);
   T* sC = sAY;  // reused once the accumulation is done
   const int site = (blockIdx.x * wpb + wave) * SPW + sub;
   if (site >= n_sites) return;
@@ -164,6 +205,7 @@ __global__ __launch_bounds__(256) void soap_fwd_kernel(
   const Cell cell = geo.of(i);
   const int si = species[i] - 1;
   const int cnt = min(count[i], maxnb);
+  const int* row = list + (size_t)i * maxnb;
 
   T acc[NS][NRP];
 #pragma unroll
@@ -171,24 +213,20 @@ __global__ __launch_bounds__(256) void soap_fwd_kernel(
 #pragma unroll
     for (int n = 0; n < NRP; n++) acc[k][n] = (T)0;
 
-  for (int base = 0; base < cnt; base += LW) {
-    const int slot = base + lane;
-    double dv[3] = {0, 0, 0};
-    double rd = 1e30;
-    int s = 0;
-    if (slot < cnt) {
-      const int e = list[(size_t)i * maxnb + slot];
-      rd = pair_vec_fast(pos, i, e, cell, dv);
-      s = species[e & TGX_ATOM_MASK] - 1;
-    }
-    const bool valid = slot < cnt && rd < sd->rh[s];
-    unsigned long long mask = __ballot(valid);
-    if (SPW == 2) mask = (mask >> (sub * 32)) & 0xffffffffull;
-    const int jc = __popcll(mask & ((1ull << lane) - 1ull));
-    const int npv = __popcll(mask);
-    for (int c0 = 0; c0 < npv; c0 += CH) {
-      if (valid && jc >= c0 && jc < c0 + CH) {
-        const int jj = jc - c0;
+  int next = 0;
+  while (next < cnt) {
+    // valid pairs first, so that the expensive basis evaluation runs on
+    // packed lanes (list slots beyond the cutoff are skipped)
+    const int nv = scan_pairs<LW, SPW, GEO>(sd, pos, species, cell, row, i, cnt, &next, sIdx,
+                                            CAPV, sub, lane, [](int) {});
+    wave_sync();
+    for (int c0 = 0; c0 < nv; c0 += CH) {
+      if (lane < CH && c0 + lane < nv) {
+        const int jj = lane;
+        const int e = row[sIdx[c0 + jj]];
+        double dv[3];
+        const double rd = pair_vec_fast(pos, i, e, cell, dv);
+        const int s = species[e & TGX_ATOM_MASK] - 1;
         const T r = (T)rd;
         const T ir = (T)1 / r;
         const T ux = (T)dv[0] * ir, uy = (T)dv[1] * ir, uz = (T)dv[2] * ir;
@@ -232,7 +270,7 @@ __global__ __launch_bounds__(256) void soap_fwd_kernel(
         }
       }
       wave_sync();
-      const int nj = min(CH, npv - c0);
+      const int nj = min(CH, nv - c0);
       for (int j = 0; j < nj; j++) {
         T y[NS];
 #pragma unroll
@@ -312,16 +350,18 @@ __global__ __launch_bounds__(256) void soap_bwd_kernel(
   const int L = sd->l_max, L2 = sd->L2, dim = sd->dim, Lp = L + 1;
   const int wave = threadIdx.x >> 6, lane64 = threadIdx.x & 63;
   const int sub = lane64 / LW, lane = lane64 % LW;
-  T* sC = reinterpret_cast<T*>(smem_raw) + (size_t)(wave * SPW + sub) * bwd_site_elems<NRP>(L2, Lp);
-  T* sGC = sC + NRP * L2;  // [L2][NRP]
-  T* sM = sGC + NRP * L2;  // [Lp][NRP][NRP]: dE/dc = M_l c
+  constexpr int CAPV = Capv<NRP>::v;
+  T* sGC = reinterpret_cast<T*>(smem_raw + (size_t)(wave * SPW + sub) * bwd_site_bytes<T, NRP>(L2, Lp));
+  // sGC [L2][NRP] = dE/dc; sM [Lp][NRP][NRP]: dE/dc = M_l c
+  T* sM = sGC + NRP * L2;
+  int* sIdx = reinterpret_cast<int*>(sM + Lp * NRP * NRP);
   const int site = (blockIdx.x * wpb + wave) * SPW + sub;
   if (site >= n_sites) return;
   const int i = site_atom[site];
   const Cell cell = geo.of(i);
   const int cnt = min(count[i], maxnb);
+  const int* row = list + (size_t)i * maxnb;
   const T* cs = cbuf + (size_t)site * NRP * L2;
-  for (int k = lane; k < NRP * L2; k += LW) sC[k] = cs[k];
   for (int k = lane; k < Lp * NRP * NRP; k += LW) sM[k] = (T)0;
   const T* qs = q + (size_t)site * q_ld;
   const T* gs = G + (size_t)site * g_ld;
@@ -351,7 +391,7 @@ __global__ __launch_bounds__(256) void soap_bwd_kernel(
     const T* Ml = sM + l * NRP * NRP;
     T cl[NRP];
 #pragma unroll
-    for (int n2 = 0; n2 < NRP; n2++) cl[n2] = sC[n2 * L2 + lm];
+    for (int n2 = 0; n2 < NRP; n2++) cl[n2] = cs[n2 * L2 + lm];
 #pragma unroll
     for (int n = 0; n < NRP; n++) {
       T a = 0;
@@ -363,15 +403,25 @@ __global__ __launch_bounds__(256) void soap_bwd_kernel(
   wave_sync();
   double v[6] = {0, 0, 0, 0, 0, 0};
   T* gp_out = gpair + (size_t)i * maxnb * 3;
-  for (int base = 0; base < cnt; base += LW) {
-    const int slot = base + lane;
-    if (slot >= cnt) continue;
-    const int e = list[(size_t)i * maxnb + slot];
-    double dv[3];
-    const double rd = pair_vec_fast(pos, i, e, cell, dv);
-    const int s = species[e & TGX_ATOM_MASK] - 1;
-    T g0 = 0, g1 = 0, g2 = 0;
-    if (rd < sd->rh[s]) {
+  int next = 0;
+  while (next < cnt) {
+    // valid pairs are packed onto lanes; skipped slots get a zero gradient
+    const int nv = scan_pairs<LW, SPW, GEO>(sd, pos, species, cell, row, i, cnt, &next, sIdx,
+                                            CAPV, sub, lane, [&](int slot) {
+                                              gp_out[slot * 3 + 0] = (T)0;
+                                              gp_out[slot * 3 + 1] = (T)0;
+                                              gp_out[slot * 3 + 2] = (T)0;
+                                            });
+    wave_sync();
+    for (int c0 = 0; c0 < nv; c0 += LW) {
+      if (c0 + lane >= nv) continue;
+      const int slot = sIdx[c0 + lane];
+      const int e = row[slot];
+      double dv[3];
+      const double rd = pair_vec_fast(pos, i, e, cell, dv);
+      const int s = species[e & TGX_ATOM_MASK] - 1;
+      T g0 = 0, g1 = 0, g2 = 0;
+      {
       const T r = (T)rd;
       const T ir = (T)1 / r;
       const T ux = (T)dv[0] * ir, uy = (T)dv[1] * ir, uz = (T)dv[2] * ir;
@@ -457,9 +507,11 @@ __global__ __launch_bounds__(256) void soap_bwd_kernel(
       v[4] -= 0.5 * (gd[0] * dv[2] + gd[2] * dv[0]);
       v[5] -= 0.5 * (gd[1] * dv[2] + gd[2] * dv[1]);
     }
-    gp_out[slot * 3 + 0] = g0;
-    gp_out[slot * 3 + 1] = g1;
-    gp_out[slot * 3 + 2] = g2;
+      gp_out[slot * 3 + 0] = g0;
+      gp_out[slot * 3 + 1] = g1;
+      gp_out[slot * 3 + 2] = g2;
+    }
+    wave_sync();  // the slot list is refilled by the next scan round
   }
 #pragma unroll
   for (int k = 0; k < 6; k++) {
@@ -496,7 +548,7 @@ static hipError_t fwd_inst(const SoapDev& sdh, const SoapDev* sd, const double* 
                            const int* site_atom, int n_sites, const int* dmap, T* q, int q_ld,
                            u16* qb, int qb_ld, T* cbuf, T* pnorm, hipStream_t st) {
   constexpr int SPW = Spw<NRP>::v;
-  const size_t per_wave = SPW * sizeof(T) * fwd_site_elems<T, NRP>(sdh.L2);
+  const size_t per_wave = SPW * fwd_site_bytes<T, NRP>(sdh.L2);
   const int wpb = waves_per_block(per_wave);
   const size_t sm = wpb * per_wave;
   static size_t sm_set = 65536;
@@ -518,7 +570,7 @@ static hipError_t bwd_inst(const SoapDev& sdh, const SoapDev* sd, const double* 
                            int q_ld, const T* cbuf, const T* pnorm, const T* G, int g_ld,
                            T* gpair, double* vsite, hipStream_t st) {
   constexpr int SPW = Spw<NRP>::v;
-  const size_t per_wave = SPW * sizeof(T) * bwd_site_elems<NRP>(sdh.L2, sdh.l_max + 1);
+  const size_t per_wave = SPW * bwd_site_bytes<T, NRP>(sdh.L2, sdh.l_max + 1);
   const int wpb = waves_per_block(per_wave);
   const size_t sm = wpb * per_wave;
   static size_t sm_set = 65536;
