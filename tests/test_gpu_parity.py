@@ -213,3 +213,14 @@ def test_list_buffer_does_not_change_results(eng):
     assert np.abs(e0 - e1).max() <= 1e-13
     assert np.abs(f0 - f1).max() <= 1e-12
     assert np.abs(v0 - v1).max() <= 1e-12 * np.abs(v0).max()
+
+
+@pytest.mark.parametrize("rcut2b", [3.2, 4.6])
+def test_two_body_fused_and_separate(eng, rcut2b):
+    """The SOAP adjoint adds the 2-body terms itself when their cutoff lies
+    within the SOAP cutoff (3.2 here); a longer 2-body cutoff (4.6) takes the
+    separate pair kernel.  Both against the oracle in FP64."""
+    pos, sp, lat, model = W.carbon216()
+    m = tgx.GapModel(model.hypers, model.Qs, model.alphas, model.delta, model.zeta,
+                     two_body=[S.carbon_2b(1000, rcut=rcut2b)])
+    _ef_check(eng, m, pos, sp, lat, 1e-8, 1e-6, 1e-6)

@@ -37,7 +37,8 @@ hipError_t launch_soap_bwd(const SoapDev& sdh, const SoapDev* sd, const double* 
                            const Geom& g, int maxnb, const int* count, const int* list,
                            const int* site_atom, int n_sites, const int* dmap, const T* q,
                            int q_ld, const T* cbuf, const T* pnorm, const T* G, int g_ld,
-                           T* gpair, double* vsite, hipStream_t st);
+                           T* gpair, double* vsite, const TwoBodyDev* twob, int n2b,
+                           double* eatom, hipStream_t st);
 hipError_t launch_two_body(const TwoBodyDev& b, const double* pos, const int* species, int n,
                            const Geom& g, int maxnb, const int* count, const int* list,
                            double* eatom, double* forces, double* vatom, int do_forces,
@@ -343,6 +344,18 @@ static int run_ef(tgx_ctx* c, int n, const double* lattice, const int* species_h
   return ef_launch(c, n, n_sites, Geom{cell}, do_forces, true);
 }
 
+// The SOAP adjoint adds the distance_2b terms itself (one pass over the
+// pairs instead of two) when every atom is a SOAP site and every 2b block's
+// cutoff lies within the SOAP cutoff of both its species.
+static bool fuse_two_body(const tgx_ctx* c, int n, int n_sites, int do_forces) {
+  if (!do_forces || !c->has_soap || c->twob.empty() || n_sites != n) return false;
+  for (const auto& b : c->twob) {
+    if (b.sp1 > c->sd.n_species || b.sp2 > c->sd.n_species) return false;
+    if (b.rcut > c->sd.rh[b.sp1 - 1] || b.rcut > c->sd.rh[b.sp2 - 1]) return false;
+  }
+  return true;
+}
+
 // Energy/force launch sequence on the current list and sites.  Asynchronous:
 // no host synchronisation, no allocation once the buffers are sized (the MD
 // step graph captures it).
@@ -374,6 +387,7 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
                                 (c->has_soap && n_sites > 0) ? c->esite.as<double>() : nullptr,
                                 c->eatom.as<double>(), c->forces.as<double>(),
                                 c->virial.as<double>(), c->stream));
+  const bool fuse = fuse_two_body(c, n, n_sites, do_forces);
   if (do_forces && c->has_soap && n_sites > 0) {
     const size_t ts = tsize(prec);
     ENSURE(c, gpair, ts * 3 * (size_t)n * c->maxnb);
@@ -386,7 +400,10 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
                                   c->site_atom.as<int>(), n_sites, x.pairn.as<int>(),
                                   c->q.as<double>(), c->dim_pad, c->cbuf.as<double>(),
                                   c->pnorm.as<double>(), c->gmat.as<double>(), c->dim_pad,
-                                  c->gpair.as<double>(), c->vsite.as<double>(), c->stream);
+                                  c->gpair.as<double>(), c->vsite.as<double>(),
+                                  fuse ? x.twob_store.as<TwoBodyDev>() : nullptr,
+                                  fuse ? (int)c->twob.size() : 0, c->eatom.as<double>(),
+                                  c->stream);
       kev_rec(c, 5);
       kev_rec(c, 6);
       if (e == hipSuccess)
@@ -400,7 +417,10 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
                                  c->site_atom.as<int>(), n_sites, x.pairn.as<int>(),
                                  c->q.as<float>(), c->dim_pad, c->cbuf.as<float>(),
                                  c->pnorm.as<float>(), c->gmat.as<float>(), c->dim_pad,
-                                 c->gpair.as<float>(), c->vsite.as<double>(), c->stream);
+                                 c->gpair.as<float>(), c->vsite.as<double>(),
+                                 fuse ? x.twob_store.as<TwoBodyDev>() : nullptr,
+                                 fuse ? (int)c->twob.size() : 0, c->eatom.as<double>(),
+                                 c->stream);
       kev_rec(c, 5);
       kev_rec(c, 6);
       if (e == hipSuccess)
@@ -413,7 +433,8 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
   }
   ev_rec(c, 4);
   for (const auto& b : c->twob)
-    TGX_HIP(c, launch_two_body(b, c->pos.as<double>(), c->species.as<int>(), n, cell, c->maxnb,
+    if (!fuse)
+      TGX_HIP(c, launch_two_body(b, c->pos.as<double>(), c->species.as<int>(), n, cell, c->maxnb,
                                c->nb_count.as<int>(), c->nb_list.as<int>(), c->eatom.as<double>(),
                                c->forces.as<double>(), c->virial.as<double>(), do_forces,
                                prec == TGX_FP64, c->stream));
@@ -518,16 +539,20 @@ int tgx_destroy(tgx_ctx* c) {
                     &c->csr_out, &c->md_vel, &c->md_fprev, &c->md_mass, &c->md_pos0,
                     &c->md_ep, &c->md_ek, &c->md_reb, &c->md_disp, &f->x.sd_dev, &f->x.pairn,
                     &f->x.e0_dev, &f->x.iota, &f->x.cell_count, &f->x.cell_fill, &f->x.out,
-                    &f->x.stage, &f->x.twob_store};
+                    &f->x.stage, &f->x.twob_store, &f->x.b_cells, &f->x.b_sid,
+                    &f->x.b_boff, &f->x.b_aoff, &f->x.b_out, &f->x.mc_pref, &f->x.mc_ctl,
+                    &f->x.mc_dctl, &f->x.mc_ecur, &f->x.mc_naff, &f->x.mc_trace,
+                    &f->x.mc_twob};
   for (DevBuf* b : bufs) b->release();
   for (auto& b : c->twob) {
     if (b.Qs) (void)hipFree(b.Qs);
     if (b.alphas) (void)hipFree(b.alphas);
     if (b.tab) (void)hipFree(b.tab);
   }
-  if (f->x.ev_ready)
+  if (f->x.ev_ready) {
     for (int k = 0; k < 8; k++) (void)hipEventDestroy(f->x.ev[k]);
     for (int k = 0; k < 8; k++) (void)hipEventDestroy(f->x.kev[k]);
+  }
   (void)hipStreamDestroy(c->stream);
   delete f;
   return TGX_OK;
@@ -751,6 +776,12 @@ int tgx_add_2b_model(tgx_ctx* c, int sp1, int sp2, double rcut, double buffer, d
   c->twob.push_back(b);
   c->maxnb = 0;
   Extra& x = X(c);
+  // device copy of all blocks, read by the SOAP adjoint when it adds the
+  // pair terms itself
+  if (x.twob_store.ensure(sizeof(TwoBodyDev) * c->twob.size()))
+    return set_err(c, TGX_ERR_HIP, "allocation failed");
+  TGX_HIP(c, hipMemcpy(x.twob_store.p, c->twob.data(), sizeof(TwoBodyDev) * c->twob.size(),
+                       hipMemcpyHostToDevice));
   if (!x.e0_dev.p) {
     if (x.e0_dev.ensure(8 * TGX_MAX_SPECIES)) return set_err(c, TGX_ERR_HIP, "allocation failed");
     TGX_HIP(c, hipMemcpy(x.e0_dev.p, c->e0, 8 * TGX_MAX_SPECIES, hipMemcpyHostToDevice));

@@ -304,4 +304,61 @@ __device__ __forceinline__ void ang_params(const SoapDev* sd, int s, T r,
 __device__ __forceinline__ float exp2_t(float x) { return exp2f(x); }
 __device__ __forceinline__ double exp2_t(double x) { return exp2(x); }
 
+// distance_2b pair term (gap.f90:383-435): phi(r) = delta^2 fcut(r)
+// sum_s alpha_s c_s exp(-(r-Q_s)^2 / (2 sigma^2)) for one side of the pair and
+// dphi/dr.  EXACT: the reference sum over sparse points in FP64; otherwise the
+// smooth part from the FP32 cubic-Hermite table with the cosine cutoff applied
+// analytically.  Zero beyond rcut.
+template <int EXACT>
+__device__ __forceinline__ void two_body_pair(const TwoBodyDev& b, double r, double* phi,
+                                              double* dphi) {
+  const double pi = 3.14159265358979323846;
+  if (!(r < b.rcut)) {
+    *phi = *dphi = 0.0;
+    return;
+  }
+  if (EXACT) {
+    double fcut, dfcut;
+    if (r < b.rcut - b.buffer) {
+      fcut = 1.0;
+      dfcut = 0.0;
+    } else {
+      const double arg = pi * (r - b.rcut + b.buffer) / b.buffer;
+      fcut = (cos(arg) + 1.0) / 2.0;
+      dfcut = pi / 2.0 / b.buffer * sin(arg);
+    }
+    const double d2 = b.delta * b.delta, is2 = 1.0 / (b.sigma * b.sigma);
+    double es = 0, fs = 0;
+    for (int k = 0; k < b.n_sparse; k++) {
+      const double dr = r - b.Qs[k];
+      const double ex = exp(-0.5 * dr * dr * is2);
+      es += b.alphas[k] * ex;
+      fs += b.alphas[k] * ex * (dr * is2 * fcut + dfcut);
+    }
+    *phi = d2 * fcut * es;
+    *dphi = -d2 * fs;
+  } else {
+    const float rf = (float)r;
+    const float hinv = (float)b.nt / (float)b.rcut;
+    const float xg = rf * hinv;
+    int k = (int)xg;
+    k = k > b.nt - 1 ? b.nt - 1 : k;
+    const float t = xg - (float)k, t2 = t * t, t3 = t2 * t;
+    const float2 p0 = b.tab[k], p1 = b.tab[k + 1];
+    const float g = (2.f * t3 - 3.f * t2 + 1.f) * p0.x + (t3 - 2.f * t2 + t) * p0.y +
+                    (-2.f * t3 + 3.f * t2) * p1.x + (t3 - t2) * p1.y;
+    const float dg = ((6.f * t2 - 6.f * t) * p0.x + (3.f * t2 - 4.f * t + 1.f) * p0.y +
+                      (-6.f * t2 + 6.f * t) * p1.x + (3.f * t2 - 2.f * t) * p1.y) * hinv;
+    float fc = 1.f, dfc = 0.f;
+    const float r1 = (float)(b.rcut - b.buffer);
+    if (rf >= r1) {
+      const float arg = (float)pi * (rf - r1) / (float)b.buffer;
+      fc = 0.5f * (__cosf(arg) + 1.f);
+      dfc = -0.5f * (float)pi / (float)b.buffer * __sinf(arg);
+    }
+    *phi = (double)(fc * g);
+    *dphi = (double)(dfc * g + fc * dg);
+  }
+}
+
 }  // namespace tgx

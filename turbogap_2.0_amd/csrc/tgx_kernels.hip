@@ -525,7 +525,6 @@ __global__ __launch_bounds__(256) void two_body_kernel(TwoBodyDev b, const doubl
   const int lane = threadIdx.x & 63;
   const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (i >= n) return;
-  const double pi = 3.14159265358979323846;
   const int si = species[i];
   if (si != b.sp1 && si != b.sp2) return;
   const Cell c = geo.of(i);
@@ -539,48 +538,7 @@ __global__ __launch_bounds__(256) void two_body_kernel(TwoBodyDev b, const doubl
     const double r = pair_vec(pos, i, ent, c, x);
     if (!(r < b.rcut)) continue;
     double phi, dphi;  // pair energy (one side) and d phi/dr
-    if (EXACT) {
-      double fcut, dfcut;
-      if (r < b.rcut - b.buffer) {
-        fcut = 1.0;
-        dfcut = 0.0;
-      } else {
-        const double arg = pi * (r - b.rcut + b.buffer) / b.buffer;
-        fcut = (cos(arg) + 1.0) / 2.0;
-        dfcut = pi / 2.0 / b.buffer * sin(arg);
-      }
-      const double d2 = b.delta * b.delta, is2 = 1.0 / (b.sigma * b.sigma);
-      double es = 0, fs = 0;
-      for (int k = 0; k < b.n_sparse; k++) {
-        const double dr = r - b.Qs[k];
-        const double ex = exp(-0.5 * dr * dr * is2);
-        es += b.alphas[k] * ex;
-        fs += b.alphas[k] * ex * (dr * is2 * fcut + dfcut);
-      }
-      phi = d2 * fcut * es;
-      dphi = -d2 * fs;
-    } else {
-      const float rf = (float)r;
-      const float hinv = (float)b.nt / (float)b.rcut;
-      const float xg = rf * hinv;
-      int k = (int)xg;
-      k = k > b.nt - 1 ? b.nt - 1 : k;
-      const float t = xg - (float)k, t2 = t * t, t3 = t2 * t;
-      const float2 p0 = b.tab[k], p1 = b.tab[k + 1];
-      const float g = (2.f * t3 - 3.f * t2 + 1.f) * p0.x + (t3 - 2.f * t2 + t) * p0.y +
-                      (-2.f * t3 + 3.f * t2) * p1.x + (t3 - t2) * p1.y;
-      const float dg = ((6.f * t2 - 6.f * t) * p0.x + (3.f * t2 - 4.f * t + 1.f) * p0.y +
-                        (-6.f * t2 + 6.f * t) * p1.x + (3.f * t2 - 2.f * t) * p1.y) * hinv;
-      float fc = 1.f, dfc = 0.f;
-      const float r1 = (float)(b.rcut - b.buffer);
-      if (rf >= r1) {
-        const float arg = (float)pi * (rf - r1) / (float)b.buffer;
-        fc = 0.5f * (__cosf(arg) + 1.f);
-        dfc = -0.5f * (float)pi / (float)b.buffer * __sinf(arg);
-      }
-      phi = (double)(fc * g);
-      dphi = (double)(dfc * g + fc * dg);
-    }
+    two_body_pair<EXACT>(b, r, &phi, &dphi);
     e += phi;
     if (do_forces) {
       // reference: F_i += -2 delta^2 alpha c exp(.) x/r (...) = 2 phi'(r) x/r

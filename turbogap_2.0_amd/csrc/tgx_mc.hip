@@ -92,17 +92,9 @@ __global__ void mc_affected_kernel(const int* ctl, const int* count, const int* 
 
 // Exact FP64 pair energy of one distance_2b block (gap.f90:383-399).
 __device__ double mc_phi(const TwoBodyDev& b, double r) {
-  if (!(r < b.rcut)) return 0.0;
-  const double pi = 3.14159265358979323846;
-  double fcut = 1.0;
-  if (r >= b.rcut - b.buffer) fcut = (cos(pi * (r - b.rcut + b.buffer) / b.buffer) + 1.0) / 2.0;
-  const double is2 = 1.0 / (b.sigma * b.sigma);
-  double es = 0.0;
-  for (int k = 0; k < b.n_sparse; k++) {
-    const double dr = r - b.Qs[k];
-    es += b.alphas[k] * exp(-0.5 * dr * dr * is2);
-  }
-  return b.delta * b.delta * fcut * es;
+  double phi, dphi;
+  two_body_pair<1>(b, r, &phi, &dphi);
+  return phi;
 }
 
 __global__ __launch_bounds__(64) void mc_decide_kernel(

@@ -105,7 +105,7 @@ __device__ __forceinline__ double pair_vec_fast(const double* __restrict__ pos, 
 // radial channels, else one.  Pairs per LDS chunk and the per-site LDS
 // footprint (elements of T).
 template <int NRP> struct Spw { static constexpr int v = NRP <= 16 ? 2 : 1; };
-template <typename T, int SPW> struct Chunk { static constexpr int CH = SPW == 2 ? 24 : 32; };
+template <typename T, int SPW> struct Chunk { static constexpr int CH = SPW == 2 ? 12 : 32; };
 template <int SPW> struct Chunk<double, SPW> { static constexpr int CH = 16; };
 
 // Compacted valid-pair slots per site (two scan rounds of the site's lanes).
@@ -179,8 +179,13 @@ __device__ __forceinline__ void ang_factors(T log2t, T* Al) {
   }
 }
 
+// Minimum waves per SIMD the FP32 forward kernel is register-limited to
+// (measured best on gfx950 with 12-pair chunks); FP64 is left unconstrained.
+template <typename T> struct FwdWaves { static constexpr int v = 4; };
+template <> struct FwdWaves<double> { static constexpr int v = 1; };
+
 template <typename T, int NRP, int NMX, int LMX, typename GEO>
-__global__ __launch_bounds__(256) void soap_fwd_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FwdWaves<T>::v))) void soap_fwd_kernel(
     const SoapDev* __restrict__ sd, const double* __restrict__ W,
     const float2* __restrict__ rtab, const double* __restrict__ pos,
     const int* __restrict__ species, GEO geo, int maxnb, const int* __restrict__ count,
@@ -344,7 +349,8 @@ __global__ __launch_bounds__(256) void soap_bwd_kernel(
     const int* __restrict__ list, const int* __restrict__ site_atom, int n_sites,
     const int* __restrict__ dmap, const T* __restrict__ q, int q_ld,
     const T* __restrict__ cbuf, const T* __restrict__ pnorm, const T* __restrict__ G, int g_ld,
-    T* __restrict__ gpair, double* __restrict__ vsite, int wpb) {
+    T* __restrict__ gpair, double* __restrict__ vsite, const TwoBodyDev* __restrict__ twob,
+    int n2b, double* __restrict__ eatom, int wpb) {
   constexpr int SPW = Spw<NRP>::v, LW = 64 / SPW;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int L = sd->l_max, L2 = sd->L2, dim = sd->dim, Lp = L + 1;
@@ -402,6 +408,7 @@ __global__ __launch_bounds__(256) void soap_bwd_kernel(
   }
   wave_sync();
   double v[6] = {0, 0, 0, 0, 0, 0};
+  double e2b = 0.0;
   T* gp_out = gpair + (size_t)i * maxnb * 3;
   int next = 0;
   while (next < cnt) {
@@ -498,7 +505,28 @@ __global__ __launch_bounds__(256) void soap_bwd_kernel(
       g0 = s1 * ux + (Vx - uV * ux) * ir;
       g1 = s1 * uy + (Vy - uV * uy) * ir;
       g2 = s1 * uz + (Vz - uV * uz) * ir;
-      const double gd[3] = {(double)g0, (double)g1, (double)g2};
+      double gd[3] = {(double)g0, (double)g1, (double)g2};
+      if (n2b) {
+        // distance_2b blocks whose cutoff lies inside the SOAP cutoff: the
+        // pair term phi(r) enters E_i, its gradient phi'(r) r/|r| the slot
+        const int si1 = species[i], sj1 = s + 1;
+        double gsc = 0.0;
+        for (int b = 0; b < n2b; b++) {
+          const TwoBodyDev& B = twob[b];
+          if (!((si1 == B.sp1 && sj1 == B.sp2) || (si1 == B.sp2 && sj1 == B.sp1))) continue;
+          double phi, dphi;
+          two_body_pair<sizeof(T) == 8>(B, rd, &phi, &dphi);
+          e2b += phi;
+          gsc += dphi;
+        }
+        const double ird = 1.0 / rd;
+        gd[0] += gsc * dv[0] * ird;
+        gd[1] += gsc * dv[1] * ird;
+        gd[2] += gsc * dv[2] * ird;
+        g0 = (T)gd[0];
+        g1 = (T)gd[1];
+        g2 = (T)gd[2];
+      }
       // virial += 1/2 (F x + x F), F = -g the force on the neighbour
       v[0] -= gd[0] * dv[0];
       v[1] -= gd[1] * dv[1];
@@ -517,6 +545,10 @@ __global__ __launch_bounds__(256) void soap_bwd_kernel(
   for (int k = 0; k < 6; k++) {
     const double t = sub_sum<LW>(v[k]);
     if (lane == 0) vsite[(size_t)site * 6 + k] = t;
+  }
+  if (n2b) {
+    const double t = sub_sum<LW>(e2b);
+    if (lane == 0) eatom[i] += t;
   }
 }
 
@@ -568,7 +600,8 @@ static hipError_t bwd_inst(const SoapDev& sdh, const SoapDev* sd, const double* 
                            const GEO& cell, int maxnb, const int* count, const int* list,
                            const int* site_atom, int n_sites, const int* dmap, const T* q,
                            int q_ld, const T* cbuf, const T* pnorm, const T* G, int g_ld,
-                           T* gpair, double* vsite, hipStream_t st) {
+                           T* gpair, double* vsite, const TwoBodyDev* twob, int n2b,
+                           double* eatom, hipStream_t st) {
   constexpr int SPW = Spw<NRP>::v;
   const size_t per_wave = SPW * bwd_site_bytes<T, NRP>(sdh.L2, sdh.l_max + 1);
   const int wpb = waves_per_block(per_wave);
@@ -580,7 +613,7 @@ static hipError_t bwd_inst(const SoapDev& sdh, const SoapDev* sd, const double* 
   hipLaunchKernelGGL((soap_bwd_kernel<T, NRP, NMX, LMX, GEO>), dim3((n_sites + per_block - 1) / per_block),
                      dim3(64 * wpb), sm, st, sd, W, reinterpret_cast<const float2*>(rtab), pos,
                      species, cell, maxnb, count, list, site_atom, n_sites, dmap, q, q_ld, cbuf,
-                     pnorm, G, g_ld, gpair, vsite, wpb);
+                     pnorm, G, g_ld, gpair, vsite, twob, n2b, eatom, wpb);
   return hipGetLastError();
 }
 
@@ -617,10 +650,12 @@ hipError_t launch_soap_bwd_g(const SoapDev& sdh, const SoapDev* sd, const double
                            const GEO& cell, int maxnb, const int* count, const int* list,
                            const int* site_atom, int n_sites, const int* dmap, const T* q,
                            int q_ld, const T* cbuf, const T* pnorm, const T* G, int g_ld,
-                           T* gpair, double* vsite, hipStream_t st) {
+                           T* gpair, double* vsite, const TwoBodyDev* twob, int n2b,
+                           double* eatom, hipStream_t st) {
   if (n_sites == 0) return hipSuccess;
   TGX_SOAP_DISPATCH(bwd_inst, sdh, sd, W, rtab, pos, species, cell, maxnb, count, list,
-                    site_atom, n_sites, dmap, q, q_ld, cbuf, pnorm, G, g_ld, gpair, vsite, st);
+                    site_atom, n_sites, dmap, q, q_ld, cbuf, pnorm, G, g_ld, gpair, vsite, twob,
+                    n2b, eatom, st);
 }
 
 }  // namespace tgx

@@ -110,12 +110,12 @@ def cho_hypers() -> SoapHypers:
                               atom_sigma_r=0.5, atom_sigma_t=0.5)
 
 
-def carbon_2b(seed: int, n_sparse: int = 50) -> TwoBody:
+def carbon_2b(seed: int, n_sparse: int = 50, rcut: float = 3.7) -> TwoBody:
     """C-C distance_2b block: rcut 3.7, buffer 0.5, sigma 0.5, delta 1,
-    sparse distances uniform in [1.0, 3.7], cutoff 1, alphas ~ N(0, 0.01)."""
+    sparse distances uniform in [1.0, rcut], cutoff 1, alphas ~ N(0, 0.01)."""
     rng = np.random.default_rng(seed)
-    return TwoBody(sp1=1, sp2=1, rcut=3.7, delta=1.0, sigma=0.5,
-                   Qs=np.sort(rng.uniform(1.0, 3.7, n_sparse)),
+    return TwoBody(sp1=1, sp2=1, rcut=rcut, delta=1.0, sigma=0.5,
+                   Qs=np.sort(rng.uniform(1.0, rcut, n_sparse)),
                    alphas=rng.normal(0.0, 0.01, n_sparse),
                    cutoff=np.ones(n_sparse))
 
