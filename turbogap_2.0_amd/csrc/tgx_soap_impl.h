@@ -19,6 +19,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "tgx_device.h"
 #include "tgx_internal.h"
 
@@ -115,11 +117,19 @@ template <int NRP> struct Capv { static constexpr int v = 2 * (64 / Spw<NRP>::v)
 // and radial values of a chunk of pairs (the first part reused for c) plus
 // the compacted slot list; adjoint = dE/dc [L2][NRP], the coupling matrices
 // M_l [Lp][NRP][NRP] and the slot list.
+// Forward region A holds the harmonics of a chunk (CH x L2), later c with
+// rows padded to a multiple of four (NRP x L2P).
+template <typename T, int NRP>
+__host__ __device__ constexpr size_t fwd_region_a(int L2) {
+  constexpr int CH = Chunk<T, Spw<NRP>::v>::CH;
+  return (size_t)CH * L2 > (size_t)NRP * ((L2 + 3) & ~3) ? (size_t)CH * L2
+                                                         : (size_t)NRP * ((L2 + 3) & ~3);
+}
 template <typename T, int NRP>
 __host__ __device__ constexpr size_t fwd_site_bytes(int L2) {
   constexpr int CH = Chunk<T, Spw<NRP>::v>::CH;
-  return ((sizeof(T) * ((size_t)(CH > NRP ? CH : NRP) * L2 + (size_t)CH * NRP) +
-           4 * Capv<NRP>::v) + 15) / 16 * 16;
+  return ((sizeof(T) * (fwd_region_a<T, NRP>(L2) + (size_t)CH * NRP) + 4 * Capv<NRP>::v) + 15) /
+         16 * 16;
 }
 template <typename T, int NRP>
 __host__ __device__ constexpr size_t bwd_site_bytes(int L2, int Lp) {
@@ -184,7 +194,7 @@ __device__ __forceinline__ void ang_factors(T log2t, T* Al) {
 template <typename T> struct FwdWaves { static constexpr int v = 4; };
 template <> struct FwdWaves<double> { static constexpr int v = 1; };
 
-template <typename T, int NRP, int NMX, int LMX, typename GEO>
+template <typename T, int NRP, int NMX, int LMX, bool EXL, typename GEO>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FwdWaves<T>::v))) void soap_fwd_kernel(
     const SoapDev* __restrict__ sd, const double* __restrict__ W,
     const float2* __restrict__ rtab, const double* __restrict__ pos,
@@ -196,12 +206,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FwdWaves<T>
   constexpr int CH = Chunk<T, SPW>::CH;
   constexpr int NS = ((LMX + 1) * (LMX + 1) + LW - 1) / LW;  // lm slots per lane
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int L = sd->l_max, L2 = sd->L2;
+  // EXL: l_max equals the bound LMX, so L and L2 are compile-time constants
+  const int L = EXL ? LMX : sd->l_max, L2 = EXL ? (LMX + 1) * (LMX + 1) : sd->L2;
   const int wave = threadIdx.x >> 6, lane64 = threadIdx.x & 63;
   const int sub = lane64 / LW, lane = lane64 % LW;
   constexpr int CAPV = Capv<NRP>::v;
   T* sAY = reinterpret_cast<T*>(smem_raw + (size_t)(wave * SPW + sub) * fwd_site_bytes<T, NRP>(L2));
-  T* sR = sAY + (size_t)(CH > NRP ? CH : NRP) * L2;
+  T* sR = sAY + fwd_region_a<T, NRP>(L2);
   int* sIdx = reinterpret_cast<int*>(sR + CH * NRP);
   T* sC = sAY;  // reused once the accumulation is done
   const int site = (blockIdx.x * wpb + wave) * SPW + sub;
@@ -277,12 +288,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FwdWaves<T>
       wave_sync();
       const int nj = min(CH, nv - c0);
       for (int j = 0; j < nj; j++) {
+        // unconditional loads: a slot past L2 reads a valid entry and its
+        // accumulator is never stored
         T y[NS];
 #pragma unroll
-        for (int k = 0; k < NS; k++) {
-          const int lm = lane + k * LW;
-          y[k] = lm < L2 ? sAY[j * L2 + lm] : (T)0;
-        }
+        for (int k = 0; k < NS; k++) y[k] = sAY[j * L2 + min(lane + k * LW, L2 - 1)];
 #pragma unroll
         for (int n = 0; n < NRP; n++) {
           const T rv = sR[j * NRP + n];
@@ -302,6 +312,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FwdWaves<T>
         if (k == o + n) acc[0][k] += (T)sd->rself[si * TGX_MAX_NR + n];
     }
   }
+  // c to HBM for the adjoint ([n][lm]) and to LDS with rows padded to a
+  // multiple of four for the power spectrum
+  const int L2P = (L2 + 3) & ~3;
   T* cs = cbuf + (size_t)site * NRP * L2;
 #pragma unroll
   for (int k = 0; k < NS; k++) {
@@ -309,39 +322,60 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FwdWaves<T>
     if (lm < L2) {
 #pragma unroll
       for (int n = 0; n < NRP; n++) {
-        sC[n * L2 + lm] = acc[k][n];
+        sC[n * L2P + lm] = acc[k][n];
         cs[n * L2 + lm] = acc[k][n];
       }
     }
   }
   wave_sync();
-  const int dim = sd->dim;
+  // power spectrum, lanes = radial pairs (n1 <= n2): p_l = w sum_m
+  // c[n1][l,m] c[n2][l,m] for every l, w = 1 or sqrt 2 (no divergence across
+  // lanes); pass 1 the norm, pass 2 q = p / |p| written as T and BF16
+  const int dim = sd->dim, Lp = L + 1, npair = sd->npair;
   T* qs = q + (size_t)site * q_ld;
+  u16* qbs = qb ? qb + (size_t)site * qb_ld : nullptr;
   T ss = 0;
-  for (int d = lane; d < dim; d += LW) {
-    const int dm = dmap[d];
-    const int n1 = dm & 255, n2 = (dm >> 8) & 255, l = dm >> 16;
-    const T* ca = sC + n1 * L2 + l * l;
-    const T* cb = sC + n2 * L2 + l * l;
-    T a = 0;
-    for (int m = 0; m <= 2 * l; m++) a += ca[m] * cb[m];
-    const T p = (n1 == n2 ? (T)1 : (T)1.41421356237309504880) * a;
-    qs[d] = p;
-    ss += p * p;
+  for (int kp = lane; kp < npair; kp += LW) {
+    const int dm = dmap[kp * Lp];
+    const T* ca = sC + (dm & 255) * L2P;
+    const T* cb = sC + ((dm >> 8) & 255) * L2P;
+    const T w = (dm & 255) == ((dm >> 8) & 255) ? (T)1 : (T)1.41421356237309504880;
+#pragma unroll
+    for (int l = 0; l <= LMX; l++) {
+      if (l <= L) {
+        T a = 0;
+#pragma unroll
+        for (int m = 0; m <= 2 * l; m++) a += ca[l * l + m] * cb[l * l + m];
+        ss += (w * a) * (w * a);
+      }
+    }
   }
   ss = sub_sum<LW>(ss);
   const T inv = (T)1 / sqrt(ss);
-  for (int d = lane; d < q_ld; d += LW) {
-    const T v = d < dim ? qs[d] * inv : (T)0;
-    qs[d] = v;
-    if (qb) qb[(size_t)site * qb_ld + d] = f2bf((float)v);
+  for (int kp = lane; kp < npair; kp += LW) {
+    const int dm = dmap[kp * Lp];
+    const T* ca = sC + (dm & 255) * L2P;
+    const T* cb = sC + ((dm >> 8) & 255) * L2P;
+    const T w = ((dm & 255) == ((dm >> 8) & 255) ? (T)1 : (T)1.41421356237309504880) * inv;
+#pragma unroll
+    for (int l = 0; l <= LMX; l++) {
+      if (l <= L) {
+        T a = 0;
+#pragma unroll
+        for (int m = 0; m <= 2 * l; m++) a += ca[l * l + m] * cb[l * l + m];
+        const T v = w * a;
+        qs[kp * Lp + l] = v;
+        if (qbs) qbs[kp * Lp + l] = f2bf((float)v);
+      }
+    }
   }
-  if (qb)
-    for (int d = q_ld + lane; d < qb_ld; d += LW) qb[(size_t)site * qb_ld + d] = 0;
+  for (int d = dim + lane; d < q_ld; d += LW) qs[d] = (T)0;
+  if (qbs)
+    for (int d = dim + lane; d < qb_ld; d += LW) qbs[d] = 0;
   if (lane == 0) pnorm[site] = sqrt(ss);
 }
 
-template <typename T, int NRP, int NMX, int LMX, typename GEO>
+template <typename T, int NRP, int NMX, int LMX, bool EXL, typename GEO>
 __global__ __launch_bounds__(256) void soap_bwd_kernel(
     const SoapDev* __restrict__ sd, const double* __restrict__ W,
     const float2* __restrict__ rtab, const double* __restrict__ pos,
@@ -353,7 +387,8 @@ __global__ __launch_bounds__(256) void soap_bwd_kernel(
     int n2b, double* __restrict__ eatom, int wpb) {
   constexpr int SPW = Spw<NRP>::v, LW = 64 / SPW;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int L = sd->l_max, L2 = sd->L2, dim = sd->dim, Lp = L + 1;
+  const int L = EXL ? LMX : sd->l_max, L2 = EXL ? (LMX + 1) * (LMX + 1) : sd->L2;
+  const int dim = sd->dim, Lp = L + 1;
   const int wave = threadIdx.x >> 6, lane64 = threadIdx.x & 63;
   const int sub = lane64 / LW, lane = lane64 % LW;
   constexpr int CAPV = Capv<NRP>::v;
@@ -573,7 +608,7 @@ static hipError_t ensure_lds(K kernel, size_t sm, size_t* set) {
   return hipSuccess;
 }
 
-template <typename T, int NRP, int NMX, int LMX, typename GEO>
+template <typename T, int NRP, int NMX, int LMX, bool EXL, typename GEO>
 static hipError_t fwd_inst(const SoapDev& sdh, const SoapDev* sd, const double* W,
                            const float* rtab, const double* pos, const int* species,
                            const GEO& cell, int maxnb, const int* count, const int* list,
@@ -584,17 +619,17 @@ static hipError_t fwd_inst(const SoapDev& sdh, const SoapDev* sd, const double* 
   const int wpb = waves_per_block(per_wave);
   const size_t sm = wpb * per_wave;
   static size_t sm_set = 65536;
-  hipError_t e = ensure_lds(soap_fwd_kernel<T, NRP, NMX, LMX, GEO>, sm, &sm_set);
+  hipError_t e = ensure_lds(soap_fwd_kernel<T, NRP, NMX, LMX, EXL, GEO>, sm, &sm_set);
   if (e != hipSuccess) return e;
   const int per_block = wpb * SPW;
-  hipLaunchKernelGGL((soap_fwd_kernel<T, NRP, NMX, LMX, GEO>), dim3((n_sites + per_block - 1) / per_block),
+  hipLaunchKernelGGL((soap_fwd_kernel<T, NRP, NMX, LMX, EXL, GEO>), dim3((n_sites + per_block - 1) / per_block),
                      dim3(64 * wpb), sm, st, sd, W, reinterpret_cast<const float2*>(rtab), pos,
                      species, cell, maxnb, count, list, site_atom, n_sites, dmap, q, q_ld, qb,
                      qb_ld, cbuf, pnorm, wpb);
   return hipGetLastError();
 }
 
-template <typename T, int NRP, int NMX, int LMX, typename GEO>
+template <typename T, int NRP, int NMX, int LMX, bool EXL, typename GEO>
 static hipError_t bwd_inst(const SoapDev& sdh, const SoapDev* sd, const double* W,
                            const float* rtab, const double* pos, const int* species,
                            const GEO& cell, int maxnb, const int* count, const int* list,
@@ -607,10 +642,10 @@ static hipError_t bwd_inst(const SoapDev& sdh, const SoapDev* sd, const double* 
   const int wpb = waves_per_block(per_wave);
   const size_t sm = wpb * per_wave;
   static size_t sm_set = 65536;
-  hipError_t e = ensure_lds(soap_bwd_kernel<T, NRP, NMX, LMX, GEO>, sm, &sm_set);
+  hipError_t e = ensure_lds(soap_bwd_kernel<T, NRP, NMX, LMX, EXL, GEO>, sm, &sm_set);
   if (e != hipSuccess) return e;
   const int per_block = wpb * SPW;
-  hipLaunchKernelGGL((soap_bwd_kernel<T, NRP, NMX, LMX, GEO>), dim3((n_sites + per_block - 1) / per_block),
+  hipLaunchKernelGGL((soap_bwd_kernel<T, NRP, NMX, LMX, EXL, GEO>), dim3((n_sites + per_block - 1) / per_block),
                      dim3(64 * wpb), sm, st, sd, W, reinterpret_cast<const float2*>(rtab), pos,
                      species, cell, maxnb, count, list, site_atom, n_sites, dmap, q, q_ld, cbuf,
                      pnorm, G, g_ld, gpair, vsite, twob, n2b, eatom, wpb);
@@ -618,20 +653,27 @@ static hipError_t bwd_inst(const SoapDev& sdh, const SoapDev* sd, const double* 
 }
 
 // Compile-time shape of a model: NRP = padded radial channels (SoapDev::nrp),
-// NMX = per-species n_max bound (8 or 16), LMX = l_max bound (8 or 10).
-#define TGX_SOAP_DISPATCH(FN, ...)                                                      \
-  do {                                                                                 \
-    const int nrp = sdh.nrp, nmx = sdh.nmx;                                            \
-    const bool l8 = sdh.l_max <= 8;                                                    \
-    if (nrp == 8) return l8 ? FN<T, 8, 8, 8, GEO>(__VA_ARGS__) : FN<T, 8, 8, 10, GEO>(__VA_ARGS__); \
-    if (nrp == 16 && nmx == 8) return l8 ? FN<T, 16, 8, 8, GEO>(__VA_ARGS__) : FN<T, 16, 8, 10, GEO>(__VA_ARGS__); \
-    if (nrp == 16) return l8 ? FN<T, 16, 16, 8, GEO>(__VA_ARGS__) : FN<T, 16, 16, 10, GEO>(__VA_ARGS__); \
-    if (nrp == 24 && nmx == 8) return l8 ? FN<T, 24, 8, 8, GEO>(__VA_ARGS__) : FN<T, 24, 8, 10, GEO>(__VA_ARGS__); \
-    if (nrp == 32 && nmx == 8) return l8 ? FN<T, 32, 8, 8, GEO>(__VA_ARGS__) : FN<T, 32, 8, 10, GEO>(__VA_ARGS__); \
-    if (nrp == 24) return l8 ? FN<T, 24, 16, 8, GEO>(__VA_ARGS__) : FN<T, 24, 16, 10, GEO>(__VA_ARGS__); \
-    if (nrp == 32) return l8 ? FN<T, 32, 16, 8, GEO>(__VA_ARGS__) : FN<T, 32, 16, 10, GEO>(__VA_ARGS__); \
-    return hipErrorInvalidValue;                                                       \
-  } while (0)
+// NMX = per-species n_max bound (8 or 16), LMX = l_max bound (8 or 10), EXL =
+// l_max equals LMX.  f is called with the four as integral constants.
+template <int V> using IC = std::integral_constant<int, V>;
+template <typename F>
+static hipError_t soap_dispatch(const SoapDev& sdh, F&& f) {
+  const int nrp = sdh.nrp, nmx = sdh.nmx, lmax = sdh.l_max;
+  auto by_l = [&](auto NR, auto NM) -> hipError_t {
+    if (lmax == 8) return f(NR, NM, IC<8>{}, std::true_type{});
+    if (lmax == 10) return f(NR, NM, IC<10>{}, std::true_type{});
+    if (lmax < 8) return f(NR, NM, IC<8>{}, std::false_type{});
+    return f(NR, NM, IC<10>{}, std::false_type{});
+  };
+  if (nrp == 8) return by_l(IC<8>{}, IC<8>{});
+  if (nrp == 16 && nmx == 8) return by_l(IC<16>{}, IC<8>{});
+  if (nrp == 16) return by_l(IC<16>{}, IC<16>{});
+  if (nrp == 24 && nmx == 8) return by_l(IC<24>{}, IC<8>{});
+  if (nrp == 24) return by_l(IC<24>{}, IC<16>{});
+  if (nrp == 32 && nmx == 8) return by_l(IC<32>{}, IC<8>{});
+  if (nrp == 32) return by_l(IC<32>{}, IC<16>{});
+  return hipErrorInvalidValue;
+}
 
 template <typename T, typename GEO>
 hipError_t launch_soap_fwd_g(const SoapDev& sdh, const SoapDev* sd, const double* W,
@@ -640,8 +682,12 @@ hipError_t launch_soap_fwd_g(const SoapDev& sdh, const SoapDev* sd, const double
                            const int* site_atom, int n_sites, const int* dmap, T* q, int q_ld,
                            u16* qb, int qb_ld, T* cbuf, T* pnorm, hipStream_t st) {
   if (n_sites == 0) return hipSuccess;
-  TGX_SOAP_DISPATCH(fwd_inst, sdh, sd, W, rtab, pos, species, cell, maxnb, count, list,
-                    site_atom, n_sites, dmap, q, q_ld, qb, qb_ld, cbuf, pnorm, st);
+  return soap_dispatch(sdh, [&](auto NR, auto NM, auto LM, auto EX) {
+    return fwd_inst<T, decltype(NR)::value, decltype(NM)::value, decltype(LM)::value,
+                    decltype(EX)::value, GEO>(sdh, sd, W, rtab, pos, species, cell, maxnb, count,
+                                              list, site_atom, n_sites, dmap, q, q_ld, qb, qb_ld,
+                                              cbuf, pnorm, st);
+  });
 }
 
 template <typename T, typename GEO>
@@ -653,9 +699,12 @@ hipError_t launch_soap_bwd_g(const SoapDev& sdh, const SoapDev* sd, const double
                            T* gpair, double* vsite, const TwoBodyDev* twob, int n2b,
                            double* eatom, hipStream_t st) {
   if (n_sites == 0) return hipSuccess;
-  TGX_SOAP_DISPATCH(bwd_inst, sdh, sd, W, rtab, pos, species, cell, maxnb, count, list,
-                    site_atom, n_sites, dmap, q, q_ld, cbuf, pnorm, G, g_ld, gpair, vsite, twob,
-                    n2b, eatom, st);
+  return soap_dispatch(sdh, [&](auto NR, auto NM, auto LM, auto EX) {
+    return bwd_inst<T, decltype(NR)::value, decltype(NM)::value, decltype(LM)::value,
+                    decltype(EX)::value, GEO>(sdh, sd, W, rtab, pos, species, cell, maxnb, count,
+                                              list, site_atom, n_sites, dmap, q, q_ld, cbuf, pnorm,
+                                              G, g_ld, gpair, vsite, twob, n2b, eatom, st);
+  });
 }
 
 }  // namespace tgx
