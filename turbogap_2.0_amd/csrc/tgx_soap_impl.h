@@ -131,10 +131,13 @@ __host__ __device__ constexpr size_t fwd_site_bytes(int L2) {
   return ((sizeof(T) * (fwd_region_a<T, NRP>(L2) + (size_t)CH * NRP) + 4 * Capv<NRP>::v) + 15) /
          16 * 16;
 }
+// Stride of one coupling matrix M_l in LDS: NRP^2 + 4 elements, so that the
+// matrices of consecutive l start in different banks (16-byte aligned rows).
+template <int NRP> struct MStride { static constexpr int v = NRP * NRP + 4; };
 template <typename T, int NRP>
 __host__ __device__ constexpr size_t bwd_site_bytes(int L2, int Lp) {
-  return ((sizeof(T) * ((size_t)NRP * L2 + (size_t)Lp * NRP * NRP) + 4 * Capv<NRP>::v) + 15) /
-         16 * 16;
+  return ((sizeof(T) * ((size_t)NRP * L2 + (size_t)Lp * MStride<NRP>::v) + 4 * Capv<NRP>::v) +
+          15) / 16 * 16;
 }
 
 // Appends, in slot order, the slots whose pair lies within the neighbour
@@ -395,7 +398,8 @@ __global__ __launch_bounds__(256) void soap_bwd_kernel(
   T* sGC = reinterpret_cast<T*>(smem_raw + (size_t)(wave * SPW + sub) * bwd_site_bytes<T, NRP>(L2, Lp));
   // sGC [L2][NRP] = dE/dc; sM [Lp][NRP][NRP]: dE/dc = M_l c
   T* sM = sGC + NRP * L2;
-  int* sIdx = reinterpret_cast<int*>(sM + Lp * NRP * NRP);
+  constexpr int MS = MStride<NRP>::v;
+  int* sIdx = reinterpret_cast<int*>(sM + Lp * MS);
   const int site = (blockIdx.x * wpb + wave) * SPW + sub;
   if (site >= n_sites) return;
   const int i = site_atom[site];
@@ -403,7 +407,7 @@ __global__ __launch_bounds__(256) void soap_bwd_kernel(
   const int cnt = min(count[i], maxnb);
   const int* row = list + (size_t)i * maxnb;
   const T* cs = cbuf + (size_t)site * NRP * L2;
-  for (int k = lane; k < Lp * NRP * NRP; k += LW) sM[k] = (T)0;
+  for (int k = lane; k < Lp * MS; k += LW) sM[k] = (T)0;
   const T* qs = q + (size_t)site * q_ld;
   const T* gs = G + (size_t)site * g_ld;
   T gq = 0;
@@ -417,19 +421,15 @@ __global__ __launch_bounds__(256) void soap_bwd_kernel(
     const T gp = (gs[d] - gq * qs[d]) * inv;
     const int dm = dmap[d];
     const int n1 = dm & 255, n2 = (dm >> 8) & 255, l = dm >> 16;
-    T* Ml = sM + l * NRP * NRP;
-    if (n1 == n2) {
-      Ml[n1 * NRP + n1] = (T)2 * gp;
-    } else {
-      const T w = (T)1.41421356237309504880 * gp;
-      Ml[n1 * NRP + n2] = w;
-      Ml[n2 * NRP + n1] = w;
-    }
+    T* Ml = sM + l * MS;
+    const T w = (n1 == n2 ? (T)2 : (T)1.41421356237309504880) * gp;
+    Ml[n1 * NRP + n2] = w;  // both writes hit one entry on the diagonal
+    Ml[n2 * NRP + n1] = w;
   }
   wave_sync();
   for (int lm = lane; lm < L2; lm += LW) {
     const int l = kSph.lm2l[lm];
-    const T* Ml = sM + l * NRP * NRP;
+    const T* Ml = sM + l * MS;
     T cl[NRP];
 #pragma unroll
     for (int n2 = 0; n2 < NRP; n2++) cl[n2] = cs[n2 * L2 + lm];
