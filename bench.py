@@ -95,6 +95,150 @@ def soap_costs(h, n_sites, n_pairs):
     return fwd, bwd
 
 
+def bench_mc512(args):
+    """Metropolis MC, 512 C at 2.0 g/cm^3, 1,500 K, single-atom moves with
+    per-component bound 0.3 A (move_max 0.6): plain GAP Metropolis against
+    delayed acceptance with the surrogate trained on the chain's own first
+    20,000 trials.  Accepted moves/s of the timed trials of each mode from
+    the same start state."""
+    n = 512
+    pos, sp, lat = S.carbon_cell(n, 2.0, 1.2, 1004)
+    eng = tgx.Engine(0, "fp64")
+    model = build_model(eng)
+    eng.set_model(model)
+    eng.set_precision("mixed")
+    T, mm, skin = 1500.0, 0.6, 1.2
+    # surrogate trained online on plain-MC trials (GAP site energies)
+    eng.surrogate_init(model.hypers.dim, seed=4)
+    t0 = time.perf_counter()
+    tr = eng.mc_run_da(pos, sp, lat, 20000, T, mm, skin, 1, n_train_trials=20000,
+                       train_every=2048, epochs=3, batch=512, lr=2e-3, max_rmse=0.2)
+    t_train = time.perf_counter() - t0
+    start = tr["pos"]
+    eng.mc_run(start, sp, lat, 64, T, mm, skin, 2)  # warm-up
+    t0 = time.perf_counter()
+    pl = eng.mc_run(start, sp, lat, args.trials, T, mm, skin, 3)
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    da = eng.mc_run_da(start, sp, lat, args.trials, T, mm, skin, 3, n_train_trials=0,
+                       max_rmse=0.2)
+    t_da = time.perf_counter() - t0
+    acc_pl, acc_da = pl["n_accepted"], da["n_accepted"]
+    v_pl, v_da = acc_pl / t_plain, acc_da / t_da
+    print(f"[bench] mc512 plain {v_pl:.4g} acc/s ({acc_pl}/{args.trials}), DA {v_da:.4g} acc/s "
+          f"({acc_da}/{args.trials}, stage-1 rejections {da['n_stage1_rejected']})",
+          file=sys.stderr, flush=True)
+    line = {"metric": "accepted Monte Carlo moves/s, 512 C at 1500 K, delayed acceptance",
+            "value": round(v_da, 1), "unit": "accepted moves/s", "n_gpus": 1,
+            "steps": args.trials, "warmup": 64, "higher_is_better": True, "scaling": "strong",
+            "vs_baseline": None, "dtype": "f32+bf16", "data": "synthetic",
+            "config": {"workload": "Metropolis MC, 512 C at 2.0 g/cm3, 1500 K, single-atom "
+                                   "moves, per-component bound 0.3 A, skin 1.2 A",
+                       "atoms": n, "trials": args.trials},
+            "plain_value": round(v_pl, 1), "speedup_over_plain": round(v_da / v_pl, 3),
+            "acceptance_plain": acc_pl / args.trials, "acceptance_da": acc_da / args.trials,
+            "stage1_rejected": da["n_stage1_rejected"],
+            "gap_evaluations": da["n_gap_evaluations"],
+            "surrogate_train_rmse_eV_per_site": tr["train_rmse"],
+            "surrogate_dE_rmse_eV": da["da_rmse"], "fallback": da["fallback"],
+            "training_phase_s": round(t_train, 3), "training_trials": 20000,
+            "training_accepted": tr["n_accepted"]}
+    print(json.dumps(line))
+    eng.close()
+
+
+def bench_batch256(args):
+    """Batched predict: 256 periodic carbon cells of 32-1,024 atoms
+    (log-uniform), 1.5-3.2 g/cm^3, carbon SOAP, 2,000 sparse from the batch's
+    own SOAPs; all structures in one launch sequence (host buffers in and
+    out, lists built per call)."""
+    structs = S.batch_cells(256, 1002)
+    n_atoms = sum(len(p) for p, _, _ in structs)
+    eng = tgx.Engine(0, "fp64")
+    h = S.carbon_hypers()
+    tmp = S.model_from_soaps(h, np.eye(h.dim)[:1], 1, seed=0)
+    eng.set_model(tmp)
+    qs = [eng.soap(p, s_, l)[0] for p, s_, l in structs[:64]]
+    model = S.model_from_soaps(h, np.concatenate(qs), 2000, seed=1002,
+                               two_body=[S.carbon_2b(1002)])
+    eng.set_model(model)
+    out = {}
+    for prec in ("mixed", "fp64"):
+        eng.set_precision(prec)
+        eng.energy_forces_batched(structs)
+        reps = 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            res = eng.energy_forces_batched(structs)
+        dt = (time.perf_counter() - t0) / reps
+        out[prec] = (dt, res, eng.kernel_times())
+    dt, (offs, e_m, f_m, v_m), kt = out["mixed"]
+    _, (_, e_d, f_d, v_d), _ = out["fp64"]
+    value = n_atoms / dt
+    print(f"[bench] batch256: {1e3 * dt:.3f} ms per batch, {value:.4g} atom-evals/s",
+          file=sys.stderr, flush=True)
+    line = {"metric": "atom-evaluations/s (E+F), 256-structure batched predict, mixed",
+            "value": round(value, 1), "unit": "atom-evals/s", "n_gpus": 1, "steps": 5,
+            "warmup": 1, "ms_per_step": round(1e3 * dt, 4), "higher_is_better": True,
+            "scaling": "strong", "vs_baseline": None, "dtype": "f32+bf16", "data": "synthetic",
+            "config": {"workload": "256 carbon cells, 32-1024 atoms log-uniform, 1.5-3.2 g/cm3, "
+                                   "carbon SOAP, 2000 sparse, + C-C 2b", "atoms": n_atoms,
+                       "structures": 256},
+            "structures_per_s": round(256 / dt, 1),
+            "fp64_ms_per_batch": round(1e3 * out["fp64"][0], 4),
+            "mixed_vs_fp64": {"force_mae_eV_A": float(np.abs(f_m - f_d).mean()),
+                              "energy_max_eV_atom": float(np.abs(e_m - e_d).max())},
+            "kernel_ms": {k: round(v, 4) for k, v in kt.items()}}
+    print(json.dumps(line))
+    eng.close()
+
+
+def bench_cho4096(args):
+    """C/H/O 60/30/10 at 1.5 g/cm^3, 4,096 atoms, rcut 4.0, n_max 8 per
+    species (dim 2,700), 4,000 sparse from the cell's own SOAPs, zeta 4:
+    E+F with the kernel GEMM in FP64, FP32 and BF16 MFMA, errors against the
+    FP64 path (itself parity-tested against the oracle)."""
+    pos, sp, lat = S.cho_cell(4096, 1.5, 1.0, 1001)
+    eng = tgx.Engine(0, "fp64")
+    h = S.cho_hypers()
+    tmp = S.model_from_soaps(h, np.eye(h.dim)[:1], 1, seed=0)
+    eng.set_model(tmp)
+    q, _ = eng.soap(pos, sp, lat)
+    model = S.model_from_soaps(h, q, 4000, seed=1001)
+    eng.set_model(model)
+    res = {}
+    for prec in ("fp64", "fp32", "mixed"):
+        eng.set_precision(prec)
+        eng.energy_forces(pos, sp, lat)
+        reps = 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            e, f, v = eng.energy_forces(pos, sp, lat)
+        dt = (time.perf_counter() - t0) / reps
+        res[prec] = dict(dt=dt, e=e, f=f, v=v, kt=eng.kernel_times())
+    ref = res["fp64"]
+    modes = {}
+    for prec, r in res.items():
+        modes[prec] = {"ms_per_eval": round(1e3 * r["dt"], 4),
+                       "atom_evals_per_s": round(4096 / r["dt"], 1),
+                       "kernel_ms": {k: round(v, 4) for k, v in r["kt"].items()},
+                       "energy_max_eV_atom": float(np.abs(r["e"] - ref["e"]).max()),
+                       "force_mae_eV_A": float(np.abs(r["f"] - ref["f"]).mean()),
+                       "virial_max_eV": float(np.abs(r["v"] - ref["v"]).max())}
+    print(f"[bench] cho4096: " + ", ".join(f"{k} {v['ms_per_eval']} ms" for k, v in modes.items()),
+          file=sys.stderr, flush=True)
+    line = {"metric": "atom-evaluations/s (E+F), 4096-atom C/H/O, 4000 sparse, mixed",
+            "value": modes["mixed"]["atom_evals_per_s"], "unit": "atom-evals/s", "n_gpus": 1,
+            "steps": 5, "warmup": 1, "higher_is_better": True, "scaling": "strong",
+            "vs_baseline": None, "dtype": "f32+bf16", "data": "synthetic",
+            "config": {"workload": "C/H/O 60/30/10 4096 atoms 1.5 g/cm3, rcut 4.0, n_max 8 per "
+                                   "species, l_max 8, dim 2700, 4000 sparse, zeta 4",
+                       "atoms": 4096},
+            "modes": modes}
+    print(json.dumps(line))
+    eng.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,14 +250,27 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="only run a plain device copy of MB MiB (PMC calibration)")
+    ap.add_argument("--workload", default="md32k", choices=["md32k", "mc512", "batch256",
+                                                            "cho4096"],
+                    help="md32k: the headline NVE MD line; the others print their own line")
+    ap.add_argument("--trials", type=int, default=4096, help="mc512: timed trials per mode")
     args = ap.parse_args()
-    if args.copy_probe:
-        copy_probe(args.copy_probe)
-        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "mc512":
+        bench_mc512(args)
+        return
+    if args.workload == "batch256":
+        bench_batch256(args)
+        return
+    if args.workload == "cho4096":
+        bench_cho4096(args)
+        return
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     n = args.atoms
     pos, sp, lat = S.carbon_cell(n, 2.0, 1.2, 1003)

@@ -202,7 +202,7 @@ int tgx_md_run(tgx_ctx* ctx, int n, int n_steps, double dt, double skin,
  * mc_get_atom_disp mc/mc_utils.f90:259-277 and the acceptance of
  * get_mc_acceptance / monte_carlo_move :172-180, 215-257).  Trial t
  * (0-based) draws u_k = u(seed, t, k), k = 0..4, where u is the top 53 bits
- * of the splitmix64 finaliser of seed + 0x9E3779B97F4A7C15 * (5t + k + 1)
+ * of the splitmix64 finaliser of seed + 0x9E3779B97F4A7C15 * (6t + k + 1)
  * divided by 2^53; it moves atom floor(u_0 n) by move_max (u_1..u_3 - 1/2)
  * (positions + disp, mc_utils.f90:696-698) and accepts when
  * min(1, exp(-(E_trial - E)/(kB T))) > u_4, kB = 8.617333262e-5 rounded to
@@ -218,6 +218,70 @@ int tgx_mc_run(tgx_ctx* ctx, int n, const int* species, const double* lattice,
                double* pos, int n_trials, double temp, double move_max,
                double skin, uint64_t seed, double* e_trace, int* n_accepted,
                int* n_rebuilds);
+
+/*
+ * Delayed-acceptance Metropolis MC (no counterpart in the reference; the
+ * chain samples the same distribution as tgx_mc_run).  A per-atom MLP
+ * surrogate e_s(q) screens every trial first: stage 1 accepts with
+ * min(1, exp(-dE_s/(kB T))) > u_4, dE_s = surrogate change of the affected
+ * sites + the exact 2-body change; only stage-1 survivors are evaluated with
+ * the GAP and accepted with min(1, exp(-(dE - dE_s)/(kB T))) > u_5 (Christen
+ * and Fox 2005).  The surrogate (tgx_surrogate_init first, input dimension =
+ * SOAP dimension) is trained on the GPU on the GAP site energies of the
+ * first n_train_trials trials, which run as plain Metropolis; training
+ * rounds every train_every trials.  When the running RMS of dE - dE_s over
+ * GAP-evaluated trials exceeds max_rmse the run falls back to plain
+ * Metropolis (u_4) for the remaining trials.
+ */
+typedef struct {
+  int32_t n_train_trials; /* plain-MC trials whose GAP site energies train  */
+  int32_t train_every;    /* trials between training rounds (>= 64)         */
+  int32_t epochs;         /* Adam passes over the samples per round         */
+  int32_t batch;          /* sites per Adam step                            */
+  double lr;              /* Adam learning rate                             */
+  double max_rmse;        /* eV; fall back to plain Metropolis above this   */
+} tgx_da_options;
+
+typedef struct {
+  int64_t n_accepted;
+  int64_t n_stage1_rejected; /* trials the surrogate rejected (no GAP)     */
+  int64_t n_gap_evaluations; /* trials evaluated with the GAP              */
+  int64_t n_rebuilds;
+  double train_rmse;         /* eV per site, last training round, all samples */
+  double da_rmse;            /* eV, running RMS of dE - dE_s (stage 2)       */
+  int32_t fallback;          /* 1: the run fell back to plain Metropolis    */
+  int32_t pad_;
+} tgx_mc_stats;
+
+int tgx_mc_run_da(tgx_ctx* ctx, int n, const int* species, const double* lattice,
+                  double* pos, int n_trials, double temp, double move_max,
+                  double skin, uint64_t seed, const tgx_da_options* opt,
+                  double* e_trace, tgx_mc_stats* stats);
+
+/*
+ * The surrogate network e_s(q) = mu + sigma (w3 . s(W2 s(W1 q + b1) + b2) + b3),
+ * s(x) = x / (1 + exp(-x)), 128 hidden units; q, W1 and W2 enter as BF16,
+ * products accumulate in FP32.  Parameters P (FP32 on the device, doubles at
+ * the boundary): W1 (128 x dim, row-major), b1, W2 (128 x 128), b2, w3, b3.
+ *   tgx_surrogate_init      Xavier-uniform weights from seed, dim <= 448;
+ *   tgx_surrogate_set/get   parameters and the output scaling (mu, sigma);
+ *   tgx_surrogate_predict   e_s(q) for n descriptors (n x dim, row-major);
+ *   tgx_surrogate_gradient  d/dP of L = (1/n) sum ((e_s - y)/sigma)^2;
+ *   tgx_surrogate_fit       Adam epochs on (q, y), rmse (eV) afterwards; the
+ *                           first fit sets mu, sigma to the mean and spread of y.
+ * None has a counterpart in the reference.
+ */
+int tgx_surrogate_init(tgx_ctx* ctx, int dim, uint64_t seed);
+int tgx_surrogate_set(tgx_ctx* ctx, int n_params, const double* P, double mu,
+                      double sigma);
+int tgx_surrogate_get(tgx_ctx* ctx, int n_params, double* P, double* mu,
+                      double* sigma);
+int tgx_surrogate_predict(tgx_ctx* ctx, int n, const double* q, double* e);
+int tgx_surrogate_gradient(tgx_ctx* ctx, int n, const double* q, const double* y,
+                           double* grad, double* loss);
+int tgx_surrogate_fit(tgx_ctx* ctx, int n, const double* q, const double* y,
+                      int epochs, int batch, double lr, uint64_t seed,
+                      double* rmse);
 
 /* Overlap S(alpha_max,alpha_max) and orthonormalisation W = S^(-1/2) of the
  * poly3 radial basis, both column-major (symmetric); replaces

@@ -304,3 +304,43 @@ def mc_run(model: GapModel, pos, species, lattice, n_trials, temp, move_max, see
     if rc != 0:
         raise RuntimeError(f"orc_mc_run failed {rc}")
     return pos, tr, acc.value
+
+
+def mlp(P, q, mu, sigma, H=128, round_bf16=True, y=None, grad=False):
+    """Surrogate MLP energies (and the loss gradient when y is given)."""
+    q = _f64(q)
+    n, D = q.shape
+    P = _f64(P)
+    e = np.zeros(n)
+    g = np.zeros_like(P) if grad else None
+    fn = lib().orc_mlp
+    fn.restype = C.c_double
+    fn.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                   C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    yy = None if y is None else _f64(y)
+    loss = fn(n, D, H, P.ctypes.data, q.ctypes.data, mu, sigma, int(round_bf16),
+              None if yy is None else yy.ctypes.data, e.ctypes.data,
+              None if g is None else g.ctypes.data)
+    return (e, loss, g) if grad else e
+
+
+def mc_run_da(model: GapModel, pos, species, lattice, n_trials, temp, move_max, seed, P, mu,
+              sigma):
+    """Delayed-acceptance MC with a fixed surrogate (P, mu, sigma)."""
+    pos = _f64(pos).copy()
+    lat = _f64(lattice)
+    sp = np.ascontiguousarray(species, np.int32)
+    cm, keep = _cmodel(model)
+    P = _f64(P)
+    tr = np.zeros(n_trials + 1)
+    acc, r1 = C.c_int(0), C.c_int(0)
+    fn = lib().orc_mc_run_da
+    fn.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
+                   C.c_double, C.c_uint64, C.c_void_p, C.c_double, C.c_double, C.c_void_p,
+                   C.c_void_p, C.c_void_p]
+    rc = fn(C.addressof(cm), len(pos), pos.ctypes.data, sp.ctypes.data, lat.ctypes.data,
+            n_trials, temp, move_max, seed, P.ctypes.data, mu, sigma, tr.ctypes.data,
+            C.addressof(acc), C.addressof(r1))
+    if rc != 0:
+        raise RuntimeError(f"orc_mc_run_da failed {rc}")
+    return pos, tr, acc.value, r1.value

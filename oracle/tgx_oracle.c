@@ -35,13 +35,20 @@
  *   orc_velocity_verlet   md/md_utils.f90:486-532 (harmonic dimer test).
  *   orc_mc_accept         mc/mc_utils.f90:172-180, 215-257.
  *   orc_mc_uniform        the trial stream of tgx_mc_run (splitmix64 of
- *       seed + golden * (5t + k + 1)); pinned by splitmix64's published
+ *       seed + golden * (6t + k + 1)); pinned by splitmix64's published
  *       outputs for state 0.
  *   orc_mc_run            mc/mc_interface.f90:183-447 with single-atom
  *       moves (mc_utils.f90:259-277, 696-698): one full-system energy per
  *       trial, accept when min(1, p) > u.  Pinned by its limits (T -> inf
  *       accepts every move, T -> 0 only downhill ones) and by recomputing
  *       the final energy.
+ *   orc_mlp               the surrogate network (no reference counterpart);
+ *       its gradient pinned by central finite differences of the loss.
+ *   orc_mc_run_da         delayed-acceptance Metropolis (Christen & Fox,
+ *       J. Comput. Graph. Stat. 14, 795 (2005)) with a fixed surrogate;
+ *       pinned by its limit: with a flat surrogate (zero network, no 2-body
+ *       term) stage 1 passes every trial and the chain is plain Metropolis
+ *       drawn with u_5.
  */
 #include "tgx_oracle.h"
 
@@ -1132,9 +1139,9 @@ double orc_mc_accept(double e_new, double e_prev, double temp) {
 }
 
 /* u(seed, t, k) in [0, 1): the top 53 bits of the splitmix64 finaliser
- * applied to seed + 0x9E3779B97F4A7C15 * (5t + k + 1). */
+ * applied to seed + 0x9E3779B97F4A7C15 * (6t + k + 1). */
 double orc_mc_uniform(uint64_t seed, int64_t t, int k) {
-  uint64_t z = seed + 0x9E3779B97F4A7C15ull * (uint64_t)(5 * t + k + 1);
+  uint64_t z = seed + 0x9E3779B97F4A7C15ull * (uint64_t)(6 * t + k + 1);
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
   z ^= z >> 31;
@@ -1183,5 +1190,182 @@ int orc_mc_run(const orc_model* m, int n, double* pos, const int* species,
   }
   free(ea);
   if (n_accepted) *n_accepted = acc;
+  return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Surrogate MLP                                                            */
+/* ------------------------------------------------------------------------ */
+
+static double orc_bf16(double x, int round) {
+  if (!round) return x;
+  float f = (float)x;
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  u = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
+  memcpy(&f, &u, 4);
+  return (double)f;
+}
+static double orc_sig(double x) { return 1.0 / (1.0 + exp(-x)); }
+
+/* e(q) = mu + sigma (w3 . s(W2 s(W1 q + b1) + b2) + b3), s(x) = x sigmoid(x);
+ * P = [W1 (H x D), b1, W2 (H x H), b2, w3, b3].  round_bf16: q, W1, W2 enter
+ * rounded to bfloat16 (round to nearest even), as the device kernels use
+ * them.  grad (may be NULL): d/dP of L = (1/n) sum (e - y)^2 / sigma^2 with
+ * the rounding taken as the identity. */
+double orc_mlp(int n, int D, int H, const double* P, const double* q, double mu,
+               double sigma, int round_bf16, const double* y, double* e,
+               double* grad) {
+  const double* W1 = P;
+  const double* b1 = W1 + (size_t)H * D;
+  const double* W2 = b1 + H;
+  const double* b2 = W2 + (size_t)H * H;
+  const double* w3 = b2 + H;
+  const double b3 = w3[H];
+  const size_t NP = (size_t)H * D + (size_t)H * H + 3 * (size_t)H + 1;
+  double* z1 = (double*)malloc(sizeof(double) * H);
+  double* z2 = (double*)malloc(sizeof(double) * H);
+  double* h1 = (double*)malloc(sizeof(double) * H);
+  double* d2 = (double*)malloc(sizeof(double) * H);
+  double* qr = (double*)malloc(sizeof(double) * D);
+  double loss = 0.0;
+  if (grad) memset(grad, 0, sizeof(double) * NP);
+  for (int s = 0; s < n; s++) {
+    for (int k = 0; k < D; k++) qr[k] = orc_bf16(q[(size_t)s * D + k], round_bf16);
+    for (int u = 0; u < H; u++) {
+      double a = 0.0;
+      for (int k = 0; k < D; k++) a += orc_bf16(W1[(size_t)u * D + k], round_bf16) * qr[k];
+      z1[u] = a + b1[u];
+      h1[u] = z1[u] * orc_sig(z1[u]);
+    }
+    double out = b3;
+    for (int u = 0; u < H; u++) {
+      double a = 0.0;
+      for (int k = 0; k < H; k++) a += orc_bf16(W2[(size_t)u * H + k], round_bf16) * h1[k];
+      z2[u] = a + b2[u];
+      out += w3[u] * z2[u] * orc_sig(z2[u]);
+    }
+    const double es = mu + sigma * out;
+    if (e) e[s] = es;
+    if (y) {
+      const double r = (es - y[s]) / sigma;
+      loss += r * r / n;
+      if (grad) {
+        const double g = 2.0 * r / sigma / n * sigma; /* dL/d(out) */
+        double* gW1 = grad;
+        double* gb1 = gW1 + (size_t)H * D;
+        double* gW2 = gb1 + H;
+        double* gb2 = gW2 + (size_t)H * H;
+        double* gw3 = gb2 + H;
+        gw3[H] += g;
+        for (int u = 0; u < H; u++) {
+          const double sg = orc_sig(z2[u]);
+          gw3[u] += g * z2[u] * sg;
+          d2[u] = g * w3[u] * sg * (1.0 + z2[u] * (1.0 - sg));
+          gb2[u] += d2[u];
+          for (int k = 0; k < H; k++) gW2[(size_t)u * H + k] += d2[u] * h1[k];
+        }
+        for (int k = 0; k < H; k++) {
+          double a = 0.0;
+          for (int u = 0; u < H; u++) a += orc_bf16(W2[(size_t)u * H + k], round_bf16) * d2[u];
+          const double sg = orc_sig(z1[k]);
+          const double d1 = a * sg * (1.0 + z1[k] * (1.0 - sg));
+          gb1[k] += d1;
+          for (int j = 0; j < D; j++) gW1[(size_t)k * D + j] += d1 * qr[j];
+        }
+      }
+    }
+  }
+  free(z1);
+  free(z2);
+  free(h1);
+  free(d2);
+  free(qr);
+  return loss;
+}
+
+/* Surrogate energy of a whole structure: the MLP on every site's SOAP plus
+ * the exact 2-body terms (the parts delayed-acceptance stage 1 sees). */
+static double orc_surrogate_energy(const orc_model* m, int n, const double* pos,
+                                   const int* species, const double* lat,
+                                   const double* P, double mu, double sigma,
+                                   int* rc) {
+  const int H = 128;
+  soap_geom g;
+  soap_geom_init(&m->h, &g);
+  double* q = (double*)malloc(sizeof(double) * (size_t)n * g.dim);
+  int n_sites = 0;
+  int64_t n_pairs = 0;
+  *rc = orc_soap(&m->h, n, pos, lat, species, q, NULL, 0, &n_sites, &n_pairs, NULL, NULL,
+                 NULL, NULL);
+  double e = 0.0;
+  if (*rc == 0) {
+    double* es = (double*)malloc(sizeof(double) * (size_t)n_sites);
+    orc_mlp(n_sites, g.dim, H, P, q, mu, sigma, 1, NULL, es, NULL);
+    for (int i = 0; i < n_sites; i++) e += es[i];
+    free(es);
+    /* 2-body part: the whole-path energy without the SOAP term (minus e0) */
+    if (m->n2b > 0) {
+      orc_model m2 = *m;
+      m2.has_soap = 0;
+      double* ea = (double*)malloc(sizeof(double) * (size_t)n);
+      *rc = orc_energy_forces(&m2, n, pos, species, lat, 0, 0, ea, NULL, NULL);
+      for (int i = 0; i < n; i++) e += ea[i] - m->e0[species[i] - 1];
+      free(ea);
+    }
+  }
+  free(q);
+  return e;
+}
+
+/* Delayed-acceptance Metropolis (Christen & Fox 2005) with a fixed surrogate:
+ * stage 1 accepts with min(1, exp(-(E*_trial - E*_current)/(kB T))) > u4,
+ * stage 2 with min(1, exp(-((E_trial - E_current) - (E*_trial - E*_current))
+ * /(kB T))) > u5; whole-structure energies per trial as orc_mc_run.  Returns
+ * the trace and counts (accepted, stage-1 rejections). */
+int orc_mc_run_da(const orc_model* m, int n, double* pos, const int* species,
+                  const double* lat, int n_trials, double temp, double move_max,
+                  uint64_t seed, const double* P, double mu, double sigma,
+                  double* e_trace, int* n_accepted, int* n_rej1) {
+  double* ea = (double*)malloc(sizeof(double) * (size_t)n);
+  int rc = 0, acc = 0, r1 = 0;
+  const double kB = (double)8.617333262e-5f;
+  double e_cur = orc_total_energy(m, n, pos, species, lat, ea, &rc);
+  double s_cur = rc == 0 ? orc_surrogate_energy(m, n, pos, species, lat, P, mu, sigma, &rc) : 0;
+  if (e_trace) e_trace[0] = e_cur;
+  for (int t = 0; t < n_trials && rc == 0; t++) {
+    int idx = (int)floor(orc_mc_uniform(seed, t, 0) * n);
+    if (idx > n - 1) idx = n - 1;
+    double old[3];
+    for (int k = 0; k < 3; k++) {
+      const double disp = move_max * (orc_mc_uniform(seed, t, 1 + k) - 0.5);
+      old[k] = pos[3 * idx + k];
+      pos[3 * idx + k] = old[k] + disp;
+    }
+    const double s_new = orc_surrogate_energy(m, n, pos, species, lat, P, mu, sigma, &rc);
+    double p1 = exp(-(s_new - s_cur) / (kB * temp));
+    p1 = p1 < 1.0 ? p1 : 1.0;
+    int accept = 0;
+    double e_new = e_cur;
+    if (p1 > orc_mc_uniform(seed, t, 4)) {
+      e_new = orc_total_energy(m, n, pos, species, lat, ea, &rc);
+      double p2 = exp(-((e_new - e_cur) - (s_new - s_cur)) / (kB * temp));
+      p2 = p2 < 1.0 ? p2 : 1.0;
+      accept = p2 > orc_mc_uniform(seed, t, 5);
+    } else {
+      r1++;
+    }
+    if (accept) {
+      e_cur = e_new;
+      s_cur = s_new;
+      acc++;
+    } else {
+      for (int k = 0; k < 3; k++) pos[3 * idx + k] = old[k];
+    }
+    if (e_trace) e_trace[t + 1] = e_cur;
+  }
+  free(ea);
+  if (n_accepted) *n_accepted = acc;
+  if (n_rej1) *n_rej1 = r1;
   return rc;
 }

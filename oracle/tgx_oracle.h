@@ -134,6 +134,18 @@ int orc_mc_run(const orc_model* m, int n, double* pos, const int* species,
                const double* lat, int n_trials, double temp, double move_max,
                uint64_t seed, double* e_trace, int* n_accepted);
 
+/* Surrogate MLP e(q) and, with y and grad, the gradient of the mean squared
+ * error in normalised units; returns the loss (0 without y). */
+double orc_mlp(int n, int D, int H, const double* P, const double* q, double mu,
+               double sigma, int round_bf16, const double* y, double* e,
+               double* grad);
+
+/* Delayed-acceptance MC with a fixed surrogate (P, mu, sigma). */
+int orc_mc_run_da(const orc_model* m, int n, double* pos, const int* species,
+                  const double* lat, int n_trials, double temp, double move_max,
+                  uint64_t seed, const double* P, double mu, double sigma,
+                  double* e_trace, int* n_accepted, int* n_rej1);
+
 #ifdef __cplusplus
 }
 #endif

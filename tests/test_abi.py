@@ -120,3 +120,29 @@ def test_create_without_device_fails_cleanly():
         lib.tgx_destroy(ctx)
         pytest.skip("device present")
     assert rc == tgx.TGX_ERR_HIP and not ctx.value
+
+
+@pytest.mark.parametrize("cls,cname", [(tgx.DaOptions, "tgx_da_options"),
+                                       (tgx.McStats, "tgx_mc_stats")])
+def test_mc_structs_layout_matches_c(tmp_path, cls, cname):
+    fields = [f[0] for f in cls._fields_]
+    body = "\n".join(f'  printf("{f} %zu\\n", offsetof({cname}, {f}));' for f in fields)
+    src = tmp_path / "layout.c"
+    src.write_text(f"""#include <stdio.h>
+#include <stddef.h>
+#include "tgx.h"
+int main(void) {{
+  printf("size %zu\\n", sizeof({cname}));
+{body}
+  return 0;
+}}
+""")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True,
+                                                         text=True, check=True).stdout.split("\n")
+               if line)
+    assert int(got["size"]) == C.sizeof(cls)
+    for f in fields:
+        assert int(got[f]) == getattr(cls, f).offset, f

@@ -15,6 +15,7 @@ from scipy.special import sph_harm_y
 
 import oracle
 import structures as S
+import tgx
 import workloads as W
 
 KB_SINGLE = float(np.float32(8.617333262e-5))
@@ -495,8 +496,8 @@ def test_mc_uniform_is_splitmix64():
     53 bits over 2^53."""
     for k, v in enumerate([0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]):
         assert oracle.mc_uniform(0, 0, k) == (v >> 11) / 2.0 ** 53
-    # trial t continues the same counter: u(0, t, k) = output 5t + k + 1
-    assert oracle.mc_uniform(0, 1, 0) == oracle.mc_uniform(0, 0, 5)
+    # trial t continues the same counter: u(0, t, k) = output 6t + k + 1
+    assert oracle.mc_uniform(0, 1, 0) == oracle.mc_uniform(0, 0, 6)
     u = np.array([oracle.mc_uniform(7, t, k) for t in range(2000) for k in range(5)])
     assert 0.0 <= u.min() and u.max() < 1.0 and abs(u.mean() - 0.5) < 0.01
 
@@ -529,3 +530,84 @@ def test_mc_cold_limit_only_downhill():
     assert acc == int(np.sum(d < 0))
     e, _, _ = oracle.energy_forces(model, p, sp, lat, do_forces=False)
     assert abs(e.sum() - tr[-1]) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# Surrogate network and delayed acceptance
+# ---------------------------------------------------------------------------
+
+def _mlp_case(n=6, D=20, seed=3):
+    rng = np.random.default_rng(seed)
+    H = 128
+    NP = H * D + H * H + 3 * H + 1
+    P = rng.normal(0, 0.2, NP)
+    q = rng.normal(0, 1, (n, D))
+    y = rng.normal(0, 1, n)
+    return P, q, y
+
+
+def test_mlp_gradient_matches_finite_differences():
+    """orc_mlp's loss gradient (exact arithmetic, no BF16 rounding) against
+    central differences on a spread of parameters of every block."""
+    P, q, y = _mlp_case()
+    mu, sigma = 0.3, 1.7
+    e, loss, g = oracle.mlp(P, q, mu, sigma, round_bf16=False, y=y, grad=True)
+    assert loss == pytest.approx(np.mean(((e - y) / sigma) ** 2), rel=1e-12)
+    D, H = q.shape[1], 128
+    idx = [0, 7, H * D - 1, H * D + 5, H * D + H + 11, H * D + H + H * H + 3,
+           H * D + H + H * H + H + 9, H * D + H * H + 3 * H]
+    for k in idx:
+        h = 1e-6
+        Pp, Pm = P.copy(), P.copy()
+        Pp[k] += h
+        Pm[k] -= h
+        _, lp, _ = oracle.mlp(Pp, q, mu, sigma, round_bf16=False, y=y, grad=True)
+        _, lm, _ = oracle.mlp(Pm, q, mu, sigma, round_bf16=False, y=y, grad=True)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - g[k]) <= 1e-6 * max(1.0, abs(fd)), (k, fd, g[k])
+
+
+def test_mlp_closed_form_single_unit():
+    """A network whose only path is W1[0][0] -> unit 0 -> w3[0] computes
+    mu + sigma (w3 s(W2 s(W1 q + b1) + b2) + b3) by hand."""
+    D, H = 3, 128
+    P = np.zeros(H * D + H * H + 3 * H + 1)
+    P[0] = 0.75                      # W1[0][0]
+    P[H * D] = -0.2                  # b1[0]
+    P[H * D + H] = 1.5               # W2[0][0]
+    P[H * D + H + H * H] = 0.1       # b2[0]
+    P[H * D + H + H * H + H] = 2.0   # w3[0]
+    P[-1] = 0.05                     # b3
+    s = lambda x: x / (1 + np.exp(-x))  # noqa: E731
+    q = np.array([[0.4, -1.0, 2.0]])
+    z1 = 0.75 * 0.4 - 0.2
+    want = 1.0 + 0.5 * (2.0 * s(1.5 * s(z1) + 0.1) + 0.05 + 127 * 0.0)
+    # units 1..127 have s(0) = 0 in layer 1 and s(b2 = 0) = 0 in layer 2
+    got = oracle.mlp(P, q, 1.0, 0.5, round_bf16=False)
+    assert got[0] == pytest.approx(want, rel=1e-14)
+
+
+def test_delayed_acceptance_flat_surrogate_is_plain_metropolis():
+    """A zero network and no 2-body term make stage 1 pass every trial; the
+    chain is then plain Metropolis drawn with u_5 (composition of the pinned
+    energy, acceptance and stream)."""
+    pos, sp, lat, model = _mc_system()
+    m = tgx.GapModel(model.hypers, model.Qs, model.alphas, model.delta, model.zeta)
+    D = model.hypers.dim
+    P = np.zeros(128 * D + 128 * 128 + 3 * 128 + 1)
+    p_da, tr_da, acc, r1 = oracle.mc_run_da(m, pos, sp, lat, 10, 1500.0, 0.6, 17, P, 0.0, 1.0)
+    assert r1 == 0
+    q = pos.copy()
+    e_cur = oracle.energy_forces(m, q, sp, lat, do_forces=False)[0].sum()
+    n_acc = 0
+    for t in range(10):
+        k = int(np.floor(oracle.mc_uniform(17, t, 0) * len(q)))
+        old = q[k].copy()
+        q[k] = old + [0.6 * (oracle.mc_uniform(17, t, 1 + j) - 0.5) for j in range(3)]
+        e_new = oracle.energy_forces(m, q, sp, lat, do_forces=False)[0].sum()
+        if oracle.mc_accept(e_new, e_cur, 1500.0) > oracle.mc_uniform(17, t, 5):
+            e_cur, n_acc = e_new, n_acc + 1
+        else:
+            q[k] = old
+        assert abs(tr_da[t + 1] - e_cur) < 1e-12
+    assert n_acc == acc and np.array_equal(q, p_da)

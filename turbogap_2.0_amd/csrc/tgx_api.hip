@@ -54,14 +54,40 @@ hipError_t launch_reduce_many(int n_struct, const int* aoff, const int* site_of_
 hipError_t launch_mc_propose(int n, double* pos, const double* pref, unsigned long long seed,
                              double move_max, double lim2, int* ctl, double* dctl,
                              hipStream_t st);
-hipError_t launch_mc_affected(const int* ctl, const int* count, const int* list, int maxnb,
-                              const int* species, int central, int* aff, int cap, int* n_aff,
-                              hipStream_t st);
-hipError_t launch_mc_decide(int n, double* pos, const int* species, const Cell& cell, int maxnb,
+hipError_t launch_mc_affected(const int* ctl, const double* dctl, const double* pos,
+                              const Cell& cell, double rsoap, const int* count, const int* list,
+                              int maxnb, const int* species, int central, int* aff, int cap,
+                              int* n_aff, hipStream_t st);
+hipError_t launch_mc_stage1(double* pos, const int* species, const Cell& cell, int maxnb,
                             const int* count, const int* list, const int* aff, const int* n_aff,
-                            const double* esite, double* e_cur, const TwoBodyDev* twob, int n2b,
+                            const double* esur, const double* esur_cur, const TwoBodyDev* twob,
+                            int n2b, unsigned long long seed, double kT, int* ctl, double* dctl,
+                            double* trace, hipStream_t st);
+hipError_t launch_mc_stage2(double* pos, const int* aff, const int* n_aff, const double* esite,
+                            double* e_cur, const double* esur, double* esur_cur,
                             unsigned long long seed, double kT, int* ctl, double* dctl,
                             double* trace, hipStream_t st);
+template <typename T>
+hipError_t launch_mc_collect(const int* ctl, const int* n_aff, const T* q, int q_ld, int D,
+                             const double* esite, float* sq, double* sy, int cap_s, int* cursor,
+                             hipStream_t st);
+size_t mlp_nparams(int D);
+int mlp_hidden();
+int mlp_max_dim();
+template <typename T>
+hipError_t launch_mlp_forward(int n, int D, const T* q, int q_ld, const int* rows,
+                              const float* P, const unsigned short* W1t,
+                              const unsigned short* W2t, const float* norm, double* e,
+                              float* z1, float* z2, const int* guard, hipStream_t st);
+template <typename T>
+hipError_t launch_mlp_gradient(int n, int D, const T* q, int q_ld, const int* rows,
+                               const double* y, const double* e, const float* z1,
+                               const float* z2, const float* P, const unsigned short* W2t,
+                               const float* norm, float* part, float* grad, hipStream_t st);
+hipError_t launch_mlp_adam(int D, float* P, const float* grad, float* m, float* v, float lr,
+                           int step, unsigned short* W1t, unsigned short* W2t, hipStream_t st);
+hipError_t launch_mlp_cast(int D, const float* P, unsigned short* W1t, unsigned short* W2t,
+                           hipStream_t st);
 hipError_t launch_mc_init(int n, const int* site_of_atom, const double* esite, double* e_cur,
                           hipStream_t st);
 hipError_t launch_energy_init(int n, const int* species, const double* e0,
@@ -142,6 +168,13 @@ struct Extra {
   DevBuf sd_dev, pairn, e0_dev, iota, cell_count, cell_fill, out, stage;
   DevBuf b_cells, b_sid, b_boff, b_aoff, b_out;  // batched structures
   DevBuf mc_pref, mc_ctl, mc_dctl, mc_ecur, mc_naff, mc_trace, mc_twob;  // Monte Carlo
+  DevBuf mc_esur, mc_esur_cur, mc_sq, mc_sy;                          // delayed acceptance
+  // surrogate MLP: FP32 master parameters, Adam moments, BF16 copies, work
+  DevBuf sg_P, sg_m, sg_v, sg_grad, sg_W1t, sg_W2t, sg_part, sg_z1, sg_z2, sg_e, sg_q, sg_y,
+      sg_rows, sg_norm /* mu, sigma on the device */;
+  int sg_dim = 0, sg_step = 0;
+  float sg_mu = 0.f, sg_sigma = 1.f;
+  bool sg_norm_set = false;
   DevBuf twob_store;
   std::vector<int> species_host;
   hipEvent_t ev[8];
@@ -279,7 +312,7 @@ static int soap_forward(tgx_ctx* c, int n, const Geom& cell, int n_sites) {
 }
 
 // Kernel step on the descriptors in c->q / c->qb.
-static int kernel_step(tgx_ctx* c, int n_sites, bool want_grad) {
+static int kernel_step(tgx_ctx* c, int n_sites, bool want_grad, const int* guard = nullptr) {
   const int prec = c->precision;
   const size_t tin = prec == TGX_FP64 ? 8 : (prec == TGX_FP32 ? 4 : 2);
   const size_t tacc = prec == TGX_FP64 ? 8 : 4;
@@ -300,7 +333,7 @@ static int kernel_step(tgx_ctx* c, int n_sites, bool want_grad) {
   TGX_HIP(c, launch_gap_kernel(prec, n_sites, c->dim, c->dim_pad, c->n_sparse, c->sp_pad, qa, qs,
                                qsT, al, c->delta, c->zeta, c->wmat.p, c->epart.p,
                                c->esite.as<double>(), want_grad ? c->gmat.p : nullptr,
-                               want_grad ? 1 : 0, c->stream));
+                               want_grad ? 1 : 0, guard, c->stream));
   kev_rec(c, 3);
   return TGX_OK;
 }
@@ -542,7 +575,10 @@ int tgx_destroy(tgx_ctx* c) {
                     &f->x.stage, &f->x.twob_store, &f->x.b_cells, &f->x.b_sid,
                     &f->x.b_boff, &f->x.b_aoff, &f->x.b_out, &f->x.mc_pref, &f->x.mc_ctl,
                     &f->x.mc_dctl, &f->x.mc_ecur, &f->x.mc_naff, &f->x.mc_trace,
-                    &f->x.mc_twob};
+                    &f->x.mc_twob, &f->x.mc_esur, &f->x.mc_esur_cur, &f->x.mc_sq,
+                    &f->x.mc_sy, &f->x.sg_P, &f->x.sg_m, &f->x.sg_v, &f->x.sg_grad,
+                    &f->x.sg_W1t, &f->x.sg_W2t, &f->x.sg_part, &f->x.sg_z1, &f->x.sg_z2,
+                    &f->x.sg_e, &f->x.sg_q, &f->x.sg_y, &f->x.sg_rows, &f->x.sg_norm};
   for (DevBuf* b : bufs) b->release();
   for (auto& b : c->twob) {
     if (b.Qs) (void)hipFree(b.Qs);
@@ -1082,19 +1118,172 @@ int tgx_energy_forces_batched(tgx_ctx* c, int n_struct, const int* offsets, cons
   return TGX_OK;
 }
 
+// ---- surrogate MLP -----------------------------------------------------------
+
+// Xavier-uniform weights from a splitmix64 stream, zero biases.
+static void sg_init_params(int D, int H, uint64_t seed, std::vector<float>* P) {
+  const size_t NP = mlp_nparams(D);
+  P->assign(NP, 0.f);
+  uint64_t st = seed;
+  auto uni = [&]() {
+    uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  };
+  const double a1 = std::sqrt(6.0 / (D + H)), a2 = std::sqrt(6.0 / (2.0 * H)),
+               a3 = std::sqrt(6.0 / (H + 1.0));
+  size_t k = 0;
+  for (size_t i = 0; i < (size_t)H * D; i++) (*P)[k++] = (float)(a1 * (2 * uni() - 1));
+  k += H;
+  for (size_t i = 0; i < (size_t)H * H; i++) (*P)[k++] = (float)(a2 * (2 * uni() - 1));
+  k += H;
+  for (int i = 0; i < H; i++) (*P)[k++] = (float)(a3 * (2 * uni() - 1));
+}
+
+// mu, sigma to the device copy the kernels read.
+static int sg_push_norm(tgx_ctx* c) {
+  Extra& x = X(c);
+  const float nm[2] = {x.sg_mu, x.sg_sigma};
+  if (x.sg_norm.ensure(sizeof(nm))) return set_err(c, TGX_ERR_HIP, "allocation failed");
+  TGX_HIP(c, hipMemcpyAsync(x.sg_norm.p, nm, sizeof(nm), hipMemcpyHostToDevice, c->stream));
+  TGX_HIP(c, hipStreamSynchronize(c->stream));
+  return TGX_OK;
+}
+
+static int sg_ready(tgx_ctx* c) {
+  if (X(c).sg_dim <= 0) return set_err(c, TGX_ERR_STATE, "surrogate not initialised");
+  return TGX_OK;
+}
+
+// Work buffers for n training/inference sites.
+static int sg_work(tgx_ctx* c, int n) {
+  Extra& x = X(c);
+  const int H = mlp_hidden(), D = x.sg_dim;
+  const int nb = (n + 15) / 16;
+  if (x.sg_part.ensure(sizeof(float) * (size_t)nb * mlp_nparams(D)) ||
+      x.sg_z1.ensure(sizeof(float) * (size_t)n * H) ||
+      x.sg_z2.ensure(sizeof(float) * (size_t)n * H) || x.sg_e.ensure(8 * (size_t)n))
+    return set_err(c, TGX_ERR_HIP, "allocation failed (surrogate)");
+  return TGX_OK;
+}
+
+// One Adam step on the sites rows[0..n) of the sample matrix sq (FP32, D
+// wide) with targets sy.
+static int sg_adam_step(tgx_ctx* c, int n, const float* sq, const double* sy, const int* rows,
+                        float lr) {
+  Extra& x = X(c);
+  const int D = x.sg_dim;
+  int rc = sg_work(c, n);
+  if (rc) return rc;
+  // rows index both the descriptors and the targets: gather the targets
+  TGX_HIP(c, launch_mlp_forward<float>(n, D, sq, D, rows, x.sg_P.as<float>(),
+                                       x.sg_W1t.as<unsigned short>(),
+                                       x.sg_W2t.as<unsigned short>(), x.sg_norm.as<float>(),
+                                       x.sg_e.as<double>(), x.sg_z1.as<float>(),
+                                       x.sg_z2.as<float>(), nullptr, c->stream));
+  TGX_HIP(c, launch_mlp_gradient<float>(n, D, sq, D, rows, sy, x.sg_e.as<double>(),
+                                        x.sg_z1.as<float>(), x.sg_z2.as<float>(),
+                                        x.sg_P.as<float>(), x.sg_W2t.as<unsigned short>(),
+                                        x.sg_norm.as<float>(), x.sg_part.as<float>(),
+                                        x.sg_grad.as<float>(), c->stream));
+  x.sg_step += 1;
+  TGX_HIP(c, launch_mlp_adam(D, x.sg_P.as<float>(), x.sg_grad.as<float>(), x.sg_m.as<float>(),
+                             x.sg_v.as<float>(), lr, x.sg_step, x.sg_W1t.as<unsigned short>(),
+                             x.sg_W2t.as<unsigned short>(), c->stream));
+  return TGX_OK;
+}
+
+// Targets in row order for a minibatch (the gradient kernel reads y[site]).
+__global__ void sg_gather_y_kernel(int n, const int* rows, const double* sy, double* y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = sy[rows[i]];
+}
+
+// epochs passes over samples [0, n_s) of (sq, sy) in minibatches of `batch`
+// in a seeded random order; returns the RMS error (eV) of the final weights
+// on all samples.
+static int sg_train(tgx_ctx* c, int n_s, const float* sq, const double* sy, int epochs,
+                    int batch, float lr, uint64_t seed, double* rmse) {
+  Extra& x = X(c);
+  if (n_s <= 0) return set_err(c, TGX_ERR_ARG, "no samples");
+  batch = std::max(16, std::min(batch, n_s));
+  if (x.sg_rows.ensure(sizeof(int) * (size_t)n_s) || x.sg_y.ensure(8 * (size_t)n_s))
+    return set_err(c, TGX_ERR_HIP, "allocation failed (surrogate)");
+  if (!x.sg_norm_set) {
+    std::vector<double> y(n_s);
+    TGX_HIP(c, hipMemcpyAsync(y.data(), sy, 8 * (size_t)n_s, hipMemcpyDeviceToHost, c->stream));
+    TGX_HIP(c, hipStreamSynchronize(c->stream));
+    double m = 0, v = 0;
+    for (double t : y) m += t;
+    m /= n_s;
+    for (double t : y) v += (t - m) * (t - m);
+    v = std::sqrt(v / n_s);
+    x.sg_mu = (float)m;
+    x.sg_sigma = (float)(v > 1e-6 ? v : 1e-6);
+    x.sg_norm_set = true;
+    int rc = sg_push_norm(c);
+    if (rc) return rc;
+  }
+  std::vector<int> perm(n_s);
+  uint64_t st = seed;
+  for (int e = 0; e < epochs; e++) {
+    for (int i = 0; i < n_s; i++) perm[i] = i;
+    for (int i = n_s - 1; i > 0; i--) {  // Fisher-Yates with a splitmix64 stream
+      uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      z ^= z >> 31;
+      std::swap(perm[i], perm[(size_t)(z % (uint64_t)(i + 1))]);
+    }
+    TGX_HIP(c, hipMemcpyAsync(x.sg_rows.p, perm.data(), sizeof(int) * (size_t)n_s,
+                              hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(sg_gather_y_kernel, dim3((n_s + 255) / 256), dim3(256), 0, c->stream, n_s,
+                       x.sg_rows.as<int>(), sy, x.sg_y.as<double>());
+    for (int b0 = 0; b0 + batch <= n_s; b0 += batch) {
+      int rc = sg_adam_step(c, batch, sq, x.sg_y.as<double>() + b0, x.sg_rows.as<int>() + b0, lr);
+      if (rc) return rc;
+    }
+    // the ordered copy must not be overwritten before the queued steps ran
+    TGX_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  if (rmse) {
+    int rc = sg_work(c, n_s);
+    if (rc) return rc;
+    TGX_HIP(c, launch_mlp_forward<float>(n_s, x.sg_dim, sq, x.sg_dim, nullptr,
+                                         x.sg_P.as<float>(), x.sg_W1t.as<unsigned short>(),
+                                         x.sg_W2t.as<unsigned short>(), x.sg_norm.as<float>(),
+                                         x.sg_e.as<double>(), nullptr, nullptr, nullptr,
+                                         c->stream));
+    std::vector<double> e(n_s), y(n_s);
+    TGX_HIP(c, hipMemcpyAsync(e.data(), x.sg_e.p, 8 * (size_t)n_s, hipMemcpyDeviceToHost,
+                              c->stream));
+    TGX_HIP(c, hipMemcpyAsync(y.data(), sy, 8 * (size_t)n_s, hipMemcpyDeviceToHost, c->stream));
+    TGX_HIP(c, hipStreamSynchronize(c->stream));
+    double a = 0;
+    for (int i = 0; i < n_s; i++) a += (e[i] - y[i]) * (e[i] - y[i]);
+    *rmse = std::sqrt(a / n_s);
+  }
+  return TGX_OK;
+}
+
 // ---- Monte Carlo -----------------------------------------------------------
 
 namespace {
 struct McRun {
   int n, cap, central, n2b;
   Cell cell;
-  double rcut_list, move_max, lim2, kT;
+  double rcut_list, rsoap, move_max, lim2, kT;
   unsigned long long seed;
+  bool da;      // surrogate stage in the trial sequence
+  int cap_s;    // sample ring capacity (0: no collection)
 };
 }  // namespace
 
-// One trial: propose, guarded list rebuild, affected sites, SOAP forward and
-// energy GEMM on them, decision.  Fixed launch shapes (capturable).
+// One trial: propose, guarded list rebuild, affected sites, SOAP forward,
+// surrogate stage, guarded GAP energy GEMM, sample collection, decision.
+// Fixed launch shapes (capturable).
 static int mc_trial_launch(tgx_ctx* c, const McRun& r) {
   Extra& x = X(c);
   int* ctl = x.mc_ctl.as<int>();
@@ -1109,27 +1298,95 @@ static int mc_trial_launch(tgx_ctx* c, const McRun& r) {
                               ctl + MC_REBUILD));
   TGX_HIP(c, launch_md_ref_copy(r.n, c->pos.as<double>(), x.mc_pref.as<double>(),
                                 ctl + MC_REBUILD, c->stream));
+  const int D = c->dim;
   if (c->has_soap) {
-    TGX_HIP(c, launch_mc_affected(ctl, c->nb_count.as<int>(), c->nb_list.as<int>(), c->maxnb,
+    TGX_HIP(c, launch_mc_affected(ctl, dctl, c->pos.as<double>(), r.cell, r.rsoap,
+                                  c->nb_count.as<int>(), c->nb_list.as<int>(), c->maxnb,
                                   c->species.as<int>(), r.central, c->site_atom.as<int>(), r.cap,
                                   x.mc_naff.as<int>(), c->stream));
     int rcode = soap_forward(c, r.n, Geom{r.cell}, r.cap);
     if (rcode) return rcode;
-    rcode = kernel_step(c, r.cap, false);
-    if (rcode) return rcode;
+    if (r.da) {
+      hipError_t e =
+          c->precision == TGX_FP64
+              ? launch_mlp_forward<double>(r.cap, D, c->q.as<double>(), c->dim_pad, nullptr,
+                                           x.sg_P.as<float>(), x.sg_W1t.as<unsigned short>(),
+                                           x.sg_W2t.as<unsigned short>(), x.sg_norm.as<float>(),
+                                           x.mc_esur.as<double>(), nullptr, nullptr,
+                                           ctl + MC_DA, c->stream)
+              : launch_mlp_forward<float>(r.cap, D, c->q.as<float>(), c->dim_pad, nullptr,
+                                          x.sg_P.as<float>(), x.sg_W1t.as<unsigned short>(),
+                                          x.sg_W2t.as<unsigned short>(), x.sg_norm.as<float>(),
+                                          x.mc_esur.as<double>(), nullptr, nullptr, ctl + MC_DA,
+                                          c->stream);
+      TGX_HIP(c, e);
+    }
   }
-  TGX_HIP(c, launch_mc_decide(r.n, c->pos.as<double>(), c->species.as<int>(), r.cell, c->maxnb,
-                              c->nb_count.as<int>(), c->nb_list.as<int>(),
-                              c->site_atom.as<int>(), x.mc_naff.as<int>(),
+  TGX_HIP(c, launch_mc_stage1(c->pos.as<double>(), c->species.as<int>(), r.cell, c->maxnb,
+                              c->nb_count.as<int>(), c->nb_list.as<int>(), c->site_atom.as<int>(),
+                              x.mc_naff.as<int>(), x.mc_esur.as<double>(),
+                              x.mc_esur_cur.as<double>(), x.mc_twob.as<TwoBodyDev>(), r.n2b,
+                              r.seed, r.kT, ctl, dctl, x.mc_trace.as<double>(), c->stream));
+  if (c->has_soap) {
+    int rcode = kernel_step(c, r.cap, false, ctl + MC_GATE);
+    if (rcode) return rcode;
+    if (r.cap_s > 0) {
+      hipError_t e =
+          c->precision == TGX_FP64
+              ? launch_mc_collect<double>(ctl, x.mc_naff.as<int>(), c->q.as<double>(), c->dim_pad,
+                                          D, c->esite.as<double>(), x.mc_sq.as<float>(),
+                                          x.mc_sy.as<double>(), r.cap_s, ctl + MC_CURSOR,
+                                          c->stream)
+              : launch_mc_collect<float>(ctl, x.mc_naff.as<int>(), c->q.as<float>(), c->dim_pad,
+                                         D, c->esite.as<double>(), x.mc_sq.as<float>(),
+                                         x.mc_sy.as<double>(), r.cap_s, ctl + MC_CURSOR,
+                                         c->stream);
+      TGX_HIP(c, e);
+    }
+  }
+  TGX_HIP(c, launch_mc_stage2(c->pos.as<double>(), c->site_atom.as<int>(), x.mc_naff.as<int>(),
                               c->has_soap ? c->esite.as<double>() : nullptr,
-                              x.mc_ecur.as<double>(), x.mc_twob.as<TwoBodyDev>(), r.n2b, r.seed,
-                              r.kT, ctl, dctl, x.mc_trace.as<double>(), c->stream));
+                              x.mc_ecur.as<double>(), x.mc_esur.as<double>(),
+                              x.mc_esur_cur.as<double>(), r.seed, r.kT, ctl, dctl,
+                              x.mc_trace.as<double>(), c->stream));
   return TGX_OK;
 }
 
-int tgx_mc_run(tgx_ctx* c, int n, const int* species, const double* lattice, double* pos,
-               int n_trials, double temp, double move_max, double skin, uint64_t seed,
-               double* e_trace, int* n_accepted, int* n_rebuilds) {
+// Surrogate energies of every atom of the current state: SOAP of all sites
+// and the MLP (the trial sequence then keeps them current).
+static int mc_refresh_esur(tgx_ctx* c, const McRun& r, int n_sites) {
+  Extra& x = X(c);
+  // all sites: site_atom is the trial's site list, so the iota list is used
+  if (x.iota.ensure(sizeof(int) * (size_t)r.n)) return set_err(c, TGX_ERR_HIP, "allocation failed");
+  std::vector<int> io(r.n);
+  for (int i = 0; i < r.n; i++) io[i] = i;
+  TGX_HIP(c, hipMemcpyAsync(x.iota.p, io.data(), sizeof(int) * r.n, hipMemcpyHostToDevice,
+                            c->stream));
+  TGX_HIP(c, hipMemcpyAsync(c->site_atom.p, x.iota.p, sizeof(int) * r.n,
+                            hipMemcpyDeviceToDevice, c->stream));
+  int rcode = soap_forward(c, r.n, Geom{r.cell}, n_sites);
+  if (rcode) return rcode;
+  const int D = c->dim;
+  hipError_t e =
+      c->precision == TGX_FP64
+          ? launch_mlp_forward<double>(n_sites, D, c->q.as<double>(), c->dim_pad, nullptr,
+                                       x.sg_P.as<float>(), x.sg_W1t.as<unsigned short>(),
+                                       x.sg_W2t.as<unsigned short>(), x.sg_norm.as<float>(),
+                                       x.mc_esur_cur.as<double>(), nullptr, nullptr, nullptr,
+                                       c->stream)
+          : launch_mlp_forward<float>(n_sites, D, c->q.as<float>(), c->dim_pad, nullptr,
+                                      x.sg_P.as<float>(), x.sg_W1t.as<unsigned short>(),
+                                      x.sg_W2t.as<unsigned short>(), x.sg_norm.as<float>(),
+                                      x.mc_esur_cur.as<double>(), nullptr, nullptr, nullptr,
+                                      c->stream);
+  TGX_HIP(c, e);
+  return TGX_OK;
+}
+
+static int mc_run_impl(tgx_ctx* c, int n, const int* species, const double* lattice,
+                       double* pos, int n_trials, double temp, double move_max, double skin,
+                       uint64_t seed, const tgx_da_options* da, double* e_trace,
+                       tgx_mc_stats* stats) {
   if (!c) return TGX_ERR_ARG;
   if (n < 2 || !species || !lattice || !pos || n_trials < 0 || !(temp > 0) ||
       !(move_max > 0))
@@ -1139,8 +1396,21 @@ int tgx_mc_run(tgx_ctx* c, int n, const int* species, const double* lattice, dou
                    std::sqrt(3.0) * move_max);
   if (!c->has_soap && c->twob.empty()) return set_err(c, TGX_ERR_STATE, "no model loaded");
   if (X(c).md.active) return set_err(c, TGX_ERR_STATE, "end the MD run first (tgx_md_end)");
-  (void)hipSetDevice(c->device);
   Extra& x = X(c);
+  if (da) {
+    if (!c->has_soap) return set_err(c, TGX_ERR_STATE, "delayed acceptance needs a SOAP model");
+    int rc = sg_ready(c);
+    if (rc) return rc;
+    if (x.sg_dim != c->dim)
+      return set_err(c, TGX_ERR_ARG, "surrogate dimension %d != SOAP dimension %d", x.sg_dim,
+                     c->dim);
+    if (c->sd.central_species != 0)
+      return set_err(c, TGX_ERR_UNSUPPORTED, "delayed acceptance needs every atom as a site");
+    if (da->n_train_trials < 0 || da->train_every < 64 || da->epochs < 1 || da->batch < 16 ||
+        !(da->lr > 0) || !(da->max_rmse > 0))
+      return set_err(c, TGX_ERR_ARG, "bad delayed-acceptance options");
+  }
+  (void)hipSetDevice(c->device);
   int rcode = upload_structure(c, n, pos, species);
   if (rcode) return rcode;
   McRun r{};
@@ -1151,6 +1421,8 @@ int tgx_mc_run(tgx_ctx* c, int n, const int* species, const double* lattice, dou
   r.seed = seed;
   r.n2b = (int)c->twob.size();
   r.central = c->has_soap ? c->sd.central_species : 0;
+  r.rsoap = c->has_soap ? c->sd.rcut_max : 0.0;
+  r.da = da != nullptr;
   r.rcut_list = rcut_all(c) + skin;
   std::string err;
   rcode = make_cell(lattice, r.rcut_list, &r.cell, &err);
@@ -1175,6 +1447,8 @@ int tgx_mc_run(tgx_ctx* c, int n, const int* species, const double* lattice, dou
   }
   r.cap = c->maxnb + 1;
   if (r.cap > 1024) return set_err(c, TGX_ERR_UNSUPPORTED, "neighbour lists too long for MC");
+  const int n_train = da ? std::min(da->n_train_trials, n_trials) : 0;
+  r.cap_s = (da && n_train > 0) ? 65536 : 0;
   int n_sites = 0;
   rcode = setup_sites(c, n, species, &n_sites);
   if (rcode) return rcode;
@@ -1182,12 +1456,26 @@ int tgx_mc_run(tgx_ctx* c, int n, const int* species, const double* lattice, dou
   x.ev_on = false;
   rcode = ef_launch(c, n, n_sites, Geom{r.cell}, 0, false);
   if (rcode) return rcode;
-  if (x.mc_pref.ensure(24 * (size_t)n) || x.mc_ctl.ensure(sizeof(int) * 16) ||
+  x.ev_on = false;
+  if (x.mc_pref.ensure(24 * (size_t)n) || x.mc_ctl.ensure(sizeof(int) * MC_NCTL) ||
       x.mc_dctl.ensure(sizeof(double) * MC_NDCTL) || x.mc_ecur.ensure(8 * (size_t)n) ||
       x.mc_naff.ensure(sizeof(int) * 4) || x.mc_trace.ensure(8 * ((size_t)n_trials + 1)) ||
-      x.mc_twob.ensure(sizeof(TwoBodyDev) * std::max(1, r.n2b)))
+      x.mc_twob.ensure(sizeof(TwoBodyDev) * std::max(1, r.n2b)) ||
+      x.mc_esur.ensure(8 * (size_t)r.cap) || x.mc_esur_cur.ensure(8 * (size_t)n) ||
+      (r.cap_s && (x.mc_sq.ensure(sizeof(float) * (size_t)r.cap_s * c->dim) ||
+                   x.mc_sy.ensure(8 * (size_t)r.cap_s))))
     return set_err(c, TGX_ERR_HIP, "device allocation failed (mc)");
   ENSURE(c, site_atom, sizeof(int) * (size_t)std::max(n, r.cap));
+  if (c->has_soap) {
+    // descriptor buffers for the larger of a trial's sites and all sites, so
+    // that refreshing the surrogate energies never moves a buffer the
+    // captured trial graph uses
+    const size_t rows = (size_t)std::max(n_sites, r.cap), ts = tsize(c->precision);
+    ENSURE(c, q, ts * rows * c->dim_pad);
+    ENSURE(c, cbuf, ts * rows * c->sd.nrp * c->sd.L2);
+    ENSURE(c, pnorm, ts * rows);
+    if (c->precision == TGX_MIXED) ENSURE(c, qb, 2 * rows * c->dim_pad);
+  }
   TGX_HIP(c, launch_mc_init(n, c->site_of_atom.as<int>(),
                             (c->has_soap && n_sites > 0) ? c->esite.as<double>() : nullptr,
                             x.mc_ecur.as<double>(), c->stream));
@@ -1198,20 +1486,31 @@ int tgx_mc_run(tgx_ctx* c, int n, const int* species, const double* lattice, dou
                             c->stream));
   double e0;
   TGX_HIP(c, hipMemcpyAsync(&e0, x.out.as<double>() + 9, 8, hipMemcpyDeviceToHost, c->stream));
-  TGX_HIP(c, hipMemsetAsync(x.mc_ctl.p, 0, sizeof(int) * 16, c->stream));
+  TGX_HIP(c, hipMemsetAsync(x.mc_ctl.p, 0, sizeof(int) * MC_NCTL, c->stream));
   TGX_HIP(c, hipMemsetAsync(c->nb_flag.p, 0, sizeof(int) * 4, c->stream));
   TGX_HIP(c, hipMemsetAsync(x.mc_naff.p, 0, sizeof(int) * 4, c->stream));
   TGX_HIP(c, hipMemsetAsync(c->site_atom.p, 0, sizeof(int) * (size_t)r.cap, c->stream));
+  TGX_HIP(c, hipMemsetAsync(x.mc_esur.p, 0, 8 * (size_t)r.cap, c->stream));
   TGX_HIP(c, hipStreamSynchronize(c->stream));
   // a dry trial (MC_END = 0: every control kernel returns at once) sizes the
-  // SOAP and GEMM buffers before anything is captured; no timing events
-  x.ev_on = false;
+  // SOAP and GEMM buffers before anything is captured
   rcode = mc_trial_launch(c, r);
   if (rcode) return rcode;
+  if (r.da) {
+    rcode = sg_work(c, std::max(n, r.cap));
+    if (rcode) return rcode;
+  }
   std::vector<double> d(MC_NDCTL, 0.0);
   d[MC_ETOT] = e0;
-  int ctl_h[16] = {0};
+  d[MC_MAXMS] = da ? da->max_rmse * da->max_rmse : 0.0;
+  int ctl_h[MC_NCTL] = {0};
   ctl_h[MC_END] = n_trials;
+  ctl_h[MC_COLLECT] = n_train > 0 ? 1 : 0;
+  ctl_h[MC_DA] = (da && n_train == 0) ? 1 : 0;
+  if (ctl_h[MC_DA]) {
+    rcode = mc_refresh_esur(c, r, n_sites);
+    if (rcode) return rcode;
+  }
   TGX_HIP(c, hipMemcpyAsync(x.mc_dctl.p, d.data(), sizeof(double) * MC_NDCTL,
                             hipMemcpyHostToDevice, c->stream));
   TGX_HIP(c, hipMemcpyAsync(x.mc_ctl.p, ctl_h, sizeof(ctl_h), hipMemcpyHostToDevice, c->stream));
@@ -1222,36 +1521,71 @@ int tgx_mc_run(tgx_ctx* c, int n, const int* species, const double* lattice, dou
   const bool use_graph = !(ng && ng[0] == '1') && n_trials > 0;
   hipGraph_t g = nullptr;
   hipGraphExec_t ge = nullptr;
+  auto cleanup = [&]() {
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (g) (void)hipGraphDestroy(g);
+    ge = nullptr;
+    g = nullptr;
+    x.ev_on = true;
+  };
   if (use_graph) {
     TGX_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     for (int b = 0; b < TB && !rcode; b++) rcode = mc_trial_launch(c, r);
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (rcode || e != hipSuccess) {
-      if (g) (void)hipGraphDestroy(g);
-      x.ev_on = true;
+      cleanup();
       return rcode ? rcode
                    : set_err(c, TGX_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
     }
     e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
     if (e != hipSuccess) {
-      (void)hipGraphDestroy(g);
-      x.ev_on = true;
+      cleanup();
       return set_err(c, TGX_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
     }
-    for (int t = 0; t < n_trials; t += TB) {
+  }
+  // trials in blocks of TB (one replay each); while training data are
+  // collected, the surrogate is trained every train_every trials and the
+  // surrogate stage switched on after n_train trials
+  double train_rmse = 0.0;
+  const int every = da ? std::max(TB, da->train_every / TB * TB) : 0;
+  int done = 0, last_train = 0;
+  while (done < n_trials && !rcode) {
+    if (ge) {
       hipError_t le = hipGraphLaunch(ge, c->stream);
-      if (le != hipSuccess) {
-        rcode = set_err(c, TGX_ERR_HIP, "graph launch failed: %s", hipGetErrorString(le));
-        break;
+      if (le != hipSuccess) rcode = set_err(c, TGX_ERR_HIP, "graph launch: %s", hipGetErrorString(le));
+    } else {
+      for (int b = 0; b < TB && !rcode; b++) rcode = mc_trial_launch(c, r);
+    }
+    done = std::min(n_trials, done + TB);
+    if (rcode || n_train == 0 || last_train >= n_train) continue;
+    if (done - last_train >= every || done >= n_train) {
+      int cur = 0;
+      TGX_HIP(c, hipMemcpyAsync(&cur, x.mc_ctl.as<int>() + MC_CURSOR, sizeof(int),
+                                hipMemcpyDeviceToHost, c->stream));
+      TGX_HIP(c, hipStreamSynchronize(c->stream));
+      const int n_s = std::min(cur, r.cap_s);
+      if (n_s >= 16) {
+        rcode = sg_train(c, n_s, x.mc_sq.as<float>(), x.mc_sy.as<double>(), da->epochs,
+                         da->batch, (float)da->lr, seed ^ (uint64_t)done, &train_rmse);
+        if (rcode) break;
+      }
+      last_train = done;
+      if (done >= n_train) {
+        // switch to delayed acceptance from the next trial on
+        rcode = mc_refresh_esur(c, r, n_sites);
+        if (rcode) break;
+        int on[2] = {1, 0};
+        TGX_HIP(c, hipMemcpyAsync(x.mc_ctl.as<int>() + MC_DA, &on[0], sizeof(int),
+                                  hipMemcpyHostToDevice, c->stream));
+        TGX_HIP(c, hipMemcpyAsync(x.mc_ctl.as<int>() + MC_COLLECT, &on[1], sizeof(int),
+                                  hipMemcpyHostToDevice, c->stream));
+        TGX_HIP(c, hipStreamSynchronize(c->stream));
+        last_train = n_train;
       }
     }
-  } else {
-    for (int t = 0; t < n_trials && !rcode; t++) rcode = mc_trial_launch(c, r);
   }
   hipError_t se = hipStreamSynchronize(c->stream);
-  if (ge) (void)hipGraphExecDestroy(ge);
-  if (g) (void)hipGraphDestroy(g);
-  x.ev_on = true;
+  cleanup();
   if (rcode) return rcode;
   if (se != hipSuccess) return set_err(c, TGX_ERR_HIP, "%s", hipGetErrorString(se));
   int flag[4];
@@ -1260,15 +1594,198 @@ int tgx_mc_run(tgx_ctx* c, int n, const int* species, const double* lattice, dou
     return set_err(c, TGX_ERR_CAPACITY, "neighbour capacity exceeded during the run (%d > %d)",
                    flag[0], c->maxnb);
   TGX_HIP(c, hipMemcpy(ctl_h, x.mc_ctl.p, sizeof(ctl_h), hipMemcpyDeviceToHost));
+  TGX_HIP(c, hipMemcpy(d.data(), x.mc_dctl.p, sizeof(double) * MC_NDCTL, hipMemcpyDeviceToHost));
   TGX_HIP(c, hipMemcpy(pos, c->pos.p, 24 * (size_t)n, hipMemcpyDeviceToHost));
   if (e_trace)
     TGX_HIP(c, hipMemcpy(e_trace, x.mc_trace.p, 8 * ((size_t)n_trials + 1),
                          hipMemcpyDeviceToHost));
-  if (n_accepted) *n_accepted = ctl_h[MC_NACC];
-  if (n_rebuilds) *n_rebuilds = ctl_h[MC_NREB];
+  if (stats) {
+    stats->n_accepted = ctl_h[MC_NACC];
+    stats->n_stage1_rejected = ctl_h[MC_NREJ1];
+    stats->n_gap_evaluations = ctl_h[MC_NGAP];
+    stats->n_rebuilds = ctl_h[MC_NREB];
+    stats->train_rmse = train_rmse;
+    stats->da_rmse = std::sqrt(d[MC_EMA]);
+    stats->fallback = ctl_h[MC_FALLBACK];
+    stats->pad_ = 0;
+  }
   if (ctl_h[MC_STEP] != n_trials)
     return set_err(c, TGX_ERR_STATE, "ran %d of %d trials", ctl_h[MC_STEP], n_trials);
   return TGX_OK;
+}
+
+int tgx_mc_run(tgx_ctx* c, int n, const int* species, const double* lattice, double* pos,
+               int n_trials, double temp, double move_max, double skin, uint64_t seed,
+               double* e_trace, int* n_accepted, int* n_rebuilds) {
+  tgx_mc_stats st{};
+  const int rc = mc_run_impl(c, n, species, lattice, pos, n_trials, temp, move_max, skin, seed,
+                             nullptr, e_trace, &st);
+  if (rc == TGX_OK) {
+    if (n_accepted) *n_accepted = (int)st.n_accepted;
+    if (n_rebuilds) *n_rebuilds = (int)st.n_rebuilds;
+  }
+  return rc;
+}
+
+int tgx_mc_run_da(tgx_ctx* c, int n, const int* species, const double* lattice, double* pos,
+                  int n_trials, double temp, double move_max, double skin, uint64_t seed,
+                  const tgx_da_options* opt, double* e_trace, tgx_mc_stats* stats) {
+  if (!c) return TGX_ERR_ARG;
+  if (!opt) return set_err(c, TGX_ERR_ARG, "options required");
+  return mc_run_impl(c, n, species, lattice, pos, n_trials, temp, move_max, skin, seed, opt,
+                     e_trace, stats);
+}
+
+int tgx_surrogate_init(tgx_ctx* c, int dim, uint64_t seed) {
+  if (!c) return TGX_ERR_ARG;
+  if (dim < 1 || dim > mlp_max_dim())
+    return set_err(c, TGX_ERR_UNSUPPORTED, "surrogate input dimension %d not in 1..%d", dim,
+                   mlp_max_dim());
+  (void)hipSetDevice(c->device);
+  Extra& x = X(c);
+  const int H = mlp_hidden();
+  const size_t NP = mlp_nparams(dim);
+  std::vector<float> P;
+  sg_init_params(dim, H, seed, &P);
+  if (x.sg_P.ensure(4 * NP) || x.sg_m.ensure(4 * NP) || x.sg_v.ensure(4 * NP) ||
+      x.sg_grad.ensure(4 * NP) || x.sg_W1t.ensure(2 * (size_t)dim * H) ||
+      x.sg_W2t.ensure(2 * (size_t)H * H))
+    return set_err(c, TGX_ERR_HIP, "allocation failed (surrogate)");
+  TGX_HIP(c, hipMemcpyAsync(x.sg_P.p, P.data(), 4 * NP, hipMemcpyHostToDevice, c->stream));
+  TGX_HIP(c, hipMemsetAsync(x.sg_m.p, 0, 4 * NP, c->stream));
+  TGX_HIP(c, hipMemsetAsync(x.sg_v.p, 0, 4 * NP, c->stream));
+  TGX_HIP(c, launch_mlp_cast(dim, x.sg_P.as<float>(), x.sg_W1t.as<unsigned short>(),
+                             x.sg_W2t.as<unsigned short>(), c->stream));
+  TGX_HIP(c, hipStreamSynchronize(c->stream));
+  x.sg_dim = dim;
+  x.sg_step = 0;
+  x.sg_mu = 0.f;
+  x.sg_sigma = 1.f;
+  x.sg_norm_set = false;
+  return sg_push_norm(c);
+}
+
+int tgx_surrogate_set(tgx_ctx* c, int n_params, const double* P, double mu, double sigma) {
+  if (!c) return TGX_ERR_ARG;
+  int rc = sg_ready(c);
+  if (rc) return rc;
+  Extra& x = X(c);
+  const size_t NP = mlp_nparams(x.sg_dim);
+  if (!P || n_params != (int)NP || !(sigma > 0))
+    return set_err(c, TGX_ERR_ARG, "expected %zu parameters and sigma > 0", NP);
+  std::vector<float> f(P, P + NP);
+  TGX_HIP(c, hipMemcpyAsync(x.sg_P.p, f.data(), 4 * NP, hipMemcpyHostToDevice, c->stream));
+  TGX_HIP(c, launch_mlp_cast(x.sg_dim, x.sg_P.as<float>(), x.sg_W1t.as<unsigned short>(),
+                             x.sg_W2t.as<unsigned short>(), c->stream));
+  TGX_HIP(c, hipStreamSynchronize(c->stream));
+  x.sg_mu = (float)mu;
+  x.sg_sigma = (float)sigma;
+  x.sg_norm_set = true;
+  return sg_push_norm(c);
+}
+
+int tgx_surrogate_get(tgx_ctx* c, int n_params, double* P, double* mu, double* sigma) {
+  if (!c) return TGX_ERR_ARG;
+  int rc = sg_ready(c);
+  if (rc) return rc;
+  Extra& x = X(c);
+  const size_t NP = mlp_nparams(x.sg_dim);
+  if (n_params != (int)NP || !P) return set_err(c, TGX_ERR_ARG, "expected %zu parameters", NP);
+  std::vector<float> f(NP);
+  TGX_HIP(c, hipMemcpy(f.data(), x.sg_P.p, 4 * NP, hipMemcpyDeviceToHost));
+  for (size_t k = 0; k < NP; k++) P[k] = f[k];
+  if (mu) *mu = x.sg_mu;
+  if (sigma) *sigma = x.sg_sigma;
+  return TGX_OK;
+}
+
+// Host descriptors (n x dim, row-major, FP64) to the device sample matrix.
+static int sg_upload(tgx_ctx* c, int n, const double* q, const double* y) {
+  Extra& x = X(c);
+  const int D = x.sg_dim;
+  if (x.sg_q.ensure(sizeof(float) * (size_t)n * D) || x.mc_sy.ensure(8 * (size_t)n))
+    return set_err(c, TGX_ERR_HIP, "allocation failed (surrogate)");
+  std::vector<float> f((size_t)n * D);
+  for (size_t k = 0; k < f.size(); k++) f[k] = (float)q[k];
+  TGX_HIP(c, hipMemcpyAsync(x.sg_q.p, f.data(), f.size() * 4, hipMemcpyHostToDevice, c->stream));
+  if (y) TGX_HIP(c, hipMemcpyAsync(x.mc_sy.p, y, 8 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  TGX_HIP(c, hipStreamSynchronize(c->stream));
+  return TGX_OK;
+}
+
+int tgx_surrogate_predict(tgx_ctx* c, int n, const double* q, double* e) {
+  if (!c) return TGX_ERR_ARG;
+  int rc = sg_ready(c);
+  if (rc) return rc;
+  if (n < 1 || !q || !e) return set_err(c, TGX_ERR_ARG, "bad arguments");
+  (void)hipSetDevice(c->device);
+  Extra& x = X(c);
+  rc = sg_upload(c, n, q, nullptr);
+  if (!rc) rc = sg_work(c, n);
+  if (rc) return rc;
+  TGX_HIP(c, launch_mlp_forward<float>(n, x.sg_dim, x.sg_q.as<float>(), x.sg_dim, nullptr,
+                                       x.sg_P.as<float>(), x.sg_W1t.as<unsigned short>(),
+                                       x.sg_W2t.as<unsigned short>(), x.sg_norm.as<float>(),
+                                       x.sg_e.as<double>(), nullptr, nullptr, nullptr, c->stream));
+  TGX_HIP(c, hipMemcpyAsync(e, x.sg_e.p, 8 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  TGX_HIP(c, hipStreamSynchronize(c->stream));
+  return TGX_OK;
+}
+
+int tgx_surrogate_gradient(tgx_ctx* c, int n, const double* q, const double* y, double* grad,
+                           double* loss) {
+  if (!c) return TGX_ERR_ARG;
+  int rc = sg_ready(c);
+  if (rc) return rc;
+  if (n < 1 || !q || !y || !grad) return set_err(c, TGX_ERR_ARG, "bad arguments");
+  (void)hipSetDevice(c->device);
+  Extra& x = X(c);
+  const int D = x.sg_dim;
+  rc = sg_upload(c, n, q, y);
+  if (!rc) rc = sg_work(c, n);
+  if (rc) return rc;
+  TGX_HIP(c, launch_mlp_forward<float>(n, D, x.sg_q.as<float>(), D, nullptr, x.sg_P.as<float>(),
+                                       x.sg_W1t.as<unsigned short>(),
+                                       x.sg_W2t.as<unsigned short>(), x.sg_norm.as<float>(),
+                                       x.sg_e.as<double>(), x.sg_z1.as<float>(),
+                                       x.sg_z2.as<float>(), nullptr, c->stream));
+  TGX_HIP(c, launch_mlp_gradient<float>(n, D, x.sg_q.as<float>(), D, nullptr,
+                                        x.mc_sy.as<double>(), x.sg_e.as<double>(),
+                                        x.sg_z1.as<float>(), x.sg_z2.as<float>(),
+                                        x.sg_P.as<float>(), x.sg_W2t.as<unsigned short>(),
+                                        x.sg_norm.as<float>(), x.sg_part.as<float>(),
+                                        x.sg_grad.as<float>(), c->stream));
+  const size_t NP = mlp_nparams(D);
+  std::vector<float> g(NP);
+  std::vector<double> e(n);
+  TGX_HIP(c, hipMemcpyAsync(g.data(), x.sg_grad.p, 4 * NP, hipMemcpyDeviceToHost, c->stream));
+  TGX_HIP(c, hipMemcpyAsync(e.data(), x.sg_e.p, 8 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  TGX_HIP(c, hipStreamSynchronize(c->stream));
+  for (size_t k = 0; k < NP; k++) grad[k] = g[k];
+  if (loss) {
+    double a = 0;
+    for (int i = 0; i < n; i++) {
+      const double r = (e[i] - y[i]) / x.sg_sigma;
+      a += r * r;
+    }
+    *loss = a / n;
+  }
+  return TGX_OK;
+}
+
+int tgx_surrogate_fit(tgx_ctx* c, int n, const double* q, const double* y, int epochs,
+                      int batch, double lr, uint64_t seed, double* rmse) {
+  if (!c) return TGX_ERR_ARG;
+  int rc = sg_ready(c);
+  if (rc) return rc;
+  if (n < 16 || !q || !y || epochs < 1 || batch < 16 || !(lr > 0))
+    return set_err(c, TGX_ERR_ARG, "bad arguments");
+  (void)hipSetDevice(c->device);
+  Extra& x = X(c);
+  rc = sg_upload(c, n, q, y);
+  if (rc) return rc;
+  return sg_train(c, n, x.sg_q.as<float>(), x.mc_sy.as<double>(), epochs, batch, (float)lr, seed,
+                  rmse);
 }
 
 // ---- molecular dynamics ---------------------------------------------------

@@ -97,7 +97,8 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(int M, int N, int K,
                                                      const TI* __restrict__ A, int lda,
                                                      const TI* __restrict__ B, int ldb,
                                                      EpiKernel ek, EpiStore es, int ntx,
-                                                     int ntiles) {
+                                                     int ntiles, const int* __restrict__ guard) {
+  if (guard && *guard == 0) return;
   typedef MF<TI> F;
   typedef typename F::acc_t acc_t;
   typedef typename F::vacc vacc;
@@ -234,7 +235,8 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(int M, int N, int K,
 
 template <typename TA>
 __global__ void esum_kernel(int M, int ntiles, const TA* epart, double d2,
-                            double* esite) {
+                            double* esite, const int* guard) {
+  if (guard && *guard == 0) return;
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= M) return;
   double s = 0.0;
@@ -246,7 +248,7 @@ template <typename TI>
 static hipError_t run(int n_sites, int dim, int dim_pad, int n_sparse, int sp_pad,
                       const void* q, const void* qs, const void* qsT, const void* alphas,
                       double delta, double zeta, void* wmat, void* epart, double* esite,
-                      void* G, int want_grad, hipStream_t st) {
+                      void* G, int want_grad, const int* guard, hipStream_t st) {
   typedef typename MF<TI>::acc_t acc_t;
   const int M = n_sites;
   if (M == 0) return hipSuccess;
@@ -258,14 +260,14 @@ static hipError_t run(int n_sites, int dim, int dim_pad, int n_sparse, int sp_pa
   const int nt1 = sp_pad / 64;
   hipLaunchKernelGGL((gemm_nt_kernel<TI, 0>), dim3((nt1 * mt + 7) / 8 * 8), dim3(256), 0, st, M,
                      sp_pad, dim_pad, (const TI*)q, dim_pad, (const TI*)qs, dim_pad, ek, es, nt1,
-                     nt1 * mt);
+                     nt1 * mt, guard);
   hipLaunchKernelGGL(esum_kernel<acc_t>, dim3((M + 255) / 256), dim3(256), 0, st, M, nt1,
-                     (const acc_t*)epart, delta * delta, esite);
+                     (const acc_t*)epart, delta * delta, esite, guard);
   if (want_grad) {
     const int nt2 = (dim_pad + 63) / 64;
     hipLaunchKernelGGL((gemm_nt_kernel<TI, 1>), dim3((nt2 * mt + 7) / 8 * 8), dim3(256), 0, st,
                        M, dim_pad, sp_pad, (const TI*)wmat, sp_pad, (const TI*)qsT, sp_pad, ek,
-                       es, nt2, nt2 * mt);
+                       es, nt2, nt2 * mt, guard);
   }
   return hipGetLastError();
 }
@@ -274,15 +276,15 @@ hipError_t launch_gap_kernel(int mode, int n_sites, int dim, int dim_pad, int n_
                              int sp_pad, const void* q, const void* qs, const void* qsT,
                              const void* alphas, double delta, double zeta, void* wmat,
                              void* epart, double* esite, void* G, int want_grad,
-                             hipStream_t st) {
+                             const int* guard, hipStream_t st) {
   if (mode == TGX_FP64)
     return run<double>(n_sites, dim, dim_pad, n_sparse, sp_pad, q, qs, qsT, alphas, delta, zeta,
-                       wmat, epart, esite, G, want_grad, st);
+                       wmat, epart, esite, G, want_grad, guard, st);
   if (mode == TGX_FP32)
     return run<float>(n_sites, dim, dim_pad, n_sparse, sp_pad, q, qs, qsT, alphas, delta, zeta,
-                      wmat, epart, esite, G, want_grad, st);
+                      wmat, epart, esite, G, want_grad, guard, st);
   return run<u16>(n_sites, dim, dim_pad, n_sparse, sp_pad, q, qs, qsT, alphas, delta, zeta, wmat,
-                  epart, esite, G, want_grad, st);
+                  epart, esite, G, want_grad, guard, st);
 }
 
 }  // namespace tgx

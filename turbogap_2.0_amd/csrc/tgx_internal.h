@@ -68,8 +68,32 @@ struct TwoBodyDev {
 };
 
 // Monte Carlo control words on the device (int and double arrays).
-enum { MC_REBUILD = 0, MC_STEP, MC_END, MC_LIVE, MC_ATOM, MC_NREB, MC_NACC, MC_NCTL };
-enum { MC_OLD = 0, MC_ETOT = 3, MC_NDCTL = 8 };
+enum {
+  MC_REBUILD = 0,  // list rebuild requested by this trial
+  MC_STEP,         // trials done
+  MC_END,          // trials to run
+  MC_LIVE,         // this trial runs
+  MC_ATOM,         // moved atom
+  MC_NREB,         // list rebuilds
+  MC_NACC,         // accepted trials
+  MC_GATE,         // this trial reaches the GAP evaluation
+  MC_DA,           // delayed acceptance active (surrogate first stage)
+  MC_FALLBACK,     // surrogate drifted: plain Metropolis from here on
+  MC_COLLECT,      // store (q, GAP site energy) samples for training
+  MC_NREJ1,        // trials rejected by the surrogate stage
+  MC_NGAP,         // trials evaluated with the GAP
+  MC_CURSOR,       // sample ring cursor (total samples written)
+  MC_NCTL = 16
+};
+enum {
+  MC_OLD = 0,   // 0..2 position of the moved atom before the move
+  MC_ETOT = 3,  // current total energy
+  MC_DESUR,     // surrogate energy change of this trial (stage 1)
+  MC_DE2B,      // exact 2-body energy change of this trial
+  MC_EMA,       // running mean of (dE_gap - dE_sur)^2 over GAP-evaluated trials
+  MC_MAXMS,     // fall back when MC_EMA exceeds this
+  MC_NDCTL = 8
+};
 
 // Growable device buffer.
 struct DevBuf {
@@ -141,7 +165,7 @@ hipError_t launch_gap_kernel(int mode, int n_sites, int dim, int dim_pad,
                              const void* qs, const void* qsT,
                              const void* alphas, double delta, double zeta,
                              void* wmat, void* epart, double* esite, void* G,
-                             int want_grad, hipStream_t st);
+                             int want_grad, const int* guard, hipStream_t st);
 template <typename T>
 hipError_t launch_force_gather(int n, int maxnb, const int* count,
                                const int* list, const int* rev,

@@ -11,12 +11,21 @@
 //             its list neighbours (the list holds every atom within
 //             rcut + skin of the build positions, which covers rcut around
 //             both the old and the new position);
-//   SOAP forward and the energy GEMM on those sites (library kernels);
-//   decide    dE = sum of site-energy changes + the exact 2-body change,
-//             p = min(1, exp(-dE/(kB T))), accept when p > u; on rejection
-//             the atom goes back, on acceptance the site energies are kept.
+//   SOAP forward on those sites (library kernel);
+//   stage 1   (delayed acceptance only) the surrogate MLP energy change
+//             dE_s of the same sites plus the exact 2-body change; accepted
+//             with min(1, exp(-dE_s/(kB T))) > u_4, else the trial ends here;
+//   energy GEMM on the sites (skipped by a device flag after a stage-1
+//             rejection) and, while training data are collected, a copy of
+//             (q, GAP site energy) into the sample ring;
+//   stage 2   dE = sum of site-energy changes + the exact 2-body change;
+//             plain Metropolis accepts with min(1, exp(-dE/(kB T))) > u_4,
+//             delayed acceptance (Christen & Fox, J. Comput. Graph. Stat. 14,
+//             795 (2005)) with min(1, exp(-(dE - dE_s)/(kB T))) > u_5.  On
+//             rejection the atom goes back; on acceptance the site energies
+//             are kept.
 // Sequential semantics are exact: every trial sees the state left by the
-// previous one.
+// previous one, and the chain samples exp(-E/(kB T)) in both modes.
 #include <hip/hip_runtime.h>
 
 #include "tgx_device.h"
@@ -25,10 +34,10 @@
 namespace tgx {
 
 // u(seed, t, k) in [0, 1): the top 53 bits of the splitmix64 finaliser of
-// seed + 0x9E3779B97F4A7C15 * (5t + k + 1).  k = 0 atom, 1..3 displacement,
+// seed + 0x9E3779B97F4A7C15 * (6t + k + 1).  k = 0 atom, 1..3 displacement,
 // 4 acceptance.
 __device__ __forceinline__ double mc_uniform(unsigned long long seed, long long t, int k) {
-  unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (unsigned long long)(5 * t + k + 1);
+  unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (unsigned long long)(6 * t + k + 1);
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
   z ^= z >> 31;
@@ -63,12 +72,14 @@ __global__ void mc_propose_kernel(int n, double* pos, const double* pref,
   }
 }
 
-// aff[0] = moved atom, aff[1..cnt] = its list neighbours; the remaining
+// aff[0] = moved atom, then the list neighbours within rsoap of its old or
+// new position (the only sites whose descriptor changes); the remaining
 // capacity repeats the moved atom (computed, never summed).  With a central
 // species only atoms of that species are sites.
-__global__ void mc_affected_kernel(const int* ctl, const int* count, const int* list,
-                                   int maxnb, const int* species, int central, int* aff,
-                                   int cap, int* n_aff) {
+__global__ void mc_affected_kernel(const int* ctl, const double* dctl, const double* pos,
+                                   Cell cell, double rsoap, const int* count, const int* list,
+                                   int maxnb, const int* species, int central, int* aff, int cap,
+                                   int* n_aff) {
   if (!ctl[MC_LIVE]) return;
   const int m = ctl[MC_ATOM];
   const int cnt = min(count[m], maxnb);
@@ -77,11 +88,16 @@ __global__ void mc_affected_kernel(const int* ctl, const int* count, const int* 
   const int t = threadIdx.x;
   if (t == 0) {
     // fixed order: the moved atom, then the list in slot order
+    const double po[3] = {dctl[MC_OLD], dctl[MC_OLD + 1], dctl[MC_OLD + 2]};
     int k = 0;
     if (central == 0 || species[m] == central) keep[k++] = m;
     for (int s = 0; s < cnt; s++) {
       const int j = list[(size_t)m * maxnb + s] & TGX_ATOM_MASK;
-      if (central == 0 || species[j] == central) keep[k++] = j;
+      if (central != 0 && species[j] != central) continue;
+      double dv[3];
+      const double rn = mic(pos + 3 * m, pos + 3 * j, cell.La, cell.Lb, cell.Lc, dv);
+      const double ro = mic(po, pos + 3 * j, cell.La, cell.Lb, cell.Lc, dv);
+      if (rn < rsoap || ro < rsoap) keep[k++] = j;
     }
     nk = k;
     *n_aff = k;
@@ -97,19 +113,12 @@ __device__ double mc_phi(const TwoBodyDev& b, double r) {
   return phi;
 }
 
-__global__ __launch_bounds__(64) void mc_decide_kernel(
-    int n, double* pos, const int* species, Cell cell, int maxnb, const int* count,
-    const int* list, const int* aff, const int* n_aff, const double* esite, double* e_cur,
-    const TwoBodyDev* twob, int n2b, unsigned long long seed, double kT, int* ctl, double* dctl,
-    double* trace) {
-  if (!ctl[MC_LIVE]) return;
-  const int lane = threadIdx.x;
-  const int t = ctl[MC_STEP];
-  const int m = ctl[MC_ATOM];
-  const int na = *n_aff;
+// Exact 2-body energy change of moving atom m from dctl[MC_OLD] to pos[m]:
+// every pair (m, j) enters E_m and E_j, so dE = 2 sum_j dphi.  Wave sum.
+__device__ double mc_de2b(const double* pos, const int* species, const Cell& cell, int maxnb,
+                          const int* count, const int* list, const TwoBodyDev* twob, int n2b,
+                          int m, const double* dctl, int lane) {
   double de = 0.0;
-  for (int k = lane; k < na; k += 64) de += esite[k] - e_cur[aff[k]];
-  // 2-body: every pair (m, j) enters E_m and E_j, so dE = 2 sum_j dphi
   const int cnt = min(count[m], maxnb);
   const double* pm = pos + 3 * m;
   const double po[3] = {dctl[MC_OLD], dctl[MC_OLD + 1], dctl[MC_OLD + 2]};
@@ -129,23 +138,117 @@ __global__ __launch_bounds__(64) void mc_decide_kernel(
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) de += __shfl_xor(de, o, 64);
+  return de;
+}
+
+__device__ void mc_finish_trial(int t, double* trace, int* ctl, const double* dctl) {
+  if (trace) trace[t + 1] = dctl[MC_ETOT];
+  ctl[MC_STEP] = t + 1;
+}
+
+// Stage 1: the exact 2-body change always; with delayed acceptance active the
+// surrogate change of the affected sites and the first accept/reject.
+__global__ __launch_bounds__(64) void mc_stage1_kernel(
+    double* pos, const int* species, Cell cell, int maxnb, const int* count, const int* list,
+    const int* aff, const int* n_aff, const double* esur, const double* esur_cur,
+    const TwoBodyDev* twob, int n2b, unsigned long long seed, double kT, int* ctl, double* dctl,
+    double* trace) {
+  if (!ctl[MC_LIVE]) return;
+  const int lane = threadIdx.x;
+  const int t = ctl[MC_STEP];
+  const int m = ctl[MC_ATOM];
+  const double de2 = mc_de2b(pos, species, cell, maxnb, count, list, twob, n2b, m, dctl, lane);
+  const bool da = ctl[MC_DA] && !ctl[MC_FALLBACK];
+  double ds = 0.0;
+  if (da) {
+    const int na = *n_aff;
+    for (int k = lane; k < na; k += 64) ds += esur[k] - esur_cur[aff[k]];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) ds += __shfl_xor(ds, o, 64);
+  }
+  if (lane == 0) {
+    dctl[MC_DE2B] = de2;
+    ctl[MC_GATE] = 1;
+    if (da) {
+      const double dsur = ds + de2;
+      dctl[MC_DESUR] = dsur;
+      double p = exp(-dsur / kT);
+      p = p < 1.0 ? p : 1.0;
+      if (!(p > mc_uniform(seed, t, 4))) {
+        ctl[MC_GATE] = 0;
+        ctl[MC_NREJ1] += 1;
+        for (int k = 0; k < 3; k++) pos[3 * m + k] = dctl[MC_OLD + k];
+        mc_finish_trial(t, trace, ctl, dctl);
+      }
+    }
+  }
+}
+
+// Stage 2 (GAP): plain Metropolis, or the delayed-acceptance correction.
+__global__ __launch_bounds__(64) void mc_stage2_kernel(
+    double* pos, const int* aff, const int* n_aff, const double* esite, double* e_cur,
+    const double* esur, double* esur_cur, unsigned long long seed, double kT, int* ctl,
+    double* dctl, double* trace) {
+  if (!ctl[MC_LIVE] || !ctl[MC_GATE]) return;
+  const int lane = threadIdx.x;
+  const int t = ctl[MC_STEP];
+  const int m = ctl[MC_ATOM];
+  const int na = *n_aff;
+  double de = 0.0;
+  if (esite)
+    for (int k = lane; k < na; k += 64) de += esite[k] - e_cur[aff[k]];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) de += __shfl_xor(de, o, 64);
+  const bool da = ctl[MC_DA] && !ctl[MC_FALLBACK];
   __shared__ int acc;
   if (lane == 0) {
-    double p = exp(-de / kT);
+    de += dctl[MC_DE2B];
+    ctl[MC_NGAP] += 1;
+    double p;
+    if (da) {
+      const double d = de - dctl[MC_DESUR];
+      p = exp(-d / kT);
+      dctl[MC_EMA] = 0.98 * dctl[MC_EMA] + 0.02 * d * d;
+      if (dctl[MC_EMA] > dctl[MC_MAXMS]) ctl[MC_FALLBACK] = 1;
+    } else {
+      p = exp(-de / kT);
+    }
     p = p < 1.0 ? p : 1.0;
-    acc = p > mc_uniform(seed, t, 4);
+    acc = p > mc_uniform(seed, t, da ? 5 : 4);
     if (acc) {
       dctl[MC_ETOT] += de;
       ctl[MC_NACC] += 1;
     } else {
       for (int k = 0; k < 3; k++) pos[3 * m + k] = dctl[MC_OLD + k];
     }
-    if (trace) trace[t + 1] = dctl[MC_ETOT];
-    ctl[MC_STEP] = t + 1;
+    mc_finish_trial(t, trace, ctl, dctl);
   }
   __syncthreads();
-  if (acc)
-    for (int k = lane; k < na; k += 64) e_cur[aff[k]] = esite[k];
+  if (acc) {
+    if (esite)
+      for (int k = lane; k < na; k += 64) e_cur[aff[k]] = esite[k];
+    if (da)
+      for (int k = lane; k < na; k += 64) esur_cur[aff[k]] = esur[k];
+  }
+}
+
+// Training samples: the descriptors and GAP energies of the trial's sites
+// (rows 0..n_aff-1 of q and esite) appended to a ring of cap_s samples.
+template <typename T>
+__global__ void mc_collect_kernel(const int* ctl, const int* n_aff, const T* q, int q_ld, int D,
+                                  const double* esite, float* sq, double* sy, int cap_s,
+                                  int* cursor) {
+  if (!ctl[MC_LIVE] || !ctl[MC_GATE] || !ctl[MC_COLLECT]) return;
+  const int na = *n_aff;
+  const int base = *cursor;
+  for (int k = 0; k < na; k++) {
+    const int slot = (base + k) % cap_s;
+    for (int d = threadIdx.x; d < D; d += blockDim.x)
+      sq[(size_t)slot * D + d] = (float)q[(size_t)k * q_ld + d];
+    if (threadIdx.x == 0) sy[slot] = esite[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *cursor = base + na;
 }
 
 // e_cur[atom] = SOAP site energy of the atom (0 for atoms that are no site).
@@ -165,24 +268,45 @@ hipError_t launch_mc_propose(int n, double* pos, const double* pref, unsigned lo
   return hipGetLastError();
 }
 
-hipError_t launch_mc_affected(const int* ctl, const int* count, const int* list, int maxnb,
-                              const int* species, int central, int* aff, int cap, int* n_aff,
-                              hipStream_t st) {
+hipError_t launch_mc_affected(const int* ctl, const double* dctl, const double* pos,
+                              const Cell& cell, double rsoap, const int* count, const int* list,
+                              int maxnb, const int* species, int central, int* aff, int cap,
+                              int* n_aff, hipStream_t st) {
   if (cap > 1024) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(mc_affected_kernel, dim3(1), dim3(256), 0, st, ctl, count, list, maxnb,
-                     species, central, aff, cap, n_aff);
+  hipLaunchKernelGGL(mc_affected_kernel, dim3(1), dim3(256), 0, st, ctl, dctl, pos, cell, rsoap,
+                     count, list, maxnb, species, central, aff, cap, n_aff);
   return hipGetLastError();
 }
 
-hipError_t launch_mc_decide(int n, double* pos, const int* species, const Cell& cell, int maxnb,
+hipError_t launch_mc_stage1(double* pos, const int* species, const Cell& cell, int maxnb,
                             const int* count, const int* list, const int* aff, const int* n_aff,
-                            const double* esite, double* e_cur, const TwoBodyDev* twob, int n2b,
-                            unsigned long long seed, double kT, int* ctl, double* dctl,
+                            const double* esur, const double* esur_cur, const TwoBodyDev* twob,
+                            int n2b, unsigned long long seed, double kT, int* ctl, double* dctl,
                             double* trace, hipStream_t st) {
-  hipLaunchKernelGGL(mc_decide_kernel, dim3(1), dim3(64), 0, st, n, pos, species, cell, maxnb,
-                     count, list, aff, n_aff, esite, e_cur, twob, n2b, seed, kT, ctl, dctl, trace);
+  hipLaunchKernelGGL(mc_stage1_kernel, dim3(1), dim3(64), 0, st, pos, species, cell, maxnb, count,
+                     list, aff, n_aff, esur, esur_cur, twob, n2b, seed, kT, ctl, dctl, trace);
   return hipGetLastError();
 }
+
+hipError_t launch_mc_stage2(double* pos, const int* aff, const int* n_aff, const double* esite,
+                            double* e_cur, const double* esur, double* esur_cur,
+                            unsigned long long seed, double kT, int* ctl, double* dctl,
+                            double* trace, hipStream_t st) {
+  hipLaunchKernelGGL(mc_stage2_kernel, dim3(1), dim3(64), 0, st, pos, aff, n_aff, esite, e_cur,
+                     esur, esur_cur, seed, kT, ctl, dctl, trace);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_mc_collect(const int* ctl, const int* n_aff, const T* q, int q_ld, int D,
+                             const double* esite, float* sq, double* sy, int cap_s, int* cursor,
+                             hipStream_t st) {
+  hipLaunchKernelGGL((mc_collect_kernel<T>), dim3(1), dim3(256), 0, st, ctl, n_aff, q, q_ld, D,
+                     esite, sq, sy, cap_s, cursor);
+  return hipGetLastError();
+}
+template hipError_t launch_mc_collect<float>(const int*, const int*, const float*, int, int, const double*, float*, double*, int, int*, hipStream_t);
+template hipError_t launch_mc_collect<double>(const int*, const int*, const double*, int, int, const double*, float*, double*, int, int*, hipStream_t);
 
 hipError_t launch_mc_init(int n, const int* site_of_atom, const double* esite, double* e_cur,
                           hipStream_t st) {

@@ -67,6 +67,29 @@ class CSoapHypers(C.Structure):
     ]
 
 
+class DaOptions(C.Structure):
+    """tgx_da_options (delayed-acceptance Monte Carlo)."""
+    _fields_ = [("n_train_trials", C.c_int32), ("train_every", C.c_int32),
+                ("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_double),
+                ("max_rmse", C.c_double)]
+
+
+class McStats(C.Structure):
+    """tgx_mc_stats."""
+    _fields_ = [("n_accepted", C.c_int64), ("n_stage1_rejected", C.c_int64),
+                ("n_gap_evaluations", C.c_int64), ("n_rebuilds", C.c_int64),
+                ("train_rmse", C.c_double), ("da_rmse", C.c_double), ("fallback", C.c_int32),
+                ("pad_", C.c_int32)]
+
+
+SURROGATE_HIDDEN = 128
+
+
+def surrogate_nparams(dim: int) -> int:
+    H = SURROGATE_HIDDEN
+    return H * dim + H * H + 3 * H + 1
+
+
 @dataclass
 class SoapHypers:
     """One soap_turbo block.  Defaults are the reference's
@@ -220,6 +243,14 @@ def load_library(path: str = LIB_PATH):
         "tgx_stage_times": (i, [P, dp]),
         "tgx_kernel_times": (i, [P, dp]),
         "tgx_set_list_buffer": (i, [P, d]),
+        "tgx_mc_run_da": (i, [P, i, ip, dp, dp, i, d, d, d, C.c_uint64,
+                              C.POINTER(DaOptions), dp, C.POINTER(McStats)]),
+        "tgx_surrogate_init": (i, [P, i, C.c_uint64]),
+        "tgx_surrogate_set": (i, [P, i, dp, d, d]),
+        "tgx_surrogate_get": (i, [P, i, dp, dp, dp]),
+        "tgx_surrogate_predict": (i, [P, i, dp, dp]),
+        "tgx_surrogate_gradient": (i, [P, i, dp, dp, dp, dp]),
+        "tgx_surrogate_fit": (i, [P, i, dp, dp, i, i, d, C.c_uint64, dp]),
         "tgx_mc_run": (i, [P, i, ip, dp, dp, i, d, d, d, C.c_uint64, dp, ip, ip]),
         "tgx_md_begin": (i, [P, i, ip, dp, dp, dp, dp, d, d, i]),
         "tgx_md_steps": (i, [P, i]),
@@ -396,6 +427,60 @@ class Engine:
                                         n_trials, temp, move_max, skin, seed, _dptr(tr),
                                         C.byref(acc), C.byref(nreb)))
         return dict(pos=pos, e_trace=tr, n_accepted=acc.value, n_rebuilds=nreb.value)
+
+    def mc_run_da(self, pos, species, lattice, n_trials: int, temp: float, move_max: float,
+                  skin: float, seed: int, n_train_trials: int = 20000, train_every: int = 1024,
+                  epochs: int = 4, batch: int = 512, lr: float = 1e-3, max_rmse: float = 0.5):
+        """Delayed-acceptance MC with the surrogate (tgx_surrogate_init first)."""
+        pos = _f64(pos).copy()
+        sp, lat = _i32(species), _f64(lattice)
+        tr = np.zeros(n_trials + 1)
+        opt = DaOptions(n_train_trials, train_every, epochs, batch, lr, max_rmse)
+        st = McStats()
+        self._check(self.lib.tgx_mc_run_da(self.ctx, len(pos), _iptr(sp), _dptr(lat),
+                                           _dptr(pos), n_trials, temp, move_max, skin, seed,
+                                           C.byref(opt), _dptr(tr), C.byref(st)))
+        out = {f[0]: getattr(st, f[0]) for f in McStats._fields_ if f[0] != "pad_"}
+        out.update(pos=pos, e_trace=tr)
+        return out
+
+    # surrogate network
+    def surrogate_init(self, dim: int, seed: int = 0):
+        self._check(self.lib.tgx_surrogate_init(self.ctx, dim, seed))
+        self._sg_dim = dim
+
+    def surrogate_set(self, P, mu: float = 0.0, sigma: float = 1.0):
+        P = _f64(P)
+        self._check(self.lib.tgx_surrogate_set(self.ctx, len(P), _dptr(P), mu, sigma))
+
+    def surrogate_get(self):
+        P = np.zeros(surrogate_nparams(self._sg_dim))
+        mu, sg = C.c_double(0), C.c_double(0)
+        self._check(self.lib.tgx_surrogate_get(self.ctx, len(P), _dptr(P), C.byref(mu),
+                                               C.byref(sg)))
+        return P, mu.value, sg.value
+
+    def surrogate_predict(self, q):
+        q = _f64(q)
+        e = np.zeros(len(q))
+        self._check(self.lib.tgx_surrogate_predict(self.ctx, len(q), _dptr(q), _dptr(e)))
+        return e
+
+    def surrogate_gradient(self, q, y):
+        q, y = _f64(q), _f64(y)
+        g = np.zeros(surrogate_nparams(q.shape[1]))
+        loss = C.c_double(0)
+        self._check(self.lib.tgx_surrogate_gradient(self.ctx, len(q), _dptr(q), _dptr(y),
+                                                    _dptr(g), C.byref(loss)))
+        return g, loss.value
+
+    def surrogate_fit(self, q, y, epochs: int = 10, batch: int = 256, lr: float = 1e-3,
+                      seed: int = 0):
+        q, y = _f64(q), _f64(y)
+        r = C.c_double(0)
+        self._check(self.lib.tgx_surrogate_fit(self.ctx, len(q), _dptr(q), _dptr(y), epochs,
+                                               batch, lr, seed, C.byref(r)))
+        return r.value
 
     def md_run(self, pos, vel, species, lattice, masses_amu, n_steps: int, dt: float,
                skin: float):
