@@ -680,7 +680,7 @@ int tgx_set_soap_model(tgx_ctx* c, const tgx_soap_hypers* h, int dim, int n_spar
   c->sd = sd;
   c->dim = dim;
   c->n_sparse = n_sparse;
-  c->dim_pad = (dim + 31) / 32 * 32;
+  c->dim_pad = (dim + 63) / 64 * 64;  // whole 64-deep k-slabs of the BF16 GEMM
   c->sp_pad = (n_sparse + 63) / 64 * 64;
   c->delta = delta;
   c->zeta = zeta;

@@ -91,7 +91,18 @@ __device__ __forceinline__ void kpow(T k, int zint, T zeta, T* kz, T* kz1) {
   }
 }
 
-// EPI 0: kernel epilogue; EPI 1: plain store.
+// Workgroup tile: 2 x 2 waves, each TM x TN MFMA tiles of 16 x 16, k-slab BK.
+// The BF16 store GEMM (G = W Qs) uses 128 x 128 tiles with a 64-deep slab
+// (more MFMA work per LDS read); the kernel-epilogue GEMM keeps 64 x 64,
+// where the per-element epilogue (powers, BF16 W stores, row sums) would
+// otherwise dominate; FP32 and FP64 use 64 x 64 (measured on gfx950).
+template <typename TI, int EPI> struct Tile {
+  static constexpr int TM = 2, TN = 2, BK = MF<TI>::BK;
+};
+template <> struct Tile<u16, 1> { static constexpr int TM = 4, TN = 4, BK = 64; };
+
+// EPI 0: kernel epilogue; EPI 1: plain store.  The next k-slab is read from
+// HBM into registers while the MFMAs of the current one run.
 template <typename TI, int EPI>
 __global__ __launch_bounds__(256) void gemm_nt_kernel(int M, int N, int K,
                                                      const TI* __restrict__ A, int lda,
@@ -102,8 +113,12 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(int M, int N, int K,
   typedef MF<TI> F;
   typedef typename F::acc_t acc_t;
   typedef typename F::vacc vacc;
-  constexpr int BM = 64, BN = 64, BK = F::BK, LDS = BK + F::PAD;
-  constexpr int EPR = 16 / sizeof(TI);  // elements per 16-byte load
+  constexpr int TM = Tile<TI, EPI>::TM, TN = Tile<TI, EPI>::TN, BK = Tile<TI, EPI>::BK;
+  constexpr int BM = 32 * TM, BN = 32 * TN, LDS = BK + F::PAD;
+  constexpr int EPR = 16 / sizeof(TI);     // elements per 16-byte load
+  constexpr int TPR = BK / EPR;            // threads per tile row
+  constexpr int RPP = 256 / TPR;           // rows per load pass
+  constexpr int PA = BM / RPP, PB = BN / RPP;
   __shared__ __align__(16) TI sA[BM * LDS];
   __shared__ __align__(16) TI sB[BN * LDS];
   __shared__ acc_t sE[2][BM];
@@ -118,53 +133,59 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(int M, int N, int K,
   if (t >= ntiles) return;
   const int tx = t % ntx, ty = t / ntx;
   const int row0 = ty * BM, col0 = tx * BN;
-  const int lr = tid >> 2, lc = (tid & 3) * EPR;
+  const int lr = tid / TPR, lc = (tid % TPR) * EPR;
 
-  vacc acc[2][2];
+  vacc acc[TM][TN];
 #pragma unroll
-  for (int a = 0; a < 2; a++)
+  for (int a = 0; a < TM; a++)
 #pragma unroll
-    for (int b = 0; b < 2; b++) acc[a][b] = vacc{0, 0, 0, 0};
+    for (int b = 0; b < TN; b++) acc[a][b] = vacc{0, 0, 0, 0};
 
+  uint4 ra[PA], rb[PB];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int p = 0; p < PA; p++) {
+      const int r = row0 + p * RPP + lr;
+      ra[p] = r < M ? *reinterpret_cast<const uint4*>(A + (size_t)r * lda + k0 + lc)
+                    : uint4{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int p = 0; p < PB; p++) {
+      const int c = col0 + p * RPP + lr;
+      rb[p] = c < N ? *reinterpret_cast<const uint4*>(B + (size_t)c * ldb + k0 + lc)
+                    : uint4{0, 0, 0, 0};
+    }
+  };
+  fetch(0);
   for (int k0 = 0; k0 < K; k0 += BK) {
-    {
-      const int r = row0 + lr;
-      TI tmp[EPR];
-      if (r < M) {
-        const uint4 v = *reinterpret_cast<const uint4*>(A + (size_t)r * lda + k0 + lc);
-        *reinterpret_cast<uint4*>(tmp) = v;
-      } else {
 #pragma unroll
-        for (int e = 0; e < EPR; e++) tmp[e] = (TI)0;
-      }
+    for (int p = 0; p < PA; p++) {
+      const TI* v = reinterpret_cast<const TI*>(&ra[p]);
 #pragma unroll
-      for (int e = 0; e < EPR; e++) sA[lr * LDS + lc + e] = tmp[e];
-      const int c = col0 + lr;
-      if (c < N) {
-        const uint4 v = *reinterpret_cast<const uint4*>(B + (size_t)c * ldb + k0 + lc);
-        *reinterpret_cast<uint4*>(tmp) = v;
-      } else {
+      for (int e = 0; e < EPR; e++) sA[(p * RPP + lr) * LDS + lc + e] = v[e];
+    }
 #pragma unroll
-        for (int e = 0; e < EPR; e++) tmp[e] = (TI)0;
-      }
+    for (int p = 0; p < PB; p++) {
+      const TI* v = reinterpret_cast<const TI*>(&rb[p]);
 #pragma unroll
-      for (int e = 0; e < EPR; e++) sB[lr * LDS + lc + e] = tmp[e];
+      for (int e = 0; e < EPR; e++) sB[(p * RPP + lr) * LDS + lc + e] = v[e];
     }
     __syncthreads();
+    if (k0 + BK < K) fetch(k0 + BK);
 #pragma unroll
     for (int ks = 0; ks < BK; ks += F::KSTEP) {
       const int kk = ks + (lane >> 4) * F::KPL;
-      typename F::frag af[2], bfr[2];
+      typename F::frag af[TM], bfr[TN];
 #pragma unroll
-      for (int mi = 0; mi < 2; mi++)
-        af[mi] = F::load(sA + (wm * 32 + mi * 16 + (lane & 15)) * LDS + kk);
+      for (int mi = 0; mi < TM; mi++)
+        af[mi] = F::load(sA + (wm * 16 * TM + mi * 16 + (lane & 15)) * LDS + kk);
 #pragma unroll
-      for (int ni = 0; ni < 2; ni++)
-        bfr[ni] = F::load(sB + (wn * 32 + ni * 16 + (lane & 15)) * LDS + kk);
+      for (int ni = 0; ni < TN; ni++)
+        bfr[ni] = F::load(sB + (wn * 16 * TN + ni * 16 + (lane & 15)) * LDS + kk);
 #pragma unroll
-      for (int mi = 0; mi < 2; mi++)
+      for (int mi = 0; mi < TM; mi++)
 #pragma unroll
-        for (int ni = 0; ni < 2; ni++) acc[mi][ni] = F::mma(af[mi], bfr[ni], acc[mi][ni]);
+        for (int ni = 0; ni < TN; ni++) acc[mi][ni] = F::mma(af[mi], bfr[ni], acc[mi][ni]);
     }
     __syncthreads();
   }
@@ -172,13 +193,13 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(int M, int N, int K,
   if (EPI == 1) {
     acc_t* C = reinterpret_cast<acc_t*>(es.C);
 #pragma unroll
-    for (int mi = 0; mi < 2; mi++)
+    for (int mi = 0; mi < TM; mi++)
 #pragma unroll
-      for (int ni = 0; ni < 2; ni++)
+      for (int ni = 0; ni < TN; ni++)
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-          const int r = row0 + wm * 32 + mi * 16 + F::row(lane, v);
-          const int c = col0 + wn * 32 + ni * 16 + (lane & 15);
+          const int r = row0 + wm * 16 * TM + mi * 16 + F::row(lane, v);
+          const int c = col0 + wn * 16 * TN + ni * 16 + (lane & 15);
           if (r < M && c < es.ldc) C[(size_t)r * es.ldc + c] = acc[mi][ni][v];
         }
     return;
@@ -187,35 +208,35 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(int M, int N, int K,
   const acc_t* al = reinterpret_cast<const acc_t*>(ek.alphas);
   const acc_t zeta = (acc_t)ek.zeta, zd2 = (acc_t)ek.zd2;
   TI* Wm = reinterpret_cast<TI*>(ek.W);
-  acc_t rs[2][4];
+  acc_t rs[TM][4];
 #pragma unroll
-  for (int mi = 0; mi < 2; mi++)
+  for (int mi = 0; mi < TM; mi++)
 #pragma unroll
     for (int v = 0; v < 4; v++) rs[mi][v] = 0;
 #pragma unroll
-  for (int ni = 0; ni < 2; ni++) {
-    const int c = col0 + wn * 32 + ni * 16 + (lane & 15);
+  for (int ni = 0; ni < TN; ni++) {
+    const int c = col0 + wn * 16 * TN + ni * 16 + (lane & 15);
     const bool valid = c < ek.n_sparse;
     const acc_t a = valid ? al[c] : (acc_t)0;
 #pragma unroll
-    for (int mi = 0; mi < 2; mi++)
+    for (int mi = 0; mi < TM; mi++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int r = row0 + wm * 32 + mi * 16 + F::row(lane, v);
+        const int r = row0 + wm * 16 * TM + mi * 16 + F::row(lane, v);
         const acc_t kv = acc[mi][ni][v];
         acc_t kz = 0, kz1 = 0;
         if (valid) kpow<acc_t>(kv, ek.zint, zeta, &kz, &kz1);
         rs[mi][v] += a * kz;
-        const acc_t w = valid ? zd2 * a * kz1 : (acc_t)0;
+        const acc_t wv = valid ? zd2 * a * kz1 : (acc_t)0;
         if (r < M && c < ek.ldw) {
-          if constexpr (sizeof(TI) == 8) Wm[(size_t)r * ek.ldw + c] = w;
-          else Wm[(size_t)r * ek.ldw + c] = cvt_out<TI>((float)w);
+          if constexpr (sizeof(TI) == 8) Wm[(size_t)r * ek.ldw + c] = wv;
+          else Wm[(size_t)r * ek.ldw + c] = cvt_out<TI>((float)wv);
         }
       }
   }
   // reduce row partials over the 16 lanes of a row group
 #pragma unroll
-  for (int mi = 0; mi < 2; mi++)
+  for (int mi = 0; mi < TM; mi++)
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       acc_t x = rs[mi][v];
@@ -223,7 +244,7 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(int M, int N, int K,
       x += __shfl_xor(x, 2, 64);
       x += __shfl_xor(x, 4, 64);
       x += __shfl_xor(x, 8, 64);
-      if ((lane & 15) == 0) sE[wn][wm * 32 + mi * 16 + F::row(lane, v)] = x;
+      if ((lane & 15) == 0) sE[wn][wm * 16 * TM + mi * 16 + F::row(lane, v)] = x;
     }
   __syncthreads();
   if (tid < BM) {
@@ -256,18 +277,21 @@ static hipError_t run(int n_sites, int dim, int dim_pad, int n_sparse, int sp_pa
   const double zz = zint > 0 ? (double)zint : zeta;
   EpiKernel ek{alphas, n_sparse, zint, zz, zz * delta * delta, wmat, sp_pad, epart};
   EpiStore es{G, dim_pad};
-  const int mt = (M + 63) / 64;
-  const int nt1 = sp_pad / 64;
-  hipLaunchKernelGGL((gemm_nt_kernel<TI, 0>), dim3((nt1 * mt + 7) / 8 * 8), dim3(256), 0, st, M,
+  constexpr int BM0 = 32 * Tile<TI, 0>::TM, BN0 = 32 * Tile<TI, 0>::TN;
+  constexpr int BM1 = 32 * Tile<TI, 1>::TM, BN1 = 32 * Tile<TI, 1>::TN;
+  const int mt0 = (M + BM0 - 1) / BM0;
+  const int nt1 = (sp_pad + BN0 - 1) / BN0;
+  hipLaunchKernelGGL((gemm_nt_kernel<TI, 0>), dim3((nt1 * mt0 + 7) / 8 * 8), dim3(256), 0, st, M,
                      sp_pad, dim_pad, (const TI*)q, dim_pad, (const TI*)qs, dim_pad, ek, es, nt1,
-                     nt1 * mt, guard);
+                     nt1 * mt0, guard);
   hipLaunchKernelGGL(esum_kernel<acc_t>, dim3((M + 255) / 256), dim3(256), 0, st, M, nt1,
                      (const acc_t*)epart, delta * delta, esite, guard);
   if (want_grad) {
-    const int nt2 = (dim_pad + 63) / 64;
-    hipLaunchKernelGGL((gemm_nt_kernel<TI, 1>), dim3((nt2 * mt + 7) / 8 * 8), dim3(256), 0, st,
+    const int mt1 = (M + BM1 - 1) / BM1;
+    const int nt2 = (dim_pad + BN1 - 1) / BN1;
+    hipLaunchKernelGGL((gemm_nt_kernel<TI, 1>), dim3((nt2 * mt1 + 7) / 8 * 8), dim3(256), 0, st,
                        M, dim_pad, sp_pad, (const TI*)wmat, sp_pad, (const TI*)qsT, sp_pad, ek,
-                       es, nt2, nt2 * mt, guard);
+                       es, nt2, nt2 * mt1, guard);
   }
   return hipGetLastError();
 }
