@@ -32,8 +32,11 @@ def _nb_equal(a, b, ulps_angles=True):
     assert np.array_equal(a["indices"], b["indices"])
 
 
-@pytest.mark.parametrize("case", ["carbon216", "small32", "cho300", "carbon2000"])
+@pytest.mark.parametrize("case", ["carbon216", "small32", "cho300", "carbon2000", "dense84",
+                                  "dense145"])
 def test_neighbors_bit_exact(eng, case):
+    """Lists of up to 64, 65-128 and over 128 entries (dense84, dense145:
+    1,500 C at 3.2 g/cm^3 with rcut 5.0 / 6.0 A) take the three sort paths."""
     if case == "carbon216":
         pos, sp, lat, _ = W.carbon216()
         rc = 3.7
@@ -43,12 +46,19 @@ def test_neighbors_bit_exact(eng, case):
     elif case == "cho300":
         pos, sp, lat, _ = W.cho()
         rc = 4.0
-    else:
+    elif case == "carbon2000":
         pos, sp, lat = S.carbon_cell(2000, 2.0, 1.2, 55)
         rc = 4.2
+    else:
+        pos, sp, lat = S.carbon_cell(1500, 3.2, 1.0, 56)
+        rc = 5.0 if case == "dense84" else 6.0
     print(f"case={case} n={len(pos)} rcut={rc}", flush=True)
     g = eng.neighbors(pos, lat, rc)
     o = oracle.neighbors(pos, lat, rc)
+    if case == "dense84":
+        assert 64 < o["n_neigh"].max() - 1 <= 128
+    if case == "dense145":
+        assert o["n_neigh"].max() - 1 > 128
     _nb_equal(g, o)
 
 
@@ -103,6 +113,30 @@ def test_kernel_step(eng, prec, etol, gtol):
     print(prec, np.abs(e - e_ref).max() / scale_e, np.abs(qss - qss_ref).max() / scale_g)
     assert np.abs(e - e_ref).max() <= etol * scale_e
     assert np.abs(qss - qss_ref).max() <= gtol * scale_g
+
+
+@pytest.mark.parametrize("zeta,n_sparse", [(4.0, 500), (2.0, 300), (3.0, 64)])
+def test_fused_bf16_kernel_step(eng, zeta, n_sparse, monkeypatch):
+    """The single-kernel BF16 step (both GEMMs in one workgroup per 128
+    sites) against the two separate GEMMs: dE/dq bit-identical, site
+    energies equal up to the order of their FP32 partial sums."""
+    pos, sp, lat, model = W.carbon216()
+    m2 = S.model_from_soaps(model.hypers, model.Qs, n_sparse, seed=5, zeta=zeta)
+    eng.set_model(m2)
+    q = oracle.soap(model.hypers, pos, sp, lat)["soap"]
+    eng.set_precision("mixed")
+    try:
+        e1, g1 = eng.kernel_step(q)
+        monkeypatch.setenv("TGX_GEMM_UNFUSED", "1")
+        e2, g2 = eng.kernel_step(q)
+        monkeypatch.delenv("TGX_GEMM_UNFUSED")
+        e3, g3 = eng.kernel_step(q[:77])  # a partial block of rows
+    finally:
+        eng.set_precision("fp64")
+    assert np.array_equal(g1, g2)
+    assert np.abs(e1 - e2).max() <= 1e-6 * np.abs(e2).max()
+    assert np.array_equal(g3, g1[:77])
+    assert np.abs(e3 - e1[:77]).max() <= 1e-6 * np.abs(e1).max()
 
 
 def test_kernel_step_fractional_zeta(eng):

@@ -21,7 +21,7 @@ hipError_t launch_build_rtab(const SoapDev* sd_dev, int n_species, const double*
 hipError_t launch_neighbors(const double* pos, int n, const Cell& c, double rcut, int* cell_of,
                             int* cell_count, int* cell_start, int* cell_fill, int* cell_atoms,
                             int* count, int maxnb, int* list, int* rev, int* flag,
-                            hipStream_t st, const int* guard);
+                            hipStream_t st, const int* guard, double* pref = nullptr);
 hipError_t launch_csr_out(const double* pos, int n, const Cell& c, int maxnb, const int* count,
                           const int* list, const long long* off, int* out_list, double* rjs,
                           double* xyz, double* th, double* ph, hipStream_t st);
@@ -98,11 +98,7 @@ hipError_t launch_reduce(int n_sites, const double* vsite, int n, const double* 
 
 hipError_t launch_md_vv(int n, double* pos, double* vel, const double* f, double* fprev,
                         const double* mass, double dt, const int* step, double* ke,
-                        hipStream_t st);
-hipError_t launch_md_skin(int n, const double* pos, const double* pref, double lim2, int* flag,
-                          hipStream_t st);
-hipError_t launch_md_ref_copy(int n, const double* pos, double* pref, const int* flag,
-                              hipStream_t st);
+                        const double* pref, double lim2, int* flag, hipStream_t st);
 hipError_t launch_md_finish(int n, const double* ke, const double* out, int* flag, int* nreb,
                             int* step, double* ep_hist, double* ek_hist, int cap,
                             hipStream_t st);
@@ -209,7 +205,7 @@ static int build_list(tgx_ctx* c, int n, const Cell& cell, double rcut, bool wan
   Extra& x = X(c);
   const int ncell = cell.small ? 1 : cell.mx * cell.my * cell.mz;
   ENSURE(c, cell_of, sizeof(int) * n);
-  ENSURE(c, cell_atoms, sizeof(int) * n);
+  ENSURE(c, cell_atoms, cell_atoms_bytes(n));
   ENSURE(c, cell_start, sizeof(int) * (ncell + 1));
   if (x.cell_count.ensure(sizeof(int) * ncell) || x.cell_fill.ensure(sizeof(int) * ncell))
     return set_err(c, TGX_ERR_HIP, "device allocation failed (cells)");
@@ -1295,9 +1291,7 @@ static int mc_trial_launch(tgx_ctx* c, const McRun& r) {
                               c->cell_start.as<int>(), x.cell_fill.as<int>(),
                               c->cell_atoms.as<int>(), c->nb_count.as<int>(), c->maxnb,
                               c->nb_list.as<int>(), nullptr, c->nb_flag.as<int>(), c->stream,
-                              ctl + MC_REBUILD));
-  TGX_HIP(c, launch_md_ref_copy(r.n, c->pos.as<double>(), x.mc_pref.as<double>(),
-                                ctl + MC_REBUILD, c->stream));
+                              ctl + MC_REBUILD, x.mc_pref.as<double>()));
   const int D = c->dim;
   if (c->has_soap) {
     TGX_HIP(c, launch_mc_affected(ctl, dctl, c->pos.as<double>(), r.cell, r.rsoap,
@@ -1797,15 +1791,14 @@ static int md_step_launch(tgx_ctx* c) {
   const int n = m.n;
   TGX_HIP(c, launch_md_vv(n, c->pos.as<double>(), c->md_vel.as<double>(), c->forces.as<double>(),
                           c->md_fprev.as<double>(), c->md_mass.as<double>(), m.dt, ctl + 2,
-                          m.ke.as<double>(), c->stream));
-  TGX_HIP(c, launch_md_skin(n, c->pos.as<double>(), m.pref.as<double>(),
-                            0.25 * m.skin * m.skin, ctl + 0, c->stream));
+                          m.ke.as<double>(), m.pref.as<double>(), 0.25 * m.skin * m.skin,
+                          ctl + 0, c->stream));
   TGX_HIP(c, launch_neighbors(c->pos.as<double>(), n, m.cell, m.rcut_list, c->cell_of.as<int>(),
                               x.cell_count.as<int>(), c->cell_start.as<int>(),
                               x.cell_fill.as<int>(), c->cell_atoms.as<int>(),
                               c->nb_count.as<int>(), c->maxnb, c->nb_list.as<int>(),
-                              c->nb_rev.as<int>(), c->nb_flag.as<int>(), c->stream, ctl + 0));
-  TGX_HIP(c, launch_md_ref_copy(n, c->pos.as<double>(), m.pref.as<double>(), ctl + 0, c->stream));
+                              c->nb_rev.as<int>(), c->nb_flag.as<int>(), c->stream, ctl + 0,
+                              m.pref.as<double>()));
   int rcode = ef_launch(c, n, m.n_sites, Geom{m.cell}, 1, false);
   if (rcode) return rcode;
   TGX_HIP(c, launch_md_finish(n, m.ke.as<double>(), x.out.as<double>(), ctl + 0, ctl + 1, ctl + 2,

@@ -85,6 +85,14 @@ struct CellMany {
   __device__ __forceinline__ Cell of(int i) const { return cells[sid[i]]; }
 };
 
+// LDS written by other lanes of this wave is visible after this point.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Fortran MODULO(a,p) for reals in gfortran's formulation (fmod plus sign
 // fix), then the L/2 fold and the x,y,z sum of squares in that order
 // (get_distance, reference neighbors_interface.f90:565-581).  Contraction is

@@ -8,6 +8,8 @@
 // (v_mfma_f32_16x16x32_bf16, FP32 accumulation).
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "tgx_internal.h"
 
 namespace tgx {
@@ -265,6 +267,217 @@ __global__ void esum_kernel(int M, int ntiles, const TA* epart, double d2,
   esite[r] = d2 * s;
 }
 
+// Both GEMMs of the BF16 kernel step in one workgroup per 128 sites, for
+// models with at most 512 sparse points and 512 descriptor components:
+//   S = q Qs^T (128 x N1, FP32 in registers), epilogue E_i and
+//   W = zeta delta^2 alpha S^(zeta-1) as BF16 into LDS, then G = W Qs.
+// W never goes to HBM and the energies need no separate row sum.  Eight
+// waves: stage 1 splits rows in two halves and the sparse points in four
+// quarters, stage 2 the rows in two and the output columns in four.
+// Stage 1 stages 32-deep slabs of q and Qs through two LDS buffers (rows of
+// 80 bytes: conflict-free 16-byte fragment reads), the next slab travelling
+// in registers while the current one is multiplied; the buffers live where
+// W goes afterwards.  Stage 2 reads W from LDS (rows padded by 16 bytes)
+// and the L2-resident Qs fragments straight from memory, two k-steps ahead.
+// The MFMA sequence per output element is the one of the two separate
+// GEMMs, so S, W and G are bit-identical to them; E_i differs only in the
+// order of its FP32 partial sums.  NT2 = (descriptor columns) / 64 is a
+// template parameter so that only the live accumulators take registers.
+#define TGX_FUSED_MAXT 8
+#define TGX_FUSED_SLD 40  // staged row: 32 bf16 + 8 pad
+#define TGX_FUSED_PF 5    // 16-byte chunks per thread per slab: (128 + 512) * 4 / 512
+
+__device__ __forceinline__ void kpow_i(float k, int zint, float* kz, float* kz1) {
+  float p = 1.f;
+#pragma unroll
+  for (int i = 1; i < 8; i++)
+    if (i < zint) p *= k;
+  *kz1 = p;
+  *kz = p * k;
+}
+
+template <int NT2>
+__global__ __launch_bounds__(512) void gap_fused_kernel(
+    int M, int N1, int K1, const u16* __restrict__ q, int ldq, const u16* __restrict__ qs,
+    int ldqs, const u16* __restrict__ qsT, int ldqt, EpiKernel ek, double d2,
+    double* __restrict__ esite, float* __restrict__ G, int ldg, const int* __restrict__ guard) {
+  if (guard && *guard == 0) return;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int ldw = N1 + 8;
+  u16* sW = reinterpret_cast<u16*>(smem_raw);  // [128][ldw] (stage-1 staging before)
+  const int stage_rows = 128 + N1;
+  const size_t wbytes = (size_t)128 * ldw * 2, sbytes = (size_t)2 * stage_rows * TGX_FUSED_SLD * 2;
+  float* sE = reinterpret_cast<float*>(smem_raw + (wbytes > sbytes ? wbytes : sbytes));  // [4][128]
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int mh = wave >> 2, nq = wave & 3;
+  const int row0 = blockIdx.x * 128, rw = row0 + mh * 64;
+  const int lr = lane & 15, lk = (lane >> 4) * 8;
+  const int nt1 = N1 >> 6;
+
+  // ---- stage 1: S = q Qs^T over this wave's quarter of the sparse points
+  f32x4 acc[4][TGX_FUSED_MAXT];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < TGX_FUSED_MAXT; b++) acc[a][b] = f32x4{0, 0, 0, 0};
+  const int nchunk = stage_rows * 4;  // 16-byte chunks per slab
+  uint4 pf[TGX_FUSED_PF];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int p = 0; p < TGX_FUSED_PF; p++) {
+      const int ch = tid + p * 512;
+      const int r = ch >> 2, part = (ch & 3) * 8;
+      uint4 v = uint4{0, 0, 0, 0};
+      if (ch < nchunk) {
+        if (r < 128) {
+          if (row0 + r < M) v = *reinterpret_cast<const uint4*>(q + (size_t)(row0 + r) * ldq + k0 + part);
+        } else {
+          v = *reinterpret_cast<const uint4*>(qs + (size_t)(r - 128) * ldqs + k0 + part);
+        }
+      }
+      pf[p] = v;
+    }
+  };
+  auto stash = [&](int buf) {
+    u16* st = sW + (size_t)buf * stage_rows * TGX_FUSED_SLD;
+#pragma unroll
+    for (int p = 0; p < TGX_FUSED_PF; p++) {
+      const int ch = tid + p * 512;
+      if (ch < nchunk)
+        *reinterpret_cast<uint4*>(st + (size_t)(ch >> 2) * TGX_FUSED_SLD + (ch & 3) * 8) = pf[p];
+    }
+  };
+  const int c1 = nq * (N1 >> 2);
+  const int nslab = K1 >> 5;
+  fetch(0);
+  stash(0);
+  __syncthreads();
+  for (int sl = 0; sl < nslab; sl++) {
+    if (sl + 1 < nslab) fetch((sl + 1) * 32);
+    const u16* st = sW + (size_t)(sl & 1) * stage_rows * TGX_FUSED_SLD;
+    bf16x8 af[4];
+#pragma unroll
+    for (int mi = 0; mi < 4; mi++)
+      af[mi] = *reinterpret_cast<const bf16x8*>(st + (size_t)(mh * 64 + mi * 16 + lr) * TGX_FUSED_SLD + lk);
+#pragma unroll
+    for (int ni = 0; ni < TGX_FUSED_MAXT; ni++)
+      if (ni < nt1) {
+        const bf16x8 bfr = *reinterpret_cast<const bf16x8*>(
+            st + (size_t)(128 + c1 + ni * 16 + lr) * TGX_FUSED_SLD + lk);
+#pragma unroll
+        for (int mi = 0; mi < 4; mi++)
+          acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mi], bfr, acc[mi][ni], 0, 0, 0);
+      }
+    if (sl + 1 < nslab) stash((sl + 1) & 1);
+    __syncthreads();
+  }
+  // ---- epilogue: E partials and W (BF16) into LDS (the staging is free now)
+  const float* al = reinterpret_cast<const float*>(ek.alphas);
+  const float zd2 = (float)ek.zd2;
+  const int zint = ek.zint;
+  float rs[4][4];
+#pragma unroll
+  for (int mi = 0; mi < 4; mi++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) rs[mi][v] = 0.f;
+#pragma unroll
+  for (int ni = 0; ni < TGX_FUSED_MAXT; ni++) {
+    if (ni < nt1) {
+      const int c = c1 + ni * 16 + lr;
+      const float a = c < ek.n_sparse ? al[c] : 0.f;  // padding: W = 0, no energy
+#pragma unroll
+      for (int mi = 0; mi < 4; mi++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int rl = mh * 64 + mi * 16 + (lane >> 4) * 4 + v;
+          float kz, kz1;
+          kpow_i(acc[mi][ni][v], zint, &kz, &kz1);
+          rs[mi][v] += a * kz;
+          sW[(size_t)rl * ldw + c] = to_bf16(zd2 * a * kz1);
+        }
+    }
+  }
+#pragma unroll
+  for (int mi = 0; mi < 4; mi++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      float x = rs[mi][v];
+      x += __shfl_xor(x, 1, 64);
+      x += __shfl_xor(x, 2, 64);
+      x += __shfl_xor(x, 4, 64);
+      x += __shfl_xor(x, 8, 64);
+      if (lr == 0) sE[nq * 128 + mh * 64 + mi * 16 + (lane >> 4) * 4 + v] = x;
+    }
+  __syncthreads();
+  if (tid < 128 && row0 + tid < M)
+    esite[row0 + tid] = d2 * ((double)sE[tid] + (double)sE[128 + tid] + (double)sE[256 + tid] +
+                              (double)sE[384 + tid]);
+
+  // ---- stage 2: G = W Qs over this wave's quarter of the output columns
+  f32x4 acc2[4][NT2];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < NT2; b++) acc2[a][b] = f32x4{0, 0, 0, 0};
+  const int c2 = nq * (16 * NT2);
+  const u16* bsrc = qsT + (size_t)(c2 + lr) * ldqt + lk;
+  bf16x8 b0[NT2], b1[NT2];
+  auto bload = [&](bf16x8* dst, int k0) {
+#pragma unroll
+    for (int nj = 0; nj < NT2; nj++)
+      dst[nj] = *reinterpret_cast<const bf16x8*>(bsrc + (size_t)nj * 16 * ldqt + k0);
+  };
+  auto step2 = [&](const bf16x8* bf, int k0) {
+    bf16x8 af[4];
+#pragma unroll
+    for (int mi = 0; mi < 4; mi++)
+      af[mi] = *reinterpret_cast<const bf16x8*>(sW + (size_t)(mh * 64 + mi * 16 + lr) * ldw + k0 + lk);
+#pragma unroll
+    for (int mi = 0; mi < 4; mi++)
+#pragma unroll
+      for (int nj = 0; nj < NT2; nj++)
+        acc2[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mi], bf[nj], acc2[mi][nj], 0, 0, 0);
+  };
+  // two k-steps in flight: b0 holds step k, b1 step k+32 (N1 is a multiple of 64)
+  bload(b0, 0);
+  bload(b1, 32);
+  for (int k0 = 0; k0 < N1; k0 += 64) {
+    step2(b0, k0);
+    if (k0 + 64 < N1) bload(b0, k0 + 64);
+    step2(b1, k0 + 32);
+    if (k0 + 96 < N1) bload(b1, k0 + 96);
+  }
+#pragma unroll
+  for (int mi = 0; mi < 4; mi++)
+#pragma unroll
+    for (int nj = 0; nj < NT2; nj++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = rw + mi * 16 + (lane >> 4) * 4 + v;
+        if (r < M) G[(size_t)r * ldg + c2 + nj * 16 + lr] = acc2[mi][nj][v];
+      }
+}
+
+template <int NT2>
+static hipError_t run_fused(int M, int sp_pad, int dim_pad, const void* q, const void* qs,
+                            const void* qsT, const EpiKernel& ek, double delta, double* esite,
+                            void* G, const int* guard, hipStream_t st) {
+  static size_t sm_set = 0;
+  const size_t wbytes = (size_t)128 * (sp_pad + 8) * 2;
+  const size_t sbytes = (size_t)2 * (128 + sp_pad) * TGX_FUSED_SLD * 2;
+  const size_t sm = (wbytes > sbytes ? wbytes : sbytes) + 4 * 128 * sizeof(float);
+  if (sm > sm_set) {
+    hipError_t e = hipFuncSetAttribute((const void*)gap_fused_kernel<NT2>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    if (e != hipSuccess) return e;
+    sm_set = sm;
+  }
+  hipLaunchKernelGGL(gap_fused_kernel<NT2>, dim3((M + 127) / 128), dim3(512), sm, st, M, sp_pad,
+                     dim_pad, (const u16*)q, dim_pad, (const u16*)qs, dim_pad, (const u16*)qsT,
+                     sp_pad, ek, delta * delta, esite, (float*)G, dim_pad, guard);
+  return hipGetLastError();
+}
+
 template <typename TI>
 static hipError_t run(int n_sites, int dim, int dim_pad, int n_sparse, int sp_pad,
                       const void* q, const void* qs, const void* qsT, const void* alphas,
@@ -277,6 +490,20 @@ static hipError_t run(int n_sites, int dim, int dim_pad, int n_sparse, int sp_pa
   const double zz = zint > 0 ? (double)zint : zeta;
   EpiKernel ek{alphas, n_sparse, zint, zz, zz * delta * delta, wmat, sp_pad, epart};
   EpiStore es{G, dim_pad};
+  if constexpr (sizeof(TI) == 2) {
+    // TGX_GEMM_UNFUSED=1 selects the two separate GEMMs (for comparisons)
+    const char* uf = getenv("TGX_GEMM_UNFUSED");
+    if (want_grad && zint >= 1 && zint <= 8 && sp_pad <= 64 * TGX_FUSED_MAXT &&
+        dim_pad <= 64 * TGX_FUSED_MAXT && !(uf && uf[0] == '1')) {
+      switch (dim_pad / 64) {
+#define TGX_FUSED_CASE(t) \
+  case t: return run_fused<t>(M, sp_pad, dim_pad, q, qs, qsT, ek, delta, esite, G, guard, st);
+        TGX_FUSED_CASE(1) TGX_FUSED_CASE(2) TGX_FUSED_CASE(3) TGX_FUSED_CASE(4)
+        TGX_FUSED_CASE(5) TGX_FUSED_CASE(6) TGX_FUSED_CASE(7) TGX_FUSED_CASE(8)
+#undef TGX_FUSED_CASE
+      }
+    }
+  }
   constexpr int BM0 = 32 * Tile<TI, 0>::TM, BN0 = 32 * Tile<TI, 0>::TN;
   constexpr int BM1 = 32 * Tile<TI, 1>::TM, BN1 = 32 * Tile<TI, 1>::TN;
   const int mt0 = (M + BM0 - 1) / BM0;

@@ -33,6 +33,13 @@ struct Cell {
   double iLa, iLb, iLc;  // 1 / supercell edges
 };
 
+// Bytes of the linked-cell atom buffer for n atoms: the atom ids in cell
+// order, then (16-byte aligned) their wrapped FP32 positions with the id in
+// the fourth component, read by the one-pass list build.
+inline size_t cell_atoms_bytes(int n) {
+  return 4 * (size_t)((n + 3) & ~3) + 16 * (size_t)n;
+}
+
 // Pair geometry handed to the launchers: one periodic cell for every atom,
 // or (batched structures) a device array of cells with sid[atom] = the
 // atom's structure.

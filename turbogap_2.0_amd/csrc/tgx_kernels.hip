@@ -107,14 +107,29 @@ __global__ void scan_cells_kernel(const int* count, int ncells, int* start,
   if (t == 1023) start[ncells] = part[1023];
 }
 
+// Wrapped FP32 position of atom i (x - L floor(x/L) in FP64, then rounded)
+// with its id in w.
+__device__ __forceinline__ float4 wrapped_pos4(const double* pos, int i, const Cell& c) {
+  const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+  return make_float4((float)(x - c.La * floor(x * c.iLa)), (float)(y - c.Lb * floor(y * c.iLb)),
+                     (float)(z - c.Lc * floor(z * c.iLc)), __int_as_float(i));
+}
+
+// Also clears the counts the scan consumed: every binning leaves cell_count
+// all zero, ready for the next one.  With cpos != nullptr the wrapped FP32
+// positions are stored in cell order too.
 __global__ void scatter_cells_kernel(int n, const int* cell_of, const int* start,
-                                     int* fill, int* cell_atoms, const int* guard) {
+                                     int* fill, int* cell_atoms, int* cell_count,
+                                     const double* pos, Cell cl, float4* cpos,
+                                     const int* guard) {
   GUARD_RETURN();
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int c = cell_of[i];
   int slot = atomicAdd(fill + c, 1);
   cell_atoms[start[c] + slot] = i;
+  if (cpos) cpos[start[c] + slot] = wrapped_pos4(pos, i, cl);
+  cell_count[c] = 0;
 }
 
 // Candidates of atom i within rcut, unsorted, packed (img<<26 | j).  Four
@@ -182,6 +197,191 @@ __global__ void find_neighbors_kernel(const double* pos, int n, Cell c,
   }
 }
 
+// Ascending bitonic sort of 64 values held one per lane.
+__device__ __forceinline__ int wave_sort64(int v, int lane) {
+#pragma unroll
+  for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const int o = __shfl_xor(v, j, 64);
+      const bool up = (lane & k) == 0;
+      const bool lower = (lane & j) == 0;
+      v = (lower == up) ? min(v, o) : max(v, o);
+    }
+  }
+  return v;
+}
+
+// Ascending bitonic sort of 128 values, v0 at index lane, v1 at lane + 64.
+__device__ __forceinline__ void wave_sort128(int& v0, int& v1, int lane) {
+#pragma unroll
+  for (int k = 2; k <= 128; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j == 64) {
+        // partners sit in the same lane; k == 128 here, so ascending
+        const int lo = min(v0, v1), hi = max(v0, v1);
+        v0 = lo;
+        v1 = hi;
+      } else {
+        const int o0 = __shfl_xor(v0, j, 64), o1 = __shfl_xor(v1, j, 64);
+        const bool lower = (lane & j) == 0;
+        const bool up0 = (lane & k) == 0, up1 = ((lane + 64) & k) == 0;
+        v0 = (lower == up0) ? min(v0, o0) : max(v0, o0);
+        v1 = (lower == up1) ? min(v1, o1) : max(v1, o1);
+      }
+    }
+  }
+}
+
+// The linked-cell list build for grids of at least 3 cells per direction
+// in one pass, one wave per atom: the atoms of the 27 neighbouring cells
+// are numbered across the wave and screened 64 at a time with a rounding
+// minimum image on their cell-ordered FP32 copies (coalesced reads); the
+// survivors (within rcut plus a margin far above rounding) queue in LDS and are decided 64 at a time with the reference's
+// arithmetic (mic), so the pair set is exactly the reference's.  Hits are
+// compacted with a ballot into LDS, sorted ascending (the reference
+// supercell id order) and written with their count; count needs no clearing
+// and no atomics are used.  A list longer than maxnb raises flag[0] to its
+// length.  With pref != nullptr the atom's position is also copied there
+// (the reference point of the skin test).  Every wave of the block reaches
+// both barriers.
+#define TGX_NB_CAPL 128
+__device__ __forceinline__ void nb_decide(const double* pos, const double* pi, const Cell& c,
+                                          double rcut, const int* q, int m, int lane,
+                                          int* buf, int* out, int maxnb, int* found) {
+  bool hit = false;
+  int k = 0;
+  if (lane < m) {
+    k = q[lane];
+    double dv[3];
+    hit = mic(pi, pos + 3 * k, c.La, c.Lb, c.Lc, dv) < rcut;
+  }
+  const unsigned long long b = __ballot(hit);
+  if (hit) {
+    const int slot = *found + __popcll(b & ((1ull << lane) - 1ull));
+    if (slot < TGX_NB_CAPL) buf[slot] = k;
+    else if (slot < maxnb) out[slot] = k;
+  }
+  *found += __popcll(b);
+}
+
+__global__ __launch_bounds__(256) void find_sorted_kernel(
+    const double* pos, int n, Cell c, double rcut, const int* cell_start,
+    const float4* cpos, int* count, int maxnb, int* list, int* flag, double* pref,
+    const int* guard) {
+  GUARD_RETURN();
+  __shared__ int s_pre[4][28], s_base[4][28];
+  __shared__ int s_buf[4][TGX_NB_CAPL], s_q[4][128];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = blockIdx.x * 4 + w;
+  const bool active = i < n;
+  const int ii = active ? i : 0;
+  const double* pi = pos + 3 * ii;
+  int cx, cy, cz;
+  cell_coords(pi, c, &cx, &cy, &cz);
+  // lanes 0..26: cell q = lane, its atom count and first slot
+  int cnt_q = 0, base_q = 0;
+  if (lane < 27) {
+    const int q = lane;
+    const int i3 = cx - 1 + q % 3, j3 = cy - 1 + (q / 3) % 3, k3 = cz - 1 + q / 9;
+    const int cell = imodd(i3, c.mx) + imodd(j3, c.my) * c.mx + imodd(k3, c.mz) * c.mx * c.my;
+    base_q = cell_start[cell];
+    cnt_q = cell_start[cell + 1] - base_q;
+  }
+  int incl = cnt_q;
+#pragma unroll
+  for (int d = 1; d < 32; d <<= 1) {
+    const int o = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += o;
+  }
+  const int total = active ? __shfl(incl, 26, 64) : 0;
+  if (lane < 27) {
+    s_pre[w][lane] = incl - cnt_q;
+    s_base[w][lane] = base_q;
+  }
+  if (pref && active && lane < 3) pref[3 * i + lane] = pi[lane];
+  __syncthreads();
+  // FP32 screen on wrapped coordinates: its error (a few 1e-5 A for boxes
+  // of 100 A) is far inside the 1e-3 A margin, and images near L/2 are far
+  // beyond rcut since L >= 3 rcut.
+  const float rs = (float)rcut + 1e-3f, screen2 = rs * rs;
+  const float4 pw = wrapped_pos4(pos, ii, c);
+  const float La = (float)c.La, Lb = (float)c.Lb, Lc = (float)c.Lc;
+  const float iLa = (float)c.iLa, iLb = (float)c.iLb, iLc = (float)c.iLc;
+  int found = 0, nq = 0;
+  int* out = list + (size_t)ii * maxnb;
+  int* sq = s_q[w];
+  for (int idx0 = 0; idx0 < total; idx0 += 64) {
+    const int idx = idx0 + lane;
+    bool near = false;
+    int k = 0;
+    if (idx < total) {
+      // largest q with s_pre[q] <= idx
+      int lo = 0, hi = 26;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_pre[w][mid] <= idx) lo = mid;
+        else hi = mid - 1;
+      }
+      const float4 pk = cpos[s_base[w][lo] + idx - s_pre[w][lo]];
+      k = __float_as_int(pk.w);
+      if (k != i) {
+        float dx = pk.x - pw.x, dy = pk.y - pw.y, dz = pk.z - pw.z;
+        dx -= La * rintf(dx * iLa);
+        dy -= Lb * rintf(dy * iLb);
+        dz -= Lc * rintf(dz * iLc);
+        near = dx * dx + dy * dy + dz * dz <= screen2;
+      }
+    }
+    const unsigned long long m = __ballot(near);
+    if (near) sq[nq + __popcll(m & ((1ull << lane) - 1ull))] = k;  // nq < 64 here
+    nq += __popcll(m);
+    if (nq >= 64) {
+      wave_sync();
+      nb_decide(pos, pi, c, rcut, sq, 64, lane, s_buf[w], out, maxnb, &found);
+      const int rest = nq - 64 > lane ? sq[64 + lane] : 0;
+      wave_sync();
+      if (lane < nq - 64) sq[lane] = rest;
+      nq -= 64;
+    }
+  }
+  wave_sync();
+  if (nq > 0) nb_decide(pos, pi, c, rcut, sq, nq, lane, s_buf[w], out, maxnb, &found);
+  __syncthreads();
+  if (!active) return;
+  if (lane == 0) {
+    count[i] = found;
+    if (found > maxnb) atomicMax(flag, found);
+  }
+  const int keep = min(found, maxnb);
+  if (found <= 64) {
+    int v = lane < found ? s_buf[w][lane] : 0x7fffffff;
+    v = wave_sort64(v, lane);
+    if (lane < keep) out[lane] = v;
+  } else if (found <= TGX_NB_CAPL) {
+    int v0 = s_buf[w][lane];
+    int v1 = lane + 64 < found ? s_buf[w][lane + 64] : 0x7fffffff;
+    wave_sort128(v0, v1, lane);
+    if (lane < keep) out[lane] = v0;
+    if (lane + 64 < keep) out[lane + 64] = v1;
+  } else {
+    for (int s = lane; s < TGX_NB_CAPL && s < keep; s += 64) out[s] = s_buf[w][s];
+    __threadfence_block();
+    if (lane == 0) {
+      for (int p = 1; p < keep; p++) {
+        const int v = out[p];
+        int q = p - 1;
+        while (q >= 0 && out[q] > v) {
+          out[q + 1] = out[q];
+          q--;
+        }
+        out[q + 1] = v;
+      }
+    }
+  }
+}
+
 // Sort each atom's list ascending (== ascending reference supercell id):
 // one wave per atom, bitonic network over 64 lanes; lists longer than 64
 // entries fall back to an insertion sort by lane 0.
@@ -219,16 +419,18 @@ __global__ __launch_bounds__(256) void sort_lists_kernel(int n, int maxnb, const
   }
 }
 
-// rev[i][slot] = flat index of the entry (neg img, i) in j's list.
+// rev[i][slot] = flat index of the entry (neg img, i) in j's list; one
+// wave per atom, four atoms per workgroup.
 template <typename G>
-__global__ void reverse_kernel(int n, int maxnb, G geo, const int* count,
-                               const int* list, int* rev, int* flag,
-                               const int* guard) {
+__global__ __launch_bounds__(256) void reverse_kernel(int n, int maxnb, G geo, const int* count,
+                                                      const int* list, int* rev, int* flag,
+                                                      const int* guard) {
   GUARD_RETURN();
-  int i = blockIdx.x;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= n) return;
   const Cell c = geo.of(i);
   int cnt = min(count[i], maxnb);
-  for (int s = threadIdx.x; s < cnt; s += blockDim.x) {
+  for (int s = threadIdx.x & 63; s < cnt; s += 64) {
     int e = list[(size_t)i * maxnb + s];
     int j = e & TGX_ATOM_MASK, img = e >> TGX_IMG_SHIFT;
     int nimg = 0;
@@ -292,34 +494,58 @@ __global__ void csr_out_kernel(const double* pos, int n, Cell c, int maxnb,
   }
 }
 
+__global__ void copy_pos_kernel(int n, const double* pos, double* pref, const int* guard) {
+  GUARD_RETURN();
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < 3 * n) pref[k] = pos[k];
+}
+
 // The whole list build; with guard != nullptr every kernel returns at once
 // unless *guard is non-zero (rebuild requested on the device), so the
-// sequence can sit in a captured graph.
+// sequence can sit in a captured graph.  pref (optional) receives a copy of
+// the positions the list was built from.
+
 hipError_t launch_neighbors(const double* pos, int n, const Cell& c, double rcut,
                             int* cell_of, int* cell_count, int* cell_start,
                             int* cell_fill, int* cell_atoms, int* count,
                             int maxnb, int* list, int* rev, int* flag,
-                            hipStream_t st, const int* guard) {
+                            hipStream_t st, const int* guard, double* pref) {
   const int B = 256;
   int nb = (n + B - 1) / B;
+  const bool fused = !c.small && c.mx >= 3 && c.my >= 3 && c.mz >= 3;
+  // wrapped FP32 positions in cell order, after the ids (cell_atoms_bytes)
+  float4* cpos = reinterpret_cast<float4*>(cell_atoms + ((n + 3) & ~3));
   if (!c.small) {
     int ncell = c.mx * c.my * c.mz;
-    hipLaunchKernelGGL(zero_ints_kernel, dim3((ncell + B - 1) / B), dim3(B), 0, st, cell_count,
-                       ncell, guard);
+    // a guarded (captured) build relies on the previous build of the same
+    // grid having left cell_count zero
+    if (!guard)
+      hipLaunchKernelGGL(zero_ints_kernel, dim3((ncell + B - 1) / B), dim3(B), 0, st, cell_count,
+                         ncell, nullptr);
     hipLaunchKernelGGL(bin_atoms_kernel, dim3(nb), dim3(B), 0, st, pos, n, c, cell_of, cell_count,
                        guard);
     hipLaunchKernelGGL(scan_cells_kernel, dim3(1), dim3(1024), 0, st, cell_count, ncell, cell_start,
                        cell_fill, guard);
     hipLaunchKernelGGL(scatter_cells_kernel, dim3(nb), dim3(B), 0, st, n, cell_of, cell_start,
-                       cell_fill, cell_atoms, guard);
+                       cell_fill, cell_atoms, cell_count, pos, c, fused ? cpos : nullptr, guard);
   }
-  hipLaunchKernelGGL(zero_ints_kernel, dim3(nb), dim3(B), 0, st, count, n, guard);
-  hipLaunchKernelGGL(find_neighbors_kernel, dim3((n * TGX_NB_LANES + B - 1) / B), dim3(B), 0, st,
-                     pos, n, c, rcut, cell_start, cell_atoms, count, maxnb, list, flag, guard);
-  hipLaunchKernelGGL(sort_lists_kernel, dim3((n + 3) / 4), dim3(256), 0, st, n, maxnb, count,
-                     list, guard);
+  if (fused) {
+    hipLaunchKernelGGL(find_sorted_kernel, dim3((n + 3) / 4), dim3(256), 0, st, pos, n, c, rcut,
+                       cell_start, cpos, count, maxnb, list, flag, pref, guard);
+  } else {
+    hipLaunchKernelGGL(zero_ints_kernel, dim3(nb), dim3(B), 0, st, count, n, guard);
+    hipLaunchKernelGGL(find_neighbors_kernel, dim3((n * TGX_NB_LANES + B - 1) / B), dim3(B), 0,
+                       st, pos, n, c, rcut, cell_start, cell_atoms, count, maxnb, list, flag,
+                       guard);
+    hipLaunchKernelGGL(sort_lists_kernel, dim3((n + 3) / 4), dim3(256), 0, st, n, maxnb, count,
+                       list, guard);
+    if (pref)
+      hipLaunchKernelGGL(copy_pos_kernel, dim3((3 * n + B - 1) / B), dim3(B), 0, st, n, pos,
+                         pref, guard);
+  }
   if (rev)
-    hipLaunchKernelGGL(reverse_kernel<CellOne>, dim3(n), dim3(64), 0, st, n, maxnb, CellOne{c},
+    hipLaunchKernelGGL(reverse_kernel<CellOne>, dim3((n + 3) / 4), dim3(256), 0, st, n, maxnb,
+                       CellOne{c},
                        count, list, rev, flag, guard);
   return hipGetLastError();
 }
@@ -420,7 +646,7 @@ hipError_t launch_neighbors_many(const double* pos, int n, const Cell* cells, co
   hipLaunchKernelGGL(scan_cells_kernel, dim3(1), dim3(1024), 0, st, cell_count, n_bins,
                      cell_start, cell_fill, nullptr);
   hipLaunchKernelGGL(scatter_cells_kernel, dim3(nb), dim3(B), 0, st, n, cell_of, cell_start,
-                     cell_fill, cell_atoms, nullptr);
+                     cell_fill, cell_atoms, cell_count, pos, Cell{}, nullptr, nullptr);
   hipLaunchKernelGGL(zero_ints_kernel, dim3(nb), dim3(B), 0, st, count, n, nullptr);
   hipLaunchKernelGGL(find_neighbors_many_kernel, dim3((n * TGX_NB_LANES + B - 1) / B), dim3(B),
                      0, st, pos, n, cells, sid, boff, aoff, rcut, cell_start, cell_atoms, count,
@@ -428,7 +654,7 @@ hipError_t launch_neighbors_many(const double* pos, int n, const Cell* cells, co
   hipLaunchKernelGGL(sort_lists_kernel, dim3((n + 3) / 4), dim3(256), 0, st, n, maxnb, count,
                      list, nullptr);
   if (rev)
-    hipLaunchKernelGGL(reverse_kernel<CellMany>, dim3(n), dim3(64), 0, st, n, maxnb,
+    hipLaunchKernelGGL(reverse_kernel<CellMany>, dim3((n + 3) / 4), dim3(256), 0, st, n, maxnb,
                        CellMany{cells, sid}, count, list, rev, flag, nullptr);
   return hipGetLastError();
 }

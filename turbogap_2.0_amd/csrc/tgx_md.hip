@@ -11,11 +11,15 @@ namespace tgx {
 
 // v(t) = v(t-dt) + 1/2 (F(t) + F(t-dt))/m dt  (skipped on the first step),
 // x(t+dt) = x(t) + v dt + 1/2 F/m dt^2, with the reference's operation order.
-// ke_part[block] = kinetic energy of the block's atoms (fixed tree).
+// ke_part[block] = kinetic energy of the block's atoms (fixed tree).  The
+// skin test follows on the new positions: *flag is raised when an atom
+// moved more than sqrt(lim2) from pref, its position at the last list build.
 __global__ __launch_bounds__(256) void md_vv_kernel(int n, double* pos, double* vel,
                                                     const double* f, double* fprev,
                                                     const double* mass, double dt,
-                                                    const int* step, double* ke_part) {
+                                                    const int* step, double* ke_part,
+                                                    const double* pref, double lim2,
+                                                    int* flag) {
 #pragma clang fp contract(off)
   __shared__ double sh[256];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -34,6 +38,9 @@ __global__ __launch_bounds__(256) void md_vv_kernel(int n, double* pos, double* 
       pos[a] = pos[a] + v * dt + 0.5 * f[a] / m * (dt * dt);
     }
     kei = 0.5 * m * v2;
+    const double dx = pos[3 * i] - pref[3 * i], dy = pos[3 * i + 1] - pref[3 * i + 1],
+                 dz = pos[3 * i + 2] - pref[3 * i + 2];
+    if (dx * dx + dy * dy + dz * dz > lim2) atomicOr(flag, 1);
   }
   sh[threadIdx.x] = kei;
   __syncthreads();
@@ -42,23 +49,6 @@ __global__ __launch_bounds__(256) void md_vv_kernel(int n, double* pos, double* 
     __syncthreads();
   }
   if (threadIdx.x == 0) ke_part[blockIdx.x] = sh[0];
-}
-
-// Rebuild when any atom moved more than skin/2 since the last build.
-__global__ void md_skin_kernel(int n, const double* pos, const double* pref,
-                               double lim2, int* flag) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double dx = pos[3 * i] - pref[3 * i], dy = pos[3 * i + 1] - pref[3 * i + 1],
-               dz = pos[3 * i + 2] - pref[3 * i + 2];
-  if (dx * dx + dy * dy + dz * dz > lim2) atomicOr(flag, 1);
-}
-
-__global__ void md_ref_copy_kernel(int n, const double* pos, double* pref,
-                                   const int* flag) {
-  if (*flag == 0) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 3 * n) pref[i] = pos[i];
 }
 
 // End of step: kinetic energy of v(t) (fixed-order tree), potential energy
@@ -127,23 +117,9 @@ __global__ void md_sum_kernel(int n, const double* x, double* out) {
 
 hipError_t launch_md_vv(int n, double* pos, double* vel, const double* f, double* fprev,
                         const double* mass, double dt, const int* step, double* ke,
-                        hipStream_t st) {
+                        const double* pref, double lim2, int* flag, hipStream_t st) {
   hipLaunchKernelGGL(md_vv_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, pos, vel, f,
-                     fprev, mass, dt, step, ke);
-  return hipGetLastError();
-}
-
-hipError_t launch_md_skin(int n, const double* pos, const double* pref, double lim2, int* flag,
-                          hipStream_t st) {
-  hipLaunchKernelGGL(md_skin_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, pos, pref, lim2,
-                     flag);
-  return hipGetLastError();
-}
-
-hipError_t launch_md_ref_copy(int n, const double* pos, double* pref, const int* flag,
-                              hipStream_t st) {
-  hipLaunchKernelGGL(md_ref_copy_kernel, dim3((3 * n + 255) / 256), dim3(256), 0, st, n, pos,
-                     pref, flag);
+                     fprev, mass, dt, step, ke, pref, lim2, flag);
   return hipGetLastError();
 }
 
