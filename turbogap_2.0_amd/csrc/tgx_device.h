@@ -93,15 +93,34 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Fortran MODULO(a,p) for reals in gfortran's formulation (fmod plus sign
-// fix), then the L/2 fold and the x,y,z sum of squares in that order
-// (get_distance, reference neighbors_interface.f90:565-581).  Contraction is
-// off so the FP64 result is bit-identical to the reference arithmetic.
-__device__ __forceinline__ double f_modulo(double a, double p) {
+// fmod(a, p) for p > 0, bit-identical to the C library's for |a/p| < 2^40:
+// the remainder a - n p of the true quotient n = trunc(a/p) is exactly
+// representable, so one fused multiply-add gives it exactly; a quotient
+// estimate off by one (a/p within rounding of an integer) puts the result
+// outside [0, p) (or (-p, 0] for negative a) and is corrected once.
+__host__ __device__ __forceinline__ double rem_exact(double a, double p) {
+  const double n = trunc(a / p);
+  double r = fma(-n, p, a);
+  if (a >= 0.0) {
+    if (r < 0.0) r = fma(-(n - 1.0), p, a);
+    else if (r >= p) r = fma(-(n + 1.0), p, a);
+  } else {
+    if (r > 0.0) r = fma(-(n + 1.0), p, a);
+    else if (r <= -p) r = fma(-(n - 1.0), p, a);
+  }
+  return r == 0.0 ? copysign(0.0, a) : r;  // fmod's zero carries the sign of a
+}
+
+// Fortran MODULO(a,p) for reals and p > 0 in gfortran's formulation (fmod
+// plus sign fix), then the L/2 fold and the x,y,z sum of squares in that
+// order (get_distance, reference neighbors_interface.f90:565-581).
+// Contraction is off so the FP64 result is bit-identical to the reference
+// arithmetic.
+__host__ __device__ __forceinline__ double f_modulo(double a, double p) {
 #pragma clang fp contract(off)
-  double r = fmod(a, p);
+  double r = rem_exact(a, p);
   if (r != 0.0) {
-    if ((a < 0.0) != (p < 0.0)) r = r + p;
+    if (a < 0.0) r = r + p;
   } else {
     r = copysign(0.0, p);
   }

@@ -57,9 +57,9 @@ template <> struct MF<u16> {
   __device__ static int row(int lane, int v) { return (lane >> 4) * 4 + v; }
 };
 
+// Round to nearest even (v_cvt_pk_bf16_f32 on gfx950).
 __device__ __forceinline__ u16 to_bf16(float f) {
-  unsigned int u = __float_as_uint(f);
-  return (u16)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+  return __builtin_bit_cast(u16, (__bf16)f);
 }
 template <typename TI> __device__ __forceinline__ TI cvt_out(float v);
 template <> __device__ __forceinline__ float cvt_out<float>(float v) { return v; }
@@ -268,32 +268,51 @@ __global__ void esum_kernel(int M, int ntiles, const TA* epart, double d2,
 }
 
 // Both GEMMs of the BF16 kernel step in one workgroup per 128 sites, for
-// models with at most 512 sparse points and 512 descriptor components:
+// models with at most 512 sparse points and 448 descriptor components:
 //   S = q Qs^T (128 x N1, FP32 in registers), epilogue E_i and
 //   W = zeta delta^2 alpha S^(zeta-1) as BF16 into LDS, then G = W Qs.
 // W never goes to HBM and the energies need no separate row sum.  Eight
-// waves: stage 1 splits rows in two halves and the sparse points in four
-// quarters, stage 2 the rows in two and the output columns in four.
-// Stage 1 stages 32-deep slabs of q and Qs through two LDS buffers (rows of
-// 80 bytes: conflict-free 16-byte fragment reads), the next slab travelling
-// in registers while the current one is multiplied; the buffers live where
-// W goes afterwards.  Stage 2 reads W from LDS (rows padded by 16 bytes)
-// and the L2-resident Qs fragments straight from memory, two k-steps ahead.
-// The MFMA sequence per output element is the one of the two separate
-// GEMMs, so S, W and G are bit-identical to them; E_i differs only in the
-// order of its FP32 partial sums.  NT2 = (descriptor columns) / 64 is a
-// template parameter so that only the live accumulators take registers.
+// waves: stage 1 splits the rows in two halves and the sparse points in
+// four quarters, stage 2 the rows in two and the output columns in four.
+// Every operand reaches LDS through coalesced 16-byte loads (whole rows of
+// a 32-deep k-slab), the next slab travelling in registers while the
+// current one is multiplied: stage 1 double-buffers q and Qs slabs where W
+// goes afterwards (rows of 80 bytes), stage 2 one Qs slab (rows of 64
+// bytes, 16-byte segments XOR-swizzled by row) next to W (rows padded by 16
+// bytes); all fragment reads are free of bank conflicts.  Both stages
+// multiply with the Qs operand first (D = Qs-tile x q-tile), so that a lane
+// owns four consecutive columns of one row: W goes to LDS as 8-byte stores
+// and G to memory as float4.  Integer zeta up to 4 has its own epilogue
+// (powers by multiplication).
 #define TGX_FUSED_MAXT 8
-#define TGX_FUSED_SLD 40  // staged row: 32 bf16 + 8 pad
-#define TGX_FUSED_PF 5    // 16-byte chunks per thread per slab: (128 + 512) * 4 / 512
+#define TGX_FUSED_SLD 40  // stage-1 staged row: 32 bf16 + 8 pad
+#define TGX_FUSED_PF 5    // stage-1 16-byte chunks per thread: (128 + 512) * 4 / 512
 
-__device__ __forceinline__ void kpow_i(float k, int zint, float* kz, float* kz1) {
+template <int ZI>
+__device__ __forceinline__ void kpow_f(float k, int zint, float* kz, float* kz1) {
   float p = 1.f;
+  if constexpr (ZI > 0) {
 #pragma unroll
-  for (int i = 1; i < 8; i++)
-    if (i < zint) p *= k;
+    for (int i = 1; i < ZI; i++) p *= k;
+  } else {
+#pragma unroll
+    for (int i = 1; i < 8; i++)
+      if (i < zint) p *= k;
+  }
   *kz1 = p;
   *kz = p * k;
+}
+
+template <int V> struct ICz { static constexpr int v = V; };
+
+// LDS bytes of the W / stage-1 region and of the stage-2 slab region.
+__host__ __device__ inline size_t fused_region_w(int N1) {
+  const size_t w = (size_t)128 * (N1 + 8) * 2, s = (size_t)2 * (128 + N1) * TGX_FUSED_SLD * 2;
+  return w > s ? w : s;
+}
+__host__ __device__ inline size_t fused_region_t(int K1) {
+  const size_t t = (size_t)K1 * 64;
+  return t > 4 * 128 * sizeof(float) ? t : 4 * 128 * sizeof(float);
 }
 
 template <int NT2>
@@ -304,158 +323,187 @@ __global__ __launch_bounds__(512) void gap_fused_kernel(
   if (guard && *guard == 0) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int ldw = N1 + 8;
-  u16* sW = reinterpret_cast<u16*>(smem_raw);  // [128][ldw] (stage-1 staging before)
-  const int stage_rows = 128 + N1;
-  const size_t wbytes = (size_t)128 * ldw * 2, sbytes = (size_t)2 * stage_rows * TGX_FUSED_SLD * 2;
-  float* sE = reinterpret_cast<float*>(smem_raw + (wbytes > sbytes ? wbytes : sbytes));  // [4][128]
+  u16* sW = reinterpret_cast<u16*>(smem_raw);  // [128][ldw]; stage-1 slabs before
+  u16* sT = reinterpret_cast<u16*>(smem_raw + fused_region_w(N1));  // stage-2 slab [K1][32]
+  float* sE = reinterpret_cast<float*>(sT);  // [4][128] row-sum partials, before stage 2
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int mh = wave >> 2, nq = wave & 3;
   const int row0 = blockIdx.x * 128, rw = row0 + mh * 64;
-  const int lr = lane & 15, lk = (lane >> 4) * 8;
+  const int lr = lane & 15, lk = (lane >> 4) * 8, lc = (lane >> 4) * 4;
   const int nt1 = N1 >> 6;
 
-  // ---- stage 1: S = q Qs^T over this wave's quarter of the sparse points
+  // ---- stage 1: D[s][m] = S[m][s] over this wave's quarter of the sparse points
   f32x4 acc[4][TGX_FUSED_MAXT];
 #pragma unroll
   for (int a = 0; a < 4; a++)
 #pragma unroll
     for (int b = 0; b < TGX_FUSED_MAXT; b++) acc[a][b] = f32x4{0, 0, 0, 0};
-  const int nchunk = stage_rows * 4;  // 16-byte chunks per slab
-  uint4 pf[TGX_FUSED_PF];
-  auto fetch = [&](int k0) {
-#pragma unroll
-    for (int p = 0; p < TGX_FUSED_PF; p++) {
-      const int ch = tid + p * 512;
-      const int r = ch >> 2, part = (ch & 3) * 8;
-      uint4 v = uint4{0, 0, 0, 0};
-      if (ch < nchunk) {
-        if (r < 128) {
-          if (row0 + r < M) v = *reinterpret_cast<const uint4*>(q + (size_t)(row0 + r) * ldq + k0 + part);
-        } else {
-          v = *reinterpret_cast<const uint4*>(qs + (size_t)(r - 128) * ldqs + k0 + part);
-        }
-      }
-      pf[p] = v;
-    }
-  };
-  auto stash = [&](int buf) {
-    u16* st = sW + (size_t)buf * stage_rows * TGX_FUSED_SLD;
-#pragma unroll
-    for (int p = 0; p < TGX_FUSED_PF; p++) {
-      const int ch = tid + p * 512;
-      if (ch < nchunk)
-        *reinterpret_cast<uint4*>(st + (size_t)(ch >> 2) * TGX_FUSED_SLD + (ch & 3) * 8) = pf[p];
-    }
-  };
   const int c1 = nq * (N1 >> 2);
-  const int nslab = K1 >> 5;
-  fetch(0);
-  stash(0);
-  __syncthreads();
-  for (int sl = 0; sl < nslab; sl++) {
-    if (sl + 1 < nslab) fetch((sl + 1) * 32);
-    const u16* st = sW + (size_t)(sl & 1) * stage_rows * TGX_FUSED_SLD;
-    bf16x8 af[4];
+  {
+    const int stage_rows = 128 + N1, nchunk = stage_rows * 4;
+    uint4 pf[TGX_FUSED_PF];
+    auto fetch = [&](int k0) {
 #pragma unroll
-    for (int mi = 0; mi < 4; mi++)
-      af[mi] = *reinterpret_cast<const bf16x8*>(st + (size_t)(mh * 64 + mi * 16 + lr) * TGX_FUSED_SLD + lk);
-#pragma unroll
-    for (int ni = 0; ni < TGX_FUSED_MAXT; ni++)
-      if (ni < nt1) {
-        const bf16x8 bfr = *reinterpret_cast<const bf16x8*>(
-            st + (size_t)(128 + c1 + ni * 16 + lr) * TGX_FUSED_SLD + lk);
-#pragma unroll
-        for (int mi = 0; mi < 4; mi++)
-          acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mi], bfr, acc[mi][ni], 0, 0, 0);
+      for (int p = 0; p < TGX_FUSED_PF; p++) {
+        const int ch = tid + p * 512;
+        const int r = ch >> 2, part = (ch & 3) * 8;
+        uint4 v = uint4{0, 0, 0, 0};
+        if (ch < nchunk) {
+          if (r < 128) {
+            if (row0 + r < M)
+              v = *reinterpret_cast<const uint4*>(q + (size_t)(row0 + r) * ldq + k0 + part);
+          } else {
+            v = *reinterpret_cast<const uint4*>(qs + (size_t)(r - 128) * ldqs + k0 + part);
+          }
+        }
+        pf[p] = v;
       }
-    if (sl + 1 < nslab) stash((sl + 1) & 1);
+    };
+    auto stash = [&](int buf) {
+      u16* st = sW + (size_t)buf * stage_rows * TGX_FUSED_SLD;
+#pragma unroll
+      for (int p = 0; p < TGX_FUSED_PF; p++) {
+        const int ch = tid + p * 512;
+        if (ch < nchunk)
+          *reinterpret_cast<uint4*>(st + (size_t)(ch >> 2) * TGX_FUSED_SLD + (ch & 3) * 8) = pf[p];
+      }
+    };
+    const int nslab = K1 >> 5;
+    fetch(0);
+    stash(0);
     __syncthreads();
-  }
-  // ---- epilogue: E partials and W (BF16) into LDS (the staging is free now)
-  const float* al = reinterpret_cast<const float*>(ek.alphas);
-  const float zd2 = (float)ek.zd2;
-  const int zint = ek.zint;
-  float rs[4][4];
-#pragma unroll
-  for (int mi = 0; mi < 4; mi++)
-#pragma unroll
-    for (int v = 0; v < 4; v++) rs[mi][v] = 0.f;
-#pragma unroll
-  for (int ni = 0; ni < TGX_FUSED_MAXT; ni++) {
-    if (ni < nt1) {
-      const int c = c1 + ni * 16 + lr;
-      const float a = c < ek.n_sparse ? al[c] : 0.f;  // padding: W = 0, no energy
+    for (int sl = 0; sl < nslab; sl++) {
+      if (sl + 1 < nslab) fetch((sl + 1) * 32);
+      const u16* st = sW + (size_t)(sl & 1) * stage_rows * TGX_FUSED_SLD;
+      bf16x8 af[4];
 #pragma unroll
       for (int mi = 0; mi < 4; mi++)
+        af[mi] = *reinterpret_cast<const bf16x8*>(
+            st + (size_t)(mh * 64 + mi * 16 + lr) * TGX_FUSED_SLD + lk);
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-          const int rl = mh * 64 + mi * 16 + (lane >> 4) * 4 + v;
-          float kz, kz1;
-          kpow_i(acc[mi][ni][v], zint, &kz, &kz1);
-          rs[mi][v] += a * kz;
-          sW[(size_t)rl * ldw + c] = to_bf16(zd2 * a * kz1);
+      for (int ni = 0; ni < TGX_FUSED_MAXT; ni++)
+        if (ni < nt1) {
+          const bf16x8 bfr = *reinterpret_cast<const bf16x8*>(
+              st + (size_t)(128 + c1 + ni * 16 + lr) * TGX_FUSED_SLD + lk);
+#pragma unroll
+          for (int mi = 0; mi < 4; mi++)
+            acc[mi][ni] =
+                __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr, af[mi], acc[mi][ni], 0, 0, 0);
         }
+      if (sl + 1 < nslab) stash((sl + 1) & 1);
+      __syncthreads();
     }
   }
+
+  // ---- epilogue: E partials and W (BF16, four columns per 8-byte store)
+  const float* al = reinterpret_cast<const float*>(ek.alphas);  // zero past n_sparse
+  const float zd2 = (float)ek.zd2;
+  float rs[4] = {0.f, 0.f, 0.f, 0.f};
+  auto epilogue = [&](auto zi) {
+    constexpr int ZI = decltype(zi)::v;
 #pragma unroll
-  for (int mi = 0; mi < 4; mi++)
+    for (int ni = 0; ni < TGX_FUSED_MAXT; ni++) {
+      if (ni < nt1) {
+        const int s0 = c1 + ni * 16 + lc;
+        const float4 a4 = *reinterpret_cast<const float4*>(al + s0);
+        const float av[4] = {a4.x, a4.y, a4.z, a4.w};
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-      float x = rs[mi][v];
-      x += __shfl_xor(x, 1, 64);
-      x += __shfl_xor(x, 2, 64);
-      x += __shfl_xor(x, 4, 64);
-      x += __shfl_xor(x, 8, 64);
-      if (lr == 0) sE[nq * 128 + mh * 64 + mi * 16 + (lane >> 4) * 4 + v] = x;
+        for (int mi = 0; mi < 4; mi++) {
+          __bf16 w[4];
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            float kz, kz1;
+            kpow_f<ZI>(acc[mi][ni][v], ek.zint, &kz, &kz1);
+            rs[mi] += av[v] * kz;
+            w[v] = (__bf16)(zd2 * av[v] * kz1);
+          }
+          *reinterpret_cast<uint2*>(sW + (size_t)(mh * 64 + mi * 16 + lr) * ldw + s0) =
+              __builtin_bit_cast(uint2, w);
+        }
+      }
     }
+  };
+  switch (ek.zint) {
+    case 1: epilogue(ICz<1>{}); break;
+    case 2: epilogue(ICz<2>{}); break;
+    case 3: epilogue(ICz<3>{}); break;
+    case 4: epilogue(ICz<4>{}); break;
+    default: epilogue(ICz<0>{}); break;
+  }
+#pragma unroll
+  for (int mi = 0; mi < 4; mi++) {
+    float x = rs[mi];
+    x += __shfl_xor(x, 16, 64);
+    x += __shfl_xor(x, 32, 64);
+    if (lane < 16) sE[nq * 128 + mh * 64 + mi * 16 + lr] = x;
+  }
   __syncthreads();
   if (tid < 128 && row0 + tid < M)
     esite[row0 + tid] = d2 * ((double)sE[tid] + (double)sE[128 + tid] + (double)sE[256 + tid] +
                               (double)sE[384 + tid]);
 
-  // ---- stage 2: G = W Qs over this wave's quarter of the output columns
+  // ---- stage 2: D[c][m] = G[m][c] over this wave's quarter of the columns
   f32x4 acc2[4][NT2];
 #pragma unroll
   for (int a = 0; a < 4; a++)
 #pragma unroll
     for (int b = 0; b < NT2; b++) acc2[a][b] = f32x4{0, 0, 0, 0};
   const int c2 = nq * (16 * NT2);
-  const u16* bsrc = qsT + (size_t)(c2 + lr) * ldqt + lk;
-  bf16x8 b0[NT2], b1[NT2];
-  auto bload = [&](bf16x8* dst, int k0) {
+  {
+    // K1 = 64 NT2 rows of four 16-byte chunks: NT2 / 2 chunks per thread
+    constexpr int nchunk = 256 * NT2, PF2 = (NT2 + 1) / 2;
+    uint4 pf[PF2];
 #pragma unroll
-    for (int nj = 0; nj < NT2; nj++)
-      dst[nj] = *reinterpret_cast<const bf16x8*>(bsrc + (size_t)nj * 16 * ldqt + k0);
-  };
-  auto step2 = [&](const bf16x8* bf, int k0) {
-    bf16x8 af[4];
+    for (int p = 0; p < PF2; p++) pf[p] = uint4{0, 0, 0, 0};
+    auto fetch = [&](int k0) {
 #pragma unroll
-    for (int mi = 0; mi < 4; mi++)
-      af[mi] = *reinterpret_cast<const bf16x8*>(sW + (size_t)(mh * 64 + mi * 16 + lr) * ldw + k0 + lk);
+      for (int p = 0; p < PF2; p++) {
+        const int ch = tid + p * 512;
+        if (ch < nchunk)
+          pf[p] = *reinterpret_cast<const uint4*>(qsT + (size_t)(ch >> 2) * ldqt + k0 +
+                                                  (ch & 3) * 8);
+      }
+    };
+    auto stash = [&]() {
 #pragma unroll
-    for (int mi = 0; mi < 4; mi++)
+      for (int p = 0; p < PF2; p++) {
+        const int ch = tid + p * 512;
+        const int r = ch >> 2;
+        if (ch < nchunk)
+          *reinterpret_cast<uint4*>(sT + (size_t)r * 32 + (((ch & 3) ^ ((r >> 2) & 3)) * 8)) =
+              pf[p];
+      }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < N1; k0 += 32) {
+      __syncthreads();  // the slab (and sE) is no longer read
+      stash();
+      __syncthreads();
+      if (k0 + 32 < N1) fetch(k0 + 32);
+      bf16x8 aw[4];
 #pragma unroll
-      for (int nj = 0; nj < NT2; nj++)
-        acc2[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mi], bf[nj], acc2[mi][nj], 0, 0, 0);
-  };
-  // two k-steps in flight: b0 holds step k, b1 step k+32 (N1 is a multiple of 64)
-  bload(b0, 0);
-  bload(b1, 32);
-  for (int k0 = 0; k0 < N1; k0 += 64) {
-    step2(b0, k0);
-    if (k0 + 64 < N1) bload(b0, k0 + 64);
-    step2(b1, k0 + 32);
-    if (k0 + 96 < N1) bload(b1, k0 + 96);
+      for (int mi = 0; mi < 4; mi++)
+        aw[mi] = *reinterpret_cast<const bf16x8*>(sW + (size_t)(mh * 64 + mi * 16 + lr) * ldw +
+                                                  k0 + lk);
+#pragma unroll
+      for (int nj = 0; nj < NT2; nj++) {
+        const int r = c2 + nj * 16 + lr;
+        const bf16x8 bq = *reinterpret_cast<const bf16x8*>(
+            sT + (size_t)r * 32 + (((lane >> 4) ^ ((r >> 2) & 3)) * 8));
+#pragma unroll
+        for (int mi = 0; mi < 4; mi++)
+          acc2[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bq, aw[mi], acc2[mi][nj], 0, 0, 0);
+      }
+    }
   }
 #pragma unroll
-  for (int mi = 0; mi < 4; mi++)
+  for (int mi = 0; mi < 4; mi++) {
+    const int r = rw + mi * 16 + lr;
+    if (r < M)
 #pragma unroll
-    for (int nj = 0; nj < NT2; nj++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int r = rw + mi * 16 + (lane >> 4) * 4 + v;
-        if (r < M) G[(size_t)r * ldg + c2 + nj * 16 + lr] = acc2[mi][nj][v];
-      }
+      for (int nj = 0; nj < NT2; nj++)
+        *reinterpret_cast<f32x4*>(G + (size_t)r * ldg + c2 + nj * 16 + lc) = acc2[mi][nj];
+  }
 }
 
 template <int NT2>
@@ -463,9 +511,7 @@ static hipError_t run_fused(int M, int sp_pad, int dim_pad, const void* q, const
                             const void* qsT, const EpiKernel& ek, double delta, double* esite,
                             void* G, const int* guard, hipStream_t st) {
   static size_t sm_set = 0;
-  const size_t wbytes = (size_t)128 * (sp_pad + 8) * 2;
-  const size_t sbytes = (size_t)2 * (128 + sp_pad) * TGX_FUSED_SLD * 2;
-  const size_t sm = (wbytes > sbytes ? wbytes : sbytes) + 4 * 128 * sizeof(float);
+  const size_t sm = fused_region_w(sp_pad) + fused_region_t(dim_pad);
   if (sm > sm_set) {
     hipError_t e = hipFuncSetAttribute((const void*)gap_fused_kernel<NT2>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
@@ -494,7 +540,9 @@ static hipError_t run(int n_sites, int dim, int dim_pad, int n_sparse, int sp_pa
     // TGX_GEMM_UNFUSED=1 selects the two separate GEMMs (for comparisons)
     const char* uf = getenv("TGX_GEMM_UNFUSED");
     if (want_grad && zint >= 1 && zint <= 8 && sp_pad <= 64 * TGX_FUSED_MAXT &&
-        dim_pad <= 64 * TGX_FUSED_MAXT && !(uf && uf[0] == '1')) {
+        dim_pad <= 64 * TGX_FUSED_MAXT &&
+        fused_region_w(sp_pad) + fused_region_t(dim_pad) <= 160 * 1024 &&
+        !(uf && uf[0] == '1')) {
       switch (dim_pad / 64) {
 #define TGX_FUSED_CASE(t) \
   case t: return run_fused<t>(M, sp_pad, dim_pad, q, qs, qsT, ek, delta, esite, G, guard, st);

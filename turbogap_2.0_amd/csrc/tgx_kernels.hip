@@ -267,7 +267,7 @@ __device__ __forceinline__ void nb_decide(const double* pos, const double* pi, c
 }
 
 __global__ __launch_bounds__(256) void find_sorted_kernel(
-    const double* pos, int n, Cell c, double rcut, const int* cell_start,
+    const double* pos, int n, Cell c, double rcut, const int* cell_of, const int* cell_start,
     const float4* cpos, int* count, int maxnb, int* list, int* flag, double* pref,
     const int* guard) {
   GUARD_RETURN();
@@ -278,8 +278,9 @@ __global__ __launch_bounds__(256) void find_sorted_kernel(
   const bool active = i < n;
   const int ii = active ? i : 0;
   const double* pi = pos + 3 * ii;
-  int cx, cy, cz;
-  cell_coords(pi, c, &cx, &cy, &cz);
+  // the atom's own cell as binned (cell_coords)
+  const int own = cell_of[ii];
+  const int cx = own % c.mx, cy = (own / c.mx) % c.my, cz = own / (c.mx * c.my);
   // lanes 0..26: cell q = lane, its atom count and first slot
   int cnt_q = 0, base_q = 0;
   if (lane < 27) {
@@ -531,7 +532,7 @@ hipError_t launch_neighbors(const double* pos, int n, const Cell& c, double rcut
   }
   if (fused) {
     hipLaunchKernelGGL(find_sorted_kernel, dim3((n + 3) / 4), dim3(256), 0, st, pos, n, c, rcut,
-                       cell_start, cpos, count, maxnb, list, flag, pref, guard);
+                       cell_of, cell_start, cpos, count, maxnb, list, flag, pref, guard);
   } else {
     hipLaunchKernelGGL(zero_ints_kernel, dim3(nb), dim3(B), 0, st, count, n, guard);
     hipLaunchKernelGGL(find_neighbors_kernel, dim3((n * TGX_NB_LANES + B - 1) / B), dim3(B), 0,
