@@ -419,7 +419,7 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
   const bool fuse = fuse_two_body(c, n, n_sites, do_forces);
   if (do_forces && c->has_soap && n_sites > 0) {
     const size_t ts = tsize(prec);
-    ENSURE(c, gpair, ts * 3 * (size_t)n * c->maxnb);
+    ENSURE(c, gpair, ts * 4 * (size_t)n * c->maxnb);  // (g, 0) per list slot
     hipError_t e;
     kev_rec(c, 4);
     if (prec == TGX_FP64) {
@@ -437,7 +437,7 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
       kev_rec(c, 6);
       if (e == hipSuccess)
         e = launch_force_gather<double>(n, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
-                                        c->nb_rev.as<int>(), c->site_of_atom.as<int>(),
+                                        c->nb_rev.as<int>(), c->site_of_atom.as<int>(), n_sites == n ? 1 : 0,
                                         c->gpair.as<double>(), c->forces.as<double>(), c->stream);
     } else {
       e = launch_soap_bwd<float>(c->sd, x.sd_dev.as<SoapDev>(), c->W_dev.as<double>(),
@@ -454,7 +454,7 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
       kev_rec(c, 6);
       if (e == hipSuccess)
         e = launch_force_gather<float>(n, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
-                                       c->nb_rev.as<int>(), c->site_of_atom.as<int>(),
+                                       c->nb_rev.as<int>(), c->site_of_atom.as<int>(), n_sites == n ? 1 : 0,
                                        c->gpair.as<float>(), c->forces.as<double>(), c->stream);
     }
     TGX_HIP(c, e);

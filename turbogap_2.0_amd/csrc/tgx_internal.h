@@ -176,7 +176,7 @@ hipError_t launch_gap_kernel(int mode, int n_sites, int dim, int dim_pad,
 template <typename T>
 hipError_t launch_force_gather(int n, int maxnb, const int* count,
                                const int* list, const int* rev,
-                               const int* site_of_atom, const T* gpair,
+                               const int* site_of_atom, int all_sites, const T* gpair,
                                double* forces, hipStream_t st);
 
 }  // namespace tgx

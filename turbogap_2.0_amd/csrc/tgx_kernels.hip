@@ -3,6 +3,8 @@
 // term, energy assembly and deterministic reductions.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "tgx_device.h"
 #include "tgx_internal.h"
 
@@ -675,31 +677,35 @@ hipError_t launch_csr_out(const double* pos, int n, const Cell& c, int maxnb,
 
 // ---------------------------------------------------------------------------
 // Forces: F_k = sum_slots (g[k][slot] - g[rev[k][slot]]), fixed order, FP64.
+// g holds one 4-vector (gx, gy, gz, 0) per slot: one 16- or 32-byte load per
+// gathered slot.  all_sites: every atom is a SOAP site (no site checks).
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void force_gather_kernel(int n, int maxnb, const int* count,
                                                            const int* list, const int* rev,
-                                                           const int* site_of_atom,
+                                                           const int* site_of_atom, int all_sites,
                                                            const T* gpair, double* forces) {
+  typedef typename std::conditional<sizeof(T) == 4, float4, double4>::type T4;
+  const T4* g4 = reinterpret_cast<const T4*>(gpair);
   const int lane = threadIdx.x & 63;
   const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (k >= n) return;
   const int cnt = min(count[k], maxnb);
-  const bool ksite = site_of_atom[k] >= 0;
+  const bool ksite = all_sites || site_of_atom[k] >= 0;
   double f0 = 0, f1 = 0, f2 = 0;
   for (int s = lane; s < cnt; s += 64) {
     const size_t p = (size_t)k * maxnb + s;
     if (ksite) {
-      f0 += (double)gpair[p * 3];
-      f1 += (double)gpair[p * 3 + 1];
-      f2 += (double)gpair[p * 3 + 2];
+      const T4 g = g4[p];
+      f0 += (double)g.x;
+      f1 += (double)g.y;
+      f2 += (double)g.z;
     }
-    const int j = list[p] & TGX_ATOM_MASK;
-    if (site_of_atom[j] >= 0) {
-      const size_t rp = (size_t)rev[p];
-      f0 -= (double)gpair[rp * 3];
-      f1 -= (double)gpair[rp * 3 + 1];
-      f2 -= (double)gpair[rp * 3 + 2];
+    if (all_sites || site_of_atom[list[p] & TGX_ATOM_MASK] >= 0) {
+      const T4 g = g4[rev[p]];
+      f0 -= (double)g.x;
+      f1 -= (double)g.y;
+      f2 -= (double)g.z;
     }
   }
 #pragma unroll
@@ -717,14 +723,14 @@ __global__ __launch_bounds__(256) void force_gather_kernel(int n, int maxnb, con
 
 template <typename T>
 hipError_t launch_force_gather(int n, int maxnb, const int* count, const int* list,
-                               const int* rev, const int* site_of_atom, const T* gpair,
-                               double* forces, hipStream_t st) {
+                               const int* rev, const int* site_of_atom, int all_sites,
+                               const T* gpair, double* forces, hipStream_t st) {
   hipLaunchKernelGGL(force_gather_kernel<T>, dim3((n + 3) / 4), dim3(256), 0, st, n, maxnb,
-                     count, list, rev, site_of_atom, gpair, forces);
+                     count, list, rev, site_of_atom, all_sites, gpair, forces);
   return hipGetLastError();
 }
-template hipError_t launch_force_gather<float>(int, int, const int*, const int*, const int*, const int*, const float*, double*, hipStream_t);
-template hipError_t launch_force_gather<double>(int, int, const int*, const int*, const int*, const int*, const double*, double*, hipStream_t);
+template hipError_t launch_force_gather<float>(int, int, const int*, const int*, const int*, const int*, int, const float*, double*, hipStream_t);
+template hipError_t launch_force_gather<double>(int, int, const int*, const int*, const int*, const int*, int, const double*, double*, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // distance_2b term (gap.f90:329-460): each pair counted from both atoms,

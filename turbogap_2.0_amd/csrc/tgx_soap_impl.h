@@ -107,8 +107,8 @@ template <int NRP> struct Capv { static constexpr int v = 2 * (64 / Spw<NRP>::v)
 
 // Per-site LDS footprint in bytes (16-byte multiple): forward = harmonics
 // and radial values of a chunk of pairs (the first part reused for c) plus
-// the compacted slot list; adjoint = dE/dc [L2][NRP], the coupling matrices
-// M_l [Lp][NRP][NRP] and the slot list.
+// the compacted slot list (16-bit slots); adjoint = dE/dc [L2][NRP], the
+// upper triangles of the coupling matrices M_l and the slot list.
 // Forward region A holds the harmonics of a chunk (CH x L2), later c with
 // rows padded to a multiple of four (NRP x L2P).
 template <typename T, int NRP>
@@ -120,15 +120,23 @@ __host__ __device__ constexpr size_t fwd_region_a(int L2) {
 template <typename T, int NRP>
 __host__ __device__ constexpr size_t fwd_site_bytes(int L2) {
   constexpr int CH = Chunk<T, Spw<NRP>::v>::CH;
-  return ((sizeof(T) * (fwd_region_a<T, NRP>(L2) + (size_t)CH * NRP) + 4 * Capv<NRP>::v) + 15) /
+  return ((sizeof(T) * (fwd_region_a<T, NRP>(L2) + (size_t)CH * NRP) + 2 * Capv<NRP>::v) + 15) /
          16 * 16;
 }
-// Stride of one coupling matrix M_l in LDS: NRP^2 + 4 elements, so that the
-// matrices of consecutive l start in different banks (16-byte aligned rows).
-template <int NRP> struct MStride { static constexpr int v = NRP * NRP + 4; };
+// The symmetric coupling matrices M_l keep their upper triangle (n <= n2) in
+// LDS, tri(n, n2) = n (2 NRP - n + 1) / 2 + n2 - n; the stride per l is odd so
+// that lanes reading the same entry of different l hit different banks.
+template <int NRP> struct MStride {
+  static constexpr int t = NRP * (NRP + 1) / 2;
+  static constexpr int v = t % 2 ? t : t + 1;
+};
+template <int NRP>
+__host__ __device__ constexpr int tri(int a, int b) {
+  return a <= b ? a * (2 * NRP - a + 1) / 2 + (b - a) : b * (2 * NRP - b + 1) / 2 + (a - b);
+}
 template <typename T, int NRP>
 __host__ __device__ constexpr size_t bwd_site_bytes(int L2, int Lp) {
-  return ((sizeof(T) * ((size_t)NRP * L2 + (size_t)Lp * MStride<NRP>::v) + 4 * Capv<NRP>::v) +
+  return ((sizeof(T) * ((size_t)NRP * L2 + (size_t)Lp * MStride<NRP>::v) + 2 * Capv<NRP>::v) +
           15) / 16 * 16;
 }
 
@@ -140,7 +148,7 @@ __host__ __device__ constexpr size_t bwd_site_bytes(int L2, int Lp) {
 template <int LW, int SPW, typename GEO, typename F>
 __device__ __forceinline__ int scan_pairs(const SoapDev* sd, const double* pos,
                                           const int* species, const Cell& cell, const int* row,
-                                          int i, int cnt, int* next, int* idx, int capv,
+                                          int i, int cnt, int* next, u16* idx, int capv,
                                           int sub, int lane, F on_skip) {
   int nv = 0;
   while (*next < cnt && nv <= capv - LW) {
@@ -156,7 +164,7 @@ __device__ __forceinline__ int scan_pairs(const SoapDev* sd, const double* pos,
     unsigned long long mask = __ballot(valid);
     if (SPW == 2) mask = (mask >> (sub * 32)) & 0xffffffffull;
     const int jc = __popcll(mask & ((1ull << lane) - 1ull));
-    if (valid) idx[nv + jc] = slot;
+    if (valid) idx[nv + jc] = (u16)slot;
     nv += __popcll(mask);
     *next += LW;
   }
@@ -208,7 +216,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FwdWaves<T>
   constexpr int CAPV = Capv<NRP>::v;
   T* sAY = reinterpret_cast<T*>(smem_raw + (size_t)(wave * SPW + sub) * fwd_site_bytes<T, NRP>(L2));
   T* sR = sAY + fwd_region_a<T, NRP>(L2);
-  int* sIdx = reinterpret_cast<int*>(sR + CH * NRP);
+  u16* sIdx = reinterpret_cast<u16*>(sR + CH * NRP);
   T* sC = sAY;  // reused once the accumulation is done
   const int site = (blockIdx.x * wpb + wave) * SPW + sub;
   if (site >= n_sites) return;
@@ -391,7 +399,7 @@ __global__ __launch_bounds__(256) void soap_bwd_kernel(
   // sGC [L2][NRP] = dE/dc; sM [Lp][NRP][NRP]: dE/dc = M_l c
   T* sM = sGC + NRP * L2;
   constexpr int MS = MStride<NRP>::v;
-  int* sIdx = reinterpret_cast<int*>(sM + Lp * MS);
+  u16* sIdx = reinterpret_cast<u16*>(sM + Lp * MS);
   const int site = (blockIdx.x * wpb + wave) * SPW + sub;
   if (site >= n_sites) return;
   const int i = site_atom[site];
@@ -415,8 +423,7 @@ __global__ __launch_bounds__(256) void soap_bwd_kernel(
     const int n1 = dm & 255, n2 = (dm >> 8) & 255, l = dm >> 16;
     T* Ml = sM + l * MS;
     const T w = (n1 == n2 ? (T)2 : (T)1.41421356237309504880) * gp;
-    Ml[n1 * NRP + n2] = w;  // both writes hit one entry on the diagonal
-    Ml[n2 * NRP + n1] = w;
+    Ml[tri<NRP>(n1, n2)] = w;
   }
   wave_sync();
   for (int lm = lane; lm < L2; lm += LW) {
@@ -429,22 +436,21 @@ __global__ __launch_bounds__(256) void soap_bwd_kernel(
     for (int n = 0; n < NRP; n++) {
       T a = 0;
 #pragma unroll
-      for (int n2 = 0; n2 < NRP; n2++) a += Ml[n * NRP + n2] * cl[n2];
+      for (int n2 = 0; n2 < NRP; n2++) a += Ml[tri<NRP>(n, n2)] * cl[n2];
       sGC[lm * NRP + n] = a;
     }
   }
   wave_sync();
   double v[6] = {0, 0, 0, 0, 0, 0};
   double e2b = 0.0;
-  T* gp_out = gpair + (size_t)i * maxnb * 3;
+  typedef typename std::conditional<sizeof(T) == 4, float4, double4>::type T4;
+  T4* gp_out = reinterpret_cast<T4*>(gpair) + (size_t)i * maxnb;  // (g, 0) per slot
   int next = 0;
   while (next < cnt) {
     // valid pairs are packed onto lanes; skipped slots get a zero gradient
     const int nv = scan_pairs<LW, SPW, GEO>(sd, pos, species, cell, row, i, cnt, &next, sIdx,
                                             CAPV, sub, lane, [&](int slot) {
-                                              gp_out[slot * 3 + 0] = (T)0;
-                                              gp_out[slot * 3 + 1] = (T)0;
-                                              gp_out[slot * 3 + 2] = (T)0;
+                                              gp_out[slot] = T4{0, 0, 0, 0};
                                             });
     wave_sync();
     for (int c0 = 0; c0 < nv; c0 += LW) {
@@ -562,9 +568,7 @@ __global__ __launch_bounds__(256) void soap_bwd_kernel(
       v[4] -= 0.5 * (gd[0] * dv[2] + gd[2] * dv[0]);
       v[5] -= 0.5 * (gd[1] * dv[2] + gd[2] * dv[1]);
     }
-      gp_out[slot * 3 + 0] = g0;
-      gp_out[slot * 3 + 1] = g1;
-      gp_out[slot * 3 + 2] = g2;
+      gp_out[slot] = T4{g0, g1, g2, (T)0};
     }
     wave_sync();  // the slot list is refilled by the next scan round
   }
@@ -607,6 +611,7 @@ static hipError_t fwd_inst(const SoapDev& sdh, const SoapDev* sd, const double* 
                            const int* site_atom, int n_sites, const int* dmap, T* q, int q_ld,
                            u16* qb, int qb_ld, T* cbuf, T* pnorm, hipStream_t st) {
   constexpr int SPW = Spw<NRP>::v;
+  if (maxnb > 65535) return hipErrorInvalidValue;  // 16-bit slot lists
   const size_t per_wave = SPW * fwd_site_bytes<T, NRP>(sdh.L2);
   const int wpb = waves_per_block(per_wave);
   const size_t sm = wpb * per_wave;
@@ -630,6 +635,7 @@ static hipError_t bwd_inst(const SoapDev& sdh, const SoapDev* sd, const double* 
                            T* gpair, double* vsite, const TwoBodyDev* twob, int n2b,
                            double* eatom, hipStream_t st) {
   constexpr int SPW = Spw<NRP>::v;
+  if (maxnb > 65535) return hipErrorInvalidValue;  // 16-bit slot lists
   const size_t per_wave = SPW * bwd_site_bytes<T, NRP>(sdh.L2, sdh.l_max + 1);
   const int wpb = waves_per_block(per_wave);
   const size_t sm = wpb * per_wave;
