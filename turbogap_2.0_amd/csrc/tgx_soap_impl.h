@@ -196,6 +196,10 @@ __device__ __forceinline__ void ang_factors(T log2t, T* Al) {
 // (measured best on gfx950 with 12-pair chunks); FP64 is left unconstrained.
 template <typename T> struct FwdWaves { static constexpr int v = 4; };
 template <> struct FwdWaves<double> { static constexpr int v = 1; };
+// The same for the FP32 adjoint (4: 148 us at 32k atoms vs 163 unconstrained
+// at 3 waves and 197 at 5 with spills).
+template <typename T> struct BwdWaves { static constexpr int v = 4; };
+template <> struct BwdWaves<double> { static constexpr int v = 1; };
 
 template <typename T, int NRP, int NMX, int LMX, bool EXL, typename GEO>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FwdWaves<T>::v))) void soap_fwd_kernel(
@@ -333,42 +337,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FwdWaves<T>
   wave_sync();
   // power spectrum, lanes = radial pairs (n1 <= n2): p_l = w sum_m
   // c[n1][l,m] c[n2][l,m] for every l, w = 1 or sqrt 2 (no divergence across
-  // lanes); pass 1 the norm, pass 2 q = p / |p| written as T and BF16
+  // lanes), kept in registers; then the norm and q = p / |p| written as T
+  // and BF16
   const int dim = sd->dim, Lp = L + 1, npair = sd->npair;
+  constexpr int KPI = (NRP * (NRP + 1) / 2 + LW - 1) / LW;  // radial pairs per lane
   T* qs = q + (size_t)site * q_ld;
   u16* qbs = qb ? qb + (size_t)site * qb_ld : nullptr;
+  T pv[KPI][LMX + 1];
   T ss = 0;
-  for (int kp = lane; kp < npair; kp += LW) {
-    const int dm = dmap[kp * Lp];
-    const T* ca = sC + (dm & 255) * L2P;
-    const T* cb = sC + ((dm >> 8) & 255) * L2P;
-    const T w = (dm & 255) == ((dm >> 8) & 255) ? (T)1 : (T)1.41421356237309504880;
 #pragma unroll
-    for (int l = 0; l <= LMX; l++) {
-      if (l <= L) {
-        T a = 0;
+  for (int t = 0; t < KPI; t++) {
+    const int kp = lane + t * LW;
+    if (kp < npair) {
+      const int dm = dmap[kp * Lp];
+      const T* ca = sC + (dm & 255) * L2P;
+      const T* cb = sC + ((dm >> 8) & 255) * L2P;
+      const T w = (dm & 255) == ((dm >> 8) & 255) ? (T)1 : (T)1.41421356237309504880;
 #pragma unroll
-        for (int m = 0; m <= 2 * l; m++) a += ca[l * l + m] * cb[l * l + m];
-        ss += (w * a) * (w * a);
+      for (int l = 0; l <= LMX; l++) {
+        if (l <= L) {
+          T a = 0;
+#pragma unroll
+          for (int m = 0; m <= 2 * l; m++) a += ca[l * l + m] * cb[l * l + m];
+          pv[t][l] = w * a;
+          ss += pv[t][l] * pv[t][l];
+        }
       }
     }
   }
   ss = sub_sum<LW>(ss);
   const T inv = (T)1 / sqrt(ss);
-  for (int kp = lane; kp < npair; kp += LW) {
-    const int dm = dmap[kp * Lp];
-    const T* ca = sC + (dm & 255) * L2P;
-    const T* cb = sC + ((dm >> 8) & 255) * L2P;
-    const T w = ((dm & 255) == ((dm >> 8) & 255) ? (T)1 : (T)1.41421356237309504880) * inv;
 #pragma unroll
-    for (int l = 0; l <= LMX; l++) {
-      if (l <= L) {
-        T a = 0;
+  for (int t = 0; t < KPI; t++) {
+    const int kp = lane + t * LW;
+    if (kp < npair) {
 #pragma unroll
-        for (int m = 0; m <= 2 * l; m++) a += ca[l * l + m] * cb[l * l + m];
-        const T v = w * a;
-        qs[kp * Lp + l] = v;
-        if (qbs) qbs[kp * Lp + l] = f2bf((float)v);
+      for (int l = 0; l <= LMX; l++) {
+        if (l <= L) {
+          const T v = pv[t][l] * inv;
+          qs[kp * Lp + l] = v;
+          if (qbs) qbs[kp * Lp + l] = f2bf((float)v);
+        }
       }
     }
   }
@@ -379,7 +388,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FwdWaves<T>
 }
 
 template <typename T, int NRP, int NMX, int LMX, bool EXL, typename GEO>
-__global__ __launch_bounds__(256) void soap_bwd_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BwdWaves<T>::v))) void soap_bwd_kernel(
     const SoapDev* __restrict__ sd, const double* __restrict__ W,
     const float2* __restrict__ rtab, const double* __restrict__ pos,
     const int* __restrict__ species, GEO geo, int maxnb, const int* __restrict__ count,
@@ -474,6 +483,12 @@ __global__ __launch_bounds__(256) void soap_bwd_kernel(
       ang_factors<T, LMX>(log2t, Al);
       T s1 = 0, Vx = 0, Vy = 0, Vz = 0;
       T Cm = 1, Sm = 0, Cp = 0, Sp = 0;  // C_m, S_m and C_{m-1}, S_{m-1}
+      T dAl[LMX + 1];  // dA_l/dr = A_l l(l+1) D
+#pragma unroll
+      for (int l = 0; l <= LMX; l++) dAl[l] = Al[l] * ((T)(l * (l + 1)) * D);
+      // per (l, +-m): the radial contractions h = sum_n dE/dc R_n and
+      // hd = sum_n dE/dc dR_n of the cosine and sine rows, then the radial
+      // (s1) and angular (V) parts of the gradient with the two rows combined
 #pragma unroll
       for (int m = 0; m <= LMX; m++) {
         if (m <= L) {
@@ -492,37 +507,32 @@ __global__ __launch_bounds__(256) void soap_bwd_kernel(
                 dm2 = dm1;
                 dm1 = dP;
               }
-              const T A = Al[l];
-              const T dA = A * (T)(l * (l + 1)) * D;
-              {
-                const T* gc = sGC + (l * l + l + m) * NRP + o;
-                T h = 0, hd = 0;
+              const T A = Al[l], dA = dAl[l];
+              const T* gcc = sGC + (l * l + l + m) * NRP + o;
+              T hc = 0, hdc = 0;
 #pragma unroll
-                for (int n = 0; n < NMX; n++) {
-                  h += gc[n] * R[n];
-                  hd += gc[n] * dR[n];
-                }
-                s1 += (hd * A + h * dA) * (P * Cm);
-                const T hA = h * A;
-                if (m > 0) {
-                  Vx += hA * P * (T)m * Cp;
-                  Vy -= hA * P * (T)m * Sp;
-                }
-                Vz += hA * dP * Cm;
+              for (int n = 0; n < NMX; n++) {
+                hc += gcc[n] * R[n];
+                hdc += gcc[n] * dR[n];
               }
-              if (m > 0) {
-                const T* gc = sGC + (l * l + l - m) * NRP + o;
-                T h = 0, hd = 0;
+              const T Xc = hdc * A + hc * dA, hAc = hc * A;
+              if (m == 0) {
+                s1 += Xc * P;
+                Vz += hAc * dP;
+              } else {
+                const T* gcs = sGC + (l * l + l - m) * NRP + o;
+                T hs = 0, hds = 0;
 #pragma unroll
                 for (int n = 0; n < NMX; n++) {
-                  h += gc[n] * R[n];
-                  hd += gc[n] * dR[n];
+                  hs += gcs[n] * R[n];
+                  hds += gcs[n] * dR[n];
                 }
-                s1 += (hd * A + h * dA) * (P * Sm);
-                const T hA = h * A;
-                Vx += hA * P * (T)m * Sp;
-                Vy += hA * P * (T)m * Cp;
-                Vz += hA * dP * Sm;
+                const T Xs = hds * A + hs * dA, hAs = hs * A;
+                const T Pm = P * (T)m;
+                s1 += P * (Xc * Cm + Xs * Sm);
+                Vx += Pm * (hAc * Cp + hAs * Sp);
+                Vy += Pm * (hAs * Cp - hAc * Sp);
+                Vz += dP * (hAc * Cm + hAs * Sm);
               }
             }
           }
