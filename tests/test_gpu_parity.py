@@ -33,10 +33,12 @@ def _nb_equal(a, b, ulps_angles=True):
 
 
 @pytest.mark.parametrize("case", ["carbon216", "small32", "cho300", "carbon2000", "dense84",
-                                  "dense145"])
+                                  "dense145", "carbon6000"])
 def test_neighbors_bit_exact(eng, case):
     """Lists of up to 64, 65-128 and over 128 entries (dense84, dense145:
-    1,500 C at 3.2 g/cm^3 with rcut 5.0 / 6.0 A) take the three sort paths."""
+    1,500 C at 3.2 g/cm^3 with rcut 5.0 / 6.0 A) take the three sort paths;
+    up to 4,096 atoms the build runs in one workgroup, carbon6000 takes the
+    multi-kernel build."""
     if case == "carbon216":
         pos, sp, lat, _ = W.carbon216()
         rc = 3.7
@@ -48,6 +50,9 @@ def test_neighbors_bit_exact(eng, case):
         rc = 4.0
     elif case == "carbon2000":
         pos, sp, lat = S.carbon_cell(2000, 2.0, 1.2, 55)
+        rc = 4.2
+    elif case == "carbon6000":
+        pos, sp, lat = S.carbon_cell(6000, 2.0, 1.2, 57)
         rc = 4.2
     else:
         pos, sp, lat = S.carbon_cell(1500, 3.2, 1.0, 56)
@@ -129,8 +134,10 @@ def test_fused_bf16_kernel_step(eng, zeta, n_sparse, monkeypatch):
     eng.set_precision("mixed")
     try:
         e1, g1 = eng.kernel_step(q)
+        e4, _ = eng.kernel_step(q, want_qss=False)  # energies-only variant
         monkeypatch.setenv("TGX_GEMM_UNFUSED", "1")
         e2, g2 = eng.kernel_step(q)
+        e5, _ = eng.kernel_step(q, want_qss=False)
         monkeypatch.delenv("TGX_GEMM_UNFUSED")
         e3, g3 = eng.kernel_step(q[:77])  # a partial block of rows
     finally:
@@ -139,6 +146,8 @@ def test_fused_bf16_kernel_step(eng, zeta, n_sparse, monkeypatch):
           "max rel diff", np.abs(g1 - g2).max() / np.abs(g2).max())
     assert np.abs(g1 - g2).max() <= 1e-6 * np.abs(g2).max()
     assert np.abs(e1 - e2).max() <= 1e-6 * np.abs(e2).max()
+    assert np.array_equal(e4, e1)
+    assert np.abs(e5 - e2).max() <= 1e-6 * np.abs(e2).max()
     assert np.array_equal(g3, g1[:77])
     assert np.abs(e3 - e1[:77]).max() <= 1e-6 * np.abs(e1).max()
 

@@ -94,11 +94,12 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // fmod(a, p) for p > 0, bit-identical to the C library's for |a/p| < 2^40:
-// the remainder a - n p of the true quotient n = trunc(a/p) is exactly
+// a itself when |a| < p; otherwise the remainder a - n p of the true quotient n = trunc(a/p) is exactly
 // representable, so one fused multiply-add gives it exactly; a quotient
 // estimate off by one (a/p within rounding of an integer) puts the result
 // outside [0, p) (or (-p, 0] for negative a) and is corrected once.
 __host__ __device__ __forceinline__ double rem_exact(double a, double p) {
+  if (fabs(a) < p) return a;  // the common case of the minimum image: no division
   const double n = trunc(a / p);
   double r = fma(-n, p, a);
   if (a >= 0.0) {

@@ -315,7 +315,7 @@ __host__ __device__ inline size_t fused_region_t(int K1) {
   return t > 4 * 128 * sizeof(float) ? t : 4 * 128 * sizeof(float);
 }
 
-template <int NT2>
+template <int NT2, bool GRAD>
 __global__ __launch_bounds__(512) void gap_fused_kernel(
     int M, int N1, int K1, const u16* __restrict__ q, int ldq, const u16* __restrict__ qs,
     int ldqs, const u16* __restrict__ qsT, int ldqt, EpiKernel ek, double d2,
@@ -417,8 +417,9 @@ __global__ __launch_bounds__(512) void gap_fused_kernel(
             rs[mi] += av[v] * kz;
             w[v] = (__bf16)(zd2 * av[v] * kz1);
           }
-          *reinterpret_cast<uint2*>(sW + (size_t)(mh * 64 + mi * 16 + lr) * ldw + s0) =
-              __builtin_bit_cast(uint2, w);
+          if (GRAD)
+            *reinterpret_cast<uint2*>(sW + (size_t)(mh * 64 + mi * 16 + lr) * ldw + s0) =
+                __builtin_bit_cast(uint2, w);
         }
       }
     }
@@ -441,6 +442,7 @@ __global__ __launch_bounds__(512) void gap_fused_kernel(
   if (tid < 128 && row0 + tid < M)
     esite[row0 + tid] = d2 * ((double)sE[tid] + (double)sE[128 + tid] + (double)sE[256 + tid] +
                               (double)sE[384 + tid]);
+  if (!GRAD) return;
 
   // ---- stage 2: D[c][m] = G[m][c] over this wave's quarter of the columns
   f32x4 acc2[4][NT2];
@@ -506,19 +508,19 @@ __global__ __launch_bounds__(512) void gap_fused_kernel(
   }
 }
 
-template <int NT2>
+template <int NT2, bool GRAD>
 static hipError_t run_fused(int M, int sp_pad, int dim_pad, const void* q, const void* qs,
                             const void* qsT, const EpiKernel& ek, double delta, double* esite,
                             void* G, const int* guard, hipStream_t st) {
   static size_t sm_set = 0;
   const size_t sm = fused_region_w(sp_pad) + fused_region_t(dim_pad);
   if (sm > sm_set) {
-    hipError_t e = hipFuncSetAttribute((const void*)gap_fused_kernel<NT2>,
+    hipError_t e = hipFuncSetAttribute((const void*)gap_fused_kernel<NT2, GRAD>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
     if (e != hipSuccess) return e;
     sm_set = sm;
   }
-  hipLaunchKernelGGL(gap_fused_kernel<NT2>, dim3((M + 127) / 128), dim3(512), sm, st, M, sp_pad,
+  hipLaunchKernelGGL((gap_fused_kernel<NT2, GRAD>), dim3((M + 127) / 128), dim3(512), sm, st, M, sp_pad,
                      dim_pad, (const u16*)q, dim_pad, (const u16*)qs, dim_pad, (const u16*)qsT,
                      sp_pad, ek, delta * delta, esite, (float*)G, dim_pad, guard);
   return hipGetLastError();
@@ -539,13 +541,17 @@ static hipError_t run(int n_sites, int dim, int dim_pad, int n_sparse, int sp_pa
   if constexpr (sizeof(TI) == 2) {
     // TGX_GEMM_UNFUSED=1 selects the two separate GEMMs (for comparisons)
     const char* uf = getenv("TGX_GEMM_UNFUSED");
-    if (want_grad && zint >= 1 && zint <= 8 && sp_pad <= 64 * TGX_FUSED_MAXT &&
+    if (zint >= 1 && zint <= 8 && sp_pad <= 64 * TGX_FUSED_MAXT &&
         dim_pad <= 64 * TGX_FUSED_MAXT &&
         fused_region_w(sp_pad) + fused_region_t(dim_pad) <= 160 * 1024 &&
         !(uf && uf[0] == '1')) {
+      if (!want_grad)  // energies only: stage 1 and the row sums
+        return run_fused<1, false>(M, sp_pad, dim_pad, q, qs, qsT, ek, delta, esite, G, guard,
+                                   st);
       switch (dim_pad / 64) {
 #define TGX_FUSED_CASE(t) \
-  case t: return run_fused<t>(M, sp_pad, dim_pad, q, qs, qsT, ek, delta, esite, G, guard, st);
+  case t:                 \
+    return run_fused<t, true>(M, sp_pad, dim_pad, q, qs, qsT, ek, delta, esite, G, guard, st);
         TGX_FUSED_CASE(1) TGX_FUSED_CASE(2) TGX_FUSED_CASE(3) TGX_FUSED_CASE(4)
         TGX_FUSED_CASE(5) TGX_FUSED_CASE(6) TGX_FUSED_CASE(7) TGX_FUSED_CASE(8)
 #undef TGX_FUSED_CASE

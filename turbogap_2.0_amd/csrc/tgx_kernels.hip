@@ -268,20 +268,19 @@ __device__ __forceinline__ void nb_decide(const double* pos, const double* pi, c
   *found += __popcll(b);
 }
 
-__global__ __launch_bounds__(256) void find_sorted_kernel(
-    const double* pos, int n, Cell c, double rcut, const int* cell_of, const int* cell_start,
-    const float4* cpos, int* count, int maxnb, int* list, int* flag, double* pref,
-    const int* guard) {
-  GUARD_RETURN();
-  __shared__ int s_pre[4][28], s_base[4][28];
-  __shared__ int s_buf[4][TGX_NB_CAPL], s_q[4][128];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int i = blockIdx.x * 4 + w;
-  const bool active = i < n;
-  const int ii = active ? i : 0;
-  const double* pi = pos + 3 * ii;
-  // the atom's own cell as binned (cell_coords)
-  const int own = cell_of[ii];
+// Per-wave LDS of the one-pass build.
+struct NbWave {
+  int pre[28], base[28];  // first candidate index and first slot of the 27 cells
+  int buf[TGX_NB_CAPL];   // hits
+  int q[128];             // screened candidates awaiting the exact decision
+};
+
+// One atom's list by one wave (all 64 lanes; LDS of this wave only).
+// own = the atom's linked cell as binned, cell_start the cell offsets.
+__device__ void nb_find_one(const double* pos, int i, int own, const Cell& c, double rcut,
+                            const int* cell_start, const float4* cpos, int* count, int maxnb,
+                            int* list, int* flag, double* pref, NbWave& L, int lane) {
+  const double* pi = pos + 3 * i;
   const int cx = own % c.mx, cy = (own / c.mx) % c.my, cz = own / (c.mx * c.my);
   // lanes 0..26: cell q = lane, its atom count and first slot
   int cnt_q = 0, base_q = 0;
@@ -298,36 +297,35 @@ __global__ __launch_bounds__(256) void find_sorted_kernel(
     const int o = __shfl_up(incl, d, 64);
     if (lane >= d) incl += o;
   }
-  const int total = active ? __shfl(incl, 26, 64) : 0;
+  const int total = __shfl(incl, 26, 64);
   if (lane < 27) {
-    s_pre[w][lane] = incl - cnt_q;
-    s_base[w][lane] = base_q;
+    L.pre[lane] = incl - cnt_q;
+    L.base[lane] = base_q;
   }
-  if (pref && active && lane < 3) pref[3 * i + lane] = pi[lane];
-  __syncthreads();
+  if (pref && lane < 3) pref[3 * i + lane] = pi[lane];
+  wave_sync();
   // FP32 screen on wrapped coordinates: its error (a few 1e-5 A for boxes
   // of 100 A) is far inside the 1e-3 A margin, and images near L/2 are far
   // beyond rcut since L >= 3 rcut.
   const float rs = (float)rcut + 1e-3f, screen2 = rs * rs;
-  const float4 pw = wrapped_pos4(pos, ii, c);
+  const float4 pw = wrapped_pos4(pos, i, c);
   const float La = (float)c.La, Lb = (float)c.Lb, Lc = (float)c.Lc;
   const float iLa = (float)c.iLa, iLb = (float)c.iLb, iLc = (float)c.iLc;
   int found = 0, nq = 0;
-  int* out = list + (size_t)ii * maxnb;
-  int* sq = s_q[w];
+  int* out = list + (size_t)i * maxnb;
   for (int idx0 = 0; idx0 < total; idx0 += 64) {
     const int idx = idx0 + lane;
     bool near = false;
     int k = 0;
     if (idx < total) {
-      // largest q with s_pre[q] <= idx
+      // largest q with pre[q] <= idx
       int lo = 0, hi = 26;
       while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (s_pre[w][mid] <= idx) lo = mid;
+        if (L.pre[mid] <= idx) lo = mid;
         else hi = mid - 1;
       }
-      const float4 pk = cpos[s_base[w][lo] + idx - s_pre[w][lo]];
+      const float4 pk = cpos[L.base[lo] + idx - L.pre[lo]];
       k = __float_as_int(pk.w);
       if (k != i) {
         float dx = pk.x - pw.x, dy = pk.y - pw.y, dz = pk.z - pw.z;
@@ -338,38 +336,37 @@ __global__ __launch_bounds__(256) void find_sorted_kernel(
       }
     }
     const unsigned long long m = __ballot(near);
-    if (near) sq[nq + __popcll(m & ((1ull << lane) - 1ull))] = k;  // nq < 64 here
+    if (near) L.q[nq + __popcll(m & ((1ull << lane) - 1ull))] = k;  // nq < 64 here
     nq += __popcll(m);
     if (nq >= 64) {
       wave_sync();
-      nb_decide(pos, pi, c, rcut, sq, 64, lane, s_buf[w], out, maxnb, &found);
-      const int rest = nq - 64 > lane ? sq[64 + lane] : 0;
+      nb_decide(pos, pi, c, rcut, L.q, 64, lane, L.buf, out, maxnb, &found);
+      const int rest = nq - 64 > lane ? L.q[64 + lane] : 0;
       wave_sync();
-      if (lane < nq - 64) sq[lane] = rest;
+      if (lane < nq - 64) L.q[lane] = rest;
       nq -= 64;
     }
   }
   wave_sync();
-  if (nq > 0) nb_decide(pos, pi, c, rcut, sq, nq, lane, s_buf[w], out, maxnb, &found);
-  __syncthreads();
-  if (!active) return;
+  if (nq > 0) nb_decide(pos, pi, c, rcut, L.q, nq, lane, L.buf, out, maxnb, &found);
+  wave_sync();
   if (lane == 0) {
     count[i] = found;
     if (found > maxnb) atomicMax(flag, found);
   }
   const int keep = min(found, maxnb);
   if (found <= 64) {
-    int v = lane < found ? s_buf[w][lane] : 0x7fffffff;
+    int v = lane < found ? L.buf[lane] : 0x7fffffff;
     v = wave_sort64(v, lane);
     if (lane < keep) out[lane] = v;
   } else if (found <= TGX_NB_CAPL) {
-    int v0 = s_buf[w][lane];
-    int v1 = lane + 64 < found ? s_buf[w][lane + 64] : 0x7fffffff;
+    int v0 = L.buf[lane];
+    int v1 = lane + 64 < found ? L.buf[lane + 64] : 0x7fffffff;
     wave_sort128(v0, v1, lane);
     if (lane < keep) out[lane] = v0;
     if (lane + 64 < keep) out[lane + 64] = v1;
   } else {
-    for (int s = lane; s < TGX_NB_CAPL && s < keep; s += 64) out[s] = s_buf[w][s];
+    for (int s = lane; s < TGX_NB_CAPL && s < keep; s += 64) out[s] = L.buf[s];
     __threadfence_block();
     if (lane == 0) {
       for (int p = 1; p < keep; p++) {
@@ -383,6 +380,77 @@ __global__ __launch_bounds__(256) void find_sorted_kernel(
       }
     }
   }
+}
+
+__global__ __launch_bounds__(256) void find_sorted_kernel(
+    const double* pos, int n, Cell c, double rcut, const int* cell_of, const int* cell_start,
+    const float4* cpos, int* count, int maxnb, int* list, int* flag, double* pref,
+    const int* guard) {
+  GUARD_RETURN();
+  __shared__ NbWave L[4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = blockIdx.x * 4 + w;
+  if (i >= n) return;
+  nb_find_one(pos, i, cell_of[i], c, rcut, cell_start, cpos, count, maxnb, list, flag, pref,
+              L[w], lane);
+}
+
+// The whole build for small systems in one workgroup of 1024 threads:
+// binning, cell offsets and the cell-ordered copies in LDS and memory,
+// then the one-pass list of every atom, one wave per atom in turn.  The
+// same lists as the multi-kernel build; one launch instead of four.
+#define TGX_NB_SMALL_N 4096
+#define TGX_NB_SMALL_CELLS 4096
+__global__ __launch_bounds__(1024) void nb_build_small_kernel(
+    const double* pos, int n, Cell c, double rcut, int* cell_of, int* cell_atoms, float4* cpos,
+    int* count, int maxnb, int* list, int* flag, double* pref, const int* guard) {
+  GUARD_RETURN();
+  __shared__ int s_cnt[TGX_NB_SMALL_CELLS];
+  __shared__ int s_start[TGX_NB_SMALL_CELLS + 1];
+  __shared__ int s_part[1024];
+  __shared__ NbWave L[16];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int ncell = c.mx * c.my * c.mz;
+  for (int k = t; k < ncell; k += 1024) s_cnt[k] = 0;
+  __syncthreads();
+  for (int i = t; i < n; i += 1024) {
+    int cx, cy, cz;
+    cell_coords(pos + 3 * i, c, &cx, &cy, &cz);
+    const int cell = imodd(cx, c.mx) + imodd(cy, c.my) * c.mx + imodd(cz, c.mz) * c.mx * c.my;
+    cell_of[i] = cell;
+    atomicAdd(s_cnt + cell, 1);
+  }
+  __syncthreads();
+  // exclusive scan: chunks of per cells per thread, then the chunk sums
+  const int per = (ncell + 1023) / 1024, b0 = t * per, b1 = min(ncell, b0 + per);
+  int sum = 0;
+  for (int k = b0; k < b1; k++) sum += s_cnt[k];
+  s_part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = t >= off ? s_part[t - off] : 0;
+    __syncthreads();
+    s_part[t] += v;
+    __syncthreads();
+  }
+  int run = s_part[t] - sum;
+  for (int k = b0; k < b1; k++) {
+    s_start[k] = run;
+    run += s_cnt[k];
+    s_cnt[k] = 0;  // becomes the fill counter
+  }
+  if (t == 1023) s_start[ncell] = s_part[1023];
+  __syncthreads();
+  for (int i = t; i < n; i += 1024) {
+    const int cell = cell_of[i];
+    const int slot = s_start[cell] + atomicAdd(s_cnt + cell, 1);
+    cell_atoms[slot] = i;
+    cpos[slot] = wrapped_pos4(pos, i, c);
+  }
+  __syncthreads();  // also makes cell_of and cpos visible across the workgroup
+  for (int i = w; i < n; i += 16)
+    nb_find_one(pos, i, cell_of[i], c, rcut, s_start, cpos, count, maxnb, list, flag, pref, L[w],
+                lane);
 }
 
 // Sort each atom's list ascending (== ascending reference supercell id):
@@ -518,6 +586,14 @@ hipError_t launch_neighbors(const double* pos, int n, const Cell& c, double rcut
   const bool fused = !c.small && c.mx >= 3 && c.my >= 3 && c.mz >= 3;
   // wrapped FP32 positions in cell order, after the ids (cell_atoms_bytes)
   float4* cpos = reinterpret_cast<float4*>(cell_atoms + ((n + 3) & ~3));
+  if (fused && n <= TGX_NB_SMALL_N && c.mx * c.my * c.mz <= TGX_NB_SMALL_CELLS) {
+    hipLaunchKernelGGL(nb_build_small_kernel, dim3(1), dim3(1024), 0, st, pos, n, c, rcut,
+                       cell_of, cell_atoms, cpos, count, maxnb, list, flag, pref, guard);
+    if (rev)
+      hipLaunchKernelGGL(reverse_kernel<CellOne>, dim3((n + 3) / 4), dim3(256), 0, st, n, maxnb,
+                         CellOne{c}, count, list, rev, flag, guard);
+    return hipGetLastError();
+  }
   if (!c.small) {
     int ncell = c.mx * c.my * c.mz;
     // a guarded (captured) build relies on the previous build of the same

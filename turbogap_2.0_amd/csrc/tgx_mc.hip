@@ -86,38 +86,62 @@ __global__ void mc_affected_kernel(const int* ctl, const double* dctl, const dou
   __shared__ int keep[1024];
   __shared__ int nk;
   const int t = threadIdx.x;
-  if (t == 0) {
-    // fixed order: the moved atom, then the list in slot order
+  if (t < 64) {
+    // fixed order: the moved atom, then the list in slot order (the first
+    // wave tests 64 slots at a time and compacts them with a ballot)
     const double po[3] = {dctl[MC_OLD], dctl[MC_OLD + 1], dctl[MC_OLD + 2]};
-    int k = 0;
-    if (central == 0 || species[m] == central) keep[k++] = m;
-    for (int s = 0; s < cnt; s++) {
-      const int j = list[(size_t)m * maxnb + s] & TGX_ATOM_MASK;
-      if (central != 0 && species[j] != central) continue;
-      double dv[3];
-      const double rn = mic(pos + 3 * m, pos + 3 * j, cell.La, cell.Lb, cell.Lc, dv);
-      const double ro = mic(po, pos + 3 * j, cell.La, cell.Lb, cell.Lc, dv);
-      if (rn < rsoap || ro < rsoap) keep[k++] = j;
+    int k = (central == 0 || species[m] == central) ? 1 : 0;
+    if (t == 0 && k) keep[0] = m;
+    for (int base = 0; base < cnt; base += 64) {
+      const int s = base + t;
+      bool ok = false;
+      int j = 0;
+      if (s < cnt) {
+        j = list[(size_t)m * maxnb + s] & TGX_ATOM_MASK;
+        if (central == 0 || species[j] == central) {
+          double dv[3];
+          const double rn = mic(pos + 3 * m, pos + 3 * j, cell.La, cell.Lb, cell.Lc, dv);
+          const double ro = mic(po, pos + 3 * j, cell.La, cell.Lb, cell.Lc, dv);
+          ok = rn < rsoap || ro < rsoap;
+        }
+      }
+      const unsigned long long b = __ballot(ok);
+      if (ok) keep[k + __popcll(b & ((1ull << t) - 1ull))] = j;
+      k += __popcll(b);
     }
-    nk = k;
-    *n_aff = k;
+    if (t == 0) {
+      nk = k;
+      *n_aff = k;
+    }
   }
   __syncthreads();
   for (int k = t; k < cap; k += blockDim.x) aff[k] = k < nk ? keep[k] : m;
 }
 
-// Exact FP64 pair energy of one distance_2b block (gap.f90:383-399).
-__device__ double mc_phi(const TwoBodyDev& b, double r) {
-  double phi, dphi;
-  two_body_pair<1>(b, r, &phi, &dphi);
-  return phi;
+// Exact FP64 pair energy of one distance_2b block (gap.f90:383-399),
+// restricted to the sparse distances k0 <= k < k1 (the cutoff factor
+// multiplies the whole sum, so partial sums add up to phi).
+__device__ double mc_phi_part(const TwoBodyDev& b, double r, int k0, int k1) {
+  const double pi = 3.14159265358979323846;
+  if (!(r < b.rcut)) return 0.0;
+  double fcut = 1.0;
+  if (!(r < b.rcut - b.buffer)) fcut = (cos(pi * (r - b.rcut + b.buffer) / b.buffer) + 1.0) / 2.0;
+  const double is2 = 1.0 / (b.sigma * b.sigma);
+  double es = 0;
+  for (int k = k0; k < k1; k++) {
+    const double dr = r - b.Qs[k];
+    es += b.alphas[k] * exp(-0.5 * dr * dr * is2);
+  }
+  return b.delta * b.delta * fcut * es;
 }
 
 // Exact 2-body energy change of moving atom m from dctl[MC_OLD] to pos[m]:
-// every pair (m, j) enters E_m and E_j, so dE = 2 sum_j dphi.  Wave sum.
+// every pair (m, j) enters E_m and E_j, so dE = 2 sum_j dphi.  The four
+// waves of the workgroup take a quarter of the sparse distances each, the
+// lanes the neighbour slots; fixed-order sum over lanes, then waves.
 __device__ double mc_de2b(const double* pos, const int* species, const Cell& cell, int maxnb,
                           const int* count, const int* list, const TwoBodyDev* twob, int n2b,
-                          int m, const double* dctl, int lane) {
+                          int m, const double* dctl, int lane, int w, double* red) {
   double de = 0.0;
   const int cnt = min(count[m], maxnb);
   const double* pm = pos + 3 * m;
@@ -126,6 +150,8 @@ __device__ double mc_de2b(const double* pos, const int* species, const Cell& cel
     const TwoBodyDev& B = twob[b];
     const int sm = species[m];
     if (sm != B.sp1 && sm != B.sp2) continue;
+    const int qn = (B.n_sparse + 3) / 4, k0 = min(B.n_sparse, w * qn),
+              k1 = min(B.n_sparse, k0 + qn);
     for (int s = lane; s < cnt; s += 64) {
       const int j = list[(size_t)m * maxnb + s] & TGX_ATOM_MASK;
       const int sj = species[j];
@@ -133,12 +159,14 @@ __device__ double mc_de2b(const double* pos, const int* species, const Cell& cel
       double dv[3];
       const double rn = mic(pm, pos + 3 * j, cell.La, cell.Lb, cell.Lc, dv);
       const double ro = mic(po, pos + 3 * j, cell.La, cell.Lb, cell.Lc, dv);
-      de += 2.0 * (mc_phi(B, rn) - mc_phi(B, ro));
+      de += 2.0 * (mc_phi_part(B, rn, k0, k1) - mc_phi_part(B, ro, k0, k1));
     }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) de += __shfl_xor(de, o, 64);
-  return de;
+  if (lane == 0) red[w] = de;
+  __syncthreads();
+  return ((red[0] + red[1]) + red[2]) + red[3];
 }
 
 __device__ void mc_finish_trial(int t, double* trace, int* ctl, const double* dctl) {
@@ -148,16 +176,19 @@ __device__ void mc_finish_trial(int t, double* trace, int* ctl, const double* dc
 
 // Stage 1: the exact 2-body change always; with delayed acceptance active the
 // surrogate change of the affected sites and the first accept/reject.
-__global__ __launch_bounds__(64) void mc_stage1_kernel(
+__global__ __launch_bounds__(256) void mc_stage1_kernel(
     double* pos, const int* species, Cell cell, int maxnb, const int* count, const int* list,
     const int* aff, const int* n_aff, const double* esur, const double* esur_cur,
     const TwoBodyDev* twob, int n2b, unsigned long long seed, double kT, int* ctl, double* dctl,
     double* trace) {
   if (!ctl[MC_LIVE]) return;
-  const int lane = threadIdx.x;
+  __shared__ double red[4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int t = ctl[MC_STEP];
   const int m = ctl[MC_ATOM];
-  const double de2 = mc_de2b(pos, species, cell, maxnb, count, list, twob, n2b, m, dctl, lane);
+  const double de2 =
+      mc_de2b(pos, species, cell, maxnb, count, list, twob, n2b, m, dctl, lane, w, red);
+  if (w != 0) return;
   const bool da = ctl[MC_DA] && !ctl[MC_FALLBACK];
   double ds = 0.0;
   if (da) {
@@ -283,7 +314,7 @@ hipError_t launch_mc_stage1(double* pos, const int* species, const Cell& cell, i
                             const double* esur, const double* esur_cur, const TwoBodyDev* twob,
                             int n2b, unsigned long long seed, double kT, int* ctl, double* dctl,
                             double* trace, hipStream_t st) {
-  hipLaunchKernelGGL(mc_stage1_kernel, dim3(1), dim3(64), 0, st, pos, species, cell, maxnb, count,
+  hipLaunchKernelGGL(mc_stage1_kernel, dim3(1), dim3(256), 0, st, pos, species, cell, maxnb, count,
                      list, aff, n_aff, esur, esur_cur, twob, n2b, seed, kT, ctl, dctl, trace);
   return hipGetLastError();
 }
