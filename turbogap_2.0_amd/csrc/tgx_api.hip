@@ -94,12 +94,13 @@ hipError_t launch_energy_init(int n, const int* species, const double* e0,
                               const int* site_of_atom, const double* esite, double* eatom,
                               double* forces, double* vatom, hipStream_t st);
 hipError_t launch_reduce(int n_sites, const double* vsite, int n, const double* vatom,
-                         const double* eatom, double* out, hipStream_t st);
+                         const double* eatom, double* out, hipStream_t st,
+                         bool with_final = true);
 
 hipError_t launch_md_vv(int n, double* pos, double* vel, const double* f, double* fprev,
                         const double* mass, double dt, const int* step, double* ke,
                         const double* pref, double lim2, int* flag, hipStream_t st);
-hipError_t launch_md_finish(int n, const double* ke, const double* out, int* flag, int* nreb,
+hipError_t launch_md_finish(int n, const double* ke, double* out, int* flag, int* nreb,
                             int* step, double* ep_hist, double* ek_hist, int cap,
                             hipStream_t st);
 hipError_t launch_md_sync_vel(int n, const double* vel, const double* f, const double* fprev,
@@ -344,7 +345,7 @@ static void kev_rec(tgx_ctx* c, int k) {
 }
 
 static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_forces,
-                     bool events, int n_struct = 0);
+                     bool events, int n_struct = 0, bool defer_final = false);
 
 // Energies/forces/virial of the structure whose positions and species are
 // already in c->pos / c->species.  Results stay on the device:
@@ -389,7 +390,7 @@ static bool fuse_two_body(const tgx_ctx* c, int n, int n_sites, int do_forces) {
 // no host synchronisation, no allocation once the buffers are sized (the MD
 // step graph captures it).
 static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_forces,
-                     bool events, int n_struct) {
+                     bool events, int n_struct, bool defer_final) {
   Extra& x = X(c);
   int rcode;
   x.ev_on = events;
@@ -480,7 +481,7 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
     TGX_HIP(c, launch_reduce((do_forces && c->has_soap) ? n_sites : 0,
                              (do_forces && c->has_soap) ? c->vsite.as<double>() : nullptr, n,
                              do_forces ? c->virial.as<double>() : nullptr, c->eatom.as<double>(),
-                             x.out.as<double>(), c->stream));
+                             x.out.as<double>(), c->stream, !defer_final));
   }
   ev_rec(c, 6);
   x.ev_on = true;
@@ -1799,7 +1800,8 @@ static int md_step_launch(tgx_ctx* c) {
                               c->nb_count.as<int>(), c->maxnb, c->nb_list.as<int>(),
                               c->nb_rev.as<int>(), c->nb_flag.as<int>(), c->stream, ctl + 0,
                               m.pref.as<double>()));
-  int rcode = ef_launch(c, n, m.n_sites, Geom{m.cell}, 1, false);
+  // the final energy/virial sum is done by the finish kernel
+  int rcode = ef_launch(c, n, m.n_sites, Geom{m.cell}, 1, false, 0, true);
   if (rcode) return rcode;
   TGX_HIP(c, launch_md_finish(n, m.ke.as<double>(), x.out.as<double>(), ctl + 0, ctl + 1, ctl + 2,
                               m.ep.as<double>(), m.ek.as<double>(), m.cap, c->stream));

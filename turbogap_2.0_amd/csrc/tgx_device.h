@@ -85,6 +85,34 @@ struct CellMany {
   __device__ __forceinline__ Cell of(int i) const { return cells[sid[i]]; }
 };
 
+// Workgroups of the deterministic energy/virial reduction (each writes one
+// row of 8 partials).
+#define TGX_RB 128
+
+// Fixed-order sum of the TGX_RB partial rows into out[0..8] (virial) and
+// out[9] (energy) by the first TGX_RB threads of a workgroup; sh holds
+// TGX_RB x 7 doubles.  Every thread of the workgroup must call it.
+__device__ inline void reduce_final_rows(const double* part, double* out, double (*sh)[7]) {
+  const int t = threadIdx.x;
+  if (t < TGX_RB)
+    for (int k = 0; k < 7; k++) sh[t][k] = part[t * 8 + k];
+  __syncthreads();
+  for (int off = TGX_RB / 2; off > 0; off >>= 1) {
+    if (t < off)
+      for (int k = 0; k < 7; k++) sh[t][k] += sh[t + off][k];
+    __syncthreads();
+  }
+  if (t == 0) {
+    out[0] = sh[0][0];
+    out[4] = sh[0][1];
+    out[8] = sh[0][2];
+    out[1] = out[3] = sh[0][3];
+    out[2] = out[6] = sh[0][4];
+    out[5] = out[7] = sh[0][5];
+    out[9] = sh[0][6];
+  }
+}
+
 // LDS written by other lanes of this wave is visible after this point.
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

@@ -931,7 +931,6 @@ hipError_t launch_energy_init(int n, const int* species, const double* e0,
 // Deterministic reduction of the per-site and per-atom 6-component virials
 // and of the atom energies: a fixed partition over RB workgroups (each a
 // fixed tree), then one workgroup sums the RB partials in order.
-#define TGX_RB 128
 __global__ void reduce_part_kernel(int n_sites, const double* vsite, int n, const double* vatom,
                                    const double* eatom, double* part /* [RB][8] */) {
   __shared__ double sh[256][7];
@@ -962,31 +961,18 @@ __global__ void reduce_part_kernel(int n_sites, const double* vsite, int n, cons
 
 __global__ void reduce_final_kernel(const double* part, double* out /* 9 virial + energy */) {
   __shared__ double sh[TGX_RB][7];
-  const int t = threadIdx.x;
-  for (int k = 0; k < 7; k++) sh[t][k] = part[t * 8 + k];
-  __syncthreads();
-  for (int off = TGX_RB / 2; off > 0; off >>= 1) {
-    if (t < off)
-      for (int k = 0; k < 7; k++) sh[t][k] += sh[t + off][k];
-    __syncthreads();
-  }
-  if (t == 0) {
-    out[0] = sh[0][0];
-    out[4] = sh[0][1];
-    out[8] = sh[0][2];
-    out[1] = out[3] = sh[0][3];
-    out[2] = out[6] = sh[0][4];
-    out[5] = out[7] = sh[0][5];
-    out[9] = sh[0][6];
-  }
+  reduce_final_rows(part, out, sh);
 }
 
 // out must hold 10 + 8*TGX_RB doubles (the partials live after the result).
+// with_final = false leaves the last step to the caller's next kernel
+// (reduce_final_rows on out + 16).
 hipError_t launch_reduce(int n_sites, const double* vsite, int n, const double* vatom,
-                         const double* eatom, double* out, hipStream_t st) {
+                         const double* eatom, double* out, hipStream_t st, bool with_final) {
   hipLaunchKernelGGL(reduce_part_kernel, dim3(TGX_RB), dim3(256), 0, st, n_sites, vsite, n, vatom,
                      eatom, out + 16);
-  hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(TGX_RB), 0, st, out + 16, out);
+  if (with_final)
+    hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(TGX_RB), 0, st, out + 16, out);
   return hipGetLastError();
 }
 

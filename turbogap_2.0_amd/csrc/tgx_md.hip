@@ -5,6 +5,7 @@
 // whole step can be captured once in a hipGraph and replayed.
 #include <hip/hip_runtime.h>
 
+#include "tgx_device.h"
 #include "tgx_internal.h"
 
 namespace tgx {
@@ -51,13 +52,17 @@ __global__ __launch_bounds__(256) void md_vv_kernel(int n, double* pos, double* 
   if (threadIdx.x == 0) ke_part[blockIdx.x] = sh[0];
 }
 
-// End of step: kinetic energy of v(t) (fixed-order tree), potential energy
-// of x(t+dt), rebuild bookkeeping, step counter.
-__global__ void md_finish_kernel(int n, const double* ke, const double* out,
+// End of step: the final energy/virial sum, kinetic energy of v(t)
+// (fixed-order tree), potential energy of x(t+dt), rebuild bookkeeping,
+// step counter.
+__global__ void md_finish_kernel(int n, const double* ke, double* out,
                                  int* flag, int* nreb, int* step,
                                  double* ep_hist, double* ek_hist, int cap) {
   __shared__ double sh[256];
+  __shared__ double shr[TGX_RB][7];
   const int t = threadIdx.x;
+  // the energy/virial partials of this step's reduction (out + 16)
+  reduce_final_rows(out + 16, out, shr);
   double a = 0.0;
   const int nb = (n + 255) / 256;  // per-block partials written by md_vv_kernel
   for (int i = t; i < nb; i += 256) a += ke[i];
@@ -123,7 +128,7 @@ hipError_t launch_md_vv(int n, double* pos, double* vel, const double* f, double
   return hipGetLastError();
 }
 
-hipError_t launch_md_finish(int n, const double* ke, const double* out, int* flag, int* nreb,
+hipError_t launch_md_finish(int n, const double* ke, double* out, int* flag, int* nreb,
                             int* step, double* ep_hist, double* ek_hist, int cap,
                             hipStream_t st) {
   hipLaunchKernelGGL(md_finish_kernel, dim3(1), dim3(256), 0, st, n, ke, out, flag, nreb, step,
