@@ -31,6 +31,13 @@ def test_fortran_struct_layout_matches_c(tmp_path):
     assert int(out["size"]) == C.sizeof(tgx.CSoapHypers)
     for f in ("n_species", "l_max", "alpha_max", "rcut_hard", "rcut_soft", "central_weight", "nf"):
         assert int(out[f]) == getattr(tgx.CSoapHypers, f).offset, f
+    assert int(out["da.size"]) == C.sizeof(tgx.DaOptions)
+    for f, _ in tgx.DaOptions._fields_:
+        assert int(out["da." + f]) == getattr(tgx.DaOptions, f).offset, f
+    assert int(out["st.size"]) == C.sizeof(tgx.McStats)
+    for f, _ in tgx.McStats._fields_:
+        if f != "pad_":
+            assert int(out["st." + f]) == getattr(tgx.McStats, f).offset, f
 
 
 def write_model_files(d, model, species=("C",)):
@@ -87,3 +94,43 @@ def test_fortran_predict_driver(tmp_path):
     assert np.abs(rows[:, 3] - e0).max() < 1e-8
     assert np.abs(rows[:, 4:7] - f0).max() < 1e-6
     assert np.abs(vir - v0).max() < 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["plain", "da"])
+def test_fortran_mc_driver(tmp_path, kind):
+    """The Fortran Monte Carlo driver (reference model files, xyz input):
+    plain FP64 Metropolis equals the oracle's chain move for move; delayed
+    acceptance with a surrogate trained on the first 256 trials reports its
+    stages consistently (every trial is either rejected by the surrogate or
+    evaluated with the GAP) and the same final energy as a fresh evaluation."""
+    exe = os.path.join(FDIR, "tgx_mc")
+    if not os.path.exists(exe):
+        pytest.fail("tgx_mc is not built (run __graft_entry__.build())")
+    pos, sp, lat, model = W.carbon216()
+    write_model_files(str(tmp_path), model)
+    write_xyz(str(tmp_path / "s.xyz"), pos, sp, lat)
+    n_trials, T, mm, skin, seed = 400, 1500.0, 0.6, 1.2, 11
+    args = [exe, "model.gap", "s.xyz", str(n_trials), str(T), str(mm), str(skin), str(seed), kind]
+    if kind == "da":
+        args += ["256"]
+    out = subprocess.run(args, capture_output=True, text=True, cwd=tmp_path, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    kv = {}
+    fpos = np.zeros_like(pos)
+    for l in out.stdout.splitlines():
+        w = l.split()
+        if w[0] == "pos":
+            fpos[int(w[1]) - 1] = [float(x) for x in w[2:5]]
+        else:
+            kv[w[0]] = float(w[1])
+    e_new, _, _ = oracle.energy_forces(model, fpos, sp, lat)
+    assert abs(kv["e_final"] - e_new.sum()) < 1e-8 * len(pos)
+    if kind == "plain":
+        p_o, tr_o, acc_o = oracle.mc_run(model, pos, sp, lat, n_trials, T, mm, seed)
+        assert int(kv["accepted"]) == acc_o
+        assert np.array_equal(fpos, p_o)
+    else:
+        assert int(kv["stage1_rejected"]) + int(kv["gap_evaluations"]) == n_trials
+        assert int(kv["fallback"]) in (0, 1)
+        assert 0 < kv["accepted"] <= kv["gap_evaluations"]

@@ -39,12 +39,36 @@ module tgx_mod
      real(c_double) :: nf(TGX_MAX_SPECIES) = 4.0d0
   end type tgx_soap_hypers
 
+  ! Delayed-acceptance options and Monte Carlo statistics (tgx_da_options,
+  ! tgx_mc_stats in tgx.h).
+  type, bind(C), public :: tgx_da_options
+     integer(c_int32_t) :: n_train_trials = 20000
+     integer(c_int32_t) :: train_every = 2048
+     integer(c_int32_t) :: epochs = 3
+     integer(c_int32_t) :: batch = 512
+     real(c_double) :: lr = 2.0d-3
+     real(c_double) :: max_rmse = 0.2d0
+  end type tgx_da_options
+
+  type, bind(C), public :: tgx_mc_stats
+     integer(c_int64_t) :: n_accepted = 0
+     integer(c_int64_t) :: n_stage1_rejected = 0
+     integer(c_int64_t) :: n_gap_evaluations = 0
+     integer(c_int64_t) :: n_rebuilds = 0
+     real(c_double) :: train_rmse = 0
+     real(c_double) :: da_rmse = 0
+     integer(c_int32_t) :: fallback = 0
+     integer(c_int32_t) :: pad_ = 0
+  end type tgx_mc_stats
+
   public :: tgx_create, tgx_destroy, tgx_last_error, tgx_set_precision, tgx_set_soap_model, &
        tgx_set_e0, tgx_add_2b_model, tgx_clear_2b_models, tgx_neighbors_build, &
        tgx_soap_descriptors, tgx_gap_kernel_step, tgx_energy_forces, &
        tgx_energy_forces_batched, tgx_md_run, tgx_md_begin, tgx_md_steps, tgx_md_read, &
        tgx_md_end, tgx_sync, tgx_poly3_orthonormalization, tgx_stage_times, tgx_check, &
-       tgx_error_string, tgx_kernel_times, tgx_mc_run, tgx_set_list_buffer
+       tgx_error_string, tgx_kernel_times, tgx_mc_run, tgx_set_list_buffer, tgx_mc_run_da, &
+       tgx_surrogate_init, tgx_surrogate_set, tgx_surrogate_get, tgx_surrogate_predict, &
+       tgx_surrogate_gradient, tgx_surrogate_fit
 
   interface
      integer(c_int) function tgx_create(device, precision, ctx) bind(C, name="tgx_create")
@@ -239,6 +263,77 @@ module tgx_mod
        real(c_double) :: e_trace(*)
        integer(c_int) :: n_accepted, n_rebuilds
      end function tgx_mc_run
+
+     ! Delayed-acceptance Metropolis with the on-device surrogate (no
+     ! reference counterpart); pos(3,n) updated in place.
+     integer(c_int) function tgx_mc_run_da(ctx, n, species, lattice, pos, n_trials, temp, &
+          move_max, skin, seed, opt, e_trace, stats) bind(C, name="tgx_mc_run_da")
+       import :: c_int, c_ptr, c_double, c_int64_t, tgx_da_options, tgx_mc_stats
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: n, n_trials
+       integer(c_int), intent(in) :: species(*)
+       real(c_double), intent(in) :: lattice(9)
+       real(c_double), intent(inout) :: pos(*)
+       real(c_double), value :: temp, move_max, skin
+       integer(c_int64_t), value :: seed
+       type(tgx_da_options), intent(in) :: opt
+       real(c_double) :: e_trace(*)
+       type(tgx_mc_stats) :: stats
+     end function tgx_mc_run_da
+
+     ! The per-site surrogate network (no reference counterpart).
+     integer(c_int) function tgx_surrogate_init(ctx, dim, seed) bind(C, name="tgx_surrogate_init")
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: dim
+       integer(c_int64_t), value :: seed
+     end function tgx_surrogate_init
+
+     integer(c_int) function tgx_surrogate_set(ctx, n_params, P, mu, sigma) &
+          bind(C, name="tgx_surrogate_set")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: n_params
+       real(c_double), intent(in) :: P(*)
+       real(c_double), value :: mu, sigma
+     end function tgx_surrogate_set
+
+     integer(c_int) function tgx_surrogate_get(ctx, n_params, P, mu, sigma) &
+          bind(C, name="tgx_surrogate_get")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: n_params
+       real(c_double) :: P(*), mu, sigma
+     end function tgx_surrogate_get
+
+     integer(c_int) function tgx_surrogate_predict(ctx, n, q, e) &
+          bind(C, name="tgx_surrogate_predict")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: n
+       real(c_double), intent(in) :: q(*)
+       real(c_double) :: e(*)
+     end function tgx_surrogate_predict
+
+     integer(c_int) function tgx_surrogate_gradient(ctx, n, q, y, grad, loss) &
+          bind(C, name="tgx_surrogate_gradient")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: n
+       real(c_double), intent(in) :: q(*), y(*)
+       real(c_double) :: grad(*), loss
+     end function tgx_surrogate_gradient
+
+     integer(c_int) function tgx_surrogate_fit(ctx, n, q, y, epochs, batch, lr, seed, rmse) &
+          bind(C, name="tgx_surrogate_fit")
+       import :: c_int, c_ptr, c_double, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: n, epochs, batch
+       real(c_double), intent(in) :: q(*), y(*)
+       real(c_double), value :: lr
+       integer(c_int64_t), value :: seed
+       real(c_double) :: rmse
+     end function tgx_surrogate_fit
 
      integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
        import :: c_size_t, c_ptr
