@@ -72,6 +72,7 @@ hipError_t launch_mc_collect(const int* ctl, const int* n_aff, const T* q, int q
                              const double* esite, float* sq, double* sy, int cap_s, int* cursor,
                              hipStream_t st);
 size_t mlp_nparams(int D);
+size_t mlp_w1_elems(int D);
 int mlp_hidden();
 int mlp_max_dim();
 template <typename T>
@@ -1643,9 +1644,11 @@ int tgx_surrogate_init(tgx_ctx* c, int dim, uint64_t seed) {
   std::vector<float> P;
   sg_init_params(dim, H, seed, &P);
   if (x.sg_P.ensure(4 * NP) || x.sg_m.ensure(4 * NP) || x.sg_v.ensure(4 * NP) ||
-      x.sg_grad.ensure(4 * NP) || x.sg_W1t.ensure(2 * (size_t)dim * H) ||
+      x.sg_grad.ensure(4 * NP) || x.sg_W1t.ensure(2 * mlp_w1_elems(dim)) ||
       x.sg_W2t.ensure(2 * (size_t)H * H))
     return set_err(c, TGX_ERR_HIP, "allocation failed (surrogate)");
+  // the zero padding of the unit-major W1 copy
+  TGX_HIP(c, hipMemsetAsync(x.sg_W1t.p, 0, 2 * mlp_w1_elems(dim), c->stream));
   TGX_HIP(c, hipMemcpyAsync(x.sg_P.p, P.data(), 4 * NP, hipMemcpyHostToDevice, c->stream));
   TGX_HIP(c, hipMemsetAsync(x.sg_m.p, 0, 4 * NP, c->stream));
   TGX_HIP(c, hipMemsetAsync(x.sg_v.p, 0, 4 * NP, c->stream));

@@ -12,7 +12,9 @@
 // Parameter vector P (FP32 master), layout:
 //   W1[H][D], b1[H], W2[H][H], b2[H], w3[H], b3
 // BF16 copies: W1t[D][H] and W2t[H][H] (input-major, so that the threads of
-// a workgroup, one per hidden unit, read consecutive words).
+// a workgroup, one per hidden unit, read consecutive words), and after W1t
+// in the same buffer W1b[H][Dp], Dp = D rounded up to 32, zero-padded
+// (unit-major: 16-byte k-fragments for the MFMA inference kernel).
 #include <hip/hip_runtime.h>
 
 #include "tgx_internal.h"
@@ -36,6 +38,8 @@ __device__ __forceinline__ float sigm(float x) { return 1.f / (1.f + __expf(-x))
 constexpr int MLP_H = 128;  // hidden width
 constexpr int MLP_SB = 16;  // sites per workgroup
 constexpr int MLP_DMAX = 448;  // static LDS budget of the gradient kernel
+
+__host__ __device__ inline int mlp_dpad(int D) { return (D + 31) / 32 * 32; }
 
 // Forward for n sites; rows of q (T) with leading dimension q_ld, optional
 // site row list rows[] (else row = site).  Writes e[n]; with z1/z2 != null
@@ -279,6 +283,7 @@ __global__ void mlp_adam_kernel(int D, size_t NP, float* P, const float* grad, f
   if (k < (size_t)H * D) {
     const int u = (int)(k / D), d = (int)(k - (size_t)u * D);
     W1t[(size_t)d * H + u] = f2bf16(p);
+    W1t[(size_t)D * H + (size_t)u * mlp_dpad(D) + d] = f2bf16(p);  // W1b
   } else {
     const size_t o = k - (size_t)H * D - H;
     if (k >= (size_t)H * D + H && o < (size_t)H * H) {
@@ -295,6 +300,7 @@ __global__ void mlp_cast_kernel(int D, const float* P, u16* W1t, u16* W2t) {
   if (k < (size_t)H * D) {
     const int u = (int)(k / D), d = (int)(k - (size_t)u * D);
     W1t[(size_t)d * H + u] = f2bf16(P[k]);
+    W1t[(size_t)D * H + (size_t)u * mlp_dpad(D) + d] = f2bf16(P[k]);  // W1b
   }
   if (k < (size_t)H * H) {
     const int u = (int)(k / H), d = (int)(k - (size_t)u * H);
@@ -302,7 +308,106 @@ __global__ void mlp_cast_kernel(int D, const float* P, u16* W1t, u16* W2t) {
   }
 }
 
+// Inference on MFMA for 16 sites per workgroup (four waves, 32 hidden
+// units each): layer 1 as BF16 products (q rounded to BF16 on load, W1b)
+// on mfma_f32_16x16x32_bf16, layer 2 with the FP32 activations and the BF16
+// W2 values on mfma_f32_16x16x4f32, so the products are those of the VALU
+// kernel above (the sums run in another order).  No pre-activations are
+// kept (the training path uses the VALU kernel).
+typedef float f32x4m __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8m __attribute__((ext_vector_type(8)));
+
+template <typename T>
+__global__ __launch_bounds__(256) void mlp_fwd_mfma_kernel(
+    int n, int D, const T* __restrict__ q, int q_ld, const int* __restrict__ rows,
+    const float* __restrict__ P, const u16* __restrict__ W1t, const u16* __restrict__ W2t,
+    const float* __restrict__ norm, double* __restrict__ e, const int* __restrict__ guard) {
+  if (guard && *guard == 0) return;
+  __shared__ __align__(16) u16 sq[MLP_SB][MLP_DMAX + 8];
+  __shared__ float sh1[MLP_SB][MLP_H + 1];
+  __shared__ float se[4][MLP_SB];
+  const int H = MLP_H, Dp = mlp_dpad(D);
+  const u16* W1b = W1t + (size_t)D * H;
+  const float mu = norm[0], sigma = norm[1];
+  const float* b1 = P + (size_t)H * D;
+  const float* b2 = b1 + H + (size_t)H * H;
+  const float* w3 = b2 + H;
+  const float b3 = w3[H];
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  const int lr = lane & 15, lg = lane >> 4;
+  const int s0 = blockIdx.x * MLP_SB;
+  for (int k = t; k < MLP_SB * Dp; k += 256) {
+    const int s = k / Dp, d = k - s * Dp;
+    const int site = s0 + s;
+    float v = 0.f;
+    if (site < n && d < D) {
+      const int r = rows ? rows[site] : site;
+      v = (float)q[(size_t)r * q_ld + d];
+    }
+    sq[s][d] = f2bf16(v);
+  }
+  __syncthreads();
+  // layer 1: D[site][unit] over units 32 wave .. +31 (two 16-wide tiles)
+  f32x4m a1[2] = {f32x4m{0, 0, 0, 0}, f32x4m{0, 0, 0, 0}};
+  for (int k0 = 0; k0 < Dp; k0 += 32) {
+    const bf16x8m af = *reinterpret_cast<const bf16x8m*>(&sq[lr][k0 + lg * 8]);
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int u = wave * 32 + j * 16 + lr;
+      const bf16x8m bw = *reinterpret_cast<const bf16x8m*>(W1b + (size_t)u * Dp + k0 + lg * 8);
+      a1[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bw, a1[j], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    const int u = wave * 32 + j * 16 + lr;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const float z = a1[j][v] + b1[u];
+      sh1[lg * 4 + v][u] = z * sigm(z);
+    }
+  }
+  __syncthreads();
+  // layer 2 (FP32 operands): D[site][unit], k over the 128 inputs
+  f32x4m a2[2] = {f32x4m{0, 0, 0, 0}, f32x4m{0, 0, 0, 0}};
+  for (int k0 = 0; k0 < H; k0 += 4) {
+    const float ah = sh1[lr][k0 + lg];
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int u = wave * 32 + j * 16 + lr;
+      const float bw = bf2f(W2t[(size_t)(k0 + lg) * H + u]);
+      a2[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bw, a2[j], 0, 0, 0);
+    }
+  }
+  // output: w3 . s(z2) over this wave's units, then over the four waves
+  float part[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    const int u = wave * 32 + j * 16 + lr;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const float z = a2[j][v] + b2[u];
+      part[v] += w3[u] * (z * sigm(z));
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    float x = part[v];
+    x += __shfl_xor(x, 1, 64);
+    x += __shfl_xor(x, 2, 64);
+    x += __shfl_xor(x, 4, 64);
+    x += __shfl_xor(x, 8, 64);
+    if (lr == 0) se[wave][lg * 4 + v] = x;
+  }
+  __syncthreads();
+  if (t < MLP_SB && s0 + t < n) {
+    const float a = ((se[0][t] + se[1][t]) + se[2][t]) + se[3][t];
+    e[s0 + t] = (double)mu + (double)sigma * (double)(a + b3);
+  }
+}
+
 size_t mlp_nparams(int D) { return (size_t)MLP_H * D + (size_t)MLP_H * MLP_H + 3 * MLP_H + 1; }
+size_t mlp_w1_elems(int D) { return (size_t)MLP_H * D + (size_t)MLP_H * mlp_dpad(D); }
 int mlp_hidden() { return MLP_H; }
 int mlp_sites_per_block() { return MLP_SB; }
 int mlp_max_dim() { return MLP_DMAX; }
@@ -314,8 +419,12 @@ hipError_t launch_mlp_forward(int n, int D, const T* q, int q_ld, const int* row
                               hipStream_t st) {
   if (n <= 0) return hipSuccess;
   if (D > MLP_DMAX) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((mlp_fwd_kernel<T>), dim3((n + MLP_SB - 1) / MLP_SB), dim3(256), 0, st, n,
-                     D, q, q_ld, rows, P, W1t, W2t, norm, e, z1, z2, guard);
+  if (!z1 && !z2)  // inference only: the MFMA kernel
+    hipLaunchKernelGGL((mlp_fwd_mfma_kernel<T>), dim3((n + MLP_SB - 1) / MLP_SB), dim3(256), 0, st,
+                       n, D, q, q_ld, rows, P, W1t, W2t, norm, e, guard);
+  else
+    hipLaunchKernelGGL((mlp_fwd_kernel<T>), dim3((n + MLP_SB - 1) / MLP_SB), dim3(256), 0, st, n,
+                       D, q, q_ld, rows, P, W1t, W2t, norm, e, z1, z2, guard);
   return hipGetLastError();
 }
 
