@@ -275,6 +275,9 @@ struct NbWave {
   int q[128];             // screened candidates awaiting the exact decision
 };
 
+__device__ void nb_finish(NbWave& L, int i, int found, int maxnb, int* count, int* flag,
+                          int* out, int lane);
+
 // One atom's list by one wave (all 64 lanes; LDS of this wave only).
 // own = the atom's linked cell as binned, cell_start the cell offsets.
 __device__ void nb_find_one(const double* pos, int i, int own, const Cell& c, double rcut,
@@ -349,6 +352,13 @@ __device__ void nb_find_one(const double* pos, int i, int own, const Cell& c, do
   }
   wave_sync();
   if (nq > 0) nb_decide(pos, pi, c, rcut, L.q, nq, lane, L.buf, out, maxnb, &found);
+  nb_finish(L, i, found, maxnb, count, flag, out, lane);
+}
+
+// The list's count, the overflow flag and the sorted entries (the first
+// TGX_NB_CAPL hits in L.buf, any further ones already in out).
+__device__ void nb_finish(NbWave& L, int i, int found, int maxnb, int* count, int* flag,
+                          int* out, int lane) {
   wave_sync();
   if (lane == 0) {
     count[i] = found;
@@ -393,6 +403,113 @@ __global__ __launch_bounds__(256) void find_sorted_kernel(
   if (i >= n) return;
   nb_find_one(pos, i, cell_of[i], c, rcut, cell_start, cpos, count, maxnb, list, flag, pref,
               L[w], lane);
+}
+
+// Batched structures, one wave per atom: the candidates are the atoms of
+// the (deduplicated) 27 neighbouring bins of the atom's structure, or for a
+// structure smaller than 2 rcut every atom of every supercell image; the
+// exact FP64 test (mic) decides, hits are compacted with a ballot and
+// sorted as in the single-structure build.
+__global__ __launch_bounds__(256) void find_sorted_many_kernel(
+    const double* pos, int n, const Cell* cells, const int* sid, const int* boff,
+    const int* aoff, double rcut, const int* cell_of, const int* cell_start,
+    const int* cell_atoms, int* count, int maxnb, int* list, int* flag) {
+  __shared__ NbWave Ls[4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = blockIdx.x * 4 + w;
+  if (i >= n) return;
+  NbWave& L = Ls[w];
+  const int s = sid[i];
+  const Cell c = cells[s];
+  const double* pi = pos + 3 * i;
+  int* out = list + (size_t)i * maxnb;
+  int found = 0;
+  auto append = [&](bool hit, int e) {
+    const unsigned long long m = __ballot(hit);
+    if (hit) {
+      const int slot = found + __popcll(m & ((1ull << lane) - 1ull));
+      if (slot < TGX_NB_CAPL) L.buf[slot] = e;
+      else if (slot < maxnb) out[slot] = e;
+    }
+    found += __popcll(m);
+  };
+  if (c.small) {
+    const int k0 = aoff[s], ns = aoff[s + 1] - k0, total = c.nrep * ns;
+    for (int idx0 = 0; idx0 < total; idx0 += 64) {
+      const int idx = idx0 + lane;
+      bool hit = false;
+      int e = 0;
+      if (idx < total) {
+        const int img = idx / ns, k = k0 + (idx - img * ns);
+        if (!(img == 0 && k == i)) {
+          double pk[3], dv[3];
+          image_pos(pos, k, img, c, pk);
+          hit = mic(pi, pk, c.La, c.Lb, c.Lc, dv) < rcut;
+          e = (img << TGX_IMG_SHIFT) | k;
+        }
+      }
+      append(hit, e);
+    }
+  } else {
+    const int own = cell_of[i] - boff[s];
+    const int cx = own % c.mx, cy = (own / c.mx) % c.my, cz = own / (c.mx * c.my);
+    int cell = -1, cnt_q = 0, base_q = 0;
+    if (lane < 27) {
+      const int q = lane;
+      const int i3 = cx - 1 + q % 3, j3 = cy - 1 + (q / 3) % 3, k3 = cz - 1 + q / 9;
+      cell = imodd(i3, c.mx) + imodd(j3, c.my) * c.mx + imodd(k3, c.mz) * c.mx * c.my;
+    }
+    // grids of 1 or 2 bins per direction repeat bins: keep the first
+    bool dup = false;
+    for (int q2 = 0; q2 < 26; q2++) {
+      const int other = __shfl(cell, q2, 64);
+      dup |= (lane < 27 && q2 < lane && other == cell);
+    }
+    if (lane < 27 && !dup) {
+      base_q = cell_start[boff[s] + cell];
+      cnt_q = cell_start[boff[s] + cell + 1] - base_q;
+    }
+    int incl = cnt_q;
+#pragma unroll
+    for (int d = 1; d < 32; d <<= 1) {
+      const int o = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += o;
+    }
+    const int total = __shfl(incl, 26, 64);
+    if (lane < 27) {
+      L.pre[lane] = incl - cnt_q;
+      L.base[lane] = base_q;
+    }
+    wave_sync();
+    const double screen2 = (rcut + 1e-6) * (rcut + 1e-6);
+    for (int idx0 = 0; idx0 < total; idx0 += 64) {
+      const int idx = idx0 + lane;
+      bool hit = false;
+      int k = 0;
+      if (idx < total) {
+        int lo = 0, hi = 26;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (L.pre[mid] <= idx) lo = mid;
+          else hi = mid - 1;
+        }
+        k = cell_atoms[L.base[lo] + idx - L.pre[lo]];
+        if (k != i) {
+          const double* pk = pos + 3 * k;
+          double dx = pk[0] - pi[0], dy = pk[1] - pi[1], dz = pk[2] - pi[2];
+          dx -= c.La * rint(dx * c.iLa);
+          dy -= c.Lb * rint(dy * c.iLb);
+          dz -= c.Lc * rint(dz * c.iLc);
+          if (dx * dx + dy * dy + dz * dz <= screen2) {
+            double dv[3];
+            hit = mic(pi, pk, c.La, c.Lb, c.Lc, dv) < rcut;
+          }
+        }
+      }
+      append(hit, k);
+    }
+  }
+  nb_finish(L, i, found, maxnb, count, flag, out, lane);
 }
 
 // The whole build for small systems in one workgroup of 1024 threads:
@@ -650,67 +767,6 @@ __global__ void bin_atoms_many_kernel(const double* pos, int n, const Cell* cell
   atomicAdd(cell_count + cell, 1);
 }
 
-__global__ void find_neighbors_many_kernel(const double* pos, int n, const Cell* cells,
-                                           const int* sid, const int* boff, const int* aoff,
-                                           double rcut, const int* cell_start,
-                                           const int* cell_atoms, int* count, int maxnb,
-                                           int* list, int* flag) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = t / TGX_NB_LANES, part = t % TGX_NB_LANES;
-  if (i >= n) return;
-  const int s = sid[i];
-  const Cell c = cells[s];
-  int* out = list + (size_t)i * maxnb;
-  const double* pi = pos + 3 * i;
-  const double screen2 = (rcut + 1e-6) * (rcut + 1e-6);
-  if (!c.small) {
-    int cx, cy, cz;
-    cell_coords(pi, c, &cx, &cy, &cz);
-    for (int q = part; q < 27; q += TGX_NB_LANES) {
-      const int i3 = cx - 1 + q % 3, j3 = cy - 1 + (q / 3) % 3, k3 = cz - 1 + q / 9;
-      const int cell = imodd(i3, c.mx) + imodd(j3, c.my) * c.mx + imodd(k3, c.mz) * c.mx * c.my;
-      bool dup = false;
-      for (int q2 = 0; q2 < q; q2++) {
-        const int a3 = cx - 1 + q2 % 3, b3 = cy - 1 + (q2 / 3) % 3, c3 = cz - 1 + q2 / 9;
-        dup |= (imodd(a3, c.mx) + imodd(b3, c.my) * c.mx + imodd(c3, c.mz) * c.mx * c.my) == cell;
-      }
-      if (dup) continue;
-      const int gc = boff[s] + cell;
-      for (int a = cell_start[gc]; a < cell_start[gc + 1]; a++) {
-        const int k = cell_atoms[a];
-        if (k == i) continue;
-        const double* pk = pos + 3 * k;
-        double dx = pk[0] - pi[0], dy = pk[1] - pi[1], dz = pk[2] - pi[2];
-        dx -= c.La * rint(dx * c.iLa);
-        dy -= c.Lb * rint(dy * c.iLb);
-        dz -= c.Lc * rint(dz * c.iLc);
-        if (dx * dx + dy * dy + dz * dz > screen2) continue;
-        double dv[3];
-        const double d = mic(pi, pk, c.La, c.Lb, c.Lc, dv);
-        if (d < rcut) {
-          const int slot = atomicAdd(count + i, 1);
-          if (slot < maxnb) out[slot] = k;
-          else atomicMax(flag, slot + 1);
-        }
-      }
-    }
-  } else {
-    const int k0 = aoff[s], k1 = aoff[s + 1];
-    for (int img = 0; img < c.nrep; img++)
-      for (int k = k0 + part; k < k1; k += TGX_NB_LANES) {
-        if (img == 0 && k == i) continue;
-        double pk[3], dv[3];
-        image_pos(pos, k, img, c, pk);
-        const double d = mic(pi, pk, c.La, c.Lb, c.Lc, dv);
-        if (d < rcut) {
-          const int slot = atomicAdd(count + i, 1);
-          if (slot < maxnb) out[slot] = (img << TGX_IMG_SHIFT) | k;
-          else atomicMax(flag, slot + 1);
-        }
-      }
-  }
-}
-
 hipError_t launch_neighbors_many(const double* pos, int n, const Cell* cells, const int* sid,
                                  const int* boff, const int* aoff, int n_bins, double rcut,
                                  int* cell_of, int* cell_count, int* cell_start, int* cell_fill,
@@ -726,12 +782,9 @@ hipError_t launch_neighbors_many(const double* pos, int n, const Cell* cells, co
                      cell_start, cell_fill, nullptr);
   hipLaunchKernelGGL(scatter_cells_kernel, dim3(nb), dim3(B), 0, st, n, cell_of, cell_start,
                      cell_fill, cell_atoms, cell_count, pos, Cell{}, nullptr, nullptr);
-  hipLaunchKernelGGL(zero_ints_kernel, dim3(nb), dim3(B), 0, st, count, n, nullptr);
-  hipLaunchKernelGGL(find_neighbors_many_kernel, dim3((n * TGX_NB_LANES + B - 1) / B), dim3(B),
-                     0, st, pos, n, cells, sid, boff, aoff, rcut, cell_start, cell_atoms, count,
-                     maxnb, list, flag);
-  hipLaunchKernelGGL(sort_lists_kernel, dim3((n + 3) / 4), dim3(256), 0, st, n, maxnb, count,
-                     list, nullptr);
+  hipLaunchKernelGGL(find_sorted_many_kernel, dim3((n + 3) / 4), dim3(256), 0, st, pos, n, cells,
+                     sid, boff, aoff, rcut, cell_of, cell_start, cell_atoms, count, maxnb, list,
+                     flag);
   if (rev)
     hipLaunchKernelGGL(reverse_kernel<CellMany>, dim3((n + 3) / 4), dim3(256), 0, st, n, maxnb,
                        CellMany{cells, sid}, count, list, rev, flag, nullptr);
