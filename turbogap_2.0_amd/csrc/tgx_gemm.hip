@@ -508,6 +508,225 @@ __global__ __launch_bounds__(512) void gap_fused_kernel(
   }
 }
 
+// The same for more than 512 sparse points: the sparse set is processed in
+// chunks of 256 (stage 1, epilogue and stage 2 per chunk), G accumulating in
+// registers across the chunks and the energies in the row sums.  The q rows
+// are staged again for every chunk (from L2).
+#define TGX_FUSED_SC 256
+#define TGX_FUSED_PFC 3  // stage-1 16-byte chunks per thread: (128 + 256) * 4 / 512
+template <int NT2, bool GRAD>
+__global__ __launch_bounds__(512) void gap_fused_chunked_kernel(
+    int M, int NS, int K1, const u16* __restrict__ q, int ldq, const u16* __restrict__ qs,
+    int ldqs, const u16* __restrict__ qsT, int ldqt, EpiKernel ek, double d2,
+    double* __restrict__ esite, float* __restrict__ G, int ldg, const int* __restrict__ guard) {
+  if (guard && *guard == 0) return;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int SC = TGX_FUSED_SC, T1 = SC / 64, ldw = SC + 8;
+  u16* sW = reinterpret_cast<u16*>(smem_raw);  // [128][ldw]; stage-1 slabs before
+  u16* sT = reinterpret_cast<u16*>(smem_raw + fused_region_w(SC));  // stage-2 slab [K1][32]
+  float* sE = reinterpret_cast<float*>(sT);  // [4][128] row-sum partials, at the end
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int mh = wave >> 2, nq = wave & 3;
+  const int row0 = blockIdx.x * 128, rw = row0 + mh * 64;
+  const int lr = lane & 15, lk = (lane >> 4) * 8, lc = (lane >> 4) * 4;
+  const float* al = reinterpret_cast<const float*>(ek.alphas);  // zero past n_sparse
+  const float zd2 = (float)ek.zd2;
+  float rs[4] = {0.f, 0.f, 0.f, 0.f};
+  f32x4 acc2[4][NT2];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < NT2; b++) acc2[a][b] = f32x4{0, 0, 0, 0};
+  const int c2 = nq * (16 * NT2);
+  for (int sc0 = 0; sc0 < NS; sc0 += SC) {
+    const int N1 = min(SC, NS - sc0), nt1 = N1 >> 6;  // N1: a multiple of 64
+    // ---- stage 1 over the chunk
+    f32x4 acc[4][T1];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int b = 0; b < T1; b++) acc[a][b] = f32x4{0, 0, 0, 0};
+    {
+      const int stage_rows = 128 + N1, nchunk = stage_rows * 4;
+      uint4 pf[TGX_FUSED_PFC];
+      auto fetch = [&](int k0) {
+#pragma unroll
+        for (int p = 0; p < TGX_FUSED_PFC; p++) {
+          const int ch = tid + p * 512;
+          const int r = ch >> 2, part = (ch & 3) * 8;
+          uint4 v = uint4{0, 0, 0, 0};
+          if (ch < nchunk) {
+            if (r < 128) {
+              if (row0 + r < M)
+                v = *reinterpret_cast<const uint4*>(q + (size_t)(row0 + r) * ldq + k0 + part);
+            } else {
+              v = *reinterpret_cast<const uint4*>(qs + (size_t)(sc0 + r - 128) * ldqs + k0 +
+                                                  part);
+            }
+          }
+          pf[p] = v;
+        }
+      };
+      auto stash = [&](int buf) {
+        u16* st = sW + (size_t)buf * stage_rows * TGX_FUSED_SLD;
+#pragma unroll
+        for (int p = 0; p < TGX_FUSED_PFC; p++) {
+          const int ch = tid + p * 512;
+          if (ch < nchunk)
+            *reinterpret_cast<uint4*>(st + (size_t)(ch >> 2) * TGX_FUSED_SLD + (ch & 3) * 8) =
+                pf[p];
+        }
+      };
+      const int nslab = K1 >> 5;
+      fetch(0);
+      stash(0);
+      __syncthreads();
+      for (int sl = 0; sl < nslab; sl++) {
+        if (sl + 1 < nslab) fetch((sl + 1) * 32);
+        const u16* st = sW + (size_t)(sl & 1) * stage_rows * TGX_FUSED_SLD;
+        bf16x8 af[4];
+#pragma unroll
+        for (int mi = 0; mi < 4; mi++)
+          af[mi] = *reinterpret_cast<const bf16x8*>(
+              st + (size_t)(mh * 64 + mi * 16 + lr) * TGX_FUSED_SLD + lk);
+#pragma unroll
+        for (int ni = 0; ni < T1; ni++)
+          if (ni < nt1) {
+            const bf16x8 bfr = *reinterpret_cast<const bf16x8*>(
+                st + (size_t)(128 + (N1 >> 2) * nq + ni * 16 + lr) * TGX_FUSED_SLD + lk);
+#pragma unroll
+            for (int mi = 0; mi < 4; mi++)
+              acc[mi][ni] =
+                  __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr, af[mi], acc[mi][ni], 0, 0, 0);
+          }
+        if (sl + 1 < nslab) stash((sl + 1) & 1);
+        __syncthreads();
+      }
+    }
+    // ---- epilogue of the chunk: energy partials and W into LDS
+    const int ldwc = N1 + 8;
+    auto epilogue = [&](auto zi) {
+      constexpr int ZI = decltype(zi)::v;
+#pragma unroll
+      for (int ni = 0; ni < T1; ni++) {
+        if (ni < nt1) {
+          const int sl0 = (N1 >> 2) * nq + ni * 16 + lc;  // column within the chunk
+          const float4 a4 = *reinterpret_cast<const float4*>(al + sc0 + sl0);
+          const float av[4] = {a4.x, a4.y, a4.z, a4.w};
+#pragma unroll
+          for (int mi = 0; mi < 4; mi++) {
+            __bf16 w[4];
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+              float kz, kz1;
+              kpow_f<ZI>(acc[mi][ni][v], ek.zint, &kz, &kz1);
+              rs[mi] += av[v] * kz;
+              w[v] = (__bf16)(zd2 * av[v] * kz1);
+            }
+            if (GRAD)
+              *reinterpret_cast<uint2*>(sW + (size_t)(mh * 64 + mi * 16 + lr) * ldwc + sl0) =
+                  __builtin_bit_cast(uint2, w);
+          }
+        }
+      }
+    };
+    switch (ek.zint) {
+      case 1: epilogue(ICz<1>{}); break;
+      case 2: epilogue(ICz<2>{}); break;
+      case 3: epilogue(ICz<3>{}); break;
+      case 4: epilogue(ICz<4>{}); break;
+      default: epilogue(ICz<0>{}); break;
+    }
+    if (GRAD) {
+      // ---- stage 2 of the chunk: G += W_chunk Qs_chunk
+      constexpr int nchunk2 = 256 * NT2, PF2 = (NT2 + 1) / 2;
+      uint4 pf[PF2];
+#pragma unroll
+      for (int p = 0; p < PF2; p++) pf[p] = uint4{0, 0, 0, 0};
+      auto fetch = [&](int k0) {
+#pragma unroll
+        for (int p = 0; p < PF2; p++) {
+          const int ch = tid + p * 512;
+          if (ch < nchunk2)
+            pf[p] = *reinterpret_cast<const uint4*>(qsT + (size_t)(ch >> 2) * ldqt + sc0 + k0 +
+                                                    (ch & 3) * 8);
+        }
+      };
+      auto stash = [&]() {
+#pragma unroll
+        for (int p = 0; p < PF2; p++) {
+          const int ch = tid + p * 512;
+          const int r = ch >> 2;
+          if (ch < nchunk2)
+            *reinterpret_cast<uint4*>(sT + (size_t)r * 32 + (((ch & 3) ^ ((r >> 2) & 3)) * 8)) =
+                pf[p];
+        }
+      };
+      fetch(0);
+      for (int k0 = 0; k0 < N1; k0 += 32) {
+        __syncthreads();  // W complete; the slab is no longer read
+        stash();
+        __syncthreads();
+        if (k0 + 32 < N1) fetch(k0 + 32);
+        bf16x8 aw[4];
+#pragma unroll
+        for (int mi = 0; mi < 4; mi++)
+          aw[mi] = *reinterpret_cast<const bf16x8*>(sW + (size_t)(mh * 64 + mi * 16 + lr) * ldwc +
+                                                    k0 + lk);
+#pragma unroll
+        for (int nj = 0; nj < NT2; nj++) {
+          const int r = c2 + nj * 16 + lr;
+          const bf16x8 bq = *reinterpret_cast<const bf16x8*>(
+              sT + (size_t)r * 32 + (((lane >> 4) ^ ((r >> 2) & 3)) * 8));
+#pragma unroll
+          for (int mi = 0; mi < 4; mi++)
+            acc2[mi][nj] =
+                __builtin_amdgcn_mfma_f32_16x16x32_bf16(bq, aw[mi], acc2[mi][nj], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();  // the next chunk's staging overwrites W and the slab
+  }
+#pragma unroll
+  for (int mi = 0; mi < 4; mi++) {
+    float x = rs[mi];
+    x += __shfl_xor(x, 16, 64);
+    x += __shfl_xor(x, 32, 64);
+    if (lane < 16) sE[nq * 128 + mh * 64 + mi * 16 + lr] = x;
+  }
+  __syncthreads();
+  if (tid < 128 && row0 + tid < M)
+    esite[row0 + tid] = d2 * ((double)sE[tid] + (double)sE[128 + tid] + (double)sE[256 + tid] +
+                              (double)sE[384 + tid]);
+  if (!GRAD) return;
+#pragma unroll
+  for (int mi = 0; mi < 4; mi++) {
+    const int r = rw + mi * 16 + lr;
+    if (r < M)
+#pragma unroll
+      for (int nj = 0; nj < NT2; nj++)
+        *reinterpret_cast<f32x4*>(G + (size_t)r * ldg + c2 + nj * 16 + lc) = acc2[mi][nj];
+  }
+}
+
+template <int NT2, bool GRAD>
+static hipError_t run_fused_chunked(int M, int sp_pad, int dim_pad, const void* q, const void* qs,
+                                    const void* qsT, const EpiKernel& ek, double delta,
+                                    double* esite, void* G, const int* guard, hipStream_t st) {
+  static size_t sm_set = 0;
+  const size_t sm = fused_region_w(TGX_FUSED_SC) + fused_region_t(dim_pad);
+  if (sm > sm_set) {
+    hipError_t e = hipFuncSetAttribute((const void*)gap_fused_chunked_kernel<NT2, GRAD>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    if (e != hipSuccess) return e;
+    sm_set = sm;
+  }
+  hipLaunchKernelGGL((gap_fused_chunked_kernel<NT2, GRAD>), dim3((M + 127) / 128), dim3(512), sm,
+                     st, M, sp_pad, dim_pad, (const u16*)q, dim_pad, (const u16*)qs, dim_pad,
+                     (const u16*)qsT, sp_pad, ek, delta * delta, esite, (float*)G, dim_pad, guard);
+  return hipGetLastError();
+}
+
 template <int NT2, bool GRAD>
 static hipError_t run_fused(int M, int sp_pad, int dim_pad, const void* q, const void* qs,
                             const void* qsT, const EpiKernel& ek, double delta, double* esite,
@@ -541,6 +760,20 @@ static hipError_t run(int n_sites, int dim, int dim_pad, int n_sparse, int sp_pa
   if constexpr (sizeof(TI) == 2) {
     // TGX_GEMM_UNFUSED=1 selects the two separate GEMMs (for comparisons)
     const char* uf = getenv("TGX_GEMM_UNFUSED");
+    if (zint >= 1 && zint <= 8 && sp_pad > 64 * TGX_FUSED_MAXT &&
+        dim_pad <= 64 * TGX_FUSED_MAXT && !(uf && uf[0] == '1')) {
+      if (!want_grad)
+        return run_fused_chunked<1, false>(M, sp_pad, dim_pad, q, qs, qsT, ek, delta, esite, G,
+                                           guard, st);
+      switch (dim_pad / 64) {
+#define TGX_FUSEDC_CASE(t) \
+  case t:                  \
+    return run_fused_chunked<t, true>(M, sp_pad, dim_pad, q, qs, qsT, ek, delta, esite, G, guard, st);
+        TGX_FUSEDC_CASE(1) TGX_FUSEDC_CASE(2) TGX_FUSEDC_CASE(3) TGX_FUSEDC_CASE(4)
+        TGX_FUSEDC_CASE(5) TGX_FUSEDC_CASE(6) TGX_FUSEDC_CASE(7)
+#undef TGX_FUSEDC_CASE
+      }
+    }
     if (zint >= 1 && zint <= 8 && sp_pad <= 64 * TGX_FUSED_MAXT &&
         dim_pad <= 64 * TGX_FUSED_MAXT &&
         fused_region_w(sp_pad) + fused_region_t(dim_pad) <= 160 * 1024 &&
