@@ -36,9 +36,9 @@ def _nb_equal(a, b, ulps_angles=True):
                                   "dense145", "carbon6000"])
 def test_neighbors_bit_exact(eng, case):
     """Lists of up to 64, 65-128 and over 128 entries (dense84, dense145:
-    1,500 C at 3.2 g/cm^3 with rcut 5.0 / 6.0 A) take the three sort paths;
-    up to 4,096 atoms the build runs in one workgroup, carbon6000 takes the
-    multi-kernel build."""
+    1,500 C at 3.2 g/cm^3 with rcut 5.0 / 6.0 A) take the three sort paths.
+    The one-workgroup build of captured Monte Carlo rebuilds is checked by
+    the Monte Carlo chains against the oracle."""
     if case == "carbon216":
         pos, sp, lat, _ = W.carbon216()
         rc = 3.7

@@ -515,7 +515,8 @@ __global__ __launch_bounds__(256) void find_sorted_many_kernel(
 // The whole build for small systems in one workgroup of 1024 threads:
 // binning, cell offsets and the cell-ordered copies in LDS and memory,
 // then the one-pass list of every atom, one wave per atom in turn.  The
-// same lists as the multi-kernel build; one launch instead of four.
+// same lists as the multi-kernel build; one launch instead of four (used
+// for builds captured in the Monte Carlo trial graph).
 #define TGX_NB_SMALL_N 4096
 #define TGX_NB_SMALL_CELLS 4096
 __global__ __launch_bounds__(1024) void nb_build_small_kernel(
@@ -703,7 +704,10 @@ hipError_t launch_neighbors(const double* pos, int n, const Cell& c, double rcut
   const bool fused = !c.small && c.mx >= 3 && c.my >= 3 && c.mz >= 3;
   // wrapped FP32 positions in cell order, after the ids (cell_atoms_bytes)
   float4* cpos = reinterpret_cast<float4*>(cell_atoms + ((n + 3) & ~3));
-  if (fused && n <= TGX_NB_SMALL_N && c.mx * c.my * c.mz <= TGX_NB_SMALL_CELLS) {
+  // captured (guarded) builds of small systems: one launch instead of four,
+  // since such builds are rare and their launches mostly exit at once; a
+  // single workgroup is slower than the grid-wide build when it does run
+  if (guard && fused && n <= TGX_NB_SMALL_N && c.mx * c.my * c.mz <= TGX_NB_SMALL_CELLS) {
     hipLaunchKernelGGL(nb_build_small_kernel, dim3(1), dim3(1024), 0, st, pos, n, c, rcut,
                        cell_of, cell_atoms, cpos, count, maxnb, list, flag, pref, guard);
     if (rev)
