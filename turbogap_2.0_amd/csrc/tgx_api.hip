@@ -310,7 +310,10 @@ static int soap_forward(tgx_ctx* c, int n, const Geom& cell, int n_sites) {
 }
 
 // Kernel step on the descriptors in c->q / c->qb.
-static int kernel_step(tgx_ctx* c, int n_sites, bool want_grad, const int* guard = nullptr) {
+// With eatom_out, every atom is a site (in atom order) and the kernel step
+// also writes eatom_out[i] = e0[species[i] - 1] + esite[i].
+static int kernel_step(tgx_ctx* c, int n_sites, bool want_grad, const int* guard = nullptr,
+                       double* eatom_out = nullptr) {
   const int prec = c->precision;
   const size_t tin = prec == TGX_FP64 ? 8 : (prec == TGX_FP32 ? 4 : 2);
   const size_t tacc = prec == TGX_FP64 ? 8 : 4;
@@ -331,7 +334,9 @@ static int kernel_step(tgx_ctx* c, int n_sites, bool want_grad, const int* guard
   TGX_HIP(c, launch_gap_kernel(prec, n_sites, c->dim, c->dim_pad, c->n_sparse, c->sp_pad, qa, qs,
                                qsT, al, c->delta, c->zeta, c->wmat.p, c->epart.p,
                                c->esite.as<double>(), want_grad ? c->gmat.p : nullptr,
-                               want_grad ? 1 : 0, guard, c->stream));
+                               want_grad ? 1 : 0, guard, c->stream, eatom_out,
+                               eatom_out ? X(c).e0_dev.as<double>() : nullptr,
+                               eatom_out ? c->species.as<int>() : nullptr));
   kev_rec(c, 3);
   return TGX_OK;
 }
@@ -402,23 +407,28 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
   ENSURE(c, virial, 48 * (size_t)n);  // per-atom 2b virial
   if (x.out.ensure((16 + 8 * 128) * sizeof(double)))
     return set_err(c, TGX_ERR_HIP, "allocation failed");
+  // fused 2b: every atom is a site, so the kernel step writes the atom
+  // energies, the force gather overwrites the forces and no per-atom virial
+  // is used -- nothing to initialise
+  const bool fuse = fuse_two_body(c, n, n_sites, do_forces);
   if (c->has_soap && n_sites > 0) {
     rcode = soap_forward(c, n, cell, n_sites);
     if (rcode) return rcode;
     ev_rec(c, 2);
-    rcode = kernel_step(c, n_sites, do_forces != 0);
+    rcode = kernel_step(c, n_sites, do_forces != 0, nullptr,
+                        fuse ? c->eatom.as<double>() : nullptr);
     if (rcode) return rcode;
     ev_rec(c, 3);
   } else {
     ev_rec(c, 2);
     ev_rec(c, 3);
   }
-  TGX_HIP(c, launch_energy_init(n, c->species.as<int>(), x.e0_dev.as<double>(),
-                                c->site_of_atom.as<int>(),
-                                (c->has_soap && n_sites > 0) ? c->esite.as<double>() : nullptr,
-                                c->eatom.as<double>(), c->forces.as<double>(),
-                                c->virial.as<double>(), c->stream));
-  const bool fuse = fuse_two_body(c, n, n_sites, do_forces);
+  if (!fuse)
+    TGX_HIP(c, launch_energy_init(n, c->species.as<int>(), x.e0_dev.as<double>(),
+                                  c->site_of_atom.as<int>(),
+                                  (c->has_soap && n_sites > 0) ? c->esite.as<double>() : nullptr,
+                                  c->eatom.as<double>(), c->forces.as<double>(),
+                                  c->virial.as<double>(), c->stream));
   if (do_forces && c->has_soap && n_sites > 0) {
     const size_t ts = tsize(prec);
     ENSURE(c, gpair, ts * 4 * (size_t)n * c->maxnb);  // (g, 0) per list slot
@@ -440,7 +450,8 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
       if (e == hipSuccess)
         e = launch_force_gather<double>(n, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
                                         c->nb_rev.as<int>(), c->site_of_atom.as<int>(), n_sites == n ? 1 : 0,
-                                        c->gpair.as<double>(), c->forces.as<double>(), c->stream);
+                                        c->gpair.as<double>(), c->forces.as<double>(), c->stream,
+                                        fuse ? 1 : 0);
     } else {
       e = launch_soap_bwd<float>(c->sd, x.sd_dev.as<SoapDev>(), c->W_dev.as<double>(),
                                  c->rtab.as<float>(), c->pos.as<double>(), c->species.as<int>(),
@@ -457,7 +468,8 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
       if (e == hipSuccess)
         e = launch_force_gather<float>(n, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
                                        c->nb_rev.as<int>(), c->site_of_atom.as<int>(), n_sites == n ? 1 : 0,
-                                       c->gpair.as<float>(), c->forces.as<double>(), c->stream);
+                                       c->gpair.as<float>(), c->forces.as<double>(), c->stream,
+                                        fuse ? 1 : 0);
     }
     TGX_HIP(c, e);
     kev_rec(c, 7);
@@ -476,13 +488,13 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
     TGX_HIP(c, launch_reduce_many(n_struct, x.b_aoff.as<int>(), c->site_of_atom.as<int>(),
                                   (do_forces && c->has_soap && n_sites > 0) ? c->vsite.as<double>()
                                                                             : nullptr,
-                                  do_forces ? c->virial.as<double>() : nullptr,
+                                  (do_forces && !fuse) ? c->virial.as<double>() : nullptr,
                                   c->eatom.as<double>(), x.b_out.as<double>(), c->stream));
   } else {
     TGX_HIP(c, launch_reduce((do_forces && c->has_soap) ? n_sites : 0,
                              (do_forces && c->has_soap) ? c->vsite.as<double>() : nullptr, n,
-                             do_forces ? c->virial.as<double>() : nullptr, c->eatom.as<double>(),
-                             x.out.as<double>(), c->stream, !defer_final));
+                             (do_forces && !fuse) ? c->virial.as<double>() : nullptr,
+                             c->eatom.as<double>(), x.out.as<double>(), c->stream, !defer_final));
   }
   ev_rec(c, 6);
   x.ev_on = true;

@@ -74,7 +74,18 @@ struct EpiKernel {
   void* W;             // TI [M][ldw]
   int ldw;
   void* epart;         // acc_t [gridDim.x][M]
+  // optional: atom energies eatom[r] = e0[species[r] - 1] + esite[r] (every
+  // atom a site, in order)
+  double* eatom;
+  const double* e0;
+  const int* species;
 };
+
+__device__ __forceinline__ void put_site_energy(const EpiKernel& ek, double* esite, int r,
+                                                double v) {
+  esite[r] = v;
+  if (ek.eatom) ek.eatom[r] = ek.e0[ek.species[r] - 1] + v;
+}
 struct EpiStore {
   void* C;  // acc_t [M][ldc]
   int ldc;
@@ -258,13 +269,13 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(int M, int N, int K,
 
 template <typename TA>
 __global__ void esum_kernel(int M, int ntiles, const TA* epart, double d2,
-                            double* esite, const int* guard) {
+                            double* esite, EpiKernel ek, const int* guard) {
   if (guard && *guard == 0) return;
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= M) return;
   double s = 0.0;
   for (int t = 0; t < ntiles; t++) s += (double)epart[(size_t)t * M + r];
-  esite[r] = d2 * s;
+  put_site_energy(ek, esite, r, d2 * s);
 }
 
 // Both GEMMs of the BF16 kernel step in one workgroup per 128 sites, for
@@ -440,8 +451,9 @@ __global__ __launch_bounds__(512) void gap_fused_kernel(
   }
   __syncthreads();
   if (tid < 128 && row0 + tid < M)
-    esite[row0 + tid] = d2 * ((double)sE[tid] + (double)sE[128 + tid] + (double)sE[256 + tid] +
-                              (double)sE[384 + tid]);
+    put_site_energy(ek, esite, row0 + tid,
+                    d2 * ((double)sE[tid] + (double)sE[128 + tid] + (double)sE[256 + tid] +
+                          (double)sE[384 + tid]));
   if (!GRAD) return;
 
   // ---- stage 2: D[c][m] = G[m][c] over this wave's quarter of the columns
@@ -696,8 +708,9 @@ __global__ __launch_bounds__(512) void gap_fused_chunked_kernel(
   }
   __syncthreads();
   if (tid < 128 && row0 + tid < M)
-    esite[row0 + tid] = d2 * ((double)sE[tid] + (double)sE[128 + tid] + (double)sE[256 + tid] +
-                              (double)sE[384 + tid]);
+    put_site_energy(ek, esite, row0 + tid,
+                    d2 * ((double)sE[tid] + (double)sE[128 + tid] + (double)sE[256 + tid] +
+                          (double)sE[384 + tid]));
   if (!GRAD) return;
 #pragma unroll
   for (int mi = 0; mi < 4; mi++) {
@@ -749,13 +762,15 @@ template <typename TI>
 static hipError_t run(int n_sites, int dim, int dim_pad, int n_sparse, int sp_pad,
                       const void* q, const void* qs, const void* qsT, const void* alphas,
                       double delta, double zeta, void* wmat, void* epart, double* esite,
-                      void* G, int want_grad, const int* guard, hipStream_t st) {
+                      void* G, int want_grad, const int* guard, double* eatom,
+                      const double* e0, const int* species, hipStream_t st) {
   typedef typename MF<TI>::acc_t acc_t;
   const int M = n_sites;
   if (M == 0) return hipSuccess;
   const int zint = fabs(zeta - (double)(int)zeta) < 1e-5 ? (int)zeta : 0;
   const double zz = zint > 0 ? (double)zint : zeta;
-  EpiKernel ek{alphas, n_sparse, zint, zz, zz * delta * delta, wmat, sp_pad, epart};
+  EpiKernel ek{alphas, n_sparse, zint, zz, zz * delta * delta, wmat, sp_pad, epart,
+               eatom, e0, species};
   EpiStore es{G, dim_pad};
   if constexpr (sizeof(TI) == 2) {
     // TGX_GEMM_UNFUSED=1 selects the two separate GEMMs (for comparisons)
@@ -799,7 +814,7 @@ static hipError_t run(int n_sites, int dim, int dim_pad, int n_sparse, int sp_pa
                      sp_pad, dim_pad, (const TI*)q, dim_pad, (const TI*)qs, dim_pad, ek, es, nt1,
                      nt1 * mt0, guard);
   hipLaunchKernelGGL(esum_kernel<acc_t>, dim3((M + 255) / 256), dim3(256), 0, st, M, nt1,
-                     (const acc_t*)epart, delta * delta, esite, guard);
+                     (const acc_t*)epart, delta * delta, esite, ek, guard);
   if (want_grad) {
     const int mt1 = (M + BM1 - 1) / BM1;
     const int nt2 = (dim_pad + BN1 - 1) / BN1;
@@ -814,15 +829,16 @@ hipError_t launch_gap_kernel(int mode, int n_sites, int dim, int dim_pad, int n_
                              int sp_pad, const void* q, const void* qs, const void* qsT,
                              const void* alphas, double delta, double zeta, void* wmat,
                              void* epart, double* esite, void* G, int want_grad,
-                             const int* guard, hipStream_t st) {
+                             const int* guard, hipStream_t st, double* eatom, const double* e0,
+                             const int* species) {
   if (mode == TGX_FP64)
     return run<double>(n_sites, dim, dim_pad, n_sparse, sp_pad, q, qs, qsT, alphas, delta, zeta,
-                       wmat, epart, esite, G, want_grad, guard, st);
+                       wmat, epart, esite, G, want_grad, guard, eatom, e0, species, st);
   if (mode == TGX_FP32)
     return run<float>(n_sites, dim, dim_pad, n_sparse, sp_pad, q, qs, qsT, alphas, delta, zeta,
-                      wmat, epart, esite, G, want_grad, guard, st);
+                      wmat, epart, esite, G, want_grad, guard, eatom, e0, species, st);
   return run<u16>(n_sites, dim, dim_pad, n_sparse, sp_pad, q, qs, qsT, alphas, delta, zeta, wmat,
-                  epart, esite, G, want_grad, guard, st);
+                  epart, esite, G, want_grad, guard, eatom, e0, species, st);
 }
 
 }  // namespace tgx

@@ -172,11 +172,14 @@ hipError_t launch_gap_kernel(int mode, int n_sites, int dim, int dim_pad,
                              const void* qs, const void* qsT,
                              const void* alphas, double delta, double zeta,
                              void* wmat, void* epart, double* esite, void* G,
-                             int want_grad, const int* guard, hipStream_t st);
+                             int want_grad, const int* guard, hipStream_t st,
+                             double* eatom = nullptr, const double* e0 = nullptr,
+                             const int* species = nullptr);
+// overwrite: forces[k] = gathered sum (otherwise added to forces[k])
 template <typename T>
 hipError_t launch_force_gather(int n, int maxnb, const int* count,
                                const int* list, const int* rev,
                                const int* site_of_atom, int all_sites, const T* gpair,
-                               double* forces, hipStream_t st);
+                               double* forces, hipStream_t st, int overwrite = 0);
 
 }  // namespace tgx

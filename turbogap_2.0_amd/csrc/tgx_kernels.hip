@@ -817,7 +817,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void force_gather_kernel(int n, int maxnb, const int* count,
                                                            const int* list, const int* rev,
                                                            const int* site_of_atom, int all_sites,
-                                                           const T* gpair, double* forces) {
+                                                           const T* gpair, double* forces,
+                                                           int overwrite) {
   typedef typename std::conditional<sizeof(T) == 4, float4, double4>::type T4;
   const T4* g4 = reinterpret_cast<const T4*>(gpair);
   const int lane = threadIdx.x & 63;
@@ -848,22 +849,28 @@ __global__ __launch_bounds__(256) void force_gather_kernel(int n, int maxnb, con
     f2 += __shfl_xor(f2, o, 64);
   }
   if (lane == 0) {
-    forces[3 * k] += f0;
-    forces[3 * k + 1] += f1;
-    forces[3 * k + 2] += f2;
+    if (overwrite) {
+      forces[3 * k] = f0;
+      forces[3 * k + 1] = f1;
+      forces[3 * k + 2] = f2;
+    } else {
+      forces[3 * k] += f0;
+      forces[3 * k + 1] += f1;
+      forces[3 * k + 2] += f2;
+    }
   }
 }
 
 template <typename T>
 hipError_t launch_force_gather(int n, int maxnb, const int* count, const int* list,
                                const int* rev, const int* site_of_atom, int all_sites,
-                               const T* gpair, double* forces, hipStream_t st) {
+                               const T* gpair, double* forces, hipStream_t st, int overwrite) {
   hipLaunchKernelGGL(force_gather_kernel<T>, dim3((n + 3) / 4), dim3(256), 0, st, n, maxnb,
-                     count, list, rev, site_of_atom, all_sites, gpair, forces);
+                     count, list, rev, site_of_atom, all_sites, gpair, forces, overwrite);
   return hipGetLastError();
 }
-template hipError_t launch_force_gather<float>(int, int, const int*, const int*, const int*, const int*, int, const float*, double*, hipStream_t);
-template hipError_t launch_force_gather<double>(int, int, const int*, const int*, const int*, const int*, int, const double*, double*, hipStream_t);
+template hipError_t launch_force_gather<float>(int, int, const int*, const int*, const int*, const int*, int, const float*, double*, hipStream_t, int);
+template hipError_t launch_force_gather<double>(int, int, const int*, const int*, const int*, const int*, int, const double*, double*, hipStream_t, int);
 
 // ---------------------------------------------------------------------------
 // distance_2b term (gap.f90:329-460): each pair counted from both atoms,
