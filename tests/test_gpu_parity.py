@@ -142,6 +142,9 @@ def test_fused_bf16_kernel_step(eng, zeta, n_sparse, monkeypatch):
         e5, _ = eng.kernel_step(q, want_qss=False)
         monkeypatch.delenv("TGX_GEMM_UNFUSED")
         e3, g3 = eng.kernel_step(q[:77])  # a partial block of rows
+        # energies of at most 128 sites: one workgroup per 32 sparse points,
+        # then the row sums (a repeated call must agree)
+        few = [eng.kernel_step(q[:m], want_qss=False)[0] for m in (40, 40, 100, 7)]
     finally:
         eng.set_precision("fp64")
     print("dE/dq bit-identical:", np.array_equal(g1, g2),
@@ -155,6 +158,9 @@ def test_fused_bf16_kernel_step(eng, zeta, n_sparse, monkeypatch):
     assert np.abs(e5 - e2).max() <= etol
     assert np.array_equal(g3, g1[:77])
     assert np.abs(e3 - e1[:77]).max() <= etol
+    assert np.array_equal(few[0], few[1])
+    for m, ef in zip((40, 40, 100, 7), few):
+        assert ef.shape == (m,) and np.abs(ef - e1[:m]).max() <= etol, m
 
 
 def test_kernel_step_fractional_zeta(eng):

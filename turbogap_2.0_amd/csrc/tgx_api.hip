@@ -319,7 +319,7 @@ static int kernel_step(tgx_ctx* c, int n_sites, bool want_grad, const int* guard
   const size_t tacc = prec == TGX_FP64 ? 8 : 4;
   const int ns = std::max(n_sites, 1);
   ENSURE(c, wmat, tin * (size_t)ns * c->sp_pad);
-  ENSURE(c, epart, tacc * (size_t)ns * (c->sp_pad / 64));
+  ENSURE(c, epart, tacc * (size_t)ns * (c->sp_pad / 32));  // per 32 sparse points at most
   ENSURE(c, esite, 8 * (size_t)ns);
   if (want_grad) ENSURE(c, gmat, tacc * (size_t)ns * c->dim_pad);
   const void *qa, *qs, *qsT, *al;

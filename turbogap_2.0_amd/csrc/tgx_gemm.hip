@@ -758,6 +758,103 @@ static hipError_t run_fused(int M, int sp_pad, int dim_pad, const void* q, const
   return hipGetLastError();
 }
 
+// Energies only, for at most 128 sites (the affected sites of a Monte Carlo
+// trial), where one workgroup per 128 rows would leave the chip idle: one
+// workgroup per 32 sparse points, its four waves owning 16 * MT rows each
+// and reading their MFMA fragments straight from memory (the few q rows and
+// the Qs slab are L2-resident).  Each workgroup writes the row sums of
+// alpha_s K_is^zeta over its sparse points to epart[workgroup][row]; the
+// row-sum kernel adds them in workgroup order.  (Summing them in the last
+// workgroup to finish instead needs device-scope fences, which write back
+// and invalidate L2 and cost more than the extra launch.)
+#define TGX_FEW_NC 2  // 16-column tiles per workgroup
+template <int MT>
+__global__ __launch_bounds__(256) void gap_energy_few_kernel(
+    int M, int dim_pad, const u16* __restrict__ q, const u16* __restrict__ qs, EpiKernel ek,
+    float* __restrict__ part, const int* __restrict__ guard) {
+  if (guard && *guard == 0) return;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int col0 = blockIdx.x * 16 * TGX_FEW_NC;
+  const int kq = (lane >> 4) * 8;
+  f32x4 acc[MT][TGX_FEW_NC];
+#pragma unroll
+  for (int mi = 0; mi < MT; mi++)
+#pragma unroll
+    for (int nj = 0; nj < TGX_FEW_NC; nj++) acc[mi][nj] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // rows past M read row M - 1 (a row of D depends only on its row of A;
+  // those rows are never stored)
+  const u16* ap[MT];
+#pragma unroll
+  for (int mi = 0; mi < MT; mi++)
+    ap[mi] = q + (size_t)min((wave * MT + mi) * 16 + (lane & 15), M - 1) * dim_pad + kq;
+  const u16* bp[TGX_FEW_NC];
+#pragma unroll
+  for (int nj = 0; nj < TGX_FEW_NC; nj++)
+    bp[nj] = qs + (size_t)(col0 + nj * 16 + (lane & 15)) * dim_pad + kq;
+  // KC k-steps of fragments are loaded before their products, so that a
+  // wave waits for memory once per KC steps (q was just written by another
+  // kernel and comes from the memory-side cache); steps past dim_pad re-read
+  // the last slab and multiply a zero Qs fragment
+  constexpr int KC = 6;
+  for (int k0 = 0; k0 < dim_pad; k0 += 32 * KC) {
+    bf16x8 a[KC][MT], b[KC][TGX_FEW_NC];
+#pragma unroll
+    for (int kc = 0; kc < KC; kc++) {
+      const int kk = min(k0 + 32 * kc, dim_pad - 32);
+#pragma unroll
+      for (int nj = 0; nj < TGX_FEW_NC; nj++)
+        b[kc][nj] = *reinterpret_cast<const bf16x8*>(bp[nj] + kk);
+#pragma unroll
+      for (int mi = 0; mi < MT; mi++) a[kc][mi] = *reinterpret_cast<const bf16x8*>(ap[mi] + kk);
+    }
+#pragma unroll
+    for (int kc = 0; kc < KC; kc++) {
+      if (k0 + 32 * kc >= dim_pad) {
+#pragma unroll
+        for (int nj = 0; nj < TGX_FEW_NC; nj++) b[kc][nj] = bf16x8{};
+      }
+#pragma unroll
+      for (int mi = 0; mi < MT; mi++)
+#pragma unroll
+        for (int nj = 0; nj < TGX_FEW_NC; nj++)
+          acc[mi][nj] =
+              __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[kc][mi], b[kc][nj], acc[mi][nj], 0, 0, 0);
+    }
+  }
+  // D[row][col]: lane holds rows (lane >> 4) * 4 + v of column lane & 15
+  const float* al = reinterpret_cast<const float*>(ek.alphas);
+  float rs[MT][4];
+#pragma unroll
+  for (int mi = 0; mi < MT; mi++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) rs[mi][v] = 0.f;
+#pragma unroll
+  for (int nj = 0; nj < TGX_FEW_NC; nj++) {
+    const int c = col0 + nj * 16 + (lane & 15);
+    const float w = c < ek.n_sparse ? al[c] : 0.f;
+#pragma unroll
+    for (int mi = 0; mi < MT; mi++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        float kz, kz1;
+        kpow_f<0>(acc[mi][nj][v], ek.zint, &kz, &kz1);
+        rs[mi][v] += w * kz;
+      }
+  }
+#pragma unroll
+  for (int mi = 0; mi < MT; mi++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      float x = rs[mi][v];
+      x += __shfl_xor(x, 1, 64);
+      x += __shfl_xor(x, 2, 64);
+      x += __shfl_xor(x, 4, 64);
+      x += __shfl_xor(x, 8, 64);
+      const int r = (wave * MT + mi) * 16 + (lane >> 4) * 4 + v;
+      if ((lane & 15) == 0 && r < M) part[(size_t)blockIdx.x * M + r] = x;
+    }
+}
+
 template <typename TI>
 static hipError_t run(int n_sites, int dim, int dim_pad, int n_sparse, int sp_pad,
                       const void* q, const void* qs, const void* qsT, const void* alphas,
@@ -775,6 +872,19 @@ static hipError_t run(int n_sites, int dim, int dim_pad, int n_sparse, int sp_pa
   if constexpr (sizeof(TI) == 2) {
     // TGX_GEMM_UNFUSED=1 selects the two separate GEMMs (for comparisons)
     const char* uf = getenv("TGX_GEMM_UNFUSED");
+    if (!want_grad && zint >= 1 && zint <= 8 && M <= 128 && sp_pad % 32 == 0 &&
+        dim_pad % 32 == 0 && !(uf && uf[0] == '1')) {
+      const int nwg = sp_pad / (16 * TGX_FEW_NC);
+      if (M <= 64)
+        hipLaunchKernelGGL(gap_energy_few_kernel<1>, dim3(nwg), dim3(256), 0, st, M, dim_pad,
+                           (const u16*)q, (const u16*)qs, ek, (float*)epart, guard);
+      else
+        hipLaunchKernelGGL(gap_energy_few_kernel<2>, dim3(nwg), dim3(256), 0, st, M, dim_pad,
+                           (const u16*)q, (const u16*)qs, ek, (float*)epart, guard);
+      hipLaunchKernelGGL(esum_kernel<float>, dim3((M + 255) / 256), dim3(256), 0, st, M, nwg,
+                         (const float*)epart, delta * delta, esite, ek, guard);
+      return hipGetLastError();
+    }
     if (zint >= 1 && zint <= 8 && sp_pad > 64 * TGX_FUSED_MAXT &&
         dim_pad <= 64 * TGX_FUSED_MAXT && !(uf && uf[0] == '1')) {
       if (!want_grad)
