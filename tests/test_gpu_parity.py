@@ -93,6 +93,31 @@ def test_soap_fp64_and_fp32(eng):
     assert np.abs(q32 - ref).max() < 2e-5
 
 
+@pytest.mark.parametrize("case", ["carbon216", "small32"])
+def test_soap_fp32_few_sites_split(eng, case, monkeypatch):
+    """The workgroup-per-site FP32 forward kernel of Monte Carlo trials (the
+    list split over four waves, partials added in wave order; TGX_FWD_SPLIT=1
+    selects it for any call of up to 256 sites) against the one-wave-per-site
+    kernel, which sums the pairs in another order, and both against the FP64
+    oracle."""
+    pos, sp, lat, model = W.carbon216() if case == "carbon216" else W.small_box()
+    eng.set_model(model)
+    ref = oracle.soap(model.hypers, pos, sp, lat)["soap"]
+    eng.set_precision("fp32")
+    try:
+        q_wave, _ = eng.soap(pos, sp, lat)
+        monkeypatch.setenv("TGX_FWD_SPLIT", "1")
+        q_split, _ = eng.soap(pos, sp, lat)
+        monkeypatch.delenv("TGX_FWD_SPLIT")
+    finally:
+        eng.set_precision("fp64")
+    print(case, len(pos), np.abs(q_split - ref).max(), np.abs(q_split - q_wave).max())
+    assert len(pos) <= 256
+    assert np.abs(q_split - ref).max() < 2e-5
+    assert np.abs(q_wave - ref).max() < 2e-5
+    assert np.abs(q_split - q_wave).max() < 2e-6
+
+
 def test_soap_multispecies(eng):
     pos, sp, lat, model = W.cho()
     eng.set_model(model)

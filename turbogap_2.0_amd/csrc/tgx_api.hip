@@ -30,7 +30,8 @@ hipError_t launch_soap_fwd(const SoapDev& sdh, const SoapDev* sd, const double* 
                            const float* rtab, const double* pos, const int* species,
                            const Geom& g, int maxnb, const int* count, const int* list,
                            const int* site_atom, int n_sites, const int* dmap, T* q, int q_ld,
-                           unsigned short* qb, int qb_ld, T* cbuf, T* pnorm, hipStream_t st);
+                           unsigned short* qb, int qb_ld, T* cbuf, T* pnorm, hipStream_t st,
+                           int split = 0);
 template <typename T>
 hipError_t launch_soap_bwd(const SoapDev& sdh, const SoapDev* sd, const double* W,
                            const float* rtab, const double* pos, const int* species,
@@ -274,7 +275,9 @@ static int setup_sites(tgx_ctx* c, int n, const int* species_host, int* n_sites)
 static void kev_rec(tgx_ctx* c, int k);
 
 // SOAP forward for the current list into c->q (and c->qb in mixed mode).
-static int soap_forward(tgx_ctx* c, int n, const Geom& cell, int n_sites) {
+// few: a few sites (a Monte Carlo trial) take the workgroup-per-site FP32
+// kernel; TGX_FWD_SPLIT=1 asks for it on every FP32 call of up to 256 sites.
+static int soap_forward(tgx_ctx* c, int n, const Geom& cell, int n_sites, bool few = false) {
   Extra& x = X(c);
   const int prec = c->precision;
   const size_t ts = tsize(prec);
@@ -287,6 +290,8 @@ static int soap_forward(tgx_ctx* c, int n, const Geom& cell, int n_sites) {
     ENSURE(c, qb, 2 * (size_t)std::max(n_sites, 1) * c->dim_pad);
     qb = c->qb.as<unsigned short>();
   }
+  const char* fs = getenv("TGX_FWD_SPLIT");
+  const int split = (few || (fs && fs[0] == '1')) ? 1 : 0;
   hipError_t e;
   kev_rec(c, 0);
   if (prec == TGX_FP64)
@@ -302,7 +307,7 @@ static int soap_forward(tgx_ctx* c, int n, const Geom& cell, int n_sites) {
                                cell, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
                                c->site_atom.as<int>(), n_sites, x.pairn.as<int>(),
                                c->q.as<float>(), c->dim_pad, qb, c->dim_pad, c->cbuf.as<float>(),
-                               c->pnorm.as<float>(), c->stream);
+                               c->pnorm.as<float>(), c->stream, split);
   TGX_HIP(c, e);
   kev_rec(c, 1);
   (void)n;
@@ -1312,7 +1317,7 @@ static int mc_trial_launch(tgx_ctx* c, const McRun& r) {
                                   c->nb_count.as<int>(), c->nb_list.as<int>(), c->maxnb,
                                   c->species.as<int>(), r.central, c->site_atom.as<int>(), r.cap,
                                   x.mc_naff.as<int>(), c->stream));
-    int rcode = soap_forward(c, r.n, Geom{r.cell}, r.cap);
+    int rcode = soap_forward(c, r.n, Geom{r.cell}, r.cap, true);
     if (rcode) return rcode;
     if (r.da) {
       hipError_t e =

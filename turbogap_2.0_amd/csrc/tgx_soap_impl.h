@@ -192,6 +192,59 @@ __device__ __forceinline__ void ang_factors(T log2t, T* Al) {
   }
 }
 
+// Basis of one pair for the forward expansion: R_n(r) into rr[0..NRP) (at
+// the neighbour species' radial offset, zeros elsewhere) and A_l(r) Y_lm(u)
+// into ay[lm], the real harmonics by the m-outer recurrence in l.
+template <typename T, int NRP, int NMX, int LMX>
+__device__ __forceinline__ void fwd_pair_basis(const SoapDev* sd, const double* W,
+                                               const float2* rtab, const double* pos,
+                                               const int* species, const Cell& cell, int i,
+                                               int e, int L, T* ay, T* rr) {
+  double dv[3];
+  const double rd = pair_vec_fast(pos, i, e, cell, dv);
+  const int s = species[e & TGX_ATOM_MASK] - 1;
+  const T r = (T)rd;
+  const T ir = (T)1 / r;
+  const T ux = (T)dv[0] * ir, uy = (T)dv[1] * ir, uz = (T)dv[2] * ir;
+  T R[NMX], dR[NMX];
+  Radial<T>::template eval<NMX>(sd, W, rtab, s, r, R, dR);
+  const int o = sd->off[s];
+  // R[n] = 0 for n >= n_max(s) and o + NMX <= NRP: no guard needed
+#pragma unroll
+  for (int n = 0; n < NRP; n++) rr[n] = (T)0;
+#pragma unroll
+  for (int n = 0; n < NMX; n++) rr[o + n] = R[n];
+  T log2t, D;
+  ang_params<T>(sd, s, r, &log2t, &D);
+  T Al[LMX + 1];
+  ang_factors<T, LMX>(log2t, Al);
+  T Cm = 1, Sm = 0;
+#pragma unroll
+  for (int m = 0; m <= LMX; m++) {
+    if (m <= L) {
+      T pm2 = 0, pm1 = SphK<T>::P(m);
+#pragma unroll
+      for (int l = m; l <= LMX; l++) {
+        if (l <= L) {
+          T P = pm1;
+          if (l > m) {
+            const int ix = l * (TGX_MAX_LMAX + 1) + m;
+            P = SphK<T>::A(ix) * uz * pm1 - SphK<T>::B(ix) * pm2;
+            pm2 = pm1;
+            pm1 = P;
+          }
+          const T AP = Al[l] * P;
+          ay[l * l + l + m] = AP * Cm;
+          if (m > 0) ay[l * l + l - m] = AP * Sm;
+        }
+      }
+      const T Cn = ux * Cm - uy * Sm;
+      Sm = ux * Sm + uy * Cm;
+      Cm = Cn;
+    }
+  }
+}
+
 // Minimum waves per SIMD the FP32 forward kernel is register-limited to
 // (measured best on gfx950 with 12-pair chunks); FP64 is left unconstrained.
 template <typename T> struct FwdWaves { static constexpr int v = 4; };
@@ -244,54 +297,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FwdWaves<T>
                                             CAPV, sub, lane, [](int) {});
     wave_sync();
     for (int c0 = 0; c0 < nv; c0 += CH) {
-      if (lane < CH && c0 + lane < nv) {
-        const int jj = lane;
-        const int e = row[sIdx[c0 + jj]];
-        double dv[3];
-        const double rd = pair_vec_fast(pos, i, e, cell, dv);
-        const int s = species[e & TGX_ATOM_MASK] - 1;
-        const T r = (T)rd;
-        const T ir = (T)1 / r;
-        const T ux = (T)dv[0] * ir, uy = (T)dv[1] * ir, uz = (T)dv[2] * ir;
-        T R[NMX], dR[NMX];
-        Radial<T>::template eval<NMX>(sd, W, rtab, s, r, R, dR);
-        const int o = sd->off[s];
-        // R[n] = 0 for n >= n_max(s) and o + NMX <= NRP: no guard needed
-#pragma unroll
-        for (int n = 0; n < NRP; n++) sR[jj * NRP + n] = (T)0;
-#pragma unroll
-        for (int n = 0; n < NMX; n++) sR[jj * NRP + o + n] = R[n];
-        T log2t, D;
-        ang_params<T>(sd, s, r, &log2t, &D);
-        T Al[LMX + 1];
-        ang_factors<T, LMX>(log2t, Al);
-        T* ay = sAY + jj * L2;
-        T Cm = 1, Sm = 0;
-#pragma unroll
-        for (int m = 0; m <= LMX; m++) {
-          if (m <= L) {
-            T pm2 = 0, pm1 = SphK<T>::P(m);
-#pragma unroll
-            for (int l = m; l <= LMX; l++) {
-              if (l <= L) {
-                T P = pm1;
-                if (l > m) {
-                  const int ix = l * (TGX_MAX_LMAX + 1) + m;
-                  P = SphK<T>::A(ix) * uz * pm1 - SphK<T>::B(ix) * pm2;
-                  pm2 = pm1;
-                  pm1 = P;
-                }
-                const T AP = Al[l] * P;
-                ay[l * l + l + m] = AP * Cm;
-                if (m > 0) ay[l * l + l - m] = AP * Sm;
-              }
-            }
-            const T Cn = ux * Cm - uy * Sm;
-            Sm = ux * Sm + uy * Cm;
-            Cm = Cn;
-          }
-        }
-      }
+      if (lane < CH && c0 + lane < nv)
+        fwd_pair_basis<T, NRP, NMX, LMX>(sd, W, rtab, pos, species, cell, i, row[sIdx[c0 + lane]],
+                                         L, sAY + lane * L2, sR + lane * NRP);
       wave_sync();
       const int nj = min(CH, nv - c0);
       for (int j = 0; j < nj; j++) {
@@ -385,6 +393,158 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FwdWaves<T>
   if (qbs)
     for (int d = dim + lane; d < qb_ld; d += LW) qbs[d] = 0;
   if (lane == 0) pnorm[site] = sqrt(ss);
+}
+
+// Forward expansion of few sites (up to TGX_FWD_SPLIT_MAX: the affected
+// sites of a Monte Carlo trial, on request), where the one-wave-per-site
+// kernel is a long serial chain on an idle chip: one workgroup per site, its four
+// waves taking a contiguous quarter of the neighbour list each (scan, basis
+// into LDS in chunks of CH pairs, c partials in registers with lanes = lm),
+// the partials added in wave order, then the power spectrum with threads
+// over the descriptor components and a workgroup reduction of the norm.
+// Output layout as soap_fwd_kernel's.  FP32 only.
+#define TGX_FWD_SPLIT_MAX 256
+template <int NRP, int LMX>
+__host__ __device__ constexpr size_t fwd_split_bytes(int L2) {
+  return sizeof(float) * ((size_t)4 * 16 * (L2 + NRP) + (size_t)5 * NRP * ((L2 + 3) & ~3)) +
+         2 * 4 * 64;
+}
+template <typename T, int NRP, int NMX, int LMX, bool EXL, typename GEO>
+__global__ __launch_bounds__(256) void soap_fwd_split_kernel(
+    const SoapDev* __restrict__ sd, const double* __restrict__ W,
+    const float2* __restrict__ rtab, const double* __restrict__ pos,
+    const int* __restrict__ species, GEO geo, int maxnb, const int* __restrict__ count,
+    const int* __restrict__ list, const int* __restrict__ site_atom, int n_sites,
+    const int* __restrict__ dmap, T* __restrict__ q, int q_ld, u16* __restrict__ qb,
+    int qb_ld, T* __restrict__ cbuf, T* __restrict__ pnorm) {
+  constexpr int CH = 16;
+  constexpr int NS = ((LMX + 1) * (LMX + 1) + 63) / 64;  // lm slots per lane
+  constexpr int IPT = (NRP * (NRP + 1) / 2 * (LMX + 1) + 255) / 256;  // components per thread
+  const int L = EXL ? LMX : sd->l_max, L2 = EXL ? (LMX + 1) * (LMX + 1) : sd->L2;
+  const int L2P = (L2 + 3) & ~3;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* sAY = reinterpret_cast<T*>(smem_raw) + (size_t)wave * CH * (L2 + NRP);  // [CH][L2]
+  T* sR = sAY + CH * L2;                                                     // [CH][NRP]
+  T* sP = reinterpret_cast<T*>(smem_raw) + (size_t)4 * CH * (L2 + NRP);     // [4][NRP][L2P]
+  T* sC = sP + (size_t)4 * NRP * L2P;                                        // [NRP][L2P]
+  u16* sIdx = reinterpret_cast<u16*>(sC + (size_t)NRP * L2P) + wave * 64;
+  __shared__ T sred[4];
+  const int site = blockIdx.x;
+  if (site >= n_sites) return;
+  const int i = site_atom[site];
+  const Cell cell = geo.of(i);
+  const int si = species[i] - 1;
+  const int cnt = min(count[i], maxnb);
+  const int* row = list + (size_t)i * maxnb;
+  T acc[NS][NRP];
+#pragma unroll
+  for (int k = 0; k < NS; k++)
+#pragma unroll
+    for (int n = 0; n < NRP; n++) acc[k][n] = (T)0;
+  const int quarter = (cnt + 3) / 4;
+  const int lo = min(cnt, wave * quarter), hi = min(cnt, lo + quarter);
+  for (int base = lo; base < hi; base += 64) {
+    // valid pairs of the next 64 slots, compacted in slot order
+    const int slot = base + lane;
+    bool valid = false;
+    if (slot < hi) {
+      const int e = row[slot];
+      double dv[3];
+      const double rd = pair_vec_fast(pos, i, e, cell, dv);
+      valid = rd < sd->rh[species[e & TGX_ATOM_MASK] - 1];
+    }
+    const unsigned long long mask = __ballot(valid);
+    if (valid) sIdx[__popcll(mask & ((1ull << lane) - 1ull))] = (u16)slot;
+    const int nv = __popcll(mask);
+    wave_sync();
+    for (int c0 = 0; c0 < nv; c0 += CH) {
+      if (lane < CH && c0 + lane < nv)
+        fwd_pair_basis<T, NRP, NMX, LMX>(sd, W, rtab, pos, species, cell, i, row[sIdx[c0 + lane]],
+                                         L, sAY + lane * L2, sR + lane * NRP);
+      wave_sync();
+      const int nj = min(CH, nv - c0);
+      for (int j = 0; j < nj; j++) {
+        T y[NS];
+#pragma unroll
+        for (int k = 0; k < NS; k++) y[k] = sAY[j * L2 + min(lane + k * 64, L2 - 1)];
+#pragma unroll
+        for (int n = 0; n < NRP; n++) {
+          const T rv = sR[j * NRP + n];
+#pragma unroll
+          for (int k = 0; k < NS; k++) acc[k][n] += rv * y[k];
+        }
+      }
+      wave_sync();
+    }
+  }
+  // central atom: l = 0 only
+  if (wave == 0 && lane == 0) {
+    const int o = sd->off[si];
+    for (int n = 0; n < sd->nmax[si]; n++) {
+#pragma unroll
+      for (int k = 0; k < NRP; k++)
+        if (k == o + n) acc[0][k] += (T)sd->rself[si * TGX_MAX_NR + n];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NS; k++) {
+    const int lm = lane + k * 64;
+    if (lm < L2) {
+#pragma unroll
+      for (int n = 0; n < NRP; n++) sP[((size_t)wave * NRP + n) * L2P + lm] = acc[k][n];
+    }
+  }
+  __syncthreads();
+  T* cs = cbuf + (size_t)site * NRP * L2;
+  for (int x = tid; x < NRP * L2; x += 256) {
+    const int n = x / L2, lm = x - n * L2;
+    const T v = ((sP[(size_t)n * L2P + lm] + sP[((size_t)NRP + n) * L2P + lm]) +
+                 sP[((size_t)2 * NRP + n) * L2P + lm]) +
+                sP[((size_t)3 * NRP + n) * L2P + lm];
+    sC[n * L2P + lm] = v;
+    cs[x] = v;
+  }
+  __syncthreads();
+  // power spectrum p_d, d = (n1 <= n2, l): w sum_m c[n1][l,m] c[n2][l,m]
+  const int dim = sd->dim;
+  T pv[IPT];
+  T ss = 0;
+#pragma unroll
+  for (int t = 0; t < IPT; t++) {
+    const int d = tid + t * 256;
+    pv[t] = 0;
+    if (d < dim) {
+      const int dm = dmap[d];
+      const int n1 = dm & 255, n2 = (dm >> 8) & 255, l = dm >> 16;
+      const T* ca = sC + n1 * L2P + l * l;
+      const T* cb = sC + n2 * L2P + l * l;
+      T a = 0;
+      for (int m = 0; m <= 2 * l; m++) a += ca[m] * cb[m];
+      pv[t] = (n1 == n2 ? (T)1 : (T)1.41421356237309504880) * a;
+      ss += pv[t] * pv[t];
+    }
+  }
+  ss = wave_sum(ss);
+  if (lane == 0) sred[wave] = ss;
+  __syncthreads();
+  ss = ((sred[0] + sred[1]) + sred[2]) + sred[3];
+  const T inv = (T)1 / sqrt(ss);
+  T* qs = q + (size_t)site * q_ld;
+  u16* qbs = qb ? qb + (size_t)site * qb_ld : nullptr;
+#pragma unroll
+  for (int t = 0; t < IPT; t++) {
+    const int d = tid + t * 256;
+    if (d < dim) {
+      const T v = pv[t] * inv;
+      qs[d] = v;
+      if (qbs) qbs[d] = f2bf((float)v);
+    }
+  }
+  for (int d = dim + tid; d < q_ld; d += 256) qs[d] = (T)0;
+  if (qbs)
+    for (int d = dim + tid; d < qb_ld; d += 256) qbs[d] = 0;
+  if (tid == 0) pnorm[site] = sqrt(ss);
 }
 
 template <typename T, int NRP, int NMX, int LMX, bool EXL, typename GEO>
@@ -619,9 +779,22 @@ static hipError_t fwd_inst(const SoapDev& sdh, const SoapDev* sd, const double* 
                            const float* rtab, const double* pos, const int* species,
                            const GEO& cell, int maxnb, const int* count, const int* list,
                            const int* site_atom, int n_sites, const int* dmap, T* q, int q_ld,
-                           u16* qb, int qb_ld, T* cbuf, T* pnorm, hipStream_t st) {
+                           u16* qb, int qb_ld, T* cbuf, T* pnorm, hipStream_t st, int split) {
   constexpr int SPW = Spw<NRP>::v;
   if (maxnb > 65535) return hipErrorInvalidValue;  // 16-bit slot lists
+  if constexpr (sizeof(T) == 4 && NRP <= 16) {
+    if (split && n_sites <= TGX_FWD_SPLIT_MAX) {
+      const size_t sm = fwd_split_bytes<NRP, LMX>(sdh.L2);
+      static size_t sm_split = 65536;
+      hipError_t e = ensure_lds(soap_fwd_split_kernel<T, NRP, NMX, LMX, EXL, GEO>, sm, &sm_split);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL((soap_fwd_split_kernel<T, NRP, NMX, LMX, EXL, GEO>), dim3(n_sites),
+                         dim3(256), sm, st, sd, W, reinterpret_cast<const float2*>(rtab), pos,
+                         species, cell, maxnb, count, list, site_atom, n_sites, dmap, q, q_ld, qb,
+                         qb_ld, cbuf, pnorm);
+      return hipGetLastError();
+    }
+  }
   const size_t per_wave = SPW * fwd_site_bytes<T, NRP>(sdh.L2);
   const int wpb = waves_per_block(per_wave);
   const size_t sm = wpb * per_wave;
@@ -688,13 +861,14 @@ hipError_t launch_soap_fwd_g(const SoapDev& sdh, const SoapDev* sd, const double
                            const float* rtab, const double* pos, const int* species,
                            const GEO& cell, int maxnb, const int* count, const int* list,
                            const int* site_atom, int n_sites, const int* dmap, T* q, int q_ld,
-                           u16* qb, int qb_ld, T* cbuf, T* pnorm, hipStream_t st) {
+                           u16* qb, int qb_ld, T* cbuf, T* pnorm, hipStream_t st,
+                           int split = 0) {
   if (n_sites == 0) return hipSuccess;
   return soap_dispatch(sdh, [&](auto NR, auto NM, auto LM, auto EX) {
     return fwd_inst<T, decltype(NR)::value, decltype(NM)::value, decltype(LM)::value,
                     decltype(EX)::value, GEO>(sdh, sd, W, rtab, pos, species, cell, maxnb, count,
                                               list, site_atom, n_sites, dmap, q, q_ld, qb, qb_ld,
-                                              cbuf, pnorm, st);
+                                              cbuf, pnorm, st, split);
   });
 }
 
