@@ -62,10 +62,10 @@ hipError_t launch_mc_affected(const int* ctl, const double* dctl, const double* 
 hipError_t launch_mc_stage1(double* pos, const int* species, const Cell& cell, int maxnb,
                             const int* count, const int* list, const int* aff, const int* n_aff,
                             const double* esur, const double* esur_cur, const TwoBodyDev* twob,
-                            int n2b, unsigned long long seed, double kT, int* ctl, double* dctl,
-                            double* trace, hipStream_t st);
+                            int n2b, int exact, unsigned long long seed, double kT, int* ctl,
+                            double* dctl, double* trace, hipStream_t st);
 hipError_t launch_mc_stage2(double* pos, const int* aff, const int* n_aff, const double* esite,
-                            double* e_cur, const double* esur, double* esur_cur,
+                            const float* parts, int nparts, int part_ld, double d2, double* e_cur, const double* esur, double* esur_cur,
                             unsigned long long seed, double kT, int* ctl, double* dctl,
                             double* trace, hipStream_t st);
 template <typename T>
@@ -318,7 +318,7 @@ static int soap_forward(tgx_ctx* c, int n, const Geom& cell, int n_sites, bool f
 // With eatom_out, every atom is a site (in atom order) and the kernel step
 // also writes eatom_out[i] = e0[species[i] - 1] + esite[i].
 static int kernel_step(tgx_ctx* c, int n_sites, bool want_grad, const int* guard = nullptr,
-                       double* eatom_out = nullptr) {
+                       double* eatom_out = nullptr, int* few_parts = nullptr) {
   const int prec = c->precision;
   const size_t tin = prec == TGX_FP64 ? 8 : (prec == TGX_FP32 ? 4 : 2);
   const size_t tacc = prec == TGX_FP64 ? 8 : 4;
@@ -341,7 +341,7 @@ static int kernel_step(tgx_ctx* c, int n_sites, bool want_grad, const int* guard
                                c->esite.as<double>(), want_grad ? c->gmat.p : nullptr,
                                want_grad ? 1 : 0, guard, c->stream, eatom_out,
                                eatom_out ? X(c).e0_dev.as<double>() : nullptr,
-                               eatom_out ? c->species.as<int>() : nullptr));
+                               eatom_out ? c->species.as<int>() : nullptr, few_parts));
   kev_rec(c, 3);
   return TGX_OK;
 }
@@ -1339,9 +1339,14 @@ static int mc_trial_launch(tgx_ctx* c, const McRun& r) {
                               c->nb_count.as<int>(), c->nb_list.as<int>(), c->site_atom.as<int>(),
                               x.mc_naff.as<int>(), x.mc_esur.as<double>(),
                               x.mc_esur_cur.as<double>(), x.mc_twob.as<TwoBodyDev>(), r.n2b,
-                              r.seed, r.kT, ctl, dctl, x.mc_trace.as<double>(), c->stream));
+                              c->precision == TGX_FP64 ? 1 : 0, r.seed, r.kT, ctl, dctl,
+                              x.mc_trace.as<double>(), c->stream));
+  // without sample collection the decision kernel sums the few-site
+  // kernel's partial energies itself (one launch less per trial)
+  int nparts = 0;
   if (c->has_soap) {
-    int rcode = kernel_step(c, r.cap, false, ctl + MC_GATE);
+    int rcode = kernel_step(c, r.cap, false, ctl + MC_GATE, nullptr,
+                            r.cap_s > 0 ? nullptr : &nparts);
     if (rcode) return rcode;
     if (r.cap_s > 0) {
       hipError_t e =
@@ -1359,6 +1364,8 @@ static int mc_trial_launch(tgx_ctx* c, const McRun& r) {
   }
   TGX_HIP(c, launch_mc_stage2(c->pos.as<double>(), c->site_atom.as<int>(), x.mc_naff.as<int>(),
                               c->has_soap ? c->esite.as<double>() : nullptr,
+                              nparts ? c->epart.as<float>() : nullptr, nparts, r.cap,
+                              c->delta * c->delta,
                               x.mc_ecur.as<double>(), x.mc_esur.as<double>(),
                               x.mc_esur_cur.as<double>(), r.seed, r.kT, ctl, dctl,
                               x.mc_trace.as<double>(), c->stream));

@@ -860,9 +860,10 @@ static hipError_t run(int n_sites, int dim, int dim_pad, int n_sparse, int sp_pa
                       const void* q, const void* qs, const void* qsT, const void* alphas,
                       double delta, double zeta, void* wmat, void* epart, double* esite,
                       void* G, int want_grad, const int* guard, double* eatom,
-                      const double* e0, const int* species, hipStream_t st) {
+                      const double* e0, const int* species, int* few_parts, hipStream_t st) {
   typedef typename MF<TI>::acc_t acc_t;
   const int M = n_sites;
+  if (few_parts) *few_parts = 0;
   if (M == 0) return hipSuccess;
   const int zint = fabs(zeta - (double)(int)zeta) < 1e-5 ? (int)zeta : 0;
   const double zz = zint > 0 ? (double)zint : zeta;
@@ -881,8 +882,11 @@ static hipError_t run(int n_sites, int dim, int dim_pad, int n_sparse, int sp_pa
       else
         hipLaunchKernelGGL(gap_energy_few_kernel<2>, dim3(nwg), dim3(256), 0, st, M, dim_pad,
                            (const u16*)q, (const u16*)qs, ek, (float*)epart, guard);
-      hipLaunchKernelGGL(esum_kernel<float>, dim3((M + 255) / 256), dim3(256), 0, st, M, nwg,
-                         (const float*)epart, delta * delta, esite, ek, guard);
+      if (few_parts)  // the caller sums the partials itself
+        *few_parts = nwg;
+      else
+        hipLaunchKernelGGL(esum_kernel<float>, dim3((M + 255) / 256), dim3(256), 0, st, M, nwg,
+                           (const float*)epart, delta * delta, esite, ek, guard);
       return hipGetLastError();
     }
     if (zint >= 1 && zint <= 8 && sp_pad > 64 * TGX_FUSED_MAXT &&
@@ -940,15 +944,15 @@ hipError_t launch_gap_kernel(int mode, int n_sites, int dim, int dim_pad, int n_
                              const void* alphas, double delta, double zeta, void* wmat,
                              void* epart, double* esite, void* G, int want_grad,
                              const int* guard, hipStream_t st, double* eatom, const double* e0,
-                             const int* species) {
+                             const int* species, int* few_parts) {
   if (mode == TGX_FP64)
     return run<double>(n_sites, dim, dim_pad, n_sparse, sp_pad, q, qs, qsT, alphas, delta, zeta,
-                       wmat, epart, esite, G, want_grad, guard, eatom, e0, species, st);
+                       wmat, epart, esite, G, want_grad, guard, eatom, e0, species, few_parts, st);
   if (mode == TGX_FP32)
     return run<float>(n_sites, dim, dim_pad, n_sparse, sp_pad, q, qs, qsT, alphas, delta, zeta,
-                      wmat, epart, esite, G, want_grad, guard, eatom, e0, species, st);
+                      wmat, epart, esite, G, want_grad, guard, eatom, e0, species, few_parts, st);
   return run<u16>(n_sites, dim, dim_pad, n_sparse, sp_pad, q, qs, qsT, alphas, delta, zeta, wmat,
-                  epart, esite, G, want_grad, guard, eatom, e0, species, st);
+                  epart, esite, G, want_grad, guard, eatom, e0, species, few_parts, st);
 }
 
 }  // namespace tgx

@@ -167,6 +167,12 @@ void radial_closed_form(const SoapDev& sd, const double* W, int s, double r,
 int make_cell(const double* lattice, double rcut, Cell* cell, std::string* err);
 
 // Kernel launchers are declared next to their callers in tgx_api.hip.
+// launch_gap_kernel: eatom/e0/species (every atom a site, in order) also
+// write eatom[i] = e0[species[i] - 1] + esite[i].  few_parts (energies only):
+// when the few-site kernel runs, its row-sum launch is left out and
+// *few_parts = the number of FP32 partials per site in epart[part][site]
+// (E_i = delta^2 times their sum in part order, in FP64); otherwise
+// *few_parts = 0 and esite is written.
 hipError_t launch_gap_kernel(int mode, int n_sites, int dim, int dim_pad,
                              int n_sparse, int sp_pad, const void* q,
                              const void* qs, const void* qsT,
@@ -174,7 +180,7 @@ hipError_t launch_gap_kernel(int mode, int n_sites, int dim, int dim_pad,
                              void* wmat, void* epart, double* esite, void* G,
                              int want_grad, const int* guard, hipStream_t st,
                              double* eatom = nullptr, const double* e0 = nullptr,
-                             const int* species = nullptr);
+                             const int* species = nullptr, int* few_parts = nullptr);
 // overwrite: forces[k] = gathered sum (otherwise added to forces[k])
 template <typename T>
 hipError_t launch_force_gather(int n, int maxnb, const int* count,

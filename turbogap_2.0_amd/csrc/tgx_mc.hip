@@ -141,7 +141,7 @@ __device__ double mc_phi_part(const TwoBodyDev& b, double r, int k0, int k1) {
 // lanes the neighbour slots; fixed-order sum over lanes, then waves.
 __device__ double mc_de2b(const double* pos, const int* species, const Cell& cell, int maxnb,
                           const int* count, const int* list, const TwoBodyDev* twob, int n2b,
-                          int m, const double* dctl, int lane, int w, double* red) {
+                          int m, const double* dctl, int lane, int w, double* red, bool exact) {
   double de = 0.0;
   const int cnt = min(count[m], maxnb);
   const double* pm = pos + 3 * m;
@@ -150,16 +150,26 @@ __device__ double mc_de2b(const double* pos, const int* species, const Cell& cel
     const TwoBodyDev& B = twob[b];
     const int sm = species[m];
     if (sm != B.sp1 && sm != B.sp2) continue;
+    // exact: every wave takes a quarter of the sparse distances for all
+    // pairs; otherwise (the FP32/mixed modes, as their full evaluation) the
+    // tabulated phi, the waves taking every fourth group of 64 pairs
     const int qn = (B.n_sparse + 3) / 4, k0 = min(B.n_sparse, w * qn),
               k1 = min(B.n_sparse, k0 + qn);
-    for (int s = lane; s < cnt; s += 64) {
+    for (int s = exact ? lane : lane + 64 * w; s < cnt; s += exact ? 64 : 256) {
       const int j = list[(size_t)m * maxnb + s] & TGX_ATOM_MASK;
       const int sj = species[j];
       if (!((sm == B.sp1 && sj == B.sp2) || (sm == B.sp2 && sj == B.sp1))) continue;
       double dv[3];
       const double rn = mic(pm, pos + 3 * j, cell.La, cell.Lb, cell.Lc, dv);
       const double ro = mic(po, pos + 3 * j, cell.La, cell.Lb, cell.Lc, dv);
-      de += 2.0 * (mc_phi_part(B, rn, k0, k1) - mc_phi_part(B, ro, k0, k1));
+      if (exact) {
+        de += 2.0 * (mc_phi_part(B, rn, k0, k1) - mc_phi_part(B, ro, k0, k1));
+      } else {
+        double pn, po_, d;
+        two_body_pair<false>(B, rn, &pn, &d);
+        two_body_pair<false>(B, ro, &po_, &d);
+        de += 2.0 * (pn - po_);
+      }
     }
   }
 #pragma unroll
@@ -179,15 +189,15 @@ __device__ void mc_finish_trial(int t, double* trace, int* ctl, const double* dc
 __global__ __launch_bounds__(256) void mc_stage1_kernel(
     double* pos, const int* species, Cell cell, int maxnb, const int* count, const int* list,
     const int* aff, const int* n_aff, const double* esur, const double* esur_cur,
-    const TwoBodyDev* twob, int n2b, unsigned long long seed, double kT, int* ctl, double* dctl,
-    double* trace) {
+    const TwoBodyDev* twob, int n2b, int exact, unsigned long long seed, double kT, int* ctl,
+    double* dctl, double* trace) {
   if (!ctl[MC_LIVE]) return;
   __shared__ double red[4];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int t = ctl[MC_STEP];
   const int m = ctl[MC_ATOM];
   const double de2 =
-      mc_de2b(pos, species, cell, maxnb, count, list, twob, n2b, m, dctl, lane, w, red);
+      mc_de2b(pos, species, cell, maxnb, count, list, twob, n2b, m, dctl, lane, w, red, exact);
   if (w != 0) return;
   const bool da = ctl[MC_DA] && !ctl[MC_FALLBACK];
   double ds = 0.0;
@@ -216,10 +226,21 @@ __global__ __launch_bounds__(256) void mc_stage1_kernel(
 }
 
 // Stage 2 (GAP): plain Metropolis, or the delayed-acceptance correction.
+// New site energy k of the trial: esite[k], or, from the few-site kernel's
+// FP32 partials, d2 times their sum in part order (the row-sum kernel's
+// arithmetic).
+__device__ __forceinline__ double mc_site_e(const double* esite, const float* parts, int nparts,
+                                            int part_ld, double d2, int k) {
+  if (!parts) return esite[k];
+  double s = 0.0;
+  for (int t = 0; t < nparts; t++) s += (double)parts[(size_t)t * part_ld + k];
+  return d2 * s;
+}
+
 __global__ __launch_bounds__(64) void mc_stage2_kernel(
-    double* pos, const int* aff, const int* n_aff, const double* esite, double* e_cur,
-    const double* esur, double* esur_cur, unsigned long long seed, double kT, int* ctl,
-    double* dctl, double* trace) {
+    double* pos, const int* aff, const int* n_aff, const double* esite, const float* parts,
+    int nparts, int part_ld, double d2, double* e_cur, const double* esur, double* esur_cur,
+    unsigned long long seed, double kT, int* ctl, double* dctl, double* trace) {
   if (!ctl[MC_LIVE] || !ctl[MC_GATE]) return;
   const int lane = threadIdx.x;
   const int t = ctl[MC_STEP];
@@ -227,7 +248,8 @@ __global__ __launch_bounds__(64) void mc_stage2_kernel(
   const int na = *n_aff;
   double de = 0.0;
   if (esite)
-    for (int k = lane; k < na; k += 64) de += esite[k] - e_cur[aff[k]];
+    for (int k = lane; k < na; k += 64)
+      de += mc_site_e(esite, parts, nparts, part_ld, d2, k) - e_cur[aff[k]];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) de += __shfl_xor(de, o, 64);
   const bool da = ctl[MC_DA] && !ctl[MC_FALLBACK];
@@ -257,7 +279,8 @@ __global__ __launch_bounds__(64) void mc_stage2_kernel(
   __syncthreads();
   if (acc) {
     if (esite)
-      for (int k = lane; k < na; k += 64) e_cur[aff[k]] = esite[k];
+      for (int k = lane; k < na; k += 64)
+        e_cur[aff[k]] = mc_site_e(esite, parts, nparts, part_ld, d2, k);
     if (da)
       for (int k = lane; k < na; k += 64) esur_cur[aff[k]] = esur[k];
   }
@@ -312,19 +335,20 @@ hipError_t launch_mc_affected(const int* ctl, const double* dctl, const double* 
 hipError_t launch_mc_stage1(double* pos, const int* species, const Cell& cell, int maxnb,
                             const int* count, const int* list, const int* aff, const int* n_aff,
                             const double* esur, const double* esur_cur, const TwoBodyDev* twob,
-                            int n2b, unsigned long long seed, double kT, int* ctl, double* dctl,
-                            double* trace, hipStream_t st) {
+                            int n2b, int exact, unsigned long long seed, double kT, int* ctl,
+                            double* dctl, double* trace, hipStream_t st) {
   hipLaunchKernelGGL(mc_stage1_kernel, dim3(1), dim3(256), 0, st, pos, species, cell, maxnb, count,
-                     list, aff, n_aff, esur, esur_cur, twob, n2b, seed, kT, ctl, dctl, trace);
+                     list, aff, n_aff, esur, esur_cur, twob, n2b, exact, seed, kT, ctl, dctl,
+                     trace);
   return hipGetLastError();
 }
 
 hipError_t launch_mc_stage2(double* pos, const int* aff, const int* n_aff, const double* esite,
-                            double* e_cur, const double* esur, double* esur_cur,
-                            unsigned long long seed, double kT, int* ctl, double* dctl,
-                            double* trace, hipStream_t st) {
-  hipLaunchKernelGGL(mc_stage2_kernel, dim3(1), dim3(64), 0, st, pos, aff, n_aff, esite, e_cur,
-                     esur, esur_cur, seed, kT, ctl, dctl, trace);
+                            const float* parts, int nparts, int part_ld, double d2, double* e_cur,
+                            const double* esur, double* esur_cur, unsigned long long seed,
+                            double kT, int* ctl, double* dctl, double* trace, hipStream_t st) {
+  hipLaunchKernelGGL(mc_stage2_kernel, dim3(1), dim3(64), 0, st, pos, aff, n_aff, esite, parts,
+                     nparts, part_ld, d2, e_cur, esur, esur_cur, seed, kT, ctl, dctl, trace);
   return hipGetLastError();
 }
 
