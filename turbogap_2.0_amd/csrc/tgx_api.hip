@@ -55,15 +55,16 @@ hipError_t launch_reduce_many(int n_struct, const int* aoff, const int* site_of_
 hipError_t launch_mc_propose(int n, double* pos, const double* pref, unsigned long long seed,
                              double move_max, double lim2, int* ctl, double* dctl,
                              hipStream_t st);
-hipError_t launch_mc_affected(const int* ctl, const double* dctl, const double* pos,
-                              const Cell& cell, double rsoap, const int* count, const int* list,
-                              int maxnb, const int* species, int central, int* aff, int cap,
-                              int* n_aff, hipStream_t st);
+hipError_t launch_mc_affected(int* ctl, double* dctl, const double* pos, const Cell& cell,
+                              double rsoap, const int* count, const int* list, int maxnb,
+                              const int* species, int central, int* aff, int cap, int* n_aff,
+                              const TwoBodyDev* twob, int n2b, int exact, int with_2b,
+                              int open_gate, hipStream_t st);
 hipError_t launch_mc_stage1(double* pos, const int* species, const Cell& cell, int maxnb,
                             const int* count, const int* list, const int* aff, const int* n_aff,
                             const double* esur, const double* esur_cur, const TwoBodyDev* twob,
-                            int n2b, int exact, unsigned long long seed, double kT, int* ctl,
-                            double* dctl, double* trace, hipStream_t st);
+                            int n2b, int exact, int have_2b, unsigned long long seed, double kT,
+                            int* ctl, double* dctl, double* trace, hipStream_t st);
 hipError_t launch_mc_stage2(double* pos, const int* aff, const int* n_aff, const double* esite,
                             const float* parts, int nparts, int part_ld, double d2, double* e_cur, const double* esur, double* esur_cur,
                             unsigned long long seed, double kT, int* ctl, double* dctl,
@@ -1312,11 +1313,15 @@ static int mc_trial_launch(tgx_ctx* c, const McRun& r) {
                               c->nb_list.as<int>(), nullptr, c->nb_flag.as<int>(), c->stream,
                               ctl + MC_REBUILD, x.mc_pref.as<double>()));
   const int D = c->dim;
+  const int exact = c->precision == TGX_FP64 ? 1 : 0;
   if (c->has_soap) {
+    // the affected-site scan also computes the 2-body change; for plain
+    // Metropolis that is all of stage 1, which then needs no launch
     TGX_HIP(c, launch_mc_affected(ctl, dctl, c->pos.as<double>(), r.cell, r.rsoap,
                                   c->nb_count.as<int>(), c->nb_list.as<int>(), c->maxnb,
                                   c->species.as<int>(), r.central, c->site_atom.as<int>(), r.cap,
-                                  x.mc_naff.as<int>(), c->stream));
+                                  x.mc_naff.as<int>(), x.mc_twob.as<TwoBodyDev>(), r.n2b, exact,
+                                  1, r.da ? 0 : 1, c->stream));
     int rcode = soap_forward(c, r.n, Geom{r.cell}, r.cap, true);
     if (rcode) return rcode;
     if (r.da) {
@@ -1335,12 +1340,13 @@ static int mc_trial_launch(tgx_ctx* c, const McRun& r) {
       TGX_HIP(c, e);
     }
   }
-  TGX_HIP(c, launch_mc_stage1(c->pos.as<double>(), c->species.as<int>(), r.cell, c->maxnb,
-                              c->nb_count.as<int>(), c->nb_list.as<int>(), c->site_atom.as<int>(),
-                              x.mc_naff.as<int>(), x.mc_esur.as<double>(),
-                              x.mc_esur_cur.as<double>(), x.mc_twob.as<TwoBodyDev>(), r.n2b,
-                              c->precision == TGX_FP64 ? 1 : 0, r.seed, r.kT, ctl, dctl,
-                              x.mc_trace.as<double>(), c->stream));
+  if (r.da || !c->has_soap)
+    TGX_HIP(c, launch_mc_stage1(c->pos.as<double>(), c->species.as<int>(), r.cell, c->maxnb,
+                                c->nb_count.as<int>(), c->nb_list.as<int>(),
+                                c->site_atom.as<int>(), x.mc_naff.as<int>(),
+                                x.mc_esur.as<double>(), x.mc_esur_cur.as<double>(),
+                                x.mc_twob.as<TwoBodyDev>(), r.n2b, exact, c->has_soap ? 1 : 0,
+                                r.seed, r.kT, ctl, dctl, x.mc_trace.as<double>(), c->stream));
   // without sample collection the decision kernel sums the few-site
   // kernel's partial energies itself (one launch less per trial)
   int nparts = 0;

@@ -76,10 +76,18 @@ __global__ void mc_propose_kernel(int n, double* pos, const double* pref,
 // new position (the only sites whose descriptor changes); the remaining
 // capacity repeats the moved atom (computed, never summed).  With a central
 // species only atoms of that species are sites.
-__global__ void mc_affected_kernel(const int* ctl, const double* dctl, const double* pos,
-                                   Cell cell, double rsoap, const int* count, const int* list,
-                                   int maxnb, const int* species, int central, int* aff, int cap,
-                                   int* n_aff) {
+__device__ double mc_de2b(const double* pos, const int* species, const Cell& cell, int maxnb,
+                          const int* count, const int* list, const TwoBodyDev* twob, int n2b,
+                          int m, const double* dctl, int lane, int w, double* red, bool exact);
+
+// With WITH_2B the same workgroup then computes the 2-body change of the move
+// into dctl[MC_DE2B] (and opens the GAP gate when open_gate: plain
+// Metropolis, whose stage 1 is nothing else).
+template <bool WITH_2B>
+__global__ __launch_bounds__(256) void mc_affected_kernel(
+    int* ctl, double* dctl, const double* pos, Cell cell, double rsoap, const int* count,
+    const int* list, int maxnb, const int* species, int central, int* aff, int cap, int* n_aff,
+    const TwoBodyDev* twob, int n2b, int exact, int open_gate) {
   if (!ctl[MC_LIVE]) return;
   const int m = ctl[MC_ATOM];
   const int cnt = min(count[m], maxnb);
@@ -116,6 +124,15 @@ __global__ void mc_affected_kernel(const int* ctl, const double* dctl, const dou
   }
   __syncthreads();
   for (int k = t; k < cap; k += blockDim.x) aff[k] = k < nk ? keep[k] : m;
+  if constexpr (WITH_2B) {
+    __shared__ double red[4];
+    const double de2 = mc_de2b(pos, species, cell, maxnb, count, list, twob, n2b, m, dctl,
+                               t & 63, t >> 6, red, exact != 0);
+    if (t == 0) {
+      dctl[MC_DE2B] = de2;
+      if (open_gate) ctl[MC_GATE] = 1;
+    }
+  }
 }
 
 // Exact FP64 pair energy of one distance_2b block (gap.f90:383-399),
@@ -189,15 +206,18 @@ __device__ void mc_finish_trial(int t, double* trace, int* ctl, const double* dc
 __global__ __launch_bounds__(256) void mc_stage1_kernel(
     double* pos, const int* species, Cell cell, int maxnb, const int* count, const int* list,
     const int* aff, const int* n_aff, const double* esur, const double* esur_cur,
-    const TwoBodyDev* twob, int n2b, int exact, unsigned long long seed, double kT, int* ctl,
-    double* dctl, double* trace) {
+    const TwoBodyDev* twob, int n2b, int exact, int have_2b, unsigned long long seed, double kT,
+    int* ctl, double* dctl, double* trace) {
   if (!ctl[MC_LIVE]) return;
   __shared__ double red[4];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int t = ctl[MC_STEP];
   const int m = ctl[MC_ATOM];
+  // have_2b: the affected-site kernel already computed the 2-body change
   const double de2 =
-      mc_de2b(pos, species, cell, maxnb, count, list, twob, n2b, m, dctl, lane, w, red, exact);
+      have_2b ? dctl[MC_DE2B]
+              : mc_de2b(pos, species, cell, maxnb, count, list, twob, n2b, m, dctl, lane, w, red,
+                        exact);
   if (w != 0) return;
   const bool da = ctl[MC_DA] && !ctl[MC_FALLBACK];
   double ds = 0.0;
@@ -322,24 +342,31 @@ hipError_t launch_mc_propose(int n, double* pos, const double* pref, unsigned lo
   return hipGetLastError();
 }
 
-hipError_t launch_mc_affected(const int* ctl, const double* dctl, const double* pos,
-                              const Cell& cell, double rsoap, const int* count, const int* list,
-                              int maxnb, const int* species, int central, int* aff, int cap,
-                              int* n_aff, hipStream_t st) {
+hipError_t launch_mc_affected(int* ctl, double* dctl, const double* pos, const Cell& cell,
+                              double rsoap, const int* count, const int* list, int maxnb,
+                              const int* species, int central, int* aff, int cap, int* n_aff,
+                              const TwoBodyDev* twob, int n2b, int exact, int with_2b,
+                              int open_gate, hipStream_t st) {
   if (cap > 1024) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(mc_affected_kernel, dim3(1), dim3(256), 0, st, ctl, dctl, pos, cell, rsoap,
-                     count, list, maxnb, species, central, aff, cap, n_aff);
+  if (with_2b)
+    hipLaunchKernelGGL(mc_affected_kernel<true>, dim3(1), dim3(256), 0, st, ctl, dctl, pos, cell,
+                       rsoap, count, list, maxnb, species, central, aff, cap, n_aff, twob, n2b,
+                       exact, open_gate);
+  else
+    hipLaunchKernelGGL(mc_affected_kernel<false>, dim3(1), dim3(256), 0, st, ctl, dctl, pos, cell,
+                       rsoap, count, list, maxnb, species, central, aff, cap, n_aff, twob, n2b,
+                       exact, open_gate);
   return hipGetLastError();
 }
 
 hipError_t launch_mc_stage1(double* pos, const int* species, const Cell& cell, int maxnb,
                             const int* count, const int* list, const int* aff, const int* n_aff,
                             const double* esur, const double* esur_cur, const TwoBodyDev* twob,
-                            int n2b, int exact, unsigned long long seed, double kT, int* ctl,
-                            double* dctl, double* trace, hipStream_t st) {
+                            int n2b, int exact, int have_2b, unsigned long long seed, double kT,
+                            int* ctl, double* dctl, double* trace, hipStream_t st) {
   hipLaunchKernelGGL(mc_stage1_kernel, dim3(1), dim3(256), 0, st, pos, species, cell, maxnb, count,
-                     list, aff, n_aff, esur, esur_cur, twob, n2b, exact, seed, kT, ctl, dctl,
-                     trace);
+                     list, aff, n_aff, esur, esur_cur, twob, n2b, exact, have_2b, seed, kT, ctl,
+                     dctl, trace);
   return hipGetLastError();
 }
 
