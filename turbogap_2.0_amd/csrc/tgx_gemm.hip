@@ -352,8 +352,12 @@ __global__ __launch_bounds__(512) void gap_fused_kernel(
   const int c1 = nq * (N1 >> 2);
   {
     const int stage_rows = 128 + N1, nchunk = stage_rows * 4;
-    uint4 pf[TGX_FUSED_PF];
-    auto fetch = [&](int k0) {
+    // two slabs in flight in registers (pa, pb) besides the one in LDS: the
+    // q rows come from HBM and one slab of products does not cover that
+    // latency; the loop runs in pairs of slabs so that the buffers keep
+    // static names
+    uint4 pa[TGX_FUSED_PF], pb[TGX_FUSED_PF];
+    auto fetch = [&](uint4 (&pf)[TGX_FUSED_PF], int k0) {
 #pragma unroll
       for (int p = 0; p < TGX_FUSED_PF; p++) {
         const int ch = tid + p * 512;
@@ -370,7 +374,7 @@ __global__ __launch_bounds__(512) void gap_fused_kernel(
         pf[p] = v;
       }
     };
-    auto stash = [&](int buf) {
+    auto stash = [&](const uint4 (&pf)[TGX_FUSED_PF], int buf) {
       u16* st = sW + (size_t)buf * stage_rows * TGX_FUSED_SLD;
 #pragma unroll
       for (int p = 0; p < TGX_FUSED_PF; p++) {
@@ -379,13 +383,8 @@ __global__ __launch_bounds__(512) void gap_fused_kernel(
           *reinterpret_cast<uint4*>(st + (size_t)(ch >> 2) * TGX_FUSED_SLD + (ch & 3) * 8) = pf[p];
       }
     };
-    const int nslab = K1 >> 5;
-    fetch(0);
-    stash(0);
-    __syncthreads();
-    for (int sl = 0; sl < nslab; sl++) {
-      if (sl + 1 < nslab) fetch((sl + 1) * 32);
-      const u16* st = sW + (size_t)(sl & 1) * stage_rows * TGX_FUSED_SLD;
+    auto mma_slab = [&](int buf) {
+      const u16* st = sW + (size_t)buf * stage_rows * TGX_FUSED_SLD;
       bf16x8 af[4];
 #pragma unroll
       for (int mi = 0; mi < 4; mi++)
@@ -401,7 +400,22 @@ __global__ __launch_bounds__(512) void gap_fused_kernel(
             acc[mi][ni] =
                 __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr, af[mi], acc[mi][ni], 0, 0, 0);
         }
-      if (sl + 1 < nslab) stash((sl + 1) & 1);
+    };
+    const int nslab = K1 >> 5;
+    fetch(pa, 0);
+    stash(pa, 0);
+    if (nslab > 1) fetch(pb, 32);
+    __syncthreads();
+    // slab sl lives in LDS buffer sl & 1
+    for (int sl = 0; sl < nslab; sl += 2) {
+      if (sl + 2 < nslab) fetch(pa, (sl + 2) * 32);
+      mma_slab(0);
+      if (sl + 1 < nslab) stash(pb, 1);
+      __syncthreads();
+      if (sl + 1 >= nslab) break;
+      if (sl + 3 < nslab) fetch(pb, (sl + 3) * 32);
+      mma_slab(1);
+      if (sl + 2 < nslab) stash(pa, 0);
       __syncthreads();
     }
   }
@@ -466,47 +480,104 @@ __global__ __launch_bounds__(512) void gap_fused_kernel(
   {
     // K1 = 64 NT2 rows of four 16-byte chunks: NT2 / 2 chunks per thread
     constexpr int nchunk = 256 * NT2, PF2 = (NT2 + 1) / 2;
-    uint4 pf[PF2];
-#pragma unroll
-    for (int p = 0; p < PF2; p++) pf[p] = uint4{0, 0, 0, 0};
-    auto fetch = [&](int k0) {
-#pragma unroll
-      for (int p = 0; p < PF2; p++) {
-        const int ch = tid + p * 512;
-        if (ch < nchunk)
-          pf[p] = *reinterpret_cast<const uint4*>(qsT + (size_t)(ch >> 2) * ldqt + k0 +
-                                                  (ch & 3) * 8);
-      }
-    };
-    auto stash = [&]() {
-#pragma unroll
-      for (int p = 0; p < PF2; p++) {
-        const int ch = tid + p * 512;
-        const int r = ch >> 2;
-        if (ch < nchunk)
-          *reinterpret_cast<uint4*>(sT + (size_t)r * 32 + (((ch & 3) ^ ((r >> 2) & 3)) * 8)) =
-              pf[p];
-      }
-    };
-    fetch(0);
-    for (int k0 = 0; k0 < N1; k0 += 32) {
-      __syncthreads();  // the slab (and sE) is no longer read
-      stash();
-      __syncthreads();
-      if (k0 + 32 < N1) fetch(k0 + 32);
-      bf16x8 aw[4];
-#pragma unroll
-      for (int mi = 0; mi < 4; mi++)
-        aw[mi] = *reinterpret_cast<const bf16x8*>(sW + (size_t)(mh * 64 + mi * 16 + lr) * ldw +
-                                                  k0 + lk);
-#pragma unroll
-      for (int nj = 0; nj < NT2; nj++) {
-        const int r = c2 + nj * 16 + lr;
-        const bf16x8 bq = *reinterpret_cast<const bf16x8*>(
-            sT + (size_t)r * 32 + (((lane >> 4) ^ ((r >> 2) & 3)) * 8));
-#pragma unroll
+    if constexpr (NT2 <= 6) {
+      // two slabs in flight in registers besides the one in LDS, as in stage 1
+      uint4 pa[PF2], pb[PF2];
+  #pragma unroll
+      for (int p = 0; p < PF2; p++) pa[p] = pb[p] = uint4{0, 0, 0, 0};
+      auto fetch = [&](uint4 (&pf)[PF2], int k0) {
+  #pragma unroll
+        for (int p = 0; p < PF2; p++) {
+          const int ch = tid + p * 512;
+          if (ch < nchunk)
+            pf[p] = *reinterpret_cast<const uint4*>(qsT + (size_t)(ch >> 2) * ldqt + k0 +
+                                                    (ch & 3) * 8);
+        }
+      };
+      auto stash = [&](const uint4 (&pf)[PF2]) {
+  #pragma unroll
+        for (int p = 0; p < PF2; p++) {
+          const int ch = tid + p * 512;
+          const int r = ch >> 2;
+          if (ch < nchunk)
+            *reinterpret_cast<uint4*>(sT + (size_t)r * 32 + (((ch & 3) ^ ((r >> 2) & 3)) * 8)) =
+                pf[p];
+        }
+      };
+      auto mma_slab = [&](int k0) {
+        bf16x8 aw[4];
+  #pragma unroll
         for (int mi = 0; mi < 4; mi++)
-          acc2[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bq, aw[mi], acc2[mi][nj], 0, 0, 0);
+          aw[mi] = *reinterpret_cast<const bf16x8*>(sW + (size_t)(mh * 64 + mi * 16 + lr) * ldw +
+                                                    k0 + lk);
+  #pragma unroll
+        for (int nj = 0; nj < NT2; nj++) {
+          const int r = c2 + nj * 16 + lr;
+          const bf16x8 bq = *reinterpret_cast<const bf16x8*>(
+              sT + (size_t)r * 32 + (((lane >> 4) ^ ((r >> 2) & 3)) * 8));
+  #pragma unroll
+          for (int mi = 0; mi < 4; mi++)
+            acc2[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bq, aw[mi], acc2[mi][nj], 0, 0, 0);
+        }
+      };
+      fetch(pa, 0);
+      if (32 < N1) fetch(pb, 32);
+      for (int k0 = 0; k0 < N1; k0 += 64) {
+        __syncthreads();  // the slab (and sE) is no longer read
+        stash(pa);
+        __syncthreads();
+        if (k0 + 64 < N1) fetch(pa, k0 + 64);
+        mma_slab(k0);
+        if (k0 + 32 >= N1) break;
+        __syncthreads();
+        stash(pb);
+        __syncthreads();
+        if (k0 + 96 < N1) fetch(pb, k0 + 96);
+        mma_slab(k0 + 32);
+      }
+    } else {  // wider G tiles: no registers left for a second slab
+      uint4 pf[PF2];
+  #pragma unroll
+      for (int p = 0; p < PF2; p++) pf[p] = uint4{0, 0, 0, 0};
+      auto fetch = [&](int k0) {
+  #pragma unroll
+        for (int p = 0; p < PF2; p++) {
+          const int ch = tid + p * 512;
+          if (ch < nchunk)
+            pf[p] = *reinterpret_cast<const uint4*>(qsT + (size_t)(ch >> 2) * ldqt + k0 +
+                                                    (ch & 3) * 8);
+        }
+      };
+      auto stash = [&]() {
+  #pragma unroll
+        for (int p = 0; p < PF2; p++) {
+          const int ch = tid + p * 512;
+          const int r = ch >> 2;
+          if (ch < nchunk)
+            *reinterpret_cast<uint4*>(sT + (size_t)r * 32 + (((ch & 3) ^ ((r >> 2) & 3)) * 8)) =
+                pf[p];
+        }
+      };
+      fetch(0);
+      for (int k0 = 0; k0 < N1; k0 += 32) {
+        __syncthreads();  // the slab (and sE) is no longer read
+        stash();
+        __syncthreads();
+        if (k0 + 32 < N1) fetch(k0 + 32);
+        bf16x8 aw[4];
+  #pragma unroll
+        for (int mi = 0; mi < 4; mi++)
+          aw[mi] = *reinterpret_cast<const bf16x8*>(sW + (size_t)(mh * 64 + mi * 16 + lr) * ldw +
+                                                    k0 + lk);
+  #pragma unroll
+        for (int nj = 0; nj < NT2; nj++) {
+          const int r = c2 + nj * 16 + lr;
+          const bf16x8 bq = *reinterpret_cast<const bf16x8*>(
+              sT + (size_t)r * 32 + (((lane >> 4) ^ ((r >> 2) & 3)) * 8));
+  #pragma unroll
+          for (int mi = 0; mi < 4; mi++)
+            acc2[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bq, aw[mi], acc2[mi][nj], 0, 0, 0);
+        }
       }
     }
   }
