@@ -286,17 +286,25 @@ __global__ void esum_kernel(int M, int ntiles, const TA* epart, double d2,
 // waves: stage 1 splits the rows in two halves and the sparse points in
 // four quarters, stage 2 the rows in two and the output columns in four.
 // Every operand reaches LDS through coalesced 16-byte loads (whole rows of
-// a 32-deep k-slab), the next slab travelling in registers while the
+// a 32-deep k-slab), the next slabs travelling in registers while the
 // current one is multiplied: stage 1 double-buffers q and Qs slabs where W
-// goes afterwards (rows of 80 bytes), stage 2 one Qs slab (rows of 64
-// bytes, 16-byte segments XOR-swizzled by row) next to W (rows padded by 16
-// bytes); all fragment reads are free of bank conflicts.  Both stages
+// goes afterwards, stage 2 keeps one Qs slab next to W (rows padded by 16
+// bytes); slab rows are 64 bytes with their 16-byte segments XOR-swizzled
+// by row, so that the staging stores and the fragment reads are free of
+// bank conflicts.  Both stages
 // multiply with the Qs operand first (D = Qs-tile x q-tile), so that a lane
 // owns four consecutive columns of one row: W goes to LDS as 8-byte stores
 // and G to memory as float4.  Integer zeta up to 4 has its own epilogue
 // (powers by multiplication).
 #define TGX_FUSED_MAXT 8
-#define TGX_FUSED_SLD 40  // stage-1 staged row: 32 bf16 + 8 pad
+#define TGX_FUSED_SLD 32  // stage-1 staged row: 32 bf16 (16-byte segments XOR-swizzled)
+// Offset (in bf16) of 16-byte segment seg of staged row r: segments are
+// XOR-swizzled by (r >> 2) & 3, so that both the coalesced stores (four rows
+// of four segments per 16 lanes) and the fragment reads (sixteen rows of one
+// segment) touch 64 distinct banks.
+__device__ __forceinline__ int slab_off(int r, int seg) {
+  return r * TGX_FUSED_SLD + ((seg ^ ((r >> 2) & 3)) << 3);
+}
 #define TGX_FUSED_PF 5    // stage-1 16-byte chunks per thread: (128 + 512) * 4 / 512
 
 template <int ZI>
@@ -380,7 +388,7 @@ __global__ __launch_bounds__(512) void gap_fused_kernel(
       for (int p = 0; p < TGX_FUSED_PF; p++) {
         const int ch = tid + p * 512;
         if (ch < nchunk)
-          *reinterpret_cast<uint4*>(st + (size_t)(ch >> 2) * TGX_FUSED_SLD + (ch & 3) * 8) = pf[p];
+          *reinterpret_cast<uint4*>(st + slab_off(ch >> 2, ch & 3)) = pf[p];
       }
     };
     auto mma_slab = [&](int buf) {
@@ -389,12 +397,12 @@ __global__ __launch_bounds__(512) void gap_fused_kernel(
 #pragma unroll
       for (int mi = 0; mi < 4; mi++)
         af[mi] = *reinterpret_cast<const bf16x8*>(
-            st + (size_t)(mh * 64 + mi * 16 + lr) * TGX_FUSED_SLD + lk);
+            st + slab_off(mh * 64 + mi * 16 + lr, lane >> 4));
 #pragma unroll
       for (int ni = 0; ni < TGX_FUSED_MAXT; ni++)
         if (ni < nt1) {
           const bf16x8 bfr = *reinterpret_cast<const bf16x8*>(
-              st + (size_t)(128 + c1 + ni * 16 + lr) * TGX_FUSED_SLD + lk);
+              st + slab_off(128 + c1 + ni * 16 + lr, lane >> 4));
 #pragma unroll
           for (int mi = 0; mi < 4; mi++)
             acc[mi][ni] =
@@ -656,7 +664,7 @@ __global__ __launch_bounds__(512) void gap_fused_chunked_kernel(
         for (int p = 0; p < TGX_FUSED_PFC; p++) {
           const int ch = tid + p * 512;
           if (ch < nchunk)
-            *reinterpret_cast<uint4*>(st + (size_t)(ch >> 2) * TGX_FUSED_SLD + (ch & 3) * 8) =
+            *reinterpret_cast<uint4*>(st + slab_off(ch >> 2, ch & 3)) =
                 pf[p];
         }
       };
@@ -671,12 +679,12 @@ __global__ __launch_bounds__(512) void gap_fused_chunked_kernel(
 #pragma unroll
         for (int mi = 0; mi < 4; mi++)
           af[mi] = *reinterpret_cast<const bf16x8*>(
-              st + (size_t)(mh * 64 + mi * 16 + lr) * TGX_FUSED_SLD + lk);
+              st + slab_off(mh * 64 + mi * 16 + lr, lane >> 4));
 #pragma unroll
         for (int ni = 0; ni < T1; ni++)
           if (ni < nt1) {
             const bf16x8 bfr = *reinterpret_cast<const bf16x8*>(
-                st + (size_t)(128 + (N1 >> 2) * nq + ni * 16 + lr) * TGX_FUSED_SLD + lk);
+                st + slab_off(128 + (N1 >> 2) * nq + ni * 16 + lr, lane >> 4));
 #pragma unroll
             for (int mi = 0; mi < 4; mi++)
               acc[mi][ni] =
