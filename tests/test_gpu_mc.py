@@ -69,6 +69,27 @@ def test_mc_mixed_precision(eng):
     assert abs(e.sum() - g["e_trace"][-1]) < 5e-3 * len(pos)
 
 
+def test_mc_mixed_partial_energies_match_row_sums(eng):
+    """Mixed precision: a plain trial's decision kernel adds the few-site
+    energy kernel's FP32 partials itself; with sample collection on (the
+    training phase of delayed acceptance, same random streams) the row-sum
+    kernel adds them first.  The same arithmetic in the same order, so the
+    two chains agree bit for bit."""
+    pos, sp, lat, model = _system()
+    eng.set_precision("mixed")
+    try:
+        a = eng.mc_run(pos, sp, lat, 300, 1500.0, 0.6, 1.2, 41)
+        eng.surrogate_init(model.hypers.dim, seed=4)
+        b = eng.mc_run_da(pos, sp, lat, 300, 1500.0, 0.6, 1.2, 41, n_train_trials=300,
+                          train_every=1 << 20, epochs=1, batch=64, lr=1e-3, max_rmse=1.0)
+    finally:
+        eng.set_precision("fp64")
+    print("accepted", a["n_accepted"], b["n_accepted"], flush=True)
+    assert a["n_accepted"] == b["n_accepted"] and a["n_accepted"] > 50
+    assert np.array_equal(a["pos"], b["pos"])
+    assert np.array_equal(a["e_trace"], b["e_trace"])
+
+
 def test_mc_rejects_bad_arguments(eng):
     pos, sp, lat, _ = _system()
     with pytest.raises(tgx.TgxError):  # skin too small for the move
