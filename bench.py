@@ -18,6 +18,7 @@ import argparse
 import json
 import os
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -64,6 +65,18 @@ def copy_probe(mbytes):
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n,
                       "launches": 3}))
+
+
+def read_tune(path):
+    """The autotuner's decisions of this run (shape key -> index of the
+    chosen candidate launch shape), and the file removed."""
+    try:
+        with open(path) as fh:
+            dec = dict(ln.rsplit(" ", 1) for ln in fh.read().splitlines() if " " in ln)
+        os.remove(path)
+    except OSError:
+        return None
+    return {k: int(v) for k, v in dec.items()}
 
 
 def build_model(eng):
@@ -249,6 +262,8 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp32", "fp64"])
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="md32k: built-in launch shapes instead of the autotuner's")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="only run a plain device copy of MB MiB (PMC calibration)")
     ap.add_argument("--workload", default="md32k", choices=["md32k", "mc512", "batch256",
@@ -258,7 +273,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
+              "(a run over several GPUs is not part of this benchmark).", file=sys.stderr)
         sys.exit(2)
     if args.workload == "mc512":
         bench_mc512(args)
@@ -281,6 +296,11 @@ def main():
     eng = tgx.Engine(0, "fp64")
     model = build_model(eng)
     eng.set_model(model)
+    # launch shapes from the library's autotuner (timed on the first, eager
+    # evaluation of each shape, before the MD step graph is captured)
+    tune_file = os.path.join(tempfile.gettempdir(), f"tgx_autotune_{os.getpid()}.txt")
+    if not args.no_autotune:
+        eng.set_autotune(True, tune_file)
 
     def run_md(prec, steps, warmup):
         eng.set_precision(prec)
@@ -396,6 +416,7 @@ def main():
         "host_buffer_atom_evals_per_s": round(host_rate, 1),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "autotune": read_tune(tune_file),
     }
     print(json.dumps(line))
     eng.close()

@@ -91,6 +91,18 @@ int tgx_destroy(tgx_ctx* ctx);
 const char* tgx_last_error(const tgx_ctx* ctx);
 int tgx_set_precision(tgx_ctx* ctx, int precision);
 
+/* Autotuner (no counterpart in the reference, whose launch shapes are fixed
+ * in gpu_interface.f90; it implements the per-shape tile choice of the
+ * engine): on = 1 times the candidate launch shapes of the kernel step
+ * (single kernel or separate GEMMs, GEMM tile shapes) and of the SOAP
+ * kernels (waves per workgroup) on the first call of each shape and keeps
+ * the fastest; on = 0 uses stored decisions or the built-in shapes.
+ * cache_path (may be NULL): decisions are read from this text file now and
+ * appended to it as they are made.  TGX_AUTOTUNE=1 in the environment turns
+ * tuning on for contexts that never call this.  Decisions are never made
+ * inside graph capture or for guarded (Monte Carlo) steps. */
+int tgx_set_autotune(tgx_ctx* ctx, int on, const char* cache_path);
+
 /* Models ------------------------------------------------------------------- */
 
 /* One soap_turbo GAP block: hypers + sparse set Qs(dim,n_sparse) + alphas,

@@ -7,7 +7,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "tgx_internal.h"
@@ -31,7 +33,7 @@ hipError_t launch_soap_fwd(const SoapDev& sdh, const SoapDev* sd, const double* 
                            const Geom& g, int maxnb, const int* count, const int* list,
                            const int* site_atom, int n_sites, const int* dmap, T* q, int q_ld,
                            unsigned short* qb, int qb_ld, T* cbuf, T* pnorm, hipStream_t st,
-                           int split = 0);
+                           int split = 0, int wpb = 0);
 template <typename T>
 hipError_t launch_soap_bwd(const SoapDev& sdh, const SoapDev* sd, const double* W,
                            const float* rtab, const double* pos, const int* species,
@@ -39,7 +41,7 @@ hipError_t launch_soap_bwd(const SoapDev& sdh, const SoapDev* sd, const double* 
                            const int* site_atom, int n_sites, const int* dmap, const T* q,
                            int q_ld, const T* cbuf, const T* pnorm, const T* G, int g_ld,
                            T* gpair, double* vsite, const TwoBodyDev* twob, int n2b,
-                           double* eatom, hipStream_t st);
+                           double* eatom, hipStream_t st, int wpb = 0);
 hipError_t launch_two_body(const TwoBodyDev& b, const double* pos, const int* species, int n,
                            const Geom& g, int maxnb, const int* count, const int* list,
                            double* eatom, double* forces, double* vatom, int do_forces,
@@ -177,6 +179,13 @@ struct Extra {
   bool sg_norm_set = false;
   DevBuf twob_store;
   std::vector<int> species_host;
+  // autotuner: on (-1 = not yet read from TGX_AUTOTUNE), cache file, the
+  // decisions by shape key, and a copy of the atom energies (the adjoint
+  // with the fused 2-body term adds to them, so its candidates restore them)
+  int tune_on = -1;
+  std::string tune_path;
+  std::map<std::string, int> tune_choice;
+  DevBuf tune_save;
   hipEvent_t ev[8];
   hipEvent_t kev[8];  // brackets of single kernels: fwd, GEMMs, adjoint, gather
   bool ev_ready = false;
@@ -275,6 +284,96 @@ static int setup_sites(tgx_ctx* c, int n, const int* species_host, int* n_sites)
 
 static void kev_rec(tgx_ctx* c, int k);
 
+// ---- autotuner -------------------------------------------------------------
+// Launch shapes with more than one good choice (the kernel-step path and its
+// GEMM tiles, the SOAP kernels' waves per workgroup) are picked by timing the
+// candidates on the call's own buffers, once per shape key, when tuning is
+// on (tgx_set_autotune, or TGX_AUTOTUNE=1).  Decisions are kept per context
+// and, with a cache file, appended to it and read back by later contexts.
+// With tuning off the stored decisions or else the built-in choices apply,
+// so that results never depend on timing noise.
+static bool tune_enabled(tgx_ctx* c) {
+  Extra& x = X(c);
+  if (x.tune_on < 0) {
+    const char* e = getenv("TGX_AUTOTUNE");
+    x.tune_on = (e && e[0] == '1') ? 1 : 0;
+  }
+  return x.tune_on == 1;
+}
+
+static void tune_load(Extra& x) {
+  if (x.tune_path.empty()) return;
+  FILE* f = fopen(x.tune_path.c_str(), "r");
+  if (!f) return;
+  char line[512];
+  while (fgets(line, sizeof line, f)) {
+    char* sp = strrchr(line, ' ');
+    if (!sp) continue;
+    *sp = 0;
+    x.tune_choice[line] = atoi(sp + 1);
+  }
+  fclose(f);
+}
+
+static std::string bucket_key(const char* kind, int a, int b, int c2, int d, int n) {
+  int m = 64;
+  while (m < n) m <<= 1;
+  char buf[160];
+  snprintf(buf, sizeof buf, "%s/%d/%d/%d/%d/%d", kind, a, b, c2, d, m);
+  return buf;
+}
+
+// The decision for key among n candidates: the stored one; else, when tuning
+// is on and the stream is not being captured, the fastest candidate (one
+// warm-up launch, then three timed with events; launch(k) enqueues candidate
+// k and must give the same results when repeated); else dflt.  The caller
+// launches the decision afterwards, so its results stand.
+template <typename F>
+static int tune_pick(tgx_ctx* c, const std::string& key, int n, int dflt, F&& launch) {
+  Extra& x = X(c);
+  auto it = x.tune_choice.find(key);
+  if (it != x.tune_choice.end() && it->second >= 0 && it->second < n) return it->second;
+  if (n < 2 || !tune_enabled(c)) return dflt;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(c->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+    return dflt;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return dflt;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return dflt;
+  }
+  int best = dflt;
+  float best_ms = 1e30f;
+  for (int k = 0; k < n; k++) {
+    bool ok = launch(k) == hipSuccess;
+    ok = ok && hipEventRecord(e0, c->stream) == hipSuccess;
+    for (int r = 0; r < 3 && ok; r++) ok = launch(k) == hipSuccess;
+    ok = ok && hipEventRecord(e1, c->stream) == hipSuccess &&
+         hipEventSynchronize(e1) == hipSuccess;
+    float ms = 0.f;
+    if (ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < best_ms) {
+      best_ms = ms;
+      best = k;
+    }
+    if (!ok) (void)hipGetLastError();  // a candidate that does not launch is skipped
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  x.tune_choice[key] = best;
+  if (!x.tune_path.empty()) {
+    FILE* f = fopen(x.tune_path.c_str(), "a");
+    if (f) {
+      fprintf(f, "%s %d\n", key.c_str(), best);
+      fclose(f);
+    }
+  }
+  return best;
+}
+
+// Waves per workgroup of the SOAP kernels: 0 = built-in (as many as fit).
+static const int kTuneWpb[4] = {0, 1, 2, 4};
+
 // SOAP forward for the current list into c->q (and c->qb in mixed mode).
 // few: a few sites (a Monte Carlo trial) take the workgroup-per-site FP32
 // kernel; TGX_FWD_SPLIT=1 asks for it on every FP32 call of up to 256 sites.
@@ -293,23 +392,32 @@ static int soap_forward(tgx_ctx* c, int n, const Geom& cell, int n_sites, bool f
   }
   const char* fs = getenv("TGX_FWD_SPLIT");
   const int split = (few || (fs && fs[0] == '1')) ? 1 : 0;
-  hipError_t e;
+  auto fwd = [&](int wpb) -> hipError_t {
+    if (prec == TGX_FP64)
+      return launch_soap_fwd<double>(c->sd, x.sd_dev.as<SoapDev>(), c->W_dev.as<double>(),
+                                     c->rtab.as<float>(), c->pos.as<double>(),
+                                     c->species.as<int>(), cell, c->maxnb,
+                                     c->nb_count.as<int>(), c->nb_list.as<int>(),
+                                     c->site_atom.as<int>(), n_sites, x.pairn.as<int>(),
+                                     c->q.as<double>(), c->dim_pad, nullptr, 0,
+                                     c->cbuf.as<double>(), c->pnorm.as<double>(), c->stream, 0,
+                                     wpb);
+    return launch_soap_fwd<float>(c->sd, x.sd_dev.as<SoapDev>(), c->W_dev.as<double>(),
+                                  c->rtab.as<float>(), c->pos.as<double>(), c->species.as<int>(),
+                                  cell, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
+                                  c->site_atom.as<int>(), n_sites, x.pairn.as<int>(),
+                                  c->q.as<float>(), c->dim_pad, qb, c->dim_pad,
+                                  c->cbuf.as<float>(), c->pnorm.as<float>(), c->stream, split,
+                                  wpb);
+  };
+  int wpb = 0;
+  if (!split) {
+    const std::string key =
+        bucket_key("fwd", prec, c->sd.nrp, c->sd.L2, cell.many ? 1 : 0, n_sites);
+    wpb = kTuneWpb[tune_pick(c, key, 4, 0, [&](int k) { return fwd(kTuneWpb[k]); })];
+  }
   kev_rec(c, 0);
-  if (prec == TGX_FP64)
-    e = launch_soap_fwd<double>(c->sd, x.sd_dev.as<SoapDev>(), c->W_dev.as<double>(),
-                                c->rtab.as<float>(), c->pos.as<double>(), c->species.as<int>(),
-                                cell, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
-                                c->site_atom.as<int>(), n_sites, x.pairn.as<int>(),
-                                c->q.as<double>(), c->dim_pad, nullptr, 0, c->cbuf.as<double>(),
-                                c->pnorm.as<double>(), c->stream);
-  else
-    e = launch_soap_fwd<float>(c->sd, x.sd_dev.as<SoapDev>(), c->W_dev.as<double>(),
-                               c->rtab.as<float>(), c->pos.as<double>(), c->species.as<int>(),
-                               cell, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
-                               c->site_atom.as<int>(), n_sites, x.pairn.as<int>(),
-                               c->q.as<float>(), c->dim_pad, qb, c->dim_pad, c->cbuf.as<float>(),
-                               c->pnorm.as<float>(), c->stream, split);
-  TGX_HIP(c, e);
+  TGX_HIP(c, fwd(wpb));
   kev_rec(c, 1);
   (void)n;
   return TGX_OK;
@@ -336,13 +444,29 @@ static int kernel_step(tgx_ctx* c, int n_sites, bool want_grad, const int* guard
   } else {
     qa = c->qb.p; qs = c->Qs16.p; qsT = c->QsT16.p; al = c->alph32.p;
   }
+  auto step = [&](const GemmPlan* plan) {
+    return launch_gap_kernel(prec, n_sites, c->dim, c->dim_pad, c->n_sparse, c->sp_pad, qa, qs,
+                             qsT, al, c->delta, c->zeta, c->wmat.p, c->epart.p,
+                             c->esite.as<double>(), want_grad ? c->gmat.p : nullptr,
+                             want_grad ? 1 : 0, guard, c->stream, eatom_out,
+                             eatom_out ? X(c).e0_dev.as<double>() : nullptr,
+                             eatom_out ? c->species.as<int>() : nullptr, few_parts, plan);
+  };
+  // candidates: the built-in path, then the separate GEMMs with each tile
+  GemmPlan plans[5] = {{0, 0, 0}};
+  int np = 1;
+  if (prec == TGX_MIXED) plans[np++] = GemmPlan{2, 0, 0};
+  plans[np++] = GemmPlan{2, 1, 0};
+  plans[np++] = GemmPlan{2, 2, 0};
+  if (want_grad && prec != TGX_FP64) plans[np++] = GemmPlan{2, 0, 1};
+  int pk = 0;
+  if (!guard) {  // a guarded (Monte Carlo) step may not run: nothing to time
+    const std::string key =
+        bucket_key("step", prec, want_grad ? 1 : 0, c->sp_pad, c->dim_pad, n_sites);
+    pk = tune_pick(c, key, np, 0, [&](int k) { return step(&plans[k]); });
+  }
   kev_rec(c, 2);
-  TGX_HIP(c, launch_gap_kernel(prec, n_sites, c->dim, c->dim_pad, c->n_sparse, c->sp_pad, qa, qs,
-                               qsT, al, c->delta, c->zeta, c->wmat.p, c->epart.p,
-                               c->esite.as<double>(), want_grad ? c->gmat.p : nullptr,
-                               want_grad ? 1 : 0, guard, c->stream, eatom_out,
-                               eatom_out ? X(c).e0_dev.as<double>() : nullptr,
-                               eatom_out ? c->species.as<int>() : nullptr, few_parts));
+  TGX_HIP(c, step(&plans[pk]));
   kev_rec(c, 3);
   return TGX_OK;
 }
@@ -438,44 +562,68 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
   if (do_forces && c->has_soap && n_sites > 0) {
     const size_t ts = tsize(prec);
     ENSURE(c, gpair, ts * 4 * (size_t)n * c->maxnb);  // (g, 0) per list slot
-    hipError_t e;
+    auto bwd = [&](int wpb) -> hipError_t {
+      if (prec == TGX_FP64)
+        return launch_soap_bwd<double>(
+            c->sd, x.sd_dev.as<SoapDev>(), c->W_dev.as<double>(), c->rtab.as<float>(),
+            c->pos.as<double>(), c->species.as<int>(), cell, c->maxnb, c->nb_count.as<int>(),
+            c->nb_list.as<int>(), c->site_atom.as<int>(), n_sites, x.pairn.as<int>(),
+            c->q.as<double>(), c->dim_pad, c->cbuf.as<double>(), c->pnorm.as<double>(),
+            c->gmat.as<double>(), c->dim_pad, c->gpair.as<double>(), c->vsite.as<double>(),
+            fuse ? x.twob_store.as<TwoBodyDev>() : nullptr, fuse ? (int)c->twob.size() : 0,
+            c->eatom.as<double>(), c->stream, wpb);
+      return launch_soap_bwd<float>(
+          c->sd, x.sd_dev.as<SoapDev>(), c->W_dev.as<double>(), c->rtab.as<float>(),
+          c->pos.as<double>(), c->species.as<int>(), cell, c->maxnb, c->nb_count.as<int>(),
+          c->nb_list.as<int>(), c->site_atom.as<int>(), n_sites, x.pairn.as<int>(),
+          c->q.as<float>(), c->dim_pad, c->cbuf.as<float>(), c->pnorm.as<float>(),
+          c->gmat.as<float>(), c->dim_pad, c->gpair.as<float>(), c->vsite.as<double>(),
+          fuse ? x.twob_store.as<TwoBodyDev>() : nullptr, fuse ? (int)c->twob.size() : 0,
+          c->eatom.as<double>(), c->stream, wpb);
+    };
+    // the fused 2-body term adds to the atom energies: while candidates are
+    // timed, every launch starts from a copy of them
+    const std::string bkey = bucket_key("bwd", prec, c->sd.nrp, c->sd.L2,
+                                        (fuse ? 2 : 0) + (cell.many ? 1 : 0), n_sites);
+    bool snap = false;
+    if (fuse && tune_enabled(c) && !x.tune_choice.count(bkey)) {
+      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+      if (hipStreamIsCapturing(c->stream, &cs) == hipSuccess &&
+          cs == hipStreamCaptureStatusNone) {
+        if (x.tune_save.ensure(8 * (size_t)n))
+          return set_err(c, TGX_ERR_HIP, "allocation failed");
+        TGX_HIP(c, hipMemcpyAsync(x.tune_save.p, c->eatom.p, 8 * (size_t)n,
+                                  hipMemcpyDeviceToDevice, c->stream));
+        snap = true;
+      }
+    }
+    auto restore = [&]() -> hipError_t {
+      return snap ? hipMemcpyAsync(c->eatom.p, x.tune_save.p, 8 * (size_t)n,
+                                   hipMemcpyDeviceToDevice, c->stream)
+                  : hipSuccess;
+    };
+    const int wk = tune_pick(c, bkey, 4, 0, [&](int k) {
+      const hipError_t r = restore();
+      return r == hipSuccess ? bwd(kTuneWpb[k]) : r;
+    });
+    TGX_HIP(c, restore());
     kev_rec(c, 4);
-    if (prec == TGX_FP64) {
-      e = launch_soap_bwd<double>(c->sd, x.sd_dev.as<SoapDev>(), c->W_dev.as<double>(),
-                                  c->rtab.as<float>(), c->pos.as<double>(), c->species.as<int>(),
-                                  cell, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
-                                  c->site_atom.as<int>(), n_sites, x.pairn.as<int>(),
-                                  c->q.as<double>(), c->dim_pad, c->cbuf.as<double>(),
-                                  c->pnorm.as<double>(), c->gmat.as<double>(), c->dim_pad,
-                                  c->gpair.as<double>(), c->vsite.as<double>(),
-                                  fuse ? x.twob_store.as<TwoBodyDev>() : nullptr,
-                                  fuse ? (int)c->twob.size() : 0, c->eatom.as<double>(),
-                                  c->stream);
-      kev_rec(c, 5);
-      kev_rec(c, 6);
-      if (e == hipSuccess)
-        e = launch_force_gather<double>(n, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
-                                        c->nb_rev.as<int>(), c->site_of_atom.as<int>(), n_sites == n ? 1 : 0,
-                                        c->gpair.as<double>(), c->forces.as<double>(), c->stream,
-                                        fuse ? 1 : 0);
-    } else {
-      e = launch_soap_bwd<float>(c->sd, x.sd_dev.as<SoapDev>(), c->W_dev.as<double>(),
-                                 c->rtab.as<float>(), c->pos.as<double>(), c->species.as<int>(),
-                                 cell, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
-                                 c->site_atom.as<int>(), n_sites, x.pairn.as<int>(),
-                                 c->q.as<float>(), c->dim_pad, c->cbuf.as<float>(),
-                                 c->pnorm.as<float>(), c->gmat.as<float>(), c->dim_pad,
-                                 c->gpair.as<float>(), c->vsite.as<double>(),
-                                 fuse ? x.twob_store.as<TwoBodyDev>() : nullptr,
-                                 fuse ? (int)c->twob.size() : 0, c->eatom.as<double>(),
-                                 c->stream);
-      kev_rec(c, 5);
-      kev_rec(c, 6);
-      if (e == hipSuccess)
-        e = launch_force_gather<float>(n, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
-                                       c->nb_rev.as<int>(), c->site_of_atom.as<int>(), n_sites == n ? 1 : 0,
+    hipError_t e = bwd(kTuneWpb[wk]);
+    kev_rec(c, 5);
+    kev_rec(c, 6);
+    if (e == hipSuccess) {
+      if (prec == TGX_FP64)
+        e = launch_force_gather<double>(n, c->maxnb, c->nb_count.as<int>(),
+                                        c->nb_list.as<int>(), c->nb_rev.as<int>(),
+                                        c->site_of_atom.as<int>(), n_sites == n ? 1 : 0,
+                                        c->gpair.as<double>(), c->forces.as<double>(),
+                                        c->stream, fuse ? 1 : 0);
+      else
+        e = launch_force_gather<float>(n, c->maxnb, c->nb_count.as<int>(),
+                                       c->nb_list.as<int>(), c->nb_rev.as<int>(),
+                                       c->site_of_atom.as<int>(), n_sites == n ? 1 : 0,
                                        c->gpair.as<float>(), c->forces.as<double>(), c->stream,
-                                        fuse ? 1 : 0);
+                                       fuse ? 1 : 0);
     }
     TGX_HIP(c, e);
     kev_rec(c, 7);
@@ -617,6 +765,18 @@ int tgx_set_precision(tgx_ctx* c, int precision) {
   if (precision < TGX_FP64 || precision > TGX_MIXED)
     return set_err(c, TGX_ERR_ARG, "precision %d not in 0..2", precision);
   c->precision = precision;
+  return TGX_OK;
+}
+
+int tgx_set_autotune(tgx_ctx* c, int on, const char* cache_path) {
+  if (!c) return TGX_ERR_ARG;
+  if (on != 0 && on != 1) return set_err(c, TGX_ERR_ARG, "autotune %d not 0 or 1", on);
+  Extra& x = X(c);
+  x.tune_on = on;
+  if (cache_path) {
+    x.tune_path = cache_path;
+    tune_load(x);
+  }
   return TGX_OK;
 }
 

@@ -115,8 +115,9 @@ template <typename TI, int EPI> struct Tile {
 template <> struct Tile<u16, 1> { static constexpr int TM = 4, TN = 4, BK = 64; };
 
 // EPI 0: kernel epilogue; EPI 1: plain store.  The next k-slab is read from
-// HBM into registers while the MFMAs of the current one run.
-template <typename TI, int EPI>
+// HBM into registers while the MFMAs of the current one run.  TM_ x TN_
+// MFMA tiles per wave (the autotuner's choice; Tile<> gives the default).
+template <typename TI, int EPI, int TM_ = Tile<TI, EPI>::TM, int TN_ = Tile<TI, EPI>::TN>
 __global__ __launch_bounds__(256) void gemm_nt_kernel(int M, int N, int K,
                                                      const TI* __restrict__ A, int lda,
                                                      const TI* __restrict__ B, int ldb,
@@ -126,7 +127,7 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(int M, int N, int K,
   typedef MF<TI> F;
   typedef typename F::acc_t acc_t;
   typedef typename F::vacc vacc;
-  constexpr int TM = Tile<TI, EPI>::TM, TN = Tile<TI, EPI>::TN, BK = Tile<TI, EPI>::BK;
+  constexpr int TM = TM_, TN = TN_, BK = Tile<TI, EPI>::BK;
   constexpr int BM = 32 * TM, BN = 32 * TN, LDS = BK + F::PAD;
   constexpr int EPR = 16 / sizeof(TI);     // elements per 16-byte load
   constexpr int TPR = BK / EPR;            // threads per tile row
@@ -323,6 +324,7 @@ __device__ __forceinline__ void kpow_f(float k, int zint, float* kz, float* kz1)
 }
 
 template <int V> struct ICz { static constexpr int v = V; };
+template <int V> struct ICv { static constexpr int value = V; };
 
 // LDS bytes of the W / stage-1 region and of the stage-2 slab region.
 __host__ __device__ inline size_t fused_region_w(int N1) {
@@ -939,7 +941,8 @@ static hipError_t run(int n_sites, int dim, int dim_pad, int n_sparse, int sp_pa
                       const void* q, const void* qs, const void* qsT, const void* alphas,
                       double delta, double zeta, void* wmat, void* epart, double* esite,
                       void* G, int want_grad, const int* guard, double* eatom,
-                      const double* e0, const int* species, int* few_parts, hipStream_t st) {
+                      const double* e0, const int* species, int* few_parts,
+                      const GemmPlan* plan, hipStream_t st) {
   typedef typename MF<TI>::acc_t acc_t;
   const int M = n_sites;
   if (few_parts) *few_parts = 0;
@@ -951,7 +954,10 @@ static hipError_t run(int n_sites, int dim, int dim_pad, int n_sparse, int sp_pa
   EpiStore es{G, dim_pad};
   if constexpr (sizeof(TI) == 2) {
     // TGX_GEMM_UNFUSED=1 selects the two separate GEMMs (for comparisons)
-    const char* uf = getenv("TGX_GEMM_UNFUSED");
+    // TGX_GEMM_UNFUSED=1 or plan path 2: the separate GEMMs
+    const char* ufe = getenv("TGX_GEMM_UNFUSED");
+    const bool uf_sep = (ufe && ufe[0] == '1') || (plan && plan->path == 2);
+    const char* uf = uf_sep ? "1" : nullptr;
     if (!want_grad && zint >= 1 && zint <= 8 && M <= 128 && sp_pad % 32 == 0 &&
         dim_pad % 32 == 0 && !(uf && uf[0] == '1')) {
       const int nwg = sp_pad / (16 * TGX_FEW_NC);
@@ -999,21 +1005,41 @@ static hipError_t run(int n_sites, int dim, int dim_pad, int n_sparse, int sp_pa
       }
     }
   }
-  constexpr int BM0 = 32 * Tile<TI, 0>::TM, BN0 = 32 * Tile<TI, 0>::TN;
-  constexpr int BM1 = 32 * Tile<TI, 1>::TM, BN1 = 32 * Tile<TI, 1>::TN;
-  const int mt0 = (M + BM0 - 1) / BM0;
-  const int nt1 = (sp_pad + BN0 - 1) / BN0;
-  hipLaunchKernelGGL((gemm_nt_kernel<TI, 0>), dim3((nt1 * mt0 + 7) / 8 * 8), dim3(256), 0, st, M,
-                     sp_pad, dim_pad, (const TI*)q, dim_pad, (const TI*)qs, dim_pad, ek, es, nt1,
-                     nt1 * mt0, guard);
+  // separate GEMMs, tiles per the plan (epilogue GEMM t0: 64x64, 128x64,
+  // 64x128; store GEMM t1: the default of the type or its alternative)
+  const int t0 = plan ? plan->t0 : 0, t1 = plan ? plan->t1 : 0;
+  int nt1 = 0;
+  auto gemm0 = [&](auto TMc, auto TNc) {
+    constexpr int BM = 32 * decltype(TMc)::value, BN = 32 * decltype(TNc)::value;
+    const int mt = (M + BM - 1) / BM;
+    nt1 = (sp_pad + BN - 1) / BN;
+    hipLaunchKernelGGL((gemm_nt_kernel<TI, 0, decltype(TMc)::value, decltype(TNc)::value>),
+                       dim3((nt1 * mt + 7) / 8 * 8), dim3(256), 0, st, M, sp_pad, dim_pad,
+                       (const TI*)q, dim_pad, (const TI*)qs, dim_pad, ek, es, nt1, nt1 * mt,
+                       guard);
+  };
+  if (t0 == 1) gemm0(ICv<4>{}, ICv<2>{});
+  else if (t0 == 2) gemm0(ICv<2>{}, ICv<4>{});
+  else gemm0(ICv<2>{}, ICv<2>{});
   hipLaunchKernelGGL(esum_kernel<acc_t>, dim3((M + 255) / 256), dim3(256), 0, st, M, nt1,
                      (const acc_t*)epart, delta * delta, esite, ek, guard);
   if (want_grad) {
-    const int mt1 = (M + BM1 - 1) / BM1;
-    const int nt2 = (dim_pad + BN1 - 1) / BN1;
-    hipLaunchKernelGGL((gemm_nt_kernel<TI, 1>), dim3((nt2 * mt1 + 7) / 8 * 8), dim3(256), 0, st,
-                       M, dim_pad, sp_pad, (const TI*)wmat, sp_pad, (const TI*)qsT, sp_pad, ek,
-                       es, nt2, nt2 * mt1, guard);
+    auto gemm1 = [&](auto TMc, auto TNc) {
+      constexpr int BM = 32 * decltype(TMc)::value, BN = 32 * decltype(TNc)::value;
+      const int mt = (M + BM - 1) / BM;
+      const int nt2 = (dim_pad + BN - 1) / BN;
+      hipLaunchKernelGGL((gemm_nt_kernel<TI, 1, decltype(TMc)::value, decltype(TNc)::value>),
+                         dim3((nt2 * mt + 7) / 8 * 8), dim3(256), 0, st, M, dim_pad, sp_pad,
+                         (const TI*)wmat, sp_pad, (const TI*)qsT, sp_pad, ek, es, nt2, nt2 * mt,
+                         guard);
+    };
+    constexpr int DM = Tile<TI, 1>::TM, DN = Tile<TI, 1>::TN;
+    if (t1 == 1 && sizeof(TI) < 8) {
+      if constexpr (DM == 4) gemm1(ICv<2>{}, ICv<2>{});
+      else gemm1(ICv<4>{}, ICv<4>{});
+    } else {
+      gemm1(ICv<DM>{}, ICv<DN>{});
+    }
   }
   return hipGetLastError();
 }
@@ -1023,15 +1049,15 @@ hipError_t launch_gap_kernel(int mode, int n_sites, int dim, int dim_pad, int n_
                              const void* alphas, double delta, double zeta, void* wmat,
                              void* epart, double* esite, void* G, int want_grad,
                              const int* guard, hipStream_t st, double* eatom, const double* e0,
-                             const int* species, int* few_parts) {
+                             const int* species, int* few_parts, const GemmPlan* plan) {
   if (mode == TGX_FP64)
     return run<double>(n_sites, dim, dim_pad, n_sparse, sp_pad, q, qs, qsT, alphas, delta, zeta,
-                       wmat, epart, esite, G, want_grad, guard, eatom, e0, species, few_parts, st);
+                       wmat, epart, esite, G, want_grad, guard, eatom, e0, species, few_parts, plan, st);
   if (mode == TGX_FP32)
     return run<float>(n_sites, dim, dim_pad, n_sparse, sp_pad, q, qs, qsT, alphas, delta, zeta,
-                      wmat, epart, esite, G, want_grad, guard, eatom, e0, species, few_parts, st);
+                      wmat, epart, esite, G, want_grad, guard, eatom, e0, species, few_parts, plan, st);
   return run<u16>(n_sites, dim, dim_pad, n_sparse, sp_pad, q, qs, qsT, alphas, delta, zeta, wmat,
-                  epart, esite, G, want_grad, guard, eatom, e0, species, few_parts, st);
+                  epart, esite, G, want_grad, guard, eatom, e0, species, few_parts, plan, st);
 }
 
 }  // namespace tgx

@@ -166,6 +166,15 @@ void radial_closed_form(const SoapDev& sd, const double* W, int s, double r,
                         double* R, double* dR);   // FP64 host evaluation
 int make_cell(const double* lattice, double rcut, Cell* cell, std::string* err);
 
+// Launch plan of the kernel step, chosen by the autotuner (or all zero: the
+// built-in choice).  path: 0 built-in, 2 separate GEMMs.  t0: tile of the
+// kernel-epilogue GEMM on the separate path (0: 64x64, 1: 128x64, 2:
+// 64x128).  t1: tile of the store GEMM (0: the type's default, 1: the
+// alternative, 64x64 for BF16 and 128x128 for FP32).
+struct GemmPlan {
+  int path, t0, t1;
+};
+
 // Kernel launchers are declared next to their callers in tgx_api.hip.
 // launch_gap_kernel: eatom/e0/species (every atom a site, in order) also
 // write eatom[i] = e0[species[i] - 1] + esite[i].  few_parts (energies only):
@@ -180,7 +189,8 @@ hipError_t launch_gap_kernel(int mode, int n_sites, int dim, int dim_pad,
                              void* wmat, void* epart, double* esite, void* G,
                              int want_grad, const int* guard, hipStream_t st,
                              double* eatom = nullptr, const double* e0 = nullptr,
-                             const int* species = nullptr, int* few_parts = nullptr);
+                             const int* species = nullptr, int* few_parts = nullptr,
+                             const GemmPlan* plan = nullptr);
 // overwrite: forces[k] = gathered sum (otherwise added to forces[k])
 template <typename T>
 hipError_t launch_force_gather(int n, int maxnb, const int* count,

@@ -759,6 +759,11 @@ static int waves_per_block(size_t per_wave_bytes) {
   while (w > 1 && (size_t)w * per_wave_bytes > 65536) w >>= 1;
   return w;
 }
+// The autotuner's request (1, 2 or 4 waves) when it fits, else the default.
+static int pick_wpb(size_t per_wave_bytes, int req) {
+  const int w = waves_per_block(per_wave_bytes);
+  return (req == 1 || req == 2 || req == 4) && req <= w ? req : w;
+}
 
 template <typename K>
 static hipError_t ensure_lds(K kernel, size_t sm, size_t* set) {
@@ -779,7 +784,8 @@ static hipError_t fwd_inst(const SoapDev& sdh, const SoapDev* sd, const double* 
                            const float* rtab, const double* pos, const int* species,
                            const GEO& cell, int maxnb, const int* count, const int* list,
                            const int* site_atom, int n_sites, const int* dmap, T* q, int q_ld,
-                           u16* qb, int qb_ld, T* cbuf, T* pnorm, hipStream_t st, int split) {
+                           u16* qb, int qb_ld, T* cbuf, T* pnorm, hipStream_t st, int split,
+                           int wpb_req) {
   constexpr int SPW = Spw<NRP>::v;
   if (maxnb > 65535) return hipErrorInvalidValue;  // 16-bit slot lists
   if constexpr (sizeof(T) == 4 && NRP <= 16) {
@@ -796,7 +802,7 @@ static hipError_t fwd_inst(const SoapDev& sdh, const SoapDev* sd, const double* 
     }
   }
   const size_t per_wave = SPW * fwd_site_bytes<T, NRP>(sdh.L2);
-  const int wpb = waves_per_block(per_wave);
+  const int wpb = pick_wpb(per_wave, wpb_req);
   const size_t sm = wpb * per_wave;
   static size_t sm_set = 65536;
   hipError_t e = ensure_lds(soap_fwd_kernel<T, NRP, NMX, LMX, EXL, GEO>, sm, &sm_set);
@@ -816,11 +822,11 @@ static hipError_t bwd_inst(const SoapDev& sdh, const SoapDev* sd, const double* 
                            const int* site_atom, int n_sites, const int* dmap, const T* q,
                            int q_ld, const T* cbuf, const T* pnorm, const T* G, int g_ld,
                            T* gpair, double* vsite, const TwoBodyDev* twob, int n2b,
-                           double* eatom, hipStream_t st) {
+                           double* eatom, hipStream_t st, int wpb_req) {
   constexpr int SPW = Spw<NRP>::v;
   if (maxnb > 65535) return hipErrorInvalidValue;  // 16-bit slot lists
   const size_t per_wave = SPW * bwd_site_bytes<T, NRP>(sdh.L2, sdh.l_max + 1);
-  const int wpb = waves_per_block(per_wave);
+  const int wpb = pick_wpb(per_wave, wpb_req);
   const size_t sm = wpb * per_wave;
   static size_t sm_set = 65536;
   hipError_t e = ensure_lds(soap_bwd_kernel<T, NRP, NMX, LMX, EXL, GEO>, sm, &sm_set);
@@ -862,13 +868,13 @@ hipError_t launch_soap_fwd_g(const SoapDev& sdh, const SoapDev* sd, const double
                            const GEO& cell, int maxnb, const int* count, const int* list,
                            const int* site_atom, int n_sites, const int* dmap, T* q, int q_ld,
                            u16* qb, int qb_ld, T* cbuf, T* pnorm, hipStream_t st,
-                           int split = 0) {
+                           int split = 0, int wpb = 0) {
   if (n_sites == 0) return hipSuccess;
   return soap_dispatch(sdh, [&](auto NR, auto NM, auto LM, auto EX) {
     return fwd_inst<T, decltype(NR)::value, decltype(NM)::value, decltype(LM)::value,
                     decltype(EX)::value, GEO>(sdh, sd, W, rtab, pos, species, cell, maxnb, count,
                                               list, site_atom, n_sites, dmap, q, q_ld, qb, qb_ld,
-                                              cbuf, pnorm, st, split);
+                                              cbuf, pnorm, st, split, wpb);
   });
 }
 
@@ -879,13 +885,14 @@ hipError_t launch_soap_bwd_g(const SoapDev& sdh, const SoapDev* sd, const double
                            const int* site_atom, int n_sites, const int* dmap, const T* q,
                            int q_ld, const T* cbuf, const T* pnorm, const T* G, int g_ld,
                            T* gpair, double* vsite, const TwoBodyDev* twob, int n2b,
-                           double* eatom, hipStream_t st) {
+                           double* eatom, hipStream_t st, int wpb = 0) {
   if (n_sites == 0) return hipSuccess;
   return soap_dispatch(sdh, [&](auto NR, auto NM, auto LM, auto EX) {
     return bwd_inst<T, decltype(NR)::value, decltype(NM)::value, decltype(LM)::value,
                     decltype(EX)::value, GEO>(sdh, sd, W, rtab, pos, species, cell, maxnb, count,
                                               list, site_atom, n_sites, dmap, q, q_ld, cbuf, pnorm,
-                                              G, g_ld, gpair, vsite, twob, n2b, eatom, st);
+                                              G, g_ld, gpair, vsite, twob, n2b, eatom, st,
+                                              wpb);
   });
 }
 

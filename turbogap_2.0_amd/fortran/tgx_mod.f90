@@ -61,7 +61,8 @@ module tgx_mod
      integer(c_int32_t) :: pad_ = 0
   end type tgx_mc_stats
 
-  public :: tgx_create, tgx_destroy, tgx_last_error, tgx_set_precision, tgx_set_soap_model, &
+  public :: tgx_create, tgx_destroy, tgx_last_error, tgx_set_precision, tgx_set_autotune, &
+       tgx_set_soap_model, &
        tgx_set_e0, tgx_add_2b_model, tgx_clear_2b_models, tgx_neighbors_build, &
        tgx_soap_descriptors, tgx_gap_kernel_step, tgx_energy_forces, &
        tgx_energy_forces_batched, tgx_md_run, tgx_md_begin, tgx_md_steps, tgx_md_read, &
@@ -92,6 +93,14 @@ module tgx_mod
        type(c_ptr), value :: ctx
        integer(c_int), value :: precision
      end function tgx_set_precision
+
+     ! cache_path: a C string (trim(path)//c_null_char) or c_null_ptr
+     integer(c_int) function tgx_set_autotune(ctx, on, cache_path) bind(C, name="tgx_set_autotune")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: on
+       type(c_ptr), value :: cache_path
+     end function tgx_set_autotune
 
      integer(c_int) function tgx_set_soap_model(ctx, h, dim, n_sparse, Qs, alphas, delta, zeta) &
           bind(C, name="tgx_set_soap_model")

@@ -229,6 +229,7 @@ def load_library(path: str = LIB_PATH):
         "tgx_destroy": (i, [P]),
         "tgx_last_error": (C.c_char_p, [P]),
         "tgx_set_precision": (i, [P, i]),
+        "tgx_set_autotune": (i, [P, i, C.c_char_p]),
         "tgx_set_soap_model": (i, [P, C.POINTER(CSoapHypers), i, i, dp, dp, d, d]),
         "tgx_set_e0": (i, [P, i, dp]),
         "tgx_add_2b_model": (i, [P, i, i, d, d, d, d, i, dp, dp, dp]),
@@ -323,6 +324,12 @@ class Engine:
 
     def set_precision(self, precision: str):
         self._check(self.lib.tgx_set_precision(self.ctx, PRECISIONS[precision]))
+
+    def set_autotune(self, on: bool, cache_path=None):
+        """Time the candidate launch shapes on first use of each shape
+        (tgx_set_autotune); decisions go to cache_path when given."""
+        path = None if cache_path is None else os.fsencode(cache_path)
+        self._check(self.lib.tgx_set_autotune(self.ctx, int(on), path))
 
     # -- models ------------------------------------------------------------
     def set_model(self, model: GapModel):
