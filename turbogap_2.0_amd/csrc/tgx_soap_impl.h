@@ -759,10 +759,14 @@ static int waves_per_block(size_t per_wave_bytes) {
   while (w > 1 && (size_t)w * per_wave_bytes > 65536) w >>= 1;
   return w;
 }
-// The autotuner's request (1, 2 or 4 waves) when it fits, else the default.
-static int pick_wpb(size_t per_wave_bytes, int req) {
+// The autotuner's request (1, 2 or 4 waves) when it fits, else the built-in
+// choice: one wave per workgroup (measured fastest for the FP32 kernels and
+// the FP64 forward at 32k sites, 8 % on the MD step) unless dflt_max asks
+// for as many as fit (the FP64 adjoint).
+static int pick_wpb(size_t per_wave_bytes, int req, bool dflt_max) {
   const int w = waves_per_block(per_wave_bytes);
-  return (req == 1 || req == 2 || req == 4) && req <= w ? req : w;
+  if ((req == 1 || req == 2 || req == 4) && req <= w) return req;
+  return dflt_max ? w : 1;
 }
 
 template <typename K>
@@ -802,7 +806,7 @@ static hipError_t fwd_inst(const SoapDev& sdh, const SoapDev* sd, const double* 
     }
   }
   const size_t per_wave = SPW * fwd_site_bytes<T, NRP>(sdh.L2);
-  const int wpb = pick_wpb(per_wave, wpb_req);
+  const int wpb = pick_wpb(per_wave, wpb_req, false);
   const size_t sm = wpb * per_wave;
   static size_t sm_set = 65536;
   hipError_t e = ensure_lds(soap_fwd_kernel<T, NRP, NMX, LMX, EXL, GEO>, sm, &sm_set);
@@ -826,7 +830,7 @@ static hipError_t bwd_inst(const SoapDev& sdh, const SoapDev* sd, const double* 
   constexpr int SPW = Spw<NRP>::v;
   if (maxnb > 65535) return hipErrorInvalidValue;  // 16-bit slot lists
   const size_t per_wave = SPW * bwd_site_bytes<T, NRP>(sdh.L2, sdh.l_max + 1);
-  const int wpb = pick_wpb(per_wave, wpb_req);
+  const int wpb = pick_wpb(per_wave, wpb_req, sizeof(T) == 8);
   const size_t sm = wpb * per_wave;
   static size_t sm_set = 65536;
   hipError_t e = ensure_lds(soap_bwd_kernel<T, NRP, NMX, LMX, EXL, GEO>, sm, &sm_set);
