@@ -168,6 +168,9 @@ def bench_batch256(args):
     structs = S.batch_cells(256, 1002)
     n_atoms = sum(len(p) for p, _, _ in structs)
     eng = tgx.Engine(0, "fp64")
+    tune_file = os.path.join(tempfile.gettempdir(), f"tgx_autotune_{os.getpid()}.txt")
+    if not args.no_autotune:  # decided in the untimed first call of each mode
+        eng.set_autotune(True, tune_file)
     h = S.carbon_hypers()
     tmp = S.model_from_soaps(h, np.eye(h.dim)[:1], 1, seed=0)
     eng.set_model(tmp)
@@ -201,7 +204,8 @@ def bench_batch256(args):
             "fp64_ms_per_batch": round(1e3 * out["fp64"][0], 4),
             "mixed_vs_fp64": {"force_mae_eV_A": float(np.abs(f_m - f_d).mean()),
                               "energy_max_eV_atom": float(np.abs(e_m - e_d).max())},
-            "kernel_ms": {k: round(v, 4) for k, v in kt.items()}}
+            "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
+            "autotune": read_tune(tune_file)}
     print(json.dumps(line))
     eng.close()
 
@@ -213,6 +217,9 @@ def bench_cho4096(args):
     FP64 path (itself parity-tested against the oracle)."""
     pos, sp, lat = S.cho_cell(4096, 1.5, 1.0, 1001)
     eng = tgx.Engine(0, "fp64")
+    tune_file = os.path.join(tempfile.gettempdir(), f"tgx_autotune_{os.getpid()}.txt")
+    if not args.no_autotune:  # decided in the untimed first call of each mode
+        eng.set_autotune(True, tune_file)
     h = S.cho_hypers()
     tmp = S.model_from_soaps(h, np.eye(h.dim)[:1], 1, seed=0)
     eng.set_model(tmp)
@@ -248,7 +255,8 @@ def bench_cho4096(args):
             "config": {"workload": "C/H/O 60/30/10 4096 atoms 1.5 g/cm3, rcut 4.0, n_max 8 per "
                                    "species, l_max 8, dim 2700, 4000 sparse, zeta 4",
                        "atoms": 4096},
-            "modes": modes}
+            "modes": modes,
+            "autotune": read_tune(tune_file)}
     print(json.dumps(line))
     eng.close()
 
