@@ -11,6 +11,10 @@ for p in (PKG, os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests")):
     if p not in sys.path:
         sys.path.insert(0, p)
 
+# the tests compare launch paths bit for bit: launch shapes must not depend on
+# timing (tests/test_gpu_autotune.py turns the tuner on for its own contexts)
+os.environ.pop("TGX_AUTOTUNE", None)
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libtgx.so)")
