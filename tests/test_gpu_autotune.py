@@ -54,3 +54,29 @@ def test_autotune_rejects_bad_mode():
     with pytest.raises(tgx.TgxError):
         e.set_autotune(2)
     e.close()
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp32", "fp64"])
+def test_every_kernel_step_plan(monkeypatch, prec):
+    """Each launch plan the tuner may choose (TGX_GEMM_PLAN="path t0 t1"
+    forces one): the separate GEMMs in every tile shape, against the
+    built-in plan.  Tile shapes change neither the products nor their k
+    order per element; the single BF16 kernel and the separate GEMMs sum
+    the site energies in another order (FP32 rounding)."""
+    pos, sp, lat, model = W.carbon216()
+    eng = tgx.Engine(0, prec)
+    eng.set_model(model)
+    e0, f0, v0 = eng.energy_forces(pos, sp, lat)
+    te, tf, tv = TOL[prec]
+    plans = ["2 0 0", "2 1 0", "2 2 0", "2 0 1", "2 1 1", "2 2 1"]
+    try:
+        for plan in plans:
+            monkeypatch.setenv("TGX_GEMM_PLAN", plan)
+            e, f, v = eng.energy_forces(pos, sp, lat)
+            print(prec, plan, np.abs(f - f0).max(), flush=True)
+            assert np.abs(e - e0).max() <= te, plan
+            assert np.abs(f - f0).max() <= tf, plan
+            assert np.abs(v - v0).max() <= tv, plan
+    finally:
+        monkeypatch.delenv("TGX_GEMM_PLAN", raising=False)
+        eng.close()

@@ -466,13 +466,17 @@ static int kernel_step(tgx_ctx* c, int n_sites, bool want_grad, const int* guard
   plans[np++] = GemmPlan{2, 2, 0};
   if (want_grad && prec != TGX_FP64) plans[np++] = GemmPlan{2, 0, 1};
   int pk = 0;
-  if (!guard) {  // a guarded (Monte Carlo) step may not run: nothing to time
+  // TGX_GEMM_PLAN="path t0 t1" forces one plan (for tests of every shape)
+  const char* fp = getenv("TGX_GEMM_PLAN");
+  GemmPlan forced{0, 0, 0};
+  const bool force = fp && sscanf(fp, "%d %d %d", &forced.path, &forced.t0, &forced.t1) == 3;
+  if (!guard && !force) {  // a guarded (Monte Carlo) step may not run: nothing to time
     const std::string key =
         bucket_key("step", prec, want_grad ? 1 : 0, c->sp_pad, c->dim_pad, n_sites);
     pk = tune_pick(c, key, np, 0, [&](int k) { return step(&plans[k]); });
   }
   kev_rec(c, 2);
-  TGX_HIP(c, step(&plans[pk]));
+  TGX_HIP(c, step(force ? &forced : &plans[pk]));
   kev_rec(c, 3);
   return TGX_OK;
 }
