@@ -80,3 +80,26 @@ def test_every_kernel_step_plan(monkeypatch, prec):
     finally:
         monkeypatch.delenv("TGX_GEMM_PLAN", raising=False)
         eng.close()
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+def test_soap_waves_per_workgroup(monkeypatch, prec):
+    """The SOAP kernels with 1, 2 and 4 waves per workgroup (the tuner's
+    candidates; TGX_SOAP_WPB forces one) give the same results bit for bit:
+    the shape changes only which sites share a workgroup."""
+    pos, sp, lat, model = W.cho(120)  # N_R = 24: one site per wave
+    pos2, sp2, lat2, model2 = W.carbon216()  # N_R = 8: two sites per wave
+    eng = tgx.Engine(0, prec)
+    try:
+        for (p_, s_, l_, m_) in ((pos, sp, lat, model), (pos2, sp2, lat2, model2)):
+            eng.set_model(m_)
+            out = []
+            for w in ("1", "2", "4"):
+                monkeypatch.setenv("TGX_SOAP_WPB", w)
+                out.append(eng.energy_forces(p_, s_, l_))
+            for e, f, v in out[1:]:
+                assert np.array_equal(e, out[0][0]) and np.array_equal(f, out[0][1])
+                assert np.array_equal(v, out[0][2])
+    finally:
+        monkeypatch.delenv("TGX_SOAP_WPB", raising=False)
+        eng.close()

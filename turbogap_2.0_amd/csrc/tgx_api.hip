@@ -416,8 +416,10 @@ static int soap_forward(tgx_ctx* c, int n, const Geom& cell, int n_sites, bool f
                                   c->cbuf.as<float>(), c->pnorm.as<float>(), c->stream, split,
                                   wpb);
   };
-  int wpb = 0;
-  if (!split) {
+  // TGX_SOAP_WPB=1|2|4 forces the waves per workgroup (for tests)
+  const char* fw = getenv("TGX_SOAP_WPB");
+  int wpb = fw ? atoi(fw) : 0;
+  if (!split && !fw) {
     const std::string key =
         bucket_key("fwd", prec, c->sd.nrp, c->sd.L2, cell.many ? 1 : 0, n_sites);
     wpb = kTuneWpb[tune_pick(c, key, 4, 0, [&](int k) { return fwd(kTuneWpb[k]); })];
@@ -612,13 +614,14 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
                                    hipMemcpyDeviceToDevice, c->stream)
                   : hipSuccess;
     };
-    const int wk = tune_pick(c, bkey, 4, 0, [&](int k) {
+    const char* fw = getenv("TGX_SOAP_WPB");
+    const int wk = fw ? 0 : tune_pick(c, bkey, 4, 0, [&](int k) {
       const hipError_t r = restore();
       return r == hipSuccess ? bwd(kTuneWpb[k]) : r;
     });
     TGX_HIP(c, restore());
     kev_rec(c, 4);
-    hipError_t e = bwd(kTuneWpb[wk]);
+    hipError_t e = bwd(fw ? atoi(fw) : kTuneWpb[wk]);
     kev_rec(c, 5);
     kev_rec(c, 6);
     if (e == hipSuccess) {
