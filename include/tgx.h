@@ -91,9 +91,8 @@ int tgx_destroy(tgx_ctx* ctx);
 const char* tgx_last_error(const tgx_ctx* ctx);
 int tgx_set_precision(tgx_ctx* ctx, int precision);
 
-/* Autotuner (no counterpart in the reference, whose launch shapes are fixed
- * in gpu_interface.f90; it implements the per-shape tile choice of the
- * engine): on = 1 times the candidate launch shapes of the kernel step
+/* Autotuner (no counterpart in the reference; the engine's per-shape choice
+ * of launch shapes): on = 1 times the candidate launch shapes of the kernel step
  * (single kernel or separate GEMMs, GEMM tile shapes) and of the SOAP
  * kernels (waves per workgroup) on the first call of each shape and keeps
  * the fastest; on = 0 uses stored decisions or the built-in shapes.
