@@ -104,15 +104,18 @@ __device__ __forceinline__ void kpow(T k, int zint, T zeta, T* kz, T* kz1) {
   }
 }
 
-// Workgroup tile: 2 x 2 waves, each TM x TN MFMA tiles of 16 x 16, k-slab BK.
-// The BF16 store GEMM (G = W Qs) uses 128 x 128 tiles with a 64-deep slab
-// (more MFMA work per LDS read); the kernel-epilogue GEMM keeps 64 x 64,
-// where the per-element epilogue (powers, BF16 W stores, row sums) would
-// otherwise dominate; FP32 and FP64 use 64 x 64 (measured on gfx950).
+// Workgroup tile: 2 x 2 waves, each TM x TN MFMA tiles of 16 x 16, k-slab BK
+// (the built-in shapes; the autotuner may pick others per problem shape).
+// The BF16 store GEMM (G = W Qs) uses 128 x 128 tiles (more MFMA work per
+// LDS read); the kernel-epilogue GEMM keeps 64 x 64, where the per-element
+// epilogue (powers, BF16 W stores, row sums) would otherwise dominate; both
+// BF16 GEMMs take 64-deep slabs.  FP32 and FP64 use 64 x 64 (measured on
+// gfx950).
 template <typename TI, int EPI> struct Tile {
   static constexpr int TM = 2, TN = 2, BK = MF<TI>::BK;
 };
 template <> struct Tile<u16, 1> { static constexpr int TM = 4, TN = 4, BK = 64; };
+template <> struct Tile<u16, 0> { static constexpr int TM = 2, TN = 2, BK = 64; };
 
 // EPI 0: kernel epilogue; EPI 1: plain store.  The next k-slab is read from
 // HBM into registers while the MFMAs of the current one run.  TM_ x TN_
