@@ -109,13 +109,14 @@ __device__ __forceinline__ void kpow(T k, int zint, T zeta, T* kz, T* kz1) {
 // The BF16 store GEMM (G = W Qs) uses 128 x 128 tiles (more MFMA work per
 // LDS read); the kernel-epilogue GEMM keeps 64 x 64, where the per-element
 // epilogue (powers, BF16 W stores, row sums) would otherwise dominate; both
-// BF16 GEMMs take 64-deep slabs.  FP32 and FP64 use 64 x 64 (measured on
-// gfx950).
+// BF16 GEMMs take 64-deep slabs.  FP32 and FP64 use 64 x 64, FP64 with
+// 16-deep slabs (measured on gfx950).
 template <typename TI, int EPI> struct Tile {
   static constexpr int TM = 2, TN = 2, BK = MF<TI>::BK;
 };
 template <> struct Tile<u16, 1> { static constexpr int TM = 4, TN = 4, BK = 64; };
 template <> struct Tile<u16, 0> { static constexpr int TM = 2, TN = 2, BK = 64; };
+template <int EPI> struct Tile<double, EPI> { static constexpr int TM = 2, TN = 2, BK = 16; };
 
 // EPI 0: kernel epilogue; EPI 1: plain store.  The next k-slab is read from
 // HBM into registers while the MFMAs of the current one run.  TM_ x TN_
