@@ -1,6 +1,6 @@
 """Benchmark: NVE molecular dynamics of 32,768 carbon atoms (E+F every step).
 
-Workload (BASELINE metric "atom-steps/s (E+F) at 32k C atoms, mixed vs FP64;
+Workload (headline metric "atom-steps/s (E+F) at 32k C atoms, mixed vs FP64;
 force MAE vs FP64 oracle"): 32,768 C at 2.0 g/cm^3 (min distance 1.2 A,
 seed 1003), Maxwell-Boltzmann velocities at 3,000 K, velocity Verlet with
 dt = 1 fs, neighbour list with 0.5 A skin rebuilt on displacement.  GAP
