@@ -278,10 +278,22 @@ __global__ void esum_kernel(int M, int ntiles, const TA* epart, double d2,
   if (guard && *guard == 0) return;
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= M) return;
+  // partials added in tile order; their loads go out eight at a time
   double s = 0.0;
-  for (int t = 0; t < ntiles; t++) s += (double)epart[(size_t)t * M + r];
+  int t = 0;
+  for (; t + 8 <= ntiles; t += 8) {
+    TA v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = epart[(size_t)(t + k) * M + r];
+#pragma unroll
+    for (int k = 0; k < 8; k++) s += (double)v[k];
+  }
+  for (; t < ntiles; t++) s += (double)epart[(size_t)t * M + r];
   put_site_energy(ek, esite, r, d2 * s);
 }
+// esum_kernel workgroup size: small, so that a few thousand sites still
+// spread over many CUs (the kernel is bound by load latency)
+#define TGX_ESUM_WG 64
 
 // Both GEMMs of the BF16 kernel step in one workgroup per 128 sites, for
 // models with at most 512 sparse points and 448 descriptor components:
@@ -974,7 +986,7 @@ static hipError_t run(int n_sites, int dim, int dim_pad, int n_sparse, int sp_pa
       if (few_parts)  // the caller sums the partials itself
         *few_parts = nwg;
       else
-        hipLaunchKernelGGL(esum_kernel<float>, dim3((M + 255) / 256), dim3(256), 0, st, M, nwg,
+        hipLaunchKernelGGL(esum_kernel<float>, dim3((M + TGX_ESUM_WG - 1) / TGX_ESUM_WG), dim3(TGX_ESUM_WG), 0, st, M, nwg,
                            (const float*)epart, delta * delta, esite, ek, guard);
       return hipGetLastError();
     }
@@ -1025,7 +1037,7 @@ static hipError_t run(int n_sites, int dim, int dim_pad, int n_sparse, int sp_pa
   if (t0 == 1) gemm0(ICv<4>{}, ICv<2>{});
   else if (t0 == 2) gemm0(ICv<2>{}, ICv<4>{});
   else gemm0(ICv<2>{}, ICv<2>{});
-  hipLaunchKernelGGL(esum_kernel<acc_t>, dim3((M + 255) / 256), dim3(256), 0, st, M, nt1,
+  hipLaunchKernelGGL(esum_kernel<acc_t>, dim3((M + TGX_ESUM_WG - 1) / TGX_ESUM_WG), dim3(TGX_ESUM_WG), 0, st, M, nt1,
                      (const acc_t*)epart, delta * delta, esite, ek, guard);
   if (want_grad) {
     auto gemm1 = [&](auto TMc, auto TNc) {
