@@ -67,6 +67,20 @@ def copy_probe(mbytes):
                       "launches": 3}))
 
 
+def cpu_model():
+    """The host CPU's model name and logical CPU count (as lscpu reports)."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    name = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{name}, {os.cpu_count()} logical CPUs"
+
+
 def read_tune(path):
     """The autotuner's decisions of this run (shape key -> index of the
     chosen candidate launch shape), and the file removed."""
@@ -271,6 +285,8 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp32", "fp64"])
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu-1core", action="store_true",
+                    help="skip the one-core oracle evaluation of the CPU baseline")
     ap.add_argument("--no-autotune", action="store_true",
                     help="md32k: built-in launch shapes instead of the autotuner's")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
@@ -334,7 +350,7 @@ def main():
 
     fp64 = None
     if not args.no_fp64 and args.precision != "fp64":
-        k64 = max(5, args.steps // 5)
+        k64 = max(20, args.steps // 5)
         t64, _ = run_md("fp64", k64, 2)
         fp64 = n * k64 / t64
         print(f"[bench] fp64: {1e3 * t64 / k64:.3f} ms/step, {fp64:.4g} atom-steps/s",
@@ -370,7 +386,13 @@ def main():
     peak = PEAK_FP32_TFLOPS if dom != "gemm" or args.precision != "mixed" else 2500.0
     if args.precision == "fp64":
         peak = PEAK_FP64_TFLOPS
-    roofline = {"bound": "mfma", "kernel": dom, "achieved": round(ach, 3), "peak": peak,
+    # the dominant kernel's arithmetic unit: the kernel GEMM runs on MFMA in
+    # every mode; the SOAP kernels on the FP32 (or FP64) vector ALUs
+    if dom == "gemm":
+        bound = "mfma"
+    else:
+        bound = "valu_fp64" if args.precision == "fp64" else "valu_fp32"
+    roofline = {"bound": bound, "kernel": dom, "achieved": round(ach, 3), "peak": peak,
                 "unit": "TFLOP/s", "frac": round(ach / peak, 5),
                 "traffic": recorded_traffic(dom, args.precision, n),
                 "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
@@ -383,19 +405,30 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
 
-        # repeat full FP64 E+F evaluations of the whole cell for >= 10 s of CPU work
+        # repeat full FP64 E+F evaluations of the whole cell for >= 10 s of CPU
+        # work on every thread the host gives, then one evaluation on one core
+        cores = oracle.threads()
         reps, t_cpu = 0, 0.0
         while reps == 0 or (t_cpu < 10.0 and reps < 20):
             t0 = time.perf_counter()
             e_o, f_o, v_o = oracle.energy_forces(model, pos, sp, lat, chunk_sites=2048)
             t_cpu += time.perf_counter() - t0
             reps += 1
-        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        one = None
+        if not args.no_cpu_1core:
+            oracle.set_threads(1)
+            t0 = time.perf_counter()
+            oracle.energy_forces(model, pos, sp, lat, chunk_sites=2048)
+            t1 = time.perf_counter() - t0
+            oracle.set_threads(cores)
+            one = {"value": round(n / t1, 2), "cores": 1,
+                   "sample": f"1 FP64 E+F evaluation of the full {n}-atom cell, {t1:.1f} s"}
         cpu = {"value": round(n * reps / t_cpu, 2), "unit": "atom-steps/s", "cores": cores,
-               "kind": "port",
+               "kind": "port", "cpu_model": cpu_model(),
                "sample": f"{reps} FP64 E+F evaluations of the full {n}-atom cell by the "
                          f"oracle (linked cells, SOAP with materialised dq/dr, kernel GEMMs, "
-                         f"force loop), {t_cpu:.1f} s"}
+                         f"force loop), {t_cpu:.1f} s",
+               "one_core": one}
         mae = {"force_mae_eV_A": float(np.abs(f_m - f_o).mean()),
                "force_max_eV_A": float(np.abs(f_m - f_o).max()),
                "energy_max_eV_atom": float(np.abs(e_m - e_o).max()),

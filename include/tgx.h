@@ -79,7 +79,8 @@ typedef struct {
   double amplitude_scaling[TGX_MAX_SPECIES];
   double central_weight[TGX_MAX_SPECIES];
   double global_scaling[TGX_MAX_SPECIES];
-  double nf[TGX_MAX_SPECIES];
+  double nf[TGX_MAX_SPECIES]; /* only the default 4 is accepted (else
+                                 TGX_ERR_UNSUPPORTED)                    */
 } tgx_soap_hypers;
 
 /* Context ------------------------------------------------------------------ */

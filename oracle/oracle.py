@@ -54,6 +54,21 @@ def lib():
     return _lib
 
 
+def set_threads(n: int) -> int:
+    """OpenMP threads of the oracle's parallel loops (omp_set_num_threads of
+    the libgomp the oracle links); returns the count in effect."""
+    L = lib()
+    L.omp_set_num_threads(int(n))
+    L.omp_get_max_threads.restype = C.c_int
+    return int(L.omp_get_max_threads())
+
+
+def threads() -> int:
+    L = lib()
+    L.omp_get_max_threads.restype = C.c_int
+    return int(L.omp_get_max_threads())
+
+
 def _d(a):
     return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
 
@@ -153,7 +168,8 @@ def soap(h: SoapHypers, pos, species, lattice, derivatives=False):
     if not derivatives:
         rc = L.orc_soap(C.byref(ch), n, _d(pos), _d(lat), _i(sp), _d(out), None, C.c_int64(0),
                         C.byref(ns), C.byref(npairs), _i(site), None, None, _i(nn))
-        assert rc == 0, rc
+        if rc != 0:
+            raise RuntimeError(f"orc_soap failed {rc}")
         return dict(soap=out[: ns.value], site_atom=site[: ns.value])
     rc = L.orc_soap(C.byref(ch), n, _d(pos), _d(lat), _i(sp), _d(out), None, C.c_int64(0),
                     C.byref(ns), C.byref(npairs), None, None, None, None)
@@ -325,22 +341,48 @@ def mlp(P, q, mu, sigma, H=128, round_bf16=True, y=None, grad=False):
 
 
 def mc_run_da(model: GapModel, pos, species, lattice, n_trials, temp, move_max, seed, P, mu,
-              sigma):
-    """Delayed-acceptance MC with a fixed surrogate (P, mu, sigma)."""
+              sigma, max_rmse=0.5, with_fallback=False):
+    """Delayed-acceptance MC with a fixed surrogate (P, mu, sigma), falling
+    back to plain Metropolis once the running RMS of the stage-2 energy
+    difference exceeds max_rmse (max_rmse < 0: never).  Returns positions,
+    trace, accepted and stage-1 rejection counts (and, with_fallback, the
+    first plain trial or -1)."""
     pos = _f64(pos).copy()
     lat = _f64(lattice)
     sp = np.ascontiguousarray(species, np.int32)
     cm, keep = _cmodel(model)
     P = _f64(P)
     tr = np.zeros(n_trials + 1)
-    acc, r1 = C.c_int(0), C.c_int(0)
+    acc, r1, fb = C.c_int(0), C.c_int(0), C.c_int(-1)
     fn = lib().orc_mc_run_da
     fn.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
-                   C.c_double, C.c_uint64, C.c_void_p, C.c_double, C.c_double, C.c_void_p,
-                   C.c_void_p, C.c_void_p]
+                   C.c_double, C.c_uint64, C.c_void_p, C.c_double, C.c_double, C.c_double,
+                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     rc = fn(C.addressof(cm), len(pos), pos.ctypes.data, sp.ctypes.data, lat.ctypes.data,
-            n_trials, temp, move_max, seed, P.ctypes.data, mu, sigma, tr.ctypes.data,
-            C.addressof(acc), C.addressof(r1))
+            n_trials, temp, move_max, seed, P.ctypes.data, mu, sigma, max_rmse, tr.ctypes.data,
+            C.addressof(acc), C.addressof(r1), C.addressof(fb))
     if rc != 0:
         raise RuntimeError(f"orc_mc_run_da failed {rc}")
+    if with_fallback:
+        return pos, tr, acc.value, r1.value, fb.value
     return pos, tr, acc.value, r1.value
+
+
+def mc_toy(n, prm, x0, n_steps, temp, move_max, seed, use_surrogate=False, variant=0):
+    """Independent chains (one per row of x0, n particles each) of the shared
+    Metropolis / delayed-acceptance driver on the toy potential
+    V = sum a (x^2-1)^2 + c x0 x1, surrogate V + eps sum sin(w x);
+    prm = (a, c, eps, w).  Returns the final x (n_chains, n) and the
+    accepted and stage-1-rejected counts."""
+    x0 = _f64(x0).reshape(-1, n)
+    prm = _f64(prm)
+    out = np.zeros_like(x0)
+    acc, r1 = C.c_int64(0), C.c_int64(0)
+    fn = lib().orc_mc_toy
+    fn.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_uint64,
+                   C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    rc = fn(n, prm.ctypes.data, len(x0), n_steps, temp, move_max, seed, int(use_surrogate),
+            variant, x0.ctypes.data, out.ctypes.data, C.addressof(acc), C.addressof(r1))
+    if rc != 0:
+        raise RuntimeError(f"orc_mc_toy failed {rc}")
+    return out, acc.value, r1.value

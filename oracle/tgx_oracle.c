@@ -742,6 +742,17 @@ static void soap_site(const tgx_soap_hypers* h, const soap_geom* g, int si,
   free(dAl);
 }
 
+/* nf enters soap_turbo's radial expansion (read_gap.f90:164-165,192-194
+ * read it, default 4; gap_interface.f90:301-306 pass it to get_soap).  Its
+ * role in soap_turbo's arithmetic is not in the reference tree, so only the
+ * default is accepted: any other value is refused rather than silently
+ * treated as 4. */
+static int hypers_supported(const tgx_soap_hypers* h) {
+  for (int s = 0; s < h->n_species; s++)
+    if (h->nf[s] != 4.0) return TGX_ERR_UNSUPPORTED;
+  return TGX_OK;
+}
+
 static double soap_rcut_max(const tgx_soap_hypers* h) {
   double r = 0.0;
   for (int s = 0; s < h->n_species; s++) r = h->rcut_hard[s] > r ? h->rcut_hard[s] : r;
@@ -847,6 +858,7 @@ int orc_soap(const tgx_soap_hypers* h, int n, const double* pos,
              double* soap_der, int64_t pair_cap, int* n_sites,
              int64_t* n_pairs, int* site_atom, int* pair_atom,
              double* pair_xyz, int* site_n_neigh) {
+  if (hypers_supported(h) != TGX_OK) return TGX_ERR_UNSUPPORTED;
   double rc = soap_rcut_max(h);
   int64_t np = 0;
   int rc0 = orc_neighbors(n, pos, lat, rc, 0, 0, NULL, NULL, NULL, NULL, NULL, NULL, &np, NULL);
@@ -1046,6 +1058,7 @@ int orc_energy_forces(const orc_model* m, int n, const double* pos,
                       int chunk_sites, double* e_atom, double* forces,
                       double* virial) {
   double rc = 0.0;
+  if (m->has_soap && hypers_supported(&m->h) != TGX_OK) return TGX_ERR_UNSUPPORTED;
   if (m->has_soap) rc = soap_rcut_max(&m->h);
   for (int b = 0; b < m->n2b; b++) rc = m->b2[b].rcut > rc ? m->b2[b].rcut : rc;
   int64_t np = 0;
@@ -1157,39 +1170,168 @@ static double orc_total_energy(const orc_model* m, int n, const double* pos,
   return e;
 }
 
-/* Trial t: atom floor(u0 n), disp = move_max (u1..u3 - 1/2), positions +
- * disp (mc_utils.f90:267-273, 696-698); E of the whole trial structure;
- * p = min(1, exp(-(E_trial - E_current)/(kB T))), accepted when p > u4
- * (mc_interface.f90:245-299).  e_trace[0] = initial energy, e_trace[t+1]
- * = current energy after trial t. */
-int orc_mc_run(const orc_model* m, int n, double* pos, const int* species,
-               const double* lat, int n_trials, double temp, double move_max,
-               uint64_t seed, double* e_trace, int* n_accepted) {
-  double* ea = (double*)malloc(sizeof(double) * (size_t)n);
-  int rc = 0, acc = 0;
-  double e_cur = orc_total_energy(m, n, pos, species, lat, ea, &rc);
+/* One Markov chain of single-atom moves on any energy function: the part of
+ * orc_mc_run / orc_mc_run_da that decides moves, shared with the toy-
+ * potential chains that pin it by sampling statistics.
+ *   trial t: atom floor(u0 n), disp = move_max (u1..u3 - 1/2), positions +
+ *   disp (mc_utils.f90:267-273, 696-698).
+ *   plain (sur == NULL): E of the whole trial structure, p = min(1,
+ *   exp(-(E_trial - E_current)/(kB T))), accepted when p > u4
+ *   (mc_interface.f90:245-299, mc_utils.f90:172-180).
+ *   delayed acceptance (Christen & Fox 2005): stage 1 accepts with
+ *   min(1, exp(-(S_trial - S_current)/(kB T))) > u4, stage 2 with
+ *   min(1, exp(-((E_trial - E_current) - (S_trial - S_current))/(kB T)))
+ *   > u5.  Each stage-2 trial updates ema = 0.98 ema + 0.02 d^2 with
+ *   d = (E_trial - E_current) - (S_trial - S_current); once ema exceeds
+ *   max_rmse^2 (max_rmse >= 0) the following trials are plain Metropolis
+ *   (u4), and *fallback_at = the first such trial (-1: none).
+ *   variant 1 (tests only): stage 2 with the plain ratio
+ *   min(1, exp(-(E_trial - E_current)/(kB T))), which does not preserve the
+ *   Boltzmann distribution -- the sampling tests must reject it.
+ * e_trace[0] = initial energy, e_trace[t+1] = current energy after trial t. */
+typedef double (*orc_efn)(void* ctx, int n, const double* pos, int* rc);
+
+static int mc_chain(int n, double* pos, int n_trials, double temp, double move_max,
+                    uint64_t seed, orc_efn E, void* ectx, orc_efn S, void* sctx,
+                    double max_rmse, int variant, double* e_trace, int* n_accepted,
+                    int* n_rej1, int* fallback_at) {
+  int rc = 0, acc = 0, r1 = 0, fb = -1;
+  const double kB = (double)8.617333262e-5f;
+  double ema = 0.0;
+  double e_cur = E(ectx, n, pos, &rc);
+  double s_cur = (rc == 0 && S) ? S(sctx, n, pos, &rc) : 0.0;
   if (e_trace) e_trace[0] = e_cur;
   for (int t = 0; t < n_trials && rc == 0; t++) {
     int idx = (int)floor(orc_mc_uniform(seed, t, 0) * n);
     if (idx > n - 1) idx = n - 1;
-    double old[3], disp[3];
+    double old[3];
     for (int k = 0; k < 3; k++) {
-      disp[k] = move_max * (orc_mc_uniform(seed, t, 1 + k) - 0.5);
+      const double disp = move_max * (orc_mc_uniform(seed, t, 1 + k) - 0.5);
       old[k] = pos[3 * idx + k];
-      pos[3 * idx + k] = old[k] + disp[k];
+      pos[3 * idx + k] = old[k] + disp;
     }
-    const double e_new = orc_total_energy(m, n, pos, species, lat, ea, &rc);
-    const double p = orc_mc_accept(e_new, e_cur, temp);
-    if (p > orc_mc_uniform(seed, t, 4)) {
+    const int da = S != NULL && fb < 0;
+    int accept = 0;
+    double e_new = e_cur, s_new = s_cur;
+    if (da) {
+      s_new = S(sctx, n, pos, &rc);
+      double p1 = exp(-(s_new - s_cur) / (kB * temp));
+      p1 = p1 < 1.0 ? p1 : 1.0;
+      if (p1 > orc_mc_uniform(seed, t, 4)) {
+        e_new = E(ectx, n, pos, &rc);
+        const double d = (e_new - e_cur) - (s_new - s_cur);
+        double p2 = exp(-(variant == 1 ? e_new - e_cur : d) / (kB * temp));
+        p2 = p2 < 1.0 ? p2 : 1.0;
+        accept = p2 > orc_mc_uniform(seed, t, 5);
+        ema = 0.98 * ema + 0.02 * d * d;
+        if (max_rmse >= 0.0 && ema > max_rmse * max_rmse) fb = t + 1;
+      } else {
+        r1++;
+      }
+    } else {
+      e_new = E(ectx, n, pos, &rc);
+      accept = orc_mc_accept(e_new, e_cur, temp) > orc_mc_uniform(seed, t, 4);
+    }
+    if (accept) {
       e_cur = e_new;
       acc++;
+      if (da) s_cur = s_new;
     } else {
       for (int k = 0; k < 3; k++) pos[3 * idx + k] = old[k];
     }
+    /* after the switch the surrogate is not evaluated; re-seed its current
+     * value if it is ever needed again (it is not: the switch is final) */
     if (e_trace) e_trace[t + 1] = e_cur;
   }
-  free(ea);
   if (n_accepted) *n_accepted = acc;
+  if (n_rej1) *n_rej1 = r1;
+  if (fallback_at) *fallback_at = fb;
+  return rc;
+}
+
+typedef struct {
+  const orc_model* m;
+  const int* species;
+  const double* lat;
+  double* ea;
+  const double* P;
+  double mu, sigma;
+} gap_chain_ctx;
+
+static double gap_chain_energy(void* ctx, int n, const double* pos, int* rc) {
+  gap_chain_ctx* g = (gap_chain_ctx*)ctx;
+  return orc_total_energy(g->m, n, pos, g->species, g->lat, g->ea, rc);
+}
+
+static double orc_surrogate_energy(const orc_model* m, int n, const double* pos,
+                                   const int* species, const double* lat,
+                                   const double* P, double mu, double sigma,
+                                   int* rc);
+
+static double gap_chain_surrogate(void* ctx, int n, const double* pos, int* rc) {
+  gap_chain_ctx* g = (gap_chain_ctx*)ctx;
+  return orc_surrogate_energy(g->m, n, pos, g->species, g->lat, g->P, g->mu, g->sigma, rc);
+}
+
+/* Sequential Metropolis with one whole-structure GAP energy per trial
+ * (mc_interface.f90:183-447): mc_chain without a surrogate. */
+int orc_mc_run(const orc_model* m, int n, double* pos, const int* species,
+               const double* lat, int n_trials, double temp, double move_max,
+               uint64_t seed, double* e_trace, int* n_accepted) {
+  gap_chain_ctx g = {m, species, lat, (double*)malloc(sizeof(double) * (size_t)n), NULL, 0, 1};
+  int rc = mc_chain(n, pos, n_trials, temp, move_max, seed, gap_chain_energy, &g, NULL, NULL,
+                    -1.0, 0, e_trace, n_accepted, NULL, NULL);
+  free(g.ea);
+  return rc;
+}
+
+/* Toy potentials for the sampling tests of mc_chain: n (1 or 2) particles,
+ * only their x coordinates enter,
+ *   V = sum_i a (x_i^2 - 1)^2 + c x_0 x_1 (n = 2),
+ * and the surrogate V + eps sum_i sin(w x_i) (inexact, not flat).
+ * prm = {a, c, eps, w}. */
+static double toy_v(void* ctx, int n, const double* pos, int* rc) {
+  const double* p = (const double*)ctx;
+  double v = 0.0;
+  for (int i = 0; i < n; i++) {
+    const double x = pos[3 * i];
+    v += p[0] * (x * x - 1.0) * (x * x - 1.0);
+  }
+  if (n == 2) v += p[1] * pos[0] * pos[3];
+  *rc = 0;
+  return v;
+}
+static double toy_s(void* ctx, int n, const double* pos, int* rc) {
+  const double* p = (const double*)ctx;
+  double v = toy_v(ctx, n, pos, rc);
+  for (int i = 0; i < n; i++) v += p[2] * sin(p[3] * pos[3 * i]);
+  return v;
+}
+
+/* n_chains independent chains of the toy potential, chain k from x0[k*n ..]
+ * with seed + k, n_steps trials each; x_end receives the final x
+ * coordinates.  use_sur: delayed acceptance with the toy surrogate
+ * (no fallback); variant as mc_chain. */
+int orc_mc_toy(int n, const double* prm, int n_chains, int n_steps, double temp,
+               double move_max, uint64_t seed, int use_sur, int variant, const double* x0,
+               double* x_end, int64_t* n_accepted, int64_t* n_rej1) {
+  if (n < 1 || n > 2) return TGX_ERR_ARG;
+  int64_t acc = 0, r1 = 0;
+  int rc = 0;
+#pragma omp parallel for reduction(+ : acc, r1) schedule(static)
+  for (int k = 0; k < n_chains; k++) {
+    double pos[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < n; i++) pos[3 * i] = x0[(size_t)k * n + i];
+    int a = 0, b = 0;
+    int r = mc_chain(n, pos, n_steps, temp, move_max, seed + (uint64_t)k, toy_v, (void*)prm,
+                     use_sur ? toy_s : NULL, (void*)prm, -1.0, variant, NULL, &a, &b, NULL);
+    if (r) rc = r;
+    for (int i = 0; i < n; i++) x_end[(size_t)k * n + i] = pos[3 * i];
+    acc += a;
+    r1 += b;
+  }
+  if (n_accepted) *n_accepted = acc;
+  if (n_rej1) *n_rej1 = r1;
   return rc;
 }
 
@@ -1318,54 +1460,20 @@ static double orc_surrogate_energy(const orc_model* m, int n, const double* pos,
   return e;
 }
 
-/* Delayed-acceptance Metropolis (Christen & Fox 2005) with a fixed surrogate:
- * stage 1 accepts with min(1, exp(-(E*_trial - E*_current)/(kB T))) > u4,
- * stage 2 with min(1, exp(-((E_trial - E_current) - (E*_trial - E*_current))
- * /(kB T))) > u5; whole-structure energies per trial as orc_mc_run.  Returns
- * the trace and counts (accepted, stage-1 rejections). */
+/* Delayed-acceptance Metropolis with a fixed surrogate (P, mu, sigma): the
+ * MLP on every site's SOAP plus the exact 2-body terms as stage 1, the GAP
+ * as stage 2, whole-structure energies per trial; mc_chain with the fallback
+ * to plain Metropolis once the running mean square of the stage-2 energy
+ * difference exceeds max_rmse^2 (max_rmse < 0: never). */
 int orc_mc_run_da(const orc_model* m, int n, double* pos, const int* species,
                   const double* lat, int n_trials, double temp, double move_max,
                   uint64_t seed, const double* P, double mu, double sigma,
-                  double* e_trace, int* n_accepted, int* n_rej1) {
-  double* ea = (double*)malloc(sizeof(double) * (size_t)n);
-  int rc = 0, acc = 0, r1 = 0;
-  const double kB = (double)8.617333262e-5f;
-  double e_cur = orc_total_energy(m, n, pos, species, lat, ea, &rc);
-  double s_cur = rc == 0 ? orc_surrogate_energy(m, n, pos, species, lat, P, mu, sigma, &rc) : 0;
-  if (e_trace) e_trace[0] = e_cur;
-  for (int t = 0; t < n_trials && rc == 0; t++) {
-    int idx = (int)floor(orc_mc_uniform(seed, t, 0) * n);
-    if (idx > n - 1) idx = n - 1;
-    double old[3];
-    for (int k = 0; k < 3; k++) {
-      const double disp = move_max * (orc_mc_uniform(seed, t, 1 + k) - 0.5);
-      old[k] = pos[3 * idx + k];
-      pos[3 * idx + k] = old[k] + disp;
-    }
-    const double s_new = orc_surrogate_energy(m, n, pos, species, lat, P, mu, sigma, &rc);
-    double p1 = exp(-(s_new - s_cur) / (kB * temp));
-    p1 = p1 < 1.0 ? p1 : 1.0;
-    int accept = 0;
-    double e_new = e_cur;
-    if (p1 > orc_mc_uniform(seed, t, 4)) {
-      e_new = orc_total_energy(m, n, pos, species, lat, ea, &rc);
-      double p2 = exp(-((e_new - e_cur) - (s_new - s_cur)) / (kB * temp));
-      p2 = p2 < 1.0 ? p2 : 1.0;
-      accept = p2 > orc_mc_uniform(seed, t, 5);
-    } else {
-      r1++;
-    }
-    if (accept) {
-      e_cur = e_new;
-      s_cur = s_new;
-      acc++;
-    } else {
-      for (int k = 0; k < 3; k++) pos[3 * idx + k] = old[k];
-    }
-    if (e_trace) e_trace[t + 1] = e_cur;
-  }
-  free(ea);
-  if (n_accepted) *n_accepted = acc;
-  if (n_rej1) *n_rej1 = r1;
+                  double max_rmse, double* e_trace, int* n_accepted, int* n_rej1,
+                  int* fallback_at) {
+  gap_chain_ctx g = {m, species, lat, (double*)malloc(sizeof(double) * (size_t)n), P, mu, sigma};
+  int rc = mc_chain(n, pos, n_trials, temp, move_max, seed, gap_chain_energy, &g,
+                    gap_chain_surrogate, &g, max_rmse, 0, e_trace, n_accepted, n_rej1,
+                    fallback_at);
+  free(g.ea);
   return rc;
 }

@@ -144,7 +144,14 @@ double orc_mlp(int n, int D, int H, const double* P, const double* q, double mu,
 int orc_mc_run_da(const orc_model* m, int n, double* pos, const int* species,
                   const double* lat, int n_trials, double temp, double move_max,
                   uint64_t seed, const double* P, double mu, double sigma,
-                  double* e_trace, int* n_accepted, int* n_rej1);
+                  double max_rmse, double* e_trace, int* n_accepted, int* n_rej1,
+                  int* fallback_at);
+
+/* Independent chains of the shared Metropolis / delayed-acceptance driver on
+ * a toy potential of 1 or 2 particles (sampling tests). */
+int orc_mc_toy(int n, const double* prm, int n_chains, int n_steps, double temp,
+               double move_max, uint64_t seed, int use_sur, int variant, const double* x0,
+               double* x_end, int64_t* n_accepted, int64_t* n_rej1);
 
 #ifdef __cplusplus
 }

@@ -127,3 +127,31 @@ def test_delayed_acceptance_online_training(eng):
     assert abs(e.sum() - g["e_trace"][-1]) < 1e-8
     a_da, a_pl = g["n_accepted"] / n, plain["n_accepted"] / n
     assert abs(a_da - a_pl) < 0.06
+
+
+@pytest.mark.parametrize("max_rmse", [1e-9, 2e-3])
+def test_delayed_acceptance_fallback_matches_oracle(eng, max_rmse):
+    """The drift fallback: once the running mean square of the stage-2
+    energy difference exceeds max_rmse^2 the run continues as plain
+    Metropolis.  With max_rmse ~ 0 it switches after the first GAP-evaluated
+    trial; the chain (before and after the switch) equals the oracle's
+    two-stage chain with the same rule, move for move."""
+    pos, sp, lat, model = _mc_system()
+    eng.set_model(model)
+    P, mu, sigma = _fitted(eng, model)
+    n_trials = 150
+    print(f"max_rmse={max_rmse}", flush=True)
+    g = eng.mc_run_da(pos, sp, lat, n_trials, 1500.0, 0.6, 1.2, 77, n_train_trials=0,
+                      max_rmse=max_rmse)
+    p_o, tr_o, acc_o, r1_o, fb = oracle.mc_run_da(model, pos, sp, lat, n_trials, 1500.0, 0.6,
+                                                  77, P, mu, sigma, max_rmse=max_rmse,
+                                                  with_fallback=True)
+    print(f"fallback at {fb}: gpu {g['fallback']} acc {g['n_accepted']}/{acc_o} "
+          f"stage-1 rejections {g['n_stage1_rejected']}/{r1_o}", flush=True)
+    if max_rmse < 1e-6:
+        assert 0 < fb < n_trials
+    assert g["fallback"] == (1 if fb >= 0 else 0)
+    assert g["n_accepted"] == acc_o and g["n_stage1_rejected"] == r1_o
+    assert g["n_gap_evaluations"] == n_trials - r1_o
+    assert np.array_equal(g["pos"], p_o)
+    assert np.abs(g["e_trace"] - tr_o).max() < 1e-8

@@ -611,3 +611,117 @@ def test_delayed_acceptance_flat_surrogate_is_plain_metropolis():
             q[k] = old
         assert abs(tr_da[t + 1] - e_cur) < 1e-12
     assert n_acc == acc and np.array_equal(q, p_da)
+
+
+def test_non_default_nf_is_refused():
+    """nf is read by read_gap.f90:164-165,192-194 (default 4) and handed to
+    soap_turbo, whose use of it is not in the reference tree: the oracle
+    refuses other values instead of treating them as 4."""
+    import dataclasses
+
+    pos, sp, lat = S.carbon_cell(32, 2.0, 1.2, 3)
+    h = dataclasses.replace(S.carbon_hypers(), nf=(4.5,))
+    with pytest.raises(RuntimeError, match="failed 4"):
+        oracle.soap(h, pos, sp, lat)
+    model = S.model_from_soaps(S.carbon_hypers(), oracle.soap(S.carbon_hypers(), pos, sp, lat)["soap"], 8, seed=3)
+    model = dataclasses.replace(model, hypers=h)
+    with pytest.raises(RuntimeError, match="failed 4"):
+        oracle.energy_forces(model, pos, sp, lat)
+
+
+# ---------------------------------------------------------------------------
+# Sampling statistics of the Metropolis / delayed-acceptance driver
+# (mc_utils.f90:172-180, 215-277; Christen & Fox 2005).  orc_mc_run and
+# orc_mc_run_da decide moves with the same driver the toy chains use here.
+# Chains start from the exact Boltzmann distribution, so after any number of
+# steps their end points are independent draws of it when the acceptance
+# rule is right: a chi-squared test on those end points checks that.
+
+KB_SINGLE = float(np.float32(8.617333262e-5))
+
+
+# Bin edges of the tests (outer bins catch the tails) and a midpoint-rule
+# grid whose cells tile every bin exactly, so that the expected bin masses
+# and the cell-wise start draws carry no edge error.
+_STEP1, _STEP2 = 0.15, 0.4
+_EDGES1 = np.concatenate([[-9.0], np.arange(-12, 13) * _STEP1, [9.0]])
+_EDGES2 = np.concatenate([[-9.0], np.arange(-4, 5) * _STEP2, [9.0]])
+
+
+def _boltzmann_grid(prm, n, temp):
+    a, c = prm[0], prm[1]
+    step = _STEP1 if n == 1 else _STEP2
+    lo = (-12 - 6) * _STEP1 if n == 1 else (-4 - 3) * _STEP2
+    h = step / 32
+    x = lo + (np.arange(int(round(-2 * lo / h))) + 0.5) * h
+    kT = KB_SINGLE * temp
+    if n == 1:
+        v = a * (x * x - 1) ** 2
+        return x, h, np.exp(-(v - v.min()) / kT)
+    X0, X1 = np.meshgrid(x, x, indexing="ij")
+    v = a * (X0 ** 2 - 1) ** 2 + a * (X1 ** 2 - 1) ** 2 + c * X0 * X1
+    return x, h, np.exp(-(v - v.min()) / kT)
+
+
+def _draw(prm, n, temp, k, seed):
+    """k draws from the Boltzmann density: a grid cell by its mass, then a
+    uniform point in it."""
+    x, h, w = _boltzmann_grid(prm, n, temp)
+    rng = np.random.default_rng(seed)
+    p = (w / w.sum()).ravel()
+    idx = rng.choice(p.size, size=k, p=p)
+    if n == 1:
+        return (x[idx] + (rng.random(k) - 0.5) * h).reshape(k, 1)
+    i0, i1 = np.unravel_index(idx, w.shape)
+    return np.stack([x[i0], x[i1]], 1) + (rng.random((k, 2)) - 0.5) * h
+
+
+def _chi2_pvalue(samples, prm, n, temp, edges):
+    from scipy.stats import chi2
+
+    x, _, w = _boltzmann_grid(prm, n, temp)
+    w = w / w.sum()
+    if n == 1:
+        obs, _ = np.histogram(samples[:, 0], bins=edges)
+        exp, _ = np.histogram(x, bins=edges, weights=w)
+    else:
+        obs, _, _ = np.histogram2d(samples[:, 0], samples[:, 1], bins=[edges, edges])
+        X0, X1 = np.meshgrid(x, x, indexing="ij")
+        exp, _, _ = np.histogram2d(X0.ravel(), X1.ravel(), bins=[edges, edges], weights=w.ravel())
+        obs, exp = obs.ravel(), exp.ravel()
+    exp = exp / exp.sum() * obs.sum()
+    keep = exp >= 5
+    stat = np.sum((obs[keep] - exp[keep]) ** 2 / exp[keep])
+    return float(chi2.sf(stat, keep.sum() - 1)), stat, int(keep.sum())
+
+
+@pytest.mark.parametrize("n,da", [(1, False), (1, True), (2, False), (2, True)])
+def test_mc_driver_samples_boltzmann(n, da):
+    """Plain Metropolis and delayed acceptance with a surrogate that is
+    neither exact nor flat (V + 0.03 sin 5x) sample exp(-V/kT): 1-D double
+    well and two coupled wells."""
+    prm = (0.1, 0.04, 0.03, 5.0)
+    T = 1000.0
+    k = 20000
+    x0 = _draw(prm, n, T, k, seed=10 + n)
+    xe, acc, r1 = oracle.mc_toy(n, prm, x0, 200, T, 1.0, 1234 + n, use_surrogate=da)
+    p, stat, nb = _chi2_pvalue(xe, prm, n, T, _EDGES1 if n == 1 else _EDGES2)
+    print(f"n={n} da={da} acc={acc / (k * 200):.3f} rej1={r1} chi2={stat:.1f}/{nb - 1} p={p:.3g}")
+    assert p > 1e-3
+    if da:
+        assert r1 > 0  # the surrogate stage does reject trials
+
+
+@pytest.mark.parametrize("n", [1, 2])
+def test_mc_driver_chi2_rejects_wrong_stage2_ratio(n):
+    """The same test has the power to see a plausible mistake: stage 2 with
+    the plain ratio exp(-dE/kT) instead of exp(-(dE - dS)/kT) (planted in the
+    oracle's driver, variant 1) fails it."""
+    prm = (0.1, 0.04, 0.03, 5.0)
+    T = 1000.0
+    k = 20000
+    x0 = _draw(prm, n, T, k, seed=20 + n)
+    xe, _, _ = oracle.mc_toy(n, prm, x0, 200, T, 1.0, 99 + n, use_surrogate=True, variant=1)
+    p, stat, nb = _chi2_pvalue(xe, prm, n, T, _EDGES1 if n == 1 else _EDGES2)
+    print(f"n={n} wrong ratio chi2={stat:.1f}/{nb - 1} p={p:.3g}")
+    assert p < 1e-6

@@ -707,6 +707,17 @@ static int upload_structure(tgx_ctx* c, int n, const double* pos, const int* spe
   return TGX_OK;
 }
 
+// Entry points that write the context's structure, list or model buffers
+// refuse to run while an MD run holds them: the captured step graph reads
+// those buffers at fixed addresses.
+static void md_release_graph(MdState& m);
+static int md_busy(tgx_ctx* c) {
+  if (X(c).md.active)
+    return set_err(c, TGX_ERR_STATE,
+                   "an MD run is active on this context (call tgx_md_end first)");
+  return TGX_OK;
+}
+
 // ---------------------------------------------------------------------------
 // C-ABI
 // ---------------------------------------------------------------------------
@@ -740,6 +751,8 @@ int tgx_destroy(tgx_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   tgx_ctx_full* f = reinterpret_cast<tgx_ctx_full*>(c);
+  md_release_graph(f->x.md);
+  f->x.md.active = false;
   DevBuf* bufs[] = {&c->W_dev, &c->rtab, &c->Qs64, &c->QsT64, &c->Qs32, &c->QsT32, &c->Qs16,
                     &c->QsT16, &c->alph64, &c->alph32, &c->pos, &c->species, &c->nb_count,
                     &c->nb_list, &c->nb_rev, &c->nb_flag, &c->cell_of, &c->cell_start,
@@ -755,7 +768,9 @@ int tgx_destroy(tgx_ctx* c) {
                     &f->x.mc_twob, &f->x.mc_esur, &f->x.mc_esur_cur, &f->x.mc_sq,
                     &f->x.mc_sy, &f->x.sg_P, &f->x.sg_m, &f->x.sg_v, &f->x.sg_grad,
                     &f->x.sg_W1t, &f->x.sg_W2t, &f->x.sg_part, &f->x.sg_z1, &f->x.sg_z2,
-                    &f->x.sg_e, &f->x.sg_q, &f->x.sg_y, &f->x.sg_rows, &f->x.sg_norm};
+                    &f->x.sg_e, &f->x.sg_q, &f->x.sg_y, &f->x.sg_rows, &f->x.sg_norm,
+                    &f->x.tune_save, &f->x.md.pref, &f->x.md.ke, &f->x.md.ctl, &f->x.md.ep,
+                    &f->x.md.ek, &f->x.md.vout, &f->x.md.kesum};
   for (DevBuf* b : bufs) b->release();
   for (auto& b : c->twob) {
     if (b.Qs) (void)hipFree(b.Qs);
@@ -775,6 +790,7 @@ const char* tgx_last_error(const tgx_ctx* c) { return c ? c->err.c_str() : "null
 
 int tgx_set_precision(tgx_ctx* c, int precision) {
   if (!c) return TGX_ERR_ARG;
+  if (md_busy(c)) return TGX_ERR_STATE;
   if (precision < TGX_FP64 || precision > TGX_MIXED)
     return set_err(c, TGX_ERR_ARG, "precision %d not in 0..2", precision);
   c->precision = precision;
@@ -796,6 +812,7 @@ int tgx_set_autotune(tgx_ctx* c, int on, const char* cache_path) {
 int tgx_set_soap_model(tgx_ctx* c, const tgx_soap_hypers* h, int dim, int n_sparse,
                        const double* Qs, const double* alphas, double delta, double zeta) {
   if (!c) return TGX_ERR_ARG;
+  if (md_busy(c)) return TGX_ERR_STATE;
   if (!h || !Qs || !alphas) return set_err(c, TGX_ERR_ARG, "null argument");
   (void)hipSetDevice(c->device);
   if (h->n_species < 1 || h->n_species > TGX_MAX_SPECIES)
@@ -820,6 +837,12 @@ int tgx_set_soap_model(tgx_ctx* c, const tgx_soap_hypers* h, int dim, int n_spar
       return set_err(c, TGX_ERR_ARG, "need 0 <= rcut_soft <= rcut_hard (species %d)", s + 1);
     if (!(h->atom_sigma_r[s] > 0) || !(h->atom_sigma_t[s] > 0))
       return set_err(c, TGX_ERR_ARG, "atom_sigma must be positive (species %d)", s + 1);
+    // nf (read_gap.f90:164-165,192-194, default 4) is passed to soap_turbo's
+    // get_soap; its role there is not in the reference tree, so only the
+    // default is accepted rather than silently treating another value as 4
+    if (h->nf[s] != 4.0)
+      return set_err(c, TGX_ERR_UNSUPPORTED, "nf = %g (species %d): only nf = 4 is implemented",
+                     h->nf[s], s + 1);
     sd.nmax[s] = h->alpha_max[s];
     sd.off[s] = nr;
     nr += h->alpha_max[s];
@@ -946,6 +969,7 @@ int tgx_set_soap_model(tgx_ctx* c, const tgx_soap_hypers* h, int dim, int n_spar
 
 int tgx_set_e0(tgx_ctx* c, int n_species, const double* e0) {
   if (!c) return TGX_ERR_ARG;
+  if (md_busy(c)) return TGX_ERR_STATE;
   if (n_species < 0 || n_species > TGX_MAX_SPECIES || (n_species > 0 && !e0))
     return set_err(c, TGX_ERR_ARG, "bad e0 arguments");
   (void)hipSetDevice(c->device);
@@ -960,6 +984,7 @@ int tgx_add_2b_model(tgx_ctx* c, int sp1, int sp2, double rcut, double buffer, d
                      double sigma, int n_sparse, const double* Qs, const double* alphas,
                      const double* cutoff) {
   if (!c) return TGX_ERR_ARG;
+  if (md_busy(c)) return TGX_ERR_STATE;
   if (sp1 < 1 || sp2 < 1 || sp1 > TGX_MAX_SPECIES || sp2 > TGX_MAX_SPECIES || n_sparse < 1 ||
       !Qs || !alphas || !cutoff || !(rcut > 0) || !(sigma > 0) || buffer < 0)
     return set_err(c, TGX_ERR_ARG, "bad distance_2b arguments");
@@ -1016,6 +1041,7 @@ int tgx_add_2b_model(tgx_ctx* c, int sp1, int sp2, double rcut, double buffer, d
 
 int tgx_clear_2b_models(tgx_ctx* c) {
   if (!c) return TGX_ERR_ARG;
+  if (md_busy(c)) return TGX_ERR_STATE;
   (void)hipSetDevice(c->device);
   for (auto& b : c->twob) {
     (void)hipFree(b.Qs);
@@ -1031,6 +1057,7 @@ int tgx_neighbors_build(tgx_ctx* c, int n, const double* pos, const double* latt
                         int64_t cap, int* n_neigh, int* list, double* rjs, double* xyz,
                         double* thetas, double* phis, int64_t* n_pairs, int* indices_out) {
   if (!c) return TGX_ERR_ARG;
+  if (md_busy(c)) return TGX_ERR_STATE;
   if (n < 1 || !pos || !lattice || !n_pairs) return set_err(c, TGX_ERR_ARG, "bad arguments");
   (void)hipSetDevice(c->device);
   Cell cell;
@@ -1085,6 +1112,7 @@ int tgx_neighbors_build(tgx_ctx* c, int n, const double* pos, const double* latt
 int tgx_soap_descriptors(tgx_ctx* c, int n, const double* pos, const int* species,
                          const double* lattice, double* soap, int* n_sites_out, int* site_index) {
   if (!c) return TGX_ERR_ARG;
+  if (md_busy(c)) return TGX_ERR_STATE;
   if (n < 1 || !pos || !species || !lattice || !soap || !n_sites_out)
     return set_err(c, TGX_ERR_ARG, "bad arguments");
   if (!c->has_soap) return set_err(c, TGX_ERR_STATE, "no SOAP model loaded");
@@ -1133,6 +1161,7 @@ int tgx_soap_descriptors(tgx_ctx* c, int n, const double* pos, const int* specie
 int tgx_gap_kernel_step(tgx_ctx* c, int n_sites, const double* soap, double* energies,
                         double* qss) {
   if (!c) return TGX_ERR_ARG;
+  if (md_busy(c)) return TGX_ERR_STATE;
   if (n_sites < 0 || (n_sites > 0 && (!soap || !energies)))
     return set_err(c, TGX_ERR_ARG, "bad arguments");
   if (!c->has_soap) return set_err(c, TGX_ERR_STATE, "no SOAP model loaded");
@@ -1189,6 +1218,7 @@ int tgx_energy_forces(tgx_ctx* c, int n, const double* pos, const int* species,
                       const double* lattice, int do_forces, double* e_atom, double* forces,
                       double* virial) {
   if (!c) return TGX_ERR_ARG;
+  if (md_busy(c)) return TGX_ERR_STATE;
   if (n < 1 || !pos || !species || !lattice || !e_atom || (do_forces && (!forces || !virial)))
     return set_err(c, TGX_ERR_ARG, "bad arguments");
   (void)hipSetDevice(c->device);
@@ -1210,6 +1240,7 @@ int tgx_energy_forces_batched(tgx_ctx* c, int n_struct, const int* offsets, cons
                               const int* species, const double* lattices, int do_forces,
                               double* e_atom, double* forces, double* virials) {
   if (!c) return TGX_ERR_ARG;
+  if (md_busy(c)) return TGX_ERR_STATE;
   if (n_struct < 1 || !offsets || !pos || !species || !lattices || !e_atom ||
       (do_forces && (!forces || !virials)))
     return set_err(c, TGX_ERR_ARG, "bad arguments");
@@ -2219,6 +2250,7 @@ int tgx_stage_times(const tgx_ctx* c, double* ms7) {
 
 int tgx_set_list_buffer(tgx_ctx* c, double buffer) {
   if (!c) return TGX_ERR_ARG;
+  if (md_busy(c)) return TGX_ERR_STATE;
   if (!(buffer >= 0.0) || buffer > 10.0) return set_err(c, TGX_ERR_ARG, "buffer out of range");
   c->list_buffer = buffer;
   c->maxnb = 0;  // re-sized at the next build

@@ -1,25 +1,29 @@
-// SOAP launchers for one periodic cell, and the entry that picks the pair
-// geometry (one cell, or a cell per structure for batched structures).
-#include "tgx_soap_impl.h"
+// SOAP launchers: the entry that picks the pair geometry (one periodic cell,
+// or a cell per structure for batched structures).  The kernels and their
+// launchers are instantiated in tgx_soap_inst.hip (one object per direction,
+// precision and geometry).
+#include "tgx_device.h"
+#include "tgx_internal.h"
 
 namespace tgx {
 
-// Defined in tgx_soap_batch.hip.
-template <typename T>
-hipError_t launch_soap_fwd_many(const SoapDev& sdh, const SoapDev* sd, const double* W,
-                                const float* rtab, const double* pos, const int* species,
-                                const CellMany& g, int maxnb, const int* count, const int* list,
-                                const int* site_atom, int n_sites, const int* dmap, T* q,
-                                int q_ld, u16* qb, int qb_ld, T* cbuf, T* pnorm, hipStream_t st,
-                                int wpb);
-template <typename T>
-hipError_t launch_soap_bwd_many(const SoapDev& sdh, const SoapDev* sd, const double* W,
-                                const float* rtab, const double* pos, const int* species,
-                                const CellMany& g, int maxnb, const int* count, const int* list,
-                                const int* site_atom, int n_sites, const int* dmap, const T* q,
-                                int q_ld, const T* cbuf, const T* pnorm, const T* G, int g_ld,
-                                T* gpair, double* vsite, const TwoBodyDev* twob, int n2b,
-                                double* eatom, hipStream_t st, int wpb);
+typedef unsigned short u16;
+
+template <typename T, typename GEO>
+hipError_t launch_soap_fwd_g(const SoapDev& sdh, const SoapDev* sd, const double* W,
+                             const float* rtab, const double* pos, const int* species,
+                             const GEO& cell, int maxnb, const int* count, const int* list,
+                             const int* site_atom, int n_sites, const int* dmap, T* q, int q_ld,
+                             u16* qb, int qb_ld, T* cbuf, T* pnorm, hipStream_t st, int split,
+                             int wpb);
+template <typename T, typename GEO>
+hipError_t launch_soap_bwd_g(const SoapDev& sdh, const SoapDev* sd, const double* W,
+                             const float* rtab, const double* pos, const int* species,
+                             const GEO& cell, int maxnb, const int* count, const int* list,
+                             const int* site_atom, int n_sites, const int* dmap, const T* q,
+                             int q_ld, const T* cbuf, const T* pnorm, const T* G, int g_ld,
+                             T* gpair, double* vsite, const TwoBodyDev* twob, int n2b,
+                             double* eatom, hipStream_t st, int wpb);
 
 template <typename T>
 hipError_t launch_soap_fwd(const SoapDev& sdh, const SoapDev* sd, const double* W,
@@ -29,9 +33,10 @@ hipError_t launch_soap_fwd(const SoapDev& sdh, const SoapDev* sd, const double* 
                            u16* qb, int qb_ld, T* cbuf, T* pnorm, hipStream_t st, int split,
                            int wpb) {
   if (g.many)
-    return launch_soap_fwd_many<T>(sdh, sd, W, rtab, pos, species, CellMany{g.many, g.sid},
-                                   maxnb, count, list, site_atom, n_sites, dmap, q, q_ld, qb,
-                                   qb_ld, cbuf, pnorm, st, wpb);
+    return launch_soap_fwd_g<T, CellMany>(sdh, sd, W, rtab, pos, species,
+                                          CellMany{g.many, g.sid}, maxnb, count, list, site_atom,
+                                          n_sites, dmap, q, q_ld, qb, qb_ld, cbuf, pnorm, st, 0,
+                                          wpb);
   return launch_soap_fwd_g<T, CellOne>(sdh, sd, W, rtab, pos, species, CellOne{g.one}, maxnb,
                                        count, list, site_atom, n_sites, dmap, q, q_ld, qb, qb_ld,
                                        cbuf, pnorm, st, split, wpb);
@@ -46,9 +51,10 @@ hipError_t launch_soap_bwd(const SoapDev& sdh, const SoapDev* sd, const double* 
                            T* gpair, double* vsite, const TwoBodyDev* twob, int n2b,
                            double* eatom, hipStream_t st, int wpb) {
   if (g.many)
-    return launch_soap_bwd_many<T>(sdh, sd, W, rtab, pos, species, CellMany{g.many, g.sid},
-                                   maxnb, count, list, site_atom, n_sites, dmap, q, q_ld, cbuf,
-                                   pnorm, G, g_ld, gpair, vsite, twob, n2b, eatom, st, wpb);
+    return launch_soap_bwd_g<T, CellMany>(sdh, sd, W, rtab, pos, species,
+                                          CellMany{g.many, g.sid}, maxnb, count, list, site_atom,
+                                          n_sites, dmap, q, q_ld, cbuf, pnorm, G, g_ld, gpair,
+                                          vsite, twob, n2b, eatom, st, wpb);
   return launch_soap_bwd_g<T, CellOne>(sdh, sd, W, rtab, pos, species, CellOne{g.one}, maxnb,
                                        count, list, site_atom, n_sites, dmap, q, q_ld, cbuf,
                                        pnorm, G, g_ld, gpair, vsite, twob, n2b, eatom, st, wpb);

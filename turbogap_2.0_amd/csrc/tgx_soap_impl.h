@@ -872,7 +872,7 @@ hipError_t launch_soap_fwd_g(const SoapDev& sdh, const SoapDev* sd, const double
                            const GEO& cell, int maxnb, const int* count, const int* list,
                            const int* site_atom, int n_sites, const int* dmap, T* q, int q_ld,
                            u16* qb, int qb_ld, T* cbuf, T* pnorm, hipStream_t st,
-                           int split = 0, int wpb = 0) {
+                           int split, int wpb) {
   if (n_sites == 0) return hipSuccess;
   return soap_dispatch(sdh, [&](auto NR, auto NM, auto LM, auto EX) {
     return fwd_inst<T, decltype(NR)::value, decltype(NM)::value, decltype(LM)::value,
@@ -889,7 +889,7 @@ hipError_t launch_soap_bwd_g(const SoapDev& sdh, const SoapDev* sd, const double
                            const int* site_atom, int n_sites, const int* dmap, const T* q,
                            int q_ld, const T* cbuf, const T* pnorm, const T* G, int g_ld,
                            T* gpair, double* vsite, const TwoBodyDev* twob, int n2b,
-                           double* eatom, hipStream_t st, int wpb = 0) {
+                           double* eatom, hipStream_t st, int wpb) {
   if (n_sites == 0) return hipSuccess;
   return soap_dispatch(sdh, [&](auto NR, auto NM, auto LM, auto EX) {
     return bwd_inst<T, decltype(NR)::value, decltype(NM)::value, decltype(LM)::value,
