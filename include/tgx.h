@@ -181,6 +181,27 @@ int tgx_energy_forces(tgx_ctx* ctx, int n, const double* pos,
                       double* virial);
 
 /*
+ * tgx_energy_forces without any device allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory -- code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * Same arguments, results and errors as tgx_energy_forces (host buffers in
+ * and out; the model set beforehand stays where it is).  Every work buffer
+ * of the call (structure copy, neighbour list and cells, descriptors,
+ * kernel-step and adjoint work, outputs before the copy back) comes from one
+ * 64 MiB array per GPU that the library holds until the process ends; the
+ * call makes no hipMalloc/hipFree.  A call that would need more (roughly
+ * more than 3,500 atoms of a 324-dim, 500-sparse model in FP64, 7,000 in
+ * mixed precision) returns TGX_ERR_CAPACITY and leaves the context usable.
+ * All calls on one GPU share that array, across contexts: calls that
+ * overlap on one GPU (other host threads) race and must be serialised by
+ * the caller.  The autotuner does not time candidates inside this call.
+ */
+int tgx_energy_forces_noalloc(tgx_ctx* ctx, int n, const double* pos,
+                              const int* species, const double* lattice,
+                              int do_forces, double* e_atom, double* forces,
+                              double* virial);
+
+/*
  * Many independent structures in one call (structures s = 0..n_struct-1 own
  * atoms offsets[s]..offsets[s+1]-1 of pos/species/e_atom/forces; lattices
  * holds 9 doubles per structure, virials 9 per structure).

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -150,10 +151,18 @@ namespace {
 
 #define ENSURE(ctx, buf, nbytes)                                                     \
   do {                                                                               \
-    if ((ctx)->buf.ensure(nbytes))                                                   \
-      return set_err(ctx, TGX_ERR_HIP, "device allocation of %zu bytes failed (%s)", \
-                     (size_t)(nbytes), #buf);                                        \
+    if (grow((ctx), (ctx)->buf, (nbytes))) return grow_err((ctx), (nbytes), #buf);   \
   } while (0)
+
+// Bump region of the per-device scratch array that the no-allocation entry
+// point hands its call: while `on`, buffers are carved from it instead of
+// allocated, and the context's own buffers are restored afterwards.
+struct Arena {
+  bool on = false, overflow = false;
+  char* base = nullptr;
+  size_t size = 0, off = 0;
+  std::vector<std::pair<DevBuf*, DevBuf>> saved;  // carved buffer, its own value
+};
 
 struct MdState {
   bool active = false;
@@ -186,6 +195,7 @@ struct Extra {
   std::string tune_path;
   std::map<std::string, int> tune_choice;
   DevBuf tune_save;
+  Arena arena;
   hipEvent_t ev[8];
   hipEvent_t kev[8];  // brackets of single kernels: fwd, GEMMs, adjoint, gather
   bool ev_ready = false;
@@ -202,6 +212,38 @@ struct tgx_ctx_full {
 };
 
 static Extra& X(tgx_ctx* c) { return reinterpret_cast<tgx_ctx_full*>(c)->x; }
+
+// Makes b hold at least want bytes: DevBuf::ensure (which may allocate), or,
+// while the context's arena is on, a region of the scratch array (a buffer
+// carved earlier in the same call is reused when large enough).  Non-zero on
+// failure.
+static int grow(tgx_ctx* c, DevBuf& b, size_t want) {
+  Arena& a = X(c).arena;
+  if (!a.on) return b.ensure(want);
+  bool carved = false;
+  for (const auto& e : a.saved)
+    if (e.first == &b) carved = true;
+  if (carved && b.bytes >= want) return 0;
+  const size_t off = (a.off + 255) & ~(size_t)255;
+  if (want > a.size || off > a.size - want) {
+    a.overflow = true;
+    return 1;
+  }
+  if (!carved) a.saved.emplace_back(&b, b);
+  b.p = a.base + off;
+  b.bytes = want;
+  a.off = off + want;
+  return 0;
+}
+
+static int grow_err(tgx_ctx* c, size_t want, const char* what) {
+  if (X(c).arena.on && X(c).arena.overflow)
+    return set_err(c, TGX_ERR_CAPACITY,
+                   "the call needs more than the %zu-byte scratch array (%s: %zu bytes at "
+                   "offset %zu)",
+                   X(c).arena.size, what, want, X(c).arena.off);
+  return set_err(c, TGX_ERR_HIP, "device allocation of %zu bytes failed (%s)", want, what);
+}
 
 static size_t tsize(int prec) { return prec == TGX_FP64 ? 8 : 4; }
 
@@ -220,8 +262,8 @@ static int build_list(tgx_ctx* c, int n, const Cell& cell, double rcut, bool wan
   ENSURE(c, cell_of, sizeof(int) * n);
   ENSURE(c, cell_atoms, cell_atoms_bytes(n));
   ENSURE(c, cell_start, sizeof(int) * (ncell + 1));
-  if (x.cell_count.ensure(sizeof(int) * ncell) || x.cell_fill.ensure(sizeof(int) * ncell))
-    return set_err(c, TGX_ERR_HIP, "device allocation failed (cells)");
+  if (grow(c, x.cell_count, sizeof(int) * ncell) || grow(c, x.cell_fill, sizeof(int) * ncell))
+    return grow_err(c, sizeof(int) * ncell, "cells");
   ENSURE(c, nb_count, sizeof(int) * n);
   ENSURE(c, nb_flag, sizeof(int) * 4);
   int maxnb = c->maxnb;
@@ -547,8 +589,8 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
   ENSURE(c, forces, 24 * (size_t)n);
   ENSURE(c, vsite, 48 * (size_t)std::max(n_sites, 1));
   ENSURE(c, virial, 48 * (size_t)n);  // per-atom 2b virial
-  if (x.out.ensure((16 + 8 * 128) * sizeof(double)))
-    return set_err(c, TGX_ERR_HIP, "allocation failed");
+  if (grow(c, x.out, (16 + 8 * 128) * sizeof(double)))
+    return grow_err(c, (16 + 8 * 128) * sizeof(double), "out");
   // fused 2b: every atom is a site, so the kernel step writes the atom
   // energies, the force gather overwrites the forces and no per-atom virial
   // is used -- nothing to initialise
@@ -602,8 +644,7 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
       hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
       if (hipStreamIsCapturing(c->stream, &cs) == hipSuccess &&
           cs == hipStreamCaptureStatusNone) {
-        if (x.tune_save.ensure(8 * (size_t)n))
-          return set_err(c, TGX_ERR_HIP, "allocation failed");
+        if (grow(c, x.tune_save, 8 * (size_t)n)) return grow_err(c, 8 * (size_t)n, "tune_save");
         TGX_HIP(c, hipMemcpyAsync(x.tune_save.p, c->eatom.p, 8 * (size_t)n,
                                   hipMemcpyDeviceToDevice, c->stream));
         snap = true;
@@ -707,6 +748,36 @@ static int upload_structure(tgx_ctx* c, int n, const double* pos, const int* spe
   return TGX_OK;
 }
 
+// Scratch array of tgx_energy_forces_noalloc: every device buffer that call
+// needs beyond the model (structure, list, cells, descriptors, kernel-step
+// and adjoint work, outputs) is carved from it, so the call allocates
+// nothing.  Each GPU holds its own copy for the life of the process; host
+// code uses only the per-device address taken with hipGetSymbolAddress at
+// tgx_create, never the array's name.
+#define TGX_SCRATCH_BYTES ((size_t)64 << 20)
+#define TGX_SCRATCH_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[TGX_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[TGX_SCRATCH_DEVICES];
+
+// Address of this device's scratch array (the current device must be
+// `device`); taken once per device.
+static hipError_t scratch_init(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0 || device >= TGX_SCRATCH_DEVICES) return hipErrorInvalidDevice;
+  if (g_scratch_ptr[device]) return hipSuccess;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch_ptr[device] = p;
+  return hipSuccess;
+}
+
+static void* scratch_ptr(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return (device >= 0 && device < TGX_SCRATCH_DEVICES) ? g_scratch_ptr[device] : nullptr;
+}
+
 // Entry points that write the context's structure, list or model buffers
 // refuse to run while an MD run holds them: the captured step graph reads
 // those buffers at fixed addresses.
@@ -734,7 +805,8 @@ int tgx_create(int device, int precision, tgx_ctx** out) {
   tgx_ctx* c = &f->c;
   c->device = device;
   c->precision = precision;
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+  if (hipSetDevice(device) != hipSuccess || scratch_init(device) != hipSuccess ||
+      hipStreamCreate(&c->stream) != hipSuccess) {
     delete f;
     return TGX_ERR_HIP;
   }
@@ -1234,6 +1306,51 @@ int tgx_energy_forces(tgx_ctx* c, int n, const double* pos, const int* species,
     TGX_HIP(c, hipMemcpy(virial, X(c).out.p, 72, hipMemcpyDeviceToHost));
   }
   return TGX_OK;
+}
+
+int tgx_energy_forces_noalloc(tgx_ctx* c, int n, const double* pos, const int* species,
+                              const double* lattice, int do_forces, double* e_atom,
+                              double* forces, double* virial) {
+  if (!c) return TGX_ERR_ARG;
+  if (md_busy(c)) return TGX_ERR_STATE;
+  if (n < 1 || !pos || !species || !lattice || !e_atom || (do_forces && (!forces || !virial)))
+    return set_err(c, TGX_ERR_ARG, "bad arguments");
+  (void)hipSetDevice(c->device);
+  void* base = scratch_ptr(c->device);
+  if (!base) return set_err(c, TGX_ERR_STATE, "no scratch array on device %d", c->device);
+  Extra& x = X(c);
+  Arena& a = x.arena;
+  a.on = true;
+  a.overflow = false;
+  a.base = static_cast<char*>(base);
+  a.size = TGX_SCRATCH_BYTES;
+  a.off = 0;
+  a.saved.clear();
+  // no autotuner timing here: its candidates would need buffers of their own
+  const int tune = x.tune_on;
+  x.tune_on = 0;
+  int rcode = upload_structure(c, n, pos, species);
+  if (!rcode) rcode = run_ef(c, n, lattice, species, do_forces);
+  if (!rcode) {
+    if (hipStreamSynchronize(c->stream) != hipSuccess)
+      rcode = set_err(c, TGX_ERR_HIP, "stream synchronisation failed");
+  }
+  if (!rcode) {
+    collect_times(c);
+    if (hipMemcpy(e_atom, c->eatom.p, 8 * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess ||
+        (do_forces &&
+         (hipMemcpy(forces, c->forces.p, 24 * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(virial, x.out.p, 72, hipMemcpyDeviceToHost) != hipSuccess)))
+      rcode = set_err(c, TGX_ERR_HIP, "copy of the results failed");
+  } else {
+    (void)hipStreamSynchronize(c->stream);  // nothing of this call still runs
+  }
+  // the context gets its own buffers back
+  a.on = false;
+  for (auto it = a.saved.rbegin(); it != a.saved.rend(); ++it) *it->first = it->second;
+  a.saved.clear();
+  x.tune_on = tune;
+  return rcode;
 }
 
 int tgx_energy_forces_batched(tgx_ctx* c, int n_struct, const int* offsets, const double* pos,

@@ -239,6 +239,7 @@ def load_library(path: str = LIB_PATH):
         "tgx_soap_descriptors": (i, [P, i, dp, ip, dp, dp, ip, ip]),
         "tgx_gap_kernel_step": (i, [P, i, dp, dp, dp]),
         "tgx_energy_forces": (i, [P, i, dp, ip, dp, i, dp, dp, dp]),
+        "tgx_energy_forces_noalloc": (i, [P, i, dp, ip, dp, i, dp, dp, dp]),
         "tgx_energy_forces_batched": (i, [P, i, ip, dp, ip, dp, i, dp, dp, dp]),
         "tgx_md_run": (i, [P, i, i, d, d, ip, dp, dp, dp, dp, dp, dp, dp, ip]),
         "tgx_stage_times": (i, [P, dp]),
@@ -401,6 +402,19 @@ class Engine:
         v = np.zeros((3, 3))
         self._check(self.lib.tgx_energy_forces(self.ctx, n, _dptr(pos), _iptr(sp), _dptr(lat),
                                                int(do_forces), _dptr(e), _dptr(f), _dptr(v)))
+        return e, f, v.T.copy()
+
+    def energy_forces_noalloc(self, pos, species, lattice, do_forces: bool = True):
+        """energy_forces with every work buffer in the library's per-GPU
+        scratch array (tgx_energy_forces_noalloc): no device allocation."""
+        pos, lat, sp = _f64(pos), _f64(lattice), _i32(species)
+        n = pos.shape[0]
+        e = np.zeros(n)
+        f = np.zeros((n, 3))
+        v = np.zeros((3, 3))
+        self._check(self.lib.tgx_energy_forces_noalloc(self.ctx, n, _dptr(pos), _iptr(sp),
+                                                       _dptr(lat), int(do_forces), _dptr(e),
+                                                       _dptr(f), _dptr(v)))
         return e, f, v.T.copy()
 
     def energy_forces_batched(self, structures, do_forces: bool = True):
