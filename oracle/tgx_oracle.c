@@ -911,31 +911,61 @@ void orc_gap_kernel_step(int n_sites, int dim, int n_sparse,
   int is_int = fabs(zeta0 - (double)(int)zeta0) < 1e-5;
   int zi = (int)zeta0;
   double zeta = is_int ? (double)zi : zeta0;
-  double* kd = (double*)malloc(sizeof(double) * (size_t)(n_sparse > 0 ? n_sparse : 1));
-#pragma omp parallel for schedule(static) firstprivate(kd)
-  for (int i = 0; i < n_sites; i++) {
-    double* kr = (double*)malloc(sizeof(double) * (size_t)(n_sparse > 0 ? n_sparse : 1));
-    const double* qi = soap + (size_t)i * dim;
-    double e = 0.0;
-    for (int s = 0; s < n_sparse; s++) {
-      const double* qs = Qs + (size_t)s * dim;
-      double k = 0.0;
-      for (int d = 0; d < dim; d++) k += qi[d] * qs[d];
-      double kz = is_int ? ipow(k, zi) : pow(k, zeta);
-      e += kz * alphas[s];
-      kr[s] = (is_int ? ipow(k, zi - 1) : pow(k, zeta - 1.0)) * alphas[s];
-    }
-    energies[i] = delta * delta * e;
-    if (qss) {
+  /* Blocks of 4 sites x 4 sparse points share their row loads; every K(i,s)
+   * is still the sum over d in ascending order, every energy the sum over s
+   * in ascending order, as the unblocked loops give. */
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int i0 = 0; i0 < n_sites; i0 += 4) {
+    const int ni = n_sites - i0 < 4 ? n_sites - i0 : 4;
+    double* kr = (double*)malloc(sizeof(double) * 4 * (size_t)(n_sparse > 0 ? n_sparse : 1));
+    const double* qi[4];
+    for (int a = 0; a < 4; a++) qi[a] = soap + (size_t)(i0 + (a < ni ? a : ni - 1)) * dim;
+    double e[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int s0 = 0; s0 < n_sparse; s0 += 4) {
+      const int nsb = n_sparse - s0 < 4 ? n_sparse - s0 : 4;
+      const double* qs[4];
+      for (int b = 0; b < 4; b++) qs[b] = Qs + (size_t)(s0 + (b < nsb ? b : nsb - 1)) * dim;
+      double k[4][4];
+      for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) k[a][b] = 0.0;
       for (int d = 0; d < dim; d++) {
-        double acc = 0.0;
-        for (int s = 0; s < n_sparse; s++) acc += kr[s] * Qs[(size_t)s * dim + d];
-        qss[(size_t)d * n_sites + i] = -zeta * delta * delta * acc;
+        const double x0 = qi[0][d], x1 = qi[1][d], x2 = qi[2][d], x3 = qi[3][d];
+        for (int b = 0; b < 4; b++) {
+          const double y = qs[b][d];
+          k[0][b] += x0 * y;
+          k[1][b] += x1 * y;
+          k[2][b] += x2 * y;
+          k[3][b] += x3 * y;
+        }
       }
+      for (int a = 0; a < ni; a++)
+        for (int b = 0; b < nsb; b++) {
+          const int s = s0 + b;
+          const double kk = k[a][b];
+          double kz = is_int ? ipow(kk, zi) : pow(kk, zeta);
+          e[a] += kz * alphas[s];
+          kr[(size_t)a * n_sparse + s] = (is_int ? ipow(kk, zi - 1) : pow(kk, zeta - 1.0)) * alphas[s];
+        }
+    }
+    for (int a = 0; a < ni; a++) energies[i0 + a] = delta * delta * e[a];
+    if (qss) {
+      /* acc[d] = sum over s (ascending) of kr[s] Qs[s][d], Qs read row by row */
+      double* acc = (double*)calloc((size_t)4 * dim, sizeof(double));
+      for (int s = 0; s < n_sparse; s++) {
+        const double* qr = Qs + (size_t)s * dim;
+        for (int a = 0; a < ni; a++) {
+          const double ks = kr[(size_t)a * n_sparse + s];
+          double* ac = acc + (size_t)a * dim;
+          for (int d = 0; d < dim; d++) ac[d] += ks * qr[d];
+        }
+      }
+      for (int a = 0; a < ni; a++)
+        for (int d = 0; d < dim; d++)
+          qss[(size_t)d * n_sites + i0 + a] = -zeta * delta * delta * acc[(size_t)a * dim + d];
+      free(acc);
     }
     free(kr);
   }
-  free(kd);
 }
 
 void orc_gap_soap_ef(int n_sites, int dim, int n_sparse, const double* soap,
@@ -967,15 +997,16 @@ void orc_gap_soap_ef(int n_sites, int dim, int n_sparse, const double* soap,
 #endif
     double* F = fth + (size_t)t * 3 * n_atoms;
     double* V = vth + (size_t)t * 9;
+    double* qrow = (double*)malloc(sizeof(double) * (size_t)dim);
 #pragma omp for schedule(dynamic, 8)
     for (int i = 0; i < n_sites; i++) {
+      for (int d = 0; d < dim; d++) qrow[d] = qss[(size_t)d * n_sites + i];
       for (int64_t l = off[i]; l < off[i + 1]; l++) {
         int j2 = pair_atom[l];
         double f[3];
         for (int k = 0; k < 3; k++) {
           double acc = 0.0;
-          for (int d = 0; d < dim; d++)
-            acc += qss[(size_t)d * n_sites + i] * soap_der[((size_t)l * dim + d) * 3 + k];
+          for (int d = 0; d < dim; d++) acc += qrow[d] * soap_der[((size_t)l * dim + d) * 3 + k];
           f[k] = acc;
           F[3 * j2 + k] += acc;
         }
@@ -984,6 +1015,7 @@ void orc_gap_soap_ef(int n_sites, int dim, int n_sparse, const double* soap,
             V[k2 * 3 + k1] += 0.5 * (f[k1] * xyz[3 * l + k2] + f[k2] * xyz[3 * l + k1]);
       }
     }
+    free(qrow);
   }
   for (int t = 0; t < nth; t++) {
     for (size_t a = 0; a < (size_t)3 * n_atoms; a++) forces[a] += fth[(size_t)t * 3 * n_atoms + a];
