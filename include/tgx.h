@@ -36,6 +36,7 @@ extern "C" {
 #define TGX_MAX_SPECIES 8
 #define TGX_MAX_LMAX 10  /* (l_max+1)^2 <= 121 real harmonics      */
 #define TGX_MAX_NR 32    /* sum over species of alpha_max (n_max)  */
+#define TGX_MAX_SOAP_BLOCKS 8  /* soap_turbo blocks of one model        */
 
 enum {
   TGX_OK = 0,
@@ -111,6 +112,25 @@ int tgx_set_autotune(tgx_ctx* ctx, int on, const char* cache_path);
 int tgx_set_soap_model(tgx_ctx* ctx, const tgx_soap_hypers* h, int dim,
                        int n_sparse, const double* Qs, const double* alphas,
                        double delta, double zeta);
+
+/* Further soap_turbo blocks of the same model.  The reference reads any
+ * number of soap_turbo blocks from the .gap file (read_gap.f90:150-299,
+ * n_soap_turbo) and calculate_gap_soap.f90:324-447 evaluates them in turn,
+ * adding each block's energies, forces and virial.  tgx_set_soap_model
+ * replaces the model by one block; tgx_add_soap_model appends a block
+ * (arguments as tgx_set_soap_model; with no block loaded it acts as
+ * tgx_set_soap_model).  Every block uses the model's species list (the same
+ * n_species and species order, h->n_species of the first block); its
+ * central_species selects its sites.  At most TGX_MAX_SOAP_BLOCKS blocks.
+ * tgx_clear_soap_models removes all blocks; tgx_n_soap_models returns how
+ * many are loaded.  tgx_soap_descriptors, tgx_gap_kernel_step and the Monte
+ * Carlo entry points work on the first block (Monte Carlo refuses models of
+ * more than one block with TGX_ERR_UNSUPPORTED). */
+int tgx_add_soap_model(tgx_ctx* ctx, const tgx_soap_hypers* h, int dim,
+                       int n_sparse, const double* Qs, const double* alphas,
+                       double delta, double zeta);
+int tgx_clear_soap_models(tgx_ctx* ctx);
+int tgx_n_soap_models(const tgx_ctx* ctx);
 
 /* Per-species isolated-atom energies added to every atom of that species
  * (calculate_e0, gap/calculate_gap_soap.f90:135-169). */

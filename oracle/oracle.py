@@ -34,11 +34,15 @@ class C2b(C.Structure):
 
 
 class CModel(C.Structure):
-    _fields_ = [("h", CSoapHypers), ("has_soap", C.c_int32), ("dim", C.c_int32),
-                ("n_sparse", C.c_int32), ("n2b", C.c_int32),
-                ("Qs", C.POINTER(C.c_double)), ("alphas", C.POINTER(C.c_double)),
-                ("delta", C.c_double), ("zeta", C.c_double),
-                ("e0", C.c_double * 8), ("b2", C.POINTER(C2b))]
+    pass
+
+
+CModel._fields_ = [("h", CSoapHypers), ("has_soap", C.c_int32), ("dim", C.c_int32),
+                   ("n_sparse", C.c_int32), ("n2b", C.c_int32),
+                   ("Qs", C.POINTER(C.c_double)), ("alphas", C.POINTER(C.c_double)),
+                   ("delta", C.c_double), ("zeta", C.c_double),
+                   ("e0", C.c_double * 8), ("b2", C.POINTER(C2b)),
+                   ("n_more", C.c_int32), ("pad_", C.c_int32), ("more", C.POINTER(CModel))]
 
 
 def build():
@@ -240,6 +244,19 @@ def _cmodel(model: GapModel):
     keep.append(arr)
     cm.n2b = len(model.two_body)
     cm.b2 = C.cast(arr, C.POINTER(C2b))
+    more = list(getattr(model, "more_soap", []) or [])
+    marr = (CModel * max(1, len(more)))()
+    for k, blk in enumerate(more):
+        q, a = _f64(blk.Qs), _f64(blk.alphas)
+        keep += [q, a]
+        marr[k].h = blk.hypers.to_c()
+        marr[k].has_soap = 1
+        marr[k].dim, marr[k].n_sparse = q.shape[1], q.shape[0]
+        marr[k].Qs, marr[k].alphas = _d(q), _d(a)
+        marr[k].delta, marr[k].zeta = blk.delta, blk.zeta
+    keep.append(marr)
+    cm.n_more = len(more)
+    cm.more = C.cast(marr, C.POINTER(CModel))
     return cm, keep
 
 

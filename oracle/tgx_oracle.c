@@ -1085,6 +1085,44 @@ void orc_gap_2b_ef(const orc_2b* b, int n, const int* species,
 /* Whole path                                                               */
 /* ------------------------------------------------------------------------ */
 
+/* The SOAP term of one soap_turbo block on a full neighbour list, added to
+ * e_atom, forces and virial (get_gap_soap + get_soap_energy_and_forces as
+ * calculate_gap_soap.f90:359-447 call them; one chunk of chunk_sites sites
+ * at a time, the reference's memory split). */
+static void soap_block_ef(const orc_model* m, int n, const int* species, const int* nn,
+                          const int* list, const double* rjs, const double* xyz,
+                          int do_forces, int chunk_sites, double* e_atom, double* forces,
+                          double* virial, double* tf) {
+  double tv[9];
+  site_pairs sp;
+  select_sites(&m->h, n, species, nn, list, rjs, xyz, &sp);
+  soap_geom g;
+  soap_geom_init(&m->h, &g);
+  int dim = g.dim;
+  int cs = chunk_sites > 0 ? chunk_sites : (sp.n_sites > 0 ? sp.n_sites : 1);
+  for (int s0 = 0; s0 < sp.n_sites; s0 += cs) {
+    int s1 = s0 + cs < sp.n_sites ? s0 + cs : sp.n_sites;
+    int ns = s1 - s0;
+    int64_t p0 = sp.site_off[s0], p1 = sp.site_off[s1];
+    double* soap = (double*)malloc(sizeof(double) * (size_t)ns * dim);
+    double* der = do_forces ? (double*)malloc(sizeof(double) * (size_t)(p1 - p0) * dim * 3) : NULL;
+    soap_chunk(&m->h, &g, &sp, species, s0, s1, soap, der);
+    double* en = (double*)malloc(sizeof(double) * (size_t)ns);
+    orc_gap_soap_ef(ns, dim, m->n_sparse, soap, der, m->alphas, m->delta, m->zeta, 0.0,
+                    m->Qs, sp.site_nn + s0, sp.pair_atom + p0, sp.pair_xyz + 3 * p0, n,
+                    do_forces, en, tf, tv);
+    for (int i = 0; i < ns; i++) e_atom[sp.site_atom[s0 + i]] += en[i];
+    if (do_forces) {
+      for (size_t a = 0; a < (size_t)3 * n; a++) forces[a] += tf[a];
+      for (int a = 0; a < 9; a++) virial[a] += tv[a];
+    }
+    free(en);
+    free(soap);
+    free(der);
+  }
+  free_sites(&sp);
+}
+
 int orc_energy_forces(const orc_model* m, int n, const double* pos,
                       const int* species, const double* lat, int do_forces,
                       int chunk_sites, double* e_atom, double* forces,
@@ -1092,6 +1130,11 @@ int orc_energy_forces(const orc_model* m, int n, const double* pos,
   double rc = 0.0;
   if (m->has_soap && hypers_supported(&m->h) != TGX_OK) return TGX_ERR_UNSUPPORTED;
   if (m->has_soap) rc = soap_rcut_max(&m->h);
+  for (int k = 0; m->has_soap && k < m->n_more; k++) {
+    if (hypers_supported(&m->more[k].h) != TGX_OK) return TGX_ERR_UNSUPPORTED;
+    const double r = soap_rcut_max(&m->more[k].h);
+    rc = r > rc ? r : rc;
+  }
   for (int b = 0; b < m->n2b; b++) rc = m->b2[b].rcut > rc ? m->b2[b].rcut : rc;
   int64_t np = 0;
   int r0 = orc_neighbors(n, pos, lat, rc, 0, 0, NULL, NULL, NULL, NULL, NULL, NULL, &np, NULL);
@@ -1110,33 +1153,12 @@ int orc_energy_forces(const orc_model* m, int n, const double* pos,
   double* tf = (double*)malloc(sizeof(double) * 3 * (size_t)n);
   double tv[9];
   if (m->has_soap) {
-    site_pairs sp;
-    select_sites(&m->h, n, species, nn, list, rjs, xyz, &sp);
-    soap_geom g;
-    soap_geom_init(&m->h, &g);
-    int dim = g.dim;
-    int cs = chunk_sites > 0 ? chunk_sites : (sp.n_sites > 0 ? sp.n_sites : 1);
-    for (int s0 = 0; s0 < sp.n_sites; s0 += cs) {
-      int s1 = s0 + cs < sp.n_sites ? s0 + cs : sp.n_sites;
-      int ns = s1 - s0;
-      int64_t p0 = sp.site_off[s0], p1 = sp.site_off[s1];
-      double* soap = (double*)malloc(sizeof(double) * (size_t)ns * dim);
-      double* der = do_forces ? (double*)malloc(sizeof(double) * (size_t)(p1 - p0) * dim * 3) : NULL;
-      soap_chunk(&m->h, &g, &sp, species, s0, s1, soap, der);
-      double* en = (double*)malloc(sizeof(double) * (size_t)ns);
-      orc_gap_soap_ef(ns, dim, m->n_sparse, soap, der, m->alphas, m->delta, m->zeta, 0.0,
-                      m->Qs, sp.site_nn + s0, sp.pair_atom + p0, sp.pair_xyz + 3 * p0, n,
-                      do_forces, en, tf, tv);
-      for (int i = 0; i < ns; i++) e_atom[sp.site_atom[s0 + i]] += en[i];
-      if (do_forces) {
-        for (size_t a = 0; a < (size_t)3 * n; a++) forces[a] += tf[a];
-        for (int a = 0; a < 9; a++) virial[a] += tv[a];
-      }
-      free(en);
-      free(soap);
-      free(der);
-    }
-    free_sites(&sp);
+    /* the soap_turbo blocks in turn (calculate_gap_soap.f90:324-447) */
+    soap_block_ef(m, n, species, nn, list, rjs, xyz, do_forces, chunk_sites, e_atom, forces,
+                  virial, tf);
+    for (int k = 0; k < m->n_more; k++)
+      soap_block_ef(&m->more[k], n, species, nn, list, rjs, xyz, do_forces, chunk_sites, e_atom,
+                    forces, virial, tf);
   }
   double* te = (double*)malloc(sizeof(double) * (size_t)n);
   for (int b = 0; b < m->n2b; b++) {

@@ -29,8 +29,10 @@ typedef struct {
   const double* cutoff;
 } orc_2b;
 
-/* Whole model: one soap_turbo block, optional 2b blocks, e0 per species. */
-typedef struct {
+/* Whole model: a soap_turbo block, optional further soap_turbo blocks
+ * (more[0..n_more-1], of which only the SOAP fields are read), optional 2b
+ * blocks, e0 per species. */
+typedef struct orc_model {
   tgx_soap_hypers h;
   int32_t has_soap;
   int32_t dim, n_sparse;
@@ -40,6 +42,9 @@ typedef struct {
   double delta, zeta;
   double e0[TGX_MAX_SPECIES];
   const orc_2b* b2;
+  int32_t n_more;
+  int32_t pad_;
+  const struct orc_model* more;
 } orc_model;
 
 /* get_distance, orthorhombic branch (neighbors_interface.f90:547-581). */

@@ -725,3 +725,30 @@ def test_mc_driver_chi2_rejects_wrong_stage2_ratio(n):
     p, stat, nb = _chi2_pvalue(xe, prm, n, T, _EDGES1 if n == 1 else _EDGES2)
     print(f"n={n} wrong ratio chi2={stat:.1f}/{nb - 1} p={p:.3g}")
     assert p < 1e-6
+
+
+def test_two_soap_blocks_add_up():
+    """Several soap_turbo blocks (calculate_gap_soap.f90:324-447): the model
+    of two blocks gives e0 plus the sum of each block's energies, the sum of
+    their forces and virials -- checked against the two one-block models."""
+    import dataclasses
+
+    pos, sp, lat = S.cho_cell(60, 1.5, 1.0, 21)
+    h1 = dataclasses.replace(S.cho_hypers(), central_species=1)
+    h2 = dataclasses.replace(S.cho_hypers(), central_species=2, l_max=5, alpha_max=(4, 5, 3),
+                             rcut_hard=(3.4, 3.6, 3.5))
+    m1 = S.model_from_soaps(h1, oracle.soap(h1, pos, sp, lat)["soap"], 20, seed=1)
+    m2 = S.model_from_soaps(h2, oracle.soap(h2, pos, sp, lat)["soap"], 15, seed=2, zeta=3.0,
+                            delta=0.7)
+    e0 = np.array([-1.0, -0.5, -2.0])
+    m1.e0 = e0
+    m2.e0 = np.zeros(3)
+    both = dataclasses.replace(m1, more_soap=[m2])
+    e, f, v = oracle.energy_forces(both, pos, sp, lat)
+    e1, f1, v1 = oracle.energy_forces(m1, pos, sp, lat)
+    e2, f2, v2 = oracle.energy_forces(m2, pos, sp, lat)
+    assert np.abs(e - (e1 + e2)).max() < 1e-12
+    assert np.abs(f - (f1 + f2)).max() < 1e-12
+    assert np.abs(v - (v1 + v2)).max() < 1e-12
+    # each block's sites: block 2 contributes only to its central species' energies
+    assert np.all(e2[sp != 2] == 0.0) and np.any(e2[sp == 2] != 0.0)

@@ -96,6 +96,8 @@ hipError_t launch_mlp_cast(int D, const float* P, unsigned short* W1t, unsigned 
                            hipStream_t st);
 hipError_t launch_mc_init(int n, const int* site_of_atom, const double* esite, double* e_cur,
                           hipStream_t st);
+hipError_t launch_site_add(int n_sites, const int* site_atom, const double* esite,
+                           const double* vsite, double* eatom, double* vatom, hipStream_t st);
 hipError_t launch_energy_init(int n, const int* species, const double* e0,
                               const int* site_of_atom, const double* esite, double* eatom,
                               double* forces, double* vatom, hipStream_t st);
@@ -176,7 +178,7 @@ struct MdState {
 };
 
 struct Extra {
-  DevBuf sd_dev, pairn, e0_dev, iota, cell_count, cell_fill, out, stage;
+  DevBuf e0_dev, iota, cell_count, cell_fill, out, stage;
   DevBuf b_cells, b_sid, b_boff, b_aoff, b_out;  // batched structures
   DevBuf mc_pref, mc_ctl, mc_dctl, mc_ecur, mc_naff, mc_trace, mc_twob;  // Monte Carlo
   DevBuf mc_esur, mc_esur_cur, mc_sq, mc_sy;                          // delayed acceptance
@@ -293,34 +295,42 @@ static int build_list(tgx_ctx* c, int n, const Cell& cell, double rcut, bool wan
 }
 
 static double rcut_all(const tgx_ctx* c) {
-  double r = c->has_soap ? c->sd.rcut_max : 0.0;
+  double r = 0.0;
+  if (c->has_soap)
+    for (const auto& b : c->blocks) r = std::max(r, b->sd.rcut_max);
   for (const auto& b : c->twob) r = std::max(r, b.rcut);
   return r;
 }
 
-// Sites of the SOAP block: all atoms, or those of central_species.
+// Sites of every SOAP block: all atoms, or those of its central_species.
+// *n_sites receives the first block's count (each block keeps its own).
 static int setup_sites(tgx_ctx* c, int n, const int* species_host, int* n_sites) {
-  Extra& x = X(c);
-  ENSURE(c, site_of_atom, sizeof(int) * n);
-  ENSURE(c, site_atom, sizeof(int) * n);
-  std::vector<int> soa(n), sa;
-  sa.reserve(n);
-  for (int i = 0; i < n; i++) {
-    if (c->has_soap && (c->sd.central_species == 0 || species_host[i] == c->sd.central_species)) {
-      soa[i] = (int)sa.size();
-      sa.push_back(i);
-    } else {
-      soa[i] = -1;
+  SoapBlock* const keep = c->blk;
+  for (auto& bp : c->blocks) {
+    c->blk = bp.get();
+    SoapBlock& b = *bp;
+    ENSURE(c, blk->site_of_atom, sizeof(int) * n);
+    ENSURE(c, blk->site_atom, sizeof(int) * n);
+    std::vector<int> soa(n), sa;
+    sa.reserve(n);
+    for (int i = 0; i < n; i++) {
+      if (c->has_soap && (b.sd.central_species == 0 || species_host[i] == b.sd.central_species)) {
+        soa[i] = (int)sa.size();
+        sa.push_back(i);
+      } else {
+        soa[i] = -1;
+      }
     }
-  }
-  (void)x;
-  TGX_HIP(c, hipMemcpyAsync(c->site_of_atom.p, soa.data(), sizeof(int) * n, hipMemcpyHostToDevice,
-                            c->stream));
-  if (!sa.empty())
-    TGX_HIP(c, hipMemcpyAsync(c->site_atom.p, sa.data(), sizeof(int) * sa.size(),
+    TGX_HIP(c, hipMemcpyAsync(b.site_of_atom.p, soa.data(), sizeof(int) * n,
                               hipMemcpyHostToDevice, c->stream));
-  TGX_HIP(c, hipStreamSynchronize(c->stream));
-  *n_sites = (int)sa.size();
+    if (!sa.empty())
+      TGX_HIP(c, hipMemcpyAsync(b.site_atom.p, sa.data(), sizeof(int) * sa.size(),
+                                hipMemcpyHostToDevice, c->stream));
+    TGX_HIP(c, hipStreamSynchronize(c->stream));
+    b.n_sites = (int)sa.size();
+  }
+  c->blk = keep;
+  *n_sites = c->blocks[0]->n_sites;
   return TGX_OK;
 }
 
@@ -422,40 +432,40 @@ static int tune_pick(tgx_ctx* c, const std::string& key, int n, int dflt, F&& la
 // Waves per workgroup of the SOAP kernels: 0 = the built-in choice.
 static const int kTuneWpb[4] = {0, 1, 2, 4};
 
-// SOAP forward for the current list into c->q (and c->qb in mixed mode).
+// SOAP forward for the current list into c->blk->q (and c->blk->qb in mixed mode).
 // few: a few sites (a Monte Carlo trial) take the workgroup-per-site FP32
 // kernel; TGX_FWD_SPLIT=1 asks for it on every FP32 call of up to 256 sites.
 static int soap_forward(tgx_ctx* c, int n, const Geom& cell, int n_sites, bool few = false) {
   Extra& x = X(c);
   const int prec = c->precision;
   const size_t ts = tsize(prec);
-  const int NRP = c->sd.nrp;
-  ENSURE(c, q, ts * (size_t)std::max(n_sites, 1) * c->dim_pad);
-  ENSURE(c, cbuf, ts * (size_t)std::max(n_sites, 1) * NRP * c->sd.L2);
-  ENSURE(c, pnorm, ts * (size_t)std::max(n_sites, 1));
+  const int NRP = c->blk->sd.nrp;
+  ENSURE(c, blk->q, ts * (size_t)std::max(n_sites, 1) * c->blk->dim_pad);
+  ENSURE(c, blk->cbuf, ts * (size_t)std::max(n_sites, 1) * NRP * c->blk->sd.L2);
+  ENSURE(c, blk->pnorm, ts * (size_t)std::max(n_sites, 1));
   unsigned short* qb = nullptr;
   if (prec == TGX_MIXED) {
-    ENSURE(c, qb, 2 * (size_t)std::max(n_sites, 1) * c->dim_pad);
-    qb = c->qb.as<unsigned short>();
+    ENSURE(c, blk->qb, 2 * (size_t)std::max(n_sites, 1) * c->blk->dim_pad);
+    qb = c->blk->qb.as<unsigned short>();
   }
   const char* fs = getenv("TGX_FWD_SPLIT");
   const int split = (few || (fs && fs[0] == '1')) ? 1 : 0;
   auto fwd = [&](int wpb) -> hipError_t {
     if (prec == TGX_FP64)
-      return launch_soap_fwd<double>(c->sd, x.sd_dev.as<SoapDev>(), c->W_dev.as<double>(),
-                                     c->rtab.as<float>(), c->pos.as<double>(),
+      return launch_soap_fwd<double>(c->blk->sd, c->blk->sd_dev.as<SoapDev>(), c->blk->W_dev.as<double>(),
+                                     c->blk->rtab.as<float>(), c->pos.as<double>(),
                                      c->species.as<int>(), cell, c->maxnb,
                                      c->nb_count.as<int>(), c->nb_list.as<int>(),
-                                     c->site_atom.as<int>(), n_sites, x.pairn.as<int>(),
-                                     c->q.as<double>(), c->dim_pad, nullptr, 0,
-                                     c->cbuf.as<double>(), c->pnorm.as<double>(), c->stream, 0,
+                                     c->blk->site_atom.as<int>(), n_sites, c->blk->pairn.as<int>(),
+                                     c->blk->q.as<double>(), c->blk->dim_pad, nullptr, 0,
+                                     c->blk->cbuf.as<double>(), c->blk->pnorm.as<double>(), c->stream, 0,
                                      wpb);
-    return launch_soap_fwd<float>(c->sd, x.sd_dev.as<SoapDev>(), c->W_dev.as<double>(),
-                                  c->rtab.as<float>(), c->pos.as<double>(), c->species.as<int>(),
+    return launch_soap_fwd<float>(c->blk->sd, c->blk->sd_dev.as<SoapDev>(), c->blk->W_dev.as<double>(),
+                                  c->blk->rtab.as<float>(), c->pos.as<double>(), c->species.as<int>(),
                                   cell, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
-                                  c->site_atom.as<int>(), n_sites, x.pairn.as<int>(),
-                                  c->q.as<float>(), c->dim_pad, qb, c->dim_pad,
-                                  c->cbuf.as<float>(), c->pnorm.as<float>(), c->stream, split,
+                                  c->blk->site_atom.as<int>(), n_sites, c->blk->pairn.as<int>(),
+                                  c->blk->q.as<float>(), c->blk->dim_pad, qb, c->blk->dim_pad,
+                                  c->blk->cbuf.as<float>(), c->blk->pnorm.as<float>(), c->stream, split,
                                   wpb);
   };
   // TGX_SOAP_WPB=1|2|4 forces the waves per workgroup (for tests)
@@ -463,7 +473,7 @@ static int soap_forward(tgx_ctx* c, int n, const Geom& cell, int n_sites, bool f
   int wpb = fw ? atoi(fw) : 0;
   if (!split && !fw) {
     const std::string key =
-        bucket_key("fwd", prec, c->sd.nrp, c->sd.L2, cell.many ? 1 : 0, n_sites);
+        bucket_key("fwd", prec, c->blk->sd.nrp, c->blk->sd.L2, cell.many ? 1 : 0, n_sites);
     wpb = kTuneWpb[tune_pick(c, key, 4, 0, [&](int k) { return fwd(kTuneWpb[k]); })];
   }
   kev_rec(c, 0);
@@ -473,7 +483,7 @@ static int soap_forward(tgx_ctx* c, int n, const Geom& cell, int n_sites, bool f
   return TGX_OK;
 }
 
-// Kernel step on the descriptors in c->q / c->qb.
+// Kernel step on the descriptors in c->blk->q / c->blk->qb.
 // With eatom_out, every atom is a site (in atom order) and the kernel step
 // also writes eatom_out[i] = e0[species[i] - 1] + esite[i].
 static int kernel_step(tgx_ctx* c, int n_sites, bool want_grad, const int* guard = nullptr,
@@ -482,22 +492,22 @@ static int kernel_step(tgx_ctx* c, int n_sites, bool want_grad, const int* guard
   const size_t tin = prec == TGX_FP64 ? 8 : (prec == TGX_FP32 ? 4 : 2);
   const size_t tacc = prec == TGX_FP64 ? 8 : 4;
   const int ns = std::max(n_sites, 1);
-  ENSURE(c, wmat, tin * (size_t)ns * c->sp_pad);
-  ENSURE(c, epart, tacc * (size_t)ns * (c->sp_pad / 32));  // per 32 sparse points at most
-  ENSURE(c, esite, 8 * (size_t)ns);
-  if (want_grad) ENSURE(c, gmat, tacc * (size_t)ns * c->dim_pad);
+  ENSURE(c, blk->wmat, tin * (size_t)ns * c->blk->sp_pad);
+  ENSURE(c, blk->epart, tacc * (size_t)ns * (c->blk->sp_pad / 32));  // per 32 sparse points at most
+  ENSURE(c, blk->esite, 8 * (size_t)ns);
+  if (want_grad) ENSURE(c, blk->gmat, tacc * (size_t)ns * c->blk->dim_pad);
   const void *qa, *qs, *qsT, *al;
   if (prec == TGX_FP64) {
-    qa = c->q.p; qs = c->Qs64.p; qsT = c->QsT64.p; al = c->alph64.p;
+    qa = c->blk->q.p; qs = c->blk->Qs64.p; qsT = c->blk->QsT64.p; al = c->blk->alph64.p;
   } else if (prec == TGX_FP32) {
-    qa = c->q.p; qs = c->Qs32.p; qsT = c->QsT32.p; al = c->alph32.p;
+    qa = c->blk->q.p; qs = c->blk->Qs32.p; qsT = c->blk->QsT32.p; al = c->blk->alph32.p;
   } else {
-    qa = c->qb.p; qs = c->Qs16.p; qsT = c->QsT16.p; al = c->alph32.p;
+    qa = c->blk->qb.p; qs = c->blk->Qs16.p; qsT = c->blk->QsT16.p; al = c->blk->alph32.p;
   }
   auto step = [&](const GemmPlan* plan) {
-    return launch_gap_kernel(prec, n_sites, c->dim, c->dim_pad, c->n_sparse, c->sp_pad, qa, qs,
-                             qsT, al, c->delta, c->zeta, c->wmat.p, c->epart.p,
-                             c->esite.as<double>(), want_grad ? c->gmat.p : nullptr,
+    return launch_gap_kernel(prec, n_sites, c->blk->dim, c->blk->dim_pad, c->blk->n_sparse, c->blk->sp_pad, qa, qs,
+                             qsT, al, c->blk->delta, c->blk->zeta, c->blk->wmat.p, c->blk->epart.p,
+                             c->blk->esite.as<double>(), want_grad ? c->blk->gmat.p : nullptr,
                              want_grad ? 1 : 0, guard, c->stream, eatom_out,
                              eatom_out ? X(c).e0_dev.as<double>() : nullptr,
                              eatom_out ? c->species.as<int>() : nullptr, few_parts, plan);
@@ -516,7 +526,7 @@ static int kernel_step(tgx_ctx* c, int n_sites, bool want_grad, const int* guard
   const bool force = fp && sscanf(fp, "%d %d %d", &forced.path, &forced.t0, &forced.t1) == 3;
   if (!guard && !force) {  // a guarded (Monte Carlo) step may not run: nothing to time
     const std::string key =
-        bucket_key("step", prec, want_grad ? 1 : 0, c->sp_pad, c->dim_pad, n_sites);
+        bucket_key("step", prec, want_grad ? 1 : 0, c->blk->sp_pad, c->blk->dim_pad, n_sites);
     pk = tune_pick(c, key, np, 0, [&](int k) { return step(&plans[k]); });
   }
   kev_rec(c, 2);
@@ -568,10 +578,11 @@ static int run_ef(tgx_ctx* c, int n, const double* lattice, const int* species_h
 // pairs instead of two) when every atom is a SOAP site and every 2b block's
 // cutoff lies within the SOAP cutoff of both its species.
 static bool fuse_two_body(const tgx_ctx* c, int n, int n_sites, int do_forces) {
-  if (!do_forces || !c->has_soap || c->twob.empty() || n_sites != n) return false;
+  if (!do_forces || !c->has_soap || c->twob.empty() || n_sites != n || c->blocks.size() != 1)
+    return false;
   for (const auto& b : c->twob) {
-    if (b.sp1 > c->sd.n_species || b.sp2 > c->sd.n_species) return false;
-    if (b.rcut > c->sd.rh[b.sp1 - 1] || b.rcut > c->sd.rh[b.sp2 - 1]) return false;
+    if (b.sp1 > c->blk->sd.n_species || b.sp2 > c->blk->sd.n_species) return false;
+    if (b.rcut > c->blk->sd.rh[b.sp1 - 1] || b.rcut > c->blk->sd.rh[b.sp2 - 1]) return false;
   }
   return true;
 }
@@ -579,15 +590,20 @@ static bool fuse_two_body(const tgx_ctx* c, int n, int n_sites, int do_forces) {
 // Energy/force launch sequence on the current list and sites.  Asynchronous:
 // no host synchronisation, no allocation once the buffers are sized (the MD
 // step graph captures it).
+static int ef_launch_multi(tgx_ctx* c, int n, const Geom& cell, int do_forces, bool events,
+                           int n_struct, bool defer_final);
+
 static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_forces,
                      bool events, int n_struct, bool defer_final) {
+  if (c->has_soap && c->blocks.size() > 1)
+    return ef_launch_multi(c, n, cell, do_forces, events, n_struct, defer_final);
   Extra& x = X(c);
   int rcode;
   x.ev_on = events;
   const int prec = c->precision;
   ENSURE(c, eatom, 8 * (size_t)n);
   ENSURE(c, forces, 24 * (size_t)n);
-  ENSURE(c, vsite, 48 * (size_t)std::max(n_sites, 1));
+  ENSURE(c, blk->vsite, 48 * (size_t)std::max(n_sites, 1));
   ENSURE(c, virial, 48 * (size_t)n);  // per-atom 2b virial
   if (grow(c, x.out, (16 + 8 * 128) * sizeof(double)))
     return grow_err(c, (16 + 8 * 128) * sizeof(double), "out");
@@ -609,8 +625,8 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
   }
   if (!fuse)
     TGX_HIP(c, launch_energy_init(n, c->species.as<int>(), x.e0_dev.as<double>(),
-                                  c->site_of_atom.as<int>(),
-                                  (c->has_soap && n_sites > 0) ? c->esite.as<double>() : nullptr,
+                                  c->blk->site_of_atom.as<int>(),
+                                  (c->has_soap && n_sites > 0) ? c->blk->esite.as<double>() : nullptr,
                                   c->eatom.as<double>(), c->forces.as<double>(),
                                   c->virial.as<double>(), c->stream));
   if (do_forces && c->has_soap && n_sites > 0) {
@@ -619,25 +635,25 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
     auto bwd = [&](int wpb) -> hipError_t {
       if (prec == TGX_FP64)
         return launch_soap_bwd<double>(
-            c->sd, x.sd_dev.as<SoapDev>(), c->W_dev.as<double>(), c->rtab.as<float>(),
+            c->blk->sd, c->blk->sd_dev.as<SoapDev>(), c->blk->W_dev.as<double>(), c->blk->rtab.as<float>(),
             c->pos.as<double>(), c->species.as<int>(), cell, c->maxnb, c->nb_count.as<int>(),
-            c->nb_list.as<int>(), c->site_atom.as<int>(), n_sites, x.pairn.as<int>(),
-            c->q.as<double>(), c->dim_pad, c->cbuf.as<double>(), c->pnorm.as<double>(),
-            c->gmat.as<double>(), c->dim_pad, c->gpair.as<double>(), c->vsite.as<double>(),
+            c->nb_list.as<int>(), c->blk->site_atom.as<int>(), n_sites, c->blk->pairn.as<int>(),
+            c->blk->q.as<double>(), c->blk->dim_pad, c->blk->cbuf.as<double>(), c->blk->pnorm.as<double>(),
+            c->blk->gmat.as<double>(), c->blk->dim_pad, c->gpair.as<double>(), c->blk->vsite.as<double>(),
             fuse ? x.twob_store.as<TwoBodyDev>() : nullptr, fuse ? (int)c->twob.size() : 0,
             c->eatom.as<double>(), c->stream, wpb);
       return launch_soap_bwd<float>(
-          c->sd, x.sd_dev.as<SoapDev>(), c->W_dev.as<double>(), c->rtab.as<float>(),
+          c->blk->sd, c->blk->sd_dev.as<SoapDev>(), c->blk->W_dev.as<double>(), c->blk->rtab.as<float>(),
           c->pos.as<double>(), c->species.as<int>(), cell, c->maxnb, c->nb_count.as<int>(),
-          c->nb_list.as<int>(), c->site_atom.as<int>(), n_sites, x.pairn.as<int>(),
-          c->q.as<float>(), c->dim_pad, c->cbuf.as<float>(), c->pnorm.as<float>(),
-          c->gmat.as<float>(), c->dim_pad, c->gpair.as<float>(), c->vsite.as<double>(),
+          c->nb_list.as<int>(), c->blk->site_atom.as<int>(), n_sites, c->blk->pairn.as<int>(),
+          c->blk->q.as<float>(), c->blk->dim_pad, c->blk->cbuf.as<float>(), c->blk->pnorm.as<float>(),
+          c->blk->gmat.as<float>(), c->blk->dim_pad, c->gpair.as<float>(), c->blk->vsite.as<double>(),
           fuse ? x.twob_store.as<TwoBodyDev>() : nullptr, fuse ? (int)c->twob.size() : 0,
           c->eatom.as<double>(), c->stream, wpb);
     };
     // the fused 2-body term adds to the atom energies: while candidates are
     // timed, every launch starts from a copy of them
-    const std::string bkey = bucket_key("bwd", prec, c->sd.nrp, c->sd.L2,
+    const std::string bkey = bucket_key("bwd", prec, c->blk->sd.nrp, c->blk->sd.L2,
                                         (fuse ? 2 : 0) + (cell.many ? 1 : 0), n_sites);
     bool snap = false;
     if (fuse && tune_enabled(c) && !x.tune_choice.count(bkey)) {
@@ -669,13 +685,13 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
       if (prec == TGX_FP64)
         e = launch_force_gather<double>(n, c->maxnb, c->nb_count.as<int>(),
                                         c->nb_list.as<int>(), c->nb_rev.as<int>(),
-                                        c->site_of_atom.as<int>(), n_sites == n ? 1 : 0,
+                                        c->blk->site_of_atom.as<int>(), n_sites == n ? 1 : 0,
                                         c->gpair.as<double>(), c->forces.as<double>(),
                                         c->stream, fuse ? 1 : 0);
       else
         e = launch_force_gather<float>(n, c->maxnb, c->nb_count.as<int>(),
                                        c->nb_list.as<int>(), c->nb_rev.as<int>(),
-                                       c->site_of_atom.as<int>(), n_sites == n ? 1 : 0,
+                                       c->blk->site_of_atom.as<int>(), n_sites == n ? 1 : 0,
                                        c->gpair.as<float>(), c->forces.as<double>(), c->stream,
                                        fuse ? 1 : 0);
     }
@@ -693,14 +709,14 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
   if (cell.many) {
     if (x.b_out.ensure(sizeof(double) * 10 * (size_t)n_struct))
       return set_err(c, TGX_ERR_HIP, "allocation failed");
-    TGX_HIP(c, launch_reduce_many(n_struct, x.b_aoff.as<int>(), c->site_of_atom.as<int>(),
-                                  (do_forces && c->has_soap && n_sites > 0) ? c->vsite.as<double>()
+    TGX_HIP(c, launch_reduce_many(n_struct, x.b_aoff.as<int>(), c->blk->site_of_atom.as<int>(),
+                                  (do_forces && c->has_soap && n_sites > 0) ? c->blk->vsite.as<double>()
                                                                             : nullptr,
                                   (do_forces && !fuse) ? c->virial.as<double>() : nullptr,
                                   c->eatom.as<double>(), x.b_out.as<double>(), c->stream));
   } else {
     TGX_HIP(c, launch_reduce((do_forces && c->has_soap) ? n_sites : 0,
-                             (do_forces && c->has_soap) ? c->vsite.as<double>() : nullptr, n,
+                             (do_forces && c->has_soap) ? c->blk->vsite.as<double>() : nullptr, n,
                              (do_forces && !fuse) ? c->virial.as<double>() : nullptr,
                              c->eatom.as<double>(), x.out.as<double>(), c->stream, !defer_final));
   }
@@ -733,10 +749,115 @@ static void collect_times(tgx_ctx* c) {
   (void)hipGetLastError();
 }
 
+// The same launch sequence for a model of several soap_turbo blocks
+// (calculate_gap_soap.f90:324-447 evaluates the blocks in turn and adds
+// their energies, forces and virials): e0 first, then per block the SOAP
+// forward, the kernel step and its site energies added to the atoms, the
+// adjoint with its forces gathered onto the atoms and its site virials added
+// to the per-atom virials; then the 2-body terms and the reductions.  Every
+// sum runs in a fixed order.
+static int ef_launch_multi(tgx_ctx* c, int n, const Geom& cell, int do_forces, bool events,
+                           int n_struct, bool defer_final) {
+  Extra& x = X(c);
+  x.ev_on = events;
+  const int prec = c->precision;
+  ENSURE(c, eatom, 8 * (size_t)n);
+  ENSURE(c, forces, 24 * (size_t)n);
+  ENSURE(c, virial, 48 * (size_t)n);
+  if (grow(c, x.out, (16 + 8 * 128) * sizeof(double)))
+    return grow_err(c, (16 + 8 * 128) * sizeof(double), "out");
+  TGX_HIP(c, launch_energy_init(n, c->species.as<int>(), x.e0_dev.as<double>(),
+                                c->blocks[0]->site_of_atom.as<int>(), nullptr,
+                                c->eatom.as<double>(), c->forces.as<double>(),
+                                c->virial.as<double>(), c->stream));
+  ev_rec(c, 2);
+  int rcode = TGX_OK;
+  for (auto& bp : c->blocks) {
+    c->blk = bp.get();
+    const int ns = c->blk->n_sites;
+    if (ns == 0) continue;
+    rcode = soap_forward(c, n, cell, ns);
+    if (!rcode) rcode = kernel_step(c, ns, do_forces != 0);
+    if (rcode) break;
+    TGX_HIP(c, launch_site_add(ns, c->blk->site_atom.as<int>(), c->blk->esite.as<double>(),
+                               nullptr, c->eatom.as<double>(), nullptr, c->stream));
+    if (!do_forces) continue;
+    ENSURE(c, blk->vsite, 48 * (size_t)ns);
+    const size_t ts = tsize(prec);
+    ENSURE(c, gpair, ts * 4 * (size_t)n * c->maxnb);
+    kev_rec(c, 4);
+    hipError_t e;
+    if (prec == TGX_FP64)
+      e = launch_soap_bwd<double>(
+          c->blk->sd, c->blk->sd_dev.as<SoapDev>(), c->blk->W_dev.as<double>(),
+          c->blk->rtab.as<float>(), c->pos.as<double>(), c->species.as<int>(), cell, c->maxnb,
+          c->nb_count.as<int>(), c->nb_list.as<int>(), c->blk->site_atom.as<int>(), ns,
+          c->blk->pairn.as<int>(), c->blk->q.as<double>(), c->blk->dim_pad,
+          c->blk->cbuf.as<double>(), c->blk->pnorm.as<double>(), c->blk->gmat.as<double>(),
+          c->blk->dim_pad, c->gpair.as<double>(), c->blk->vsite.as<double>(), nullptr, 0,
+          c->eatom.as<double>(), c->stream, 0);
+    else
+      e = launch_soap_bwd<float>(
+          c->blk->sd, c->blk->sd_dev.as<SoapDev>(), c->blk->W_dev.as<double>(),
+          c->blk->rtab.as<float>(), c->pos.as<double>(), c->species.as<int>(), cell, c->maxnb,
+          c->nb_count.as<int>(), c->nb_list.as<int>(), c->blk->site_atom.as<int>(), ns,
+          c->blk->pairn.as<int>(), c->blk->q.as<float>(), c->blk->dim_pad,
+          c->blk->cbuf.as<float>(), c->blk->pnorm.as<float>(), c->blk->gmat.as<float>(),
+          c->blk->dim_pad, c->gpair.as<float>(), c->blk->vsite.as<double>(), nullptr, 0,
+          c->eatom.as<double>(), c->stream, 0);
+    kev_rec(c, 5);
+    kev_rec(c, 6);
+    if (e == hipSuccess) {
+      if (prec == TGX_FP64)
+        e = launch_force_gather<double>(n, c->maxnb, c->nb_count.as<int>(),
+                                        c->nb_list.as<int>(), c->nb_rev.as<int>(),
+                                        c->blk->site_of_atom.as<int>(), ns == n ? 1 : 0,
+                                        c->gpair.as<double>(), c->forces.as<double>(),
+                                        c->stream, 0);
+      else
+        e = launch_force_gather<float>(n, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
+                                       c->nb_rev.as<int>(), c->blk->site_of_atom.as<int>(),
+                                       ns == n ? 1 : 0, c->gpair.as<float>(),
+                                       c->forces.as<double>(), c->stream, 0);
+    }
+    if (e == hipSuccess)
+      e = launch_site_add(ns, c->blk->site_atom.as<int>(), nullptr, c->blk->vsite.as<double>(),
+                          nullptr, c->virial.as<double>(), c->stream);
+    kev_rec(c, 7);
+    if (e != hipSuccess) {
+      c->blk = c->blocks[0].get();
+      TGX_HIP(c, e);
+    }
+  }
+  c->blk = c->blocks[0].get();
+  if (rcode) return rcode;
+  ev_rec(c, 3);
+  ev_rec(c, 4);
+  for (const auto& b : c->twob)
+    TGX_HIP(c, launch_two_body(b, c->pos.as<double>(), c->species.as<int>(), n, cell, c->maxnb,
+                               c->nb_count.as<int>(), c->nb_list.as<int>(), c->eatom.as<double>(),
+                               c->forces.as<double>(), c->virial.as<double>(), do_forces,
+                               prec == TGX_FP64, c->stream));
+  ev_rec(c, 5);
+  if (cell.many) {
+    if (x.b_out.ensure(sizeof(double) * 10 * (size_t)n_struct))
+      return set_err(c, TGX_ERR_HIP, "allocation failed");
+    TGX_HIP(c, launch_reduce_many(n_struct, x.b_aoff.as<int>(), c->blocks[0]->site_of_atom.as<int>(),
+                                  nullptr, do_forces ? c->virial.as<double>() : nullptr,
+                                  c->eatom.as<double>(), x.b_out.as<double>(), c->stream));
+  } else {
+    TGX_HIP(c, launch_reduce(0, nullptr, n, do_forces ? c->virial.as<double>() : nullptr,
+                             c->eatom.as<double>(), x.out.as<double>(), c->stream, !defer_final));
+  }
+  ev_rec(c, 6);
+  x.ev_on = true;
+  return TGX_OK;
+}
+
 static int upload_structure(tgx_ctx* c, int n, const double* pos, const int* species) {
   ENSURE(c, pos, 24 * (size_t)n);
   ENSURE(c, species, 4 * (size_t)n);
-  const int ns = c->has_soap ? c->sd.n_species : TGX_MAX_SPECIES;
+  const int ns = c->has_soap ? c->blk->sd.n_species : TGX_MAX_SPECIES;
   for (int i = 0; i < n; i++) {
     if (species[i] < 1 || species[i] > ns)
       return set_err(c, TGX_ERR_ARG, "species[%d] = %d out of range 1..%d", i, species[i], ns);
@@ -805,6 +926,12 @@ int tgx_create(int device, int precision, tgx_ctx** out) {
   tgx_ctx* c = &f->c;
   c->device = device;
   c->precision = precision;
+  c->blocks.emplace_back(new (std::nothrow) SoapBlock());
+  if (!c->blocks.back()) {
+    delete f;
+    return TGX_ERR_ARG;
+  }
+  c->blk = c->blocks[0].get();
   if (hipSetDevice(device) != hipSuccess || scratch_init(device) != hipSuccess ||
       hipStreamCreate(&c->stream) != hipSuccess) {
     delete f;
@@ -825,14 +952,12 @@ int tgx_destroy(tgx_ctx* c) {
   tgx_ctx_full* f = reinterpret_cast<tgx_ctx_full*>(c);
   md_release_graph(f->x.md);
   f->x.md.active = false;
-  DevBuf* bufs[] = {&c->W_dev, &c->rtab, &c->Qs64, &c->QsT64, &c->Qs32, &c->QsT32, &c->Qs16,
-                    &c->QsT16, &c->alph64, &c->alph32, &c->pos, &c->species, &c->nb_count,
-                    &c->nb_list, &c->nb_rev, &c->nb_flag, &c->cell_of, &c->cell_start,
-                    &c->cell_atoms, &c->site_of_atom, &c->site_atom, &c->q, &c->qb, &c->cbuf,
-                    &c->pnorm, &c->ksum, &c->wmat, &c->gmat, &c->epart, &c->esite, &c->gpair,
-                    &c->vsite, &c->forces, &c->eatom, &c->virial, &c->scratch, &c->csr_off,
+  for (auto& b : c->blocks) b->release();
+  DevBuf* bufs[] = {&c->pos, &c->species, &c->nb_count, &c->nb_list, &c->nb_rev, &c->nb_flag,
+                    &c->cell_of, &c->cell_start, &c->cell_atoms, &c->ksum, &c->gpair,
+                    &c->forces, &c->eatom, &c->virial, &c->scratch, &c->csr_off,
                     &c->csr_out, &c->md_vel, &c->md_fprev, &c->md_mass, &c->md_pos0,
-                    &c->md_ep, &c->md_ek, &c->md_reb, &c->md_disp, &f->x.sd_dev, &f->x.pairn,
+                    &c->md_ep, &c->md_ek, &c->md_reb, &c->md_disp,
                     &f->x.e0_dev, &f->x.iota, &f->x.cell_count, &f->x.cell_fill, &f->x.out,
                     &f->x.stage, &f->x.twob_store, &f->x.b_cells, &f->x.b_sid,
                     &f->x.b_boff, &f->x.b_aoff, &f->x.b_out, &f->x.mc_pref, &f->x.mc_ctl,
@@ -881,10 +1006,11 @@ int tgx_set_autotune(tgx_ctx* c, int on, const char* cache_path) {
   return TGX_OK;
 }
 
-int tgx_set_soap_model(tgx_ctx* c, const tgx_soap_hypers* h, int dim, int n_sparse,
-                       const double* Qs, const double* alphas, double delta, double zeta) {
-  if (!c) return TGX_ERR_ARG;
-  if (md_busy(c)) return TGX_ERR_STATE;
+}  // extern "C"
+
+// Loads one soap_turbo block into c->blk (hypers, sparse set, device tables).
+static int load_soap_block(tgx_ctx* c, const tgx_soap_hypers* h, int dim, int n_sparse,
+                           const double* Qs, const double* alphas, double delta, double zeta) {
   if (!h || !Qs || !alphas) return set_err(c, TGX_ERR_ARG, "null argument");
   (void)hipSetDevice(c->device);
   if (h->n_species < 1 || h->n_species > TGX_MAX_SPECIES)
@@ -959,20 +1085,20 @@ int tgx_set_soap_model(tgx_ctx* c, const tgx_soap_hypers* h, int dim, int n_spar
     for (int k = 0; k < h->alpha_max[s]; k++)
       sd.rself[s * TGX_MAX_NR + k] = h->central_weight[s] * R[k] * (1.0 / std::sqrt(4.0 * M_PI));
   }
-  c->W_host = W;
-  c->hyp = *h;
-  c->sd = sd;
-  c->dim = dim;
-  c->n_sparse = n_sparse;
-  c->dim_pad = (dim + 63) / 64 * 64;  // whole 64-deep k-slabs of the BF16 GEMM
-  c->sp_pad = (n_sparse + 63) / 64 * 64;
-  c->delta = delta;
-  c->zeta = zeta;
-  c->Qs_host.assign(Qs, Qs + (size_t)dim * n_sparse);
-  c->alphas_host.assign(alphas, alphas + n_sparse);
+  c->blk->W_host = W;
+  c->blk->hyp = *h;
+  c->blk->sd = sd;
+  c->blk->dim = dim;
+  c->blk->n_sparse = n_sparse;
+  c->blk->dim_pad = (dim + 63) / 64 * 64;  // whole 64-deep k-slabs of the BF16 GEMM
+  c->blk->sp_pad = (n_sparse + 63) / 64 * 64;
+  c->blk->delta = delta;
+  c->blk->zeta = zeta;
+  c->blk->Qs_host.assign(Qs, Qs + (size_t)dim * n_sparse);
+  c->blk->alphas_host.assign(alphas, alphas + n_sparse);
   Extra& x = X(c);
   // padded layouts of Qs for the three precisions
-  const int Dp = c->dim_pad, Sp = c->sp_pad;
+  const int Dp = c->blk->dim_pad, Sp = c->blk->sp_pad;
   std::vector<double> q64((size_t)Sp * Dp, 0.0), qT64((size_t)Dp * Sp, 0.0), a64(Sp, 0.0);
   std::vector<float> q32((size_t)Sp * Dp, 0.f), qT32((size_t)Dp * Sp, 0.f), a32(Sp, 0.f);
   std::vector<unsigned short> q16((size_t)Sp * Dp, 0), qT16((size_t)Dp * Sp, 0);
@@ -994,17 +1120,17 @@ int tgx_set_soap_model(tgx_ctx* c, const tgx_soap_hypers* h, int dim, int n_spar
       qT16[(size_t)d * Sp + s] = bf((float)v);
     }
   }
-  ENSURE(c, Qs64, 8 * q64.size());
-  ENSURE(c, QsT64, 8 * qT64.size());
-  ENSURE(c, Qs32, 4 * q32.size());
-  ENSURE(c, QsT32, 4 * qT32.size());
-  ENSURE(c, Qs16, 2 * q16.size());
-  ENSURE(c, QsT16, 2 * qT16.size());
-  ENSURE(c, alph64, 8 * (size_t)Sp);
-  ENSURE(c, alph32, 4 * (size_t)Sp);
-  ENSURE(c, W_dev, 8 * W.size());
-  ENSURE(c, rtab, sizeof(float) * 2 * (size_t)TGX_MAX_SPECIES * (TGX_RTAB_N + 1) * TGX_MAX_NR);
-  if (x.sd_dev.ensure(sizeof(SoapDev))) return set_err(c, TGX_ERR_HIP, "allocation failed");
+  ENSURE(c, blk->Qs64, 8 * q64.size());
+  ENSURE(c, blk->QsT64, 8 * qT64.size());
+  ENSURE(c, blk->Qs32, 4 * q32.size());
+  ENSURE(c, blk->QsT32, 4 * qT32.size());
+  ENSURE(c, blk->Qs16, 2 * q16.size());
+  ENSURE(c, blk->QsT16, 2 * qT16.size());
+  ENSURE(c, blk->alph64, 8 * (size_t)Sp);
+  ENSURE(c, blk->alph32, 4 * (size_t)Sp);
+  ENSURE(c, blk->W_dev, 8 * W.size());
+  ENSURE(c, blk->rtab, sizeof(float) * 2 * (size_t)TGX_MAX_SPECIES * (TGX_RTAB_N + 1) * TGX_MAX_NR);
+  if (c->blk->sd_dev.ensure(sizeof(SoapDev))) return set_err(c, TGX_ERR_HIP, "allocation failed");
   // component d of the power spectrum -> (n, n', l), packed n | n' << 8 | l << 16
   std::vector<int> pn(sd.dim);
   {
@@ -1013,22 +1139,22 @@ int tgx_set_soap_model(tgx_ctx* c, const tgx_soap_hypers* h, int dim, int n_spar
       for (int b = a; b < nr; b++)
         for (int l = 0; l <= sd.l_max; l++) pn[k++] = a | (b << 8) | (l << 16);
   }
-  if (x.pairn.ensure(sizeof(int) * pn.size())) return set_err(c, TGX_ERR_HIP, "allocation failed");
+  if (c->blk->pairn.ensure(sizeof(int) * pn.size())) return set_err(c, TGX_ERR_HIP, "allocation failed");
   hipStream_t st = c->stream;
-  TGX_HIP(c, hipMemcpyAsync(c->Qs64.p, q64.data(), 8 * q64.size(), hipMemcpyHostToDevice, st));
-  TGX_HIP(c, hipMemcpyAsync(c->QsT64.p, qT64.data(), 8 * qT64.size(), hipMemcpyHostToDevice, st));
-  TGX_HIP(c, hipMemcpyAsync(c->Qs32.p, q32.data(), 4 * q32.size(), hipMemcpyHostToDevice, st));
-  TGX_HIP(c, hipMemcpyAsync(c->QsT32.p, qT32.data(), 4 * qT32.size(), hipMemcpyHostToDevice, st));
-  TGX_HIP(c, hipMemcpyAsync(c->Qs16.p, q16.data(), 2 * q16.size(), hipMemcpyHostToDevice, st));
-  TGX_HIP(c, hipMemcpyAsync(c->QsT16.p, qT16.data(), 2 * qT16.size(), hipMemcpyHostToDevice, st));
-  TGX_HIP(c, hipMemcpyAsync(c->alph64.p, a64.data(), 8 * (size_t)Sp, hipMemcpyHostToDevice, st));
-  TGX_HIP(c, hipMemcpyAsync(c->alph32.p, a32.data(), 4 * (size_t)Sp, hipMemcpyHostToDevice, st));
-  TGX_HIP(c, hipMemcpyAsync(c->W_dev.p, W.data(), 8 * W.size(), hipMemcpyHostToDevice, st));
-  TGX_HIP(c, hipMemcpyAsync(x.sd_dev.p, &c->sd, sizeof(SoapDev), hipMemcpyHostToDevice, st));
-  TGX_HIP(c, hipMemcpyAsync(x.pairn.p, pn.data(), sizeof(int) * pn.size(), hipMemcpyHostToDevice, st));
-  TGX_HIP(c, hipMemsetAsync(c->rtab.p, 0, c->rtab.bytes, st));
-  TGX_HIP(c, launch_build_rtab(x.sd_dev.as<SoapDev>(), sd.n_species, c->W_dev.as<double>(),
-                               c->rtab.as<float>(), st));
+  TGX_HIP(c, hipMemcpyAsync(c->blk->Qs64.p, q64.data(), 8 * q64.size(), hipMemcpyHostToDevice, st));
+  TGX_HIP(c, hipMemcpyAsync(c->blk->QsT64.p, qT64.data(), 8 * qT64.size(), hipMemcpyHostToDevice, st));
+  TGX_HIP(c, hipMemcpyAsync(c->blk->Qs32.p, q32.data(), 4 * q32.size(), hipMemcpyHostToDevice, st));
+  TGX_HIP(c, hipMemcpyAsync(c->blk->QsT32.p, qT32.data(), 4 * qT32.size(), hipMemcpyHostToDevice, st));
+  TGX_HIP(c, hipMemcpyAsync(c->blk->Qs16.p, q16.data(), 2 * q16.size(), hipMemcpyHostToDevice, st));
+  TGX_HIP(c, hipMemcpyAsync(c->blk->QsT16.p, qT16.data(), 2 * qT16.size(), hipMemcpyHostToDevice, st));
+  TGX_HIP(c, hipMemcpyAsync(c->blk->alph64.p, a64.data(), 8 * (size_t)Sp, hipMemcpyHostToDevice, st));
+  TGX_HIP(c, hipMemcpyAsync(c->blk->alph32.p, a32.data(), 4 * (size_t)Sp, hipMemcpyHostToDevice, st));
+  TGX_HIP(c, hipMemcpyAsync(c->blk->W_dev.p, W.data(), 8 * W.size(), hipMemcpyHostToDevice, st));
+  TGX_HIP(c, hipMemcpyAsync(c->blk->sd_dev.p, &c->blk->sd, sizeof(SoapDev), hipMemcpyHostToDevice, st));
+  TGX_HIP(c, hipMemcpyAsync(c->blk->pairn.p, pn.data(), sizeof(int) * pn.size(), hipMemcpyHostToDevice, st));
+  TGX_HIP(c, hipMemsetAsync(c->blk->rtab.p, 0, c->blk->rtab.bytes, st));
+  TGX_HIP(c, launch_build_rtab(c->blk->sd_dev.as<SoapDev>(), sd.n_species, c->blk->W_dev.as<double>(),
+                               c->blk->rtab.as<float>(), st));
   TGX_HIP(c, hipStreamSynchronize(st));
   c->has_soap = true;
   c->maxnb = 0;
@@ -1037,6 +1163,72 @@ int tgx_set_soap_model(tgx_ctx* c, const tgx_soap_hypers* h, int dim, int n_spar
     TGX_HIP(c, hipMemcpy(x.e0_dev.p, c->e0, 8 * TGX_MAX_SPECIES, hipMemcpyHostToDevice));
   }
   return TGX_OK;
+}
+
+// Releases every SOAP block but the first object (kept empty).
+static void drop_soap_blocks(tgx_ctx* c) {
+  for (auto& b : c->blocks) b->release();
+  c->blocks.resize(1);
+  *c->blocks[0] = SoapBlock();
+  c->blk = c->blocks[0].get();
+  c->has_soap = false;
+  c->maxnb = 0;
+}
+
+extern "C" {
+
+int tgx_set_soap_model(tgx_ctx* c, const tgx_soap_hypers* h, int dim, int n_sparse,
+                       const double* Qs, const double* alphas, double delta, double zeta) {
+  if (!c) return TGX_ERR_ARG;
+  if (md_busy(c)) return TGX_ERR_STATE;
+  drop_soap_blocks(c);
+  const int rcode = load_soap_block(c, h, dim, n_sparse, Qs, alphas, delta, zeta);
+  if (rcode) drop_soap_blocks(c);
+  return rcode;
+}
+
+int tgx_add_soap_model(tgx_ctx* c, const tgx_soap_hypers* h, int dim, int n_sparse,
+                       const double* Qs, const double* alphas, double delta, double zeta) {
+  if (!c) return TGX_ERR_ARG;
+  if (md_busy(c)) return TGX_ERR_STATE;
+  if (!c->has_soap) return tgx_set_soap_model(c, h, dim, n_sparse, Qs, alphas, delta, zeta);
+  if (!h) return set_err(c, TGX_ERR_ARG, "null argument");
+  if (h->n_species != c->blocks[0]->hyp.n_species)
+    return set_err(c, TGX_ERR_ARG,
+                   "every soap_turbo block uses the model's species list (%d species, got %d)",
+                   c->blocks[0]->hyp.n_species, h->n_species);
+  if (c->blocks.size() >= TGX_MAX_SOAP_BLOCKS)
+    return set_err(c, TGX_ERR_ARG, "at most %d soap_turbo blocks", TGX_MAX_SOAP_BLOCKS);
+  (void)hipSetDevice(c->device);
+  c->blocks.emplace_back(new (std::nothrow) SoapBlock());
+  if (!c->blocks.back()) {
+    c->blocks.pop_back();
+    return set_err(c, TGX_ERR_ARG, "out of host memory");
+  }
+  c->blk = c->blocks.back().get();
+  const int rcode = load_soap_block(c, h, dim, n_sparse, Qs, alphas, delta, zeta);
+  if (rcode) {
+    c->blocks.back()->release();
+    c->blocks.pop_back();
+  }
+  c->blk = c->blocks[0].get();
+  c->has_soap = true;
+  c->maxnb = 0;
+  return rcode;
+}
+
+int tgx_clear_soap_models(tgx_ctx* c) {
+  if (!c) return TGX_ERR_ARG;
+  if (md_busy(c)) return TGX_ERR_STATE;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  drop_soap_blocks(c);
+  return TGX_OK;
+}
+
+int tgx_n_soap_models(const tgx_ctx* c) {
+  if (!c) return -TGX_ERR_ARG;
+  return c->has_soap ? (int)c->blocks.size() : 0;
 }
 
 int tgx_set_e0(tgx_ctx* c, int n_species, const double* e0) {
@@ -1193,11 +1385,11 @@ int tgx_soap_descriptors(tgx_ctx* c, int n, const double* pos, const int* specie
   if (rcode) return rcode;
   Cell cell;
   std::string err;
-  rcode = make_cell(lattice, c->sd.rcut_max, &cell, &err);
+  rcode = make_cell(lattice, c->blk->sd.rcut_max, &cell, &err);
   if (rcode) return set_err(c, rcode, "%s", err.c_str());
   const int saved = c->maxnb;
   c->maxnb = 0;
-  rcode = build_list(c, n, cell, c->sd.rcut_max, false);
+  rcode = build_list(c, n, cell, c->blk->sd.rcut_max, false);
   if (rcode) {
     c->maxnb = saved;
     return rcode;
@@ -1208,15 +1400,15 @@ int tgx_soap_descriptors(tgx_ctx* c, int n, const double* pos, const int* specie
   c->maxnb = saved;
   if (rcode) return rcode;
   TGX_HIP(c, hipStreamSynchronize(c->stream));
-  const int Dp = c->dim_pad, dim = c->dim;
+  const int Dp = c->blk->dim_pad, dim = c->blk->dim;
   if (c->precision == TGX_FP64) {
     std::vector<double> tmp((size_t)n_sites * Dp);
-    TGX_HIP(c, hipMemcpy(tmp.data(), c->q.p, 8 * tmp.size(), hipMemcpyDeviceToHost));
+    TGX_HIP(c, hipMemcpy(tmp.data(), c->blk->q.p, 8 * tmp.size(), hipMemcpyDeviceToHost));
     for (int s = 0; s < n_sites; s++)
       std::memcpy(soap + (size_t)s * dim, tmp.data() + (size_t)s * Dp, 8 * (size_t)dim);
   } else {
     std::vector<float> tmp((size_t)n_sites * Dp);
-    TGX_HIP(c, hipMemcpy(tmp.data(), c->q.p, 4 * tmp.size(), hipMemcpyDeviceToHost));
+    TGX_HIP(c, hipMemcpy(tmp.data(), c->blk->q.p, 4 * tmp.size(), hipMemcpyDeviceToHost));
     for (int s = 0; s < n_sites; s++)
       for (int d = 0; d < dim; d++) soap[(size_t)s * dim + d] = tmp[(size_t)s * Dp + d];
   }
@@ -1224,7 +1416,7 @@ int tgx_soap_descriptors(tgx_ctx* c, int n, const double* pos, const int* specie
   if (site_index) {
     std::vector<int> sa(n_sites);
     if (n_sites)
-      TGX_HIP(c, hipMemcpy(sa.data(), c->site_atom.p, 4 * (size_t)n_sites, hipMemcpyDeviceToHost));
+      TGX_HIP(c, hipMemcpy(sa.data(), c->blk->site_atom.p, 4 * (size_t)n_sites, hipMemcpyDeviceToHost));
     for (int s = 0; s < n_sites; s++) site_index[s] = sa[s] + 1;
   }
   return TGX_OK;
@@ -1239,10 +1431,10 @@ int tgx_gap_kernel_step(tgx_ctx* c, int n_sites, const double* soap, double* ene
   if (!c->has_soap) return set_err(c, TGX_ERR_STATE, "no SOAP model loaded");
   if (n_sites == 0) return TGX_OK;
   (void)hipSetDevice(c->device);
-  const int prec = c->precision, Dp = c->dim_pad, dim = c->dim;
+  const int prec = c->precision, Dp = c->blk->dim_pad, dim = c->blk->dim;
   const size_t ts = prec == TGX_FP64 ? 8 : (prec == TGX_FP32 ? 4 : 2);
   if (prec == TGX_MIXED) {
-    ENSURE(c, qb, 2 * (size_t)n_sites * Dp);
+    ENSURE(c, blk->qb, 2 * (size_t)n_sites * Dp);
     std::vector<unsigned short> t((size_t)n_sites * Dp, 0);
     for (int s = 0; s < n_sites; s++)
       for (int d = 0; d < dim; d++) {
@@ -1251,34 +1443,34 @@ int tgx_gap_kernel_step(tgx_ctx* c, int n_sites, const double* soap, double* ene
         std::memcpy(&u, &f, 4);
         t[(size_t)s * Dp + d] = (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
       }
-    TGX_HIP(c, hipMemcpy(c->qb.p, t.data(), 2 * t.size(), hipMemcpyHostToDevice));
+    TGX_HIP(c, hipMemcpy(c->blk->qb.p, t.data(), 2 * t.size(), hipMemcpyHostToDevice));
   } else {
-    ENSURE(c, q, ts * (size_t)n_sites * Dp);
+    ENSURE(c, blk->q, ts * (size_t)n_sites * Dp);
     if (prec == TGX_FP64) {
       std::vector<double> t((size_t)n_sites * Dp, 0.0);
       for (int s = 0; s < n_sites; s++)
         std::memcpy(t.data() + (size_t)s * Dp, soap + (size_t)s * dim, 8 * (size_t)dim);
-      TGX_HIP(c, hipMemcpy(c->q.p, t.data(), 8 * t.size(), hipMemcpyHostToDevice));
+      TGX_HIP(c, hipMemcpy(c->blk->q.p, t.data(), 8 * t.size(), hipMemcpyHostToDevice));
     } else {
       std::vector<float> t((size_t)n_sites * Dp, 0.f);
       for (int s = 0; s < n_sites; s++)
         for (int d = 0; d < dim; d++) t[(size_t)s * Dp + d] = (float)soap[(size_t)s * dim + d];
-      TGX_HIP(c, hipMemcpy(c->q.p, t.data(), 4 * t.size(), hipMemcpyHostToDevice));
+      TGX_HIP(c, hipMemcpy(c->blk->q.p, t.data(), 4 * t.size(), hipMemcpyHostToDevice));
     }
   }
   int rcode = kernel_step(c, n_sites, qss != nullptr);
   if (rcode) return rcode;
   TGX_HIP(c, hipStreamSynchronize(c->stream));
-  TGX_HIP(c, hipMemcpy(energies, c->esite.p, 8 * (size_t)n_sites, hipMemcpyDeviceToHost));
+  TGX_HIP(c, hipMemcpy(energies, c->blk->esite.p, 8 * (size_t)n_sites, hipMemcpyDeviceToHost));
   if (qss) {
     if (prec == TGX_FP64) {
       std::vector<double> g((size_t)n_sites * Dp);
-      TGX_HIP(c, hipMemcpy(g.data(), c->gmat.p, 8 * g.size(), hipMemcpyDeviceToHost));
+      TGX_HIP(c, hipMemcpy(g.data(), c->blk->gmat.p, 8 * g.size(), hipMemcpyDeviceToHost));
       for (int s = 0; s < n_sites; s++)
         for (int d = 0; d < dim; d++) qss[(size_t)d * n_sites + s] = -g[(size_t)s * Dp + d];
     } else {
       std::vector<float> g((size_t)n_sites * Dp);
-      TGX_HIP(c, hipMemcpy(g.data(), c->gmat.p, 4 * g.size(), hipMemcpyDeviceToHost));
+      TGX_HIP(c, hipMemcpy(g.data(), c->blk->gmat.p, 4 * g.size(), hipMemcpyDeviceToHost));
       for (int s = 0; s < n_sites; s++)
         for (int d = 0; d < dim; d++) qss[(size_t)d * n_sites + s] = -(double)g[(size_t)s * Dp + d];
     }
@@ -1633,14 +1825,14 @@ static int mc_trial_launch(tgx_ctx* c, const McRun& r) {
                               c->cell_atoms.as<int>(), c->nb_count.as<int>(), c->maxnb,
                               c->nb_list.as<int>(), nullptr, c->nb_flag.as<int>(), c->stream,
                               ctl + MC_REBUILD, x.mc_pref.as<double>()));
-  const int D = c->dim;
+  const int D = c->blk->dim;
   const int exact = c->precision == TGX_FP64 ? 1 : 0;
   if (c->has_soap) {
     // the affected-site scan also computes the 2-body change; for plain
     // Metropolis that is all of stage 1, which then needs no launch
     TGX_HIP(c, launch_mc_affected(ctl, dctl, c->pos.as<double>(), r.cell, r.rsoap,
                                   c->nb_count.as<int>(), c->nb_list.as<int>(), c->maxnb,
-                                  c->species.as<int>(), r.central, c->site_atom.as<int>(), r.cap,
+                                  c->species.as<int>(), r.central, c->blk->site_atom.as<int>(), r.cap,
                                   x.mc_naff.as<int>(), x.mc_twob.as<TwoBodyDev>(), r.n2b, exact,
                                   1, r.da ? 0 : 1, c->stream));
     int rcode = soap_forward(c, r.n, Geom{r.cell}, r.cap, true);
@@ -1648,12 +1840,12 @@ static int mc_trial_launch(tgx_ctx* c, const McRun& r) {
     if (r.da) {
       hipError_t e =
           c->precision == TGX_FP64
-              ? launch_mlp_forward<double>(r.cap, D, c->q.as<double>(), c->dim_pad, nullptr,
+              ? launch_mlp_forward<double>(r.cap, D, c->blk->q.as<double>(), c->blk->dim_pad, nullptr,
                                            x.sg_P.as<float>(), x.sg_W1t.as<unsigned short>(),
                                            x.sg_W2t.as<unsigned short>(), x.sg_norm.as<float>(),
                                            x.mc_esur.as<double>(), nullptr, nullptr,
                                            ctl + MC_DA, c->stream)
-              : launch_mlp_forward<float>(r.cap, D, c->q.as<float>(), c->dim_pad, nullptr,
+              : launch_mlp_forward<float>(r.cap, D, c->blk->q.as<float>(), c->blk->dim_pad, nullptr,
                                           x.sg_P.as<float>(), x.sg_W1t.as<unsigned short>(),
                                           x.sg_W2t.as<unsigned short>(), x.sg_norm.as<float>(),
                                           x.mc_esur.as<double>(), nullptr, nullptr, ctl + MC_DA,
@@ -1664,7 +1856,7 @@ static int mc_trial_launch(tgx_ctx* c, const McRun& r) {
   if (r.da || !c->has_soap)
     TGX_HIP(c, launch_mc_stage1(c->pos.as<double>(), c->species.as<int>(), r.cell, c->maxnb,
                                 c->nb_count.as<int>(), c->nb_list.as<int>(),
-                                c->site_atom.as<int>(), x.mc_naff.as<int>(),
+                                c->blk->site_atom.as<int>(), x.mc_naff.as<int>(),
                                 x.mc_esur.as<double>(), x.mc_esur_cur.as<double>(),
                                 x.mc_twob.as<TwoBodyDev>(), r.n2b, exact, c->has_soap ? 1 : 0,
                                 r.seed, r.kT, ctl, dctl, x.mc_trace.as<double>(), c->stream));
@@ -1678,21 +1870,21 @@ static int mc_trial_launch(tgx_ctx* c, const McRun& r) {
     if (r.cap_s > 0) {
       hipError_t e =
           c->precision == TGX_FP64
-              ? launch_mc_collect<double>(ctl, x.mc_naff.as<int>(), c->q.as<double>(), c->dim_pad,
-                                          D, c->esite.as<double>(), x.mc_sq.as<float>(),
+              ? launch_mc_collect<double>(ctl, x.mc_naff.as<int>(), c->blk->q.as<double>(), c->blk->dim_pad,
+                                          D, c->blk->esite.as<double>(), x.mc_sq.as<float>(),
                                           x.mc_sy.as<double>(), r.cap_s, ctl + MC_CURSOR,
                                           c->stream)
-              : launch_mc_collect<float>(ctl, x.mc_naff.as<int>(), c->q.as<float>(), c->dim_pad,
-                                         D, c->esite.as<double>(), x.mc_sq.as<float>(),
+              : launch_mc_collect<float>(ctl, x.mc_naff.as<int>(), c->blk->q.as<float>(), c->blk->dim_pad,
+                                         D, c->blk->esite.as<double>(), x.mc_sq.as<float>(),
                                          x.mc_sy.as<double>(), r.cap_s, ctl + MC_CURSOR,
                                          c->stream);
       TGX_HIP(c, e);
     }
   }
-  TGX_HIP(c, launch_mc_stage2(c->pos.as<double>(), c->site_atom.as<int>(), x.mc_naff.as<int>(),
-                              c->has_soap ? c->esite.as<double>() : nullptr,
-                              nparts ? c->epart.as<float>() : nullptr, nparts, r.cap,
-                              c->delta * c->delta,
+  TGX_HIP(c, launch_mc_stage2(c->pos.as<double>(), c->blk->site_atom.as<int>(), x.mc_naff.as<int>(),
+                              c->has_soap ? c->blk->esite.as<double>() : nullptr,
+                              nparts ? c->blk->epart.as<float>() : nullptr, nparts, r.cap,
+                              c->blk->delta * c->blk->delta,
                               x.mc_ecur.as<double>(), x.mc_esur.as<double>(),
                               x.mc_esur_cur.as<double>(), r.seed, r.kT, ctl, dctl,
                               x.mc_trace.as<double>(), c->stream));
@@ -1709,19 +1901,19 @@ static int mc_refresh_esur(tgx_ctx* c, const McRun& r, int n_sites) {
   for (int i = 0; i < r.n; i++) io[i] = i;
   TGX_HIP(c, hipMemcpyAsync(x.iota.p, io.data(), sizeof(int) * r.n, hipMemcpyHostToDevice,
                             c->stream));
-  TGX_HIP(c, hipMemcpyAsync(c->site_atom.p, x.iota.p, sizeof(int) * r.n,
+  TGX_HIP(c, hipMemcpyAsync(c->blk->site_atom.p, x.iota.p, sizeof(int) * r.n,
                             hipMemcpyDeviceToDevice, c->stream));
   int rcode = soap_forward(c, r.n, Geom{r.cell}, n_sites);
   if (rcode) return rcode;
-  const int D = c->dim;
+  const int D = c->blk->dim;
   hipError_t e =
       c->precision == TGX_FP64
-          ? launch_mlp_forward<double>(n_sites, D, c->q.as<double>(), c->dim_pad, nullptr,
+          ? launch_mlp_forward<double>(n_sites, D, c->blk->q.as<double>(), c->blk->dim_pad, nullptr,
                                        x.sg_P.as<float>(), x.sg_W1t.as<unsigned short>(),
                                        x.sg_W2t.as<unsigned short>(), x.sg_norm.as<float>(),
                                        x.mc_esur_cur.as<double>(), nullptr, nullptr, nullptr,
                                        c->stream)
-          : launch_mlp_forward<float>(n_sites, D, c->q.as<float>(), c->dim_pad, nullptr,
+          : launch_mlp_forward<float>(n_sites, D, c->blk->q.as<float>(), c->blk->dim_pad, nullptr,
                                       x.sg_P.as<float>(), x.sg_W1t.as<unsigned short>(),
                                       x.sg_W2t.as<unsigned short>(), x.sg_norm.as<float>(),
                                       x.mc_esur_cur.as<double>(), nullptr, nullptr, nullptr,
@@ -1743,15 +1935,19 @@ static int mc_run_impl(tgx_ctx* c, int n, const int* species, const double* latt
                    std::sqrt(3.0) * move_max);
   if (!c->has_soap && c->twob.empty()) return set_err(c, TGX_ERR_STATE, "no model loaded");
   if (X(c).md.active) return set_err(c, TGX_ERR_STATE, "end the MD run first (tgx_md_end)");
+  if (c->has_soap && c->blocks.size() > 1)
+    return set_err(c, TGX_ERR_UNSUPPORTED,
+                   "Monte Carlo takes models of one soap_turbo block (this one has %d)",
+                   (int)c->blocks.size());
   Extra& x = X(c);
   if (da) {
     if (!c->has_soap) return set_err(c, TGX_ERR_STATE, "delayed acceptance needs a SOAP model");
     int rc = sg_ready(c);
     if (rc) return rc;
-    if (x.sg_dim != c->dim)
+    if (x.sg_dim != c->blk->dim)
       return set_err(c, TGX_ERR_ARG, "surrogate dimension %d != SOAP dimension %d", x.sg_dim,
-                     c->dim);
-    if (c->sd.central_species != 0)
+                     c->blk->dim);
+    if (c->blk->sd.central_species != 0)
       return set_err(c, TGX_ERR_UNSUPPORTED, "delayed acceptance needs every atom as a site");
     if (da->n_train_trials < 0 || da->train_every < 64 || da->epochs < 1 || da->batch < 16 ||
         !(da->lr > 0) || !(da->max_rmse > 0))
@@ -1767,8 +1963,8 @@ static int mc_run_impl(tgx_ctx* c, int n, const int* species, const double* latt
   r.kT = (double)8.617333262e-5f * temp;  // the reference's default-real kB
   r.seed = seed;
   r.n2b = (int)c->twob.size();
-  r.central = c->has_soap ? c->sd.central_species : 0;
-  r.rsoap = c->has_soap ? c->sd.rcut_max : 0.0;
+  r.central = c->has_soap ? c->blk->sd.central_species : 0;
+  r.rsoap = c->has_soap ? c->blk->sd.rcut_max : 0.0;
   r.da = da != nullptr;
   r.rcut_list = rcut_all(c) + skin;
   std::string err;
@@ -1809,22 +2005,22 @@ static int mc_run_impl(tgx_ctx* c, int n, const int* species, const double* latt
       x.mc_naff.ensure(sizeof(int) * 4) || x.mc_trace.ensure(8 * ((size_t)n_trials + 1)) ||
       x.mc_twob.ensure(sizeof(TwoBodyDev) * std::max(1, r.n2b)) ||
       x.mc_esur.ensure(8 * (size_t)r.cap) || x.mc_esur_cur.ensure(8 * (size_t)n) ||
-      (r.cap_s && (x.mc_sq.ensure(sizeof(float) * (size_t)r.cap_s * c->dim) ||
+      (r.cap_s && (x.mc_sq.ensure(sizeof(float) * (size_t)r.cap_s * c->blk->dim) ||
                    x.mc_sy.ensure(8 * (size_t)r.cap_s))))
     return set_err(c, TGX_ERR_HIP, "device allocation failed (mc)");
-  ENSURE(c, site_atom, sizeof(int) * (size_t)std::max(n, r.cap));
+  ENSURE(c, blk->site_atom, sizeof(int) * (size_t)std::max(n, r.cap));
   if (c->has_soap) {
     // descriptor buffers for the larger of a trial's sites and all sites, so
     // that refreshing the surrogate energies never moves a buffer the
     // captured trial graph uses
     const size_t rows = (size_t)std::max(n_sites, r.cap), ts = tsize(c->precision);
-    ENSURE(c, q, ts * rows * c->dim_pad);
-    ENSURE(c, cbuf, ts * rows * c->sd.nrp * c->sd.L2);
-    ENSURE(c, pnorm, ts * rows);
-    if (c->precision == TGX_MIXED) ENSURE(c, qb, 2 * rows * c->dim_pad);
+    ENSURE(c, blk->q, ts * rows * c->blk->dim_pad);
+    ENSURE(c, blk->cbuf, ts * rows * c->blk->sd.nrp * c->blk->sd.L2);
+    ENSURE(c, blk->pnorm, ts * rows);
+    if (c->precision == TGX_MIXED) ENSURE(c, blk->qb, 2 * rows * c->blk->dim_pad);
   }
-  TGX_HIP(c, launch_mc_init(n, c->site_of_atom.as<int>(),
-                            (c->has_soap && n_sites > 0) ? c->esite.as<double>() : nullptr,
+  TGX_HIP(c, launch_mc_init(n, c->blk->site_of_atom.as<int>(),
+                            (c->has_soap && n_sites > 0) ? c->blk->esite.as<double>() : nullptr,
                             x.mc_ecur.as<double>(), c->stream));
   if (r.n2b)
     TGX_HIP(c, hipMemcpyAsync(x.mc_twob.p, c->twob.data(), sizeof(TwoBodyDev) * r.n2b,
@@ -1836,7 +2032,7 @@ static int mc_run_impl(tgx_ctx* c, int n, const int* species, const double* latt
   TGX_HIP(c, hipMemsetAsync(x.mc_ctl.p, 0, sizeof(int) * MC_NCTL, c->stream));
   TGX_HIP(c, hipMemsetAsync(c->nb_flag.p, 0, sizeof(int) * 4, c->stream));
   TGX_HIP(c, hipMemsetAsync(x.mc_naff.p, 0, sizeof(int) * 4, c->stream));
-  TGX_HIP(c, hipMemsetAsync(c->site_atom.p, 0, sizeof(int) * (size_t)r.cap, c->stream));
+  TGX_HIP(c, hipMemsetAsync(c->blk->site_atom.p, 0, sizeof(int) * (size_t)r.cap, c->stream));
   TGX_HIP(c, hipMemsetAsync(x.mc_esur.p, 0, 8 * (size_t)r.cap, c->stream));
   TGX_HIP(c, hipStreamSynchronize(c->stream));
   // a dry trial (MC_END = 0: every control kernel returns at once) sizes the

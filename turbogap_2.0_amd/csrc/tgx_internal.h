@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -114,33 +115,56 @@ struct DevBuf {
 
 }  // namespace tgx
 
+namespace tgx {
+
+// One soap_turbo descriptor block of the model (the reference keeps one
+// soap_turbo_hypers entry per block and loops over them,
+// gap/calculate_gap_soap.f90:324-447): its hyper-parameters, sparse set and
+// device tables, and the per-call buffers of its sites.  Every block uses
+// the model's species list; its own central species picks its sites.
+struct SoapBlock {
+  tgx_soap_hypers hyp{};
+  SoapDev sd{};
+  int dim = 0, n_sparse = 0, dim_pad = 0, sp_pad = 0;
+  double delta = 1.0, zeta = 2.0;
+  std::vector<double> Qs_host;      // n_sparse x dim
+  std::vector<double> alphas_host;  // n_sparse
+  // W matrices per species (n_max x n_max, row-major), FP64
+  std::vector<double> W_host;       // TGX_MAX_SPECIES * NR * NR
+  DevBuf W_dev, rtab, Qs64, QsT64, Qs32, QsT32, Qs16, QsT16, alph64, alph32, sd_dev, pairn;
+  // per call: sites (atoms of the central species, or all), descriptors,
+  // kernel-step and adjoint work
+  DevBuf site_of_atom, site_atom, q, qb, cbuf, pnorm, wmat, gmat, epart, esite, vsite;
+  int n_sites = 0;
+  void release() {
+    DevBuf* b[] = {&W_dev, &rtab, &Qs64, &QsT64, &Qs32, &QsT32, &Qs16, &QsT16, &alph64,
+                   &alph32, &sd_dev, &pairn, &site_of_atom, &site_atom, &q, &qb, &cbuf,
+                   &pnorm, &wmat, &gmat, &epart, &esite, &vsite};
+    for (DevBuf* d : b) d->release();
+  }
+};
+
+}  // namespace tgx
+
 struct tgx_ctx {
   int device = 0;
   int precision = TGX_FP64;
   hipStream_t stream = nullptr;
   std::string err;
 
-  // model
+  // model: SOAP blocks (at least one object; has_soap = a model is loaded),
+  // blk = the block the SOAP launchers work on
   bool has_soap = false;
-  tgx_soap_hypers hyp{};
-  tgx::SoapDev sd{};
-  int dim = 0, n_sparse = 0, dim_pad = 0, sp_pad = 0;
-  double delta = 1.0, zeta = 2.0;
+  std::vector<std::unique_ptr<tgx::SoapBlock>> blocks;
+  tgx::SoapBlock* blk = nullptr;
   double e0[TGX_MAX_SPECIES] = {0};
   double list_buffer = 0.0;  // list cutoff = largest model cutoff + list_buffer
-  std::vector<double> Qs_host;      // n_sparse x dim
-  std::vector<double> alphas_host;  // n_sparse
   std::vector<tgx::TwoBodyDev> twob;
-  // W matrices per species (n_max x n_max, row-major), FP64
-  std::vector<double> W_host;       // TGX_MAX_SPECIES * NR * NR
-  tgx::DevBuf W_dev, rtab, Qs64, QsT64, Qs32, QsT32, Qs16, QsT16, alph64, alph32;
-  bool rtab_ready = false;
 
   // per-call work buffers
   tgx::DevBuf pos, species, nb_count, nb_list, nb_rev, nb_flag, cell_of, cell_start,
-      cell_atoms, site_of_atom, site_atom;
-  tgx::DevBuf q, qb, cbuf, pnorm, ksum, wmat, gmat, epart, esite, gpair, vsite,
-      forces, eatom, virial, scratch, csr_off, csr_out;
+      cell_atoms;
+  tgx::DevBuf ksum, gpair, forces, eatom, virial, scratch, csr_off, csr_out;
   int maxnb = 0;
   double stage_ms[7] = {0};
   double kernel_ms[4] = {0};  // SOAP forward, kernel GEMMs, SOAP adjoint, force gather

@@ -992,6 +992,26 @@ hipError_t launch_energy_init(int n, const int* species, const double* e0,
   return hipGetLastError();
 }
 
+// Adds one SOAP block's site energies and site virials to their atoms (the
+// sites of one block are distinct atoms).
+__global__ void site_add_kernel(int n_sites, const int* site_atom, const double* esite,
+                                const double* vsite, double* eatom, double* vatom) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_sites) return;
+  const int i = site_atom[k];
+  if (esite) eatom[i] += esite[k];
+  if (vsite)
+    for (int t = 0; t < 6; t++) vatom[(size_t)i * 6 + t] += vsite[(size_t)k * 6 + t];
+}
+
+hipError_t launch_site_add(int n_sites, const int* site_atom, const double* esite,
+                           const double* vsite, double* eatom, double* vatom, hipStream_t st) {
+  if (n_sites == 0) return hipSuccess;
+  hipLaunchKernelGGL(site_add_kernel, dim3((n_sites + 255) / 256), dim3(256), 0, st, n_sites,
+                     site_atom, esite, vsite, eatom, vatom);
+  return hipGetLastError();
+}
+
 // Deterministic reduction of the per-site and per-atom 6-component virials
 // and of the atom energies: a fixed partition over RB workgroups (each a
 // fixed tree), then one workgroup sums the RB partials in order.

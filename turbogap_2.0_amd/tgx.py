@@ -188,10 +188,15 @@ class GapModel:
     zeta: float = 4.0
     e0: Optional[np.ndarray] = None
     two_body: list = field(default_factory=list)
+    # further soap_turbo blocks (GapModel objects whose hypers, Qs, alphas,
+    # delta and zeta are used), evaluated after this one and summed
+    more_soap: list = field(default_factory=list)
 
     @property
     def rcut_max(self) -> float:
         r = self.hypers.rcut_max
+        for b in self.more_soap:
+            r = max(r, b.hypers.rcut_max)
         for b in self.two_body:
             r = max(r, b.rcut)
         return r
@@ -232,6 +237,9 @@ def load_library(path: str = LIB_PATH):
         "tgx_set_autotune": (i, [P, i, C.c_char_p]),
         "tgx_set_soap_model": (i, [P, C.POINTER(CSoapHypers), i, i, dp, dp, d, d]),
         "tgx_set_e0": (i, [P, i, dp]),
+        "tgx_add_soap_model": (i, [P, C.POINTER(CSoapHypers), i, i, dp, dp, d, d]),
+        "tgx_clear_soap_models": (i, [P]),
+        "tgx_n_soap_models": (i, [P]),
         "tgx_add_2b_model": (i, [P, i, i, d, d, d, d, i, dp, dp, dp]),
         "tgx_clear_2b_models": (i, [P]),
         "tgx_neighbors_build": (i, [P, i, dp, dp, d, i64, ip, ip, dp, dp, dp, dp,
@@ -340,6 +348,12 @@ class Engine:
         n_sparse, dim = Qs.shape
         self._check(self.lib.tgx_set_soap_model(self.ctx, C.byref(h), dim, n_sparse,
                                                 _dptr(Qs), _dptr(al), model.delta, model.zeta))
+        for blk in model.more_soap:
+            hb = blk.hypers.to_c()
+            Qb, ab = _f64(blk.Qs), _f64(blk.alphas)
+            self._check(self.lib.tgx_add_soap_model(self.ctx, C.byref(hb), Qb.shape[1],
+                                                    Qb.shape[0], _dptr(Qb), _dptr(ab),
+                                                    blk.delta, blk.zeta))
         e0 = _f64(model.e0 if model.e0 is not None else np.zeros(model.hypers.n_species))
         self._check(self.lib.tgx_set_e0(self.ctx, len(e0), _dptr(e0)))
         self._check(self.lib.tgx_clear_2b_models(self.ctx))
