@@ -41,20 +41,24 @@ def test_fortran_struct_layout_matches_c(tmp_path):
 
 
 def write_model_files(d, model, species=("C",)):
-    """The reference's formats: .gap blocks, alphas (one per line), desc
-    (dim*n_sparse values, column-major), 2b alphas ("alpha cutoff")."""
-    h = model.hypers
-    np.savetxt(os.path.join(d, "alphas.dat"), model.alphas, fmt="%.17e")
-    np.savetxt(os.path.join(d, "desc.dat"), model.Qs.reshape(-1), fmt="%.17e")
-    lines = ["gap_beg soap_turbo", f"n_species = {h.n_species}",
-             "species = " + " ".join(species),
-             "rcut = " + " ".join(str(x) for x in h.rcut_hard),
-             "buffer = " + " ".join(str(x) for x in h.buffer),
-             "atom_sigma_r = " + " ".join(str(x) for x in h.atom_sigma_r),
-             "atom_sigma_t = " + " ".join(str(x) for x in h.atom_sigma_t),
-             "n_max = " + " ".join(str(x) for x in h.alpha_max), f"l_max = {h.l_max}",
-             f"zeta = {model.zeta}", f"delta = {model.delta}",
-             "alphas_sparse = alphas.dat", "desc_sparse = desc.dat", "gap_end"]
+    """The reference's formats: .gap blocks (one soap_turbo block per SOAP
+    block of the model), alphas (one per line), desc (dim*n_sparse values,
+    column-major), 2b alphas ("alpha cutoff")."""
+    lines = []
+    for k, blk in enumerate([model] + list(model.more_soap)):
+        h = blk.hypers
+        np.savetxt(os.path.join(d, f"alphas{k}.dat"), blk.alphas, fmt="%.17e")
+        np.savetxt(os.path.join(d, f"desc{k}.dat"), blk.Qs.reshape(-1), fmt="%.17e")
+        lines += ["gap_beg soap_turbo", f"n_species = {h.n_species}",
+                  "species = " + " ".join(species),
+                  "rcut = " + " ".join(str(x) for x in h.rcut_hard),
+                  "buffer = " + " ".join(str(x) for x in h.buffer),
+                  "atom_sigma_r = " + " ".join(str(x) for x in h.atom_sigma_r),
+                  "atom_sigma_t = " + " ".join(str(x) for x in h.atom_sigma_t),
+                  "n_max = " + " ".join(str(x) for x in h.alpha_max), f"l_max = {h.l_max}",
+                  f"central_species = {h.central_species}",
+                  f"zeta = {blk.zeta}", f"delta = {blk.delta}",
+                  f"alphas_sparse = alphas{k}.dat", f"desc_sparse = desc{k}.dat", "gap_end"]
     for k, b in enumerate(model.two_body):
         np.savetxt(os.path.join(d, f"a2b_{k}.dat"), np.stack([b.alphas, b.cutoff], 1), fmt="%.17e")
         np.savetxt(os.path.join(d, f"d2b_{k}.dat"), b.Qs, fmt="%.17e")
@@ -65,13 +69,36 @@ def write_model_files(d, model, species=("C",)):
     open(os.path.join(d, "model.gap"), "w").write("\n".join(lines) + "\n")
 
 
-def write_xyz(path, pos, sp, lat, symbols=("C",)):
-    with open(path, "w") as f:
+def write_xyz(path, pos, sp, lat, symbols=("C",), mode="w"):
+    with open(path, mode) as f:
         f.write(f"{len(pos)}\n")
         f.write('Lattice="' + " ".join(f"{x:.17e}" for x in lat) +
                 '" Properties=species:S:1:pos:R:3\n')
         for p, s in zip(pos, sp):
             f.write(f"{symbols[s - 1]} {p[0]:.17e} {p[1]:.17e} {p[2]:.17e}\n")
+
+
+def read_extxyz(path):
+    """Frames of an extended-XYZ file: (comment key/values, columns)."""
+    frames = []
+    lines = open(path).read().splitlines()
+    k = 0
+    while k < len(lines):
+        if not lines[k].strip():
+            k += 1
+            continue
+        n = int(lines[k])
+        head = lines[k + 1]
+        kv = {}
+        import shlex
+        for tok in shlex.split(head):
+            if "=" in tok:
+                a, b = tok.split("=", 1)
+                kv[a] = b
+        rows = [lines[k + 2 + i].split() for i in range(n)]
+        frames.append((kv, rows))
+        k += 2 + n
+    return frames
 
 
 @pytest.mark.gpu
@@ -134,3 +161,60 @@ def test_fortran_mc_driver(tmp_path, kind):
         assert int(kv["stage1_rejected"]) + int(kv["gap_evaluations"]) == n_trials
         assert int(kv["fallback"]) in (0, 1)
         assert 0 < kv["accepted"] <= kv["gap_evaluations"]
+
+
+@pytest.mark.gpu
+def test_fortran_predict_multiframe_two_blocks(tmp_path):
+    """A model of two soap_turbo blocks (C/H/O, central species C and H) and
+    three frames of different sizes in one xyz file: stdout per frame against
+    the oracle at 1e-8 eV/atom, 1e-6 eV/A; the extended-XYZ output carries
+    the same values at its printed precision (F16.8 columns, F16.6 energy)
+    with the reference's layout."""
+    import dataclasses
+
+    import structures as S
+
+    exe = os.path.join(FDIR, "tgx_predict")
+    if not os.path.exists(exe):
+        pytest.fail("tgx_predict is not built (run __graft_entry__.build())")
+    pos, sp, lat = S.cho_cell(150, 1.5, 1.0, 71)
+    h1 = dataclasses.replace(S.cho_hypers(), central_species=1)
+    h2 = dataclasses.replace(S.cho_hypers(), central_species=2, l_max=6, alpha_max=(6, 6, 6),
+                             rcut_hard=(3.5, 3.5, 3.5))
+    m1 = S.model_from_soaps(h1, oracle.soap(h1, pos, sp, lat)["soap"], 60, seed=1)
+    m2 = S.model_from_soaps(h2, oracle.soap(h2, pos, sp, lat)["soap"], 40, seed=2, zeta=3.0)
+    model = dataclasses.replace(m1, more_soap=[m2])
+    names = ("C", "H", "O")
+    write_model_files(str(tmp_path), model, species=names)
+    frames = [(pos, sp, lat), S.cho_cell(90, 1.5, 1.0, 72), S.cho_cell(120, 1.7, 1.0, 73)]
+    for k, (p, q, l) in enumerate(frames):
+        write_xyz(str(tmp_path / "s.xyz"), p, q, l, symbols=names, mode="w" if k == 0 else "a")
+    out = subprocess.run([exe, "model.gap", "s.xyz", "fp64", "out.xyz"], capture_output=True,
+                         text=True, cwd=tmp_path, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    lines = out.stdout.splitlines()
+    ext = read_extxyz(str(tmp_path / "out.xyz"))
+    assert len(ext) == len(frames)
+    k = 0
+    for fi, (p, q, l) in enumerate(frames):
+        n = len(p)
+        etot = float(lines[k].split()[1])
+        vir = np.array(lines[k + 1].split()[1:], float).reshape(3, 3).T
+        rows = np.array([r.split()[1:] for r in lines[k + 2:k + 2 + n]], float)
+        k += 2 + n
+        e0, f0, v0 = oracle.energy_forces(model, p, q, l)
+        assert abs(etot - e0.sum()) < 1e-8 * n
+        assert np.abs(rows[:, 3] - e0).max() < 1e-8
+        assert np.abs(rows[:, 4:7] - f0).max() < 1e-6
+        assert np.abs(vir - v0).max() < 1e-9 * n
+        kv, cols = ext[fi]
+        assert kv["Properties"] == "species:S:1:pos:R:3:forces:R:3:local_energy:R:1"
+        assert int(kv["i_config"]) == fi + 1
+        assert np.allclose(np.array(kv["Lattice"].split(), float), l, atol=1e-9)
+        assert abs(float(kv["energy"]) - e0.sum()) < 1e-6
+        assert np.abs(np.array(kv["virial"].split(), float).reshape(3, 3).T - v0).max() < 1e-7
+        assert [c[0] for c in cols] == [names[s - 1] for s in q]
+        arr = np.array([c[1:] for c in cols], float)
+        assert np.abs(arr[:, 0:3] - p).max() < 1e-7
+        assert np.abs(arr[:, 3:6] - f0).max() < 1e-7
+        assert np.abs(arr[:, 6] - e0).max() < 1e-7

@@ -1,8 +1,12 @@
-! Input readers for the Fortran drivers: the subset of TurboGAP's .gap model
-! format used by soap_turbo and distance_2b blocks (gap/read_gap.f90:45-771,
-! "keyword = value(s)" lines between gap_beg and gap_end), the sparse-set
-! files (read_alphas_and_descriptors, read_gap.f90:773-875) and extended-XYZ
-! frames (xyz/read_xyz.f90: Lattice="..." and species/pos[/vel] columns).
+! Input and output for the Fortran drivers: the subset of TurboGAP's .gap
+! model format used by soap_turbo and distance_2b blocks
+! (gap/read_gap.f90:45-771, "keyword = value(s)" lines between gap_beg and
+! gap_end; any number of soap_turbo blocks, as n_soap_turbo there), the
+! sparse-set files (read_alphas_and_descriptors, read_gap.f90:773-875),
+! extended-XYZ frames in (xyz/read_xyz.f90: Lattice="..." and
+! species/pos[/vel] columns; one or several frames per file) and out
+! (xyz/write_xyz.f90:177-442: species, pos, forces, local_energy columns,
+! Lattice, energy and virial on the comment line).
 module tgx_io
   use iso_c_binding
   use tgx_mod
@@ -15,18 +19,26 @@ module tgx_io
      real(c_double), allocatable :: Qs(:), alphas(:), cutoff(:)
   end type two_body_block
 
-  type, public :: gap_model
+  ! one soap_turbo block: hyper-parameters and sparse set
+  type, public :: soap_block
      type(tgx_soap_hypers) :: h
-     character(len=8) :: species(TGX_MAX_SPECIES) = ''
      integer :: dim = 0, n_sparse = 0
      real(c_double) :: delta = 1, zeta = 2
      real(c_double), allocatable :: Qs(:), alphas(:)
+  end type soap_block
+
+  type, public :: gap_model
+     integer :: n_species = 0
+     character(len=8) :: species(TGX_MAX_SPECIES) = ''
+     integer :: n_soap = 0
+     type(soap_block) :: sb(TGX_MAX_SOAP_BLOCKS)
      real(c_double) :: e0(TGX_MAX_SPECIES) = 0
      integer :: n2b = 0
      type(two_body_block) :: b2(8)
   end type gap_model
 
-  public :: read_gap_file, read_xyz_frame, species_index
+  public :: read_gap_file, read_xyz_frame, read_xyz_next, species_index, load_model, &
+       write_extxyz
 
 contains
 
@@ -47,7 +59,7 @@ contains
   ! line, column-major Qs(dim, n_sparse); dim = lines(desc)/n_sparse.
   subroutine read_soap_sparse(fa, fd, m)
     character(len=*), intent(in) :: fa, fd
-    type(gap_model), intent(inout) :: m
+    type(soap_block), intent(inout) :: m
     integer :: u, k
     m%n_sparse = count_lines(fa)
     m%dim = count_lines(fd)/m%n_sparse
@@ -89,7 +101,7 @@ contains
     character(len=*), intent(in) :: sym
     integer :: s
     species_index = 0
-    do s = 1, m%h%n_species
+    do s = 1, m%n_species
        if (trim(m%species(s)) == trim(sym)) species_index = s
     end do
   end function species_index
@@ -99,14 +111,14 @@ contains
     type(gap_model), intent(inout) :: m
     character(len=1024) :: line, key, rest, fa, fd
     character(len=64) :: kind
-    character(len=8) :: s1, s2
-    integer :: u, ios, ns, eq, k
+    character(len=8) :: s1, s2, names(TGX_MAX_SPECIES)
+    integer :: u, ios, ns, eq, k, ib
     logical :: in_soap, in_2b
     real(c_double) :: buffer(TGX_MAX_SPECIES)
     type(two_body_block) :: b
     in_soap = .false.
     in_2b = .false.
-    buffer = 0.5d0
+    ib = 0
     open (newunit=u, file=fname, status='old', action='read')
     do
        read (u, '(A)', iostat=ios) line
@@ -117,6 +129,14 @@ contains
           in_soap = trim(kind) == 'soap_turbo'
           in_2b = trim(kind) == 'distance_2b'
           if (in_soap) then
+             if (m%n_soap == TGX_MAX_SOAP_BLOCKS) then
+                write (*, *) 'ERROR: more than ', TGX_MAX_SOAP_BLOCKS, ' soap_turbo blocks'
+                stop 1
+             end if
+             m%n_soap = m%n_soap + 1
+             ib = m%n_soap
+             buffer = 0.5d0
+             names = ''
              fa = ''
              fd = ''
           end if
@@ -129,14 +149,25 @@ contains
        end if
        if (line(1:7) == 'gap_end') then
           if (in_soap) then
-             do k = 1, m%h%n_species
-                if (buffer(k) == 0.d0) then
-                   m%h%rcut_soft(k) = m%h%rcut_hard(k)
-                else
-                   m%h%rcut_soft(k) = m%h%rcut_hard(k) - buffer(k)
-                end if
-             end do
-             call read_soap_sparse(trim(fa), trim(fd), m)
+             associate (sb => m%sb(ib))
+               do k = 1, sb%h%n_species
+                  if (buffer(k) == 0.d0) then
+                     sb%h%rcut_soft(k) = sb%h%rcut_hard(k)
+                  else
+                     sb%h%rcut_soft(k) = sb%h%rcut_hard(k) - buffer(k)
+                  end if
+               end do
+               call read_soap_sparse(trim(fa), trim(fd), sb)
+               ! every block on the model's species list (the first block's)
+               if (ib == 1) then
+                  m%n_species = sb%h%n_species
+                  m%species = names
+               else if (sb%h%n_species /= m%n_species .or. &
+                        any(names(1:m%n_species) /= m%species(1:m%n_species))) then
+                  write (*, *) 'ERROR: soap_turbo block ', ib, ' has another species list'
+                  stop 1
+               end if
+             end associate
           else if (in_2b) then
              call read_2b_sparse(trim(fa), trim(fd), b)
              b%sp1 = species_index(m, s1)
@@ -152,50 +183,52 @@ contains
        if (eq == 0) cycle
        key = adjustl(line(1:eq - 1))
        rest = line(eq + 1:)
-       ns = m%h%n_species
        if (in_soap) then
-          select case (trim(key))
-          case ('n_species')
-             read (rest, *) m%h%n_species
-          case ('species')
-             read (rest, *) m%species(1:ns)
-          case ('rcut')
-             read (rest, *) m%h%rcut_hard(1:ns)
-          case ('buffer')
-             read (rest, *) buffer(1:ns)
-          case ('atom_sigma_r')
-             read (rest, *) m%h%atom_sigma_r(1:ns)
-          case ('atom_sigma_t')
-             read (rest, *) m%h%atom_sigma_t(1:ns)
-          case ('atom_sigma_r_scaling')
-             read (rest, *) m%h%atom_sigma_r_scaling(1:ns)
-          case ('atom_sigma_t_scaling')
-             read (rest, *) m%h%atom_sigma_t_scaling(1:ns)
-          case ('amplitude_scaling')
-             read (rest, *) m%h%amplitude_scaling(1:ns)
-          case ('central_weight')
-             read (rest, *) m%h%central_weight(1:ns)
-          case ('global_scaling')
-             read (rest, *) m%h%global_scaling(1:ns)
-          case ('nf')
-             read (rest, *) m%h%nf(1:ns)
-          case ('n_max')
-             read (rest, *) m%h%alpha_max(1:ns)
-          case ('l_max')
-             read (rest, *) m%h%l_max
-          case ('radial_enhancement')
-             read (rest, *) m%h%radial_enhancement
-          case ('central_species')
-             read (rest, *) m%h%central_species
-          case ('zeta')
-             read (rest, *) m%zeta
-          case ('delta')
-             read (rest, *) m%delta
-          case ('alphas_sparse')
-             read (rest, *) fa
-          case ('desc_sparse')
-             read (rest, *) fd
-          end select
+          associate (h => m%sb(ib)%h)
+            ns = h%n_species
+            select case (trim(key))
+            case ('n_species')
+               read (rest, *) h%n_species
+            case ('species')
+               read (rest, *) names(1:ns)
+            case ('rcut')
+               read (rest, *) h%rcut_hard(1:ns)
+            case ('buffer')
+               read (rest, *) buffer(1:ns)
+            case ('atom_sigma_r')
+               read (rest, *) h%atom_sigma_r(1:ns)
+            case ('atom_sigma_t')
+               read (rest, *) h%atom_sigma_t(1:ns)
+            case ('atom_sigma_r_scaling')
+               read (rest, *) h%atom_sigma_r_scaling(1:ns)
+            case ('atom_sigma_t_scaling')
+               read (rest, *) h%atom_sigma_t_scaling(1:ns)
+            case ('amplitude_scaling')
+               read (rest, *) h%amplitude_scaling(1:ns)
+            case ('central_weight')
+               read (rest, *) h%central_weight(1:ns)
+            case ('global_scaling')
+               read (rest, *) h%global_scaling(1:ns)
+            case ('nf')
+               read (rest, *) h%nf(1:ns)
+            case ('n_max')
+               read (rest, *) h%alpha_max(1:ns)
+            case ('l_max')
+               read (rest, *) h%l_max
+            case ('radial_enhancement')
+               read (rest, *) h%radial_enhancement
+            case ('central_species')
+               read (rest, *) h%central_species
+            case ('zeta')
+               read (rest, *) m%sb(ib)%zeta
+            case ('delta')
+               read (rest, *) m%sb(ib)%delta
+            case ('alphas_sparse')
+               read (rest, *) fa
+            case ('desc_sparse')
+               read (rest, *) fd
+            end select
+          end associate
        else if (in_2b) then
           select case (trim(key))
           case ('species1')
@@ -218,26 +251,65 @@ contains
     close (u)
   end subroutine read_gap_file
 
-  ! One extended-XYZ frame: Lattice="ax ay az bx by bz cx cy cz", then lines
-  ! "symbol x y z [vx vy vz]".
-  subroutine read_xyz_frame(fname, m, n, pos, species, lattice, vel)
-    character(len=*), intent(in) :: fname
+  ! Hands the model to a context: the soap_turbo blocks in file order, e0,
+  ! the distance_2b blocks.
+  subroutine load_model(ctx, m)
+    type(c_ptr), intent(in) :: ctx
+    type(gap_model), intent(in) :: m
+    integer :: k
+    do k = 1, m%n_soap
+       associate (sb => m%sb(k))
+         if (k == 1) then
+            call tgx_check(ctx, tgx_set_soap_model(ctx, sb%h, int(sb%dim, c_int), &
+                 int(sb%n_sparse, c_int), sb%Qs, sb%alphas, sb%delta, sb%zeta), &
+                 'tgx_set_soap_model')
+         else
+            call tgx_check(ctx, tgx_add_soap_model(ctx, sb%h, int(sb%dim, c_int), &
+                 int(sb%n_sparse, c_int), sb%Qs, sb%alphas, sb%delta, sb%zeta), &
+                 'tgx_add_soap_model')
+         end if
+       end associate
+    end do
+    call tgx_check(ctx, tgx_set_e0(ctx, int(m%n_species, c_int), m%e0), 'tgx_set_e0')
+    do k = 1, m%n2b
+       call tgx_check(ctx, tgx_add_2b_model(ctx, int(m%b2(k)%sp1, c_int), &
+            int(m%b2(k)%sp2, c_int), m%b2(k)%rcut, m%b2(k)%buffer, m%b2(k)%delta, &
+            m%b2(k)%sigma, int(m%b2(k)%n_sparse, c_int), m%b2(k)%Qs, m%b2(k)%alphas, &
+            m%b2(k)%cutoff), 'tgx_add_2b_model')
+    end do
+  end subroutine load_model
+
+  ! The next extended-XYZ frame on an open unit: the atom count, then a
+  ! comment line with Lattice="ax ay az bx by bz cx cy cz", then lines
+  ! "symbol x y z [vx vy vz]".  ok = .false. at the end of the file.
+  subroutine read_xyz_next(u, m, n, pos, species, lattice, vel, ok)
+    integer, intent(in) :: u
     type(gap_model), intent(in) :: m
     integer, intent(out) :: n
-    real(c_double), allocatable, intent(out) :: pos(:), vel(:)
-    integer(c_int), allocatable, intent(out) :: species(:)
+    real(c_double), allocatable, intent(inout) :: pos(:), vel(:)
+    integer(c_int), allocatable, intent(inout) :: species(:)
     real(c_double), intent(out) :: lattice(9)
+    logical, intent(out) :: ok
     character(len=4096) :: line
     character(len=8) :: sym
-    integer :: u, i, p, q, ios
+    integer :: i, p, q, ios
     real(c_double) :: v(6)
-    open (newunit=u, file=fname, status='old', action='read')
-    read (u, *) n
+    ok = .false.
+    n = 0
+    do
+       read (u, '(A)', iostat=ios) line
+       if (ios /= 0) return
+       if (len_trim(line) > 0) exit
+    end do
+    read (line, *) n
     read (u, '(A)') line
     p = index(line, 'Lattice="')
     if (p == 0) p = index(line, 'lattice="')
     q = index(line(p + 9:), '"')
     read (line(p + 9:p + 7 + q), *) lattice
+    if (allocated(pos)) deallocate (pos)
+    if (allocated(vel)) deallocate (vel)
+    if (allocated(species)) deallocate (species)
     allocate (pos(3*n), vel(3*n), species(n))
     vel = 0
     do i = 1, n
@@ -255,7 +327,58 @@ contains
           stop 1
        end if
     end do
+    ok = .true.
+  end subroutine read_xyz_next
+
+  ! The first frame of a file.
+  subroutine read_xyz_frame(fname, m, n, pos, species, lattice, vel)
+    character(len=*), intent(in) :: fname
+    type(gap_model), intent(in) :: m
+    integer, intent(out) :: n
+    real(c_double), allocatable, intent(out) :: pos(:), vel(:)
+    integer(c_int), allocatable, intent(out) :: species(:)
+    real(c_double), intent(out) :: lattice(9)
+    integer :: u
+    logical :: ok
+    open (newunit=u, file=fname, status='old', action='read')
+    call read_xyz_next(u, m, n, pos, species, lattice, vel, ok)
     close (u)
+    if (.not. ok) then
+       write (*, *) 'ERROR: no frame in ', trim(fname)
+       stop 1
+    end if
   end subroutine read_xyz_frame
+
+  ! One extended-XYZ frame of results, in the layout of the reference's
+  ! write_extxyz (xyz/write_xyz.f90:177-442) with the array properties
+  ! species, pos, forces and local_energy and the scalar properties
+  ! Lattice, energy, virial and i_config (the frame number in predict mode).
+  subroutine write_extxyz(u, m, n, species, pos, lattice, e_atom, forces, virial, i_config)
+    integer, intent(in) :: u, n, i_config
+    type(gap_model), intent(in) :: m
+    integer(c_int), intent(in) :: species(n)
+    real(c_double), intent(in) :: pos(3*n), lattice(9), e_atom(n), forces(3*n), virial(9)
+    character(len=16) :: cell(9), tmp
+    integer :: i
+    write (u, '(I8)') n
+    write (u, '(1X,A)', advance='no') 'Properties=species:S:1:pos:R:3:forces:R:3:local_energy:R:1'
+    do i = 1, 9
+       write (cell(i), '(F16.10)') lattice(i)
+    end do
+    write (u, '(1X,11A)', advance='no') 'Lattice="', adjustl(cell(1)), cell(2:9), '"'
+    write (tmp, '(F16.6)') sum(e_atom)
+    write (u, '(1X,2A)', advance='no') 'energy=', trim(adjustl(tmp))
+    do i = 1, 9
+       write (cell(i), '(F16.8)') virial(i)
+    end do
+    write (u, '(1X,11A)', advance='no') 'virial="', adjustl(cell(1)), cell(2:9), '"'
+    write (tmp, '(I10)') i_config
+    write (u, '(1X,2A)') 'i_config=', trim(adjustl(tmp))
+    do i = 1, n
+       write (u, '(1X,A8,1X,F16.8,1X,F16.8,1X,F16.8,1X,F16.8,1X,F16.8,1X,F16.8,1X,F16.8)') &
+            m%species(species(i)), pos(3*i - 2:3*i), forces(3*i - 2:3*i), e_atom(i)
+    end do
+    flush (u)
+  end subroutine write_extxyz
 
 end module tgx_io

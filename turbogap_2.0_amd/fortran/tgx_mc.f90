@@ -57,16 +57,9 @@ program tgx_mc
      write (*, *) 'ERROR: no GPU context'
      stop 1
   end if
-  call tgx_check(ctx, tgx_set_soap_model(ctx, m%h, int(m%dim, c_int), int(m%n_sparse, c_int), &
-       m%Qs, m%alphas, m%delta, m%zeta), 'tgx_set_soap_model')
-  do k = 1, m%n2b
-     call tgx_check(ctx, tgx_add_2b_model(ctx, int(m%b2(k)%sp1, c_int), int(m%b2(k)%sp2, c_int), &
-          m%b2(k)%rcut, m%b2(k)%buffer, m%b2(k)%delta, m%b2(k)%sigma, &
-          int(m%b2(k)%n_sparse, c_int), m%b2(k)%Qs, m%b2(k)%alphas, m%b2(k)%cutoff), &
-          'tgx_add_2b_model')
-  end do
+  call load_model(ctx, m)
   if (trim(kind) == 'da') then
-     call tgx_check(ctx, tgx_surrogate_init(ctx, int(m%dim, c_int), seed), 'tgx_surrogate_init')
+     call tgx_check(ctx, tgx_surrogate_init(ctx, int(m%sb(1)%dim, c_int), seed), 'tgx_surrogate_init')
      opt%train_every = max(64, min(2048, opt%n_train_trials / 4))
      call tgx_check(ctx, tgx_mc_run_da(ctx, int(n, c_int), species, lattice, pos, &
           int(n_trials, c_int), temp, move_max, skin, seed, opt, etr, st), 'tgx_mc_run_da')

@@ -51,14 +51,7 @@ program tgx_md
      write (*, *) 'ERROR: no GPU context'
      stop 1
   end if
-  call tgx_check(ctx, tgx_set_soap_model(ctx, m%h, int(m%dim, c_int), int(m%n_sparse, c_int), &
-       m%Qs, m%alphas, m%delta, m%zeta), 'tgx_set_soap_model')
-  do k = 1, m%n2b
-     call tgx_check(ctx, tgx_add_2b_model(ctx, int(m%b2(k)%sp1, c_int), int(m%b2(k)%sp2, c_int), &
-          m%b2(k)%rcut, m%b2(k)%buffer, m%b2(k)%delta, m%b2(k)%sigma, &
-          int(m%b2(k)%n_sparse, c_int), m%b2(k)%Qs, m%b2(k)%alphas, m%b2(k)%cutoff), &
-          'tgx_add_2b_model')
-  end do
+  call load_model(ctx, m)
   call tgx_check(ctx, tgx_md_begin(ctx, int(n, c_int), species, lattice, mass, pos, vel, dt, skin, &
        int(n_steps + 1, c_int)), 'tgx_md_begin')
   call tgx_check(ctx, tgx_md_steps(ctx, int(n_steps, c_int)), 'tgx_md_steps')

@@ -13,7 +13,7 @@ module tgx_mod
   implicit none
   private
 
-  integer, parameter, public :: TGX_MAX_SPECIES = 8
+  integer, parameter, public :: TGX_MAX_SPECIES = 8, TGX_MAX_SOAP_BLOCKS = 8
   integer(c_int), parameter, public :: TGX_OK = 0, TGX_ERR_ARG = 1, TGX_ERR_HIP = 2, &
        TGX_ERR_CAPACITY = 3, TGX_ERR_UNSUPPORTED = 4, TGX_ERR_STATE = 5
   integer(c_int), parameter, public :: TGX_FP64 = 0, TGX_FP32 = 1, TGX_MIXED = 2
@@ -62,7 +62,8 @@ module tgx_mod
   end type tgx_mc_stats
 
   public :: tgx_create, tgx_destroy, tgx_last_error, tgx_set_precision, tgx_set_autotune, &
-       tgx_set_soap_model, &
+       tgx_set_soap_model, tgx_add_soap_model, tgx_clear_soap_models, tgx_n_soap_models, &
+       tgx_energy_forces_noalloc, &
        tgx_set_e0, tgx_add_2b_model, tgx_clear_2b_models, tgx_neighbors_build, &
        tgx_soap_descriptors, tgx_gap_kernel_step, tgx_energy_forces, &
        tgx_energy_forces_batched, tgx_md_run, tgx_md_begin, tgx_md_steps, tgx_md_read, &
@@ -111,6 +112,26 @@ module tgx_mod
        real(c_double), intent(in) :: Qs(*), alphas(*)
        real(c_double), value :: delta, zeta
      end function tgx_set_soap_model
+
+     integer(c_int) function tgx_add_soap_model(ctx, h, dim, n_sparse, Qs, alphas, delta, zeta) &
+          bind(C, name="tgx_add_soap_model")
+       import :: c_int, c_ptr, c_double, tgx_soap_hypers
+       type(c_ptr), value :: ctx
+       type(tgx_soap_hypers), intent(in) :: h
+       integer(c_int), value :: dim, n_sparse
+       real(c_double), intent(in) :: Qs(*), alphas(*)
+       real(c_double), value :: delta, zeta
+     end function tgx_add_soap_model
+
+     integer(c_int) function tgx_clear_soap_models(ctx) bind(C, name="tgx_clear_soap_models")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function tgx_clear_soap_models
+
+     integer(c_int) function tgx_n_soap_models(ctx) bind(C, name="tgx_n_soap_models")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function tgx_n_soap_models
 
      integer(c_int) function tgx_set_e0(ctx, n_species, e0) bind(C, name="tgx_set_e0")
        import :: c_int, c_ptr, c_double
@@ -175,6 +196,16 @@ module tgx_mod
        integer(c_int), intent(in) :: species(*)
        real(c_double) :: e_atom(*), forces(*), virial(*)
      end function tgx_energy_forces
+
+     integer(c_int) function tgx_energy_forces_noalloc(ctx, n, pos, species, lattice, do_forces, &
+          e_atom, forces, virial) bind(C, name="tgx_energy_forces_noalloc")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: n, do_forces
+       real(c_double), intent(in) :: pos(*), lattice(*)
+       integer(c_int), intent(in) :: species(*)
+       real(c_double) :: e_atom(*), forces(*), virial(*)
+     end function tgx_energy_forces_noalloc
 
      integer(c_int) function tgx_energy_forces_batched(ctx, n_struct, offsets, pos, species, &
           lattices, do_forces, e_atom, forces, virials) bind(C, name="tgx_energy_forces_batched")
