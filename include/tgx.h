@@ -158,7 +158,13 @@ int tgx_clear_2b_models(tgx_ctx* ctx);
  * length of list/rjs/thetas/phis (xyz holds 3*cap); when too small,
  * TGX_ERR_CAPACITY is returned and *n_pairs holds the required size.
  * indices_out (may be NULL) receives the supercell repetitions.
- * Orthorhombic cells only (TGX_ERR_UNSUPPORTED otherwise).
+ * Orthorhombic cells take the reference's linked-cell path
+ * (neighbors_interface.f90:291-386); cells of any other shape its brute-force
+ * path (:390-421) with get_distance's general minimum image (:582-629): every
+ * supercell atom tested against every atom, O(n^2 images), lists in
+ * ascending supercell id, rjs/xyz bit-identical to that arithmetic.  The
+ * energy, force, MD and Monte Carlo entry points take orthorhombic cells
+ * only (TGX_ERR_UNSUPPORTED otherwise).
  */
 int tgx_neighbors_build(tgx_ctx* ctx, int n, const double* pos,
                         const double* lattice, double rcut, int64_t cap,

@@ -138,6 +138,177 @@ void orc_cells_for_cutoff(const double* lat, double rcut, int* indices) {
   }
 }
 
+/* get_distance, general-cell branch (neighbors_interface.f90:582-629):
+ * the matrix of lattice dot products and its explicit inverse (computed once
+ * per lattice there; here by orc_general_mic_setup), the real-valued shift
+ * from the normal equations, its nearest integer and the best of the 27
+ * integer shifts around it (first minimum in i, j, k order).  a, b, c are
+ * the supercell vectors.  Operation order as written in the reference. */
+typedef struct {
+  double a[3], b[3], c[3], mi[3][3];
+} gen_cell;
+
+static void orc_general_mic_setup(const double* a, const double* b, const double* c,
+                                  gen_cell* g) {
+  double m[3][3], md;
+  for (int k = 0; k < 3; k++) {
+    g->a[k] = a[k];
+    g->b[k] = b[k];
+    g->c[k] = c[k];
+  }
+  m[0][0] = dot3(a, a);
+  m[0][1] = dot3(a, b);
+  m[0][2] = dot3(a, c);
+  m[1][0] = m[0][1];
+  m[1][1] = dot3(b, b);
+  m[1][2] = dot3(b, c);
+  m[2][0] = m[0][2];
+  m[2][1] = m[1][2];
+  m[2][2] = dot3(c, c);
+  md = -(m[0][2] * m[0][2]) * m[1][1] + 2.0 * m[0][1] * m[0][2] * m[1][2] -
+       m[0][0] * (m[1][2] * m[1][2]) - (m[0][1] * m[0][1]) * m[2][2] +
+       m[0][0] * m[1][1] * m[2][2];
+  g->mi[0][0] = m[1][1] * m[2][2] - m[1][2] * m[1][2];
+  g->mi[0][1] = m[0][2] * m[1][2] - m[0][1] * m[2][2];
+  g->mi[0][2] = m[0][1] * m[1][2] - m[0][2] * m[1][1];
+  g->mi[1][0] = g->mi[0][1];
+  g->mi[1][1] = m[0][0] * m[2][2] - m[0][2] * m[0][2];
+  g->mi[1][2] = m[0][1] * m[0][2] - m[0][0] * m[1][2];
+  g->mi[2][0] = g->mi[0][2];
+  g->mi[2][1] = g->mi[1][2];
+  g->mi[2][2] = m[0][0] * m[1][1] - m[0][1] * m[0][1];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) g->mi[i][j] = g->mi[i][j] / md;
+}
+
+static void orc_general_mic(const double* posi, const double* posj, const gen_cell* g,
+                            double* dist, double* d) {
+  double dd[3], v[3], x[3], best[3];
+  for (int k = 0; k < 3; k++) dd[k] = posj[k] - posi[k];
+  v[0] = -1.0 * dot3(dd, g->a);
+  v[1] = -1.0 * dot3(dd, g->b);
+  v[2] = -1.0 * dot3(dd, g->c);
+  for (int i = 0; i < 3; i++) x[i] = (g->mi[i][0] * v[0] + g->mi[i][1] * v[1]) + g->mi[i][2] * v[2];
+  int nI = (int)lround(x[0]), nJ = (int)lround(x[1]), nK = (int)lround(x[2]);
+  double res_opt = 1e10;
+  best[0] = dd[0];
+  best[1] = dd[1];
+  best[2] = dd[2];
+  for (int i = nI - 1; i <= nI + 1; i++)
+    for (int j = nJ - 1; j <= nJ + 1; j++)
+      for (int k = nK - 1; k <= nK + 1; k++) {
+        double t[3];
+        for (int q = 0; q < 3; q++)
+          t[q] = ((dd[q] + (double)i * g->a[q]) + (double)j * g->b[q]) + (double)k * g->c[q];
+        const double res = dot3(t, t);
+        if (res < res_opt) {
+          res_opt = res;
+          best[0] = t[0];
+          best[1] = t[1];
+          best[2] = t[2];
+        }
+      }
+  dist[0] = best[0];
+  dist[1] = best[1];
+  dist[2] = best[2];
+  *d = sqrt(dot3(best, best));
+}
+
+/* The reference's neighbour list of a non-orthorhombic cell
+ * (neighbors_interface.f90:390-421 with get_distance :582-629 and the second
+ * get_distance pass :470-500): the supercell of read_xyz.f90:323-373 when a
+ * plane spacing is below 2 rcut, then every supercell atom j != i tested
+ * with the general minimum image against the supercell vectors; self first,
+ * then ascending supercell id. */
+static int orc_neighbors_general(int n, const double* pos, const double* lat, double rcut,
+                                 int64_t cap, int* n_neigh, int* list, double* rjs,
+                                 double* xyz, double* th, double* ph, int64_t* n_pairs,
+                                 int* indices) {
+  int idx[3];
+  orc_cells_for_cutoff(lat, rcut, idx);
+  if (indices) memcpy(indices, idx, sizeof(idx));
+  const int nrep = idx[0] * idx[1] * idx[2], nsc = n * nrep;
+  double* P = (double*)malloc(sizeof(double) * 3 * (size_t)nsc);
+  int cnt0 = 0;
+  for (int i2 = 0; i2 < idx[0]; i2++)
+    for (int j2 = 0; j2 < idx[1]; j2++)
+      for (int k2 = 0; k2 < idx[2]; k2++)
+        for (int i = 0; i < n; i++) {
+          for (int k = 0; k < 3; k++)
+            P[3 * cnt0 + k] = ((pos[3 * i + k] + (double)i2 * lat[k]) + (double)j2 * lat[3 + k]) +
+                              (double)k2 * lat[6 + k];
+          cnt0++;
+        }
+  double A[3], B[3], Cv[3];
+  for (int k = 0; k < 3; k++) {
+    A[k] = (double)idx[0] * lat[k];
+    B[k] = (double)idx[1] * lat[3 + k];
+    Cv[k] = (double)idx[2] * lat[6 + k];
+  }
+  gen_cell g;
+  orc_general_mic_setup(A, B, Cv, &g);
+  int** nb = (int**)calloc((size_t)n, sizeof(int*));
+  int* cnt = (int*)calloc((size_t)n, sizeof(int));
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < n; i++) {
+    int capi = 64, ni = 0;
+    int* buf = (int*)malloc(sizeof(int) * capi);
+    for (int k = 0; k < nsc; k++) {
+      if (k == i) continue;
+      double dv[3], dk;
+      orc_general_mic(P + 3 * i, P + 3 * k, &g, dv, &dk);
+      if (dk < rcut) {
+        if (ni == capi) buf = (int*)realloc(buf, sizeof(int) * (capi *= 2));
+        buf[ni++] = k;
+      }
+    }
+    nb[i] = buf;
+    cnt[i] = ni;
+  }
+  int64_t tot = 0;
+  for (int i = 0; i < n; i++) tot += cnt[i] + 1;
+  *n_pairs = tot;
+  int rc = TGX_OK;
+  if (tot > cap) {
+    rc = TGX_ERR_CAPACITY;
+  } else {
+    int64_t k2 = 0;
+    for (int i = 0; i < n; i++) {
+      n_neigh[i] = cnt[i] + 1;
+      list[k2] = i + 1;
+      if (rjs) rjs[k2] = 0.0;
+      if (xyz) xyz[3 * k2] = xyz[3 * k2 + 1] = xyz[3 * k2 + 2] = 0.0;
+      if (th) th[k2] = 0.0;
+      if (ph) ph[k2] = 0.0;
+      k2++;
+      for (int q = 0; q < cnt[i]; q++, k2++) {
+        const int j = nb[i][q];
+        double dv[3], d;
+        orc_general_mic(P + 3 * i, P + 3 * j, &g, dv, &d);
+        list[k2] = j + 1;
+        if (rjs) rjs[k2] = d;
+        if (xyz) {
+          xyz[3 * k2] = dv[0];
+          xyz[3 * k2 + 1] = dv[1];
+          xyz[3 * k2 + 2] = dv[2];
+        }
+        if (th) {
+          if (fabs(dv[2]) >= d)
+            th[k2] = dv[2] > 0.0 ? 0.0 : acos(-1.0);
+          else
+            th[k2] = acos(dv[2] / d);
+        }
+        if (ph) ph[k2] = atan2(dv[1], dv[0]);
+      }
+    }
+  }
+  for (int i = 0; i < n; i++) free(nb[i]);
+  free(nb);
+  free(cnt);
+  free(P);
+  return rc;
+}
+
 static int imod(int a, int m) {
   int r = a % m;
   return r < 0 ? r + m : r;
@@ -155,7 +326,8 @@ int orc_neighbors(int n, const double* pos, const double* lat, double rcut,
   const double dtol = 1e-6;
   if (!(fabs(lat[1]) < dtol && fabs(lat[2]) < dtol && fabs(lat[3]) < dtol &&
         fabs(lat[5]) < dtol && fabs(lat[6]) < dtol && fabs(lat[7]) < dtol))
-    return TGX_ERR_UNSUPPORTED;
+    return orc_neighbors_general(n, pos, lat, rcut, cap, n_neigh, list, rjs, xyz, th, ph,
+                                 n_pairs, indices);
   double A[9] = {lat[0], 0, 0, 0, lat[4], 0, 0, 0, lat[8]};
   int idx[3];
   orc_cells_for_cutoff(A, rcut, idx);

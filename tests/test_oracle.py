@@ -752,3 +752,62 @@ def test_two_soap_blocks_add_up():
     assert np.abs(v - (v1 + v2)).max() < 1e-12
     # each block's sites: block 2 contributes only to its central species' energies
     assert np.all(e2[sp != 2] == 0.0) and np.any(e2[sp == 2] != 0.0)
+
+
+def _numpy_neighbors_general(pos, lat, rc, reach=4):
+    """Independent brute force for any cell: every image r_j + T M (T in
+    [-reach, reach]^3, M rows = lattice vectors) within rc of r_i."""
+    M = np.asarray(lat, float).reshape(3, 3)
+    T = np.array([[a, b, c] for a in range(-reach, reach + 1) for b in range(-reach, reach + 1)
+                  for c in range(-reach, reach + 1)], float)
+    sh = T @ M
+    out = []
+    for i in range(len(pos)):
+        d = (pos[None, :, :] + sh[:, None, :]) - pos[i]
+        r = np.linalg.norm(d, axis=2)
+        hits = []
+        for t, j in zip(*np.nonzero(r < rc)):
+            if not (j == i and np.all(T[t] == 0)):
+                hits.append((j, round(r[t, j], 9)))
+        out.append(sorted(hits))
+    return out
+
+
+@pytest.mark.parametrize("n,rho,rc,skew,rot", [
+    (40, 2.0, 3.0, (0.3, -0.2, 0.25), False),
+    (24, 2.0, 3.7, (0.45, 0.1, -0.3), False),   # supercell images
+    (12, 1.6, 3.5, (-0.2, 0.35, 0.4), True),    # rotated, supercell
+    (60, 2.4, 2.5, (0.5, 0.5, 0.5), True),
+])
+def test_neighbors_general_cell_vs_numpy(n, rho, rc, skew, rot):
+    """General cells (get_distance's non-orthorhombic branch,
+    neighbors_interface.f90:582-629; brute force :390-421): the pair set and
+    distances against an exhaustive image search; self first, then ascending
+    supercell id; xyz, rjs, theta consistent."""
+    pos, sp, lat = S.triclinic_cell(n, rho, 1.1, 300 + n, skew=skew, rotate=rot)
+    nb = oracle.neighbors(pos, lat, rc)
+    ref = _numpy_neighbors_general(pos, lat, rc)
+    off = np.concatenate([[0], np.cumsum(nb["n_neigh"])])
+    for i in range(n):
+        lst = nb["list"][off[i]:off[i + 1]]
+        assert lst[0] == i + 1 and nb["rjs"][off[i]] == 0.0
+        got = sorted(((j - 1) % n, round(r, 9)) for j, r in zip(lst[1:], nb["rjs"][off[i] + 1:off[i + 1]]))
+        assert got == ref[i]
+        assert np.all(np.diff(lst[1:]) > 0)
+        x = nb["xyz"][off[i] + 1:off[i + 1]]
+        r = nb["rjs"][off[i] + 1:off[i + 1]]
+        assert np.abs(np.linalg.norm(x, axis=1) - r).max() < 1e-12
+        th = nb["thetas"][off[i] + 1:off[i + 1]]
+        assert np.abs(np.cos(th) * r - x[:, 2]).max() < 1e-12
+    assert np.prod(nb["indices"]) >= 1
+
+
+def test_neighbors_general_cell_rotation_invariant():
+    """Rotating the whole system leaves pair set and distances unchanged."""
+    pos, sp, lat = S.triclinic_cell(30, 2.0, 1.1, 41, skew=(0.2, 0.3, -0.1))
+    q, _ = np.linalg.qr(np.random.default_rng(5).normal(size=(3, 3)))
+    a = oracle.neighbors(pos, lat, 3.2)
+    b = oracle.neighbors(pos @ q.T, (lat.reshape(3, 3) @ q.T).reshape(9), 3.2)
+    assert np.array_equal(a["n_neigh"], b["n_neigh"])
+    assert np.array_equal(a["list"], b["list"])
+    assert np.abs(a["rjs"] - b["rjs"]).max() < 1e-12

@@ -96,6 +96,13 @@ hipError_t launch_mlp_cast(int D, const float* P, unsigned short* W1t, unsigned 
                            hipStream_t st);
 hipError_t launch_mc_init(int n, const int* site_of_atom, const double* esite, double* e_cur,
                           hipStream_t st);
+hipError_t launch_neighbors_general(const double* pos, int n, const GenCell& g, double rcut,
+                                    int maxnb, int* count, int* list, int* flag,
+                                    hipStream_t st);
+hipError_t launch_csr_out_general(const double* pos, int n, const GenCell& g, int maxnb,
+                                  const int* count, const int* list, const long long* off,
+                                  int* out_list, double* rjs, double* xyz, double* th,
+                                  double* ph, hipStream_t st);
 hipError_t launch_site_add(int n_sites, const int* site_atom, const double* esite,
                            const double* vsite, double* eatom, double* vatom, hipStream_t st);
 hipError_t launch_energy_init(int n, const int* species, const double* e0,
@@ -869,6 +876,83 @@ static int upload_structure(tgx_ctx* c, int n, const double* pos, const int* spe
   return TGX_OK;
 }
 
+// tgx_neighbors_build for a cell of any shape: the reference's brute-force
+// search (neighbors_interface.f90:390-421) over the supercell with
+// get_distance's general minimum image, padded rows on the device, then the
+// CSR export.  O(n^2 images), as in the reference.
+static int neighbors_general(tgx_ctx* c, int n, const double* pos, const double* lattice,
+                             double rcut, int64_t cap, int* n_neigh, int* list, double* rjs,
+                             double* xyz, double* thetas, double* phis, int64_t* n_pairs,
+                             int* indices_out) {
+  GenCell g;
+  std::string err;
+  int rcode = make_general_cell(lattice, rcut, &g, &err);
+  if (rcode) return set_err(c, rcode, "%s", err.c_str());
+  if ((long long)n * g.nrep >= (1LL << TGX_IMG_SHIFT))
+    return set_err(c, TGX_ERR_ARG, "too many atoms");
+  if (indices_out) {
+    indices_out[0] = g.ia;
+    indices_out[1] = g.ib;
+    indices_out[2] = g.ic;
+  }
+  for (int i = 0; i < 3 * n; i++)
+    if (!std::isfinite(pos[i])) return set_err(c, TGX_ERR_ARG, "non-finite position");
+  ENSURE(c, pos, 24 * (size_t)n);
+  ENSURE(c, nb_count, sizeof(int) * n);
+  ENSURE(c, nb_flag, sizeof(int) * 4);
+  TGX_HIP(c, hipMemcpyAsync(c->pos.p, pos, 24 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  const double vol = std::fabs(g.u[0] * (g.u[4] * g.u[8] - g.u[5] * g.u[7]) -
+                               g.u[1] * (g.u[3] * g.u[8] - g.u[5] * g.u[6]) +
+                               g.u[2] * (g.u[3] * g.u[7] - g.u[4] * g.u[6]));
+  int maxnb = guess_maxnb(n, vol, rcut);
+  bool sized = false;
+  for (int attempt = 0; attempt < 3 && !sized; attempt++) {
+    ENSURE(c, nb_list, sizeof(int) * (size_t)n * maxnb);
+    TGX_HIP(c, hipMemsetAsync(c->nb_flag.p, 0, sizeof(int) * 4, c->stream));
+    TGX_HIP(c, launch_neighbors_general(c->pos.as<double>(), n, g, rcut, maxnb,
+                                        c->nb_count.as<int>(), c->nb_list.as<int>(),
+                                        c->nb_flag.as<int>(), c->stream));
+    int flag[4];
+    TGX_HIP(c, hipMemcpyAsync(flag, c->nb_flag.p, sizeof(flag), hipMemcpyDeviceToHost, c->stream));
+    TGX_HIP(c, hipStreamSynchronize(c->stream));
+    if (flag[0] > maxnb)
+      maxnb = (flag[0] + 7) / 8 * 8;
+    else
+      sized = true;
+  }
+  if (!sized) return set_err(c, TGX_ERR_CAPACITY, "neighbour capacity could not be sized");
+  std::vector<int> cnt(n);
+  TGX_HIP(c, hipMemcpy(cnt.data(), c->nb_count.p, 4 * (size_t)n, hipMemcpyDeviceToHost));
+  std::vector<long long> off(n + 1, 0);
+  for (int i = 0; i < n; i++) off[i + 1] = off[i] + cnt[i] + 1;
+  *n_pairs = off[n];
+  if (off[n] > cap)
+    return set_err(c, TGX_ERR_CAPACITY, "need %lld pairs, cap %lld", off[n], (long long)cap);
+  if (!n_neigh || !list) return set_err(c, TGX_ERR_ARG, "null output arrays");
+  const size_t np = (size_t)off[n];
+  ENSURE(c, csr_off, 8 * (size_t)(n + 1));
+  ENSURE(c, csr_out, (4 + 8 + 24 + 8 + 8) * np + 8);
+  char* base = c->csr_out.as<char>();
+  int* d_list = reinterpret_cast<int*>(base);
+  double* d_rjs = reinterpret_cast<double*>(base + ((4 * np + 7) / 8 * 8));
+  double* d_xyz = d_rjs + np;
+  double* d_th = d_xyz + 3 * np;
+  double* d_ph = d_th + np;
+  TGX_HIP(c, hipMemcpyAsync(c->csr_off.p, off.data(), 8 * (size_t)(n + 1), hipMemcpyHostToDevice,
+                            c->stream));
+  TGX_HIP(c, launch_csr_out_general(c->pos.as<double>(), n, g, maxnb, c->nb_count.as<int>(),
+                                    c->nb_list.as<int>(), c->csr_off.as<long long>(), d_list,
+                                    d_rjs, d_xyz, d_th, d_ph, c->stream));
+  TGX_HIP(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < n; i++) n_neigh[i] = cnt[i] + 1;
+  TGX_HIP(c, hipMemcpy(list, d_list, 4 * np, hipMemcpyDeviceToHost));
+  if (rjs) TGX_HIP(c, hipMemcpy(rjs, d_rjs, 8 * np, hipMemcpyDeviceToHost));
+  if (xyz) TGX_HIP(c, hipMemcpy(xyz, d_xyz, 24 * np, hipMemcpyDeviceToHost));
+  if (thetas) TGX_HIP(c, hipMemcpy(thetas, d_th, 8 * np, hipMemcpyDeviceToHost));
+  if (phis) TGX_HIP(c, hipMemcpy(phis, d_ph, 8 * np, hipMemcpyDeviceToHost));
+  return TGX_OK;
+}
+
 // Scratch array of tgx_energy_forces_noalloc: every device buffer that call
 // needs beyond the model (structure, list, cells, descriptors, kernel-step
 // and adjoint work, outputs) is carved from it, so the call allocates
@@ -1324,6 +1408,9 @@ int tgx_neighbors_build(tgx_ctx* c, int n, const double* pos, const double* latt
   if (md_busy(c)) return TGX_ERR_STATE;
   if (n < 1 || !pos || !lattice || !n_pairs) return set_err(c, TGX_ERR_ARG, "bad arguments");
   (void)hipSetDevice(c->device);
+  if (!is_orthorhombic(lattice))
+    return neighbors_general(c, n, pos, lattice, rcut, cap, n_neigh, list, rjs, xyz, thetas,
+                             phis, n_pairs, indices_out);
   Cell cell;
   std::string err;
   int rcode = make_cell(lattice, rcut, &cell, &err);

@@ -117,21 +117,11 @@ void radial_closed_form(const SoapDev& sd, const double* W, int s, double r, dou
 // Orthorhombic cell, supercell repetitions as number_of_unit_cells_for_given_
 // cutoff (neighbors_interface.f90:655-708) and the linked-cell grid
 // mx = int(L/rcut) (:299-301).
-int make_cell(const double* lat, double rcut, Cell* c, std::string* err) {
-  const double tol = 1e-6;
-  if (!(std::fabs(lat[1]) < tol && std::fabs(lat[2]) < tol && std::fabs(lat[3]) < tol &&
-        std::fabs(lat[5]) < tol && std::fabs(lat[6]) < tol && std::fabs(lat[7]) < tol)) {
-    *err = "non-orthorhombic cells are not supported";
-    return TGX_ERR_UNSUPPORTED;
-  }
-  if (!(lat[0] > 0 && lat[4] > 0 && lat[8] > 0 && rcut > 0)) {
-    *err = "cell edges and cutoff must be positive";
-    return TGX_ERR_ARG;
-  }
-  // Reference arithmetic of number_of_unit_cells_for_given_cutoff on the
-  // (zeroed off-diagonal) lattice: unit normals, the 3x3 matrix of normal.edge,
-  // its explicit inverse, times 2*rcut, rounded up.
-  const double a[3] = {lat[0], 0, 0}, b[3] = {0, lat[4], 0}, cc[3] = {0, 0, lat[8]};
+// number_of_unit_cells_for_given_cutoff (neighbors_interface.f90:655-708)
+// in the reference's arithmetic: unit normals of the cell faces, the 3x3
+// matrix of normal.edge, its explicit inverse, times 2 rcut, rounded up.
+static void cells_for_cutoff(const double* a, const double* b, const double* cc, double rcut,
+                             int* idx) {
   auto cross = [](const double* x, const double* y, double* o) {
     o[0] = x[1] * y[2] - x[2] * y[1];
     o[1] = x[2] * y[0] - x[0] * y[2];
@@ -165,7 +155,6 @@ int make_cell(const double* lat, double rcut, Cell* c, std::string* err) {
   mi[2][0] = m[1][0] * m[2][1] - m[1][1] * m[2][0];
   mi[2][1] = m[0][1] * m[2][0] - m[0][0] * m[2][1];
   mi[2][2] = m[0][0] * m[1][1] - m[0][1] * m[1][0];
-  int idx[3];
   for (int i = 0; i < 3; i++) {
     double v = 0;
     for (int j = 0; j < 3; j++) v += (mi[i][j] / md) * (2.0 * rcut);
@@ -174,6 +163,85 @@ int make_cell(const double* lat, double rcut, Cell* c, std::string* err) {
     if (v > (double)idx[i]) idx[i] += 1;
     if (idx[i] < 1) idx[i] = 1;
   }
+}
+
+bool is_orthorhombic(const double* lat) {
+  const double tol = 1e-6;  // get_distance's d_tol
+  return std::fabs(lat[1]) < tol && std::fabs(lat[2]) < tol && std::fabs(lat[3]) < tol &&
+         std::fabs(lat[5]) < tol && std::fabs(lat[6]) < tol && std::fabs(lat[7]) < tol;
+}
+
+int make_general_cell(const double* lat, double rcut, GenCell* g, std::string* err) {
+  const double* a = lat;
+  const double* b = lat + 3;
+  const double* cc = lat + 6;
+  const double vol = a[0] * (b[1] * cc[2] - b[2] * cc[1]) - a[1] * (b[0] * cc[2] - b[2] * cc[0]) +
+                     a[2] * (b[0] * cc[1] - b[1] * cc[0]);
+  if (!(std::fabs(vol) > 0) || !(rcut > 0)) {
+    *err = "the cell vectors must span a volume and the cutoff must be positive";
+    return TGX_ERR_ARG;
+  }
+  int idx[3];
+  cells_for_cutoff(a, b, cc, rcut, idx);
+  g->ia = idx[0];
+  g->ib = idx[1];
+  g->ic = idx[2];
+  g->nrep = idx[0] * idx[1] * idx[2];
+  if (g->nrep > (1 << (31 - TGX_IMG_SHIFT))) {
+    *err = "cell too small for the cutoff (too many images)";
+    return TGX_ERR_UNSUPPORTED;
+  }
+  for (int k = 0; k < 9; k++) g->u[k] = lat[k];
+  // supercell vectors (set_supercell, read_xyz.f90:370-372)
+  for (int k = 0; k < 3; k++) {
+    g->s[k] = (double)idx[0] * a[k];
+    g->s[3 + k] = (double)idx[1] * b[k];
+    g->s[6 + k] = (double)idx[2] * cc[k];
+  }
+  // get_distance's matrix of supercell-vector dot products and its explicit
+  // inverse (neighbors_interface.f90:593-613), in the reference's order
+  auto dot = [](const double* x, const double* y) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
+  const double *A = g->s, *B = g->s + 3, *C = g->s + 6;
+  double m[3][3];
+  m[0][0] = dot(A, A);
+  m[0][1] = dot(A, B);
+  m[0][2] = dot(A, C);
+  m[1][0] = m[0][1];
+  m[1][1] = dot(B, B);
+  m[1][2] = dot(B, C);
+  m[2][0] = m[0][2];
+  m[2][1] = m[1][2];
+  m[2][2] = dot(C, C);
+  const double md = -(m[0][2] * m[0][2]) * m[1][1] + 2.0 * m[0][1] * m[0][2] * m[1][2] -
+                    m[0][0] * (m[1][2] * m[1][2]) - (m[0][1] * m[0][1]) * m[2][2] +
+                    m[0][0] * m[1][1] * m[2][2];
+  double* mi = g->mi;
+  mi[0] = m[1][1] * m[2][2] - m[1][2] * m[1][2];
+  mi[1] = m[0][2] * m[1][2] - m[0][1] * m[2][2];
+  mi[2] = m[0][1] * m[1][2] - m[0][2] * m[1][1];
+  mi[3] = mi[1];
+  mi[4] = m[0][0] * m[2][2] - m[0][2] * m[0][2];
+  mi[5] = m[0][1] * m[0][2] - m[0][0] * m[1][2];
+  mi[6] = mi[2];
+  mi[7] = mi[5];
+  mi[8] = m[0][0] * m[1][1] - m[0][1] * m[0][1];
+  for (int k = 0; k < 9; k++) mi[k] = mi[k] / md;
+  return TGX_OK;
+}
+
+int make_cell(const double* lat, double rcut, Cell* c, std::string* err) {
+  if (!is_orthorhombic(lat)) {
+    *err = "non-orthorhombic cells are supported by tgx_neighbors_build only";
+    return TGX_ERR_UNSUPPORTED;
+  }
+  if (!(lat[0] > 0 && lat[4] > 0 && lat[8] > 0 && rcut > 0)) {
+    *err = "cell edges and cutoff must be positive";
+    return TGX_ERR_ARG;
+  }
+  // the (zeroed off-diagonal) lattice, as get_distance reads it
+  const double a[3] = {lat[0], 0, 0}, b[3] = {0, lat[4], 0}, cc[3] = {0, 0, lat[8]};
+  int idx[3];
+  cells_for_cutoff(a, b, cc, rcut, idx);
   c->a = lat[0];
   c->b = lat[4];
   c->c = lat[8];

@@ -34,6 +34,15 @@ struct Cell {
   double iLa, iLb, iLc;  // 1 / supercell edges
 };
 
+// A periodic cell of any shape, for the reference's brute-force neighbour
+// search (neighbors_interface.f90:390-421): unit-cell vectors u (rows a, b,
+// c), the supercell vectors s = indices * u (read_xyz.f90:370-372) and
+// get_distance's inverse of their dot-product matrix mi (row-major).
+struct GenCell {
+  double u[9], s[9], mi[9];
+  int ia, ib, ic, nrep;
+};
+
 // Bytes of the linked-cell atom buffer for n atoms: the atom ids in cell
 // order, then (16-byte aligned) their wrapped FP32 positions with the id in
 // the fourth component, read by the one-pass list build.
@@ -189,6 +198,8 @@ void poly3_W(int nmax, double* W);                 // S^(-1/2), binary128
 void radial_closed_form(const SoapDev& sd, const double* W, int s, double r,
                         double* R, double* dR);   // FP64 host evaluation
 int make_cell(const double* lattice, double rcut, Cell* cell, std::string* err);
+bool is_orthorhombic(const double* lattice);
+int make_general_cell(const double* lattice, double rcut, GenCell* cell, std::string* err);
 
 // Launch plan of the kernel step, chosen by the autotuner (or all zero: the
 // built-in choice).  path: 0 built-in, 2 separate GEMMs.  t0: tile of the

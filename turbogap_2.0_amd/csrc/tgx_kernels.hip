@@ -683,6 +683,155 @@ __global__ void csr_out_kernel(const double* pos, int n, Cell c, int maxnb,
   }
 }
 
+// ---- Cells of any shape: the reference's brute-force search -------------
+
+// Supercell image img = (i2 ib + j2) ic + k2 of atom j, summed as
+// set_supercell does with the unit-cell vectors: ((p + i2 a) + j2 b) + k2 c.
+__device__ __forceinline__ void gen_image_pos(const double* pos, int j, int img,
+                                              const GenCell& g, double* out) {
+#pragma clang fp contract(off)
+  const double* p = pos + 3 * j;
+  const int k2 = img % g.ic, j2 = (img / g.ic) % g.ib, i2 = img / (g.ic * g.ib);
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+    out[k] = ((p[k] + (double)i2 * g.u[k]) + (double)j2 * g.u[3 + k]) + (double)k2 * g.u[6 + k];
+}
+
+__device__ __forceinline__ double dot3d(const double* x, const double* y) {
+#pragma clang fp contract(off)
+  return x[0] * y[0] + x[1] * y[1] + x[2] * y[2];
+}
+
+// get_distance's general branch (neighbors_interface.f90:582-629) against
+// the supercell vectors: the real shift from the normal equations, its
+// nearest integer (half away from zero) and the best of the 27 integer
+// shifts around it, the first minimum in i, j, k order.
+__device__ __forceinline__ double gen_mic(const double* pi, const double* pj, const GenCell& g,
+                                          double* dv) {
+#pragma clang fp contract(off)
+  double dd[3], v[3];
+  for (int k = 0; k < 3; k++) dd[k] = pj[k] - pi[k];
+  v[0] = -1.0 * dot3d(dd, g.s);
+  v[1] = -1.0 * dot3d(dd, g.s + 3);
+  v[2] = -1.0 * dot3d(dd, g.s + 6);
+  int nn[3];
+  for (int i = 0; i < 3; i++) {
+    const double x = (g.mi[3 * i] * v[0] + g.mi[3 * i + 1] * v[1]) + g.mi[3 * i + 2] * v[2];
+    nn[i] = (int)round(x);
+  }
+  double res_opt = 1e10, b0 = dd[0], b1 = dd[1], b2 = dd[2];
+  for (int i = nn[0] - 1; i <= nn[0] + 1; i++)
+    for (int j = nn[1] - 1; j <= nn[1] + 1; j++)
+      for (int k = nn[2] - 1; k <= nn[2] + 1; k++) {
+        double t[3];
+        for (int q = 0; q < 3; q++)
+          t[q] = ((dd[q] + (double)i * g.s[q]) + (double)j * g.s[3 + q]) + (double)k * g.s[6 + q];
+        const double res = dot3d(t, t);
+        if (res < res_opt) {
+          res_opt = res;
+          b0 = t[0];
+          b1 = t[1];
+          b2 = t[2];
+        }
+      }
+  dv[0] = b0;
+  dv[1] = b1;
+  dv[2] = b2;
+  return sqrt(dot3d(dv, dv));
+}
+
+// One wave per atom i of the unit cell: every supercell atom, 64 at a time
+// in ascending supercell id, tested with the general minimum image; hits
+// are compacted in that order (so rows are sorted, packed as atom | image
+// << TGX_IMG_SHIFT).  A row longer than maxnb raises flag[0] to its length.
+__global__ __launch_bounds__(64) void nb_general_kernel(const double* pos, int n, GenCell g,
+                                                        double rcut, int maxnb, int* count,
+                                                        int* list, int* flag) {
+  const int i = blockIdx.x, lane = threadIdx.x;
+  if (i >= n) return;
+  const int nsc = n * g.nrep;
+  double pi[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+  int found = 0;
+  int* row = list + (size_t)i * maxnb;
+  for (int base = 0; base < nsc; base += 64) {
+    const int J = base + lane;
+    bool hit = false;
+    if (J < nsc && J != i) {
+      const int img = J / n, j = J - img * n;
+      double pj[3], dv[3];
+      gen_image_pos(pos, j, img, g, pj);
+      hit = gen_mic(pi, pj, g, dv) < rcut;
+    }
+    const unsigned long long m = __ballot(hit);
+    if (hit) {
+      const int slot = found + __popcll(m & ((1ull << lane) - 1ull));
+      const int img = J / n;
+      if (slot < maxnb) row[slot] = (J - img * n) | (img << TGX_IMG_SHIFT);
+    }
+    found += __popcll(m);
+  }
+  if (lane == 0) {
+    count[i] = found;
+    if (found > maxnb) atomicMax(flag, found);
+  }
+}
+
+// CSR export of the general-cell rows: self first, then the entries with
+// their supercell ids, rjs, xyz (the second get_distance pass,
+// neighbors_interface.f90:470-500) and the angles with the pole guard.
+__global__ void csr_out_general_kernel(const double* pos, int n, GenCell g, int maxnb,
+                                       const int* count, const int* list, const long long* off,
+                                       int* out_list, double* rjs, double* xyz, double* th,
+                                       double* ph) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x;
+  const long long base = off[i];
+  const int cnt = count[i];
+  for (int s = threadIdx.x; s <= cnt; s += blockDim.x) {
+    const long long k = base + s;
+    if (s == 0) {
+      out_list[k] = i + 1;
+      rjs[k] = 0.0;
+      xyz[3 * k] = xyz[3 * k + 1] = xyz[3 * k + 2] = 0.0;
+      th[k] = 0.0;
+      ph[k] = 0.0;
+      continue;
+    }
+    const int e = list[(size_t)i * maxnb + s - 1];
+    const int j = e & TGX_ATOM_MASK, img = e >> TGX_IMG_SHIFT;
+    double pj[3], dv[3];
+    gen_image_pos(pos, j, img, g, pj);
+    const double d = gen_mic(pos + 3 * i, pj, g, dv);
+    out_list[k] = img * n + j + 1;
+    rjs[k] = d;
+    xyz[3 * k] = dv[0];
+    xyz[3 * k + 1] = dv[1];
+    xyz[3 * k + 2] = dv[2];
+    if (fabs(dv[2]) >= d)
+      th[k] = dv[2] > 0.0 ? 0.0 : acos(-1.0);
+    else
+      th[k] = acos(dv[2] / d);
+    ph[k] = atan2(dv[1], dv[0]);
+  }
+}
+
+hipError_t launch_neighbors_general(const double* pos, int n, const GenCell& g, double rcut,
+                                    int maxnb, int* count, int* list, int* flag,
+                                    hipStream_t st) {
+  hipLaunchKernelGGL(nb_general_kernel, dim3(n), dim3(64), 0, st, pos, n, g, rcut, maxnb, count,
+                     list, flag);
+  return hipGetLastError();
+}
+
+hipError_t launch_csr_out_general(const double* pos, int n, const GenCell& g, int maxnb,
+                                  const int* count, const int* list, const long long* off,
+                                  int* out_list, double* rjs, double* xyz, double* th,
+                                  double* ph, hipStream_t st) {
+  hipLaunchKernelGGL(csr_out_general_kernel, dim3(n), dim3(64), 0, st, pos, n, g, maxnb, count,
+                     list, off, out_list, rjs, xyz, th, ph);
+  return hipGetLastError();
+}
+
 __global__ void copy_pos_kernel(int n, const double* pos, double* pref, const int* guard) {
   GUARD_RETURN();
   const int k = blockIdx.x * blockDim.x + threadIdx.x;

@@ -76,6 +76,38 @@ def random_cell(n: int, L: float, min_dist: float, seed: int,
     return pos
 
 
+def triclinic_cell(n: int, density: float, min_dist: float, seed: int,
+                   skew=(0.3, -0.2, 0.25), rotate: bool = False, mass: float = MASS["C"]):
+    """n atoms in a non-orthorhombic periodic cell of the given density:
+    lattice a = (L,0,0), b = (s0 L, L, 0), c = (s1 L, s2 L, L) (volume L^3),
+    optionally rotated by a random rotation; atoms at uniform fractional
+    coordinates with every periodic distance >= min_dist (exhaustive image
+    search; for small cells).  Returns pos, species (all 1), lattice (9)."""
+    rng = np.random.default_rng(seed)
+    L = box_length(n, mass, density)
+    M = np.array([[L, 0, 0], [skew[0] * L, L, 0], [skew[1] * L, skew[2] * L, L]])
+    if rotate:
+        q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        if np.linalg.det(q) < 0:
+            q[:, 0] = -q[:, 0]
+        M = M @ q.T
+    shifts = np.array([[a, b, c] for a in range(-2, 3) for b in range(-2, 3)
+                       for c in range(-2, 3)], float) @ M
+    pos = np.zeros((n, 3))
+    for i in range(n):
+        for _ in range(20000):
+            p = rng.random(3) @ M
+            if i == 0:
+                break
+            d = (pos[:i, None, :] + shifts[None, :, :]) - p
+            if np.min(np.einsum("ijk,ijk->ij", d, d)) >= min_dist * min_dist:
+                break
+        else:
+            raise RuntimeError("triclinic_cell: could not insert atom %d" % i)
+        pos[i] = p
+    return pos, np.ones(n, np.int32), M.reshape(9).copy()
+
+
 def cubic_lattice(L: float) -> np.ndarray:
     return np.array([L, 0, 0, 0, L, 0, 0, 0, L], dtype=np.float64)
 
