@@ -156,6 +156,18 @@ def sph(lmax, u):
     return Y, dF
 
 
+def soap_coeffs(h: SoapHypers, si: int, x, sj):
+    """c_nlm of one site (central species si, neighbour vectors x (np, 3)
+    with species sj, 0-based) as (n_r, (l_max+1)^2), lm = l*l + l + m."""
+    x = _f64(np.atleast_2d(x)) if len(x) else np.zeros((0, 3))
+    sj = np.ascontiguousarray(sj, dtype=np.int32)
+    c = np.zeros((h.n_r, (h.l_max + 1) ** 2))
+    ch = h.to_c()
+    rc = lib().orc_soap_coeffs(C.byref(ch), si, len(x), _d(x), _i(sj), _d(c))
+    assert rc == 0, rc
+    return c
+
+
 def soap(h: SoapHypers, pos, species, lattice, derivatives=False):
     """get_soap: descriptors (n_sites, dim) and optionally soap_cart_der as
     (n_pairs, dim, 3), with pair_atom / site_atom / site_n_neigh."""

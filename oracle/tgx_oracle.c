@@ -777,21 +777,15 @@ static void soap_geom_init(const tgx_soap_hypers* h, soap_geom* g) {
   g->dim = g->npair * (h->l_max + 1);
 }
 
-/* One site: central species si, np neighbour vectors x (3 each) with species
- * sj (0-based).  Writes q (dim) and, if dq != NULL, dq[(j*dim + d)*3 + k] =
- * dq_d / dr_j,k for each neighbour j; the self derivative is -sum_j. */
-static void soap_site(const tgx_soap_hypers* h, const soap_geom* g, int si,
-                      int np, const double* x, const int* sj, double* q,
-                      double* dq) {
-  int NR = g->nr, L = g->L, L2 = g->L2, dim = g->dim;
-  double* c = (double*)calloc((size_t)NR * L2, sizeof(double));
-  double* Rv = (double*)malloc(sizeof(double) * (size_t)np * TGX_MAX_NR);
-  double* dRv = (double*)malloc(sizeof(double) * (size_t)np * TGX_MAX_NR);
-  double* AY = (double*)malloc(sizeof(double) * (size_t)np * L2);
-  double* Yv = (double*)malloc(sizeof(double) * (size_t)np * L2);
-  double* dFv = (double*)malloc(sizeof(double) * (size_t)np * L2 * 3);
-  double* Al = (double*)malloc(sizeof(double) * (size_t)np * (L + 1));
-  double* dAl = (double*)malloc(sizeof(double) * (size_t)np * (L + 1));
+/* Expansion coefficients c[(n*L2 + lm)] of one site (c zeroed by the
+ * caller): the central atom's c_n00 plus, per neighbour j, R_n(r_j) A_l(r_j)
+ * Y_lm(u_j) with the angular smearing factor A_l = exp(-l(l+1) st^2 /
+ * (2 r_j^2)).  Keeps R, dR/dr, Y, dY, A, dA/dr per neighbour for the
+ * derivative pass. */
+static void site_expand(const tgx_soap_hypers* h, const soap_geom* g, int si, int np,
+                        const double* x, const int* sj, double* c, double* Rv, double* dRv,
+                        double* AY, double* Yv, double* dFv, double* Al, double* dAl) {
+  int L = g->L, L2 = g->L2;
   /* central atom */
   {
     double Rs[TGX_MAX_NR];
@@ -826,6 +820,24 @@ static void soap_site(const tgx_soap_hypers* h, const soap_geom* g, int si,
       for (int lm = 0; lm < L2; lm++) cn[lm] += Rn * AY[(size_t)j * L2 + lm];
     }
   }
+}
+
+/* One site: central species si, np neighbour vectors x (3 each) with species
+ * sj (0-based).  Writes q (dim) and, if dq != NULL, dq[(j*dim + d)*3 + k] =
+ * dq_d / dr_j,k for each neighbour j; the self derivative is -sum_j. */
+static void soap_site(const tgx_soap_hypers* h, const soap_geom* g, int si,
+                      int np, const double* x, const int* sj, double* q,
+                      double* dq) {
+  int NR = g->nr, L = g->L, L2 = g->L2, dim = g->dim;
+  double* c = (double*)calloc((size_t)NR * L2, sizeof(double));
+  double* Rv = (double*)malloc(sizeof(double) * (size_t)np * TGX_MAX_NR);
+  double* dRv = (double*)malloc(sizeof(double) * (size_t)np * TGX_MAX_NR);
+  double* AY = (double*)malloc(sizeof(double) * (size_t)np * L2);
+  double* Yv = (double*)malloc(sizeof(double) * (size_t)np * L2);
+  double* dFv = (double*)malloc(sizeof(double) * (size_t)np * L2 * 3);
+  double* Al = (double*)malloc(sizeof(double) * (size_t)np * (L + 1));
+  double* dAl = (double*)malloc(sizeof(double) * (size_t)np * (L + 1));
+  site_expand(h, g, si, np, x, sj, c, Rv, dRv, AY, Yv, dFv, Al, dAl);
   /* power spectrum */
   double* p = (double*)calloc((size_t)dim, sizeof(double));
   int k = 0;
@@ -912,6 +924,34 @@ static void soap_site(const tgx_soap_hypers* h, const soap_geom* g, int si,
   free(dFv);
   free(Al);
   free(dAl);
+}
+
+/* Expansion coefficients c_nlm of one site (species si, np neighbour
+ * vectors x with species sj, 0-based): c[(n*(l_max+1)^2 + l*l+l+m)], n
+ * running over all species' radial channels.  Exposed so that the tests can
+ * check them against a 3-D quadrature of the smeared density. */
+int orc_soap_coeffs(const tgx_soap_hypers* h, int si, int np, const double* x, const int* sj,
+                    double* c) {
+  soap_geom g;
+  soap_geom_init(h, &g);
+  int L = g.L, L2 = g.L2, m = np > 0 ? np : 1;
+  memset(c, 0, sizeof(double) * (size_t)g.nr * L2);
+  double* Rv = (double*)malloc(sizeof(double) * (size_t)m * TGX_MAX_NR);
+  double* dRv = (double*)malloc(sizeof(double) * (size_t)m * TGX_MAX_NR);
+  double* AY = (double*)malloc(sizeof(double) * (size_t)m * L2);
+  double* Yv = (double*)malloc(sizeof(double) * (size_t)m * L2);
+  double* dFv = (double*)malloc(sizeof(double) * (size_t)m * L2 * 3);
+  double* Al = (double*)malloc(sizeof(double) * (size_t)m * (L + 1));
+  double* dAl = (double*)malloc(sizeof(double) * (size_t)m * (L + 1));
+  site_expand(h, &g, si, np, x, sj, c, Rv, dRv, AY, Yv, dFv, Al, dAl);
+  free(Rv);
+  free(dRv);
+  free(AY);
+  free(Yv);
+  free(dFv);
+  free(Al);
+  free(dAl);
+  return TGX_OK;
 }
 
 /* nf enters soap_turbo's radial expansion (read_gap.f90:164-165,192-194

@@ -78,6 +78,9 @@ void orc_radial_raw(double r, double rh, double sig, int p, int nk,
 /* Real spherical harmonics Y[l*l+l+m] at unit vector u and the gradient
  * (3 x (lmax+1)^2, component fastest) of their polynomial extension. */
 void orc_sph(int lmax, const double* u, double* Y, double* dF);
+/* c_nlm of one site: c[(n*(l_max+1)^2 + l*l+l+m)] over all radial channels. */
+int orc_soap_coeffs(const tgx_soap_hypers* h, int si, int np, const double* x, const int* sj,
+                    double* c);
 
 /* SOAP vectors (dim x n_sites) and, when soap_der != NULL, the Cartesian
  * derivatives soap_cart_der(3,dim,n_pairs) over the site pair list (pairs

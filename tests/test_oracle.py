@@ -811,3 +811,181 @@ def test_neighbors_general_cell_rotation_invariant():
     assert np.array_equal(a["n_neigh"], b["n_neigh"])
     assert np.array_equal(a["list"], b["list"])
     assert np.abs(a["rjs"] - b["rjs"]).max() < 1e-12
+
+
+# --------------------------------------------------------------------------
+# c_nlm against a 3-D quadrature of the smeared density
+# --------------------------------------------------------------------------
+def _real_sph_scipy(L, th, ph):
+    """Real spherical harmonics from scipy's complex ones, lm = l*l + l + m
+    (the convention pinned by test_real_harmonics_vs_scipy)."""
+    out = np.zeros(((L + 1) ** 2,) + np.shape(th))
+    for l in range(L + 1):
+        for m in range(-l, l + 1):
+            yc = sph_harm_y(l, abs(m), th, ph)
+            if m == 0:
+                v = yc.real
+            elif m > 0:
+                v = math.sqrt(2) * (-1) ** m * yc.real
+            else:
+                v = math.sqrt(2) * (-1) ** m * yc.imag
+            out[l * l + l + m] = v
+    return out
+
+
+def _sphere_grid(nz=200, nphi=400):
+    z, wz = np.polynomial.legendre.leggauss(nz)
+    ph = 2 * math.pi * np.arange(nphi) / nphi
+    Z, PH = np.meshgrid(z, ph, indexing="ij")
+    W = np.repeat(wz[:, None], nphi, axis=1) * (2 * math.pi / nphi)
+    th = np.arccos(Z)
+    U = np.stack([np.sin(th) * np.cos(PH), np.sin(th) * np.sin(PH), Z], axis=-1)
+    return th, PH, U, W
+
+
+def _radial_quad(h, s, r, sig):
+    """sum_a W_na int_0^rh phi_a(u) u^p G_sig(u - r) du by adaptive
+    quadrature (G the unit-area Gaussian), phi_a = N_a (rh - u)^(a+2)."""
+    rh = h.rcut_hard[s]
+    p = h.radial_enhancement
+    nmax = h.alpha_max[s]
+    _, W = oracle.poly3_overlap(nmax)
+    I = np.zeros(nmax)
+    for a in range(1, nmax + 1):
+        Na = math.sqrt((2 * a + 5) / rh ** (2 * a + 5))
+        f = lambda u: (Na * (rh - u) ** (a + 2) * u ** p
+                       * math.exp(-(u - r) ** 2 / (2 * sig * sig)) / (math.sqrt(2 * math.pi) * sig))
+        I[a - 1] = quad(f, 0.0, rh, points=[min(max(r, 1e-9), rh)], epsabs=1e-15, epsrel=1e-13,
+                        limit=200)[0]
+    return W @ I
+
+
+def _cnlm_by_quadrature(h, si, x, sj, grid):
+    """c_nlm of the density: central atom (isotropic, radial Gaussian at 0,
+    weight central_weight) plus, per neighbour, a radial Gaussian at r_j
+    times the normalised angular profile exp(kappa (u.u_j - 1)) of a 3-D
+    Gaussian of width st on the sphere of radius r_j, kappa = r_j^2/st^2.
+    Radial integrals by adaptive quadrature, angular ones by product
+    Gauss-Legendre x trapezoid quadrature on the sphere with scipy's
+    harmonics: nothing shared with the oracle's closed forms but W."""
+    th, PH, U, Wq = grid
+    L = h.l_max
+    Y = _real_sph_scipy(L, th, PH)
+    off = np.concatenate([[0], np.cumsum(h.alpha_max)])
+    c = np.zeros((h.n_r, (L + 1) ** 2))
+    c[off[si]:off[si + 1], 0] = (h.central_weight[si] * _radial_quad(h, si, 0.0, h.atom_sigma_r[si])
+                                 / math.sqrt(4 * math.pi))
+    for v, s in zip(x, sj):
+        r = np.linalg.norm(v)
+        kap = r * r / h.atom_sigma_t[s] ** 2
+        f = np.exp(kap * (U @ (v / r) - 1.0))
+        ang = np.einsum("kij,ij->k", Y, f * Wq) / np.sum(f * Wq)
+        c[off[s]:off[s + 1]] += np.outer(_radial_quad(h, s, r, h.atom_sigma_r[s]), ang)
+    return c
+
+
+def _cnlm_cases():
+    rng = np.random.default_rng(77)
+
+    def dirs(k):
+        d = rng.normal(size=(k, 3))
+        return d / np.linalg.norm(d, axis=1)[:, None]
+
+    cases = []
+    for st, p in ((0.3, 0), (0.3, 2), (0.5, 1), (0.25, 0)):
+        for rs in ((2.1,), (1.7,), (2.9,), (1.8, 2.6), (1.6, 2.2, 3.0)):
+            x = dirs(len(rs)) * np.array(rs)[:, None]
+            cases.append((st, p, 1, x, [0] * len(rs)))
+    # two species: neighbours of both, central species 1
+    x = dirs(3) * np.array([1.9, 2.4, 2.8])[:, None]
+    cases.append((0.3, 0, 2, x, [0, 1, 1]))
+    return cases
+
+
+@pytest.fixture(scope="module")
+def sphere_grid():
+    return _sphere_grid()
+
+
+@pytest.mark.parametrize("case", _cnlm_cases())
+def test_cnlm_vs_3d_quadrature(case, sphere_grid):
+    """Expansion coefficients of 1-3 neighbours equal the 3-D quadrature of
+    the smeared density up to the heat-kernel approximation of the angular
+    profile: exp(-l(l+1)/(2 kappa)) against the exact e^-k i_l(k)/(e^-k
+    i_0(k)), whose log differs by -l(l+1)/(4 kappa^2) + O(kappa^-3); the
+    test allows 0.6 l(l+1)/kappa^2 (tabulated for kappa >= 10, l <= 8).
+    Amplitude scaling off and every r_j inside rcut_soft, so the envelope
+    is 1; sigma_r 0.4 so the radial Gaussian is not a delta."""
+    st, p, nsp, x, sj = case
+    h = tgx.SoapHypers.uniform(nsp, alpha_max=(6 if nsp == 1 else 5), l_max=8, rcut_hard=4.0,
+                           buffer=0.5, atom_sigma_r=0.4, atom_sigma_t=st, amplitude_scaling=0.0,
+                           central_weight=0.7)
+    h = tgx.SoapHypers(**{**h.__dict__, "radial_enhancement": p})
+    si = nsp - 1
+    got = oracle.soap_coeffs(h, si, x, sj)
+    ref = _cnlm_by_quadrature(h, si, x, sj, sphere_grid)
+    # bound: per neighbour |R_n Y_lm A_l| * 0.6 l(l+1)/kappa^2
+    th, PH, U, Wq = sphere_grid
+    off = np.concatenate([[0], np.cumsum(h.alpha_max)])
+    bound = np.zeros_like(got)
+    for v, s in zip(x, sj):
+        r = np.linalg.norm(v)
+        kap = r * r / st ** 2
+        u = v / r
+        Yj = _real_sph_scipy(h.l_max, np.arccos(u[2]), math.atan2(u[1], u[0]))
+        Rj = _radial_quad(h, s, r, h.atom_sigma_r[s])
+        for l in range(h.l_max + 1):
+            A = math.exp(-l * (l + 1) / (2 * kap))
+            tol = 0.6 * l * (l + 1) / kap ** 2
+            sl = slice(l * l, (l + 1) ** 2)
+            bound[off[s]:off[s + 1], sl] += np.abs(np.outer(Rj, Yj[sl])) * A * tol
+    floor = 1e-9 * np.abs(ref).max()
+    assert np.all(np.abs(got - ref) <= bound + floor)
+    # the l = 0 column carries no angular approximation: tight
+    assert np.abs(got[:, 0] - ref[:, 0]).max() <= floor
+
+
+def test_cnlm_quadrature_rejects_planted_mistakes(sphere_grid):
+    """The bound above fails for plausible mistakes in the angular factor
+    or the harmonic convention, applied to the oracle's coefficients on the
+    CPU: l^2 for l(l+1), sigma_t instead of sigma_t/r, Y_l,m and Y_l,-m
+    swapped."""
+    st, r = 0.3, 2.2
+    h = tgx.SoapHypers.uniform(1, alpha_max=6, l_max=8, rcut_hard=4.0, buffer=0.5, atom_sigma_r=0.4,
+                           atom_sigma_t=st, amplitude_scaling=0.0, central_weight=0.7)
+    u = np.array([0.3, -0.5, 0.81])
+    u /= np.linalg.norm(u)
+    x = (r * u)[None, :]
+    got = oracle.soap_coeffs(h, 0, x, [0])
+    ref = _cnlm_by_quadrature(h, 0, x, [0], sphere_grid)
+    kap = r * r / st ** 2
+    Rj = _radial_quad(h, 0, r, 0.4)
+    Yj = _real_sph_scipy(8, math.acos(u[2]), math.atan2(u[1], u[0]))
+    cen = np.zeros_like(got)
+    cen[:, 0] = got[:, 0] - Rj * Yj[0]
+
+    def with_A(Afun, Y=Yj):
+        c = cen.copy()
+        for l in range(9):
+            sl = slice(l * l, (l + 1) ** 2)
+            c[:, sl] += np.outer(Rj, Y[sl]) * Afun(l)
+        return c
+
+    def within(c):
+        b = np.zeros_like(c)
+        for l in range(9):
+            sl = slice(l * l, (l + 1) ** 2)
+            b[:, sl] = np.abs(np.outer(Rj, Yj[sl])) * math.exp(-l * (l + 1) / (2 * kap)) * (
+                0.6 * l * (l + 1) / kap ** 2)
+        return np.all(np.abs(c - ref) <= b + 1e-9 * np.abs(ref).max())
+
+    # the oracle's own factor, rebuilt from its pieces, passes
+    assert np.abs(with_A(lambda l: math.exp(-l * (l + 1) * st * st / (2 * r * r))) - got).max() < 1e-12
+    assert within(got)
+    assert not within(with_A(lambda l: math.exp(-l * l * st * st / (2 * r * r))))
+    assert not within(with_A(lambda l: math.exp(-l * (l + 1) * st * st / 2)))
+    Ysw = Yj.copy()
+    for l in range(1, 9):
+        for m in range(1, l + 1):
+            Ysw[l * l + l + m], Ysw[l * l + l - m] = Yj[l * l + l - m], Yj[l * l + l + m]
+    assert not within(with_A(lambda l: math.exp(-l * (l + 1) * st * st / (2 * r * r)), Ysw))
