@@ -43,7 +43,7 @@ enum {
   TGX_ERR_ARG = 1,         /* bad argument or model/shape mismatch           */
   TGX_ERR_HIP = 2,         /* HIP runtime failure                            */
   TGX_ERR_CAPACITY = 3,    /* caller buffer too small (size is reported)     */
-  TGX_ERR_UNSUPPORTED = 4, /* e.g. non-orthorhombic cell                     */
+  TGX_ERR_UNSUPPORTED = 4, /* e.g. non-orthorhombic cell for MD / MC        */
   TGX_ERR_STATE = 5        /* no model loaded, wrong call order              */
 };
 
@@ -162,8 +162,9 @@ int tgx_clear_2b_models(tgx_ctx* ctx);
  * (neighbors_interface.f90:291-386); cells of any other shape its brute-force
  * path (:390-421) with get_distance's general minimum image (:582-629): every
  * supercell atom tested against every atom, O(n^2 images), lists in
- * ascending supercell id, rjs/xyz bit-identical to that arithmetic.  The
- * energy, force, MD and Monte Carlo entry points take orthorhombic cells
+ * ascending supercell id, rjs/xyz bit-identical to that arithmetic.
+ * tgx_energy_forces(_noalloc) and tgx_soap_descriptors take cells of any
+ * shape too; MD, Monte Carlo and batched structures take orthorhombic cells
  * only (TGX_ERR_UNSUPPORTED otherwise).
  */
 int tgx_neighbors_build(tgx_ctx* ctx, int n, const double* pos,
@@ -199,7 +200,10 @@ int tgx_gap_kernel_step(tgx_ctx* ctx, int n_sites, const double* soap,
  * calculate_gap_2b gap.f90:246-327 + calculate_e0, summed as
  * turbogap_main.f90:993-1109 does).  e_atom(n), forces(3,n), virial(3,3).
  * With do_forces = 0 only e_atom is written (MC mode,
- * control/read_control.f90:47-51).
+ * control/read_control.f90:47-51).  A cell of any shape is accepted: its
+ * atoms are wrapped into the cell and every lattice translation within reach
+ * of the cutoff becomes a set of image atoms (the pairs of the reference's
+ * supercell list, neighbors_interface.f90:390-421), searched brute force.
  */
 int tgx_energy_forces(tgx_ctx* ctx, int n, const double* pos,
                       const int* species, const double* lattice,

@@ -275,10 +275,14 @@ def test_bad_arguments_raise(eng):
     pos = np.zeros((2, 3))
     with pytest.raises(tgx.TgxError):
         eng.energy_forces(pos, np.array([1, 5], np.int32), S.cubic_lattice(10.0))
-    tri = np.array([10.0, 0, 0, 1.0, 10.0, 0, 0, 0, 10.0])
     with pytest.raises(tgx.TgxError) as ei:
-        eng.energy_forces(pos + [[0, 0, 0], [1.5, 0, 0]], np.array([1, 1], np.int32), tri)
-    assert ei.value.code == tgx.TGX_ERR_UNSUPPORTED
+        eng.energy_forces(pos + [[0, 0, 0], [np.nan, 0, 0]], np.array([1, 1], np.int32),
+                          S.cubic_lattice(10.0))
+    assert ei.value.code == tgx.TGX_ERR_ARG
+    flat = np.array([10.0, 0, 0, 1.0, 10.0, 0, 11.0, 1.0, 0.0])  # no volume
+    with pytest.raises(tgx.TgxError) as ei:
+        eng.energy_forces(pos + [[0, 0, 0], [1.5, 0, 0]], np.array([1, 1], np.int32), flat)
+    assert ei.value.code == tgx.TGX_ERR_ARG
 
 
 def test_list_buffer_does_not_change_results(eng):

@@ -989,3 +989,22 @@ def test_cnlm_quadrature_rejects_planted_mistakes(sphere_grid):
         for m in range(1, l + 1):
             Ysw[l * l + l + m], Ysw[l * l + l - m] = Yj[l * l + l - m], Yj[l * l + l + m]
     assert not within(with_A(lambda l: math.exp(-l * (l + 1) * st * st / (2 * r * r)), Ysw))
+
+
+@pytest.mark.parametrize("rot", [None, 4])
+def test_general_cell_energy_forces_same_crystal(rot):
+    """The oracle's energies, forces and virial on the reference's
+    general-cell list equal those of the orthorhombic path for the same
+    crystal in another basis (a, a + b, c - 2b), rotated or not."""
+    pos, sp, lat, model = W.carbon216()
+    L = np.asarray(lat).reshape(3, 3)
+    M = np.array([L[0], L[1] + L[0], L[2] - 2 * L[1]])
+    A = np.eye(3)
+    if rot is not None:
+        A = np.linalg.qr(np.random.default_rng(rot).normal(size=(3, 3)))[0]
+    e1, f1, v1 = oracle.energy_forces(model, pos, sp, lat)[:3]
+    e2, f2, v2 = oracle.energy_forces(model, pos @ A.T, sp, (M @ A.T).reshape(9))[:3]
+    assert oracle.neighbors(pos @ A.T, (M @ A.T).reshape(9), 4.2)["indices"].prod() > 1
+    assert np.abs(e1 - e2).max() < 1e-10
+    assert np.abs(f1 @ A.T - f2).max() < 1e-9
+    assert np.abs(A @ v1 @ A.T - v2).max() < 1e-8

@@ -103,6 +103,11 @@ hipError_t launch_csr_out_general(const double* pos, int n, const GenCell& g, in
                                   const int* count, const int* list, const long long* off,
                                   int* out_list, double* rjs, double* xyz, double* th,
                                   double* ph, hipStream_t st);
+hipError_t launch_image_atoms(const double* pos0, const int* sp0, int n, const ImgCell& g,
+                              double* pos, int* sp, hipStream_t st);
+hipError_t launch_neighbors_image(const double* pos, int n, const ImgCell& g, double rcut,
+                                  int maxnb, int* count, int* list, int* rev, int* flag,
+                                  hipStream_t st);
 hipError_t launch_site_add(int n_sites, const int* site_atom, const double* esite,
                            const double* vsite, double* eatom, double* vatom, hipStream_t st);
 hipError_t launch_energy_init(int n, const int* species, const double* e0,
@@ -196,6 +201,7 @@ struct Extra {
   float sg_mu = 0.f, sg_sigma = 1.f;
   bool sg_norm_set = false;
   DevBuf twob_store;
+  DevBuf g_pos, g_sp;  // the structure itself while its image atoms fill pos / species
   std::vector<int> species_host;
   // autotuner: on (-1 = not yet read from TGX_AUTOTUNE), cache file, the
   // decisions by shape key, and a copy of the atom energies (the adjoint
@@ -301,6 +307,62 @@ static int build_list(tgx_ctx* c, int n, const Cell& cell, double rcut, bool wan
   return set_err(c, TGX_ERR_CAPACITY, "neighbour capacity could not be sized");
 }
 
+// Image atoms and their list for a cell of any shape (energies, forces,
+// descriptors).  The uploaded structure moves to g_pos / g_sp, the n * nt
+// image atoms (wrapped atoms plus each lattice translation) fill pos /
+// species, and the rows over them (with the reverse index when forces are
+// wanted) are built; the pair kernels then take the vectors as they are
+// (*open_out has no minimum image).  Brute force over all images, as the
+// reference's search for such cells.
+static int prepare_image_cell(tgx_ctx* c, int n, const double* lattice, double rcut,
+                              bool want_rev, Cell* open_out, int* n_ext_out) {
+  Extra& x = X(c);
+  ImgCell g;
+  std::string err;
+  int rcode = make_image_cell(lattice, rcut, &g, &err);
+  if (rcode) return set_err(c, rcode, "%s", err.c_str());
+  const long long next = (long long)n * g.nt;
+  if (next >= (1LL << TGX_IMG_SHIFT))
+    return set_err(c, TGX_ERR_UNSUPPORTED, "%d atoms x %d lattice translations: too many image atoms",
+                   n, g.nt);
+  if (grow(c, x.g_pos, 24 * (size_t)n) || grow(c, x.g_sp, 4 * (size_t)n))
+    return grow_err(c, 28 * (size_t)n, "structure copy");
+  TGX_HIP(c, hipMemcpyAsync(x.g_pos.p, c->pos.p, 24 * (size_t)n, hipMemcpyDeviceToDevice,
+                            c->stream));
+  TGX_HIP(c, hipMemcpyAsync(x.g_sp.p, c->species.p, 4 * (size_t)n, hipMemcpyDeviceToDevice,
+                            c->stream));
+  TGX_HIP(c, hipStreamSynchronize(c->stream));  // before pos / species may be replaced
+  ENSURE(c, pos, 24 * (size_t)next);
+  ENSURE(c, species, 4 * (size_t)next);
+  TGX_HIP(c, launch_image_atoms(x.g_pos.as<double>(), x.g_sp.as<int>(), n, g, c->pos.as<double>(),
+                                c->species.as<int>(), c->stream));
+  ENSURE(c, nb_count, sizeof(int) * n);
+  ENSURE(c, nb_flag, sizeof(int) * 4);
+  int maxnb = c->maxnb > 0 ? c->maxnb : guess_maxnb(n, g.vol, rcut);
+  for (int attempt = 0; attempt < 3; attempt++) {
+    ENSURE(c, nb_list, sizeof(int) * (size_t)n * maxnb);
+    if (want_rev) ENSURE(c, nb_rev, sizeof(int) * (size_t)n * maxnb);
+    TGX_HIP(c, hipMemsetAsync(c->nb_flag.p, 0, sizeof(int) * 4, c->stream));
+    TGX_HIP(c, launch_neighbors_image(c->pos.as<double>(), n, g, rcut, maxnb,
+                                      c->nb_count.as<int>(), c->nb_list.as<int>(),
+                                      want_rev ? c->nb_rev.as<int>() : nullptr,
+                                      c->nb_flag.as<int>(), c->stream));
+    int flag[4];
+    TGX_HIP(c, hipMemcpyAsync(flag, c->nb_flag.p, sizeof(flag), hipMemcpyDeviceToHost, c->stream));
+    TGX_HIP(c, hipStreamSynchronize(c->stream));
+    if (flag[0] > maxnb) {
+      maxnb = (flag[0] + flag[0] / 4 + 8 + 7) / 8 * 8;
+      continue;
+    }
+    if (flag[1]) return set_err(c, TGX_ERR_STATE, "neighbour list is not symmetric");
+    c->maxnb = maxnb;
+    *open_out = open_cell();
+    *n_ext_out = (int)next;
+    return TGX_OK;
+  }
+  return set_err(c, TGX_ERR_CAPACITY, "neighbour capacity could not be sized");
+}
+
 static double rcut_all(const tgx_ctx* c) {
   double r = 0.0;
   if (c->has_soap)
@@ -311,14 +373,18 @@ static double rcut_all(const tgx_ctx* c) {
 
 // Sites of every SOAP block: all atoms, or those of its central_species.
 // *n_sites receives the first block's count (each block keeps its own).
-static int setup_sites(tgx_ctx* c, int n, const int* species_host, int* n_sites) {
+// With image atoms (n_ext > n) site_of_atom also covers them: the site of
+// image J is that of atom J % n.
+static int setup_sites(tgx_ctx* c, int n, const int* species_host, int* n_sites,
+                       int n_ext = 0) {
   SoapBlock* const keep = c->blk;
+  if (n_ext < n) n_ext = n;
   for (auto& bp : c->blocks) {
     c->blk = bp.get();
     SoapBlock& b = *bp;
-    ENSURE(c, blk->site_of_atom, sizeof(int) * n);
+    ENSURE(c, blk->site_of_atom, sizeof(int) * n_ext);
     ENSURE(c, blk->site_atom, sizeof(int) * n);
-    std::vector<int> soa(n), sa;
+    std::vector<int> soa(n_ext), sa;
     sa.reserve(n);
     for (int i = 0; i < n; i++) {
       if (c->has_soap && (b.sd.central_species == 0 || species_host[i] == b.sd.central_species)) {
@@ -328,7 +394,8 @@ static int setup_sites(tgx_ctx* c, int n, const int* species_host, int* n_sites)
         soa[i] = -1;
       }
     }
-    TGX_HIP(c, hipMemcpyAsync(b.site_of_atom.p, soa.data(), sizeof(int) * n,
+    for (int J = n; J < n_ext; J++) soa[J] = soa[J % n];
+    TGX_HIP(c, hipMemcpyAsync(b.site_of_atom.p, soa.data(), sizeof(int) * n_ext,
                               hipMemcpyHostToDevice, c->stream));
     if (!sa.empty())
       TGX_HIP(c, hipMemcpyAsync(b.site_atom.p, sa.data(), sizeof(int) * sa.size(),
@@ -564,6 +631,18 @@ static int run_ef(tgx_ctx* c, int n, const double* lattice, const int* species_h
   Cell cell;
   std::string err;
   const double rc = rcut_all(c) + c->list_buffer;
+  if (!is_orthorhombic(lattice)) {
+    x.ev_on = true;
+    ev_rec(c, 0);
+    int n_ext = n;
+    int rcode = prepare_image_cell(c, n, lattice, rc, do_forces != 0, &cell, &n_ext);
+    if (rcode) return rcode;
+    ev_rec(c, 1);
+    int n_sites = 0;
+    rcode = setup_sites(c, n, species_host, &n_sites, n_ext);
+    if (rcode) return rcode;
+    return ef_launch(c, n, n_sites, Geom{cell}, do_forces, true);
+  }
   int rcode = make_cell(lattice, rc, &cell, &err);
   if (rcode) return set_err(c, rcode, "%s", err.c_str());
   if ((long long)n * cell.nrep >= (1LL << TGX_IMG_SHIFT))
@@ -1472,17 +1551,22 @@ int tgx_soap_descriptors(tgx_ctx* c, int n, const double* pos, const int* specie
   if (rcode) return rcode;
   Cell cell;
   std::string err;
-  rcode = make_cell(lattice, c->blk->sd.rcut_max, &cell, &err);
-  if (rcode) return set_err(c, rcode, "%s", err.c_str());
+  const bool general = !is_orthorhombic(lattice);
+  if (!general) {
+    rcode = make_cell(lattice, c->blk->sd.rcut_max, &cell, &err);
+    if (rcode) return set_err(c, rcode, "%s", err.c_str());
+  }
   const int saved = c->maxnb;
   c->maxnb = 0;
-  rcode = build_list(c, n, cell, c->blk->sd.rcut_max, false);
+  int n_ext = n;
+  rcode = general ? prepare_image_cell(c, n, lattice, c->blk->sd.rcut_max, false, &cell, &n_ext)
+                  : build_list(c, n, cell, c->blk->sd.rcut_max, false);
   if (rcode) {
     c->maxnb = saved;
     return rcode;
   }
   int n_sites = 0;
-  rcode = setup_sites(c, n, species, &n_sites);
+  rcode = setup_sites(c, n, species, &n_sites, n_ext);
   if (!rcode && n_sites > 0) rcode = soap_forward(c, n, Geom{cell}, n_sites);
   c->maxnb = saved;
   if (rcode) return rcode;

@@ -164,8 +164,11 @@ __device__ __forceinline__ double mic(const double* pi, const double* pj,
   double d2 = 0.0;
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    double t = f_modulo(pj[k] - pi[k], L[k]);
-    if (t > L[k] / 2.0) t = t - L[k];
+    double t = pj[k] - pi[k];
+    if (L[k] > 0.0) {  // an edge of 0 (open cell): the vector as it is
+      t = f_modulo(t, L[k]);
+      if (t > L[k] / 2.0) t = t - L[k];
+    }
     dv[k] = t;
     d2 = d2 + t * t;
   }

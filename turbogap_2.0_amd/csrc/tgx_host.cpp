@@ -229,9 +229,65 @@ int make_general_cell(const double* lat, double rcut, GenCell* g, std::string* e
   return TGX_OK;
 }
 
+int make_image_cell(const double* lat, double rcut, ImgCell* g, std::string* err) {
+  const double* a = lat;
+  const double* b = lat + 3;
+  const double* c = lat + 6;
+  auto cross = [](const double* x, const double* y, double* z) {
+    z[0] = x[1] * y[2] - x[2] * y[1];
+    z[1] = x[2] * y[0] - x[0] * y[2];
+    z[2] = x[0] * y[1] - x[1] * y[0];
+  };
+  double bc[3], ca[3], ab[3];
+  cross(b, c, bc);
+  cross(c, a, ca);
+  cross(a, b, ab);
+  const double det = a[0] * bc[0] + a[1] * bc[1] + a[2] * bc[2];
+  if (!(std::fabs(det) > 0) || !(rcut > 0)) {
+    *err = "the cell vectors must span a volume and the cutoff must be positive";
+    return TGX_ERR_ARG;
+  }
+  // inverse of the row matrix (rows a, b, c): columns are the reciprocal
+  // vectors b x c, c x a, a x b over det
+  for (int k = 0; k < 3; k++) {
+    g->ui[3 * k + 0] = bc[k] / det;
+    g->ui[3 * k + 1] = ca[k] / det;
+    g->ui[3 * k + 2] = ab[k] / det;
+  }
+  for (int k = 0; k < 9; k++) g->u[k] = lat[k];
+  g->vol = std::fabs(det);
+  // a pair within rcut of wrapped atoms (fractional coordinates in [0, 1])
+  // needs |t_k| <= rcut / w_k + 1, w_k the cell's width across axis k
+  const double* rec[3] = {bc, ca, ab};
+  long long nt = 1;
+  for (int k = 0; k < 3; k++) {
+    const double w = g->vol / std::sqrt(rec[k][0] * rec[k][0] + rec[k][1] * rec[k][1] +
+                                        rec[k][2] * rec[k][2]);
+    const double mk = std::floor(rcut / w) + 1.0;
+    if (!(mk < 64)) {
+      *err = "cell too thin for the cutoff";
+      return TGX_ERR_UNSUPPORTED;
+    }
+    g->m[k] = (int)mk;
+    nt *= 2 * g->m[k] + 1;
+  }
+  g->nt = (int)nt;
+  return TGX_OK;
+}
+
+Cell open_cell() {
+  Cell c{};
+  c.ia = c.ib = c.ic = 1;
+  c.nrep = 1;
+  c.small = 0;
+  c.mx = c.my = c.mz = 1;
+  return c;  // edges and inverse edges 0: no minimum image
+}
+
 int make_cell(const double* lat, double rcut, Cell* c, std::string* err) {
   if (!is_orthorhombic(lat)) {
-    *err = "non-orthorhombic cells are supported by tgx_neighbors_build only";
+    *err = "non-orthorhombic cells are supported by tgx_neighbors_build, tgx_energy_forces, "
+           "tgx_energy_forces_noalloc and tgx_soap_descriptors only";
     return TGX_ERR_UNSUPPORTED;
   }
   if (!(lat[0] > 0 && lat[4] > 0 && lat[8] > 0 && rcut > 0)) {

@@ -43,6 +43,18 @@ struct GenCell {
   int ia, ib, ic, nrep;
 };
 
+// A periodic cell of any shape for energies, forces and descriptors: the
+// atoms are wrapped into the cell and every lattice translation t with
+// |t_k| <= m[k] of them becomes an image atom J = img * n + j.  img 0 is
+// t = 0 (the atoms themselves); the image of -t is (nt - img) % nt.
+struct ImgCell {
+  double u[9];   // rows a, b, c
+  double ui[9];  // inverse of the row matrix: fractional f = p * ui
+  int m[3];      // translation range per axis
+  int nt;        // (2 m0 + 1)(2 m1 + 1)(2 m2 + 1)
+  double vol;    // |det|
+};
+
 // Bytes of the linked-cell atom buffer for n atoms: the atom ids in cell
 // order, then (16-byte aligned) their wrapped FP32 positions with the id in
 // the fourth component, read by the one-pass list build.
@@ -200,6 +212,9 @@ void radial_closed_form(const SoapDev& sd, const double* W, int s, double r,
 int make_cell(const double* lattice, double rcut, Cell* cell, std::string* err);
 bool is_orthorhombic(const double* lattice);
 int make_general_cell(const double* lattice, double rcut, GenCell* cell, std::string* err);
+int make_image_cell(const double* lattice, double rcut, ImgCell* cell, std::string* err);
+// Orthorhombic Cell with no periodicity (pair vectors taken as they are).
+Cell open_cell();
 
 // Launch plan of the kernel step, chosen by the autotuner (or all zero: the
 // built-in choice).  path: 0 built-in, 2 separate GEMMs.  t0: tile of the

@@ -3,6 +3,7 @@
 // term, energy assembly and deterministic reductions.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "tgx_device.h"
@@ -829,6 +830,139 @@ hipError_t launch_csr_out_general(const double* pos, int n, const GenCell& g, in
                                   double* ph, hipStream_t st) {
   hipLaunchKernelGGL(csr_out_general_kernel, dim3(n), dim3(64), 0, st, pos, n, g, maxnb, count,
                      list, off, out_list, rjs, xyz, th, ph);
+  return hipGetLastError();
+}
+
+// ---- Cells of any shape for energies and forces: image atoms -------------
+
+// Lattice translation of image img (img 0 = t 0): t1 a + t2 b + t3 c.  The
+// translation of -t is the exact negation (every operation commutes with it).
+__device__ __forceinline__ void img_translation(const ImgCell& g, int img, double* t) {
+#pragma clang fp contract(off)
+  const int n2 = 2 * g.m[1] + 1, n3 = 2 * g.m[2] + 1;
+  int idx = img + (g.nt - 1) / 2;
+  if (idx >= g.nt) idx -= g.nt;
+  const double t3 = (double)(idx % n3 - g.m[2]);
+  const double t2 = (double)((idx / n3) % n2 - g.m[1]);
+  const double t1 = (double)(idx / (n2 * n3) - g.m[0]);
+#pragma unroll
+  for (int k = 0; k < 3; k++) t[k] = (t1 * g.u[k] + t2 * g.u[3 + k]) + t3 * g.u[6 + k];
+}
+
+// Image atoms J = img * n + j: atom j wrapped into the cell (fractional
+// coordinates floored), plus translation img; species copied.
+__global__ void image_atoms_kernel(const double* pos0, const int* sp0, int n, ImgCell g,
+                                   double* pos, int* sp) {
+#pragma clang fp contract(off)
+  const long long total = (long long)n * g.nt;
+  for (long long J = blockIdx.x * (long long)blockDim.x + threadIdx.x; J < total;
+       J += (long long)gridDim.x * blockDim.x) {
+    const int img = (int)(J / n), j = (int)(J - (long long)img * n);
+    const double* p = pos0 + 3 * j;
+    double f[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      f[k] = (p[0] * g.ui[k] + p[1] * g.ui[3 + k]) + p[2] * g.ui[6 + k];
+      f[k] = f[k] - floor(f[k]);
+    }
+    double t[3] = {0.0, 0.0, 0.0};
+    if (img) img_translation(g, img, t);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const double w = (f[0] * g.u[k] + f[1] * g.u[3 + k]) + f[2] * g.u[6 + k];
+      pos[3 * J + k] = img ? w + t[k] : w;
+    }
+    sp[J] = sp0[j];
+  }
+}
+
+// Rows of the image-atom list: one wave per atom i, images in ascending J
+// (translation-major), pair vector (w_j - w_i) + t so that the pair (j, -t)
+// of atom j sees exactly the negated vector and the list is symmetric.
+// A row longer than maxnb raises flag[0] to its length.
+__global__ __launch_bounds__(64) void nb_image_kernel(const double* pos, int n, ImgCell g,
+                                                      double rcut2, int maxnb, int* count,
+                                                      int* list, int* flag) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x, lane = threadIdx.x;
+  if (i >= n) return;
+  const double pi0 = pos[3 * i], pi1 = pos[3 * i + 1], pi2 = pos[3 * i + 2];
+  int found = 0;
+  int* row = list + (size_t)i * maxnb;
+  for (int img = 0; img < g.nt; img++) {
+    double t[3] = {0.0, 0.0, 0.0};
+    if (img) img_translation(g, img, t);
+    for (int jb = 0; jb < n; jb += 64) {
+      const int j = jb + lane;
+      const int J = img * n + j;
+      bool hit = false;
+      if (j < n && J != i) {
+        const double d0 = (pos[3 * j] - pi0) + t[0];
+        const double d1 = (pos[3 * j + 1] - pi1) + t[1];
+        const double d2 = (pos[3 * j + 2] - pi2) + t[2];
+        hit = (d0 * d0 + d1 * d1) + d2 * d2 < rcut2;
+      }
+      const unsigned long long m = __ballot(hit);
+      if (hit) {
+        const int slot = found + __popcll(m & ((1ull << lane) - 1ull));
+        if (slot < maxnb) row[slot] = J;
+      }
+      found += __popcll(m);
+    }
+  }
+  if (lane == 0) {
+    count[i] = found;
+    if (found > maxnb) atomicMax(flag, found);
+  }
+}
+
+// rev[i][slot] = flat index of the entry (image of -t, atom i) in row j;
+// one wave per atom, four atoms per workgroup.  A missing counterpart sets
+// flag[1] and points the slot at itself.
+__global__ __launch_bounds__(256) void reverse_image_kernel(int n, int nt, int maxnb,
+                                                            const int* count, const int* list,
+                                                            int* rev, int* flag) {
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= n) return;
+  const int cnt = min(count[i], maxnb);
+  for (int s = threadIdx.x & 63; s < cnt; s += 64) {
+    const int J = list[(size_t)i * maxnb + s];
+    const int img = J / n, j = J - img * n;
+    const int key = (img ? nt - img : 0) * n + i;
+    const int* a = list + (size_t)j * maxnb;
+    int lo = 0, hi = min(count[j], maxnb) - 1, found = -1;
+    while (lo <= hi) {
+      const int mid = (lo + hi) >> 1;
+      const int v = a[mid];
+      if (v == key) {
+        found = mid;
+        break;
+      }
+      if (v < key) lo = mid + 1;
+      else hi = mid - 1;
+    }
+    if (found < 0) atomicOr(flag + 1, 1);
+    rev[(size_t)i * maxnb + s] = found < 0 ? (int)((size_t)i * maxnb + s) : (int)((size_t)j * maxnb + found);
+  }
+}
+
+hipError_t launch_image_atoms(const double* pos0, const int* sp0, int n, const ImgCell& g,
+                              double* pos, int* sp, hipStream_t st) {
+  const long long total = (long long)n * g.nt;
+  const int blocks = (int)std::min<long long>((total + 255) / 256, 4096);
+  hipLaunchKernelGGL(image_atoms_kernel, dim3(blocks), dim3(256), 0, st, pos0, sp0, n, g, pos, sp);
+  return hipGetLastError();
+}
+
+hipError_t launch_neighbors_image(const double* pos, int n, const ImgCell& g, double rcut,
+                                  int maxnb, int* count, int* list, int* rev, int* flag,
+                                  hipStream_t st) {
+  hipLaunchKernelGGL(nb_image_kernel, dim3(n), dim3(64), 0, st, pos, n, g, rcut * rcut, maxnb,
+                     count, list, flag);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !rev) return e;
+  hipLaunchKernelGGL(reverse_image_kernel, dim3((n + 3) / 4), dim3(256), 0, st, n, g.nt, maxnb,
+                     count, list, rev, flag);
   return hipGetLastError();
 }
 
