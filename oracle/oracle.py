@@ -33,6 +33,20 @@ class C2b(C.Structure):
                 ("cutoff", C.POINTER(C.c_double))]
 
 
+class C3b(C.Structure):
+    _fields_ = [("sp0", C.c_int32), ("sp1", C.c_int32), ("sp2", C.c_int32),
+                ("n_sparse", C.c_int32), ("kernel", C.c_int32), ("pad_", C.c_int32),
+                ("rcut", C.c_double), ("delta", C.c_double), ("sigma", C.c_double * 3),
+                ("Qs", C.POINTER(C.c_double)), ("alphas", C.POINTER(C.c_double)),
+                ("cutoff", C.POINTER(C.c_double))]
+
+
+class CCore(C.Structure):
+    _fields_ = [("sp1", C.c_int32), ("sp2", C.c_int32), ("n", C.c_int32), ("pad_", C.c_int32),
+                ("yp1", C.c_double), ("ypn", C.c_double), ("x", C.POINTER(C.c_double)),
+                ("V", C.POINTER(C.c_double)), ("d2", C.POINTER(C.c_double))]
+
+
 class CModel(C.Structure):
     pass
 
@@ -42,7 +56,9 @@ CModel._fields_ = [("h", CSoapHypers), ("has_soap", C.c_int32), ("dim", C.c_int3
                    ("Qs", C.POINTER(C.c_double)), ("alphas", C.POINTER(C.c_double)),
                    ("delta", C.c_double), ("zeta", C.c_double),
                    ("e0", C.c_double * 8), ("b2", C.POINTER(C2b)),
-                   ("n_more", C.c_int32), ("pad_", C.c_int32), ("more", C.POINTER(CModel))]
+                   ("n_more", C.c_int32), ("pad_", C.c_int32), ("more", C.POINTER(CModel)),
+                   ("n3b", C.c_int32), ("ncore", C.c_int32), ("b3", C.POINTER(C3b)),
+                   ("core", C.POINTER(CCore))]
 
 
 def build():
@@ -269,7 +285,53 @@ def _cmodel(model: GapModel):
     keep.append(marr)
     cm.n_more = len(more)
     cm.more = C.cast(marr, C.POINTER(CModel))
+    a3 = list(getattr(model, "angle_3b", []) or [])
+    arr3 = (C3b * max(1, len(a3)))()
+    for k, b in enumerate(a3):
+        q = _f64(np.asarray(b.Qs, float).T.copy())  # Qs(n_sparse, 3) column-major
+        a, c = _f64(b.alphas), _f64(b.cutoff)
+        keep += [q, a, c]
+        arr3[k].sp0, arr3[k].sp1, arr3[k].sp2 = b.sp0, b.sp1, b.sp2
+        arr3[k].n_sparse, arr3[k].kernel = len(a), {"exp": 0, "pol": 1}[b.kernel]
+        arr3[k].rcut, arr3[k].delta = b.rcut, b.delta
+        for t in range(3):
+            arr3[k].sigma[t] = float(b.sigma[t])
+        arr3[k].Qs, arr3[k].alphas, arr3[k].cutoff = _d(q), _d(a), _d(c)
+    keep.append(arr3)
+    cm.n3b = len(a3)
+    cm.b3 = C.cast(arr3, C.POINTER(C3b))
+    cp = list(getattr(model, "core_pot", []) or [])
+    arrc = (CCore * max(1, len(cp)))()
+    for k, b in enumerate(cp):
+        x, v, d2, y1, yn = core_pot_setup(b.x, b.V, b.yp1, b.ypn)
+        keep += [x, v, d2]
+        arrc[k].sp1, arrc[k].sp2, arrc[k].n = b.sp1, b.sp2, len(x)
+        arrc[k].yp1, arrc[k].ypn = y1, yn
+        arrc[k].x, arrc[k].V, arrc[k].d2 = _d(x), _d(v), _d(d2)
+    keep.append(arrc)
+    cm.ncore = len(cp)
+    cm.core = C.cast(arrc, C.POINTER(CCore))
     return cm, keep
+
+
+def core_pot_setup(x, V, yp1, ypn):
+    """read_gap.f90:701-752: spline second derivatives and end slopes."""
+    x, V = _f64(x), _f64(V)
+    d2 = np.zeros(len(x))
+    y1, yn = C.c_double(yp1), C.c_double(ypn)
+    lib().orc_core_pot_setup(len(x), _d(x), _d(V), C.byref(y1), C.byref(yn), _d(d2))
+    return x, V, d2, y1.value, yn.value
+
+
+def spline(x, V, d2, yp1, ypn, r, rcut, derivative=False):
+    """spline / spline_der (utils/splines.f90:33-156) at each r."""
+    fn = lib().orc_spline
+    fn.restype = C.c_double
+    fn.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                   C.c_double, C.c_double, C.c_int]
+    x, V, d2 = _f64(x), _f64(V), _f64(d2)
+    return np.array([fn(len(x), _d(x), _d(V), _d(d2), yp1, ypn, float(t), rcut, int(derivative))
+                     for t in np.atleast_1d(r)])
 
 
 def gap_2b(b, pos, species, lattice, do_forces=True):

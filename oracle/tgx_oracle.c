@@ -1294,6 +1294,266 @@ void orc_gap_2b_ef(const orc_2b* b, int n, const int* species,
 }
 
 /* ------------------------------------------------------------------------ */
+/* core_pot and angle_3b terms                                              */
+/* ------------------------------------------------------------------------ */
+
+/* read_gap.f90:701-752: the tridiagonal sweep for the spline's second
+ * derivatives (natural end where the slope exceeds 0.99e30, else clamped),
+ * then natural-end slopes replaced by spline_der at the end points.  At
+ * x(n) the reference's interval search runs past the table; the last
+ * interval is used here. */
+void orc_core_pot_setup(int n, const double* x, const double* V, double* yp1, double* ypn,
+                        double* d2) {
+  double* u = (double*)malloc(sizeof(double) * (size_t)n);
+  if (*yp1 > 0.99e30) {
+    d2[0] = 0.0;
+    u[0] = 0.0;
+  } else {
+    d2[0] = -0.5;
+    u[0] = (3.0 / (x[1] - x[0])) * ((V[1] - V[0]) / (x[1] - x[0]) - *yp1);
+  }
+  for (int i = 1; i < n - 1; i++) {
+    double sig = (x[i] - x[i - 1]) / (x[i + 1] - x[i - 1]);
+    double p = sig * d2[i - 1] + 2.0;
+    d2[i] = (sig - 1.0) / p;
+    u[i] = (V[i + 1] - V[i]) / (x[i + 1] - x[i]) - (V[i] - V[i - 1]) / (x[i] - x[i - 1]);
+    u[i] = (6.0 * u[i] / (x[i + 1] - x[i - 1]) - sig * u[i - 1]) / p;
+  }
+  double qn, un;
+  if (*ypn > 0.99e30) {
+    qn = 0.0;
+    un = 0.0;
+  } else {
+    qn = 0.5;
+    un = (3.0 / (x[n - 1] - x[n - 2])) * (*ypn - (V[n - 1] - V[n - 2]) / (x[n - 1] - x[n - 2]));
+  }
+  d2[n - 1] = (un - qn * u[n - 2]) / (qn * d2[n - 2] + 1.0);
+  for (int i = n - 2; i >= 0; i--) d2[i] = d2[i] * d2[i + 1] + u[i];
+  if (*yp1 > 0.99e30) *yp1 = orc_spline(n, x, V, d2, *yp1, *ypn, x[0], 1e100, 1);
+  if (*ypn > 0.99e30) *ypn = orc_spline(n, x, V, d2, *yp1, *ypn, x[n - 1], 1e100, 1);
+  free(u);
+}
+
+/* splines.f90:33-156: linear continuation outside [x(1), x(n)], else the
+ * cubic on the interval with x(j) <= r < x(j+1) (the first j whose x(j+1)
+ * exceeds r; r = x(n) takes the last interval). */
+double orc_spline(int n, const double* x, const double* y, const double* y2, double yp1,
+                  double ypn, double r, double rcut, int derivative) {
+  if (!(r < rcut)) return 0.0;
+  if (r < x[0]) return derivative ? yp1 : y[0] + (r - x[0]) * yp1;
+  if (r > x[n - 1]) return derivative ? ypn : y[n - 1] + (r - x[n - 1]) * ypn;
+  int j = 0;
+  while (j < n - 2 && !(r < x[j + 1])) j++;
+  double h = x[j + 1] - x[j];
+  double A = (x[j + 1] - r) / h;
+  double B = 1.0 - A;
+  if (!derivative) {
+    double h26 = h * h / 6.0;
+    double C = (A * A * A - A) * h26;
+    double D = (B * B * B - B) * h26;
+    return A * y[j] + B * y[j + 1] + C * y2[j] + D * y2[j + 1];
+  }
+  double h6 = h / 6.0;
+  double dA = -1.0 / h, dB = -dA;
+  double dC = (1.0 - 3.0 * A * A) * h6;
+  double dD = (3.0 * B * B - 1.0) * h6;
+  return dA * y[j] + dB * y[j + 1] + dC * y2[j] + dD * y2[j + 1];
+}
+
+/* gap.f90:552-686: for every pair of the right species within max(x),
+ * E_i += V(r)/2; F_i += V'(r) xyz/r; virial -= sym(F x xyz)/2, then x 1/2. */
+void orc_core_pot_ef(const orc_core* b, int n, const int* species, const int* n_neigh,
+                     const int* list, const double* rjs, const double* xyz, int do_forces,
+                     double* energies, double* forces, double* virial) {
+  double rcut = b->x[0];
+  for (int a = 1; a < b->n; a++) rcut = b->x[a] > rcut ? b->x[a] : rcut;
+  int sp1 = b->sp1, sp2 = b->sp2;
+  double vir[9] = {0};
+  int64_t k = 0;
+  for (int i = 0; i < n; i++) {
+    energies[i] = 0.0;
+    if (do_forces) forces[3 * i] = forces[3 * i + 1] = forces[3 * i + 2] = 0.0;
+  }
+  for (int i = 0; i < n; i++) {
+    if (species[i] != sp1 && species[i] != sp2) {
+      k += n_neigh[i];
+      continue;
+    }
+    k++; /* self */
+    for (int j = 1; j < n_neigh[i]; j++, k++) {
+      int sj = species[(list[k] - 1) % n];
+      if (!((species[i] == sp1 && sj == sp2) || (species[i] == sp2 && sj == sp1))) continue;
+      double r = rjs[k];
+      if (!(r < rcut)) continue;
+      energies[i] += 0.5 * orc_spline(b->n, b->x, b->V, b->d2, b->yp1, b->ypn, r, rcut, 0);
+      if (do_forces) {
+        double dV = orc_spline(b->n, b->x, b->V, b->d2, b->yp1, b->ypn, r, rcut, 1);
+        double f[3];
+        for (int a = 0; a < 3; a++) f[a] = dV * xyz[3 * k + a] / r;
+        for (int a = 0; a < 3; a++) forces[3 * i + a] += f[a];
+        for (int k1 = 0; k1 < 3; k1++)
+          for (int k2 = 0; k2 < 3; k2++)
+            vir[k2 * 3 + k1] -= 0.5 * (f[k1] * xyz[3 * k + k2] + f[k2] * xyz[3 * k + k1]);
+      }
+    }
+  }
+  if (do_forces)
+    for (int a = 0; a < 9; a++) virial[a] = 0.5 * vir[a];
+}
+
+/* cov_pp / cov_pp_der with d = 3, q = 1 (gap.f90:1022-1081): j = 3,
+ * (1 - r)^4 (4 r + 1) on (0, 1), 1 below, 0 above; the derivative holds the
+ * 1/r factor. */
+static double cov_pp31(double r) {
+  if (r >= 1.0) return 0.0;
+  if (r <= 0.0) return 1.0;
+  return pow(1.0 - r, 4) * (4.0 * r + 1.0);
+}
+static double cov_pp31_der(double r) {
+  if (r >= 1.0 || r <= 0.0) return 0.0;
+  return (-4.0 * pow(1.0 - r, 3) * (4.0 * r + 1.0) + pow(1.0 - r, 4) * 4.0) / r;
+}
+
+/* gap.f90:756-1015 with buffer 0.5 and e0 0 (the call at :730-751): for a
+ * centre of species sp0 and every pair of its neighbours (list order) of
+ * species (sp1, sp2) in either order within rcut, q = (r12 + r13,
+ * (r12 - r13)^2, r23), kernel pref_s k(|q - Q_s|_sigma), pref = 2 delta^2
+ * alpha c, both cosine cutoffs; forces on the three atoms and the virial
+ * sym(F2 x12 + F3 x13)/2. */
+void orc_angle_3b_ef(const orc_3b* b, int n, const int* species, const int* n_neigh,
+                     const int* list, const double* rjs, const double* xyz, int do_forces,
+                     double* energies, double* forces, double* virial) {
+  const double pi = acos(-1.0), buffer = 0.5, rcut = b->rcut;
+  const int S = b->n_sparse;
+  const double* Q1 = b->Qs;
+  const double* Q2 = b->Qs + S;
+  const double* Q3 = b->Qs + 2 * S;
+  double* pref = (double*)malloc(sizeof(double) * (size_t)S);
+  double* kern = (double*)malloc(sizeof(double) * (size_t)S);
+  double* kder = (double*)malloc(sizeof(double) * (size_t)S);
+  double* rr = (double*)malloc(sizeof(double) * (size_t)S);
+  double* dq = (double*)malloc(sizeof(double) * 3 * (size_t)S);
+  for (int s = 0; s < S; s++) pref[s] = 2.0 * b->delta * b->delta * b->alphas[s] * b->cutoff[s];
+  for (int i = 0; i < n; i++) energies[i] = 0.0;
+  if (do_forces) {
+    memset(forces, 0, sizeof(double) * 3 * (size_t)n);
+    memset(virial, 0, sizeof(double) * 9);
+  }
+  int64_t k = 0;
+  for (int i = 0; i < n; i++) {
+    if (species[i] != b->sp0) {
+      k += n_neigh[i];
+      continue;
+    }
+    int i3 = (list[k] - 1) % n;
+    k++;
+    for (int j = 1; j < n_neigh[i]; j++) {
+      int64_t kj = k + j - 1;
+      double r12 = rjs[kj];
+      int sj = species[(list[kj] - 1) % n];
+      if ((sj != b->sp1 && sj != b->sp2) || r12 > rcut) continue;
+      int j3 = (list[kj] - 1) % n;
+      const double* x12 = xyz + 3 * kj;
+      double u12[3] = {x12[0] / r12, x12[1] / r12, x12[2] / r12};
+      for (int j2 = j + 1; j2 < n_neigh[i]; j2++) {
+        int64_t kk = k + j2 - 1;
+        double r13 = rjs[kk];
+        int sk = species[(list[kk] - 1) % n];
+        if (!(((sj == b->sp1 && sk == b->sp2) || (sj == b->sp2 && sk == b->sp1)) && r13 < rcut))
+          continue;
+        int k3 = (list[kk] - 1) % n;
+        const double* x13 = xyz + 3 * kk;
+        double u13[3] = {x13[0] / r13, x13[1] / r13, x13[2] / r13};
+        double x23[3] = {x13[0] - x12[0], x13[1] - x12[1], x13[2] - x12[2]};
+        double r23 = sqrt(x23[0] * x23[0] + x23[1] * x23[1] + x23[2] * x23[2]);
+        double u23[3] = {x23[0] / r23, x23[1] / r23, x23[2] / r23};
+        if (!(r12 < rcut && r13 < rcut)) continue;
+        double fc12, fc13, df12[3], df13[3];
+        if (r12 < rcut - buffer) {
+          fc12 = 1.0;
+          df12[0] = df12[1] = df12[2] = 0.0;
+        } else {
+          fc12 = (cos(pi * (r12 - rcut + buffer) / buffer) + 1.0) / 2.0;
+          for (int a = 0; a < 3; a++)
+            df12[a] = sin(pi * (r12 - rcut + buffer) / buffer) / 2.0 * pi / buffer * u12[a];
+        }
+        if (r13 < rcut - buffer) {
+          fc13 = 1.0;
+          df13[0] = df13[1] = df13[2] = 0.0;
+        } else {
+          fc13 = (cos(pi * (r13 - rcut + buffer) / buffer) + 1.0) / 2.0;
+          for (int a = 0; a < 3; a++)
+            df13[a] = sin(pi * (r13 - rcut + buffer) / buffer) / 2.0 * pi / buffer * u13[a];
+        }
+        double fcut = fc12 * fc13, dfc[3];
+        for (int a = 0; a < 3; a++) dfc[a] = fc12 * df13[a] + fc13 * df12[a];
+        double q[3] = {r12 + r13, (r12 - r13) * (r12 - r13), r23};
+        for (int s = 0; s < S; s++) {
+          double d1 = q[0] - Q1[s], d2 = q[1] - Q2[s], d3 = q[2] - Q3[s];
+          double t = d1 * d1 / (b->sigma[0] * b->sigma[0]);
+          t = t + d2 * d2 / (b->sigma[1] * b->sigma[1]);
+          t = t + d3 * d3 / (b->sigma[2] * b->sigma[2]);
+          rr[s] = sqrt(t);
+          dq[3 * s] = d1 / (b->sigma[0] * b->sigma[0]);
+          dq[3 * s + 1] = d2 / (b->sigma[1] * b->sigma[1]);
+          dq[3 * s + 2] = d3 / (b->sigma[2] * b->sigma[2]);
+          if (b->kernel == 0) {
+            kern[s] = pref[s] * exp(-0.5 * rr[s] * rr[s]);
+          } else {
+            kern[s] = pref[s] * cov_pp31(rr[s]);
+            kder[s] = pref[s] * cov_pp31_der(rr[s]);
+          }
+        }
+        double ks = 0.0;
+        for (int s = 0; s < S; s++) ks += kern[s];
+        energies[i] += fcut * ks;
+        if (!do_forces) continue;
+        double F1[3], F2[3], F3[3];
+        for (int l = 0; l < 3; l++) {
+          double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+          for (int s = 0; s < S; s++) {
+            double dx1 = dq[3 * s] * (u12[l] + u13[l]) +
+                         dq[3 * s + 1] * 2.0 * (r12 - r13) * (u12[l] - u13[l]);
+            double dx2 = -dq[3 * s] * u12[l] - dq[3 * s + 1] * 2.0 * (r12 - r13) * u12[l] +
+                         dq[3 * s + 2] * u23[l];
+            double dx3 = -dq[3 * s] * u13[l] - dq[3 * s + 1] * 2.0 * (r13 - r12) * u13[l] -
+                         dq[3 * s + 2] * u23[l];
+            if (b->kernel == 0) {
+              s1 += kern[s] * (dfc[l] + fcut * dx1);
+              s2 += kern[s] * (-fc13 * df12[l] + fcut * dx2);
+              s3 += kern[s] * (-fc12 * df13[l] + fcut * dx3);
+            } else {
+              s1 += kern[s] * dfc[l] - kder[s] * fcut * dx1;
+              s2 += -kern[s] * fc13 * df12[l] - kder[s] * fcut * dx2;
+              s3 += -kern[s] * fc12 * df13[l] - kder[s] * fcut * dx3;
+            }
+          }
+          F1[l] = -s1;
+          F2[l] = -s2;
+          F3[l] = -s3;
+        }
+        for (int a = 0; a < 3; a++) {
+          forces[3 * i3 + a] += F1[a];
+          forces[3 * j3 + a] += F2[a];
+          forces[3 * k3 + a] += F3[a];
+        }
+        for (int k1 = 0; k1 < 3; k1++)
+          for (int k4 = 0; k4 < 3; k4++) {
+            virial[k4 * 3 + k1] += 0.5 * (F2[k1] * x12[k4] + F2[k4] * x12[k1]);
+            virial[k4 * 3 + k1] += 0.5 * (F3[k1] * x13[k4] + F3[k4] * x13[k1]);
+          }
+      }
+    }
+    k += n_neigh[i] - 1;
+  }
+  free(pref);
+  free(kern);
+  free(kder);
+  free(rr);
+  free(dq);
+}
+
+/* ------------------------------------------------------------------------ */
 /* Whole path                                                               */
 /* ------------------------------------------------------------------------ */
 
@@ -1348,6 +1608,9 @@ int orc_energy_forces(const orc_model* m, int n, const double* pos,
     rc = r > rc ? r : rc;
   }
   for (int b = 0; b < m->n2b; b++) rc = m->b2[b].rcut > rc ? m->b2[b].rcut : rc;
+  for (int b = 0; b < m->n3b; b++) rc = m->b3[b].rcut > rc ? m->b3[b].rcut : rc;
+  for (int b = 0; b < m->ncore; b++)
+    for (int a = 0; a < m->core[b].n; a++) rc = m->core[b].x[a] > rc ? m->core[b].x[a] : rc;
   int64_t np = 0;
   int r0 = orc_neighbors(n, pos, lat, rc, 0, 0, NULL, NULL, NULL, NULL, NULL, NULL, &np, NULL);
   if (r0 != TGX_OK && r0 != TGX_ERR_CAPACITY) return r0;
@@ -1375,6 +1638,19 @@ int orc_energy_forces(const orc_model* m, int n, const double* pos,
   double* te = (double*)malloc(sizeof(double) * (size_t)n);
   for (int b = 0; b < m->n2b; b++) {
     orc_gap_2b_ef(&m->b2[b], n, species, nn, list, rjs, xyz, do_forces, te, tf, tv);
+    for (int i = 0; i < n; i++) e_atom[i] += te[i];
+    if (do_forces) {
+      for (size_t a = 0; a < (size_t)3 * n; a++) forces[a] += tf[a];
+      for (int a = 0; a < 9; a++) virial[a] += tv[a];
+    }
+  }
+  /* angle_3b then core_pot, as turbogap_main.f90:809-840 calls them */
+  for (int b = 0; b < m->n3b + m->ncore; b++) {
+    if (b < m->n3b)
+      orc_angle_3b_ef(&m->b3[b], n, species, nn, list, rjs, xyz, do_forces, te, tf, tv);
+    else
+      orc_core_pot_ef(&m->core[b - m->n3b], n, species, nn, list, rjs, xyz, do_forces, te, tf,
+                      tv);
     for (int i = 0; i < n; i++) e_atom[i] += te[i];
     if (do_forces) {
       for (size_t a = 0; a < (size_t)3 * n; a++) forces[a] += tf[a];

@@ -29,6 +29,31 @@ typedef struct {
   const double* cutoff;
 } orc_2b;
 
+/* One angle_3b block (gap_3b_t; read_gap.f90:599-643, sparse set
+ * :850-872).  Qs is n_sparse x 3 column-major as the reference stores it;
+ * kernel 0 = "exp", 1 = "pol" (cov_pp with d = 3, q = 1). */
+typedef struct {
+  int32_t sp0, sp1, sp2; /* 1-based: centre and the two neighbour species */
+  int32_t n_sparse;
+  int32_t kernel;
+  int32_t pad_;
+  double rcut, delta;
+  double sigma[3];
+  const double* Qs;
+  const double* alphas;
+  const double* cutoff;
+} orc_3b;
+
+/* One core_pot block (gap_core_pot_t): tabulated V(x) with its spline
+ * second derivatives d2 and end slopes as orc_core_pot_setup leaves them. */
+typedef struct {
+  int32_t sp1, sp2, n, pad_;
+  double yp1, ypn;
+  const double* x;
+  const double* V;
+  const double* d2;
+} orc_core;
+
 /* Whole model: a soap_turbo block, optional further soap_turbo blocks
  * (more[0..n_more-1], of which only the SOAP fields are read), optional 2b
  * blocks, e0 per species. */
@@ -45,6 +70,9 @@ typedef struct orc_model {
   int32_t n_more;
   int32_t pad_;
   const struct orc_model* more;
+  int32_t n3b, ncore;
+  const orc_3b* b3;
+  const orc_core* core;
 } orc_model;
 
 /* get_distance, orthorhombic branch (neighbors_interface.f90:547-581). */
@@ -119,6 +147,23 @@ void orc_gap_2b_ef(const orc_2b* b, int n, const int* species,
 
 /* Whole path for one structure: neighbours, SOAP term (sites processed in
  * chunks of at most chunk_sites, 0 = all), 2b terms, e0. */
+/* Spline second derivatives of a core_pot table and the end slopes
+ * (read_gap.f90:701-752): a slope above 0.99e30 means a natural end and is
+ * then replaced by the spline's own derivative there. */
+void orc_core_pot_setup(int n, const double* x, const double* V, double* yp1, double* ypn,
+                        double* d2);
+/* spline / spline_der (utils/splines.f90:33-156) at r < rcut, else 0. */
+double orc_spline(int n, const double* x, const double* y, const double* y2, double yp1,
+                  double ypn, double r, double rcut, int derivative);
+/* get_core_pot_energy_and_forces (gap.f90:552-686) on a full list. */
+void orc_core_pot_ef(const orc_core* b, int n, const int* species, const int* n_neigh,
+                     const int* list, const double* rjs, const double* xyz, int do_forces,
+                     double* energies, double* forces, double* virial);
+/* get_3b_energy_and_forces (gap.f90:756-1015) on a full list. */
+void orc_angle_3b_ef(const orc_3b* b, int n, const int* species, const int* n_neigh,
+                     const int* list, const double* rjs, const double* xyz, int do_forces,
+                     double* energies, double* forces, double* virial);
+
 int orc_energy_forces(const orc_model* m, int n, const double* pos,
                       const int* species, const double* lat, int do_forces,
                       int chunk_sites, double* e_atom, double* forces,

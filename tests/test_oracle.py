@@ -1008,3 +1008,166 @@ def test_general_cell_energy_forces_same_crystal(rot):
     assert np.abs(e1 - e2).max() < 1e-10
     assert np.abs(f1 @ A.T - f2).max() < 1e-9
     assert np.abs(A @ v1 @ A.T - v2).max() < 1e-8
+
+
+# --------------------------------------------------------------------------
+# core_pot and angle_3b terms
+# --------------------------------------------------------------------------
+def _core_table(n=120, lo=0.3, hi=2.1):
+    """A screened-Coulomb-like repulsion tabulated on a non-uniform grid."""
+    x = lo + (hi - lo) * np.linspace(0, 1, n) ** 1.3
+    V = 30.0 * np.exp(-x / 0.35) / x
+    dV0 = -30.0 * np.exp(-lo / 0.35) * (1 / (0.35 * lo) + 1 / lo ** 2)
+    return x, V, dV0
+
+
+@pytest.mark.parametrize("ends", ["clamped", "natural", "mixed"])
+def test_core_pot_spline_vs_scipy(ends):
+    """orc_core_pot_setup + spline/spline_der equal scipy's cubic spline with
+    the same end conditions; below x(1) the continuation is linear with
+    slope yp1; natural ends get the spline's own end slope."""
+    from scipy.interpolate import CubicSpline
+    x, V, dV0 = _core_table()
+    yp1 = dV0 if ends != "natural" else 1e31
+    ypn = 0.0 if ends == "clamped" else 1e31
+    xs, Vs, d2, y1, yn = oracle.core_pot_setup(x, V, yp1, ypn)
+    bc = ((1, yp1) if yp1 < 1e30 else (2, 0.0), (1, ypn) if ypn < 1e30 else (2, 0.0))
+    cs = CubicSpline(x, V, bc_type=bc)
+    assert np.abs(d2 - cs(x, 2)).max() < 1e-7 * np.abs(d2).max()
+    r = np.random.default_rng(0).uniform(x[0], x[-1], 200)
+    s = oracle.spline(x, V, d2, y1, yn, r, x[-1] + 1.0)
+    ds = oracle.spline(x, V, d2, y1, yn, r, x[-1] + 1.0, derivative=True)
+    assert np.abs(s - cs(r)).max() < 1e-10 * np.abs(V).max()
+    assert np.abs(ds - cs(r, 1)).max() < 1e-8 * np.abs(cs(r, 1)).max()
+    assert abs(y1 - cs(x[0], 1)) < 1e-8 * abs(y1) and abs(yn - cs(x[-1], 1)) < 1e-8 * max(1, abs(yn))
+    lo = x[0] - np.array([0.05, 0.1])
+    assert np.allclose(oracle.spline(x, V, d2, y1, yn, lo, 5.0), V[0] + (lo - x[0]) * y1,
+                       rtol=1e-14)
+    assert np.all(oracle.spline(x, V, d2, y1, yn, [x[-1], x[-1] + 0.1], x[-1]) == 0.0)
+
+
+def _terms_model(kernel="exp", sp=(1, 1, 1), seed=3):
+    """A model of angle_3b + core_pot (no SOAP) for carbon cells: sparse
+    points spread over the descriptor range of neighbour pairs within
+    2.6 A."""
+    rng = np.random.default_rng(seed)
+    S3 = 40
+    q1 = rng.uniform(2.4, 5.0, S3)
+    q2 = rng.uniform(0.0, 1.0, S3)
+    q3 = rng.uniform(1.2, 4.5, S3)
+    sig = (0.5, 0.4, 0.5) if kernel == "exp" else (2.0, 1.2, 2.0)
+    a3 = tgx.Angle3b(sp0=sp[0], sp1=sp[1], sp2=sp[2], rcut=2.6, delta=0.8, sigma=sig,
+                     Qs=np.stack([q1, q2, q3], 1), alphas=rng.normal(0, 0.05, S3),
+                     cutoff=rng.uniform(0.5, 1.0, S3), kernel=kernel)
+    x, V, dV0 = _core_table()
+    cp = tgx.CorePot(sp1=1, sp2=1, x=x, V=V, yp1=dV0, ypn=1e31)
+    h = S.carbon_hypers()
+    return tgx.GapModel(hypers=h, Qs=np.zeros((1, h.dim)), alphas=np.zeros(1),
+                        angle_3b=[a3], core_pot=[cp])
+
+
+def _ef_terms(model, pos, sp, lat, do_forces=True):
+    return oracle.energy_forces(model, pos, sp, lat, do_forces=do_forces, soap=False)[:3]
+
+
+def test_core_pot_dimer_closed_form():
+    x, V, dV0 = _core_table()
+    m = _terms_model()
+    m.angle_3b = []
+    lat = S.cubic_lattice(30.0)
+    for r in (0.2, 0.75, 1.33, 2.0, 2.2):
+        pos = np.array([[5.0, 5.0, 5.0], [5.0 + r * 0.6, 5.0 + r * 0.8, 5.0]])
+        e, f, v = _ef_terms(m, pos, np.ones(2, np.int32), lat)
+        xs, Vs, d2, y1, yn = oracle.core_pot_setup(x, V, dV0, 1e31)
+        Vr = oracle.spline(x, V, d2, y1, yn, [r], x[-1])[0]
+        dVr = oracle.spline(x, V, d2, y1, yn, [r], x[-1], derivative=True)[0]
+        assert np.allclose(e, [0.5 * Vr, 0.5 * Vr], rtol=1e-14, atol=0)
+        u = np.array([0.6, 0.8, 0.0])
+        assert np.abs(f[0] - dVr * u).max() <= 1e-12 * max(1, abs(dVr))
+        assert np.abs(f[0] + f[1]).max() <= 1e-12 * max(1, abs(dVr))
+        # pair virial: -r (x) F_j summed = -dV/dr r u u^T
+        assert np.abs(v - (-dVr * r * np.outer(u, u))).max() <= 1e-12 * max(1, abs(dVr * r))
+
+
+def _trimer_energy(a3, p0, p1, p2):
+    """Independent numpy form of one angle_3b centre with two neighbours."""
+    x12, x13 = p1 - p0, p2 - p0
+    r12, r13, r23 = np.linalg.norm(x12), np.linalg.norm(x13), np.linalg.norm(p2 - p1)
+    if not (r12 < a3.rcut and r13 < a3.rcut):
+        return 0.0
+    fc = 1.0
+    for r in (r12, r13):
+        if r > a3.rcut - 0.5:
+            fc *= 0.5 * (1 + math.cos(math.pi * (r - a3.rcut + 0.5) / 0.5))
+    q = np.array([r12 + r13, (r12 - r13) ** 2, r23])
+    d = np.sqrt((((q - a3.Qs) / np.asarray(a3.sigma)) ** 2).sum(1))
+    if a3.kernel == "exp":
+        k = np.exp(-0.5 * d * d)
+    else:
+        k = np.where(d < 1, (1 - np.minimum(d, 1)) ** 4 * (4 * d + 1), 0.0)
+    return fc * np.sum(2 * a3.delta ** 2 * a3.alphas * a3.cutoff * k)
+
+
+@pytest.mark.parametrize("kernel", ["exp", "pol"])
+def test_angle_3b_trimer_closed_form(kernel):
+    m = _terms_model(kernel)
+    m.core_pot = []
+    a3 = m.angle_3b[0]
+    lat = S.cubic_lattice(30.0)
+    rng = np.random.default_rng(8)
+    for _ in range(6):
+        p = np.array([10.0, 10.0, 10.0]) + rng.uniform(-1.3, 1.3, (3, 3))
+        e, f, v = _ef_terms(m, p, np.ones(3, np.int32), lat)
+        ref = [_trimer_energy(a3, p[0], p[1], p[2]), _trimer_energy(a3, p[1], p[0], p[2]),
+               _trimer_energy(a3, p[2], p[0], p[1])]
+        assert np.allclose(e, ref, rtol=1e-12, atol=1e-15)
+        assert np.abs(f.sum(0)).max() < 1e-12
+
+
+@pytest.mark.parametrize("kernel", ["exp", "pol"])
+def test_angle_3b_core_pot_forces_and_virial_fd(kernel):
+    """Forces = -dE/dr and virial = -dE/d(strain), all nine components (the
+    strained cells are triclinic), for angle_3b + core_pot together."""
+    m = _terms_model(kernel)
+    pos, sp, lat = S.carbon_cell(48, 2.2, 1.05, 41)
+    e, f, v = _ef_terms(m, pos, sp, lat)
+    assert abs(e.sum()) > 1e-3
+    h = 1e-5
+    rng = np.random.default_rng(1)
+    for i in rng.choice(len(pos), 6, replace=False):
+        for a in range(3):
+            Es = []
+            for s in (1, -1):
+                p = pos.copy()
+                p[i, a] += s * h
+                Es.append(_ef_terms(m, p, sp, lat, False)[0].sum())
+            fd = -(Es[0] - Es[1]) / (2 * h)
+            assert abs(fd - f[i, a]) < 1e-6 * max(1.0, abs(f[i, a])), (i, a, fd, f[i, a])
+    for a in range(3):
+        for b in range(3):
+            Es = []
+            for s in (1, -1):
+                eps = np.zeros((3, 3))
+                eps[a, b] = s * h
+                p, l = _strained(pos, lat, eps)
+                Es.append(_ef_terms(m, p, sp, l, False)[0].sum())
+            fd = -(Es[0] - Es[1]) / (2 * h)
+            assert abs(fd - v[a, b]) < 1e-6 * max(1.0, abs(v[a, b])), (a, b, fd, v[a, b])
+
+
+def test_angle_3b_species_filter_and_invariance():
+    """Centre / neighbour species filters (a C-centred C-H block on a C/H
+    cell) and rotation + permutation invariance."""
+    pos, sp, lat = S.cho_cell(80, 1.5, 1.0, 5)
+    sp = np.where(sp == 3, 1, sp).astype(np.int32)  # C/H only
+    m = _terms_model("exp", sp=(1, 1, 2))
+    m.core_pot = []
+    e, f, v = _ef_terms(m, pos, sp, lat)
+    assert np.all(e[sp != 1] == 0.0) and np.abs(e).max() > 0
+    rng = np.random.default_rng(3)
+    A = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+    perm = rng.permutation(len(pos))
+    e2, f2, v2 = _ef_terms(m, pos[perm] @ A.T, sp[perm], (lat.reshape(3, 3) @ A.T).reshape(9))
+    assert np.abs(e2 - e[perm]).max() < 1e-12
+    assert np.abs(f2 - f[perm] @ A.T).max() < 1e-11
+    assert np.abs(v2 - A @ v @ A.T).max() < 1e-10

@@ -178,6 +178,38 @@ class TwoBody:
 
 
 @dataclass
+class Angle3b:
+    """One angle_3b block (gap_3b_t; read_gap.f90:599-643): centre species
+    sp0, neighbour species sp1/sp2 (1-based), sparse descriptors Qs (S, 3)
+    of q = (r12 + r13, (r12 - r13)^2, r23), kernel "exp" or "pol"."""
+
+    sp0: int
+    sp1: int
+    sp2: int
+    rcut: float
+    delta: float
+    sigma: Sequence[float]
+    Qs: np.ndarray
+    alphas: np.ndarray
+    cutoff: np.ndarray
+    kernel: str = "exp"
+
+
+@dataclass
+class CorePot:
+    """One core_pot block (gap_core_pot_t): tabulated V(x), ascending x,
+    end slopes yp1/ypn (above 0.99e30: natural end; read_gap.f90:644-752).
+    The table is taken as given (after any core_pot_cutoff taper)."""
+
+    sp1: int
+    sp2: int
+    x: np.ndarray
+    V: np.ndarray
+    yp1: float = 1e31
+    ypn: float = 1e31
+
+
+@dataclass
 class GapModel:
     """A soap_turbo GAP block plus optional 2b blocks and e0 per species."""
 
@@ -191,6 +223,8 @@ class GapModel:
     # further soap_turbo blocks (GapModel objects whose hypers, Qs, alphas,
     # delta and zeta are used), evaluated after this one and summed
     more_soap: list = field(default_factory=list)
+    angle_3b: list = field(default_factory=list)
+    core_pot: list = field(default_factory=list)
 
     @property
     def rcut_max(self) -> float:
@@ -199,6 +233,10 @@ class GapModel:
             r = max(r, b.hypers.rcut_max)
         for b in self.two_body:
             r = max(r, b.rcut)
+        for b in self.angle_3b:
+            r = max(r, b.rcut)
+        for b in self.core_pot:
+            r = max(r, float(np.max(b.x)))
         return r
 
 
@@ -242,6 +280,9 @@ def load_library(path: str = LIB_PATH):
         "tgx_n_soap_models": (i, [P]),
         "tgx_add_2b_model": (i, [P, i, i, d, d, d, d, i, dp, dp, dp]),
         "tgx_clear_2b_models": (i, [P]),
+        "tgx_add_angle_3b": (i, [P, i, i, i, d, d, dp, i, i, dp, dp, dp]),
+        "tgx_add_core_pot": (i, [P, i, i, i, dp, dp, d, d]),
+        "tgx_clear_extra_terms": (i, [P]),
         "tgx_neighbors_build": (i, [P, i, dp, dp, d, i64, ip, ip, dp, dp, dp, dp,
                                     C.POINTER(i64), ip]),
         "tgx_soap_descriptors": (i, [P, i, dp, ip, dp, dp, ip, ip]),
@@ -362,6 +403,18 @@ class Engine:
             self._check(self.lib.tgx_add_2b_model(self.ctx, b.sp1, b.sp2, b.rcut, b.buffer,
                                                   b.delta, b.sigma, len(q), _dptr(q), _dptr(a),
                                                   _dptr(c)))
+        self._check(self.lib.tgx_clear_extra_terms(self.ctx))
+        for b in model.angle_3b:
+            q = _f64(np.asarray(b.Qs, float).T.copy())  # Qs(n_sparse, 3) column-major
+            a, c = _f64(b.alphas), _f64(b.cutoff)
+            sg = _f64(b.sigma)
+            self._check(self.lib.tgx_add_angle_3b(self.ctx, b.sp0, b.sp1, b.sp2, b.rcut, b.delta,
+                                                  _dptr(sg), {"exp": 0, "pol": 1}[b.kernel],
+                                                  len(a), _dptr(q), _dptr(a), _dptr(c)))
+        for b in model.core_pot:
+            x, v = _f64(b.x), _f64(b.V)
+            self._check(self.lib.tgx_add_core_pot(self.ctx, b.sp1, b.sp2, len(x), _dptr(x),
+                                                  _dptr(v), b.yp1, b.ypn))
         self.model = model
 
     # -- hot path ----------------------------------------------------------
