@@ -144,6 +144,26 @@ int tgx_add_2b_model(tgx_ctx* ctx, int sp1, int sp2, double rcut,
                      const double* cutoff);
 int tgx_clear_2b_models(tgx_ctx* ctx);
 
+/* One angle_3b block (gap_3b_t, types.f90; read_gap.f90:599-643, sparse set
+ * :850-872; evaluated as get_3b_energy_and_forces, gap.f90:756-1015, with
+ * buffer 0.5 as calculate_gap_3b passes it, :730-751).  sp_center, sp1, sp2
+ * 1-based; sigma[3]; kernel 0 = "exp", 1 = "pol"; Qs(n_sparse,3)
+ * column-major as read_alphas_and_descriptors stores it. */
+int tgx_add_angle_3b(tgx_ctx* ctx, int sp_center, int sp1, int sp2, double rcut,
+                     double delta, const double* sigma, int kernel, int n_sparse,
+                     const double* Qs, const double* alphas, const double* cutoff);
+
+/* One core_pot block (gap_core_pot_t; read_gap.f90:644-752; evaluated as
+ * get_core_pot_energy_and_forces, gap.f90:552-686): V tabulated at n >= 2
+ * strictly ascending x (after any core_pot_cutoff taper, which the reader
+ * applies), end slopes yp1/ypn; a slope above 0.99e30 makes that end of the
+ * cubic spline natural.  The cutoff is the largest x. */
+int tgx_add_core_pot(tgx_ctx* ctx, int sp1, int sp2, int n, const double* x,
+                     const double* V, double yp1, double ypn);
+
+/* Drops every angle_3b and core_pot block. */
+int tgx_clear_extra_terms(tgx_ctx* ctx);
+
 /* Hot path ----------------------------------------------------------------- */
 
 /*

@@ -108,6 +108,14 @@ hipError_t launch_image_atoms(const double* pos0, const int* sp0, int n, const I
 hipError_t launch_neighbors_image(const double* pos, int n, const ImgCell& g, double rcut,
                                   int maxnb, int* count, int* list, int* rev, int* flag,
                                   hipStream_t st);
+hipError_t launch_core_pot(const CorePotDev& b, const double* pos, const int* species, int n,
+                           const Geom& g, int maxnb, const int* count, const int* list,
+                           double* eatom, double* forces, double* vatom, int do_forces,
+                           hipStream_t st);
+hipError_t launch_angle_3b(const Angle3bDev& b, const double* pos, const int* species, int n,
+                           const Geom& g, int maxnb, const int* count, const int* list,
+                           double* eatom, double* vatom, double* g3, int do_forces,
+                           hipStream_t st);
 hipError_t launch_site_add(int n_sites, const int* site_atom, const double* esite,
                            const double* vsite, double* eatom, double* vatom, hipStream_t st);
 hipError_t launch_energy_init(int n, const int* species, const double* e0,
@@ -202,6 +210,7 @@ struct Extra {
   bool sg_norm_set = false;
   DevBuf twob_store;
   DevBuf g_pos, g_sp;  // the structure itself while its image atoms fill pos / species
+  DevBuf g3;           // angle_3b forces per list slot (double4)
   std::vector<int> species_host;
   // autotuner: on (-1 = not yet read from TGX_AUTOTUNE), cache file, the
   // decisions by shape key, and a copy of the atom energies (the adjoint
@@ -368,7 +377,40 @@ static double rcut_all(const tgx_ctx* c) {
   if (c->has_soap)
     for (const auto& b : c->blocks) r = std::max(r, b->sd.rcut_max);
   for (const auto& b : c->twob) r = std::max(r, b.rcut);
+  for (const auto& b : c->a3b) r = std::max(r, b.rcut);
+  for (const auto& b : c->corep) r = std::max(r, b.rcut);
   return r;
+}
+
+static bool has_model(const tgx_ctx* c) {
+  return c->has_soap || !c->twob.empty() || !c->a3b.empty() || !c->corep.empty();
+}
+
+// angle_3b and core_pot blocks in turn (turbogap_main.f90:809-840), after
+// the 2-body terms: energies, forces and per-atom virials added.
+static int launch_extra_terms(tgx_ctx* c, int n, const Geom& cell, int do_forces) {
+  Extra& x = X(c);
+  for (const auto& b : c->a3b) {
+    if (do_forces) {
+      if (grow(c, x.g3, 32 * (size_t)n * c->maxnb))
+        return grow_err(c, 32 * (size_t)n * c->maxnb, "angle_3b slots");
+    }
+    TGX_HIP(c, launch_angle_3b(b, c->pos.as<double>(), c->species.as<int>(), n, cell, c->maxnb,
+                               c->nb_count.as<int>(), c->nb_list.as<int>(), c->eatom.as<double>(),
+                               c->virial.as<double>(), do_forces ? x.g3.as<double>() : nullptr,
+                               do_forces, c->stream));
+    if (do_forces)
+      TGX_HIP(c, launch_force_gather<double>(n, c->maxnb, c->nb_count.as<int>(),
+                                             c->nb_list.as<int>(), c->nb_rev.as<int>(), nullptr,
+                                             1, x.g3.as<double>(), c->forces.as<double>(),
+                                             c->stream, 0));
+  }
+  for (const auto& b : c->corep)
+    TGX_HIP(c, launch_core_pot(b, c->pos.as<double>(), c->species.as<int>(), n, cell, c->maxnb,
+                               c->nb_count.as<int>(), c->nb_list.as<int>(), c->eatom.as<double>(),
+                               c->forces.as<double>(), c->virial.as<double>(), do_forces,
+                               c->stream));
+  return TGX_OK;
 }
 
 // Sites of every SOAP block: all atoms, or those of its central_species.
@@ -627,7 +669,7 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
 static int run_ef(tgx_ctx* c, int n, const double* lattice, const int* species_host,
                   int do_forces) {
   Extra& x = X(c);
-  if (!c->has_soap && c->twob.empty()) return set_err(c, TGX_ERR_STATE, "no model loaded");
+  if (!has_model(c)) return set_err(c, TGX_ERR_STATE, "no model loaded");
   Cell cell;
   std::string err;
   const double rc = rcut_all(c) + c->list_buffer;
@@ -664,7 +706,8 @@ static int run_ef(tgx_ctx* c, int n, const double* lattice, const int* species_h
 // pairs instead of two) when every atom is a SOAP site and every 2b block's
 // cutoff lies within the SOAP cutoff of both its species.
 static bool fuse_two_body(const tgx_ctx* c, int n, int n_sites, int do_forces) {
-  if (!do_forces || !c->has_soap || c->twob.empty() || n_sites != n || c->blocks.size() != 1)
+  if (!do_forces || !c->has_soap || c->twob.empty() || n_sites != n || c->blocks.size() != 1 ||
+      !c->a3b.empty() || !c->corep.empty())
     return false;
   for (const auto& b : c->twob) {
     if (b.sp1 > c->blk->sd.n_species || b.sp2 > c->blk->sd.n_species) return false;
@@ -791,6 +834,8 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
                                c->nb_count.as<int>(), c->nb_list.as<int>(), c->eatom.as<double>(),
                                c->forces.as<double>(), c->virial.as<double>(), do_forces,
                                prec == TGX_FP64, c->stream));
+  rcode = launch_extra_terms(c, n, cell, do_forces);
+  if (rcode) return rcode;
   ev_rec(c, 5);
   if (cell.many) {
     if (x.b_out.ensure(sizeof(double) * 10 * (size_t)n_struct))
@@ -924,6 +969,8 @@ static int ef_launch_multi(tgx_ctx* c, int n, const Geom& cell, int do_forces, b
                                c->nb_count.as<int>(), c->nb_list.as<int>(), c->eatom.as<double>(),
                                c->forces.as<double>(), c->virial.as<double>(), do_forces,
                                prec == TGX_FP64, c->stream));
+  rcode = launch_extra_terms(c, n, cell, do_forces);
+  if (rcode) return rcode;
   ev_rec(c, 5);
   if (cell.many) {
     if (x.b_out.ensure(sizeof(double) * 10 * (size_t)n_struct))
@@ -1130,13 +1177,14 @@ int tgx_destroy(tgx_ctx* c) {
                     &f->x.sg_W1t, &f->x.sg_W2t, &f->x.sg_part, &f->x.sg_z1, &f->x.sg_z2,
                     &f->x.sg_e, &f->x.sg_q, &f->x.sg_y, &f->x.sg_rows, &f->x.sg_norm,
                     &f->x.tune_save, &f->x.md.pref, &f->x.md.ke, &f->x.md.ctl, &f->x.md.ep,
-                    &f->x.md.ek, &f->x.md.vout, &f->x.md.kesum};
+                    &f->x.md.ek, &f->x.md.vout, &f->x.md.kesum, &f->x.g_pos, &f->x.g_sp, &f->x.g3};
   for (DevBuf* b : bufs) b->release();
   for (auto& b : c->twob) {
     if (b.Qs) (void)hipFree(b.Qs);
     if (b.alphas) (void)hipFree(b.alphas);
     if (b.tab) (void)hipFree(b.tab);
   }
+  (void)tgx_clear_extra_terms(c);
   if (f->x.ev_ready) {
     for (int k = 0; k < 8; k++) (void)hipEventDestroy(f->x.ev[k]);
     for (int k = 0; k < 8; k++) (void)hipEventDestroy(f->x.kev[k]);
@@ -1480,6 +1528,122 @@ int tgx_clear_2b_models(tgx_ctx* c) {
   return TGX_OK;
 }
 
+int tgx_add_angle_3b(tgx_ctx* c, int sp_center, int sp1, int sp2, double rcut, double delta,
+                     const double* sigma, int kernel, int n_sparse, const double* Qs,
+                     const double* alphas, const double* cutoff) {
+  if (!c) return TGX_ERR_ARG;
+  if (md_busy(c)) return TGX_ERR_STATE;
+  auto sp_ok = [](int s) { return s >= 1 && s <= TGX_MAX_SPECIES; };
+  if (!sp_ok(sp_center) || !sp_ok(sp1) || !sp_ok(sp2) || !(rcut > 0) || !sigma ||
+      !(sigma[0] > 0 && sigma[1] > 0 && sigma[2] > 0) || (kernel != 0 && kernel != 1) ||
+      n_sparse < 1 || !Qs || !alphas || !cutoff)
+    return set_err(c, TGX_ERR_ARG, "bad angle_3b arguments");
+  (void)hipSetDevice(c->device);
+  Angle3bDev b{};
+  b.sp0 = sp_center;
+  b.sp1 = sp1;
+  b.sp2 = sp2;
+  b.n_sparse = n_sparse;
+  b.kernel = kernel;
+  b.rcut = rcut;
+  b.buffer = 0.5;  // calculate_gap_3b passes 0.5 (gap.f90:738)
+  for (int k = 0; k < 3; k++) b.is2[k] = 1.0 / (sigma[k] * sigma[k]);
+  std::vector<double> qp(4 * (size_t)n_sparse);
+  for (int s = 0; s < n_sparse; s++) {
+    for (int k = 0; k < 3; k++) qp[4 * s + k] = Qs[(size_t)k * n_sparse + s];
+    qp[4 * s + 3] = 2.0 * delta * delta * alphas[s] * cutoff[s];
+  }
+  TGX_HIP(c, hipMalloc(&b.Qp, 8 * qp.size()));
+  TGX_HIP(c, hipMemcpy(b.Qp, qp.data(), 8 * qp.size(), hipMemcpyHostToDevice));
+  c->a3b.push_back(b);
+  c->maxnb = 0;
+  return TGX_OK;
+}
+
+// Second derivatives M of the cubic spline through (x, V) by one tridiagonal
+// solve: interior rows h_{i-1} M_{i-1} + 2 (h_{i-1} + h_i) M_i + h_i M_{i+1}
+// = 6 (s_i - s_{i-1}) (s_i the slope of interval i); an end is clamped to
+// its slope, or natural (M = 0) when the slope exceeds 0.99e30, and then
+// takes the spline's own end derivative.
+static void core_pot_spline(int n, const double* x, const double* V, double* yp1, double* ypn,
+                            double* M) {
+  std::vector<double> h(n - 1), sl(n - 1), lo(n), di(n), up(n), rhs(n);
+  for (int i = 0; i + 1 < n; i++) {
+    h[i] = x[i + 1] - x[i];
+    sl[i] = (V[i + 1] - V[i]) / h[i];
+  }
+  const bool nat0 = *yp1 > 0.99e30, natn = *ypn > 0.99e30;
+  lo[0] = 0.0;
+  di[0] = nat0 ? 1.0 : 2.0 * h[0];
+  up[0] = nat0 ? 0.0 : h[0];
+  rhs[0] = nat0 ? 0.0 : 6.0 * (sl[0] - *yp1);
+  for (int i = 1; i + 1 < n; i++) {
+    lo[i] = h[i - 1];
+    di[i] = 2.0 * (h[i - 1] + h[i]);
+    up[i] = h[i];
+    rhs[i] = 6.0 * (sl[i] - sl[i - 1]);
+  }
+  lo[n - 1] = natn ? 0.0 : h[n - 2];
+  di[n - 1] = natn ? 1.0 : 2.0 * h[n - 2];
+  up[n - 1] = 0.0;
+  rhs[n - 1] = natn ? 0.0 : 6.0 * (*ypn - sl[n - 2]);
+  // Thomas algorithm (the matrix is diagonally dominant)
+  for (int i = 1; i < n; i++) {
+    const double w = lo[i] / di[i - 1];
+    di[i] -= w * up[i - 1];
+    rhs[i] -= w * rhs[i - 1];
+  }
+  M[n - 1] = rhs[n - 1] / di[n - 1];
+  for (int i = n - 2; i >= 0; i--) M[i] = (rhs[i] - up[i] * M[i + 1]) / di[i];
+  if (nat0) *yp1 = sl[0] - h[0] * (2.0 * M[0] + M[1]) / 6.0;
+  if (natn) *ypn = sl[n - 2] + h[n - 2] * (M[n - 2] + 2.0 * M[n - 1]) / 6.0;
+}
+
+int tgx_add_core_pot(tgx_ctx* c, int sp1, int sp2, int n, const double* x, const double* V,
+                     double yp1, double ypn) {
+  if (!c) return TGX_ERR_ARG;
+  if (md_busy(c)) return TGX_ERR_STATE;
+  if (sp1 < 1 || sp2 < 1 || sp1 > TGX_MAX_SPECIES || sp2 > TGX_MAX_SPECIES || n < 2 || !x || !V)
+    return set_err(c, TGX_ERR_ARG, "bad core_pot arguments");
+  for (int i = 0; i + 1 < n; i++)
+    if (!(x[i + 1] > x[i])) return set_err(c, TGX_ERR_ARG, "core_pot x must ascend strictly");
+  (void)hipSetDevice(c->device);
+  CorePotDev b{};
+  b.sp1 = sp1;
+  b.sp2 = sp2;
+  b.n = n;
+  b.rcut = x[n - 1];
+  b.yp1 = yp1;
+  b.ypn = ypn;
+  std::vector<double> M(n);
+  core_pot_spline(n, x, V, &b.yp1, &b.ypn, M.data());
+  TGX_HIP(c, hipMalloc(&b.x, 8 * (size_t)n));
+  TGX_HIP(c, hipMalloc(&b.V, 8 * (size_t)n));
+  TGX_HIP(c, hipMalloc(&b.d2, 8 * (size_t)n));
+  TGX_HIP(c, hipMemcpy(b.x, x, 8 * (size_t)n, hipMemcpyHostToDevice));
+  TGX_HIP(c, hipMemcpy(b.V, V, 8 * (size_t)n, hipMemcpyHostToDevice));
+  TGX_HIP(c, hipMemcpy(b.d2, M.data(), 8 * (size_t)n, hipMemcpyHostToDevice));
+  c->corep.push_back(b);
+  c->maxnb = 0;
+  return TGX_OK;
+}
+
+int tgx_clear_extra_terms(tgx_ctx* c) {
+  if (!c) return TGX_ERR_ARG;
+  if (md_busy(c)) return TGX_ERR_STATE;
+  (void)hipSetDevice(c->device);
+  for (auto& b : c->a3b) (void)hipFree(b.Qp);
+  for (auto& b : c->corep) {
+    (void)hipFree(b.x);
+    (void)hipFree(b.V);
+    (void)hipFree(b.d2);
+  }
+  c->a3b.clear();
+  c->corep.clear();
+  c->maxnb = 0;
+  return TGX_OK;
+}
+
 int tgx_neighbors_build(tgx_ctx* c, int n, const double* pos, const double* lattice, double rcut,
                         int64_t cap, int* n_neigh, int* list, double* rjs, double* xyz,
                         double* thetas, double* phis, int64_t* n_pairs, int* indices_out) {
@@ -1729,7 +1893,7 @@ int tgx_energy_forces_batched(tgx_ctx* c, int n_struct, const int* offsets, cons
     if (offsets[s + 1] <= offsets[s]) return set_err(c, TGX_ERR_ARG, "structure %d is empty", s);
   const int n = offsets[n_struct];
   if (n >= (1 << TGX_IMG_SHIFT)) return set_err(c, TGX_ERR_ARG, "too many atoms");
-  if (!c->has_soap && c->twob.empty()) return set_err(c, TGX_ERR_STATE, "no model loaded");
+  if (!has_model(c)) return set_err(c, TGX_ERR_STATE, "no model loaded");
   (void)hipSetDevice(c->device);
   Extra& x = X(c);
   int rcode = upload_structure(c, n, pos, species);
@@ -2104,8 +2268,10 @@ static int mc_run_impl(tgx_ctx* c, int n, const int* species, const double* latt
   if (!(skin > std::sqrt(3.0) * move_max))
     return set_err(c, TGX_ERR_ARG, "skin must exceed sqrt(3) * move_max (%g)",
                    std::sqrt(3.0) * move_max);
-  if (!c->has_soap && c->twob.empty()) return set_err(c, TGX_ERR_STATE, "no model loaded");
+  if (!has_model(c)) return set_err(c, TGX_ERR_STATE, "no model loaded");
   if (X(c).md.active) return set_err(c, TGX_ERR_STATE, "end the MD run first (tgx_md_end)");
+  if (!c->a3b.empty() || !c->corep.empty())
+    return set_err(c, TGX_ERR_UNSUPPORTED, "Monte Carlo takes no angle_3b or core_pot blocks");
   if (c->has_soap && c->blocks.size() > 1)
     return set_err(c, TGX_ERR_UNSUPPORTED,
                    "Monte Carlo takes models of one soap_turbo block (this one has %d)",
@@ -2553,7 +2719,7 @@ int tgx_md_begin(tgx_ctx* c, int n, const int* species, const double* lattice,
   if (n < 2 || !species || !lattice || !mass || !pos || !vel || !(dt > 0) || skin < 0 ||
       history < 1)
     return set_err(c, TGX_ERR_ARG, "bad arguments");
-  if (!c->has_soap && c->twob.empty()) return set_err(c, TGX_ERR_STATE, "no model loaded");
+  if (!has_model(c)) return set_err(c, TGX_ERR_STATE, "no model loaded");
   (void)hipSetDevice(c->device);
   Extra& x = X(c);
   MdState& m = x.md;

@@ -96,6 +96,25 @@ struct TwoBodyDev {
   int nt;
 };
 
+// One angle_3b block on the device: per sparse point (q1, q2, q3, pref),
+// pref = 2 delta^2 alpha c; is2 = 1 / sigma^2 per descriptor component.
+struct Angle3bDev {
+  int sp0, sp1, sp2, n_sparse, kernel;  // kernel 0 "exp", 1 "pol"
+  double rcut, buffer;
+  double is2[3];
+  double* Qp;  // device, 4 doubles per sparse point
+};
+
+// One core_pot block on the device: the table, its spline second
+// derivatives and end slopes; rcut = largest x.
+struct CorePotDev {
+  int sp1, sp2, n;
+  double rcut, yp1, ypn;
+  double* x;   // device
+  double* V;   // device
+  double* d2;  // device
+};
+
 // Monte Carlo control words on the device (int and double arrays).
 enum {
   MC_REBUILD = 0,  // list rebuild requested by this trial
@@ -181,6 +200,8 @@ struct tgx_ctx {
   double e0[TGX_MAX_SPECIES] = {0};
   double list_buffer = 0.0;  // list cutoff = largest model cutoff + list_buffer
   std::vector<tgx::TwoBodyDev> twob;
+  std::vector<tgx::Angle3bDev> a3b;
+  std::vector<tgx::CorePotDev> corep;
 
   // per-call work buffers
   tgx::DevBuf pos, species, nb_count, nb_list, nb_rev, nb_flag, cell_of, cell_start,

@@ -1254,6 +1254,242 @@ hipError_t launch_two_body(const TwoBodyDev& b, const double* pos, const int* sp
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// core_pot term (gap.f90:552-686): cubic spline of a tabulated pair
+// potential, each pair counted from both atoms (E_i += V/2), force on the
+// central atom V'(r) x/r, virial -1/2 sym(F x) then x 1/2.  One wave per
+// atom, as the distance_2b kernel.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double core_spline(const CorePotDev& b, double r, double* dV) {
+  const double* x = b.x;
+  const int n = b.n;
+  if (r < x[0]) {
+    *dV = b.yp1;
+    return b.V[0] + (r - x[0]) * b.yp1;
+  }
+  if (r > x[n - 1]) {
+    *dV = b.ypn;
+    return b.V[n - 1] + (r - x[n - 1]) * b.ypn;
+  }
+  // first interval j with r < x[j+1] (the last one when r = x[n-1])
+  int lo = 0, hi = n - 2;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (r < x[mid + 1]) hi = mid;
+    else lo = mid + 1;
+  }
+  const double h = x[lo + 1] - x[lo];
+  const double A = (x[lo + 1] - r) / h, B = 1.0 - A;
+  const double y0 = b.V[lo], y1 = b.V[lo + 1], m0 = b.d2[lo], m1 = b.d2[lo + 1];
+  *dV = (y1 - y0) / h + ((1.0 - 3.0 * A * A) * m0 + (3.0 * B * B - 1.0) * m1) * (h / 6.0);
+  return A * y0 + B * y1 + ((A * A * A - A) * m0 + (B * B * B - B) * m1) * (h * h / 6.0);
+}
+
+template <typename G>
+__global__ __launch_bounds__(256) void core_pot_kernel(CorePotDev b, const double* pos,
+                                                       const int* species, int n, G geo,
+                                                       int maxnb, const int* count,
+                                                       const int* list, double* eatom,
+                                                       double* forces, double* vatom,
+                                                       int do_forces) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= n) return;
+  const int si = species[i];
+  if (si != b.sp1 && si != b.sp2) return;
+  const Cell c = geo.of(i);
+  const int cnt = min(count[i], maxnb);
+  double e = 0, f[3] = {0, 0, 0}, v[6] = {0, 0, 0, 0, 0, 0};
+  for (int s = lane; s < cnt; s += 64) {
+    const int ent = list[(size_t)i * maxnb + s];
+    const int sj = species[ent & TGX_ATOM_MASK];
+    if (!((si == b.sp1 && sj == b.sp2) || (si == b.sp2 && sj == b.sp1))) continue;
+    double x[3];
+    const double r = pair_vec(pos, i, ent, c, x);
+    if (!(r < b.rcut)) continue;
+    double dV;
+    e += 0.5 * core_spline(b, r, &dV);
+    if (do_forces) {
+      const double pre = dV / r;
+      const double ff[3] = {pre * x[0], pre * x[1], pre * x[2]};
+      f[0] += ff[0];
+      f[1] += ff[1];
+      f[2] += ff[2];
+      v[0] -= ff[0] * x[0];
+      v[1] -= ff[1] * x[1];
+      v[2] -= ff[2] * x[2];
+      v[3] -= 0.5 * (ff[0] * x[1] + ff[1] * x[0]);
+      v[4] -= 0.5 * (ff[0] * x[2] + ff[2] * x[0]);
+      v[5] -= 0.5 * (ff[1] * x[2] + ff[2] * x[1]);
+    }
+  }
+  e = wsum(e);
+  if (lane == 0) eatom[i] += e;
+  if (do_forces) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const double t = wsum(f[k]);
+      if (lane == 0) forces[3 * i + k] += t;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      const double t = wsum(v[k]);
+      if (lane == 0) vatom[(size_t)i * 6 + k] += 0.5 * t;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// angle_3b term (gap.f90:756-1015): for a centre of species sp0, every
+// unordered pair of neighbours within rcut of species (sp1, sp2) in either
+// order forms q = (r12 + r13, (r12 - r13)^2, r23); energy fcut12 fcut13
+// sum_s pref_s k(|q - Q_s|_sigma).  One wave per centre; lane a owns
+// neighbour slot a and sums, over every partner b, the force on its atom,
+// so each slot's force is written once, in a fixed order: g[i][a] = -(force
+// on slot a's atom from centre i), which the force gather turns into the
+// forces on all three atoms (the centre's is minus the sum of its slots').
+// Per-atom virial: sum_a sym(F_a x_a)/2, the reference's sym(F2 x12 + F3
+// x13)/2 per triplet.  The energy counts each pair once (b > a).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void cut3b(double r, double rcut, double buffer, double* fc,
+                                      double* dfc) {
+  if (r < rcut - buffer) {
+    *fc = 1.0;
+    *dfc = 0.0;
+  } else {
+    const double t = M_PI * (r - rcut + buffer) / buffer;
+    *fc = (cos(t) + 1.0) / 2.0;
+    *dfc = sin(t) / 2.0 * M_PI / buffer;  // -d fc / dr
+  }
+}
+
+template <typename G>
+__global__ __launch_bounds__(256) void angle3b_kernel(Angle3bDev b, const double* pos,
+                                                      const int* species, int n, G geo,
+                                                      int maxnb, const int* count,
+                                                      const int* list, double* eatom,
+                                                      double* vatom, double4* g3, int do_forces) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= n) return;
+  const Cell c = geo.of(i);
+  const int cnt = min(count[i], maxnb);
+  const bool centre = species[i] == b.sp0;
+  const int* row = list + (size_t)i * maxnb;
+  double e = 0, v[6] = {0, 0, 0, 0, 0, 0};
+  for (int a = lane; a < cnt; a += 64) {
+    double F[3] = {0, 0, 0}, xa[3] = {0, 0, 0};
+    const int ea = row[a];
+    const int sa = species[ea & TGX_ATOM_MASK];
+    const double ra = centre ? pair_vec(pos, i, ea, c, xa) : 0.0;
+    if (centre && (sa == b.sp1 || sa == b.sp2) && ra < b.rcut) {
+      double fa, sfa;
+      cut3b(ra, b.rcut, b.buffer, &fa, &sfa);
+      const double ua[3] = {xa[0] / ra, xa[1] / ra, xa[2] / ra};
+      for (int bb = 0; bb < cnt; bb++) {
+        if (bb == a) continue;
+        const int eb = row[bb];
+        const int sb = species[eb & TGX_ATOM_MASK];
+        if (!((sa == b.sp1 && sb == b.sp2) || (sa == b.sp2 && sb == b.sp1))) continue;
+        double xb[3];
+        const double rb = pair_vec(pos, i, eb, c, xb);
+        if (!(rb < b.rcut)) continue;
+        double fb, sfb;
+        cut3b(rb, b.rcut, b.buffer, &fb, &sfb);
+        const double x23[3] = {xb[0] - xa[0], xb[1] - xa[1], xb[2] - xa[2]};
+        const double r23 = sqrt(x23[0] * x23[0] + x23[1] * x23[1] + x23[2] * x23[2]);
+        const double q1 = ra + rb, q2 = (ra - rb) * (ra - rb), q3 = r23;
+        double ks = 0.0, C1 = 0.0, C2 = 0.0, C3 = 0.0;
+        for (int s = 0; s < b.n_sparse; s++) {
+          const double4 Q = reinterpret_cast<const double4*>(b.Qp)[s];
+          const double d1 = (q1 - Q.x) * b.is2[0], d2 = (q2 - Q.y) * b.is2[1],
+                       d3 = (q3 - Q.z) * b.is2[2];
+          const double rr2 = d1 * (q1 - Q.x) + d2 * (q2 - Q.y) + d3 * (q3 - Q.z);
+          double k, kd;  // kernel and the factor of its q-gradient
+          if (b.kernel == 0) {
+            k = Q.w * exp(-0.5 * rr2);
+            kd = k;
+          } else {
+            const double rs = sqrt(rr2);
+            if (rs >= 1.0) {
+              k = 0.0;
+              kd = 0.0;
+            } else {
+              const double om = 1.0 - rs;
+              k = rs <= 0.0 ? Q.w : Q.w * om * om * om * om * (4.0 * rs + 1.0);
+              kd = rs <= 0.0 ? 0.0 : 20.0 * Q.w * om * om * om;  // -cov_pp_der
+            }
+          }
+          ks += k;
+          C1 += kd * d1;
+          C2 += kd * d2;
+          C3 += kd * d3;
+        }
+        const double fcut = fa * fb;
+        if (bb > a) e += fcut * ks;
+        if (do_forces) {
+          // force on slot a's atom, -d(fa fb ks)/dx_a: d fa/dx_a = -sfa ua and
+          // d ks/dx_a = -(C . dq/dx_a), dq/dx_a = (ua, 2 (ra - rb) ua, -u23)
+          const double t = C1 + 2.0 * (ra - rb) * C2;
+#pragma unroll
+          for (int l = 0; l < 3; l++) {
+            const double u23 = x23[l] / r23;
+            F[l] += fb * ks * sfa * ua[l] + fcut * (t * ua[l] - C3 * u23);
+          }
+        }
+      }
+    }
+    if (do_forces) {
+      g3[(size_t)i * maxnb + a] = make_double4(-F[0], -F[1], -F[2], 0.0);
+      v[0] += F[0] * xa[0];
+      v[1] += F[1] * xa[1];
+      v[2] += F[2] * xa[2];
+      v[3] += 0.5 * (F[0] * xa[1] + F[1] * xa[0]);
+      v[4] += 0.5 * (F[0] * xa[2] + F[2] * xa[0]);
+      v[5] += 0.5 * (F[1] * xa[2] + F[2] * xa[1]);
+    }
+  }
+  e = wsum(e);
+  if (lane == 0 && centre) eatom[i] += e;
+  if (do_forces) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      const double t = wsum(v[k]);
+      if (lane == 0) vatom[(size_t)i * 6 + k] += t;
+    }
+  }
+}
+
+hipError_t launch_core_pot(const CorePotDev& b, const double* pos, const int* species, int n,
+                           const Geom& g, int maxnb, const int* count, const int* list,
+                           double* eatom, double* forces, double* vatom, int do_forces,
+                           hipStream_t st) {
+  const dim3 grid((n + 3) / 4), block(256);
+  if (g.many)
+    hipLaunchKernelGGL(core_pot_kernel<CellMany>, grid, block, 0, st, b, pos, species, n,
+                       CellMany{g.many, g.sid}, maxnb, count, list, eatom, forces, vatom,
+                       do_forces);
+  else
+    hipLaunchKernelGGL(core_pot_kernel<CellOne>, grid, block, 0, st, b, pos, species, n,
+                       CellOne{g.one}, maxnb, count, list, eatom, forces, vatom, do_forces);
+  return hipGetLastError();
+}
+
+hipError_t launch_angle_3b(const Angle3bDev& b, const double* pos, const int* species, int n,
+                           const Geom& g, int maxnb, const int* count, const int* list,
+                           double* eatom, double* vatom, double* g3, int do_forces,
+                           hipStream_t st) {
+  const dim3 grid((n + 3) / 4), block(256);
+  double4* g4 = reinterpret_cast<double4*>(g3);
+  if (g.many)
+    hipLaunchKernelGGL(angle3b_kernel<CellMany>, grid, block, 0, st, b, pos, species, n,
+                       CellMany{g.many, g.sid}, maxnb, count, list, eatom, vatom, g4, do_forces);
+  else
+    hipLaunchKernelGGL(angle3b_kernel<CellOne>, grid, block, 0, st, b, pos, species, n,
+                       CellOne{g.one}, maxnb, count, list, eatom, vatom, g4, do_forces);
+  return hipGetLastError();
+}
+
 // e_atom = e0(species) + SOAP site energy; forces and per-atom virial zeroed.
 __global__ void energy_init_kernel(int n, const int* species, const double* e0,
                                    const int* site_of_atom, const double* esite,
