@@ -66,6 +66,23 @@ def write_model_files(d, model, species=("C",)):
                   f"species2 = {species[b.sp2 - 1]}", f"rcut = {b.rcut}", f"delta = {b.delta}",
                   f"sigma = {b.sigma}", f"alphas_sparse = a2b_{k}.dat",
                   f"desc_sparse = d2b_{k}.dat", "gap_end"]
+    for k, b in enumerate(getattr(model, "angle_3b", [])):
+        np.savetxt(os.path.join(d, f"a3b_{k}.dat"), np.stack([b.alphas, b.cutoff], 1),
+                   fmt="%.17e")
+        np.savetxt(os.path.join(d, f"d3b_{k}.dat"), np.asarray(b.Qs).reshape(-1), fmt="%.17e")
+        lines += ["gap_beg angle_3b", f"species_center = {species[b.sp0 - 1]}",
+                  f"species1 = {species[b.sp1 - 1]}", f"species2 = {species[b.sp2 - 1]}",
+                  f"rcut = {b.rcut}", f"delta = {b.delta}",
+                  "sigma = " + " ".join(repr(float(x)) for x in b.sigma),
+                  f"kernel_type = {b.kernel}", f"alphas_sparse = a3b_{k}.dat",
+                  f"desc_sparse = d3b_{k}.dat", "gap_end"]
+    for k, b in enumerate(getattr(model, "core_pot", [])):
+        with open(os.path.join(d, f"core_{k}.dat"), "w") as f:
+            f.write(f"{len(b.x)} {float(b.yp1):.17e} {float(b.ypn):.17e}\n")
+            for x, v in zip(b.x, b.V):
+                f.write(f"{x:.17e} {v:.17e}\n")
+        lines += ["gap_beg core_pot", f"species1 = {species[b.sp1 - 1]}",
+                  f"species2 = {species[b.sp2 - 1]}", f"core_pot_file = core_{k}.dat", "gap_end"]
     open(os.path.join(d, "model.gap"), "w").write("\n".join(lines) + "\n")
 
 
@@ -218,3 +235,30 @@ def test_fortran_predict_multiframe_two_blocks(tmp_path):
         assert np.abs(arr[:, 0:3] - p).max() < 1e-7
         assert np.abs(arr[:, 3:6] - f0).max() < 1e-7
         assert np.abs(arr[:, 6] - e0).max() < 1e-7
+
+
+@pytest.mark.gpu
+def test_fortran_predict_angle_3b_core_pot(tmp_path):
+    """angle_3b ("exp" and "pol") and core_pot blocks read from .gap files in
+    the reference's formats, evaluated with SOAP and 2b, against the
+    oracle."""
+    import test_gpu_terms as T
+    exe = os.path.join(FDIR, "tgx_predict")
+    if not os.path.exists(exe):
+        pytest.fail("tgx_predict is not built (run __graft_entry__.build())")
+    pos, sp, lat, base = W.carbon216()
+    model = T._with_terms(base, [T._a3("exp"), T._a3("pol", seed=5, rcut=2.3)], [T._core()])
+    write_model_files(str(tmp_path), model)
+    write_xyz(str(tmp_path / "s.xyz"), pos, sp, lat)
+    out = subprocess.run([exe, "model.gap", "s.xyz", "fp64"], capture_output=True, text=True,
+                         cwd=tmp_path, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    lines = out.stdout.splitlines()
+    etot = float(lines[0].split()[1])
+    vir = np.array(lines[1].split()[1:], float).reshape(3, 3).T
+    rows = np.array([l.split()[1:] for l in lines[2:2 + len(pos)]], float)
+    e0, f0, v0 = oracle.energy_forces(model, pos, sp, lat)[:3]
+    assert abs(etot - e0.sum()) < 1e-8 * len(pos)
+    assert np.abs(rows[:, 3] - e0).max() < 1e-8
+    assert np.abs(rows[:, 4:7] - f0).max() < 1e-6
+    assert np.abs(vir - v0).max() < 1e-6

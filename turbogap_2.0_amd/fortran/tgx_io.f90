@@ -1,5 +1,5 @@
-! Input and output for the Fortran drivers: the subset of TurboGAP's .gap
-! model format used by soap_turbo and distance_2b blocks
+! Input and output for the Fortran drivers: TurboGAP's .gap model format
+! for soap_turbo, distance_2b, angle_3b and core_pot blocks
 ! (gap/read_gap.f90:45-771, "keyword = value(s)" lines between gap_beg and
 ! gap_end; any number of soap_turbo blocks, as n_soap_turbo there), the
 ! sparse-set files (read_alphas_and_descriptors, read_gap.f90:773-875),
@@ -27,6 +27,18 @@ module tgx_io
      real(c_double), allocatable :: Qs(:), alphas(:)
   end type soap_block
 
+  type, public :: angle_3b_block
+     integer :: sp0 = 1, sp1 = 1, sp2 = 1, n_sparse = 0, kernel = 0
+     real(c_double) :: rcut = 0, delta = 1, sigma(3) = 1
+     real(c_double), allocatable :: Qs(:), alphas(:), cutoff(:)
+  end type angle_3b_block
+
+  type, public :: core_pot_block
+     integer :: sp1 = 1, sp2 = 1, n = 0
+     real(c_double) :: yp1 = 0, ypn = 0
+     real(c_double), allocatable :: x(:), V(:)
+  end type core_pot_block
+
   type, public :: gap_model
      integer :: n_species = 0
      character(len=8) :: species(TGX_MAX_SPECIES) = ''
@@ -35,7 +47,16 @@ module tgx_io
      real(c_double) :: e0(TGX_MAX_SPECIES) = 0
      integer :: n2b = 0
      type(two_body_block) :: b2(8)
+     integer :: n3b = 0
+     type(angle_3b_block) :: b3(8)
+     integer :: ncore = 0
+     type(core_pot_block) :: core(8)
   end type gap_model
+
+  ! core_pot tables are cut at core_pot_cutoff and tapered by a cosine over
+  ! the last core_pot_buffer (read_gap.f90:663-682); the defaults are the
+  ! reference's (types.f90:55-56), input-file keywords there.
+  real(c_double), public :: core_pot_cutoff = 1.0d10, core_pot_buffer = 0.0d0
 
   public :: read_gap_file, read_xyz_frame, read_xyz_next, species_index, load_model, &
        write_extxyz
@@ -96,6 +117,61 @@ contains
     close (u)
   end subroutine read_2b_sparse
 
+  ! angle_3b: alphas file "alpha cutoff" per line, desc file three values
+  ! per sparse point, one per line; Qs kept (n_sparse, 3) column-major
+  ! (read_gap.f90:850-872).
+  subroutine read_3b_sparse(fa, fd, b)
+    character(len=*), intent(in) :: fa, fd
+    type(angle_3b_block), intent(inout) :: b
+    integer :: u, k, j
+    b%n_sparse = count_lines(fa)
+    allocate (b%alphas(b%n_sparse), b%cutoff(b%n_sparse), b%Qs(3*b%n_sparse))
+    open (newunit=u, file=fa, status='old', action='read')
+    do k = 1, b%n_sparse
+       read (u, *) b%alphas(k), b%cutoff(k)
+    end do
+    close (u)
+    open (newunit=u, file=fd, status='old', action='read')
+    do k = 1, b%n_sparse
+       do j = 1, 3
+          read (u, *) b%Qs((j - 1)*b%n_sparse + k)
+       end do
+    end do
+    close (u)
+  end subroutine read_3b_sparse
+
+  ! core_pot file: "n yp1 ypn", then n lines "x V"; points beyond
+  ! core_pot_cutoff dropped, the last core_pot_buffer tapered.
+  subroutine read_core_pot(fname, b)
+    character(len=*), intent(in) :: fname
+    type(core_pot_block), intent(inout) :: b
+    integer :: u, k, n, kept
+    real(c_double) :: xk, vk
+    real(c_double), allocatable :: x(:), V(:)
+    open (newunit=u, file=fname, status='old', action='read')
+    read (u, *) n, b%yp1, b%ypn
+    allocate (x(n), V(n))
+    kept = 0
+    do k = 1, n
+       read (u, *) xk, vk
+       if (xk <= core_pot_cutoff) then
+          kept = kept + 1
+          x(kept) = xk
+          if (xk <= core_pot_cutoff - core_pot_buffer) then
+             V(kept) = vk
+          else
+             V(kept) = vk*0.5d0*(cos(acos(-1.d0)/core_pot_buffer*(xk - core_pot_cutoff + &
+                  core_pot_buffer)) + 1.d0)
+          end if
+       end if
+    end do
+    close (u)
+    b%n = kept
+    allocate (b%x(kept), b%V(kept))
+    b%x = x(1:kept)
+    b%V = V(1:kept)
+  end subroutine read_core_pot
+
   integer function species_index(m, sym)
     type(gap_model), intent(in) :: m
     character(len=*), intent(in) :: sym
@@ -113,11 +189,17 @@ contains
     character(len=64) :: kind
     character(len=8) :: s1, s2, names(TGX_MAX_SPECIES)
     integer :: u, ios, ns, eq, k, ib
-    logical :: in_soap, in_2b
+    character(len=8) :: s0
+    character(len=16) :: ktype
+    logical :: in_soap, in_2b, in_3b, in_core
     real(c_double) :: buffer(TGX_MAX_SPECIES)
     type(two_body_block) :: b
+    type(angle_3b_block) :: b3
+    type(core_pot_block) :: cp
     in_soap = .false.
     in_2b = .false.
+    in_3b = .false.
+    in_core = .false.
     ib = 0
     open (newunit=u, file=fname, status='old', action='read')
     do
@@ -128,6 +210,18 @@ contains
           read (line(8:), *) kind
           in_soap = trim(kind) == 'soap_turbo'
           in_2b = trim(kind) == 'distance_2b'
+          in_3b = trim(kind) == 'angle_3b'
+          in_core = trim(kind) == 'core_pot'
+          if (in_3b) then
+             b3 = angle_3b_block()
+             ktype = 'exp'
+             fa = ''
+             fd = ''
+          end if
+          if (in_core) then
+             cp = core_pot_block()
+             fa = ''
+          end if
           if (in_soap) then
              if (m%n_soap == TGX_MAX_SOAP_BLOCKS) then
                 write (*, *) 'ERROR: more than ', TGX_MAX_SOAP_BLOCKS, ' soap_turbo blocks'
@@ -174,9 +268,32 @@ contains
              b%sp2 = species_index(m, s2)
              m%n2b = m%n2b + 1
              m%b2(m%n2b) = b
+          else if (in_3b) then
+             call read_3b_sparse(trim(fa), trim(fd), b3)
+             b3%sp0 = species_index(m, s0)
+             b3%sp1 = species_index(m, s1)
+             b3%sp2 = species_index(m, s2)
+             if (trim(ktype) == 'exp') then
+                b3%kernel = 0
+             else if (trim(ktype) == 'pol') then
+                b3%kernel = 1
+             else
+                write (*, *) 'ERROR: angle_3b kernel_type must be exp or pol'
+                stop 1
+             end if
+             m%n3b = m%n3b + 1
+             m%b3(m%n3b) = b3
+          else if (in_core) then
+             call read_core_pot(trim(fa), cp)
+             cp%sp1 = species_index(m, s1)
+             cp%sp2 = species_index(m, s2)
+             m%ncore = m%ncore + 1
+             m%core(m%ncore) = cp
           end if
           in_soap = .false.
           in_2b = .false.
+          in_3b = .false.
+          in_core = .false.
           cycle
        end if
        eq = index(line, '=')
@@ -229,11 +346,41 @@ contains
                read (rest, *) fd
             end select
           end associate
+       else if (in_3b) then
+          select case (trim(key))
+          case ('species1', 'Z1', 'z1')
+             read (rest, *) s1
+          case ('species2', 'Z2', 'z2')
+             read (rest, *) s2
+          case ('species_center', 'Z_center', 'z_center')
+             read (rest, *) s0
+          case ('rcut')
+             read (rest, *) b3%rcut
+          case ('delta')
+             read (rest, *) b3%delta
+          case ('sigma')
+             read (rest, *) b3%sigma
+          case ('kernel_type')
+             read (rest, *) ktype
+          case ('alphas_sparse')
+             read (rest, *) fa
+          case ('desc_sparse')
+             read (rest, *) fd
+          end select
+       else if (in_core) then
+          select case (trim(key))
+          case ('species1', 'Z1', 'z1')
+             read (rest, *) s1
+          case ('species2', 'Z2', 'z2')
+             read (rest, *) s2
+          case ('core_pot_file')
+             read (rest, *) fa
+          end select
        else if (in_2b) then
           select case (trim(key))
-          case ('species1')
+          case ('species1', 'Z1', 'z1')
              read (rest, *) s1
-          case ('species2')
+          case ('species2', 'Z2', 'z2')
              read (rest, *) s2
           case ('rcut')
              read (rest, *) b%rcut
@@ -252,7 +399,7 @@ contains
   end subroutine read_gap_file
 
   ! Hands the model to a context: the soap_turbo blocks in file order, e0,
-  ! the distance_2b blocks.
+  ! the distance_2b, angle_3b and core_pot blocks.
   subroutine load_model(ctx, m)
     type(c_ptr), intent(in) :: ctx
     type(gap_model), intent(in) :: m
@@ -276,6 +423,20 @@ contains
             int(m%b2(k)%sp2, c_int), m%b2(k)%rcut, m%b2(k)%buffer, m%b2(k)%delta, &
             m%b2(k)%sigma, int(m%b2(k)%n_sparse, c_int), m%b2(k)%Qs, m%b2(k)%alphas, &
             m%b2(k)%cutoff), 'tgx_add_2b_model')
+    end do
+    call tgx_check(ctx, tgx_clear_extra_terms(ctx), 'tgx_clear_extra_terms')
+    do k = 1, m%n3b
+       associate (b => m%b3(k))
+         call tgx_check(ctx, tgx_add_angle_3b(ctx, int(b%sp0, c_int), int(b%sp1, c_int), &
+              int(b%sp2, c_int), b%rcut, b%delta, b%sigma, int(b%kernel, c_int), &
+              int(b%n_sparse, c_int), b%Qs, b%alphas, b%cutoff), 'tgx_add_angle_3b')
+       end associate
+    end do
+    do k = 1, m%ncore
+       associate (b => m%core(k))
+         call tgx_check(ctx, tgx_add_core_pot(ctx, int(b%sp1, c_int), int(b%sp2, c_int), &
+              int(b%n, c_int), b%x, b%V, b%yp1, b%ypn), 'tgx_add_core_pot')
+       end associate
     end do
   end subroutine load_model
 

@@ -64,7 +64,8 @@ module tgx_mod
   public :: tgx_create, tgx_destroy, tgx_last_error, tgx_set_precision, tgx_set_autotune, &
        tgx_set_soap_model, tgx_add_soap_model, tgx_clear_soap_models, tgx_n_soap_models, &
        tgx_energy_forces_noalloc, &
-       tgx_set_e0, tgx_add_2b_model, tgx_clear_2b_models, tgx_neighbors_build, &
+       tgx_set_e0, tgx_add_2b_model, tgx_clear_2b_models, tgx_add_angle_3b, &
+       tgx_add_core_pot, tgx_clear_extra_terms, tgx_neighbors_build, &
        tgx_soap_descriptors, tgx_gap_kernel_step, tgx_energy_forces, &
        tgx_energy_forces_batched, tgx_md_run, tgx_md_begin, tgx_md_steps, tgx_md_read, &
        tgx_md_end, tgx_sync, tgx_poly3_orthonormalization, tgx_stage_times, tgx_check, &
@@ -153,6 +154,29 @@ module tgx_mod
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx
      end function tgx_clear_2b_models
+
+     integer(c_int) function tgx_add_angle_3b(ctx, sp_center, sp1, sp2, rcut, delta, sigma, &
+          kernel, n_sparse, Qs, alphas, cutoff) bind(C, name="tgx_add_angle_3b")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: sp_center, sp1, sp2, kernel, n_sparse
+       real(c_double), value :: rcut, delta
+       real(c_double), intent(in) :: sigma(3), Qs(*), alphas(*), cutoff(*)
+     end function tgx_add_angle_3b
+
+     integer(c_int) function tgx_add_core_pot(ctx, sp1, sp2, n, x, V, yp1, ypn) &
+          bind(C, name="tgx_add_core_pot")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: sp1, sp2, n
+       real(c_double), intent(in) :: x(*), V(*)
+       real(c_double), value :: yp1, ypn
+     end function tgx_add_core_pot
+
+     integer(c_int) function tgx_clear_extra_terms(ctx) bind(C, name="tgx_clear_extra_terms")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function tgx_clear_extra_terms
 
      integer(c_int) function tgx_neighbors_build(ctx, n, pos, lattice, rcut, cap, n_neigh, &
           list, rjs, xyz, thetas, phis, n_pairs, indices) bind(C, name="tgx_neighbors_build")
