@@ -127,7 +127,8 @@ def bench_mc512(args):
     per-component bound 0.3 A (move_max 0.6): plain GAP Metropolis against
     delayed acceptance with the surrogate trained on the chain's own first
     20,000 trials.  Accepted moves/s of the timed trials of each mode from
-    the same start state."""
+    the same start state, in speculative batches of 8 trials; plain
+    Metropolis also one trial per launch sequence (the same chain)."""
     n = 512
     pos, sp, lat = S.carbon_cell(n, 2.0, 1.2, 1004)
     eng = tgx.Engine(0, "fp64")
@@ -142,6 +143,14 @@ def bench_mc512(args):
                        train_every=2048, epochs=3, batch=512, lr=2e-3, max_rmse=0.2)
     t_train = time.perf_counter() - t0
     start = tr["pos"]
+    # the one-trial launch sequence first, then speculative batches (the
+    # default): the same chain, decided move for move
+    eng.set_mc_batch(1)
+    eng.mc_run(start, sp, lat, 64, T, mm, skin, 2)  # warm-up
+    t0 = time.perf_counter()
+    one = eng.mc_run(start, sp, lat, args.trials, T, mm, skin, 3)
+    t_one = time.perf_counter() - t0
+    eng.set_mc_batch(0)
     eng.mc_run(start, sp, lat, 64, T, mm, skin, 2)  # warm-up
     t0 = time.perf_counter()
     pl = eng.mc_run(start, sp, lat, args.trials, T, mm, skin, 3)
@@ -163,6 +172,8 @@ def bench_mc512(args):
                                    "moves, per-component bound 0.3 A, skin 1.2 A",
                        "atoms": n, "trials": args.trials},
             "plain_value": round(v_pl, 1), "speedup_over_plain": round(v_da / v_pl, 3),
+            "plain_one_trial_value": round(one["n_accepted"] / t_one, 1),
+            "batch_speedup_plain": round(v_pl / (one["n_accepted"] / t_one), 3),
             "acceptance_plain": acc_pl / args.trials, "acceptance_da": acc_da / args.trials,
             "stage1_rejected": da["n_stage1_rejected"],
             "gap_evaluations": da["n_gap_evaluations"],

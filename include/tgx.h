@@ -164,6 +164,12 @@ int tgx_add_core_pot(tgx_ctx* ctx, int sp1, int sp2, int n, const double* x,
 /* Drops every angle_3b and core_pot block. */
 int tgx_clear_extra_terms(tgx_ctx* ctx);
 
+/* Monte Carlo trials per speculative batch (tgx_mc_run, tgx_mc_run_da after
+ * the surrogate's training trials): 0 = automatic (8), 1 = one trial per
+ * launch sequence, at most 64.  No counterpart in the reference (its trials
+ * run one by one, mc/mc_interface.f90:183-447). */
+int tgx_set_mc_batch(tgx_ctx* ctx, int trials_per_batch);
+
 /* Hot path ----------------------------------------------------------------- */
 
 /*
@@ -296,6 +302,11 @@ int tgx_md_run(tgx_ctx* ctx, int n, int n_steps, double dt, double skin,
  * an atom leaves it; the box must be at least 2 (rcut + skin).  pos (3,n) is
  * updated in place.  e_trace (n_trials + 1, may be NULL): energy before the
  * first trial and after every trial.  n_accepted / n_rebuilds may be NULL.
+ * Trials are proposed and evaluated in speculative batches
+ * (tgx_set_mc_batch) and decided in order; a trial whose energy change an
+ * earlier accepted trial of its batch could have altered (an atom moved
+ * within 2 rcut + sqrt(3) move_max) is evaluated again in the next batch,
+ * so the decisions are those of the one-trial sequence.
  */
 int tgx_mc_run(tgx_ctx* ctx, int n, const int* species, const double* lattice,
                double* pos, int n_trials, double temp, double move_max,

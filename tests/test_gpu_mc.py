@@ -77,6 +77,7 @@ def test_mc_mixed_partial_energies_match_row_sums(eng):
     two chains agree bit for bit."""
     pos, sp, lat, model = _system()
     eng.set_precision("mixed")
+    eng.set_mc_batch(1)  # the one-trial sequence on both sides
     try:
         a = eng.mc_run(pos, sp, lat, 300, 1500.0, 0.6, 1.2, 41)
         eng.surrogate_init(model.hypers.dim, seed=4)
@@ -84,6 +85,7 @@ def test_mc_mixed_partial_energies_match_row_sums(eng):
                           train_every=1 << 20, epochs=1, batch=64, lr=1e-3, max_rmse=1.0)
     finally:
         eng.set_precision("fp64")
+        eng.set_mc_batch(0)
     print("accepted", a["n_accepted"], b["n_accepted"], flush=True)
     assert a["n_accepted"] == b["n_accepted"] and a["n_accepted"] > 50
     assert np.array_equal(a["pos"], b["pos"])

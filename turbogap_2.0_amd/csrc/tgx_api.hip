@@ -94,6 +94,18 @@ hipError_t launch_mlp_adam(int D, float* P, const float* grad, float* m, float* 
                            int step, unsigned short* W1t, unsigned short* W2t, hipStream_t st);
 hipError_t launch_mlp_cast(int D, const float* P, unsigned short* W1t, unsigned short* W2t,
                            hipStream_t st);
+hipError_t launch_mcb_propose(int n, int nb, double* pos, int* species, const double* pref,
+                              unsigned long long seed, double move_max, double lim2,
+                              int* ctl, int* mi, double* md, hipStream_t st);
+hipError_t launch_mcb_affected(int n, int nb, int cap, int vb, double* pos, int* species,
+                               int* count, int* list, int maxnb, const Cell& cell, double rsoap,
+                               int central, int* site_atom, int* vatom, int* mi, double* md,
+                               const TwoBodyDev* twob, int n2b, int exact, hipStream_t st);
+hipError_t launch_mcb_walk(int nb, int cap, double* pos, const Cell& cell, double rdep,
+                           const int* vatom, const double* esite, double* e_cur,
+                           const double* esur, double* esur_cur, unsigned long long seed,
+                           double kT, int* ctl, double* dctl, const int* mi, const double* md,
+                           double* trace, hipStream_t st);
 hipError_t launch_mc_init(int n, const int* site_of_atom, const double* esite, double* e_cur,
                           hipStream_t st);
 hipError_t launch_neighbors_general(const double* pos, int n, const GenCell& g, double rcut,
@@ -202,6 +214,8 @@ struct Extra {
   DevBuf b_cells, b_sid, b_boff, b_aoff, b_out;  // batched structures
   DevBuf mc_pref, mc_ctl, mc_dctl, mc_ecur, mc_naff, mc_trace, mc_twob;  // Monte Carlo
   DevBuf mc_esur, mc_esur_cur, mc_sq, mc_sy;                          // delayed acceptance
+  DevBuf mcb_mi, mcb_md, mcb_vatom;  // speculative batches: trial data, virtual sites' atoms
+  int mc_batch = 0;                  // trials per batch (0: automatic, 1: one at a time)
   // surrogate MLP: FP32 master parameters, Adam moments, BF16 copies, work
   DevBuf sg_P, sg_m, sg_v, sg_grad, sg_W1t, sg_W2t, sg_part, sg_z1, sg_z2, sg_e, sg_q, sg_y,
       sg_rows, sg_norm /* mu, sigma on the device */;
@@ -2137,8 +2151,10 @@ static int sg_train(tgx_ctx* c, int n_s, const float* sq, const double* sy, int 
 namespace {
 struct McRun {
   int n, cap, central, n2b;
+  int nb;  // trials per speculative batch (1: the one-trial sequence)
   Cell cell;
   double rcut_list, rsoap, move_max, lim2, kT;
+  double rdep;  // range of atoms an energy change depends on
   unsigned long long seed;
   bool da;      // surrogate stage in the trial sequence
   int cap_s;    // sample ring capacity (0: no collection)
@@ -2223,6 +2239,60 @@ static int mc_trial_launch(tgx_ctx* c, const McRun& r) {
                               x.mc_ecur.as<double>(), x.mc_esur.as<double>(),
                               x.mc_esur_cur.as<double>(), r.seed, r.kT, ctl, dctl,
                               x.mc_trace.as<double>(), c->stream));
+  return TGX_OK;
+}
+
+// One speculative batch of r.nb trials: the rebuild requested by the last
+// walk, proposals, affected sites as virtual atoms with the 2-body changes,
+// SOAP and (delayed acceptance) surrogate energies and the kernel step of
+// all virtual sites, then the walk that decides the trials in order.
+static int mc_batch_launch(tgx_ctx* c, const McRun& r) {
+  Extra& x = X(c);
+  int* ctl = x.mc_ctl.as<int>();
+  double* dctl = x.mc_dctl.as<double>();
+  const int vb = r.n + r.nb, ns = r.nb * r.cap;
+  TGX_HIP(c, launch_neighbors(c->pos.as<double>(), r.n, r.cell, r.rcut_list,
+                              c->cell_of.as<int>(), x.cell_count.as<int>(),
+                              c->cell_start.as<int>(), x.cell_fill.as<int>(),
+                              c->cell_atoms.as<int>(), c->nb_count.as<int>(), c->maxnb,
+                              c->nb_list.as<int>(), nullptr, c->nb_flag.as<int>(), c->stream,
+                              ctl + MC_REBUILD, x.mc_pref.as<double>()));
+  TGX_HIP(c, launch_mcb_propose(r.n, r.nb, c->pos.as<double>(), c->species.as<int>(),
+                                x.mc_pref.as<double>(), r.seed, r.move_max, r.lim2, ctl,
+                                x.mcb_mi.as<int>(), x.mcb_md.as<double>(), c->stream));
+  const int exact = c->precision == TGX_FP64 ? 1 : 0;
+  TGX_HIP(c, launch_mcb_affected(r.n, r.nb, r.cap, vb, c->pos.as<double>(), c->species.as<int>(),
+                                 c->nb_count.as<int>(), c->nb_list.as<int>(), c->maxnb, r.cell,
+                                 r.rsoap, r.central, c->blk->site_atom.as<int>(),
+                                 x.mcb_vatom.as<int>(), x.mcb_mi.as<int>(),
+                                 x.mcb_md.as<double>(), x.mc_twob.as<TwoBodyDev>(), r.n2b, exact,
+                                 c->stream));
+  int rcode = soap_forward(c, r.n, Geom{r.cell}, ns);
+  if (rcode) return rcode;
+  const int D = c->blk->dim;
+  if (r.da) {
+    hipError_t e =
+        c->precision == TGX_FP64
+            ? launch_mlp_forward<double>(ns, D, c->blk->q.as<double>(), c->blk->dim_pad, nullptr,
+                                         x.sg_P.as<float>(), x.sg_W1t.as<unsigned short>(),
+                                         x.sg_W2t.as<unsigned short>(), x.sg_norm.as<float>(),
+                                         x.mc_esur.as<double>(), nullptr, nullptr, ctl + MC_DA,
+                                         c->stream)
+            : launch_mlp_forward<float>(ns, D, c->blk->q.as<float>(), c->blk->dim_pad, nullptr,
+                                        x.sg_P.as<float>(), x.sg_W1t.as<unsigned short>(),
+                                        x.sg_W2t.as<unsigned short>(), x.sg_norm.as<float>(),
+                                        x.mc_esur.as<double>(), nullptr, nullptr, ctl + MC_DA,
+                                        c->stream);
+    TGX_HIP(c, e);
+  }
+  rcode = kernel_step(c, ns, false);
+  if (rcode) return rcode;
+  TGX_HIP(c, launch_mcb_walk(r.nb, r.cap, c->pos.as<double>(), r.cell, r.rdep,
+                             x.mcb_vatom.as<int>(), c->blk->esite.as<double>(),
+                             x.mc_ecur.as<double>(), x.mc_esur.as<double>(),
+                             x.mc_esur_cur.as<double>(), r.seed, r.kT, ctl, dctl,
+                             x.mcb_mi.as<int>(), x.mcb_md.as<double>(), x.mc_trace.as<double>(),
+                             c->stream));
   return TGX_OK;
 }
 
@@ -2327,6 +2397,28 @@ static int mc_run_impl(tgx_ctx* c, int n, const int* species, const double* latt
   }
   r.cap = c->maxnb + 1;
   if (r.cap > 1024) return set_err(c, TGX_ERR_UNSUPPORTED, "neighbour lists too long for MC");
+  // speculative batches need a SOAP model; the virtual atoms follow the
+  // structure in pos / species / the list, which grow to hold them
+  r.nb = c->has_soap ? (x.mc_batch > 0 ? x.mc_batch : 8) : 1;
+  r.nb = std::min(r.nb, 64);
+  r.rdep = 2.0 * rcut_all(c) + 1e-9;  // the walk adds each trial's displacement
+  if (r.nb > 1) {
+    const size_t n_ext = (size_t)n + r.nb + (size_t)r.nb * r.cap;
+    if (n_ext >= ((size_t)1 << TGX_IMG_SHIFT)) return set_err(c, TGX_ERR_ARG, "too many atoms");
+    ENSURE(c, pos, 24 * n_ext);
+    ENSURE(c, species, 4 * n_ext);
+    ENSURE(c, nb_count, 4 * n_ext);
+    ENSURE(c, nb_list, 4 * n_ext * c->maxnb);
+    if (x.mcb_mi.ensure(sizeof(int) * 4 * r.nb) || x.mcb_md.ensure(sizeof(double) * 8 * r.nb) ||
+        x.mcb_vatom.ensure(sizeof(int) * (size_t)r.nb * r.cap))
+      return set_err(c, TGX_ERR_HIP, "device allocation failed (mc batches)");
+    TGX_HIP(c, hipMemcpyAsync(c->pos.p, pos, 24 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    TGX_HIP(c, hipMemcpyAsync(c->species.p, species, 4 * (size_t)n, hipMemcpyHostToDevice,
+                              c->stream));
+    rcode = build_list(c, n, r.cell, r.rcut_list, false);
+    if (rcode) return rcode;
+  }
+  const int nsb = r.nb > 1 ? r.nb * r.cap : r.cap;  // sites of one launch sequence
   const int n_train = da ? std::min(da->n_train_trials, n_trials) : 0;
   r.cap_s = (da && n_train > 0) ? 65536 : 0;
   int n_sites = 0;
@@ -2341,16 +2433,16 @@ static int mc_run_impl(tgx_ctx* c, int n, const int* species, const double* latt
       x.mc_dctl.ensure(sizeof(double) * MC_NDCTL) || x.mc_ecur.ensure(8 * (size_t)n) ||
       x.mc_naff.ensure(sizeof(int) * 4) || x.mc_trace.ensure(8 * ((size_t)n_trials + 1)) ||
       x.mc_twob.ensure(sizeof(TwoBodyDev) * std::max(1, r.n2b)) ||
-      x.mc_esur.ensure(8 * (size_t)r.cap) || x.mc_esur_cur.ensure(8 * (size_t)n) ||
+      x.mc_esur.ensure(8 * (size_t)nsb) || x.mc_esur_cur.ensure(8 * (size_t)n) ||
       (r.cap_s && (x.mc_sq.ensure(sizeof(float) * (size_t)r.cap_s * c->blk->dim) ||
                    x.mc_sy.ensure(8 * (size_t)r.cap_s))))
     return set_err(c, TGX_ERR_HIP, "device allocation failed (mc)");
-  ENSURE(c, blk->site_atom, sizeof(int) * (size_t)std::max(n, r.cap));
+  ENSURE(c, blk->site_atom, sizeof(int) * (size_t)std::max(n, nsb));
   if (c->has_soap) {
-    // descriptor buffers for the larger of a trial's sites and all sites, so
-    // that refreshing the surrogate energies never moves a buffer the
-    // captured trial graph uses
-    const size_t rows = (size_t)std::max(n_sites, r.cap), ts = tsize(c->precision);
+    // descriptor buffers for the larger of a launch sequence's sites and
+    // all sites, so that refreshing the surrogate energies never moves a
+    // buffer the captured trial graphs use
+    const size_t rows = (size_t)std::max(n_sites, nsb), ts = tsize(c->precision);
     ENSURE(c, blk->q, ts * rows * c->blk->dim_pad);
     ENSURE(c, blk->cbuf, ts * rows * c->blk->sd.nrp * c->blk->sd.L2);
     ENSURE(c, blk->pnorm, ts * rows);
@@ -2370,14 +2462,18 @@ static int mc_run_impl(tgx_ctx* c, int n, const int* species, const double* latt
   TGX_HIP(c, hipMemsetAsync(c->nb_flag.p, 0, sizeof(int) * 4, c->stream));
   TGX_HIP(c, hipMemsetAsync(x.mc_naff.p, 0, sizeof(int) * 4, c->stream));
   TGX_HIP(c, hipMemsetAsync(c->blk->site_atom.p, 0, sizeof(int) * (size_t)r.cap, c->stream));
-  TGX_HIP(c, hipMemsetAsync(x.mc_esur.p, 0, 8 * (size_t)r.cap, c->stream));
+  TGX_HIP(c, hipMemsetAsync(x.mc_esur.p, 0, 8 * (size_t)nsb, c->stream));
   TGX_HIP(c, hipStreamSynchronize(c->stream));
-  // a dry trial (MC_END = 0: every control kernel returns at once) sizes the
-  // SOAP and GEMM buffers before anything is captured
+  // a dry trial and batch (MC_END = 0: every control kernel returns at
+  // once) size the SOAP and GEMM buffers before anything is captured
+  if (r.nb > 1) {
+    rcode = mc_batch_launch(c, r);
+    if (rcode) return rcode;
+  }
   rcode = mc_trial_launch(c, r);
   if (rcode) return rcode;
   if (r.da) {
-    rcode = sg_work(c, std::max(n, r.cap));
+    rcode = sg_work(c, std::max(n, nsb));
     if (rcode) return rcode;
   }
   std::vector<double> d(MC_NDCTL, 0.0);
@@ -2397,39 +2493,72 @@ static int mc_run_impl(tgx_ctx* c, int n, const int* species, const double* latt
   TGX_HIP(c, hipMemcpyAsync(x.mc_trace.p, &e0, 8, hipMemcpyHostToDevice, c->stream));
   TGX_HIP(c, hipStreamSynchronize(c->stream));
   const int TB = std::max(1, std::min(64, n_trials));
+  const int KB = 8;  // speculative batches per replay
   const char* ng = getenv("TGX_NO_GRAPH");
   const bool use_graph = !(ng && ng[0] == '1') && n_trials > 0;
-  hipGraph_t g = nullptr;
-  hipGraphExec_t ge = nullptr;
+  const bool batched = r.nb > 1;
+  hipGraph_t g = nullptr, gb = nullptr;
+  hipGraphExec_t ge = nullptr, gbe = nullptr;
   auto cleanup = [&]() {
     if (ge) (void)hipGraphExecDestroy(ge);
     if (g) (void)hipGraphDestroy(g);
-    ge = nullptr;
-    g = nullptr;
+    if (gbe) (void)hipGraphExecDestroy(gbe);
+    if (gb) (void)hipGraphDestroy(gb);
+    ge = gbe = nullptr;
+    g = gb = nullptr;
     x.ev_on = true;
   };
-  if (use_graph) {
+  auto capture = [&](bool batch, hipGraph_t* gr, hipGraphExec_t* ex) -> int {
     TGX_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    for (int b = 0; b < TB && !rcode; b++) rcode = mc_trial_launch(c, r);
-    hipError_t e = hipStreamEndCapture(c->stream, &g);
-    if (rcode || e != hipSuccess) {
-      cleanup();
-      return rcode ? rcode
-                   : set_err(c, TGX_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
-    }
-    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
-      cleanup();
+    int rc = TGX_OK;
+    for (int b = 0; b < (batch ? KB : TB) && !rc; b++)
+      rc = batch ? mc_batch_launch(c, r) : mc_trial_launch(c, r);
+    hipError_t e = hipStreamEndCapture(c->stream, gr);
+    if (rc) return rc;
+    if (e != hipSuccess)
+      return set_err(c, TGX_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(ex, *gr, nullptr, nullptr, 0);
+    if (e != hipSuccess)
       return set_err(c, TGX_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+    return TGX_OK;
+  };
+  if (use_graph) {
+    if (!batched || n_train > 0) rcode = capture(false, &g, &ge);
+    if (!rcode && batched) rcode = capture(true, &gb, &gbe);
+    if (rcode) {
+      cleanup();
+      return rcode;
     }
   }
-  // trials in blocks of TB (one replay each); while training data are
-  // collected, the surrogate is trained every train_every trials and the
-  // surrogate stage switched on after n_train trials
+  // trials in blocks of TB (one replay each) while training data are
+  // collected (the surrogate is trained every train_every trials and its
+  // stage switched on after n_train trials), else in speculative batches,
+  // KB per replay, until the device's trial counter reaches n_trials
   double train_rmse = 0.0;
   const int every = da ? std::max(TB, da->train_every / TB * TB) : 0;
   int done = 0, last_train = 0;
   while (done < n_trials && !rcode) {
+    const bool collecting = n_train > 0 && last_train < n_train;
+    if (batched && !collecting) {
+      if (gbe) {
+        hipError_t le = hipGraphLaunch(gbe, c->stream);
+        if (le != hipSuccess)
+          rcode = set_err(c, TGX_ERR_HIP, "graph launch: %s", hipGetErrorString(le));
+      } else {
+        for (int b = 0; b < KB && !rcode; b++) rcode = mc_batch_launch(c, r);
+      }
+      if (rcode) break;
+      int step = 0;
+      TGX_HIP(c, hipMemcpyAsync(&step, x.mc_ctl.as<int>() + MC_STEP, sizeof(int),
+                                hipMemcpyDeviceToHost, c->stream));
+      TGX_HIP(c, hipStreamSynchronize(c->stream));
+      if (step <= done) {
+        rcode = set_err(c, TGX_ERR_STATE, "Monte Carlo batches made no progress at trial %d", step);
+        break;
+      }
+      done = step;
+      continue;
+    }
     if (ge) {
       hipError_t le = hipGraphLaunch(ge, c->stream);
       if (le != hipSuccess) rcode = set_err(c, TGX_ERR_HIP, "graph launch: %s", hipGetErrorString(le));
@@ -2904,6 +3033,15 @@ int tgx_set_list_buffer(tgx_ctx* c, double buffer) {
   if (!(buffer >= 0.0) || buffer > 10.0) return set_err(c, TGX_ERR_ARG, "buffer out of range");
   c->list_buffer = buffer;
   c->maxnb = 0;  // re-sized at the next build
+  return TGX_OK;
+}
+
+int tgx_set_mc_batch(tgx_ctx* c, int trials_per_batch) {
+  if (!c) return TGX_ERR_ARG;
+  if (md_busy(c)) return TGX_ERR_STATE;
+  if (trials_per_batch < 0 || trials_per_batch > 64)
+    return set_err(c, TGX_ERR_ARG, "trials per batch must be 0..64");
+  X(c).mc_batch = trials_per_batch;
   return TGX_OK;
 }
 

@@ -78,7 +78,8 @@ __global__ void mc_propose_kernel(int n, double* pos, const double* pref,
 // species only atoms of that species are sites.
 __device__ double mc_de2b(const double* pos, const int* species, const Cell& cell, int maxnb,
                           const int* count, const int* list, const TwoBodyDev* twob, int n2b,
-                          int m, const double* dctl, int lane, int w, double* red, bool exact);
+                          int m, const double* pm, const double* po, int lane, int w, double* red,
+                          bool exact);
 
 // With WITH_2B the same workgroup then computes the 2-body change of the move
 // into dctl[MC_DE2B] (and opens the GAP gate when open_gate: plain
@@ -126,8 +127,8 @@ __global__ __launch_bounds__(256) void mc_affected_kernel(
   for (int k = t; k < cap; k += blockDim.x) aff[k] = k < nk ? keep[k] : m;
   if constexpr (WITH_2B) {
     __shared__ double red[4];
-    const double de2 = mc_de2b(pos, species, cell, maxnb, count, list, twob, n2b, m, dctl,
-                               t & 63, t >> 6, red, exact != 0);
+    const double de2 = mc_de2b(pos, species, cell, maxnb, count, list, twob, n2b, m, pos + 3 * m,
+                               dctl + MC_OLD, t & 63, t >> 6, red, exact != 0);
     if (t == 0) {
       dctl[MC_DE2B] = de2;
       if (open_gate) ctl[MC_GATE] = 1;
@@ -152,17 +153,16 @@ __device__ double mc_phi_part(const TwoBodyDev& b, double r, int k0, int k1) {
   return b.delta * b.delta * fcut * es;
 }
 
-// Exact 2-body energy change of moving atom m from dctl[MC_OLD] to pos[m]:
+// Exact 2-body energy change of moving atom m from po to pm:
 // every pair (m, j) enters E_m and E_j, so dE = 2 sum_j dphi.  The four
 // waves of the workgroup take a quarter of the sparse distances each, the
 // lanes the neighbour slots; fixed-order sum over lanes, then waves.
 __device__ double mc_de2b(const double* pos, const int* species, const Cell& cell, int maxnb,
                           const int* count, const int* list, const TwoBodyDev* twob, int n2b,
-                          int m, const double* dctl, int lane, int w, double* red, bool exact) {
+                          int m, const double* pm, const double* po, int lane, int w, double* red,
+                          bool exact) {
   double de = 0.0;
   const int cnt = min(count[m], maxnb);
-  const double* pm = pos + 3 * m;
-  const double po[3] = {dctl[MC_OLD], dctl[MC_OLD + 1], dctl[MC_OLD + 2]};
   for (int b = 0; b < n2b; b++) {
     const TwoBodyDev& B = twob[b];
     const int sm = species[m];
@@ -216,8 +216,8 @@ __global__ __launch_bounds__(256) void mc_stage1_kernel(
   // have_2b: the affected-site kernel already computed the 2-body change
   const double de2 =
       have_2b ? dctl[MC_DE2B]
-              : mc_de2b(pos, species, cell, maxnb, count, list, twob, n2b, m, dctl, lane, w, red,
-                        exact);
+              : mc_de2b(pos, species, cell, maxnb, count, list, twob, n2b, m, pos + 3 * m,
+                        dctl + MC_OLD, lane, w, red, exact);
   if (w != 0) return;
   const bool da = ctl[MC_DA] && !ctl[MC_FALLBACK];
   double ds = 0.0;
@@ -332,6 +332,302 @@ __global__ void mc_init_kernel(int n, const int* site_of_atom, const double* esi
   if (i >= n) return;
   const int s = site_of_atom[i];
   e_cur[i] = (s >= 0 && esite) ? esite[s] : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Speculative batches: nb trials proposed and evaluated at once against the
+// current state, then decided in trial order by one wave.  Trial t of a
+// batch sees the state its predecessors left only if none of the accepted
+// ones moved an atom within rdep of its moved atom (the range its energy
+// change depends on); the walk stops at the first trial for which that
+// fails, or that needs a list rebuild, and the next batch starts there,
+// with the same random numbers.  The decisions are those of the one-trial
+// sequence.
+//
+// Each trial's new descriptors come from virtual atoms: ghost n + t holds
+// the moved atom at its new position; virtual atom vb + t cap + k holds the
+// trial's k-th affected site (the moved atom at its new position, the others
+// where they are) with its list row, in which the moved atom is replaced by
+// the ghost.  The SOAP and kernel-step launches then run unchanged on the
+// virtual sites.
+// ---------------------------------------------------------------------------
+enum { MB_ATOM = 0, MB_NAFF, MB_REB, MB_LIVE, MB_NI = 4 };
+enum { MB_OLD = 0, MB_NEW = 3, MB_DE2B = 6, MB_ND = 8 };
+
+__global__ void mcb_propose_kernel(int n, int nb, double* pos, int* species, const double* pref,
+                                   unsigned long long seed, double move_max, double lim2,
+                                   int* ctl, int* mi, double* md) {
+#pragma clang fp contract(off)
+  const int t = threadIdx.x;
+  const int T = ctl[MC_STEP] + t;
+  __syncthreads();  // every lane has read the step before the flag is cleared
+  if (t == 0) ctl[MC_REBUILD] = 0;  // the batch's rebuild (if any) has run
+  if (t >= nb) return;
+  const int live = T < ctl[MC_END];
+  mi[t * MB_NI + MB_LIVE] = live;
+  mi[t * MB_NI + MB_REB] = 0;
+  if (!live) {
+    mi[t * MB_NI + MB_ATOM] = 0;
+    return;
+  }
+  int m = (int)floor(mc_uniform(seed, T, 0) * (double)n);
+  if (m > n - 1) m = n - 1;
+  mi[t * MB_NI + MB_ATOM] = m;
+  double d2 = 0.0;
+  for (int k = 0; k < 3; k++) {
+    const double disp = move_max * (mc_uniform(seed, T, 1 + k) - 0.5);
+    const double old = pos[3 * m + k];
+    const double nw = old + disp;
+    md[t * MB_ND + MB_OLD + k] = old;
+    md[t * MB_ND + MB_NEW + k] = nw;
+    pos[3 * (n + t) + k] = nw;  // ghost
+    const double dx = nw - pref[3 * m + k];
+    d2 += dx * dx;
+  }
+  species[n + t] = species[m];
+  if (d2 > lim2) mi[t * MB_NI + MB_REB] = 1;
+}
+
+// One workgroup per trial: affected sites (the moved atom, then its list
+// neighbours within rsoap of its old or new position, in slot order, as the
+// one-trial scan), the virtual atoms and rows, and the exact 2-body change.
+// Trials that do not run get isolated virtual sites (no neighbours).
+__global__ __launch_bounds__(256) void mcb_affected_kernel(
+    int n, int nb, int cap, int vb, double* pos, int* species, int* count, int* list, int maxnb,
+    Cell cell, double rsoap, int central, int* site_atom, int* vatom, int* mi, double* md,
+    const TwoBodyDev* twob, int n2b, int exact) {
+  const int t = blockIdx.x;
+  const int th = threadIdx.x;
+  __shared__ int keep[1024];
+  __shared__ int nk;
+  __shared__ double red[4];
+  const int live = mi[t * MB_NI + MB_LIVE];
+  const int m = mi[t * MB_NI + MB_ATOM];
+  const int ghost = n + t;
+  if (!live) {
+    if (th == 0) {
+      nk = 0;
+      mi[t * MB_NI + MB_NAFF] = 0;
+    }
+  } else if (th < 64) {
+    const int cnt = min(count[m], maxnb);
+    const double* po = md + t * MB_ND + MB_OLD;
+    const double* pn = md + t * MB_ND + MB_NEW;
+    int k = (central == 0 || species[m] == central) ? 1 : 0;
+    if (th == 0 && k) keep[0] = m;
+    for (int base = 0; base < cnt; base += 64) {
+      const int s = base + th;
+      bool ok = false;
+      int j = 0;
+      if (s < cnt) {
+        j = list[(size_t)m * maxnb + s] & TGX_ATOM_MASK;
+        if (central == 0 || species[j] == central) {
+          double dv[3];
+          const double rn = mic(pn, pos + 3 * j, cell.La, cell.Lb, cell.Lc, dv);
+          const double ro = mic(po, pos + 3 * j, cell.La, cell.Lb, cell.Lc, dv);
+          ok = rn < rsoap || ro < rsoap;
+        }
+      }
+      const unsigned long long b = __ballot(ok);
+      if (ok) keep[k + __popcll(b & ((1ull << th) - 1ull))] = j;
+      k += __popcll(b);
+    }
+    if (th == 0) {
+      nk = k;
+      mi[t * MB_NI + MB_NAFF] = k;
+    }
+  }
+  __syncthreads();
+  const int na = nk;
+  // virtual atoms: position, species, count; site list and real atom
+  for (int k = th; k < cap; k += blockDim.x) {
+    const int V = vb + t * cap + k;
+    int a, c;
+    const double* p;
+    if (!live || k >= na) {
+      // padding sites (computed, never summed): isolated copies
+      a = live ? m : 0;
+      c = 0;
+      p = pos + 3 * a;
+    } else {
+      a = keep[k];
+      c = min(count[a], maxnb);
+      p = a == m ? md + t * MB_ND + MB_NEW : pos + 3 * a;
+    }
+    pos[3 * V] = p[0];
+    pos[3 * V + 1] = p[1];
+    pos[3 * V + 2] = p[2];
+    species[V] = species[a];
+    count[V] = c;
+    site_atom[t * cap + k] = V;
+    vatom[t * cap + k] = a;
+  }
+  // rows of the affected sites: the real row with the moved atom replaced
+  // by the ghost; a wave per site, lanes over its slots
+  if (live) {
+    for (int k = th >> 6; k < na; k += blockDim.x >> 6) {
+      const int a = keep[k];
+      const int c = min(count[a], maxnb);
+      const int* src = list + (size_t)a * maxnb;
+      int* dst = list + (size_t)(vb + t * cap + k) * maxnb;
+      for (int s = th & 63; s < c; s += 64) {
+        const int ent = src[s];
+        dst[s] = (ent & TGX_ATOM_MASK) == m ? ghost : ent;
+      }
+    }
+  }
+  if (n2b && live) {
+    const double de2 = mc_de2b(pos, species, cell, maxnb, count, list, twob, n2b, m,
+                               md + t * MB_ND + MB_NEW, md + t * MB_ND + MB_OLD, th & 63, th >> 6,
+                               red, exact != 0);
+    if (th == 0) md[t * MB_ND + MB_DE2B] = de2;
+  } else if (th == 0) {
+    md[t * MB_ND + MB_DE2B] = 0.0;
+  }
+}
+
+// The walk: trials in order, each decided exactly as the one-trial kernels
+// decide it (same sums, same random numbers), until one is stale or needs a
+// rebuild.  One wave.
+__global__ __launch_bounds__(64) void mcb_walk_kernel(
+    int nb, int cap, double* pos, const Cell cell, double rdep, const int* vatom,
+    const double* esite, double* e_cur, const double* esur, double* esur_cur,
+    unsigned long long seed, double kT, int* ctl, double* dctl, const int* mi, const double* md,
+    double* trace) {
+  const int lane = threadIdx.x;
+  __shared__ double acc_old[64 * 3], acc_new[64 * 3];
+  __shared__ int acc_atom[64];
+  __shared__ int n_acc, take;
+  if (lane == 0) n_acc = 0;
+  __syncthreads();
+  int t = 0;
+  for (; t < nb; t++) {
+    if (!mi[t * MB_NI + MB_LIVE]) break;
+    const int m = mi[t * MB_NI + MB_ATOM];
+    const double* po = md + t * MB_ND + MB_OLD;
+    if (mi[t * MB_NI + MB_REB]) {
+      if (lane == 0) {
+        ctl[MC_REBUILD] = 1;
+        ctl[MC_NREB] += 1;
+      }
+      break;
+    }
+    // stale: an accepted trial of this batch moved an atom within rdep +
+    // |d_t| (the trial's sites lie within rcut + |d_t| of its old position,
+    // their environments within rcut of those)
+    const double* pn0 = md + t * MB_ND + MB_NEW;
+    const double dd = sqrt((pn0[0] - po[0]) * (pn0[0] - po[0]) + (pn0[1] - po[1]) * (pn0[1] - po[1]) +
+                           (pn0[2] - po[2]) * (pn0[2] - po[2]));
+    const double rt = rdep + dd, rdep2 = rt * rt;
+    bool st = false;
+    for (int a = lane; a < n_acc && !st; a += 64) {
+      double dv[3];
+      const double ro = mic(acc_old + 3 * a, po, cell.La, cell.Lb, cell.Lc, dv);
+      const double rn = mic(acc_new + 3 * a, po, cell.La, cell.Lb, cell.Lc, dv);
+      st = acc_atom[a] == m || ro * ro < rdep2 || rn * rn < rdep2;
+    }
+    if (__any(st)) break;
+    const int T = ctl[MC_STEP];
+    const int na = mi[t * MB_NI + MB_NAFF];
+    const double de2 = md[t * MB_ND + MB_DE2B];
+    const bool da = ctl[MC_DA] && !ctl[MC_FALLBACK];
+    // stage 1 (delayed acceptance): surrogate change of the sites
+    double dsur = 0.0;
+    bool gate = true;
+    if (da) {
+      double ds = 0.0;
+      for (int k = lane; k < na; k += 64) ds += esur[t * cap + k] - esur_cur[vatom[t * cap + k]];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) ds += __shfl_xor(ds, o, 64);
+      dsur = ds + de2;
+      double p = exp(-dsur / kT);
+      p = p < 1.0 ? p : 1.0;
+      gate = p > mc_uniform(seed, T, 4);
+    }
+    bool accept = false;
+    double de = 0.0;
+    if (gate) {
+      for (int k = lane; k < na; k += 64) de += esite[t * cap + k] - e_cur[vatom[t * cap + k]];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) de += __shfl_xor(de, o, 64);
+      de += de2;
+      double p;
+      if (da) {
+        const double d = de - dsur;
+        p = exp(-d / kT);
+      } else {
+        p = exp(-de / kT);
+      }
+      p = p < 1.0 ? p : 1.0;
+      accept = p > mc_uniform(seed, T, da ? 5 : 4);
+    }
+    if (lane == 0) {
+      if (!gate) {
+        ctl[MC_NREJ1] += 1;
+      } else {
+        ctl[MC_NGAP] += 1;
+        if (da) {
+          const double d = de - dsur;
+          dctl[MC_EMA] = 0.98 * dctl[MC_EMA] + 0.02 * d * d;
+          if (dctl[MC_EMA] > dctl[MC_MAXMS]) ctl[MC_FALLBACK] = 1;
+        }
+      }
+      if (accept) {
+        dctl[MC_ETOT] += de;
+        ctl[MC_NACC] += 1;
+        const double* pn = md + t * MB_ND + MB_NEW;
+        const int a = n_acc;
+        acc_atom[a] = m;
+        for (int k = 0; k < 3; k++) {
+          acc_old[3 * a + k] = po[k];
+          acc_new[3 * a + k] = pn[k];
+          pos[3 * m + k] = pn[k];
+        }
+        n_acc = a + 1;
+      }
+      if (trace) trace[T + 1] = dctl[MC_ETOT];
+      ctl[MC_STEP] = T + 1;
+      take = accept ? 1 : 0;
+    }
+    __syncthreads();
+    if (take) {
+      for (int k = lane; k < na; k += 64) e_cur[vatom[t * cap + k]] = esite[t * cap + k];
+      if (da)
+        for (int k = lane; k < na; k += 64) esur_cur[vatom[t * cap + k]] = esur[t * cap + k];
+    }
+    __syncthreads();
+  }
+}
+
+hipError_t launch_mcb_propose(int n, int nb, double* pos, int* species, const double* pref,
+                              unsigned long long seed, double move_max, double lim2,
+                              int* ctl, int* mi, double* md, hipStream_t st) {
+  if (nb > 64) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(mcb_propose_kernel, dim3(1), dim3(64), 0, st, n, nb, pos, species, pref, seed,
+                     move_max, lim2, ctl, mi, md);
+  return hipGetLastError();
+}
+
+hipError_t launch_mcb_affected(int n, int nb, int cap, int vb, double* pos, int* species,
+                               int* count, int* list, int maxnb, const Cell& cell, double rsoap,
+                               int central, int* site_atom, int* vatom, int* mi, double* md,
+                               const TwoBodyDev* twob, int n2b, int exact, hipStream_t st) {
+  if (cap > 1024 || nb > 64) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(mcb_affected_kernel, dim3(nb), dim3(256), 0, st, n, nb, cap, vb, pos, species,
+                     count, list, maxnb, cell, rsoap, central, site_atom, vatom, mi, md, twob, n2b,
+                     exact);
+  return hipGetLastError();
+}
+
+hipError_t launch_mcb_walk(int nb, int cap, double* pos, const Cell& cell, double rdep,
+                           const int* vatom, const double* esite, double* e_cur,
+                           const double* esur, double* esur_cur, unsigned long long seed,
+                           double kT, int* ctl, double* dctl, const int* mi, const double* md,
+                           double* trace, hipStream_t st) {
+  if (nb > 64) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(mcb_walk_kernel, dim3(1), dim3(64), 0, st, nb, cap, pos, cell, rdep, vatom,
+                     esite, e_cur, esur, esur_cur, seed, kT, ctl, dctl, mi, md, trace);
+  return hipGetLastError();
 }
 
 hipError_t launch_mc_propose(int n, double* pos, const double* pref, unsigned long long seed,

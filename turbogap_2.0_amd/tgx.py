@@ -283,6 +283,7 @@ def load_library(path: str = LIB_PATH):
         "tgx_add_angle_3b": (i, [P, i, i, i, d, d, dp, i, i, dp, dp, dp]),
         "tgx_add_core_pot": (i, [P, i, i, i, dp, dp, d, d]),
         "tgx_clear_extra_terms": (i, [P]),
+        "tgx_set_mc_batch": (i, [P, i]),
         "tgx_neighbors_build": (i, [P, i, dp, dp, d, i64, ip, ip, dp, dp, dp, dp,
                                     C.POINTER(i64), ip]),
         "tgx_soap_descriptors": (i, [P, i, dp, ip, dp, dp, ip, ip]),
@@ -618,6 +619,11 @@ class Engine:
         self._check(self.lib.tgx_stage_times(self.ctx, _dptr(out)))
         return dict(zip(["neighbors", "soap_fwd", "gemm", "soap_bwd", "two_body", "integrator",
                          "total"], out))
+
+    def set_mc_batch(self, trials_per_batch: int):
+        """Monte Carlo trials per speculative batch (0 automatic, 1 one at a
+        time)."""
+        self._check(self.lib.tgx_set_mc_batch(self.ctx, int(trials_per_batch)))
 
     def set_list_buffer(self, buffer: float):
         self._check(self.lib.tgx_set_list_buffer(self.ctx, float(buffer)))
