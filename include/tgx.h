@@ -164,6 +164,32 @@ int tgx_add_core_pot(tgx_ctx* ctx, int sp1, int sp2, int n, const double* x,
 /* Drops every angle_3b and core_pot block. */
 int tgx_clear_extra_terms(tgx_ctx* ctx);
 
+/* A local-property model (local_property_models of a soap_turbo block,
+ * read_gap.f90:296-380, e.g. Hirshfeld volumes) on SOAP block `block`
+ * (0-based, in the order the blocks were set and added), predicted as
+ * local_property_predict does (gap/local_properties.f90:46-146): V_i =
+ * delta^2 sum_s alphas_s K(i,s)^zeta + V0 with K = q_i . Qs_s, clipped at 0.
+ * Qs(dim, n_sparse) column-major.  *index_out (may be NULL) receives its
+ * index.  Integer zeta follows the GAP kernel's rule (|zeta - int(zeta)| <
+ * 1e-5), where the reference's property code takes |zeta - nint(zeta)| <
+ * 1e-10. */
+int tgx_add_local_property(tgx_ctx* ctx, int block, int dim, int n_sparse,
+                           const double* Qs, const double* alphas, double delta,
+                           double zeta, double V0, int* index_out);
+int tgx_clear_local_properties(tgx_ctx* ctx);
+
+/* Local property `index` of one structure (get_local_properties,
+ * gap/gap_interface.f90:300-476): V(n), 0 for atoms that are no site of its
+ * block; with V_der (may be NULL; orthorhombic cells) dV_i/dr_j (3, n_pairs)
+ * for every entry of the list tgx_neighbors_build returns at rcut_list (at
+ * least the block's cutoff), the self entry being dV_i/dr_i, all zero where
+ * V_i = 0 or atom i is no site (local_properties.f90:125-137).  cap is the
+ * capacity of V_der in pairs; TGX_ERR_CAPACITY leaves the size needed in
+ * *n_pairs. */
+int tgx_local_property(tgx_ctx* ctx, int index, int n, const double* pos,
+                       const int* species, const double* lattice, double rcut_list,
+                       double* V, int64_t cap, double* V_der, int64_t* n_pairs);
+
 /* Monte Carlo trials per speculative batch (tgx_mc_run, tgx_mc_run_da after
  * the surrogate's training trials): 0 = automatic (8), 1 = one trial per
  * launch sequence, at most 64.  No counterpart in the reference (its trials

@@ -217,6 +217,49 @@ def soap(h: SoapHypers, pos, species, lattice, derivatives=False):
                 site_n_neigh=nn[:k])
 
 
+def local_property(h: SoapHypers, Qs, alphas, delta, zeta, V0, pos, species, lattice,
+                   rcut_list=None, derivatives=False):
+    """local_property_predict (local_properties.f90:46-146) on get_soap's
+    descriptors: V_i = delta^2 sum_s alphas_s K_is^zeta + V0 clipped at 0
+    (zeta integer when |zeta - nint(zeta)| < 1e-10); with derivatives
+    V_der = Qss(i,:) . soap_cart_der(:,:,pair), zero where V_i = 0, scattered
+    onto the full list at rcut_list (get_local_properties,
+    gap_interface.f90:342-351), pairs matched by atom and xyz."""
+    pos, lat = _f64(pos), _f64(lattice)
+    sp = np.ascontiguousarray(species, np.int32)
+    n = len(pos)
+    s = soap(h, pos, sp, lat, derivatives=derivatives)
+    q = s["soap"]
+    Qs, alphas = _f64(Qs), _f64(alphas)
+    K = q @ Qs.T
+    zi = int(round(zeta))
+    integer = abs(zi - zeta) < 1e-10
+    Kz = K ** zi if integer else K ** zeta
+    Vs = delta ** 2 * (Kz @ alphas) + V0
+    Vs = np.where(Vs < 0.0, 0.0, Vs)
+    V = np.zeros(n)
+    V[s["site_atom"]] = Vs
+    if not derivatives:
+        return V, None
+    Kd = zi * K ** (zi - 1) if integer else zeta * K ** (zeta - 1.0)
+    Qss = delta ** 2 * (Kd * alphas[None, :]) @ Qs
+    full = neighbors(pos, lat, rcut_list)
+    off = np.concatenate([[0], np.cumsum(full["n_neigh"])])
+    lookup = {}
+    for i in range(n):
+        for k in range(off[i], off[i + 1]):
+            lookup[(i, (full["list"][k] - 1) % n, tuple(full["xyz"][k]))] = k
+    der = np.zeros((len(full["list"]), 3))
+    soff = np.concatenate([[0], np.cumsum(s["site_n_neigh"])])
+    for si, i in enumerate(s["site_atom"]):
+        if Vs[si] == 0.0:
+            continue
+        for k in range(soff[si], soff[si + 1]):
+            key = (int(i), int(s["pair_atom"][k]), tuple(s["pair_xyz"][k]))
+            der[lookup[key]] = Qss[si] @ s["soap_der"][k]
+    return V, der
+
+
 def gap_kernel_step(soap_, Qs, alphas, delta, zeta):
     """gap.f90:118-171: energies and Qss (n_sites, dim)."""
     q = _f64(soap_)

@@ -1171,3 +1171,49 @@ def test_angle_3b_species_filter_and_invariance():
     assert np.abs(e2 - e[perm]).max() < 1e-12
     assert np.abs(f2 - f[perm] @ A.T).max() < 1e-11
     assert np.abs(v2 - A @ v @ A.T).max() < 1e-10
+
+
+# --------------------------------------------------------------------------
+# local-property models
+# --------------------------------------------------------------------------
+def _lp_model(h, q, seed=5, n_sparse=60):
+    rng = np.random.default_rng(seed)
+    Qs = q[rng.choice(len(q), n_sparse, replace=len(q) < n_sparse)]
+    return Qs, rng.normal(0.0, 0.5, n_sparse)
+
+
+def test_local_property_values_and_derivatives():
+    """V equals the C kernel step's site energies plus V0, clipped at 0;
+    dV_i/dr_j summed over a unit atom's entries equals the central finite
+    difference of V_i in that atom's position (images included: small cell);
+    entries of clipped sites are zero."""
+    pos, sp, lat = S.carbon_cell(24, 2.6, 1.2, 61)  # 8.1 A box: supercell images
+    h = S.carbon_hypers()
+    q = oracle.soap(h, pos, sp, lat)["soap"]
+    Qs, al = _lp_model(h, q)
+    e, _ = oracle.gap_kernel_step(q, Qs, al, 0.9, 3.0)
+    V0 = -float(np.median(e))  # about half the sites clip
+    V, der = oracle.local_property(h, Qs, al, 0.9, 3.0, V0, pos, sp, lat, 4.2, derivatives=True)
+    assert np.allclose(V, np.maximum(e + V0, 0.0), rtol=0, atol=1e-13)
+    assert 0 < np.sum(V == 0.0) < len(V)
+    full = oracle.neighbors(pos, lat, 4.2)
+    off = np.concatenate([[0], np.cumsum(full["n_neigh"])])
+    hh = 1e-5
+    for i in range(len(pos)):
+        rows = der[off[i]:off[i + 1]]
+        if V[i] == 0.0:
+            assert np.all(rows == 0.0)
+            continue
+        if i % 3:
+            continue
+        units = (full["list"][off[i]:off[i + 1]] - 1) % len(pos)
+        for j in sorted(set(units.tolist()))[:4]:
+            an = rows[units == j].sum(0)
+            for a in range(3):
+                vs = []
+                for sgn in (1, -1):
+                    p = pos.copy()
+                    p[j, a] += sgn * hh
+                    vs.append(oracle.local_property(h, Qs, al, 0.9, 3.0, V0, p, sp, lat)[0][i])
+                fd = (vs[0] - vs[1]) / (2 * hh)
+                assert abs(fd - an[a]) < 1e-6 * max(1.0, abs(an[a])), (i, j, a, fd, an[a])

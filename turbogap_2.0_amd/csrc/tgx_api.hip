@@ -215,6 +215,11 @@ struct Extra {
   DevBuf mc_pref, mc_ctl, mc_dctl, mc_ecur, mc_naff, mc_trace, mc_twob;  // Monte Carlo
   DevBuf mc_esur, mc_esur_cur, mc_sq, mc_sy;                          // delayed acceptance
   DevBuf mcb_mi, mcb_md, mcb_vatom;  // speculative batches: trial data, virtual sites' atoms
+  // local-property models: a block each (the descriptor block's hypers, the
+  // property's sparse set), its descriptor block and V0
+  std::vector<std::unique_ptr<SoapBlock>> lp;
+  std::vector<int> lp_block;
+  std::vector<double> lp_v0;
   int mc_batch = 0;                  // trials per batch (0: automatic, 1: one at a time)
   // surrogate MLP: FP32 master parameters, Adam moments, BF16 copies, work
   DevBuf sg_P, sg_m, sg_v, sg_grad, sg_W1t, sg_W2t, sg_part, sg_z1, sg_z2, sg_e, sg_q, sg_y,
@@ -431,12 +436,26 @@ static int launch_extra_terms(tgx_ctx* c, int n, const Geom& cell, int do_forces
 // *n_sites receives the first block's count (each block keeps its own).
 // With image atoms (n_ext > n) site_of_atom also covers them: the site of
 // image J is that of atom J % n.
+static int setup_block_sites(tgx_ctx* c, SoapBlock* bp, int n, const int* species_host,
+                             int n_ext);
+
 static int setup_sites(tgx_ctx* c, int n, const int* species_host, int* n_sites,
                        int n_ext = 0) {
+  for (auto& bp : c->blocks) {
+    const int rcode = setup_block_sites(c, bp.get(), n, species_host, n_ext);
+    if (rcode) return rcode;
+  }
+  *n_sites = c->blocks[0]->n_sites;
+  return TGX_OK;
+}
+
+// Sites of one block (see setup_sites).
+static int setup_block_sites(tgx_ctx* c, SoapBlock* bp, int n, const int* species_host,
+                             int n_ext) {
   SoapBlock* const keep = c->blk;
   if (n_ext < n) n_ext = n;
-  for (auto& bp : c->blocks) {
-    c->blk = bp.get();
+  {
+    c->blk = bp;
     SoapBlock& b = *bp;
     ENSURE(c, blk->site_of_atom, sizeof(int) * n_ext);
     ENSURE(c, blk->site_atom, sizeof(int) * n);
@@ -460,7 +479,6 @@ static int setup_sites(tgx_ctx* c, int n, const int* species_host, int* n_sites,
     b.n_sites = (int)sa.size();
   }
   c->blk = keep;
-  *n_sites = c->blocks[0]->n_sites;
   return TGX_OK;
 }
 
@@ -1199,6 +1217,7 @@ int tgx_destroy(tgx_ctx* c) {
     if (b.tab) (void)hipFree(b.tab);
   }
   (void)tgx_clear_extra_terms(c);
+  for (auto& b : f->x.lp) b->release();
   if (f->x.ev_ready) {
     for (int k = 0; k < 8; k++) (void)hipEventDestroy(f->x.ev[k]);
     for (int k = 0; k < 8; k++) (void)hipEventDestroy(f->x.kev[k]);
@@ -3034,6 +3053,176 @@ int tgx_set_list_buffer(tgx_ctx* c, double buffer) {
   c->list_buffer = buffer;
   c->maxnb = 0;  // re-sized at the next build
   return TGX_OK;
+}
+
+int tgx_add_local_property(tgx_ctx* c, int block, int dim, int n_sparse, const double* Qs,
+                           const double* alphas, double delta, double zeta, double V0,
+                           int* index_out) {
+  if (!c) return TGX_ERR_ARG;
+  if (md_busy(c)) return TGX_ERR_STATE;
+  if (!c->has_soap || block < 0 || block >= (int)c->blocks.size())
+    return set_err(c, TGX_ERR_ARG, "no SOAP block %d", block);
+  Extra& x = X(c);
+  std::unique_ptr<SoapBlock> b(new (std::nothrow) SoapBlock());
+  if (!b) return set_err(c, TGX_ERR_ARG, "out of host memory");
+  SoapBlock* const keep = c->blk;
+  c->blk = b.get();
+  const int rcode = load_soap_block(c, &c->blocks[block]->hyp, dim, n_sparse, Qs, alphas, delta,
+                                    zeta);
+  c->blk = keep;
+  if (rcode) {
+    b->release();
+    return rcode;
+  }
+  x.lp.push_back(std::move(b));
+  x.lp_block.push_back(block);
+  x.lp_v0.push_back(V0);
+  if (index_out) *index_out = (int)x.lp.size() - 1;
+  return TGX_OK;
+}
+
+int tgx_clear_local_properties(tgx_ctx* c) {
+  if (!c) return TGX_ERR_ARG;
+  if (md_busy(c)) return TGX_ERR_STATE;
+  Extra& x = X(c);
+  (void)hipSetDevice(c->device);
+  for (auto& b : x.lp) b->release();
+  x.lp.clear();
+  x.lp_block.clear();
+  x.lp_v0.clear();
+  return TGX_OK;
+}
+
+// local_property_predict (gap/local_properties.f90:46-146) on the device:
+// the block's descriptors, the property's kernel step V = delta^2 sum_s
+// alphas_s K^zeta + V0 clipped at 0, and with V_der the adjoint's per-slot
+// gradients dV_i/dr_j (the self entry minus their sum), in the order of the
+// list tgx_neighbors_build returns at rcut_list.
+static int local_property_impl(tgx_ctx* c, int index, int n, const double* pos,
+                               const int* species, const double* lattice, double rcut_list,
+                               double* V, int64_t cap, double* V_der, int64_t* n_pairs) {
+  Extra& x = X(c);
+  SoapBlock* P = x.lp[index].get();
+  const double rc = P->sd.rcut_max;
+  if (V_der && !(rcut_list >= rc))
+    return set_err(c, TGX_ERR_ARG, "rcut_list %g below the descriptor cutoff %g", rcut_list, rc);
+  int rcode = upload_structure(c, n, pos, species);
+  if (rcode) return rcode;
+  Cell cell;
+  std::string err;
+  int n_ext = n;
+  const bool general = !is_orthorhombic(lattice);
+  if (general && V_der)
+    return set_err(c, TGX_ERR_UNSUPPORTED, "V_der needs an orthorhombic cell");
+  const double rl = V_der ? rcut_list : rc;
+  if (general) {
+    rcode = prepare_image_cell(c, n, lattice, rl, false, &cell, &n_ext);
+  } else {
+    rcode = make_cell(lattice, rl, &cell, &err);
+    if (rcode) return set_err(c, rcode, "%s", err.c_str());
+    rcode = build_list(c, n, cell, rl, false);
+  }
+  if (rcode) return rcode;
+  rcode = setup_block_sites(c, P, n, species, n_ext);
+  if (rcode) return rcode;
+  c->blk = P;
+  const int ns = P->n_sites;
+  const int prec = c->precision;
+  if (ns > 0) {
+    rcode = soap_forward(c, n, Geom{cell}, ns);
+    if (!rcode) rcode = kernel_step(c, ns, V_der != nullptr);
+    if (rcode) return rcode;
+    if (V_der) {
+      const size_t ts = tsize(prec);
+      ENSURE(c, gpair, ts * 4 * (size_t)n * c->maxnb);
+      ENSURE(c, blk->vsite, 48 * (size_t)ns);
+      hipError_t e;
+      if (prec == TGX_FP64)
+        e = launch_soap_bwd<double>(
+            P->sd, P->sd_dev.as<SoapDev>(), P->W_dev.as<double>(), P->rtab.as<float>(),
+            c->pos.as<double>(), c->species.as<int>(), Geom{cell}, c->maxnb,
+            c->nb_count.as<int>(), c->nb_list.as<int>(), P->site_atom.as<int>(), ns,
+            P->pairn.as<int>(), P->q.as<double>(), P->dim_pad, P->cbuf.as<double>(), P->pnorm.as<double>(),
+            P->gmat.as<double>(), P->dim_pad, c->gpair.as<double>(), P->vsite.as<double>(),
+            nullptr, 0, c->eatom.as<double>(), c->stream, 0);
+      else
+        e = launch_soap_bwd<float>(
+            P->sd, P->sd_dev.as<SoapDev>(), P->W_dev.as<double>(), P->rtab.as<float>(),
+            c->pos.as<double>(), c->species.as<int>(), Geom{cell}, c->maxnb,
+            c->nb_count.as<int>(), c->nb_list.as<int>(), P->site_atom.as<int>(), ns,
+            P->pairn.as<int>(), P->q.as<float>(), P->dim_pad, P->cbuf.as<float>(), P->pnorm.as<float>(),
+            P->gmat.as<float>(), P->dim_pad, c->gpair.as<float>(), P->vsite.as<double>(),
+            nullptr, 0, c->eatom.as<double>(), c->stream, 0);
+      TGX_HIP(c, e);
+    }
+  }
+  TGX_HIP(c, hipStreamSynchronize(c->stream));
+  std::vector<double> es(ns);
+  std::vector<int> sa(ns);
+  if (ns > 0) {
+    TGX_HIP(c, hipMemcpy(es.data(), P->esite.p, 8 * (size_t)ns, hipMemcpyDeviceToHost));
+    TGX_HIP(c, hipMemcpy(sa.data(), P->site_atom.p, 4 * (size_t)ns, hipMemcpyDeviceToHost));
+  }
+  std::vector<double> vv(n, 0.0);
+  std::vector<char> live(n, 0);
+  for (int s = 0; s < ns; s++) {
+    const double v = es[s] + x.lp_v0[index];
+    vv[sa[s]] = v < 0.0 ? 0.0 : v;
+    live[sa[s]] = vv[sa[s]] != 0.0;
+  }
+  std::memcpy(V, vv.data(), 8 * (size_t)n);
+  if (!V_der) return TGX_OK;
+  std::vector<int> cnt(n);
+  TGX_HIP(c, hipMemcpy(cnt.data(), c->nb_count.p, 4 * (size_t)n, hipMemcpyDeviceToHost));
+  std::vector<long long> off(n + 1, 0);
+  for (int i = 0; i < n; i++) off[i + 1] = off[i] + std::min(cnt[i], c->maxnb) + 1;
+  *n_pairs = off[n];
+  if (off[n] > cap)
+    return set_err(c, TGX_ERR_CAPACITY, "need %lld pairs, cap %lld", off[n], (long long)cap);
+  const size_t slots = (size_t)n * c->maxnb;
+  std::vector<double> g(4 * slots, 0.0);
+  if (ns > 0) {
+    if (prec == TGX_FP64) {
+      TGX_HIP(c, hipMemcpy(g.data(), c->gpair.p, 32 * slots, hipMemcpyDeviceToHost));
+    } else {
+      std::vector<float> gf(4 * slots);
+      TGX_HIP(c, hipMemcpy(gf.data(), c->gpair.p, 16 * slots, hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < gf.size(); k++) g[k] = gf[k];
+    }
+  }
+  for (int i = 0; i < n; i++) {
+    double* row = V_der + 3 * off[i];
+    const int m = (int)(off[i + 1] - off[i]) - 1;
+    for (int k = 0; k < 3 * (m + 1); k++) row[k] = 0.0;
+    if (!live[i]) continue;
+    for (int s = 0; s < m; s++)
+      for (int k = 0; k < 3; k++) {
+        const double v = g[4 * ((size_t)i * c->maxnb + s) + k];
+        row[3 * (s + 1) + k] = v;
+        row[k] -= v;
+      }
+  }
+  return TGX_OK;
+}
+
+int tgx_local_property(tgx_ctx* c, int index, int n, const double* pos, const int* species,
+                       const double* lattice, double rcut_list, double* V, int64_t cap,
+                       double* V_der, int64_t* n_pairs) {
+  if (!c) return TGX_ERR_ARG;
+  if (md_busy(c)) return TGX_ERR_STATE;
+  Extra& x = X(c);
+  if (index < 0 || index >= (int)x.lp.size())
+    return set_err(c, TGX_ERR_ARG, "no local property %d", index);
+  if (n < 1 || !pos || !species || !lattice || !V || (V_der && !n_pairs))
+    return set_err(c, TGX_ERR_ARG, "bad arguments");
+  (void)hipSetDevice(c->device);
+  const int saved = c->maxnb;
+  c->maxnb = 0;
+  const int rcode = local_property_impl(c, index, n, pos, species, lattice, rcut_list, V, cap,
+                                        V_der, n_pairs);
+  c->maxnb = saved;
+  c->blk = c->blocks[0].get();
+  return rcode;
 }
 
 int tgx_set_mc_batch(tgx_ctx* c, int trials_per_batch) {

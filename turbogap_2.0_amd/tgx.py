@@ -284,6 +284,9 @@ def load_library(path: str = LIB_PATH):
         "tgx_add_core_pot": (i, [P, i, i, i, dp, dp, d, d]),
         "tgx_clear_extra_terms": (i, [P]),
         "tgx_set_mc_batch": (i, [P, i]),
+        "tgx_add_local_property": (i, [P, i, i, i, dp, dp, d, d, d, ip]),
+        "tgx_clear_local_properties": (i, [P]),
+        "tgx_local_property": (i, [P, i, i, dp, ip, dp, d, dp, i64, dp, C.POINTER(i64)]),
         "tgx_neighbors_build": (i, [P, i, dp, dp, d, i64, ip, ip, dp, dp, dp, dp,
                                     C.POINTER(i64), ip]),
         "tgx_soap_descriptors": (i, [P, i, dp, ip, dp, dp, ip, ip]),
@@ -619,6 +622,45 @@ class Engine:
         self._check(self.lib.tgx_stage_times(self.ctx, _dptr(out)))
         return dict(zip(["neighbors", "soap_fwd", "gemm", "soap_bwd", "two_body", "integrator",
                          "total"], out))
+
+    def add_local_property(self, block: int, Qs, alphas, delta: float, zeta: float,
+                           V0: float = 0.0) -> int:
+        """A local-property model (e.g. Hirshfeld volumes) on SOAP block
+        `block`; Qs (n_sparse, dim).  Returns its index."""
+        q, a = _f64(Qs), _f64(alphas)
+        idx = C.c_int(-1)
+        self._check(self.lib.tgx_add_local_property(self.ctx, block, q.shape[1], q.shape[0],
+                                                    _dptr(q), _dptr(a), delta, zeta, V0,
+                                                    C.byref(idx)))
+        return idx.value
+
+    def clear_local_properties(self):
+        self._check(self.lib.tgx_clear_local_properties(self.ctx))
+
+    def local_property(self, index: int, pos, species, lattice, rcut_list: float = 0.0,
+                       derivatives: bool = False):
+        """V (n,) and, with derivatives, dV_i/dr_j (n_pairs, 3) in the order
+        of neighbors(pos, lattice, rcut_list)."""
+        pos, lat, sp = _f64(pos), _f64(lattice), _i32(species)
+        n = pos.shape[0]
+        V = np.zeros(n)
+        if not derivatives:
+            self._check(self.lib.tgx_local_property(self.ctx, index, n, _dptr(pos), _iptr(sp),
+                                                    _dptr(lat), rcut_list, _dptr(V), 0, None,
+                                                    None))
+            return V, None
+        need = C.c_int64(0)
+        cap = max(64 * n, 1024)
+        while True:
+            der = np.zeros((cap, 3))
+            rc = self.lib.tgx_local_property(self.ctx, index, n, _dptr(pos), _iptr(sp),
+                                             _dptr(lat), rcut_list, _dptr(V), cap, _dptr(der),
+                                             C.byref(need))
+            if rc == TGX_ERR_CAPACITY:
+                cap = int(need.value)
+                continue
+            self._check(rc)
+            return V, der[: need.value]
 
     def set_mc_batch(self, trials_per_batch: int):
         """Monte Carlo trials per speculative batch (0 automatic, 1 one at a
