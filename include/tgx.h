@@ -190,6 +190,33 @@ int tgx_local_property(tgx_ctx* ctx, int index, int n, const double* pos,
                        const int* species, const double* lattice, double rcut_list,
                        double* V, int64_t cap, double* V_der, int64_t* n_pairs);
 
+/* Tkatchenko-Scheffler dispersion (vdw_type "ts"; get_vdw_energy_and_forces
+ * and get_ts_energy_and_forces, vdw/vdw_interface.f90:40-385) on the
+ * Hirshfeld volumes of local property `local_property`.  Per species s
+ * (1-based, s-1 in the arrays): free-atom C6, R0 and alpha0 (eV A^6, A,
+ * A^3; get_vdw_ref_params :388-466 holds the reference's defaults for H, C
+ * and P).  The cutoffs and damping are vdw_t's (vdw/vdw_types.f90:32-48;
+ * defaults rcut 10, buffer 1, rcut_inner 0.5, buffer_inner 0.5, sR 0.94,
+ * d 20).  Replaces an earlier setting; tgx_clear_local_properties clears
+ * it. */
+int tgx_set_vdw_ts(tgx_ctx* ctx, int local_property, int n_species, const double* c6_ref,
+                   const double* r0_ref, const double* alpha0_ref, double rcut,
+                   double buffer, double rcut_inner, double buffer_inner, double sR,
+                   double d);
+int tgx_clear_vdw(tgx_ctx* ctx);
+
+/* The TS term of one structure, as get_vdw_energy_and_forces returns it:
+ * energies(n) = -1/2 sum_j C6_ij f_damp r^-6 (switched in the buffers) over
+ * rcut_inner < r_ij < rcut; with do_forces, forces(3,n) and virial(3,3)
+ * (overwritten; not touched otherwise), the pair force plus the Hirshfeld
+ * term dE/dV_i dV_i/dr_j over the list entries of i with rcut_inner < r <
+ * rcut (so not its own entry, r = 0, as in the reference, :345-363).  Cells
+ * of any shape.  Species with zero references contribute nothing (the
+ * reference's 0/0 there would be NaN). */
+int tgx_vdw_energy_forces(tgx_ctx* ctx, int n, const double* pos, const int* species,
+                          const double* lattice, int do_forces, double* energies,
+                          double* forces, double* virial);
+
 /* Monte Carlo trials per speculative batch (tgx_mc_run, tgx_mc_run_da after
  * the surrogate's training trials): 0 = automatic (8), 1 = one trial per
  * launch sequence, at most 64.  No counterpart in the reference (its trials

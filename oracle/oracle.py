@@ -245,18 +245,24 @@ def local_property(h: SoapHypers, Qs, alphas, delta, zeta, V0, pos, species, lat
     Qss = delta ** 2 * (Kd * alphas[None, :]) @ Qs
     full = neighbors(pos, lat, rcut_list)
     off = np.concatenate([[0], np.cumsum(full["n_neigh"])])
+    # entries are matched by (i, j) and the vector; the vectors of the two
+    # lists agree to rounding (a list at a larger cutoff may use a larger
+    # supercell), images of one j are far apart
     lookup = {}
     for i in range(n):
         for k in range(off[i], off[i + 1]):
-            lookup[(i, (full["list"][k] - 1) % n, tuple(full["xyz"][k]))] = k
+            lookup.setdefault((i, (full["list"][k] - 1) % n), []).append(k)
     der = np.zeros((len(full["list"]), 3))
     soff = np.concatenate([[0], np.cumsum(s["site_n_neigh"])])
     for si, i in enumerate(s["site_atom"]):
         if Vs[si] == 0.0:
             continue
         for k in range(soff[si], soff[si + 1]):
-            key = (int(i), int(s["pair_atom"][k]), tuple(s["pair_xyz"][k]))
-            der[lookup[key]] = Qss[si] @ s["soap_der"][k]
+            cand = lookup[(int(i), int(s["pair_atom"][k]))]
+            dist = [np.abs(full["xyz"][kk] - s["pair_xyz"][k]).max() for kk in cand]
+            kk = cand[int(np.argmin(dist))]
+            assert min(dist) < 1e-9
+            der[kk] = Qss[si] @ s["soap_der"][k]
     return V, der
 
 
@@ -392,6 +398,42 @@ def gap_2b(b, pos, species, lattice, do_forces=True):
     sp = np.ascontiguousarray(species, np.int32)
     lib().orc_gap_2b_ef(C.byref(arr), n, _i(sp), _i(nb["n_neigh"]), _i(nb["list"]),
                         _d(nb["rjs"]), _d(_f64(nb["xyz"])), int(do_forces), _d(e), _d(f), _d(v))
+    return e, f, v.T.copy()
+
+
+class CTs(C.Structure):
+    _fields_ = [("n_species", C.c_int32), ("pad_", C.c_int32),
+                ("c6_ref", C.POINTER(C.c_double)), ("r0_ref", C.POINTER(C.c_double)),
+                ("alpha0_ref", C.POINTER(C.c_double)), ("rcut", C.c_double),
+                ("buffer", C.c_double), ("rcut_inner", C.c_double),
+                ("buffer_inner", C.c_double), ("sR", C.c_double), ("d", C.c_double)]
+
+
+def ts_vdw(c6_ref, r0_ref, alpha0_ref, pos, species, lattice, V, V_der, rcut=10.0, buffer=1.0,
+           rcut_inner=0.5, buffer_inner=0.5, sR=0.94, d=20.0, rcut_list=None, do_forces=True):
+    """get_ts_energy_and_forces (vdw/vdw_interface.f90:127-385) on the full
+    list at rcut_list (default rcut).  V (n,) Hirshfeld volumes, V_der
+    (n_pairs, 3) dV_i/dr_j in that list's order (local_property with the same
+    rcut_list), or None when do_forces is false.  Returns energies, forces
+    (n, 3) and the virial (3, 3)."""
+    pos, lat = _f64(pos), _f64(lattice)
+    sp = np.ascontiguousarray(species, np.int32)
+    n = len(pos)
+    nb = neighbors(pos, lat, rcut if rcut_list is None else rcut_list)
+    c6, r0, a0 = _f64(c6_ref), _f64(r0_ref), _f64(alpha0_ref)
+    t = CTs()
+    t.n_species = len(c6)
+    t.c6_ref, t.r0_ref, t.alpha0_ref = _d(c6), _d(r0), _d(a0)
+    t.rcut, t.buffer, t.rcut_inner, t.buffer_inner, t.sR, t.d = (
+        rcut, buffer, rcut_inner, buffer_inner, sR, d)
+    V = _f64(V)
+    der = _f64(np.zeros((len(nb["list"]), 3)) if V_der is None else V_der)
+    assert der.shape == (len(nb["list"]), 3)
+    e = np.zeros(n)
+    f = np.zeros((n, 3))
+    v = np.zeros((3, 3))
+    lib().orc_ts_ef(C.byref(t), n, _i(sp), _i(nb["n_neigh"]), _i(nb["list"]), _d(nb["rjs"]),
+                    _d(_f64(nb["xyz"])), _d(V), _d(der), int(do_forces), _d(e), _d(f), _d(v))
     return e, f, v.T.copy()
 
 

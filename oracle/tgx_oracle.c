@@ -1554,6 +1554,120 @@ void orc_angle_3b_ef(const orc_3b* b, int n, const int* species, const int* n_ne
 }
 
 /* ------------------------------------------------------------------------ */
+/* Tkatchenko-Scheffler dispersion                                          */
+/* ------------------------------------------------------------------------ */
+
+/* get_ts_energy_and_forces (vdw/vdw_interface.f90:127-385) with its
+ * conventions kept:
+ *  - per pair k of row i (j >= 2): C6_ij = 2 C6_i C6_j / (a_j/a_i C6_i +
+ *    a_i/a_j C6_j) from C6 = c6_ref V^2, a = alpha0_ref V, R0_ij = R0_i + R0_j
+ *    with R0 = r0_ref V^(1/3) (:235-276); zero when either C6 is zero;
+ *  - r^-6 times the cubic switch 1 - 3 x^2 + 2 x^3 in the outer buffer
+ *    (rcut - buffer, rcut) and the inner one (rcut_inner, rcut_inner +
+ *    buffer_inner) (:212-226); its derivative is r6' of the unswitched
+ *    r^-6 plus r^-6 times the switch's slope (:199-211);
+ *  - E_i = -1/2 sum_j C6_ij r6 f_damp over rcut_inner < r < rcut, f_damp =
+ *    1/(1 + exp(-d (r/(sR R0_ij) - 1))) (:287-300);
+ *  - the Hirshfeld term: dE_i/dV_i prefactors (:314-339) times dV_i/dr_j
+ *    for every entry of row i with rcut_inner < r < rcut, so the self entry
+ *    (r = 0) takes no part (:345-363), virial + 1/2 sym(F x);
+ *  - the pair force on i from each entry (j not i itself), virial - 1/4
+ *    sym(F x) (:365-382).
+ * The pair parameters are formed for pairs where they take part only, so
+ * species with zero references yield zeros where the reference's 0/0 would
+ * give NaN. */
+void orc_ts_ef(const orc_ts* t, int n, const int* species, const int* n_neigh, const int* list,
+               const double* rjs, const double* xyz, const double* V, const double* V_der,
+               int do_forces, double* energies, double* forces, double* virial) {
+  const double rc = t->rcut, b = t->buffer, rci = t->rcut_inner, bi = t->buffer_inner;
+  double* pref = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+  if (do_forces) {
+    memset(forces, 0, sizeof(double) * 3 * (size_t)n);
+    memset(virial, 0, sizeof(double) * 9);
+  }
+  int64_t k = 0;
+  for (int i = 0; i < n; i++) {
+    const int si = species[i] - 1;
+    const double vi = V[i];
+    const double c6i = t->c6_ref[si] * vi * vi;
+    const double r0i = t->r0_ref[si] * pow(vi, 1.0 / 3.0);
+    const double ai = t->alpha0_ref[si] * vi;
+    double e = 0.0, p1 = 0.0, p2 = 0.0;
+    for (int jj = 1; jj < n_neigh[i]; jj++) {
+      const int64_t kk = k + jj;
+      const double r = rjs[kk];
+      if (!(r > rci && r < rc)) continue;
+      const int j = (list[kk] - 1) % n;
+      const int sj = species[j] - 1;
+      const double vj = V[j];
+      const double c6j = t->c6_ref[sj] * vj * vj;
+      const double aj = t->alpha0_ref[sj] * vj;
+      const double c6ij =
+          (c6i == 0.0 || c6j == 0.0) ? 0.0 : 2.0 * c6i * c6j / (aj / ai * c6i + ai / aj * c6j);
+      const double r0ij = r0i + t->r0_ref[sj] * pow(vj, 1.0 / 3.0);
+      const double r2 = r * r;
+      double r6 = 1.0 / (r2 * r2 * r2);
+      double r6d = -6.0 / r * r6;
+      if ((b > 0.0 || bi > 0.0) && ((r > rc - b && r < rc) || (r < rci + bi && r > rci))) {
+        double x, dsw;
+        if (r <= rci + bi) {
+          x = (rci + bi - r) / bi;
+          dsw = 6.0 * r6 / (-bi) * (-x + x * x);
+        } else {
+          x = (r - rc + b) / b;
+          dsw = 6.0 * r6 / b * (-x + x * x);
+        }
+        r6d += dsw;
+        r6 *= 1.0 - 3.0 * x * x + 2.0 * x * x * x;
+      }
+      if (c6ij == 0.0) continue;
+      const double ed = exp(-t->d * (r / (t->sR * r0ij) - 1.0));
+      const double fd = 1.0 / (1.0 + ed);
+      e += c6ij * r6 * fd;
+      if (!do_forces) continue;
+      p1 += c6ij * fd * r6;
+      p2 -= c6ij * fd * fd * r * r6 * ed * t->d / t->sR / (r0ij * r0ij);
+      const double s = c6ij / r * fd * (-r6d - r6 * fd * ed * t->d / t->sR / r0ij);
+      const double f[3] = {s * xyz[3 * kk], s * xyz[3 * kk + 1], s * xyz[3 * kk + 2]};
+      if (j != i) {
+        forces[3 * i] += f[0];
+        forces[3 * i + 1] += f[1];
+        forces[3 * i + 2] += f[2];
+      }
+      for (int a = 0; a < 3; a++)
+        for (int c = 0; c < 3; c++)
+          virial[3 * c + a] -= 0.25 * (f[a] * xyz[3 * kk + c] + f[c] * xyz[3 * kk + a]);
+    }
+    energies[i] = -0.5 * e;
+    if (do_forces)
+      pref[i] = vi == 0.0 ? 0.0 : p1 / vi + p2 * t->r0_ref[si] / 3.0 / pow(vi, 2.0 / 3.0);
+    k += n_neigh[i];
+  }
+  if (do_forces) {
+    k = 0;
+    for (int i = 0; i < n; i++) {
+      for (int jj = 0; jj < n_neigh[i]; jj++) {
+        const int64_t kk = k + jj;
+        const double r = rjs[kk];
+        const double* g = V_der + 3 * kk;
+        if (!(r > rci && r < rc) || pref[i] == 0.0) continue;
+        if (g[0] == 0.0 && g[1] == 0.0 && g[2] == 0.0) continue;
+        const int j = (list[kk] - 1) % n;
+        const double f[3] = {g[0] * pref[i], g[1] * pref[i], g[2] * pref[i]};
+        forces[3 * j] += f[0];
+        forces[3 * j + 1] += f[1];
+        forces[3 * j + 2] += f[2];
+        for (int a = 0; a < 3; a++)
+          for (int c = 0; c < 3; c++)
+            virial[3 * c + a] += 0.5 * (f[a] * xyz[3 * kk + c] + f[c] * xyz[3 * kk + a]);
+      }
+      k += n_neigh[i];
+    }
+  }
+  free(pref);
+}
+
+/* ------------------------------------------------------------------------ */
 /* Whole path                                                               */
 /* ------------------------------------------------------------------------ */
 

@@ -164,6 +164,25 @@ void orc_angle_3b_ef(const orc_3b* b, int n, const int* species, const int* n_ne
                      const int* list, const double* rjs, const double* xyz, int do_forces,
                      double* energies, double* forces, double* virial);
 
+/* Tkatchenko-Scheffler dispersion parameters (vdw_t, vdw/vdw_types.f90:32-48):
+ * per-species free-atom references (1-based species index s -> [s-1]). */
+typedef struct {
+  int n_species;
+  const double* c6_ref;
+  const double* r0_ref;
+  const double* alpha0_ref;
+  double rcut, buffer, rcut_inner, buffer_inner, sR, d;
+} orc_ts;
+
+/* get_ts_energy_and_forces (vdw/vdw_interface.f90:127-385) on a full list in
+ * the reference's layout (self first, 1-based supercell ids folded modulo n):
+ * V (n) the Hirshfeld volumes per atom, V_der (3 per list entry) dV_i/dr_j.
+ * energies (n) are overwritten; forces (3n) and the virial are overwritten
+ * when do_forces. */
+void orc_ts_ef(const orc_ts* t, int n, const int* species, const int* n_neigh, const int* list,
+               const double* rjs, const double* xyz, const double* V, const double* V_der,
+               int do_forces, double* energies, double* forces, double* virial);
+
 int orc_energy_forces(const orc_model* m, int n, const double* pos,
                       const int* species, const double* lat, int do_forces,
                       int chunk_sites, double* e_atom, double* forces,

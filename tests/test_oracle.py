@@ -1217,3 +1217,181 @@ def test_local_property_values_and_derivatives():
                     vs.append(oracle.local_property(h, Qs, al, 0.9, 3.0, V0, p, sp, lat)[0][i])
                 fd = (vs[0] - vs[1]) / (2 * hh)
                 assert abs(fd - an[a]) < 1e-6 * max(1.0, abs(an[a])), (i, j, a, fd, an[a])
+
+
+# --------------------------------------------------------------------------
+# Tkatchenko-Scheffler dispersion
+# --------------------------------------------------------------------------
+HARTREE, BOHR = 27.211386024367243, 0.5291772105638411
+# free-atom references (C6 eV A^6, R0 A, alpha0 A^3): the reference's
+# defaults for C and H (get_vdw_ref_params); O from the TS tables
+TS_REF = np.array([[46.6 * HARTREE * BOHR ** 6, 1.900, 12.0 * BOHR ** 3],
+                   [6.5 * HARTREE * BOHR ** 6, 1.64, 4.5 * BOHR ** 3],
+                   [15.6 * HARTREE * BOHR ** 6, 1.71, 5.4 * BOHR ** 3]])
+
+
+def _ts_pair_energy(r, sa, sb, Va, Vb, sR=0.94, d=20.0, mpm=False):
+    """Total TS energy of one pair (Tkatchenko & Scheffler, PRL 102, 073005
+    (2009)): -f_damp(r) C6_AB / r^6 with the combination rule C6_AB =
+    2 C6_A C6_B / (a_B/a_A C6_A + a_A/a_B C6_B) and R0_AB = R0_A + R0_B, the
+    free-atom references scaled by V^2 (C6), V (alpha) and V^(1/3) (R0)."""
+    ex = mp.exp if mpm else math.exp
+    c6a, r0a, aa = TS_REF[sa - 1]
+    c6b, r0b, ab = TS_REF[sb - 1]
+    C6A, C6B = c6a * Va ** 2, c6b * Vb ** 2
+    aA, aB = aa * Va, ab * Vb
+    C6 = 2 * C6A * C6B / (aB / aA * C6A + aA / aB * C6B)
+    R0 = r0a * Va ** (1.0 / 3.0) + r0b * Vb ** (1.0 / 3.0)
+    return -C6 / (1 + ex(-d * (r / (sR * R0) - 1))) / r ** 6, C6
+
+
+def _ts(pos, sp, lat, V, der=None, do_forces=True, **kw):
+    return oracle.ts_vdw(TS_REF[:, 0], TS_REF[:, 1], TS_REF[:, 2], pos, sp, lat, V, der,
+                         do_forces=do_forces, **kw)
+
+
+@pytest.mark.parametrize("sa,sb,r,Va,Vb", [(1, 1, 3.0, 0.8, 0.8), (1, 2, 4.5, 0.7, 1.1),
+                                           (2, 3, 7.3, 0.9, 0.6), (1, 1, 1.9, 1.2, 0.5)])
+def test_ts_dimer_closed_form(sa, sb, r, Va, Vb):
+    """Energy of a dimer outside the switching buffers equals the published
+    TS pair energy, half on each atom; with fixed volumes the force on A is
+    E'(r) along the bond (mpmath derivative of the closed form)."""
+    rng = np.random.default_rng(int(100 * r))
+    u = rng.normal(size=3)
+    u /= np.linalg.norm(u)
+    pos = np.array([[20.0, 20.0, 20.0], [20.0, 20.0, 20.0] + r * u])
+    sp = np.array([sa, sb], np.int32)
+    lat = np.diag([40.0] * 3).reshape(9)
+    V = np.array([Va, Vb])
+    e, f, v = _ts(pos, sp, lat, V)
+    E, _ = _ts_pair_energy(r, sa, sb, Va, Vb)
+    assert np.allclose(e, [E / 2, E / 2], rtol=1e-13, atol=0)
+    mp.mp.dps = 30
+    dE = float(mp.diff(lambda t: _ts_pair_energy(t, sa, sb, Va, Vb, mpm=True)[0], mp.mpf(r)))
+    assert np.allclose(f[0], dE * u, rtol=1e-10, atol=1e-14)
+    assert np.allclose(f[1], -dE * u, rtol=1e-10, atol=1e-14)
+    # virial of a pair force: -F_A (x) d
+    assert np.allclose(v, -np.outer(f[0], r * u) * 1.0, rtol=1e-10, atol=1e-14)
+
+
+def test_ts_buffers_switch_and_reference_slope():
+    """In the buffers the energy is the pair energy times 1 - 3x^2 + 2x^3
+    (x from 0 at the buffer's inner edge to 1 at the cutoff); the force uses
+    the reference's slope, the exact derivative plus 6 C6 f_damp (1 - sw)
+    / r^7 along the bond (vdw_interface.f90:199-211 differentiates the
+    unswitched r^-6)."""
+    lat = np.diag([40.0] * 3).reshape(9)
+    sp = np.array([1, 2], np.int32)
+    V = np.array([0.9, 1.05])
+    kw = dict(rcut=10.0, buffer=1.0, rcut_inner=0.5, buffer_inner=0.5)
+
+    def sw(x):
+        return 1 - 3 * x ** 2 + 2 * x ** 3
+
+    def exact(t, mpm=False):
+        E, C6 = _ts_pair_energy(t, 1, 2, V[0], V[1], mpm=mpm)
+        x = (t - 9.0) / 1.0 if t > 5 else (1.0 - t) / 0.5
+        return E * sw(x), E, C6, sw(x)
+
+    mp.mp.dps = 30
+    for r in (9.4, 9.97, 0.8, 0.55):
+        pos = np.array([[20.0, 20.0, 20.0], [20.0 + r, 20.0, 20.0]])
+        e, f, _ = _ts(pos, sp, lat, V, **kw)
+        Es, E, C6, s = exact(r)
+        assert abs(e.sum() - Es) < 1e-13 * abs(E)
+        dEs = float(mp.diff(lambda t: exact(t, mpm=True)[0], mp.mpf(r)))
+        fd = 1.0 / (1.0 + math.exp(-20.0 * (r / (0.94 * (TS_REF[0, 1] * V[0] ** (1 / 3) +
+                                                         TS_REF[1, 1] * V[1] ** (1 / 3))) - 1)))
+        slope = dEs + 6.0 * C6 * fd * (1.0 - s) / r ** 7
+        assert abs(f[0, 0] - slope) < 1e-9 * max(abs(slope), abs(dEs)), (r, f[0, 0], slope, dEs)
+        assert abs(slope - dEs) > 1e-6 * abs(dEs)  # the convention is visible here
+    # beyond the cutoff and inside rcut_inner: nothing
+    for r in (10.0, 10.3, 0.5, 0.3):
+        pos = np.array([[20.0, 20.0, 20.0], [20.0 + r, 20.0, 20.0]])
+        e, f, v = _ts(pos, sp, lat, V, **kw)
+        assert np.all(e == 0) and np.all(f == 0) and np.all(v == 0)
+
+
+def test_ts_fixed_volumes_forces_and_virial_fd():
+    """Fixed volumes (no Hirshfeld term), no buffers: forces = -dE/dr and
+    virial = -dE/d(strain) in all nine components on a C/H/O cell."""
+    pos, sp, lat = S.cho_cell(40, 1.5, 1.0, 5)
+    V = np.random.default_rng(2).uniform(0.5, 1.2, len(pos))
+    kw = dict(rcut=5.0, buffer=0.0, rcut_inner=0.5, buffer_inner=0.0)
+    e, f, v = _ts(pos, sp, lat, V, **kw)
+    assert e.sum() < 0
+    h = 1e-5
+    for i in (0, 9, 27):
+        for a in range(3):
+            Es = []
+            for s in (1, -1):
+                p = pos.copy()
+                p[i, a] += s * h
+                Es.append(_ts(p, sp, lat, V, do_forces=False, **kw)[0].sum())
+            fdv = -(Es[0] - Es[1]) / (2 * h)
+            assert abs(fdv - f[i, a]) < 1e-7 * max(1.0, abs(f[i, a])), (i, a, fdv, f[i, a])
+    for a in range(3):
+        for b in range(3):
+            Es = []
+            for s in (1, -1):
+                eps = np.zeros((3, 3))
+                eps[a, b] = s * h
+                p, l = _strained(pos, lat, eps)
+                Es.append(_ts(p, sp, l, V, do_forces=False, **kw)[0].sum())
+            fdv = -(Es[0] - Es[1]) / (2 * h)
+            assert abs(fdv - v[a, b]) < 1e-7 * max(1.0, abs(v[a, b])), (a, b, fdv, v[a, b])
+
+
+def test_ts_hirshfeld_term_fd():
+    """Volumes from a local-property model (supercell images): the TS force
+    plus the self entry's term the reference leaves out, (-dE/dV_a)
+    dV_a/dr_a with dE/dV_a by finite differences of the energy in V, equals
+    -dE/dr_a of the full energy E(V(r), r); the virial (the self entry's
+    vector is zero) equals -dE/d(strain)."""
+    pos, sp, lat = S.carbon_cell(16, 2.0, 1.2, 7)
+    h = S.carbon_hypers()
+    q = oracle.soap(h, pos, sp, lat)["soap"]
+    Qs, al = _lp_model(h, q, n_sparse=16)
+    al = 0.05 * al
+    rl = 4.5
+    kw = dict(rcut=rl, buffer=0.0, rcut_inner=0.5, buffer_inner=0.0)
+
+    def vol(p, l):
+        return oracle.local_property(h, Qs, al, 0.9, 3.0, 1.0, p, sp, l)[0]
+
+    def energy(p, l):
+        return _ts(p, sp, l, vol(p, l), do_forces=False, **kw)[0].sum()
+
+    V, der = oracle.local_property(h, Qs, al, 0.9, 3.0, 1.0, pos, sp, lat, rl, derivatives=True)
+    assert np.all(V > 0.5)
+    e, f, v = _ts(pos, sp, lat, V, der, **kw)
+    off = np.concatenate([[0], np.cumsum(oracle.neighbors(pos, lat, rl)["n_neigh"])])
+    hh = 1e-5
+    for a_ in (0, 5, 11):
+        Es = []
+        for s in (1, -1):
+            Vp = V.copy()
+            Vp[a_] += s * hh
+            Es.append(_ts(pos, sp, lat, Vp, do_forces=False, **kw)[0].sum())
+        mdEdV = -(Es[0] - Es[1]) / (2 * hh)
+        self_term = mdEdV * der[off[a_]]
+        for a in range(3):
+            Es = []
+            for s in (1, -1):
+                p = pos.copy()
+                p[a_, a] += s * hh
+                Es.append(energy(p, lat))
+            fdv = -(Es[0] - Es[1]) / (2 * hh)
+            assert abs(fdv - (f[a_, a] + self_term[a])) < 1e-7 * max(1.0, abs(fdv)), \
+                (a_, a, fdv, f[a_, a], self_term[a])
+        assert np.abs(self_term).max() > 1e-4
+    for a in range(3):
+        for b in range(3):
+            Es = []
+            for s in (1, -1):
+                eps = np.zeros((3, 3))
+                eps[a, b] = s * hh
+                p, l = _strained(pos, lat, eps)
+                Es.append(energy(p, l))
+            fdv = -(Es[0] - Es[1]) / (2 * hh)
+            assert abs(fdv - v[a, b]) < 1e-7 * max(1.0, abs(v[a, b])), (a, b, fdv, v[a, b])

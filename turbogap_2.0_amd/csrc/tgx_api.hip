@@ -128,6 +128,16 @@ hipError_t launch_angle_3b(const Angle3bDev& b, const double* pos, const int* sp
                            const Geom& g, int maxnb, const int* count, const int* list,
                            double* eatom, double* vatom, double* g3, int do_forces,
                            hipStream_t st);
+hipError_t launch_ts_volume(int n, const int* site_of_atom, const double* esite, double v0,
+                            double* V, hipStream_t st);
+hipError_t launch_ts_pairs(const TsDev& t, const double* pos, const int* species, int n,
+                           const Geom& g, int maxnb, const int* count, const int* list,
+                           const double* V, double* eatom, double* forces, double* vatom,
+                           double* pref, int do_forces, hipStream_t st);
+template <typename T>
+hipError_t launch_ts_gather(const TsDev& t, const double* pos, int n, const Geom& g, int maxnb,
+                            const int* count, const int* list, const int* rev, const T* gpair,
+                            const double* pref, double* forces, double* vatom, hipStream_t st);
 hipError_t launch_site_add(int n_sites, const int* site_atom, const double* esite,
                            const double* vsite, double* eatom, double* vatom, hipStream_t st);
 hipError_t launch_energy_init(int n, const int* species, const double* e0,
@@ -220,6 +230,10 @@ struct Extra {
   std::vector<std::unique_ptr<SoapBlock>> lp;
   std::vector<int> lp_block;
   std::vector<double> lp_v0;
+  // Tkatchenko-Scheffler dispersion on local property ts_lp (-1: not set)
+  int ts_lp = -1;
+  TsDev ts{};
+  DevBuf ts_V, ts_pref, ts_vatom;
   int mc_batch = 0;                  // trials per batch (0: automatic, 1: one at a time)
   // surrogate MLP: FP32 master parameters, Adam moments, BF16 copies, work
   DevBuf sg_P, sg_m, sg_v, sg_grad, sg_W1t, sg_W2t, sg_part, sg_z1, sg_z2, sg_e, sg_q, sg_y,
@@ -1209,7 +1223,9 @@ int tgx_destroy(tgx_ctx* c) {
                     &f->x.sg_W1t, &f->x.sg_W2t, &f->x.sg_part, &f->x.sg_z1, &f->x.sg_z2,
                     &f->x.sg_e, &f->x.sg_q, &f->x.sg_y, &f->x.sg_rows, &f->x.sg_norm,
                     &f->x.tune_save, &f->x.md.pref, &f->x.md.ke, &f->x.md.ctl, &f->x.md.ep,
-                    &f->x.md.ek, &f->x.md.vout, &f->x.md.kesum, &f->x.g_pos, &f->x.g_sp, &f->x.g3};
+                    &f->x.md.ek, &f->x.md.vout, &f->x.md.kesum, &f->x.g_pos, &f->x.g_sp, &f->x.g3,
+                    &f->x.mcb_mi, &f->x.mcb_md, &f->x.mcb_vatom, &f->x.ts_V, &f->x.ts_pref,
+                    &f->x.ts_vatom};
   for (DevBuf* b : bufs) b->release();
   for (auto& b : c->twob) {
     if (b.Qs) (void)hipFree(b.Qs);
@@ -3090,6 +3106,35 @@ int tgx_clear_local_properties(tgx_ctx* c) {
   x.lp.clear();
   x.lp_block.clear();
   x.lp_v0.clear();
+  x.ts_lp = -1;  // the vdW term read one of them
+  return TGX_OK;
+}
+
+// The property's adjoint on block P (its kernel step already run with the
+// gradient): dV_i/dr_j of every slot of every site row into c->gpair (zero
+// beyond the descriptor cutoff), rows of atoms that are no site untouched.
+static int lp_adjoint(tgx_ctx* c, SoapBlock* P, const Geom& cell, int n, int ns) {
+  const int prec = c->precision;
+  ENSURE(c, gpair, tsize(prec) * 4 * (size_t)n * c->maxnb);
+  ENSURE(c, blk->vsite, 48 * (size_t)ns);
+  hipError_t e;
+  if (prec == TGX_FP64)
+    e = launch_soap_bwd<double>(
+        P->sd, P->sd_dev.as<SoapDev>(), P->W_dev.as<double>(), P->rtab.as<float>(),
+        c->pos.as<double>(), c->species.as<int>(), cell, c->maxnb, c->nb_count.as<int>(),
+        c->nb_list.as<int>(), P->site_atom.as<int>(), ns, P->pairn.as<int>(), P->q.as<double>(),
+        P->dim_pad, P->cbuf.as<double>(), P->pnorm.as<double>(), P->gmat.as<double>(),
+        P->dim_pad, c->gpair.as<double>(), P->vsite.as<double>(), nullptr, 0,
+        c->eatom.as<double>(), c->stream, 0);
+  else
+    e = launch_soap_bwd<float>(
+        P->sd, P->sd_dev.as<SoapDev>(), P->W_dev.as<double>(), P->rtab.as<float>(),
+        c->pos.as<double>(), c->species.as<int>(), cell, c->maxnb, c->nb_count.as<int>(),
+        c->nb_list.as<int>(), P->site_atom.as<int>(), ns, P->pairn.as<int>(), P->q.as<float>(),
+        P->dim_pad, P->cbuf.as<float>(), P->pnorm.as<float>(), P->gmat.as<float>(), P->dim_pad,
+        c->gpair.as<float>(), P->vsite.as<double>(), nullptr, 0, c->eatom.as<double>(),
+        c->stream, 0);
+  TGX_HIP(c, e);
   return TGX_OK;
 }
 
@@ -3132,29 +3177,8 @@ static int local_property_impl(tgx_ctx* c, int index, int n, const double* pos,
     rcode = soap_forward(c, n, Geom{cell}, ns);
     if (!rcode) rcode = kernel_step(c, ns, V_der != nullptr);
     if (rcode) return rcode;
-    if (V_der) {
-      const size_t ts = tsize(prec);
-      ENSURE(c, gpair, ts * 4 * (size_t)n * c->maxnb);
-      ENSURE(c, blk->vsite, 48 * (size_t)ns);
-      hipError_t e;
-      if (prec == TGX_FP64)
-        e = launch_soap_bwd<double>(
-            P->sd, P->sd_dev.as<SoapDev>(), P->W_dev.as<double>(), P->rtab.as<float>(),
-            c->pos.as<double>(), c->species.as<int>(), Geom{cell}, c->maxnb,
-            c->nb_count.as<int>(), c->nb_list.as<int>(), P->site_atom.as<int>(), ns,
-            P->pairn.as<int>(), P->q.as<double>(), P->dim_pad, P->cbuf.as<double>(), P->pnorm.as<double>(),
-            P->gmat.as<double>(), P->dim_pad, c->gpair.as<double>(), P->vsite.as<double>(),
-            nullptr, 0, c->eatom.as<double>(), c->stream, 0);
-      else
-        e = launch_soap_bwd<float>(
-            P->sd, P->sd_dev.as<SoapDev>(), P->W_dev.as<double>(), P->rtab.as<float>(),
-            c->pos.as<double>(), c->species.as<int>(), Geom{cell}, c->maxnb,
-            c->nb_count.as<int>(), c->nb_list.as<int>(), P->site_atom.as<int>(), ns,
-            P->pairn.as<int>(), P->q.as<float>(), P->dim_pad, P->cbuf.as<float>(), P->pnorm.as<float>(),
-            P->gmat.as<float>(), P->dim_pad, c->gpair.as<float>(), P->vsite.as<double>(),
-            nullptr, 0, c->eatom.as<double>(), c->stream, 0);
-      TGX_HIP(c, e);
-    }
+    if (V_der) rcode = lp_adjoint(c, P, Geom{cell}, n, ns);
+    if (rcode) return rcode;
   }
   TGX_HIP(c, hipStreamSynchronize(c->stream));
   std::vector<double> es(ns);
@@ -3220,6 +3244,155 @@ int tgx_local_property(tgx_ctx* c, int index, int n, const double* pos, const in
   c->maxnb = 0;
   const int rcode = local_property_impl(c, index, n, pos, species, lattice, rcut_list, V, cap,
                                         V_der, n_pairs);
+  c->maxnb = saved;
+  c->blk = c->blocks[0].get();
+  return rcode;
+}
+
+// get_vdw_energy_and_forces / get_ts_energy_and_forces
+// (vdw/vdw_interface.f90:40-385) on the device: one list at max(rcut, the
+// property's descriptor cutoff), the property's descriptors, kernel step
+// and (with forces) adjoint, then the volume, pair and Hirshfeld-gather
+// kernels.  Energies per atom, forces and the virial in FP64.
+static int vdw_ts_impl(tgx_ctx* c, int n, const double* pos, const int* species,
+                       const double* lattice, int do_forces, double* energies, double* forces,
+                       double* virial) {
+  Extra& x = X(c);
+  SoapBlock* P = x.lp[x.ts_lp].get();
+  const double rl = std::max(x.ts.rcut, P->sd.rcut_max);
+  int rcode = upload_structure(c, n, pos, species);
+  if (rcode) return rcode;
+  Cell cell;
+  int n_ext = n;
+  if (!is_orthorhombic(lattice)) {
+    rcode = prepare_image_cell(c, n, lattice, rl, do_forces != 0, &cell, &n_ext);
+  } else {
+    std::string err;
+    rcode = make_cell(lattice, rl, &cell, &err);
+    if (rcode) return set_err(c, rcode, "%s", err.c_str());
+    if ((long long)n * cell.nrep >= (1LL << TGX_IMG_SHIFT))
+      return set_err(c, TGX_ERR_ARG, "too many atoms");
+    if (cell.nrep >= (1 << (31 - TGX_IMG_SHIFT)))
+      return set_err(c, TGX_ERR_UNSUPPORTED, "box too small for the cutoff (%d images)",
+                     cell.nrep);
+    rcode = build_list(c, n, cell, rl, do_forces != 0);
+  }
+  if (rcode) return rcode;
+  rcode = setup_block_sites(c, P, n, species, n_ext);
+  if (rcode) return rcode;
+  c->blk = P;
+  const int ns = P->n_sites;
+  const Geom geo{cell};
+  if (ns > 0) {
+    rcode = soap_forward(c, n, geo, ns);
+    if (!rcode) rcode = kernel_step(c, ns, do_forces != 0);
+    if (!rcode && do_forces) rcode = lp_adjoint(c, P, geo, n, ns);
+    if (rcode) return rcode;
+  }
+  if (grow(c, x.ts_V, 8 * (size_t)n) || grow(c, x.ts_pref, 8 * (size_t)n) ||
+      grow(c, x.ts_vatom, 48 * (size_t)n))
+    return grow_err(c, 64 * (size_t)n, "vdW buffers");
+  ENSURE(c, eatom, 8 * (size_t)n);
+  ENSURE(c, forces, 24 * (size_t)n);
+  if (ns > 0)
+    TGX_HIP(c, launch_ts_volume(n, P->site_of_atom.as<int>(), P->esite.as<double>(),
+                                x.lp_v0[x.ts_lp], x.ts_V.as<double>(), c->stream));
+  else
+    TGX_HIP(c, hipMemsetAsync(x.ts_V.p, 0, 8 * (size_t)n, c->stream));
+  TGX_HIP(c, launch_ts_pairs(x.ts, c->pos.as<double>(), c->species.as<int>(), n, geo, c->maxnb,
+                             c->nb_count.as<int>(), c->nb_list.as<int>(), x.ts_V.as<double>(),
+                             c->eatom.as<double>(), c->forces.as<double>(),
+                             x.ts_vatom.as<double>(), x.ts_pref.as<double>(), do_forces,
+                             c->stream));
+  if (do_forces && ns > 0) {
+    if (c->precision == TGX_FP64)
+      TGX_HIP(c, launch_ts_gather<double>(x.ts, c->pos.as<double>(), n, geo, c->maxnb,
+                                          c->nb_count.as<int>(), c->nb_list.as<int>(),
+                                          c->nb_rev.as<int>(), c->gpair.as<double>(),
+                                          x.ts_pref.as<double>(), c->forces.as<double>(),
+                                          x.ts_vatom.as<double>(), c->stream));
+    else
+      TGX_HIP(c, launch_ts_gather<float>(x.ts, c->pos.as<double>(), n, geo, c->maxnb,
+                                         c->nb_count.as<int>(), c->nb_list.as<int>(),
+                                         c->nb_rev.as<int>(), c->gpair.as<float>(),
+                                         x.ts_pref.as<double>(), c->forces.as<double>(),
+                                         x.ts_vatom.as<double>(), c->stream));
+  }
+  TGX_HIP(c, hipStreamSynchronize(c->stream));
+  TGX_HIP(c, hipMemcpy(energies, c->eatom.p, 8 * (size_t)n, hipMemcpyDeviceToHost));
+  if (!do_forces) return TGX_OK;
+  TGX_HIP(c, hipMemcpy(forces, c->forces.p, 24 * (size_t)n, hipMemcpyDeviceToHost));
+  std::vector<double> va(6 * (size_t)n);
+  TGX_HIP(c, hipMemcpy(va.data(), x.ts_vatom.p, 48 * (size_t)n, hipMemcpyDeviceToHost));
+  double sv[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k < 6; k++) sv[k] += va[6 * (size_t)i + k];
+  virial[0] = sv[0];
+  virial[4] = sv[1];
+  virial[8] = sv[2];
+  virial[1] = virial[3] = sv[3];
+  virial[2] = virial[6] = sv[4];
+  virial[5] = virial[7] = sv[5];
+  return TGX_OK;
+}
+
+int tgx_set_vdw_ts(tgx_ctx* c, int local_property, int n_species, const double* c6_ref,
+                   const double* r0_ref, const double* alpha0_ref, double rcut, double buffer,
+                   double rcut_inner, double buffer_inner, double sR, double d) {
+  if (!c) return TGX_ERR_ARG;
+  if (md_busy(c)) return TGX_ERR_STATE;
+  Extra& x = X(c);
+  if (local_property < 0 || local_property >= (int)x.lp.size())
+    return set_err(c, TGX_ERR_ARG, "no local property %d", local_property);
+  if (n_species < 1 || n_species > TGX_MAX_SPECIES || !c6_ref || !r0_ref || !alpha0_ref)
+    return set_err(c, TGX_ERR_ARG, "bad species references");
+  TsDev t{};
+  for (int s = 0; s < n_species; s++) {
+    if (!std::isfinite(c6_ref[s]) || !std::isfinite(r0_ref[s]) || !std::isfinite(alpha0_ref[s]))
+      return set_err(c, TGX_ERR_ARG, "reference of species %d is not finite", s + 1);
+    t.c6[s] = c6_ref[s];
+    t.r0[s] = r0_ref[s];
+    t.a0[s] = alpha0_ref[s];
+  }
+  if (!(rcut > 0.0) || !std::isfinite(rcut) || !(buffer >= 0.0) || !(buffer <= rcut) ||
+      !(rcut_inner >= 0.0) || !(buffer_inner >= 0.0) || !std::isfinite(buffer_inner) ||
+      !(rcut_inner < rcut) || !(sR > 0.0) || !std::isfinite(sR) || !std::isfinite(d))
+    return set_err(c, TGX_ERR_ARG, "bad vdW cutoffs or damping parameters");
+  t.n_species = n_species;
+  t.rcut = rcut;
+  t.buffer = buffer;
+  t.rcut_inner = rcut_inner;
+  t.buffer_inner = buffer_inner;
+  t.sR = sR;
+  t.d = d;
+  x.ts = t;
+  x.ts_lp = local_property;
+  return TGX_OK;
+}
+
+int tgx_clear_vdw(tgx_ctx* c) {
+  if (!c) return TGX_ERR_ARG;
+  if (md_busy(c)) return TGX_ERR_STATE;
+  X(c).ts_lp = -1;
+  return TGX_OK;
+}
+
+int tgx_vdw_energy_forces(tgx_ctx* c, int n, const double* pos, const int* species,
+                          const double* lattice, int do_forces, double* energies,
+                          double* forces, double* virial) {
+  if (!c) return TGX_ERR_ARG;
+  if (md_busy(c)) return TGX_ERR_STATE;
+  Extra& x = X(c);
+  if (x.ts_lp < 0) return set_err(c, TGX_ERR_STATE, "no vdW model set");
+  if (n < 1 || !pos || !species || !lattice || !energies || (do_forces && (!forces || !virial)))
+    return set_err(c, TGX_ERR_ARG, "bad arguments");
+  for (int i = 0; i < n; i++)
+    if (species[i] < 1 || species[i] > x.ts.n_species)
+      return set_err(c, TGX_ERR_ARG, "species[%d] = %d has no vdW reference", i, species[i]);
+  (void)hipSetDevice(c->device);
+  const int saved = c->maxnb;
+  c->maxnb = 0;
+  const int rcode = vdw_ts_impl(c, n, pos, species, lattice, do_forces, energies, forces, virial);
   c->maxnb = saved;
   c->blk = c->blocks[0].get();
   return rcode;

@@ -115,6 +115,14 @@ struct CorePotDev {
   double* d2;  // device
 };
 
+// Tkatchenko-Scheffler dispersion: per-species free-atom references and the
+// cutoffs of the pair sum (vdw_t, vdw/vdw_types.f90:32-48).
+struct TsDev {
+  double c6[TGX_MAX_SPECIES], r0[TGX_MAX_SPECIES], a0[TGX_MAX_SPECIES];
+  double rcut, buffer, rcut_inner, buffer_inner, sR, d;
+  int n_species;
+};
+
 // Monte Carlo control words on the device (int and double arrays).
 enum {
   MC_REBUILD = 0,  // list rebuild requested by this trial

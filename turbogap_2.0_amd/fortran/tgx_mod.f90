@@ -178,6 +178,63 @@ module tgx_mod
        type(c_ptr), value :: ctx
      end function tgx_clear_extra_terms
 
+     ! local-property models (read_gap.f90:296-380, local_properties.f90:46-146)
+     integer(c_int) function tgx_add_local_property(ctx, block, dim, n_sparse, Qs, alphas, &
+          delta, zeta, V0, index_out) bind(C, name="tgx_add_local_property")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: block, dim, n_sparse
+       real(c_double), intent(in) :: Qs(*), alphas(*)
+       real(c_double), value :: delta, zeta, V0
+       integer(c_int) :: index_out
+     end function tgx_add_local_property
+
+     integer(c_int) function tgx_clear_local_properties(ctx) &
+          bind(C, name="tgx_clear_local_properties")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function tgx_clear_local_properties
+
+     integer(c_int) function tgx_local_property(ctx, idx, n, pos, species, lattice, &
+          rcut_list, V, cap, V_der, n_pairs) bind(C, name="tgx_local_property")
+       import :: c_int, c_ptr, c_double, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: idx, n
+       real(c_double), intent(in) :: pos(*), lattice(*)
+       integer(c_int), intent(in) :: species(*)
+       real(c_double), value :: rcut_list
+       real(c_double) :: V(*)
+       integer(c_int64_t), value :: cap
+       type(c_ptr), value :: V_der
+       type(c_ptr), value :: n_pairs
+     end function tgx_local_property
+
+     ! Tkatchenko-Scheffler dispersion (vdw/vdw_interface.f90:40-385)
+     integer(c_int) function tgx_set_vdw_ts(ctx, local_property, n_species, c6_ref, r0_ref, &
+          alpha0_ref, rcut, buffer, rcut_inner, buffer_inner, sR, d) &
+          bind(C, name="tgx_set_vdw_ts")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: local_property, n_species
+       real(c_double), intent(in) :: c6_ref(*), r0_ref(*), alpha0_ref(*)
+       real(c_double), value :: rcut, buffer, rcut_inner, buffer_inner, sR, d
+     end function tgx_set_vdw_ts
+
+     integer(c_int) function tgx_clear_vdw(ctx) bind(C, name="tgx_clear_vdw")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function tgx_clear_vdw
+
+     integer(c_int) function tgx_vdw_energy_forces(ctx, n, pos, species, lattice, do_forces, &
+          energies, forces, virial) bind(C, name="tgx_vdw_energy_forces")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: n, do_forces
+       real(c_double), intent(in) :: pos(*), lattice(*)
+       integer(c_int), intent(in) :: species(*)
+       real(c_double) :: energies(*), forces(*), virial(*)
+     end function tgx_vdw_energy_forces
+
      integer(c_int) function tgx_neighbors_build(ctx, n, pos, lattice, rcut, cap, n_neigh, &
           list, rjs, xyz, thetas, phis, n_pairs, indices) bind(C, name="tgx_neighbors_build")
        import :: c_int, c_ptr, c_double, c_int64_t

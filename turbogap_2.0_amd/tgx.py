@@ -286,6 +286,9 @@ def load_library(path: str = LIB_PATH):
         "tgx_set_mc_batch": (i, [P, i]),
         "tgx_add_local_property": (i, [P, i, i, i, dp, dp, d, d, d, ip]),
         "tgx_clear_local_properties": (i, [P]),
+        "tgx_set_vdw_ts": (i, [P, i, i, dp, dp, dp, d, d, d, d, d, d]),
+        "tgx_clear_vdw": (i, [P]),
+        "tgx_vdw_energy_forces": (i, [P, i, dp, ip, dp, i, dp, dp, dp]),
         "tgx_local_property": (i, [P, i, i, dp, ip, dp, d, dp, i64, dp, C.POINTER(i64)]),
         "tgx_neighbors_build": (i, [P, i, dp, dp, d, i64, ip, ip, dp, dp, dp, dp,
                                     C.POINTER(i64), ip]),
@@ -661,6 +664,33 @@ class Engine:
                 continue
             self._check(rc)
             return V, der[: need.value]
+
+    def set_vdw_ts(self, local_property: int, c6_ref, r0_ref, alpha0_ref, rcut=10.0,
+                   buffer=1.0, rcut_inner=0.5, buffer_inner=0.5, sR=0.94, d=20.0):
+        """Tkatchenko-Scheffler dispersion on the Hirshfeld volumes of local
+        property `local_property`; per-species references (eV A^6, A, A^3)."""
+        c6, r0, a0 = _f64(c6_ref), _f64(r0_ref), _f64(alpha0_ref)
+        self._check(self.lib.tgx_set_vdw_ts(self.ctx, int(local_property), len(c6), _dptr(c6),
+                                            _dptr(r0), _dptr(a0), rcut, buffer, rcut_inner,
+                                            buffer_inner, sR, d))
+
+    def clear_vdw(self):
+        self._check(self.lib.tgx_clear_vdw(self.ctx))
+
+    def vdw_energy_forces(self, pos, species, lattice, do_forces=True):
+        """TS energies (n,), forces (n, 3) and virial (3, 3) of one structure
+        (forces and virial None without do_forces)."""
+        pos, lat, sp = _f64(pos), _f64(lattice), _i32(species)
+        n = pos.shape[0]
+        e = np.zeros(n)
+        f = np.zeros((n, 3))
+        v = np.zeros((3, 3))
+        self._check(self.lib.tgx_vdw_energy_forces(self.ctx, n, _dptr(pos), _iptr(sp),
+                                                   _dptr(lat), int(do_forces), _dptr(e),
+                                                   _dptr(f), _dptr(v)))
+        if not do_forces:
+            return e, None, None
+        return e, f, v.T.copy()
 
     def set_mc_batch(self, trials_per_batch: int):
         """Monte Carlo trials per speculative batch (0 automatic, 1 one at a
