@@ -68,7 +68,7 @@ typedef struct {
   int32_t l_max;
   int32_t radial_enhancement; /* 0, 1 or 2                                  */
   int32_t central_species;    /* 0 = all atoms are sites, else 1-based       */
-  int32_t basis;              /* 0 = poly3 (the only basis implemented)      */
+  int32_t basis;              /* 0 = poly3, 1 = poly3gauss                   */
   int32_t pad_;
   int32_t alpha_max[TGX_MAX_SPECIES];
   double rcut_hard[TGX_MAX_SPECIES];
@@ -82,6 +82,18 @@ typedef struct {
   double global_scaling[TGX_MAX_SPECIES];
   double nf[TGX_MAX_SPECIES]; /* only the default 4 is accepted (else
                                  TGX_ERR_UNSUPPORTED)                    */
+  /* compress_soap (types.f90:268-305, read_gap.f90:485-549): with
+   * compress_P_nonzero > 0 the descriptor is q = P p / |P p|, p the
+   * uncompressed power spectrum and P the sparse matrix of entries
+   * (compress_P_i[k], compress_P_j[k], compress_P_el[k]), 1-based row in
+   * 1..dim (the model's descriptor dimension) and column in 1..n_max
+   * (n_max+1)/2 (l_max+1); 0 = no compression.  The arrays are copied when
+   * the model is set. */
+  int32_t compress_P_nonzero;
+  int32_t pad2_;
+  const int32_t* compress_P_i;
+  const int32_t* compress_P_j;
+  const double* compress_P_el;
 } tgx_soap_hypers;
 
 /* Context ------------------------------------------------------------------ */

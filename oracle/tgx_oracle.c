@@ -757,7 +757,7 @@ void orc_sph(int L, const double* u, double* Y, double* dF) {
 /* ------------------------------------------------------------------------ */
 
 typedef struct {
-  int nr, L, L2, npair, dim;
+  int nr, L, L2, npair, dim, dim_full; /* dim: descriptor, dim_full: power spectrum */
   int off[TGX_MAX_SPECIES], nmax[TGX_MAX_SPECIES];
   double W[TGX_MAX_SPECIES][TGX_MAX_NR * TGX_MAX_NR];
 } soap_geom;
@@ -774,7 +774,25 @@ static void soap_geom_init(const tgx_soap_hypers* h, soap_geom* g) {
   g->L = h->l_max;
   g->L2 = (h->l_max + 1) * (h->l_max + 1);
   g->npair = g->nr * (g->nr + 1) / 2;
-  g->dim = g->npair * (h->l_max + 1);
+  g->dim_full = g->npair * (h->l_max + 1);
+  g->dim = g->dim_full;
+  if (h->compress_P_nonzero > 0) {
+    g->dim = 0;
+    for (int k = 0; k < h->compress_P_nonzero; k++)
+      g->dim = h->compress_P_i[k] > g->dim ? h->compress_P_i[k] : g->dim;
+  }
+}
+
+/* compress_soap: out[P_i - 1] += P_el x[P_j - 1] over the entries, each of
+ * `stride` components (1 for p, 3 for dp/dr) (soap_turbo's P
+ * transformation, read_gap.f90:485-549 reads it). */
+static void compress_apply(const tgx_soap_hypers* h, int dim_out, const double* x, int stride,
+                           double* out) {
+  memset(out, 0, sizeof(double) * (size_t)dim_out * stride);
+  for (int k = 0; k < h->compress_P_nonzero; k++) {
+    const int i = h->compress_P_i[k] - 1, j = h->compress_P_j[k] - 1;
+    for (int t = 0; t < stride; t++) out[i * stride + t] += h->compress_P_el[k] * x[j * stride + t];
+  }
 }
 
 /* Expansion coefficients c[(n*L2 + lm)] of one site (c zeroed by the
@@ -828,7 +846,8 @@ static void site_expand(const tgx_soap_hypers* h, const soap_geom* g, int si, in
 static void soap_site(const tgx_soap_hypers* h, const soap_geom* g, int si,
                       int np, const double* x, const int* sj, double* q,
                       double* dq) {
-  int NR = g->nr, L = g->L, L2 = g->L2, dim = g->dim;
+  int NR = g->nr, L = g->L, L2 = g->L2, dim = g->dim, dimf = g->dim_full;
+  const int cmp = h->compress_P_nonzero > 0;
   double* c = (double*)calloc((size_t)NR * L2, sizeof(double));
   double* Rv = (double*)malloc(sizeof(double) * (size_t)np * TGX_MAX_NR);
   double* dRv = (double*)malloc(sizeof(double) * (size_t)np * TGX_MAX_NR);
@@ -838,8 +857,9 @@ static void soap_site(const tgx_soap_hypers* h, const soap_geom* g, int si,
   double* Al = (double*)malloc(sizeof(double) * (size_t)np * (L + 1));
   double* dAl = (double*)malloc(sizeof(double) * (size_t)np * (L + 1));
   site_expand(h, g, si, np, x, sj, c, Rv, dRv, AY, Yv, dFv, Al, dAl);
-  /* power spectrum */
-  double* p = (double*)calloc((size_t)dim, sizeof(double));
+  /* power spectrum (compressed: pc = P p) */
+  double* pf = (double*)calloc((size_t)dimf, sizeof(double));
+  double* p = pf;
   int k = 0;
   for (int n = 0; n < NR; n++)
     for (int n2 = n; n2 < NR; n2++) {
@@ -850,10 +870,14 @@ static void soap_site(const tgx_soap_hypers* h, const soap_geom* g, int si,
           int lm = l * l + l + m;
           acc += c[n * L2 + lm] * c[n2 * L2 + lm];
         }
-        p[k * (L + 1) + l] = w * acc;
+        pf[k * (L + 1) + l] = w * acc;
       }
       k++;
     }
+  if (cmp) {
+    p = (double*)malloc(sizeof(double) * (size_t)dim);
+    compress_apply(h, dim, pf, 1, p);
+  }
   double nrm = 0.0;
   for (int d = 0; d < dim; d++) nrm += p[d] * p[d];
   nrm = sqrt(nrm);
@@ -861,7 +885,8 @@ static void soap_site(const tgx_soap_hypers* h, const soap_geom* g, int si,
 
   if (dq) {
     double* dc = (double*)malloc(sizeof(double) * TGX_MAX_NR * L2 * 3);
-    double* dp = (double*)malloc(sizeof(double) * (size_t)dim * 3);
+    double* dpf = (double*)malloc(sizeof(double) * (size_t)dimf * 3);
+    double* dp = cmp ? (double*)malloc(sizeof(double) * (size_t)dim * 3) : dpf;
     for (int j = 0; j < np; j++) {
       const double* v = x + 3 * j;
       double r = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
@@ -884,7 +909,7 @@ static void soap_site(const tgx_soap_hypers* h, const soap_geom* g, int si,
           }
         }
       }
-      memset(dp, 0, sizeof(double) * (size_t)dim * 3);
+      memset(dpf, 0, sizeof(double) * (size_t)dimf * 3);
       int kp = 0;
       for (int n = 0; n < NR; n++)
         for (int n2 = n; n2 < NR; n2++) {
@@ -900,11 +925,12 @@ static void soap_site(const tgx_soap_hypers* h, const soap_geom* g, int si,
                   if (in2) acc[kk] += c[n * L2 + lm] * dc[((size_t)(n2 - o) * L2 + lm) * 3 + kk];
                 }
               }
-              for (int kk = 0; kk < 3; kk++) dp[(kp * (L + 1) + l) * 3 + kk] = w * acc[kk];
+              for (int kk = 0; kk < 3; kk++) dpf[(kp * (L + 1) + l) * 3 + kk] = w * acc[kk];
             }
           }
           kp++;
         }
+      if (cmp) compress_apply(h, dim, dpf, 3, dp);
       double qdp[3] = {0, 0, 0};
       for (int d = 0; d < dim; d++)
         for (int kk = 0; kk < 3; kk++) qdp[kk] += q[d] * dp[d * 3 + kk];
@@ -913,9 +939,11 @@ static void soap_site(const tgx_soap_hypers* h, const soap_geom* g, int si,
           dq[((size_t)j * dim + d) * 3 + kk] = (dp[d * 3 + kk] - q[d] * qdp[kk]) / nrm;
     }
     free(dc);
-    free(dp);
+    if (cmp) free(dp);
+    free(dpf);
   }
-  free(p);
+  if (cmp) free(p);
+  free(pf);
   free(c);
   free(Rv);
   free(dRv);
@@ -962,6 +990,15 @@ int orc_soap_coeffs(const tgx_soap_hypers* h, int si, int np, const double* x, c
 static int hypers_supported(const tgx_soap_hypers* h) {
   for (int s = 0; s < h->n_species; s++)
     if (h->nf[s] != 4.0) return TGX_ERR_UNSUPPORTED;
+  if (h->basis != 0) return TGX_ERR_UNSUPPORTED;
+  if (h->compress_P_nonzero > 0) {
+    int nr = 0;
+    for (int s = 0; s < h->n_species; s++) nr += h->alpha_max[s];
+    const int dimf = nr * (nr + 1) / 2 * (h->l_max + 1);
+    for (int k = 0; k < h->compress_P_nonzero; k++)
+      if (h->compress_P_i[k] < 1 || h->compress_P_j[k] < 1 || h->compress_P_j[k] > dimf)
+        return TGX_ERR_ARG;
+  }
   return TGX_OK;
 }
 

@@ -49,6 +49,20 @@ def write_model_files(d, model, species=("C",)):
         h = blk.hypers
         np.savetxt(os.path.join(d, f"alphas{k}.dat"), blk.alphas, fmt="%.17e")
         np.savetxt(os.path.join(d, f"desc{k}.dat"), blk.Qs.reshape(-1), fmt="%.17e")
+        extra = []
+        if h.compress_P is not None:
+            pi, pj, pe = h.compress_P
+            with open(os.path.join(d, f"compress{k}.dat"), "w") as f:
+                f.write(" ".join(str(x) for x in h.alpha_max) + f" {h.l_max} {h.compress_dim}\n")
+                if np.array_equal(pi, np.arange(1, len(pi) + 1)) and np.all(pe == 1.0):
+                    f.writelines(f"{j}\n" for j in pj)  # the older format: kept indices
+                else:
+                    f.write("P_transformation\n")
+                    f.writelines(f"{i} {j} {e:.17e}\n" for i, j, e in zip(pi, pj, pe))
+                    f.write("end_transformation\n")
+            extra += ["compress_soap = .true.", f"file_compress_soap = compress{k}.dat"]
+        if h.basis != "poly3":
+            extra.append(f"basis = {h.basis}")
         lines += ["gap_beg soap_turbo", f"n_species = {h.n_species}",
                   "species = " + " ".join(species),
                   "rcut = " + " ".join(str(x) for x in h.rcut_hard),
@@ -58,7 +72,8 @@ def write_model_files(d, model, species=("C",)):
                   "n_max = " + " ".join(str(x) for x in h.alpha_max), f"l_max = {h.l_max}",
                   f"central_species = {h.central_species}",
                   f"zeta = {blk.zeta}", f"delta = {blk.delta}",
-                  f"alphas_sparse = alphas{k}.dat", f"desc_sparse = desc{k}.dat", "gap_end"]
+                  f"alphas_sparse = alphas{k}.dat", f"desc_sparse = desc{k}.dat"] + extra + \
+            ["gap_end"]
     for k, b in enumerate(model.two_body):
         np.savetxt(os.path.join(d, f"a2b_{k}.dat"), np.stack([b.alphas, b.cutoff], 1), fmt="%.17e")
         np.savetxt(os.path.join(d, f"d2b_{k}.dat"), b.Qs, fmt="%.17e")
@@ -258,6 +273,46 @@ def test_fortran_predict_angle_3b_core_pot(tmp_path):
     vir = np.array(lines[1].split()[1:], float).reshape(3, 3).T
     rows = np.array([l.split()[1:] for l in lines[2:2 + len(pos)]], float)
     e0, f0, v0 = oracle.energy_forces(model, pos, sp, lat)[:3]
+    assert abs(etot - e0.sum()) < 1e-8 * len(pos)
+    assert np.abs(rows[:, 3] - e0).max() < 1e-8
+    assert np.abs(rows[:, 4:7] - f0).max() < 1e-6
+    assert np.abs(vir - v0).max() < 1e-6
+
+
+@pytest.mark.gpu
+def test_fortran_predict_compressed_blocks(tmp_path):
+    """Two compressed soap_turbo blocks read from .gap files: a general P in
+    the P_transformation format (block 1) and a selection of components in
+    the older one-index-per-line format (block 2, central species H); the
+    predict driver against the oracle."""
+    import dataclasses
+
+    import structures as S
+    import test_gpu_compress as TC
+
+    exe = os.path.join(FDIR, "tgx_predict")
+    if not os.path.exists(exe):
+        pytest.fail("tgx_predict is not built (run __graft_entry__.build())")
+    pos, sp, lat = S.cho_cell(120, 1.5, 1.0, 81)
+    h = S.cho_hypers()
+    h1 = dataclasses.replace(h, compress_P=TC._random_P(h, 200, 3), compress_dim=200)
+    sel = np.sort(np.random.default_rng(5).choice(h.full_dim, 300, replace=False)) + 1
+    h2 = dataclasses.replace(h, central_species=2, compress_dim=300,
+                             compress_P=(np.arange(1, 301), sel, np.ones(300)))
+    m1 = S.model_from_soaps(h1, oracle.soap(h1, pos, sp, lat)["soap"], 50, seed=1)
+    m2 = S.model_from_soaps(h2, oracle.soap(h2, pos, sp, lat)["soap"], 30, seed=2)
+    model = dataclasses.replace(m1, more_soap=[m2])
+    names = ("C", "H", "O")
+    write_model_files(str(tmp_path), model, species=names)
+    write_xyz(str(tmp_path / "s.xyz"), pos, sp, lat, symbols=names)
+    out = subprocess.run([exe, "model.gap", "s.xyz", "fp64"], capture_output=True, text=True,
+                         cwd=tmp_path, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    lines = out.stdout.splitlines()
+    etot = float(lines[0].split()[1])
+    vir = np.array(lines[1].split()[1:], float).reshape(3, 3).T
+    rows = np.array([l.split()[1:] for l in lines[2:2 + len(pos)]], float)
+    e0, f0, v0 = oracle.energy_forces(model, pos, sp, lat)
     assert abs(etot - e0.sum()) < 1e-8 * len(pos)
     assert np.abs(rows[:, 3] - e0).max() < 1e-8
     assert np.abs(rows[:, 4:7] - f0).max() < 1e-6

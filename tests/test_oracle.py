@@ -1395,3 +1395,96 @@ def test_ts_hirshfeld_term_fd():
                 Es.append(energy(p, l))
             fdv = -(Es[0] - Es[1]) / (2 * hh)
             assert abs(fdv - v[a, b]) < 1e-7 * max(1.0, abs(v[a, b])), (a, b, fdv, v[a, b])
+
+
+# --------------------------------------------------------------------------
+# compress_soap
+# --------------------------------------------------------------------------
+def _compressed(h, P, dim=0):
+    import dataclasses
+    return dataclasses.replace(h, compress_P=P, compress_dim=dim)
+
+
+def _random_P(h, dim, seed, per_row=3):
+    rng = np.random.default_rng(seed)
+    pi, pj, pe = [], [], []
+    for i in range(1, dim + 1):
+        for j in rng.choice(h.full_dim, per_row, replace=False):
+            pi.append(i)
+            pj.append(j + 1)
+            pe.append(rng.normal())
+    return np.array(pi), np.array(pj), np.array(pe)
+
+
+def test_compress_identity_and_selection():
+    """The identity transformation gives the uncompressed descriptors and
+    derivatives bit for bit; a selection of components (the old file format:
+    one kept index per line, weight 1) gives q[sel] / |q[sel]|."""
+    pos, sp, lat = S.carbon_cell(20, 2.0, 1.2, 3)
+    h = S.carbon_hypers()
+    ref = oracle.soap(h, pos, sp, lat, derivatives=True)
+    d = h.full_dim
+    ident = _compressed(h, (np.arange(1, d + 1), np.arange(1, d + 1), np.ones(d)))
+    got = oracle.soap(ident, pos, sp, lat, derivatives=True)
+    assert np.array_equal(got["soap"], ref["soap"])
+    assert np.array_equal(got["soap_der"], ref["soap_der"])
+    sel = np.sort(np.random.default_rng(4).choice(d, 100, replace=False))
+    hs = _compressed(h, (np.arange(1, 101), sel + 1, np.ones(100)))
+    got = oracle.soap(hs, pos, sp, lat)
+    want = ref["soap"][:, sel]
+    want /= np.linalg.norm(want, axis=1, keepdims=True)
+    assert np.abs(got["soap"] - want).max() < 1e-14
+
+
+def test_compress_general_values_derivatives_and_forces():
+    """A general sparse transformation (3 weighted entries per row):
+    q = P p / |P p| (numpy on the uncompressed descriptors: P p is P q times
+    |p|), dq/dr by finite differences, and forces = -dE/dr, virial =
+    -dE/d(strain) for a GAP model on the compressed descriptors."""
+    pos, sp, lat = S.carbon_cell(24, 2.0, 1.2, 9)
+    h = S.carbon_hypers()
+    P = _random_P(h, 90, 2)
+    hc = _compressed(h, P, 90)
+    full = oracle.soap(h, pos, sp, lat)["soap"]
+    M = np.zeros((90, h.full_dim))
+    np.add.at(M, (P[0] - 1, P[1] - 1), P[2])
+    want = full @ M.T
+    want /= np.linalg.norm(want, axis=1, keepdims=True)
+    got = oracle.soap(hc, pos, sp, lat, derivatives=True)
+    assert got["soap"].shape == (24, 90)
+    assert np.abs(got["soap"] - want).max() < 1e-13
+    # dq_i/dr_j of one pair by central differences
+    k = 5
+    i, j = int(np.repeat(np.arange(24), got["site_n_neigh"])[k]), int(got["pair_atom"][k])
+    assert i != j
+    hh = 1e-5
+    for a in range(3):
+        qs = []
+        for s in (1, -1):
+            p = pos.copy()
+            p[j, a] += s * hh
+            qs.append(oracle.soap(hc, p, sp, lat)["soap"][i])
+        fd = (qs[0] - qs[1]) / (2 * hh)
+        # every entry of row i pointing at j (images included)
+        rows = np.arange(np.sum(got["site_n_neigh"][:i]), np.sum(got["site_n_neigh"][:i + 1]))
+        an = got["soap_der"][rows[got["pair_atom"][rows] == j], :, a].sum(0)
+        assert np.abs(fd - an).max() < 1e-7, a
+    model = S.model_from_soaps(hc, got["soap"], 30, seed=3)
+    e, f, v = oracle.energy_forces(model, pos, sp, lat)
+    for at in (0, 11):
+        for a in range(3):
+            Es = []
+            for s in (1, -1):
+                p = pos.copy()
+                p[at, a] += s * hh
+                Es.append(oracle.energy_forces(model, p, sp, lat, do_forces=False)[0].sum())
+            fd = -(Es[0] - Es[1]) / (2 * hh)
+            assert abs(fd - f[at, a]) < 1e-7 * max(1.0, abs(f[at, a])), (at, a, fd, f[at, a])
+    for a in range(3):
+        def strained_energy(t):
+            eps = np.zeros((3, 3))
+            eps[a, a] = t
+            p, l = _strained(pos, lat, eps)
+            return oracle.energy_forces(model, p, sp, l, do_forces=False)[0].sum()
+        fd = -_fd5(strained_energy, 0.0, 1e-4)
+        assert abs(fd - v[a, a]) < 5e-8 * max(1.0, abs(v[a, a])), (a, fd, v[a, a])

@@ -128,6 +128,16 @@ hipError_t launch_angle_3b(const Angle3bDev& b, const double* pos, const int* sp
                            const Geom& g, int maxnb, const int* count, const int* list,
                            double* eatom, double* vatom, double* g3, int do_forces,
                            hipStream_t st);
+template <typename T>
+hipError_t launch_compress_fwd(int n_sites, int dimf, int dimf_ld, const T* qf, const T* pnf,
+                               int dim, int dim_ld, const int* roff, const int* rcol,
+                               const double* rval, T* q, unsigned short* qb, int qb_ld, T* pn,
+                               hipStream_t st);
+template <typename T>
+hipError_t launch_compress_bwd(int n_sites, int dim, int dim_ld, const T* q, const T* pn,
+                               const T* G, int g_ld, int dimf, int dimf_ld, const T* pnf,
+                               const int* coff, const int* crow, const double* cval, T* Gf,
+                               hipStream_t st);
 hipError_t launch_ts_volume(int n, const int* site_of_atom, const double* esite, double v0,
                             double* V, hipStream_t st);
 hipError_t launch_ts_pairs(const TsDev& t, const double* pos, const int* species, int n,
@@ -602,6 +612,10 @@ static int soap_forward(tgx_ctx* c, int n, const Geom& cell, int n_sites, bool f
   const int prec = c->precision;
   const size_t ts = tsize(prec);
   const int NRP = c->blk->sd.nrp;
+  SoapBlock* const b = c->blk;
+  const bool cmp = b->cmp > 0;
+  if (cmp && few)
+    return set_err(c, TGX_ERR_UNSUPPORTED, "compress_soap blocks are not implemented for Monte Carlo");
   ENSURE(c, blk->q, ts * (size_t)std::max(n_sites, 1) * c->blk->dim_pad);
   ENSURE(c, blk->cbuf, ts * (size_t)std::max(n_sites, 1) * NRP * c->blk->sd.L2);
   ENSURE(c, blk->pnorm, ts * (size_t)std::max(n_sites, 1));
@@ -610,6 +624,17 @@ static int soap_forward(tgx_ctx* c, int n, const Geom& cell, int n_sites, bool f
     ENSURE(c, blk->qb, 2 * (size_t)std::max(n_sites, 1) * c->blk->dim_pad);
     qb = c->blk->qb.as<unsigned short>();
   }
+  // with compress_soap the expansion writes the uncompressed descriptor and
+  // its norm (q_f, |p|); the compression pass below writes q (and its BF16
+  // copy) and |P p|
+  if (cmp) {
+    ENSURE(c, blk->qf, ts * (size_t)std::max(n_sites, 1) * b->dimf_pad);
+    ENSURE(c, blk->pnf, ts * (size_t)std::max(n_sites, 1));
+  }
+  void* const fq = cmp ? b->qf.p : b->q.p;
+  void* const fpn = cmp ? b->pnf.p : b->pnorm.p;
+  const int fld = cmp ? b->dimf_pad : b->dim_pad;
+  unsigned short* const fqb = cmp ? nullptr : qb;
   const char* fs = getenv("TGX_FWD_SPLIT");
   const int split = (few || (fs && fs[0] == '1')) ? 1 : 0;
   auto fwd = [&](int wpb) -> hipError_t {
@@ -619,15 +644,15 @@ static int soap_forward(tgx_ctx* c, int n, const Geom& cell, int n_sites, bool f
                                      c->species.as<int>(), cell, c->maxnb,
                                      c->nb_count.as<int>(), c->nb_list.as<int>(),
                                      c->blk->site_atom.as<int>(), n_sites, c->blk->pairn.as<int>(),
-                                     c->blk->q.as<double>(), c->blk->dim_pad, nullptr, 0,
-                                     c->blk->cbuf.as<double>(), c->blk->pnorm.as<double>(), c->stream, 0,
+                                     static_cast<double*>(fq), fld, nullptr, 0,
+                                     c->blk->cbuf.as<double>(), static_cast<double*>(fpn), c->stream, 0,
                                      wpb);
     return launch_soap_fwd<float>(c->blk->sd, c->blk->sd_dev.as<SoapDev>(), c->blk->W_dev.as<double>(),
                                   c->blk->rtab.as<float>(), c->pos.as<double>(), c->species.as<int>(),
                                   cell, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
                                   c->blk->site_atom.as<int>(), n_sites, c->blk->pairn.as<int>(),
-                                  c->blk->q.as<float>(), c->blk->dim_pad, qb, c->blk->dim_pad,
-                                  c->blk->cbuf.as<float>(), c->blk->pnorm.as<float>(), c->stream, split,
+                                  static_cast<float*>(fq), fld, fqb, c->blk->dim_pad,
+                                  c->blk->cbuf.as<float>(), static_cast<float*>(fpn), c->stream, split,
                                   wpb);
   };
   // TGX_SOAP_WPB=1|2|4 forces the waves per workgroup (for tests)
@@ -641,7 +666,60 @@ static int soap_forward(tgx_ctx* c, int n, const Geom& cell, int n_sites, bool f
   kev_rec(c, 0);
   TGX_HIP(c, fwd(wpb));
   kev_rec(c, 1);
+  if (cmp && n_sites > 0) {
+    if (prec == TGX_FP64)
+      TGX_HIP(c, launch_compress_fwd<double>(n_sites, b->sd.dim, b->dimf_pad, b->qf.as<double>(),
+                                             b->pnf.as<double>(), b->dim, b->dim_pad,
+                                             b->c_roff.as<int>(), b->c_rcol.as<int>(),
+                                             b->c_rval.as<double>(), b->q.as<double>(), nullptr,
+                                             0, b->pnorm.as<double>(), c->stream));
+    else
+      TGX_HIP(c, launch_compress_fwd<float>(n_sites, b->sd.dim, b->dimf_pad, b->qf.as<float>(),
+                                            b->pnf.as<float>(), b->dim, b->dim_pad,
+                                            b->c_roff.as<int>(), b->c_rcol.as<int>(),
+                                            b->c_rval.as<double>(), b->q.as<float>(), qb,
+                                            b->dim_pad, b->pnorm.as<float>(), c->stream));
+  }
   (void)n;
+  return TGX_OK;
+}
+
+// Inputs of the SOAP adjoint of block c->blk: the descriptor, its norm and
+// dE/dq, each of the power spectrum.  With compress_soap the kernel step's
+// dE/dq of the compressed descriptor is first taken back to the power
+// spectrum (after the kernel step with the gradient).
+struct AdjIn {
+  void* q;
+  int q_ld;
+  void* pn;
+  void* G;
+  int g_ld;
+};
+
+static int adjoint_inputs(tgx_ctx* c, int n_sites, AdjIn* a) {
+  SoapBlock* const b = c->blk;
+  if (!b->cmp) {
+    *a = AdjIn{b->q.p, b->dim_pad, b->pnorm.p, b->gmat.p, b->dim_pad};
+    return TGX_OK;
+  }
+  ENSURE(c, blk->gf, tsize(c->precision) * (size_t)std::max(n_sites, 1) * b->dimf_pad);
+  if (n_sites > 0) {
+    if (c->precision == TGX_FP64)
+      TGX_HIP(c, launch_compress_bwd<double>(n_sites, b->dim, b->dim_pad, b->q.as<double>(),
+                                             b->pnorm.as<double>(), b->gmat.as<double>(),
+                                             b->dim_pad, b->sd.dim, b->dimf_pad,
+                                             b->pnf.as<double>(), b->c_coff.as<int>(),
+                                             b->c_crow.as<int>(), b->c_cval.as<double>(),
+                                             b->gf.as<double>(), c->stream));
+    else
+      TGX_HIP(c, launch_compress_bwd<float>(n_sites, b->dim, b->dim_pad, b->q.as<float>(),
+                                            b->pnorm.as<float>(), b->gmat.as<float>(),
+                                            b->dim_pad, b->sd.dim, b->dimf_pad,
+                                            b->pnf.as<float>(), b->c_coff.as<int>(),
+                                            b->c_crow.as<int>(), b->c_cval.as<double>(),
+                                            b->gf.as<float>(), c->stream));
+  }
+  *a = AdjIn{b->qf.p, b->dimf_pad, b->pnf.p, b->gf.p, b->dimf_pad};
   return TGX_OK;
 }
 
@@ -807,22 +885,27 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
   if (do_forces && c->has_soap && n_sites > 0) {
     const size_t ts = tsize(prec);
     ENSURE(c, gpair, ts * 4 * (size_t)n * c->maxnb);  // (g, 0) per list slot
+    AdjIn ai;
+    int arc = adjoint_inputs(c, n_sites, &ai);
+    if (arc) return arc;
     auto bwd = [&](int wpb) -> hipError_t {
       if (prec == TGX_FP64)
         return launch_soap_bwd<double>(
             c->blk->sd, c->blk->sd_dev.as<SoapDev>(), c->blk->W_dev.as<double>(), c->blk->rtab.as<float>(),
             c->pos.as<double>(), c->species.as<int>(), cell, c->maxnb, c->nb_count.as<int>(),
             c->nb_list.as<int>(), c->blk->site_atom.as<int>(), n_sites, c->blk->pairn.as<int>(),
-            c->blk->q.as<double>(), c->blk->dim_pad, c->blk->cbuf.as<double>(), c->blk->pnorm.as<double>(),
-            c->blk->gmat.as<double>(), c->blk->dim_pad, c->gpair.as<double>(), c->blk->vsite.as<double>(),
+            static_cast<const double*>(ai.q), ai.q_ld, c->blk->cbuf.as<double>(),
+            static_cast<const double*>(ai.pn), static_cast<const double*>(ai.G), ai.g_ld,
+            c->gpair.as<double>(), c->blk->vsite.as<double>(),
             fuse ? x.twob_store.as<TwoBodyDev>() : nullptr, fuse ? (int)c->twob.size() : 0,
             c->eatom.as<double>(), c->stream, wpb);
       return launch_soap_bwd<float>(
           c->blk->sd, c->blk->sd_dev.as<SoapDev>(), c->blk->W_dev.as<double>(), c->blk->rtab.as<float>(),
           c->pos.as<double>(), c->species.as<int>(), cell, c->maxnb, c->nb_count.as<int>(),
           c->nb_list.as<int>(), c->blk->site_atom.as<int>(), n_sites, c->blk->pairn.as<int>(),
-          c->blk->q.as<float>(), c->blk->dim_pad, c->blk->cbuf.as<float>(), c->blk->pnorm.as<float>(),
-          c->blk->gmat.as<float>(), c->blk->dim_pad, c->gpair.as<float>(), c->blk->vsite.as<double>(),
+          static_cast<const float*>(ai.q), ai.q_ld, c->blk->cbuf.as<float>(),
+          static_cast<const float*>(ai.pn), static_cast<const float*>(ai.G), ai.g_ld,
+          c->gpair.as<float>(), c->blk->vsite.as<double>(),
           fuse ? x.twob_store.as<TwoBodyDev>() : nullptr, fuse ? (int)c->twob.size() : 0,
           c->eatom.as<double>(), c->stream, wpb);
     };
@@ -962,6 +1045,9 @@ static int ef_launch_multi(tgx_ctx* c, int n, const Geom& cell, int do_forces, b
     ENSURE(c, blk->vsite, 48 * (size_t)ns);
     const size_t ts = tsize(prec);
     ENSURE(c, gpair, ts * 4 * (size_t)n * c->maxnb);
+    AdjIn ai;
+    rcode = adjoint_inputs(c, ns, &ai);
+    if (rcode) break;
     kev_rec(c, 4);
     hipError_t e;
     if (prec == TGX_FP64)
@@ -969,19 +1055,19 @@ static int ef_launch_multi(tgx_ctx* c, int n, const Geom& cell, int do_forces, b
           c->blk->sd, c->blk->sd_dev.as<SoapDev>(), c->blk->W_dev.as<double>(),
           c->blk->rtab.as<float>(), c->pos.as<double>(), c->species.as<int>(), cell, c->maxnb,
           c->nb_count.as<int>(), c->nb_list.as<int>(), c->blk->site_atom.as<int>(), ns,
-          c->blk->pairn.as<int>(), c->blk->q.as<double>(), c->blk->dim_pad,
-          c->blk->cbuf.as<double>(), c->blk->pnorm.as<double>(), c->blk->gmat.as<double>(),
-          c->blk->dim_pad, c->gpair.as<double>(), c->blk->vsite.as<double>(), nullptr, 0,
-          c->eatom.as<double>(), c->stream, 0);
+          c->blk->pairn.as<int>(), static_cast<const double*>(ai.q), ai.q_ld,
+          c->blk->cbuf.as<double>(), static_cast<const double*>(ai.pn),
+          static_cast<const double*>(ai.G), ai.g_ld, c->gpair.as<double>(),
+          c->blk->vsite.as<double>(), nullptr, 0, c->eatom.as<double>(), c->stream, 0);
     else
       e = launch_soap_bwd<float>(
           c->blk->sd, c->blk->sd_dev.as<SoapDev>(), c->blk->W_dev.as<double>(),
           c->blk->rtab.as<float>(), c->pos.as<double>(), c->species.as<int>(), cell, c->maxnb,
           c->nb_count.as<int>(), c->nb_list.as<int>(), c->blk->site_atom.as<int>(), ns,
-          c->blk->pairn.as<int>(), c->blk->q.as<float>(), c->blk->dim_pad,
-          c->blk->cbuf.as<float>(), c->blk->pnorm.as<float>(), c->blk->gmat.as<float>(),
-          c->blk->dim_pad, c->gpair.as<float>(), c->blk->vsite.as<double>(), nullptr, 0,
-          c->eatom.as<double>(), c->stream, 0);
+          c->blk->pairn.as<int>(), static_cast<const float*>(ai.q), ai.q_ld,
+          c->blk->cbuf.as<float>(), static_cast<const float*>(ai.pn),
+          static_cast<const float*>(ai.G), ai.g_ld, c->gpair.as<float>(),
+          c->blk->vsite.as<double>(), nullptr, 0, c->eatom.as<double>(), c->stream, 0);
     kev_rec(c, 5);
     kev_rec(c, 6);
     if (e == hipSuccess) {
@@ -1331,8 +1417,22 @@ static int load_soap_block(tgx_ctx* c, const tgx_soap_hypers* h, int dim, int n_
   sd.rcut_max = rcm;
   if (h->central_species < 0 || h->central_species > h->n_species)
     return set_err(c, TGX_ERR_ARG, "central_species out of range");
-  if (dim != sd.dim)
+  const int ncmp = h->compress_P_nonzero;
+  if (ncmp < 0) return set_err(c, TGX_ERR_ARG, "compress_P_nonzero < 0");
+  if (ncmp > 0) {
+    if (!h->compress_P_i || !h->compress_P_j || !h->compress_P_el)
+      return set_err(c, TGX_ERR_ARG, "compress_soap: P arrays missing");
+    if (dim < 1 || dim > 6144)
+      return set_err(c, TGX_ERR_ARG, "compressed dim %d not in 1..6144", dim);
+    for (int k = 0; k < ncmp; k++)
+      if (h->compress_P_i[k] < 1 || h->compress_P_i[k] > dim || h->compress_P_j[k] < 1 ||
+          h->compress_P_j[k] > sd.dim || !std::isfinite(h->compress_P_el[k]))
+        return set_err(c, TGX_ERR_ARG,
+                       "compress_soap entry %d (%d, %d) outside %d x %d or not finite", k + 1,
+                       h->compress_P_i[k], h->compress_P_j[k], dim, sd.dim);
+  } else if (dim != sd.dim) {
     return set_err(c, TGX_ERR_ARG, "dim %d != n_max(n_max+1)/2*(l_max+1) = %d", dim, sd.dim);
+  }
   if (n_sparse < 1) return set_err(c, TGX_ERR_ARG, "n_sparse must be >= 1");
   // W per species and the central-atom coefficients
   std::vector<double> W((size_t)TGX_MAX_SPECIES * TGX_MAX_NR * TGX_MAX_NR, 0.0);
@@ -1347,6 +1447,47 @@ static int load_soap_block(tgx_ctx* c, const tgx_soap_hypers* h, int dim, int n_
   }
   c->blk->W_host = W;
   c->blk->hyp = *h;
+  // the block keeps its own copy of P (its hypers point there, so that
+  // blocks loaded from them later, e.g. local properties, find it)
+  c->blk->cmp = ncmp;
+  c->blk->cP_i.assign(h->compress_P_i, h->compress_P_i + ncmp);
+  c->blk->cP_j.assign(h->compress_P_j, h->compress_P_j + ncmp);
+  c->blk->cP_el.assign(h->compress_P_el, h->compress_P_el + ncmp);
+  c->blk->hyp.compress_P_i = ncmp ? c->blk->cP_i.data() : nullptr;
+  c->blk->hyp.compress_P_j = ncmp ? c->blk->cP_j.data() : nullptr;
+  c->blk->hyp.compress_P_el = ncmp ? c->blk->cP_el.data() : nullptr;
+  c->blk->dimf_pad = (sd.dim + 63) / 64 * 64;
+  if (ncmp) {
+    // P by rows (forward) and by columns (adjoint), entries in input order
+    std::vector<int> roff(dim + 1, 0), coff(sd.dim + 1, 0), rcol(ncmp), crow(ncmp);
+    std::vector<double> rval(ncmp), cval(ncmp);
+    for (int k = 0; k < ncmp; k++) {
+      roff[h->compress_P_i[k]]++;
+      coff[h->compress_P_j[k]]++;
+    }
+    for (int i = 0; i < dim; i++) roff[i + 1] += roff[i];
+    for (int j = 0; j < sd.dim; j++) coff[j + 1] += coff[j];
+    std::vector<int> rf(roff.begin(), roff.end() - 1), cf(coff.begin(), coff.end() - 1);
+    for (int k = 0; k < ncmp; k++) {
+      const int i = h->compress_P_i[k] - 1, j = h->compress_P_j[k] - 1;
+      rcol[rf[i]] = j;
+      rval[rf[i]++] = h->compress_P_el[k];
+      crow[cf[j]] = i;
+      cval[cf[j]++] = h->compress_P_el[k];
+    }
+    ENSURE(c, blk->c_roff, 4 * roff.size());
+    ENSURE(c, blk->c_coff, 4 * coff.size());
+    ENSURE(c, blk->c_rcol, 4 * (size_t)ncmp);
+    ENSURE(c, blk->c_crow, 4 * (size_t)ncmp);
+    ENSURE(c, blk->c_rval, 8 * (size_t)ncmp);
+    ENSURE(c, blk->c_cval, 8 * (size_t)ncmp);
+    TGX_HIP(c, hipMemcpy(c->blk->c_roff.p, roff.data(), 4 * roff.size(), hipMemcpyHostToDevice));
+    TGX_HIP(c, hipMemcpy(c->blk->c_coff.p, coff.data(), 4 * coff.size(), hipMemcpyHostToDevice));
+    TGX_HIP(c, hipMemcpy(c->blk->c_rcol.p, rcol.data(), 4 * (size_t)ncmp, hipMemcpyHostToDevice));
+    TGX_HIP(c, hipMemcpy(c->blk->c_crow.p, crow.data(), 4 * (size_t)ncmp, hipMemcpyHostToDevice));
+    TGX_HIP(c, hipMemcpy(c->blk->c_rval.p, rval.data(), 8 * (size_t)ncmp, hipMemcpyHostToDevice));
+    TGX_HIP(c, hipMemcpy(c->blk->c_cval.p, cval.data(), 8 * (size_t)ncmp, hipMemcpyHostToDevice));
+  }
   c->blk->sd = sd;
   c->blk->dim = dim;
   c->blk->n_sparse = n_sparse;
@@ -2377,6 +2518,8 @@ static int mc_run_impl(tgx_ctx* c, int n, const int* species, const double* latt
   if (X(c).md.active) return set_err(c, TGX_ERR_STATE, "end the MD run first (tgx_md_end)");
   if (!c->a3b.empty() || !c->corep.empty())
     return set_err(c, TGX_ERR_UNSUPPORTED, "Monte Carlo takes no angle_3b or core_pot blocks");
+  if (c->has_soap && c->blocks[0]->cmp > 0)
+    return set_err(c, TGX_ERR_UNSUPPORTED, "Monte Carlo takes no compress_soap block");
   if (c->has_soap && c->blocks.size() > 1)
     return set_err(c, TGX_ERR_UNSUPPORTED,
                    "Monte Carlo takes models of one soap_turbo block (this one has %d)",
@@ -3117,21 +3260,26 @@ static int lp_adjoint(tgx_ctx* c, SoapBlock* P, const Geom& cell, int n, int ns)
   const int prec = c->precision;
   ENSURE(c, gpair, tsize(prec) * 4 * (size_t)n * c->maxnb);
   ENSURE(c, blk->vsite, 48 * (size_t)ns);
+  AdjIn ai;
+  const int arc = adjoint_inputs(c, ns, &ai);
+  if (arc) return arc;
   hipError_t e;
   if (prec == TGX_FP64)
     e = launch_soap_bwd<double>(
         P->sd, P->sd_dev.as<SoapDev>(), P->W_dev.as<double>(), P->rtab.as<float>(),
         c->pos.as<double>(), c->species.as<int>(), cell, c->maxnb, c->nb_count.as<int>(),
-        c->nb_list.as<int>(), P->site_atom.as<int>(), ns, P->pairn.as<int>(), P->q.as<double>(),
-        P->dim_pad, P->cbuf.as<double>(), P->pnorm.as<double>(), P->gmat.as<double>(),
-        P->dim_pad, c->gpair.as<double>(), P->vsite.as<double>(), nullptr, 0,
-        c->eatom.as<double>(), c->stream, 0);
+        c->nb_list.as<int>(), P->site_atom.as<int>(), ns, P->pairn.as<int>(),
+        static_cast<const double*>(ai.q), ai.q_ld, P->cbuf.as<double>(),
+        static_cast<const double*>(ai.pn), static_cast<const double*>(ai.G), ai.g_ld,
+        c->gpair.as<double>(), P->vsite.as<double>(), nullptr, 0, c->eatom.as<double>(),
+        c->stream, 0);
   else
     e = launch_soap_bwd<float>(
         P->sd, P->sd_dev.as<SoapDev>(), P->W_dev.as<double>(), P->rtab.as<float>(),
         c->pos.as<double>(), c->species.as<int>(), cell, c->maxnb, c->nb_count.as<int>(),
-        c->nb_list.as<int>(), P->site_atom.as<int>(), ns, P->pairn.as<int>(), P->q.as<float>(),
-        P->dim_pad, P->cbuf.as<float>(), P->pnorm.as<float>(), P->gmat.as<float>(), P->dim_pad,
+        c->nb_list.as<int>(), P->site_atom.as<int>(), ns, P->pairn.as<int>(),
+        static_cast<const float*>(ai.q), ai.q_ld, P->cbuf.as<float>(),
+        static_cast<const float*>(ai.pn), static_cast<const float*>(ai.G), ai.g_ld,
         c->gpair.as<float>(), P->vsite.as<double>(), nullptr, 0, c->eatom.as<double>(),
         c->stream, 0);
   TGX_HIP(c, e);

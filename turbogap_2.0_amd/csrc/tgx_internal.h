@@ -184,10 +184,19 @@ struct SoapBlock {
   // kernel-step and adjoint work
   DevBuf site_of_atom, site_atom, q, qb, cbuf, pnorm, wmat, gmat, epart, esite, vsite;
   int n_sites = 0;
+  // compress_soap (cmp = number of entries of P, 0 = none): the entries
+  // (1-based, as given), P by rows and by columns (0-based) on the device,
+  // and per call the uncompressed descriptor q_f, its norm and dE/dq_f, in
+  // rows of dimf_pad = roundup(power-spectrum dimension, 64)
+  int cmp = 0, dimf_pad = 0;
+  std::vector<int32_t> cP_i, cP_j;
+  std::vector<double> cP_el;
+  DevBuf c_roff, c_rcol, c_rval, c_coff, c_crow, c_cval, qf, pnf, gf;
   void release() {
     DevBuf* b[] = {&W_dev, &rtab, &Qs64, &QsT64, &Qs32, &QsT32, &Qs16, &QsT16, &alph64,
                    &alph32, &sd_dev, &pairn, &site_of_atom, &site_atom, &q, &qb, &cbuf,
-                   &pnorm, &wmat, &gmat, &epart, &esite, &vsite};
+                   &pnorm, &wmat, &gmat, &epart, &esite, &vsite, &c_roff, &c_rcol,
+                   &c_rval, &c_coff, &c_crow, &c_cval, &qf, &pnf, &gf};
     for (DevBuf* d : b) d->release();
   }
 };

@@ -19,12 +19,17 @@ module tgx_io
      real(c_double), allocatable :: Qs(:), alphas(:), cutoff(:)
   end type two_body_block
 
-  ! one soap_turbo block: hyper-parameters and sparse set
+  ! one soap_turbo block: hyper-parameters and sparse set; with
+  ! compress_soap the P transformation read from file_compress_soap
   type, public :: soap_block
      type(tgx_soap_hypers) :: h
      integer :: dim = 0, n_sparse = 0
      real(c_double) :: delta = 1, zeta = 2
      real(c_double), allocatable :: Qs(:), alphas(:)
+     logical :: compress = .false.
+     integer :: dim_compressed = 0
+     integer(c_int32_t), allocatable :: cP_i(:), cP_j(:)
+     real(c_double), allocatable :: cP_el(:)
   end type soap_block
 
   type, public :: angle_3b_block
@@ -182,11 +187,57 @@ contains
     end do
   end function species_index
 
+  ! file_compress_soap (read_gap.f90:489-527): the first line holds n_max of
+  ! every species, l_max and the compressed dimension; then either a line
+  ! "P_transformation", lines "i j value" and a line "end_transformation",
+  ! or (the older format) one kept component index per line, weight 1.
+  subroutine read_compress_file(fname, ns, sb)
+    character(len=*), intent(in) :: fname
+    integer, intent(in) :: ns
+    type(soap_block), intent(inout) :: sb
+    integer :: u, k, n, ios, idum(TGX_MAX_SPECIES + 1)
+    character(len=1024) :: line
+    open (newunit=u, file=fname, status='old', action='read')
+    read (u, *) idum(1:ns + 1), sb%dim_compressed
+    read (u, '(A)') line
+    if (trim(adjustl(line)) == 'P_transformation') then
+       n = 0
+       do
+          read (u, '(A)', iostat=ios) line
+          if (ios /= 0) then
+             write (*, *) 'ERROR: no end_transformation in ', trim(fname)
+             stop 1
+          end if
+          if (trim(adjustl(line)) == 'end_transformation') exit
+          n = n + 1
+       end do
+       rewind (u)
+       read (u, *)
+       read (u, *)
+       allocate (sb%cP_i(n), sb%cP_j(n), sb%cP_el(n))
+       do k = 1, n
+          read (u, *) sb%cP_i(k), sb%cP_j(k), sb%cP_el(k)
+       end do
+    else
+       n = sb%dim_compressed
+       allocate (sb%cP_i(n), sb%cP_j(n), sb%cP_el(n))
+       read (line, *) sb%cP_j(1)
+       do k = 2, n
+          read (u, *) sb%cP_j(k)
+       end do
+       do k = 1, n
+          sb%cP_i(k) = k
+          sb%cP_el(k) = 1.0d0
+       end do
+    end if
+    close (u)
+  end subroutine read_compress_file
+
   subroutine read_gap_file(fname, m)
     character(len=*), intent(in) :: fname
     type(gap_model), intent(inout) :: m
-    character(len=1024) :: line, key, rest, fa, fd
-    character(len=64) :: kind
+    character(len=1024) :: line, key, rest, fa, fd, fcomp
+    character(len=64) :: kind, cmode, basis
     character(len=8) :: s1, s2, names(TGX_MAX_SPECIES)
     integer :: u, ios, ns, eq, k, ib
     character(len=8) :: s0
@@ -233,6 +284,8 @@ contains
              names = ''
              fa = ''
              fd = ''
+             fcomp = 'none'
+             cmode = 'none'
           end if
           if (in_2b) then
              b = two_body_block()
@@ -252,6 +305,25 @@ contains
                   end if
                end do
                call read_soap_sparse(trim(fa), trim(fd), sb)
+               if (sb%compress) then
+                  ! the recipes of compress_mode live in soap_turbo
+                  ! (get_compress_indices), which is not part of this engine
+                  if (trim(fcomp) /= 'none') then
+                     call read_compress_file(trim(fcomp), sb%h%n_species, sb)
+                  else if (trim(cmode) /= 'none') then
+                     write (*, *) 'ERROR: compress_mode = ', trim(cmode), &
+                          ' is not implemented; give file_compress_soap'
+                     stop 1
+                  else
+                     write (*, *) 'ERROR: compress_soap needs file_compress_soap'
+                     stop 1
+                  end if
+                  if (sb%dim_compressed /= sb%dim) then
+                     write (*, *) 'ERROR: compressed dimension ', sb%dim_compressed, &
+                          ' but the sparse set has ', sb%dim
+                     stop 1
+                  end if
+               end if
                ! every block on the model's species list (the first block's)
                if (ib == 1) then
                   m%n_species = sb%h%n_species
@@ -344,6 +416,21 @@ contains
                read (rest, *) fa
             case ('desc_sparse')
                read (rest, *) fd
+            case ('compress_soap')
+               read (rest, *) m%sb(ib)%compress
+            case ('file_compress_soap')
+               read (rest, *) fcomp
+            case ('compress_mode')
+               read (rest, *) cmode
+            case ('basis')
+               read (rest, *) basis
+               if (trim(basis) == 'poly3gauss') then
+                  h%basis = 1
+               else
+                  if (trim(basis) /= 'poly3') write (*, *) &
+                       'WARNING: basis ', trim(basis), ' not understood; using poly3'
+                  h%basis = 0
+               end if
             end select
           end associate
        else if (in_3b) then
@@ -402,16 +489,25 @@ contains
   ! the distance_2b, angle_3b and core_pot blocks.
   subroutine load_model(ctx, m)
     type(c_ptr), intent(in) :: ctx
-    type(gap_model), intent(in) :: m
+    type(gap_model), intent(in), target :: m
+    type(tgx_soap_hypers) :: h
     integer :: k
     do k = 1, m%n_soap
        associate (sb => m%sb(k))
+         h = sb%h
+         if (sb%compress) then
+            ! the library copies P when the block is set
+            h%compress_P_nonzero = int(size(sb%cP_i), c_int32_t)
+            h%compress_P_i = c_loc(m%sb(k)%cP_i)
+            h%compress_P_j = c_loc(m%sb(k)%cP_j)
+            h%compress_P_el = c_loc(m%sb(k)%cP_el)
+         end if
          if (k == 1) then
-            call tgx_check(ctx, tgx_set_soap_model(ctx, sb%h, int(sb%dim, c_int), &
+            call tgx_check(ctx, tgx_set_soap_model(ctx, h, int(sb%dim, c_int), &
                  int(sb%n_sparse, c_int), sb%Qs, sb%alphas, sb%delta, sb%zeta), &
                  'tgx_set_soap_model')
          else
-            call tgx_check(ctx, tgx_add_soap_model(ctx, sb%h, int(sb%dim, c_int), &
+            call tgx_check(ctx, tgx_add_soap_model(ctx, h, int(sb%dim, c_int), &
                  int(sb%n_sparse, c_int), sb%Qs, sb%alphas, sb%delta, sb%zeta), &
                  'tgx_add_soap_model')
          end if

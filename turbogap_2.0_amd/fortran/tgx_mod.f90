@@ -37,6 +37,13 @@ module tgx_mod
      real(c_double) :: central_weight(TGX_MAX_SPECIES) = 1.0d0
      real(c_double) :: global_scaling(TGX_MAX_SPECIES) = 1.0d0
      real(c_double) :: nf(TGX_MAX_SPECIES) = 4.0d0
+     ! compress_soap: number of entries of P and their arrays (1-based row,
+     ! column, value; c_loc of arrays that live while the model is set)
+     integer(c_int32_t) :: compress_P_nonzero = 0
+     integer(c_int32_t) :: pad2_ = 0
+     type(c_ptr) :: compress_P_i = c_null_ptr
+     type(c_ptr) :: compress_P_j = c_null_ptr
+     type(c_ptr) :: compress_P_el = c_null_ptr
   end type tgx_soap_hypers
 
   ! Delayed-acceptance options and Monte Carlo statistics (tgx_da_options,
@@ -71,7 +78,9 @@ module tgx_mod
        tgx_md_end, tgx_sync, tgx_poly3_orthonormalization, tgx_stage_times, tgx_check, &
        tgx_error_string, tgx_kernel_times, tgx_mc_run, tgx_set_list_buffer, tgx_mc_run_da, &
        tgx_surrogate_init, tgx_surrogate_set, tgx_surrogate_get, tgx_surrogate_predict, &
-       tgx_surrogate_gradient, tgx_surrogate_fit
+       tgx_surrogate_gradient, tgx_surrogate_fit, tgx_add_local_property, &
+       tgx_clear_local_properties, tgx_local_property, tgx_set_vdw_ts, tgx_clear_vdw, &
+       tgx_vdw_energy_forces
 
   interface
      integer(c_int) function tgx_create(device, precision, ctx) bind(C, name="tgx_create")

@@ -17,7 +17,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 from dataclasses import dataclass, field
-from typing import Optional, Sequence
+from typing import Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -64,6 +64,11 @@ class CSoapHypers(C.Structure):
         ("central_weight", C.c_double * MAX_SPECIES),
         ("global_scaling", C.c_double * MAX_SPECIES),
         ("nf", C.c_double * MAX_SPECIES),
+        ("compress_P_nonzero", C.c_int32),
+        ("pad2_", C.c_int32),
+        ("compress_P_i", C.POINTER(C.c_int32)),
+        ("compress_P_j", C.POINTER(C.c_int32)),
+        ("compress_P_el", C.POINTER(C.c_double)),
     ]
 
 
@@ -111,6 +116,19 @@ class SoapHypers:
     nf: Sequence[float] = (4.0,)
     radial_enhancement: int = 0
     central_species: int = 0
+    basis: str = "poly3"  # or "poly3gauss"
+    # compress_soap: (P_i, P_j, P_el) 1-based, and the compressed dimension
+    compress_P: Optional[Tuple[np.ndarray, np.ndarray, np.ndarray]] = None
+    compress_dim: int = 0
+
+    def __post_init__(self):
+        if self.compress_P is not None:
+            pi, pj, pe = self.compress_P
+            self.compress_P = (np.ascontiguousarray(pi, np.int32),
+                               np.ascontiguousarray(pj, np.int32),
+                               np.ascontiguousarray(pe, np.float64))
+            if not self.compress_dim:
+                self.compress_dim = int(self.compress_P[0].max())
 
     @classmethod
     def uniform(cls, n_species: int, **kw) -> "SoapHypers":
@@ -131,9 +149,15 @@ class SoapHypers:
         return int(sum(self.alpha_max))
 
     @property
-    def dim(self) -> int:
+    def full_dim(self) -> int:
+        """Dimension of the uncompressed power spectrum."""
         nr = self.n_r
         return nr * (nr + 1) // 2 * (self.l_max + 1)
+
+    @property
+    def dim(self) -> int:
+        """Descriptor dimension (the compressed one with compress_soap)."""
+        return self.compress_dim if self.compress_P is not None else self.full_dim
 
     @property
     def rcut_max(self) -> float:
@@ -145,7 +169,13 @@ class SoapHypers:
         h.l_max = self.l_max
         h.radial_enhancement = self.radial_enhancement
         h.central_species = self.central_species
-        h.basis = 0
+        h.basis = {"poly3": 0, "poly3gauss": 1}[self.basis]
+        if self.compress_P is not None:
+            pi, pj, pe = self.compress_P
+            h.compress_P_nonzero = len(pi)
+            h.compress_P_i = pi.ctypes.data_as(C.POINTER(C.c_int32))
+            h.compress_P_j = pj.ctypes.data_as(C.POINTER(C.c_int32))
+            h.compress_P_el = pe.ctypes.data_as(C.POINTER(C.c_double))
         for s in range(self.n_species):
             h.alpha_max[s] = int(self.alpha_max[s])
             h.rcut_hard[s] = float(self.rcut_hard[s])
