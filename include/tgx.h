@@ -449,6 +449,15 @@ int tgx_surrogate_fit(tgx_ctx* ctx, int n, const double* q, const double* y,
  * cond(S) ~ 1e13 at alpha_max = 8. */
 int tgx_poly3_orthonormalization(int alpha_max, double* S, double* W);
 
+/* The same for the poly3gauss basis (a normalised Gaussian of width
+ * atom_sigma at the origin, then alpha_max - 1 normalised poly3 functions on
+ * [0, rcut_hard]); replaces get_orthonormalization_matrix_poly3gauss
+ * (gpu/gpu_interface.f90:431-437, implemented in soap_turbo, which is not
+ * in the reference tree: this basis is the engine's reading of it).
+ * Host-only.  S is formed in binary128, so W is exact to FP64 rounding. */
+int tgx_poly3gauss_orthonormalization(int alpha_max, double atom_sigma, double rcut_hard,
+                                      double* S, double* W);
+
 /*
  * The same dynamics split for callers that keep the state on the device
  * between calls (the Fortran md driver, benchmarks):

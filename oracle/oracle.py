@@ -147,6 +147,15 @@ def poly3_overlap(nmax):
     return S, W
 
 
+def poly3gauss_overlap(nmax, rh, sigma):
+    """Overlap S and W = S^(-1/2) of the poly3gauss basis (a Gaussian of
+    width sigma at the origin, then nmax - 1 poly3 functions)."""
+    S = np.zeros((nmax, nmax))
+    W = np.zeros((nmax, nmax))
+    lib().orc_poly3gauss_overlap(nmax, C.c_double(rh), C.c_double(sigma), _d(S), _d(W))
+    return S, W
+
+
 def radial(h: SoapHypers, s: int, r):
     """R_n(r), dR_n/dr of species s (0-based)."""
     r = _f64(np.atleast_1d(r))

@@ -53,6 +53,7 @@
 #include "tgx_oracle.h"
 
 #include <math.h>
+#include <quadmath.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -556,6 +557,57 @@ void orc_poly3_overlap(int nmax, double* S, double* W) {
     }
 }
 
+/* poly3gauss basis (soap_turbo's get_orthonormalization_matrix_poly3gauss,
+ * called at gpu/gpu_interface.f90:431): phi_1 = N_g exp(-r^2/(2 sigma^2)),
+ * the atom's Gaussian at the origin, then the first alpha_max - 1 poly3
+ * functions N_k (rh - r)^(k+2), all normalised on [0, rh] with weight 1:
+ *   S_11 = 1, S_1k = N_g N_k int_0^rh exp(-r^2/(2 sigma^2)) (rh - r)^(k+2) dr,
+ *   S_kl = sqrt((2k+5)(2l+5)) / (k+l+5);
+ * N_g^-2 = int_0^rh exp(-r^2/sigma^2) dr = sqrt(pi)/2 sigma erf(rh/sigma).
+ * The Gaussian-polynomial integrals come from the recurrence of radial_J
+ * at r = 0, all in binary128 (cond(S) is about 1e13, so W must come from an
+ * S exact to far below FP64 rounding). */
+static q128 q_expq(q128 x) { return expq(x); }
+
+void orc_poly3gauss_overlap(int nmax, double rh_in, double sigma_in, double* S, double* W) {
+  q128 A[TGX_MAX_NR * TGX_MAX_NR], V[TGX_MAX_NR * TGX_MAX_NR], w[TGX_MAX_NR];
+  const q128 rh = rh_in, sg = sigma_in, s2 = sg * sg;
+  const q128 pi = M_PIq;
+  /* J_m = int_0^rh v^m exp(-(v - rh)^2/(2 s^2)) dv, v = rh - r */
+  q128 J[TGX_MAX_NR + 8];
+  const q128 er = 1, ec = q_expq(-rh * rh / (2 * s2));
+  J[0] = sg * q_sqrt(pi / 2) * erfq(rh / (sg * q_sqrt((q128)2)));
+  J[1] = rh * J[0] + s2 * (ec - er);
+  q128 rhp = 1;
+  for (int k = 2; k < nmax + 4; k++) {
+    rhp *= rh;
+    J[k] = rh * J[k - 1] + (k - 1) * s2 * J[k - 2] - s2 * rhp * er;
+  }
+  const q128 Ng = 1 / q_sqrt(q_sqrt(pi) / 2 * sg * erfq(rh / sg));
+  for (int a = 0; a < nmax; a++)
+    for (int b = 0; b < nmax; b++) {
+      q128 v;
+      if (a == 0 && b == 0) {
+        v = 1;
+      } else if (a == 0 || b == 0) {
+        const int k = a + b; /* the polynomial's index */
+        const q128 Nk = q_sqrt((q128)(2 * k + 5)) / powq(rh, (q128)(2 * k + 5) / 2);
+        v = Ng * Nk * J[k + 2];
+      } else {
+        v = q_sqrt((q128)((2 * a + 5) * (2 * b + 5))) / (q128)(a + b + 5);
+      }
+      A[a * nmax + b] = v;
+      S[a * nmax + b] = (double)v;
+    }
+  jacobi_eig_q(nmax, A, V, w);
+  for (int i = 0; i < nmax; i++)
+    for (int j = 0; j < nmax; j++) {
+      q128 acc = 0;
+      for (int k = 0; k < nmax; k++) acc += V[i * nmax + k] * V[j * nmax + k] / q_sqrt(w[k]);
+      W[i * nmax + j] = (double)acc;
+    }
+}
+
 /* J_k = int_0^rh u^k exp(-(u-c)^2/(2 s^2)) du, c = rh - r, k = 0..kmax. */
 static void radial_J(double r, double rh, double s, int kmax, double* J) {
   double c = rh - r, s2 = s * s;
@@ -588,7 +640,7 @@ void orc_radial_raw(double r, double rh, double sig, int p, int nk,
 /* Unnormalised (no W) basis integrals I_a(r) (a = 1..nmax) and dI_a/dr for
  * a Gaussian of width sig0 + sigs*r centred at r. */
 static void radial_I(double r, double rh, double sig0, double sigs, int p,
-                     int nmax, double* I, double* dI) {
+                     int nmax, int gauss, double* I, double* dI) {
   double s = sig0 + sigs * r;
   double J[64];
   int kmax = nmax + 2 + p + 2;
@@ -600,16 +652,56 @@ static void radial_I(double r, double rh, double sig0, double sigs, int p,
     dJc[k] = (J[k + 1] - c * J[k]) / s2;
     dJs[k] = (J[k + 2] - 2.0 * c * J[k + 1] + c * c * J[k]) / (s2 * s);
   }
-  for (int a = 1; a <= nmax; a++) {
+  /* poly3gauss: slot 0 is the Gaussian, slots 1.. the poly3 functions 1.. */
+  for (int a = 1; a <= nmax - gauss; a++) {
     int d = a + 2;
     double Na = sqrt((2.0 * a + 5.0) / pow(rh, 2.0 * a + 5.0));
     double T = poly_T(J, d, p, rh);
     double Tc = poly_T(dJc, d, p, rh);
     double Ts = poly_T(dJs, d, p, rh);
-    I[a - 1] = Na * pre * T;
+    I[a - 1 + gauss] = Na * pre * T;
     if (dI)
-      dI[a - 1] = Na * (pre * (-Tc + Ts * sigs) - pre / s * sigs * T);
+      dI[a - 1 + gauss] = Na * (pre * (-Tc + Ts * sigs) - pre / s * sigs * T);
   }
+  if (gauss) {
+    /* int_0^rh exp(-u^2/(2 sb^2)) u^p G(u; r, s) du with sb = sig0: the
+     * product of the two Gaussians is exp(-r^2/(2 (sb^2 + s^2))) times a
+     * Gaussian of centre mu = r sb^2/(sb^2+s^2) and width tau = sb s /
+     * sqrt(sb^2+s^2); M_k = int_0^rh u^k exp(-(u-mu)^2/(2 tau^2)) du by
+     * parts: M_k = mu M_(k-1) + (k-1) tau^2 M_(k-2) - tau^2 [u^(k-1) e]_0^rh. */
+    const double sb2 = sig0 * sig0, Sg = sb2 + s * s;
+    const double A = exp(-r * r / (2.0 * Sg));
+    const double mu = r * sb2 / Sg, t2 = sb2 * s * s / Sg, t = sqrt(t2);
+    const double e0 = exp(-mu * mu / (2.0 * t2)), eh = exp(-(rh - mu) * (rh - mu) / (2.0 * t2));
+    double M[5];
+    M[0] = t * sqrt(PI / 2.0) * (erf((rh - mu) / (t * sqrt(2.0))) + erf(mu / (t * sqrt(2.0))));
+    M[1] = mu * M[0] - t2 * (eh - e0);
+    double rp = 1.0;
+    for (int k = 2; k < 5; k++) {
+      rp *= rh;
+      M[k] = mu * M[k - 1] + (k - 1) * t2 * M[k - 2] - t2 * rp * eh;
+    }
+    const double Ng = 1.0 / sqrt(sqrt(PI) / 2.0 * sig0 * erf(rh / sig0));
+    I[0] = Ng * pre * A * M[p];
+    if (dI) {
+      const double dMmu = (M[p + 1] - mu * M[p]) / t2;
+      const double dMt = (M[p + 2] - 2.0 * mu * M[p + 1] + mu * mu * M[p]) / (t2 * t);
+      const double dmu = sb2 / Sg - r * sb2 * 2.0 * s * sigs / (Sg * Sg);
+      const double dt = sigs * sig0 * sb2 / (Sg * sqrt(Sg));
+      const double dA = A * (-r / Sg + r * r * s * sigs / (Sg * Sg));
+      dI[0] = Ng * (-pre * sigs / s * A * M[p] + pre * dA * M[p] +
+                    pre * A * (dMmu * dmu + dMt * dt));
+    }
+  }
+}
+
+/* W = S^(-1/2) of species s's basis (alpha_max x alpha_max, row-major). */
+static void basis_W(const tgx_soap_hypers* h, int s, double* W) {
+  double S[TGX_MAX_NR * TGX_MAX_NR];
+  if (h->basis == 1)
+    orc_poly3gauss_overlap(h->alpha_max[s], h->rcut_hard[s], h->atom_sigma_r[s], S, W);
+  else
+    orc_poly3_overlap(h->alpha_max[s], S, W);
 }
 
 /* Amplitude (polynomial scaling, global scaling) times soft cutoff. */
@@ -656,7 +748,7 @@ static void radial_W(const tgx_soap_hypers* h, int s, const double* W, int nr,
       continue;
     }
     radial_I(rr, h->rcut_hard[s], h->atom_sigma_r[s],
-             h->atom_sigma_r_scaling[s], h->radial_enhancement, nmax, I, dI);
+             h->atom_sigma_r_scaling[s], h->radial_enhancement, nmax, h->basis == 1, I, dI);
     radial_envelope(h, s, rr, &e, &de);
     for (int n = 0; n < nmax; n++) {
       double v = 0.0, dv = 0.0;
@@ -672,8 +764,8 @@ static void radial_W(const tgx_soap_hypers* h, int s, const double* W, int nr,
 
 void orc_radial(const tgx_soap_hypers* h, int s, int nr, const double* r,
                 double* R, double* dR) {
-  double S[TGX_MAX_NR * TGX_MAX_NR], W[TGX_MAX_NR * TGX_MAX_NR];
-  orc_poly3_overlap(h->alpha_max[s], S, W);
+  double W[TGX_MAX_NR * TGX_MAX_NR];
+  basis_W(h, s, W);
   radial_W(h, s, W, nr, r, R, dR);
 }
 
@@ -683,7 +775,7 @@ static void radial_self(const tgx_soap_hypers* h, int s, const double* W,
   int nmax = h->alpha_max[s];
   double I[TGX_MAX_NR];
   radial_I(0.0, h->rcut_hard[s], h->atom_sigma_r[s], 0.0,
-           h->radial_enhancement, nmax, I, NULL);
+           h->radial_enhancement, nmax, h->basis == 1, I, NULL);
   for (int n = 0; n < nmax; n++) {
     double v = 0.0;
     for (int a = 0; a < nmax; a++) v += W[n * nmax + a] * I[a];
@@ -768,8 +860,7 @@ static void soap_geom_init(const tgx_soap_hypers* h, soap_geom* g) {
     g->off[s] = g->nr;
     g->nmax[s] = h->alpha_max[s];
     g->nr += h->alpha_max[s];
-    double S[TGX_MAX_NR * TGX_MAX_NR];
-    orc_poly3_overlap(h->alpha_max[s], S, g->W[s]);
+    basis_W(h, s, g->W[s]);
   }
   g->L = h->l_max;
   g->L2 = (h->l_max + 1) * (h->l_max + 1);
@@ -990,7 +1081,7 @@ int orc_soap_coeffs(const tgx_soap_hypers* h, int si, int np, const double* x, c
 static int hypers_supported(const tgx_soap_hypers* h) {
   for (int s = 0; s < h->n_species; s++)
     if (h->nf[s] != 4.0) return TGX_ERR_UNSUPPORTED;
-  if (h->basis != 0) return TGX_ERR_UNSUPPORTED;
+  if (h->basis != 0 && h->basis != 1) return TGX_ERR_UNSUPPORTED;
   if (h->compress_P_nonzero > 0) {
     int nr = 0;
     for (int s = 0; s < h->n_species; s++) nr += h->alpha_max[s];

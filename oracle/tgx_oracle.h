@@ -91,6 +91,10 @@ int orc_neighbors(int n, const double* pos, const double* lat, double rcut,
 
 /* Overlap S and W = S^(-1/2) of the poly3 basis (alpha_max x alpha_max). */
 void orc_poly3_overlap(int alpha_max, double* S, double* W);
+/* The poly3gauss basis' overlap S and W = S^(-1/2) (alpha_max x alpha_max,
+ * row-major): a Gaussian of width sigma at the origin, then alpha_max - 1
+ * poly3 functions, normalised on [0, rh]. */
+void orc_poly3gauss_overlap(int alpha_max, double rh, double sigma, double* S, double* W);
 
 /* Radial coefficients R_n(r) and dR_n/dr of neighbour species s (0-based)
  * for nr distances; R and dR are nr x alpha_max[s] (n fastest). */

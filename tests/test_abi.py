@@ -68,6 +68,20 @@ def test_orthonormalization_matches_oracle_bitwise(nmax):
     assert np.abs(S1 - S2).max() <= 2.3e-16
 
 
+@pytest.mark.parametrize("nmax,sg,rh", [(1, 0.5, 3.7), (4, 0.5, 3.7), (8, 0.5, 3.7),
+                                         (8, 0.3, 5.0), (12, 0.45, 4.0), (16, 0.5, 4.5)])
+def test_poly3gauss_orthonormalization_matches_oracle(nmax, sg, rh):
+    """Library (binary128, libquadmath) and oracle (its own binary128) W of
+    the poly3gauss basis agree to FP64 rounding."""
+    S1, W1 = tgx.poly3gauss_orthonormalization(nmax, sg, rh)
+    S2, W2 = oracle.poly3gauss_overlap(nmax, rh, sg)
+    assert np.abs(S1 - S2).max() <= 4.5e-16
+    # at n_max = 16 cond(S) exceeds 1e20, so the two binary128 computations
+    # (their own square roots and Jacobi sweeps) part at about 1e-12
+    tol = 4e-16 if nmax <= 14 else 1e-11
+    assert np.abs(W1 - W2).max() <= tol * np.abs(W2).max()
+
+
 def test_orthonormalization_rejects_bad_size():
     lib = tgx.load_library()
     W = np.zeros(4)

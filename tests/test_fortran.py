@@ -283,8 +283,8 @@ def test_fortran_predict_angle_3b_core_pot(tmp_path):
 def test_fortran_predict_compressed_blocks(tmp_path):
     """Two compressed soap_turbo blocks read from .gap files: a general P in
     the P_transformation format (block 1) and a selection of components in
-    the older one-index-per-line format (block 2, central species H); the
-    predict driver against the oracle."""
+    the older one-index-per-line format (block 2, central species H, basis
+    poly3gauss); the predict driver against the oracle."""
     import dataclasses
 
     import structures as S
@@ -297,7 +297,7 @@ def test_fortran_predict_compressed_blocks(tmp_path):
     h = S.cho_hypers()
     h1 = dataclasses.replace(h, compress_P=TC._random_P(h, 200, 3), compress_dim=200)
     sel = np.sort(np.random.default_rng(5).choice(h.full_dim, 300, replace=False)) + 1
-    h2 = dataclasses.replace(h, central_species=2, compress_dim=300,
+    h2 = dataclasses.replace(h, central_species=2, compress_dim=300, basis="poly3gauss",
                              compress_P=(np.arange(1, 301), sel, np.ones(300)))
     m1 = S.model_from_soaps(h1, oracle.soap(h1, pos, sp, lat)["soap"], 50, seed=1)
     m2 = S.model_from_soaps(h2, oracle.soap(h2, pos, sp, lat)["soap"], 30, seed=2)

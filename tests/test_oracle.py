@@ -42,6 +42,59 @@ def test_poly3_overlap_vs_quadrature(nmax):
             assert abs(v - S_[a - 1, b - 1]) < 1e-12
 
 
+@pytest.mark.parametrize("nmax,rh,sg", [(1, 3.7, 0.5), (4, 3.7, 0.5), (8, 3.7, 0.5),
+                                         (8, 5.0, 0.3), (12, 4.0, 0.45)])
+def test_poly3gauss_overlap_vs_quadrature(nmax, rh, sg):
+    """The poly3gauss overlap: a Gaussian of width sigma at the origin and
+    n_max - 1 poly3 functions, each normalised on [0, rh] by quadrature."""
+    h = tgx.SoapHypers.uniform(1, alpha_max=nmax, rcut_hard=rh, atom_sigma_r=sg)
+    h = tgx.SoapHypers(**{**h.__dict__, "basis": "poly3gauss"})
+    S_, W_ = oracle.poly3gauss_overlap(nmax, rh, sg)
+    fs = _basis_functions(h, 0)
+    for a in range(nmax):
+        for b in range(nmax):
+            v = quad(lambda u: fs[a](u) * fs[b](u), 0, rh, epsabs=1e-15, epsrel=1e-13)[0]
+            assert abs(v - S_[a, b]) < 2e-12, (a, b, v, S_[a, b])
+
+
+@pytest.mark.parametrize("nmax,tol", [(4, 1e-13), (8, 1e-9), (12, 1e-6)])
+def test_poly3gauss_W_is_inverse_sqrt(nmax, tol):
+    """W S W = I with S evaluated exactly (50 digits, mpmath's erf and
+    quadrature) and the FP64 W of the oracle."""
+    mp.mp.dps = 50
+    rh, sg = mp.mpf("3.7"), mp.mpf("0.5")
+    _, W_ = oracle.poly3gauss_overlap(nmax, 3.7, 0.5)
+    assert np.array_equal(W_, W_.T)
+    g = lambda u: mp.exp(-u * u / (2 * sg * sg))
+    fs = [g] + [lambda u, a=a: (rh - u) ** (a + 2) for a in range(1, nmax)]
+    nrm = [1 / mp.sqrt(mp.quad(lambda u: f(u) ** 2, [0, rh])) for f in fs]
+    Sm = mp.matrix(nmax, nmax)
+    for a in range(nmax):
+        for b in range(nmax):
+            Sm[a, b] = nrm[a] * nrm[b] * mp.quad(lambda u: fs[a](u) * fs[b](u), [0, rh])
+    R = mp.matrix(W_.tolist()) * Sm * mp.matrix(W_.tolist()) - mp.eye(nmax)
+    assert float(max(abs(x) for x in R)) < tol
+    assert np.all(np.linalg.eigvalsh(W_) > 0)
+
+
+@pytest.mark.parametrize("p,sigs", [(0, 0.0), (1, 0.0), (2, 0.0), (0, 0.08), (2, 0.1)])
+def test_poly3gauss_radial_vs_quadrature_and_derivative(p, sigs):
+    """R_n of the poly3gauss basis (W times the integrals of each basis
+    function against the smeared density u^p G) against adaptive
+    quadrature, with r-dependent width sigma_r + sigma_r_scaling r; dR/dr by
+    five-point differences."""
+    h = tgx.SoapHypers.uniform(1, alpha_max=8, rcut_hard=3.7, buffer=0.0, atom_sigma_r=0.5,
+                               atom_sigma_r_scaling=sigs, amplitude_scaling=0.0)
+    h = tgx.SoapHypers(**{**h.__dict__, "radial_enhancement": p, "basis": "poly3gauss"})
+    r = np.array([0.05, 0.6, 1.3, 2.2, 3.0, 3.6])
+    R, dR = oracle.radial(h, 0, r)
+    for k, rk in enumerate(r):
+        ref = _radial_quad(h, 0, rk, 0.5 + sigs * rk)
+        assert np.abs(R[k] - ref).max() < 1e-8 * max(1.0, np.abs(ref).max()), (rk, R[k], ref)
+    fd = _fd5(lambda x: oracle.radial(h, 0, x)[0], r, 1e-3)
+    assert np.abs(fd - dR).max() < 1e-6 * max(1.0, np.abs(dR).max())
+
+
 @pytest.mark.parametrize("nmax,tol", [(4, 1e-13), (8, 1e-9), (12, 1e-6)])
 def test_W_is_inverse_sqrt(nmax, tol):
     """W S W = I evaluated exactly (50 digits) with the FP64 W; W symmetric."""
@@ -843,20 +896,40 @@ def _sphere_grid(nz=200, nphi=400):
     return th, PH, U, W
 
 
+def _basis_functions(h, s):
+    """The unnormalised radial basis functions of species s, normalised here
+    by quadrature: poly3 (rh - u)^(a+2), a = 1..n_max; poly3gauss the
+    Gaussian exp(-u^2/(2 sigma_r^2)) then (rh - u)^(a+2), a = 1..n_max-1."""
+    rh, nmax = h.rcut_hard[s], h.alpha_max[s]
+    if h.basis == "poly3gauss":
+        sg = h.atom_sigma_r[s]
+        fs = [lambda u: math.exp(-u * u / (2 * sg * sg))]
+        fs += [lambda u, a=a: (rh - u) ** (a + 2) for a in range(1, nmax)]
+    else:
+        fs = [lambda u, a=a: (rh - u) ** (a + 2) for a in range(1, nmax + 1)]
+    norms = [1 / math.sqrt(quad(lambda u: f(u) ** 2, 0, rh, epsabs=1e-16, epsrel=1e-13)[0])
+             for f in fs]
+    return [lambda u, f=f, c=c: c * f(u) for f, c in zip(fs, norms)]
+
+
+def _basis_W(h, s):
+    if h.basis == "poly3gauss":
+        return oracle.poly3gauss_overlap(h.alpha_max[s], h.rcut_hard[s], h.atom_sigma_r[s])[1]
+    return oracle.poly3_overlap(h.alpha_max[s])[1]
+
+
 def _radial_quad(h, s, r, sig):
     """sum_a W_na int_0^rh phi_a(u) u^p G_sig(u - r) du by adaptive
-    quadrature (G the unit-area Gaussian), phi_a = N_a (rh - u)^(a+2)."""
+    quadrature (G the unit-area Gaussian), phi_a the normalised basis."""
     rh = h.rcut_hard[s]
     p = h.radial_enhancement
-    nmax = h.alpha_max[s]
-    _, W = oracle.poly3_overlap(nmax)
-    I = np.zeros(nmax)
-    for a in range(1, nmax + 1):
-        Na = math.sqrt((2 * a + 5) / rh ** (2 * a + 5))
-        f = lambda u: (Na * (rh - u) ** (a + 2) * u ** p
+    W = _basis_W(h, s)
+    I = np.zeros(h.alpha_max[s])
+    for a, phi in enumerate(_basis_functions(h, s)):
+        f = lambda u: (phi(u) * u ** p
                        * math.exp(-(u - r) ** 2 / (2 * sig * sig)) / (math.sqrt(2 * math.pi) * sig))
-        I[a - 1] = quad(f, 0.0, rh, points=[min(max(r, 1e-9), rh)], epsabs=1e-15, epsrel=1e-13,
-                        limit=200)[0]
+        I[a] = quad(f, 0.0, rh, points=[min(max(r, 1e-9), rh)], epsabs=1e-15, epsrel=1e-13,
+                    limit=200)[0]
     return W @ I
 
 
@@ -895,10 +968,16 @@ def _cnlm_cases():
     for st, p in ((0.3, 0), (0.3, 2), (0.5, 1), (0.25, 0)):
         for rs in ((2.1,), (1.7,), (2.9,), (1.8, 2.6), (1.6, 2.2, 3.0)):
             x = dirs(len(rs)) * np.array(rs)[:, None]
-            cases.append((st, p, 1, x, [0] * len(rs)))
+            cases.append((st, p, 1, x, [0] * len(rs), "poly3"))
     # two species: neighbours of both, central species 1
     x = dirs(3) * np.array([1.9, 2.4, 2.8])[:, None]
-    cases.append((0.3, 0, 2, x, [0, 1, 1]))
+    cases.append((0.3, 0, 2, x, [0, 1, 1], "poly3"))
+    # the poly3gauss basis
+    for st, p, rs in ((0.3, 0, (2.1,)), (0.3, 1, (1.8, 2.6)), (0.5, 2, (1.6, 2.2, 3.0))):
+        x = dirs(len(rs)) * np.array(rs)[:, None]
+        cases.append((st, p, 1, x, [0] * len(rs), "poly3gauss"))
+    x = dirs(3) * np.array([1.9, 2.4, 2.8])[:, None]
+    cases.append((0.3, 0, 2, x, [0, 1, 1], "poly3gauss"))
     return cases
 
 
@@ -916,11 +995,11 @@ def test_cnlm_vs_3d_quadrature(case, sphere_grid):
     test allows 0.6 l(l+1)/kappa^2 (tabulated for kappa >= 10, l <= 8).
     Amplitude scaling off and every r_j inside rcut_soft, so the envelope
     is 1; sigma_r 0.4 so the radial Gaussian is not a delta."""
-    st, p, nsp, x, sj = case
+    st, p, nsp, x, sj, basis = case
     h = tgx.SoapHypers.uniform(nsp, alpha_max=(6 if nsp == 1 else 5), l_max=8, rcut_hard=4.0,
                            buffer=0.5, atom_sigma_r=0.4, atom_sigma_t=st, amplitude_scaling=0.0,
                            central_weight=0.7)
-    h = tgx.SoapHypers(**{**h.__dict__, "radial_enhancement": p})
+    h = tgx.SoapHypers(**{**h.__dict__, "radial_enhancement": p, "basis": basis})
     si = nsp - 1
     got = oracle.soap_coeffs(h, si, x, sj)
     ref = _cnlm_by_quadrature(h, si, x, sj, sphere_grid)

@@ -1363,7 +1363,8 @@ static int load_soap_block(tgx_ctx* c, const tgx_soap_hypers* h, int dim, int n_
     return set_err(c, TGX_ERR_ARG, "n_species %d not in 1..%d", h->n_species, TGX_MAX_SPECIES);
   if (h->l_max < 0 || h->l_max > TGX_MAX_LMAX)
     return set_err(c, TGX_ERR_ARG, "l_max %d not in 0..%d", h->l_max, TGX_MAX_LMAX);
-  if (h->basis != 0) return set_err(c, TGX_ERR_UNSUPPORTED, "only the poly3 basis is implemented");
+  if (h->basis != 0 && h->basis != 1)
+    return set_err(c, TGX_ERR_ARG, "basis %d: 0 (poly3) or 1 (poly3gauss)", h->basis);
   if (h->radial_enhancement < 0 || h->radial_enhancement > 2)
     return set_err(c, TGX_ERR_ARG, "radial_enhancement must be 0, 1 or 2");
   SoapDev sd{};
@@ -1372,6 +1373,7 @@ static int load_soap_block(tgx_ctx* c, const tgx_soap_hypers* h, int dim, int n_
   sd.L2 = (h->l_max + 1) * (h->l_max + 1);
   sd.radial_enhancement = h->radial_enhancement;
   sd.central_species = h->central_species;
+  sd.basis = h->basis;
   int nr = 0;
   double rcm = 0;
   for (int s = 0; s < h->n_species; s++) {
@@ -1398,6 +1400,9 @@ static int load_soap_block(tgx_ctx* c, const tgx_soap_hypers* h, int dim, int n_
     sd.sig_ts[s] = h->atom_sigma_t_scaling[s];
     sd.amp_scal[s] = h->amplitude_scaling[s];
     sd.glob[s] = h->global_scaling[s];
+    // the poly3gauss Gaussian's normalisation: (int_0^rh exp(-r^2/sig^2) dr)^(-1/2)
+    sd.gnorm[s] = 1.0 / std::sqrt(std::sqrt(M_PI) / 2.0 * h->atom_sigma_r[s] *
+                                  std::erf(h->rcut_hard[s] / h->atom_sigma_r[s]));
     rcm = std::max(rcm, h->rcut_hard[s]);
   }
   if (nr > TGX_MAX_NR) return set_err(c, TGX_ERR_ARG, "sum of alpha_max %d > %d", nr, TGX_MAX_NR);
@@ -1436,7 +1441,13 @@ static int load_soap_block(tgx_ctx* c, const tgx_soap_hypers* h, int dim, int n_
   if (n_sparse < 1) return set_err(c, TGX_ERR_ARG, "n_sparse must be >= 1");
   // W per species and the central-atom coefficients
   std::vector<double> W((size_t)TGX_MAX_SPECIES * TGX_MAX_NR * TGX_MAX_NR, 0.0);
-  for (int s = 0; s < h->n_species; s++) poly3_W(h->alpha_max[s], W.data() + (size_t)s * TGX_MAX_NR * TGX_MAX_NR);
+  for (int s = 0; s < h->n_species; s++) {
+    double* Ws = W.data() + (size_t)s * TGX_MAX_NR * TGX_MAX_NR;
+    if (h->basis == 1)
+      poly3gauss_W(h->alpha_max[s], h->rcut_hard[s], h->atom_sigma_r[s], nullptr, Ws);
+    else
+      poly3_W(h->alpha_max[s], Ws);
+  }
   for (int s = 0; s < h->n_species; s++) {
     double R[TGX_MAX_NR];
     SoapDev tmp = sd;
@@ -3195,6 +3206,21 @@ int tgx_poly3_orthonormalization(int alpha_max, double* S, double* W) {
       if (S)
         S[j * alpha_max + i] =
             std::sqrt((2.0 * i + 7.0) * (2.0 * j + 7.0)) / (double)(i + j + 7);
+    }
+  return TGX_OK;
+}
+
+int tgx_poly3gauss_orthonormalization(int alpha_max, double atom_sigma, double rcut_hard,
+                                      double* S, double* W) {
+  if (alpha_max < 1 || alpha_max > TGX_MAX_NR || !W || !(atom_sigma > 0.0) ||
+      !(rcut_hard > 0.0) || !std::isfinite(atom_sigma) || !std::isfinite(rcut_hard))
+    return TGX_ERR_ARG;
+  std::vector<double> w((size_t)TGX_MAX_NR * TGX_MAX_NR), sm((size_t)alpha_max * alpha_max);
+  poly3gauss_W(alpha_max, rcut_hard, atom_sigma, sm.data(), w.data());
+  for (int i = 0; i < alpha_max; i++)
+    for (int j = 0; j < alpha_max; j++) {
+      W[j * alpha_max + i] = w[(size_t)i * TGX_MAX_NR + j];
+      if (S) S[j * alpha_max + i] = sm[(size_t)i * alpha_max + j];
     }
   return TGX_OK;
 }

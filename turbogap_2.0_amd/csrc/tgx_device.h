@@ -239,14 +239,17 @@ __device__ inline void radial_basis_fp64(const SoapDev* sd, const double* W,
   }
   const double pre = 1.0 / (2.5066282746310005024 * sg);
   double I[NMX], dI[NMX];
+  // poly3gauss: slot 0 is the Gaussian, slot a the poly3 function a
+  const int g = sd->basis == 1 ? 1 : 0;
 #pragma unroll
   for (int a = 1; a <= NMX; a++) {
     if (a > nmax) {
       I[a - 1] = dI[a - 1] = 0.0;
       continue;
     }
-    const int d = a + 2;
-    double Na = sqrt((2.0 * a + 5.0) / pow(rh, 2.0 * a + 5.0));
+    const int d = a + 2 - g;  // degree: (rh - r)^(d) of poly3 function a - g
+    const int ak = a - g;
+    double Na = sqrt((2.0 * ak + 5.0) / pow(rh, 2.0 * ak + 5.0));
     double T, Tc, Ts;
     double j0 = J[d], j1 = J[d + 1], j2 = J[d + 2], j3 = J[d + 3], j4 = J[d + 4];
     // dJ_k/dc = (J_{k+1} - c J_k)/s2 ; dJ_k/dsig = (J_{k+2} - 2c J_{k+1} + c^2 J_k)/s2/sg
@@ -265,6 +268,38 @@ __device__ inline void radial_basis_fp64(const SoapDev* sd, const double* W,
     }
     I[a - 1] = Na * pre * T;
     dI[a - 1] = Na * (pre * (-Tc + Ts * sigs) - pre / sg * sigs * T);
+  }
+  if (g) {
+    // slot 0: int_0^rh exp(-u^2/(2 sb^2)) u^p G(u; r, sg) du, sb = sig_r.
+    // The product of the two Gaussians is exp(-r^2/(2 Sg)), Sg = sb^2 +
+    // sg^2, times a Gaussian of centre mu = r sb^2/Sg and width t = sb sg /
+    // sqrt(Sg); M_k = int_0^rh u^k exp(-(u-mu)^2/(2 t^2)) du by parts:
+    // M_k = mu M_(k-1) + (k-1) t^2 M_(k-2) - t^2 [u^(k-1) e]_0^rh.  The r
+    // derivative goes through mu, t (sg = sig_r + sig_rs r) and the factor.
+    const double sb = sd->sig_r[s], sb2 = sb * sb, Sg = sb2 + s2;
+    const double A = exp(-r * r / (2.0 * Sg));
+    const double mu = r * sb2 / Sg, t2 = sb2 * s2 / Sg, t = sqrt(t2);
+    const double e0 = exp(-mu * mu / (2.0 * t2));
+    const double eh = exp(-(rh - mu) * (rh - mu) / (2.0 * t2));
+    double M[5];
+    M[0] = t * 1.2533141373155002512 *
+           (erf((rh - mu) / (t * 1.4142135623730950488)) + erf(mu / (t * 1.4142135623730950488)));
+    M[1] = mu * M[0] - t2 * (eh - e0);
+    double rp = 1.0;
+#pragma unroll
+    for (int k = 2; k < 5; k++) {
+      rp *= rh;
+      M[k] = mu * M[k - 1] + (k - 1) * t2 * M[k - 2] - t2 * rp * eh;
+    }
+    const double Mp = M[p], Mp1 = M[p + 1], Mp2 = M[p + 2];
+    const double dMmu = (Mp1 - mu * Mp) / t2;
+    const double dMt = (Mp2 - 2.0 * mu * Mp1 + mu * mu * Mp) / (t2 * t);
+    const double dmu = sb2 / Sg - r * sb2 * 2.0 * sg * sigs / (Sg * Sg);
+    const double dt = sigs * sb * sb2 / (Sg * sqrt(Sg));
+    const double dA = A * (-r / Sg + r * r * sg * sigs / (Sg * Sg));
+    const double ng = sd->gnorm[s];
+    I[0] = ng * pre * A * Mp;
+    dI[0] = ng * (-pre * sigs / sg * A * Mp + pre * dA * Mp + pre * A * (dMmu * dmu + dMt * dt));
   }
   const double* Ws = W + s * TGX_MAX_NR * TGX_MAX_NR;
 #pragma unroll

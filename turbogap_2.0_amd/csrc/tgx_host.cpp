@@ -2,6 +2,8 @@
 // matrix W = S^(-1/2) (computed in binary128 and rounded once, because
 // cond(S) ~ 1e13 at n_max = 8), the FP64 radial closed form used for the
 // central-atom term, spherical-harmonic recurrence constants and the cell.
+#include <quadmath.h>
+
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -78,6 +80,49 @@ void poly3_W(int nmax, double* W) {
     }
 }
 
+// The poly3gauss basis on [0, rh], every function normalised (weight 1):
+// slot 0 the Gaussian g(r) = exp(-r^2/(2 sigma^2)), slot k = 1..nmax-1 the
+// poly3 function (rh - r)^(k+2).  S_00 = 1, S_kl as poly3, and
+// S_0k = N_g N_k int_0^rh g(r) (rh - r)^(k+2) dr, the integrals by the
+// recurrence J_m = rh J_(m-1) + (m-1) sigma^2 J_(m-2) - sigma^2 rh^(m-1) of
+// J_m = int_0^rh v^m exp(-(v - rh)^2/(2 sigma^2)) dv (v = rh - r).  All in
+// binary128: cond(S) is about 1e13, so S must be exact far below FP64
+// rounding for W = S^(-1/2) to be exact to FP64 rounding.
+void poly3gauss_W(int nmax, double rh_in, double sigma_in, double* Sout, double* W) {
+  qd S[TGX_MAX_NR * TGX_MAX_NR], V[TGX_MAX_NR * TGX_MAX_NR], w[TGX_MAX_NR];
+  const qd rh = rh_in, sg = sigma_in, s2 = sg * sg, pi = acosq((qd)-1);
+  qd J[TGX_MAX_NR + 8];
+  J[0] = sg * sqrtq(pi / 2) * erfq(rh / (sg * sqrtq((qd)2)));
+  J[1] = rh * J[0] + s2 * (expq(-rh * rh / (2 * s2)) - 1);
+  qd rhp = 1;
+  for (int m = 2; m < nmax + 4; m++) {
+    rhp *= rh;
+    J[m] = rh * J[m - 1] + (m - 1) * s2 * J[m - 2] - s2 * rhp;
+  }
+  const qd ng = 1 / sqrtq(sqrtq(pi) / 2 * sg * erfq(rh / sg));
+  for (int a = 0; a < nmax; a++)
+    for (int b = 0; b < nmax; b++) {
+      qd v;
+      if (a == 0 && b == 0) {
+        v = 1;
+      } else if (a == 0 || b == 0) {
+        const int k = a + b;
+        v = ng * sqrtq((qd)(2 * k + 5)) / powq(rh, (qd)(2 * k + 5) / 2) * J[k + 2];
+      } else {
+        v = sqrtq((qd)((2 * a + 5) * (2 * b + 5))) / (qd)(a + b + 5);
+      }
+      S[a * nmax + b] = v;
+      if (Sout) Sout[a * nmax + b] = (double)v;
+    }
+  eig_sym(nmax, S, V, w);
+  for (int i = 0; i < nmax; i++)
+    for (int j = 0; j < nmax; j++) {
+      qd acc = 0;
+      for (int k = 0; k < nmax; k++) acc += V[i * nmax + k] * V[j * nmax + k] / sqrtq(w[k]);
+      W[i * TGX_MAX_NR + j] = (double)acc;
+    }
+}
+
 // FP64 closed form of R_n(r) (no envelope) for species s: see the device
 // version in tgx_device.h; used on the host for the central-atom term.
 void radial_closed_form(const SoapDev& sd, const double* W, int s, double r, double* R,
@@ -97,14 +142,32 @@ void radial_closed_form(const SoapDev& sd, const double* W, int s, double r, dou
   }
   const double pre = 1.0 / (std::sqrt(2 * M_PI) * sg);
   double I[TGX_MAX_NR];
-  for (int a = 1; a <= nmax; a++) {
+  const int g = sd.basis == 1 ? 1 : 0;  // poly3gauss: slot 0 is the Gaussian
+  for (int a = 1; a <= nmax - g; a++) {
     const int d = a + 2;
     const double Na = std::sqrt((2.0 * a + 5.0) / std::pow(rh, 2.0 * a + 5.0));
     double T;
     if (p == 0) T = J[d];
     else if (p == 1) T = rh * J[d] - J[d + 1];
     else T = rh * rh * J[d] - 2 * rh * J[d + 1] + J[d + 2];
-    I[a - 1] = Na * pre * T;
+    I[a - 1 + g] = Na * pre * T;
+  }
+  if (g) {
+    // the Gaussian times the smeared density: exp(-r^2/(2(sb^2+s^2))) times
+    // M_p = int_0^rh u^p exp(-(u-mu)^2/(2 t^2)) du (see the device version)
+    const double sb2 = sd.sig_r[s] * sd.sig_r[s], Sg = sb2 + s2;
+    const double mu = r * sb2 / Sg, t2 = sb2 * s2 / Sg, t = std::sqrt(t2);
+    const double e0 = std::exp(-mu * mu / (2 * t2)), eh = std::exp(-(rh - mu) * (rh - mu) / (2 * t2));
+    double M[5];
+    M[0] = t * std::sqrt(M_PI / 2) *
+           (std::erf((rh - mu) / (t * std::sqrt(2.0))) + std::erf(mu / (t * std::sqrt(2.0))));
+    M[1] = mu * M[0] - t2 * (eh - e0);
+    double rp = 1;
+    for (int k = 2; k < 5; k++) {
+      rp *= rh;
+      M[k] = mu * M[k - 1] + (k - 1) * t2 * M[k - 2] - t2 * rp * eh;
+    }
+    I[0] = sd.gnorm[s] * pre * std::exp(-r * r / (2 * Sg)) * M[p];
   }
   const double* Ws = W + s * TGX_MAX_NR * TGX_MAX_NR;
   for (int n = 0; n < nmax; n++) {

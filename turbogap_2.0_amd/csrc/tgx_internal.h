@@ -85,6 +85,10 @@ struct SoapDev {
   // channels padded so that off[s] + nmx <= nrp for every species (rows of
   // width nrp can then be read nmx wide from any species offset).
   int nmx, nrp;
+  // radial basis: 0 poly3, 1 poly3gauss (slot 0 the normalised Gaussian
+  // gnorm[s] exp(-r^2/(2 sig_r[s]^2)), then the poly3 functions)
+  int basis;
+  double gnorm[TGX_MAX_SPECIES];
 };
 
 struct TwoBodyDev {
@@ -245,6 +249,8 @@ int set_err(tgx_ctx* c, int code, const char* fmt, ...);
 
 // Host helpers (tgx_host.cpp)
 void poly3_W(int nmax, double* W);                 // S^(-1/2), binary128
+// poly3gauss: S and W = S^(-1/2) (binary128, rounded once); S may be NULL
+void poly3gauss_W(int nmax, double rh, double sigma, double* S, double* W);
 void radial_closed_form(const SoapDev& sd, const double* W, int s, double r,
                         double* R, double* dR);   // FP64 host evaluation
 int make_cell(const double* lattice, double rcut, Cell* cell, std::string* err);

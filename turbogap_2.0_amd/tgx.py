@@ -346,6 +346,7 @@ def load_library(path: str = LIB_PATH):
         "tgx_md_end": (i, [P]),
         "tgx_sync": (i, [P]),
         "tgx_poly3_orthonormalization": (i, [i, dp, dp]),
+        "tgx_poly3gauss_orthonormalization": (i, [i, d, d, dp, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -363,6 +364,18 @@ def poly3_orthonormalization(alpha_max: int):
     rc = lib.tgx_poly3_orthonormalization(alpha_max, _dptr(S), _dptr(W))
     if rc != TGX_OK:
         raise TgxError(rc, "tgx_poly3_orthonormalization")
+    return S.T.copy(), W.T.copy()
+
+
+def poly3gauss_orthonormalization(alpha_max: int, atom_sigma: float, rcut_hard: float):
+    """S and W = S^(-1/2) of the poly3gauss basis (host-only library call)."""
+    lib = load_library()
+    S = np.zeros((alpha_max, alpha_max))
+    W = np.zeros((alpha_max, alpha_max))
+    rc = lib.tgx_poly3gauss_orthonormalization(alpha_max, atom_sigma, rcut_hard, _dptr(S),
+                                               _dptr(W))
+    if rc != TGX_OK:
+        raise TgxError(rc, "tgx_poly3gauss_orthonormalization")
     return S.T.copy(), W.T.copy()
 
 
