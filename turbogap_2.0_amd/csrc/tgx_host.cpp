@@ -349,8 +349,8 @@ Cell open_cell() {
 
 int make_cell(const double* lat, double rcut, Cell* c, std::string* err) {
   if (!is_orthorhombic(lat)) {
-    *err = "non-orthorhombic cells are supported by tgx_neighbors_build, tgx_energy_forces, "
-           "tgx_energy_forces_noalloc and tgx_soap_descriptors only";
+    *err = "non-orthorhombic cells are supported by the single-structure calls (neighbours, "
+           "descriptors, energies and forces, local-property values, TS dispersion) only";
     return TGX_ERR_UNSUPPORTED;
   }
   if (!(lat[0] > 0 && lat[4] > 0 && lat[8] > 0 && rcut > 0)) {
