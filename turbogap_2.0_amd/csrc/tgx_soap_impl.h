@@ -8,7 +8,9 @@
 // channels), each site with its own LDS region, so nothing waits across
 // sites.  Forward, per site:
 //   A  lanes = neighbour slots: minimum-image vector, radial values R_n(r)
-//      and A_l(r) Y_lm(u) for the valid pairs, compacted into LDS in chunks;
+//      and A_l(r) Y_lm(u) for the valid pairs, compacted into LDS in chunks
+//      (FP32 with two sites per wave: 28 pairs a chunk, the harmonics one
+//      block of degrees at a time, each block followed by its B step);
 //   B  lanes = lm: c[n][lm] += R_n(j) A_l Y_lm(j) accumulated in registers;
 //   C  power spectrum p[(n<=n'),l], its norm and q = p/|p|.
 // Adjoint, per site: dE/dp from G = dE/dq, dE/dc (lanes = lm), then lanes =
@@ -99,8 +101,35 @@ __device__ __forceinline__ double pair_vec_fast(const double* __restrict__ pos, 
 // radial channels, else one.  Pairs per LDS chunk and the per-site LDS
 // footprint (elements of T).
 template <int NRP> struct Spw { static constexpr int v = NRP <= 16 ? 2 : 1; };
-template <typename T, int SPW> struct Chunk { static constexpr int CH = SPW == 2 ? 12 : 32; };
-template <int SPW> struct Chunk<double, SPW> { static constexpr int CH = 16; };
+// FP32 with two sites per wave: the forward evaluates the harmonics of a chunk
+// in blocks of degrees (hblk_*), so that LDS holds one block of a chunk at a
+// time and a chunk can take 28 pairs instead of 12.
+template <typename T, int NRP> struct FwdBlocked {
+  static constexpr bool v = sizeof(T) == 4 && Spw<NRP>::v == 2;
+};
+template <typename T, int NRP> struct FwdChunk {
+  static constexpr int v = FwdBlocked<T, NRP>::v ? (NRP <= 8 ? 28 : 24)
+                                                 : (sizeof(T) == 8 ? 16 : Spw<NRP>::v == 2 ? 12 : 32);
+};
+// Harmonic blocks: consecutive degrees with at most 32 (l, m) values each
+// (l_max 8: l = 0-4, 5-6, 7-8; l_max 10 adds 9 and 10).  A pair's block row
+// in LDS has the odd stride HSTR, so lanes storing different pairs' values
+// hit different banks.
+__host__ __device__ constexpr int hblk_of(int l) {
+  int b = 0, l0 = 0;
+  for (int k = 1; k <= l; k++)
+    if ((k + 1) * (k + 1) - l0 * l0 > 32) {
+      b++;
+      l0 = k;
+    }
+  return b;
+}
+__host__ __device__ constexpr int hblk_first(int b) {
+  int l = 0;
+  while (hblk_of(l) < b) l++;
+  return l;
+}
+constexpr int HSTR = 33;
 
 // Compacted valid-pair slots per site (two scan rounds of the site's lanes).
 template <int NRP> struct Capv { static constexpr int v = 2 * (64 / Spw<NRP>::v); };
@@ -109,17 +138,17 @@ template <int NRP> struct Capv { static constexpr int v = 2 * (64 / Spw<NRP>::v)
 // and radial values of a chunk of pairs (the first part reused for c) plus
 // the compacted slot list (16-bit slots); adjoint = dE/dc [L2][NRP], the
 // upper triangles of the coupling matrices M_l and the slot list.
-// Forward region A holds the harmonics of a chunk (CH x L2), later c with
-// rows padded to a multiple of four (NRP x L2P).
+// Forward region A holds the harmonics of a chunk (CH x L2, or one block of
+// them, CH x HSTR), later c with rows padded to a multiple of four (NRP x L2P).
 template <typename T, int NRP>
 __host__ __device__ constexpr size_t fwd_region_a(int L2) {
-  constexpr int CH = Chunk<T, Spw<NRP>::v>::CH;
-  return (size_t)CH * L2 > (size_t)NRP * ((L2 + 3) & ~3) ? (size_t)CH * L2
-                                                         : (size_t)NRP * ((L2 + 3) & ~3);
+  constexpr int CH = FwdChunk<T, NRP>::v;
+  const size_t rows = (size_t)CH * (FwdBlocked<T, NRP>::v ? HSTR : L2);
+  return rows > (size_t)NRP * ((L2 + 3) & ~3) ? rows : (size_t)NRP * ((L2 + 3) & ~3);
 }
 template <typename T, int NRP>
 __host__ __device__ constexpr size_t fwd_site_bytes(int L2) {
-  constexpr int CH = Chunk<T, Spw<NRP>::v>::CH;
+  constexpr int CH = FwdChunk<T, NRP>::v;
   return ((sizeof(T) * (fwd_region_a<T, NRP>(L2) + (size_t)CH * NRP) + 2 * Capv<NRP>::v) + 15) /
          16 * 16;
 }
@@ -245,8 +274,104 @@ __device__ __forceinline__ void fwd_pair_basis(const SoapDev* sd, const double* 
   }
 }
 
+// The same basis evaluated block by block (blocked forward): the pair state
+// that carries from one harmonic block to the next -- u_z, A_l, the trig
+// factors C_m + i S_m = (u_x + i u_y)^m and the two last Legendre values of
+// every m-chain -- stays in registers.  The values and the arithmetic are
+// those of fwd_pair_basis.
+template <typename T, int LMX>
+struct PairHarm {
+  T uz;
+  T Al[LMX + 1], C[LMX + 1], S[LMX + 1], p1[LMX + 1], p2[LMX + 1];
+};
+
+// R_n(r) into rr[0..NRP) and the pair state of the harmonic recurrences.
+template <typename T, int NRP, int NMX, int LMX>
+__device__ __forceinline__ void fwd_pair_setup(const SoapDev* sd, const double* W,
+                                               const float2* rtab, const double* pos,
+                                               const int* species, const Cell& cell, int i,
+                                               int e, T* rr, PairHarm<T, LMX>& h) {
+  double dv[3];
+  const double rd = pair_vec_fast(pos, i, e, cell, dv);
+  const int s = species[e & TGX_ATOM_MASK] - 1;
+  const T r = (T)rd;
+  const T ir = (T)1 / r;
+  const T ux = (T)dv[0] * ir, uy = (T)dv[1] * ir;
+  h.uz = (T)dv[2] * ir;
+  T R[NMX], dR[NMX];
+  Radial<T>::template eval<NMX>(sd, W, rtab, s, r, R, dR);
+  const int o = sd->off[s];
+#pragma unroll
+  for (int n = 0; n < NRP; n++) rr[n] = (T)0;
+#pragma unroll
+  for (int n = 0; n < NMX; n++) rr[o + n] = R[n];
+  T log2t, D;
+  ang_params<T>(sd, s, r, &log2t, &D);
+  ang_factors<T, LMX>(log2t, h.Al);
+  h.C[0] = 1;
+  h.S[0] = 0;
+#pragma unroll
+  for (int m = 0; m < LMX; m++) {
+    h.C[m + 1] = ux * h.C[m] - uy * h.S[m];
+    h.S[m + 1] = ux * h.S[m] + uy * h.C[m];
+  }
+#pragma unroll
+  for (int m = 0; m <= LMX; m++) {
+    h.p1[m] = SphK<T>::P(m);
+    h.p2[m] = 0;
+  }
+}
+
+// State of a chunk lane without a pair: zero radial row and zero harmonics.
+template <typename T, int NRP, int LMX>
+__device__ __forceinline__ void fwd_pair_zero(T* rr, PairHarm<T, LMX>& h) {
+#pragma unroll
+  for (int n = 0; n < NRP; n++) rr[n] = (T)0;
+  h.uz = 0;
+#pragma unroll
+  for (int m = 0; m <= LMX; m++) h.Al[m] = h.C[m] = h.S[m] = h.p1[m] = h.p2[m] = 0;
+}
+
+// Harmonic block B (degrees hblk_first(B) .. hblk_first(B+1)-1, at most
+// l_max): A_l Y_lm into ay[lm - l0^2], advancing each m-chain's recurrence.
+template <typename T, int LMX, int B>
+__device__ __forceinline__ void fwd_harm_block(PairHarm<T, LMX>& h, int L, T* ay) {
+  constexpr int l0 = hblk_first(B), l1 = hblk_first(B + 1) - 1 < LMX ? hblk_first(B + 1) - 1 : LMX;
+  constexpr int base = l0 * l0;
+#pragma unroll
+  for (int m = 0; m <= l1; m++) {
+    if (m <= L) {
+#pragma unroll
+      for (int l = m > l0 ? m : l0; l <= l1; l++) {
+        if (l <= L) {
+          T P = h.p1[m];
+          if (l > m) {
+            const int ix = l * (TGX_MAX_LMAX + 1) + m;
+            P = SphK<T>::A(ix) * h.uz * h.p1[m] - SphK<T>::B(ix) * h.p2[m];
+            h.p2[m] = h.p1[m];
+            h.p1[m] = P;
+          }
+          const T AP = h.Al[l] * P;
+          ay[l * l + l + m - base] = AP * h.C[m];
+          if (m > 0) ay[l * l + l - m - base] = AP * h.S[m];
+        }
+      }
+    }
+  }
+}
+
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
 // Minimum waves per SIMD the FP32 forward kernel is register-limited to
-// (measured best on gfx950 with 12-pair chunks); FP64 is left unconstrained.
+// (measured on gfx950 at 32k sites: 98.5 us at four waves with 28-pair
+// chunks, 151 us at five with 20-pair chunks, which spills); FP64 is left
+// unconstrained.
 template <typename T> struct FwdWaves { static constexpr int v = 4; };
 template <> struct FwdWaves<double> { static constexpr int v = 1; };
 // The same for the FP32 adjoint (4: 148 us at 32k atoms vs 163 unconstrained
@@ -263,8 +388,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FwdWaves<T>
     const int* __restrict__ dmap, T* __restrict__ q, int q_ld, u16* __restrict__ qb,
     int qb_ld, T* __restrict__ cbuf, T* __restrict__ pnorm, int wpb) {
   constexpr int SPW = Spw<NRP>::v, LW = 64 / SPW;
-  constexpr int CH = Chunk<T, SPW>::CH;
-  constexpr int NS = ((LMX + 1) * (LMX + 1) + LW - 1) / LW;  // lm slots per lane
+  constexpr bool BLK = FwdBlocked<T, NRP>::v;
+  constexpr int CH = FwdChunk<T, NRP>::v;
+  // lm slots per lane: lm = lane + k LW, or (blocked) the lane-th value of
+  // harmonic block k
+  constexpr int NS = BLK ? hblk_of(LMX) + 1 : ((LMX + 1) * (LMX + 1) + LW - 1) / LW;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   // EXL: l_max equals the bound LMX, so L and L2 are compile-time constants
   const int L = EXL ? LMX : sd->l_max, L2 = EXL ? (LMX + 1) * (LMX + 1) : sd->L2;
@@ -297,25 +425,65 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FwdWaves<T>
                                             CAPV, sub, lane, [](int) {});
     wave_sync();
     for (int c0 = 0; c0 < nv; c0 += CH) {
-      if (lane < CH && c0 + lane < nv)
-        fwd_pair_basis<T, NRP, NMX, LMX>(sd, W, rtab, pos, species, cell, i, row[sIdx[c0 + lane]],
-                                         L, sAY + lane * L2, sR + lane * NRP);
-      wave_sync();
+      const bool act = lane < CH && c0 + lane < nv;
       const int nj = min(CH, nv - c0);
-      for (int j = 0; j < nj; j++) {
-        // unconditional loads: a slot past L2 reads a valid entry and its
-        // accumulator is never stored
-        T y[NS];
+      if constexpr (BLK) {
+        // one harmonic block of the chunk in LDS at a time: lanes = pairs
+        // advance their recurrences by one block, then lanes = the block's
+        // lm accumulate it
+        // lanes of the chunk without a pair write zero rows, so that the
+        // accumulation runs over whole groups of four rows (CH is a multiple
+        // of four) with its loads issued together
+        PairHarm<T, LMX> h;
+        if (act)
+          fwd_pair_setup<T, NRP, NMX, LMX>(sd, W, rtab, pos, species, cell, i,
+                                           row[sIdx[c0 + lane]], sR + lane * NRP, h);
+        else if (lane < CH)
+          fwd_pair_zero<T, NRP, LMX>(sR + lane * NRP, h);
+        const int nj4 = (nj + 3) & ~3;
+        static_for<0, NS>([&](auto bc) {
+          constexpr int B = decltype(bc)::value;
+          constexpr int base = hblk_first(B) * hblk_first(B);
+          constexpr int end = hblk_first(B + 1) * hblk_first(B + 1);
+          const int nval = min(end, L2) - base;
+          if (nval > 0) {
+            if (lane < CH) fwd_harm_block<T, LMX, B>(h, L, sAY + lane * HSTR);
+            wave_sync();
+            for (int j = 0; j < nj4; j += 4) {
+              // unconditional loads: a lane past the block reads a valid
+              // entry and its accumulator is never stored
+              T y[4];
 #pragma unroll
-        for (int k = 0; k < NS; k++) y[k] = sAY[j * L2 + min(lane + k * LW, L2 - 1)];
+              for (int u = 0; u < 4; u++) y[u] = sAY[(j + u) * HSTR + min(lane, nval - 1)];
 #pragma unroll
-        for (int n = 0; n < NRP; n++) {
-          const T rv = sR[j * NRP + n];
+              for (int u = 0; u < 4; u++)
 #pragma unroll
-          for (int k = 0; k < NS; k++) acc[k][n] += rv * y[k];
+                for (int n = 0; n < NRP; n++) acc[B][n] += sR[(j + u) * NRP + n] * y[u];
+            }
+            wave_sync();
+          }
+        });
+      } else {
+        if (act)
+          fwd_pair_basis<T, NRP, NMX, LMX>(sd, W, rtab, pos, species, cell, i,
+                                           row[sIdx[c0 + lane]], L, sAY + lane * L2,
+                                           sR + lane * NRP);
+        wave_sync();
+        for (int j = 0; j < nj; j++) {
+          // unconditional loads: a slot past L2 reads a valid entry and its
+          // accumulator is never stored
+          T y[NS];
+#pragma unroll
+          for (int k = 0; k < NS; k++) y[k] = sAY[j * L2 + min(lane + k * LW, L2 - 1)];
+#pragma unroll
+          for (int n = 0; n < NRP; n++) {
+            const T rv = sR[j * NRP + n];
+#pragma unroll
+            for (int k = 0; k < NS; k++) acc[k][n] += rv * y[k];
+          }
         }
+        wave_sync();
       }
-      wave_sync();
     }
   }
   // central atom: l = 0 only
@@ -331,14 +499,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FwdWaves<T>
   // multiple of four for the power spectrum
   const int L2P = (L2 + 3) & ~3;
   T* cs = cbuf + (size_t)site * NRP * L2;
+  if constexpr (BLK) {
+    static_for<0, NS>([&](auto bc) {
+      constexpr int B = decltype(bc)::value;
+      constexpr int base = hblk_first(B) * hblk_first(B);
+      constexpr int end = hblk_first(B + 1) * hblk_first(B + 1);
+      const int lm = base + lane;
+      if (lm < min(end, L2)) {
 #pragma unroll
-  for (int k = 0; k < NS; k++) {
-    const int lm = lane + k * LW;
-    if (lm < L2) {
+        for (int n = 0; n < NRP; n++) {
+          sC[n * L2P + lm] = acc[B][n];
+          cs[n * L2 + lm] = acc[B][n];
+        }
+      }
+    });
+  } else {
 #pragma unroll
-      for (int n = 0; n < NRP; n++) {
-        sC[n * L2P + lm] = acc[k][n];
-        cs[n * L2 + lm] = acc[k][n];
+    for (int k = 0; k < NS; k++) {
+      const int lm = lane + k * LW;
+      if (lm < L2) {
+#pragma unroll
+        for (int n = 0; n < NRP; n++) {
+          sC[n * L2P + lm] = acc[k][n];
+          cs[n * L2 + lm] = acc[k][n];
+        }
       }
     }
   }
