@@ -1413,10 +1413,22 @@ static int load_soap_block(tgx_ctx* c, const tgx_soap_hypers* h, int dim, int n_
     for (int s = 0; s < h->n_species; s++) nmx = std::max(nmx, h->alpha_max[s]);
     sd.nmx = nmx <= 8 ? 8 : 16;
     for (int s = 0; s < h->n_species; s++) need = std::max(need, sd.off[s] + sd.nmx);
+    // the adjoint's species sub-rows at 4-aligned offsets must fit as well
+    int oa = 0;
+    for (int s = 0; s < h->n_species; s++) {
+      sd.offa[s] = oa;
+      need = std::max(need, oa + sd.nmx);
+      oa += (h->alpha_max[s] + 3) / 4 * 4;
+    }
     sd.nrp = (std::max(need, nr) + 7) / 8 * 8;
     if (sd.nrp > TGX_MAX_NR)
       return set_err(c, TGX_ERR_UNSUPPORTED, "radial channels (padded %d) exceed %d", sd.nrp,
                      TGX_MAX_NR);
+    for (int k = 0; k < TGX_MAX_NR; k++) sd.chana[k] = -1;
+    for (int s = 0; s < h->n_species; s++)
+      for (int k = 0; k < h->alpha_max[s]; k++)
+        sd.chana[sd.off[s] + k] = (signed char)(sd.offa[s] + k);
+    sd.lds_gap = sd.nrp != nr;
   }
   sd.dim = sd.npair * (h->l_max + 1);
   sd.rcut_max = rcm;

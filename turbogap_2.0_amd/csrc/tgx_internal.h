@@ -89,6 +89,14 @@ struct SoapDev {
   // gnorm[s] exp(-r^2/(2 sig_r[s]^2)), then the poly3 functions)
   int basis;
   double gnorm[TGX_MAX_SPECIES];
+  // The adjoint's dE/dc rows in LDS (nrp wide): species sub-rows at offsets
+  // offa[s], multiples of four with offa[s] + nmx <= nrp, so that the
+  // nmx-wide reads of a pair are 16-byte aligned; chana[n] = LDS slot of
+  // channel n (-1: no species' channel); lds_gap = some slot of a row is
+  // filled by no channel (and is cleared).
+  int lds_gap;
+  int offa[TGX_MAX_SPECIES];
+  signed char chana[TGX_MAX_NR];
 };
 
 struct TwoBodyDev {

@@ -136,8 +136,9 @@ template <int NRP> struct Capv { static constexpr int v = 2 * (64 / Spw<NRP>::v)
 
 // Per-site LDS footprint in bytes (16-byte multiple): forward = harmonics
 // and radial values of a chunk of pairs (the first part reused for c) plus
-// the compacted slot list (16-bit slots); adjoint = dE/dc [L2][NRP], the
-// upper triangles of the coupling matrices M_l and the slot list.
+// the compacted slot list (16-bit slots); adjoint = dE/dc [L2][NRP] (rows
+// of species sub-rows at 4-aligned offsets), the upper triangles of the
+// coupling matrices M_l and the slot list.
 // Forward region A holds the harmonics of a chunk (CH x L2, or one block of
 // them, CH x HSTR), later c with rows padded to a multiple of four (NRP x L2P).
 template <typename T, int NRP>
@@ -198,6 +199,20 @@ __device__ __forceinline__ int scan_pairs(const SoapDev* sd, const double* pos,
     *next += LW;
   }
   return nv;
+}
+
+// N values from LDS at p, which is 16-byte aligned, as 16-byte loads.
+template <typename T, int N>
+__device__ __forceinline__ void lds_row16(const T* p, T* out) {
+  typedef typename std::conditional<sizeof(T) == 4, float4, double2>::type V;
+  constexpr int PER = 16 / sizeof(T);
+  static_assert(N % PER == 0, "row of whole 16-byte units");
+  const V* v = reinterpret_cast<const V*>(p);
+#pragma unroll
+  for (int k = 0; k < N / PER; k++) {
+    const V x = v[k];
+    __builtin_memcpy(out + k * PER, &x, 16);
+  }
 }
 
 // Sum over the LW lanes of a sub-wave.
@@ -749,7 +764,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BwdWaves<T>
   const int sub = lane64 / LW, lane = lane64 % LW;
   constexpr int CAPV = Capv<NRP>::v;
   T* sGC = reinterpret_cast<T*>(smem_raw + (size_t)(wave * SPW + sub) * bwd_site_bytes<T, NRP>(L2, Lp));
-  // sGC [L2][NRP] = dE/dc; sM [Lp][NRP][NRP]: dE/dc = M_l c
+  // sGC [L2][NRP] = dE/dc, species s at offa[s]; sM [Lp][NRP][NRP]: dE/dc = M_l c
   T* sM = sGC + NRP * L2;
   constexpr int MS = MStride<NRP>::v;
   u16* sIdx = reinterpret_cast<u16*>(sM + Lp * MS);
@@ -761,6 +776,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BwdWaves<T>
   const int* row = list + (size_t)i * maxnb;
   const T* cs = cbuf + (size_t)site * NRP * L2;
   for (int k = lane; k < Lp * MS; k += LW) sM[k] = (T)0;
+  // row slots that no channel fills are read with R_n = 0: they must be finite
+  if (sd->lds_gap)
+    for (int k = lane; k < L2 * NRP; k += LW) sGC[k] = (T)0;
   const T* qs = q + (size_t)site * q_ld;
   const T* gs = G + (size_t)site * g_ld;
   T gq = 0;
@@ -790,7 +808,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BwdWaves<T>
       T a = 0;
 #pragma unroll
       for (int n2 = 0; n2 < NRP; n2++) a += Ml[tri<NRP>(n, n2)] * cl[n2];
-      sGC[lm * NRP + n] = a;
+      const int dst = sd->chana[n];
+      if (dst >= 0) sGC[lm * NRP + dst] = a;
     }
   }
   wave_sync();
@@ -820,7 +839,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BwdWaves<T>
       const T ux = (T)dv[0] * ir, uy = (T)dv[1] * ir, uz = (T)dv[2] * ir;
       T R[NMX], dR[NMX];
       Radial<T>::template eval<NMX>(sd, W, rtab, s, r, R, dR);
-      const int o = sd->off[s];
+      const int o = sd->offa[s];
       T log2t, D;
       ang_params<T>(sd, s, r, &log2t, &D);
       T Al[LMX + 1];
@@ -852,7 +871,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BwdWaves<T>
                 dm1 = dP;
               }
               const T A = Al[l], dA = dAl[l];
-              const T* gcc = sGC + (l * l + l + m) * NRP + o;
+              T gcc[NMX];
+              lds_row16<T, NMX>(sGC + (l * l + l + m) * NRP + o, gcc);
               T hc = 0, hdc = 0;
 #pragma unroll
               for (int n = 0; n < NMX; n++) {
@@ -864,7 +884,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BwdWaves<T>
                 s1 += Xc * P;
                 Vz += hAc * dP;
               } else {
-                const T* gcs = sGC + (l * l + l - m) * NRP + o;
+                T gcs[NMX];
+                lds_row16<T, NMX>(sGC + (l * l + l - m) * NRP + o, gcs);
                 T hs = 0, hds = 0;
 #pragma unroll
                 for (int n = 0; n < NMX; n++) {
