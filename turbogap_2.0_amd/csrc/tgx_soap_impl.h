@@ -846,12 +846,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BwdWaves<T>
       ang_factors<T, LMX>(log2t, Al);
       T s1 = 0, Vx = 0, Vy = 0, Vz = 0;
       T Cm = 1, Sm = 0, Cp = 0, Sp = 0;  // C_m, S_m and C_{m-1}, S_{m-1}
-      T dAl[LMX + 1];  // dA_l/dr = A_l l(l+1) D
+      T kap[LMX + 1];  // (dA_l/dr) / A_l = l(l+1) D
 #pragma unroll
-      for (int l = 0; l <= LMX; l++) dAl[l] = Al[l] * ((T)(l * (l + 1)) * D);
+      for (int l = 0; l <= LMX; l++) kap[l] = (T)(l * (l + 1)) * D;
       // per (l, +-m): the radial contractions h = sum_n dE/dc R_n and
       // hd = sum_n dE/dc dR_n of the cosine and sine rows, then the radial
       // (s1) and angular (V) parts of the gradient with the two rows combined
+      // and A_l, dA_l/dr = A_l kap_l factored out of both
 #pragma unroll
       for (int m = 0; m <= LMX; m++) {
         if (m <= L) {
@@ -870,7 +871,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BwdWaves<T>
                 dm2 = dm1;
                 dm1 = dP;
               }
-              const T A = Al[l], dA = dAl[l];
+              const T A = Al[l];
               T gcc[NMX];
               lds_row16<T, NMX>(sGC + (l * l + l + m) * NRP + o, gcc);
               T hc = 0, hdc = 0;
@@ -879,10 +880,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BwdWaves<T>
                 hc += gcc[n] * R[n];
                 hdc += gcc[n] * dR[n];
               }
-              const T Xc = hdc * A + hc * dA, hAc = hc * A;
+              const T PA = P * A, dPA = dP * A;
               if (m == 0) {
-                s1 += Xc * P;
-                Vz += hAc * dP;
+                s1 += PA * (hdc + kap[l] * hc);
+                Vz += dPA * hc;
               } else {
                 T gcs[NMX];
                 lds_row16<T, NMX>(sGC + (l * l + l - m) * NRP + o, gcs);
@@ -892,12 +893,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BwdWaves<T>
                   hs += gcs[n] * R[n];
                   hds += gcs[n] * dR[n];
                 }
-                const T Xs = hds * A + hs * dA, hAs = hs * A;
-                const T Pm = P * (T)m;
-                s1 += P * (Xc * Cm + Xs * Sm);
-                Vx += Pm * (hAc * Cp + hAs * Sp);
-                Vy += Pm * (hAs * Cp - hAc * Sp);
-                Vz += dP * (hAc * Cm + hAs * Sm);
+                const T t1 = hc * Cm + hs * Sm, t2 = hdc * Cm + hds * Sm;
+                const T mPA = PA * (T)m;
+                s1 += PA * (t2 + kap[l] * t1);
+                Vx += mPA * (hc * Cp + hs * Sp);
+                Vy += mPA * (hs * Cp - hc * Sp);
+                Vz += dPA * t1;
               }
             }
           }
