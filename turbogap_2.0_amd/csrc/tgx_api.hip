@@ -156,6 +156,7 @@ hipError_t launch_energy_init(int n, const int* species, const double* e0,
 hipError_t launch_reduce(int n_sites, const double* vsite, int n, const double* vatom,
                          const double* eatom, double* out, hipStream_t st,
                          bool with_final = true);
+hipError_t launch_reduce_final(double* out, hipStream_t st);
 
 hipError_t launch_md_vv(int n, double* pos, double* vel, const double* f, double* fprev,
                         const double* mass, double dt, const int* step, double* ke,
@@ -864,6 +865,7 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
   // energies, the force gather overwrites the forces and no per-atom virial
   // is used -- nothing to initialise
   const bool fuse = fuse_two_body(c, n, n_sites, do_forces);
+  bool red_in_gather = false;  // the reduction's partial rows done by the force gather
   if (c->has_soap && n_sites > 0) {
     rcode = soap_forward(c, n, cell, n_sites);
     if (rcode) return rcode;
@@ -939,19 +941,25 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
     hipError_t e = bwd(fw ? atoi(fw) : kTuneWpb[wk]);
     kev_rec(c, 5);
     kev_rec(c, 6);
+    // with the 2-body term fused and no further terms, nothing after the
+    // gather writes the energies or virials: the same launch computes the
+    // partial rows of their reduction
+    red_in_gather = fuse && c->a3b.empty() && c->corep.empty() && !cell.many;
+    const ReduceArgs ra{n_sites, n, c->blk->vsite.as<double>(), nullptr, c->eatom.as<double>(),
+                        x.out.as<double>() + 16};
     if (e == hipSuccess) {
       if (prec == TGX_FP64)
         e = launch_force_gather<double>(n, c->maxnb, c->nb_count.as<int>(),
                                         c->nb_list.as<int>(), c->nb_rev.as<int>(),
                                         c->blk->site_of_atom.as<int>(), n_sites == n ? 1 : 0,
                                         c->gpair.as<double>(), c->forces.as<double>(),
-                                        c->stream, fuse ? 1 : 0);
+                                        c->stream, fuse ? 1 : 0, red_in_gather ? &ra : nullptr);
       else
         e = launch_force_gather<float>(n, c->maxnb, c->nb_count.as<int>(),
                                        c->nb_list.as<int>(), c->nb_rev.as<int>(),
                                        c->blk->site_of_atom.as<int>(), n_sites == n ? 1 : 0,
                                        c->gpair.as<float>(), c->forces.as<double>(), c->stream,
-                                       fuse ? 1 : 0);
+                                       fuse ? 1 : 0, red_in_gather ? &ra : nullptr);
     }
     TGX_HIP(c, e);
     kev_rec(c, 7);
@@ -974,6 +982,8 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
                                                                             : nullptr,
                                   (do_forces && !fuse) ? c->virial.as<double>() : nullptr,
                                   c->eatom.as<double>(), x.b_out.as<double>(), c->stream));
+  } else if (red_in_gather) {
+    if (!defer_final) TGX_HIP(c, launch_reduce_final(x.out.as<double>(), c->stream));
   } else {
     TGX_HIP(c, launch_reduce((do_forces && c->has_soap) ? n_sites : 0,
                              (do_forces && c->has_soap) ? c->blk->vsite.as<double>() : nullptr, n,

@@ -92,6 +92,39 @@ struct CellMany {
 // Fixed-order sum of the TGX_RB partial rows into out[0..8] (virial) and
 // out[9] (energy) by the first TGX_RB threads of a workgroup; sh holds
 // TGX_RB x 7 doubles.  Every thread of the workgroup must call it.
+// Partial row b of the deterministic energy/virial reduction: a fixed
+// partition of the sites (6-component virials vsite) and atoms (virials
+// vatom, energies eatom; each input may be null) over TGX_RB workgroups of
+// 256 threads, each summed by a fixed tree into part[b*8 + 0..6].  sh holds
+// 256 x 7 doubles.  Every thread of the workgroup must call it.
+__device__ inline void reduce_part_rows(int b, int n_sites, const double* vsite, int n,
+                                        const double* vatom, const double* eatom,
+                                        double* part, double (*sh)[7]) {
+  const int t = threadIdx.x;
+  double a[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (vsite) {
+    const int per = (n_sites + TGX_RB - 1) / TGX_RB, lo = b * per, hi = min(n_sites, lo + per);
+    for (int s = lo + t; s < hi; s += 256)
+      for (int k = 0; k < 6; k++) a[k] += vsite[(size_t)s * 6 + k];
+  }
+  {
+    const int per = (n + TGX_RB - 1) / TGX_RB, lo = b * per, hi = min(n, lo + per);
+    for (int s = lo + t; s < hi; s += 256) {
+      if (vatom)
+        for (int k = 0; k < 6; k++) a[k] += vatom[(size_t)s * 6 + k];
+      if (eatom) a[6] += eatom[s];
+    }
+  }
+  for (int k = 0; k < 7; k++) sh[t][k] = a[k];
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (t < off)
+      for (int k = 0; k < 7; k++) sh[t][k] += sh[t + off][k];
+    __syncthreads();
+  }
+  if (t < 7) part[b * 8 + t] = sh[0][t];
+}
+
 __device__ inline void reduce_final_rows(const double* part, double* out, double (*sh)[7]) {
   const int t = threadIdx.x;
   if (t < TGX_RB)

@@ -294,10 +294,21 @@ hipError_t launch_gap_kernel(int mode, int n_sites, int dim, int dim_pad,
                              const int* species = nullptr, int* few_parts = nullptr,
                              const GemmPlan* plan = nullptr);
 // overwrite: forces[k] = gathered sum (otherwise added to forces[k])
+// The partial rows of the energy/virial reduction (reduce_part_rows), done
+// by extra workgroups of the same launch when nothing later in the
+// evaluation writes its inputs.
+struct ReduceArgs {
+  int n_sites, n;
+  const double* vsite;
+  const double* vatom;
+  const double* eatom;
+  double* part;
+};
 template <typename T>
 hipError_t launch_force_gather(int n, int maxnb, const int* count,
                                const int* list, const int* rev,
                                const int* site_of_atom, int all_sites, const T* gpair,
-                               double* forces, hipStream_t st, int overwrite = 0);
+                               double* forces, hipStream_t st, int overwrite = 0,
+                               const ReduceArgs* red = nullptr);
 
 }  // namespace tgx

@@ -1096,16 +1096,27 @@ hipError_t launch_csr_out(const double* pos, int n, const Cell& c, int maxnb,
 // g holds one 4-vector (gx, gy, gz, 0) per slot: one 16- or 32-byte load per
 // gathered slot.  all_sites: every atom is a SOAP site (no site checks).
 // ---------------------------------------------------------------------------
+// With red.part, the first TGX_RB workgroups compute the partial rows of
+// the energy/virial reduction instead (the same rows as reduce_part_kernel).
 template <typename T>
 __global__ __launch_bounds__(256) void force_gather_kernel(int n, int maxnb, const int* count,
                                                            const int* list, const int* rev,
                                                            const int* site_of_atom, int all_sites,
                                                            const T* gpair, double* forces,
-                                                           int overwrite) {
+                                                           int overwrite, ReduceArgs red) {
+  int blk = blockIdx.x;
+  if (red.part) {
+    if (blk < TGX_RB) {
+      __shared__ double sh[256][7];
+      reduce_part_rows(blk, red.n_sites, red.vsite, red.n, red.vatom, red.eatom, red.part, sh);
+      return;
+    }
+    blk -= TGX_RB;
+  }
   typedef typename std::conditional<sizeof(T) == 4, float4, double4>::type T4;
   const T4* g4 = reinterpret_cast<const T4*>(gpair);
   const int lane = threadIdx.x & 63;
-  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int k = blk * 4 + (threadIdx.x >> 6);
   if (k >= n) return;
   const int cnt = min(count[k], maxnb);
   const bool ksite = all_sites || site_of_atom[k] >= 0;
@@ -1147,13 +1158,16 @@ __global__ __launch_bounds__(256) void force_gather_kernel(int n, int maxnb, con
 template <typename T>
 hipError_t launch_force_gather(int n, int maxnb, const int* count, const int* list,
                                const int* rev, const int* site_of_atom, int all_sites,
-                               const T* gpair, double* forces, hipStream_t st, int overwrite) {
-  hipLaunchKernelGGL(force_gather_kernel<T>, dim3((n + 3) / 4), dim3(256), 0, st, n, maxnb,
-                     count, list, rev, site_of_atom, all_sites, gpair, forces, overwrite);
+                               const T* gpair, double* forces, hipStream_t st, int overwrite,
+                               const ReduceArgs* red) {
+  const ReduceArgs r = red ? *red : ReduceArgs{0, 0, nullptr, nullptr, nullptr, nullptr};
+  hipLaunchKernelGGL(force_gather_kernel<T>, dim3((n + 3) / 4 + (r.part ? TGX_RB : 0)), dim3(256),
+                     0, st, n, maxnb, count, list, rev, site_of_atom, all_sites, gpair, forces,
+                     overwrite, r);
   return hipGetLastError();
 }
-template hipError_t launch_force_gather<float>(int, int, const int*, const int*, const int*, const int*, int, const float*, double*, hipStream_t, int);
-template hipError_t launch_force_gather<double>(int, int, const int*, const int*, const int*, const int*, int, const double*, double*, hipStream_t, int);
+template hipError_t launch_force_gather<float>(int, int, const int*, const int*, const int*, const int*, int, const float*, double*, hipStream_t, int, const ReduceArgs*);
+template hipError_t launch_force_gather<double>(int, int, const int*, const int*, const int*, const int*, int, const double*, double*, hipStream_t, int, const ReduceArgs*);
 
 // ---------------------------------------------------------------------------
 // distance_2b term (gap.f90:329-460): each pair counted from both atoms,
@@ -1537,34 +1551,18 @@ hipError_t launch_site_add(int n_sites, const int* site_atom, const double* esit
 __global__ void reduce_part_kernel(int n_sites, const double* vsite, int n, const double* vatom,
                                    const double* eatom, double* part /* [RB][8] */) {
   __shared__ double sh[256][7];
-  const int t = threadIdx.x, b = blockIdx.x;
-  double a[7] = {0, 0, 0, 0, 0, 0, 0};
-  if (vsite) {
-    const int per = (n_sites + TGX_RB - 1) / TGX_RB, lo = b * per, hi = min(n_sites, lo + per);
-    for (int s = lo + t; s < hi; s += 256)
-      for (int k = 0; k < 6; k++) a[k] += vsite[(size_t)s * 6 + k];
-  }
-  {
-    const int per = (n + TGX_RB - 1) / TGX_RB, lo = b * per, hi = min(n, lo + per);
-    for (int s = lo + t; s < hi; s += 256) {
-      if (vatom)
-        for (int k = 0; k < 6; k++) a[k] += vatom[(size_t)s * 6 + k];
-      if (eatom) a[6] += eatom[s];
-    }
-  }
-  for (int k = 0; k < 7; k++) sh[t][k] = a[k];
-  __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if (t < off)
-      for (int k = 0; k < 7; k++) sh[t][k] += sh[t + off][k];
-    __syncthreads();
-  }
-  if (t < 7) part[b * 8 + t] = sh[0][t];
+  reduce_part_rows(blockIdx.x, n_sites, vsite, n, vatom, eatom, part, sh);
 }
 
 __global__ void reduce_final_kernel(const double* part, double* out /* 9 virial + energy */) {
   __shared__ double sh[TGX_RB][7];
   reduce_final_rows(part, out, sh);
+}
+
+// The final step alone, the partial rows at out + 16 already written.
+hipError_t launch_reduce_final(double* out, hipStream_t st) {
+  hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(TGX_RB), 0, st, out + 16, out);
+  return hipGetLastError();
 }
 
 // out must hold 10 + 8*TGX_RB doubles (the partials live after the result).
