@@ -89,9 +89,6 @@ struct CellMany {
 // row of 8 partials).
 #define TGX_RB 128
 
-// Fixed-order sum of the TGX_RB partial rows into out[0..8] (virial) and
-// out[9] (energy) by the first TGX_RB threads of a workgroup; sh holds
-// TGX_RB x 7 doubles.  Every thread of the workgroup must call it.
 // Partial row b of the deterministic energy/virial reduction: a fixed
 // partition of the sites (6-component virials vsite) and atoms (virials
 // vatom, energies eatom; each input may be null) over TGX_RB workgroups of
@@ -125,6 +122,9 @@ __device__ inline void reduce_part_rows(int b, int n_sites, const double* vsite,
   if (t < 7) part[b * 8 + t] = sh[0][t];
 }
 
+// Fixed-order sum of the TGX_RB partial rows into out[0..8] (virial) and
+// out[9] (energy) by the first TGX_RB threads of a workgroup; sh holds
+// TGX_RB x 7 doubles.  Every thread of the workgroup must call it.
 __device__ inline void reduce_final_rows(const double* part, double* out, double (*sh)[7]) {
   const int t = threadIdx.x;
   if (t < TGX_RB)
