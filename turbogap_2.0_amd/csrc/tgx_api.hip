@@ -502,6 +502,7 @@ static int setup_block_sites(tgx_ctx* c, SoapBlock* bp, int n, const int* specie
                                 hipMemcpyHostToDevice, c->stream));
     TGX_HIP(c, hipStreamSynchronize(c->stream));
     b.n_sites = (int)sa.size();
+    b.site_ident = b.n_sites == n;
   }
   c->blk = keep;
   return TGX_OK;
@@ -608,7 +609,10 @@ static const int kTuneWpb[4] = {0, 1, 2, 4};
 // SOAP forward for the current list into c->blk->q (and c->blk->qb in mixed mode).
 // few: a few sites (a Monte Carlo trial) take the workgroup-per-site FP32
 // kernel; TGX_FWD_SPLIT=1 asks for it on every FP32 call of up to 256 sites.
-static int soap_forward(tgx_ctx* c, int n, const Geom& cell, int n_sites, bool few = false) {
+// ident_ok: the sites are the block's own (setup_block_sites), not a Monte
+// Carlo trial's list, so an all-atom block may skip the site_atom lookup.
+static int soap_forward(tgx_ctx* c, int n, const Geom& cell, int n_sites, bool few = false,
+                        bool ident_ok = false) {
   Extra& x = X(c);
   const int prec = c->precision;
   const size_t ts = tsize(prec);
@@ -638,20 +642,21 @@ static int soap_forward(tgx_ctx* c, int n, const Geom& cell, int n_sites, bool f
   unsigned short* const fqb = cmp ? nullptr : qb;
   const char* fs = getenv("TGX_FWD_SPLIT");
   const int split = (few || (fs && fs[0] == '1')) ? 1 : 0;
+  const int* sat = (ident_ok && !few && b->site_ident) ? nullptr : b->site_atom.as<int>();
   auto fwd = [&](int wpb) -> hipError_t {
     if (prec == TGX_FP64)
       return launch_soap_fwd<double>(c->blk->sd, c->blk->sd_dev.as<SoapDev>(), c->blk->W_dev.as<double>(),
                                      c->blk->rtab.as<float>(), c->pos.as<double>(),
                                      c->species.as<int>(), cell, c->maxnb,
                                      c->nb_count.as<int>(), c->nb_list.as<int>(),
-                                     c->blk->site_atom.as<int>(), n_sites, c->blk->pairn.as<int>(),
+                                     sat, n_sites, c->blk->pairn.as<int>(),
                                      static_cast<double*>(fq), fld, nullptr, 0,
                                      c->blk->cbuf.as<double>(), static_cast<double*>(fpn), c->stream, 0,
                                      wpb);
     return launch_soap_fwd<float>(c->blk->sd, c->blk->sd_dev.as<SoapDev>(), c->blk->W_dev.as<double>(),
                                   c->blk->rtab.as<float>(), c->pos.as<double>(), c->species.as<int>(),
                                   cell, c->maxnb, c->nb_count.as<int>(), c->nb_list.as<int>(),
-                                  c->blk->site_atom.as<int>(), n_sites, c->blk->pairn.as<int>(),
+                                  sat, n_sites, c->blk->pairn.as<int>(),
                                   static_cast<float*>(fq), fld, fqb, c->blk->dim_pad,
                                   c->blk->cbuf.as<float>(), static_cast<float*>(fpn), c->stream, split,
                                   wpb);
@@ -867,7 +872,7 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
   const bool fuse = fuse_two_body(c, n, n_sites, do_forces);
   bool red_in_gather = false;  // the reduction's partial rows done by the force gather
   if (c->has_soap && n_sites > 0) {
-    rcode = soap_forward(c, n, cell, n_sites);
+    rcode = soap_forward(c, n, cell, n_sites, false, true);
     if (rcode) return rcode;
     ev_rec(c, 2);
     rcode = kernel_step(c, n_sites, do_forces != 0, nullptr,
@@ -895,7 +900,7 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
         return launch_soap_bwd<double>(
             c->blk->sd, c->blk->sd_dev.as<SoapDev>(), c->blk->W_dev.as<double>(), c->blk->rtab.as<float>(),
             c->pos.as<double>(), c->species.as<int>(), cell, c->maxnb, c->nb_count.as<int>(),
-            c->nb_list.as<int>(), c->blk->site_atom.as<int>(), n_sites, c->blk->pairn.as<int>(),
+            c->nb_list.as<int>(), (c->blk->site_ident ? nullptr : c->blk->site_atom.as<int>()), n_sites, c->blk->pairn.as<int>(),
             static_cast<const double*>(ai.q), ai.q_ld, c->blk->cbuf.as<double>(),
             static_cast<const double*>(ai.pn), static_cast<const double*>(ai.G), ai.g_ld,
             c->gpair.as<double>(), c->blk->vsite.as<double>(),
@@ -904,7 +909,7 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
       return launch_soap_bwd<float>(
           c->blk->sd, c->blk->sd_dev.as<SoapDev>(), c->blk->W_dev.as<double>(), c->blk->rtab.as<float>(),
           c->pos.as<double>(), c->species.as<int>(), cell, c->maxnb, c->nb_count.as<int>(),
-          c->nb_list.as<int>(), c->blk->site_atom.as<int>(), n_sites, c->blk->pairn.as<int>(),
+          c->nb_list.as<int>(), (c->blk->site_ident ? nullptr : c->blk->site_atom.as<int>()), n_sites, c->blk->pairn.as<int>(),
           static_cast<const float*>(ai.q), ai.q_ld, c->blk->cbuf.as<float>(),
           static_cast<const float*>(ai.pn), static_cast<const float*>(ai.G), ai.g_ld,
           c->gpair.as<float>(), c->blk->vsite.as<double>(),
@@ -1046,7 +1051,7 @@ static int ef_launch_multi(tgx_ctx* c, int n, const Geom& cell, int do_forces, b
     c->blk = bp.get();
     const int ns = c->blk->n_sites;
     if (ns == 0) continue;
-    rcode = soap_forward(c, n, cell, ns);
+    rcode = soap_forward(c, n, cell, ns, false, true);
     if (!rcode) rcode = kernel_step(c, ns, do_forces != 0);
     if (rcode) break;
     TGX_HIP(c, launch_site_add(ns, c->blk->site_atom.as<int>(), c->blk->esite.as<double>(),
@@ -1064,7 +1069,7 @@ static int ef_launch_multi(tgx_ctx* c, int n, const Geom& cell, int do_forces, b
       e = launch_soap_bwd<double>(
           c->blk->sd, c->blk->sd_dev.as<SoapDev>(), c->blk->W_dev.as<double>(),
           c->blk->rtab.as<float>(), c->pos.as<double>(), c->species.as<int>(), cell, c->maxnb,
-          c->nb_count.as<int>(), c->nb_list.as<int>(), c->blk->site_atom.as<int>(), ns,
+          c->nb_count.as<int>(), c->nb_list.as<int>(), (c->blk->site_ident ? nullptr : c->blk->site_atom.as<int>()), ns,
           c->blk->pairn.as<int>(), static_cast<const double*>(ai.q), ai.q_ld,
           c->blk->cbuf.as<double>(), static_cast<const double*>(ai.pn),
           static_cast<const double*>(ai.G), ai.g_ld, c->gpair.as<double>(),
@@ -1073,7 +1078,7 @@ static int ef_launch_multi(tgx_ctx* c, int n, const Geom& cell, int do_forces, b
       e = launch_soap_bwd<float>(
           c->blk->sd, c->blk->sd_dev.as<SoapDev>(), c->blk->W_dev.as<double>(),
           c->blk->rtab.as<float>(), c->pos.as<double>(), c->species.as<int>(), cell, c->maxnb,
-          c->nb_count.as<int>(), c->nb_list.as<int>(), c->blk->site_atom.as<int>(), ns,
+          c->nb_count.as<int>(), c->nb_list.as<int>(), (c->blk->site_ident ? nullptr : c->blk->site_atom.as<int>()), ns,
           c->blk->pairn.as<int>(), static_cast<const float*>(ai.q), ai.q_ld,
           c->blk->cbuf.as<float>(), static_cast<const float*>(ai.pn),
           static_cast<const float*>(ai.G), ai.g_ld, c->gpair.as<float>(),
@@ -1954,7 +1959,7 @@ int tgx_soap_descriptors(tgx_ctx* c, int n, const double* pos, const int* specie
   }
   int n_sites = 0;
   rcode = setup_sites(c, n, species, &n_sites, n_ext);
-  if (!rcode && n_sites > 0) rcode = soap_forward(c, n, Geom{cell}, n_sites);
+  if (!rcode && n_sites > 0) rcode = soap_forward(c, n, Geom{cell}, n_sites, false, true);
   c->maxnb = saved;
   if (rcode) return rcode;
   TGX_HIP(c, hipStreamSynchronize(c->stream));
@@ -3370,7 +3375,7 @@ static int local_property_impl(tgx_ctx* c, int index, int n, const double* pos,
   const int ns = P->n_sites;
   const int prec = c->precision;
   if (ns > 0) {
-    rcode = soap_forward(c, n, Geom{cell}, ns);
+    rcode = soap_forward(c, n, Geom{cell}, ns, false, true);
     if (!rcode) rcode = kernel_step(c, ns, V_der != nullptr);
     if (rcode) return rcode;
     if (V_der) rcode = lp_adjoint(c, P, Geom{cell}, n, ns);
@@ -3480,7 +3485,7 @@ static int vdw_ts_impl(tgx_ctx* c, int n, const double* pos, const int* species,
   const int ns = P->n_sites;
   const Geom geo{cell};
   if (ns > 0) {
-    rcode = soap_forward(c, n, geo, ns);
+    rcode = soap_forward(c, n, geo, ns, false, true);
     if (!rcode) rcode = kernel_step(c, ns, do_forces != 0);
     if (!rcode && do_forces) rcode = lp_adjoint(c, P, geo, n, ns);
     if (rcode) return rcode;

@@ -195,6 +195,9 @@ struct SoapBlock {
   // per call: sites (atoms of the central species, or all), descriptors,
   // kernel-step and adjoint work
   DevBuf site_of_atom, site_atom, q, qb, cbuf, pnorm, wmat, gmat, epart, esite, vsite;
+  // every atom is a site, so site s is atom s (the SOAP kernels then take no
+  // site_atom lookup)
+  bool site_ident = false;
   int n_sites = 0;
   // compress_soap (cmp = number of entries of P, 0 = none): the entries
   // (1-based, as given), P by rows and by columns (0-based) on the device,

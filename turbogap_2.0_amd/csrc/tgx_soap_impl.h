@@ -420,7 +420,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FwdWaves<T>
   T* sC = sAY;  // reused once the accumulation is done
   const int site = (blockIdx.x * wpb + wave) * SPW + sub;
   if (site >= n_sites) return;
-  const int i = site_atom[site];
+  const int i = site_atom ? site_atom[site] : site;  // null: site s is atom s
   const Cell cell = geo.of(i);
   const int si = species[i] - 1;
   const int cnt = min(count[i], maxnb);
@@ -631,7 +631,7 @@ __global__ __launch_bounds__(256) void soap_fwd_split_kernel(
   __shared__ T sred[4];
   const int site = blockIdx.x;
   if (site >= n_sites) return;
-  const int i = site_atom[site];
+  const int i = site_atom ? site_atom[site] : site;  // null: site s is atom s
   const Cell cell = geo.of(i);
   const int si = species[i] - 1;
   const int cnt = min(count[i], maxnb);
@@ -770,7 +770,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BwdWaves<T>
   u16* sIdx = reinterpret_cast<u16*>(sM + Lp * MS);
   const int site = (blockIdx.x * wpb + wave) * SPW + sub;
   if (site >= n_sites) return;
-  const int i = site_atom[site];
+  const int i = site_atom ? site_atom[site] : site;  // null: site s is atom s
   const Cell cell = geo.of(i);
   const int cnt = min(count[i], maxnb);
   const int* row = list + (size_t)i * maxnb;
