@@ -167,7 +167,7 @@ __host__ __device__ constexpr int tri(int a, int b) {
 template <typename T, int NRP>
 __host__ __device__ constexpr size_t bwd_site_bytes(int L2, int Lp) {
   return ((sizeof(T) * ((size_t)NRP * L2 + (size_t)Lp * MStride<NRP>::v) + 2 * Capv<NRP>::v) +
-          15) / 16 * 16;
+          15) / 16 * 16 + 64;  // + the site's running virial and 2-body energy (8 doubles)
 }
 
 // Appends, in slot order, the slots whose pair lies within the neighbour
@@ -813,8 +813,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BwdWaves<T>
     }
   }
   wave_sync();
-  double v[6] = {0, 0, 0, 0, 0, 0};
-  double e2b = 0.0;
+  // the site's virial and 2-body energy, summed per round of pairs into LDS
+  // (not carried in registers through the pair loop)
+  double* sV = reinterpret_cast<double*>(reinterpret_cast<unsigned char*>(sGC) +
+                                         bwd_site_bytes<T, NRP>(L2, Lp) - 64);
+  if (lane < 7) sV[lane] = 0.0;
   typedef typename std::conditional<sizeof(T) == 4, float4, double4>::type T4;
   T4* gp_out = reinterpret_cast<T4*>(gpair) + (size_t)i * maxnb;  // (g, 0) per slot
   int next = 0;
@@ -826,7 +829,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BwdWaves<T>
                                             });
     wave_sync();
     for (int c0 = 0; c0 < nv; c0 += LW) {
-      if (c0 + lane >= nv) continue;
+      double v[6] = {0, 0, 0, 0, 0, 0};
+      double e2b = 0.0;
+      if (c0 + lane < nv) {
       const int slot = sIdx[c0 + lane];
       const int e = row[slot];
       double dv[3];
@@ -945,18 +950,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BwdWaves<T>
       v[5] -= 0.5 * (gd[1] * dv[2] + gd[2] * dv[1]);
     }
       gp_out[slot] = T4{g0, g1, g2, (T)0};
+      }
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+        const double t = sub_sum<LW>(v[k]);
+        if (lane == 0) sV[k] += t;
+      }
+      if (n2b) {
+        const double t = sub_sum<LW>(e2b);
+        if (lane == 0) sV[6] += t;
+      }
     }
     wave_sync();  // the slot list is refilled by the next scan round
   }
-#pragma unroll
-  for (int k = 0; k < 6; k++) {
-    const double t = sub_sum<LW>(v[k]);
-    if (lane == 0) vsite[(size_t)site * 6 + k] = t;
-  }
-  if (n2b) {
-    const double t = sub_sum<LW>(e2b);
-    if (lane == 0) eatom[i] += t;
-  }
+  if (lane < 6) vsite[(size_t)site * 6 + lane] = sV[lane];
+  if (n2b && lane == 0) eatom[i] += sV[6];
 }
 
 // Sites per workgroup: as many waves as fit in 64 KB of LDS (at least one).
