@@ -225,6 +225,9 @@ struct MdState {
   double dt = 1.0, skin = 0.0, rcut_list = 0.0;
   Cell cell{};
   DevBuf pref, ke, ctl /* flag, nreb, step, pad */, ep, ek, vout, kesum;
+  // internal atom k is the caller's atom perm[k] (a Z-order of the atoms'
+  // positions at tgx_md_begin, so that neighbours sit close in memory)
+  std::vector<int> perm;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool use_graph = true;
@@ -3047,6 +3050,30 @@ static void md_release_graph(MdState& m) {
   m.graph = nullptr;
 }
 
+// Atom order along a Z-order (Morton) curve of the wrapped positions, 10 bits
+// per axis of the orthorhombic box (La, Lb, Lc); ties keep the input order.
+static std::vector<int> zorder_atoms(int n, const double* pos, double La, double Lb, double Lc) {
+  std::vector<std::pair<uint64_t, int>> key(n);
+  const double L[3] = {La, Lb, Lc};
+  for (int i = 0; i < n; i++) {
+    uint64_t k = 0;
+    for (int d = 0; d < 3; d++) {
+      double f = pos[3 * (size_t)i + d] / L[d];
+      f -= std::floor(f);
+      const uint64_t q = std::min<uint64_t>(1023, (uint64_t)(f * 1024.0));
+      for (int b = 0; b < 10; b++) k |= ((q >> b) & 1) << (3 * b + d);
+    }
+    key[i] = {k, i};
+  }
+  std::stable_sort(key.begin(), key.end(),
+                   [](const std::pair<uint64_t, int>& x, const std::pair<uint64_t, int>& y) {
+                     return x.first < y.first;
+                   });
+  std::vector<int> perm(n);
+  for (int i = 0; i < n; i++) perm[i] = key[i].second;
+  return perm;
+}
+
 int tgx_md_end(tgx_ctx* c) {
   if (!c) return TGX_ERR_ARG;
   (void)hipSetDevice(c->device);
@@ -3071,18 +3098,37 @@ int tgx_md_begin(tgx_ctx* c, int n, const int* species, const double* lattice,
   if (m.active) tgx_md_end(c);
   for (int i = 0; i < n; i++)
     if (!(mass[i] > 0)) return set_err(c, TGX_ERR_ARG, "mass[%d] must be positive", i);
-  int rcode = upload_structure(c, n, pos, species);
-  if (rcode) return rcode;
   m.n = n;
   m.dt = dt;
   m.skin = skin;
   m.cap = history;
   m.rcut_list = rcut_all(c) + skin;
   std::string err;
-  rcode = make_cell(lattice, m.rcut_list, &m.cell, &err);
+  int rcode = make_cell(lattice, m.rcut_list, &m.cell, &err);
   if (rcode) return set_err(c, rcode, "%s", err.c_str());
   if ((long long)n * m.cell.nrep >= (1LL << TGX_IMG_SHIFT))
     return set_err(c, TGX_ERR_ARG, "too many atoms");
+  // the run works on the atoms in a Z-order of their wrapped positions:
+  // the SOAP kernels, the force gather and the list build then read the
+  // data of nearby atoms from nearby memory; tgx_md_read restores the order
+  m.perm = zorder_atoms(n, pos, m.cell.La, m.cell.Lb, m.cell.Lc);
+  std::vector<double> pos_p(3 * (size_t)n), vel_p(3 * (size_t)n), mass_p(n);
+  std::vector<int> sp_p(n);
+  for (int k = 0; k < n; k++) {
+    const int a = m.perm[k];
+    for (int d = 0; d < 3; d++) {
+      pos_p[3 * (size_t)k + d] = pos[3 * (size_t)a + d];
+      vel_p[3 * (size_t)k + d] = vel[3 * (size_t)a + d];
+    }
+    mass_p[k] = mass[a];
+    sp_p[k] = species[a];
+  }
+  pos = pos_p.data();
+  vel = vel_p.data();
+  mass = mass_p.data();
+  species = sp_p.data();
+  rcode = upload_structure(c, n, pos, species);
+  if (rcode) return rcode;
   // size the list from a first build, with head room for density changes
   c->maxnb = 0;
   rcode = build_list(c, n, m.cell, m.rcut_list, true);
@@ -3186,9 +3232,18 @@ int tgx_md_read(tgx_ctx* c, double* pos, double* vel, double* forces, double* e_
   const int steps = ctl[2];
   if (n_steps_done) *n_steps_done = steps;
   if (n_rebuilds) *n_rebuilds = ctl[1];
-  if (pos) TGX_HIP(c, hipMemcpy(pos, c->pos.p, 24 * (size_t)n, hipMemcpyDeviceToHost));
-  if (vel) TGX_HIP(c, hipMemcpy(vel, m.vout.p, 24 * (size_t)n, hipMemcpyDeviceToHost));
-  if (forces) TGX_HIP(c, hipMemcpy(forces, c->forces.p, 24 * (size_t)n, hipMemcpyDeviceToHost));
+  // per-atom arrays back in the caller's atom order
+  std::vector<double> tmp(3 * (size_t)n);
+  auto out3 = [&](double* dst, const void* src) -> hipError_t {
+    const hipError_t e = hipMemcpy(tmp.data(), src, 24 * (size_t)n, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return e;
+    for (int k = 0; k < n; k++)
+      for (int d = 0; d < 3; d++) dst[3 * (size_t)m.perm[k] + d] = tmp[3 * (size_t)k + d];
+    return hipSuccess;
+  };
+  if (pos) TGX_HIP(c, out3(pos, c->pos.p));
+  if (vel) TGX_HIP(c, out3(vel, m.vout.p));
+  if (forces) TGX_HIP(c, out3(forces, c->forces.p));
   if (e_pot || e_kin) {
     const int cap = m.cap;
     std::vector<double> ep(cap), ek(cap);
