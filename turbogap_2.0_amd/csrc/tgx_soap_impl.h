@@ -574,23 +574,56 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FwdWaves<T>
   }
   ss = sub_sum<LW>(ss);
   const T inv = (T)1 / sqrt(ss);
+  if ((size_t)q_ld <= fwd_region_a<T, NRP>(L2) && (!qbs || qb_ld <= q_ld)) {
+    // q (with its zero padding) staged in LDS, where c was, and written in
+    // whole 16-byte units: lanes writing their own Lp consecutive components
+    // would leave every store uncoalesced
+    T* sQ = sAY;
+    wave_sync();
 #pragma unroll
-  for (int t = 0; t < KPI; t++) {
-    const int kp = lane + t * LW;
-    if (kp < npair) {
+    for (int t = 0; t < KPI; t++) {
+      const int kp = lane + t * LW;
+      if (kp < npair) {
 #pragma unroll
-      for (int l = 0; l <= LMX; l++) {
-        if (l <= L) {
-          const T v = pv[t][l] * inv;
-          qs[kp * Lp + l] = v;
-          if (qbs) qbs[kp * Lp + l] = f2bf((float)v);
+        for (int l = 0; l <= LMX; l++)
+          if (l <= L) sQ[kp * Lp + l] = pv[t][l] * inv;
+      }
+    }
+    for (int d = dim + lane; d < q_ld; d += LW) sQ[d] = (T)0;
+    wave_sync();
+    typedef typename std::conditional<sizeof(T) == 4, float4, double2>::type V;
+    constexpr int PER = 16 / sizeof(T);
+    for (int d = lane * PER; d < q_ld; d += LW * PER)
+      *reinterpret_cast<V*>(qs + d) = *reinterpret_cast<const V*>(sQ + d);
+    if (qbs) {
+      for (int d = lane * 8; d < qb_ld; d += LW * 8) {
+        unsigned int w[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+          w[k] = (unsigned int)f2bf((float)sQ[d + 2 * k]) |
+                 ((unsigned int)f2bf((float)sQ[d + 2 * k + 1]) << 16);
+        *reinterpret_cast<uint4*>(qbs + d) = uint4{w[0], w[1], w[2], w[3]};
+      }
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < KPI; t++) {
+      const int kp = lane + t * LW;
+      if (kp < npair) {
+#pragma unroll
+        for (int l = 0; l <= LMX; l++) {
+          if (l <= L) {
+            const T v = pv[t][l] * inv;
+            qs[kp * Lp + l] = v;
+            if (qbs) qbs[kp * Lp + l] = f2bf((float)v);
+          }
         }
       }
     }
+    for (int d = dim + lane; d < q_ld; d += LW) qs[d] = (T)0;
+    if (qbs)
+      for (int d = dim + lane; d < qb_ld; d += LW) qbs[d] = 0;
   }
-  for (int d = dim + lane; d < q_ld; d += LW) qs[d] = (T)0;
-  if (qbs)
-    for (int d = dim + lane; d < qb_ld; d += LW) qbs[d] = 0;
   if (lane == 0) pnorm[site] = sqrt(ss);
 }
 
