@@ -309,9 +309,11 @@ __device__ void nb_find_one(const double* pos, int i, int own, const Cell& c, do
   if (pref && lane < 3) pref[3 * i + lane] = pi[lane];
   wave_sync();
   // FP32 screen on wrapped coordinates: its error (a few 1e-5 A for boxes
-  // of 100 A) is far inside the 1e-3 A margin, and images near L/2 are far
-  // beyond rcut since L >= 3 rcut.
+  // of 100 A) is far inside a 1e-3 A margin, and images near L/2 are far
+  // beyond rcut since L >= 3 rcut.  Candidates within rcut - 1e-3 are hits
+  // for certain; those within 1e-3 of rcut take the exact FP64 test.
   const float rs = (float)rcut + 1e-3f, screen2 = rs * rs;
+  const float ri = (float)rcut - 1e-3f, inner2 = ri > 0.f ? ri * ri : 0.f;
   const float4 pw = wrapped_pos4(pos, i, c);
   const float La = (float)c.La, Lb = (float)c.Lb, Lc = (float)c.Lc;
   const float iLa = (float)c.iLa, iLb = (float)c.iLb, iLc = (float)c.iLc;
@@ -319,7 +321,7 @@ __device__ void nb_find_one(const double* pos, int i, int own, const Cell& c, do
   int* out = list + (size_t)i * maxnb;
   for (int idx0 = 0; idx0 < total; idx0 += 64) {
     const int idx = idx0 + lane;
-    bool near = false;
+    bool near = false, sure = false;
     int k = 0;
     if (idx < total) {
       // largest q with pre[q] <= idx
@@ -336,9 +338,20 @@ __device__ void nb_find_one(const double* pos, int i, int own, const Cell& c, do
         dx -= La * rintf(dx * iLa);
         dy -= Lb * rintf(dy * iLb);
         dz -= Lc * rintf(dz * iLc);
-        near = dx * dx + dy * dy + dz * dz <= screen2;
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        sure = d2 <= inner2;
+        near = !sure && d2 <= screen2;
       }
     }
+    // certain hits straight to the list (its final sort fixes the order);
+    // the band of +-1e-3 A around rcut to the exact FP64 test
+    const unsigned long long ms = __ballot(sure);
+    if (sure) {
+      const int slot = found + __popcll(ms & ((1ull << lane) - 1ull));
+      if (slot < TGX_NB_CAPL) L.buf[slot] = k;
+      else if (slot < maxnb) out[slot] = k;
+    }
+    found += __popcll(ms);
     const unsigned long long m = __ballot(near);
     if (near) L.q[nq + __popcll(m & ((1ull << lane) - 1ull))] = k;  // nq < 64 here
     nq += __popcll(m);
