@@ -136,9 +136,9 @@ template <int NRP> struct Capv { static constexpr int v = 2 * (64 / Spw<NRP>::v)
 
 // Per-site LDS footprint in bytes (16-byte multiple): forward = harmonics
 // and radial values of a chunk of pairs (the first part reused for c) plus
-// the compacted slot list (16-bit slots); adjoint = dE/dc [L2][NRP] (rows
+// the compacted list entries of the valid pairs (32 bits); adjoint = dE/dc [L2][NRP] (rows
 // of species sub-rows at 4-aligned offsets), the upper triangles of the
-// coupling matrices M_l and the slot list.
+// coupling matrices M_l and the list entries and slots of the valid pairs.
 // Forward region A holds the harmonics of a chunk (CH x L2, or one block of
 // them, CH x HSTR), later c with rows padded to a multiple of four (NRP x L2P).
 template <typename T, int NRP>
@@ -150,7 +150,7 @@ __host__ __device__ constexpr size_t fwd_region_a(int L2) {
 template <typename T, int NRP>
 __host__ __device__ constexpr size_t fwd_site_bytes(int L2) {
   constexpr int CH = FwdChunk<T, NRP>::v;
-  return ((sizeof(T) * (fwd_region_a<T, NRP>(L2) + (size_t)CH * NRP) + 2 * Capv<NRP>::v) + 15) /
+  return ((sizeof(T) * (fwd_region_a<T, NRP>(L2) + (size_t)CH * NRP) + 4 * Capv<NRP>::v) + 15) /
          16 * 16;
 }
 // The symmetric coupling matrices M_l keep their upper triangle (n <= n2) in
@@ -166,26 +166,28 @@ __host__ __device__ constexpr int tri(int a, int b) {
 }
 template <typename T, int NRP>
 __host__ __device__ constexpr size_t bwd_site_bytes(int L2, int Lp) {
-  return ((sizeof(T) * ((size_t)NRP * L2 + (size_t)Lp * MStride<NRP>::v) + 2 * Capv<NRP>::v) +
+  return ((sizeof(T) * ((size_t)NRP * L2 + (size_t)Lp * MStride<NRP>::v) + 6 * Capv<NRP>::v) +
           15) / 16 * 16 + 64;  // + the site's running virial and 2-body energy (8 doubles)
 }
 
-// Appends, in slot order, the slots whose pair lies within the neighbour
-// species' cutoff to idx[], scanning the list from *next in steps of LW
-// slots while at least LW entries are free.  Returns the count appended;
-// the same on every lane of the site.  on_skip(slot) runs for slots that
-// are scanned but not appended.
+// Appends, in slot order, the pairs within the neighbour species' cutoff --
+// their slots to slots[] and their list entries to ents[], either of which
+// may be null -- scanning the list from *next in steps of LW slots while at
+// least LW entries are free.  Returns the count appended; the same on every
+// lane of the site.  on_skip(slot) runs for slots that are scanned but not
+// appended.
 template <int LW, int SPW, typename GEO, typename F>
 __device__ __forceinline__ int scan_pairs(const SoapDev* sd, const double* pos,
                                           const int* species, const Cell& cell, const int* row,
-                                          int i, int cnt, int* next, u16* idx, int capv,
-                                          int sub, int lane, F on_skip) {
+                                          int i, int cnt, int* next, u16* slots, int* ents,
+                                          int capv, int sub, int lane, F on_skip) {
   int nv = 0;
   while (*next < cnt && nv <= capv - LW) {
     const int slot = *next + lane;
     bool valid = false;
+    int e = 0;
     if (slot < cnt) {
-      const int e = row[slot];
+      e = row[slot];
       double dv[3];
       const double rd = pair_vec_fast(pos, i, e, cell, dv);
       valid = rd < sd->rh[species[e & TGX_ATOM_MASK] - 1];
@@ -194,7 +196,8 @@ __device__ __forceinline__ int scan_pairs(const SoapDev* sd, const double* pos,
     unsigned long long mask = __ballot(valid);
     if (SPW == 2) mask = (mask >> (sub * 32)) & 0xffffffffull;
     const int jc = __popcll(mask & ((1ull << lane) - 1ull));
-    if (valid) idx[nv + jc] = (u16)slot;
+    if (valid && slots) slots[nv + jc] = (u16)slot;
+    if (valid && ents) ents[nv + jc] = e;
     nv += __popcll(mask);
     *next += LW;
   }
@@ -416,7 +419,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FwdWaves<T>
   constexpr int CAPV = Capv<NRP>::v;
   T* sAY = reinterpret_cast<T*>(smem_raw + (size_t)(wave * SPW + sub) * fwd_site_bytes<T, NRP>(L2));
   T* sR = sAY + fwd_region_a<T, NRP>(L2);
-  u16* sIdx = reinterpret_cast<u16*>(sR + CH * NRP);
+  // list entries of the valid pairs: the chunk lanes take them from LDS
+  // instead of reloading the list row
+  int* sEnt = reinterpret_cast<int*>(sR + CH * NRP);
   T* sC = sAY;  // reused once the accumulation is done
   const int site = (blockIdx.x * wpb + wave) * SPW + sub;
   if (site >= n_sites) return;
@@ -436,8 +441,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FwdWaves<T>
   while (next < cnt) {
     // valid pairs first, so that the expensive basis evaluation runs on
     // packed lanes (list slots beyond the cutoff are skipped)
-    const int nv = scan_pairs<LW, SPW, GEO>(sd, pos, species, cell, row, i, cnt, &next, sIdx,
-                                            CAPV, sub, lane, [](int) {});
+    const int nv = scan_pairs<LW, SPW, GEO>(sd, pos, species, cell, row, i, cnt, &next, nullptr,
+                                            sEnt, CAPV, sub, lane, [](int) {});
     wave_sync();
     for (int c0 = 0; c0 < nv; c0 += CH) {
       const bool act = lane < CH && c0 + lane < nv;
@@ -452,7 +457,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FwdWaves<T>
         PairHarm<T, LMX> h;
         if (act)
           fwd_pair_setup<T, NRP, NMX, LMX>(sd, W, rtab, pos, species, cell, i,
-                                           row[sIdx[c0 + lane]], sR + lane * NRP, h);
+                                           sEnt[c0 + lane], sR + lane * NRP, h);
         else if (lane < CH)
           fwd_pair_zero<T, NRP, LMX>(sR + lane * NRP, h);
         const int nj4 = (nj + 3) & ~3;
@@ -481,7 +486,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FwdWaves<T>
       } else {
         if (act)
           fwd_pair_basis<T, NRP, NMX, LMX>(sd, W, rtab, pos, species, cell, i,
-                                           row[sIdx[c0 + lane]], L, sAY + lane * L2,
+                                           sEnt[c0 + lane], L, sAY + lane * L2,
                                            sR + lane * NRP);
         wave_sync();
         for (int j = 0; j < nj; j++) {
@@ -800,7 +805,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BwdWaves<T>
   // sGC [L2][NRP] = dE/dc, species s at offa[s]; sM [Lp][NRP][NRP]: dE/dc = M_l c
   T* sM = sGC + NRP * L2;
   constexpr int MS = MStride<NRP>::v;
-  u16* sIdx = reinterpret_cast<u16*>(sM + Lp * MS);
+  // compacted valid pairs: list entries (so that the pair lanes need not
+  // reload the list row) and slots (where their gradients go)
+  int* sEnt = reinterpret_cast<int*>(sM + Lp * MS);
+  u16* sIdx = reinterpret_cast<u16*>(sEnt + CAPV);
   const int site = (blockIdx.x * wpb + wave) * SPW + sub;
   if (site >= n_sites) return;
   const int i = site_atom ? site_atom[site] : site;  // null: site s is atom s
@@ -857,7 +865,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BwdWaves<T>
   while (next < cnt) {
     // valid pairs are packed onto lanes; skipped slots get a zero gradient
     const int nv = scan_pairs<LW, SPW, GEO>(sd, pos, species, cell, row, i, cnt, &next, sIdx,
-                                            CAPV, sub, lane, [&](int slot) {
+                                            sEnt, CAPV, sub, lane, [&](int slot) {
                                               gp_out[slot] = T4{0, 0, 0, 0};
                                             });
     wave_sync();
@@ -866,7 +874,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BwdWaves<T>
       double e2b = 0.0;
       if (c0 + lane < nv) {
       const int slot = sIdx[c0 + lane];
-      const int e = row[slot];
+      const int e = sEnt[c0 + lane];
       double dv[3];
       const double rd = pair_vec_fast(pos, i, e, cell, dv);
       const int s = species[e & TGX_ATOM_MASK] - 1;
