@@ -33,7 +33,7 @@ def _nb_equal(a, b, ulps_angles=True):
 
 
 @pytest.mark.parametrize("case", ["carbon216", "small32", "cho300", "carbon2000", "dense84",
-                                  "dense145", "carbon6000"])
+                                  "dense145", "carbon6000", "vacuum1100"])
 def test_neighbors_bit_exact(eng, case):
     """Lists of up to 64, 65-128 and over 128 entries (dense84, dense145:
     1,500 C at 3.2 g/cm^3 with rcut 5.0 / 6.0 A) take the three sort paths.
@@ -54,6 +54,13 @@ def test_neighbors_bit_exact(eng, case):
     elif case == "carbon6000":
         pos, sp, lat = S.carbon_cell(6000, 2.0, 1.2, 57)
         rc = 4.2
+    elif case == "vacuum1100":
+        # a 300-atom cluster straddling the faces of a 1,100 A box (the
+        # build for boxes beyond the FP32 screen's range)
+        pos, sp, _ = S.carbon_cell(300, 2.0, 1.2, 58)
+        lat = np.diag([1100.0] * 3).ravel()
+        pos = pos - 7.0
+        rc = 3.7
     else:
         pos, sp, lat = S.carbon_cell(1500, 3.2, 1.0, 56)
         rc = 5.0 if case == "dense84" else 6.0

@@ -997,7 +997,11 @@ hipError_t launch_neighbors(const double* pos, int n, const Cell& c, double rcut
                             hipStream_t st, const int* guard, double* pref) {
   const int B = 256;
   int nb = (n + B - 1) / B;
-  const bool fused = !c.small && c.mx >= 3 && c.my >= 3 && c.mz >= 3;
+  // the one-pass build screens on FP32 wrapped positions with a 1e-3 A
+  // margin either side of the cutoff: boxes up to 1,024 A keep the FP32
+  // error below a third of it; larger ones take the FP64 build
+  const bool fp32_ok = c.La < 1024.0 && c.Lb < 1024.0 && c.Lc < 1024.0;
+  const bool fused = !c.small && c.mx >= 3 && c.my >= 3 && c.mz >= 3 && fp32_ok;
   // wrapped FP32 positions in cell order, after the ids (cell_atoms_bytes)
   float4* cpos = reinterpret_cast<float4*>(cell_atoms + ((n + 3) & ~3));
   // captured (guarded) builds of small systems: one launch instead of four,
