@@ -309,6 +309,31 @@ def test_list_buffer_does_not_change_results(eng):
     assert np.abs(v0 - v1).max() <= 1e-12 * np.abs(v0).max()
 
 
+
+@pytest.mark.parametrize("rows", [20000, 28000])
+def test_chunked_kernel_step_tile_heights(eng, rows):
+    """The chunked single-kernel step (600 sparse points) takes 96-row
+    workgroups where 128-row ones would leave the last round of workgroups
+    mostly empty: on 256 CUs the 216- and 20,000-site calls use 96 rows,
+    the 28,000-site call 128.  A site's E and dE/dq depend on its own row
+    only, so tiled copies of the 216 descriptor rows must reproduce the
+    216-row call bit for bit."""
+    pos, sp, lat, model = W.carbon216()
+    m2 = S.model_from_soaps(model.hypers, model.Qs, 600, seed=5, zeta=4.0)
+    eng.set_model(m2)
+    q = oracle.soap(model.hypers, pos, sp, lat)["soap"]
+    big = np.ascontiguousarray(np.tile(q, (rows // len(q) + 1, 1))[:rows])
+    print(f"rows={rows}", flush=True)
+    eng.set_precision("mixed")
+    try:
+        e1, g1 = eng.kernel_step(q)
+        e2, g2 = eng.kernel_step(big)
+    finally:
+        eng.set_precision("fp64")
+    idx = np.arange(rows) % len(q)
+    assert np.array_equal(e2, e1[idx])
+    assert np.array_equal(g2, g1[idx])
+
 @pytest.mark.parametrize("rcut2b", [3.2, 4.6])
 def test_two_body_fused_and_separate(eng, rcut2b):
     """The SOAP adjoint adds the 2-body terms itself when their cutoff lies

@@ -621,9 +621,11 @@ __global__ __launch_bounds__(512) void gap_fused_kernel(
 // chunks of 256 (stage 1, epilogue and stage 2 per chunk), G accumulating in
 // registers across the chunks and the energies in the row sums.  The q rows
 // are staged again for every chunk (from L2).
+// Workgroups take 128 or 96 rows (MI = 4 or 3 row tiles per wave half); a
+// row's arithmetic is the same either way.
 #define TGX_FUSED_SC 256
 #define TGX_FUSED_PFC 3  // stage-1 16-byte chunks per thread: (128 + 256) * 4 / 512
-template <int NT2, bool GRAD>
+template <int NT2, bool GRAD, int MI>
 __global__ __launch_bounds__(512) void gap_fused_chunked_kernel(
     int M, int NS, int K1, const u16* __restrict__ q, int ldq, const u16* __restrict__ qs,
     int ldqs, const u16* __restrict__ qsT, int ldqt, EpiKernel ek, double d2,
@@ -631,32 +633,36 @@ __global__ __launch_bounds__(512) void gap_fused_chunked_kernel(
   if (guard && *guard == 0) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int SC = TGX_FUSED_SC, T1 = SC / 64, ldw = SC + 8;
-  u16* sW = reinterpret_cast<u16*>(smem_raw);  // [128][ldw]; stage-1 slabs before
+  // MI row tiles of 16 per wave half: HR rows per half, ROWS per workgroup
+  constexpr int HR = 16 * MI, ROWS = 2 * HR;
+  u16* sW = reinterpret_cast<u16*>(smem_raw);  // [ROWS][ldw]; stage-1 slabs before
   u16* sT = reinterpret_cast<u16*>(smem_raw + fused_region_w(SC));  // stage-2 slab [K1][32]
-  float* sE = reinterpret_cast<float*>(sT);  // [4][128] row-sum partials, at the end
+  float* sE = reinterpret_cast<float*>(sT);  // [4][ROWS] row-sum partials, at the end
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int mh = wave >> 2, nq = wave & 3;
-  const int row0 = blockIdx.x * 128, rw = row0 + mh * 64;
+  const int row0 = blockIdx.x * ROWS, rw = row0 + mh * HR;
   const int lr = lane & 15, lk = (lane >> 4) * 8, lc = (lane >> 4) * 4;
   const float* al = reinterpret_cast<const float*>(ek.alphas);  // zero past n_sparse
   const float zd2 = (float)ek.zd2;
-  float rs[4] = {0.f, 0.f, 0.f, 0.f};
-  f32x4 acc2[4][NT2];
+  float rs[MI];
 #pragma unroll
-  for (int a = 0; a < 4; a++)
+  for (int a = 0; a < MI; a++) rs[a] = 0.f;
+  f32x4 acc2[MI][NT2];
+#pragma unroll
+  for (int a = 0; a < MI; a++)
 #pragma unroll
     for (int b = 0; b < NT2; b++) acc2[a][b] = f32x4{0, 0, 0, 0};
   const int c2 = nq * (16 * NT2);
   for (int sc0 = 0; sc0 < NS; sc0 += SC) {
     const int N1 = min(SC, NS - sc0), nt1 = N1 >> 6;  // N1: a multiple of 64
     // ---- stage 1 over the chunk
-    f32x4 acc[4][T1];
+    f32x4 acc[MI][T1];
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int a = 0; a < MI; a++)
 #pragma unroll
       for (int b = 0; b < T1; b++) acc[a][b] = f32x4{0, 0, 0, 0};
     {
-      const int stage_rows = 128 + N1, nchunk = stage_rows * 4;
+      const int stage_rows = ROWS + N1, nchunk = stage_rows * 4;
       uint4 pf[TGX_FUSED_PFC];
       auto fetch = [&](int k0) {
 #pragma unroll
@@ -665,11 +671,11 @@ __global__ __launch_bounds__(512) void gap_fused_chunked_kernel(
           const int r = ch >> 2, part = (ch & 3) * 8;
           uint4 v = uint4{0, 0, 0, 0};
           if (ch < nchunk) {
-            if (r < 128) {
+            if (r < ROWS) {
               if (row0 + r < M)
                 v = *reinterpret_cast<const uint4*>(q + (size_t)(row0 + r) * ldq + k0 + part);
             } else {
-              v = *reinterpret_cast<const uint4*>(qs + (size_t)(sc0 + r - 128) * ldqs + k0 +
+              v = *reinterpret_cast<const uint4*>(qs + (size_t)(sc0 + r - ROWS) * ldqs + k0 +
                                                   part);
             }
           }
@@ -693,18 +699,18 @@ __global__ __launch_bounds__(512) void gap_fused_chunked_kernel(
       for (int sl = 0; sl < nslab; sl++) {
         if (sl + 1 < nslab) fetch((sl + 1) * 32);
         const u16* st = sW + (size_t)(sl & 1) * stage_rows * TGX_FUSED_SLD;
-        bf16x8 af[4];
+        bf16x8 af[MI];
 #pragma unroll
-        for (int mi = 0; mi < 4; mi++)
+        for (int mi = 0; mi < MI; mi++)
           af[mi] = *reinterpret_cast<const bf16x8*>(
-              st + slab_off(mh * 64 + mi * 16 + lr, lane >> 4));
+              st + slab_off(mh * HR + mi * 16 + lr, lane >> 4));
 #pragma unroll
         for (int ni = 0; ni < T1; ni++)
           if (ni < nt1) {
             const bf16x8 bfr = *reinterpret_cast<const bf16x8*>(
-                st + slab_off(128 + (N1 >> 2) * nq + ni * 16 + lr, lane >> 4));
+                st + slab_off(ROWS + (N1 >> 2) * nq + ni * 16 + lr, lane >> 4));
 #pragma unroll
-            for (int mi = 0; mi < 4; mi++)
+            for (int mi = 0; mi < MI; mi++)
               acc[mi][ni] =
                   __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr, af[mi], acc[mi][ni], 0, 0, 0);
           }
@@ -723,7 +729,7 @@ __global__ __launch_bounds__(512) void gap_fused_chunked_kernel(
           const float4 a4 = *reinterpret_cast<const float4*>(al + sc0 + sl0);
           const float av[4] = {a4.x, a4.y, a4.z, a4.w};
 #pragma unroll
-          for (int mi = 0; mi < 4; mi++) {
+          for (int mi = 0; mi < MI; mi++) {
             __bf16 w[4];
 #pragma unroll
             for (int v = 0; v < 4; v++) {
@@ -733,7 +739,7 @@ __global__ __launch_bounds__(512) void gap_fused_chunked_kernel(
               w[v] = (__bf16)(zd2 * av[v] * kz1);
             }
             if (GRAD)
-              *reinterpret_cast<uint2*>(sW + (size_t)(mh * 64 + mi * 16 + lr) * ldwc + sl0) =
+              *reinterpret_cast<uint2*>(sW + (size_t)(mh * HR + mi * 16 + lr) * ldwc + sl0) =
                   __builtin_bit_cast(uint2, w);
           }
         }
@@ -777,10 +783,10 @@ __global__ __launch_bounds__(512) void gap_fused_chunked_kernel(
         stash();
         __syncthreads();
         if (k0 + 32 < N1) fetch(k0 + 32);
-        bf16x8 aw[4];
+        bf16x8 aw[MI];
 #pragma unroll
-        for (int mi = 0; mi < 4; mi++)
-          aw[mi] = *reinterpret_cast<const bf16x8*>(sW + (size_t)(mh * 64 + mi * 16 + lr) * ldwc +
+        for (int mi = 0; mi < MI; mi++)
+          aw[mi] = *reinterpret_cast<const bf16x8*>(sW + (size_t)(mh * HR + mi * 16 + lr) * ldwc +
                                                     k0 + lk);
 #pragma unroll
         for (int nj = 0; nj < NT2; nj++) {
@@ -788,7 +794,7 @@ __global__ __launch_bounds__(512) void gap_fused_chunked_kernel(
           const bf16x8 bq = *reinterpret_cast<const bf16x8*>(
               sT + (size_t)r * 32 + (((lane >> 4) ^ ((r >> 2) & 3)) * 8));
 #pragma unroll
-          for (int mi = 0; mi < 4; mi++)
+          for (int mi = 0; mi < MI; mi++)
             acc2[mi][nj] =
                 __builtin_amdgcn_mfma_f32_16x16x32_bf16(bq, aw[mi], acc2[mi][nj], 0, 0, 0);
         }
@@ -797,20 +803,20 @@ __global__ __launch_bounds__(512) void gap_fused_chunked_kernel(
     __syncthreads();  // the next chunk's staging overwrites W and the slab
   }
 #pragma unroll
-  for (int mi = 0; mi < 4; mi++) {
+  for (int mi = 0; mi < MI; mi++) {
     float x = rs[mi];
     x += __shfl_xor(x, 16, 64);
     x += __shfl_xor(x, 32, 64);
-    if (lane < 16) sE[nq * 128 + mh * 64 + mi * 16 + lr] = x;
+    if (lane < 16) sE[nq * ROWS + mh * HR + mi * 16 + lr] = x;
   }
   __syncthreads();
-  if (tid < 128 && row0 + tid < M)
+  if (tid < ROWS && row0 + tid < M)
     put_site_energy(ek, esite, row0 + tid,
-                    d2 * ((double)sE[tid] + (double)sE[128 + tid] + (double)sE[256 + tid] +
-                          (double)sE[384 + tid]));
+                    d2 * ((double)sE[tid] + (double)sE[ROWS + tid] + (double)sE[2 * ROWS + tid] +
+                          (double)sE[3 * ROWS + tid]));
   if (!GRAD) return;
 #pragma unroll
-  for (int mi = 0; mi < 4; mi++) {
+  for (int mi = 0; mi < MI; mi++) {
     const int r = rw + mi * 16 + lr;
     if (r < M)
 #pragma unroll
@@ -819,22 +825,55 @@ __global__ __launch_bounds__(512) void gap_fused_chunked_kernel(
   }
 }
 
-template <int NT2, bool GRAD>
-static hipError_t run_fused_chunked(int M, int sp_pad, int dim_pad, const void* q, const void* qs,
-                                    const void* qsT, const EpiKernel& ek, double delta,
-                                    double* esite, void* G, const int* guard, hipStream_t st) {
+template <int NT2, bool GRAD, int MI>
+static hipError_t launch_fused_chunked(int M, int sp_pad, int dim_pad, const void* q,
+                                       const void* qs, const void* qsT, const EpiKernel& ek,
+                                       double delta, double* esite, void* G, const int* guard,
+                                       hipStream_t st) {
   static size_t sm_set = 0;
+  // sized for 128-row tiles, which also holds 96
   const size_t sm = fused_region_w(TGX_FUSED_SC) + fused_region_t(dim_pad);
   if (sm > sm_set) {
-    hipError_t e = hipFuncSetAttribute((const void*)gap_fused_chunked_kernel<NT2, GRAD>,
+    hipError_t e = hipFuncSetAttribute((const void*)gap_fused_chunked_kernel<NT2, GRAD, MI>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
     if (e != hipSuccess) return e;
     sm_set = sm;
   }
-  hipLaunchKernelGGL((gap_fused_chunked_kernel<NT2, GRAD>), dim3((M + 127) / 128), dim3(512), sm,
-                     st, M, sp_pad, dim_pad, (const u16*)q, dim_pad, (const u16*)qs, dim_pad,
-                     (const u16*)qsT, sp_pad, ek, delta * delta, esite, (float*)G, dim_pad, guard);
+  constexpr int ROWS = 32 * MI;
+  hipLaunchKernelGGL((gap_fused_chunked_kernel<NT2, GRAD, MI>), dim3((M + ROWS - 1) / ROWS),
+                     dim3(512), sm, st, M, sp_pad, dim_pad, (const u16*)q, dim_pad,
+                     (const u16*)qs, dim_pad, (const u16*)qsT, sp_pad, ek, delta * delta, esite,
+                     (float*)G, dim_pad, guard);
   return hipGetLastError();
+}
+
+// Rows per workgroup, 128 or 96: one workgroup fits a CU (LDS), so the
+// launch takes ceil(workgroups / CUs) rounds of a workgroup's time, which
+// scales with its rows; 96-row tiles win where 128-row ones leave the last
+// round mostly empty (71,255 rows on 256 CUs: 3 rounds of 96 rows against 3
+// of 128).
+static int chunked_tile_rows(int M) {
+  static int ncu = 0;
+  if (ncu <= 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        ncu <= 0)
+      ncu = 256;
+  }
+  auto cost = [&](long rows) { return ((M + rows - 1) / rows + ncu - 1) / ncu * rows; };
+  return cost(96) < cost(128) ? 96 : 128;
+}
+
+template <int NT2, bool GRAD>
+static hipError_t run_fused_chunked(int M, int sp_pad, int dim_pad, const void* q, const void* qs,
+                                    const void* qsT, const EpiKernel& ek, double delta,
+                                    double* esite, void* G, const int* guard, hipStream_t st) {
+  if (chunked_tile_rows(M) == 96)
+    return launch_fused_chunked<NT2, GRAD, 3>(M, sp_pad, dim_pad, q, qs, qsT, ek, delta, esite,
+                                              G, guard, st);
+  return launch_fused_chunked<NT2, GRAD, 4>(M, sp_pad, dim_pad, q, qs, qsT, ek, delta, esite, G,
+                                            guard, st);
 }
 
 template <int NT2, bool GRAD>
