@@ -488,13 +488,22 @@ int tgx_sync(tgx_ctx* ctx);
  * beyond a term's cutoff contribute nothing. */
 int tgx_set_list_buffer(tgx_ctx* ctx, double buffer);
 
-/* Per-stage timing of the last call, milliseconds (HIP events):
+/* Timing of the library's own launches (the reference's do_timing switch,
+ * gap/gap.f90:230-242): on, tgx_energy_forces and tgx_energy_forces_batched
+ * bracket their stages and their main kernels with HIP events for
+ * tgx_stage_times and tgx_kernel_times.  Off by default: the events cost the
+ * GPU several microseconds of idle time each; off, those calls report 0. */
+int tgx_set_timing(tgx_ctx* ctx, int on);
+
+/* Per-stage timing of the last call, milliseconds (HIP events, with
+ * tgx_set_timing on):
  * [0] neighbours, [1] SOAP forward, [2] kernel GEMMs, [3] SOAP backward and
  * force gather, [4] 2-body, [5] integrator, [6] total. */
 int tgx_stage_times(const tgx_ctx* ctx, double* ms7);
 
 /* Device time of single kernels in the last tgx_energy_forces call,
- * milliseconds, each bracketed by HIP events on the context's stream:
+ * milliseconds, each bracketed by HIP events on the context's stream (with
+ * tgx_set_timing on):
  * [0] SOAP forward, [1] kernel GEMMs (both), [2] SOAP adjoint, [3] force
  * gather.  0 for a kernel that did not run.  (Instrumentation; the
  * reference's equivalent is the do_timing wall-clock printout,

@@ -343,3 +343,25 @@ def test_two_body_fused_and_separate(eng, rcut2b):
     m = tgx.GapModel(model.hypers, model.Qs, model.alphas, model.delta, model.zeta,
                      two_body=[S.carbon_2b(1000, rcut=rcut2b)])
     _ef_check(eng, m, pos, sp, lat, 1e-8, 1e-6, 1e-6)
+
+
+def test_timing_events_are_opt_in(eng):
+    """Stage and kernel timing events only with tgx_set_timing on (0 figures
+    otherwise); the results do not depend on them."""
+    pos, sp, lat, model = W.carbon216()
+    eng.set_model(model)
+    e0, f0, v0 = eng.energy_forces(pos, sp, lat)
+    assert all(t == 0.0 for t in eng.kernel_times().values())
+    assert all(t == 0.0 for t in eng.stage_times().values())
+    eng.set_timing(True)
+    try:
+        e1, f1, v1 = eng.energy_forces(pos, sp, lat)
+        kt, st = eng.kernel_times(), eng.stage_times()
+    finally:
+        eng.set_timing(False)
+    print(kt, st, flush=True)
+    assert kt["soap_fwd"] > 0 and kt["soap_bwd"] > 0 and kt["force_gather"] > 0
+    assert st["total"] >= st["soap_fwd"] > 0
+    assert np.array_equal(e0, e1) and np.array_equal(f0, f1) and np.array_equal(v0, v1)
+    eng.energy_forces(pos, sp, lat)
+    assert all(t == 0.0 for t in eng.kernel_times().values())

@@ -271,6 +271,9 @@ struct Extra {
   hipEvent_t kev[8];  // brackets of single kernels: fwd, GEMMs, adjoint, gather
   bool ev_ready = false;
   bool ev_on = true;
+  // stage and kernel timing events (tgx_set_timing; off by default: every
+  // event between two kernels leaves the GPU idle for several microseconds)
+  bool timing = false;
   MdState md;
 };
 
@@ -786,11 +789,11 @@ static int kernel_step(tgx_ctx* c, int n_sites, bool want_grad, const int* guard
 
 static void ev_rec(tgx_ctx* c, int k) {
   Extra& x = X(c);
-  if (x.ev_ready && x.ev_on) (void)hipEventRecord(x.ev[k], c->stream);
+  if (x.ev_ready && x.ev_on && x.timing) (void)hipEventRecord(x.ev[k], c->stream);
 }
 static void kev_rec(tgx_ctx* c, int k) {
   Extra& x = X(c);
-  if (x.ev_ready && x.ev_on) (void)hipEventRecord(x.kev[k], c->stream);
+  if (x.ev_ready && x.ev_on && x.timing) (void)hipEventRecord(x.kev[k], c->stream);
 }
 
 static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_forces,
@@ -1006,6 +1009,11 @@ static int ef_launch(tgx_ctx* c, int n, int n_sites, const Geom& cell, int do_fo
 static void collect_times(tgx_ctx* c) {
   Extra& x = X(c);
   if (!x.ev_ready || !x.ev_on) return;
+  if (!x.timing) {  // no figures of this call
+    for (int k = 0; k < 7; k++) c->stage_ms[k] = 0.0;
+    for (int k = 0; k < 4; k++) c->kernel_ms[k] = 0.0;
+    return;
+  }
   float ms[6];
   for (int k = 0; k < 6; k++) {
     ms[k] = 0.f;
@@ -3660,6 +3668,12 @@ int tgx_set_mc_batch(tgx_ctx* c, int trials_per_batch) {
   if (trials_per_batch < 0 || trials_per_batch > 64)
     return set_err(c, TGX_ERR_ARG, "trials per batch must be 0..64");
   X(c).mc_batch = trials_per_batch;
+  return TGX_OK;
+}
+
+int tgx_set_timing(tgx_ctx* c, int on) {
+  if (!c) return TGX_ERR_ARG;
+  X(c).timing = on != 0;
   return TGX_OK;
 }
 

@@ -76,7 +76,7 @@ module tgx_mod
        tgx_soap_descriptors, tgx_gap_kernel_step, tgx_energy_forces, &
        tgx_energy_forces_batched, tgx_md_run, tgx_md_begin, tgx_md_steps, tgx_md_read, &
        tgx_md_end, tgx_sync, tgx_poly3_orthonormalization, tgx_stage_times, tgx_check, &
-       tgx_error_string, tgx_kernel_times, tgx_mc_run, tgx_set_list_buffer, tgx_mc_run_da, &
+       tgx_error_string, tgx_kernel_times, tgx_set_timing, tgx_mc_run, tgx_set_list_buffer, tgx_mc_run_da, &
        tgx_surrogate_init, tgx_surrogate_set, tgx_surrogate_get, tgx_surrogate_predict, &
        tgx_surrogate_gradient, tgx_surrogate_fit, tgx_add_local_property, &
        tgx_clear_local_properties, tgx_local_property, tgx_set_vdw_ts, tgx_clear_vdw, &
@@ -371,6 +371,12 @@ module tgx_mod
        type(c_ptr), value :: ctx
        real(c_double), value :: buffer
      end function tgx_set_list_buffer
+
+     integer(c_int) function tgx_set_timing(ctx, on) bind(C, name="tgx_set_timing")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: on
+     end function tgx_set_timing
 
      integer(c_int) function tgx_kernel_times(ctx, ms4) bind(C, name="tgx_kernel_times")
        import :: c_int, c_ptr, c_double

@@ -331,6 +331,7 @@ def load_library(path: str = LIB_PATH):
         "tgx_stage_times": (i, [P, dp]),
         "tgx_kernel_times": (i, [P, dp]),
         "tgx_set_list_buffer": (i, [P, d]),
+        "tgx_set_timing": (i, [P, i]),
         "tgx_mc_run_da": (i, [P, i, ip, dp, dp, i, d, d, d, C.c_uint64,
                               C.POINTER(DaOptions), dp, C.POINTER(McStats)]),
         "tgx_surrogate_init": (i, [P, i, C.c_uint64]),
@@ -742,6 +743,10 @@ class Engine:
 
     def set_list_buffer(self, buffer: float):
         self._check(self.lib.tgx_set_list_buffer(self.ctx, float(buffer)))
+
+    def set_timing(self, on: bool = True):
+        """Stage and kernel timing events on or off (off by default)."""
+        self._check(self.lib.tgx_set_timing(self.ctx, int(bool(on))))
 
     def kernel_times(self):
         """Event-bracketed device time (ms) of single kernels in the last
