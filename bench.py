@@ -105,8 +105,8 @@ def build_model(eng):
 
 def zorder(pos, lat):
     """Atom order along a Z-order (Morton) curve of the wrapped positions,
-    10 bits per axis of the orthorhombic box: the order the MD driver runs
-    its atoms in, so that single-call kernel timings see the MD step's
+    10 bits per axis of the orthorhombic box: the order the library's MD
+    run (tgx_md_begin) keeps its atoms in, so that single-call kernel timings see the MD step's
     memory locality."""
     L = np.array([lat[0], lat[4], lat[8]])
     f = np.mod(pos / L, 1.0)
