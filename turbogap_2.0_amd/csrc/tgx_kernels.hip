@@ -1132,13 +1132,15 @@ __global__ __launch_bounds__(256) void force_gather_kernel(int n, int maxnb, con
   }
   typedef typename std::conditional<sizeof(T) == 4, float4, double4>::type T4;
   const T4* g4 = reinterpret_cast<const T4*>(gpair);
-  const int lane = threadIdx.x & 63;
-  const int k = blk * 4 + (threadIdx.x >> 6);
+  // 32 lanes per atom (a typical list has about 30 slots), eight atoms per
+  // workgroup
+  const int lane = threadIdx.x & 31;
+  const int k = blk * 8 + (threadIdx.x >> 5);
   if (k >= n) return;
   const int cnt = min(count[k], maxnb);
   const bool ksite = all_sites || site_of_atom[k] >= 0;
   double f0 = 0, f1 = 0, f2 = 0;
-  for (int s = lane; s < cnt; s += 64) {
+  for (int s = lane; s < cnt; s += 32) {
     const size_t p = (size_t)k * maxnb + s;
     if (ksite) {
       const T4 g = g4[p];
@@ -1154,7 +1156,7 @@ __global__ __launch_bounds__(256) void force_gather_kernel(int n, int maxnb, con
     }
   }
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
+  for (int o = 16; o > 0; o >>= 1) {
     f0 += __shfl_xor(f0, o, 64);
     f1 += __shfl_xor(f1, o, 64);
     f2 += __shfl_xor(f2, o, 64);
@@ -1178,7 +1180,7 @@ hipError_t launch_force_gather(int n, int maxnb, const int* count, const int* li
                                const T* gpair, double* forces, hipStream_t st, int overwrite,
                                const ReduceArgs* red) {
   const ReduceArgs r = red ? *red : ReduceArgs{0, 0, nullptr, nullptr, nullptr, nullptr};
-  hipLaunchKernelGGL(force_gather_kernel<T>, dim3((n + 3) / 4 + (r.part ? TGX_RB : 0)), dim3(256),
+  hipLaunchKernelGGL(force_gather_kernel<T>, dim3((n + 7) / 8 + (r.part ? TGX_RB : 0)), dim3(256),
                      0, st, n, maxnb, count, list, rev, site_of_atom, all_sites, gpair, forces,
                      overwrite, r);
   return hipGetLastError();
