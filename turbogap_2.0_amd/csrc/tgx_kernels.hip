@@ -629,11 +629,12 @@ __global__ __launch_bounds__(256) void reverse_kernel(int n, int maxnb, G geo, c
                                                       const int* list, int* rev, int* flag,
                                                       const int* guard) {
   GUARD_RETURN();
-  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  // 32 lanes per atom, eight atoms per workgroup
+  const int i = blockIdx.x * 8 + (threadIdx.x >> 5);
   if (i >= n) return;
   const Cell c = geo.of(i);
   int cnt = min(count[i], maxnb);
-  for (int s = threadIdx.x & 63; s < cnt; s += 64) {
+  for (int s = threadIdx.x & 31; s < cnt; s += 32) {
     int e = list[(size_t)i * maxnb + s];
     int j = e & TGX_ATOM_MASK, img = e >> TGX_IMG_SHIFT;
     int nimg = 0;
@@ -1011,7 +1012,7 @@ hipError_t launch_neighbors(const double* pos, int n, const Cell& c, double rcut
     hipLaunchKernelGGL(nb_build_small_kernel, dim3(1), dim3(1024), 0, st, pos, n, c, rcut,
                        cell_of, cell_atoms, cpos, count, maxnb, list, flag, pref, guard);
     if (rev)
-      hipLaunchKernelGGL(reverse_kernel<CellOne>, dim3((n + 3) / 4), dim3(256), 0, st, n, maxnb,
+      hipLaunchKernelGGL(reverse_kernel<CellOne>, dim3((n + 7) / 8), dim3(256), 0, st, n, maxnb,
                          CellOne{c}, count, list, rev, flag, guard);
     return hipGetLastError();
   }
@@ -1044,7 +1045,7 @@ hipError_t launch_neighbors(const double* pos, int n, const Cell& c, double rcut
                          pref, guard);
   }
   if (rev)
-    hipLaunchKernelGGL(reverse_kernel<CellOne>, dim3((n + 3) / 4), dim3(256), 0, st, n, maxnb,
+    hipLaunchKernelGGL(reverse_kernel<CellOne>, dim3((n + 7) / 8), dim3(256), 0, st, n, maxnb,
                        CellOne{c},
                        count, list, rev, flag, guard);
   return hipGetLastError();
@@ -1090,7 +1091,7 @@ hipError_t launch_neighbors_many(const double* pos, int n, const Cell* cells, co
                      sid, boff, aoff, rcut, cell_of, cell_start, cell_atoms, count, maxnb, list,
                      flag);
   if (rev)
-    hipLaunchKernelGGL(reverse_kernel<CellMany>, dim3((n + 3) / 4), dim3(256), 0, st, n, maxnb,
+    hipLaunchKernelGGL(reverse_kernel<CellMany>, dim3((n + 7) / 8), dim3(256), 0, st, n, maxnb,
                        CellMany{cells, sid}, count, list, rev, flag, nullptr);
   return hipGetLastError();
 }
